@@ -1,0 +1,129 @@
+"""ctypes mirror of include/ge_rt.h: the engine's GPU records and the backend's own.
+
+Layouts are asserted against the engine's size tables on import (ClusterDesc 48, MeshDesc 64,
+InstanceDesc 96, FrameEye 48, ResolveMaterial 128, ResolveLight 64, PathTraceParams 368,
+RayVisibilityParams 256 — domain/gfx/tests/size_table.cpp:41,122,125,388,397,452 and
+domain/geometry/tests/size_table.cpp:13 in the reference).
+"""
+import ctypes as C
+
+import numpy as np
+
+u32, i32, u64, f32, f64 = C.c_uint32, C.c_int32, C.c_uint64, C.c_float, C.c_double
+
+
+class ClusterDesc(C.Structure):
+    _fields_ = [("vertex_offset", u32), ("triangle_offset", u32), ("vertex_count", u32),
+                ("triangle_count", u32), ("center", f32 * 3), ("radius", f32),
+                ("cone_apex", f32 * 3), ("cone", u32)]
+
+
+class VertexAttributes(C.Structure):
+    _fields_ = [("normal_oct", u32), ("uv_half2", u32)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("quant", f32 * 4), ("quantized", u64), ("first_cluster", u32),
+                ("cluster_count", u32), ("deform_pool", u64), ("deform_slots", u64),
+                ("templates", u64), ("skin", u64)]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("rows", (f32 * 4) * 3), ("cell", i32 * 3), ("mesh", u32), ("material_base", u32),
+                ("first_pair", u32), ("scale_max", f32), ("flags", u32), ("deform", u32),
+                ("bounds_padding", f32), ("terrain", u32), ("band", u32)]
+
+
+class FrameEye(C.Structure):
+    _fields_ = [("cell", i32 * 3), ("masks", u32), ("local", f32 * 3), ("pad1", f32),
+                ("residual", f32 * 3), ("pad2", f32)]
+
+
+class ResolveMaterial(C.Structure):
+    _fields_ = [("albedo", f32 * 4), ("emissive", f32 * 4), ("albedo_texture", u32),
+                ("sampler", u32), ("uv_scale", f32), ("flags", u32),
+                ("metallic_roughness_texture", u32), ("normal_texture", u32),
+                ("normal_scale", f32), ("occlusion_texture", u32), ("emissive_texture", u32),
+                ("occlusion_strength", f32), ("samplers_mr_normal", u32),
+                ("samplers_occlusion_emissive", u32), ("uv_transform", f32 * 4),
+                ("uv_offset", f32 * 4), ("specular", f32 * 4)]
+
+
+class ResolveLight(C.Structure):
+    _fields_ = [("position_radius", f32 * 4), ("color_intensity", f32 * 4),
+                ("direction_type", f32 * 4), ("spot", f32 * 4)]
+
+
+class PathTraceParams(C.Structure):
+    _fields_ = [("clip_to_ray", f32 * 16), ("camera", f32 * 4), ("sky", f32 * 4), ("sun", f32 * 4),
+                ("ground", f32 * 4), ("accum", u64), ("output", u64), ("clusters", u64),
+                ("mesh", u64), ("triangles", u64), ("instances", u64), ("attributes", u64),
+                ("materials", u64), ("cluster_materials", u64), ("visible", u64), ("lights", u64),
+                ("width", u32), ("height", u32), ("light_count", u32), ("scene", u32),
+                ("spp", u32), ("sample_base", u32), ("max_bounces", u32), ("seed", u32),
+                ("flags", u32), ("ray_bias", f32), ("coverage", u64), ("background", u32),
+                ("ray_bias_steps", f32), ("ground_detail", u64), ("sky_params", u64),
+                ("eye", FrameEye), ("terrain_uv_x0_mm", i32), ("terrain_uv_z0_mm", i32),
+                ("terrain_uv_per_mm", f32), ("extra_vertices", u32), ("hit_entries", u64),
+                ("extra", u64)]
+
+
+class RayVisibilityParams(C.Structure):
+    _fields_ = [("view_proj", f32 * 16), ("clip_to_ray", f32 * 16), ("camera", f32 * 4),
+                ("output", u64), ("instance_base", u64), ("width", u32), ("height", u32),
+                ("scene", u32), ("caster_base", u32), ("visible", u64), ("instances", u64),
+                ("meshes", u64), ("triangles", u64), ("clusters", u64), ("hit_entries", u64),
+                ("masks", u64 * 4)]
+
+
+class ShadowParams(C.Structure):
+    _fields_ = [("sun", f32 * 4), ("offset", f32), ("offset_steps", f32), ("t_max", f32),
+                ("count", u32), ("pad", u32 * 4)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("closest_rays", u64), ("any_rays", u64), ("nodes", u64), ("triangles", u64),
+                ("instances", u64), ("last_kernel_ms", f64), ("last_build_ms", f64)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("clusters", C.c_void_p), ("cluster_count", u32), ("vertex_count", u32),
+                ("vertices", C.c_void_p), ("quantized", C.c_void_p), ("attributes", C.c_void_p),
+                ("triangles", C.c_void_p), ("triangle_count", u32), ("mesh_count", u32),
+                ("meshes", C.c_void_p), ("instances", C.c_void_p), ("instance_count", u32),
+                ("material_count", u32), ("materials", C.c_void_p),
+                ("cluster_materials", C.c_void_p), ("lights", C.c_void_p), ("light_count", u32),
+                ("pad", u32)]
+
+
+# numpy views of the backend's own records.
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instance_id", "<u4"),
+                      ("geometry", "<u4"), ("primitive", "<u4"), ("key", "<u4"), ("pad", "<u4")])
+NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3), ("hi1", "<f4", 3),
+                       ("child", "<u4", 2), ("parent", "<u4"), ("count", "<u4")])
+TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3), ("prim", "<u4"),
+                      ("index", "<u4"), ("pad", "<u4")])
+RECEIVER_DTYPE = np.dtype([("position", "<f4", 3), ("instance_id", "<u4"), ("geometric", "<f4", 3),
+                           ("pad", "<u4")])
+
+SIZES = {ClusterDesc: 48, VertexAttributes: 8, MeshDesc: 64, InstanceDesc: 96, FrameEye: 48,
+         ResolveMaterial: 128, ResolveLight: 64, PathTraceParams: 368, RayVisibilityParams: 256,
+         ShadowParams: 48}
+for _t, _n in SIZES.items():
+    assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
+assert NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and RECEIVER_DTYPE.itemsize == 32
+
+OK = 0
+DEVICE_PTR = 0x1
+COUNT = 0x2
+BVH_LEAF = 0x80000000
+BVH_NONE = 0xFFFFFFFF
+MISS = 0xFFFFFFFF
+PT_UNIFORM_SKY = 1
+PT_PIXEL_CENTER = 2
+PT_DITHER = 4
+PT_ALPHA_MASKS = 8
+BUILD_FULL, BUILD_SUBSET, BUILD_TOP = 0, 1, 2
+INSTANCE_STATIC_RT = 128

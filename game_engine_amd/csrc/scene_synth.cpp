@@ -1,0 +1,860 @@
+// scene_synth.cpp — seeded synthetic scenes in the engine's GPU formats (host code, no GPU).
+//
+// Cluster records, the packed triangles, the cluster-ordered vertices, the quantized grid and the
+// octahedral normals follow domain/geometry/include/domain/geometry/cluster.h:45-60,350-443 and
+// domain/geometry/src/cluster.cpp:437-466; the camera follows core/math/include/core/math/math.h
+// (look_at :564-584, perspective_reversed_z :586-600, inverse :336-370) and
+// domain/gfx/include/domain/gfx/view_ray.h:26-32; the ray bias follows
+// systems/renderer/include/systems/renderer/lighting.h:122-141.
+#include "scene_synth.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+struct Rng {
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s(seed) {}
+  double next() { return double(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+  float uniform(float a, float b) { return float(a + (b - a) * next()); }
+};
+
+uint32_t pcg_hash(uint32_t v) {
+  const uint32_t state = v * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+struct F3 {
+  float x, y, z;
+};
+F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+F3 norm3(F3 a) {
+  const float l = std::sqrt(dot3(a, a));
+  return l > 0 ? a * (1.0f / l) : F3{0, 0, 1};
+}
+
+// The inverse of scene.slang's decode_oct_normal (scene.slang:612-626, encode_oct_normal).
+uint32_t encode_oct(F3 n) {
+  const float l1 = std::fabs(n.x) + std::fabs(n.y) + std::fabs(n.z);
+  float px = l1 > 0 ? n.x / l1 : 0.0f, py = l1 > 0 ? n.y / l1 : 0.0f;
+  if (n.z < 0.0f) {
+    const float qx = (1.0f - std::fabs(py)) * (px < 0.0f ? -1.0f : 1.0f);
+    const float qy = (1.0f - std::fabs(px)) * (py < 0.0f ? -1.0f : 1.0f);
+    px = qx;
+    py = qy;
+  }
+  const int x = int(std::round(std::min(std::max(px, -1.0f), 1.0f) * 32767.0f));
+  const int y = int(std::round(std::min(std::max(py, -1.0f), 1.0f) * 32767.0f));
+  return (uint32_t(x) & 0xffffu) | ((uint32_t(y) & 0xffffu) << 16);
+}
+
+// A source mesh before clustering: positions, normals, and triangle corner indices.
+struct Source {
+  std::vector<F3> pos, nrm;
+  std::vector<uint32_t> idx;
+};
+
+void icosphere(Source& m, F3 c, float radius, int subdivisions) {
+  const float t = (1.0f + std::sqrt(5.0f)) * 0.5f;
+  std::vector<F3> v = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+                       {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+  std::vector<uint32_t> f = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6,
+                             7, 1, 8, 3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10,
+                             8, 6, 7, 9, 8, 1};
+  for (F3& p : v) p = norm3(p);
+  for (int s = 0; s < subdivisions; ++s) {
+    std::vector<uint32_t> g;
+    std::vector<std::pair<uint64_t, uint32_t>> mid;
+    auto midpoint = [&](uint32_t a, uint32_t b) {
+      const uint64_t key = (uint64_t(std::min(a, b)) << 32) | std::max(a, b);
+      for (auto& e : mid)
+        if (e.first == key) return e.second;
+      v.push_back(norm3((v[a] + v[b]) * 0.5f));
+      mid.push_back({key, uint32_t(v.size() - 1)});
+      return uint32_t(v.size() - 1);
+    };
+    for (size_t i = 0; i < f.size(); i += 3) {
+      const uint32_t a = f[i], b = f[i + 1], c2 = f[i + 2];
+      const uint32_t ab = midpoint(a, b), bc = midpoint(b, c2), ca = midpoint(c2, a);
+      const uint32_t tris[12] = {a, ab, ca, b, bc, ab, c2, ca, bc, ab, bc, ca};
+      g.insert(g.end(), tris, tris + 12);
+    }
+    f.swap(g);
+  }
+  const uint32_t base = uint32_t(m.pos.size());
+  for (const F3& p : v) {
+    m.pos.push_back(c + p * radius);
+    m.nrm.push_back(p);
+  }
+  for (uint32_t i : f) m.idx.push_back(base + i);
+}
+
+// A box with each face split n x n (2 n^2 triangles a face), flat normals.
+void split_box(Source& m, F3 c, F3 half, int n) {
+  const F3 axes[6][3] = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, {{-1, 0, 0}, {0, 0, 1}, {0, 1, 0}},
+                         {{0, 1, 0}, {0, 0, 1}, {1, 0, 0}}, {{0, -1, 0}, {1, 0, 0}, {0, 0, 1}},
+                         {{0, 0, 1}, {1, 0, 0}, {0, 1, 0}}, {{0, 0, -1}, {0, 1, 0}, {1, 0, 0}}};
+  for (int face = 0; face < 6; ++face) {
+    const F3 nrm = axes[face][0], u = axes[face][1], w = axes[face][2];
+    const uint32_t base = uint32_t(m.pos.size());
+    for (int j = 0; j <= n; ++j) {
+      for (int i = 0; i <= n; ++i) {
+        const float a = -1.0f + 2.0f * float(i) / float(n), b = -1.0f + 2.0f * float(j) / float(n);
+        const F3 local = nrm + u * a + w * b;
+        m.pos.push_back(c + F3{local.x * half.x, local.y * half.y, local.z * half.z});
+        m.nrm.push_back(nrm);
+      }
+    }
+    for (int j = 0; j < n; ++j) {
+      for (int i = 0; i < n; ++i) {
+        const uint32_t a = base + uint32_t(j * (n + 1) + i), b = a + 1, d = a + uint32_t(n + 1), e = d + 1;
+        // Outward winding: u x w = normal for every face above.
+        const uint32_t tris[6] = {a, b, e, a, e, d};
+        m.idx.insert(m.idx.end(), tris, tris + 6);
+      }
+    }
+  }
+}
+
+// A torus of `major` x `minor` quads, radius R, tube r, its tube wobbled by a per-mesh harmonic.
+void torus(Source& m, float R, float r, int major, int minor, float wobble, int k1, int k2) {
+  const uint32_t base = uint32_t(m.pos.size());
+  const float two_pi = 6.283185307179586f;
+  for (int i = 0; i <= major; ++i) {
+    const float th = two_pi * float(i % major) / float(major);
+    for (int j = 0; j <= minor; ++j) {
+      const float ph = two_pi * float(j % minor) / float(minor);
+      const float rr = r * (1.0f + wobble * std::sin(float(k1) * th + float(k2) * ph));
+      const F3 ring{std::cos(th), 0.0f, std::sin(th)};
+      const F3 n{std::cos(ph) * ring.x, std::sin(ph), std::cos(ph) * ring.z};
+      m.pos.push_back(ring * R + n * rr);
+      m.nrm.push_back(n);
+    }
+  }
+  for (int i = 0; i < major; ++i) {
+    for (int j = 0; j < minor; ++j) {
+      const uint32_t a = base + uint32_t(i * (minor + 1) + j), b = a + 1;
+      const uint32_t d = a + uint32_t(minor + 1), e = d + 1;
+      const uint32_t tris[6] = {a, d, e, a, e, b};
+      m.idx.insert(m.idx.end(), tris, tris + 6);
+    }
+  }
+}
+
+}  // namespace
+
+struct ge_synth {
+  std::vector<ge_cluster_desc> clusters;
+  std::vector<float> vertices;
+  std::vector<uint16_t> quantized;
+  std::vector<ge_vertex_attributes> attributes;
+  bool has_attributes = true;
+  std::vector<uint32_t> triangles;
+  std::vector<ge_mesh_desc> meshes;
+  std::vector<ge_instance_desc> instances;
+  std::vector<ge_resolve_material> materials;
+  std::vector<uint32_t> cluster_materials;
+  std::vector<ge_resolve_light> lights;
+  // Frame.
+  F3 eye{0, 0, 0}, target{0, 0, 1};
+  float fov_deg = 60.0f;
+  float sun[4] = {0, 1, 0, 3};
+  float sky[3] = {0.55f, 0.70f, 0.90f};
+  float ground[3] = {0.3f, 0.3f, 0.3f};
+  uint32_t spp = 1, max_bounces = 4;
+  float reach = 1.0f;
+  uint32_t config = 0;
+};
+
+namespace {
+
+// Appends a mesh: the source's triangles in clusters of `cluster_size` in source order, each
+// cluster with its own block of vertices (cluster-ordered copies), quantized onto the mesh's own
+// 16-bit grid (quantize_positions, cluster.cpp:437-466), with `material` per cluster.
+uint32_t add_mesh(ge_synth& s, const Source& src, uint32_t cluster_size, const std::vector<uint32_t>& tri_material) {
+  ge_mesh_desc mesh{};
+  mesh.first_cluster = uint32_t(s.clusters.size());
+  const uint32_t vbase = uint32_t(s.vertices.size() / 3);
+  const uint32_t ntri = uint32_t(src.idx.size() / 3);
+  std::vector<F3> local_pos;
+  for (uint32_t t0 = 0; t0 < ntri; t0 += cluster_size) {
+    const uint32_t t1 = std::min(ntri, t0 + cluster_size);
+    ge_cluster_desc cl{};
+    cl.vertex_offset = uint32_t(s.vertices.size() / 3);
+    cl.triangle_offset = uint32_t(s.triangles.size());
+    std::vector<uint32_t> remap;  // source vertex -> local
+    std::vector<uint32_t> locals;
+    for (uint32_t t = t0; t < t1; ++t) {
+      uint32_t packed = 0;
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t sv = src.idx[size_t(t) * 3 + k];
+        uint32_t li = 0xffffffffu;
+        for (uint32_t q = 0; q < locals.size(); ++q)
+          if (locals[q] == sv) {
+            li = q;
+            break;
+          }
+        if (li == 0xffffffffu) {
+          li = uint32_t(locals.size());
+          locals.push_back(sv);
+        }
+        packed |= li << (8 * k);
+      }
+      s.triangles.push_back(packed);
+    }
+    F3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t sv : locals) {
+      const F3 p = src.pos[sv];
+      s.vertices.insert(s.vertices.end(), {p.x, p.y, p.z});
+      s.attributes.push_back({encode_oct(norm3(src.nrm[sv])), 0u});
+      lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+      hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+    }
+    cl.vertex_count = uint32_t(locals.size());
+    cl.triangle_count = t1 - t0;
+    const F3 c = (lo + hi) * 0.5f;
+    float rad = 0;
+    for (uint32_t sv : locals) rad = std::max(rad, std::sqrt(dot3(src.pos[sv] - c, src.pos[sv] - c)));
+    cl.center[0] = c.x;
+    cl.center[1] = c.y;
+    cl.center[2] = c.z;
+    cl.radius = rad;
+    cl.cone = 0x7f000000u;  // k_cone_none: never culled
+    s.clusters.push_back(cl);
+    s.cluster_materials.push_back(tri_material.empty() ? 0u : tri_material[t0]);
+  }
+  mesh.cluster_count = uint32_t(s.clusters.size()) - mesh.first_cluster;
+  // The mesh's grid (cluster.cpp:437-466).
+  const uint32_t vend = uint32_t(s.vertices.size() / 3);
+  F3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = vbase; v < vend; ++v) {
+    const F3 p{s.vertices[v * 3], s.vertices[v * 3 + 1], s.vertices[v * 3 + 2]};
+    lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+    hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+  }
+  const float extent = std::max(std::max(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+  const float scale = extent > 0.0f ? extent / 65535.0f : 1.0f;
+  const float inverse_scale = 1.0f / scale;
+  mesh.quant[0] = lo.x;
+  mesh.quant[1] = lo.y;
+  mesh.quant[2] = lo.z;
+  mesh.quant[3] = scale;
+  s.quantized.resize(size_t(vend) * 3);
+  for (uint32_t v = vbase; v < vend; ++v) {
+    const float axis[3] = {s.vertices[v * 3] - lo.x, s.vertices[v * 3 + 1] - lo.y, s.vertices[v * 3 + 2] - lo.z};
+    for (int k = 0; k < 3; ++k) {
+      const float grid = std::round(axis[k] * inverse_scale);
+      const float clamped = grid < 0.0f ? 0.0f : (grid > 65535.0f ? 65535.0f : grid);
+      s.quantized[size_t(v) * 3 + k] = uint16_t(uint32_t(clamped));
+    }
+  }
+  s.meshes.push_back(mesh);
+  return uint32_t(s.meshes.size() - 1);
+}
+
+ge_instance_desc make_instance(uint32_t mesh, const float rot[3][3], F3 t, uint32_t material_base, uint32_t first_pair) {
+  ge_instance_desc in{};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) in.rows[r][c] = rot[r][c];
+  in.rows[0][3] = t.x;
+  in.rows[1][3] = t.y;
+  in.rows[2][3] = t.z;
+  in.mesh = mesh;
+  in.material_base = material_base;
+  in.first_pair = first_pair;
+  in.scale_max = 1.0f;
+  in.flags = GE_INSTANCE_STATIC_RT | 1u;  // static structure, uniform scale
+  in.deform = GE_INVALID_DEFORM;
+  return in;
+}
+
+// Shoemake 1992: a uniform rotation from three uniforms.
+void random_rotation(Rng& rng, float m[3][3]) {
+  const double u1 = rng.next(), u2 = rng.next(), u3 = rng.next();
+  const double a = std::sqrt(1.0 - u1), b = std::sqrt(u1);
+  const double w = a * std::sin(2 * M_PI * u2), x = a * std::cos(2 * M_PI * u2);
+  const double y = b * std::sin(2 * M_PI * u3), z = b * std::cos(2 * M_PI * u3);
+  m[0][0] = float(1 - 2 * (y * y + z * z));
+  m[0][1] = float(2 * (x * y - z * w));
+  m[0][2] = float(2 * (x * z + y * w));
+  m[1][0] = float(2 * (x * y + z * w));
+  m[1][1] = float(1 - 2 * (x * x + z * z));
+  m[1][2] = float(2 * (y * z - x * w));
+  m[2][0] = float(2 * (x * z - y * w));
+  m[2][1] = float(2 * (y * z + x * w));
+  m[2][2] = float(1 - 2 * (x * x + y * y));
+}
+
+ge_resolve_material make_material(float r, float g, float b, float roughness, float metallic) {
+  ge_resolve_material m{};
+  m.albedo[0] = r;
+  m.albedo[1] = g;
+  m.albedo[2] = b;
+  m.albedo[3] = roughness;
+  m.emissive[3] = metallic;
+  m.albedo_texture = GE_NO_TEXTURE;
+  m.metallic_roughness_texture = GE_NO_TEXTURE;
+  m.normal_texture = GE_NO_TEXTURE;
+  m.occlusion_texture = GE_NO_TEXTURE;
+  m.emissive_texture = GE_NO_TEXTURE;
+  m.uv_scale = 1.0f;
+  m.normal_scale = 1.0f;
+  m.occlusion_strength = 1.0f;
+  m.samplers_mr_normal = 0xffffffffu;
+  m.samplers_occlusion_emissive = 0xffffffffu;
+  m.uv_transform[0] = 1.0f;
+  m.uv_transform[3] = 1.0f;
+  return m;
+}
+
+void set_sun(ge_synth& s, float x, float y, float z, float intensity) {
+  const F3 d = norm3({x, y, z});
+  s.sun[0] = d.x;
+  s.sun[1] = d.y;
+  s.sun[2] = d.z;
+  s.sun[3] = intensity;
+}
+
+void build_config1(ge_synth& s, uint64_t seed) {
+  Rng rng(seed);
+  Source src;
+  for (uint32_t o = 0; o < 256; ++o) {
+    const F3 c{rng.uniform(0, 100), rng.uniform(0, 100), rng.uniform(0, 100)};
+    if (o % 2 == 0) {
+      icosphere(src, c, rng.uniform(1.5f, 5.0f), 2);  // 320 triangles
+    } else {
+      split_box(src, c, {rng.uniform(1, 4), rng.uniform(1, 4), rng.uniform(1, 4)}, 4);  // 192 triangles
+    }
+    s.materials.push_back(make_material(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), 1.0f, 0.0f));
+  }
+  // Clusters never straddle two objects: each object's run of triangles is clustered on its own
+  // (objects are 320 or 192 triangles, so sequential clusters of 128 would straddle), into one mesh
+  // whose grid is fit once over every object afterwards, as quantize_positions fits one per mesh.
+  const uint32_t mesh = uint32_t(s.meshes.size());
+  {
+    ge_mesh_desc m{};
+    m.first_cluster = uint32_t(s.clusters.size());
+    s.meshes.push_back(m);
+  }
+  // Build clusters per object into the shared arrays, then quantize once over the whole mesh.
+  size_t tri_cursor = 0;
+  for (uint32_t o = 0; o < 256; ++o) {
+    const uint32_t count = (o % 2 == 0) ? 320u : 192u;
+    Source obj;
+    std::vector<uint32_t> remap(src.pos.size(), 0xffffffffu);
+    for (size_t t = tri_cursor; t < tri_cursor + count; ++t) {
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t sv = src.idx[t * 3 + k];
+        if (remap[sv] == 0xffffffffu) {
+          remap[sv] = uint32_t(obj.pos.size());
+          obj.pos.push_back(src.pos[sv]);
+          obj.nrm.push_back(src.nrm[sv]);
+        }
+        obj.idx.push_back(remap[sv]);
+      }
+    }
+    tri_cursor += count;
+    const std::vector<uint32_t> mats(count, o);
+    const uint32_t before_meshes = uint32_t(s.meshes.size());
+    add_mesh(s, obj, 128, mats);
+    s.meshes.resize(before_meshes);  // fold the object's clusters into the one mesh
+  }
+  // One grid over every object (quantize_positions runs once per mesh).
+  ge_mesh_desc& m = s.meshes[mesh];
+  m.cluster_count = uint32_t(s.clusters.size()) - m.first_cluster;
+  const uint32_t nv = uint32_t(s.vertices.size() / 3);
+  F3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < nv; ++v) {
+    lo = {std::min(lo.x, s.vertices[v * 3]), std::min(lo.y, s.vertices[v * 3 + 1]), std::min(lo.z, s.vertices[v * 3 + 2])};
+    hi = {std::max(hi.x, s.vertices[v * 3]), std::max(hi.y, s.vertices[v * 3 + 1]), std::max(hi.z, s.vertices[v * 3 + 2])};
+  }
+  const float extent = std::max(std::max(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+  const float scale = extent > 0.0f ? extent / 65535.0f : 1.0f;
+  const float inv = 1.0f / scale;
+  m.quant[0] = lo.x;
+  m.quant[1] = lo.y;
+  m.quant[2] = lo.z;
+  m.quant[3] = scale;
+  s.quantized.assign(size_t(nv) * 3, 0);
+  for (uint32_t v = 0; v < nv; ++v) {
+    const float axis[3] = {s.vertices[v * 3] - lo.x, s.vertices[v * 3 + 1] - lo.y, s.vertices[v * 3 + 2] - lo.z};
+    for (int k = 0; k < 3; ++k) {
+      const float grid = std::round(axis[k] * inv);
+      const float clamped = grid < 0.0f ? 0.0f : (grid > 65535.0f ? 65535.0f : grid);
+      s.quantized[size_t(v) * 3 + k] = uint16_t(uint32_t(clamped));
+    }
+  }
+  const float identity[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  s.instances.push_back(make_instance(mesh, identity, {0, 0, 0}, 0, 0));
+  set_sun(s, 0.3f, 0.8f, 0.5f, 3.0f);
+  s.eye = {50.0f + 125.0f * std::sin(0.3839724f), 95.0f, 50.0f - 125.0f * std::cos(0.3839724f)};  // a 22 degree orbit
+  s.target = {50, 45, 50};
+  s.fov_deg = 60.0f;
+  s.spp = 1;
+  s.max_bounces = 4;
+  s.reach = std::sqrt(3.0f) * 110.0f;
+}
+
+void build_config2(ge_synth& s, uint64_t seed, uint32_t count) {
+  Rng rng(seed);
+  const float box = 200.0f * std::cbrt(float(count) / 4096.0f);
+  for (uint32_t k = 0; k < 16; ++k) {
+    Source src;
+    torus(src, rng.uniform(1.5f, 3.0f), rng.uniform(0.35f, 0.8f), 32, 16, rng.uniform(0.05f, 0.3f),
+          int(1 + rng.next() * 5), int(1 + rng.next() * 4));  // 1,024 triangles
+    add_mesh(s, src, 128, std::vector<uint32_t>(src.idx.size() / 3, 0u));
+  }
+  uint32_t first_pair = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    float rot[3][3];
+    random_rotation(rng, rot);
+    const F3 t{rng.uniform(0, box), rng.uniform(0, box), rng.uniform(0, box)};
+    const uint32_t mesh = uint32_t(rng.next() * 16) & 15u;
+    s.instances.push_back(make_instance(mesh, rot, t, i, first_pair));
+    first_pair += s.meshes[mesh].cluster_count;
+    const float roughness = rng.uniform(0.05f, 1.0f);
+    const float metallic = rng.next() < 0.2 ? 1.0f : 0.0f;
+    s.materials.push_back(make_material(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), roughness, metallic));
+  }
+  for (uint32_t l = 0; l < 64; ++l) {
+    ge_resolve_light light{};
+    light.position_radius[0] = rng.uniform(0, box);
+    light.position_radius[1] = rng.uniform(0, box);
+    light.position_radius[2] = rng.uniform(0, box);
+    light.position_radius[3] = 0.175f * box;  // radius of influence
+    light.color_intensity[0] = rng.uniform(0.6f, 1.0f);
+    light.color_intensity[1] = rng.uniform(0.5f, 0.9f);
+    light.color_intensity[2] = rng.uniform(0.4f, 0.8f);
+    light.color_intensity[3] = rng.uniform(100.0f, 400.0f);
+    light.spot[2] = 0.05f;  // the emitter's radius: shadow rays stop that far short
+    s.lights.push_back(light);
+  }
+  set_sun(s, -0.4f, 0.75f, 0.35f, 3.0f);
+  s.eye = {0.5f * box, 0.55f * box, -0.12f * box};
+  s.target = {0.5f * box, 0.45f * box, 0.6f * box};
+  s.fov_deg = 60.0f;
+  s.spp = 2;
+  s.max_bounces = 8;
+  s.reach = std::sqrt(3.0f) * (box + 10.0f);
+}
+
+// PCG-hashed value noise, smoothstep-interpolated, summed over octaves (fBm).
+float value_noise(float x, float z, uint32_t seed) {
+  const float fx = std::floor(x), fz = std::floor(z);
+  const int ix = int(fx), iz = int(fz);
+  const float tx = x - fx, tz = z - fz;
+  auto h = [&](int a, int b) {
+    return float(pcg_hash(uint32_t(a) * 73856093u ^ uint32_t(b) * 19349663u ^ seed * 83492791u)) * (1.0f / 4294967296.0f);
+  };
+  const float sx = tx * tx * (3 - 2 * tx), sz = tz * tz * (3 - 2 * tz);
+  const float a = h(ix, iz) + (h(ix + 1, iz) - h(ix, iz)) * sx;
+  const float b = h(ix, iz + 1) + (h(ix + 1, iz + 1) - h(ix, iz + 1)) * sx;
+  return a + (b - a) * sz;
+}
+float fbm(float x, float z, uint32_t seed) {
+  float sum = 0, amp = 0.5f, freq = 1.0f;
+  for (int o = 0; o < 6; ++o) {
+    sum += amp * value_noise(x * freq, z * freq, seed + uint32_t(o));
+    freq *= 2.0f;   // lacunarity
+    amp *= 0.5f;    // gain
+  }
+  return sum;
+}
+
+void build_config3(ge_synth& s, uint64_t seed, uint32_t groups, uint32_t n) {
+  Rng rng(seed);
+  const float extent = 4096.0f;  // metres across the terrain
+  const float cell = extent / float(n - 1);
+  const float height = 220.0f;
+  // The terrain: one mesh, one instance, rows of quads in clusters of 128 triangles (64 quads: a
+  // 8 x 8 quad block, so each cluster is compact).
+  {
+    std::vector<float> hgt(size_t(n) * n);
+    for (uint32_t j = 0; j < n; ++j)
+      for (uint32_t i = 0; i < n; ++i)
+        hgt[size_t(j) * n + i] = height * fbm(float(i) / 256.0f, float(j) / 256.0f, uint32_t(seed) ^ 0x51u);
+    ge_mesh_desc mesh{};
+    mesh.first_cluster = uint32_t(s.clusters.size());
+    const uint32_t vstart = uint32_t(s.vertices.size() / 3);
+    const uint32_t blocks = (n - 1 + 7) / 8;
+    for (uint32_t bj = 0; bj < blocks; ++bj) {
+      for (uint32_t bi = 0; bi < blocks; ++bi) {
+        const uint32_t i0 = bi * 8, j0 = bj * 8;
+        const uint32_t i1 = std::min(i0 + 8, n - 1), j1 = std::min(j0 + 8, n - 1);
+        ge_cluster_desc cl{};
+        cl.vertex_offset = uint32_t(s.vertices.size() / 3);
+        cl.triangle_offset = uint32_t(s.triangles.size());
+        const uint32_t w = i1 - i0 + 1;
+        F3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t j = j0; j <= j1; ++j) {
+          for (uint32_t i = i0; i <= i1; ++i) {
+            const F3 p{float(i) * cell, hgt[size_t(j) * n + i], float(j) * cell};
+            const float hx = hgt[size_t(j) * n + std::min(i + 1, n - 1)] - hgt[size_t(j) * n + (i > 0 ? i - 1 : 0)];
+            const float hz = hgt[size_t(std::min(j + 1, n - 1)) * n + i] - hgt[size_t(j > 0 ? j - 1 : 0) * n + i];
+            const F3 nrm = norm3({-hx, 2.0f * cell, -hz});
+            s.vertices.insert(s.vertices.end(), {p.x, p.y, p.z});
+            s.attributes.push_back({encode_oct(nrm), 0u});
+            lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+            hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+          }
+        }
+        for (uint32_t j = 0; j < j1 - j0; ++j) {
+          for (uint32_t i = 0; i < i1 - i0; ++i) {
+            const uint32_t a = j * w + i, b = a + 1, d = a + w, e = d + 1;
+            s.triangles.push_back(a | (d << 8) | (e << 16));
+            s.triangles.push_back(a | (e << 8) | (b << 16));
+          }
+        }
+        cl.vertex_count = w * (j1 - j0 + 1);
+        cl.triangle_count = 2 * (i1 - i0) * (j1 - j0);
+        const F3 c = (lo + hi) * 0.5f;
+        cl.center[0] = c.x;
+        cl.center[1] = c.y;
+        cl.center[2] = c.z;
+        cl.radius = std::sqrt(dot3(hi - c, hi - c));
+        cl.cone = 0x7f000000u;
+        s.clusters.push_back(cl);
+        s.cluster_materials.push_back(0);
+      }
+    }
+    mesh.cluster_count = uint32_t(s.clusters.size()) - mesh.first_cluster;
+    const uint32_t vend = uint32_t(s.vertices.size() / 3);
+    F3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t v = vstart; v < vend; ++v) {
+      lo = {std::min(lo.x, s.vertices[v * 3]), std::min(lo.y, s.vertices[v * 3 + 1]), std::min(lo.z, s.vertices[v * 3 + 2])};
+      hi = {std::max(hi.x, s.vertices[v * 3]), std::max(hi.y, s.vertices[v * 3 + 1]), std::max(hi.z, s.vertices[v * 3 + 2])};
+    }
+    const float ext = std::max(std::max(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+    const float scale = ext > 0.0f ? ext / 65535.0f : 1.0f;
+    const float inv = 1.0f / scale;
+    mesh.quant[0] = lo.x;
+    mesh.quant[1] = lo.y;
+    mesh.quant[2] = lo.z;
+    mesh.quant[3] = scale;
+    s.quantized.resize(size_t(vend) * 3);
+    for (uint32_t v = vstart; v < vend; ++v) {
+      const float axis[3] = {s.vertices[v * 3] - lo.x, s.vertices[v * 3 + 1] - lo.y, s.vertices[v * 3 + 2] - lo.z};
+      for (int k = 0; k < 3; ++k) {
+        const float grid = std::round(axis[k] * inv);
+        const float clamped = grid < 0.0f ? 0.0f : (grid > 65535.0f ? 65535.0f : grid);
+        s.quantized[size_t(v) * 3 + k] = uint16_t(uint32_t(clamped));
+      }
+    }
+    s.meshes.push_back(mesh);
+    s.materials.push_back(make_material(0.35f, 0.3f, 0.22f, 0.9f, 0.0f));
+    const float identity[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    s.instances.push_back(make_instance(0, identity, {0, 0, 0}, 0, 0));
+  }
+  // The cluster groups: unique 1,024-triangle meshes (8 clusters of 128) standing on the terrain.
+  uint32_t first_pair = s.meshes[0].cluster_count;
+  for (uint32_t g = 0; g < groups; ++g) {
+    Source src;
+    torus(src, rng.uniform(2.0f, 6.0f), rng.uniform(0.6f, 1.8f), 32, 16, rng.uniform(0.05f, 0.4f),
+          int(1 + rng.next() * 5), int(1 + rng.next() * 4));
+    const uint32_t mesh = add_mesh(s, src, 128, std::vector<uint32_t>(src.idx.size() / 3, 0u));
+    float rot[3][3];
+    random_rotation(rng, rot);
+    const float x = rng.uniform(0.02f * extent, 0.98f * extent), z = rng.uniform(0.02f * extent, 0.98f * extent);
+    const uint32_t ci = std::min(uint32_t(x / cell), n - 1), cj = std::min(uint32_t(z / cell), n - 1);
+    const float y = height * fbm(float(ci) / 256.0f, float(cj) / 256.0f, uint32_t(seed) ^ 0x51u) + rng.uniform(2.0f, 9.0f);
+    s.instances.push_back(make_instance(mesh, rot, {x, y, z}, 1 + (g & 63u), first_pair));
+    first_pair += s.meshes[mesh].cluster_count;
+  }
+  for (uint32_t k = 0; k < 64; ++k)
+    s.materials.push_back(make_material(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.2f, 1.0f), 0.0f));
+  set_sun(s, 0.45f, 0.55f, 0.3f, 3.0f);  // a low sun: long shadows
+  s.eye = {0.5f * extent, 0.16f * extent, -0.05f * extent};
+  s.target = {0.5f * extent, 0.0f, 0.55f * extent};
+  s.fov_deg = 55.0f;
+  s.spp = 1;
+  s.max_bounces = 1;
+  s.reach = std::sqrt(2.0f) * extent + 300.0f;
+}
+
+// --- camera (core/math/math.h) ---------------------------------------------------------------
+struct M4 {
+  float c[4][4];  // columns
+  float& at(int r, int col) { return c[col][r]; }
+  float at(int r, int col) const { return c[col][r]; }
+};
+M4 identity4() {
+  M4 m{};
+  for (int i = 0; i < 4; ++i) m.c[i][i] = 1.0f;
+  return m;
+}
+M4 mul(const M4& a, const M4& b) {
+  M4 r{};
+  for (int col = 0; col < 4; ++col)
+    for (int row = 0; row < 4; ++row) {
+      float v = 0;
+      for (int k = 0; k < 4; ++k) v += a.at(row, k) * b.at(k, col);
+      r.at(row, col) = v;
+    }
+  return r;
+}
+M4 look_at(F3 eye, F3 target, F3 up) {
+  const F3 f = norm3(target - eye);
+  const F3 s = norm3(cross3(f, up));
+  const F3 u = cross3(s, f);
+  M4 m = identity4();
+  m.at(0, 0) = s.x;
+  m.at(0, 1) = s.y;
+  m.at(0, 2) = s.z;
+  m.at(1, 0) = u.x;
+  m.at(1, 1) = u.y;
+  m.at(1, 2) = u.z;
+  m.at(2, 0) = -f.x;
+  m.at(2, 1) = -f.y;
+  m.at(2, 2) = -f.z;
+  m.at(0, 3) = -dot3(s, eye);
+  m.at(1, 3) = -dot3(u, eye);
+  m.at(2, 3) = dot3(f, eye);
+  return m;
+}
+M4 perspective_reversed_z(float fov_y, float aspect, float near) {
+  const float t = 1.0f / std::tan(fov_y * 0.5f);
+  M4 m = identity4();
+  m.at(0, 0) = t / aspect;
+  m.at(1, 1) = t;
+  m.at(2, 2) = 0.0f;
+  m.at(2, 3) = near;
+  m.at(3, 2) = -1.0f;
+  m.at(3, 3) = 0.0f;
+  return m;
+}
+M4 inverse4(const M4& m) {  // cofactor expansion (math.h:336-370)
+  const float a00 = m.at(0, 0), a01 = m.at(0, 1), a02 = m.at(0, 2), a03 = m.at(0, 3);
+  const float a10 = m.at(1, 0), a11 = m.at(1, 1), a12 = m.at(1, 2), a13 = m.at(1, 3);
+  const float a20 = m.at(2, 0), a21 = m.at(2, 1), a22 = m.at(2, 2), a23 = m.at(2, 3);
+  const float a30 = m.at(3, 0), a31 = m.at(3, 1), a32 = m.at(3, 2), a33 = m.at(3, 3);
+  const float b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10, b02 = a00 * a13 - a03 * a10;
+  const float b03 = a01 * a12 - a02 * a11, b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+  const float b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30, b08 = a20 * a33 - a23 * a30;
+  const float b09 = a21 * a32 - a22 * a31, b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+  const float det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+  if (det > -1.0e-12f && det < 1.0e-12f) return identity4();
+  const float inv = 1.0f / det;
+  M4 r;
+  r.at(0, 0) = (a11 * b11 - a12 * b10 + a13 * b09) * inv;
+  r.at(0, 1) = (-a01 * b11 + a02 * b10 - a03 * b09) * inv;
+  r.at(0, 2) = (a31 * b05 - a32 * b04 + a33 * b03) * inv;
+  r.at(0, 3) = (-a21 * b05 + a22 * b04 - a23 * b03) * inv;
+  r.at(1, 0) = (-a10 * b11 + a12 * b08 - a13 * b07) * inv;
+  r.at(1, 1) = (a00 * b11 - a02 * b08 + a03 * b07) * inv;
+  r.at(1, 2) = (-a30 * b05 + a32 * b02 - a33 * b01) * inv;
+  r.at(1, 3) = (a20 * b05 - a22 * b02 + a23 * b01) * inv;
+  r.at(2, 0) = (a10 * b10 - a11 * b08 + a13 * b06) * inv;
+  r.at(2, 1) = (-a00 * b10 + a01 * b08 - a03 * b06) * inv;
+  r.at(2, 2) = (a30 * b04 - a31 * b02 + a33 * b00) * inv;
+  r.at(2, 3) = (-a20 * b04 + a21 * b02 - a23 * b00) * inv;
+  r.at(3, 0) = (-a10 * b09 + a11 * b07 - a12 * b06) * inv;
+  r.at(3, 1) = (a00 * b09 - a01 * b07 + a02 * b06) * inv;
+  r.at(3, 2) = (-a30 * b03 + a31 * b01 - a32 * b00) * inv;
+  r.at(3, 3) = (a20 * b03 - a21 * b01 + a22 * b00) * inv;
+  return r;
+}
+void store(const M4& m, float out[16]) {
+  for (int col = 0; col < 4; ++col)
+    for (int row = 0; row < 4; ++row) out[col * 4 + row] = m.at(row, col);
+}
+uint32_t pack_unorm_rgba8(const float c[3]) {  // path_trace.h:135-141
+  auto q = [](float v) {
+    const double clamped = v < 0.0f ? 0.0 : (v > 1.0f ? 1.0 : double(v));
+    return uint32_t(clamped * 255.0 + 0.5);
+  };
+  return q(c[0]) | (q(c[1]) << 8) | (q(c[2]) << 16) | (255u << 24);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t ge_synth_splitmix64(uint64_t* state) { return splitmix64(*state); }
+
+ge_synth* ge_synth_create(uint32_t config, uint64_t seed, uint32_t detail, uint32_t terrain) {
+  auto* s = new ge_synth();
+  s->config = config;
+  switch (config) {
+    case 1: build_config1(*s, seed); break;
+    case 2: build_config2(*s, seed, detail ? detail : 4096u); break;
+    case 3: build_config3(*s, seed, detail ? detail : 16384u, terrain ? std::max(terrain, 2u) : 4096u); break;
+    default: delete s; return nullptr;
+  }
+  if (s->quantized.size() & 1u) s->quantized.push_back(0);  // padded to an even count (cluster.h:436)
+  return s;
+}
+
+ge_synth* ge_synth_from_triangles(const float* positions, uint32_t triangle_count, uint32_t cluster_size,
+                                  int32_t attributes, uint64_t seed) {
+  if (!positions || triangle_count == 0 || cluster_size == 0 || cluster_size > 128) return nullptr;
+  auto* s = new ge_synth();
+  s->config = 0;
+  Source src;
+  for (uint32_t t = 0; t < triangle_count; ++t) {
+    const F3 a{positions[t * 9], positions[t * 9 + 1], positions[t * 9 + 2]};
+    const F3 b{positions[t * 9 + 3], positions[t * 9 + 4], positions[t * 9 + 5]};
+    const F3 c{positions[t * 9 + 6], positions[t * 9 + 7], positions[t * 9 + 8]};
+    const F3 n = norm3(cross3(b - a, c - a));
+    for (const F3& p : {a, b, c}) {
+      src.pos.push_back(p);
+      src.nrm.push_back(n);
+      src.idx.push_back(uint32_t(src.idx.size()));
+    }
+  }
+  std::vector<uint32_t> mats(triangle_count);
+  for (uint32_t t = 0; t < triangle_count; ++t) mats[t] = t / cluster_size;
+  add_mesh(*s, src, cluster_size, mats);
+  s->has_attributes = attributes != 0;
+  if (!s->has_attributes) s->attributes.clear();
+  Rng rng(seed);
+  for (uint32_t c = 0; c < s->clusters.size(); ++c)
+    s->materials.push_back(make_material(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), 1.0f, 0.0f));
+  const float identity[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  s->instances.push_back(make_instance(0, identity, {0, 0, 0}, 0, 0));
+  set_sun(*s, 0.3f, 0.8f, 0.5f, 3.0f);
+  F3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+  for (const F3& p : src.pos) {
+    lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+    hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+  }
+  const F3 c = (lo + hi) * 0.5f;
+  const float r = std::max(1e-3f, std::sqrt(dot3(hi - c, hi - c)));
+  s->target = c;
+  s->eye = c + F3{0.35f, 0.45f, -1.0f} * (2.2f * r);
+  s->reach = std::sqrt(dot3(hi, hi)) + r;
+  if (s->quantized.size() & 1u) s->quantized.push_back(0);
+  return s;
+}
+
+void ge_synth_destroy(ge_synth* s) { delete s; }
+
+void ge_synth_scene(const ge_synth* s, ge_scene_desc* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->clusters = s->clusters.data();
+  out->cluster_count = uint32_t(s->clusters.size());
+  out->vertex_count = uint32_t(s->vertices.size() / 3);
+  out->vertices = s->vertices.data();
+  out->quantized = s->quantized.data();
+  out->attributes = s->has_attributes ? s->attributes.data() : nullptr;
+  out->triangles = s->triangles.data();
+  out->triangle_count = uint32_t(s->triangles.size());
+  out->mesh_count = uint32_t(s->meshes.size());
+  out->meshes = s->meshes.data();
+  out->instances = s->instances.data();
+  out->instance_count = uint32_t(s->instances.size());
+  out->material_count = uint32_t(s->materials.size());
+  out->materials = s->materials.data();
+  out->cluster_materials = s->cluster_materials.data();
+  out->lights = s->lights.empty() ? nullptr : s->lights.data();
+  out->light_count = uint32_t(s->lights.size());
+}
+
+void ge_synth_frame(const ge_synth* s, uint32_t width, uint32_t height, ge_path_trace_params* pt,
+                    ge_ray_visibility_params* rv) {
+  const M4 view = look_at(s->eye, s->target, {0, 1, 0});
+  const M4 proj = perspective_reversed_z(s->fov_deg * 0.017453292519943295f, float(width) / float(height), 0.1f);
+  M4 rotation = view;
+  rotation.at(0, 3) = 0.0f;
+  rotation.at(1, 3) = 0.0f;
+  rotation.at(2, 3) = 0.0f;
+  const M4 clip_to_ray = inverse4(mul(proj, rotation));
+  const M4 view_proj = mul(proj, view);
+  if (pt) {
+    std::memset(pt, 0, sizeof(*pt));
+    store(clip_to_ray, pt->clip_to_ray);
+    pt->camera[0] = s->eye.x;
+    pt->camera[1] = s->eye.y;
+    pt->camera[2] = s->eye.z;
+    std::memcpy(pt->sky, s->sky, 12);
+    pt->sky[3] = 1.0f;
+    std::memcpy(pt->sun, s->sun, 16);
+    std::memcpy(pt->ground, s->ground, 12);
+    pt->width = width;
+    pt->height = height;
+    pt->light_count = uint32_t(s->lights.size());
+    pt->spp = s->spp;
+    pt->sample_base = 0;
+    pt->max_bounces = s->max_bounces;
+    pt->seed = 0;
+    pt->flags = 0;
+    pt->ray_bias = s->reach * (1.0f / 262144.0f);  // k_shadow_bias_relative (lighting.h:141)
+    pt->ray_bias_steps = 1.0f;                     // k_shadow_bias_steps (lighting.h:140)
+    pt->background = pack_unorm_rgba8(s->sky);
+  }
+  if (rv) {
+    std::memset(rv, 0, sizeof(*rv));
+    store(view_proj, rv->view_proj);
+    store(clip_to_ray, rv->clip_to_ray);
+    rv->camera[0] = s->eye.x;
+    rv->camera[1] = s->eye.y;
+    rv->camera[2] = s->eye.z;
+    rv->width = width;
+    rv->height = height;
+    rv->caster_base = 0xffffffffu;
+  }
+}
+
+void ge_synth_set_camera(ge_synth* s, const float eye[3], const float target[3], float fov_y_deg) {
+  s->eye = {eye[0], eye[1], eye[2]};
+  s->target = {target[0], target[1], target[2]};
+  s->fov_deg = fov_y_deg;
+}
+
+void ge_synth_move_instances(ge_synth* s, uint32_t first, uint32_t count, uint64_t seed, float step, float angle,
+                             ge_instance_desc* out) {
+  Rng rng(seed);
+  for (uint32_t i = first; i < first + count && i < s->instances.size(); ++i) {
+    ge_instance_desc& in = s->instances[i];
+    const float a = angle * float(2.0 * rng.next() - 1.0);
+    const float c = std::cos(a), sn = std::sin(a);
+    const float spin[3][3] = {{c, 0, sn}, {0, 1, 0}, {-sn, 0, c}};  // about y
+    float rows[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) rows[r][k] = spin[r][0] * in.rows[0][k] + spin[r][1] * in.rows[1][k] + spin[r][2] * in.rows[2][k];
+    for (int r = 0; r < 3; ++r) {
+      for (int k = 0; k < 3; ++k) in.rows[r][k] = rows[r][k];
+      in.rows[r][3] += step * float(2.0 * rng.next() - 1.0);
+    }
+    out[i - first] = in;
+  }
+}
+
+void ge_synth_primary_rays(const ge_synth* s, uint32_t width, uint32_t height, uint32_t seed, ge_ray* rays) {
+  ge_path_trace_params pt;
+  ge_synth_frame(s, width, height, &pt, nullptr);
+  const float* m = pt.clip_to_ray;
+  for (uint32_t y = 0; y < height; ++y) {
+    for (uint32_t x = 0; x < width; ++x) {
+      const uint32_t h = pcg_hash(x * 73856093u ^ y * 19349663u ^ seed * 83492791u);
+      const float jx = float(h & 0xffffu) * (1.0f / 65536.0f), jy = float(h >> 16) * (1.0f / 65536.0f);
+      const float ndx = (float(x) + jx) / float(width) * 2.0f - 1.0f;
+      const float ndy = 1.0f - (float(y) + jy) / float(height) * 2.0f;
+      float d[3];
+      for (int r = 0; r < 3; ++r) d[r] = ((m[r] * ndx + m[4 + r] * ndy) + m[8 + r] * 0.0f) + m[12 + r];
+      const float inv = 1.0f / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      ge_ray& ray = rays[size_t(y) * width + x];
+      for (int r = 0; r < 3; ++r) {
+        ray.origin[r] = pt.camera[r];
+        ray.dir[r] = d[r] * inv;
+      }
+      ray.tmin = 0.0f;
+      ray.tmax = 1e30f;
+    }
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,424 @@
+/*
+ * ge_rt.h — the C ABI of the MI355X ray-tracing backend.
+ *
+ * This is the drop-in boundary for the renderer's ray-traced hot path: BVH construction over the
+ * cluster-geometry triangles, closest-hit and any-hit traversal, the reference path tracer and
+ * the ray-traced shadow pass. On the engine side these are today Vulkan calls (the driver's
+ * acceleration-structure builds and `RayQuery` traversal) driven by
+ * `systems/renderer/src/reference.cpp` and the resolve's shadow query; here they are plain C entry
+ * points over plain pointers and sizes, no exceptions, `int32_t` status (0 = ok) and a per-context
+ * error string, which is the engine's own `bool + std::string* error` convention
+ * (`systems/renderer/include/systems/renderer/reference.h:111-146`) flattened for a C ABI.
+ *
+ * Every record the GPU reads keeps the engine's byte layout, so the host can hand over the arrays
+ * it already builds for its GPU scene unchanged. The layouts are pinned by the static assertions
+ * at the end of this file, against the engine's size tables
+ * (`domain/gfx/tests/size_table.cpp:41,122,125,388,397,452`, `domain/geometry/tests/size_table.cpp:13`).
+ *
+ * Threading: a context is owned by one host thread; calls on a context are serialized and run on
+ * the context's own HIP stream. Nothing here allocates on a caller's stream.
+ */
+#ifndef GE_RT_H
+#define GE_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------- */
+enum {
+  GE_RT_OK = 0,
+  GE_RT_ERROR_INVALID_ARGUMENT = 1, /* a null pointer, a zero size where one is needed, a bad index */
+  GE_RT_ERROR_INVALID_SCENE = 2,    /* offsets or counts out of range, no BVH built yet */
+  GE_RT_ERROR_OUT_OF_MEMORY = 3,
+  GE_RT_ERROR_DEVICE = 4,           /* a HIP call failed; the text names it */
+  GE_RT_ERROR_UNSUPPORTED = 5       /* a feature flag this backend does not implement */
+};
+
+/* Memory flag: every caller pointer of the call is a device pointer of the context's device.
+ * Without it, caller pointers are host memory and the call copies in and out. */
+#define GE_RT_DEVICE_PTR 0x1u
+/* Trace flag: count internal nodes popped, triangle tests and instance entries into the context's
+ * counters (ge_rt_stats). Off in timed runs. */
+#define GE_RT_COUNT 0x2u
+
+/* ---- the engine's GPU records (byte layouts mirrored, sizes asserted below) ----------------- */
+
+/* geometry::ClusterDesc, 48 bytes (domain/geometry/include/domain/geometry/cluster.h:45-60). */
+typedef struct ge_cluster_desc {
+  uint32_t vertex_offset;   /* first vertex, scene-wide */
+  uint32_t triangle_offset; /* first packed triangle, scene-wide */
+  uint32_t vertex_count;
+  uint32_t triangle_count;
+  float center[3];
+  float radius;
+  float cone_apex[3];
+  uint32_t cone;
+} ge_cluster_desc;
+
+/* geometry::VertexAttributes, 8 bytes (cluster.h:75-78): octahedral normal in two snorm16 and a UV
+ * in two halves. */
+typedef struct ge_vertex_attributes {
+  uint32_t normal_oct;
+  uint32_t uv_half2;
+} ge_vertex_attributes;
+
+/* gfx::MeshDesc, 64 bytes (domain/gfx/shaders/scene.slang:46-55). `quantized` is an address the
+ * backend fills in from the uploaded stream; the deform addresses must be 0 (rigid scenes). */
+typedef struct ge_mesh_desc {
+  float quant[4]; /* xyz grid origin, w grid step */
+  uint64_t quantized;
+  uint32_t first_cluster;
+  uint32_t cluster_count;
+  uint64_t deform_pool;
+  uint64_t deform_slots;
+  uint64_t templates;
+  uint64_t skin;
+} ge_mesh_desc;
+
+/* gfx::InstanceDesc, 96 bytes (scene.slang:61-80). rows = mesh space to the cell's frame,
+ * (linear row r | translation r). */
+typedef struct ge_instance_desc {
+  float rows[3][4];
+  int32_t cell[3];
+  uint32_t mesh;
+  uint32_t material_base;
+  uint32_t first_pair;
+  float scale_max;
+  uint32_t flags;
+  uint32_t deform;
+  float bounds_padding;
+  uint32_t terrain;
+  uint32_t band;
+} ge_instance_desc;
+
+/* InstanceDesc::flags bit 7 (scene.slang:694): the instance traces its mesh's own structure and a
+ * hit's geometry index is the cluster within the mesh. Every instance this backend builds is one. */
+#define GE_INSTANCE_STATIC_RT 128u
+#define GE_INVALID_DEFORM 0xffffffffu
+
+/* gfx::FrameEye, 48 bytes (scene.slang:96-104): the frame's origin. All zero is the world's. */
+typedef struct ge_frame_eye {
+  int32_t cell[3];
+  uint32_t masks;
+  float local[3];
+  float pad1;
+  float residual[3];
+  float pad2;
+} ge_frame_eye;
+
+/* gfx::ResolveMaterial, 128 bytes (domain/gfx/shaders/material.slang:71-90). */
+typedef struct ge_resolve_material {
+  float albedo[4];   /* rgb linear, w roughness */
+  float emissive[4]; /* rgb linear factor, w metallic */
+  uint32_t albedo_texture;
+  uint32_t sampler;
+  float uv_scale;
+  uint32_t flags;
+  uint32_t metallic_roughness_texture;
+  uint32_t normal_texture;
+  float normal_scale;
+  uint32_t occlusion_texture;
+  uint32_t emissive_texture;
+  float occlusion_strength;
+  uint32_t samplers_mr_normal;
+  uint32_t samplers_occlusion_emissive;
+  float uv_transform[4];
+  float uv_offset[4];
+  float specular[4]; /* rgb F0 - 0.04, w 1 - F90; zero: the 4 % and 1 */
+} ge_resolve_material;
+#define GE_NO_TEXTURE 0xffffffffu
+
+/* gfx::ResolveLight, 64 bytes (material.slang:150-157). */
+typedef struct ge_resolve_light {
+  float position_radius[4]; /* xyz position (directional: towards the light), w radius */
+  float color_intensity[4];
+  float direction_type[4]; /* xyz spot direction, w 0 point, 1 spot, 2 directional */
+  float spot[4];           /* x cos inner, y cos outer, z emitter radius */
+} ge_resolve_light;
+
+/* gfx::PathTraceParams, 368 bytes (domain/gfx/include/domain/gfx/path_trace.h:51-131 and
+ * domain/gfx/shaders/path_trace.slang:55-98). The host fills it exactly as it fills the engine's
+ * block; the scene addresses (clusters .. lights, hit_entries, extra) are written by the backend
+ * from the scene handle at dispatch, and `accum`, `output`, `coverage` are the caller's device
+ * pointers when GE_RT_DEVICE_PTR is given and ignored otherwise. */
+typedef struct ge_path_trace_params {
+  float clip_to_ray[16]; /* column-major: element (row r, col c) at [c * 4 + r] */
+  float camera[4];
+  float sky[4];
+  float sun[4]; /* xyz normalized direction towards the light, w intensity */
+  float ground[4];
+  uint64_t accum;
+  uint64_t output;
+  uint64_t clusters;
+  uint64_t mesh;
+  uint64_t triangles;
+  uint64_t instances;
+  uint64_t attributes;
+  uint64_t materials;
+  uint64_t cluster_materials;
+  uint64_t visible;
+  uint64_t lights;
+  uint32_t width;
+  uint32_t height;
+  uint32_t light_count;
+  uint32_t scene;
+  uint32_t spp;
+  uint32_t sample_base;
+  uint32_t max_bounces;
+  uint32_t seed;
+  uint32_t flags;
+  float ray_bias;
+  uint64_t coverage;
+  uint32_t background;
+  float ray_bias_steps;
+  uint64_t ground_detail;
+  uint64_t sky_params;
+  ge_frame_eye eye;
+  int32_t terrain_uv_x0_mm;
+  int32_t terrain_uv_z0_mm;
+  float terrain_uv_per_mm;
+  uint32_t extra_vertices;
+  uint64_t hit_entries;
+  uint64_t extra;
+} ge_path_trace_params;
+
+/* PathTraceParams::flags (path_trace.h:35-48). */
+#define GE_PT_UNIFORM_SKY 1u
+#define GE_PT_PIXEL_CENTER 2u
+#define GE_PT_DITHER 4u
+#define GE_PT_ALPHA_MASKS 8u /* unsupported: returns GE_RT_ERROR_UNSUPPORTED */
+#define GE_PT_ROULETTE_DEPTH 3u
+
+/* gfx::RayVisibilityParams, 256 bytes (domain/gfx/include/domain/gfx/ray_visibility.h:39-61,
+ * ray_visibility.slang:39-57). `output` is the caller's device pointer under GE_RT_DEVICE_PTR. */
+typedef struct ge_ray_visibility_params {
+  float view_proj[16];   /* column-major */
+  float clip_to_ray[16]; /* column-major */
+  float camera[4];
+  uint64_t output;
+  uint64_t instance_base;
+  uint32_t width;
+  uint32_t height;
+  uint32_t scene;
+  uint32_t caster_base;
+  uint64_t visible;
+  uint64_t instances;
+  uint64_t meshes;
+  uint64_t triangles;
+  uint64_t clusters;
+  uint64_t hit_entries;
+  uint64_t masks[4]; /* MaskTables (cluster_cull.h:154-167); must be 0: no alpha-tested material */
+} ge_ray_visibility_params;
+
+/* ---- the backend's own records --------------------------------------------------------------- */
+
+/* One ray of the test and bench surface: the fields of a RayDesc (origin, TMin, direction, TMax).
+ * 32 bytes. Hits are accepted for tmin <= t <= tmax, as a Vulkan ray query accepts them. */
+typedef struct ge_ray {
+  float origin[3];
+  float tmin;
+  float dir[3];
+  float tmax;
+} ge_ray;
+
+/* A committed closest hit: what `CommittedRayT`, `CommittedInstanceID`, `CommittedGeometryIndex`,
+ * `CommittedPrimitiveIndex` and `CommittedTriangleBarycentrics` return, plus the visibility id
+ * `pair << 8 | triangle` (ray_visibility.slang:88-93). `instance_id` is ~0 on a miss. 32 bytes. */
+typedef struct ge_hit {
+  float t;
+  float u; /* barycentric weight of the triangle's second corner */
+  float v; /* barycentric weight of the third corner */
+  uint32_t instance_id;
+  uint32_t geometry;  /* cluster index within the instance's mesh */
+  uint32_t primitive; /* triangle index within the cluster */
+  uint32_t key;       /* pair << 8 | triangle */
+  uint32_t pad;
+} ge_hit;
+#define GE_RT_MISS 0xffffffffu
+
+/* The shadow pass's frame: one sun ray per receiver, as the resolve's shadow_visibility and
+ * shadow_ray_blocked ask it (domain/gfx/shaders/visibility_resolve.slang:516-605,742-755). */
+typedef struct ge_shadow_params {
+  float sun[4];       /* xyz normalized direction towards the light */
+  float offset;       /* world units along the geometric normal (ray_offset's floor) */
+  float offset_steps; /* steps of the receiver's own grid (ray_offset) */
+  float t_max;        /* 1e30 for the sun */
+  uint32_t count;     /* receivers */
+  uint32_t pad[4];
+} ge_shadow_params;
+
+/* One receiver of the shadow pass: a surface point in the frame's space, the geometric normal
+ * facing the eye, and the instance whose grid sets the offset. 32 bytes. */
+typedef struct ge_shadow_receiver {
+  float position[3];
+  uint32_t instance_id;
+  float geometric[3];
+  uint32_t pad;
+} ge_shadow_receiver;
+
+/* Traversal counters (GE_RT_COUNT) and the last dispatch's kernel time. */
+typedef struct ge_rt_counters {
+  uint64_t closest_rays;   /* closest-hit queries issued */
+  uint64_t any_rays;       /* any-hit queries issued */
+  uint64_t nodes;          /* internal BVH nodes popped (TLAS + BLAS) */
+  uint64_t triangles;      /* ray/triangle tests */
+  uint64_t instances;      /* TLAS leaves entered (ray transformed into a BLAS) */
+  double last_kernel_ms;   /* the last traced dispatch's device time, hipEvent on the ctx stream */
+  double last_build_ms;    /* the last BVH build's device time */
+} ge_rt_counters;
+
+/* The built structure, read back for topology parity. Node and triangle records are the backend's
+ * 64-byte and 48-byte formats. */
+typedef struct ge_bvh_node {
+  float lo0[3];
+  float hi0[3];
+  float lo1[3];
+  float hi1[3];
+  uint32_t child[2]; /* bit 31 set: a leaf, the rest its primitive slot; else an internal node */
+  uint32_t parent;   /* ~0 at the root */
+  uint32_t count;    /* primitives below this node */
+} ge_bvh_node;
+#define GE_BVH_LEAF 0x80000000u
+#define GE_BVH_NONE 0xffffffffu
+
+typedef struct ge_bvh_triangle {
+  float v0[3];
+  float v1[3];
+  float v2[3];
+  uint32_t prim; /* cluster-within-mesh << 8 | triangle-within-cluster */
+  uint32_t index; /* the triangle's index within its mesh (sort order's tie break) */
+  uint32_t pad;
+} ge_bvh_triangle;
+
+/* The scene, as the engine's GpuScene holds it (systems/renderer/include/systems/renderer/gpu_scene.h
+ * :171-191): global arrays with the offsets baked into each cluster. Copied in from host memory. */
+typedef struct ge_scene_desc {
+  const ge_cluster_desc* clusters;
+  uint32_t cluster_count;
+  uint32_t vertex_count;
+  const float* vertices;     /* fp32 xyz per vertex, mesh space, what the AS builds read */
+  const uint16_t* quantized; /* 3 per vertex, padded to an even count: the shading grid */
+  const ge_vertex_attributes* attributes; /* per vertex, or NULL for flat shading */
+  const uint32_t* triangles; /* packed i0 | i1 << 8 | i2 << 16 */
+  uint32_t triangle_count;
+  uint32_t mesh_count;
+  const ge_mesh_desc* meshes;
+  const ge_instance_desc* instances;
+  uint32_t instance_count;
+  uint32_t material_count;
+  const ge_resolve_material* materials;
+  const uint32_t* cluster_materials; /* per cluster */
+  const ge_resolve_light* lights;
+  uint32_t light_count;
+  uint32_t pad;
+} ge_scene_desc;
+
+typedef struct ge_rt_ctx ge_rt_ctx;
+typedef struct ge_rt_scene ge_rt_scene;
+
+/* ---- entry points -------------------------------------------------------------------------- */
+
+/* A context on HIP device `device_ordinal`, with its own stream. Replaces the engine's device and
+ * pipeline creation in ReferenceRenderer::create (systems/renderer/src/reference.cpp:81-157). */
+int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal);
+/* Idempotent; null is fine (ReferenceRenderer::destroy, reference.h:132). */
+void ge_rt_destroy(ge_rt_ctx* ctx);
+/* The last error's text, NUL-terminated and cut to `size`. Returns the status it described. */
+int32_t ge_rt_last_error(const ge_rt_ctx* ctx, char* buffer, size_t size);
+/* Waits for the context's stream. */
+int32_t ge_rt_synchronize(ge_rt_ctx* ctx);
+
+/* Uploads a scene (the GpuScene buffers: clusters, meshes, triangles, instances, attributes,
+ * materials, cluster_materials, lights; gpu_scene.h:1693-1804). Validates offsets and counts. */
+int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* desc, ge_rt_scene** out);
+void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene);
+
+/* Builds the structures: one LBVH per mesh over its clusters' triangles (the engine's BLAS with one
+ * geometry per cluster: create_blas/build_blas, domain/gfx/include/domain/gfx/acceleration.h:46-110,
+ * domain/gfx/src/acceleration.cpp:134-169) and one LBVH over the instances in the frame of `eye`
+ * (create_tlas/write_instances/build_tlas, acceleration.cpp:196-225, with the eye-relative 3x4 of
+ * domain/gfx/shaders/tlas_references.slang:75-104). `eye` may be NULL (the world's origin).
+ * flags: GE_RT_BUILD_FULL builds everything; GE_RT_BUILD_SUBSET rebuilds the meshes named in
+ * `meshes[0..count)` and then the top level; GE_RT_BUILD_TOP rebuilds the top level only (for
+ * instances that moved: re-upload them with ge_rt_scene_update_instances first). */
+#define GE_RT_BUILD_FULL 0u
+#define GE_RT_BUILD_SUBSET 1u
+#define GE_RT_BUILD_TOP 2u
+int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, const ge_frame_eye* eye,
+                        const uint32_t* meshes, uint32_t count);
+/* Replaces instance records [first, first + count) (the per-frame placement the engine's
+ * tlas_references pass writes); follow with ge_rt_bvh_build(GE_RT_BUILD_TOP). */
+int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first,
+                                     uint32_t count, const ge_instance_desc* instances);
+/* Sizes of the built structure of mesh `mesh` (or of the top level for mesh == ~0). */
+int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, uint32_t* node_count,
+                       uint32_t* prim_count);
+/* Copies the built structure out to host memory: nodes (node_count), sorted primitive slots
+ * (prim_count: triangles for a mesh, instance ids for the top level), 30-bit Morton codes in sorted
+ * order (prim_count). Any output may be NULL. */
+int32_t ge_rt_bvh_download(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh,
+                           ge_bvh_node* nodes, ge_bvh_triangle* triangles, uint32_t* instance_slots,
+                           uint32_t* morton);
+
+/* Closest-hit queries (RayQuery<RAY_FLAG_NONE> committed hit, path_trace.slang:268-298; the tie
+ * between hits at one t goes to the larger pair << 8 | triangle, ray_visibility.slang:13-30).
+ * `mask` is the ray's 8-bit instance mask (0xfe camera, 0x01 caster). */
+int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* rays,
+                            uint32_t count, uint32_t mask, ge_hit* hits, uint32_t flags);
+/* Any-hit queries (RayQuery<FORCE_OPAQUE | ACCEPT_FIRST_HIT_AND_END_SEARCH>, path_trace.slang:180-207,
+ * visibility_resolve.slang:599-603): bit i of occluded[i / 32] is set when anything lies in
+ * [tmin, tmax]. A ray with tmax <= 0 is never occluded (occluded(), path_trace.slang:181). */
+int32_t ge_rt_trace_any(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* rays, uint32_t count,
+                        uint32_t mask, uint32_t* occluded, uint32_t flags);
+
+/* The traced primary visibility pass (trace_main, domain/gfx/shaders/ray_visibility.slang:106-217):
+ * one ray per pixel centre, the 64-bit word asuint(depth) << 32 | pair << 8 | triangle, 0 empty.
+ * `words` receives width * height words (host memory, or device under GE_RT_DEVICE_PTR). */
+int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene,
+                             const ge_ray_visibility_params* params, uint64_t* words, uint32_t flags);
+
+/* One batch of the reference path tracer (path_trace_main, path_trace.slang:463-495), replacing one
+ * iteration of ReferenceRenderer::render's batch loop (reference.cpp:323-347). Adds params->spp
+ * samples numbered from params->sample_base into accum (float4 per pixel: rgb sum, w count) and
+ * ORs the primary coverage into coverage (u32 per pixel). With GE_RT_DEVICE_PTR the block's own
+ * accum/coverage addresses are used and the pointers given here are ignored. */
+int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
+                         float* accum, uint32_t* coverage, uint32_t flags);
+/* The tonemap (tonemap_main, path_trace.slang:511-543): accum + coverage -> packed RGBA8. */
+int32_t ge_rt_tonemap(ge_rt_ctx* ctx, const ge_path_trace_params* params, const float* accum,
+                      const uint32_t* coverage, uint32_t* output, uint32_t flags);
+
+/* The ray-traced sun shadow pass: one any-hit ray per receiver, origin = position + geometric *
+ * ray_offset (shadow_visibility, visibility_resolve.slang:742-755), bit i of blocked[i / 32] set
+ * when the sun is blocked (shadow_ray_blocked, :516-605). */
+int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_params* params,
+                          const ge_shadow_receiver* receivers, uint32_t* blocked, uint32_t flags);
+
+/* Counters and the last kernel/build times (ReferenceFrame::trace_ms, reference.h:95). */
+int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out);
+int32_t ge_rt_reset_stats(ge_rt_ctx* ctx);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+static_assert(sizeof(ge_cluster_desc) == 48, "geometry::ClusterDesc is 48 bytes");
+static_assert(sizeof(ge_vertex_attributes) == 8, "geometry::VertexAttributes is 8 bytes");
+static_assert(sizeof(ge_mesh_desc) == 64, "gfx::MeshDesc is 64 bytes");
+static_assert(sizeof(ge_instance_desc) == 96, "gfx::InstanceDesc is 96 bytes");
+static_assert(sizeof(ge_frame_eye) == 48, "gfx::FrameEye is 48 bytes");
+static_assert(sizeof(ge_resolve_material) == 128, "gfx::ResolveMaterial is 128 bytes");
+static_assert(sizeof(ge_resolve_light) == 64, "gfx::ResolveLight is 64 bytes");
+static_assert(sizeof(ge_path_trace_params) == 368, "gfx::PathTraceParams is 368 bytes");
+static_assert(sizeof(ge_ray_visibility_params) == 256, "gfx::RayVisibilityParams is 256 bytes");
+static_assert(sizeof(ge_ray) == 32, "ge_ray is 32 bytes");
+static_assert(sizeof(ge_hit) == 32, "ge_hit is 32 bytes");
+static_assert(sizeof(ge_bvh_node) == 64, "a BVH node is one 64-byte fetch");
+static_assert(sizeof(ge_bvh_triangle) == 48, "a BVH triangle is 48 bytes");
+static_assert(sizeof(ge_shadow_receiver) == 32, "a shadow receiver is 32 bytes");
+#endif
+
+#endif /* GE_RT_H */

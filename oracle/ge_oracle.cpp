@@ -1,0 +1,1521 @@
+// ge_oracle.cpp — the CPU oracle. TEST INFRASTRUCTURE ONLY: the checker the HIP backend is
+// compared against and the CPU baseline the bench times beside it; the product never loads it.
+//
+// A scalar C++ restatement of the reference path tracer (domain/gfx/shaders/path_trace.slang and
+// the files it includes), with a software LBVH standing in for the Vulkan driver's acceleration
+// structures. Every floating-point expression is written in the operation order of the Slang
+// source it restates, and the file is compiled with -ffp-contract=off, the engine's own CPU policy
+// (cmake/EngineFpContraction.cmake:5-53, docs/adr/0035-no-floating-point-contraction.md).
+//
+// Parity pins for each part are in tests/ (closed forms, invariants, brute force, the reference's
+// own sampler identities and furnace numbers); the header of each section names its pin.
+#include "ge_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <type_traits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---- counters --------------------------------------------------------------------------------
+std::atomic<uint64_t> g_counters[5];  // closest, any, nodes, triangles, instances
+struct Tally {
+  uint64_t closest = 0, any = 0, nodes = 0, tris = 0, inst = 0;
+  void flush() {
+    g_counters[0] += closest;
+    g_counters[1] += any;
+    g_counters[2] += nodes;
+    g_counters[3] += tris;
+    g_counters[4] += inst;
+    closest = any = nodes = tris = inst = 0;
+  }
+};
+
+// ---- constants -------------------------------------------------------------------------------
+// A literal of the shader as the arithmetic mode reads it: the float literal in FP32 mode, the
+// decimal in double in FP64 mode.
+template <class R>
+constexpr R K(float f, double d) {
+  if constexpr (std::is_same_v<R, float>) {
+    (void)d;
+    return f;
+  } else {
+    (void)f;
+    return d;
+  }
+}
+#define KC(x) K<R>(x##f, x)
+
+template <class R>
+struct V3 {
+  R x, y, z;
+};
+template <class R> inline V3<R> v3(R x, R y, R z) { return {x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline R rsqrt_(R x) { return R(1) / std::sqrt(x); }
+template <class R> inline V3<R> normalize(V3<R> v) { return v * (R(1) / std::sqrt(dot(v, v))); }
+template <class R> inline R clampr(R v, R lo, R hi) { return std::min(std::max(v, lo), hi); }
+template <class R> inline R maxr(R a, R b) { return a > b ? a : b; }  // HLSL max on non-NaN operands
+template <class R> inline R minr(R a, R b) { return a < b ? a : b; }
+template <class R> inline R lerp_(R a, R b, R t) { return a + (b - a) * t; }
+template <class R> inline V3<R> lerp3(V3<R> a, V3<R> b, R t) {
+  return {lerp_(a.x, b.x, t), lerp_(a.y, b.y, t), lerp_(a.z, b.z, t)};
+}
+template <class R> inline V3<R> maxv(V3<R> a, R s) { return {maxr(a.x, s), maxr(a.y, s), maxr(a.z, s)}; }
+template <class R> inline V3<R> splat(R s) { return {s, s, s}; }
+
+template <class R> inline V3<R> load3(const float* p) { return {R(p[0]), R(p[1]), R(p[2])}; }
+
+// ---- sin and cos ---------------------------------------------------------------------------------
+// FP32 mode: the backend's own sincos, restated. Both sides reduce by pi/2 and evaluate the Taylor
+// series of sin and cos on [-pi/4, pi/4] in double, then round once to float, so the result is the
+// correctly rounded value except within ~1e-10 relative of a rounding boundary (pinned against
+// libm's double sin/cos in tests/test_oracle_math.py). FP64 mode: libm.
+inline void sincos_f32(float x, float* s, float* c) {
+  const double k_half_pi = 1.5707963267948966;
+  const double xd = double(x);
+  const double q = std::nearbyint(xd * 0.6366197723675814);
+  const double r = xd - q * k_half_pi;
+  const double r2 = r * r;
+  const double sp =
+      r + r * r2 * (-1.0 / 6.0 + r2 * (1.0 / 120.0 + r2 * (-1.0 / 5040.0 + r2 * (1.0 / 362880.0 +
+                                                                                  r2 * (-1.0 / 39916800.0)))));
+  const double cp =
+      1.0 + r2 * (-0.5 + r2 * (1.0 / 24.0 + r2 * (-1.0 / 720.0 + r2 * (1.0 / 40320.0 +
+                                                                         r2 * (-1.0 / 3628800.0 + r2 * (1.0 / 479001600.0))))));
+  const int quadrant = int(int64_t(q) & 3);
+  double sv, cv;
+  switch (quadrant) {
+    case 0: sv = sp; cv = cp; break;
+    case 1: sv = cp; cv = -sp; break;
+    case 2: sv = -sp; cv = -cp; break;
+    default: sv = -cp; cv = sp; break;
+  }
+  *s = float(sv);
+  *c = float(cv);
+}
+template <class R> inline void sincos_r(R x, R* s, R* c) {
+  if constexpr (std::is_same_v<R, float>) {
+    sincos_f32(x, s, c);
+  } else {
+    *s = std::sin(x);
+    *c = std::cos(x);
+  }
+}
+
+// ---- random numbers (sampling.slang:27-51) -----------------------------------------------------
+inline uint32_t pcg_hash(uint32_t v) {
+  const uint32_t state = v * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+inline uint32_t rng_seed(uint32_t px, uint32_t py, uint32_t sample_index, uint32_t seed) {
+  const uint32_t mixed = pcg_hash(px * 73856093u ^ py * 19349663u ^ seed * 83492791u);
+  return pcg_hash(mixed + sample_index * 2654435761u);
+}
+template <class R> inline R rng_next(uint32_t& state) {
+  state = state * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  word = (word >> 22u) ^ word;
+  if constexpr (std::is_same_v<R, float>) {
+    return float(word) * (1.0f / 4294967296.0f);
+  } else {
+    // The shader's float draw, held in double: the FP64 mode follows the same stream.
+    return double(float(word) * (1.0f / 4294967296.0f));
+  }
+}
+
+// ---- BVH build: LBVH (Karras 2012) -------------------------------------------------------------
+// Pins: tests/test_oracle_bvh.py — Morton codes against a bit-interleave written from the
+// definition, the sorted order against numpy's stable argsort, every internal node's key range
+// contiguous with the split at the highest differing bit, every leaf reached once, parents
+// containing children, and traversal equal to brute force.
+struct Box {
+  float lo[3], hi[3];
+};
+
+inline uint32_t expand10(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+inline uint32_t quantize10(float c, float lo, float scale) {
+  const float q = std::fmin(std::fmax((c - lo) * scale, 0.0f), 1023.0f);
+  return uint32_t(q);
+}
+inline uint32_t morton3(const float c[3], const float lo[3], const float scale[3]) {
+  return (expand10(quantize10(c[0], lo[0], scale[0])) << 2) |
+         (expand10(quantize10(c[1], lo[1], scale[1])) << 1) | expand10(quantize10(c[2], lo[2], scale[2]));
+}
+
+struct Lbvh {
+  std::vector<ge_bvh_node> nodes;
+  std::vector<uint32_t> order;   // sorted slot -> primitive
+  std::vector<uint32_t> morton;  // sorted codes
+  Box root{};
+  uint32_t n = 0;
+};
+
+inline int delta(const std::vector<uint32_t>& codes, int n, int i, int j) {
+  if (j < 0 || j > n - 1) return -1;
+  const uint32_t a = codes[size_t(i)], b = codes[size_t(j)];
+  if (a != b) return __builtin_clz(a ^ b);
+  return 32 + __builtin_clz(uint32_t(i) ^ uint32_t(j));
+}
+
+void lbvh_build(const std::vector<Box>& boxes, Lbvh& out) {
+  const uint32_t n = uint32_t(boxes.size());
+  out = Lbvh{};
+  out.n = n;
+  if (n == 0) return;
+  // Centroids and their bounds.
+  std::vector<float> cen(size_t(n) * 3);
+  float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      const float c = (boxes[i].lo[a] + boxes[i].hi[a]) * 0.5f;
+      cen[size_t(i) * 3 + a] = c;
+      clo[a] = std::fmin(clo[a], c);
+      chi[a] = std::fmax(chi[a], c);
+    }
+  }
+  float scale[3];
+  for (int a = 0; a < 3; ++a) {
+    const float ext = chi[a] - clo[a];
+    scale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+  }
+  std::vector<uint32_t> codes(n);
+  for (uint32_t i = 0; i < n; ++i) codes[i] = morton3(&cen[size_t(i) * 3], clo, scale);
+  // Stable sort by code: equal codes keep primitive order.
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return codes[a] < codes[b]; });
+  std::vector<uint32_t> sorted(n);
+  for (uint32_t i = 0; i < n; ++i) sorted[i] = codes[order[i]];
+  out.order = order;
+  out.morton = sorted;
+
+  auto leaf_box = [&](uint32_t slot) { return boxes[order[slot]]; };
+  if (n == 1) {
+    ge_bvh_node node{};
+    const Box b = leaf_box(0);
+    std::memcpy(node.lo0, b.lo, 12);
+    std::memcpy(node.hi0, b.hi, 12);
+    for (int a = 0; a < 3; ++a) {
+      node.lo1[a] = INFINITY;
+      node.hi1[a] = -INFINITY;
+    }
+    node.child[0] = GE_BVH_LEAF | 0u;
+    node.child[1] = GE_BVH_NONE;
+    node.parent = GE_BVH_NONE;
+    node.count = 1;
+    out.nodes.push_back(node);
+    out.root = b;
+    return;
+  }
+  // Karras 2012: internal node i covers [min(i,j), max(i,j)], split at gamma.
+  const int ni = int(n);
+  out.nodes.assign(n - 1, ge_bvh_node{});
+  out.nodes[0].parent = GE_BVH_NONE;
+  for (int i = 0; i < ni - 1; ++i) {
+    const int d = (delta(sorted, ni, i, i + 1) - delta(sorted, ni, i, i - 1)) > 0 ? 1 : -1;
+    const int dmin = delta(sorted, ni, i, i - d);
+    int lmax = 2;
+    while (delta(sorted, ni, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+      if (delta(sorted, ni, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(sorted, ni, i, j);
+    int s = 0;
+    int t = l;
+    do {
+      t = (t + 1) >> 1;
+      if (delta(sorted, ni, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + std::min(d, 0);
+    const int first = std::min(i, j), last = std::max(i, j);
+    ge_bvh_node& node = out.nodes[size_t(i)];
+    node.child[0] = (first == gamma) ? (GE_BVH_LEAF | uint32_t(gamma)) : uint32_t(gamma);
+    node.child[1] = (last == gamma + 1) ? (GE_BVH_LEAF | uint32_t(gamma + 1)) : uint32_t(gamma + 1);
+    node.count = uint32_t(last - first + 1);
+    if (!(node.child[0] & GE_BVH_LEAF)) out.nodes[size_t(gamma)].parent = uint32_t(i);
+    if (!(node.child[1] & GE_BVH_LEAF)) out.nodes[size_t(gamma + 1)].parent = uint32_t(i);
+  }
+  // Bottom-up fit, written as a post-order walk; min and max are exact, so any order is the same.
+  std::vector<Box> nb(n - 1);
+  std::vector<uint8_t> done(n - 1, 0);
+  std::vector<uint32_t> stack;
+  stack.push_back(0);
+  while (!stack.empty()) {
+    const uint32_t i = stack.back();
+    ge_bvh_node& node = out.nodes[i];
+    bool ready = true;
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t ch = node.child[c];
+      if (!(ch & GE_BVH_LEAF) && !done[ch]) {
+        stack.push_back(ch);
+        ready = false;
+      }
+    }
+    if (!ready) continue;
+    stack.pop_back();
+    Box cb[2];
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t ch = node.child[c];
+      cb[c] = (ch & GE_BVH_LEAF) ? leaf_box(ch & ~GE_BVH_LEAF) : nb[ch];
+    }
+    std::memcpy(node.lo0, cb[0].lo, 12);
+    std::memcpy(node.hi0, cb[0].hi, 12);
+    std::memcpy(node.lo1, cb[1].lo, 12);
+    std::memcpy(node.hi1, cb[1].hi, 12);
+    for (int a = 0; a < 3; ++a) {
+      nb[i].lo[a] = std::fmin(cb[0].lo[a], cb[1].lo[a]);
+      nb[i].hi[a] = std::fmax(cb[0].hi[a], cb[1].hi[a]);
+    }
+    done[i] = 1;
+  }
+  out.root = nb[0];
+}
+
+// ---- the scene ---------------------------------------------------------------------------------
+struct InstRec {
+  float w2o[3][4];  // frame space to mesh space
+  uint32_t mesh;
+  uint32_t first_pair;
+  bool live;
+};
+
+struct Blas {
+  Lbvh bvh;
+  std::vector<ge_bvh_triangle> tris;  // in sorted order
+};
+
+}  // namespace
+
+struct ge_oracle_scene {
+  std::vector<ge_cluster_desc> clusters;
+  std::vector<float> vertices;
+  std::vector<uint16_t> quantized;
+  std::vector<ge_vertex_attributes> attributes;
+  bool has_attributes = false;
+  std::vector<uint32_t> triangles;
+  std::vector<ge_mesh_desc> meshes;
+  std::vector<ge_instance_desc> instances;
+  std::vector<ge_resolve_material> materials;
+  std::vector<uint32_t> cluster_materials;
+  std::vector<ge_resolve_light> lights;
+  std::vector<Blas> blas;
+  std::vector<InstRec> inst;
+  Lbvh tlas;
+  std::vector<uint32_t> tlas_slots;  // sorted slot -> instance id
+};
+
+namespace {
+
+// engine::relative_axis (core/math/world.h:181-195; scene.slang:120-126), operation for operation.
+inline float relative_axis(int cell, float local, int eye_cell, float eye_local, float eye_residual) {
+  const float cells = float(cell - eye_cell) * 64.0f;
+  const float from_local = cells > 0.0f ? (cells - eye_local) + local : (cells + local) - eye_local;
+  return from_local - eye_residual;
+}
+inline void instance_from_eye(const ge_instance_desc& inst, const ge_frame_eye& eye, float out[3]) {
+  for (int r = 0; r < 3; ++r)
+    out[r] = relative_axis(inst.cell[r], inst.rows[r][3], eye.cell[r], eye.local[r], eye.residual[r]);
+}
+
+// The instance's frame-to-mesh 3x4: the inverse of the linear part by its adjugate, in double,
+// rounded once to float; the translation is -inverse * from_eye, in double. (The backend builds
+// the same record on its host with the same double arithmetic.)
+void make_w2o(const ge_instance_desc& inst, const float T[3], float w2o[3][4]) {
+  double m[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[r][c] = double(inst.rows[r][c]);
+  const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+  const double inv_det = det != 0.0 ? 1.0 / det : 0.0;
+  double inv[3][3];
+  inv[0][0] = c00 * inv_det;
+  inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv_det;
+  inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv_det;
+  inv[1][0] = c01 * inv_det;
+  inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv_det;
+  inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv_det;
+  inv[2][0] = c02 * inv_det;
+  inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv_det;
+  inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv_det;
+  for (int r = 0; r < 3; ++r) {
+    const double t = -(inv[r][0] * double(T[0]) + inv[r][1] * double(T[1]) + inv[r][2] * double(T[2]));
+    for (int c = 0; c < 3; ++c) w2o[r][c] = float(inv[r][c]);
+    w2o[r][3] = float(t);
+  }
+}
+
+// The instance's box in the frame: the 8 corners of its mesh's root box through the 3x4 (linear
+// part as `instance_direction`, scene.slang:180-183, then the translation), padded by 2^-20 of its
+// largest coordinate so that a ray transformed into the mesh's space never finds a triangle the
+// box missed by rounding.
+Box instance_box(const ge_instance_desc& inst, const float T[3], const Box& b) {
+  Box r;
+  for (int a = 0; a < 3; ++a) {
+    r.lo[a] = INFINITY;
+    r.hi[a] = -INFINITY;
+  }
+  for (int i = 0; i < 8; ++i) {
+    const float p[3] = {(i & 1) ? b.hi[0] : b.lo[0], (i & 2) ? b.hi[1] : b.lo[1], (i & 4) ? b.hi[2] : b.lo[2]};
+    for (int a = 0; a < 3; ++a) {
+      const float v = (inst.rows[a][0] * p[0] + inst.rows[a][1] * p[1] + inst.rows[a][2] * p[2]) + T[a];
+      r.lo[a] = std::fmin(r.lo[a], v);
+      r.hi[a] = std::fmax(r.hi[a], v);
+    }
+  }
+  float m = 0.0f;
+  for (int a = 0; a < 3; ++a) m = std::fmax(m, std::fmax(std::fabs(r.lo[a]), std::fabs(r.hi[a])));
+  const float pad = m * 9.5367431640625e-7f + 1e-30f;
+  for (int a = 0; a < 3; ++a) {
+    r.lo[a] = r.lo[a] - pad;
+    r.hi[a] = r.hi[a] + pad;
+  }
+  return r;
+}
+
+void build_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
+  const uint32_t ni = uint32_t(s.instances.size());
+  s.inst.assign(ni, InstRec{});
+  std::vector<Box> boxes;
+  std::vector<uint32_t> ids;
+  for (uint32_t i = 0; i < ni; ++i) {
+    const ge_instance_desc& in = s.instances[i];
+    float T[3];
+    instance_from_eye(in, eye, T);
+    InstRec& rec = s.inst[i];
+    make_w2o(in, T, rec.w2o);
+    rec.mesh = in.mesh;
+    rec.first_pair = in.first_pair;
+    rec.live = s.blas[in.mesh].bvh.n > 0;
+    if (!rec.live) continue;
+    boxes.push_back(instance_box(in, T, s.blas[in.mesh].bvh.root));
+    ids.push_back(i);
+  }
+  lbvh_build(boxes, s.tlas);
+  s.tlas_slots.resize(ids.size());
+  for (size_t k = 0; k < ids.size(); ++k) s.tlas_slots[k] = ids[s.tlas.order[k]];
+}
+
+void build_blas(ge_oracle_scene& s, uint32_t m) {
+  const ge_mesh_desc& mesh = s.meshes[m];
+  std::vector<Box> boxes;
+  std::vector<ge_bvh_triangle> tris;
+  uint32_t index = 0;
+  for (uint32_t c = 0; c < mesh.cluster_count; ++c) {
+    const ge_cluster_desc& cl = s.clusters[mesh.first_cluster + c];
+    for (uint32_t t = 0; t < cl.triangle_count; ++t, ++index) {
+      const uint32_t packed = s.triangles[cl.triangle_offset + t];
+      ge_bvh_triangle tri{};
+      float* v[3] = {tri.v0, tri.v1, tri.v2};
+      Box b;
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = cl.vertex_offset + ((packed >> (8 * k)) & 0xffu);
+        std::memcpy(v[k], &s.vertices[size_t(vi) * 3], 12);
+      }
+      for (int a = 0; a < 3; ++a) {
+        b.lo[a] = std::fmin(std::fmin(tri.v0[a], tri.v1[a]), tri.v2[a]);
+        b.hi[a] = std::fmax(std::fmax(tri.v0[a], tri.v1[a]), tri.v2[a]);
+      }
+      tri.prim = (c << 8) | t;
+      tri.index = index;
+      boxes.push_back(b);
+      tris.push_back(tri);
+    }
+  }
+  Blas& out = s.blas[m];
+  lbvh_build(boxes, out.bvh);
+  out.tris.resize(tris.size());
+  for (size_t k = 0; k < tris.size(); ++k) out.tris[k] = tris[out.bvh.order[k]];
+}
+
+// ---- traversal ---------------------------------------------------------------------------------
+// Pins: tests/test_oracle_bvh.py (BVH == brute force on every ray of random and fixture scenes;
+// the watertight test against a double-precision Moller-Trumbore away from edges, and closed
+// under edge-sharing: a ray through a shared edge hits exactly one side).
+constexpr float k_box_robust = 1.00000095367431640625f;  // 1 + 2^-20 on the far distance (Ize 2013)
+
+struct TRay {
+  float o[3], d[3], inv[3];
+  float tmin;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+inline void prep_ray(TRay& r) {
+  for (int a = 0; a < 3; ++a) {
+    const float d = std::fabs(r.d[a]) < 1e-20f ? std::copysign(1e-20f, r.d[a]) : r.d[a];
+    r.inv[a] = 1.0f / d;
+  }
+  int kz = 0;
+  if (std::fabs(r.d[1]) > std::fabs(r.d[kz])) kz = 1;
+  if (std::fabs(r.d[2]) > std::fabs(r.d[kz])) kz = 2;
+  int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+  if (r.d[kz] < 0.0f) std::swap(kx, ky);
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  r.Sx = r.d[kx] / r.d[kz];
+  r.Sy = r.d[ky] / r.d[kz];
+  r.Sz = 1.0f / r.d[kz];
+}
+
+inline bool slab(const float lo[3], const float hi[3], const TRay& r, float tmax, float& tnear) {
+  const float tx0 = (lo[0] - r.o[0]) * r.inv[0], tx1 = (hi[0] - r.o[0]) * r.inv[0];
+  const float ty0 = (lo[1] - r.o[1]) * r.inv[1], ty1 = (hi[1] - r.o[1]) * r.inv[1];
+  const float tz0 = (lo[2] - r.o[2]) * r.inv[2], tz1 = (hi[2] - r.o[2]) * r.inv[2];
+  const float tn = std::fmax(std::fmax(std::fmin(tx0, tx1), std::fmin(ty0, ty1)),
+                             std::fmax(std::fmin(tz0, tz1), r.tmin));
+  const float tf = std::fmin(std::fmin(std::fmax(tx0, tx1), std::fmax(ty0, ty1)),
+                             std::fmin(std::fmax(tz0, tz1), tmax));
+  tnear = tn;
+  return tn <= tf * k_box_robust;
+}
+
+// Woop, Benthin, Wald 2013, "Watertight Ray/Triangle Intersection", with the double-precision
+// fallback for an edge function that is exactly zero. No face culling.
+inline bool intersect(const TRay& r, const float* v0, const float* v1, const float* v2, float& t,
+                      float& u, float& v) {
+  const float A[3] = {v0[0] - r.o[0], v0[1] - r.o[1], v0[2] - r.o[2]};
+  const float B[3] = {v1[0] - r.o[0], v1[1] - r.o[1], v1[2] - r.o[2]};
+  const float C[3] = {v2[0] - r.o[0], v2[1] - r.o[1], v2[2] - r.o[2]};
+  const float Ax = A[r.kx] - r.Sx * A[r.kz], Ay = A[r.ky] - r.Sy * A[r.kz];
+  const float Bx = B[r.kx] - r.Sx * B[r.kz], By = B[r.ky] - r.Sy * B[r.kz];
+  const float Cx = C[r.kx] - r.Sx * C[r.kz], Cy = C[r.ky] - r.Sy * C[r.kz];
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+    V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+    W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * A[r.kz], Bz = r.Sz * B[r.kz], Cz = r.Sz * C[r.kz];
+  const float T = U * Az + V * Bz + W * Cz;
+  t = T / det;
+  u = V / det;
+  v = W / det;
+  return true;
+}
+
+struct HitRec {
+  float t, u, v;
+  uint32_t inst, prim, key;
+  bool found;
+};
+
+inline void to_object(const InstRec& rec, const TRay& w, TRay& o) {
+  for (int r = 0; r < 3; ++r) {
+    o.o[r] = ((rec.w2o[r][0] * w.o[0] + rec.w2o[r][1] * w.o[1]) + rec.w2o[r][2] * w.o[2]) + rec.w2o[r][3];
+    o.d[r] = (rec.w2o[r][0] * w.d[0] + rec.w2o[r][1] * w.d[1]) + rec.w2o[r][2] * w.d[2];
+  }
+  o.tmin = w.tmin;
+  prep_ray(o);
+}
+
+constexpr uint32_t k_sentinel = 0xfffffffeu;
+
+// The canonical walk: the nearer child first (child 0 on a tie), the other pushed; leaves of a
+// mesh's structure are tested as their node is visited; a top-level leaf enters its instance. The
+// backend's kernels walk in this order, so their counters equal these.
+template <bool AnyHit>
+bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit,
+              Tally& tally) {
+  hit.found = false;
+  hit.t = tmax;
+  hit.key = 0;
+  if (mask == 0 || s.tlas.nodes.empty()) return false;
+  uint32_t stack[192];
+  int sp = 0;
+  TRay ray = world;
+  bool in_blas = false;
+  const Blas* blas = nullptr;
+  const InstRec* rec = nullptr;
+  uint32_t inst_id = 0;
+  uint32_t cur = 0;
+  for (;;) {
+    if (!in_blas && (cur & GE_BVH_LEAF)) {
+      inst_id = s.tlas_slots[cur & ~GE_BVH_LEAF];
+      rec = &s.inst[inst_id];
+      blas = &s.blas[rec->mesh];
+      to_object(*rec, world, ray);
+      stack[sp++] = k_sentinel;
+      in_blas = true;
+      ++tally.inst;
+      cur = 0;
+    }
+    ++tally.nodes;
+    const ge_bvh_node& node = in_blas ? blas->bvh.nodes[cur] : s.tlas.nodes[cur];
+    float tn0 = 0, tn1 = 0;
+    bool h0 = node.child[0] != GE_BVH_NONE && slab(node.lo0, node.hi0, ray, hit.t, tn0);
+    bool h1 = node.child[1] != GE_BVH_NONE && slab(node.lo1, node.hi1, ray, hit.t, tn1);
+    if (in_blas) {
+      for (int c = 0; c < 2; ++c) {
+        bool& hc = c == 0 ? h0 : h1;
+        if (!hc || !(node.child[c] & GE_BVH_LEAF)) continue;
+        hc = false;
+        const ge_bvh_triangle& tri = blas->tris[node.child[c] & ~GE_BVH_LEAF];
+        ++tally.tris;
+        float t, u, v;
+        if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
+        if (!(t >= ray.tmin && t <= hit.t)) continue;
+        const uint32_t key = (rec->first_pair << 8) + tri.prim;
+        if (AnyHit) {
+          hit.found = true;
+          hit.t = t;
+          hit.u = u;
+          hit.v = v;
+          hit.inst = inst_id;
+          hit.prim = tri.prim;
+          hit.key = key;
+          return true;
+        }
+        if (t < hit.t || !hit.found || key > hit.key) {
+          hit.found = true;
+          hit.t = t;
+          hit.u = u;
+          hit.v = v;
+          hit.inst = inst_id;
+          hit.prim = tri.prim;
+          hit.key = key;
+        }
+      }
+    }
+    if (h0 && h1) {
+      const bool second_first = tn1 < tn0;
+      stack[sp++] = second_first ? node.child[0] : node.child[1];
+      cur = second_first ? node.child[1] : node.child[0];
+      continue;
+    }
+    if (h0) {
+      cur = node.child[0];
+      continue;
+    }
+    if (h1) {
+      cur = node.child[1];
+      continue;
+    }
+    for (;;) {
+      if (sp == 0) return hit.found;
+      const uint32_t e = stack[--sp];
+      if (e == k_sentinel) {
+        in_blas = false;
+        ray = world;
+        continue;
+      }
+      cur = e;
+      break;
+    }
+  }
+}
+
+template <bool AnyHit>
+bool brute(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit) {
+  hit.found = false;
+  hit.t = tmax;
+  hit.key = 0;
+  if (mask == 0) return false;
+  for (uint32_t i = 0; i < s.instances.size(); ++i) {
+    const InstRec& rec = s.inst[i];
+    if (!rec.live) continue;
+    TRay ray;
+    to_object(rec, world, ray);
+    for (const ge_bvh_triangle& tri : s.blas[rec.mesh].tris) {
+      float t, u, v;
+      if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
+      if (!(t >= ray.tmin && t <= hit.t)) continue;
+      const uint32_t key = (rec.first_pair << 8) + tri.prim;
+      if (t < hit.t || !hit.found || key > hit.key) {
+        hit = HitRec{t, u, v, i, tri.prim, key, true};
+        if (AnyHit) return true;
+      }
+    }
+  }
+  return hit.found;
+}
+
+inline TRay make_ray(const float o[3], const float d[3], float tmin) {
+  TRay r;
+  for (int a = 0; a < 3; ++a) {
+    r.o[a] = o[a];
+    r.d[a] = d[a];
+  }
+  r.tmin = tmin;
+  prep_ray(r);
+  return r;
+}
+
+int resolve_threads(int32_t threads) {
+#ifdef _OPENMP
+  return threads > 0 ? threads : omp_get_max_threads();
+#else
+  (void)threads;
+  return 1;
+#endif
+}
+
+// ---- BSDF (brdf.slang:39-264) and sampler (sampling.slang:58-160) ------------------------------
+// Pins: tests/test_oracle_sampling.py — the reference's own sampler identities and furnace
+// numbers (domain/gfx/tests/sampling_tests.cpp:140-369): densities integrate to one, the VNDF
+// histogram, f cos / pdf = F G2 / G1 for a metal, the white furnace (0.94 < Lambert < 1,
+// 0.28 < metal r=1 < 0.36), MIS sums to one.
+template <class R> struct BrdfSurface {
+  V3<R> albedo, normal, view;
+  R roughness, metallic;
+  R specular[4];
+};
+
+template <class R> inline R k_pi() { return K<R>(3.14159265358979323846f, 3.14159265358979323846); }
+
+template <class R> inline R brdf_alpha(R roughness) {
+  const R r = clampr(roughness, KC(0.045), R(1));
+  return r * r;
+}
+template <class R> inline V3<R> brdf_f0(V3<R> albedo, R metallic) { return lerp3(splat(KC(0.04)), albedo, metallic); }
+template <class R> inline R brdf_d_ggx(R n_dot_h, R alpha) {
+  const R a2 = alpha * alpha;
+  const R d = n_dot_h * n_dot_h * (a2 - R(1)) + R(1);
+  return a2 / maxr(k_pi<R>() * d * d, KC(1e-9));
+}
+template <class R> inline R brdf_v_smith(R n_dot_v, R n_dot_l, R alpha) {
+  const R a2 = alpha * alpha;
+  const R v = n_dot_l * std::sqrt(n_dot_v * n_dot_v * (R(1) - a2) + a2);
+  const R l = n_dot_v * std::sqrt(n_dot_l * n_dot_l * (R(1) - a2) + a2);
+  return KC(0.5) / maxr(v + l, KC(1e-9));
+}
+template <class R> inline V3<R> f_schlick_f90(V3<R> f0, R f90, R cos_theta) {
+  const R m = clampr(R(1) - cos_theta, R(0), R(1));
+  const R m2 = m * m;
+  const R w = m2 * m2 * m;
+  return f0 + (splat(f90) - f0) * w;
+}
+template <class R> inline V3<R> f_schlick(V3<R> f0, R cos_theta) {
+  const R m = clampr(R(1) - cos_theta, R(0), R(1));
+  const R m2 = m * m;
+  return f0 + (splat(R(1)) - f0) * (m2 * m2 * m);
+}
+template <class R> inline bool declares_specular(const BrdfSurface<R>& s) {
+  return s.specular[0] != R(0) || s.specular[1] != R(0) || s.specular[2] != R(0) || s.specular[3] != R(0);
+}
+template <class R> inline V3<R> dielectric_f0(const BrdfSurface<R>& s) {
+  return splat(KC(0.04)) + v3(s.specular[0], s.specular[1], s.specular[2]);
+}
+template <class R> inline V3<R> surface_f0(const BrdfSurface<R>& s) {
+  if (!declares_specular(s)) return brdf_f0(s.albedo, s.metallic);
+  return lerp3(dielectric_f0(s), s.albedo, s.metallic);
+}
+template <class R> inline R surface_f90(const BrdfSurface<R>& s) {
+  return R(1) - s.specular[3] * (R(1) - s.metallic);
+}
+template <class R> inline V3<R> brdf_fresnel(const BrdfSurface<R>& s, R cos_theta) {
+  if (!declares_specular(s)) return f_schlick(brdf_f0(s.albedo, s.metallic), cos_theta);
+  return f_schlick_f90(surface_f0(s), surface_f90(s), cos_theta);
+}
+template <class R> inline R dielectric_f0_max(const BrdfSurface<R>& s) {
+  const V3<R> f0 = dielectric_f0(s);
+  return maxr(f0.x, maxr(f0.y, f0.z));
+}
+template <class R> inline V3<R> diffuse_fresnel(const BrdfSurface<R>& s, V3<R> f, R cos_theta) {
+  if (!declares_specular(s)) return f;
+  return f_schlick_f90(lerp3(splat(dielectric_f0_max(s)), s.albedo, s.metallic), surface_f90(s), cos_theta);
+}
+template <class R> inline V3<R> brdf_eval(const BrdfSurface<R>& s, V3<R> l) {
+  const R n_dot_l = dot(s.normal, l);
+  if (n_dot_l <= R(0)) return splat(R(0));
+  const R n_dot_v = maxr(dot(s.normal, s.view), KC(1e-4));
+  const V3<R> half_sum = l + s.view;
+  const V3<R> h = dot(half_sum, half_sum) > KC(1e-12) ? normalize(half_sum) : s.normal;
+  const R n_dot_h = clampr(dot(s.normal, h), R(0), R(1));
+  const R v_dot_h = clampr(dot(s.view, h), R(0), R(1));
+  const R alpha = brdf_alpha(s.roughness);
+  const V3<R> f = brdf_fresnel(s, v_dot_h);
+  const V3<R> specular = f * (brdf_d_ggx(n_dot_h, alpha) * brdf_v_smith(n_dot_v, n_dot_l, alpha));
+  const V3<R> diffuse = (splat(R(1)) - diffuse_fresnel(s, f, v_dot_h)) * ((R(1) - s.metallic) / k_pi<R>()) * s.albedo;
+  return diffuse + specular;
+}
+template <class R> inline V3<R> brdf_direct(const BrdfSurface<R>& s, V3<R> l, V3<R> radiance) {
+  return brdf_eval(s, l) * maxr(dot(s.normal, l), R(0)) * radiance;
+}
+
+template <class R> inline void onb_from_normal(V3<R> n, V3<R>& t, V3<R>& b) {
+  const R s = n.z >= R(0) ? R(1) : R(-1);
+  const R a = R(-1) / (s + n.z);
+  const R bb = n.x * n.y * a;
+  t = v3(R(1) + s * n.x * n.x * a, s * bb, -s * n.x);
+  b = v3(bb, s + n.y * n.y * a, -n.y);
+}
+template <class R> inline V3<R> to_local(V3<R> v, V3<R> t, V3<R> b, V3<R> n) { return v3(dot(v, t), dot(v, b), dot(v, n)); }
+template <class R> inline V3<R> to_world(V3<R> v, V3<R> t, V3<R> b, V3<R> n) { return t * v.x + b * v.y + n * v.z; }
+template <class R> inline R luminance(V3<R> c) { return dot(c, v3(KC(0.2126), KC(0.7152), KC(0.0722))); }
+template <class R> inline V3<R> sample_cosine_hemisphere(R ux, R uy) {
+  const R r = std::sqrt(ux);
+  const R phi = R(2) * k_pi<R>() * uy;
+  R sn, cs;
+  sincos_r(phi, &sn, &cs);
+  return v3(r * cs, r * sn, std::sqrt(maxr(R(0), R(1) - ux)));
+}
+template <class R> inline R cosine_pdf(R n_dot_l) { return maxr(n_dot_l, R(0)) / k_pi<R>(); }
+template <class R> inline R ggx_g1(R n_dot_v, R alpha) {
+  const R a2 = alpha * alpha;
+  const R denom = n_dot_v + std::sqrt(a2 + (R(1) - a2) * n_dot_v * n_dot_v);
+  return R(2) * n_dot_v / maxr(denom, KC(1e-9));
+}
+template <class R> inline V3<R> sample_ggx_vndf(V3<R> v, R alpha, R ux, R uy) {
+  const V3<R> vh = normalize(v3(alpha * v.x, alpha * v.y, v.z));
+  const R len_sq = vh.x * vh.x + vh.y * vh.y;
+  const V3<R> t1 = len_sq > R(0) ? v3(-vh.y, vh.x, R(0)) * rsqrt_(len_sq) : v3(R(1), R(0), R(0));
+  const V3<R> t2 = cross(vh, t1);
+  const R r = std::sqrt(ux);
+  const R phi = R(2) * k_pi<R>() * uy;
+  R sn, cs;
+  sincos_r(phi, &sn, &cs);
+  const R p1 = r * cs;
+  R p2 = r * sn;
+  const R s = KC(0.5) * (R(1) + vh.z);
+  p2 = (R(1) - s) * std::sqrt(maxr(R(0), R(1) - p1 * p1)) + s * p2;
+  const V3<R> nh = p1 * t1 + p2 * t2 + std::sqrt(maxr(R(0), R(1) - p1 * p1 - p2 * p2)) * vh;
+  return normalize(v3(alpha * nh.x, alpha * nh.y, maxr(R(0), nh.z)));
+}
+template <class R> inline R ggx_vndf_pdf(V3<R> v, V3<R> l, R alpha) {
+  if (v.z <= R(0) || l.z <= R(0)) return R(0);
+  const V3<R> h = normalize(v + l);
+  if (dot(v, h) <= R(0)) return R(0);
+  return ggx_g1(v.z, alpha) * brdf_d_ggx(h.z, alpha) / maxr(R(4) * v.z, KC(1e-9));
+}
+template <class R> inline R specular_probability(V3<R> albedo, R metallic, R /*n_dot_v*/, V3<R> fresnel) {
+  const R specular = luminance(fresnel);
+  const R diffuse = (R(1) - metallic) * luminance(albedo) * (R(1) - specular);
+  const R total = specular + diffuse;
+  if (total <= KC(1e-6)) return KC(0.5);
+  return clampr(specular / total, KC(0.05), KC(0.95));
+}
+template <class R> inline R bsdf_pdf(V3<R> v, V3<R> l, R alpha, R p_specular) {
+  return p_specular * ggx_vndf_pdf(v, l, alpha) + (R(1) - p_specular) * cosine_pdf(l.z);
+}
+template <class R> inline R mis_power_heuristic(R a, R b) {
+  const R aa = a * a, bb = b * b, total = aa + bb;
+  return total > R(0) ? aa / total : R(0);
+}
+// HLSL reflect: i - 2 n dot(i, n).
+template <class R> inline V3<R> reflect_(V3<R> i, V3<R> n) { return i - (n * R(2)) * dot(i, n); }
+
+// ---- lights (material.slang:960-986, brdf.slang:370-383) ----------------------------------------
+template <class R> inline R distance_attenuation(R distance_sq, R radius) {
+  const R ratio = clampr(distance_sq / maxr(radius * radius, KC(1e-8)), R(0), R(1));
+  const R window = R(1) - ratio * ratio;
+  return window * window / maxr(distance_sq, KC(1e-8));
+}
+template <class R> inline R spot_attenuation(V3<R> to_surface, V3<R> direction, R cos_inner, R cos_outer) {
+  const R t = clampr((dot(to_surface, direction) - cos_outer) / maxr(cos_inner - cos_outer, KC(1e-4)), R(0), R(1));
+  return t * t;
+}
+template <class R>
+inline bool light_sample(const ge_resolve_light& light, V3<R> position, V3<R>& direction, V3<R>& radiance,
+                         R& distance) {
+  if (R(light.direction_type[3]) >= KC(1.5)) {
+    direction = normalize(load3<R>(light.position_radius));
+    radiance = load3<R>(light.color_intensity) * R(light.color_intensity[3]);
+    distance = KC(1e30);
+    return true;
+  }
+  const V3<R> to_light = load3<R>(light.position_radius) - position;
+  const R distance_sq = dot(to_light, to_light);
+  R attenuation = distance_attenuation(distance_sq, R(light.position_radius[3]));
+  direction = v3(R(0), R(1), R(0));
+  radiance = splat(R(0));
+  distance = R(0);
+  if (attenuation <= R(0)) return false;
+  direction = normalize(to_light);
+  if (R(light.direction_type[3]) >= KC(0.5))
+    attenuation *= spot_attenuation(-direction, normalize(load3<R>(light.direction_type)), R(light.spot[0]), R(light.spot[1]));
+  radiance = load3<R>(light.color_intensity) * (R(light.color_intensity[3]) * attenuation);
+  distance = std::sqrt(distance_sq);
+  return true;
+}
+
+// ---- hit decode (material.slang:244-302, scene.slang:566-610,789-796) ----------------------------
+template <class R> inline V3<R> decode_normal_oct(uint32_t packed) {
+  const R x = R(float(int32_t(packed << 16) >> 16)) / KC(32767.0);
+  const R y = R(float(int32_t(packed) >> 16)) / KC(32767.0);
+  V3<R> n = v3(x, y, R(1) - std::fabs(x) - std::fabs(y));
+  if (n.z < R(0)) {
+    const R sx = x < R(0) ? R(-1) : R(1), sy = y < R(0) ? R(-1) : R(1);
+    n.x = (R(1) - std::fabs(y)) * sx;
+    n.y = (R(1) - std::fabs(x)) * sy;
+  }
+  return normalize(n);
+}
+template <class R> inline V3<R> transform_normal(const ge_instance_desc& in, V3<R> n) {
+  const V3<R> c0 = v3(R(in.rows[0][0]), R(in.rows[1][0]), R(in.rows[2][0]));
+  const V3<R> c1 = v3(R(in.rows[0][1]), R(in.rows[1][1]), R(in.rows[2][1]));
+  const V3<R> c2 = v3(R(in.rows[0][2]), R(in.rows[1][2]), R(in.rows[2][2]));
+  const V3<R> x12 = cross(c1, c2);
+  const V3<R> cof = x12 * n.x + cross(c2, c0) * n.y + cross(c0, c1) * n.z;
+  return normalize(dot(c0, x12) < R(0) ? -cof : cof);
+}
+template <class R> inline V3<R> load_quantized_position(const ge_oracle_scene& s, const ge_mesh_desc& mesh, uint32_t vertex) {
+  const uint32_t i = vertex * 3;
+  const V3<R> q = v3(R(float(s.quantized[i])), R(float(s.quantized[i + 1])), R(float(s.quantized[i + 2])));
+  return v3(R(mesh.quant[0]), R(mesh.quant[1]), R(mesh.quant[2])) + q * R(mesh.quant[3]);
+}
+
+template <class R> struct Surf {
+  V3<R> position, normal, geometric;
+};
+
+template <class R>
+Surf<R> surface_of_hit(const ge_oracle_scene& s, const HitRec& hit, V3<R> towards, const ge_frame_eye& eye,
+                       const ge_instance_desc** inst_out, uint32_t* cluster_out) {
+  const ge_instance_desc& in = s.instances[hit.inst];
+  const ge_mesh_desc& mesh = s.meshes[in.mesh];
+  const uint32_t cluster_index = mesh.first_cluster + (hit.prim >> 8);
+  const ge_cluster_desc& cl = s.clusters[cluster_index];
+  const uint32_t packed = s.triangles[cl.triangle_offset + (hit.prim & 0xffu)];
+  float Tf[3];
+  instance_from_eye(in, eye, Tf);
+  const V3<R> T = v3(R(Tf[0]), R(Tf[1]), R(Tf[2]));
+  V3<R> p[3];
+  uint32_t local[3];
+  for (int k = 0; k < 3; ++k) {
+    local[k] = cl.vertex_offset + ((packed >> (8 * k)) & 0xffu);
+    const V3<R> m = load_quantized_position<R>(s, mesh, local[k]);
+    V3<R> d;
+    d.x = R(in.rows[0][0]) * m.x + R(in.rows[0][1]) * m.y + R(in.rows[0][2]) * m.z;
+    d.y = R(in.rows[1][0]) * m.x + R(in.rows[1][1]) * m.y + R(in.rows[1][2]) * m.z;
+    d.z = R(in.rows[2][0]) * m.x + R(in.rows[2][1]) * m.y + R(in.rows[2][2]) * m.z;
+    p[k] = d + T;
+  }
+  const R bu = R(hit.u), bv = R(hit.v);
+  const V3<R> b = v3(R(1) - bu - bv, bu, bv);
+  Surf<R> out;
+  out.position = b.x * p[0] + b.y * p[1] + b.z * p[2];
+  out.geometric = normalize(cross(p[1] - p[0], p[2] - p[0]));
+  if (dot(out.geometric, towards - out.position) < R(0)) out.geometric = -out.geometric;
+  if (s.has_attributes) {
+    const V3<R> n0 = decode_normal_oct<R>(s.attributes[local[0]].normal_oct);
+    const V3<R> n1 = decode_normal_oct<R>(s.attributes[local[1]].normal_oct);
+    const V3<R> n2 = decode_normal_oct<R>(s.attributes[local[2]].normal_oct);
+    const V3<R> n = b.x * n0 + b.y * n1 + b.z * n2;
+    out.normal = transform_normal(in, n);
+  } else {
+    out.normal = out.geometric;
+  }
+  *inst_out = &in;
+  *cluster_out = cluster_index;
+  return out;
+}
+
+// ---- the integrator (path_trace.slang:166-495) ----------------------------------------------------
+template <class R> struct PtCtx {
+  const ge_oracle_scene* s;
+  const ge_path_trace_params* p;
+  Tally* tally;
+};
+
+template <class R> inline bool occluded(const PtCtx<R>& c, V3<R> origin, V3<R> direction, R t_max) {
+  if (t_max <= R(0)) return false;
+  ++c.tally->any;
+  const float o[3] = {float(origin.x), float(origin.y), float(origin.z)};
+  const float d[3] = {float(direction.x), float(direction.y), float(direction.z)};
+  const TRay ray = make_ray(o, d, 0.0f);
+  HitRec hit;
+  return traverse<true>(*c.s, ray, float(t_max), 0x01u, hit, *c.tally);
+}
+
+template <class R> inline V3<R> environment(const PtCtx<R>& c, V3<R> direction, bool primary) {
+  const ge_path_trace_params& p = *c.p;
+  const V3<R> sky = load3<R>(p.sky);
+  if (p.flags & GE_PT_UNIFORM_SKY) return sky;
+  if (primary) {
+    const V3<R> m = maxv(sky, R(0));
+    if constexpr (std::is_same_v<R, float>)
+      return v3(std::pow(m.x, 2.2f), std::pow(m.y, 2.2f), std::pow(m.z, 2.2f));
+    else
+      return v3(std::pow(m.x, 2.2), std::pow(m.y, 2.2), std::pow(m.z, 2.2));
+  }
+  const V3<R> sky_r = sky * KC(0.35);
+  const V3<R> ground = load3<R>(p.ground) * (splat(R(p.sun[3]) * maxr(R(p.sun[1]), R(0)) / k_pi<R>()) + sky_r);
+  return direction.y >= R(0) ? sky_r : ground;
+}
+
+template <class R>
+V3<R> next_event(const PtCtx<R>& c, const BrdfSurface<R>& surface, V3<R> position, V3<R> front, R bias) {
+  const ge_path_trace_params& p = *c.p;
+  V3<R> direct = splat(R(0));
+  const V3<R> shadow_origin = position + front * bias;
+  const V3<R> sun = load3<R>(p.sun);
+  if (R(p.sun[3]) > R(0) && dot(surface.normal, sun) > R(0) && !occluded(c, shadow_origin, sun, KC(1e30))) {
+    direct = direct + brdf_direct(surface, sun, splat(R(p.sun[3])));
+  }
+  for (uint32_t i = 0; i < p.light_count; ++i) {
+    const ge_resolve_light& light = c.s->lights[i];
+    V3<R> light_dir, radiance;
+    R light_distance;
+    if (!light_sample(light, position, light_dir, radiance, light_distance)) continue;
+    if (dot(surface.normal, light_dir) <= R(0)) continue;
+    if (occluded(c, shadow_origin, light_dir, light_distance - bias - R(light.spot[2]))) continue;
+    direct = direct + brdf_direct(surface, light_dir, radiance);
+  }
+  return direct;
+}
+
+template <class R>
+V3<R> trace_path(const PtCtx<R>& c, V3<R> origin, V3<R> direction, uint32_t& rng, bool& hit_primary) {
+  const ge_path_trace_params& p = *c.p;
+  const ge_oracle_scene& s = *c.s;
+  hit_primary = false;
+  V3<R> radiance = splat(R(0));
+  V3<R> throughput = splat(R(1));
+  for (uint32_t bounce = 0;; ++bounce) {
+    ++c.tally->closest;
+    const float o[3] = {float(origin.x), float(origin.y), float(origin.z)};
+    const float d[3] = {float(direction.x), float(direction.y), float(direction.z)};
+    const TRay ray = make_ray(o, d, 0.0f);
+    HitRec hit;
+    traverse<false>(s, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, *c.tally);
+    if (!hit.found) {
+      radiance = radiance + throughput * environment(c, direction, bounce == 0);
+      break;
+    }
+    if (bounce == 0) hit_primary = true;
+    const ge_instance_desc* inst;
+    uint32_t cluster_index;
+    const Surf<R> sf = surface_of_hit<R>(s, hit, origin, p.eye, &inst, &cluster_index);
+    const ge_resolve_material& mat = s.materials[s.cluster_materials[cluster_index] + inst->material_base];
+    V3<R> shading_normal = sf.normal;
+    if (dot(shading_normal, sf.geometric) < R(0)) shading_normal = -shading_normal;
+    radiance = radiance + throughput * load3<R>(mat.emissive);
+    BrdfSurface<R> surface;
+    surface.albedo = load3<R>(mat.albedo);
+    surface.normal = shading_normal;
+    surface.view = -direction;
+    surface.roughness = R(mat.albedo[3]);
+    surface.metallic = R(mat.emissive[3]);
+    for (int k = 0; k < 4; ++k) surface.specular[k] = R(mat.specular[k]);
+    const ge_mesh_desc& mesh = s.meshes[inst->mesh];
+    const R bias = R(p.ray_bias) + R(p.ray_bias_steps) * R(mesh.quant[3]) * R(inst->scale_max);
+    const V3<R> front = dot(sf.geometric, surface.view) >= R(0) ? sf.geometric : -sf.geometric;
+    radiance = radiance + throughput * next_event(c, surface, sf.position, front, bias);
+    if (bounce >= p.max_bounces) break;
+    const R n_dot_v = dot(surface.normal, surface.view);
+    if (n_dot_v <= R(0)) break;
+    V3<R> tangent, bitangent;
+    onb_from_normal(surface.normal, tangent, bitangent);
+    const V3<R> v_local = to_local(surface.view, tangent, bitangent, surface.normal);
+    const R alpha = brdf_alpha(surface.roughness);
+    const V3<R> fresnel = brdf_fresnel(surface, n_dot_v);
+    const R p_specular = specular_probability(surface.albedo, surface.metallic, n_dot_v, fresnel);
+    V3<R> l_local;
+    if (rng_next<R>(rng) < p_specular) {
+      const R u0 = rng_next<R>(rng);
+      const R u1 = rng_next<R>(rng);
+      const V3<R> h = sample_ggx_vndf(v_local, alpha, u0, u1);
+      l_local = reflect_(-v_local, h);
+    } else {
+      const R u0 = rng_next<R>(rng);
+      const R u1 = rng_next<R>(rng);
+      l_local = sample_cosine_hemisphere(u0, u1);
+    }
+    if (l_local.z <= R(0)) break;
+    const R pdf = bsdf_pdf(v_local, l_local, alpha, p_specular);
+    if (!(pdf > R(0))) break;
+    const V3<R> next = to_world(l_local, tangent, bitangent, surface.normal);
+    const V3<R> weight = brdf_eval(surface, next) * (l_local.z / pdf);
+    throughput = throughput * weight;
+    if (throughput.x <= R(0) && throughput.y <= R(0) && throughput.z <= R(0)) break;
+    if (bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
+      const R survive = clampr(maxr(throughput.x, maxr(throughput.y, throughput.z)), R(0), KC(0.95));
+      if (rng_next<R>(rng) >= survive) break;
+      throughput = throughput / maxr(survive, KC(1e-6));
+    }
+    origin = sf.position + (dot(next, front) >= R(0) ? front : -front) * bias;
+    direction = next;
+  }
+  return radiance;
+}
+
+template <class R>
+void path_trace_pixel(const PtCtx<R>& c, uint32_t x, uint32_t y, R sum[3], bool& covered) {
+  const ge_path_trace_params& p = *c.p;
+  const R sx = R(float(p.width)), sy = R(float(p.height));
+  const bool centered = (p.flags & GE_PT_PIXEL_CENTER) != 0;
+  V3<R> acc = splat(R(0));
+  covered = false;
+  for (uint32_t si = 0; si < p.spp; ++si) {
+    uint32_t rng = rng_seed(x, y, p.sample_base + si, p.seed);
+    R jx = KC(0.5), jy = KC(0.5);
+    if (!centered) {
+      jx = rng_next<R>(rng);
+      jy = rng_next<R>(rng);
+    }
+    const R ndx = (R(float(x)) + jx) / sx * R(2) - R(1);
+    const R ndy = R(1) - (R(float(y)) + jy) / sy * R(2);
+    const float* m = p.clip_to_ray;  // column-major
+    V3<R> dir;
+    dir.x = ((R(m[0]) * ndx + R(m[4]) * ndy) + R(m[8]) * R(0)) + R(m[12]) * R(1);
+    dir.y = ((R(m[1]) * ndx + R(m[5]) * ndy) + R(m[9]) * R(0)) + R(m[13]) * R(1);
+    dir.z = ((R(m[2]) * ndx + R(m[6]) * ndy) + R(m[10]) * R(0)) + R(m[14]) * R(1);
+    dir = normalize(dir);
+    const V3<R> origin = load3<R>(p.camera);
+    bool hit_primary = false;
+    acc = acc + trace_path(c, origin, dir, rng, hit_primary);
+    covered = covered || hit_primary;
+  }
+  sum[0] = acc.x;
+  sum[1] = acc.y;
+  sum[2] = acc.z;
+}
+
+// ---- display (display.slang:1-60, path_trace.slang:511-543) -------------------------------------
+inline float display_ign(uint32_t px, uint32_t py) {
+  const uint32_t inner = px * 288237660u + py * 25070368u;
+  const uint32_t outer = uint32_t((uint64_t(inner) * uint64_t(3472289u)) >> 16);
+  return float(outer >> 8) * (1.0f / 16777216.0f);
+}
+inline float display_dither(uint32_t px, uint32_t py) {
+  const float u = display_ign(px, py);
+  return u < 0.5f ? std::sqrt(2.0f * u) - 1.0f : 1.0f - std::sqrt(2.0f * (1.0f - u));
+}
+inline float saturate_(float v) { return std::fmin(std::fmax(v, 0.0f), 1.0f); }
+inline void display_output(const float lin[3], uint32_t px, uint32_t py, uint32_t steps, float out[3]) {
+  float e[3];
+  for (int k = 0; k < 3; ++k) e[k] = std::pow(std::fmax(lin[k], 0.0f), 1.0f / 2.2f);
+  if (steps == 0) {
+    for (int k = 0; k < 3; ++k) out[k] = e[k];
+    return;
+  }
+  const float s = float(steps);
+  const float dither = display_dither(px, py);
+  for (int k = 0; k < 3; ++k) {
+    const float v = saturate_(e[k]);
+    const float room = std::fmin(std::fmin(v * s, (1.0f - v) * s), 1.0f);
+    out[k] = v + dither * room / s;
+  }
+}
+
+}  // namespace
+
+// ---- C ABI ------------------------------------------------------------------------------------
+extern "C" {
+
+ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* d, const ge_frame_eye* eye) {
+  if (!d) return nullptr;
+  auto* s = new ge_oracle_scene();
+  s->clusters.assign(d->clusters, d->clusters + d->cluster_count);
+  s->vertices.assign(d->vertices, d->vertices + size_t(d->vertex_count) * 3);
+  const size_t qn = (size_t(d->vertex_count) * 3 + 1) & ~size_t(1);
+  s->quantized.assign(d->quantized, d->quantized + qn);
+  s->has_attributes = d->attributes != nullptr;
+  if (s->has_attributes) s->attributes.assign(d->attributes, d->attributes + d->vertex_count);
+  s->triangles.assign(d->triangles, d->triangles + d->triangle_count);
+  s->meshes.assign(d->meshes, d->meshes + d->mesh_count);
+  s->instances.assign(d->instances, d->instances + d->instance_count);
+  s->materials.assign(d->materials, d->materials + d->material_count);
+  s->cluster_materials.assign(d->cluster_materials, d->cluster_materials + d->cluster_count);
+  if (d->light_count) s->lights.assign(d->lights, d->lights + d->light_count);
+  s->blas.resize(d->mesh_count);
+  for (uint32_t m = 0; m < d->mesh_count; ++m) build_blas(*s, m);
+  const ge_frame_eye zero{};
+  build_tlas(*s, eye ? *eye : zero);
+  return s;
+}
+
+void ge_oracle_scene_destroy(ge_oracle_scene* s) { delete s; }
+
+int32_t ge_oracle_bvh_info(const ge_oracle_scene* s, uint32_t mesh, uint32_t* node_count, uint32_t* prim_count) {
+  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (mesh < s->blas.size() ? &s->blas[mesh].bvh : nullptr);
+  if (!b) return GE_RT_ERROR_INVALID_ARGUMENT;
+  if (node_count) *node_count = uint32_t(b->nodes.size());
+  if (prim_count) *prim_count = b->n;
+  return GE_RT_OK;
+}
+
+int32_t ge_oracle_bvh_download(const ge_oracle_scene* s, uint32_t mesh, ge_bvh_node* nodes, ge_bvh_triangle* tris,
+                               uint32_t* slots, uint32_t* morton) {
+  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (mesh < s->blas.size() ? &s->blas[mesh].bvh : nullptr);
+  if (!b) return GE_RT_ERROR_INVALID_ARGUMENT;
+  if (nodes) std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(ge_bvh_node));
+  if (morton) std::memcpy(morton, b->morton.data(), b->morton.size() * 4);
+  if (mesh == 0xffffffffu) {
+    if (slots) std::memcpy(slots, s->tlas_slots.data(), s->tlas_slots.size() * 4);
+  } else if (tris) {
+    std::memcpy(tris, s->blas[mesh].tris.data(), s->blas[mesh].tris.size() * sizeof(ge_bvh_triangle));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_oracle_update_instances(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_instance_desc* in,
+                                   const ge_frame_eye* eye) {
+  if (first + count > s->instances.size()) return GE_RT_ERROR_INVALID_ARGUMENT;
+  std::memcpy(&s->instances[first], in, size_t(count) * sizeof(ge_instance_desc));
+  const ge_frame_eye zero{};
+  build_tlas(*s, eye ? *eye : zero);
+  return GE_RT_OK;
+}
+
+void ge_oracle_trace_closest(const ge_oracle_scene* s, const ge_ray* rays, uint32_t count, uint32_t mask, ge_hit* hits,
+                             int32_t use_brute, int32_t threads) {
+  const int nt = resolve_threads(threads);
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t i = 0; i < int64_t(count); ++i) {
+      const ge_ray& r = rays[i];
+      const TRay ray = make_ray(r.origin, r.dir, r.tmin);
+      HitRec h;
+      ++tally.closest;
+      const bool found = use_brute ? brute<false>(*s, ray, r.tmax, mask, h) : traverse<false>(*s, ray, r.tmax, mask, h, tally);
+      ge_hit out{};
+      if (found) {
+        out.t = h.t;
+        out.u = h.u;
+        out.v = h.v;
+        out.instance_id = h.inst;
+        out.geometry = h.prim >> 8;
+        out.primitive = h.prim & 0xffu;
+        out.key = h.key;
+      } else {
+        out.instance_id = GE_RT_MISS;
+        out.geometry = GE_RT_MISS;
+        out.primitive = GE_RT_MISS;
+      }
+      hits[i] = out;
+    }
+    tally.flush();
+  }
+}
+
+void ge_oracle_trace_any(const ge_oracle_scene* s, const ge_ray* rays, uint32_t count, uint32_t mask,
+                         uint32_t* occluded_bits, int32_t use_brute, int32_t threads) {
+  const uint32_t words = (count + 31) / 32;
+  std::memset(occluded_bits, 0, size_t(words) * 4);
+  const int nt = resolve_threads(threads);
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+#pragma omp for schedule(dynamic, 8)
+    for (int64_t w = 0; w < int64_t(words); ++w) {
+      uint32_t bits = 0;
+      for (uint32_t k = 0; k < 32; ++k) {
+        const uint64_t i = uint64_t(w) * 32 + k;
+        if (i >= count) break;
+        const ge_ray& r = rays[i];
+        if (!(r.tmax > 0.0f)) continue;
+        ++tally.any;
+        const TRay ray = make_ray(r.origin, r.dir, r.tmin);
+        HitRec h;
+        const bool hit = use_brute ? brute<true>(*s, ray, r.tmax, mask, h) : traverse<true>(*s, ray, r.tmax, mask, h, tally);
+        if (hit) bits |= 1u << k;
+      }
+      occluded_bits[w] = bits;
+    }
+    tally.flush();
+  }
+}
+
+void ge_oracle_ray_visibility(const ge_oracle_scene* s, const ge_ray_visibility_params* p, uint64_t* words,
+                              int32_t threads) {
+  const int nt = resolve_threads(threads);
+  const uint32_t W = p->width, H = p->height;
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t y = 0; y < int64_t(H); ++y) {
+      for (uint32_t x = 0; x < W; ++x) {
+        const float ndx = (float(x) + 0.5f) / float(W) * 2.0f - 1.0f;
+        const float ndy = 1.0f - (float(uint32_t(y)) + 0.5f) / float(H) * 2.0f;
+        const float* m = p->clip_to_ray;
+        float d[3];
+        for (int r = 0; r < 3; ++r) d[r] = ((m[r] * ndx + m[4 + r] * ndy) + m[8 + r] * 0.0f) + m[12 + r] * 1.0f;
+        const float len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const float inv = 1.0f / len;
+        for (int r = 0; r < 3; ++r) d[r] = d[r] * inv;
+        const float o[3] = {p->camera[0], p->camera[1], p->camera[2]};
+        const TRay ray = make_ray(o, d, 0.0f);
+        HitRec h;
+        ++tally.closest;
+        uint64_t value = 0;
+        if (traverse<false>(*s, ray, 1e30f, 0xfeu, h, tally)) {
+          float pt[3];
+          for (int r = 0; r < 3; ++r) pt[r] = o[r] + d[r] * h.t;
+          const float* vp = p->view_proj;
+          const float cz = ((vp[2] * pt[0] + vp[6] * pt[1]) + vp[10] * pt[2]) + vp[14] * 1.0f;
+          const float cw = ((vp[3] * pt[0] + vp[7] * pt[1]) + vp[11] * pt[2]) + vp[15] * 1.0f;
+          const float depth = cz / cw;
+          uint32_t bits;
+          std::memcpy(&bits, &depth, 4);
+          value = (uint64_t(bits) << 32) | uint64_t(h.key);
+        }
+        words[size_t(y) * W + x] = value;
+      }
+    }
+    tally.flush();
+  }
+}
+
+void ge_oracle_path_trace(const ge_oracle_scene* s, const ge_path_trace_params* p, float* accum, uint32_t* coverage,
+                          uint32_t mode, uint32_t row0, uint32_t row1, int32_t threads) {
+  const int nt = resolve_threads(threads);
+  const uint32_t W = p->width;
+  row1 = std::min(row1, p->height);
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+    // 8x8 pixel tiles, dynamically scheduled (the CPU baseline's policy).
+    const uint32_t tiles_x = (W + 7) / 8, tiles_y = (row1 - row0 + 7) / 8;
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t tile = 0; tile < int64_t(tiles_x) * tiles_y; ++tile) {
+      const uint32_t tx = uint32_t(tile % tiles_x), ty = uint32_t(tile / tiles_x);
+      for (uint32_t y = row0 + ty * 8; y < std::min(row0 + ty * 8 + 8, row1); ++y) {
+        for (uint32_t x = tx * 8; x < std::min(tx * 8 + 8, W); ++x) {
+          const size_t pix = size_t(y) * W + x;
+          bool covered = false;
+          float sum[3];
+          if (mode == GE_ORACLE_FP64) {
+            PtCtx<double> c{s, p, &tally};
+            double sd[3];
+            path_trace_pixel<double>(c, x, y, sd, covered);
+            for (int k = 0; k < 3; ++k) sum[k] = float(sd[k]);
+          } else {
+            PtCtx<float> c{s, p, &tally};
+            path_trace_pixel<float>(c, x, y, sum, covered);
+          }
+          float prev[4] = {0, 0, 0, 0};
+          if (p->sample_base != 0) std::memcpy(prev, &accum[pix * 4], 16);
+          accum[pix * 4 + 0] = prev[0] + sum[0];
+          accum[pix * 4 + 1] = prev[1] + sum[1];
+          accum[pix * 4 + 2] = prev[2] + sum[2];
+          accum[pix * 4 + 3] = prev[3] + float(p->spp);
+          const uint32_t before = p->sample_base == 0 ? 0u : coverage[pix];
+          coverage[pix] = before | (covered ? 1u : 0u);
+        }
+      }
+    }
+    tally.flush();
+  }
+}
+
+void ge_oracle_path_trace64(const ge_oracle_scene* s, const ge_path_trace_params* p, double* accum64,
+                            uint32_t* coverage, uint32_t row0, uint32_t row1, int32_t threads) {
+  const int nt = resolve_threads(threads);
+  const uint32_t W = p->width;
+  row1 = std::min(row1, p->height);
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t y = row0; y < int64_t(row1); ++y) {
+      for (uint32_t x = 0; x < W; ++x) {
+        const size_t pix = size_t(y) * W + x;
+        PtCtx<double> c{s, p, &tally};
+        double sd[3];
+        bool covered = false;
+        path_trace_pixel<double>(c, x, uint32_t(y), sd, covered);
+        double prev[4] = {0, 0, 0, 0};
+        if (p->sample_base != 0) std::memcpy(prev, &accum64[pix * 4], 32);
+        for (int k = 0; k < 3; ++k) accum64[pix * 4 + k] = prev[k] + sd[k];
+        accum64[pix * 4 + 3] = prev[3] + double(p->spp);
+        const uint32_t before = p->sample_base == 0 ? 0u : coverage[pix];
+        coverage[pix] = before | (covered ? 1u : 0u);
+      }
+    }
+    tally.flush();
+  }
+}
+
+void ge_oracle_tonemap(const ge_path_trace_params* p, const float* accum, const uint32_t* coverage, uint32_t* output) {
+  const uint32_t W = p->width, H = p->height;
+  for (uint32_t y = 0; y < H; ++y) {
+    for (uint32_t x = 0; x < W; ++x) {
+      const size_t pix = size_t(y) * W + x;
+      const float* v = &accum[pix * 4];
+      float mean[3] = {0, 0, 0};
+      if (v[3] > 0.0f)
+        for (int k = 0; k < 3; ++k) mean[k] = v[k] / v[3];
+      const bool background = coverage[pix] == 0 && (p->flags & GE_PT_UNIFORM_SKY) == 0;
+      if (background) {
+        output[pix] = p->background;
+        continue;
+      }
+      const uint32_t steps = (p->flags & GE_PT_DITHER) ? 255u : 0u;
+      float e[3];
+      display_output(mean, x, y, steps, e);
+      uint32_t rgb[3];
+      for (int k = 0; k < 3; ++k) rgb[k] = uint32_t(saturate_(e[k]) * 255.0f + 0.5f);
+      output[pix] = rgb[0] | (rgb[1] << 8) | (rgb[2] << 16) | (255u << 24);
+    }
+  }
+}
+
+void ge_oracle_shadow_pass(const ge_oracle_scene* s, const ge_shadow_params* p, const ge_shadow_receiver* rec,
+                           uint32_t* blocked, int32_t threads) {
+  const uint32_t words = (p->count + 31) / 32;
+  std::memset(blocked, 0, size_t(words) * 4);
+  const int nt = resolve_threads(threads);
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+#pragma omp for schedule(dynamic, 8)
+    for (int64_t w = 0; w < int64_t(words); ++w) {
+      uint32_t bits = 0;
+      for (uint32_t k = 0; k < 32; ++k) {
+        const uint64_t i = uint64_t(w) * 32 + k;
+        if (i >= p->count) break;
+        const ge_shadow_receiver& r = rec[i];
+        if (r.instance_id >= s->instances.size()) continue;
+        const ge_instance_desc& in = s->instances[r.instance_id];
+        const float offset = p->offset + p->offset_steps * s->meshes[in.mesh].quant[3] * in.scale_max;
+        float o[3];
+        for (int a = 0; a < 3; ++a) o[a] = r.position[a] + r.geometric[a] * offset;
+        if (!(p->t_max > 0.0f)) continue;
+        ++tally.any;
+        const TRay ray = make_ray(o, p->sun, 0.0f);
+        HitRec h;
+        if (traverse<true>(*s, ray, p->t_max, 0x01u, h, tally)) bits |= 1u << k;
+      }
+      blocked[w] = bits;
+    }
+    tally.flush();
+  }
+}
+
+void ge_oracle_counters(uint64_t out[5]) {
+  for (int i = 0; i < 5; ++i) out[i] = g_counters[i].load();
+}
+void ge_oracle_reset_counters(void) {
+  for (int i = 0; i < 5; ++i) g_counters[i] = 0;
+}
+int32_t ge_oracle_max_threads(void) { return resolve_threads(0); }
+
+uint32_t ge_oracle_pcg_hash(uint32_t v) { return pcg_hash(v); }
+uint32_t ge_oracle_rng_seed(uint32_t x, uint32_t y, uint32_t sample_index, uint32_t seed) {
+  return rng_seed(x, y, sample_index, seed);
+}
+float ge_oracle_rng_next(uint32_t* state) { return rng_next<float>(*state); }
+uint32_t ge_oracle_morton3(float cx, float cy, float cz, const float lo[3], const float scale[3]) {
+  const float c[3] = {cx, cy, cz};
+  return morton3(c, lo, scale);
+}
+void ge_oracle_sincos(float x, float* s, float* c) { sincos_f32(x, s, c); }
+
+void ge_oracle_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
+                         float tmin, float tmax, double out[4]) {
+  const TRay r = make_ray(o, d, tmin);
+  float t = 0, u = 0, v = 0;
+  const bool hit = intersect(r, v0, v1, v2, t, u, v) && t >= tmin && t <= tmax;
+  out[0] = hit ? 1.0 : 0.0;
+  out[1] = t;
+  out[2] = u;
+  out[3] = v;
+}
+
+}  // extern "C"
+
+namespace {
+template <class R> int32_t probe(uint32_t kind, const double* in, double* out) {
+  auto V = [&](int i) { return v3(R(in[i]), R(in[i + 1]), R(in[i + 2])); };
+  auto put = [&](int i, V3<R> v) {
+    out[i] = double(v.x);
+    out[i + 1] = double(v.y);
+    out[i + 2] = double(v.z);
+  };
+  auto surface = [&](int i) {  // albedo3 normal3 view3 roughness metallic specular4
+    BrdfSurface<R> s;
+    s.albedo = V(i);
+    s.normal = V(i + 3);
+    s.view = V(i + 6);
+    s.roughness = R(in[i + 9]);
+    s.metallic = R(in[i + 10]);
+    for (int k = 0; k < 4; ++k) s.specular[k] = R(in[i + 11 + k]);
+    return s;
+  };
+  switch (kind) {
+    case 0: put(0, sample_cosine_hemisphere(R(in[0]), R(in[1]))); return 0;
+    case 1: put(0, sample_ggx_vndf(V(0), R(in[3]), R(in[4]), R(in[5]))); return 0;
+    case 2: out[0] = double(ggx_vndf_pdf(V(0), V(3), R(in[6]))); return 0;
+    case 3: put(0, brdf_eval(surface(0), V(15))); return 0;
+    case 4: out[0] = double(bsdf_pdf(V(0), V(3), R(in[6]), R(in[7]))); return 0;
+    case 5: out[0] = double(specular_probability(V(0), R(in[3]), R(in[4]), V(5))); return 0;
+    case 6: {
+      V3<R> t, b;
+      onb_from_normal(V(0), t, b);
+      put(0, t);
+      put(3, b);
+      return 0;
+    }
+    case 7: out[0] = double(ggx_g1(R(in[0]), R(in[1]))); return 0;
+    case 8: out[0] = double(brdf_v_smith(R(in[0]), R(in[1]), R(in[2]))); return 0;
+    case 9: out[0] = double(brdf_d_ggx(R(in[0]), R(in[1]))); return 0;
+    case 10: put(0, brdf_fresnel(surface(0), R(in[15]))); return 0;
+    case 11: out[0] = double(mis_power_heuristic(R(in[0]), R(in[1]))); return 0;
+    case 12: {
+      ge_resolve_light l;
+      for (int k = 0; k < 4; ++k) {
+        l.position_radius[k] = float(in[k]);
+        l.color_intensity[k] = float(in[4 + k]);
+        l.direction_type[k] = float(in[8 + k]);
+        l.spot[k] = float(in[12 + k]);
+      }
+      V3<R> dir, rad;
+      R dist;
+      out[0] = light_sample(l, V(16), dir, rad, dist) ? 1.0 : 0.0;
+      put(1, dir);
+      put(4, rad);
+      out[7] = double(dist);
+      return 0;
+    }
+    case 13: put(0, decode_normal_oct<R>(uint32_t(in[0]))); return 0;
+    case 14: {
+      const float lin[3] = {float(in[0]), float(in[1]), float(in[2])};
+      float e[3];
+      display_output(lin, uint32_t(in[3]), uint32_t(in[4]), uint32_t(in[5]), e);
+      for (int k = 0; k < 3; ++k) out[k] = e[k];
+      return 0;
+    }
+    case 15: {
+      ge_instance_desc inst{};
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) inst.rows[r][c] = float(in[r * 3 + c]);
+      put(0, transform_normal(inst, V(9)));
+      return 0;
+    }
+    default: return GE_RT_ERROR_INVALID_ARGUMENT;
+  }
+}
+}  // namespace
+
+extern "C" int32_t ge_oracle_probe(uint32_t kind, uint32_t mode, const double* in, double* out) {
+  return mode == GE_ORACLE_FP64 ? probe<double>(kind, in, out) : probe<float>(kind, in, out);
+}

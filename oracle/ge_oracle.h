@@ -1,0 +1,90 @@
+/*
+ * ge_oracle.h — the CPU oracle's C ABI. TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker the HIP backend is compared against, and the CPU baseline the bench
+ * times beside it. Only the tests, the smoke check and the bench's cpu_baseline leg load it; the
+ * product (game_engine_amd/) never does.
+ *
+ * It restates, in scalar C++ compiled with -ffp-contract=off (the engine's CPU policy,
+ * cmake/EngineFpContraction.cmake:5-53), the reference's path-tracer sources:
+ *   domain/gfx/shaders/path_trace.slang, sampling.slang, brdf.slang, the material.slang and
+ *   scene.slang subsets the configurations use, view_ray.slang, ray_visibility.slang, and the shadow
+ *   query of visibility_resolve.slang:516-605,742-755,
+ * with a software BVH (LBVH: Morton codes, stable radix sort, Karras 2012 emission, bottom-up fit)
+ * standing in for the driver's acceleration structures and `RayQuery`, and a watertight
+ * ray/triangle test (Woop, Benthin, Wald 2013) standing in for the hardware's.
+ *
+ * Two arithmetic modes: GE_ORACLE_FP32 mirrors the shader's operation order in float; GE_ORACLE_FP64
+ * runs the same integrator in double (the BVH and traversal stay the float ones).
+ */
+#ifndef GE_ORACLE_H
+#define GE_ORACLE_H
+
+#include "../include/ge_rt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GE_ORACLE_FP32 0u
+#define GE_ORACLE_FP64 1u
+
+typedef struct ge_oracle_scene ge_oracle_scene;
+
+/* Builds every mesh's LBVH and the top level in the frame of `eye` (NULL: origin). */
+ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* desc, const ge_frame_eye* eye);
+void ge_oracle_scene_destroy(ge_oracle_scene* scene);
+int32_t ge_oracle_bvh_info(const ge_oracle_scene* scene, uint32_t mesh, uint32_t* node_count,
+                           uint32_t* prim_count);
+int32_t ge_oracle_bvh_download(const ge_oracle_scene* scene, uint32_t mesh, ge_bvh_node* nodes,
+                               ge_bvh_triangle* triangles, uint32_t* instance_slots,
+                               uint32_t* morton);
+/* Replaces instance records and rebuilds the top level (the per-frame placement). */
+int32_t ge_oracle_update_instances(ge_oracle_scene* scene, uint32_t first, uint32_t count,
+                                   const ge_instance_desc* instances, const ge_frame_eye* eye);
+
+/* Queries. `brute` = 1 tests every triangle of every instance instead of walking the BVH. */
+void ge_oracle_trace_closest(const ge_oracle_scene* scene, const ge_ray* rays, uint32_t count,
+                             uint32_t mask, ge_hit* hits, int32_t brute, int32_t threads);
+void ge_oracle_trace_any(const ge_oracle_scene* scene, const ge_ray* rays, uint32_t count,
+                         uint32_t mask, uint32_t* occluded, int32_t brute, int32_t threads);
+void ge_oracle_ray_visibility(const ge_oracle_scene* scene, const ge_ray_visibility_params* params,
+                              uint64_t* words, int32_t threads);
+/* One batch of path_trace_main over rows [row0, row1) (all rows: 0, height). */
+void ge_oracle_path_trace(const ge_oracle_scene* scene, const ge_path_trace_params* params,
+                          float* accum, uint32_t* coverage, uint32_t mode, uint32_t row0,
+                          uint32_t row1, int32_t threads);
+/* Same, FP64 accumulation into a double[4] per pixel (mode FP64 only; accum64 is w*h*4). */
+void ge_oracle_path_trace64(const ge_oracle_scene* scene, const ge_path_trace_params* params,
+                            double* accum64, uint32_t* coverage, uint32_t row0, uint32_t row1,
+                            int32_t threads);
+void ge_oracle_tonemap(const ge_path_trace_params* params, const float* accum,
+                       const uint32_t* coverage, uint32_t* output);
+void ge_oracle_shadow_pass(const ge_oracle_scene* scene, const ge_shadow_params* params,
+                           const ge_shadow_receiver* receivers, uint32_t* blocked, int32_t threads);
+/* Traversal counters accumulated since the last reset (closest, any, nodes, triangles, instances). */
+void ge_oracle_counters(uint64_t out[5]);
+void ge_oracle_reset_counters(void);
+int32_t ge_oracle_max_threads(void);
+
+/* ---- unit probes, for golden vectors and closed-form checks ---- */
+uint32_t ge_oracle_pcg_hash(uint32_t v);
+uint32_t ge_oracle_rng_seed(uint32_t x, uint32_t y, uint32_t sample_index, uint32_t seed);
+/* Advances `state` and returns the draw; `state` is updated in place. */
+float ge_oracle_rng_next(uint32_t* state);
+uint32_t ge_oracle_morton3(float cx, float cy, float cz, const float lo[3], const float scale[3]);
+void ge_oracle_sincos(float x, float* s, float* c);
+/* Sampler and BSDF, in double (mode FP64) or float (FP32): `in`/`out` layouts per kind, see the
+ * source. kind: 0 cosine hemisphere, 1 GGX VNDF sample, 2 ggx_vndf_pdf, 3 brdf_eval, 4 bsdf_pdf,
+ * 5 specular_probability, 6 onb_from_normal, 7 ggx_g1, 8 v_smith, 9 d_ggx, 10 fresnel,
+ * 11 mis_power_heuristic, 12 light_sample, 13 decode_normal_oct, 14 display_output,
+ * 15 transform_normal. */
+int32_t ge_oracle_probe(uint32_t kind, uint32_t mode, const double* in, double* out);
+/* Watertight ray/triangle test: out = {hit, t, u, v}. */
+void ge_oracle_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3],
+                         const float v2[3], float tmin, float tmax, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
