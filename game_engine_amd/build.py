@@ -1,0 +1,39 @@
+"""Builds the backend in-tree: libge_rt.so (HIP, gfx950) and libge_synth.so (host C++).
+
+HIP flags: -ffp-contract=off (no fused multiply-add: every kernel's arithmetic is the IEEE
+operation order written in the source), -fno-fast-math, correctly rounded division and square root
+(hipcc's HIP default), and -packed-fp32-ops disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+ARCH = os.environ.get("GE_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["rt_api.hip", "rt_bvh.hip", "rt_kernels.hip"]
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=CSRC)
+
+
+def build(verbose=True):
+    os.makedirs(LIB, exist_ok=True)
+    rt = os.path.join(LIB, "libge_rt.so")
+    synth = os.path.join(LIB, "libge_synth.so")
+    _run(["hipcc", f"--offload-arch={ARCH}", *HIP_FLAGS, "-o", rt + ".tmp", *HIP_SOURCES])
+    os.replace(rt + ".tmp", rt)
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-o", synth + ".tmp", "scene_synth.cpp"])
+    os.replace(synth + ".tmp", synth)
+    return rt, synth
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

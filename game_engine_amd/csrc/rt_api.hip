@@ -1,0 +1,768 @@
+// rt_api.hip — the C ABI (include/ge_rt.h): context, scene upload with validation, the build
+// orchestration (every mesh's LBVH in one segmented batch, then the instance level), and the
+// traced passes. Errors are a status and a per-context message, never an exception across the ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rt_internal.h"
+
+using namespace ge;
+
+namespace {
+
+int32_t set_error(ge_rt_ctx* ctx, int32_t code, const std::string& msg) {
+  if (ctx) {
+    ctx->error = msg;
+    ctx->last_status = code;
+  }
+  return code;
+}
+
+#define GE_HIP(ctx, call)                                                                                   \
+  do {                                                                                                      \
+    const hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess)                                                                                   \
+      return set_error(ctx, e_ == hipErrorOutOfMemory ? GE_RT_ERROR_OUT_OF_MEMORY : GE_RT_ERROR_DEVICE,     \
+                       std::string(#call) + ": " + hipGetErrorString(e_));                                 \
+  } while (0)
+
+#define GE_CHECK(ctx, cond, code, msg) \
+  do {                                 \
+    if (!(cond)) return set_error(ctx, code, msg); \
+  } while (0)
+
+template <class T>
+hipError_t upload(DevBuf& buf, const T* host, size_t count, hipStream_t stream) {
+  const size_t bytes = count * sizeof(T);
+  hipError_t e = buf.reserve(bytes ? bytes : 256);
+  if (e != hipSuccess || bytes == 0 || host == nullptr) return e;
+  return hipMemcpyAsync(buf.ptr, host, bytes, hipMemcpyHostToDevice, stream);
+}
+
+// engine::relative_axis (core/math/world.h:181-195), operation for operation.
+float relative_axis_host(int cell, float local, int eye_cell, float eye_local, float eye_residual) {
+  const float cells = float(cell - eye_cell) * 64.0f;
+  const float from_local = cells > 0.0f ? (cells - eye_local) + local : (cells + local) - eye_local;
+  return from_local - eye_residual;
+}
+
+// The frame-to-mesh 3x4: the linear part's inverse by its adjugate in double, rounded once to
+// float, and -inverse * translation in double.
+void make_w2o(const ge_instance_desc& inst, const float T[3], float w2o[3][4]) {
+  double m[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[r][c] = double(inst.rows[r][c]);
+  const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+  const double inv_det = det != 0.0 ? 1.0 / det : 0.0;
+  double inv[3][3];
+  inv[0][0] = c00 * inv_det;
+  inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv_det;
+  inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv_det;
+  inv[1][0] = c01 * inv_det;
+  inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv_det;
+  inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv_det;
+  inv[2][0] = c02 * inv_det;
+  inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv_det;
+  inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv_det;
+  for (int r = 0; r < 3; ++r) {
+    const double t = -(inv[r][0] * double(T[0]) + inv[r][1] * double(T[1]) + inv[r][2] * double(T[2]));
+    for (int c = 0; c < 3; ++c) w2o[r][c] = float(inv[r][c]);
+    w2o[r][3] = float(t);
+  }
+}
+
+Bvh bvh_view(const SceneDev& s) {
+  Bvh b{};
+  b.tlas = s.tlas_nodes.as<ge_bvh_node>();
+  b.tlas_slots = s.tlas_slots.as<uint32_t>();
+  b.inst = s.inst_recs.as<InstRec>();
+  b.nodes = s.blas_nodes.as<ge_bvh_node>();
+  b.tris = s.blas_tris.as<ge_bvh_triangle>();
+  b.tlas_node_count = s.tlas_node_count;
+  return b;
+}
+
+SceneView scene_view(const SceneDev& s) {
+  SceneView v{};
+  v.clusters = s.clusters.as<ge_cluster_desc>();
+  v.quantized = s.quantized.as<uint16_t>();
+  v.attributes = s.has_attributes ? s.attributes.as<ge_vertex_attributes>() : nullptr;
+  v.triangles = s.triangles.as<uint32_t>();
+  v.meshes = s.meshes.as<ge_mesh_desc>();
+  v.instances = s.instances.as<ge_instance_desc>();
+  v.materials = s.materials.as<ge_resolve_material>();
+  v.cluster_materials = s.cluster_materials.as<uint32_t>();
+  v.lights = s.lights.as<ge_resolve_light>();
+  v.light_count = s.light_count;
+  return v;
+}
+
+int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_desc* in, uint32_t count,
+                           const uint32_t* cluster_materials) {
+  for (uint32_t i = 0; i < count; ++i) {
+    const ge_instance_desc& inst = in[i];
+    if (inst.mesh >= s.mesh_count)
+      return set_error(ctx, GE_RT_ERROR_INVALID_SCENE, "instance " + std::to_string(i) + " names mesh " +
+                                                           std::to_string(inst.mesh) + " of " + std::to_string(s.mesh_count));
+    if (!(inst.flags & GE_INSTANCE_STATIC_RT))
+      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED,
+                       "instance " + std::to_string(i) + " is not a static structure (InstanceDesc::flags bit 7); "
+                       "the backend traces every instance through its mesh's own structure");
+    if (inst.deform != GE_INVALID_DEFORM)
+      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, "instance " + std::to_string(i) + " is deformed");
+    const ge_mesh_desc& m = s.h_meshes[inst.mesh];
+    for (uint32_t c = m.first_cluster; c < m.first_cluster + m.cluster_count; ++c) {
+      if (uint64_t(cluster_materials[c]) + inst.material_base >= s.material_count)
+        return set_error(ctx, GE_RT_ERROR_INVALID_SCENE,
+                         "instance " + std::to_string(i) + " cluster " + std::to_string(c) + " has no material");
+    }
+  }
+  return GE_RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal) {
+  if (!out) return GE_RT_ERROR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GE_RT_ERROR_DEVICE;
+  if (device_ordinal < 0 || device_ordinal >= count) return GE_RT_ERROR_INVALID_ARGUMENT;
+  auto* ctx = new (std::nothrow) ge_rt_ctx();
+  if (!ctx) return GE_RT_ERROR_OUT_OF_MEMORY;
+  ctx->device = device_ordinal;
+  if (hipSetDevice(device_ordinal) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+      hipEventCreate(&ctx->bev0) != hipSuccess || hipEventCreate(&ctx->bev1) != hipSuccess ||
+      ctx->counters.reserve(64) != hipSuccess || hipMemset(ctx->counters.ptr, 0, 64) != hipSuccess) {
+    ge_rt_destroy(ctx);
+    return GE_RT_ERROR_DEVICE;
+  }
+  *out = ctx;
+  return GE_RT_OK;
+}
+
+void ge_rt_destroy(ge_rt_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (DevBuf* b : {&ctx->build.keys[0], &ctx->build.keys[1], &ctx->build.vals[0], &ctx->build.vals[1], &ctx->build.hist,
+                    &ctx->build.scan_tmp, &ctx->build.flags, &ctx->build.node_box, &ctx->build.leaf_parent,
+                    &ctx->build.node_parent, &ctx->build.seg_cbox, &ctx->build.seg_scale, &ctx->counters})
+    b->release();
+  for (DevBuf& b : ctx->io) b.release();
+  for (DevBuf& b : ctx->tmp) b.release();
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->bev0) (void)hipEventDestroy(ctx->bev0);
+  if (ctx->bev1) (void)hipEventDestroy(ctx->bev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int32_t ge_rt_last_error(const ge_rt_ctx* ctx, char* buffer, size_t size) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  if (buffer && size) {
+    const size_t n = std::min(size - 1, ctx->error.size());
+    std::memcpy(buffer, ctx->error.data(), n);
+    buffer[n] = 0;
+  }
+  return ctx->last_status;
+}
+
+int32_t ge_rt_synchronize(ge_rt_ctx* ctx) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene** out) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, d && out, GE_RT_ERROR_INVALID_ARGUMENT, "null scene or output");
+  *out = nullptr;
+  GE_CHECK(ctx, (d->clusters || !d->cluster_count) && (d->vertices || !d->vertex_count) &&
+                    (d->quantized || !d->vertex_count) && (d->triangles || !d->triangle_count) &&
+                    (d->meshes || !d->mesh_count) && (d->instances || !d->instance_count) &&
+                    (d->materials || !d->material_count) && (d->cluster_materials || !d->cluster_count) &&
+                    (d->lights || !d->light_count),
+           GE_RT_ERROR_INVALID_ARGUMENT, "a scene array is null with a non-zero count");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  auto* scene = new (std::nothrow) ge_rt_scene();
+  GE_CHECK(ctx, scene, GE_RT_ERROR_OUT_OF_MEMORY, "host allocation");
+  SceneDev& s = scene->dev;
+  s.cluster_count = d->cluster_count;
+  s.vertex_count = d->vertex_count;
+  s.triangle_count = d->triangle_count;
+  s.mesh_count = d->mesh_count;
+  s.instance_count = d->instance_count;
+  s.material_count = d->material_count;
+  s.light_count = d->light_count;
+  s.has_attributes = d->attributes != nullptr;
+  // Validation: every offset and count inside its array, every local index inside its cluster.
+  auto fail = [&](int32_t code, const std::string& msg) {
+    delete scene;
+    return set_error(ctx, code, msg);
+  };
+  for (uint32_t c = 0; c < d->cluster_count; ++c) {
+    const ge_cluster_desc& cl = d->clusters[c];
+    if (uint64_t(cl.vertex_offset) + cl.vertex_count > d->vertex_count ||
+        uint64_t(cl.triangle_offset) + cl.triangle_count > d->triangle_count)
+      return fail(GE_RT_ERROR_INVALID_SCENE, "cluster " + std::to_string(c) + " reaches past the vertex or triangle array");
+    if (cl.triangle_count > 256 || cl.vertex_count > 256)
+      return fail(GE_RT_ERROR_INVALID_SCENE, "cluster " + std::to_string(c) + " holds more than 256 triangles or vertices");
+    for (uint32_t t = 0; t < cl.triangle_count; ++t) {
+      const uint32_t p = d->triangles[cl.triangle_offset + t];
+      if ((p & 0xffu) >= cl.vertex_count || ((p >> 8) & 0xffu) >= cl.vertex_count || ((p >> 16) & 0xffu) >= cl.vertex_count)
+        return fail(GE_RT_ERROR_INVALID_SCENE, "cluster " + std::to_string(c) + " triangle " + std::to_string(t) +
+                                                   " names a vertex outside the cluster");
+    }
+  }
+  s.h_clusters.assign(d->clusters, d->clusters + d->cluster_count);
+  s.h_meshes.assign(d->meshes, d->meshes + d->mesh_count);
+  s.mesh_tri_count.assign(d->mesh_count, 0);
+  s.mesh_tri_base.assign(d->mesh_count, 0);
+  s.mesh_node_base.assign(d->mesh_count, 0);
+  s.mesh_node_count.assign(d->mesh_count, 0);
+  uint64_t tri_pool = 0, node_pool = 0;
+  for (uint32_t m = 0; m < d->mesh_count; ++m) {
+    const ge_mesh_desc& mesh = d->meshes[m];
+    if (uint64_t(mesh.first_cluster) + mesh.cluster_count > d->cluster_count || mesh.cluster_count >= (1u << 24))
+      return fail(GE_RT_ERROR_INVALID_SCENE, "mesh " + std::to_string(m) + " reaches past the cluster array");
+    if (mesh.deform_pool || mesh.deform_slots || mesh.templates || mesh.skin)
+      return fail(GE_RT_ERROR_UNSUPPORTED, "mesh " + std::to_string(m) + " carries deformation streams");
+    uint64_t n = 0;
+    for (uint32_t c = mesh.first_cluster; c < mesh.first_cluster + mesh.cluster_count; ++c) n += d->clusters[c].triangle_count;
+    if (n >= (1u << 30)) return fail(GE_RT_ERROR_INVALID_SCENE, "mesh " + std::to_string(m) + " is too large");
+    s.mesh_tri_count[m] = uint32_t(n);
+    s.mesh_tri_base[m] = uint32_t(tri_pool);
+    s.mesh_node_base[m] = uint32_t(node_pool);
+    s.mesh_node_count[m] = n == 0 ? 0u : (n == 1 ? 1u : uint32_t(n - 1));
+    tri_pool += n;
+    node_pool += s.mesh_node_count[m];
+  }
+  if (tri_pool >= (1ull << 31) || node_pool >= (1ull << 31))
+    return fail(GE_RT_ERROR_INVALID_SCENE, "the scene's meshes hold more than 2^31 triangles");
+  s.tri_pool = uint32_t(tri_pool);
+  s.node_pool = uint32_t(node_pool);
+  for (uint32_t k = 0; k < d->material_count; ++k) {
+    const ge_resolve_material& mat = d->materials[k];
+    if (mat.albedo_texture != GE_NO_TEXTURE || mat.metallic_roughness_texture != GE_NO_TEXTURE ||
+        mat.normal_texture != GE_NO_TEXTURE || mat.emissive_texture != GE_NO_TEXTURE)
+      return fail(GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(k) + " is textured");
+    if ((mat.flags >> 24) != 0u) return fail(GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(k) + " declares foliage optics");
+  }
+  {
+    const int32_t st = validate_instances(ctx, s, d->instances, d->instance_count, d->cluster_materials);
+    if (st != GE_RT_OK) {
+      const std::string msg = ctx->error;
+      return fail(st, msg);
+    }
+  }
+  s.h_instances.assign(d->instances, d->instances + d->instance_count);
+  const hipStream_t st = ctx->stream;
+  const size_t qn = (size_t(d->vertex_count) * 3 + 1) & ~size_t(1);
+  std::vector<ge_mesh_desc> meshes(d->meshes, d->meshes + d->mesh_count);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = upload(s.clusters, d->clusters, d->cluster_count, st);
+  if (e == hipSuccess) e = upload(s.vertices, d->vertices, size_t(d->vertex_count) * 3, st);
+  if (e == hipSuccess) e = upload(s.quantized, d->quantized, qn, st);
+  if (e == hipSuccess) e = upload(s.attributes, d->attributes, s.has_attributes ? d->vertex_count : 0, st);
+  if (e == hipSuccess) e = upload(s.triangles, d->triangles, d->triangle_count, st);
+  for (ge_mesh_desc& m : meshes) m.quantized = reinterpret_cast<uint64_t>(s.quantized.ptr);
+  if (e == hipSuccess) e = upload(s.meshes, meshes.data(), meshes.size(), st);
+  if (e == hipSuccess) e = upload(s.instances, d->instances, d->instance_count, st);
+  if (e == hipSuccess) e = upload(s.materials, d->materials, d->material_count, st);
+  if (e == hipSuccess) e = upload(s.cluster_materials, d->cluster_materials, d->cluster_count, st);
+  if (e == hipSuccess) e = upload(s.lights, d->lights, d->light_count, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? GE_RT_ERROR_OUT_OF_MEMORY : GE_RT_ERROR_DEVICE,
+                std::string("scene upload: ") + hipGetErrorString(e));
+  *out = scene;
+  return GE_RT_OK;
+}
+
+void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
+  if (!scene) return;
+  if (ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  SceneDev& s = scene->dev;
+  for (DevBuf* b : {&s.clusters, &s.vertices, &s.quantized, &s.attributes, &s.triangles, &s.meshes, &s.instances,
+                    &s.materials, &s.cluster_materials, &s.lights, &s.tri_prefix, &s.tri_mesh_of, &s.blas_nodes,
+                    &s.blas_tris, &s.blas_morton, &s.mesh_roots, &s.mesh_height, &s.tlas_nodes, &s.tlas_slots,
+                    &s.tlas_morton, &s.inst_recs, &s.inst_boxes, &s.live_ids, &s.tlas_root, &s.tlas_height})
+    b->release();
+  delete scene;
+}
+
+int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
+                                     const ge_instance_desc* instances) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && (instances || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or instances");
+  SceneDev& s = scene->dev;
+  GE_CHECK(ctx, uint64_t(first) + count <= s.instance_count, GE_RT_ERROR_INVALID_ARGUMENT, "instance range out of bounds");
+  std::vector<uint32_t> cm(s.cluster_count);
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  GE_HIP(ctx, hipMemcpy(cm.data(), s.cluster_materials.ptr, size_t(s.cluster_count) * 4, hipMemcpyDeviceToHost));
+  const int32_t st = validate_instances(ctx, s, instances, count, cm.data());
+  if (st != GE_RT_OK) return st;
+  std::memcpy(&s.h_instances[first], instances, size_t(count) * sizeof(ge_instance_desc));
+  GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, instances, size_t(count) * sizeof(ge_instance_desc),
+                             hipMemcpyHostToDevice, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, const ge_frame_eye* eye, const uint32_t* meshes,
+                        uint32_t count) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene, GE_RT_ERROR_INVALID_ARGUMENT, "null scene");
+  GE_CHECK(ctx, flags <= GE_RT_BUILD_TOP, GE_RT_ERROR_INVALID_ARGUMENT, "unknown build flags");
+  SceneDev& s = scene->dev;
+  GE_CHECK(ctx, flags == GE_RT_BUILD_FULL || s.built, GE_RT_ERROR_INVALID_SCENE, "a partial build needs a full one first");
+  GE_CHECK(ctx, flags != GE_RT_BUILD_SUBSET || meshes || !count, GE_RT_ERROR_INVALID_ARGUMENT, "null mesh list");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  const hipStream_t st = ctx->stream;
+  s.eye = eye ? *eye : ge_frame_eye{};
+  GE_HIP(ctx, s.blas_nodes.reserve(size_t(std::max(s.node_pool, 1u)) * sizeof(ge_bvh_node)));
+  GE_HIP(ctx, s.blas_tris.reserve(size_t(std::max(s.tri_pool, 1u)) * sizeof(ge_bvh_triangle)));
+  GE_HIP(ctx, s.blas_morton.reserve(size_t(std::max(s.tri_pool, 1u)) * 4));
+  GE_HIP(ctx, s.mesh_roots.reserve(size_t(std::max(s.mesh_count, 1u)) * 24));
+  GE_HIP(ctx, s.mesh_height.reserve(size_t(std::max(s.mesh_count, 1u)) * 4));
+  GE_HIP(ctx, hipEventRecord(ctx->bev0, st));
+
+  // ---- the meshes ----
+  std::vector<uint32_t> list;
+  if (flags == GE_RT_BUILD_FULL) {
+    for (uint32_t m = 0; m < s.mesh_count; ++m) list.push_back(m);
+  } else if (flags == GE_RT_BUILD_SUBSET) {
+    for (uint32_t k = 0; k < count; ++k) {
+      GE_CHECK(ctx, meshes[k] < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
+      list.push_back(meshes[k]);
+    }
+    std::sort(list.begin(), list.end());
+    list.erase(std::unique(list.begin(), list.end()), list.end());
+  }
+  std::vector<uint32_t> seg_first, seg_count, seg_node, seg_out, seg_root;
+  std::vector<uint32_t> cl_list, cl_out, cl_seg, cl_local;
+  uint32_t n = 0;
+  for (uint32_t m : list) {
+    if (s.mesh_tri_count[m] == 0) continue;
+    const uint32_t sidx = uint32_t(seg_first.size());
+    seg_first.push_back(n);
+    seg_count.push_back(s.mesh_tri_count[m]);
+    seg_node.push_back(s.mesh_node_base[m]);
+    seg_out.push_back(s.mesh_tri_base[m]);
+    seg_root.push_back(m);
+    const ge_mesh_desc& mesh = s.h_meshes[m];
+    uint32_t off = n;
+    for (uint32_t c = 0; c < mesh.cluster_count; ++c) {
+      const uint32_t gc = mesh.first_cluster + c;
+      const uint32_t tc = s.h_clusters[gc].triangle_count;
+      if (tc == 0) continue;
+      cl_list.push_back(gc);
+      cl_out.push_back(off);
+      cl_seg.push_back(sidx);
+      cl_local.push_back(c);
+      off += tc;
+    }
+    n = off;
+  }
+  std::vector<uint32_t> meta;  // every small array of the batch, packed, one upload
+  auto pack = [&](const std::vector<uint32_t>& v) {
+    const size_t at = meta.size();
+    meta.insert(meta.end(), v.begin(), v.end());
+    while (meta.size() % 64) meta.push_back(0);
+    return at;
+  };
+  if (n > 0) {
+    const size_t o_sf = pack(seg_first), o_sc = pack(seg_count), o_sn = pack(seg_node), o_so = pack(seg_out),
+                 o_sr = pack(seg_root), o_cl = pack(cl_list), o_co = pack(cl_out), o_cs = pack(cl_seg), o_cc = pack(cl_local);
+    GE_HIP(ctx, upload(ctx->tmp[0], meta.data(), meta.size(), st));
+    const uint32_t* md = ctx->tmp[0].as<uint32_t>();
+    GE_HIP(ctx, ctx->tmp[1].reserve(size_t(n) * sizeof(ge_bvh_triangle)));  // records in batch order
+    GE_HIP(ctx, ctx->tmp[2].reserve(size_t(n) * 24));                       // boxes
+    GE_HIP(ctx, ctx->tmp[3].reserve(size_t(n) * 4));                        // segment of each primitive
+    GE_HIP(ctx, ctx->tmp[4].reserve(size_t(n) * 4));                        // sorted order
+    GE_HIP(ctx, ctx->tmp[5].reserve(size_t(n) * 4));                        // sorted codes
+    GE_HIP(ctx, launch_triangle_records(st, s, uint32_t(cl_list.size()), md + o_cl, md + o_co, md + o_cs, md + o_cc,
+                                        ctx->tmp[1].as<ge_bvh_triangle>(), ctx->tmp[2].as<float>(), ctx->tmp[3].as<uint32_t>()));
+    GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), uint32_t(seg_first.size()), ctx->tmp[3].as<uint32_t>(),
+                               md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
+                               ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
+                               std::max(s.node_pool, 1u)));
+    GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
+                                        ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
+                                        ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));
+    GE_HIP(ctx, hipStreamSynchronize(st));  // the packed metadata lives in host memory until here
+  }
+
+  // ---- the instances ----
+  const uint32_t ni = s.instance_count;
+  std::vector<InstRec> recs(ni);
+  std::vector<float> T(size_t(std::max(ni, 1u)) * 3, 0.0f);
+  std::vector<uint32_t> live;
+  for (uint32_t i = 0; i < ni; ++i) {
+    const ge_instance_desc& in = s.h_instances[i];
+    for (int r = 0; r < 3; ++r)
+      T[size_t(i) * 3 + r] = relative_axis_host(in.cell[r], in.rows[r][3], s.eye.cell[r], s.eye.local[r], s.eye.residual[r]);
+    InstRec& rec = recs[i];
+    make_w2o(in, &T[size_t(i) * 3], rec.w2o);
+    rec.node_base = s.mesh_node_base[in.mesh];
+    rec.tri_base = s.mesh_tri_base[in.mesh];
+    rec.first_pair = in.first_pair;
+    rec.live = s.mesh_tri_count[in.mesh] > 0 ? 1u : 0u;
+    if (rec.live) live.push_back(i);
+  }
+  const uint32_t nl = uint32_t(live.size());
+  s.tlas_prims = nl;
+  s.tlas_node_count = nl == 0 ? 0u : (nl == 1 ? 1u : nl - 1);
+  GE_HIP(ctx, upload(s.inst_recs, recs.data(), recs.size(), st));
+  GE_HIP(ctx, upload(s.live_ids, live.data(), live.size(), st));
+  GE_HIP(ctx, s.tlas_nodes.reserve(size_t(std::max(s.tlas_node_count, 1u)) * sizeof(ge_bvh_node)));
+  GE_HIP(ctx, s.tlas_slots.reserve(size_t(std::max(nl, 1u)) * 4));
+  GE_HIP(ctx, s.tlas_morton.reserve(size_t(std::max(nl, 1u)) * 4));
+  GE_HIP(ctx, s.tlas_root.reserve(32));
+  GE_HIP(ctx, s.tlas_height.reserve(4));
+  if (nl > 0) {
+    const std::vector<uint32_t> tmeta = {0u, nl, 0u};  // seg_first, seg_count, node_base
+    GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + size_t(ni) * 12 + 256));
+    float* boxes = ctx->tmp[6].as<float>();
+    float* d_T = boxes + size_t(nl) * 6;
+    GE_HIP(ctx, hipMemcpyAsync(d_T, T.data(), size_t(ni) * 12, hipMemcpyHostToDevice, st));
+    GE_HIP(ctx, upload(ctx->tmp[7], tmeta.data(), tmeta.size(), st));
+    GE_HIP(ctx, ctx->tmp[8].reserve(size_t(nl) * 12));
+    uint32_t* seg_of = ctx->tmp[8].as<uint32_t>();
+    uint32_t* perm = seg_of + nl;
+    uint32_t* codes = perm + nl;
+    const uint32_t* tm = ctx->tmp[7].as<uint32_t>();
+    GE_HIP(ctx, launch_instance_boxes(st, nl, s.live_ids.as<uint32_t>(), s.instances.as<ge_instance_desc>(), d_T,
+                                      s.mesh_roots.as<float>(), boxes, seg_of));
+    GE_HIP(ctx, build_segments(ctx->build, st, nl, boxes, 1, seg_of, tm, tm + 1, tm + 2, s.tlas_nodes.as<ge_bvh_node>(), perm,
+                               codes, s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), nullptr, s.tlas_node_count));
+    GE_HIP(ctx, launch_gather_slots(st, nl, perm, s.live_ids.as<uint32_t>(), s.tlas_slots.as<uint32_t>()));
+    GE_HIP(ctx, hipMemcpyAsync(s.tlas_morton.ptr, codes, size_t(nl) * 4, hipMemcpyDeviceToDevice, st));
+  }
+  GE_HIP(ctx, hipEventRecord(ctx->bev1, st));
+  GE_HIP(ctx, hipStreamSynchronize(st));
+  float ms = 0.0f;
+  GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->bev0, ctx->bev1));
+  ctx->last_build_ms = ms;
+  // The traversal stack holds at most one deferred sibling per level of each structure plus the
+  // sentinel between them.
+  s.h_mesh_height.assign(s.mesh_count, 0);
+  if (s.mesh_count) GE_HIP(ctx, hipMemcpy(s.h_mesh_height.data(), s.mesh_height.ptr, size_t(s.mesh_count) * 4, hipMemcpyDeviceToHost));
+  s.tlas_height_value = 0;
+  if (nl > 0) GE_HIP(ctx, hipMemcpy(&s.tlas_height_value, s.tlas_height.ptr, 4, hipMemcpyDeviceToHost));
+  uint32_t blas_max = 0;
+  for (uint32_t m = 0; m < s.mesh_count; ++m)
+    if (s.mesh_tri_count[m] > 0) blas_max = std::max(blas_max, s.h_mesh_height[m]);
+  s.built = false;
+  GE_CHECK(ctx, s.tlas_height_value + blas_max + 2 <= uint32_t(k_stack), GE_RT_ERROR_UNSUPPORTED,
+           "the structures are " + std::to_string(s.tlas_height_value) + " + " + std::to_string(blas_max) +
+               " levels deep, past the traversal stack of " + std::to_string(k_stack));
+  s.built = true;
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, uint32_t* node_count, uint32_t* prim_count) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && scene->dev.built, GE_RT_ERROR_INVALID_SCENE, "no structure built");
+  const SceneDev& s = scene->dev;
+  if (mesh == 0xffffffffu) {
+    if (node_count) *node_count = s.tlas_node_count;
+    if (prim_count) *prim_count = s.tlas_prims;
+    return GE_RT_OK;
+  }
+  GE_CHECK(ctx, mesh < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
+  if (node_count) *node_count = s.mesh_node_count[mesh];
+  if (prim_count) *prim_count = s.mesh_tri_count[mesh];
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_bvh_download(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, ge_bvh_node* nodes,
+                           ge_bvh_triangle* triangles, uint32_t* instance_slots, uint32_t* morton) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && scene->dev.built, GE_RT_ERROR_INVALID_SCENE, "no structure built");
+  const SceneDev& s = scene->dev;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  if (mesh == 0xffffffffu) {
+    if (nodes && s.tlas_node_count)
+      GE_HIP(ctx, hipMemcpy(nodes, s.tlas_nodes.ptr, size_t(s.tlas_node_count) * sizeof(ge_bvh_node), hipMemcpyDeviceToHost));
+    if (instance_slots && s.tlas_prims)
+      GE_HIP(ctx, hipMemcpy(instance_slots, s.tlas_slots.ptr, size_t(s.tlas_prims) * 4, hipMemcpyDeviceToHost));
+    if (morton && s.tlas_prims) GE_HIP(ctx, hipMemcpy(morton, s.tlas_morton.ptr, size_t(s.tlas_prims) * 4, hipMemcpyDeviceToHost));
+    return GE_RT_OK;
+  }
+  GE_CHECK(ctx, mesh < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
+  const uint32_t nn = s.mesh_node_count[mesh], np = s.mesh_tri_count[mesh];
+  if (nodes && nn)
+    GE_HIP(ctx, hipMemcpy(nodes, s.blas_nodes.as<ge_bvh_node>() + s.mesh_node_base[mesh], size_t(nn) * sizeof(ge_bvh_node),
+                          hipMemcpyDeviceToHost));
+  if (triangles && np)
+    GE_HIP(ctx, hipMemcpy(triangles, s.blas_tris.as<ge_bvh_triangle>() + s.mesh_tri_base[mesh],
+                          size_t(np) * sizeof(ge_bvh_triangle), hipMemcpyDeviceToHost));
+  if (morton && np)
+    GE_HIP(ctx, hipMemcpy(morton, s.blas_morton.as<uint32_t>() + s.mesh_tri_base[mesh], size_t(np) * 4, hipMemcpyDeviceToHost));
+  return GE_RT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host-pointer staging: device pointer for a caller buffer, copied in when `in` is set.
+template <class T>
+int32_t stage(ge_rt_ctx* ctx, int slot, const T* host, size_t count, bool copy_in, T** dev) {
+  GE_HIP(ctx, ctx->io[slot].reserve(std::max<size_t>(count * sizeof(T), 256)));
+  *dev = ctx->io[slot].as<T>();
+  if (copy_in && count) GE_HIP(ctx, hipMemcpyAsync(*dev, host, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t finish(ge_rt_ctx* ctx, bool sync) {
+  GE_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->timing_pending = true;
+  if (sync) GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t ready(ge_rt_ctx* ctx, const ge_rt_scene* scene) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && scene->dev.built, GE_RT_ERROR_INVALID_SCENE, "no structure built: call ge_rt_bvh_build");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  return GE_RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* rays, uint32_t count, uint32_t mask,
+                            ge_hit* hits, uint32_t flags) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, (rays && hits) || count == 0, GE_RT_ERROR_INVALID_ARGUMENT, "null rays or hits");
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  const ge_ray* d_rays = rays;
+  ge_hit* d_hits = hits;
+  if (!dev) {
+    ge_ray* dr;
+    if ((r = stage(ctx, 0, rays, count, true, &dr)) != GE_RT_OK) return r;
+    d_rays = dr;
+    if ((r = stage(ctx, 1, hits, count, false, &d_hits)) != GE_RT_OK) return r;
+  }
+  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, launch_trace_closest(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_hits,
+                                   (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) {
+    GE_HIP(ctx, hipMemcpyAsync(hits, d_hits, size_t(count) * sizeof(ge_hit), hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_trace_any(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* rays, uint32_t count, uint32_t mask,
+                        uint32_t* occluded, uint32_t flags) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, (rays && occluded) || count == 0, GE_RT_ERROR_INVALID_ARGUMENT, "null rays or output");
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  const size_t words = (size_t(count) + 31) / 32;
+  const ge_ray* d_rays = rays;
+  uint32_t* d_bits = occluded;
+  if (!dev) {
+    ge_ray* dr;
+    if ((r = stage(ctx, 0, rays, count, true, &dr)) != GE_RT_OK) return r;
+    d_rays = dr;
+    if ((r = stage(ctx, 1, occluded, words, false, &d_bits)) != GE_RT_OK) return r;
+  }
+  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, launch_trace_any(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_bits,
+                               (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) {
+    GE_HIP(ctx, hipMemcpyAsync(occluded, d_bits, words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray_visibility_params* params,
+                             uint64_t* words, uint32_t flags) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
+  GE_CHECK(ctx, params->masks[0] == 0, GE_RT_ERROR_UNSUPPORTED, "alpha-tested materials (MaskTables) are not supported");
+  GE_CHECK(ctx, params->width > 0 && params->height > 0, GE_RT_ERROR_INVALID_ARGUMENT, "empty extent");
+  const size_t n = size_t(params->width) * params->height;
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  uint64_t* out = dev ? (words ? words : reinterpret_cast<uint64_t*>(params->output)) : nullptr;
+  GE_CHECK(ctx, !dev || out, GE_RT_ERROR_INVALID_ARGUMENT, "null output");
+  GE_CHECK(ctx, dev || words, GE_RT_ERROR_INVALID_ARGUMENT, "null output");
+  if (!dev && (r = stage(ctx, 0, words, n, false, &out)) != GE_RT_OK) return r;
+  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, launch_ray_visibility(ctx->stream, bvh_view(scene->dev), *params, out,
+                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) {
+    GE_HIP(ctx, hipMemcpyAsync(words, out, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params, float* accum,
+                         uint32_t* coverage, uint32_t flags) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
+  GE_CHECK(ctx, params->width > 0 && params->height > 0, GE_RT_ERROR_INVALID_ARGUMENT, "empty extent");
+  GE_CHECK(ctx, !(params->flags & GE_PT_ALPHA_MASKS), GE_RT_ERROR_UNSUPPORTED, "alpha-tested materials are not supported");
+  GE_CHECK(ctx, params->sky_params == 0, GE_RT_ERROR_UNSUPPORTED, "the physical sky (sky_params) is not supported");
+  GE_CHECK(ctx, params->ground_detail == 0, GE_RT_ERROR_UNSUPPORTED, "ground detail is not supported");
+  GE_CHECK(ctx, params->light_count <= scene->dev.light_count, GE_RT_ERROR_INVALID_ARGUMENT,
+           "light_count exceeds the scene's uploaded lights");
+  const size_t n = size_t(params->width) * params->height;
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  float4* d_accum;
+  uint32_t* d_cov;
+  if (dev) {
+    d_accum = reinterpret_cast<float4*>(accum ? accum : reinterpret_cast<float*>(params->accum));
+    d_cov = coverage ? coverage : reinterpret_cast<uint32_t*>(params->coverage);
+    GE_CHECK(ctx, d_accum && d_cov, GE_RT_ERROR_INVALID_ARGUMENT, "null accumulator or coverage");
+  } else {
+    GE_CHECK(ctx, accum && coverage, GE_RT_ERROR_INVALID_ARGUMENT, "null accumulator or coverage");
+    const bool resume = params->sample_base != 0;
+    float* da;
+    if ((r = stage(ctx, 0, accum, n * 4, resume, &da)) != GE_RT_OK) return r;
+    d_accum = reinterpret_cast<float4*>(da);
+    if ((r = stage(ctx, 1, coverage, n, resume, &d_cov)) != GE_RT_OK) return r;
+  }
+  ge_path_trace_params p = *params;
+  const SceneView sv = scene_view(scene->dev);
+  // The block's scene addresses, as the renderer's GpuScene would have filled them.
+  p.clusters = reinterpret_cast<uint64_t>(sv.clusters);
+  p.mesh = reinterpret_cast<uint64_t>(sv.meshes);
+  p.triangles = reinterpret_cast<uint64_t>(sv.triangles);
+  p.instances = reinterpret_cast<uint64_t>(sv.instances);
+  p.attributes = reinterpret_cast<uint64_t>(sv.attributes);
+  p.materials = reinterpret_cast<uint64_t>(sv.materials);
+  p.cluster_materials = reinterpret_cast<uint64_t>(sv.cluster_materials);
+  p.lights = reinterpret_cast<uint64_t>(sv.lights);
+  p.accum = reinterpret_cast<uint64_t>(d_accum);
+  p.coverage = reinterpret_cast<uint64_t>(d_cov);
+  SceneView svl = sv;
+  svl.light_count = params->light_count;
+  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, launch_path_trace(ctx->stream, bvh_view(scene->dev), svl, p, d_accum, d_cov,
+                                (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) {
+    GE_HIP(ctx, hipMemcpyAsync(accum, d_accum, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipMemcpyAsync(coverage, d_cov, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_tonemap(ge_rt_ctx* ctx, const ge_path_trace_params* params, const float* accum, const uint32_t* coverage,
+                      uint32_t* output, uint32_t flags) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, params && accum && coverage && output, GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  GE_CHECK(ctx, params->sky_params == 0, GE_RT_ERROR_UNSUPPORTED, "the physical sky (sky_params) is not supported");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  int32_t r;
+  const size_t n = size_t(params->width) * params->height;
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  const float* da = accum;
+  const uint32_t* dc = coverage;
+  uint32_t* dout = output;
+  if (!dev) {
+    float* a;
+    uint32_t* c;
+    if ((r = stage(ctx, 0, accum, n * 4, true, &a)) != GE_RT_OK) return r;
+    if ((r = stage(ctx, 1, coverage, n, true, &c)) != GE_RT_OK) return r;
+    if ((r = stage(ctx, 2, output, n, false, &dout)) != GE_RT_OK) return r;
+    da = a;
+    dc = c;
+  }
+  GE_HIP(ctx, launch_tonemap(ctx->stream, *params, reinterpret_cast<const float4*>(da), dc, dout));
+  if (!dev) GE_HIP(ctx, hipMemcpyAsync(output, dout, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_params* params,
+                          const ge_shadow_receiver* receivers, uint32_t* blocked, uint32_t flags) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, params && ((receivers && blocked) || params->count == 0), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  const uint32_t n = params->count;
+  const size_t words = (size_t(n) + 31) / 32;
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  const ge_shadow_receiver* dr = receivers;
+  uint32_t* db = blocked;
+  if (!dev) {
+    ge_shadow_receiver* rr;
+    if ((r = stage(ctx, 0, receivers, n, true, &rr)) != GE_RT_OK) return r;
+    dr = rr;
+    if ((r = stage(ctx, 1, blocked, words, false, &db)) != GE_RT_OK) return r;
+  }
+  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, launch_shadow_pass(ctx->stream, bvh_view(scene->dev), scene_view(scene->dev), *params, dr, db,
+                                 (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) {
+    GE_HIP(ctx, hipMemcpyAsync(blocked, db, words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, out, GE_RT_ERROR_INVALID_ARGUMENT, "null output");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->timing_pending) {
+    float ms = 0.0f;
+    GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    ctx->timing_pending = false;
+  }
+  unsigned long long c[5];
+  GE_HIP(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+  out->closest_rays = c[0];
+  out->any_rays = c[1];
+  out->nodes = c[2];
+  out->triangles = c[3];
+  out->instances = c[4];
+  out->last_kernel_ms = ctx->last_kernel_ms;
+  out->last_build_ms = ctx->last_build_ms;
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_reset_stats(ge_rt_ctx* ctx) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 64, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+}  // extern "C"

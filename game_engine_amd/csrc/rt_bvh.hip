@@ -1,0 +1,533 @@
+// rt_bvh.hip — the LBVH build on the GPU: segmented over many meshes in one set of launches.
+//
+// Per primitive: a box and the 30-bit Morton code of its centroid, quantized on its segment's
+// centroid bounds (10 bits an axis). A stable LSD radix sort on (segment << 30 | code) orders each
+// segment by code with ties in primitive order. Karras 2012 emits each segment's binary radix tree
+// from the sorted codes (one thread per internal node, the index breaking equal codes), and a
+// bottom-up pass fits the boxes (the second child to finish computes its parent; min and max are
+// exact, so the result does not depend on which one that is). Everything is integer work or exact
+// float min/max, so the structure is bit-identical to the oracle's sequential build.
+//
+// Memory: keys 8 B + values 4 B per primitive, double-buffered, a 256 x tiles histogram, and the
+// nodes themselves (64 B each). Bytes per triangle of the whole build are stated beside the
+// measured build time.
+#include "rt_internal.h"
+
+namespace ge {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Float <-> order-preserving uint, for min/max by integer atomics.
+__device__ __forceinline__ uint32_t ordered(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unordered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t quantize10(float c, float lo, float scale) {
+  const float q = __builtin_fminf(__builtin_fmaxf((c - lo) * scale, 0.0f), 1023.0f);
+  return uint32_t(q);
+}
+
+__global__ void k_init_bounds(uint32_t nseg, uint32_t* cbox) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  for (int a = 0; a < 3; ++a) {
+    cbox[s * 6 + a] = 0xffffffffu;  // min accumulator
+    cbox[s * 6 + 3 + a] = 0u;       // max accumulator
+  }
+}
+
+__global__ void k_centroid_bounds(uint32_t n, const float* __restrict__ boxes, const uint32_t* __restrict__ seg_of,
+                                  uint32_t* cbox) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = seg_of[i];
+  for (int a = 0; a < 3; ++a) {
+    const float c = (boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f;
+    atomicMin(&cbox[s * 6 + a], ordered(c));
+    atomicMax(&cbox[s * 6 + 3 + a], ordered(c));
+  }
+}
+
+__global__ void k_segment_scale(uint32_t nseg, const uint32_t* cbox, float* lo_scale) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  for (int a = 0; a < 3; ++a) {
+    const float lo = unordered(cbox[s * 6 + a]);
+    const float hi = unordered(cbox[s * 6 + 3 + a]);
+    const float ext = hi - lo;
+    lo_scale[s * 6 + a] = lo;
+    lo_scale[s * 6 + 3 + a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+  }
+}
+
+__global__ void k_morton(uint32_t n, const float* __restrict__ boxes, const uint32_t* __restrict__ seg_of,
+                         const float* __restrict__ lo_scale, uint64_t* keys, uint32_t* vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = seg_of[i];
+  uint32_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    const float c = (boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f;
+    q[a] = quantize10(c, lo_scale[s * 6 + a], lo_scale[s * 6 + 3 + a]);
+  }
+  const uint32_t code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  keys[i] = (uint64_t(s) << 30) | code;
+  vals[i] = i;
+}
+
+// ---- stable LSD radix sort, 8-bit digits -------------------------------------------------------
+__global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* __restrict__ keys, uint32_t n, int shift,
+                                                            uint32_t* hist, uint32_t tiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t i = base + uint32_t(k) * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[size_t(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* __restrict__ kin,
+                                                               const uint32_t* __restrict__ vin, uint64_t* kout,
+                                                               uint32_t* vout, uint32_t n, int shift,
+                                                               const uint32_t* __restrict__ offsets, uint32_t tiles) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wave_counts[kSortThreads / 64][256];
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & 63u, wave = t >> 6;
+  running[t] = offsets[size_t(t) * tiles + blockIdx.x];
+  for (int w = 0; w < kSortThreads / 64; ++w) wave_counts[w][t] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t i = base + uint32_t(k) * kSortThreads + t;
+    const bool valid = i < n;
+    const uint64_t key = valid ? kin[i] : 0ull;
+    const uint32_t d = uint32_t(key >> shift) & 255u;
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bits = __ballot((d >> b) & 1u);
+      match &= ((d >> b) & 1u) ? bits : ~bits;
+    }
+    const uint32_t below = uint32_t(__popcll(match & lt));
+    if (valid && below == 0) wave_counts[wave][d] = uint32_t(__popcll(match));
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + below;
+      for (uint32_t w = 0; w < wave; ++w) pos += wave_counts[w][d];
+      kout[pos] = key;
+      vout[pos] = vin[i];
+    }
+    __syncthreads();
+    uint32_t sum = 0;
+    for (int w = 0; w < kSortThreads / 64; ++w) {
+      sum += wave_counts[w][t];
+      wave_counts[w][t] = 0;
+    }
+    running[t] += sum;
+    __syncthreads();
+  }
+}
+
+// Exclusive scan of u32, in tiles of kScanTile with a recursive pass over the tile sums.
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(uint32_t* data, uint32_t n, uint32_t* sums) {
+  __shared__ uint32_t s[kScanThreads];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < n) ? data[base + k] : 0u;
+    local += v[k];
+  }
+  s[threadIdx.x] = local;
+  __syncthreads();
+  for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
+    const uint32_t add = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += add;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - local;
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) data[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kScanThreads - 1 && sums) sums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void k_scan_add(uint32_t* data, uint32_t n, const uint32_t* sums) {
+  const uint32_t i = blockIdx.x * kScanThreads * kScanItems + threadIdx.x;
+  const uint32_t add = sums[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t j = i + uint32_t(k) * kScanThreads;
+    if (j < n) data[j] += add;
+  }
+}
+
+hipError_t exclusive_scan(uint32_t* data, uint32_t n, DevBuf& tmp, size_t tmp_offset, hipStream_t stream) {
+  const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles <= 1) {
+    k_scan_tiles<<<1, kScanThreads, 0, stream>>>(data, n, nullptr);
+    return hipGetLastError();
+  }
+  uint32_t* sums = reinterpret_cast<uint32_t*>(static_cast<char*>(tmp.ptr) + tmp_offset);
+  k_scan_tiles<<<tiles, kScanThreads, 0, stream>>>(data, n, sums);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = exclusive_scan(sums, tiles, tmp, tmp_offset + ((size_t(tiles) * 4 + 255) & ~size_t(255)), stream);
+  if (e != hipSuccess) return e;
+  k_scan_add<<<tiles, kScanThreads, 0, stream>>>(data, n, sums);
+  return hipGetLastError();
+}
+
+// ---- Karras 2012 --------------------------------------------------------------------------------
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, int n, int i, int j) {
+  if (j < 0 || j > n - 1) return -1;
+  const uint32_t a = uint32_t(keys[i]) & 0x3fffffffu, b = uint32_t(keys[j]) & 0x3fffffffu;
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz(uint32_t(i) ^ uint32_t(j));
+}
+
+__global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_keys, const uint32_t* __restrict__ seg_of,
+                         const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
+                         const uint32_t* __restrict__ node_base, ge_bvh_node* nodes, uint32_t* leaf_parent,
+                         uint32_t* node_parent, uint32_t* flags) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_total) return;
+  const uint32_t s = seg_of[g];
+  const uint32_t first_g = seg_first[s];
+  const int n = int(seg_count[s]);
+  const int i = int(g - first_g);
+  const uint64_t* keys = sorted_keys + first_g;
+  ge_bvh_node* seg_nodes = nodes + node_base[s];
+  uint32_t* seg_flags = flags + node_base[s];
+  if (n == 1) {
+    ge_bvh_node node;
+    node.child[0] = GE_BVH_LEAF;
+    node.child[1] = GE_BVH_NONE;
+    node.parent = GE_BVH_NONE;
+    node.count = 1;
+    for (int a = 0; a < 3; ++a) {
+      node.lo0[a] = node.hi0[a] = 0.0f;  // written by the fit
+      node.lo1[a] = __uint_as_float(0x7f800000u);
+      node.hi1[a] = __uint_as_float(0xff800000u);
+    }
+    seg_nodes[0] = node;
+    leaf_parent[g] = 0;
+    seg_flags[0] = 1;  // one child only: the leaf's arrival completes it
+    return;
+  }
+  if (i >= n - 1) return;
+  seg_flags[i] = 0;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int sp = 0;
+  int t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (delta(keys, n, i, i + (sp + t) * d) > dnode) sp += t;
+  } while (t > 1);
+  const int gamma = i + sp * d + (d < 0 ? d : 0);
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  ge_bvh_node& node = seg_nodes[i];
+  const uint32_t c0 = (lo == gamma) ? (GE_BVH_LEAF | uint32_t(gamma)) : uint32_t(gamma);
+  const uint32_t c1 = (hi == gamma + 1) ? (GE_BVH_LEAF | uint32_t(gamma + 1)) : uint32_t(gamma + 1);
+  node.child[0] = c0;
+  node.child[1] = c1;
+  node.count = uint32_t(hi - lo + 1);
+  if (i == 0) node.parent = GE_BVH_NONE;
+  if (c0 & GE_BVH_LEAF) leaf_parent[first_g + gamma] = uint32_t(i); else node_parent[node_base[s] + gamma] = uint32_t(i);
+  if (c1 & GE_BVH_LEAF) leaf_parent[first_g + gamma + 1] = uint32_t(i); else node_parent[node_base[s] + gamma + 1] = uint32_t(i);
+}
+
+__device__ __forceinline__ void leaf_box(const float* boxes, const uint32_t* perm, uint32_t g, float lo[3], float hi[3]) {
+  const uint32_t p = perm[g];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = boxes[size_t(p) * 6 + a];
+    hi[a] = boxes[size_t(p) * 6 + 3 + a];
+  }
+}
+
+// Bottom-up fit: each leaf walks up; at each parent the first arrival stops and the second, which
+// sees both children finished, writes the parent's child boxes and its own box and height.
+__global__ void k_fit(uint32_t n_total, const float* __restrict__ boxes, const uint32_t* __restrict__ perm,
+                      const uint32_t* __restrict__ seg_of, const uint32_t* __restrict__ seg_first,
+                      const uint32_t* __restrict__ seg_count, const uint32_t* __restrict__ node_base,
+                      const uint32_t* __restrict__ root_index, ge_bvh_node* nodes, const uint32_t* leaf_parent,
+                      const uint32_t* node_parent, uint32_t* flags, float* node_box, float* roots, uint32_t* heights) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_total) return;
+  const uint32_t s = seg_of[g];
+  const uint32_t first_g = seg_first[s];
+  const uint32_t n = seg_count[s];
+  const uint32_t nb = node_base[s];
+  ge_bvh_node* seg_nodes = nodes + nb;
+  const uint32_t out = root_index ? root_index[s] : s;
+  if (n == 1) {
+    float lo[3], hi[3];
+    leaf_box(boxes, perm, g, lo, hi);
+    for (int a = 0; a < 3; ++a) {
+      seg_nodes[0].lo0[a] = lo[a];
+      seg_nodes[0].hi0[a] = hi[a];
+      roots[out * 6 + a] = lo[a];
+      roots[out * 6 + 3 + a] = hi[a];
+    }
+    heights[out] = 1;
+    return;
+  }
+  uint32_t p = leaf_parent[g];
+  for (;;) {
+    __threadfence();
+    const uint32_t arrived = atomicAdd(&flags[nb + p], 1u);
+    if (arrived == 0) return;
+    __threadfence();
+    ge_bvh_node& node = seg_nodes[p];
+    const volatile ge_bvh_node& vnode = seg_nodes[p];
+    float clo[2][3], chi[2][3];
+    uint32_t ch_height[2];
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t ch = vnode.child[c];
+      if (ch & GE_BVH_LEAF) {
+        leaf_box(boxes, perm, first_g + (ch & ~GE_BVH_LEAF), clo[c], chi[c]);
+        ch_height[c] = 0;
+      } else {
+        const volatile float* b = node_box + size_t(nb + ch) * 8;
+        for (int a = 0; a < 3; ++a) {
+          clo[c][a] = b[a];
+          chi[c][a] = b[3 + a];
+        }
+        ch_height[c] = __float_as_uint(b[6]);
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      node.lo0[a] = clo[0][a];
+      node.hi0[a] = chi[0][a];
+      node.lo1[a] = clo[1][a];
+      node.hi1[a] = chi[1][a];
+    }
+    const uint32_t h = 1 + (ch_height[0] > ch_height[1] ? ch_height[0] : ch_height[1]);
+    float* b = node_box + size_t(nb + p) * 8;
+    for (int a = 0; a < 3; ++a) {
+      b[a] = __builtin_fminf(clo[0][a], clo[1][a]);
+      b[3 + a] = __builtin_fmaxf(chi[0][a], chi[1][a]);
+    }
+    b[6] = __uint_as_float(h);
+    if (p == 0) {
+      node.parent = GE_BVH_NONE;
+      for (int a = 0; a < 6; ++a) roots[out * 6 + a] = b[a];
+      heights[out] = h;
+      return;
+    }
+    node.parent = node_parent[nb + p];
+    p = node_parent[nb + p];
+  }
+}
+
+__global__ void k_triangle_records(const ge_cluster_desc* __restrict__ clusters, const float* __restrict__ vertices,
+                                   const uint32_t* __restrict__ triangles, const uint32_t* __restrict__ cluster_list,
+                                   const uint32_t* __restrict__ cluster_out, const uint32_t* __restrict__ cluster_seg,
+                                   const uint32_t* __restrict__ cluster_local, ge_bvh_triangle* tris, float* boxes,
+                                   uint32_t* seg_of) {
+  const uint32_t k = blockIdx.x;
+  const uint32_t c = cluster_list[k];
+  const ge_cluster_desc cl = clusters[c];
+  const uint32_t out0 = cluster_out[k];
+  const uint32_t local_cluster = cluster_local[k];
+  const uint32_t s = cluster_seg[k];
+  for (uint32_t t = threadIdx.x; t < cl.triangle_count; t += blockDim.x) {
+    const uint32_t packed = triangles[cl.triangle_offset + t];
+    ge_bvh_triangle tri;
+    float* v[3] = {tri.v0, tri.v1, tri.v2};
+    for (int q = 0; q < 3; ++q) {
+      const uint32_t vi = cl.vertex_offset + ((packed >> (8 * q)) & 0xffu);
+      v[q][0] = vertices[size_t(vi) * 3];
+      v[q][1] = vertices[size_t(vi) * 3 + 1];
+      v[q][2] = vertices[size_t(vi) * 3 + 2];
+    }
+    const uint32_t o = out0 + t;
+    tri.prim = (local_cluster << 8) | t;
+    tri.index = o;  // rewritten to the mesh-local index by the gather
+    tri.pad = 0;
+    tris[o] = tri;
+    for (int a = 0; a < 3; ++a) {
+      boxes[size_t(o) * 6 + a] = __builtin_fminf(__builtin_fminf(tri.v0[a], tri.v1[a]), tri.v2[a]);
+      boxes[size_t(o) * 6 + 3 + a] = __builtin_fmaxf(__builtin_fmaxf(tri.v0[a], tri.v1[a]), tri.v2[a]);
+    }
+    seg_of[o] = s;
+  }
+}
+
+__global__ void k_gather_triangles(uint32_t n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ seg_of,
+                                   const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_out,
+                                   const ge_bvh_triangle* __restrict__ in, ge_bvh_triangle* out,
+                                   const uint32_t* __restrict__ morton_in, uint32_t* morton_out) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t s = seg_of[g];
+  const uint32_t o = seg_out[s] + (g - seg_first[s]);
+  ge_bvh_triangle tri = in[perm[g]];
+  tri.index = perm[g] - seg_first[s];
+  out[o] = tri;
+  morton_out[o] = morton_in[g];
+}
+
+__global__ void k_codes(uint32_t n, const uint64_t* keys, uint32_t* codes) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) codes[g] = uint32_t(keys[g]) & 0x3fffffffu;
+}
+
+__global__ void k_instance_boxes(uint32_t n, const uint32_t* __restrict__ live, const ge_instance_desc* __restrict__ inst,
+                                 const float* __restrict__ T, const float* __restrict__ roots, float* boxes,
+                                 uint32_t* seg_of) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t id = live[k];
+  const ge_instance_desc in = inst[id];
+  const float* b = roots + size_t(in.mesh) * 6;
+  float lo[3] = {__uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u)};
+  float hi[3] = {__uint_as_float(0xff800000u), __uint_as_float(0xff800000u), __uint_as_float(0xff800000u)};
+  for (int i = 0; i < 8; ++i) {
+    const float p[3] = {(i & 1) ? b[3] : b[0], (i & 2) ? b[4] : b[1], (i & 4) ? b[5] : b[2]};
+    for (int a = 0; a < 3; ++a) {
+      const float v = (in.rows[a][0] * p[0] + in.rows[a][1] * p[1] + in.rows[a][2] * p[2]) + T[size_t(id) * 3 + a];
+      lo[a] = __builtin_fminf(lo[a], v);
+      hi[a] = __builtin_fmaxf(hi[a], v);
+    }
+  }
+  float m = 0.0f;
+  for (int a = 0; a < 3; ++a) m = __builtin_fmaxf(m, __builtin_fmaxf(__builtin_fabsf(lo[a]), __builtin_fabsf(hi[a])));
+  const float pad = m * 9.5367431640625e-7f + 1e-30f;
+  for (int a = 0; a < 3; ++a) {
+    boxes[size_t(k) * 6 + a] = lo[a] - pad;
+    boxes[size_t(k) * 6 + 3 + a] = hi[a] + pad;
+  }
+  seg_of[k] = 0;
+}
+
+__global__ void k_gather_slots(uint32_t n, const uint32_t* perm, const uint32_t* live, uint32_t* slots) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) slots[g] = live[perm[g]];
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, const float* boxes, uint32_t nseg,
+                          const uint32_t* d_seg_of, const uint32_t* d_seg_first, const uint32_t* d_seg_count,
+                          const uint32_t* d_node_base, ge_bvh_node* nodes, uint32_t* perm, uint32_t* morton,
+                          float* roots, uint32_t* heights, const uint32_t* d_root_index, uint32_t node_pool) {
+  if (n == 0) return hipSuccess;
+  hipError_t e;
+  const uint32_t tiles = blocks(n, kSortTile);
+  if ((e = sc.keys[0].reserve(size_t(n) * 8)) != hipSuccess) return e;
+  if ((e = sc.keys[1].reserve(size_t(n) * 8)) != hipSuccess) return e;
+  if ((e = sc.vals[0].reserve(size_t(n) * 4)) != hipSuccess) return e;
+  if ((e = sc.vals[1].reserve(size_t(n) * 4)) != hipSuccess) return e;
+  if ((e = sc.hist.reserve(size_t(tiles) * 256 * 4)) != hipSuccess) return e;
+  if ((e = sc.scan_tmp.reserve(size_t(tiles) * 256 * 4 / 256 + 65536)) != hipSuccess) return e;
+  if ((e = sc.flags.reserve(size_t(node_pool) * 4)) != hipSuccess) return e;
+  if ((e = sc.node_box.reserve(size_t(node_pool) * 32)) != hipSuccess) return e;
+  if ((e = sc.leaf_parent.reserve(size_t(n) * 4)) != hipSuccess) return e;
+  if ((e = sc.node_parent.reserve(size_t(node_pool) * 4)) != hipSuccess) return e;
+  if ((e = sc.seg_cbox.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
+  if ((e = sc.seg_scale.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
+
+  k_init_bounds<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>());
+  k_centroid_bounds<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, d_seg_of, sc.seg_cbox.as<uint32_t>());
+  k_segment_scale<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>(), sc.seg_scale.as<float>());
+  k_morton<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, d_seg_of, sc.seg_scale.as<float>(), sc.keys[0].as<uint64_t>(),
+                                               sc.vals[0].as<uint32_t>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+
+  uint32_t seg_bits = 0;
+  while ((1u << seg_bits) < nseg) ++seg_bits;
+  const int key_bits = 30 + int(seg_bits);
+  int cur = 0;
+  for (int shift = 0; shift < key_bits; shift += 8) {
+    k_sort_hist<<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), n, shift, sc.hist.as<uint32_t>(), tiles);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = exclusive_scan(sc.hist.as<uint32_t>(), tiles * 256, sc.scan_tmp, 0, stream)) != hipSuccess) return e;
+    k_sort_scatter<<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), sc.vals[cur].as<uint32_t>(),
+                                                       sc.keys[cur ^ 1].as<uint64_t>(), sc.vals[cur ^ 1].as<uint32_t>(),
+                                                       n, shift, sc.hist.as<uint32_t>(), tiles);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    cur ^= 1;
+  }
+  const uint64_t* sorted_keys = sc.keys[cur].as<uint64_t>();
+  const uint32_t* sorted_vals = sc.vals[cur].as<uint32_t>();
+  k_karras<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, d_seg_of, d_seg_first, d_seg_count, d_node_base, nodes,
+                                               sc.leaf_parent.as<uint32_t>(), sc.node_parent.as<uint32_t>(),
+                                               sc.flags.as<uint32_t>());
+  k_fit<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, sorted_vals, d_seg_of, d_seg_first, d_seg_count, d_node_base,
+                                            d_root_index, nodes, sc.leaf_parent.as<uint32_t>(),
+                                            sc.node_parent.as<uint32_t>(), sc.flags.as<uint32_t>(),
+                                            sc.node_box.as<float>(), roots, heights);
+  k_codes<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, morton);
+  if ((e = hipMemcpyAsync(perm, sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,
+                                   const uint32_t* cluster_out, const uint32_t* cluster_seg, const uint32_t* cluster_local,
+                                   ge_bvh_triangle* tris, float* boxes, uint32_t* seg_of) {
+  if (ncl == 0) return hipSuccess;
+  k_triangle_records<<<ncl, 128, 0, stream>>>(s.clusters.as<ge_cluster_desc>(), s.vertices.as<float>(),
+                                               s.triangles.as<uint32_t>(), cluster_list, cluster_out, cluster_seg,
+                                               cluster_local, tris, boxes, seg_of);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_triangles(hipStream_t stream, uint32_t n, const uint32_t* perm, const uint32_t* seg_of,
+                                   const uint32_t* seg_first, const uint32_t* seg_out, const ge_bvh_triangle* in,
+                                   ge_bvh_triangle* out, const uint32_t* morton_in, uint32_t* morton_out) {
+  if (n == 0) return hipSuccess;
+  k_gather_triangles<<<blocks(n, 256), 256, 0, stream>>>(n, perm, seg_of, seg_first, seg_out, in, out, morton_in,
+                                                         morton_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_instance_boxes(hipStream_t stream, uint32_t n, const uint32_t* live, const ge_instance_desc* inst,
+                                 const float* T, const float* roots, float* boxes, uint32_t* seg_of) {
+  if (n == 0) return hipSuccess;
+  k_instance_boxes<<<blocks(n, 256), 256, 0, stream>>>(n, live, inst, T, roots, boxes, seg_of);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_slots(hipStream_t stream, uint32_t n, const uint32_t* perm, const uint32_t* live,
+                               uint32_t* slots) {
+  if (n == 0) return hipSuccess;
+  k_gather_slots<<<blocks(n, 256), 256, 0, stream>>>(n, perm, live, slots);
+  return hipGetLastError();
+}
+
+}  // namespace ge

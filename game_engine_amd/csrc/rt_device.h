@@ -1,0 +1,343 @@
+// rt_device.h — device code shared by the backend's kernels: vector arithmetic in the shaders'
+// operation order, the random stream, the robust box test, the watertight triangle test and the
+// two-level traversal.
+//
+// Floating point: every kernel is compiled with -ffp-contract=off and correctly rounded division
+// and square root (hipcc's default for HIP), so the arithmetic below is IEEE single precision
+// operation for operation and a hit, a Morton code or a radiance sum is the same float on every
+// launch. Nothing here is fused or reassociated.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ge_rt.h"
+
+#define GE_DEV __device__ __forceinline__
+
+namespace ge {
+
+struct f3 {
+  float x, y, z;
+};
+GE_DEV f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+GE_DEV f3 splat(float s) { return f3{s, s, s}; }
+GE_DEV f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+GE_DEV f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+GE_DEV f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+GE_DEV f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+GE_DEV f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+GE_DEV f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+GE_DEV f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+GE_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+GE_DEV f3 cross(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+GE_DEV f3 normalize(f3 v) { return v * (1.0f / __builtin_sqrtf(dot(v, v))); }
+GE_DEV float rsqrt_cr(float x) { return 1.0f / __builtin_sqrtf(x); }
+// HLSL max/min/clamp on non-NaN operands.
+GE_DEV float maxr(float a, float b) { return a > b ? a : b; }
+GE_DEV float minr(float a, float b) { return a < b ? a : b; }
+GE_DEV float clampr(float v, float lo, float hi) {
+  const float m = v < lo ? lo : v;
+  return hi < m ? hi : m;
+}
+GE_DEV float lerp1(float a, float b, float t) { return a + (b - a) * t; }
+GE_DEV f3 lerp3(f3 a, f3 b, float t) { return f3{lerp1(a.x, b.x, t), lerp1(a.y, b.y, t), lerp1(a.z, b.z, t)}; }
+GE_DEV f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+
+constexpr float k_pi = 3.14159265358979323846f;
+
+// sin and cos: reduction by pi/2 and the Taylor series of both on [-pi/4, pi/4], in double,
+// rounded once to float. Correctly rounded except within ~1e-10 relative of a rounding boundary.
+GE_DEV void sincos_cr(float x, float* s, float* c) {
+  const double xd = double(x);
+  const double q = __builtin_rint(xd * 0.6366197723675814);
+  const double r = xd - q * 1.5707963267948966;
+  const double r2 = r * r;
+  const double sp =
+      r + r * r2 * (-1.0 / 6.0 + r2 * (1.0 / 120.0 + r2 * (-1.0 / 5040.0 + r2 * (1.0 / 362880.0 +
+                                                                                  r2 * (-1.0 / 39916800.0)))));
+  const double cp =
+      1.0 + r2 * (-0.5 + r2 * (1.0 / 24.0 + r2 * (-1.0 / 720.0 + r2 * (1.0 / 40320.0 +
+                                                                         r2 * (-1.0 / 3628800.0 + r2 * (1.0 / 479001600.0))))));
+  const int quadrant = int(int64_t(q) & 3);
+  const double sv = quadrant == 0 ? sp : (quadrant == 1 ? cp : (quadrant == 2 ? -sp : -cp));
+  const double cv = quadrant == 0 ? cp : (quadrant == 1 ? -sp : (quadrant == 2 ? -cp : sp));
+  *s = float(sv);
+  *c = float(cv);
+}
+
+// ---- the random stream: PCG hash over (pixel, sample, seed) ------------------------------------
+GE_DEV uint32_t pcg_hash(uint32_t v) {
+  const uint32_t state = v * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+GE_DEV uint32_t rng_seed(uint32_t px, uint32_t py, uint32_t sample_index, uint32_t seed) {
+  const uint32_t mixed = pcg_hash(px * 73856093u ^ py * 19349663u ^ seed * 83492791u);
+  return pcg_hash(mixed + sample_index * 2654435761u);
+}
+GE_DEV float rng_next(uint32_t& state) {
+  state = state * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  word = (word >> 22u) ^ word;
+  return float(word) * (1.0f / 4294967296.0f);
+}
+
+// ---- the built structures, as the kernels read them ---------------------------------------------
+// A top-level leaf's record: the frame-to-mesh 3x4 and where its mesh's structure lives. 64 bytes.
+struct InstRec {
+  float w2o[3][4];
+  uint32_t node_base;  // the mesh's root in the node pool
+  uint32_t tri_base;   // the mesh's first triangle in the triangle pool
+  uint32_t first_pair;
+  uint32_t live;
+};
+static_assert(sizeof(InstRec) == 64, "one instance record is one 64-byte fetch");
+
+struct Bvh {
+  const ge_bvh_node* tlas;       // the top level's nodes (node 0 the root)
+  const uint32_t* tlas_slots;    // sorted slot -> instance id
+  const InstRec* inst;           // per instance id
+  const ge_bvh_node* nodes;      // every mesh's nodes, pooled
+  const ge_bvh_triangle* tris;   // every mesh's sorted triangles, pooled
+  uint32_t tlas_node_count;      // 0: an empty scene
+  uint32_t pad;
+};
+
+struct Tally {
+  uint32_t nodes, tris, inst;
+};
+
+constexpr float k_box_robust = 1.00000095367431640625f;  // 1 + 2^-20 on the far distance (Ize 2013)
+constexpr uint32_t k_sentinel = 0xfffffffeu;
+constexpr int k_stack = 96;
+
+struct Ray {
+  float o[3], d[3], inv[3];
+  float tmin;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+GE_DEV void prep(Ray& r) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float d = __builtin_fabsf(r.d[a]) < 1e-20f ? __builtin_copysignf(1e-20f, r.d[a]) : r.d[a];
+    r.inv[a] = 1.0f / d;
+  }
+  int kz = 0;
+  if (__builtin_fabsf(r.d[1]) > __builtin_fabsf(r.d[kz])) kz = 1;
+  if (__builtin_fabsf(r.d[2]) > __builtin_fabsf(r.d[kz])) kz = 2;
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  if (r.d[kz] < 0.0f) {
+    const int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  r.Sx = r.d[kx] / r.d[kz];
+  r.Sy = r.d[ky] / r.d[kz];
+  r.Sz = 1.0f / r.d[kz];
+}
+
+GE_DEV Ray make_ray(f3 o, f3 d, float tmin) {
+  Ray r;
+  r.o[0] = o.x;
+  r.o[1] = o.y;
+  r.o[2] = o.z;
+  r.d[0] = d.x;
+  r.d[1] = d.y;
+  r.d[2] = d.z;
+  r.tmin = tmin;
+  prep(r);
+  return r;
+}
+
+GE_DEV float sel3(const float v[3], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
+// The slab test with the far distance widened by 1 + 2^-20, so a box is never missed by a ray
+// that hits a triangle inside it (Ize 2013, "Robust BVH Ray Traversal").
+GE_DEV bool slab(const float lo[3], const float hi[3], const Ray& r, float tmax, float& tnear) {
+  const float tx0 = (lo[0] - r.o[0]) * r.inv[0], tx1 = (hi[0] - r.o[0]) * r.inv[0];
+  const float ty0 = (lo[1] - r.o[1]) * r.inv[1], ty1 = (hi[1] - r.o[1]) * r.inv[1];
+  const float tz0 = (lo[2] - r.o[2]) * r.inv[2], tz1 = (hi[2] - r.o[2]) * r.inv[2];
+  const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
+                                   __builtin_fmaxf(__builtin_fminf(tz0, tz1), r.tmin));
+  const float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)),
+                                   __builtin_fminf(__builtin_fmaxf(tz0, tz1), tmax));
+  tnear = tn;
+  return tn <= tf * k_box_robust;
+}
+
+// Watertight ray/triangle intersection (Woop, Benthin, Wald 2013), with the double-precision
+// recomputation of an edge function that is exactly zero. No face culling.
+GE_DEV bool intersect(const Ray& r, const float* v0, const float* v1, const float* v2, float& t, float& u,
+                      float& v) {
+  const float A[3] = {v0[0] - r.o[0], v0[1] - r.o[1], v0[2] - r.o[2]};
+  const float B[3] = {v1[0] - r.o[0], v1[1] - r.o[1], v1[2] - r.o[2]};
+  const float C[3] = {v2[0] - r.o[0], v2[1] - r.o[1], v2[2] - r.o[2]};
+  const float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
+  const float Ax = sel3(A, r.kx) - r.Sx * Akz, Ay = sel3(A, r.ky) - r.Sy * Akz;
+  const float Bx = sel3(B, r.kx) - r.Sx * Bkz, By = sel3(B, r.ky) - r.Sy * Bkz;
+  const float Cx = sel3(C, r.kx) - r.Sx * Ckz, Cy = sel3(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+    V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+    W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  const float T = U * Az + V * Bz + W * Cz;
+  t = T / det;
+  u = V / det;
+  v = W / det;
+  return true;
+}
+
+struct Hit {
+  float t, u, v;
+  uint32_t inst, prim, key;
+  bool found;
+};
+
+GE_DEV void load_node(const ge_bvh_node* nodes, uint32_t i, float lo0[3], float hi0[3], float lo1[3], float hi1[3],
+                      uint32_t child[2]) {
+  const float4* p = reinterpret_cast<const float4*>(nodes + i);
+  const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+  lo0[0] = a.x; lo0[1] = a.y; lo0[2] = a.z;
+  hi0[0] = a.w; hi0[1] = b.x; hi0[2] = b.y;
+  lo1[0] = b.z; lo1[1] = b.w; lo1[2] = c.x;
+  hi1[0] = c.y; hi1[1] = c.z; hi1[2] = c.w;
+  child[0] = __float_as_uint(d.x);
+  child[1] = __float_as_uint(d.y);
+}
+
+GE_DEV void to_object(const InstRec& rec, const Ray& w, Ray& o) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    o.o[r] = ((rec.w2o[r][0] * w.o[0] + rec.w2o[r][1] * w.o[1]) + rec.w2o[r][2] * w.o[2]) + rec.w2o[r][3];
+    o.d[r] = (rec.w2o[r][0] * w.d[0] + rec.w2o[r][1] * w.d[1]) + rec.w2o[r][2] * w.d[2];
+  }
+  o.tmin = w.tmin;
+  prep(o);
+}
+
+// The two-level walk in its canonical order: the nearer child first (child 0 on a tie) and the
+// other pushed; a mesh's leaves tested as their node is visited; a top-level leaf enters its
+// instance (the ray goes into mesh space and a sentinel marks the way back). Closest hits are
+// ordered by (t, then the larger pair << 8 | triangle), a total order, so the answer does not depend
+// on the walk; the counters do, and match the oracle's walk.
+template <bool AnyHit, bool Count>
+GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask, Hit& hit, Tally& tally) {
+  hit.found = false;
+  hit.t = tmax;
+  hit.key = 0;
+  hit.u = hit.v = 0.0f;
+  hit.inst = hit.prim = GE_RT_MISS;
+  if (mask == 0 || bvh.tlas_node_count == 0) return false;
+  uint32_t stack[k_stack];
+  int sp = 0;
+  Ray ray = world;
+  bool in_blas = false;
+  uint32_t node_base = 0, tri_base = 0, first_pair = 0, inst_id = 0;
+  uint32_t cur = 0;
+  for (;;) {
+    if (!in_blas && (cur & GE_BVH_LEAF)) {
+      inst_id = bvh.tlas_slots[cur & ~GE_BVH_LEAF];
+      const InstRec rec = bvh.inst[inst_id];
+      node_base = rec.node_base;
+      tri_base = rec.tri_base;
+      first_pair = rec.first_pair;
+      to_object(rec, world, ray);
+      stack[sp++] = k_sentinel;
+      in_blas = true;
+      if (Count) ++tally.inst;
+      cur = 0;
+    }
+    if (Count) ++tally.nodes;
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    uint32_t child[2];
+    load_node(in_blas ? bvh.nodes : bvh.tlas, in_blas ? node_base + cur : cur, lo0, hi0, lo1, hi1, child);
+    float tn0 = 0.0f, tn1 = 0.0f;
+    bool h0 = child[0] != GE_BVH_NONE && slab(lo0, hi0, ray, hit.t, tn0);
+    bool h1 = child[1] != GE_BVH_NONE && slab(lo1, hi1, ray, hit.t, tn1);
+    if (in_blas) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const bool hc = c == 0 ? h0 : h1;
+        if (!hc || !(child[c] & GE_BVH_LEAF)) continue;
+        if (c == 0) h0 = false; else h1 = false;
+        const ge_bvh_triangle* tri = bvh.tris + tri_base + (child[c] & ~GE_BVH_LEAF);
+        const float4* tp = reinterpret_cast<const float4*>(tri);
+        const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
+        const float v0[3] = {q0.x, q0.y, q0.z}, v1[3] = {q0.w, q1.x, q1.y}, v2[3] = {q1.z, q1.w, q2.x};
+        const uint32_t prim = __float_as_uint(q2.y);
+        if (Count) ++tally.tris;
+        float t, u, v;
+        if (!intersect(ray, v0, v1, v2, t, u, v)) continue;
+        if (!(t >= ray.tmin && t <= hit.t)) continue;
+        const uint32_t key = (first_pair << 8) + prim;
+        if (AnyHit) {
+          hit.found = true;
+          hit.t = t;
+          hit.u = u;
+          hit.v = v;
+          hit.inst = inst_id;
+          hit.prim = prim;
+          hit.key = key;
+          return true;
+        }
+        if (t < hit.t || !hit.found || key > hit.key) {
+          hit.found = true;
+          hit.t = t;
+          hit.u = u;
+          hit.v = v;
+          hit.inst = inst_id;
+          hit.prim = prim;
+          hit.key = key;
+        }
+      }
+    }
+    if (h0 && h1) {
+      const bool second_first = tn1 < tn0;
+      stack[sp++] = second_first ? child[0] : child[1];
+      cur = second_first ? child[1] : child[0];
+      continue;
+    }
+    if (h0) {
+      cur = child[0];
+      continue;
+    }
+    if (h1) {
+      cur = child[1];
+      continue;
+    }
+    bool done = false;
+    for (;;) {
+      if (sp == 0) {
+        done = true;
+        break;
+      }
+      const uint32_t e = stack[--sp];
+      if (e == k_sentinel) {
+        in_blas = false;
+        ray = world;
+        continue;
+      }
+      cur = e;
+      break;
+    }
+    if (done) return hit.found;
+  }
+}
+
+}  // namespace ge

@@ -1,0 +1,144 @@
+// rt_internal.h — host-side state of the backend: the context, the scene on the device, device
+// buffers, and the launchers each kernel file exports to rt_api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ge_rt.h"
+#include "rt_device.h"
+
+namespace ge {
+
+// A device allocation that grows and never shrinks (a scratch buffer reused across calls).
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (need == 0) return hipSuccess;
+    const hipError_t e = hipMalloc(&ptr, need);
+    if (e == hipSuccess) bytes = need;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Everything a segmented LBVH build needs besides its inputs. One per context, reused.
+struct BuildScratch {
+  DevBuf keys[2], vals[2], hist, scan_tmp, flags, node_box, leaf_parent, node_parent, seg_cbox, seg_scale;
+};
+
+struct SceneDev {
+  // The engine's arrays, as uploaded.
+  DevBuf clusters, vertices, quantized, attributes, triangles, meshes, instances, materials,
+      cluster_materials, lights;
+  std::vector<ge_cluster_desc> h_clusters;
+  std::vector<ge_mesh_desc> h_meshes;
+  std::vector<ge_instance_desc> h_instances;
+  uint32_t cluster_count = 0, vertex_count = 0, triangle_count = 0, mesh_count = 0, instance_count = 0,
+           material_count = 0, light_count = 0;
+  bool has_attributes = false;
+  // Per mesh: its triangles' place in the pools and its cluster-triangle prefix.
+  std::vector<uint32_t> mesh_tri_count, mesh_tri_base, mesh_node_base, mesh_node_count;
+  DevBuf tri_prefix;   // per cluster: the mesh-local index of its first triangle
+  DevBuf tri_mesh_of;  // per cluster: its mesh
+  uint32_t tri_pool = 0, node_pool = 0;
+  // The built structures.
+  DevBuf blas_nodes, blas_tris, blas_morton, mesh_roots, mesh_height;  // roots: 6 floats per mesh
+  DevBuf tlas_nodes, tlas_slots, tlas_morton, inst_recs, inst_boxes, live_ids, tlas_root, tlas_height;
+  uint32_t tlas_prims = 0, tlas_node_count = 0;
+  std::vector<uint32_t> h_mesh_height;
+  uint32_t tlas_height_value = 0;
+  bool built = false;
+  ge_frame_eye eye{};
+};
+
+}  // namespace ge
+
+struct ge_rt_scene {
+  ge::SceneDev dev;
+};
+
+struct ge_rt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, bev0 = nullptr, bev1 = nullptr;
+  std::string error;
+  int32_t last_status = 0;
+  ge::BuildScratch build;
+  ge::DevBuf io[6];    // host-pointer staging
+  ge::DevBuf tmp[9];   // build batches: metadata, records, boxes, segments, order, codes, instances
+  ge::DevBuf counters; // 5 x u64
+  double last_kernel_ms = 0.0, last_build_ms = 0.0;
+  bool timing_pending = false;
+};
+
+namespace ge {
+
+// ---- launchers (rt_bvh.hip) ----
+// Builds one LBVH per segment over `n` primitive boxes (6 floats each, lo xyz then hi xyz, grouped
+// by segment in order; seg_of[i] names primitive i's segment). Segment s covers primitives
+// [seg_first[s], seg_first[s] + seg_count[s]); its nodes go to nodes[node_base[s] ..], its sorted
+// order (batch indices) to perm[seg_first[s] ..], its sorted codes to morton[seg_first[s] ..], and
+// its root box and height to roots[6 r ..] and heights[r], r = root_index[s] (or s when null).
+// `node_pool` bounds every node_base[s] + node count. All pointers are device pointers.
+hipError_t build_segments(BuildScratch& scratch, hipStream_t stream, uint32_t n, const float* boxes,
+                          uint32_t nseg, const uint32_t* d_seg_of, const uint32_t* d_seg_first,
+                          const uint32_t* d_seg_count, const uint32_t* d_node_base, ge_bvh_node* nodes,
+                          uint32_t* perm, uint32_t* morton, float* roots, uint32_t* heights,
+                          const uint32_t* d_root_index, uint32_t node_pool);
+// Triangle records, boxes and segment ids of the clusters in `cluster_list` (cluster k's triangles
+// at batch index cluster_out[k] on; cluster_local[k] its index within its mesh).
+hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,
+                                   const uint32_t* cluster_out, const uint32_t* cluster_seg,
+                                   const uint32_t* cluster_local, ge_bvh_triangle* tris, float* boxes,
+                                   uint32_t* seg_of);
+hipError_t launch_gather_triangles(hipStream_t stream, uint32_t n, const uint32_t* perm, const uint32_t* seg_of,
+                                   const uint32_t* seg_first, const uint32_t* seg_out, const ge_bvh_triangle* in,
+                                   ge_bvh_triangle* out, const uint32_t* morton_in, uint32_t* morton_out);
+hipError_t launch_instance_boxes(hipStream_t stream, uint32_t n, const uint32_t* live_ids, const ge_instance_desc* inst,
+                                 const float* translations, const float* mesh_roots, float* boxes, uint32_t* seg_of);
+hipError_t launch_gather_slots(hipStream_t stream, uint32_t n, const uint32_t* perm, const uint32_t* live_ids,
+                               uint32_t* slots);
+
+// ---- launchers (rt_kernels.hip) ----
+struct SceneView {
+  const ge_cluster_desc* clusters;
+  const uint16_t* quantized;
+  const ge_vertex_attributes* attributes;  // null: flat shading
+  const uint32_t* triangles;
+  const ge_mesh_desc* meshes;
+  const ge_instance_desc* instances;
+  const ge_resolve_material* materials;
+  const uint32_t* cluster_materials;
+  const ge_resolve_light* lights;
+  uint32_t light_count;
+  uint32_t pad;
+};
+
+hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
+                                ge_hit* hits, unsigned long long* counters);
+hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
+                            uint32_t* bits, unsigned long long* counters);
+hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
+                                 unsigned long long* counters);
+hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
+                             float4* accum, uint32_t* coverage, unsigned long long* counters);
+hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, const float4* accum,
+                          const uint32_t* coverage, uint32_t* out);
+hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
+                              const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters);
+
+}  // namespace ge

@@ -1,0 +1,474 @@
+// rt_kernels.hip — the traced passes: closest-hit and any-hit queries, the primary visibility word,
+// the reference path tracer (one path per lane, the megakernel form of path_trace_main), its
+// tonemap, and the sun shadow pass.
+//
+// Launch shapes: 64-lane workgroups (one wave64). The path tracer's workgroup is an 8 x 8 pixel
+// tile, the reference's numthreads(8, 8, 1) (path_trace.h:149-154), so one wave traces a compact
+// block of coherent primaries. Counters, when asked for, are summed per wave before one atomic.
+#include "rt_internal.h"
+#include "rt_shade.h"
+
+namespace ge {
+namespace {
+
+__device__ __forceinline__ void flush_counters(unsigned long long* counters, uint64_t closest, uint64_t any,
+                                               const Tally& t) {
+  // Reduce across the wave, then one lane adds.
+  uint64_t v[5] = {closest, any, t.nodes, t.tris, t.inst};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    uint64_t x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    v[k] = x;
+  }
+  if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (v[k]) atomicAdd(&counters[k], (unsigned long long)v[k]);
+  }
+}
+
+template <bool Count>
+__global__ void __launch_bounds__(64) k_trace_closest(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n,
+                                                      uint32_t mask, ge_hit* __restrict__ hits,
+                                                      unsigned long long* counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  Tally tally{0, 0, 0};
+  uint64_t issued = 0;
+  if (i < n) {
+    const ge_ray r = rays[i];
+    const Ray ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
+    Hit h;
+    issued = 1;
+    const bool found = traverse<false, Count>(bvh, ray, r.tmax, mask, h, tally);
+    ge_hit out;
+    out.pad = 0;
+    if (found) {
+      out.t = h.t;
+      out.u = h.u;
+      out.v = h.v;
+      out.instance_id = h.inst;
+      out.geometry = h.prim >> 8;
+      out.primitive = h.prim & 0xffu;
+      out.key = h.key;
+    } else {
+      out.t = 0.0f;
+      out.u = 0.0f;
+      out.v = 0.0f;
+      out.instance_id = GE_RT_MISS;
+      out.geometry = GE_RT_MISS;
+      out.primitive = GE_RT_MISS;
+      out.key = 0;
+    }
+    hits[i] = out;
+  }
+  if (Count) flush_counters(counters, issued, 0, tally);
+}
+
+template <bool Count>
+__global__ void __launch_bounds__(64) k_trace_any(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n, uint32_t mask,
+                                                  uint32_t* __restrict__ bits, unsigned long long* counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  Tally tally{0, 0, 0};
+  uint64_t issued = 0;
+  bool blocked = false;
+  if (i < n) {
+    const ge_ray r = rays[i];
+    if (r.tmax > 0.0f) {
+      issued = 1;
+      const Ray ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
+      Hit h;
+      blocked = traverse<true, Count>(bvh, ray, r.tmax, mask, h, tally);
+    }
+  }
+  const uint64_t ballot = __ballot(blocked);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t word0 = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) >> 5;
+  if (lane == 0 && word0 * 32 < n) bits[word0] = uint32_t(ballot);
+  if (lane == 32 && (word0 + 1) * 32 < n) bits[word0 + 1] = uint32_t(ballot >> 32);
+  if (Count) flush_counters(counters, 0, issued, tally);
+}
+
+// trace_main (ray_visibility.slang:106-217): the pixel centre's ray, the committed hit (whose
+// (t, key) order already settles a depth tie to the larger pair << 8 | triangle, the tie scan's
+// answer), and the word asuint(depth) << 32 | key.
+template <bool Count>
+__global__ void __launch_bounds__(64) k_ray_visibility(Bvh bvh, ge_ray_visibility_params p, uint64_t* __restrict__ words,
+                                                       unsigned long long* counters) {
+  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
+  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+  Tally tally{0, 0, 0};
+  uint64_t issued = 0;
+  if (x < p.width && y < p.height) {
+    const float ndx = (float(x) + 0.5f) / float(p.width) * 2.0f - 1.0f;
+    const float ndy = 1.0f - (float(y) + 0.5f) / float(p.height) * 2.0f;
+    const float* m = p.clip_to_ray;
+    f3 d;
+    d.x = ((m[0] * ndx + m[4] * ndy) + m[8] * 0.0f) + m[12] * 1.0f;
+    d.y = ((m[1] * ndx + m[5] * ndy) + m[9] * 0.0f) + m[13] * 1.0f;
+    d.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
+    const float len = __builtin_sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    const float inv = 1.0f / len;
+    d = f3{d.x * inv, d.y * inv, d.z * inv};
+    const f3 o = f3{p.camera[0], p.camera[1], p.camera[2]};
+    const Ray ray = make_ray(o, d, 0.0f);
+    Hit h;
+    issued = 1;
+    uint64_t value = 0;
+    if (traverse<false, Count>(bvh, ray, 1e30f, 0xfeu, h, tally)) {
+      const float pt[3] = {o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t};
+      const float* vp = p.view_proj;
+      const float cz = ((vp[2] * pt[0] + vp[6] * pt[1]) + vp[10] * pt[2]) + vp[14] * 1.0f;
+      const float cw = ((vp[3] * pt[0] + vp[7] * pt[1]) + vp[11] * pt[2]) + vp[15] * 1.0f;
+      const float depth = cz / cw;
+      value = (uint64_t(__float_as_uint(depth)) << 32) | uint64_t(h.key);
+    }
+    words[size_t(y) * p.width + x] = value;
+  }
+  if (Count) flush_counters(counters, issued, 0, tally);
+}
+
+// ---- the path tracer ---------------------------------------------------------------------------
+struct Pt {
+  Bvh bvh;
+  SceneView sv;
+  ge_path_trace_params p;
+};
+
+template <bool Count>
+__device__ bool occluded(const Pt& c, f3 origin, f3 direction, float t_max, uint64_t& any, Tally& tally) {
+  if (t_max <= 0.0f) return false;
+  ++any;
+  const Ray ray = make_ray(origin, direction, 0.0f);
+  Hit h;
+  return traverse<true, Count>(c.bvh, ray, t_max, 0x01u, h, tally);
+}
+
+__device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool primary) {
+  const f3 sky = ld3(p.sky);
+  if (p.flags & GE_PT_UNIFORM_SKY) return sky;
+  if (primary) {
+    return f3{__builtin_powf(maxr(sky.x, 0.0f), 2.2f), __builtin_powf(maxr(sky.y, 0.0f), 2.2f),
+              __builtin_powf(maxr(sky.z, 0.0f), 2.2f)};
+  }
+  const f3 sky_r = sky * 0.35f;
+  const f3 ground = ld3(p.ground) * (splat(p.sun[3] * maxr(p.sun[1], 0.0f) / k_pi) + sky_r);
+  return direction.y >= 0.0f ? sky_r : ground;
+}
+
+struct HitSurface {
+  f3 position, normal, geometric;
+  uint32_t cluster;
+};
+
+// fetch_triangle + surface_at (material.slang:244-302): the triangle's corners off the mesh's
+// 16-bit grid into the frame through the instance, the geometric normal turned to `towards`, the
+// shading normal from the octahedral vertex normals through the cofactor matrix.
+__device__ HitSurface surface_of_hit(const SceneView& sv, const ge_path_trace_params& p, const Hit& hit, f3 towards,
+                                     const ge_instance_desc& in) {
+  const ge_mesh_desc mesh = sv.meshes[in.mesh];
+  const uint32_t cluster_index = mesh.first_cluster + (hit.prim >> 8);
+  const ge_cluster_desc cl = sv.clusters[cluster_index];
+  const uint32_t packed = sv.triangles[cl.triangle_offset + (hit.prim & 0xffu)];
+  const f3 T = f3{relative_axis(in.cell[0], in.rows[0][3], p.eye.cell[0], p.eye.local[0], p.eye.residual[0]),
+                  relative_axis(in.cell[1], in.rows[1][3], p.eye.cell[1], p.eye.local[1], p.eye.residual[1]),
+                  relative_axis(in.cell[2], in.rows[2][3], p.eye.cell[2], p.eye.local[2], p.eye.residual[2])};
+  f3 pc[3];
+  uint32_t local[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    local[k] = cl.vertex_offset + ((packed >> (8 * k)) & 0xffu);
+    const uint32_t i = local[k] * 3;
+    // Three u16 of the grid, read as the 32-bit words that hold them (load_quantized_position).
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(sv.quantized);
+    const uint32_t w0 = words[i >> 1], w1 = words[(i + 1) >> 1], w2 = words[(i + 2) >> 1];
+    const uint32_t qx = (i & 1) ? (w0 >> 16) : (w0 & 0xffffu);
+    const uint32_t qy = ((i + 1) & 1) ? (w1 >> 16) : (w1 & 0xffffu);
+    const uint32_t qz = ((i + 2) & 1) ? (w2 >> 16) : (w2 & 0xffffu);
+    const f3 m = f3{mesh.quant[0], mesh.quant[1], mesh.quant[2]} + f3{float(qx), float(qy), float(qz)} * mesh.quant[3];
+    f3 d;
+    d.x = in.rows[0][0] * m.x + in.rows[0][1] * m.y + in.rows[0][2] * m.z;
+    d.y = in.rows[1][0] * m.x + in.rows[1][1] * m.y + in.rows[1][2] * m.z;
+    d.z = in.rows[2][0] * m.x + in.rows[2][1] * m.y + in.rows[2][2] * m.z;
+    pc[k] = d + T;
+  }
+  const f3 b = f3{1.0f - hit.u - hit.v, hit.u, hit.v};
+  HitSurface out;
+  out.position = b.x * pc[0] + b.y * pc[1] + b.z * pc[2];
+  out.geometric = normalize(cross(pc[1] - pc[0], pc[2] - pc[0]));
+  if (dot(out.geometric, towards - out.position) < 0.0f) out.geometric = -out.geometric;
+  if (sv.attributes) {
+    const f3 n0 = decode_normal_oct(sv.attributes[local[0]].normal_oct);
+    const f3 n1 = decode_normal_oct(sv.attributes[local[1]].normal_oct);
+    const f3 n2 = decode_normal_oct(sv.attributes[local[2]].normal_oct);
+    const f3 n = b.x * n0 + b.y * n1 + b.z * n2;
+    out.normal = transform_normal(in, n);
+  } else {
+    out.normal = out.geometric;
+  }
+  out.cluster = cluster_index;
+  return out;
+}
+
+template <bool Count>
+__device__ f3 next_event(const Pt& c, const Surface& surface, f3 position, f3 front, float bias, uint64_t& any,
+                         Tally& tally) {
+  const ge_path_trace_params& p = c.p;
+  f3 direct = splat(0.0f);
+  const f3 shadow_origin = position + front * bias;
+  const f3 sun = ld3(p.sun);
+  if (p.sun[3] > 0.0f && dot(surface.normal, sun) > 0.0f && !occluded<Count>(c, shadow_origin, sun, 1e30f, any, tally))
+    direct = direct + brdf_direct(surface, sun, splat(p.sun[3]));
+  for (uint32_t i = 0; i < c.sv.light_count; ++i) {
+    const ge_resolve_light light = c.sv.lights[i];
+    f3 light_dir, radiance;
+    float light_distance;
+    if (!light_sample(light, position, light_dir, radiance, light_distance)) continue;
+    if (dot(surface.normal, light_dir) <= 0.0f) continue;
+    if (occluded<Count>(c, shadow_origin, light_dir, light_distance - bias - light.spot[2], any, tally)) continue;
+    direct = direct + brdf_direct(surface, light_dir, radiance);
+  }
+  return direct;
+}
+
+template <bool Count>
+__device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bool& hit_primary, uint64_t& closest,
+                         uint64_t& any, Tally& tally) {
+  const ge_path_trace_params& p = c.p;
+  hit_primary = false;
+  f3 radiance = splat(0.0f);
+  f3 throughput = splat(1.0f);
+  for (uint32_t bounce = 0;; ++bounce) {
+    ++closest;
+    const Ray ray = make_ray(origin, direction, 0.0f);
+    Hit hit;
+    traverse<false, Count>(c.bvh, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, tally);
+    if (!hit.found) {
+      radiance = radiance + throughput * environment(p, direction, bounce == 0);
+      break;
+    }
+    if (bounce == 0) hit_primary = true;
+    const ge_instance_desc in = c.sv.instances[hit.inst];
+    const HitSurface sf = surface_of_hit(c.sv, p, hit, origin, in);
+    const ge_resolve_material* mat = c.sv.materials + (c.sv.cluster_materials[sf.cluster] + in.material_base);
+    f3 shading_normal = sf.normal;
+    if (dot(shading_normal, sf.geometric) < 0.0f) shading_normal = -shading_normal;
+    const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
+    const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
+    const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
+    radiance = radiance + throughput * f3{emi.x, emi.y, emi.z};
+    Surface surface;
+    surface.albedo = f3{alb.x, alb.y, alb.z};
+    surface.normal = shading_normal;
+    surface.view = -direction;
+    surface.roughness = alb.w;
+    surface.metallic = emi.w;
+    surface.spec[0] = spc.x;
+    surface.spec[1] = spc.y;
+    surface.spec[2] = spc.z;
+    surface.spec[3] = spc.w;
+    const float bias = p.ray_bias + p.ray_bias_steps * c.sv.meshes[in.mesh].quant[3] * in.scale_max;
+    const f3 front = dot(sf.geometric, surface.view) >= 0.0f ? sf.geometric : -sf.geometric;
+    radiance = radiance + throughput * next_event<Count>(c, surface, sf.position, front, bias, any, tally);
+    if (bounce >= p.max_bounces) break;
+    const float n_dot_v = dot(surface.normal, surface.view);
+    if (n_dot_v <= 0.0f) break;
+    f3 tangent, bitangent;
+    onb(surface.normal, tangent, bitangent);
+    const f3 v_local = f3{dot(surface.view, tangent), dot(surface.view, bitangent), dot(surface.view, surface.normal)};
+    const float alpha = brdf_alpha(surface.roughness);
+    const f3 f = fresnel(surface, n_dot_v);
+    const float p_specular = specular_probability(surface.albedo, surface.metallic, f);
+    f3 l_local;
+    if (rng_next(rng) < p_specular) {
+      const float u0 = rng_next(rng);
+      const float u1 = rng_next(rng);
+      const f3 h = ggx_vndf(v_local, alpha, u0, u1);
+      l_local = reflect3(-v_local, h);
+    } else {
+      const float u0 = rng_next(rng);
+      const float u1 = rng_next(rng);
+      l_local = cosine_hemisphere(u0, u1);
+    }
+    if (l_local.z <= 0.0f) break;
+    const float pdf = bsdf_pdf(v_local, l_local, alpha, p_specular);
+    if (!(pdf > 0.0f)) break;
+    const f3 next = tangent * l_local.x + bitangent * l_local.y + surface.normal * l_local.z;
+    const f3 weight = brdf_eval(surface, next) * (l_local.z / pdf);
+    throughput = throughput * weight;
+    if (throughput.x <= 0.0f && throughput.y <= 0.0f && throughput.z <= 0.0f) break;
+    if (bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
+      const float survive = clampr(maxr(throughput.x, maxr(throughput.y, throughput.z)), 0.0f, 0.95f);
+      if (rng_next(rng) >= survive) break;
+      throughput = throughput / maxr(survive, 1e-6f);
+    }
+    origin = sf.position + (dot(next, front) >= 0.0f ? front : -front) * bias;
+    direction = next;
+  }
+  return radiance;
+}
+
+template <bool Count>
+__global__ void __launch_bounds__(64) k_path_trace(Pt c, float4* __restrict__ accum, uint32_t* __restrict__ coverage,
+                                                   unsigned long long* counters) {
+  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
+  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+  const ge_path_trace_params& p = c.p;
+  Tally tally{0, 0, 0};
+  uint64_t closest = 0, any = 0;
+  if (x < p.width && y < p.height) {
+    const size_t pixel = size_t(y) * p.width + x;
+    const float sx = float(p.width), sy = float(p.height);
+    const bool centered = (p.flags & GE_PT_PIXEL_CENTER) != 0;
+    f3 sum = splat(0.0f);
+    bool covered = false;
+    for (uint32_t s = 0; s < p.spp; ++s) {
+      uint32_t rng = rng_seed(x, y, p.sample_base + s, p.seed);
+      float jx = 0.5f, jy = 0.5f;
+      if (!centered) {
+        jx = rng_next(rng);
+        jy = rng_next(rng);
+      }
+      const float ndx = (float(x) + jx) / sx * 2.0f - 1.0f;
+      const float ndy = 1.0f - (float(y) + jy) / sy * 2.0f;
+      const float* m = p.clip_to_ray;
+      f3 dir;
+      dir.x = ((m[0] * ndx + m[4] * ndy) + m[8] * 0.0f) + m[12] * 1.0f;
+      dir.y = ((m[1] * ndx + m[5] * ndy) + m[9] * 0.0f) + m[13] * 1.0f;
+      dir.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
+      dir = normalize(dir);
+      bool hit_primary = false;
+      sum = sum + trace_path<Count>(c, ld3(p.camera), dir, rng, hit_primary, closest, any, tally);
+      covered = covered || hit_primary;
+    }
+    float4 prev = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (p.sample_base != 0) prev = accum[pixel];
+    accum[pixel] = make_float4(prev.x + sum.x, prev.y + sum.y, prev.z + sum.z, prev.w + float(p.spp));
+    const uint32_t before = p.sample_base == 0 ? 0u : coverage[pixel];
+    coverage[pixel] = before | (covered ? 1u : 0u);
+  }
+  if (Count) flush_counters(counters, closest, any, tally);
+}
+
+// tonemap_main (path_trace.slang:511-543) with display_output (display.slang).
+__global__ void __launch_bounds__(64) k_tonemap(ge_path_trace_params p, const float4* __restrict__ accum,
+                                                const uint32_t* __restrict__ coverage, uint32_t* __restrict__ out) {
+  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
+  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+  if (x >= p.width || y >= p.height) return;
+  const size_t pixel = size_t(y) * p.width + x;
+  const float4 v = accum[pixel];
+  float mean[3] = {0.0f, 0.0f, 0.0f};
+  if (v.w > 0.0f) {
+    mean[0] = v.x / v.w;
+    mean[1] = v.y / v.w;
+    mean[2] = v.z / v.w;
+  }
+  if (coverage[pixel] == 0 && (p.flags & GE_PT_UNIFORM_SKY) == 0) {
+    out[pixel] = p.background;
+    return;
+  }
+  float e[3];
+  for (int k = 0; k < 3; ++k) e[k] = __builtin_powf(__builtin_fmaxf(mean[k], 0.0f), 1.0f / 2.2f);
+  if (p.flags & GE_PT_DITHER) {
+    const uint32_t inner = x * 288237660u + y * 25070368u;
+    const uint32_t outer = uint32_t((uint64_t(inner) * uint64_t(3472289u)) >> 16);
+    const float u = float(outer >> 8) * (1.0f / 16777216.0f);
+    const float dither = u < 0.5f ? __builtin_sqrtf(2.0f * u) - 1.0f : 1.0f - __builtin_sqrtf(2.0f * (1.0f - u));
+    const float s = 255.0f;
+    for (int k = 0; k < 3; ++k) {
+      const float vv = __builtin_fminf(__builtin_fmaxf(e[k], 0.0f), 1.0f);
+      const float room = __builtin_fminf(__builtin_fminf(vv * s, (1.0f - vv) * s), 1.0f);
+      e[k] = vv + dither * room / s;
+    }
+  }
+  uint32_t rgb[3];
+  for (int k = 0; k < 3; ++k) rgb[k] = uint32_t(__builtin_fminf(__builtin_fmaxf(e[k], 0.0f), 1.0f) * 255.0f + 0.5f);
+  out[pixel] = rgb[0] | (rgb[1] << 8) | (rgb[2] << 16) | (255u << 24);
+}
+
+// The resolve's sun query (shadow_visibility + shadow_ray_blocked, visibility_resolve.slang:516-605,
+// 742-755): origin = position + geometric * ray_offset, one terminate-on-first-hit ray with the caster
+// mask, a ballot of the answers per 32 receivers.
+template <bool Count>
+__global__ void __launch_bounds__(64) k_shadow(Bvh bvh, SceneView sv, ge_shadow_params p,
+                                               const ge_shadow_receiver* __restrict__ rec, uint32_t* __restrict__ bits,
+                                               unsigned long long* counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  Tally tally{0, 0, 0};
+  uint64_t issued = 0;
+  bool blocked = false;
+  if (i < p.count) {
+    const ge_shadow_receiver r = rec[i];
+    if (r.instance_id != GE_RT_MISS && p.t_max > 0.0f) {
+      const ge_instance_desc in = sv.instances[r.instance_id];
+      const float offset = p.offset + p.offset_steps * sv.meshes[in.mesh].quant[3] * in.scale_max;
+      const f3 o = f3{r.position[0] + r.geometric[0] * offset, r.position[1] + r.geometric[1] * offset,
+                      r.position[2] + r.geometric[2] * offset};
+      issued = 1;
+      const Ray ray = make_ray(o, f3{p.sun[0], p.sun[1], p.sun[2]}, 0.0f);
+      Hit h;
+      blocked = traverse<true, Count>(bvh, ray, p.t_max, 0x01u, h, tally);
+    }
+  }
+  const uint64_t ballot = __ballot(blocked);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t word0 = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) >> 5;
+  if (lane == 0 && word0 * 32 < p.count) bits[word0] = uint32_t(ballot);
+  if (lane == 32 && (word0 + 1) * 32 < p.count) bits[word0 + 1] = uint32_t(ballot >> 32);
+  if (Count) flush_counters(counters, 0, issued, tally);
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
+                                ge_hit* hits, unsigned long long* counters) {
+  if (n == 0) return hipSuccess;
+  if (counters) k_trace_closest<true><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, hits, counters);
+  else k_trace_closest<false><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, hits, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
+                            uint32_t* bits, unsigned long long* counters) {
+  if (n == 0) return hipSuccess;
+  if (counters) k_trace_any<true><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, bits, counters);
+  else k_trace_any<false><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, bits, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
+                                 unsigned long long* counters) {
+  const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
+  if (counters) k_ray_visibility<true><<<grid, 64, 0, stream>>>(bvh, p, words, counters);
+  else k_ray_visibility<false><<<grid, 64, 0, stream>>>(bvh, p, words, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
+                             float4* accum, uint32_t* coverage, unsigned long long* counters) {
+  const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
+  Pt c{bvh, sv, p};
+  if (counters) k_path_trace<true><<<grid, 64, 0, stream>>>(c, accum, coverage, counters);
+  else k_path_trace<false><<<grid, 64, 0, stream>>>(c, accum, coverage, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, const float4* accum,
+                          const uint32_t* coverage, uint32_t* out) {
+  const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
+  k_tonemap<<<grid, 64, 0, stream>>>(p, accum, coverage, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
+                              const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters) {
+  if (p.count == 0) return hipSuccess;
+  if (counters) k_shadow<true><<<blocks(p.count, 64), 64, 0, stream>>>(bvh, sv, p, rec, bits, counters);
+  else k_shadow<false><<<blocks(p.count, 64), 64, 0, stream>>>(bvh, sv, p, rec, bits, counters);
+  return hipGetLastError();
+}
+
+}  // namespace ge

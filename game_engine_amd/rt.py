@@ -1,0 +1,309 @@
+"""Python host over the C ABI of the HIP backend (game_engine_amd/lib/libge_rt.so).
+
+`Context`, `Scene` and the trace calls map one to one onto include/ge_rt.h. `ReferenceRenderer`
+mirrors the engine's `renderer::ReferenceRenderer` (systems/renderer/include/systems/renderer/
+reference.h:51-170): `ReferenceSettings` in, a `ReferenceFrame` out (the accumulated linear
+radiance mean with its sample count, the display bytes, the trace milliseconds), the batch loop of
+`ceil(spp / batch)` dispatches with the same `(pixel, sample, seed)` streams, and an optional
+progress callback that can stop the render early.
+
+The backend has no CPU fallback: if the shared library is missing, or no GPU is present, every
+call raises.
+"""
+import ctypes as C
+import dataclasses
+import math
+import os
+import time
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ERRORS = {0: "ok", 1: "invalid argument", 2: "invalid scene", 3: "out of memory", 4: "device error",
+          5: "unsupported"}
+
+
+class RtError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{ERRORS.get(status, status)}: {message}")
+        self.status = status
+
+
+def library_path():
+    return os.path.join(_HERE, "lib", "libge_rt.so")
+
+
+def lib():
+    """Loads libge_rt.so; raises if it is not built (there is no fallback path)."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built: run __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+        sig = {
+            "ge_rt_create": ([C.POINTER(vp), i32], i32),
+            "ge_rt_destroy": ([vp], None),
+            "ge_rt_last_error": ([vp, C.c_char_p, C.c_size_t], i32),
+            "ge_rt_synchronize": ([vp], i32),
+            "ge_rt_scene_upload": ([vp, C.POINTER(abi.SceneDesc), C.POINTER(vp)], i32),
+            "ge_rt_scene_destroy": ([vp, vp], None),
+            "ge_rt_bvh_build": ([vp, vp, u32, vp, vp, u32], i32),
+            "ge_rt_scene_update_instances": ([vp, vp, u32, u32, vp], i32),
+            "ge_rt_bvh_info": ([vp, vp, u32, C.POINTER(u32), C.POINTER(u32)], i32),
+            "ge_rt_bvh_download": ([vp, vp, u32, vp, vp, vp, vp], i32),
+            "ge_rt_trace_closest": ([vp, vp, vp, u32, u32, vp, u32], i32),
+            "ge_rt_trace_any": ([vp, vp, vp, u32, u32, vp, u32], i32),
+            "ge_rt_ray_visibility": ([vp, vp, C.POINTER(abi.RayVisibilityParams), vp, u32], i32),
+            "ge_rt_path_trace": ([vp, vp, C.POINTER(abi.PathTraceParams), vp, vp, u32], i32),
+            "ge_rt_tonemap": ([vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, u32], i32),
+            "ge_rt_shadow_pass": ([vp, vp, C.POINTER(abi.ShadowParams), vp, vp, u32], i32),
+            "ge_rt_stats": ([vp, C.POINTER(abi.Counters)], i32),
+            "ge_rt_reset_stats": ([vp], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchronize", "ge_rt_scene_upload",
+           "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_bvh_info",
+           "ge_rt_bvh_download", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
+           "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_stats", "ge_rt_reset_stats")
+
+
+def _ptr(a):
+    return None if a is None else (a if isinstance(a, int) else a.ctypes.data)
+
+
+class Context:
+    """One HIP device and its stream (ge_rt_create)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        st = lib().ge_rt_create(C.byref(h), device)
+        if st != 0:
+            raise RtError(st, f"ge_rt_create(device={device}) failed")
+        self.h = h
+
+    def check(self, status):
+        if status != 0:
+            buf = C.create_string_buffer(1024)
+            lib().ge_rt_last_error(self.h, buf, 1024)
+            raise RtError(status, buf.value.decode())
+
+    def close(self):
+        if self.h:
+            lib().ge_rt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        self.check(lib().ge_rt_synchronize(self.h))
+
+    def stats(self):
+        c = abi.Counters()
+        self.check(lib().ge_rt_stats(self.h, C.byref(c)))
+        return {"closest": c.closest_rays, "any": c.any_rays, "nodes": c.nodes, "triangles": c.triangles,
+                "instances": c.instances, "kernel_ms": c.last_kernel_ms, "build_ms": c.last_build_ms}
+
+    def reset_stats(self):
+        self.check(lib().ge_rt_reset_stats(self.h))
+
+    def upload(self, desc):
+        return Scene(self, desc)
+
+
+class Scene:
+    """A scene on the device (ge_rt_scene_upload) and its structures (ge_rt_bvh_build)."""
+
+    def __init__(self, ctx, desc):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().ge_rt_scene_upload(ctx.h, C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().ge_rt_scene_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, flags=abi.BUILD_FULL, eye=None, meshes=None):
+        m = None if meshes is None else np.ascontiguousarray(meshes, dtype=np.uint32)
+        self.ctx.check(lib().ge_rt_bvh_build(self.ctx.h, self.h, flags, C.byref(eye) if eye else None, _ptr(m),
+                                             0 if m is None else len(m)))
+        return self
+
+    def update_instances(self, first, records):
+        self.ctx.check(lib().ge_rt_scene_update_instances(self.ctx.h, self.h, first, len(records), C.byref(records[0])))
+
+    def bvh(self, mesh):
+        n, p = C.c_uint32(), C.c_uint32()
+        self.ctx.check(lib().ge_rt_bvh_info(self.ctx.h, self.h, mesh, C.byref(n), C.byref(p)))
+        nodes = np.zeros(n.value, dtype=abi.NODE_DTYPE)
+        morton = np.zeros(p.value, dtype=np.uint32)
+        if mesh == 0xFFFFFFFF:
+            prims = np.zeros(p.value, dtype=np.uint32)
+            self.ctx.check(lib().ge_rt_bvh_download(self.ctx.h, self.h, mesh, _ptr(nodes), None, _ptr(prims), _ptr(morton)))
+        else:
+            prims = np.zeros(p.value, dtype=abi.TRI_DTYPE)
+            self.ctx.check(lib().ge_rt_bvh_download(self.ctx.h, self.h, mesh, _ptr(nodes), _ptr(prims), None, _ptr(morton)))
+        return nodes, prims, morton
+
+    def trace_closest(self, rays, mask=0xFF, count=False, hits=None, device=False, n=None):
+        n = len(rays) if n is None else n
+        hits = np.zeros(n, dtype=abi.HIT_DTYPE) if hits is None else hits
+        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        self.ctx.check(lib().ge_rt_trace_closest(self.ctx.h, self.h, _ptr(rays), n, mask, _ptr(hits), flags))
+        return hits
+
+    def trace_any(self, rays, mask=0xFF, count=False, bits=None, device=False, n=None):
+        n = len(rays) if n is None else n
+        bits = np.zeros((n + 31) // 32, dtype=np.uint32) if bits is None else bits
+        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        self.ctx.check(lib().ge_rt_trace_any(self.ctx.h, self.h, _ptr(rays), n, mask, _ptr(bits), flags))
+        return bits
+
+    def ray_visibility(self, params, count=False, words=None, device=False):
+        words = np.zeros(params.width * params.height, dtype=np.uint64) if words is None else words
+        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        self.ctx.check(lib().ge_rt_ray_visibility(self.ctx.h, self.h, C.byref(params), _ptr(words), flags))
+        return words
+
+    def path_trace(self, params, accum=None, coverage=None, count=False, device=False):
+        n = params.width * params.height
+        if not device:
+            accum = np.zeros((n, 4), dtype=np.float32) if accum is None else accum
+            coverage = np.zeros(n, dtype=np.uint32) if coverage is None else coverage
+        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        self.ctx.check(lib().ge_rt_path_trace(self.ctx.h, self.h, C.byref(params), _ptr(accum), _ptr(coverage), flags))
+        return accum, coverage
+
+    def shadow_pass(self, params, receivers, count=False, bits=None, device=False):
+        bits = np.zeros((params.count + 31) // 32, dtype=np.uint32) if bits is None else bits
+        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        self.ctx.check(lib().ge_rt_shadow_pass(self.ctx.h, self.h, C.byref(params), _ptr(receivers), _ptr(bits), flags))
+        return bits
+
+
+def tonemap(ctx, params, accum, coverage, device=False, output=None):
+    n = params.width * params.height
+    output = np.zeros(n, dtype=np.uint32) if output is None else output
+    ctx.check(lib().ge_rt_tonemap(ctx.h, C.byref(params), _ptr(accum), _ptr(coverage), _ptr(output),
+                                  abi.DEVICE_PTR if device else 0))
+    return output
+
+
+# ---- the engine's ReferenceRenderer, over the C ABI --------------------------------------------
+
+@dataclasses.dataclass
+class ReferenceSettings:
+    """renderer::ReferenceSettings (reference.h:51-84): what a reference render is asked for."""
+    spp: int = 64
+    max_bounces: int = 3
+    seed: int = 1
+    batch: int = 8
+    uniform_sky: bool = False
+    pixel_center: bool = False
+
+
+@dataclasses.dataclass
+class ReferenceFrame:
+    """renderer::ReferenceFrame (reference.h:86-98): the linear mean (rgb + count), RGBA8, timings."""
+    width: int = 0
+    height: int = 0
+    samples: int = 0
+    hdr: np.ndarray = None
+    color: np.ndarray = None
+    trace_ms: float = 0.0
+    seconds: float = 0.0
+
+
+class ReferenceRenderer:
+    """renderer::ReferenceRenderer (reference.h:114-170), with the HIP backend as its device.
+
+    `create` takes a context and an uploaded, built scene; `render` fills `PathTraceParams` from a
+    frame template (camera, sky, sun, lights, ray bias — what the engine's `frame_lighting` and
+    camera give it) and the settings, runs ceil(spp / batch) batches, then the tonemap and the
+    readback. Errors are returned as (False, message), as the engine's bool + error string.
+    """
+
+    def __init__(self):
+        self.ctx = None
+        self.scene = None
+        self.width = self.height = 0
+
+    def create(self, ctx, scene, width, height):
+        if ctx is None or scene is None:
+            return False, "no device or scene"
+        self.ctx, self.scene = ctx, scene
+        return self.resize(width, height)
+
+    def resize(self, width, height):
+        if width <= 0 or height <= 0:
+            return False, "empty extent"
+        self.width, self.height = width, height
+        return True, ""
+
+    def destroy(self):
+        self.ctx = self.scene = None
+
+    def valid(self):
+        return self.ctx is not None
+
+    def render(self, frame_params, settings, progress=None):
+        if not self.valid():
+            return False, "not created", None
+        p = abi.PathTraceParams.from_buffer_copy(frame_params)
+        p.width, p.height = self.width, self.height
+        p.max_bounces = settings.max_bounces
+        p.seed = settings.seed
+        p.flags = (abi.PT_UNIFORM_SKY if settings.uniform_sky else 0) | (abi.PT_PIXEL_CENTER if settings.pixel_center else 0)
+        if settings.uniform_sky:  # the furnace: the sun and the analytic lights are off (reference.h:61-66)
+            p.sun[3] = 0.0
+            p.light_count = 0
+        n = self.width * self.height
+        accum = np.zeros((n, 4), dtype=np.float32)
+        coverage = np.zeros(n, dtype=np.uint32)
+        batch = max(1, settings.batch)
+        total = max(0, settings.spp)
+        start = time.perf_counter()
+        trace_ms = 0.0
+        done = 0
+        try:
+            while done < total:
+                p.spp = min(batch, total - done)
+                p.sample_base = done
+                self.scene.path_trace(p, accum, coverage)
+                trace_ms += self.ctx.stats()["kernel_ms"]
+                done += p.spp
+                if progress is not None and not progress(done, total):
+                    break
+            out = tonemap(self.ctx, p, accum, coverage)
+        except RtError as e:
+            return False, str(e), None
+        hdr = accum.copy()
+        nz = hdr[:, 3] > 0
+        hdr[nz, :3] /= hdr[nz, 3:4]
+        color = np.stack([(out >> s) & 255 for s in (0, 8, 16, 24)], axis=-1).astype(np.uint8)
+        frame = ReferenceFrame(self.width, self.height, done, hdr, color, trace_ms, time.perf_counter() - start)
+        return True, "", frame

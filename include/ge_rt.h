@@ -384,8 +384,8 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene,
 /* One batch of the reference path tracer (path_trace_main, path_trace.slang:463-495), replacing one
  * iteration of ReferenceRenderer::render's batch loop (reference.cpp:323-347). Adds params->spp
  * samples numbered from params->sample_base into accum (float4 per pixel: rgb sum, w count) and
- * ORs the primary coverage into coverage (u32 per pixel). With GE_RT_DEVICE_PTR the block's own
- * accum/coverage addresses are used and the pointers given here are ignored. */
+ * ORs the primary coverage into coverage (u32 per pixel). With GE_RT_DEVICE_PTR both are device
+ * pointers (null takes the block's own `accum` / `coverage` address). */
 int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
                          float* accum, uint32_t* coverage, uint32_t flags);
 /* The tonemap (tonemap_main, path_trace.slang:511-543): accum + coverage -> packed RGBA8. */
