@@ -487,14 +487,16 @@ inline void prep_ray(TRay& r) {
   r.Sz = 1.0f / r.d[kz];
 }
 
+// min/max as compares: the operands are never NaN here, and a -0/+0 choice cannot change the
+// comparisons the walk makes with these values (the box fit uses the IEEE fmin/fmax).
+inline float mn(float a, float b) { return b < a ? b : a; }
+inline float mx(float a, float b) { return a < b ? b : a; }
 inline bool slab(const float lo[3], const float hi[3], const TRay& r, float tmax, float& tnear) {
   const float tx0 = (lo[0] - r.o[0]) * r.inv[0], tx1 = (hi[0] - r.o[0]) * r.inv[0];
   const float ty0 = (lo[1] - r.o[1]) * r.inv[1], ty1 = (hi[1] - r.o[1]) * r.inv[1];
   const float tz0 = (lo[2] - r.o[2]) * r.inv[2], tz1 = (hi[2] - r.o[2]) * r.inv[2];
-  const float tn = std::fmax(std::fmax(std::fmin(tx0, tx1), std::fmin(ty0, ty1)),
-                             std::fmax(std::fmin(tz0, tz1), r.tmin));
-  const float tf = std::fmin(std::fmin(std::fmax(tx0, tx1), std::fmax(ty0, ty1)),
-                             std::fmin(std::fmax(tz0, tz1), tmax));
+  const float tn = mx(mx(mn(tx0, tx1), mn(ty0, ty1)), mx(mn(tz0, tz1), r.tmin));
+  const float tf = mn(mn(mx(tx0, tx1), mx(ty0, ty1)), mn(mx(tz0, tz1), tmax));
   tnear = tn;
   return tn <= tf * k_box_robust;
 }
