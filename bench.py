@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--megakernel", action="store_true", help="one path per lane instead of the wavefront")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the path tracer runs on one MI355X and has no multi-GPU path "
@@ -110,7 +111,7 @@ def main():
 
     def frame(k, count=False):
         pt.sample_base = k * pt.spp
-        scene.path_trace(pt, A, Cv, count=count, device=True)
+        scene.path_trace(pt, A, Cv, count=count, device=True, megakernel=args.megakernel)
 
     for w in range(args.warmup):
         frame(w)

@@ -118,6 +118,7 @@ assert NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and RECEIVER_DTYPE
 OK = 0
 DEVICE_PTR = 0x1
 COUNT = 0x2
+PT_MEGAKERNEL = 0x4
 BVH_LEAF = 0x80000000
 BVH_NONE = 0xFFFFFFFF
 MISS = 0xFFFFFFFF

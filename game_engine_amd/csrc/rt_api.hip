@@ -87,6 +87,7 @@ Bvh bvh_view(const SceneDev& s) {
   b.nodes = s.blas_nodes.as<ge_bvh_node>();
   b.tris = s.blas_tris.as<ge_bvh_triangle>();
   b.tlas_node_count = s.tlas_node_count;
+  b.stack_cap = s.stack_cap;
   return b;
 }
 
@@ -162,6 +163,10 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
     b->release();
   for (DevBuf& b : ctx->io) b.release();
   for (DevBuf& b : ctx->tmp) b.release();
+  for (DevBuf* b : {&ctx->wave.state[0], &ctx->wave.state[1], &ctx->wave.hits, &ctx->wave.nee, &ctx->wave.result,
+                    &ctx->wave.queue[0], &ctx->wave.queue[1], &ctx->wave.counts, &ctx->wave.shadow, &ctx->wave.blocked})
+    b->release();
+  if (ctx->wave.host_counts) (void)hipHostFree(ctx->wave.host_counts);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->bev0) (void)hipEventDestroy(ctx->bev0);
@@ -474,6 +479,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_CHECK(ctx, s.tlas_height_value + blas_max + 2 <= uint32_t(k_stack), GE_RT_ERROR_UNSUPPORTED,
            "the structures are " + std::to_string(s.tlas_height_value) + " + " + std::to_string(blas_max) +
                " levels deep, past the traversal stack of " + std::to_string(k_stack));
+  s.stack_cap = s.tlas_height_value + blas_max + 2;
   s.built = true;
   return GE_RT_OK;
 }
@@ -669,8 +675,12 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
   SceneView svl = sv;
   svl.light_count = params->light_count;
   GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  GE_HIP(ctx, launch_path_trace(ctx->stream, bvh_view(scene->dev), svl, p, d_accum, d_cov,
-                                (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+  unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
+  if (flags & GE_RT_PT_MEGAKERNEL) {
+    GE_HIP(ctx, launch_path_trace(ctx->stream, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
+  } else {
+    GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
+  }
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(accum, d_accum, n * 16, hipMemcpyDeviceToHost, ctx->stream));

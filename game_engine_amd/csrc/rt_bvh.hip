@@ -53,15 +53,41 @@ __global__ void k_init_bounds(uint32_t nseg, uint32_t* cbox) {
   }
 }
 
+// Each wave reduces its primitives' centroid bounds with cross-lane min/max when all of its lanes
+// are in one segment (the common case: segments are contiguous), so a segment of n primitives
+// costs n / 64 atomics per bound instead of n.
 __global__ void k_centroid_bounds(uint32_t n, const float* __restrict__ boxes, const uint32_t* __restrict__ seg_of,
                                   uint32_t* cbox) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t s = seg_of[i];
+  const bool valid = i < n;
+  const uint32_t s = valid ? seg_of[i] : 0xffffffffu;
+  uint32_t lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
-    const float c = (boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f;
-    atomicMin(&cbox[s * 6 + a], ordered(c));
-    atomicMax(&cbox[s * 6 + 3 + a], ordered(c));
+    const float c = valid ? (boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f : 0.0f;
+    lo[a] = valid ? ordered(c) : 0xffffffffu;
+    hi[a] = valid ? ordered(c) : 0u;
+  }
+  const uint32_t s0 = __shfl(s, 0);
+  const bool uniform = __all(!valid || s == s0);
+  if (uniform) {
+    for (int a = 0; a < 3; ++a) {
+      for (int off = 32; off > 0; off >>= 1) {
+        lo[a] = min(lo[a], uint32_t(__shfl_xor(int(lo[a]), off)));
+        hi[a] = max(hi[a], uint32_t(__shfl_xor(int(hi[a]), off)));
+      }
+    }
+    if ((threadIdx.x & 63u) == 0 && s0 != 0xffffffffu) {
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(&cbox[s0 * 6 + a], lo[a]);
+        atomicMax(&cbox[s0 * 6 + 3 + a], hi[a]);
+      }
+    }
+    return;
+  }
+  if (!valid) return;
+  for (int a = 0; a < 3; ++a) {
+    atomicMin(&cbox[s * 6 + a], lo[a]);
+    atomicMax(&cbox[s * 6 + 3 + a], hi[a]);
   }
 }
 

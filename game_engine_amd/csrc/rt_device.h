@@ -102,7 +102,7 @@ struct Bvh {
   const ge_bvh_node* nodes;      // every mesh's nodes, pooled
   const ge_bvh_triangle* tris;   // every mesh's sorted triangles, pooled
   uint32_t tlas_node_count;      // 0: an empty scene
-  uint32_t pad;
+  uint32_t stack_cap;            // entries of the per-lane LDS stack: both structures' heights + 2
 };
 
 struct Tally {
@@ -111,7 +111,14 @@ struct Tally {
 
 constexpr float k_box_robust = 1.00000095367431640625f;  // 1 + 2^-20 on the far distance (Ize 2013)
 constexpr uint32_t k_sentinel = 0xfffffffeu;
+// The most entries a lane's traversal stack may need: the build refuses deeper structures.
 constexpr int k_stack = 96;
+// Each kernel's workgroup is one wave of 64 lanes. The top k_ring entries of a lane's stack live in
+// an LDS ring (entry k of lane l at word 64 (k mod k_ring) + l, so a wave's pushes and pops hit 64
+// different banks); deeper entries spill to the lane's private memory and come back as the stack
+// shrinks, so LDS stays at 4 KiB a wave whatever the structures' depth.
+constexpr int k_lanes = 64;
+constexpr int k_ring = 16;
 
 struct Ray {
   float o[3], d[3], inv[3];
@@ -119,6 +126,8 @@ struct Ray {
   int kx, ky, kz;
   float Sx, Sy, Sz;
 };
+
+GE_DEV float sel3(const float v[3], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
 
 GE_DEV void prep(Ray& r) {
 #pragma unroll
@@ -131,7 +140,7 @@ GE_DEV void prep(Ray& r) {
   if (__builtin_fabsf(r.d[2]) > __builtin_fabsf(r.d[kz])) kz = 2;
   int kx = kz == 2 ? 0 : kz + 1;
   int ky = kx == 2 ? 0 : kx + 1;
-  if (r.d[kz] < 0.0f) {
+  if (sel3(r.d, kz) < 0.0f) {
     const int t = kx;
     kx = ky;
     ky = t;
@@ -139,9 +148,11 @@ GE_DEV void prep(Ray& r) {
   r.kx = kx;
   r.ky = ky;
   r.kz = kz;
-  r.Sx = r.d[kx] / r.d[kz];
-  r.Sy = r.d[ky] / r.d[kz];
-  r.Sz = 1.0f / r.d[kz];
+  // The shear shares the slab test's reciprocal of the dominant component (never clamped: it is
+  // the largest of a non-zero direction), so a ray costs three divisions.
+  r.Sz = sel3(r.inv, kz);
+  r.Sx = sel3(r.d, kx) * r.Sz;
+  r.Sy = sel3(r.d, ky) * r.Sz;
 }
 
 GE_DEV Ray make_ray(f3 o, f3 d, float tmin) {
@@ -156,8 +167,6 @@ GE_DEV Ray make_ray(f3 o, f3 d, float tmin) {
   prep(r);
   return r;
 }
-
-GE_DEV float sel3(const float v[3], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
 
 // The slab test with the far distance widened by 1 + 2^-20, so a box is never missed by a ray
 // that hits a triangle inside it (Ize 2013, "Robust BVH Ray Traversal").
@@ -237,15 +246,31 @@ GE_DEV void to_object(const InstRec& rec, const Ray& w, Ray& o) {
 // ordered by (t, then the larger pair << 8 | triangle), a total order, so the answer does not depend
 // on the walk; the counters do, and match the oracle's walk.
 template <bool AnyHit, bool Count>
-GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask, Hit& hit, Tally& tally) {
+GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask, Hit& hit, Tally& tally,
+                     uint32_t* __restrict__ stack) {
   hit.found = false;
   hit.t = tmax;
   hit.key = 0;
   hit.u = hit.v = 0.0f;
   hit.inst = hit.prim = GE_RT_MISS;
   if (mask == 0 || bvh.tlas_node_count == 0) return false;
-  uint32_t stack[k_stack];
   int sp = 0;
+  const int cap = int(bvh.stack_cap);
+  uint32_t spill[k_stack - k_ring];
+  auto push = [&](uint32_t x) {
+    if (sp >= cap) return;
+    uint32_t& slot = stack[k_lanes * (sp & (k_ring - 1))];
+    if (sp >= k_ring) spill[sp - k_ring] = slot;
+    slot = x;
+    ++sp;
+  };
+  auto pop = [&]() {
+    --sp;
+    uint32_t& slot = stack[k_lanes * (sp & (k_ring - 1))];
+    const uint32_t x = slot;
+    if (sp >= k_ring) slot = spill[sp - k_ring];
+    return x;
+  };
   Ray ray = world;
   bool in_blas = false;
   uint32_t node_base = 0, tri_base = 0, first_pair = 0, inst_id = 0;
@@ -258,7 +283,7 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
       tri_base = rec.tri_base;
       first_pair = rec.first_pair;
       to_object(rec, world, ray);
-      stack[sp++] = k_sentinel;
+      push(k_sentinel);
       in_blas = true;
       if (Count) ++tally.inst;
       cur = 0;
@@ -309,7 +334,7 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
     }
     if (h0 && h1) {
       const bool second_first = tn1 < tn0;
-      stack[sp++] = second_first ? child[0] : child[1];
+      push(second_first ? child[0] : child[1]);
       cur = second_first ? child[1] : child[0];
       continue;
     }
@@ -327,7 +352,7 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
         done = true;
         break;
       }
-      const uint32_t e = stack[--sp];
+      const uint32_t e = pop();
       if (e == k_sentinel) {
         in_blas = false;
         ray = world;

@@ -61,8 +61,44 @@ struct SceneDev {
   uint32_t tlas_prims = 0, tlas_node_count = 0;
   std::vector<uint32_t> h_mesh_height;
   uint32_t tlas_height_value = 0;
+  uint32_t stack_cap = 0;  // traversal stack entries a lane needs: both heights + 2
   bool built = false;
   ge_frame_eye eye{};
+};
+
+// ---- the wavefront path tracer's records (rt_wavefront.hip) ----
+struct PathState {  // 64 bytes: one live path
+  float o[3];
+  uint32_t pad;
+  float d[3];
+  uint32_t rng;
+  float T[3];      // throughput
+  uint32_t bounce;
+  float L[3];      // radiance so far
+  uint32_t flags;  // bit 0: the camera ray hit; bit 1: the path ended
+};
+struct PathHit {  // 20 bytes: the closest hit of the path's ray
+  float t, u, v;
+  uint32_t inst, prim;
+};
+struct NeeRec {  // 32 bytes: a vertex's next-event estimation, resolved after its shadow rays
+  float T[3];
+  uint32_t first;
+  uint32_t count;  // ~0: the ray missed, nothing to resolve
+  uint32_t pad[3];
+};
+struct ShadowRec {  // 48 bytes: one shadow ray and what its light adds if nothing blocks it
+  float o[3];
+  float tmax;
+  float d[3];
+  uint32_t pad0;
+  float c[3];
+  uint32_t pad1;
+};
+struct Wavefront {
+  DevBuf state[2], hits, nee, result, queue[2], counts, shadow, blocked;
+  uint64_t shadow_capacity = 0;
+  uint32_t* host_counts = nullptr;  // pinned
 };
 
 }  // namespace ge
@@ -83,6 +119,7 @@ struct ge_rt_ctx {
   ge::DevBuf counters; // 5 x u64
   double last_kernel_ms = 0.0, last_build_ms = 0.0;
   bool timing_pending = false;
+  ge::Wavefront wave;
 };
 
 namespace ge {
@@ -138,6 +175,10 @@ hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView
                              float4* accum, uint32_t* coverage, unsigned long long* counters);
 hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, const float4* accum,
                           const uint32_t* coverage, uint32_t* out);
+// The wavefront form of launch_path_trace (rt_wavefront.hip); synchronizes once a bounce to size
+// the next launches.
+hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
+                                float4* accum, uint32_t* coverage, unsigned long long* counters);
 hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
                               const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters);
 

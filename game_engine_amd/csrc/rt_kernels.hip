@@ -11,6 +11,10 @@
 namespace ge {
 namespace {
 
+// The per-lane traversal stack rings (dynamic LDS: k_ring entries x 64 lanes x 4 bytes).
+extern __shared__ uint32_t g_lds[];
+__device__ __forceinline__ uint32_t* lane_stack() { return g_lds + threadIdx.x; }
+
 __device__ __forceinline__ void flush_counters(unsigned long long* counters, uint64_t closest, uint64_t any,
                                                const Tally& t) {
   // Reduce across the wave, then one lane adds.
@@ -40,7 +44,7 @@ __global__ void __launch_bounds__(64) k_trace_closest(Bvh bvh, const ge_ray* __r
     const Ray ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
     Hit h;
     issued = 1;
-    const bool found = traverse<false, Count>(bvh, ray, r.tmax, mask, h, tally);
+    const bool found = traverse<false, Count>(bvh, ray, r.tmax, mask, h, tally, lane_stack());
     ge_hit out;
     out.pad = 0;
     if (found) {
@@ -78,7 +82,7 @@ __global__ void __launch_bounds__(64) k_trace_any(Bvh bvh, const ge_ray* __restr
       issued = 1;
       const Ray ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
       Hit h;
-      blocked = traverse<true, Count>(bvh, ray, r.tmax, mask, h, tally);
+      blocked = traverse<true, Count>(bvh, ray, r.tmax, mask, h, tally, lane_stack());
     }
   }
   const uint64_t ballot = __ballot(blocked);
@@ -115,7 +119,7 @@ __global__ void __launch_bounds__(64) k_ray_visibility(Bvh bvh, ge_ray_visibilit
     Hit h;
     issued = 1;
     uint64_t value = 0;
-    if (traverse<false, Count>(bvh, ray, 1e30f, 0xfeu, h, tally)) {
+    if (traverse<false, Count>(bvh, ray, 1e30f, 0xfeu, h, tally, lane_stack())) {
       const float pt[3] = {o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t};
       const float* vp = p.view_proj;
       const float cz = ((vp[2] * pt[0] + vp[6] * pt[1]) + vp[10] * pt[2]) + vp[14] * 1.0f;
@@ -141,7 +145,7 @@ __device__ bool occluded(const Pt& c, f3 origin, f3 direction, float t_max, uint
   ++any;
   const Ray ray = make_ray(origin, direction, 0.0f);
   Hit h;
-  return traverse<true, Count>(c.bvh, ray, t_max, 0x01u, h, tally);
+  return traverse<true, Count>(c.bvh, ray, t_max, 0x01u, h, tally, lane_stack());
 }
 
 __device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool primary) {
@@ -242,7 +246,7 @@ __device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bo
     ++closest;
     const Ray ray = make_ray(origin, direction, 0.0f);
     Hit hit;
-    traverse<false, Count>(c.bvh, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, tally);
+    traverse<false, Count>(c.bvh, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, tally, lane_stack());
     if (!hit.found) {
       radiance = radiance + throughput * environment(p, direction, bounce == 0);
       break;
@@ -408,7 +412,7 @@ __global__ void __launch_bounds__(64) k_shadow(Bvh bvh, SceneView sv, ge_shadow_
       issued = 1;
       const Ray ray = make_ray(o, f3{p.sun[0], p.sun[1], p.sun[2]}, 0.0f);
       Hit h;
-      blocked = traverse<true, Count>(bvh, ray, p.t_max, 0x01u, h, tally);
+      blocked = traverse<true, Count>(bvh, ray, p.t_max, 0x01u, h, tally, lane_stack());
     }
   }
   const uint64_t ballot = __ballot(blocked);
@@ -420,30 +424,31 @@ __global__ void __launch_bounds__(64) k_shadow(Bvh bvh, SceneView sv, ge_shadow_
 }
 
 inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+inline size_t lds(const Bvh&) { return size_t(k_ring) * k_lanes * 4; }
 
 }  // namespace
 
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
                                 ge_hit* hits, unsigned long long* counters) {
   if (n == 0) return hipSuccess;
-  if (counters) k_trace_closest<true><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, hits, counters);
-  else k_trace_closest<false><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, hits, counters);
+  if (counters) k_trace_closest<true><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, hits, counters);
+  else k_trace_closest<false><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, hits, counters);
   return hipGetLastError();
 }
 
 hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
                             uint32_t* bits, unsigned long long* counters) {
   if (n == 0) return hipSuccess;
-  if (counters) k_trace_any<true><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, bits, counters);
-  else k_trace_any<false><<<blocks(n, 64), 64, 0, stream>>>(bvh, rays, n, mask, bits, counters);
+  if (counters) k_trace_any<true><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, bits, counters);
+  else k_trace_any<false><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, bits, counters);
   return hipGetLastError();
 }
 
 hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
                                  unsigned long long* counters) {
   const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
-  if (counters) k_ray_visibility<true><<<grid, 64, 0, stream>>>(bvh, p, words, counters);
-  else k_ray_visibility<false><<<grid, 64, 0, stream>>>(bvh, p, words, counters);
+  if (counters) k_ray_visibility<true><<<grid, 64, lds(bvh), stream>>>(bvh, p, words, counters);
+  else k_ray_visibility<false><<<grid, 64, lds(bvh), stream>>>(bvh, p, words, counters);
   return hipGetLastError();
 }
 
@@ -451,8 +456,8 @@ hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView
                              float4* accum, uint32_t* coverage, unsigned long long* counters) {
   const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
   Pt c{bvh, sv, p};
-  if (counters) k_path_trace<true><<<grid, 64, 0, stream>>>(c, accum, coverage, counters);
-  else k_path_trace<false><<<grid, 64, 0, stream>>>(c, accum, coverage, counters);
+  if (counters) k_path_trace<true><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
+  else k_path_trace<false><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
   return hipGetLastError();
 }
 
@@ -466,8 +471,8 @@ hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, con
 hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
                               const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters) {
   if (p.count == 0) return hipSuccess;
-  if (counters) k_shadow<true><<<blocks(p.count, 64), 64, 0, stream>>>(bvh, sv, p, rec, bits, counters);
-  else k_shadow<false><<<blocks(p.count, 64), 64, 0, stream>>>(bvh, sv, p, rec, bits, counters);
+  if (counters) k_shadow<true><<<blocks(p.count, 64), 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters);
+  else k_shadow<false><<<blocks(p.count, 64), 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters);
   return hipGetLastError();
 }
 

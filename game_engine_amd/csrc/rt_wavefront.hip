@@ -1,0 +1,556 @@
+// rt_wavefront.hip — the reference path tracer as a wavefront: one kernel per stage, paths in
+// queues compacted with wave ballots, so each traversal kernel runs one kind of ray with a small
+// register footprint instead of the whole integrator in every lane.
+//
+// Per bounce, over the queue of live paths:
+//   closest  — the closest hit of each path's ray (camera mask on bounce 0, caster mask after);
+//   shade    — a miss adds the environment and ends the path; a hit decodes the surface, adds its
+//              emission, writes one shadow-ray record per light next-event estimation asks about
+//              (sun first, then the lights in order, as next_event does, path_trace.slang:215-252),
+//              samples the BSDF, plays roulette and appends the path to the next queue;
+//   any      — one terminate-on-first-hit query per shadow record;
+//   resolve  — each path sums its unblocked lights in record order and adds throughput x that sum.
+// The sums are formed in the shader's order, so every path's radiance is the same float as the
+// one-path-per-lane integrator's (and the oracle's).
+//
+// Path state is double-buffered (read from one, write the other), which makes a shade launch
+// idempotent: if its shadow records overflow their buffer, the host grows it and runs the launch
+// again.
+#include "rt_internal.h"
+#include "rt_shade.h"
+
+namespace ge {
+namespace {
+
+extern __shared__ uint32_t g_lds[];
+__device__ __forceinline__ uint32_t* lane_stack() { return g_lds + threadIdx.x; }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t lane = threadIdx.x & 63u;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Appends `count` entries per lane where `count` > 0, one atomic per wave; returns each lane's first
+// slot (in lane order within the wave).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t count) {
+  // Inclusive prefix of `count` over the wave by shuffles.
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t incl = count;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = uint32_t(__shfl_up(int(incl), off));
+    if (lane >= uint32_t(off)) incl += v;
+  }
+  const uint32_t total = uint32_t(__shfl(int(incl), 63));
+  uint32_t base = 0;
+  if (lane == 63 && total) base = atomicAdd(counter, total);
+  base = uint32_t(__shfl(int(base), 63));
+  return base + incl - count;
+}
+
+__device__ __forceinline__ void flush(unsigned long long* counters, uint64_t closest, uint64_t any, const Tally& t) {
+  uint64_t v[5] = {closest, any, t.nodes, t.tris, t.inst};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    uint64_t x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    v[k] = x;
+  }
+  if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (v[k]) atomicAdd(&counters[k], (unsigned long long)v[k]);
+  }
+}
+
+// Camera rays in 8 x 8 pixel tiles, samples outermost: path id = sample * pixels + pixel.
+__global__ void __launch_bounds__(64) k_generate(ge_path_trace_params p, uint32_t tiles_x, uint32_t tiles,
+                                                 PathState* __restrict__ state, uint32_t* __restrict__ queue,
+                                                 uint32_t* __restrict__ queue_count) {
+  const uint32_t tile = blockIdx.x % tiles, sample = blockIdx.x / tiles;
+  const uint32_t x = (tile % tiles_x) * 8 + (threadIdx.x & 7u);
+  const uint32_t y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
+  const bool valid = x < p.width && y < p.height;
+  const uint32_t pixels = p.width * p.height;
+  const uint32_t slot = wave_append(queue_count, valid ? 1u : 0u);
+  if (!valid) return;
+  const uint32_t pid = sample * pixels + y * p.width + x;
+  uint32_t rng = rng_seed(x, y, p.sample_base + sample, p.seed);
+  float jx = 0.5f, jy = 0.5f;
+  if (!(p.flags & GE_PT_PIXEL_CENTER)) {
+    jx = rng_next(rng);
+    jy = rng_next(rng);
+  }
+  const float ndx = (float(x) + jx) / float(p.width) * 2.0f - 1.0f;
+  const float ndy = 1.0f - (float(y) + jy) / float(p.height) * 2.0f;
+  const float* m = p.clip_to_ray;
+  f3 dir;
+  dir.x = ((m[0] * ndx + m[4] * ndy) + m[8] * 0.0f) + m[12] * 1.0f;
+  dir.y = ((m[1] * ndx + m[5] * ndy) + m[9] * 0.0f) + m[13] * 1.0f;
+  dir.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
+  dir = normalize(dir);
+  PathState st;
+  st.o[0] = p.camera[0];
+  st.o[1] = p.camera[1];
+  st.o[2] = p.camera[2];
+  st.d[0] = dir.x;
+  st.d[1] = dir.y;
+  st.d[2] = dir.z;
+  st.rng = rng;
+  st.T[0] = st.T[1] = st.T[2] = 1.0f;
+  st.L[0] = st.L[1] = st.L[2] = 0.0f;
+  st.bounce = 0;
+  st.flags = 0;
+  st.pad = 0;
+  state[pid] = st;
+  queue[slot] = pid;
+}
+
+template <bool Count>
+__global__ void __launch_bounds__(64) k_closest(Bvh bvh, const PathState* __restrict__ state,
+                                                const uint32_t* __restrict__ queue, uint32_t n,
+                                                PathHit* __restrict__ hits, unsigned long long* counters) {
+  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+  Tally tally{0, 0, 0};
+  uint64_t issued = 0;
+  if (g < n) {
+    const uint32_t pid = queue[g];
+    const PathState& st = state[pid];
+    const Ray ray = make_ray(f3{st.o[0], st.o[1], st.o[2]}, f3{st.d[0], st.d[1], st.d[2]}, 0.0f);
+    Hit h;
+    issued = 1;
+    traverse<false, Count>(bvh, ray, 1e30f, st.bounce == 0 ? 0xfeu : 0x01u, h, tally, lane_stack());
+    PathHit out;
+    out.t = h.t;
+    out.u = h.u;
+    out.v = h.v;
+    out.inst = h.found ? h.inst : GE_RT_MISS;
+    out.prim = h.prim;
+    hits[pid] = out;
+  }
+  if (Count) flush(counters, issued, 0, tally);
+}
+
+__device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool primary) {
+  const f3 sky = ld3(p.sky);
+  if (p.flags & GE_PT_UNIFORM_SKY) return sky;
+  if (primary) {
+    return f3{__builtin_powf(maxr(sky.x, 0.0f), 2.2f), __builtin_powf(maxr(sky.y, 0.0f), 2.2f),
+              __builtin_powf(maxr(sky.z, 0.0f), 2.2f)};
+  }
+  const f3 sky_r = sky * 0.35f;
+  const f3 ground = ld3(p.ground) * (splat(p.sun[3] * maxr(p.sun[1], 0.0f) / k_pi) + sky_r);
+  return direction.y >= 0.0f ? sky_r : ground;
+}
+
+struct Vertex {
+  f3 position, normal, geometric;
+  uint32_t cluster;
+};
+
+// fetch_triangle + surface_at (material.slang:244-302).
+__device__ Vertex vertex_of(const SceneView& sv, const ge_path_trace_params& p, const PathHit& hit, f3 towards,
+                            const ge_instance_desc& in) {
+  const ge_mesh_desc mesh = sv.meshes[in.mesh];
+  const uint32_t cluster_index = mesh.first_cluster + (hit.prim >> 8);
+  const ge_cluster_desc cl = sv.clusters[cluster_index];
+  const uint32_t packed = sv.triangles[cl.triangle_offset + (hit.prim & 0xffu)];
+  const f3 T = f3{relative_axis(in.cell[0], in.rows[0][3], p.eye.cell[0], p.eye.local[0], p.eye.residual[0]),
+                  relative_axis(in.cell[1], in.rows[1][3], p.eye.cell[1], p.eye.local[1], p.eye.residual[1]),
+                  relative_axis(in.cell[2], in.rows[2][3], p.eye.cell[2], p.eye.local[2], p.eye.residual[2])};
+  f3 pc[3];
+  uint32_t local[3];
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(sv.quantized);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    local[k] = cl.vertex_offset + ((packed >> (8 * k)) & 0xffu);
+    const uint32_t i = local[k] * 3;
+    const uint32_t w0 = words[i >> 1], w1 = words[(i + 1) >> 1], w2 = words[(i + 2) >> 1];
+    const uint32_t qx = (i & 1) ? (w0 >> 16) : (w0 & 0xffffu);
+    const uint32_t qy = ((i + 1) & 1) ? (w1 >> 16) : (w1 & 0xffffu);
+    const uint32_t qz = ((i + 2) & 1) ? (w2 >> 16) : (w2 & 0xffffu);
+    const f3 m = f3{mesh.quant[0], mesh.quant[1], mesh.quant[2]} + f3{float(qx), float(qy), float(qz)} * mesh.quant[3];
+    f3 d;
+    d.x = in.rows[0][0] * m.x + in.rows[0][1] * m.y + in.rows[0][2] * m.z;
+    d.y = in.rows[1][0] * m.x + in.rows[1][1] * m.y + in.rows[1][2] * m.z;
+    d.z = in.rows[2][0] * m.x + in.rows[2][1] * m.y + in.rows[2][2] * m.z;
+    pc[k] = d + T;
+  }
+  const f3 b = f3{1.0f - hit.u - hit.v, hit.u, hit.v};
+  Vertex out;
+  out.position = b.x * pc[0] + b.y * pc[1] + b.z * pc[2];
+  out.geometric = normalize(cross(pc[1] - pc[0], pc[2] - pc[0]));
+  if (dot(out.geometric, towards - out.position) < 0.0f) out.geometric = -out.geometric;
+  if (sv.attributes) {
+    const f3 n0 = decode_normal_oct(sv.attributes[local[0]].normal_oct);
+    const f3 n1 = decode_normal_oct(sv.attributes[local[1]].normal_oct);
+    const f3 n2 = decode_normal_oct(sv.attributes[local[2]].normal_oct);
+    const f3 n = b.x * n0 + b.y * n1 + b.z * n2;
+    out.normal = transform_normal(in, n);
+  } else {
+    out.normal = out.geometric;
+  }
+  out.cluster = cluster_index;
+  return out;
+}
+
+struct ShadeArgs {
+  SceneView sv;
+  ge_path_trace_params p;
+  const PathState* in;
+  PathState* out;
+  const uint32_t* queue;
+  uint32_t n;
+  uint32_t shadow_capacity;
+  const PathHit* hits;
+  NeeRec* nee;
+  ShadowRec* shadow;
+  uint32_t* next_queue;
+  uint32_t* counts;  // [0] next queue, [1] shadow records
+};
+
+// Which lights next-event estimation traces from this surface, in next_event's order: the sun, then
+// each light whose sample is not zero and that is in front of the shading normal.
+__device__ __forceinline__ bool sun_asked(const ge_path_trace_params& p, const Surface& s) {
+  return p.sun[3] > 0.0f && dot(s.normal, ld3(p.sun)) > 0.0f;
+}
+
+__global__ void __launch_bounds__(64) k_shade(ShadeArgs a) {
+  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+  const bool valid = g < a.n;
+  const uint32_t pid = valid ? a.queue[g] : 0u;
+  PathState st;
+  bool alive = false;
+  uint32_t nshadow = 0;
+  Surface surface;
+  Vertex vx;
+  float bias = 0.0f;
+  f3 front = splat(0.0f);
+  const ge_path_trace_params& p = a.p;
+  bool hit_found = false;
+  if (valid) {
+    st = a.in[pid];
+    const PathHit hit = a.hits[pid];
+    const f3 dir = f3{st.d[0], st.d[1], st.d[2]};
+    f3 L = f3{st.L[0], st.L[1], st.L[2]};
+    const f3 T = f3{st.T[0], st.T[1], st.T[2]};
+    if (hit.inst == GE_RT_MISS) {
+      L = L + T * environment(p, dir, st.bounce == 0);
+      st.L[0] = L.x;
+      st.L[1] = L.y;
+      st.L[2] = L.z;
+      NeeRec nr;
+      nr.T[0] = nr.T[1] = nr.T[2] = 0.0f;
+      nr.first = 0;
+      nr.count = GE_RT_MISS;
+      a.nee[pid] = nr;
+    } else {
+      hit_found = true;
+      if (st.bounce == 0) st.flags |= 1u;
+      const ge_instance_desc in = a.sv.instances[hit.inst];
+      const f3 origin = f3{st.o[0], st.o[1], st.o[2]};
+      vx = vertex_of(a.sv, p, hit, origin, in);
+      const ge_resolve_material* mat = a.sv.materials + (a.sv.cluster_materials[vx.cluster] + in.material_base);
+      f3 shading_normal = vx.normal;
+      if (dot(shading_normal, vx.geometric) < 0.0f) shading_normal = -shading_normal;
+      const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
+      const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
+      const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
+      L = L + T * f3{emi.x, emi.y, emi.z};
+      st.L[0] = L.x;
+      st.L[1] = L.y;
+      st.L[2] = L.z;
+      surface.albedo = f3{alb.x, alb.y, alb.z};
+      surface.normal = shading_normal;
+      surface.view = -dir;
+      surface.roughness = alb.w;
+      surface.metallic = emi.w;
+      surface.spec[0] = spc.x;
+      surface.spec[1] = spc.y;
+      surface.spec[2] = spc.z;
+      surface.spec[3] = spc.w;
+      bias = p.ray_bias + p.ray_bias_steps * a.sv.meshes[in.mesh].quant[3] * in.scale_max;
+      front = dot(vx.geometric, surface.view) >= 0.0f ? vx.geometric : -vx.geometric;
+      // Count the records first; the same tests run again when they are written.
+      nshadow = sun_asked(p, surface) ? 1u : 0u;
+      for (uint32_t i = 0; i < a.sv.light_count; ++i) {
+        f3 ld, rad;
+        float dist;
+        if (!light_sample(a.sv.lights[i], vx.position, ld, rad, dist)) continue;
+        if (dot(surface.normal, ld) <= 0.0f) continue;
+        ++nshadow;
+      }
+    }
+  }
+  const uint32_t first = wave_append(&a.counts[1], nshadow);
+  if (valid && hit_found) {
+    const bool fits = first + nshadow <= a.shadow_capacity;
+    NeeRec nr;
+    nr.T[0] = st.T[0];
+    nr.T[1] = st.T[1];
+    nr.T[2] = st.T[2];
+    nr.first = first;
+    nr.count = nshadow;
+    a.nee[pid] = nr;
+    if (fits && nshadow) {
+      const f3 shadow_origin = vx.position + front * bias;
+      uint32_t k = first;
+      auto emit = [&](f3 ld, float tmax, f3 c) {
+        ShadowRec r;
+        r.o[0] = shadow_origin.x;
+        r.o[1] = shadow_origin.y;
+        r.o[2] = shadow_origin.z;
+        r.tmax = tmax;
+        r.d[0] = ld.x;
+        r.d[1] = ld.y;
+        r.d[2] = ld.z;
+        r.c[0] = c.x;
+        r.c[1] = c.y;
+        r.c[2] = c.z;
+        r.pad0 = r.pad1 = 0;
+        a.shadow[k++] = r;
+      };
+      if (sun_asked(p, surface)) emit(ld3(p.sun), 1e30f, brdf_direct(surface, ld3(p.sun), splat(p.sun[3])));
+      for (uint32_t i = 0; i < a.sv.light_count; ++i) {
+        const ge_resolve_light light = a.sv.lights[i];
+        f3 ld, rad;
+        float dist;
+        if (!light_sample(light, vx.position, ld, rad, dist)) continue;
+        if (dot(surface.normal, ld) <= 0.0f) continue;
+        emit(ld, dist - bias - light.spot[2], brdf_direct(surface, ld, rad));
+      }
+    }
+    // ---- the next direction (path_trace.slang:397-455) ----
+    alive = st.bounce < p.max_bounces;
+    if (alive) {
+      const float n_dot_v = dot(surface.normal, surface.view);
+      alive = n_dot_v > 0.0f;
+      if (alive) {
+        f3 tangent, bitangent;
+        onb(surface.normal, tangent, bitangent);
+        const f3 v_local = f3{dot(surface.view, tangent), dot(surface.view, bitangent), dot(surface.view, surface.normal)};
+        const float alpha = brdf_alpha(surface.roughness);
+        const f3 f = fresnel(surface, n_dot_v);
+        const float p_specular = specular_probability(surface.albedo, surface.metallic, f);
+        uint32_t rng = st.rng;
+        f3 l_local;
+        if (rng_next(rng) < p_specular) {
+          const float u0 = rng_next(rng);
+          const float u1 = rng_next(rng);
+          const f3 h = ggx_vndf(v_local, alpha, u0, u1);
+          l_local = reflect3(-v_local, h);
+        } else {
+          const float u0 = rng_next(rng);
+          const float u1 = rng_next(rng);
+          l_local = cosine_hemisphere(u0, u1);
+        }
+        alive = l_local.z > 0.0f;
+        float pdf = 0.0f;
+        if (alive) {
+          pdf = bsdf_pdf(v_local, l_local, alpha, p_specular);
+          alive = pdf > 0.0f;
+        }
+        if (alive) {
+          const f3 next = tangent * l_local.x + bitangent * l_local.y + surface.normal * l_local.z;
+          const f3 weight = brdf_eval(surface, next) * (l_local.z / pdf);
+          f3 T = f3{st.T[0], st.T[1], st.T[2]} * weight;
+          alive = !(T.x <= 0.0f && T.y <= 0.0f && T.z <= 0.0f);
+          if (alive && st.bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
+            const float survive = clampr(maxr(T.x, maxr(T.y, T.z)), 0.0f, 0.95f);
+            if (rng_next(rng) >= survive) alive = false;
+            else T = T / maxr(survive, 1e-6f);
+          }
+          if (alive) {
+            const f3 o = vx.position + (dot(next, front) >= 0.0f ? front : -front) * bias;
+            st.o[0] = o.x;
+            st.o[1] = o.y;
+            st.o[2] = o.z;
+            st.d[0] = next.x;
+            st.d[1] = next.y;
+            st.d[2] = next.z;
+            st.T[0] = T.x;
+            st.T[1] = T.y;
+            st.T[2] = T.z;
+            st.bounce += 1;
+          }
+        }
+        st.rng = rng;
+      }
+    }
+  }
+  if (valid) {
+    if (!alive) st.flags |= 2u;
+    a.out[pid] = st;
+  }
+  const uint32_t slot = wave_append(&a.counts[0], alive ? 1u : 0u);
+  if (alive) a.next_queue[slot] = pid;
+}
+
+template <bool Count>
+__global__ void __launch_bounds__(64) k_any(Bvh bvh, const ShadowRec* __restrict__ shadow, uint32_t n,
+                                            uint8_t* __restrict__ blocked, unsigned long long* counters) {
+  const uint32_t j = blockIdx.x * 64 + threadIdx.x;
+  Tally tally{0, 0, 0};
+  uint64_t issued = 0;
+  if (j < n) {
+    const ShadowRec r = shadow[j];
+    bool b = false;
+    if (r.tmax > 0.0f) {
+      issued = 1;
+      const Ray ray = make_ray(f3{r.o[0], r.o[1], r.o[2]}, f3{r.d[0], r.d[1], r.d[2]}, 0.0f);
+      Hit h;
+      b = traverse<true, Count>(bvh, ray, r.tmax, 0x01u, h, tally, lane_stack());
+    }
+    blocked[j] = b ? 1 : 0;
+  }
+  if (Count) flush(counters, 0, issued, tally);
+}
+
+// radiance + throughput * next_event(...): the unblocked lights summed in record order. A path that
+// ended this bounce (a miss, or shade's break) leaves its radiance in `result`.
+__global__ void __launch_bounds__(64) k_resolve(PathState* __restrict__ state, const uint32_t* __restrict__ queue,
+                                                uint32_t n, const NeeRec* __restrict__ nee,
+                                                const ShadowRec* __restrict__ shadow,
+                                                const uint8_t* __restrict__ blocked, float4* __restrict__ result) {
+  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t pid = queue[g];
+  const NeeRec nr = nee[pid];
+  PathState& st = state[pid];
+  f3 L = f3{st.L[0], st.L[1], st.L[2]};
+  if (nr.count != GE_RT_MISS) {
+    f3 direct = splat(0.0f);
+    for (uint32_t k = 0; k < nr.count; ++k) {
+      const uint32_t j = nr.first + k;
+      if (blocked[j]) continue;
+      direct = direct + f3{shadow[j].c[0], shadow[j].c[1], shadow[j].c[2]};
+    }
+    L = L + f3{nr.T[0], nr.T[1], nr.T[2]} * direct;
+    st.L[0] = L.x;
+    st.L[1] = L.y;
+    st.L[2] = L.z;
+  }
+  if (st.flags & 2u) result[pid] = make_float4(L.x, L.y, L.z, __uint_as_float(st.flags & 1u));
+}
+
+// accum[p] += (sum over samples in order, spp); coverage |= any primary hit (path_trace.slang:490-494).
+__global__ void __launch_bounds__(64) k_accumulate(ge_path_trace_params p, const float4* __restrict__ result,
+                                                   float4* __restrict__ accum, uint32_t* __restrict__ coverage) {
+  const uint32_t pixel = blockIdx.x * 64 + threadIdx.x;
+  const uint32_t pixels = p.width * p.height;
+  if (pixel >= pixels) return;
+  f3 sum = splat(0.0f);
+  bool covered = false;
+  for (uint32_t s = 0; s < p.spp; ++s) {
+    const float4 r = result[size_t(s) * pixels + pixel];
+    sum = sum + f3{r.x, r.y, r.z};
+    covered = covered || (__float_as_uint(r.w) & 1u);
+  }
+  float4 prev = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (p.sample_base != 0) prev = accum[pixel];
+  accum[pixel] = make_float4(prev.x + sum.x, prev.y + sum.y, prev.z + sum.z, prev.w + float(p.spp));
+  const uint32_t before = p.sample_base == 0 ? 0u : coverage[pixel];
+  coverage[pixel] = before | (covered ? 1u : 0u);
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+constexpr size_t k_lds = size_t(k_ring) * k_lanes * 4;
+
+}  // namespace
+
+hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
+                                float4* accum, uint32_t* coverage, unsigned long long* counters) {
+  Wavefront& w = ctx->wave;
+  const hipStream_t st = ctx->stream;
+  const uint32_t pixels = p.width * p.height;
+  const uint64_t npaths64 = uint64_t(pixels) * p.spp;
+  if (npaths64 == 0) return hipSuccess;
+  if (npaths64 >= (1ull << 31)) return hipErrorInvalidValue;
+  const uint32_t npaths = uint32_t(npaths64);
+  hipError_t e;
+  if ((e = w.state[0].reserve(size_t(npaths) * sizeof(PathState))) != hipSuccess) return e;
+  if ((e = w.state[1].reserve(size_t(npaths) * sizeof(PathState))) != hipSuccess) return e;
+  if ((e = w.hits.reserve(size_t(npaths) * sizeof(PathHit))) != hipSuccess) return e;
+  if ((e = w.nee.reserve(size_t(npaths) * sizeof(NeeRec))) != hipSuccess) return e;
+  if ((e = w.result.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
+  if ((e = w.queue[0].reserve(size_t(npaths) * 4)) != hipSuccess) return e;
+  if ((e = w.queue[1].reserve(size_t(npaths) * 4)) != hipSuccess) return e;
+  if ((e = w.counts.reserve(256)) != hipSuccess) return e;
+  if (!w.host_counts && (e = hipHostMalloc(reinterpret_cast<void**>(&w.host_counts), 64)) != hipSuccess) return e;
+  if (w.shadow_capacity < 2ull * npaths) {
+    w.shadow_capacity = 2ull * npaths;
+    w.shadow.release();
+    w.blocked.release();
+    if ((e = w.shadow.reserve(w.shadow_capacity * sizeof(ShadowRec))) != hipSuccess) return e;
+    if ((e = w.blocked.reserve(w.shadow_capacity)) != hipSuccess) return e;
+  }
+  uint32_t* counts = w.counts.as<uint32_t>();  // [0] the first queue; [2] next queue, [3] shadow records
+  if ((e = hipMemsetAsync(counts, 0, 16, st)) != hipSuccess) return e;
+  const uint32_t tiles_x = (p.width + 7) / 8, tiles = tiles_x * ((p.height + 7) / 8);
+  k_generate<<<tiles * p.spp, 64, 0, st>>>(p, tiles_x, tiles, w.state[0].as<PathState>(), w.queue[0].as<uint32_t>(),
+                                           counts);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  int cur = 0;
+  uint32_t n = npaths;  // every path starts live
+  uint32_t* c2 = counts + 2;
+  while (n > 0) {
+    if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;
+    if (counters)
+      k_closest<true><<<blocks(n, 64), 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
+                                                         w.hits.as<PathHit>(), counters);
+    else
+      k_closest<false><<<blocks(n, 64), 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
+                                                          w.hits.as<PathHit>(), counters);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    ShadeArgs a;
+    a.sv = sv;
+    a.p = p;
+    a.in = w.state[cur].as<PathState>();
+    a.out = w.state[cur ^ 1].as<PathState>();
+    a.queue = w.queue[0].as<uint32_t>();
+    a.n = n;
+    a.hits = w.hits.as<PathHit>();
+    a.nee = w.nee.as<NeeRec>();
+    a.next_queue = w.queue[1].as<uint32_t>();
+    a.counts = c2;
+    uint32_t nshadow = 0, next = 0;
+    for (;;) {
+      a.shadow = w.shadow.as<ShadowRec>();
+      a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
+      k_shade<<<blocks(n, 64), 64, 0, st>>>(a);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(w.host_counts, c2, 8, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      next = w.host_counts[0];
+      nshadow = w.host_counts[1];
+      if (nshadow <= a.shadow_capacity) break;
+      // The records did not fit: grow the buffer and shade this bounce again (idempotent: the
+      // launch reads one state buffer and writes the other).
+      w.shadow_capacity = uint64_t(nshadow) + nshadow / 4;
+      w.shadow.release();
+      w.blocked.release();
+      if ((e = w.shadow.reserve(w.shadow_capacity * sizeof(ShadowRec))) != hipSuccess) return e;
+      if ((e = w.blocked.reserve(w.shadow_capacity)) != hipSuccess) return e;
+      if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;
+    }
+    if (nshadow > 0) {
+      if (counters)
+        k_any<true><<<blocks(nshadow, 64), 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow,
+                                                             w.blocked.as<uint8_t>(), counters);
+      else
+        k_any<false><<<blocks(nshadow, 64), 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow,
+                                                              w.blocked.as<uint8_t>(), counters);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    k_resolve<<<blocks(n, 64), 64, 0, st>>>(w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n,
+                                           w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(), w.blocked.as<uint8_t>(),
+                                           w.result.as<float4>());
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    cur ^= 1;
+    std::swap(w.queue[0], w.queue[1]);
+    n = next;
+  }
+  k_accumulate<<<blocks(pixels, 64), 64, 0, st>>>(p, w.result.as<float4>(), accum, coverage);
+  return hipGetLastError();
+}
+
+}  // namespace ge

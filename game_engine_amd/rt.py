@@ -189,12 +189,13 @@ class Scene:
         self.ctx.check(lib().ge_rt_ray_visibility(self.ctx.h, self.h, C.byref(params), _ptr(words), flags))
         return words
 
-    def path_trace(self, params, accum=None, coverage=None, count=False, device=False):
+    def path_trace(self, params, accum=None, coverage=None, count=False, device=False, megakernel=False):
         n = params.width * params.height
         if not device:
             accum = np.zeros((n, 4), dtype=np.float32) if accum is None else accum
             coverage = np.zeros(n, dtype=np.uint32) if coverage is None else coverage
-        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        flags = ((abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0) |
+                 (abi.PT_MEGAKERNEL if megakernel else 0))
         self.ctx.check(lib().ge_rt_path_trace(self.ctx.h, self.h, C.byref(params), _ptr(accum), _ptr(coverage), flags))
         return accum, coverage
 

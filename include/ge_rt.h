@@ -44,6 +44,9 @@ enum {
 /* Trace flag: count internal nodes popped, triangle tests and instance entries into the context's
  * counters (ge_rt_stats). Off in timed runs. */
 #define GE_RT_COUNT 0x2u
+/* Path-trace flag: run the integrator as one kernel, one path per lane, instead of the default
+ * wavefront (a kernel per stage over compacted queues). Both give the same radiance to the bit. */
+#define GE_RT_PT_MEGAKERNEL 0x4u
 
 /* ---- the engine's GPU records (byte layouts mirrored, sizes asserted below) ----------------- */
 

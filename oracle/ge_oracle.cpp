@@ -482,9 +482,11 @@ inline void prep_ray(TRay& r) {
   r.kx = kx;
   r.ky = ky;
   r.kz = kz;
-  r.Sx = r.d[kx] / r.d[kz];
-  r.Sy = r.d[ky] / r.d[kz];
-  r.Sz = 1.0f / r.d[kz];
+  // The shear shares the slab test's reciprocal of the dominant component (never clamped: it is
+  // the largest of a non-zero direction), so a ray costs three divisions.
+  r.Sz = r.inv[kz];
+  r.Sx = r.d[kx] * r.Sz;
+  r.Sy = r.d[ky] * r.Sz;
 }
 
 // min/max as compares: the operands are never NaN here, and a -0/+0 choice cannot change the

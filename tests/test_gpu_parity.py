@@ -5,6 +5,8 @@ occlusion bits, visibility words, coverage and traversal counters; radiance with
 per pixel (the FP32 oracle runs the same operation order, so in practice the sums are equal to the
 bit, which the tests also report); display bytes within one code (pow is each side's own library's).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -119,10 +121,11 @@ def _radiance_close(a, b):
     return rel.max(), exact
 
 
-def test_path_trace_c1(c1):
+@pytest.mark.parametrize("megakernel", [False, True], ids=["wavefront", "megakernel"])
+def test_path_trace_c1(c1, megakernel):
     s, g, o = c1
     pt, _ = s.frame(128, 128)
-    ga, gc = g.path_trace(pt)
+    ga, gc = g.path_trace(pt, megakernel=megakernel)
     oa, oc = o.path_trace(pt)
     assert np.array_equal(gc, oc)
     err, exact = _radiance_close(ga, oa)
@@ -130,16 +133,47 @@ def test_path_trace_c1(c1):
     assert err <= RAD_TOL
 
 
-def test_path_trace_c2_lights(c2):
+@pytest.mark.parametrize("megakernel", [False, True], ids=["wavefront", "megakernel"])
+def test_path_trace_c2_lights(ctx, c2, megakernel):
     s, g, o = c2
     pt, _ = s.frame(96, 64)
     assert pt.light_count == 64
-    ga, gc = g.path_trace(pt)
+    ctx.reset_stats()
+    ob.reset_counters()
+    ga, gc = g.path_trace(pt, megakernel=megakernel, count=True)
     oa, oc = o.path_trace(pt)
     assert np.array_equal(gc, oc)
     err, exact = _radiance_close(ga, oa)
     print(f"C2 radiance: max rel err {err:.3g}, bit-exact pixels {exact:.4f}")
     assert err <= RAD_TOL
+    gs, os_ = ctx.stats(), ob.counters()
+    for k in ("closest", "any", "nodes", "triangles", "instances"):
+        assert gs[k] == os_[k], k  # the same rays, walked the same way
+
+
+def test_path_trace_many_lights_regrows_shadow_queue(ctx):
+    """Nine shadow rays a vertex overflow the wavefront's first shadow buffer (two a path): the
+    shade pass is run again into a grown buffer and the picture is unchanged."""
+    s = synth.Synth.config(1, 0)
+    lights = (abi.ResolveLight * 8)()
+    rng = np.random.default_rng(2)
+    for l in lights:
+        d = rng.normal(size=3)
+        d[1] = abs(d[1]) + 0.5
+        l.position_radius[:] = [*d, 0.0]
+        l.color_intensity[:] = [1.0, 0.9, 0.8, 0.5]
+        l.direction_type[:] = [0.0, 0.0, 0.0, 2.0]
+    desc = abi.SceneDesc.from_buffer_copy(s.desc())
+    desc.lights = C.addressof(lights)
+    desc.light_count = 8
+    g = ctx.upload(desc).build()
+    o = ob.Scene(desc)
+    pt, _ = s.frame(96, 96)
+    pt.light_count = 8
+    ga, gc = g.path_trace(pt)
+    oa, oc = o.path_trace(pt)
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(ga, oa)
 
 
 def test_path_trace_progressive_and_deterministic(ctx, c1):
@@ -147,6 +181,8 @@ def test_path_trace_progressive_and_deterministic(ctx, c1):
     pt, _ = s.frame(64, 64)
     pt.spp = 4
     one, cov1 = g.path_trace(pt)
+    mk, _ = g.path_trace(pt, megakernel=True)
+    assert one.tobytes() == mk.tobytes()  # the two integrator forms agree to the bit
     again, _ = g.path_trace(pt)
     assert one.tobytes() == again.tobytes()  # same seed, same bytes (reference_tests.cpp:424-497)
     acc = np.zeros_like(one)
