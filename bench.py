@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--megakernel", action="store_true", help="one path per lane instead of the wavefront")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device three times: the known-size kernel a rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE pass is scaled by")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the path tracer runs on one MI355X and has no multi-GPU path "
@@ -93,6 +96,13 @@ def main():
 
     from game_engine_amd import rt, synth
 
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log("[bench] synthesizing the 4M-triangle instanced scene")
     s = synth.Synth.config(2, 0, args.instances)
     pt, _ = s.frame(WIDTH, HEIGHT)

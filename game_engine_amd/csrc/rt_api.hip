@@ -145,10 +145,13 @@ int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal) {
   if (hipSetDevice(device_ordinal) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
       hipEventCreate(&ctx->bev0) != hipSuccess || hipEventCreate(&ctx->bev1) != hipSuccess ||
-      ctx->counters.reserve(64) != hipSuccess || hipMemset(ctx->counters.ptr, 0, 64) != hipSuccess) {
+      ctx->counters.reserve(128) != hipSuccess || hipMemset(ctx->counters.ptr, 0, 128) != hipSuccess) {
     ge_rt_destroy(ctx);
     return GE_RT_ERROR_DEVICE;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_ordinal) == hipSuccess && cus > 0)
+    ctx->persistent_blocks = uint32_t(cus) * 32u;
   *out = ctx;
   return GE_RT_OK;
 }
@@ -573,7 +576,8 @@ int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_r
   }
   GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   GE_HIP(ctx, launch_trace_closest(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_hits,
-                                   (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+                                   (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
+                                   ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(hits, d_hits, size_t(count) * sizeof(ge_hit), hipMemcpyDeviceToHost, ctx->stream));
@@ -599,7 +603,8 @@ int32_t ge_rt_trace_any(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* 
   }
   GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   GE_HIP(ctx, launch_trace_any(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_bits,
-                               (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+                               (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
+                               ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(occluded, d_bits, words * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -770,7 +775,7 @@ int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {
 int32_t ge_rt_reset_stats(ge_rt_ctx* ctx) {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_HIP(ctx, hipSetDevice(ctx->device));
-  GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 64, ctx->stream));
+  GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 40, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
 }

@@ -245,37 +245,51 @@ GE_DEV void to_object(const InstRec& rec, const Ray& w, Ray& o) {
 // instance (the ray goes into mesh space and a sentinel marks the way back). Closest hits are
 // ordered by (t, then the larger pair << 8 | triangle), a total order, so the answer does not depend
 // on the walk; the counters do, and match the oracle's walk.
+//
+// The walk is a state machine, one node visit a step, so a persistent kernel can give a lane whose
+// ray has finished a new ray while the rest of its wave is still walking.
 template <bool AnyHit, bool Count>
-GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask, Hit& hit, Tally& tally,
-                     uint32_t* __restrict__ stack) {
-  hit.found = false;
-  hit.t = tmax;
-  hit.key = 0;
-  hit.u = hit.v = 0.0f;
-  hit.inst = hit.prim = GE_RT_MISS;
-  if (mask == 0 || bvh.tlas_node_count == 0) return false;
-  int sp = 0;
-  const int cap = int(bvh.stack_cap);
-  uint32_t spill[k_stack - k_ring];
-  auto push = [&](uint32_t x) {
+struct Walker {
+  Ray world, ray;
+  Hit hit;
+  int sp;
+  uint32_t cur, node_base, tri_base, first_pair, inst_id;
+  bool in_blas;
+
+  // False when there is nothing to walk (mask 0 or an empty scene): the hit is already the answer.
+  GE_DEV bool start(const Bvh& bvh, const Ray& w, float tmax, uint32_t mask) {
+    hit.found = false;
+    hit.t = tmax;
+    hit.key = 0;
+    hit.u = hit.v = 0.0f;
+    hit.inst = hit.prim = GE_RT_MISS;
+    world = w;
+    ray = w;
+    sp = 0;
+    cur = 0;
+    node_base = tri_base = first_pair = inst_id = 0;
+    in_blas = false;
+    return mask != 0 && bvh.tlas_node_count != 0;
+  }
+
+  GE_DEV void push(uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, int cap, uint32_t x) {
     if (sp >= cap) return;
     uint32_t& slot = stack[k_lanes * (sp & (k_ring - 1))];
     if (sp >= k_ring) spill[sp - k_ring] = slot;
     slot = x;
     ++sp;
-  };
-  auto pop = [&]() {
+  }
+  GE_DEV uint32_t pop(uint32_t* __restrict__ stack, const uint32_t* __restrict__ spill) {
     --sp;
     uint32_t& slot = stack[k_lanes * (sp & (k_ring - 1))];
     const uint32_t x = slot;
     if (sp >= k_ring) slot = spill[sp - k_ring];
     return x;
-  };
-  Ray ray = world;
-  bool in_blas = false;
-  uint32_t node_base = 0, tri_base = 0, first_pair = 0, inst_id = 0;
-  uint32_t cur = 0;
-  for (;;) {
+  }
+
+  // One node visit; true when the walk has ended (stack empty, or an any-hit query found a hit).
+  GE_DEV bool step(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
+    const int cap = int(bvh.stack_cap);
     if (!in_blas && (cur & GE_BVH_LEAF)) {
       inst_id = bvh.tlas_slots[cur & ~GE_BVH_LEAF];
       const InstRec rec = bvh.inst[inst_id];
@@ -283,7 +297,7 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
       tri_base = rec.tri_base;
       first_pair = rec.first_pair;
       to_object(rec, world, ray);
-      push(k_sentinel);
+      push(stack, spill, cap, k_sentinel);
       in_blas = true;
       if (Count) ++tally.inst;
       cur = 0;
@@ -311,7 +325,7 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
         if (!intersect(ray, v0, v1, v2, t, u, v)) continue;
         if (!(t >= ray.tmin && t <= hit.t)) continue;
         const uint32_t key = (first_pair << 8) + prim;
-        if (AnyHit) {
+        if (AnyHit || t < hit.t || !hit.found || key > hit.key) {
           hit.found = true;
           hit.t = t;
           hit.u = u;
@@ -319,49 +333,103 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
           hit.inst = inst_id;
           hit.prim = prim;
           hit.key = key;
-          return true;
-        }
-        if (t < hit.t || !hit.found || key > hit.key) {
-          hit.found = true;
-          hit.t = t;
-          hit.u = u;
-          hit.v = v;
-          hit.inst = inst_id;
-          hit.prim = prim;
-          hit.key = key;
+          if (AnyHit) return true;
         }
       }
     }
     if (h0 && h1) {
       const bool second_first = tn1 < tn0;
-      push(second_first ? child[0] : child[1]);
+      push(stack, spill, cap, second_first ? child[0] : child[1]);
       cur = second_first ? child[1] : child[0];
-      continue;
+      return false;
     }
-    if (h0) {
-      cur = child[0];
-      continue;
+    if (h0 || h1) {
+      cur = h0 ? child[0] : child[1];
+      return false;
     }
-    if (h1) {
-      cur = child[1];
-      continue;
-    }
-    bool done = false;
     for (;;) {
-      if (sp == 0) {
-        done = true;
-        break;
-      }
-      const uint32_t e = pop();
+      if (sp == 0) return true;
+      const uint32_t e = pop(stack, spill);
       if (e == k_sentinel) {
         in_blas = false;
         ray = world;
         continue;
       }
       cur = e;
-      break;
+      return false;
     }
-    if (done) return hit.found;
+  }
+};
+
+template <bool AnyHit, bool Count>
+GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask, Hit& hit, Tally& tally,
+                     uint32_t* __restrict__ stack) {
+  Walker<AnyHit, Count> w;
+  uint32_t spill[k_stack - k_ring];
+  if (w.start(bvh, world, tmax, mask)) {
+    while (!w.step(bvh, stack, spill, tally)) {
+    }
+  }
+  hit = w.hit;
+  return w.hit.found;
+}
+
+// A persistent walk: the wave takes rays from a shared counter, and whenever k_refill or more of its
+// lanes have finished their rays it fetches new ones for them, so lanes are not left idle until the
+// wave's longest ray ends. `load(i, slot, ray, tmax, mask)` fetches ray i (false: nothing to trace,
+// the answer is a miss) and `store(slot, hit)` writes an answer. Every wave leaves the loop once
+// the counter has passed n and its lanes are idle.
+constexpr uint32_t k_refill = 16;
+
+GE_DEV uint64_t lanes_below() {
+  const uint32_t lane = threadIdx.x & 63u;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+template <bool AnyHit, bool Count, class Load, class Store>
+GE_DEV void persistent_walk(const Bvh& bvh, uint32_t n, uint32_t* head, uint32_t* __restrict__ stack, Load load,
+                            Store store, uint64_t& issued, Tally& tally) {
+  Walker<AnyHit, Count> w;
+  uint32_t spill[k_stack - k_ring];
+  bool active = false, exhausted = false;
+  uint32_t slot = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = uint32_t(__popcll(idle));
+    if (!exhausted && nidle >= k_refill) {
+      uint32_t base = 0;
+      if ((threadIdx.x & 63u) == 0) base = atomicAdd(head, nidle);
+      base = uint32_t(__shfl(int(base), 0));
+      if (uint64_t(base) + nidle >= n) exhausted = true;
+      if (!active) {
+        const uint32_t i = base + uint32_t(__popcll(idle & lanes_below()));
+        if (i < n) {
+          Ray ray;
+          float tmax;
+          uint32_t mask;
+          if (load(i, slot, ray, tmax, mask)) {
+            ++issued;
+            active = w.start(bvh, ray, tmax, mask);
+            if (!active) store(slot, w.hit);
+          } else {
+            Hit miss;
+            miss.found = false;
+            miss.t = miss.u = miss.v = 0.0f;
+            miss.inst = miss.prim = GE_RT_MISS;
+            miss.key = 0;
+            store(slot, miss);
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (active && w.step(bvh, stack, spill, tally)) {
+      store(slot, w.hit);
+      active = false;
+    }
   }
 }
 

@@ -116,10 +116,11 @@ struct ge_rt_ctx {
   ge::BuildScratch build;
   ge::DevBuf io[6];    // host-pointer staging
   ge::DevBuf tmp[9];   // build batches: metadata, records, boxes, segments, order, codes, instances
-  ge::DevBuf counters; // 5 x u64
+  ge::DevBuf counters; // 5 x u64, then the persistent walks' head counter at byte 64
   double last_kernel_ms = 0.0, last_build_ms = 0.0;
   bool timing_pending = false;
   ge::Wavefront wave;
+  uint32_t persistent_blocks = 8192;  // one-wave workgroups a persistent walk launches: CUs x 32
 };
 
 namespace ge {
@@ -165,10 +166,11 @@ struct SceneView {
   uint32_t pad;
 };
 
+// Persistent walks: `head` is a device counter the launch clears, `max_blocks` the grid's cap.
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                                ge_hit* hits, unsigned long long* counters);
+                                ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks);
 hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                            uint32_t* bits, unsigned long long* counters);
+                            uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks);
 hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
                                  unsigned long long* counters);
 hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,

@@ -8,6 +8,8 @@
 #include "rt_internal.h"
 #include "rt_shade.h"
 
+#include <algorithm>
+
 namespace ge {
 namespace {
 
@@ -34,20 +36,22 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uin
 
 template <bool Count>
 __global__ void __launch_bounds__(64) k_trace_closest(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n,
-                                                      uint32_t mask, ge_hit* __restrict__ hits,
+                                                      uint32_t mask, uint32_t* head, ge_hit* __restrict__ hits,
                                                       unsigned long long* counters) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   Tally tally{0, 0, 0};
   uint64_t issued = 0;
-  if (i < n) {
+  auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& m) {
     const ge_ray r = rays[i];
-    const Ray ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
-    Hit h;
-    issued = 1;
-    const bool found = traverse<false, Count>(bvh, ray, r.tmax, mask, h, tally, lane_stack());
+    slot = i;
+    ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
+    tmax = r.tmax;
+    m = mask;
+    return true;
+  };
+  auto store = [&](uint32_t i, const Hit& h) {
     ge_hit out;
     out.pad = 0;
-    if (found) {
+    if (h.found) {
       out.t = h.t;
       out.u = h.u;
       out.v = h.v;
@@ -56,40 +60,37 @@ __global__ void __launch_bounds__(64) k_trace_closest(Bvh bvh, const ge_ray* __r
       out.primitive = h.prim & 0xffu;
       out.key = h.key;
     } else {
-      out.t = 0.0f;
-      out.u = 0.0f;
-      out.v = 0.0f;
-      out.instance_id = GE_RT_MISS;
-      out.geometry = GE_RT_MISS;
-      out.primitive = GE_RT_MISS;
+      out.t = out.u = out.v = 0.0f;
+      out.instance_id = out.geometry = out.primitive = GE_RT_MISS;
       out.key = 0;
     }
     hits[i] = out;
-  }
+  };
+  persistent_walk<false, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
   if (Count) flush_counters(counters, issued, 0, tally);
 }
 
+// Occlusion bits by atomic OR into words the host cleared: a persistent walk finishes rays in any
+// order.
 template <bool Count>
 __global__ void __launch_bounds__(64) k_trace_any(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n, uint32_t mask,
-                                                  uint32_t* __restrict__ bits, unsigned long long* counters) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                  uint32_t* head, uint32_t* __restrict__ bits,
+                                                  unsigned long long* counters) {
   Tally tally{0, 0, 0};
   uint64_t issued = 0;
-  bool blocked = false;
-  if (i < n) {
+  auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& m) {
     const ge_ray r = rays[i];
-    if (r.tmax > 0.0f) {
-      issued = 1;
-      const Ray ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
-      Hit h;
-      blocked = traverse<true, Count>(bvh, ray, r.tmax, mask, h, tally, lane_stack());
-    }
-  }
-  const uint64_t ballot = __ballot(blocked);
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t word0 = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) >> 5;
-  if (lane == 0 && word0 * 32 < n) bits[word0] = uint32_t(ballot);
-  if (lane == 32 && (word0 + 1) * 32 < n) bits[word0 + 1] = uint32_t(ballot >> 32);
+    slot = i;
+    tmax = r.tmax;
+    m = mask;
+    if (!(r.tmax > 0.0f)) return false;
+    ray = make_ray(f3{r.origin[0], r.origin[1], r.origin[2]}, f3{r.dir[0], r.dir[1], r.dir[2]}, r.tmin);
+    return true;
+  };
+  auto store = [&](uint32_t i, const Hit& h) {
+    if (h.found) atomicOr(&bits[i >> 5], 1u << (i & 31u));
+  };
+  persistent_walk<true, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
   if (Count) flush_counters(counters, 0, issued, tally);
 }
 
@@ -429,18 +430,25 @@ inline size_t lds(const Bvh&) { return size_t(k_ring) * k_lanes * 4; }
 }  // namespace
 
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                                ge_hit* hits, unsigned long long* counters) {
+                                ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks) {
   if (n == 0) return hipSuccess;
-  if (counters) k_trace_closest<true><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, hits, counters);
-  else k_trace_closest<false><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, hits, counters);
+  hipError_t e = hipMemsetAsync(head, 0, 4, stream);
+  if (e != hipSuccess) return e;
+  const uint32_t grid = std::min(blocks(n, 64), max_blocks);
+  if (counters) k_trace_closest<true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters);
+  else k_trace_closest<false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters);
   return hipGetLastError();
 }
 
 hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                            uint32_t* bits, unsigned long long* counters) {
+                            uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks) {
   if (n == 0) return hipSuccess;
-  if (counters) k_trace_any<true><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, bits, counters);
-  else k_trace_any<false><<<blocks(n, 64), 64, lds(bvh), stream>>>(bvh, rays, n, mask, bits, counters);
+  hipError_t e = hipMemsetAsync(head, 0, 4, stream);
+  if (e == hipSuccess) e = hipMemsetAsync(bits, 0, size_t((n + 31) / 32) * 4, stream);
+  if (e != hipSuccess) return e;
+  const uint32_t grid = std::min(blocks(n, 64), max_blocks);
+  if (counters) k_trace_any<true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters);
+  else k_trace_any<false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters);
   return hipGetLastError();
 }
 

@@ -107,18 +107,20 @@ __global__ void __launch_bounds__(64) k_generate(ge_path_trace_params p, uint32_
 
 template <bool Count>
 __global__ void __launch_bounds__(64) k_closest(Bvh bvh, const PathState* __restrict__ state,
-                                                const uint32_t* __restrict__ queue, uint32_t n,
+                                                const uint32_t* __restrict__ queue, uint32_t n, uint32_t* head,
                                                 PathHit* __restrict__ hits, unsigned long long* counters) {
-  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
   Tally tally{0, 0, 0};
   uint64_t issued = 0;
-  if (g < n) {
-    const uint32_t pid = queue[g];
+  auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
+    const uint32_t pid = queue[i];
     const PathState& st = state[pid];
-    const Ray ray = make_ray(f3{st.o[0], st.o[1], st.o[2]}, f3{st.d[0], st.d[1], st.d[2]}, 0.0f);
-    Hit h;
-    issued = 1;
-    traverse<false, Count>(bvh, ray, 1e30f, st.bounce == 0 ? 0xfeu : 0x01u, h, tally, lane_stack());
+    slot = pid;
+    ray = make_ray(f3{st.o[0], st.o[1], st.o[2]}, f3{st.d[0], st.d[1], st.d[2]}, 0.0f);
+    tmax = 1e30f;
+    mask = st.bounce == 0 ? 0xfeu : 0x01u;
+    return true;
+  };
+  auto store = [&](uint32_t pid, const Hit& h) {
     PathHit out;
     out.t = h.t;
     out.u = h.u;
@@ -126,7 +128,8 @@ __global__ void __launch_bounds__(64) k_closest(Bvh bvh, const PathState* __rest
     out.inst = h.found ? h.inst : GE_RT_MISS;
     out.prim = h.prim;
     hits[pid] = out;
-  }
+  };
+  persistent_walk<false, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
   if (Count) flush(counters, issued, 0, tally);
 }
 
@@ -386,22 +389,21 @@ __global__ void __launch_bounds__(64) k_shade(ShadeArgs a) {
 }
 
 template <bool Count>
-__global__ void __launch_bounds__(64) k_any(Bvh bvh, const ShadowRec* __restrict__ shadow, uint32_t n,
+__global__ void __launch_bounds__(64) k_any(Bvh bvh, const ShadowRec* __restrict__ shadow, uint32_t n, uint32_t* head,
                                             uint8_t* __restrict__ blocked, unsigned long long* counters) {
-  const uint32_t j = blockIdx.x * 64 + threadIdx.x;
   Tally tally{0, 0, 0};
   uint64_t issued = 0;
-  if (j < n) {
-    const ShadowRec r = shadow[j];
-    bool b = false;
-    if (r.tmax > 0.0f) {
-      issued = 1;
-      const Ray ray = make_ray(f3{r.o[0], r.o[1], r.o[2]}, f3{r.d[0], r.d[1], r.d[2]}, 0.0f);
-      Hit h;
-      b = traverse<true, Count>(bvh, ray, r.tmax, 0x01u, h, tally, lane_stack());
-    }
-    blocked[j] = b ? 1 : 0;
-  }
+  auto load = [&](uint32_t j, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
+    const ShadowRec& r = shadow[j];
+    slot = j;
+    tmax = r.tmax;
+    mask = 0x01u;
+    if (!(r.tmax > 0.0f)) return false;  // the bias ate the distance: never occluded (occluded(), :181)
+    ray = make_ray(f3{r.o[0], r.o[1], r.o[2]}, f3{r.d[0], r.d[1], r.d[2]}, 0.0f);
+    return true;
+  };
+  auto store = [&](uint32_t j, const Hit& h) { blocked[j] = h.found ? 1 : 0; };
+  persistent_walk<true, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
   if (Count) flush(counters, 0, issued, tally);
 }
 
@@ -483,8 +485,8 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     if ((e = w.shadow.reserve(w.shadow_capacity * sizeof(ShadowRec))) != hipSuccess) return e;
     if ((e = w.blocked.reserve(w.shadow_capacity)) != hipSuccess) return e;
   }
-  uint32_t* counts = w.counts.as<uint32_t>();  // [0] the first queue; [2] next queue, [3] shadow records
-  if ((e = hipMemsetAsync(counts, 0, 16, st)) != hipSuccess) return e;
+  uint32_t* counts = w.counts.as<uint32_t>();  // [0] first queue; [2] next queue, [3] shadow records, [4] [5] walk heads
+  if ((e = hipMemsetAsync(counts, 0, 32, st)) != hipSuccess) return e;
   const uint32_t tiles_x = (p.width + 7) / 8, tiles = tiles_x * ((p.height + 7) / 8);
   k_generate<<<tiles * p.spp, 64, 0, st>>>(p, tiles_x, tiles, w.state[0].as<PathState>(), w.queue[0].as<uint32_t>(),
                                            counts);
@@ -493,13 +495,14 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   uint32_t n = npaths;  // every path starts live
   uint32_t* c2 = counts + 2;
   while (n > 0) {
-    if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(c2, 0, 16, st)) != hipSuccess) return e;  // queue and shadow counts, walk heads
+    const uint32_t grid = std::min(blocks(n, 64), ctx->persistent_blocks);
     if (counters)
-      k_closest<true><<<blocks(n, 64), 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
-                                                         w.hits.as<PathHit>(), counters);
+      k_closest<true><<<grid, 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
+                                              counts + 4, w.hits.as<PathHit>(), counters);
     else
-      k_closest<false><<<blocks(n, 64), 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
-                                                          w.hits.as<PathHit>(), counters);
+      k_closest<false><<<grid, 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
+                                               counts + 4, w.hits.as<PathHit>(), counters);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     ShadeArgs a;
     a.sv = sv;
@@ -533,12 +536,13 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;
     }
     if (nshadow > 0) {
+      const uint32_t grid = std::min(blocks(nshadow, 64), ctx->persistent_blocks);
       if (counters)
-        k_any<true><<<blocks(nshadow, 64), 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow,
-                                                             w.blocked.as<uint8_t>(), counters);
+        k_any<true><<<grid, 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow, counts + 5,
+                                            w.blocked.as<uint8_t>(), counters);
       else
-        k_any<false><<<blocks(nshadow, 64), 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow,
-                                                              w.blocked.as<uint8_t>(), counters);
+        k_any<false><<<grid, 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow, counts + 5,
+                                             w.blocked.as<uint8_t>(), counters);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     k_resolve<<<blocks(n, 64), 64, 0, st>>>(w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n,
