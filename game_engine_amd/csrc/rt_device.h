@@ -123,6 +123,7 @@ constexpr int k_ring = 16;
 struct Ray {
   float o[3], d[3], inv[3];
   float tmin;
+  float teps;  // absolute slack on the box test, in t: 2^-19 of the origin's largest coordinate
   int kx, ky, kz;
   float Sx, Sy, Sz;
 };
@@ -153,6 +154,8 @@ GE_DEV void prep(Ray& r) {
   r.Sz = sel3(r.inv, kz);
   r.Sx = sel3(r.d, kx) * r.Sz;
   r.Sy = sel3(r.d, ky) * r.Sz;
+  r.teps = maxr(maxr(__builtin_fabsf(r.o[0]), __builtin_fabsf(r.o[1])), __builtin_fabsf(r.o[2])) * 1.9073486328125e-6f *
+           __builtin_fabsf(r.Sz);
 }
 
 GE_DEV Ray make_ray(f3 o, f3 d, float tmin) {
@@ -168,18 +171,18 @@ GE_DEV Ray make_ray(f3 o, f3 d, float tmin) {
   return r;
 }
 
-// The slab test with the far distance widened by 1 + 2^-20, so a box is never missed by a ray
-// that hits a triangle inside it (Ize 2013, "Robust BVH Ray Traversal").
+// The box test: the slab interval, its far end widened by 1 + 2^-20 (Ize 2013, "Robust BVH Ray
+// Traversal"), against [tmin, tmax], both widened by the ray's absolute slack `teps`, so that a
+// triangle a few coordinate ulps short of its box's computed entry (a hit very near an origin far
+// from the world's) is never culled with its box. min/max as compares: no operand is NaN.
 GE_DEV bool slab(const float lo[3], const float hi[3], const Ray& r, float tmax, float& tnear) {
   const float tx0 = (lo[0] - r.o[0]) * r.inv[0], tx1 = (hi[0] - r.o[0]) * r.inv[0];
   const float ty0 = (lo[1] - r.o[1]) * r.inv[1], ty1 = (hi[1] - r.o[1]) * r.inv[1];
   const float tz0 = (lo[2] - r.o[2]) * r.inv[2], tz1 = (hi[2] - r.o[2]) * r.inv[2];
-  const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
-                                   __builtin_fmaxf(__builtin_fminf(tz0, tz1), r.tmin));
-  const float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)),
-                                   __builtin_fminf(__builtin_fmaxf(tz0, tz1), tmax));
+  const float tn = maxr(maxr(minr(tx0, tx1), minr(ty0, ty1)), maxr(minr(tz0, tz1), r.tmin));
+  const float tf = minr(minr(minr(maxr(tx0, tx1), maxr(ty0, ty1)), maxr(tz0, tz1)) * k_box_robust, tmax);
   tnear = tn;
-  return tn <= tf * k_box_robust;
+  return tn <= tf + r.teps;
 }
 
 // Watertight ray/triangle intersection (Woop, Benthin, Wald 2013), with the double-precision

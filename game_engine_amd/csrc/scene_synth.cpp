@@ -191,12 +191,29 @@ uint32_t add_mesh(ge_synth& s, const Source& src, uint32_t cluster_size, const s
   const uint32_t vbase = uint32_t(s.vertices.size() / 3);
   const uint32_t ntri = uint32_t(src.idx.size() / 3);
   std::vector<F3> local_pos;
-  for (uint32_t t0 = 0; t0 < ntri; t0 += cluster_size) {
-    const uint32_t t1 = std::min(ntri, t0 + cluster_size);
+  for (uint32_t t0 = 0; t0 < ntri;) {
+    // A cluster takes triangles in order until it holds `cluster_size` of them or one more would
+    // take it past 255 vertices (local indices are bytes; cluster.h:350-358).
+    uint32_t t1 = t0;
+    {
+      std::vector<uint32_t> probe;
+      while (t1 < ntri && t1 - t0 < cluster_size) {
+        uint32_t fresh = 0;
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t sv = src.idx[size_t(t1) * 3 + k];
+          if (std::find(probe.begin(), probe.end(), sv) == probe.end()) ++fresh;
+        }
+        if (probe.size() + fresh > 255) break;
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t sv = src.idx[size_t(t1) * 3 + k];
+          if (std::find(probe.begin(), probe.end(), sv) == probe.end()) probe.push_back(sv);
+        }
+        ++t1;
+      }
+    }
     ge_cluster_desc cl{};
     cl.vertex_offset = uint32_t(s.vertices.size() / 3);
     cl.triangle_offset = uint32_t(s.triangles.size());
-    std::vector<uint32_t> remap;  // source vertex -> local
     std::vector<uint32_t> locals;
     for (uint32_t t = t0; t < t1; ++t) {
       uint32_t packed = 0;
@@ -236,6 +253,7 @@ uint32_t add_mesh(ge_synth& s, const Source& src, uint32_t cluster_size, const s
     cl.cone = 0x7f000000u;  // k_cone_none: never culled
     s.clusters.push_back(cl);
     s.cluster_materials.push_back(tri_material.empty() ? 0u : tri_material[t0]);
+    t0 = t1;
   }
   mesh.cluster_count = uint32_t(s.clusters.size()) - mesh.first_cluster;
   // The mesh's grid (cluster.cpp:437-466).
@@ -715,9 +733,9 @@ ge_synth* ge_synth_from_triangles(const float* positions, uint32_t triangle_coun
       src.idx.push_back(uint32_t(src.idx.size()));
     }
   }
-  std::vector<uint32_t> mats(triangle_count);
-  for (uint32_t t = 0; t < triangle_count; ++t) mats[t] = t / cluster_size;
+  std::vector<uint32_t> mats(triangle_count, 0u);
   add_mesh(*s, src, cluster_size, mats);
+  for (uint32_t c = 0; c < s->clusters.size(); ++c) s->cluster_materials[c] = c;  // a material per cluster
   s->has_attributes = attributes != 0;
   if (!s->has_attributes) s->attributes.clear();
   Rng rng(seed);

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <type_traits>
@@ -462,9 +463,15 @@ void build_blas(ge_oracle_scene& s, uint32_t m) {
 // under edge-sharing: a ray through a shared edge hits exactly one side).
 constexpr float k_box_robust = 1.00000095367431640625f;  // 1 + 2^-20 on the far distance (Ize 2013)
 
+// min/max as compares: the operands are never NaN here, and a -0/+0 choice cannot change the
+// comparisons the walk makes with these values (the box fit uses the IEEE fmin/fmax).
+inline float mn(float a, float b) { return b < a ? b : a; }
+inline float mx(float a, float b) { return a < b ? b : a; }
+
 struct TRay {
   float o[3], d[3], inv[3];
   float tmin;
+  float teps;  // absolute slack on the box test, in t: 2^-19 of the origin's largest coordinate
   int kx, ky, kz;
   float Sx, Sy, Sz;
 };
@@ -487,20 +494,21 @@ inline void prep_ray(TRay& r) {
   r.Sz = r.inv[kz];
   r.Sx = r.d[kx] * r.Sz;
   r.Sy = r.d[ky] * r.Sz;
+  r.teps = mx(mx(std::fabs(r.o[0]), std::fabs(r.o[1])), std::fabs(r.o[2])) * 1.9073486328125e-6f * std::fabs(r.Sz);
 }
 
-// min/max as compares: the operands are never NaN here, and a -0/+0 choice cannot change the
-// comparisons the walk makes with these values (the box fit uses the IEEE fmin/fmax).
-inline float mn(float a, float b) { return b < a ? b : a; }
-inline float mx(float a, float b) { return a < b ? b : a; }
+// The box test: the slab interval, its far end widened by 1 + 2^-20 (Ize 2013, "Robust BVH Ray
+// Traversal"), against [tmin, tmax], both widened by the ray's absolute slack `teps`, so that a
+// triangle a few coordinate ulps short of its box's computed entry (a hit very near an origin far
+// from the world's) is never culled with its box.
 inline bool slab(const float lo[3], const float hi[3], const TRay& r, float tmax, float& tnear) {
   const float tx0 = (lo[0] - r.o[0]) * r.inv[0], tx1 = (hi[0] - r.o[0]) * r.inv[0];
   const float ty0 = (lo[1] - r.o[1]) * r.inv[1], ty1 = (hi[1] - r.o[1]) * r.inv[1];
   const float tz0 = (lo[2] - r.o[2]) * r.inv[2], tz1 = (hi[2] - r.o[2]) * r.inv[2];
   const float tn = mx(mx(mn(tx0, tx1), mn(ty0, ty1)), mx(mn(tz0, tz1), r.tmin));
-  const float tf = mn(mn(mx(tx0, tx1), mx(ty0, ty1)), mn(mx(tz0, tz1), tmax));
+  const float tf = mn(mn(mn(mx(tx0, tx1), mx(ty0, ty1)), mx(tz0, tz1)) * k_box_robust, tmax);
   tnear = tn;
-  return tn <= tf * k_box_robust;
+  return tn <= tf + r.teps;
 }
 
 // Woop, Benthin, Wald 2013, "Watertight Ray/Triangle Intersection", with the double-precision
@@ -573,6 +581,7 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
       rec = &s.inst[inst_id];
       blas = &s.blas[rec->mesh];
       to_object(*rec, world, ray);
+      if (sp >= 190) std::abort();  // deeper than any structure the build accepts
       stack[sp++] = k_sentinel;
       in_blas = true;
       ++tally.inst;
@@ -617,6 +626,7 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
     }
     if (h0 && h1) {
       const bool second_first = tn1 < tn0;
+      if (sp >= 190) std::abort();
       stack[sp++] = second_first ? node.child[0] : node.child[1];
       cur = second_first ? node.child[1] : node.child[0];
       continue;
@@ -1134,6 +1144,24 @@ extern "C" {
 
 ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* d, const ge_frame_eye* eye) {
   if (!d) return nullptr;
+  // The same validation as the backend's upload: offsets inside their arrays, local indices inside
+  // their cluster, meshes and instances inside theirs.
+  for (uint32_t c = 0; c < d->cluster_count; ++c) {
+    const ge_cluster_desc& cl = d->clusters[c];
+    if (uint64_t(cl.vertex_offset) + cl.vertex_count > d->vertex_count ||
+        uint64_t(cl.triangle_offset) + cl.triangle_count > d->triangle_count || cl.triangle_count > 256 ||
+        cl.vertex_count > 256)
+      return nullptr;
+    for (uint32_t t = 0; t < cl.triangle_count; ++t) {
+      const uint32_t p = d->triangles[cl.triangle_offset + t];
+      if ((p & 0xffu) >= cl.vertex_count || ((p >> 8) & 0xffu) >= cl.vertex_count || ((p >> 16) & 0xffu) >= cl.vertex_count)
+        return nullptr;
+    }
+  }
+  for (uint32_t m = 0; m < d->mesh_count; ++m)
+    if (uint64_t(d->meshes[m].first_cluster) + d->meshes[m].cluster_count > d->cluster_count) return nullptr;
+  for (uint32_t i = 0; i < d->instance_count; ++i)
+    if (d->instances[i].mesh >= d->mesh_count) return nullptr;
   auto* s = new ge_oracle_scene();
   s->clusters.assign(d->clusters, d->clusters + d->cluster_count);
   s->vertices.assign(d->vertices, d->vertices + size_t(d->vertex_count) * 3);

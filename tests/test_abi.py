@@ -1,0 +1,50 @@
+"""The C ABI without a GPU: the library loads, exports every entry point include/ge_rt.h declares,
+its records have the engine's sizes, and calls that need no device fail cleanly."""
+import ctypes as C
+import os
+import re
+
+from game_engine_amd import abi, rt
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "ge_rt.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int32_t|void)\s+(ge_rt_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_entry_point():
+    names = declared()
+    assert len(names) >= 18
+    lib = rt.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(rt.EXPORTS) == names
+
+
+def test_record_sizes_match_the_engine():
+    assert C.sizeof(abi.PathTraceParams) == 368 and C.sizeof(abi.RayVisibilityParams) == 256
+    assert C.sizeof(abi.ClusterDesc) == 48 and C.sizeof(abi.InstanceDesc) == 96 and C.sizeof(abi.MeshDesc) == 64
+    assert C.sizeof(abi.ResolveMaterial) == 128 and C.sizeof(abi.ResolveLight) == 64 and C.sizeof(abi.FrameEye) == 48
+    text = open(HEADER).read()
+    for struct, size in re.findall(r"static_assert\(sizeof\((\w+)\) == (\d+)", text):
+        mirror = {"ge_cluster_desc": abi.ClusterDesc, "ge_mesh_desc": abi.MeshDesc, "ge_instance_desc": abi.InstanceDesc,
+                  "ge_frame_eye": abi.FrameEye, "ge_resolve_material": abi.ResolveMaterial,
+                  "ge_resolve_light": abi.ResolveLight, "ge_path_trace_params": abi.PathTraceParams,
+                  "ge_ray_visibility_params": abi.RayVisibilityParams, "ge_vertex_attributes": abi.VertexAttributes}
+        if struct in mirror:
+            assert C.sizeof(mirror[struct]) == int(size), struct
+        dtypes = {"ge_ray": abi.RAY_DTYPE, "ge_hit": abi.HIT_DTYPE, "ge_bvh_node": abi.NODE_DTYPE,
+                  "ge_bvh_triangle": abi.TRI_DTYPE, "ge_shadow_receiver": abi.RECEIVER_DTYPE}
+        if struct in dtypes:
+            assert dtypes[struct].itemsize == int(size), struct
+
+
+def test_null_handles_fail_cleanly():
+    lib = rt.lib()
+    lib.ge_rt_destroy(None)  # idempotent on null (ReferenceRenderer::destroy)
+    assert lib.ge_rt_last_error(None, None, 0) == 1
+    assert lib.ge_rt_synchronize(None) == 1
+    assert lib.ge_rt_create(None, 0) == 1
+    assert lib.ge_rt_stats(None, None) == 1
