@@ -11,9 +11,13 @@ Rays = closest-hit queries + any-hit queries issued, counted by the kernel's own
 separate counted replay of the same frames (counters are compiled out of the timed kernel).
 value = rays / wall seconds of the K timed frames. The BVH build is timed separately.
 
-roofline: the path-trace kernel's algorithmic bytes per launch, 48 B a ray (32 in, 16 out) + 64 B per
-node popped + 48 B per triangle tested + 64 B per instance entered, over its mean launch time
-(HIP events on the library's stream), against MI355X's 8 TB/s.
+roofline: the dominant kernel of the frame (the traversal walk with the larger share of device time:
+k_closest, the closest-hit walk of one bounce, or k_any, the any-hit shadow walk of one bounce). Its
+algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 64 B per node popped + 48 B per
+triangle tested + 64 B per instance entered, from the counted replay's per-kind counters divided by
+its launches per frame; its mean launch time is the library's own HIP events around each launch on
+its stream, over the timed frames. Peak: MI355X's 8 TB/s. `traffic` is that kernel's measured HBM
+bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, when one is there.
 
 cpu_baseline: the oracle (oracle/, a scalar C++ port of the same tracer over the same LBVH) on a
 band of rows of the same frame, all host threads, same seeds.
@@ -36,40 +40,40 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if one is there."""
-    path = os.path.join(ROOT, "profiles", "path_trace_c2_pmc.txt")
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if one is there."""
+    path = os.path.join(ROOT, "profiles", "path_trace_c2_pmc.md")
     if not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch {kernel}:"
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return float(line.split(":", 1)[1].strip())
     return None
 
 
 def cpu_baseline(s, pt, target_seconds, sample_base):
+    """The oracle on whole frames of the same workload (same scene, seeds and sample numbering) until
+    `target_seconds` of CPU time have passed: a bounded sample, at least one frame."""
     from oracle import binding as ob
     sc = ob.Scene(s.desc())
     threads = ob.lib().ge_oracle_max_threads()
     p = type(pt).from_buffer_copy(pt)
-    p.sample_base = sample_base
-    mid = HEIGHT // 2
-    # Calibrate on 8 rows, then take a band sized to the target time.
     ob.reset_counters()
+    frames = 0
     t = time.perf_counter()
-    sc.path_trace(p, rows=(mid - 4, mid + 4))
-    dt = time.perf_counter() - t
-    rows = max(8, min(HEIGHT, int(8 * target_seconds / max(dt, 1e-3))))
-    r0 = max(0, mid - rows // 2)
-    r1 = min(HEIGHT, r0 + rows)
-    ob.reset_counters()
-    t = time.perf_counter()
-    sc.path_trace(p, rows=(r0, r1))
-    dt = time.perf_counter() - t
+    while True:
+        p.sample_base = sample_base + frames * p.spp
+        sc.path_trace(p)
+        frames += 1
+        dt = time.perf_counter() - t
+        log(f"[bench] cpu frame {frames}: {dt:.1f} s")
+        if dt >= target_seconds or frames >= 64:
+            break
     c = ob.counters()
     rays = c["closest"] + c["any"]
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"rows {r0}..{r1} of the {WIDTH}x{HEIGHT} frame, {p.spp} spp, sample_base {sample_base}: "
+            "sample": f"{frames} whole {WIDTH}x{HEIGHT} frames at {p.spp} spp from sample_base {sample_base}: "
                       f"{rays} rays in {dt:.2f} s on {threads} threads (oracle/ scalar C++ port, OpenMP 8x8 tiles)"}
 
 
@@ -127,14 +131,17 @@ def main():
         frame(w)
         log(f"[bench] warmup {w} {ctx.stats()['kernel_ms']:.2f} ms")
     torch.cuda.synchronize()
-    kernel_ms = []
+    per_step = []
     t0 = time.perf_counter()
     for k in range(args.steps):
         frame(args.warmup + k)
-        kernel_ms.append(ctx.stats()["kernel_ms"])
+        per_step.append(ctx.stats())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    log(f"[bench] timed {args.steps} frames: {wall * 1e3:.1f} ms wall, kernel ms {['%.2f' % x for x in kernel_ms]}")
+    kernel_ms = [x["kernel_ms"] for x in per_step]
+    log(f"[bench] timed {args.steps} frames: {wall * 1e3:.1f} ms wall, frame ms {['%.2f' % x for x in kernel_ms]}, "
+        f"closest-hit walks ms {['%.2f' % x['closest_ms'] for x in per_step]}, "
+        f"any-hit walks ms {['%.2f' % x['any_ms'] for x in per_step]}")
 
     ctx.reset_stats()
     for k in range(args.steps):
@@ -143,9 +150,27 @@ def main():
     rays = st["closest"] + st["any"]
     log(f"[bench] counted: {st}")
     value = rays / wall / 1e6
-    per_launch = (48 * rays + 64 * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"]) / args.steps
     mean_ms = sum(kernel_ms) / len(kernel_ms)
-    achieved = per_launch / (mean_ms * 1e-3) / 1e9
+    # Algorithmic bytes of each walk kind over the timed frames (counts of the counted replay of the
+    # same frames), and their HIP-event times.
+    kinds = {
+        "k_closest": (48 * st["closest"] + 64 * (st["nodes"] - st["any_nodes"]) +
+                      48 * (st["triangles"] - st["any_triangles"]) + 64 * (st["instances"] - st["any_instances"]),
+                      sum(x["closest_ms"] for x in per_step), sum(x["closest_launches"] for x in per_step)),
+        "k_any": (48 * st["any"] + 64 * st["any_nodes"] + 48 * st["any_triangles"] + 64 * st["any_instances"],
+                  sum(x["any_ms"] for x in per_step), sum(x["any_launches"] for x in per_step)),
+    }
+    if args.megakernel:
+        kinds = {"k_path_trace": (48 * rays + 64 * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"],
+                                  sum(kernel_ms), args.steps)}
+    for name, (b, ms, launches) in kinds.items():
+        log(f"[bench] {name}: {b / args.steps / 1e6:.1f} MB a frame over {launches / args.steps:.1f} launches, "
+            f"{ms / args.steps:.3f} ms a frame, {b / (ms * 1e-3) / 1e9 if ms else 0:.1f} GB/s")
+    dominant = max(kinds, key=lambda k: kinds[k][1])
+    total_bytes, total_ms, launches = kinds[dominant]
+    per_launch = total_bytes / launches
+    launch_ms = total_ms / launches
+    achieved = per_launch / (launch_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
@@ -158,10 +183,12 @@ def main():
         "rays_per_step": rays // args.steps,
         "closest_per_step": st["closest"] // args.steps, "any_per_step": st["any"] // args.steps,
         "bvh_build_ms": round(min(builds), 3),
+        "frame_kernel_ms": round(mean_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": read_traffic(),
-                     "kernel": "k_path_trace", "kernel_ms": round(mean_ms, 3),
-                     "bytes_per_launch": int(per_launch)},
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": read_traffic(dominant),
+                     "kernel": dominant, "launch_ms": round(launch_ms, 4), "launches_per_step": launches / args.steps,
+                     "bytes_per_launch": int(per_launch),
+                     "walk_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in kinds.items()}},
     }
     if not args.no_cpu:
         log("[bench] cpu baseline")

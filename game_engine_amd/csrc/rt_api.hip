@@ -149,6 +149,12 @@ int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal) {
     ge_rt_destroy(ctx);
     return GE_RT_ERROR_DEVICE;
   }
+  for (auto& kind : ctx->stage_ev)
+    for (hipEvent_t& ev : kind)
+      if (hipEventCreate(&ev) != hipSuccess) {
+        ge_rt_destroy(ctx);
+        return GE_RT_ERROR_DEVICE;
+      }
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_ordinal) == hipSuccess && cus > 0)
     ctx->persistent_blocks = uint32_t(cus) * 32u;
@@ -174,6 +180,9 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->bev0) (void)hipEventDestroy(ctx->bev0);
   if (ctx->bev1) (void)hipEventDestroy(ctx->bev1);
+  for (auto& kind : ctx->stage_ev)
+    for (hipEvent_t ev : kind)
+      if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -542,6 +551,14 @@ int32_t stage(ge_rt_ctx* ctx, int slot, const T* host, size_t count, bool copy_i
   return GE_RT_OK;
 }
 
+// Opens a dispatch's timing: its begin event, and no stage launches timed yet.
+hipError_t begin_timing(ge_rt_ctx* ctx) {
+  ctx->stage_n[0] = ctx->stage_n[1] = 0;
+  ctx->stage_ms[0] = ctx->stage_ms[1] = 0.0;
+  ctx->stage_launches[0] = ctx->stage_launches[1] = 0;
+  return hipEventRecord(ctx->ev0, ctx->stream);
+}
+
 int32_t finish(ge_rt_ctx* ctx, bool sync) {
   GE_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   ctx->timing_pending = true;
@@ -574,7 +591,7 @@ int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_r
     d_rays = dr;
     if ((r = stage(ctx, 1, hits, count, false, &d_hits)) != GE_RT_OK) return r;
   }
-  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, begin_timing(ctx));
   GE_HIP(ctx, launch_trace_closest(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_hits,
                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
@@ -601,7 +618,7 @@ int32_t ge_rt_trace_any(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* 
     d_rays = dr;
     if ((r = stage(ctx, 1, occluded, words, false, &d_bits)) != GE_RT_OK) return r;
   }
-  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, begin_timing(ctx));
   GE_HIP(ctx, launch_trace_any(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_bits,
                                (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
                                ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
@@ -626,7 +643,7 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   GE_CHECK(ctx, !dev || out, GE_RT_ERROR_INVALID_ARGUMENT, "null output");
   GE_CHECK(ctx, dev || words, GE_RT_ERROR_INVALID_ARGUMENT, "null output");
   if (!dev && (r = stage(ctx, 0, words, n, false, &out)) != GE_RT_OK) return r;
-  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, begin_timing(ctx));
   GE_HIP(ctx, launch_ray_visibility(ctx->stream, bvh_view(scene->dev), *params, out,
                                     (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
@@ -679,7 +696,7 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
   p.coverage = reinterpret_cast<uint64_t>(d_cov);
   SceneView svl = sv;
   svl.light_count = params->light_count;
-  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, begin_timing(ctx));
   unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
   if (flags & GE_RT_PT_MEGAKERNEL) {
     GE_HIP(ctx, launch_path_trace(ctx->stream, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
@@ -738,7 +755,7 @@ int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_sha
     dr = rr;
     if ((r = stage(ctx, 1, blocked, words, false, &db)) != GE_RT_OK) return r;
   }
-  GE_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  GE_HIP(ctx, begin_timing(ctx));
   GE_HIP(ctx, launch_shadow_pass(ctx->stream, bvh_view(scene->dev), scene_view(scene->dev), *params, dr, db,
                                  (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
@@ -758,24 +775,40 @@ int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {
     float ms = 0.0f;
     GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_kernel_ms = ms;
+    for (int k = 0; k < 2; ++k) {
+      double sum = 0.0;
+      for (uint32_t i = 0; i < ctx->stage_n[k]; ++i) {
+        GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->stage_ev[k][2 * i], ctx->stage_ev[k][2 * i + 1]));
+        sum += ms;
+      }
+      ctx->stage_ms[k] = sum;
+      ctx->stage_launches[k] = ctx->stage_n[k];
+    }
     ctx->timing_pending = false;
   }
-  unsigned long long c[5];
+  unsigned long long c[8];
   GE_HIP(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
   out->closest_rays = c[0];
   out->any_rays = c[1];
-  out->nodes = c[2];
-  out->triangles = c[3];
-  out->instances = c[4];
+  out->nodes = c[2] + c[5];
+  out->triangles = c[3] + c[6];
+  out->instances = c[4] + c[7];
+  out->any_nodes = c[5];
+  out->any_triangles = c[6];
+  out->any_instances = c[7];
   out->last_kernel_ms = ctx->last_kernel_ms;
   out->last_build_ms = ctx->last_build_ms;
+  out->closest_ms = ctx->stage_ms[0];
+  out->any_ms = ctx->stage_ms[1];
+  out->closest_launches = ctx->stage_launches[0];
+  out->any_launches = ctx->stage_launches[1];
   return GE_RT_OK;
 }
 
 int32_t ge_rt_reset_stats(ge_rt_ctx* ctx) {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_HIP(ctx, hipSetDevice(ctx->device));
-  GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 40, ctx->stream));
+  GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 64, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
 }

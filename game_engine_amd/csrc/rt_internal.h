@@ -116,9 +116,16 @@ struct ge_rt_ctx {
   ge::BuildScratch build;
   ge::DevBuf io[6];    // host-pointer staging
   ge::DevBuf tmp[9];   // build batches: metadata, records, boxes, segments, order, codes, instances
-  ge::DevBuf counters; // 5 x u64, then the persistent walks' head counter at byte 64
+  // 8 x u64 (closest rays, any rays, closest-hit nodes, triangles, instances, any-hit nodes,
+  // triangles, instances), then the persistent walks' head counter at byte 64
+  ge::DevBuf counters;
   double last_kernel_ms = 0.0, last_build_ms = 0.0;
   bool timing_pending = false;
+  // Begin/end event pairs around the wavefront's traversal launches: [0] closest-hit, [1] any-hit.
+  hipEvent_t stage_ev[2][2 * GE_RT_STAGE_TIMED] = {};
+  uint32_t stage_n[2] = {0, 0};
+  double stage_ms[2] = {0.0, 0.0};
+  uint32_t stage_launches[2] = {0, 0};
   ge::Wavefront wave;
   uint32_t persistent_blocks = 8192;  // one-wave workgroups a persistent walk launches: CUs x 32
 };

@@ -17,28 +17,11 @@ namespace {
 extern __shared__ uint32_t g_lds[];
 __device__ __forceinline__ uint32_t* lane_stack() { return g_lds + threadIdx.x; }
 
-__device__ __forceinline__ void flush_counters(unsigned long long* counters, uint64_t closest, uint64_t any,
-                                               const Tally& t) {
-  // Reduce across the wave, then one lane adds.
-  uint64_t v[5] = {closest, any, t.nodes, t.tris, t.inst};
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    uint64_t x = v[k];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    v[k] = x;
-  }
-  if ((threadIdx.x & 63u) == 0) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-      if (v[k]) atomicAdd(&counters[k], (unsigned long long)v[k]);
-  }
-}
-
 template <bool Count>
-__global__ void __launch_bounds__(64) k_trace_closest(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n,
+__global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_closest(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n,
                                                       uint32_t mask, uint32_t* head, ge_hit* __restrict__ hits,
                                                       unsigned long long* counters) {
-  Tally tally{0, 0, 0};
+  Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& m) {
     const ge_ray r = rays[i];
@@ -73,10 +56,10 @@ __global__ void __launch_bounds__(64) k_trace_closest(Bvh bvh, const ge_ray* __r
 // Occlusion bits by atomic OR into words the host cleared: a persistent walk finishes rays in any
 // order.
 template <bool Count>
-__global__ void __launch_bounds__(64) k_trace_any(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n, uint32_t mask,
+__global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_any(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n, uint32_t mask,
                                                   uint32_t* head, uint32_t* __restrict__ bits,
                                                   unsigned long long* counters) {
-  Tally tally{0, 0, 0};
+  Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& m) {
     const ge_ray r = rays[i];
@@ -102,7 +85,7 @@ __global__ void __launch_bounds__(64) k_ray_visibility(Bvh bvh, ge_ray_visibilit
                                                        unsigned long long* counters) {
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
-  Tally tally{0, 0, 0};
+  Tally tally;
   uint64_t issued = 0;
   if (x < p.width && y < p.height) {
     const float ndx = (float(x) + 0.5f) / float(p.width) * 2.0f - 1.0f;
@@ -319,7 +302,7 @@ __global__ void __launch_bounds__(64) k_path_trace(Pt c, float4* __restrict__ ac
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
   const ge_path_trace_params& p = c.p;
-  Tally tally{0, 0, 0};
+  Tally tally;
   uint64_t closest = 0, any = 0;
   if (x < p.width && y < p.height) {
     const size_t pixel = size_t(y) * p.width + x;
@@ -400,7 +383,7 @@ __global__ void __launch_bounds__(64) k_shadow(Bvh bvh, SceneView sv, ge_shadow_
                                                const ge_shadow_receiver* __restrict__ rec, uint32_t* __restrict__ bits,
                                                unsigned long long* counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  Tally tally{0, 0, 0};
+  Tally tally;
   uint64_t issued = 0;
   bool blocked = false;
   if (i < p.count) {

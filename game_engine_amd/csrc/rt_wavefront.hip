@@ -1,5 +1,5 @@
 // rt_wavefront.hip — the reference path tracer as a wavefront: one kernel per stage, paths in
-// queues compacted with wave ballots, so each traversal kernel runs one kind of ray with a small
+// queues compacted with workgroup prefix sums, so each traversal kernel runs one kind of ray with a small
 // register footprint instead of the whole integrator in every lane.
 //
 // Per bounce, over the queue of live paths:
@@ -25,54 +25,53 @@ namespace {
 extern __shared__ uint32_t g_lds[];
 __device__ __forceinline__ uint32_t* lane_stack() { return g_lds + threadIdx.x; }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const uint32_t lane = threadIdx.x & 63u;
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
+#ifndef GE_SHADE_THREADS
+#define GE_SHADE_THREADS 256
+#endif
+// Shade workgroups: one queue-append atomic per workgroup (same-address atomics serialize).
+constexpr uint32_t k_shade_threads = GE_SHADE_THREADS;
+static_assert(k_shade_threads % 64 == 0 && k_shade_threads <= 1024, "whole waves, at most 1024 threads");
 
-// Appends `count` entries per lane where `count` > 0, one atomic per wave; returns each lane's first
-// slot (in lane order within the wave).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t count) {
-  // Inclusive prefix of `count` over the wave by shuffles.
-  const uint32_t lane = threadIdx.x & 63u;
-  uint32_t incl = count;
+// Appends `count` entries per lane, one atomic per workgroup of k_shade_threads; returns each lane's
+// first slot (lanes in order within a wave, waves in order within the workgroup). `part` is a
+// workgroup-shared array of k_shade_threads / 64 + 1 words, one per call site, so two appends in one
+// kernel never share it. Every thread of the workgroup must call it.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, uint32_t count, uint32_t* part) {
+  constexpr uint32_t waves = k_shade_threads / 64;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t incl = count;  // inclusive prefix over the wave by shuffles
   for (int off = 1; off < 64; off <<= 1) {
     const uint32_t v = uint32_t(__shfl_up(int(incl), off));
     if (lane >= uint32_t(off)) incl += v;
   }
-  const uint32_t total = uint32_t(__shfl(int(incl), 63));
-  uint32_t base = 0;
-  if (lane == 63 && total) base = atomicAdd(counter, total);
-  base = uint32_t(__shfl(int(base), 63));
-  return base + incl - count;
+  if (lane == 63) part[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < waves; ++k) {
+      const uint32_t c = part[k];
+      part[k] = total;
+      total += c;
+    }
+    part[waves] = total ? atomicAdd(counter, total) : 0u;
+  }
+  __syncthreads();
+  return part[waves] + part[wave] + incl - count;
 }
 
-__device__ __forceinline__ void flush(unsigned long long* counters, uint64_t closest, uint64_t any, const Tally& t) {
-  uint64_t v[5] = {closest, any, t.nodes, t.tris, t.inst};
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    uint64_t x = v[k];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    v[k] = x;
-  }
-  if ((threadIdx.x & 63u) == 0) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-      if (v[k]) atomicAdd(&counters[k], (unsigned long long)v[k]);
-  }
-}
-
-// Camera rays in 8 x 8 pixel tiles, samples outermost: path id = sample * pixels + pixel.
+// Camera rays in 8 x 8 pixel tiles, samples outermost: path id = sample * pixels + pixel. The first
+// queue lists the paths tile by tile (tiles row-major, pixels row-major within a tile), so a wave
+// of the first closest-hit walk takes one tile; each path's queue slot is that order's closed form.
 __global__ void __launch_bounds__(64) k_generate(ge_path_trace_params p, uint32_t tiles_x, uint32_t tiles,
-                                                 PathState* __restrict__ state, uint32_t* __restrict__ queue,
-                                                 uint32_t* __restrict__ queue_count) {
+                                                 PathState* __restrict__ state, uint32_t* __restrict__ queue) {
   const uint32_t tile = blockIdx.x % tiles, sample = blockIdx.x / tiles;
-  const uint32_t x = (tile % tiles_x) * 8 + (threadIdx.x & 7u);
-  const uint32_t y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
-  const bool valid = x < p.width && y < p.height;
+  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+  const uint32_t lx = threadIdx.x & 7u, ly = threadIdx.x >> 3;
+  const uint32_t x = tx * 8 + lx, y = ty * 8 + ly;
+  if (x >= p.width || y >= p.height) return;
   const uint32_t pixels = p.width * p.height;
-  const uint32_t slot = wave_append(queue_count, valid ? 1u : 0u);
-  if (!valid) return;
+  const uint32_t rows = std::min(8u, p.height - ty * 8), cols = std::min(8u, p.width - tx * 8);
+  const uint32_t slot = sample * pixels + ty * 8 * p.width + tx * 8 * rows + ly * cols + lx;
   const uint32_t pid = sample * pixels + y * p.width + x;
   uint32_t rng = rng_seed(x, y, p.sample_base + sample, p.seed);
   float jx = 0.5f, jy = 0.5f;
@@ -106,10 +105,10 @@ __global__ void __launch_bounds__(64) k_generate(ge_path_trace_params p, uint32_
 }
 
 template <bool Count>
-__global__ void __launch_bounds__(64) k_closest(Bvh bvh, const PathState* __restrict__ state,
-                                                const uint32_t* __restrict__ queue, uint32_t n, uint32_t* head,
-                                                PathHit* __restrict__ hits, unsigned long long* counters) {
-  Tally tally{0, 0, 0};
+__global__ void __launch_bounds__(64) GE_WALK_ATTR k_closest(Bvh bvh, const PathState* __restrict__ state,
+                                                const uint32_t* __restrict__ queue, uint32_t n,
+                                                uint32_t* head, PathHit* __restrict__ hits, unsigned long long* counters) {
+  Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
     const uint32_t pid = queue[i];
@@ -130,7 +129,7 @@ __global__ void __launch_bounds__(64) k_closest(Bvh bvh, const PathState* __rest
     hits[pid] = out;
   };
   persistent_walk<false, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
-  if (Count) flush(counters, issued, 0, tally);
+  if (Count) flush_counters(counters, issued, 0, tally);
 }
 
 __device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool primary) {
@@ -217,8 +216,9 @@ __device__ __forceinline__ bool sun_asked(const ge_path_trace_params& p, const S
   return p.sun[3] > 0.0f && dot(s.normal, ld3(p.sun)) > 0.0f;
 }
 
-__global__ void __launch_bounds__(64) k_shade(ShadeArgs a) {
-  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+__global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
+  __shared__ uint32_t parts[2][k_shade_threads / 64 + 1];
+  const uint32_t g = blockIdx.x * k_shade_threads + threadIdx.x;
   const bool valid = g < a.n;
   const uint32_t pid = valid ? a.queue[g] : 0u;
   PathState st;
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(64) k_shade(ShadeArgs a) {
       }
     }
   }
-  const uint32_t first = wave_append(&a.counts[1], nshadow);
+  const uint32_t first = block_append(&a.counts[1], nshadow, parts[0]);
   if (valid && hit_found) {
     const bool fits = first + nshadow <= a.shadow_capacity;
     NeeRec nr;
@@ -384,14 +384,14 @@ __global__ void __launch_bounds__(64) k_shade(ShadeArgs a) {
     if (!alive) st.flags |= 2u;
     a.out[pid] = st;
   }
-  const uint32_t slot = wave_append(&a.counts[0], alive ? 1u : 0u);
+  const uint32_t slot = block_append(&a.counts[0], alive ? 1u : 0u, parts[1]);
   if (alive) a.next_queue[slot] = pid;
 }
 
 template <bool Count>
-__global__ void __launch_bounds__(64) k_any(Bvh bvh, const ShadowRec* __restrict__ shadow, uint32_t n, uint32_t* head,
+__global__ void __launch_bounds__(64) GE_WALK_ATTR k_any(Bvh bvh, const ShadowRec* __restrict__ shadow, uint32_t n, uint32_t* head,
                                             uint8_t* __restrict__ blocked, unsigned long long* counters) {
-  Tally tally{0, 0, 0};
+  Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t j, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
     const ShadowRec& r = shadow[j];
@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(64) k_any(Bvh bvh, const ShadowRec* __restrict
   };
   auto store = [&](uint32_t j, const Hit& h) { blocked[j] = h.found ? 1 : 0; };
   persistent_walk<true, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
-  if (Count) flush(counters, 0, issued, tally);
+  if (Count) flush_counters(counters, 0, issued, tally);
 }
 
 // radiance + throughput * next_event(...): the unblocked lights summed in record order. A path that
@@ -457,6 +457,16 @@ __global__ void __launch_bounds__(64) k_accumulate(ge_path_trace_params p, const
 inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
 constexpr size_t k_lds = size_t(k_ring) * k_lanes * 4;
 
+// Records the begin (end = 0) or end (end = 1) event of a traversal launch of `kind` (0 closest-hit,
+// 1 any-hit) on the context's stream; launches past the event pool go untimed.
+hipError_t stage_mark(ge_rt_ctx* ctx, int kind, int end) {
+  const uint32_t i = ctx->stage_n[kind];
+  if (i >= GE_RT_STAGE_TIMED) return hipSuccess;
+  const hipError_t e = hipEventRecord(ctx->stage_ev[kind][2 * i + end], ctx->stream);
+  if (end) ctx->stage_n[kind] = i + 1;
+  return e;
+}
+
 }  // namespace
 
 hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
@@ -488,8 +498,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   uint32_t* counts = w.counts.as<uint32_t>();  // [0] first queue; [2] next queue, [3] shadow records, [4] [5] walk heads
   if ((e = hipMemsetAsync(counts, 0, 32, st)) != hipSuccess) return e;
   const uint32_t tiles_x = (p.width + 7) / 8, tiles = tiles_x * ((p.height + 7) / 8);
-  k_generate<<<tiles * p.spp, 64, 0, st>>>(p, tiles_x, tiles, w.state[0].as<PathState>(), w.queue[0].as<uint32_t>(),
-                                           counts);
+  k_generate<<<tiles * p.spp, 64, 0, st>>>(p, tiles_x, tiles, w.state[0].as<PathState>(), w.queue[0].as<uint32_t>());
   if ((e = hipGetLastError()) != hipSuccess) return e;
   int cur = 0;
   uint32_t n = npaths;  // every path starts live
@@ -497,6 +506,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   while (n > 0) {
     if ((e = hipMemsetAsync(c2, 0, 16, st)) != hipSuccess) return e;  // queue and shadow counts, walk heads
     const uint32_t grid = std::min(blocks(n, 64), ctx->persistent_blocks);
+    if ((e = stage_mark(ctx, 0, 0)) != hipSuccess) return e;
     if (counters)
       k_closest<true><<<grid, 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
                                               counts + 4, w.hits.as<PathHit>(), counters);
@@ -504,6 +514,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       k_closest<false><<<grid, 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
                                                counts + 4, w.hits.as<PathHit>(), counters);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = stage_mark(ctx, 0, 1)) != hipSuccess) return e;
     ShadeArgs a;
     a.sv = sv;
     a.p = p;
@@ -519,7 +530,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     for (;;) {
       a.shadow = w.shadow.as<ShadowRec>();
       a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
-      k_shade<<<blocks(n, 64), 64, 0, st>>>(a);
+      k_shade<<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if ((e = hipMemcpyAsync(w.host_counts, c2, 8, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
@@ -537,6 +548,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     }
     if (nshadow > 0) {
       const uint32_t grid = std::min(blocks(nshadow, 64), ctx->persistent_blocks);
+      if ((e = stage_mark(ctx, 1, 0)) != hipSuccess) return e;
       if (counters)
         k_any<true><<<grid, 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow, counts + 5,
                                             w.blocked.as<uint8_t>(), counters);
@@ -544,6 +556,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
         k_any<false><<<grid, 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow, counts + 5,
                                              w.blocked.as<uint8_t>(), counters);
       if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = stage_mark(ctx, 1, 1)) != hipSuccess) return e;
     }
     k_resolve<<<blocks(n, 64), 64, 0, st>>>(w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n,
                                            w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(), w.blocked.as<uint8_t>(),

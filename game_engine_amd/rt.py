@@ -118,7 +118,10 @@ class Context:
         c = abi.Counters()
         self.check(lib().ge_rt_stats(self.h, C.byref(c)))
         return {"closest": c.closest_rays, "any": c.any_rays, "nodes": c.nodes, "triangles": c.triangles,
-                "instances": c.instances, "kernel_ms": c.last_kernel_ms, "build_ms": c.last_build_ms}
+                "instances": c.instances, "any_nodes": c.any_nodes, "any_triangles": c.any_triangles,
+                "any_instances": c.any_instances, "kernel_ms": c.last_kernel_ms, "build_ms": c.last_build_ms,
+                "closest_ms": c.closest_ms, "any_ms": c.any_ms, "closest_launches": c.closest_launches,
+                "any_launches": c.any_launches}
 
     def reset_stats(self):
         self.check(lib().ge_rt_reset_stats(self.h))

@@ -263,16 +263,27 @@ typedef struct ge_shadow_receiver {
   uint32_t pad;
 } ge_shadow_receiver;
 
-/* Traversal counters (GE_RT_COUNT) and the last dispatch's kernel time. */
+/* Traversal counters (GE_RT_COUNT) and the last dispatch's kernel times. */
 typedef struct ge_rt_counters {
   uint64_t closest_rays;   /* closest-hit queries issued */
   uint64_t any_rays;       /* any-hit queries issued */
-  uint64_t nodes;          /* internal BVH nodes popped (TLAS + BLAS) */
-  uint64_t triangles;      /* ray/triangle tests */
-  uint64_t instances;      /* TLAS leaves entered (ray transformed into a BLAS) */
+  uint64_t nodes;          /* internal BVH nodes popped (TLAS + BLAS), both query kinds */
+  uint64_t triangles;      /* ray/triangle tests, both query kinds */
+  uint64_t instances;      /* TLAS leaves entered (ray transformed into a BLAS), both query kinds */
   double last_kernel_ms;   /* the last traced dispatch's device time, hipEvent on the ctx stream */
   double last_build_ms;    /* the last BVH build's device time */
+  uint64_t any_nodes;      /* the part of nodes, triangles and instances that any-hit queries made */
+  uint64_t any_triangles;
+  uint64_t any_instances;
+  /* The last wavefront path trace's traversal launches, timed with hipEvents on the ctx stream:
+   * the closest-hit walks (one a bounce) and the any-hit shadow walks, summed, and how many
+   * launches the sums cover (the first GE_RT_STAGE_TIMED of each kind are timed). */
+  double closest_ms;
+  double any_ms;
+  uint32_t closest_launches;
+  uint32_t any_launches;
 } ge_rt_counters;
+#define GE_RT_STAGE_TIMED 32
 
 /* The built structure, read back for topology parity. Node and triangle records are the backend's
  * 64-byte and 48-byte formats. */
@@ -417,6 +428,7 @@ static_assert(sizeof(ge_resolve_material) == 128, "gfx::ResolveMaterial is 128 b
 static_assert(sizeof(ge_resolve_light) == 64, "gfx::ResolveLight is 64 bytes");
 static_assert(sizeof(ge_path_trace_params) == 368, "gfx::PathTraceParams is 368 bytes");
 static_assert(sizeof(ge_ray_visibility_params) == 256, "gfx::RayVisibilityParams is 256 bytes");
+static_assert(sizeof(ge_rt_counters) == 104, "counters record");
 static_assert(sizeof(ge_ray) == 32, "ge_ray is 32 bytes");
 static_assert(sizeof(ge_hit) == 32, "ge_hit is 32 bytes");
 static_assert(sizeof(ge_bvh_node) == 64, "a BVH node is one 64-byte fetch");

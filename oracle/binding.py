@@ -16,9 +16,35 @@ _LIB = None
 FP32, FP64 = 0, 1
 
 
+REFERENCE = "/root/reference"
+_REF = None
+
+
 def build():
-    """Compiles the oracle with its Makefile (g++ -O3 -march=x86-64-v3 -ffp-contract=off)."""
+    """Compiles the oracle with its Makefile (g++ -O3 -march=x86-64-v3 -ffp-contract=off), and,
+    where the reference tree is present, its double-precision BSDF twin into oracle/_ref/."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.isdir(REFERENCE):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref", f"REFERENCE={REFERENCE}"], check=True)
+
+
+def ref_lib():
+    """The reference's BSDF twin (oracle/_ref/libbrdf_ref.so) if it was built, else None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libbrdf_ref.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        d, vp = C.c_double, C.c_void_p
+        for name, args in (("ge_ref_alpha", [d]), ("ge_ref_d_ggx", [d, d]), ("ge_ref_v_smith", [d, d, d]),
+                           ("ge_ref_distance_attenuation", [d, d]), ("ge_ref_spot_attenuation", [vp, vp, d, d])):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = args
+        for name in ("ge_ref_eval", "ge_ref_fresnel", "ge_ref_direct"):
+            getattr(L, name).argtypes = [vp, vp]
+        _REF = L
+    return _REF
 
 
 def lib():
@@ -149,9 +175,11 @@ def tonemap(params, accum, coverage):
 
 
 def counters():
-    out = np.zeros(5, dtype=np.uint64)
+    out = np.zeros(8, dtype=np.uint64)
     lib().ge_oracle_counters(out.ctypes.data)
-    return dict(zip(["closest", "any", "nodes", "triangles", "instances"], (int(v) for v in out)))
+    c = [int(v) for v in out]
+    return {"closest": c[0], "any": c[1], "nodes": c[2] + c[5], "triangles": c[3] + c[6],
+            "instances": c[4] + c[7], "any_nodes": c[5], "any_triangles": c[6], "any_instances": c[7]}
 
 
 def reset_counters():

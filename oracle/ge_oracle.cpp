@@ -27,16 +27,20 @@
 namespace {
 
 // ---- counters --------------------------------------------------------------------------------
-std::atomic<uint64_t> g_counters[5];  // closest, any, nodes, triangles, instances
+// closest rays, any rays, then nodes, triangles and instances of the closest-hit walks and of the
+// any-hit walks
+std::atomic<uint64_t> g_counters[8];
 struct Tally {
-  uint64_t closest = 0, any = 0, nodes = 0, tris = 0, inst = 0;
+  uint64_t closest = 0, any = 0, nodes[2] = {0, 0}, tris[2] = {0, 0}, inst[2] = {0, 0};
   void flush() {
     g_counters[0] += closest;
     g_counters[1] += any;
-    g_counters[2] += nodes;
-    g_counters[3] += tris;
-    g_counters[4] += inst;
-    closest = any = nodes = tris = inst = 0;
+    for (int k = 0; k < 2; ++k) {
+      g_counters[2 + 3 * k] += nodes[k];
+      g_counters[3 + 3 * k] += tris[k];
+      g_counters[4 + 3 * k] += inst[k];
+    }
+    *this = Tally();
   }
 };
 
@@ -541,9 +545,9 @@ inline bool intersect(const TRay& r, const float* v0, const float* v1, const flo
 }
 
 struct HitRec {
-  float t, u, v;
-  uint32_t inst, prim, key;
-  bool found;
+  float t = 0, u = 0, v = 0;
+  uint32_t inst = 0, prim = 0, key = 0;
+  bool found = false;
 };
 
 inline void to_object(const InstRec& rec, const TRay& w, TRay& o) {
@@ -584,10 +588,10 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
       if (sp >= 190) std::abort();  // deeper than any structure the build accepts
       stack[sp++] = k_sentinel;
       in_blas = true;
-      ++tally.inst;
+      ++tally.inst[AnyHit];
       cur = 0;
     }
-    ++tally.nodes;
+    ++tally.nodes[AnyHit];
     const ge_bvh_node& node = in_blas ? blas->bvh.nodes[cur] : s.tlas.nodes[cur];
     float tn0 = 0, tn1 = 0;
     bool h0 = node.child[0] != GE_BVH_NONE && slab(node.lo0, node.hi0, ray, hit.t, tn0);
@@ -598,7 +602,7 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
         if (!hc || !(node.child[c] & GE_BVH_LEAF)) continue;
         hc = false;
         const ge_bvh_triangle& tri = blas->tris[node.child[c] & ~GE_BVH_LEAF];
-        ++tally.tris;
+        ++tally.tris[AnyHit];
         float t, u, v;
         if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
         if (!(t >= ray.tmin && t <= hit.t)) continue;
@@ -1443,11 +1447,11 @@ void ge_oracle_shadow_pass(const ge_oracle_scene* s, const ge_shadow_params* p, 
   }
 }
 
-void ge_oracle_counters(uint64_t out[5]) {
-  for (int i = 0; i < 5; ++i) out[i] = g_counters[i].load();
+void ge_oracle_counters(uint64_t out[8]) {
+  for (int i = 0; i < 8; ++i) out[i] = g_counters[i].load();
 }
 void ge_oracle_reset_counters(void) {
-  for (int i = 0; i < 5; ++i) g_counters[i] = 0;
+  for (int i = 0; i < 8; ++i) g_counters[i] = 0;
 }
 int32_t ge_oracle_max_threads(void) { return resolve_threads(0); }
 
@@ -1543,6 +1547,7 @@ template <class R> int32_t probe(uint32_t kind, const double* in, double* out) {
       put(0, transform_normal(inst, V(9)));
       return 0;
     }
+    case 16: put(0, brdf_direct(surface(0), V(15), V(18))); return 0;
     default: return GE_RT_ERROR_INVALID_ARGUMENT;
   }
 }

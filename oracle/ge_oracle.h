@@ -62,8 +62,9 @@ void ge_oracle_tonemap(const ge_path_trace_params* params, const float* accum,
                        const uint32_t* coverage, uint32_t* output);
 void ge_oracle_shadow_pass(const ge_oracle_scene* scene, const ge_shadow_params* params,
                            const ge_shadow_receiver* receivers, uint32_t* blocked, int32_t threads);
-/* Traversal counters accumulated since the last reset (closest, any, nodes, triangles, instances). */
-void ge_oracle_counters(uint64_t out[5]);
+/* Traversal counters accumulated since the last reset: closest rays, any rays, then nodes,
+ * triangles and instances of the closest-hit walks, then those of the any-hit walks. */
+void ge_oracle_counters(uint64_t out[8]);
 void ge_oracle_reset_counters(void);
 int32_t ge_oracle_max_threads(void);
 
@@ -78,7 +79,7 @@ void ge_oracle_sincos(float x, float* s, float* c);
  * source. kind: 0 cosine hemisphere, 1 GGX VNDF sample, 2 ggx_vndf_pdf, 3 brdf_eval, 4 bsdf_pdf,
  * 5 specular_probability, 6 onb_from_normal, 7 ggx_g1, 8 v_smith, 9 d_ggx, 10 fresnel,
  * 11 mis_power_heuristic, 12 light_sample, 13 decode_normal_oct, 14 display_output,
- * 15 transform_normal. */
+ * 15 transform_normal, 16 brdf_direct. */
 int32_t ge_oracle_probe(uint32_t kind, uint32_t mode, const double* in, double* out);
 /* Watertight ray/triangle test: out = {hit, t, u, v}. */
 void ge_oracle_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3],

@@ -104,6 +104,15 @@ def test_counters_match_oracle_walk(ctx, c2):
     gs, os_ = ctx.stats(), ob.counters()
     for k in ("closest", "nodes", "triangles", "instances"):
         assert gs[k] == os_[k], k
+    assert gs["any_nodes"] == os_["any_nodes"] == 0
+    ctx.reset_stats()
+    ob.reset_counters()
+    g.trace_any(rays, count=True)
+    o.trace_any(rays)
+    gs, os_ = ctx.stats(), ob.counters()
+    for k in ("any", "nodes", "any_nodes", "any_triangles", "any_instances"):
+        assert gs[k] == os_[k], k
+    assert gs["any_nodes"] == gs["nodes"] > 0
 
 
 def test_ray_visibility_words(c1, c2):
@@ -147,8 +156,13 @@ def test_path_trace_c2_lights(ctx, c2, megakernel):
     print(f"C2 radiance: max rel err {err:.3g}, bit-exact pixels {exact:.4f}")
     assert err <= RAD_TOL
     gs, os_ = ctx.stats(), ob.counters()
-    for k in ("closest", "any", "nodes", "triangles", "instances"):
+    for k in ("closest", "any", "nodes", "triangles", "instances", "any_nodes", "any_triangles", "any_instances"):
         assert gs[k] == os_[k], k  # the same rays, walked the same way
+    if not megakernel:
+        # One closest-hit walk a bounce and one any-hit walk a bounce with shadow rays, each timed.
+        assert 1 <= gs["closest_launches"] <= pt.max_bounces + 1
+        assert 1 <= gs["any_launches"] <= gs["closest_launches"]
+        assert 0 < gs["closest_ms"] + gs["any_ms"] <= gs["kernel_ms"]
 
 
 def test_path_trace_many_lights_regrows_shadow_queue(ctx):
