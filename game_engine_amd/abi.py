@@ -81,6 +81,11 @@ class ShadowParams(C.Structure):
                 ("count", u32), ("pad", u32 * 4)]
 
 
+class ShadowResolveParams(C.Structure):
+    _fields_ = [("view_proj", f32 * 16), ("camera", f32 * 4), ("sun", f32 * 4), ("width", u32), ("height", u32),
+                ("shadow_bias", f32), ("shadow_bias_steps", f32), ("flags", u32), ("seed", u32), ("pad", u32 * 2)]
+
+
 class Counters(C.Structure):
     _fields_ = [("closest_rays", u64), ("any_rays", u64), ("nodes", u64), ("triangles", u64),
                 ("instances", u64), ("last_kernel_ms", f64), ("last_build_ms", f64), ("any_nodes", u64),
@@ -111,7 +116,7 @@ RECEIVER_DTYPE = np.dtype([("position", "<f4", 3), ("instance_id", "<u4"), ("geo
 
 SIZES = {ClusterDesc: 48, VertexAttributes: 8, MeshDesc: 64, InstanceDesc: 96, FrameEye: 48,
          ResolveMaterial: 128, ResolveLight: 64, PathTraceParams: 368, RayVisibilityParams: 256,
-         ShadowParams: 48, Counters: 104}
+         ShadowParams: 48, Counters: 104, ShadowResolveParams: 128}
 for _t, _n in SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
@@ -129,4 +134,14 @@ PT_PIXEL_CENTER = 2
 PT_DITHER = 4
 PT_ALPHA_MASKS = 8
 BUILD_FULL, BUILD_SUBSET, BUILD_TOP = 0, 1, 2
+SHADOW_HALF_RATE = 0x1
+SHADOW_JITTER = 0x2
+SHADOW_NOT_TAKEN = 0xFFFFFFFF
+
+
+def shadow_entries(params):
+    """Entries of a shadow resolve's output: pixels, or 2x2 blocks at half rate."""
+    if params.flags & SHADOW_HALF_RATE:
+        return ((params.width + 1) // 2) * ((params.height + 1) // 2)
+    return params.width * params.height
 INSTANCE_STATIC_RT = 128

@@ -171,6 +171,7 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
                     &ctx->build.node_parent, &ctx->build.seg_cbox, &ctx->build.seg_scale, &ctx->counters})
     b->release();
   for (DevBuf& b : ctx->io) b.release();
+  ctx->resolve_rays.release();
   for (DevBuf& b : ctx->tmp) b.release();
   for (DevBuf* b : {&ctx->wave.state[0], &ctx->wave.state[1], &ctx->wave.hits, &ctx->wave.nee, &ctx->wave.result,
                     &ctx->wave.queue[0], &ctx->wave.queue[1], &ctx->wave.counts, &ctx->wave.shadow, &ctx->wave.blocked})
@@ -320,7 +321,7 @@ void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
   for (DevBuf* b : {&s.clusters, &s.vertices, &s.quantized, &s.attributes, &s.triangles, &s.meshes, &s.instances,
                     &s.materials, &s.cluster_materials, &s.lights, &s.tri_prefix, &s.tri_mesh_of, &s.blas_nodes,
                     &s.blas_tris, &s.blas_morton, &s.mesh_roots, &s.mesh_height, &s.tlas_nodes, &s.tlas_slots,
-                    &s.tlas_morton, &s.inst_recs, &s.inst_boxes, &s.live_ids, &s.tlas_root, &s.tlas_height})
+                    &s.tlas_morton, &s.inst_recs, &s.inst_boxes, &s.live_ids, &s.tlas_root, &s.tlas_height, &s.pairs})
     b->release();
   delete scene;
 }
@@ -337,6 +338,7 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   const int32_t st = validate_instances(ctx, s, instances, count, cm.data());
   if (st != GE_RT_OK) return st;
   std::memcpy(&s.h_instances[first], instances, size_t(count) * sizeof(ge_instance_desc));
+  s.pairs_dirty = true;
   GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, instances, size_t(count) * sizeof(ge_instance_desc),
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -761,6 +763,67 @@ int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_sha
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(blocked, db, words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
+                             const uint64_t* visibility, uint32_t* out, uint32_t flags) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null parameters");
+  GE_CHECK(ctx, (params->flags & ~(GE_SHADOW_HALF_RATE | GE_SHADOW_JITTER)) == 0, GE_RT_ERROR_UNSUPPORTED,
+           "unknown shadow resolve flags");
+  const uint64_t pixels = uint64_t(params->width) * params->height;
+  const bool half = params->flags & GE_SHADOW_HALF_RATE;
+  const uint64_t entries = half ? uint64_t((params->width + 1) / 2) * ((params->height + 1) / 2) : pixels;
+  GE_CHECK(ctx, pixels < (1ull << 31), GE_RT_ERROR_INVALID_ARGUMENT, "more than 2^31 pixels");
+  GE_CHECK(ctx, (visibility && out) || entries == 0, GE_RT_ERROR_INVALID_ARGUMENT, "null visibility or output");
+  if (entries == 0) return GE_RT_OK;
+  SceneDev& s = const_cast<SceneDev&>(scene->dev);
+  if (s.pairs_dirty) {
+    // The engine's pair numbering: instance i owns pairs [first_pair, first_pair + its clusters).
+    uint64_t count = 0;
+    for (const ge_instance_desc& in : s.h_instances)
+      count = std::max<uint64_t>(count, uint64_t(in.first_pair) + s.h_meshes[in.mesh].cluster_count);
+    GE_CHECK(ctx, count <= (1u << 24), GE_RT_ERROR_INVALID_SCENE, "pairs past 2^24 do not fit a visibility word");
+    std::vector<uint32_t> table(size_t(count) * 2, GE_RT_MISS);
+    for (uint32_t i = 0; i < s.instance_count; ++i) {
+      const ge_instance_desc& in = s.h_instances[i];
+      for (uint32_t c = 0; c < s.h_meshes[in.mesh].cluster_count; ++c) {
+        uint32_t* e = &table[(size_t(in.first_pair) + c) * 2];
+        GE_CHECK(ctx, e[0] == GE_RT_MISS, GE_RT_ERROR_INVALID_SCENE,
+                 "instances " + std::to_string(e[0]) + " and " + std::to_string(i) + " claim pair " +
+                     std::to_string(in.first_pair + c));
+        e[0] = i;
+        e[1] = c;
+      }
+    }
+    GE_HIP(ctx, s.pairs.reserve(std::max<size_t>(table.size() * 4, 8)));
+    if (!table.empty())
+      GE_HIP(ctx, hipMemcpy(s.pairs.ptr, table.data(), table.size() * 4, hipMemcpyHostToDevice));
+    s.pair_count = uint32_t(count);
+    s.pairs_dirty = false;
+  }
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  const uint64_t* dw = visibility;
+  uint32_t* dout = out;
+  if (!dev) {
+    uint64_t* w;
+    if ((r = stage(ctx, 0, visibility, size_t(pixels), true, &w)) != GE_RT_OK) return r;
+    dw = w;
+    if ((r = stage(ctx, 1, out, size_t(entries), false, &dout)) != GE_RT_OK) return r;
+  }
+  GE_HIP(ctx, ctx->resolve_rays.reserve(size_t(entries) * 16));
+  GE_HIP(ctx, begin_timing(ctx));
+  GE_HIP(ctx, launch_shadow_resolve(ctx->stream, bvh_view(s), scene_view(s), s.eye, *params, dw,
+                                    s.pairs.as<uint2>(), s.pair_count, ctx->resolve_rays.as<float4>(), dout,
+                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
+                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) {
+    GE_HIP(ctx, hipMemcpyAsync(out, dout, size_t(entries) * 4, hipMemcpyDeviceToHost, ctx->stream));
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;

@@ -63,6 +63,11 @@ struct SceneDev {
   uint32_t tlas_height_value = 0;
   uint32_t stack_cap = 0;  // traversal stack entries a lane needs: both heights + 2
   bool built = false;
+  // The engine's pair numbering, pair -> (instance, cluster within its mesh), for visibility words;
+  // rebuilt on first use after an upload or an instance update.
+  DevBuf pairs;
+  uint32_t pair_count = 0;
+  bool pairs_dirty = true;
   ge_frame_eye eye{};
 };
 
@@ -127,6 +132,7 @@ struct ge_rt_ctx {
   double stage_ms[2] = {0.0, 0.0};
   uint32_t stage_launches[2] = {0, 0};
   ge::Wavefront wave;
+  ge::DevBuf resolve_rays;  // the visibility-buffer shadow pass's ray records
   uint32_t persistent_blocks = 8192;  // one-wave workgroups a persistent walk launches: CUs x 32
 };
 
@@ -172,6 +178,13 @@ struct SceneView {
   uint32_t light_count;
   uint32_t pad;
 };
+
+// The sun shadow pass over a visibility buffer (rt_shadow.hip): `rays` holds one 16-byte record an
+// entry, `out` the entries' answers.
+hipError_t launch_shadow_resolve(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_frame_eye& eye,
+                                 const ge_shadow_resolve_params& p, const uint64_t* words, const uint2* pairs,
+                                 uint32_t pair_count, float4* rays, uint32_t* out, unsigned long long* counters,
+                                 uint32_t* head, uint32_t max_blocks);
 
 // Persistent walks: `head` is a device counter the launch clears, `max_blocks` the grid's cap.
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,

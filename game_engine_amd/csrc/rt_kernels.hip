@@ -149,51 +149,16 @@ struct HitSurface {
   uint32_t cluster;
 };
 
-// fetch_triangle + surface_at (material.slang:244-302): the triangle's corners off the mesh's
-// 16-bit grid into the frame through the instance, the geometric normal turned to `towards`, the
-// shading normal from the octahedral vertex normals through the cofactor matrix.
+// fetch_triangle + surface_at (material.slang:244-302) at the hit's barycentrics.
 __device__ HitSurface surface_of_hit(const SceneView& sv, const ge_path_trace_params& p, const Hit& hit, f3 towards,
                                      const ge_instance_desc& in) {
-  const ge_mesh_desc mesh = sv.meshes[in.mesh];
-  const uint32_t cluster_index = mesh.first_cluster + (hit.prim >> 8);
-  const ge_cluster_desc cl = sv.clusters[cluster_index];
-  const uint32_t packed = sv.triangles[cl.triangle_offset + (hit.prim & 0xffu)];
-  const f3 T = f3{relative_axis(in.cell[0], in.rows[0][3], p.eye.cell[0], p.eye.local[0], p.eye.residual[0]),
-                  relative_axis(in.cell[1], in.rows[1][3], p.eye.cell[1], p.eye.local[1], p.eye.residual[1]),
-                  relative_axis(in.cell[2], in.rows[2][3], p.eye.cell[2], p.eye.local[2], p.eye.residual[2])};
-  f3 pc[3];
-  uint32_t local[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    local[k] = cl.vertex_offset + ((packed >> (8 * k)) & 0xffu);
-    const uint32_t i = local[k] * 3;
-    // Three u16 of the grid, read as the 32-bit words that hold them (load_quantized_position).
-    const uint32_t* words = reinterpret_cast<const uint32_t*>(sv.quantized);
-    const uint32_t w0 = words[i >> 1], w1 = words[(i + 1) >> 1], w2 = words[(i + 2) >> 1];
-    const uint32_t qx = (i & 1) ? (w0 >> 16) : (w0 & 0xffffu);
-    const uint32_t qy = ((i + 1) & 1) ? (w1 >> 16) : (w1 & 0xffffu);
-    const uint32_t qz = ((i + 2) & 1) ? (w2 >> 16) : (w2 & 0xffffu);
-    const f3 m = f3{mesh.quant[0], mesh.quant[1], mesh.quant[2]} + f3{float(qx), float(qy), float(qz)} * mesh.quant[3];
-    f3 d;
-    d.x = in.rows[0][0] * m.x + in.rows[0][1] * m.y + in.rows[0][2] * m.z;
-    d.y = in.rows[1][0] * m.x + in.rows[1][1] * m.y + in.rows[1][2] * m.z;
-    d.z = in.rows[2][0] * m.x + in.rows[2][1] * m.y + in.rows[2][2] * m.z;
-    pc[k] = d + T;
-  }
-  const f3 b = f3{1.0f - hit.u - hit.v, hit.u, hit.v};
+  const uint32_t cluster_index = sv.meshes[in.mesh].first_cluster + (hit.prim >> 8);
+  const Corners c = fetch_corners(sv, p.eye, in, cluster_index, hit.prim & 0xffu);
+  const Vertex v = surface_at(sv, in, c, f3{1.0f - hit.u - hit.v, hit.u, hit.v}, towards);
   HitSurface out;
-  out.position = b.x * pc[0] + b.y * pc[1] + b.z * pc[2];
-  out.geometric = normalize(cross(pc[1] - pc[0], pc[2] - pc[0]));
-  if (dot(out.geometric, towards - out.position) < 0.0f) out.geometric = -out.geometric;
-  if (sv.attributes) {
-    const f3 n0 = decode_normal_oct(sv.attributes[local[0]].normal_oct);
-    const f3 n1 = decode_normal_oct(sv.attributes[local[1]].normal_oct);
-    const f3 n2 = decode_normal_oct(sv.attributes[local[2]].normal_oct);
-    const f3 n = b.x * n0 + b.y * n1 + b.z * n2;
-    out.normal = transform_normal(in, n);
-  } else {
-    out.normal = out.geometric;
-  }
+  out.position = v.position;
+  out.normal = v.normal;
+  out.geometric = v.geometric;
   out.cluster = cluster_index;
   return out;
 }

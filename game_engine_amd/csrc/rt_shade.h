@@ -187,4 +187,91 @@ GE_DEV float relative_axis(int cell, float local, int eye_cell, float eye_local,
   return from_local - eye_residual;
 }
 
+// A triangle fetched into frame space (fetch_triangle, material.slang:244-266): its corners off the
+// mesh's 16-bit grid (load_quantized_position, scene.slang:126-170) through the instance's rows and
+// eye-relative translation (instance_point, scene.slang:566-610), and its vertices' indices.
+struct Corners {
+  f3 p[3];
+  uint32_t local[3];
+};
+template <class SceneV>
+GE_DEV Corners fetch_corners(const SceneV& sv, const ge_frame_eye& eye, const ge_instance_desc& in,
+                             uint32_t cluster_index, uint32_t triangle) {
+  const ge_mesh_desc mesh = sv.meshes[in.mesh];
+  const ge_cluster_desc cl = sv.clusters[cluster_index];
+  const uint32_t packed = sv.triangles[cl.triangle_offset + triangle];
+  const f3 T = f3{relative_axis(in.cell[0], in.rows[0][3], eye.cell[0], eye.local[0], eye.residual[0]),
+                  relative_axis(in.cell[1], in.rows[1][3], eye.cell[1], eye.local[1], eye.residual[1]),
+                  relative_axis(in.cell[2], in.rows[2][3], eye.cell[2], eye.local[2], eye.residual[2])};
+  Corners c;
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(sv.quantized);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    c.local[k] = cl.vertex_offset + ((packed >> (8 * k)) & 0xffu);
+    const uint32_t i = c.local[k] * 3;
+    const uint32_t w0 = words[i >> 1], w1 = words[(i + 1) >> 1], w2 = words[(i + 2) >> 1];
+    const uint32_t qx = (i & 1) ? (w0 >> 16) : (w0 & 0xffffu);
+    const uint32_t qy = ((i + 1) & 1) ? (w1 >> 16) : (w1 & 0xffffu);
+    const uint32_t qz = ((i + 2) & 1) ? (w2 >> 16) : (w2 & 0xffffu);
+    const f3 m = f3{mesh.quant[0], mesh.quant[1], mesh.quant[2]} + f3{float(qx), float(qy), float(qz)} * mesh.quant[3];
+    f3 d;
+    d.x = in.rows[0][0] * m.x + in.rows[0][1] * m.y + in.rows[0][2] * m.z;
+    d.y = in.rows[1][0] * m.x + in.rows[1][1] * m.y + in.rows[1][2] * m.z;
+    d.z = in.rows[2][0] * m.x + in.rows[2][1] * m.y + in.rows[2][2] * m.z;
+    c.p[k] = d + T;
+  }
+  return c;
+}
+
+struct Vertex {
+  f3 position, normal, geometric;
+};
+
+// surface_at (material.slang:270-302): position, shading normal and geometric normal at
+// barycentrics `b`, the geometric normal turned towards `towards`.
+template <class SceneV>
+GE_DEV Vertex surface_at(const SceneV& sv, const ge_instance_desc& in, const Corners& c, f3 b, f3 towards) {
+  Vertex out;
+  out.position = b.x * c.p[0] + b.y * c.p[1] + b.z * c.p[2];
+  out.geometric = normalize(cross(c.p[1] - c.p[0], c.p[2] - c.p[0]));
+  if (dot(out.geometric, towards - out.position) < 0.0f) out.geometric = -out.geometric;
+  if (sv.attributes) {
+    const f3 n0 = decode_normal_oct(sv.attributes[c.local[0]].normal_oct);
+    const f3 n1 = decode_normal_oct(sv.attributes[c.local[1]].normal_oct);
+    const f3 n2 = decode_normal_oct(sv.attributes[c.local[2]].normal_oct);
+    const f3 n = b.x * n0 + b.y * n1 + b.z * n2;
+    out.normal = transform_normal(in, n);
+  } else {
+    out.normal = out.geometric;
+  }
+  return out;
+}
+
+// screen_barycentrics (material.slang:386-416), the barycentrics alone: where the ray through
+// `pixel` (in pixels, y down) meets the triangle's plane, the edge functions on the plane's side
+// of zero kept, the rest zeroed, over their sum.
+GE_DEV f3 screen_barycentrics(const Corners& c, float px, float py, const float* vp, f3 eye, uint32_t width,
+                              uint32_t height) {
+  f3 cc[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const f3 v = c.p[i] - eye;
+    cc[i].x = ((vp[0] * v.x + vp[4] * v.y) + vp[8] * v.z) + vp[12] * 0.0f;
+    cc[i].y = ((vp[1] * v.x + vp[5] * v.y) + vp[9] * v.z) + vp[13] * 0.0f;
+    cc[i].z = ((vp[3] * v.x + vp[7] * v.y) + vp[11] * v.z) + vp[15] * 0.0f;
+  }
+  const float ndx = 2.0f / float(width), ndy = -2.0f / float(height);
+  const f3 p = f3{px * ndx + -1.0f, py * ndy + 1.0f, 1.0f};
+  const f3 n0 = cross(cc[1], cc[2]);
+  const f3 n1 = cross(cc[2], cc[0]);
+  const f3 n2 = cross(cc[0], cc[1]);
+  const f3 e = f3{dot(n0, p), dot(n1, p), dot(n2, p)};
+  const float plane = e.x + e.y + e.z;
+  const f3 s = plane < 0.0f ? -e : e;
+  const f3 inside = f3{maxr(s.x, 0.0f), maxr(s.y, 0.0f), maxr(s.z, 0.0f)};
+  const float total = inside.x + inside.y + inside.z;
+  if (!(total > 0.0f)) return f3{1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 3.0f};
+  return f3{inside.x / total, inside.y / total, inside.z / total};
+}
+
 }  // namespace ge

@@ -63,6 +63,7 @@ def lib():
             "ge_rt_path_trace": ([vp, vp, C.POINTER(abi.PathTraceParams), vp, vp, u32], i32),
             "ge_rt_tonemap": ([vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, u32], i32),
             "ge_rt_shadow_pass": ([vp, vp, C.POINTER(abi.ShadowParams), vp, vp, u32], i32),
+            "ge_rt_shadow_resolve": ([vp, vp, C.POINTER(abi.ShadowResolveParams), vp, vp, u32], i32),
             "ge_rt_stats": ([vp, C.POINTER(abi.Counters)], i32),
             "ge_rt_reset_stats": ([vp], i32),
         }
@@ -77,7 +78,8 @@ def lib():
 EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchronize", "ge_rt_scene_upload",
            "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_bvh_info",
            "ge_rt_bvh_download", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
-           "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_stats", "ge_rt_reset_stats")
+           "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve", "ge_rt_stats",
+           "ge_rt_reset_stats")
 
 
 def _ptr(a):
@@ -207,6 +209,14 @@ class Scene:
         flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
         self.ctx.check(lib().ge_rt_shadow_pass(self.ctx.h, self.h, C.byref(params), _ptr(receivers), _ptr(bits), flags))
         return bits
+
+    def shadow_resolve(self, params, words, count=False, out=None, device=False):
+        """The sun shadow pass over a visibility buffer: per pixel (or 2x2 block at half rate)
+        pair << 1 | blocked, or abi.SHADOW_NOT_TAKEN."""
+        out = np.zeros(abi.shadow_entries(params), dtype=np.uint32) if out is None else out
+        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        self.ctx.check(lib().ge_rt_shadow_resolve(self.ctx.h, self.h, C.byref(params), _ptr(words), _ptr(out), flags))
+        return out
 
 
 def tonemap(ctx, params, accum, coverage, device=False, output=None):

@@ -77,6 +77,19 @@ class Synth:
         lib().ge_synth_frame(self.h, width, height, C.byref(pt), C.byref(rv))
         return pt, rv
 
+    @staticmethod
+    def shadow_params(pt, rv, flags=0, seed=0):
+        """The shadow resolve's frame for a frame of this scene: its projection, eye, sun and ray
+        offset (the path tracer's ray_bias and steps are the frame's shadow bias)."""
+        p = abi.ShadowResolveParams()
+        p.view_proj[:] = list(rv.view_proj)
+        p.camera[:] = list(rv.camera)
+        p.sun[:] = list(pt.sun)
+        p.width, p.height = rv.width, rv.height
+        p.shadow_bias, p.shadow_bias_steps = pt.ray_bias, pt.ray_bias_steps
+        p.flags, p.seed = flags, seed
+        return p
+
     def set_camera(self, eye, target, fov_deg):
         e = (C.c_float * 3)(*eye)
         t = (C.c_float * 3)(*target)

@@ -254,6 +254,31 @@ typedef struct ge_shadow_params {
   uint32_t pad[4];
 } ge_shadow_params;
 
+/* The shadow pass over a visibility buffer: each covered pixel's surface reconstructed from its
+ * visibility word as the resolve does it (reconstruct_screen, material.slang:386-458), one sun ray
+ * from it (shadow_visibility, visibility_resolve.slang:742-755), or one ray per 2x2 block of one
+ * pair (half_sun_block, resolve_half_sun.slang:35-96). The frame is the one the structure was
+ * built for (ge_rt_bvh_build's eye). */
+typedef struct ge_shadow_resolve_params {
+  float view_proj[16];     /* column-major, the projection the visibility words were drawn with */
+  float camera[4];         /* xyz: the eye in the frame's space (zero for the renderer's views) */
+  float sun[4];            /* xyz: unit direction towards the sun */
+  uint32_t width, height;  /* the visibility buffer's pixels */
+  float shadow_bias;       /* ray_offset's floor (FrameLighting::shadow_bias) */
+  float shadow_bias_steps; /* ray_offset's grid steps (FrameLighting::shadow_bias_steps) */
+  uint32_t flags;          /* GE_SHADOW_HALF_RATE, GE_SHADOW_JITTER */
+  uint32_t seed;           /* the jitter's seed */
+  uint32_t pad[2];
+} ge_shadow_resolve_params;
+/* One ray per 2x2 block whose four words hold one pair (the top-left pixel's), as half_sun_block. */
+#define GE_SHADOW_HALF_RATE 0x1u
+/* The pixel's point at a per-pixel hash offset (rng_seed(x, y, 0, seed), two draws) instead of its
+ * centre: the stochastic sampling of one ray per pixel. The resolve itself uses the centre. */
+#define GE_SHADOW_JITTER 0x2u
+/* An entry with no ray: no surface, a pair with no instance, a surface facing away from the sun by
+ * both normals, or (half rate) a block of more than one pair or at the picture's edge. */
+#define GE_SHADOW_NOT_TAKEN 0xffffffffu
+
 /* One receiver of the shadow pass: a surface point in the frame's space, the geometric normal
  * facing the eye, and the instance whose grid sets the offset. 32 bytes. */
 typedef struct ge_shadow_receiver {
@@ -412,6 +437,16 @@ int32_t ge_rt_tonemap(ge_rt_ctx* ctx, const ge_path_trace_params* params, const 
 int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_params* params,
                           const ge_shadow_receiver* receivers, uint32_t* blocked, uint32_t flags);
 
+/* The sun shadow pass over a visibility buffer of width x height 64-bit words (ge_rt_ray_visibility's
+ * or the rasterizer's: depth bits << 32 | pair << 8 | triangle, zero where empty). out[i] is
+ * pair << 1 | blocked, or GE_SHADOW_NOT_TAKEN; i is the pixel (width x height entries), or with
+ * GE_SHADOW_HALF_RATE the 2x2 block ((width + 1) / 2 x (height + 1) / 2 entries). Pairs map to
+ * (instance, cluster) through the instances' first_pair (the engine's pair numbering). Replaces the
+ * resolve's sun query (visibility_resolve.slang:742-755, 898-989) and the half-rate sun pass
+ * (resolve_half_sun.slang:35-110). */
+int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
+                             const uint64_t* visibility, uint32_t* out, uint32_t flags);
+
 /* Counters and the last kernel/build times (ReferenceFrame::trace_ms, reference.h:95). */
 int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out);
 int32_t ge_rt_reset_stats(ge_rt_ctx* ctx);
@@ -434,6 +469,7 @@ static_assert(sizeof(ge_hit) == 32, "ge_hit is 32 bytes");
 static_assert(sizeof(ge_bvh_node) == 64, "a BVH node is one 64-byte fetch");
 static_assert(sizeof(ge_bvh_triangle) == 48, "a BVH triangle is 48 bytes");
 static_assert(sizeof(ge_shadow_receiver) == 32, "a shadow receiver is 32 bytes");
+static_assert(sizeof(ge_shadow_resolve_params) == 128, "shadow resolve params are 128 bytes");
 #endif
 
 #endif /* GE_RT_H */

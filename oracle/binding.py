@@ -64,6 +64,8 @@ def lib():
         L.ge_oracle_trace_closest.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32]
         L.ge_oracle_trace_any.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32]
         L.ge_oracle_ray_visibility.argtypes = [vp, C.POINTER(abi.RayVisibilityParams), vp, C.c_int32]
+        L.ge_oracle_shadow_resolve.restype = C.c_int32
+        L.ge_oracle_shadow_resolve.argtypes = [vp, C.POINTER(abi.ShadowResolveParams), vp, vp, vp, C.c_int32]
         L.ge_oracle_path_trace.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, C.c_uint32,
                                            C.c_uint32, C.c_uint32, C.c_int32]
         L.ge_oracle_path_trace64.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, C.c_uint32,
@@ -161,6 +163,17 @@ class Scene:
         lib().ge_oracle_path_trace64(self.h, C.byref(params), accum.ctypes.data, coverage.ctypes.data,
                                      r0, r1, threads)
         return accum, coverage
+
+    def shadow_resolve(self, params, words, threads=0):
+        """(out, rays): the entries' answers and their rays (origin xyz, t_max; t_max -1: none)."""
+        n = abi.shadow_entries(params)
+        out = np.zeros(n, dtype=np.uint32)
+        rays = np.zeros((n, 4), dtype=np.float32)
+        st = lib().ge_oracle_shadow_resolve(self.h, C.byref(params), words.ctypes.data, out.ctypes.data,
+                                            rays.ctypes.data, threads)
+        if st != 0:
+            raise ValueError(f"oracle shadow resolve: status {st}")
+        return out, rays
 
     def shadow_pass(self, params, receivers, threads=0):
         bits = np.zeros((params.count + 31) // 32, dtype=np.uint32)

@@ -12,6 +12,7 @@
 #include "ge_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -334,6 +335,7 @@ struct ge_oracle_scene {
   std::vector<InstRec> inst;
   Lbvh tlas;
   std::vector<uint32_t> tlas_slots;  // sorted slot -> instance id
+  ge_frame_eye eye{};                // the frame the instance structure was built in
 };
 
 namespace {
@@ -911,19 +913,16 @@ template <class R> struct Surf {
   V3<R> position, normal, geometric;
 };
 
+// fetch_triangle (material.slang:244-266): the corners in frame space, and the vertex indices.
 template <class R>
-Surf<R> surface_of_hit(const ge_oracle_scene& s, const HitRec& hit, V3<R> towards, const ge_frame_eye& eye,
-                       const ge_instance_desc** inst_out, uint32_t* cluster_out) {
-  const ge_instance_desc& in = s.instances[hit.inst];
+void corners_of(const ge_oracle_scene& s, const ge_instance_desc& in, const ge_frame_eye& eye, uint32_t cluster_index,
+                uint32_t triangle, V3<R> p[3], uint32_t local[3]) {
   const ge_mesh_desc& mesh = s.meshes[in.mesh];
-  const uint32_t cluster_index = mesh.first_cluster + (hit.prim >> 8);
   const ge_cluster_desc& cl = s.clusters[cluster_index];
-  const uint32_t packed = s.triangles[cl.triangle_offset + (hit.prim & 0xffu)];
+  const uint32_t packed = s.triangles[cl.triangle_offset + triangle];
   float Tf[3];
   instance_from_eye(in, eye, Tf);
   const V3<R> T = v3(R(Tf[0]), R(Tf[1]), R(Tf[2]));
-  V3<R> p[3];
-  uint32_t local[3];
   for (int k = 0; k < 3; ++k) {
     local[k] = cl.vertex_offset + ((packed >> (8 * k)) & 0xffu);
     const V3<R> m = load_quantized_position<R>(s, mesh, local[k]);
@@ -933,8 +932,12 @@ Surf<R> surface_of_hit(const ge_oracle_scene& s, const HitRec& hit, V3<R> toward
     d.z = R(in.rows[2][0]) * m.x + R(in.rows[2][1]) * m.y + R(in.rows[2][2]) * m.z;
     p[k] = d + T;
   }
-  const R bu = R(hit.u), bv = R(hit.v);
-  const V3<R> b = v3(R(1) - bu - bv, bu, bv);
+}
+
+// surface_at (material.slang:270-302).
+template <class R>
+Surf<R> surface_at(const ge_oracle_scene& s, const ge_instance_desc& in, const V3<R> p[3], const uint32_t local[3],
+                   V3<R> b, V3<R> towards) {
   Surf<R> out;
   out.position = b.x * p[0] + b.y * p[1] + b.z * p[2];
   out.geometric = normalize(cross(p[1] - p[0], p[2] - p[0]));
@@ -948,9 +951,45 @@ Surf<R> surface_of_hit(const ge_oracle_scene& s, const HitRec& hit, V3<R> toward
   } else {
     out.normal = out.geometric;
   }
+  return out;
+}
+
+template <class R>
+Surf<R> surface_of_hit(const ge_oracle_scene& s, const HitRec& hit, V3<R> towards, const ge_frame_eye& eye,
+                       const ge_instance_desc** inst_out, uint32_t* cluster_out) {
+  const ge_instance_desc& in = s.instances[hit.inst];
+  const uint32_t cluster_index = s.meshes[in.mesh].first_cluster + (hit.prim >> 8);
+  V3<R> p[3];
+  uint32_t local[3];
+  corners_of<R>(s, in, eye, cluster_index, hit.prim & 0xffu, p, local);
+  const R bu = R(hit.u), bv = R(hit.v);
+  const V3<R> b = v3(R(1) - bu - bv, bu, bv);
   *inst_out = &in;
   *cluster_out = cluster_index;
-  return out;
+  return surface_at<R>(s, in, p, local, b, towards);
+}
+
+// screen_barycentrics (material.slang:386-416), the barycentrics alone, in float: where the ray
+// through `pixel` meets the triangle's plane.
+V3<float> screen_barycentrics(const V3<float> p[3], float px, float py, const float* vp, V3<float> eye, uint32_t width,
+                              uint32_t height) {
+  V3<float> c[3];
+  for (int i = 0; i < 3; ++i) {
+    const V3<float> v = p[i] - eye;
+    c[i].x = ((vp[0] * v.x + vp[4] * v.y) + vp[8] * v.z) + vp[12] * 0.0f;
+    c[i].y = ((vp[1] * v.x + vp[5] * v.y) + vp[9] * v.z) + vp[13] * 0.0f;
+    c[i].z = ((vp[3] * v.x + vp[7] * v.y) + vp[11] * v.z) + vp[15] * 0.0f;
+  }
+  const float ndx = 2.0f / float(width), ndy = -2.0f / float(height);
+  const V3<float> q = v3(px * ndx + -1.0f, py * ndy + 1.0f, 1.0f);
+  const V3<float> n0 = cross(c[1], c[2]), n1 = cross(c[2], c[0]), n2 = cross(c[0], c[1]);
+  const V3<float> e = v3(dot(n0, q), dot(n1, q), dot(n2, q));
+  const float plane = e.x + e.y + e.z;
+  const V3<float> sgn = plane < 0.0f ? -e : e;
+  const V3<float> inside = maxv(sgn, 0.0f);
+  const float total = inside.x + inside.y + inside.z;
+  if (!(total > 0.0f)) return v3(1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 3.0f);
+  return v3(inside.x / total, inside.y / total, inside.z / total);
 }
 
 // ---- the integrator (path_trace.slang:166-495) ----------------------------------------------------
@@ -1182,7 +1221,8 @@ ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* d, const ge_frame_e
   s->blas.resize(d->mesh_count);
   for (uint32_t m = 0; m < d->mesh_count; ++m) build_blas(*s, m);
   const ge_frame_eye zero{};
-  build_tlas(*s, eye ? *eye : zero);
+  s->eye = eye ? *eye : zero;
+  build_tlas(*s, s->eye);
   return s;
 }
 
@@ -1215,7 +1255,8 @@ int32_t ge_oracle_update_instances(ge_oracle_scene* s, uint32_t first, uint32_t 
   if (first + count > s->instances.size()) return GE_RT_ERROR_INVALID_ARGUMENT;
   std::memcpy(&s->instances[first], in, size_t(count) * sizeof(ge_instance_desc));
   const ge_frame_eye zero{};
-  build_tlas(*s, eye ? *eye : zero);
+  s->eye = eye ? *eye : zero;
+  build_tlas(*s, s->eye);
   return GE_RT_OK;
 }
 
@@ -1445,6 +1486,115 @@ void ge_oracle_shadow_pass(const ge_oracle_scene* s, const ge_shadow_params* p, 
     }
     tally.flush();
   }
+}
+
+// The engine's pair numbering: instance i owns pairs [first_pair, first_pair + its mesh's clusters),
+// pair -> (instance, cluster within the mesh). Empty (~0) where no instance claims a pair; false
+// when two instances claim one.
+static bool pair_table(const ge_oracle_scene& s, std::vector<std::array<uint32_t, 2>>& pairs) {
+  uint64_t count = 0;
+  for (const ge_instance_desc& in : s.instances)
+    count = std::max<uint64_t>(count, uint64_t(in.first_pair) + s.meshes[in.mesh].cluster_count);
+  if (count > (1u << 24)) return false;
+  pairs.assign(size_t(count), {GE_RT_MISS, GE_RT_MISS});
+  for (uint32_t i = 0; i < s.instances.size(); ++i) {
+    const ge_instance_desc& in = s.instances[i];
+    for (uint32_t c = 0; c < s.meshes[in.mesh].cluster_count; ++c) {
+      auto& e = pairs[size_t(in.first_pair) + c];
+      if (e[0] != GE_RT_MISS) return false;
+      e = {i, c};
+    }
+  }
+  return true;
+}
+
+// One pixel's receiver (reconstruct_screen, material.slang:447-458; shadow_visibility,
+// visibility_resolve.slang:742-755): false when it asks no ray.
+static bool resolve_receiver(const ge_oracle_scene& s, const std::vector<std::array<uint32_t, 2>>& pairs,
+                             const ge_shadow_resolve_params& p, uint32_t x, uint32_t y, uint64_t word, uint32_t& pair,
+                             float ray[4]) {
+  if (word == 0) return false;
+  const uint32_t key = uint32_t(word);
+  pair = key >> 8;
+  if (pair >= pairs.size() || pairs[pair][0] == GE_RT_MISS) return false;
+  const ge_instance_desc& in = s.instances[pairs[pair][0]];
+  const uint32_t cluster_index = s.meshes[in.mesh].first_cluster + pairs[pair][1];
+  V3<float> c[3];
+  uint32_t local[3];
+  corners_of<float>(s, in, s.eye, cluster_index, key & 0xffu, c, local);
+  float px = float(x) + 0.5f, py = float(y) + 0.5f;
+  if (p.flags & GE_SHADOW_JITTER) {
+    uint32_t rng = rng_seed(x, y, 0u, p.seed);
+    px = float(x) + rng_next<float>(rng);
+    py = float(y) + rng_next<float>(rng);
+  }
+  const V3<float> cam = v3(p.camera[0], p.camera[1], p.camera[2]);
+  const V3<float> b = screen_barycentrics(c, px, py, p.view_proj, cam, p.width, p.height);
+  const Surf<float> sf = surface_at<float>(s, in, c, local, b, cam);
+  const V3<float> from_eye = b.x * (c[0] - cam) + b.y * (c[1] - cam) + b.z * (c[2] - cam);
+  const V3<float> position = cam + from_eye;
+  const V3<float> sun = v3(p.sun[0], p.sun[1], p.sun[2]);
+  if (dot(sf.normal, sun) <= 0.0f && dot(sf.geometric, sun) <= 0.0f) return false;
+  const float bias = p.shadow_bias + p.shadow_bias_steps * s.meshes[in.mesh].quant[3] * in.scale_max;
+  const V3<float> o = position + sf.geometric * bias;
+  ray[0] = o.x;
+  ray[1] = o.y;
+  ray[2] = o.z;
+  ray[3] = 1e30f;
+  return true;
+}
+
+int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resolve_params* p, const uint64_t* words,
+                                 uint32_t* out, float* rays, int32_t threads) {
+  std::vector<std::array<uint32_t, 2>> pairs;
+  if (!pair_table(*s, pairs)) return GE_RT_ERROR_INVALID_SCENE;
+  const bool half = (p->flags & GE_SHADOW_HALF_RATE) != 0;
+  const uint32_t W = p->width, H = p->height;
+  const uint32_t cols = half ? (W + 1) / 2 : W, rows = half ? (H + 1) / 2 : H;
+  const int nt = resolve_threads(threads);
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t by = 0; by < int64_t(rows); ++by) {
+      for (uint32_t bx = 0; bx < cols; ++bx) {
+        const size_t i = size_t(by) * cols + bx;
+        uint32_t pair = 0;
+        float ray[4] = {0.0f, 0.0f, 0.0f, -1.0f};
+        bool taken = false;
+        if (!half) {
+          taken = resolve_receiver(*s, pairs, *p, bx, uint32_t(by), words[i], pair, ray);
+        } else {
+          // half_sun_block (resolve_half_sun.slang:35-96): four non-empty words of one pair; the
+          // top-left pixel's ray answers for the block.
+          const uint32_t x = bx * 2, y = uint32_t(by) * 2;
+          if (x + 1 < W && y + 1 < H) {
+            const size_t row = size_t(y) * W + x;
+            const uint64_t v00 = words[row], v10 = words[row + 1], v01 = words[row + W], v11 = words[row + W + 1];
+            const uint32_t pr = uint32_t(v00) >> 8;
+            if (v00 && v10 && v01 && v11 && (uint32_t(v10) >> 8) == pr && (uint32_t(v01) >> 8) == pr &&
+                (uint32_t(v11) >> 8) == pr)
+              taken = resolve_receiver(*s, pairs, *p, x, y, v00, pair, ray);
+          }
+        }
+        uint32_t value = taken ? pair << 1 : GE_SHADOW_NOT_TAKEN;
+        if (taken) {
+          ++tally.any;
+          const float d[3] = {p->sun[0], p->sun[1], p->sun[2]};
+          const TRay r = make_ray(ray, d, 0.0f);
+          HitRec h;
+          if (traverse<true>(*s, r, ray[3], 0x01u, h, tally)) value |= 1u;
+        } else {
+          ray[3] = -1.0f;
+        }
+        out[i] = value;
+        if (rays)
+          for (int k = 0; k < 4; ++k) rays[i * 4 + k] = ray[k];
+      }
+    }
+    tally.flush();
+  }
+  return GE_RT_OK;
 }
 
 void ge_oracle_counters(uint64_t out[8]) {

@@ -62,6 +62,11 @@ void ge_oracle_tonemap(const ge_path_trace_params* params, const float* accum,
                        const uint32_t* coverage, uint32_t* output);
 void ge_oracle_shadow_pass(const ge_oracle_scene* scene, const ge_shadow_params* params,
                            const ge_shadow_receiver* receivers, uint32_t* blocked, int32_t threads);
+/* The sun shadow pass over a visibility buffer (ge_rt_shadow_resolve's contract); `rays`, when not
+ * null, receives each entry's ray origin and t_max (t_max -1: no ray), 4 floats an entry. */
+int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resolve_params* p, const uint64_t* words,
+                                 uint32_t* out, float* rays, int32_t threads);
+
 /* Traversal counters accumulated since the last reset: closest rays, any rays, then nodes,
  * triangles and instances of the closest-hit walks, then those of the any-hit walks. */
 void ge_oracle_counters(uint64_t out[8]);

@@ -260,3 +260,33 @@ def test_errors_are_reported(ctx, c1):
     with pytest.raises(rt.RtError) as e:
         unbuilt.trace_closest(random_rays(4, 0, 1, 0))
     assert e.value.status == 2
+
+
+@pytest.fixture(scope="module")
+def c3(ctx):
+    s = synth.Synth.config(3, 0, 48, 129)
+    g, o = _pair(ctx, s)
+    yield s, g, o
+    g.close()
+
+
+@pytest.mark.parametrize("flags", [0, abi.SHADOW_HALF_RATE, abi.SHADOW_JITTER],
+                         ids=["full", "half", "jitter"])
+def test_shadow_resolve(ctx, c1, c2, c3, flags):
+    """The visibility-buffer sun shadow pass: every entry's pair and blocked bit, and the counters,
+    equal to the oracle's."""
+    for name, (s, g, o) in (("c1", c1), ("c2", c2), ("c3", c3)):
+        pt, rv = s.frame(200, 120)
+        words = o.ray_visibility(rv)
+        p = synth.Synth.shadow_params(pt, rv, flags, 11)
+        ctx.reset_stats()
+        ob.reset_counters()
+        got = g.shadow_resolve(p, words, count=True)
+        want, _ = o.shadow_resolve(p, words)
+        taken = want != abi.SHADOW_NOT_TAKEN
+        print(f"{name}: {taken.sum()} rays, {(want[taken] & 1).sum()} blocked", flush=True)
+        assert taken.sum() > 0
+        assert np.array_equal(got, want), name
+        gs, os_ = ctx.stats(), ob.counters()
+        for k in ("any", "nodes", "triangles", "instances"):
+            assert gs[k] == os_[k], (name, k)

@@ -140,3 +140,113 @@ def test_primary_miss_is_sky_to_the_2_2():
     pt.flags = abi.PT_UNIFORM_SKY
     acc, _ = o.path_trace(pt)
     assert np.all(acc[:, :3] == sky * np.float32(2))
+
+
+def _visibility_frame(cfg, args, w, h):
+    s = synth.Synth.config(cfg, *args)
+    o = ob.Scene(s.desc())
+    pt, rv = s.frame(w, h)
+    return s, o, pt, rv, o.ray_visibility(rv)
+
+
+def test_shadow_resolve_rebuilds_the_pixel_point_and_traces_from_it():
+    """Each receiver is the pixel's own surface point (the camera ray's hit, rebuilt from the
+    visibility word's triangle at the pixel centre), moved off the surface by the ray offset; its
+    answer is an any-hit query along the sun from there (visibility_resolve.slang:742-755)."""
+    s, o, pt, rv, words = _visibility_frame(1, (0,), 160, 96)
+    p = synth.Synth.shadow_params(pt, rv)
+    out, rays = o.shadow_resolve(p, words)
+    taken = out != abi.SHADOW_NOT_TAKEN
+    assert 500 < taken.sum() < np.count_nonzero(words)  # some surfaces face away from the sun
+    assert np.all(~taken[words == 0])
+    assert np.array_equal(out[taken] >> 1, (words[taken] & 0xFFFFFFFF).astype(np.uint32) >> 8)
+    # The point: on the pixel's camera ray, near where that ray hits the triangle, moved off the
+    # surface by the offset along the triangle's normal turned to the camera.
+    cam = _camera_rays(rv)
+    hits = o.trace_closest(cam, mask=0xFE)
+    assert np.array_equal(hits["key"][taken], (words[taken] & 0xFFFFFFFF).astype(np.uint32))
+    t = hits["t"][taken].astype(np.float64)
+    d = cam["dir"][taken].astype(np.float64)
+    eye = cam["origin"][taken].astype(np.float64)
+    a = s.arrays()
+    inst = a["instances"][hits["instance_id"][taken]]
+    assert np.all(inst["cell"] == 0)
+    cl = a["clusters"][a["meshes"]["first_cluster"][inst["mesh"]] + hits["geometry"][taken]]
+    packed = a["triangles"][cl["triangle_offset"] + hits["primitive"][taken]]
+    corner = [a["vertices"][cl["vertex_offset"] + ((packed >> (8 * k)) & 0xFF)].astype(np.float64) for k in range(3)]
+    rows = inst["rows"].astype(np.float64)
+    world = [np.einsum("nij,nj->ni", rows[:, :, :3], c) + rows[:, :, 3] for c in corner]
+    n = np.cross(world[1] - world[0], world[2] - world[0])
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    n = np.where((np.sum(n * d, axis=1) > 0)[:, None], -n, n)  # towards the camera
+    bias = p.shadow_bias + p.shadow_bias_steps * a["meshes"]["quant"][inst["mesh"], 3] * inst["scale_max"]
+    point = rays[taken, :3] - n * bias[:, None]
+    along = np.sum((point - eye) * d, axis=1)
+    off_ray = np.linalg.norm(point - eye - d * along[:, None], axis=1)
+    # The rebuilt point lies on the mesh's 16-bit grid, the traced triangle on the float positions
+    # it was rounded from: at most sqrt(3)/2 grid steps apart (material.slang:304-318). Off the
+    # ray: the projection's rounding, or a grid step where the centre falls just outside the
+    # rounded triangle and its barycentrics are clamped to the edge.
+    grid = a["meshes"]["quant"][inst["mesh"], 3] * inst["scale_max"]
+    assert np.all(off_ray <= 1e-5 * t + grid)
+    assert np.mean(off_ray <= 1e-5 * t) > 0.95
+    cos = np.abs(np.sum(n * d, axis=1))
+    assert np.all(np.abs(along - t) <= 0.87 * grid / cos + 1e-5 * t)
+    # The answer: an any-hit ray along the sun from that origin, by the walk and by brute force.
+    shadow = np.zeros(int(taken.sum()), dtype=abi.RAY_DTYPE)
+    shadow["origin"] = rays[taken, :3]
+    shadow["dir"] = np.array(pt.sun[:3], dtype=np.float32)
+    shadow["tmax"] = rays[taken, 3]
+    blocked = (out[taken] & 1).astype(bool)
+    assert np.array_equal(blocked, unpack_bits(o.trace_any(shadow, mask=0x01), len(shadow)))
+    assert np.array_equal(blocked, unpack_bits(o.trace_any(shadow, mask=0x01, brute=True), len(shadow)))
+    assert 0.05 < blocked.mean() < 0.95
+
+
+def test_half_rate_blocks_take_the_top_left_answer():
+    """half_sun_block (resolve_half_sun.slang:35-96): a block of four non-empty words of one pair
+    is the top-left pixel's answer; any other block, and a block on the picture's odd edge, is not
+    taken."""
+    for cfg, args, w, h in ((1, (0,), 161, 97), (3, (0, 48, 129), 120, 80)):
+        s, o, pt, rv, words = _visibility_frame(cfg, args, w, h)
+        full, _ = o.shadow_resolve(synth.Synth.shadow_params(pt, rv), words)
+        half, _ = o.shadow_resolve(synth.Synth.shadow_params(pt, rv, abi.SHADOW_HALF_RATE), words)
+        cols, rows = (w + 1) // 2, (h + 1) // 2
+        assert len(half) == cols * rows
+        grid = words.reshape(h, w)
+        pairs = (grid & 0xFFFFFFFF) >> 8
+        taken_blocks = 0
+        for by in range(rows):
+            for bx in range(cols):
+                x, y = 2 * bx, 2 * by
+                v = half[by * cols + bx]
+                if x + 1 >= w or y + 1 >= h:
+                    assert v == abi.SHADOW_NOT_TAKEN
+                    continue
+                quad = grid[y:y + 2, x:x + 2]
+                one_pair = np.all(quad != 0) and np.all(pairs[y:y + 2, x:x + 2] == pairs[y, x])
+                if not one_pair:
+                    assert v == abi.SHADOW_NOT_TAKEN
+                else:
+                    assert v == full[y * w + x]
+                    taken_blocks += v != abi.SHADOW_NOT_TAKEN
+        assert taken_blocks > 30
+
+
+def test_jitter_moves_the_point_within_the_pixel():
+    s, o, pt, rv, words = _visibility_frame(1, (0,), 160, 96)
+    centre, rays0 = o.shadow_resolve(synth.Synth.shadow_params(pt, rv), words)
+    a, rays_a = o.shadow_resolve(synth.Synth.shadow_params(pt, rv, abi.SHADOW_JITTER, 3), words)
+    b, _ = o.shadow_resolve(synth.Synth.shadow_params(pt, rv, abi.SHADOW_JITTER, 3), words)
+    c, rays_c = o.shadow_resolve(synth.Synth.shadow_params(pt, rv, abi.SHADOW_JITTER, 4), words)
+    assert np.array_equal(a, b)  # seeded
+    both = (centre != abi.SHADOW_NOT_TAKEN) & (a != abi.SHADOW_NOT_TAKEN) & (c != abi.SHADOW_NOT_TAKEN)
+    assert both.sum() > 1000
+    assert np.array_equal(centre[both] >> 1, a[both] >> 1)  # same surfaces
+    # (a few points outside their small triangles clamp to the same edge or corner)
+    assert np.mean(np.any(rays_a[both, :3] != rays0[both, :3], axis=1)) > 0.9
+    assert np.mean(np.any(rays_a[both, :3] != rays_c[both, :3], axis=1)) > 0.9
+    # Sub-pixel moves: the points stay within a few pixels' footprint of the centre's.
+    span = np.linalg.norm(rays_a[both, :3] - rays0[both, :3], axis=1)
+    dist = np.linalg.norm(rays0[both, :3] - np.array(rv.camera[:3]), axis=1)
+    assert np.percentile(span / dist, 95) < 0.05
