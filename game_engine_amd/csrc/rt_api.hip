@@ -106,8 +106,7 @@ SceneView scene_view(const SceneDev& s) {
   return v;
 }
 
-int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_desc* in, uint32_t count,
-                           const uint32_t* cluster_materials) {
+int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_desc* in, uint32_t count) {
   for (uint32_t i = 0; i < count; ++i) {
     const ge_instance_desc& inst = in[i];
     if (inst.mesh >= s.mesh_count)
@@ -119,12 +118,10 @@ int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_
                        "the backend traces every instance through its mesh's own structure");
     if (inst.deform != GE_INVALID_DEFORM)
       return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, "instance " + std::to_string(i) + " is deformed");
-    const ge_mesh_desc& m = s.h_meshes[inst.mesh];
-    for (uint32_t c = m.first_cluster; c < m.first_cluster + m.cluster_count; ++c) {
-      if (uint64_t(cluster_materials[c]) + inst.material_base >= s.material_count)
-        return set_error(ctx, GE_RT_ERROR_INVALID_SCENE,
-                         "instance " + std::to_string(i) + " cluster " + std::to_string(c) + " has no material");
-    }
+    if (s.h_meshes[inst.mesh].cluster_count &&
+        uint64_t(s.mesh_max_material[inst.mesh]) + inst.material_base >= s.material_count)
+      return set_error(ctx, GE_RT_ERROR_INVALID_SCENE,
+                       "instance " + std::to_string(i) + " reaches past the material array");
   }
   return GE_RT_OK;
 }
@@ -167,9 +164,11 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->build.keys[0], &ctx->build.keys[1], &ctx->build.vals[0], &ctx->build.vals[1], &ctx->build.hist,
-                    &ctx->build.scan_tmp, &ctx->build.flags, &ctx->build.node_box, &ctx->build.leaf_parent,
-                    &ctx->build.node_parent, &ctx->build.seg_cbox, &ctx->build.seg_scale, &ctx->counters})
+                    &ctx->build.scan_tmp, &ctx->build.node_box, &ctx->build.node_parent, &ctx->build.seg_cbox,
+                    &ctx->build.seg_scale, &ctx->build.anc[0], &ctx->build.anc[1], &ctx->build.dist[0],
+                    &ctx->build.dist[1], &ctx->build.levels, &ctx->counters})
     b->release();
+  if (ctx->build.host_levels) (void)hipHostFree(ctx->build.host_levels);
   for (DevBuf& b : ctx->io) b.release();
   ctx->resolve_rays.release();
   for (DevBuf& b : ctx->tmp) b.release();
@@ -281,7 +280,14 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
     if ((mat.flags >> 24) != 0u) return fail(GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(k) + " declares foliage optics");
   }
   {
-    const int32_t st = validate_instances(ctx, s, d->instances, d->instance_count, d->cluster_materials);
+    // The largest cluster material of each mesh: an instance's materials are that plus its base.
+    s.mesh_max_material.assign(d->mesh_count, 0u);
+    for (uint32_t m = 0; m < d->mesh_count; ++m) {
+      const ge_mesh_desc& mesh = d->meshes[m];
+      for (uint32_t c = mesh.first_cluster; c < mesh.first_cluster + mesh.cluster_count; ++c)
+        s.mesh_max_material[m] = std::max(s.mesh_max_material[m], d->cluster_materials[c]);
+    }
+    const int32_t st = validate_instances(ctx, s, d->instances, d->instance_count);
     if (st != GE_RT_OK) {
       const std::string msg = ctx->error;
       return fail(st, msg);
@@ -332,10 +338,8 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_CHECK(ctx, scene && (instances || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or instances");
   SceneDev& s = scene->dev;
   GE_CHECK(ctx, uint64_t(first) + count <= s.instance_count, GE_RT_ERROR_INVALID_ARGUMENT, "instance range out of bounds");
-  std::vector<uint32_t> cm(s.cluster_count);
   GE_HIP(ctx, hipSetDevice(ctx->device));
-  GE_HIP(ctx, hipMemcpy(cm.data(), s.cluster_materials.ptr, size_t(s.cluster_count) * 4, hipMemcpyDeviceToHost));
-  const int32_t st = validate_instances(ctx, s, instances, count, cm.data());
+  const int32_t st = validate_instances(ctx, s, instances, count);
   if (st != GE_RT_OK) return st;
   std::memcpy(&s.h_instances[first], instances, size_t(count) * sizeof(ge_instance_desc));
   s.pairs_dirty = true;
@@ -647,7 +651,8 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   if (!dev && (r = stage(ctx, 0, words, n, false, &out)) != GE_RT_OK) return r;
   GE_HIP(ctx, begin_timing(ctx));
   GE_HIP(ctx, launch_ray_visibility(ctx->stream, bvh_view(scene->dev), *params, out,
-                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
+                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(words, out, n * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -817,7 +822,7 @@ int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   }
   GE_HIP(ctx, ctx->resolve_rays.reserve(size_t(entries) * 16));
   GE_HIP(ctx, begin_timing(ctx));
-  GE_HIP(ctx, launch_shadow_resolve(ctx->stream, bvh_view(s), scene_view(s), s.eye, *params, dw,
+  GE_HIP(ctx, launch_shadow_resolve(ctx, bvh_view(s), scene_view(s), s.eye, *params, dw,
                                     s.pairs.as<uint2>(), s.pair_count, ctx->resolve_rays.as<float4>(), dout,
                                     (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
                                     ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));

@@ -3,10 +3,13 @@
 // Per primitive: a box and the 30-bit Morton code of its centroid, quantized on its segment's
 // centroid bounds (10 bits an axis). A stable LSD radix sort on (segment << 30 | code) orders each
 // segment by code with ties in primitive order. Karras 2012 emits each segment's binary radix tree
-// from the sorted codes (one thread per internal node, the index breaking equal codes), and a
-// bottom-up pass fits the boxes (the second child to finish computes its parent; min and max are
-// exact, so the result does not depend on which one that is). Everything is integer work or exact
-// float min/max, so the structure is bit-identical to the oracle's sequential build.
+// from the sorted codes (one thread per internal node, the index breaking equal codes). The boxes
+// are fitted level by level from the deepest: each node's depth comes from pointer jumping over the
+// parent links, one pass of the radix sort groups the nodes by depth, and one launch per level
+// unions each node's two children, whose boxes the previous launch wrote. The kernel boundary is the
+// only synchronization: no per-node atomics or device-scope fences, which on a part whose L2 is
+// split over eight XCDs cost a cache write-back each. Everything is integer work or exact float
+// min/max, so the structure is bit-identical to the oracle's sequential build.
 //
 // Memory: keys 8 B + values 4 B per primitive, double-buffered, a 256 x tiles histogram, and the
 // nodes themselves (64 B each). Bytes per triangle of the whole build are stated beside the
@@ -53,42 +56,69 @@ __global__ void k_init_bounds(uint32_t nseg, uint32_t* cbox) {
   }
 }
 
-// Each wave reduces its primitives' centroid bounds with cross-lane min/max when all of its lanes
-// are in one segment (the common case: segments are contiguous), so a segment of n primitives
-// costs n / 64 atomics per bound instead of n.
-__global__ void k_centroid_bounds(uint32_t n, const float* __restrict__ boxes, const uint32_t* __restrict__ seg_of,
-                                  uint32_t* cbox) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  const uint32_t s = valid ? seg_of[i] : 0xffffffffu;
-  uint32_t lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) {
-    const float c = valid ? (boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f : 0.0f;
-    lo[a] = valid ? ordered(c) : 0xffffffffu;
-    hi[a] = valid ? ordered(c) : 0u;
-  }
-  const uint32_t s0 = __shfl(s, 0);
-  const bool uniform = __all(!valid || s == s0);
-  if (uniform) {
+// Centroid bounds per segment. Each wave takes a contiguous run of kBoundsSteps x 64 primitives and
+// keeps per-lane running bounds while its lanes stay in one segment (segments are contiguous), so a
+// segment costs one wave reduction and six atomics per run instead of per primitive: same-address
+// atomics serialize, and one segment can hold tens of millions of primitives.
+constexpr uint32_t kBoundsSteps = 64;
+
+__global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, const float* __restrict__ boxes,
+                                                         const uint32_t* __restrict__ seg_of, uint32_t* cbox) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64u * kBoundsSteps;
+  uint32_t cur = 0xffffffffu;  // the segment the running bounds belong to (the same in every lane)
+  uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+  auto flush = [&]() {
     for (int a = 0; a < 3; ++a) {
       for (int off = 32; off > 0; off >>= 1) {
         lo[a] = min(lo[a], uint32_t(__shfl_xor(int(lo[a]), off)));
         hi[a] = max(hi[a], uint32_t(__shfl_xor(int(hi[a]), off)));
       }
     }
-    if ((threadIdx.x & 63u) == 0 && s0 != 0xffffffffu) {
+    if (lane == 0 && cur != 0xffffffffu) {
       for (int a = 0; a < 3; ++a) {
-        atomicMin(&cbox[s0 * 6 + a], lo[a]);
-        atomicMax(&cbox[s0 * 6 + 3 + a], hi[a]);
+        atomicMin(&cbox[cur * 6 + a], lo[a]);
+        atomicMax(&cbox[cur * 6 + 3 + a], hi[a]);
       }
     }
-    return;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = 0xffffffffu;
+      hi[a] = 0u;
+    }
+  };
+  for (uint32_t k = 0; k < kBoundsSteps; ++k) {
+    const uint32_t i = base + k * 64u + lane;
+    const bool valid = i < n;
+    if (!__any(valid)) break;
+    const uint32_t s = valid ? seg_of[i] : 0xffffffffu;
+    uint32_t c[3];
+    for (int a = 0; a < 3; ++a)
+      c[a] = valid ? ordered((boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f) : 0u;
+    const uint32_t s0 = uint32_t(__shfl(int(s), 0));
+    if (__all(!valid || s == s0)) {
+      if (s0 != cur) {
+        flush();
+        cur = s0;
+      }
+      if (valid) {
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = min(lo[a], c[a]);
+          hi[a] = max(hi[a], c[a]);
+        }
+      }
+    } else {
+      // A segment boundary inside this step: close the run and add these lanes one by one.
+      flush();
+      cur = 0xffffffffu;
+      if (valid) {
+        for (int a = 0; a < 3; ++a) {
+          atomicMin(&cbox[s * 6 + a], c[a]);
+          atomicMax(&cbox[s * 6 + 3 + a], c[a]);
+        }
+      }
+    }
   }
-  if (!valid) return;
-  for (int a = 0; a < 3; ++a) {
-    atomicMin(&cbox[s * 6 + a], lo[a]);
-    atomicMax(&cbox[s * 6 + 3 + a], hi[a]);
-  }
+  flush();
 }
 
 __global__ void k_segment_scale(uint32_t nseg, const uint32_t* cbox, float* lo_scale) {
@@ -133,6 +163,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* __re
   hist[size_t(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+template <bool Keys>
 __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* __restrict__ kin,
                                                                const uint32_t* __restrict__ vin, uint64_t* kout,
                                                                uint32_t* vout, uint32_t n, int shift,
@@ -163,7 +194,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* _
     if (valid) {
       uint32_t pos = running[d] + below;
       for (uint32_t w = 0; w < wave; ++w) pos += wave_counts[w][d];
-      kout[pos] = key;
+      if (Keys) kout[pos] = key;
       vout[pos] = vin[i];
     }
     __syncthreads();
@@ -238,8 +269,7 @@ __device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, int n, i
 
 __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_keys, const uint32_t* __restrict__ seg_of,
                          const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
-                         const uint32_t* __restrict__ node_base, ge_bvh_node* nodes, uint32_t* leaf_parent,
-                         uint32_t* node_parent, uint32_t* flags) {
+                         const uint32_t* __restrict__ node_base, ge_bvh_node* nodes, uint32_t* node_parent) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_total) return;
   const uint32_t s = seg_of[g];
@@ -248,7 +278,6 @@ __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_k
   const int i = int(g - first_g);
   const uint64_t* keys = sorted_keys + first_g;
   ge_bvh_node* seg_nodes = nodes + node_base[s];
-  uint32_t* seg_flags = flags + node_base[s];
   if (n == 1) {
     ge_bvh_node node;
     node.child[0] = GE_BVH_LEAF;
@@ -261,12 +290,9 @@ __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_k
       node.hi1[a] = __uint_as_float(0xff800000u);
     }
     seg_nodes[0] = node;
-    leaf_parent[g] = 0;
-    seg_flags[0] = 1;  // one child only: the leaf's arrival completes it
     return;
   }
   if (i >= n - 1) return;
-  seg_flags[i] = 0;
   const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) > 0 ? 1 : -1;
   const int dmin = delta(keys, n, i, i - d);
   int lmax = 2;
@@ -291,8 +317,77 @@ __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_k
   node.child[1] = c1;
   node.count = uint32_t(hi - lo + 1);
   if (i == 0) node.parent = GE_BVH_NONE;
-  if (c0 & GE_BVH_LEAF) leaf_parent[first_g + gamma] = uint32_t(i); else node_parent[node_base[s] + gamma] = uint32_t(i);
-  if (c1 & GE_BVH_LEAF) leaf_parent[first_g + gamma + 1] = uint32_t(i); else node_parent[node_base[s] + gamma + 1] = uint32_t(i);
+  if (!(c0 & GE_BVH_LEAF)) node_parent[node_base[s] + gamma] = uint32_t(i);
+  if (!(c1 & GE_BVH_LEAF)) node_parent[node_base[s] + gamma + 1] = uint32_t(i);
+}
+
+
+// ---- the fit, level by level ----------------------------------------------------------------
+// A node is named by the batch index of its segment's primitive of the same index: internal node i
+// of segment s is batch entry seg_first[s] + i (i < n - 1, or i = 0 for a one-primitive segment),
+// so a batch of n primitives has at most n nodes and every array below is n long.
+constexpr uint32_t kNoNode = 0xffffffffu;
+constexpr uint32_t kNoDepth = 0xffffffffu;  // the distance of a batch entry that names no node
+constexpr uint32_t kNotANode = 255u;        // its depth key
+constexpr int kJumps = 8;                   // pointer-jumping rounds: depths up to 2^8 - 1
+
+__global__ void k_depth_init(uint32_t n_total, const uint32_t* __restrict__ seg_of,
+                             const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
+                             const uint32_t* __restrict__ node_base, const uint32_t* __restrict__ node_parent,
+                             uint32_t* anc, uint32_t* dist) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_total) return;
+  const uint32_t s = seg_of[g];
+  const uint32_t first_g = seg_first[s], n = seg_count[s];
+  const uint32_t i = g - first_g;
+  const bool node = n == 1 ? i == 0 : i < n - 1;
+  uint32_t a = kNoNode;
+  if (node && i != 0) a = first_g + node_parent[node_base[s] + i];
+  anc[g] = a;
+  dist[g] = node ? (a == kNoNode ? 0u : 1u) : kNoDepth;
+}
+
+// One round of pointer jumping: the distance to the ancestor doubles its reach.
+__global__ void k_depth_jump(uint32_t n_total, const uint32_t* __restrict__ anc, const uint32_t* __restrict__ dist,
+                             uint32_t* anc_out, uint32_t* dist_out) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_total) return;
+  const uint32_t a = anc[g];
+  uint32_t d = dist[g];
+  uint32_t up = kNoNode;
+  if (a != kNoNode) {
+    d += dist[a];
+    up = anc[a];
+  }
+  anc_out[g] = up;
+  dist_out[g] = d;
+}
+
+// The sort keys: a node's depth (kNotANode for the rest); a node short of its root after kJumps
+// rounds, or as deep as kNotANode, is deeper than any structure the traversal accepts and raises
+// the error flag.
+__global__ void k_depth_keys(uint32_t n_total, const uint32_t* __restrict__ anc, const uint32_t* __restrict__ dist,
+                             uint64_t* keys, uint32_t* vals, uint32_t* error) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_total) return;
+  uint32_t d = dist[g];
+  if (d == kNoDepth) {
+    d = kNotANode;
+  } else if (anc[g] != kNoNode || d >= kNotANode) {
+    atomicOr(error, 1u);
+    d = kNotANode - 1;
+  }
+  keys[g] = d;
+  vals[g] = g;
+}
+
+// Each level's first position in the depth-sorted order: the exclusive-scanned histogram's entry
+// for (digit d, tile 0). `out` gets 256 starts, then the error flag.
+__global__ void k_level_starts(const uint32_t* __restrict__ scanned, uint32_t tiles, const uint32_t* error,
+                               uint32_t* out) {
+  const uint32_t d = threadIdx.x;
+  out[d] = scanned[size_t(d) * tiles];
+  if (d == 0) out[256] = *error;
 }
 
 __device__ __forceinline__ void leaf_box(const float* boxes, const uint32_t* perm, uint32_t g, float lo[3], float hi[3]) {
@@ -303,79 +398,71 @@ __device__ __forceinline__ void leaf_box(const float* boxes, const uint32_t* per
   }
 }
 
-// Bottom-up fit: each leaf walks up; at each parent the first arrival stops and the second, which
-// sees both children finished, writes the parent's child boxes and its own box and height.
-__global__ void k_fit(uint32_t n_total, const float* __restrict__ boxes, const uint32_t* __restrict__ perm,
-                      const uint32_t* __restrict__ seg_of, const uint32_t* __restrict__ seg_first,
-                      const uint32_t* __restrict__ seg_count, const uint32_t* __restrict__ node_base,
-                      const uint32_t* __restrict__ root_index, ge_bvh_node* nodes, const uint32_t* leaf_parent,
-                      const uint32_t* node_parent, uint32_t* flags, float* node_box, float* roots, uint32_t* heights) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_total) return;
+// One level of the fit: each node unions its children (a leaf's box, or the box and height the
+// previous launch wrote for an internal child) into its record, its box and its height.
+__global__ void k_fit_level(const uint32_t* __restrict__ level, uint32_t count, const float* __restrict__ boxes,
+                            const uint32_t* __restrict__ perm, const uint32_t* __restrict__ seg_of,
+                            const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
+                            const uint32_t* __restrict__ node_base, const uint32_t* __restrict__ root_index,
+                            ge_bvh_node* nodes, const uint32_t* __restrict__ node_parent, float* node_box,
+                            float* roots, uint32_t* heights) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t g = level[k];
   const uint32_t s = seg_of[g];
   const uint32_t first_g = seg_first[s];
-  const uint32_t n = seg_count[s];
   const uint32_t nb = node_base[s];
-  ge_bvh_node* seg_nodes = nodes + nb;
+  const uint32_t i = g - first_g;
   const uint32_t out = root_index ? root_index[s] : s;
-  if (n == 1) {
+  ge_bvh_node& node = nodes[nb + i];
+  if (seg_count[s] == 1) {
     float lo[3], hi[3];
-    leaf_box(boxes, perm, g, lo, hi);
+    leaf_box(boxes, perm, first_g, lo, hi);
     for (int a = 0; a < 3; ++a) {
-      seg_nodes[0].lo0[a] = lo[a];
-      seg_nodes[0].hi0[a] = hi[a];
+      node.lo0[a] = lo[a];
+      node.hi0[a] = hi[a];
       roots[out * 6 + a] = lo[a];
       roots[out * 6 + 3 + a] = hi[a];
     }
     heights[out] = 1;
     return;
   }
-  uint32_t p = leaf_parent[g];
-  for (;;) {
-    __threadfence();
-    const uint32_t arrived = atomicAdd(&flags[nb + p], 1u);
-    if (arrived == 0) return;
-    __threadfence();
-    ge_bvh_node& node = seg_nodes[p];
-    const volatile ge_bvh_node& vnode = seg_nodes[p];
-    float clo[2][3], chi[2][3];
-    uint32_t ch_height[2];
-    for (int c = 0; c < 2; ++c) {
-      const uint32_t ch = vnode.child[c];
-      if (ch & GE_BVH_LEAF) {
-        leaf_box(boxes, perm, first_g + (ch & ~GE_BVH_LEAF), clo[c], chi[c]);
-        ch_height[c] = 0;
-      } else {
-        const volatile float* b = node_box + size_t(nb + ch) * 8;
-        for (int a = 0; a < 3; ++a) {
-          clo[c][a] = b[a];
-          chi[c][a] = b[3 + a];
-        }
-        ch_height[c] = __float_as_uint(b[6]);
+  float clo[2][3], chi[2][3];
+  uint32_t ch_height[2];
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t ch = node.child[c];
+    if (ch & GE_BVH_LEAF) {
+      leaf_box(boxes, perm, first_g + (ch & ~GE_BVH_LEAF), clo[c], chi[c]);
+      ch_height[c] = 0;
+    } else {
+      const float* b = node_box + size_t(nb + ch) * 8;
+      for (int a = 0; a < 3; ++a) {
+        clo[c][a] = b[a];
+        chi[c][a] = b[3 + a];
       }
+      ch_height[c] = __float_as_uint(b[6]);
     }
-    for (int a = 0; a < 3; ++a) {
-      node.lo0[a] = clo[0][a];
-      node.hi0[a] = chi[0][a];
-      node.lo1[a] = clo[1][a];
-      node.hi1[a] = chi[1][a];
-    }
-    const uint32_t h = 1 + (ch_height[0] > ch_height[1] ? ch_height[0] : ch_height[1]);
-    float* b = node_box + size_t(nb + p) * 8;
-    for (int a = 0; a < 3; ++a) {
-      b[a] = __builtin_fminf(clo[0][a], clo[1][a]);
-      b[3 + a] = __builtin_fmaxf(chi[0][a], chi[1][a]);
-    }
-    b[6] = __uint_as_float(h);
-    if (p == 0) {
-      node.parent = GE_BVH_NONE;
-      for (int a = 0; a < 6; ++a) roots[out * 6 + a] = b[a];
-      heights[out] = h;
-      return;
-    }
-    node.parent = node_parent[nb + p];
-    p = node_parent[nb + p];
   }
+  for (int a = 0; a < 3; ++a) {
+    node.lo0[a] = clo[0][a];
+    node.hi0[a] = chi[0][a];
+    node.lo1[a] = clo[1][a];
+    node.hi1[a] = chi[1][a];
+  }
+  const uint32_t h = 1 + (ch_height[0] > ch_height[1] ? ch_height[0] : ch_height[1]);
+  float* b = node_box + size_t(nb + i) * 8;
+  for (int a = 0; a < 3; ++a) {
+    b[a] = __builtin_fminf(clo[0][a], clo[1][a]);
+    b[3 + a] = __builtin_fmaxf(chi[0][a], chi[1][a]);
+  }
+  b[6] = __uint_as_float(h);
+  if (i == 0) {
+    node.parent = GE_BVH_NONE;
+    for (int a = 0; a < 6; ++a) roots[out * 6 + a] = b[a];
+    heights[out] = h;
+    return;
+  }
+  node.parent = node_parent[nb + i];
 }
 
 __global__ void k_triangle_records(const ge_cluster_desc* __restrict__ clusters, const float* __restrict__ vertices,
@@ -481,10 +568,14 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   if ((e = sc.vals[1].reserve(size_t(n) * 4)) != hipSuccess) return e;
   if ((e = sc.hist.reserve(size_t(tiles) * 256 * 4)) != hipSuccess) return e;
   if ((e = sc.scan_tmp.reserve(size_t(tiles) * 256 * 4 / 256 + 65536)) != hipSuccess) return e;
-  if ((e = sc.flags.reserve(size_t(node_pool) * 4)) != hipSuccess) return e;
   if ((e = sc.node_box.reserve(size_t(node_pool) * 32)) != hipSuccess) return e;
-  if ((e = sc.leaf_parent.reserve(size_t(n) * 4)) != hipSuccess) return e;
   if ((e = sc.node_parent.reserve(size_t(node_pool) * 4)) != hipSuccess) return e;
+  for (int k = 0; k < 2; ++k) {
+    if ((e = sc.anc[k].reserve(size_t(n) * 4)) != hipSuccess) return e;
+    if ((e = sc.dist[k].reserve(size_t(n) * 4)) != hipSuccess) return e;
+  }
+  if ((e = sc.levels.reserve(260 * 4)) != hipSuccess) return e;
+  if (!sc.host_levels && (e = hipHostMalloc(reinterpret_cast<void**>(&sc.host_levels), 260 * 4)) != hipSuccess) return e;
   if ((e = sc.seg_cbox.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
   if ((e = sc.seg_scale.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
 
@@ -503,7 +594,7 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
     k_sort_hist<<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), n, shift, sc.hist.as<uint32_t>(), tiles);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = exclusive_scan(sc.hist.as<uint32_t>(), tiles * 256, sc.scan_tmp, 0, stream)) != hipSuccess) return e;
-    k_sort_scatter<<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), sc.vals[cur].as<uint32_t>(),
+    k_sort_scatter<true><<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), sc.vals[cur].as<uint32_t>(),
                                                        sc.keys[cur ^ 1].as<uint64_t>(), sc.vals[cur ^ 1].as<uint32_t>(),
                                                        n, shift, sc.hist.as<uint32_t>(), tiles);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -512,14 +603,47 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   const uint64_t* sorted_keys = sc.keys[cur].as<uint64_t>();
   const uint32_t* sorted_vals = sc.vals[cur].as<uint32_t>();
   k_karras<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, d_seg_of, d_seg_first, d_seg_count, d_node_base, nodes,
-                                               sc.leaf_parent.as<uint32_t>(), sc.node_parent.as<uint32_t>(),
-                                               sc.flags.as<uint32_t>());
-  k_fit<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, sorted_vals, d_seg_of, d_seg_first, d_seg_count, d_node_base,
-                                            d_root_index, nodes, sc.leaf_parent.as<uint32_t>(),
-                                            sc.node_parent.as<uint32_t>(), sc.flags.as<uint32_t>(),
-                                            sc.node_box.as<float>(), roots, heights);
+                                               sc.node_parent.as<uint32_t>());
   k_codes<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, morton);
   if ((e = hipMemcpyAsync(perm, sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
+
+  // ---- the fit: depths by pointer jumping, nodes grouped by depth, one launch per level ----
+  uint32_t* error = sc.levels.as<uint32_t>() + 257;
+  if ((e = hipMemsetAsync(error, 0, 4, stream)) != hipSuccess) return e;
+  k_depth_init<<<blocks(n, 256), 256, 0, stream>>>(n, d_seg_of, d_seg_first, d_seg_count, d_node_base,
+                                                   sc.node_parent.as<uint32_t>(), sc.anc[0].as<uint32_t>(),
+                                                   sc.dist[0].as<uint32_t>());
+  int jc = 0;
+  for (int r = 0; r < kJumps; ++r, jc ^= 1)
+    k_depth_jump<<<blocks(n, 256), 256, 0, stream>>>(n, sc.anc[jc].as<uint32_t>(), sc.dist[jc].as<uint32_t>(),
+                                                     sc.anc[jc ^ 1].as<uint32_t>(), sc.dist[jc ^ 1].as<uint32_t>());
+  // The Morton keys and order stay in keys[cur] / vals[cur]; the depth sort uses the other pair
+  // and its own value buffer.
+  uint64_t* dkeys = sc.keys[cur ^ 1].as<uint64_t>();
+  uint32_t* dvals = sc.anc[jc ^ 1].as<uint32_t>();
+  uint32_t* dsorted = sc.vals[cur ^ 1].as<uint32_t>();
+  k_depth_keys<<<blocks(n, 256), 256, 0, stream>>>(n, sc.anc[jc].as<uint32_t>(), sc.dist[jc].as<uint32_t>(), dkeys,
+                                                   dvals, error);
+  k_sort_hist<<<tiles, kSortThreads, 0, stream>>>(dkeys, n, 0, sc.hist.as<uint32_t>(), tiles);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = exclusive_scan(sc.hist.as<uint32_t>(), tiles * 256, sc.scan_tmp, 0, stream)) != hipSuccess) return e;
+  // One 8-bit pass orders the nodes by depth (the sorted keys themselves are not kept).
+  k_sort_scatter<false><<<tiles, kSortThreads, 0, stream>>>(dkeys, dvals, nullptr, dsorted, n, 0,
+                                                            sc.hist.as<uint32_t>(), tiles);
+  k_level_starts<<<1, 256, 0, stream>>>(sc.hist.as<uint32_t>(), tiles, error, sc.levels.as<uint32_t>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(sc.host_levels, sc.levels.ptr, 257 * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+  if (sc.host_levels[256]) return hipErrorNotSupported;  // deeper than 2^8 - 2 levels
+  for (int d = int(kNotANode) - 1; d >= 0; --d) {
+    const uint32_t start = sc.host_levels[d];
+    const uint32_t end = sc.host_levels[d + 1];
+    if (end <= start) continue;
+    k_fit_level<<<blocks(end - start, 256), 256, 0, stream>>>(dsorted + start, end - start, boxes, sorted_vals,
+                                                              d_seg_of, d_seg_first, d_seg_count, d_node_base,
+                                                              d_root_index, nodes, sc.node_parent.as<uint32_t>(),
+                                                              sc.node_box.as<float>(), roots, heights);
+  }
   return hipGetLastError();
 }
 

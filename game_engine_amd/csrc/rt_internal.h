@@ -37,7 +37,10 @@ struct DevBuf {
 
 // Everything a segmented LBVH build needs besides its inputs. One per context, reused.
 struct BuildScratch {
-  DevBuf keys[2], vals[2], hist, scan_tmp, flags, node_box, leaf_parent, node_parent, seg_cbox, seg_scale;
+  DevBuf keys[2], vals[2], hist, scan_tmp, node_box, node_parent, seg_cbox, seg_scale;
+  DevBuf anc[2], dist[2];           // pointer jumping: each node's ancestor and distance to it
+  DevBuf levels;                    // 256 level starts, the depth error flag copy, the flag
+  uint32_t* host_levels = nullptr;  // pinned copy of `levels`
 };
 
 struct SceneDev {
@@ -47,6 +50,7 @@ struct SceneDev {
   std::vector<ge_cluster_desc> h_clusters;
   std::vector<ge_mesh_desc> h_meshes;
   std::vector<ge_instance_desc> h_instances;
+  std::vector<uint32_t> mesh_max_material;  // per mesh: its clusters' largest material index
   uint32_t cluster_count = 0, vertex_count = 0, triangle_count = 0, mesh_count = 0, instance_count = 0,
            material_count = 0, light_count = 0;
   bool has_attributes = false;
@@ -179,9 +183,13 @@ struct SceneView {
   uint32_t pad;
 };
 
+// Records the begin (end = 0) or end (end = 1) event of a traversal launch of `kind` (0 closest-hit,
+// 1 any-hit) on the context's stream, for ge_rt_stats; launches past the event pool go untimed.
+hipError_t stage_mark(ge_rt_ctx* ctx, int kind, int end);
+
 // The sun shadow pass over a visibility buffer (rt_shadow.hip): `rays` holds one 16-byte record an
-// entry, `out` the entries' answers.
-hipError_t launch_shadow_resolve(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_frame_eye& eye,
+// entry, `out` the entries' answers. The any-hit walk is timed as a stage of kind 1.
+hipError_t launch_shadow_resolve(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_frame_eye& eye,
                                  const ge_shadow_resolve_params& p, const uint64_t* words, const uint2* pairs,
                                  uint32_t pair_count, float4* rays, uint32_t* out, unsigned long long* counters,
                                  uint32_t* head, uint32_t max_blocks);
@@ -192,7 +200,7 @@ hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray
 hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
                             uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks);
 hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
-                                 unsigned long long* counters);
+                                 unsigned long long* counters, uint32_t* head, uint32_t max_blocks);
 hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
                              float4* accum, uint32_t* coverage, unsigned long long* counters);
 hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, const float4* accum,

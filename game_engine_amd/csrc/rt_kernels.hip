@@ -81,13 +81,12 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_any(Bvh bvh, const ge
 // (t, key) order already settles a depth tie to the larger pair << 8 | triangle, the tie scan's
 // answer), and the word asuint(depth) << 32 | key.
 template <bool Count>
-__global__ void __launch_bounds__(64) k_ray_visibility(Bvh bvh, ge_ray_visibility_params p, uint64_t* __restrict__ words,
-                                                       unsigned long long* counters) {
-  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
-  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
-  Tally tally;
-  uint64_t issued = 0;
-  if (x < p.width && y < p.height) {
+__global__ void __launch_bounds__(64) GE_WALK_ATTR k_ray_visibility(Bvh bvh, ge_ray_visibility_params p, uint32_t tiles_x,
+                                                                     uint32_t n, uint32_t* head,
+                                                                     uint64_t* __restrict__ words,
+                                                                     unsigned long long* counters) {
+  // Ray i is lane i % 64 of 8 x 8 tile i / 64 (tiles row-major), so a refill takes a tile's rays.
+  auto direction = [&](uint32_t x, uint32_t y) {
     const float ndx = (float(x) + 0.5f) / float(p.width) * 2.0f - 1.0f;
     const float ndy = 1.0f - (float(y) + 0.5f) / float(p.height) * 2.0f;
     const float* m = p.clip_to_ray;
@@ -97,13 +96,29 @@ __global__ void __launch_bounds__(64) k_ray_visibility(Bvh bvh, ge_ray_visibilit
     d.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
     const float len = __builtin_sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
     const float inv = 1.0f / len;
-    d = f3{d.x * inv, d.y * inv, d.z * inv};
-    const f3 o = f3{p.camera[0], p.camera[1], p.camera[2]};
-    const Ray ray = make_ray(o, d, 0.0f);
-    Hit h;
-    issued = 1;
+    return f3{d.x * inv, d.y * inv, d.z * inv};
+  };
+  Tally tally;
+  uint64_t issued = 0;
+  auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
+    const uint32_t tile = i >> 6, l = i & 63u;
+    const uint32_t x = (tile % tiles_x) * 8 + (l & 7u), y = (tile / tiles_x) * 8 + (l >> 3);
+    tmax = 1e30f;
+    mask = 0xfeu;
+    if (x >= p.width || y >= p.height) {
+      slot = GE_RT_MISS;  // past the picture's edge: no pixel
+      return false;
+    }
+    slot = y * p.width + x;
+    ray = make_ray(f3{p.camera[0], p.camera[1], p.camera[2]}, direction(x, y), 0.0f);
+    return true;
+  };
+  auto store = [&](uint32_t pixel, const Hit& h) {
+    if (pixel == GE_RT_MISS) return;
     uint64_t value = 0;
-    if (traverse<false, Count>(bvh, ray, 1e30f, 0xfeu, h, tally, lane_stack())) {
+    if (h.found) {
+      const f3 o = f3{p.camera[0], p.camera[1], p.camera[2]};
+      const f3 d = direction(pixel % p.width, pixel / p.width);
       const float pt[3] = {o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t};
       const float* vp = p.view_proj;
       const float cz = ((vp[2] * pt[0] + vp[6] * pt[1]) + vp[10] * pt[2]) + vp[14] * 1.0f;
@@ -111,8 +126,9 @@ __global__ void __launch_bounds__(64) k_ray_visibility(Bvh bvh, ge_ray_visibilit
       const float depth = cz / cw;
       value = (uint64_t(__float_as_uint(depth)) << 32) | uint64_t(h.key);
     }
-    words[size_t(y) * p.width + x] = value;
-  }
+    words[pixel] = value;
+  };
+  persistent_walk<false, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
   if (Count) flush_counters(counters, issued, 0, tally);
 }
 
@@ -401,10 +417,18 @@ hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* ra
 }
 
 hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
-                                 unsigned long long* counters) {
-  const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
-  if (counters) k_ray_visibility<true><<<grid, 64, lds(bvh), stream>>>(bvh, p, words, counters);
-  else k_ray_visibility<false><<<grid, 64, lds(bvh), stream>>>(bvh, p, words, counters);
+                                 unsigned long long* counters, uint32_t* head, uint32_t max_blocks) {
+  const uint32_t tiles_x = blocks(p.width, 8), tiles_y = blocks(p.height, 8);
+  const uint64_t n = uint64_t(tiles_x) * tiles_y * 64;
+  if (n == 0) return hipSuccess;
+  if (n >= (1ull << 32)) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(head, 0, 4, stream);
+  if (e != hipSuccess) return e;
+  const uint32_t grid = std::min(blocks(n, 64), max_blocks);
+  if (counters)
+    k_ray_visibility<true><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, uint32_t(n), head, words, counters);
+  else
+    k_ray_visibility<false><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, uint32_t(n), head, words, counters);
   return hipGetLastError();
 }
 

@@ -116,10 +116,11 @@ inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t
 
 }  // namespace
 
-hipError_t launch_shadow_resolve(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_frame_eye& eye,
+hipError_t launch_shadow_resolve(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_frame_eye& eye,
                                  const ge_shadow_resolve_params& p, const uint64_t* words, const uint2* pairs,
                                  uint32_t pair_count, float4* rays, uint32_t* out, unsigned long long* counters,
                                  uint32_t* head, uint32_t max_blocks) {
+  const hipStream_t stream = ctx->stream;
   const bool half = (p.flags & GE_SHADOW_HALF_RATE) != 0;
   ReceiverArgs a;
   a.sv = sv;
@@ -142,11 +143,13 @@ hipError_t launch_shadow_resolve(hipStream_t stream, const Bvh& bvh, const Scene
   if (e != hipSuccess) return e;
   const uint32_t grid = std::min(blocks(n, 64), max_blocks);
   const size_t lds = size_t(k_ring) * k_lanes * 4;
+  if ((e = stage_mark(ctx, 1, 0)) != hipSuccess) return e;
   if (counters)
     k_sun_walk<true><<<grid, 64, lds, stream>>>(bvh, rays, p.sun[0], p.sun[1], p.sun[2], a.n, head, out, counters);
   else
     k_sun_walk<false><<<grid, 64, lds, stream>>>(bvh, rays, p.sun[0], p.sun[1], p.sun[2], a.n, head, out, counters);
-  return hipGetLastError();
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return stage_mark(ctx, 1, 1);
 }
 
 }  // namespace ge

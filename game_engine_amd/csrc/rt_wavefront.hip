@@ -415,8 +415,9 @@ __global__ void __launch_bounds__(64) k_accumulate(ge_path_trace_params p, const
 inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
 constexpr size_t k_lds = size_t(k_ring) * k_lanes * 4;
 
-// Records the begin (end = 0) or end (end = 1) event of a traversal launch of `kind` (0 closest-hit,
-// 1 any-hit) on the context's stream; launches past the event pool go untimed.
+
+}  // namespace
+
 hipError_t stage_mark(ge_rt_ctx* ctx, int kind, int end) {
   const uint32_t i = ctx->stage_n[kind];
   if (i >= GE_RT_STAGE_TIMED) return hipSuccess;
@@ -424,8 +425,6 @@ hipError_t stage_mark(ge_rt_ctx* ctx, int kind, int end) {
   if (end) ctx->stage_n[kind] = i + 1;
   return e;
 }
-
-}  // namespace
 
 hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
                                 float4* accum, uint32_t* coverage, unsigned long long* counters) {

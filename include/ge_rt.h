@@ -300,9 +300,10 @@ typedef struct ge_rt_counters {
   uint64_t any_nodes;      /* the part of nodes, triangles and instances that any-hit queries made */
   uint64_t any_triangles;
   uint64_t any_instances;
-  /* The last wavefront path trace's traversal launches, timed with hipEvents on the ctx stream:
-   * the closest-hit walks (one a bounce) and the any-hit shadow walks, summed, and how many
-   * launches the sums cover (the first GE_RT_STAGE_TIMED of each kind are timed). */
+  /* The last dispatch's traversal launches, timed with hipEvents on the ctx stream: the wavefront
+   * path tracer's closest-hit walks (one a bounce) and any-hit shadow walks, or the shadow
+   * resolve's sun walk (an any-hit launch), summed, and how many launches the sums cover (the first
+   * GE_RT_STAGE_TIMED of each kind are timed). */
   double closest_ms;
   double any_ms;
   uint32_t closest_launches;

@@ -290,3 +290,32 @@ def test_shadow_resolve(ctx, c1, c2, c3, flags):
         gs, os_ = ctx.stats(), ob.counters()
         for k in ("any", "nodes", "triangles", "instances"):
             assert gs[k] == os_[k], (name, k)
+
+
+def test_dynamic_frames_rebuild_moved_meshes(ctx):
+    """Frames of the shadow-pass scene with a tenth of its instances moving: the moved instances'
+    meshes rebuilt (GE_RT_BUILD_SUBSET) and the instance structure after them, then the visibility
+    words and the jittered shadow pass, each equal to the oracle's after the same moves."""
+    s = synth.Synth.config(3, 0, 400, 257)
+    g, o = _pair(ctx, s)
+    n = len(s.arrays()["instances"])
+    moving = (n - 1) // 10
+    for k in range(3):
+        first = 1 + k * moving
+        recs = s.move_instances(first, moving, 100 + k, 0.5, 0.2)
+        g.update_instances(first, recs)
+        o.update_instances(first, recs)
+        meshes = [recs[i].mesh for i in range(moving)]
+        g.build(abi.BUILD_SUBSET, meshes=meshes)
+        _bvh_equal(g, o, TOP)
+        for m in meshes[:4]:
+            _bvh_equal(g, o, m)
+        pt, rv = s.frame(256, 144)
+        words = g.ray_visibility(rv)
+        assert np.array_equal(words, o.ray_visibility(rv))
+        p = synth.Synth.shadow_params(pt, rv, abi.SHADOW_JITTER, k)
+        got = g.shadow_resolve(p, words)
+        want, _ = o.shadow_resolve(p, words)
+        print(f"frame {k}: {np.count_nonzero(want != abi.SHADOW_NOT_TAKEN)} shadow rays", flush=True)
+        assert np.array_equal(got, want)
+    g.close()
