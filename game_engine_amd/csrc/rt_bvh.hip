@@ -56,16 +56,17 @@ __global__ void k_init_bounds(uint32_t nseg, uint32_t* cbox) {
   }
 }
 
-// Centroid bounds per segment. Each wave takes a contiguous run of kBoundsSteps x 64 primitives and
-// keeps per-lane running bounds while its lanes stay in one segment (segments are contiguous), so a
+// Centroid bounds per segment. Each wave takes a contiguous run of `steps` x 64 primitives and keeps
+// per-lane running bounds while its lanes stay in one segment (segments are contiguous), so a
 // segment costs one wave reduction and six atomics per run instead of per primitive: same-address
-// atomics serialize, and one segment can hold tens of millions of primitives.
-constexpr uint32_t kBoundsSteps = 64;
+// atomics serialize, and one segment can hold tens of millions of primitives. `steps` grows with
+// the batch (bounds_steps) so that small batches still spread over many waves.
+constexpr uint32_t kBoundsMaxSteps = 64;
 
-__global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, const float* __restrict__ boxes,
+__global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, uint32_t steps, const float* __restrict__ boxes,
                                                          const uint32_t* __restrict__ seg_of, uint32_t* cbox) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64u * kBoundsSteps;
+  const uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64u * steps;
   uint32_t cur = 0xffffffffu;  // the segment the running bounds belong to (the same in every lane)
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   auto flush = [&]() {
@@ -86,7 +87,7 @@ __global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, const float
       hi[a] = 0u;
     }
   };
-  for (uint32_t k = 0; k < kBoundsSteps; ++k) {
+  for (uint32_t k = 0; k < steps; ++k) {
     const uint32_t i = base + k * 64u + lane;
     const bool valid = i < n;
     if (!__any(valid)) break;
@@ -580,7 +581,11 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   if ((e = sc.seg_scale.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
 
   k_init_bounds<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>());
-  k_centroid_bounds<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, d_seg_of, sc.seg_cbox.as<uint32_t>());
+  // Runs of 64 x steps primitives a wave: at least 2,048 waves, at most 64 steps a run.
+  const uint32_t steps = std::max(1u, std::min(kBoundsMaxSteps, n / (64u * 2048u)));
+  const uint32_t bound_waves = blocks(n, 64u * steps);
+  k_centroid_bounds<<<blocks(uint64_t(bound_waves) * 64, 256), 256, 0, stream>>>(n, steps, boxes, d_seg_of,
+                                                                                  sc.seg_cbox.as<uint32_t>());
   k_segment_scale<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>(), sc.seg_scale.as<float>());
   k_morton<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, d_seg_of, sc.seg_scale.as<float>(), sc.keys[0].as<uint64_t>(),
                                                sc.vals[0].as<uint32_t>());

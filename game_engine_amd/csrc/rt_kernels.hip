@@ -200,36 +200,41 @@ __device__ f3 next_event(const Pt& c, const Surface& surface, f3 position, f3 fr
   return direct;
 }
 
+// A path in flight: where its next ray starts and goes, what it has gathered, and its sampler.
+struct PathVars {
+  f3 origin, direction, radiance, throughput;
+  uint32_t rng, bounce;
+  bool hit_primary;
+};
+
+// trace_path's bounce loop (path_trace.slang:257-461) from the path's current bounce to its end.
 template <bool Count>
-__device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bool& hit_primary, uint64_t& closest,
-                         uint64_t& any, Tally& tally) {
+__device__ void continue_path(const Pt& c, PathVars& v, uint64_t& closest, uint64_t& any, Tally& tally) {
   const ge_path_trace_params& p = c.p;
-  hit_primary = false;
-  f3 radiance = splat(0.0f);
-  f3 throughput = splat(1.0f);
-  for (uint32_t bounce = 0;; ++bounce) {
+  for (;; ++v.bounce) {
+    const uint32_t bounce = v.bounce;
     ++closest;
-    const Ray ray = make_ray(origin, direction, 0.0f);
+    const Ray ray = make_ray(v.origin, v.direction, 0.0f);
     Hit hit;
     traverse<false, Count>(c.bvh, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, tally, lane_stack());
     if (!hit.found) {
-      radiance = radiance + throughput * environment(p, direction, bounce == 0);
+      v.radiance = v.radiance + v.throughput * environment(p, v.direction, bounce == 0);
       break;
     }
-    if (bounce == 0) hit_primary = true;
+    if (bounce == 0) v.hit_primary = true;
     const ge_instance_desc in = c.sv.instances[hit.inst];
-    const HitSurface sf = surface_of_hit(c.sv, p, hit, origin, in);
+    const HitSurface sf = surface_of_hit(c.sv, p, hit, v.origin, in);
     const ge_resolve_material* mat = c.sv.materials + (c.sv.cluster_materials[sf.cluster] + in.material_base);
     f3 shading_normal = sf.normal;
     if (dot(shading_normal, sf.geometric) < 0.0f) shading_normal = -shading_normal;
     const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
     const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
     const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
-    radiance = radiance + throughput * f3{emi.x, emi.y, emi.z};
+    v.radiance = v.radiance + v.throughput * f3{emi.x, emi.y, emi.z};
     Surface surface;
     surface.albedo = f3{alb.x, alb.y, alb.z};
     surface.normal = shading_normal;
-    surface.view = -direction;
+    surface.view = -v.direction;
     surface.roughness = alb.w;
     surface.metallic = emi.w;
     surface.spec[0] = spc.x;
@@ -238,7 +243,7 @@ __device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bo
     surface.spec[3] = spc.w;
     const float bias = p.ray_bias + p.ray_bias_steps * c.sv.meshes[in.mesh].quant[3] * in.scale_max;
     const f3 front = dot(sf.geometric, surface.view) >= 0.0f ? sf.geometric : -sf.geometric;
-    radiance = radiance + throughput * next_event<Count>(c, surface, sf.position, front, bias, any, tally);
+    v.radiance = v.radiance + v.throughput * next_event<Count>(c, surface, sf.position, front, bias, any, tally);
     if (bounce >= p.max_bounces) break;
     const float n_dot_v = dot(surface.normal, surface.view);
     if (n_dot_v <= 0.0f) break;
@@ -249,14 +254,14 @@ __device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bo
     const f3 f = fresnel(surface, n_dot_v);
     const float p_specular = specular_probability(surface.albedo, surface.metallic, f);
     f3 l_local;
-    if (rng_next(rng) < p_specular) {
-      const float u0 = rng_next(rng);
-      const float u1 = rng_next(rng);
+    if (rng_next(v.rng) < p_specular) {
+      const float u0 = rng_next(v.rng);
+      const float u1 = rng_next(v.rng);
       const f3 h = ggx_vndf(v_local, alpha, u0, u1);
       l_local = reflect3(-v_local, h);
     } else {
-      const float u0 = rng_next(rng);
-      const float u1 = rng_next(rng);
+      const float u0 = rng_next(v.rng);
+      const float u1 = rng_next(v.rng);
       l_local = cosine_hemisphere(u0, u1);
     }
     if (l_local.z <= 0.0f) break;
@@ -264,17 +269,33 @@ __device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bo
     if (!(pdf > 0.0f)) break;
     const f3 next = tangent * l_local.x + bitangent * l_local.y + surface.normal * l_local.z;
     const f3 weight = brdf_eval(surface, next) * (l_local.z / pdf);
-    throughput = throughput * weight;
-    if (throughput.x <= 0.0f && throughput.y <= 0.0f && throughput.z <= 0.0f) break;
+    v.throughput = v.throughput * weight;
+    if (v.throughput.x <= 0.0f && v.throughput.y <= 0.0f && v.throughput.z <= 0.0f) break;
     if (bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
-      const float survive = clampr(maxr(throughput.x, maxr(throughput.y, throughput.z)), 0.0f, 0.95f);
-      if (rng_next(rng) >= survive) break;
-      throughput = throughput / maxr(survive, 1e-6f);
+      const float survive = clampr(maxr(v.throughput.x, maxr(v.throughput.y, v.throughput.z)), 0.0f, 0.95f);
+      if (rng_next(v.rng) >= survive) break;
+      v.throughput = v.throughput / maxr(survive, 1e-6f);
     }
-    origin = sf.position + (dot(next, front) >= 0.0f ? front : -front) * bias;
-    direction = next;
+    v.origin = sf.position + (dot(next, front) >= 0.0f ? front : -front) * bias;
+    v.direction = next;
   }
-  return radiance;
+}
+
+template <bool Count>
+__device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bool& hit_primary, uint64_t& closest,
+                         uint64_t& any, Tally& tally) {
+  PathVars v;
+  v.origin = origin;
+  v.direction = direction;
+  v.radiance = splat(0.0f);
+  v.throughput = splat(1.0f);
+  v.rng = rng;
+  v.bounce = 0;
+  v.hit_primary = false;
+  continue_path<Count>(c, v, closest, any, tally);
+  rng = v.rng;
+  hit_primary = v.hit_primary;
+  return v.radiance;
 }
 
 template <bool Count>
