@@ -106,6 +106,18 @@ SceneView scene_view(const SceneDev& s) {
   return v;
 }
 
+int32_t validate_materials(ge_rt_ctx* ctx, const ge_resolve_material* m, uint32_t count, uint32_t first) {
+  for (uint32_t k = 0; k < count; ++k) {
+    const ge_resolve_material& mat = m[k];
+    if (mat.albedo_texture != GE_NO_TEXTURE || mat.metallic_roughness_texture != GE_NO_TEXTURE ||
+        mat.normal_texture != GE_NO_TEXTURE || mat.emissive_texture != GE_NO_TEXTURE)
+      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(first + k) + " is textured");
+    if ((mat.flags >> 24) != 0u)
+      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(first + k) + " declares foliage optics");
+  }
+  return GE_RT_OK;
+}
+
 int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_desc* in, uint32_t count) {
   for (uint32_t i = 0; i < count; ++i) {
     const ge_instance_desc& inst = in[i];
@@ -272,13 +284,14 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
     return fail(GE_RT_ERROR_INVALID_SCENE, "the scene's meshes hold more than 2^31 triangles");
   s.tri_pool = uint32_t(tri_pool);
   s.node_pool = uint32_t(node_pool);
-  for (uint32_t k = 0; k < d->material_count; ++k) {
-    const ge_resolve_material& mat = d->materials[k];
-    if (mat.albedo_texture != GE_NO_TEXTURE || mat.metallic_roughness_texture != GE_NO_TEXTURE ||
-        mat.normal_texture != GE_NO_TEXTURE || mat.emissive_texture != GE_NO_TEXTURE)
-      return fail(GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(k) + " is textured");
-    if ((mat.flags >> 24) != 0u) return fail(GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(k) + " declares foliage optics");
+  {
+    const int32_t st = validate_materials(ctx, d->materials, d->material_count, 0);
+    if (st != GE_RT_OK) {
+      const std::string msg = ctx->error;
+      return fail(st, msg);
+    }
   }
+  s.h_materials.assign(d->materials, d->materials + d->material_count);
   {
     // The largest cluster material of each mesh: an instance's materials are that plus its base.
     s.mesh_max_material.assign(d->mesh_count, 0u);
@@ -714,6 +727,104 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(accum, d_accum, n * 16, hipMemcpyDeviceToHost, ctx->stream));
     GE_HIP(ctx, hipMemcpyAsync(coverage, d_cov, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
+                                     const ge_resolve_material* materials) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && (materials || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or materials");
+  SceneDev& s = scene->dev;
+  GE_CHECK(ctx, uint64_t(first) + count <= s.material_count, GE_RT_ERROR_INVALID_ARGUMENT, "material range out of bounds");
+  const int32_t st = validate_materials(ctx, materials, count, first);
+  if (st != GE_RT_OK) return st;
+  if (!count) return GE_RT_OK;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  std::memcpy(&s.h_materials[first], materials, size_t(count) * sizeof(ge_resolve_material));
+  GE_HIP(ctx, hipMemcpyAsync(s.materials.as<ge_resolve_material>() + first, materials,
+                             size_t(count) * sizeof(ge_resolve_material), hipMemcpyHostToDevice, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
+                                  const ge_resolve_light* lights) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && (lights || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or lights");
+  SceneDev& s = scene->dev;
+  GE_CHECK(ctx, uint64_t(first) + count <= s.light_count, GE_RT_ERROR_INVALID_ARGUMENT, "light range out of bounds");
+  if (!count) return GE_RT_OK;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  GE_HIP(ctx, hipMemcpyAsync(s.lights.as<ge_resolve_light>() + first, lights, size_t(count) * sizeof(ge_resolve_light),
+                             hipMemcpyHostToDevice, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
+                           const float* d_image, const ge_resolve_material* sampling, float* image,
+                           double* grad_materials, double* grad_lights, uint32_t flags) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
+  GE_CHECK(ctx, params->width > 0 && params->height > 0 && params->spp > 0, GE_RT_ERROR_INVALID_ARGUMENT,
+           "empty extent or no samples");
+  GE_CHECK(ctx, !(params->flags & GE_PT_ALPHA_MASKS), GE_RT_ERROR_UNSUPPORTED, "alpha-tested materials are not supported");
+  GE_CHECK(ctx, params->sky_params == 0 && params->ground_detail == 0, GE_RT_ERROR_UNSUPPORTED,
+           "the physical sky and ground detail are not supported");
+  GE_CHECK(ctx, params->light_count <= scene->dev.light_count, GE_RT_ERROR_INVALID_ARGUMENT,
+           "light_count exceeds the scene's uploaded lights");
+  GE_CHECK(ctx, params->max_bounces <= 15, GE_RT_ERROR_UNSUPPORTED, "the adjoint records at most 15 bounces");
+  GE_CHECK(ctx, !d_image || (grad_materials && grad_lights), GE_RT_ERROR_INVALID_ARGUMENT,
+           "a loss gradient needs both gradient outputs");
+  const SceneDev& s = scene->dev;
+  for (const ge_resolve_material& m : s.h_materials)
+    GE_CHECK(ctx, m.specular[0] == 0.0f && m.specular[1] == 0.0f && m.specular[2] == 0.0f && m.specular[3] == 0.0f,
+             GE_RT_ERROR_UNSUPPORTED, "the adjoint covers materials without their own specular record");
+  const size_t n = size_t(params->width) * params->height;
+  const size_t nm = size_t(s.material_count) * 4, nl = size_t(params->light_count) + 1;
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  const float* dd = d_image;
+  const ge_resolve_material* dsamp = sampling;
+  float* dimg = image;
+  double* dgm = grad_materials;
+  double* dgl = grad_lights;
+  if (!dev) {
+    float* t;
+    if (d_image) {
+      if ((r = stage(ctx, 0, d_image, n * 4, true, &t)) != GE_RT_OK) return r;
+      dd = t;
+    }
+    if (sampling) {
+      ge_resolve_material* sm;
+      if ((r = stage(ctx, 1, sampling, size_t(s.material_count), true, &sm)) != GE_RT_OK) return r;
+      dsamp = sm;
+    }
+    if (image && (r = stage(ctx, 2, image, n * 4, false, &dimg)) != GE_RT_OK) return r;
+    if (d_image) {
+      if ((r = stage(ctx, 3, grad_materials, nm, false, &dgm)) != GE_RT_OK) return r;
+      if ((r = stage(ctx, 4, grad_lights, nl, false, &dgl)) != GE_RT_OK) return r;
+    }
+  }
+  if (d_image) {
+    GE_HIP(ctx, hipMemsetAsync(dgm, 0, nm * 8, ctx->stream));
+    GE_HIP(ctx, hipMemsetAsync(dgl, 0, nl * 8, ctx->stream));
+  }
+  ge_path_trace_params p = *params;
+  SceneView sv = scene_view(s);
+  sv.light_count = params->light_count;
+  GE_HIP(ctx, begin_timing(ctx));
+  GE_HIP(ctx, launch_path_adjoint(ctx->stream, bvh_view(s), sv, p, dsamp, reinterpret_cast<const float4*>(dd),
+                                  reinterpret_cast<float4*>(dimg), d_image ? dgm : nullptr, d_image ? dgl : nullptr));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) {
+    if (image) GE_HIP(ctx, hipMemcpyAsync(image, dimg, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (d_image) {
+      GE_HIP(ctx, hipMemcpyAsync(grad_materials, dgm, nm * 8, hipMemcpyDeviceToHost, ctx->stream));
+      GE_HIP(ctx, hipMemcpyAsync(grad_lights, dgl, nl * 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;

@@ -51,6 +51,7 @@ struct SceneDev {
   std::vector<ge_mesh_desc> h_meshes;
   std::vector<ge_instance_desc> h_instances;
   std::vector<uint32_t> mesh_max_material;  // per mesh: its clusters' largest material index
+  std::vector<ge_resolve_material> h_materials;
   uint32_t cluster_count = 0, vertex_count = 0, triangle_count = 0, mesh_count = 0, instance_count = 0,
            material_count = 0, light_count = 0;
   bool has_attributes = false;
@@ -186,6 +187,12 @@ struct SceneView {
 // Records the begin (end = 0) or end (end = 1) event of a traversal launch of `kind` (0 closest-hit,
 // 1 any-hit) on the context's stream, for ge_rt_stats; launches past the event pool go untimed.
 hipError_t stage_mark(ge_rt_ctx* ctx, int kind, int end);
+
+// The path-replay adjoint (rt_kernels.hip): per pixel the detached estimator's sample sum into
+// `image` (when not null) and, with `d_image`, the loss gradient added into the double sums.
+hipError_t launch_path_adjoint(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
+                               const ge_resolve_material* sampling, const float4* d_image, float4* image,
+                               double* grad_materials, double* grad_lights);
 
 // The sun shadow pass over a visibility buffer (rt_shadow.hip): `rays` holds one 16-byte record an
 // entry, `out` the entries' answers. The any-hit walk is timed as a stage of kind 1.

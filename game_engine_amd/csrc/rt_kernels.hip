@@ -298,6 +298,234 @@ __device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bo
   return v.radiance;
 }
 
+// ---- the path-replay adjoint ---------------------------------------------------------------------
+// The path tracer's estimator with sampling detached (directions, lobe choices, pdfs and roulette
+// from the sampling materials; the BSDF values from the scene's), replayed from its counter RNG and
+// differentiated: dL = sum_k T_k dC_k + sum_k T_k dw_k U_{k+1}, U_k = C_k + w_k U_{k+1}, the first
+// term at each vertex's next-event sum, the second backwards over the path's recorded vertices.
+// Per-path arithmetic in float; the per-parameter sums in double.
+constexpr uint32_t k_adjoint_vertices = 17;  // max_bounces <= 15: the vertices and the escape
+
+struct AdjointArgs {
+  Pt c;
+  const ge_resolve_material* sampling;  // null: the scene's materials
+  const float4* d_image;                // dLoss/dI per pixel; null: the estimate only
+  float4* image;                        // per pixel: the samples' sum, the sample count
+  double* grad_materials;               // per material: albedo rgb, roughness
+  double* grad_lights;                  // the sun, then each light
+};
+
+struct AdjointVertex {
+  f3 C, T, w, dw_a, dw_r;
+  uint32_t mat;
+  bool cont;
+};
+
+__device__ f3 replay_path(const AdjointArgs& a, f3 origin, f3 direction, uint32_t& rng, bool grad, f3 delta) {
+  const Pt& c = a.c;
+  const ge_path_trace_params& p = c.p;
+  AdjointVertex vx[k_adjoint_vertices];
+  uint32_t nv = 0;
+  Tally tally;
+  uint64_t any = 0;
+  f3 radiance = splat(0.0f);
+  f3 throughput = splat(1.0f);
+  f3 throughput_s = splat(1.0f);
+  for (uint32_t bounce = 0;; ++bounce) {
+    const Ray ray = make_ray(origin, direction, 0.0f);
+    Hit hit;
+    traverse<false, false>(c.bvh, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, tally, lane_stack());
+    AdjointVertex& v = vx[nv++];
+    v.T = throughput;
+    v.cont = false;
+    v.mat = GE_RT_MISS;
+    if (!hit.found) {
+      v.C = environment(p, direction, bounce == 0);
+      radiance = radiance + throughput * v.C;
+      if (grad && bounce != 0 && !(p.flags & GE_PT_UNIFORM_SKY) && !(direction.y >= 0.0f))
+        atomicAdd(&a.grad_lights[0], double(dot(delta, throughput * ld3(p.ground) * (maxr(p.sun[1], 0.0f) / k_pi))));
+      break;
+    }
+    const ge_instance_desc in = c.sv.instances[hit.inst];
+    const HitSurface sf = surface_of_hit(c.sv, p, hit, origin, in);
+    const uint32_t mat_index = c.sv.cluster_materials[sf.cluster] + in.material_base;
+    const ge_resolve_material* mat = c.sv.materials + mat_index;
+    const ge_resolve_material* smat = a.sampling ? a.sampling + mat_index : mat;
+    f3 shading_normal = sf.normal;
+    if (dot(shading_normal, sf.geometric) < 0.0f) shading_normal = -shading_normal;
+    const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
+    const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
+    const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
+    const f3 emission = f3{emi.x, emi.y, emi.z};
+    radiance = radiance + throughput * emission;
+    Surface surface;
+    surface.albedo = f3{alb.x, alb.y, alb.z};
+    surface.normal = shading_normal;
+    surface.view = -direction;
+    surface.roughness = alb.w;
+    surface.metallic = emi.w;
+    surface.spec[0] = spc.x;
+    surface.spec[1] = spc.y;
+    surface.spec[2] = spc.z;
+    surface.spec[3] = spc.w;
+    Surface ssurface = surface;
+    {
+      const float4 salb = *reinterpret_cast<const float4*>(smat->albedo);
+      ssurface.albedo = f3{salb.x, salb.y, salb.z};
+      ssurface.roughness = salb.w;
+      ssurface.metallic = smat->emissive[3];
+    }
+    const float bias = p.ray_bias + p.ray_bias_steps * c.sv.meshes[in.mesh].quant[3] * in.scale_max;
+    const f3 front = dot(sf.geometric, surface.view) >= 0.0f ? sf.geometric : -sf.geometric;
+    f3 direct = splat(0.0f);
+    f3 dd_a = splat(0.0f), dd_r = splat(0.0f);
+    const f3 shadow_origin = sf.position + front * bias;
+    const f3 sun = ld3(p.sun);
+    if (p.sun[3] > 0.0f && dot(surface.normal, sun) > 0.0f && !occluded<false>(c, shadow_origin, sun, 1e30f, any, tally)) {
+      direct = direct + brdf_direct(surface, sun, splat(p.sun[3]));
+      if (grad) {
+        f3 ga, gr;
+        brdf_eval_grad(surface, sun, ga, gr);
+        const float cosl = maxr(dot(surface.normal, sun), 0.0f);
+        dd_a = dd_a + ga * (cosl * p.sun[3]);
+        dd_r = dd_r + gr * (cosl * p.sun[3]);
+        atomicAdd(&a.grad_lights[0], double(dot(delta, throughput * (brdf_eval(surface, sun) * cosl))));
+      }
+    }
+    for (uint32_t i = 0; i < c.sv.light_count; ++i) {
+      const ge_resolve_light light = c.sv.lights[i];
+      f3 light_dir, light_radiance;
+      float light_distance, att;
+      if (!light_sample(light, sf.position, light_dir, light_radiance, light_distance, &att)) continue;
+      if (dot(surface.normal, light_dir) <= 0.0f) continue;
+      if (occluded<false>(c, shadow_origin, light_dir, light_distance - bias - light.spot[2], any, tally)) continue;
+      direct = direct + brdf_direct(surface, light_dir, light_radiance);
+      if (grad) {
+        f3 ga, gr;
+        brdf_eval_grad(surface, light_dir, ga, gr);
+        const float cosl = maxr(dot(surface.normal, light_dir), 0.0f);
+        dd_a = dd_a + ga * cosl * light_radiance;
+        dd_r = dd_r + gr * cosl * light_radiance;
+        atomicAdd(&a.grad_lights[1 + i],
+                  double(dot(delta, throughput * (brdf_eval(surface, light_dir) * cosl * ld3(light.color_intensity) * att))));
+      }
+    }
+    radiance = radiance + throughput * direct;
+    v.C = emission + direct;
+    v.mat = mat_index;
+    if (grad) {
+      const f3 t_a = delta * throughput * dd_a;
+      double* gm = a.grad_materials + size_t(mat_index) * 4;
+      atomicAdd(&gm[0], double(t_a.x));
+      atomicAdd(&gm[1], double(t_a.y));
+      atomicAdd(&gm[2], double(t_a.z));
+      atomicAdd(&gm[3], double(dot(delta, throughput * dd_r)));
+    }
+    if (bounce >= p.max_bounces) break;
+    const float n_dot_v = dot(surface.normal, surface.view);
+    if (n_dot_v <= 0.0f) break;
+    f3 tangent, bitangent;
+    onb(surface.normal, tangent, bitangent);
+    const f3 v_local = f3{dot(surface.view, tangent), dot(surface.view, bitangent), dot(surface.view, surface.normal)};
+    const float alpha = brdf_alpha(ssurface.roughness);
+    const f3 f = fresnel(ssurface, n_dot_v);
+    const float p_specular = specular_probability(ssurface.albedo, ssurface.metallic, f);
+    f3 l_local;
+    if (rng_next(rng) < p_specular) {
+      const float u0 = rng_next(rng);
+      const float u1 = rng_next(rng);
+      const f3 hh = ggx_vndf(v_local, alpha, u0, u1);
+      l_local = reflect3(-v_local, hh);
+    } else {
+      const float u0 = rng_next(rng);
+      const float u1 = rng_next(rng);
+      l_local = cosine_hemisphere(u0, u1);
+    }
+    if (l_local.z <= 0.0f) break;
+    const float pdf = bsdf_pdf(v_local, l_local, alpha, p_specular);
+    if (!(pdf > 0.0f)) break;
+    const f3 next = tangent * l_local.x + bitangent * l_local.y + surface.normal * l_local.z;
+    const float scale = l_local.z / pdf;
+    const f3 weight = brdf_eval(surface, next) * scale;
+    const f3 weight_s = a.sampling ? brdf_eval(ssurface, next) * scale : weight;
+    throughput = throughput * weight;
+    throughput_s = throughput_s * weight_s;
+    if (throughput_s.x <= 0.0f && throughput_s.y <= 0.0f && throughput_s.z <= 0.0f) break;
+    float survive = 1.0f;
+    if (bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
+      survive = clampr(maxr(throughput_s.x, maxr(throughput_s.y, throughput_s.z)), 0.0f, 0.95f);
+      if (rng_next(rng) >= survive) break;
+      throughput = throughput / maxr(survive, 1e-6f);
+      throughput_s = throughput_s / maxr(survive, 1e-6f);
+      survive = maxr(survive, 1e-6f);
+    }
+    if (grad) {
+      f3 ga, gr;
+      brdf_eval_grad(surface, next, ga, gr);
+      v.w = weight / survive;
+      v.dw_a = ga * (scale / survive);
+      v.dw_r = gr * (scale / survive);
+      v.cont = true;
+    }
+    if (nv >= k_adjoint_vertices - 1) break;
+    origin = sf.position + (dot(next, front) >= 0.0f ? front : -front) * bias;
+    direction = next;
+  }
+  if (grad) {
+    f3 u = splat(0.0f);
+    for (uint32_t k = nv; k-- > 0;) {
+      const AdjointVertex& v = vx[k];
+      if (v.cont) {
+        const f3 t_a = delta * v.T * v.dw_a * u;
+        double* gm = a.grad_materials + size_t(v.mat) * 4;
+        atomicAdd(&gm[0], double(t_a.x));
+        atomicAdd(&gm[1], double(t_a.y));
+        atomicAdd(&gm[2], double(t_a.z));
+        atomicAdd(&gm[3], double(dot(delta, v.T * v.dw_r * u)));
+        u = v.C + v.w * u;
+      } else {
+        u = v.C;
+      }
+    }
+  }
+  return radiance;
+}
+
+__global__ void __launch_bounds__(64) k_path_adjoint(AdjointArgs a) {
+  const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
+  const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
+  const ge_path_trace_params& p = a.c.p;
+  if (x >= p.width || y >= p.height) return;
+  const size_t pixel = size_t(y) * p.width + x;
+  const bool grad = a.d_image != nullptr;
+  f3 delta = splat(0.0f);
+  if (grad) {
+    const float4 d = a.d_image[pixel];
+    delta = f3{d.x, d.y, d.z} / float(p.spp);
+  }
+  const float sx = float(p.width), sy = float(p.height);
+  const bool centered = (p.flags & GE_PT_PIXEL_CENTER) != 0;
+  f3 sum = splat(0.0f);
+  for (uint32_t s = 0; s < p.spp; ++s) {
+    uint32_t rng = rng_seed(x, y, p.sample_base + s, p.seed);
+    float jx = 0.5f, jy = 0.5f;
+    if (!centered) {
+      jx = rng_next(rng);
+      jy = rng_next(rng);
+    }
+    const float ndx = (float(x) + jx) / sx * 2.0f - 1.0f;
+    const float ndy = 1.0f - (float(y) + jy) / sy * 2.0f;
+    const float* m = p.clip_to_ray;
+    f3 dir;
+    dir.x = ((m[0] * ndx + m[4] * ndy) + m[8] * 0.0f) + m[12] * 1.0f;
+    dir.y = ((m[1] * ndx + m[5] * ndy) + m[9] * 0.0f) + m[13] * 1.0f;
+    dir.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
+    dir = normalize(dir);
+    sum = sum + replay_path(a, ld3(p.camera), dir, rng, grad, delta);
+  }
+  if (a.image) a.image[pixel] = make_float4(sum.x, sum.y, sum.z, float(p.spp));
+}
+
 template <bool Count>
 __global__ void __launch_bounds__(64) k_path_trace(Pt c, float4* __restrict__ accum, uint32_t* __restrict__ coverage,
                                                    unsigned long long* counters) {
@@ -450,6 +678,23 @@ hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ra
     k_ray_visibility<true><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, uint32_t(n), head, words, counters);
   else
     k_ray_visibility<false><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, uint32_t(n), head, words, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_path_adjoint(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
+                               const ge_resolve_material* sampling, const float4* d_image, float4* image,
+                               double* grad_materials, double* grad_lights) {
+  AdjointArgs a;
+  a.c.bvh = bvh;
+  a.c.sv = sv;
+  a.c.p = p;
+  a.sampling = sampling;
+  a.d_image = d_image;
+  a.image = image;
+  a.grad_materials = grad_materials;
+  a.grad_lights = grad_lights;
+  const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
+  k_path_adjoint<<<grid, 64, lds(bvh), stream>>>(a);
   return hipGetLastError();
 }
 

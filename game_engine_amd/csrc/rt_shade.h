@@ -75,6 +75,44 @@ GE_DEV f3 brdf_eval(const Surface& s, f3 l) {
 GE_DEV f3 brdf_direct(const Surface& s, f3 l, f3 radiance) {
   return brdf_eval(s, l) * maxr(dot(s.normal, l), 0.0f) * radiance;
 }
+// brdf_eval's derivatives for a surface without its own specular record: per channel c,
+// df_c/d albedo_c and df_c/d roughness (zero where brdf_alpha clamps).
+GE_DEV void brdf_eval_grad(const Surface& s, f3 l, f3& df_da, f3& df_dr) {
+  df_da = splat(0.0f);
+  df_dr = splat(0.0f);
+  const float n_dot_l = dot(s.normal, l);
+  if (n_dot_l <= 0.0f) return;
+  const float n_dot_v = maxr(dot(s.normal, s.view), 1e-4f);
+  const f3 half_sum = l + s.view;
+  const f3 h = dot(half_sum, half_sum) > 1e-12f ? normalize(half_sum) : s.normal;
+  const float n_dot_h = clampr(dot(s.normal, h), 0.0f, 1.0f);
+  const float v_dot_h = clampr(dot(s.view, h), 0.0f, 1.0f);
+  const float rc = clampr(s.roughness, 0.045f, 1.0f);
+  const float alpha = rc * rc;
+  const float dalpha = (s.roughness > 0.045f && s.roughness < 1.0f) ? 2.0f * rc : 0.0f;
+  const float m = s.metallic;
+  const float mm = clampr(1.0f - v_dot_h, 0.0f, 1.0f);
+  const float m2 = mm * mm;
+  const float w5 = m2 * m2 * mm;
+  const f3 f = fresnel(s, v_dot_h);
+  const float d = d_ggx(n_dot_h, alpha);
+  const float vis = v_smith(n_dot_v, n_dot_l, alpha);
+  const float kd = (1.0f - m) / k_pi;
+  const float df0 = m * (1.0f - w5);
+  df_da = splat(df0 * (d * vis)) - splat(df0 * kd) * s.albedo + (splat(1.0f) - f) * kd;
+  const float a2 = alpha * alpha;
+  const float dd = n_dot_h * n_dot_h * (a2 - 1.0f) + 1.0f;
+  const float dd_dalpha = k_pi * dd * dd > 1e-9f
+                              ? 2.0f * alpha / (k_pi * dd * dd) - 4.0f * alpha * a2 * n_dot_h * n_dot_h / (k_pi * dd * dd * dd)
+                              : 2.0f * alpha / 1e-9f;
+  const float xv = n_dot_v * n_dot_v * (1.0f - a2) + a2;
+  const float xl = n_dot_l * n_dot_l * (1.0f - a2) + a2;
+  const float lambda = n_dot_l * __builtin_sqrtf(xv) + n_dot_v * __builtin_sqrtf(xl);
+  const float dlambda = n_dot_l * alpha * (1.0f - n_dot_v * n_dot_v) / __builtin_sqrtf(xv) +
+                        n_dot_v * alpha * (1.0f - n_dot_l * n_dot_l) / __builtin_sqrtf(xl);
+  const float dv_dalpha = lambda > 1e-9f ? -0.5f * dlambda / (lambda * lambda) : 0.0f;
+  df_dr = f * ((dd_dalpha * vis + d * dv_dalpha) * dalpha);
+}
 
 GE_DEV void onb(f3 n, f3& t, f3& b) {
   const float s = n.z >= 0.0f ? 1.0f : -1.0f;
@@ -139,7 +177,9 @@ GE_DEV float spot_attenuation(f3 to_surface, f3 direction, float cos_inner, floa
   const float t = clampr((dot(to_surface, direction) - cos_outer) / maxr(cos_inner - cos_outer, 1e-4f), 0.0f, 1.0f);
   return t * t;
 }
-GE_DEV bool light_sample(const ge_resolve_light& light, f3 position, f3& direction, f3& radiance, float& distance) {
+GE_DEV bool light_sample(const ge_resolve_light& light, f3 position, f3& direction, f3& radiance, float& distance,
+                         float* attenuation_out = nullptr) {
+  if (attenuation_out) *attenuation_out = 1.0f;
   if (light.direction_type[3] >= 1.5f) {
     direction = normalize(ld3(light.position_radius));
     radiance = ld3(light.color_intensity) * light.color_intensity[3];
@@ -157,6 +197,7 @@ GE_DEV bool light_sample(const ge_resolve_light& light, f3 position, f3& directi
   if (light.direction_type[3] >= 0.5f)
     attenuation *= spot_attenuation(-direction, normalize(ld3(light.direction_type)), light.spot[0], light.spot[1]);
   radiance = ld3(light.color_intensity) * (light.color_intensity[3] * attenuation);
+  if (attenuation_out) *attenuation_out = attenuation;
   distance = __builtin_sqrtf(distance_sq);
   return true;
 }

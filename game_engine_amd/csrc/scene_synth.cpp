@@ -347,17 +347,26 @@ void set_sun(ge_synth& s, float x, float y, float z, float intensity) {
   s.sun[3] = intensity;
 }
 
-void build_config1(ge_synth& s, uint64_t seed) {
+// `objects` icospheres (320 triangles) and split boxes (192) alternating, at uniform positions in a
+// cube of side `extent`, in one mesh; object o takes material o mod `materials`. Plain materials are
+// Lambert-like (roughness 1, dielectric); varied ones draw roughness U[0.05, 1] and are metal one
+// time in five.
+void build_objects(ge_synth& s, uint64_t seed, uint32_t objects, float extent, uint32_t materials, bool varied) {
   Rng rng(seed);
   Source src;
-  for (uint32_t o = 0; o < 256; ++o) {
-    const F3 c{rng.uniform(0, 100), rng.uniform(0, 100), rng.uniform(0, 100)};
+  for (uint32_t o = 0; o < objects; ++o) {
+    const F3 c{rng.uniform(0, extent), rng.uniform(0, extent), rng.uniform(0, extent)};
     if (o % 2 == 0) {
       icosphere(src, c, rng.uniform(1.5f, 5.0f), 2);  // 320 triangles
     } else {
       split_box(src, c, {rng.uniform(1, 4), rng.uniform(1, 4), rng.uniform(1, 4)}, 4);  // 192 triangles
     }
-    s.materials.push_back(make_material(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), 1.0f, 0.0f));
+    if (o < materials) {
+      const float r = rng.uniform(0.1f, 0.9f), g = rng.uniform(0.1f, 0.9f), b = rng.uniform(0.1f, 0.9f);
+      const float rough = varied ? rng.uniform(0.05f, 1.0f) : 1.0f;
+      const float metal = varied ? (rng.next() < 0.2 ? 1.0f : 0.0f) : 0.0f;
+      s.materials.push_back(make_material(r, g, b, rough, metal));
+    }
   }
   // Clusters never straddle two objects: each object's run of triangles is clustered on its own
   // (objects are 320 or 192 triangles, so sequential clusters of 128 would straddle), into one mesh
@@ -370,7 +379,7 @@ void build_config1(ge_synth& s, uint64_t seed) {
   }
   // Build clusters per object into the shared arrays, then quantize once over the whole mesh.
   size_t tri_cursor = 0;
-  for (uint32_t o = 0; o < 256; ++o) {
+  for (uint32_t o = 0; o < objects; ++o) {
     const uint32_t count = (o % 2 == 0) ? 320u : 192u;
     Source obj;
     std::vector<uint32_t> remap(src.pos.size(), 0xffffffffu);
@@ -386,7 +395,7 @@ void build_config1(ge_synth& s, uint64_t seed) {
       }
     }
     tri_cursor += count;
-    const std::vector<uint32_t> mats(count, o);
+    const std::vector<uint32_t> mats(count, o % materials);
     const uint32_t before_meshes = uint32_t(s.meshes.size());
     add_mesh(s, obj, 128, mats);
     s.meshes.resize(before_meshes);  // fold the object's clusters into the one mesh
@@ -400,8 +409,8 @@ void build_config1(ge_synth& s, uint64_t seed) {
     lo = {std::min(lo.x, s.vertices[v * 3]), std::min(lo.y, s.vertices[v * 3 + 1]), std::min(lo.z, s.vertices[v * 3 + 2])};
     hi = {std::max(hi.x, s.vertices[v * 3]), std::max(hi.y, s.vertices[v * 3 + 1]), std::max(hi.z, s.vertices[v * 3 + 2])};
   }
-  const float extent = std::max(std::max(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
-  const float scale = extent > 0.0f ? extent / 65535.0f : 1.0f;
+  const float span = std::max(std::max(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+  const float scale = span > 0.0f ? span / 65535.0f : 1.0f;
   const float inv = 1.0f / scale;
   m.quant[0] = lo.x;
   m.quant[1] = lo.y;
@@ -419,12 +428,36 @@ void build_config1(ge_synth& s, uint64_t seed) {
   const float identity[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   s.instances.push_back(make_instance(mesh, identity, {0, 0, 0}, 0, 0));
   set_sun(s, 0.3f, 0.8f, 0.5f, 3.0f);
-  s.eye = {50.0f + 125.0f * std::sin(0.3839724f), 95.0f, 50.0f - 125.0f * std::cos(0.3839724f)};  // a 22 degree orbit
-  s.target = {50, 45, 50};
+  const float k = extent / 100.0f;  // the camera orbits the cube at its scale: a 22 degree orbit
+  s.eye = {k * (50.0f + 125.0f * std::sin(0.3839724f)), k * 95.0f, k * (50.0f - 125.0f * std::cos(0.3839724f))};
+  s.target = {k * 50, k * 45, k * 50};
   s.fov_deg = 60.0f;
   s.spp = 1;
   s.max_bounces = 4;
-  s.reach = std::sqrt(3.0f) * 110.0f;
+  s.reach = std::sqrt(3.0f) * 1.1f * extent;
+}
+
+void build_config1(ge_synth& s, uint64_t seed) { build_objects(s, seed, 256, 100.0f, 256, false); }
+
+// Inverse rendering: 2,048 objects (524,288 triangles) in a 200 m cube, 1,024 varied materials, the
+// sun and eight radius-limited point lights; 512 x 512 at 16 spp and 6 bounces.
+void build_config4(ge_synth& s, uint64_t seed) {
+  build_objects(s, seed, 2048, 200.0f, 1024, true);
+  Rng rng(seed ^ 0x4c49u);
+  for (uint32_t i = 0; i < 8; ++i) {
+    ge_resolve_light light{};
+    light.position_radius[0] = rng.uniform(20.0f, 180.0f);
+    light.position_radius[1] = rng.uniform(20.0f, 180.0f);
+    light.position_radius[2] = rng.uniform(20.0f, 180.0f);
+    light.position_radius[3] = 80.0f;
+    light.color_intensity[0] = rng.uniform(0.6f, 1.0f);
+    light.color_intensity[1] = rng.uniform(0.6f, 1.0f);
+    light.color_intensity[2] = rng.uniform(0.6f, 1.0f);
+    light.color_intensity[3] = rng.uniform(200.0f, 800.0f);
+    s.lights.push_back(light);
+  }
+  s.spp = 16;
+  s.max_bounces = 6;
 }
 
 void build_config2(ge_synth& s, uint64_t seed, uint32_t count) {
@@ -709,6 +742,7 @@ ge_synth* ge_synth_create(uint32_t config, uint64_t seed, uint32_t detail, uint3
   switch (config) {
     case 1: build_config1(*s, seed); break;
     case 2: build_config2(*s, seed, detail ? detail : 4096u); break;
+    case 4: build_config4(*s, seed); break;
     case 3: build_config3(*s, seed, detail ? detail : 16384u, terrain ? std::max(terrain, 2u) : 4096u); break;
     default: delete s; return nullptr;
   }

@@ -16,6 +16,9 @@
  *   config 3: `detail` (default 16,384) unique cluster-group meshes of 8 clusters of 128 triangles,
  *             one instance each, plus a (n x n) fBm heightfield terrain (n = 4096 by default, 6
  *             octaves of PCG-hashed value noise, lacunarity 2, gain 0.5) in clusters of 128.
+ *   config 4: 2,048 objects as config 1 (524,288 triangles) in a 200 m cube, 1,024 materials
+ *             (albedo U[0.1, 0.9], roughness U[0.05, 1], one in five metal), the sun and eight
+ *             point lights, 16 spp and 6 bounces: the inverse-rendering scene.
  */
 #ifndef GE_SCENE_SYNTH_H
 #define GE_SCENE_SYNTH_H

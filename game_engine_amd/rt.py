@@ -64,6 +64,9 @@ def lib():
             "ge_rt_tonemap": ([vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, u32], i32),
             "ge_rt_shadow_pass": ([vp, vp, C.POINTER(abi.ShadowParams), vp, vp, u32], i32),
             "ge_rt_shadow_resolve": ([vp, vp, C.POINTER(abi.ShadowResolveParams), vp, vp, u32], i32),
+            "ge_rt_scene_update_materials": ([vp, vp, u32, u32, vp], i32),
+            "ge_rt_scene_update_lights": ([vp, vp, u32, u32, vp], i32),
+            "ge_rt_path_adjoint": ([vp, vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, vp, vp, u32], i32),
             "ge_rt_stats": ([vp, C.POINTER(abi.Counters)], i32),
             "ge_rt_reset_stats": ([vp], i32),
         }
@@ -79,7 +82,7 @@ EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchroni
            "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_bvh_info",
            "ge_rt_bvh_download", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
            "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve", "ge_rt_stats",
-           "ge_rt_reset_stats")
+           "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint")
 
 
 def _ptr(a):
@@ -140,6 +143,8 @@ class Scene:
         h = C.c_void_p()
         ctx.check(lib().ge_rt_scene_upload(ctx.h, C.byref(desc), C.byref(h)))
         self.h = h
+        self.material_count = desc.material_count
+        self.light_count = desc.light_count
 
     def close(self):
         if self.h:
@@ -160,6 +165,38 @@ class Scene:
 
     def update_instances(self, first, records):
         self.ctx.check(lib().ge_rt_scene_update_instances(self.ctx.h, self.h, first, len(records), C.byref(records[0])))
+
+    def update_materials(self, first, records):
+        self.ctx.check(lib().ge_rt_scene_update_materials(self.ctx.h, self.h, first, len(records), C.addressof(records)))
+
+    def update_lights(self, first, records):
+        self.ctx.check(lib().ge_rt_scene_update_lights(self.ctx.h, self.h, first, len(records), C.addressof(records)))
+
+    def path_adjoint(self, params, d_image=None, sampling=None, image=True, grads=None, device=False):
+        """The path-replay adjoint (ge_rt_path_adjoint): (image, grad_materials, grad_lights).
+
+        Host arrays by default: image (n, 4) float32 sample sums, gradients (materials, 4) and
+        (1 + lights,) float64 when d_image (n, 4 float32, dLoss/dI) is given. With device=True,
+        every argument is a device pointer (int) or None and nothing is returned."""
+        n = params.width * params.height
+        flags = abi.DEVICE_PTR if device else 0
+        if device:
+            gm, gl = grads if grads is not None else (None, None)
+            self.ctx.check(lib().ge_rt_path_adjoint(self.ctx.h, self.h, C.byref(params), d_image, sampling, image,
+                                                    gm, gl, flags))
+            return None
+        img = np.zeros((n, 4), dtype=np.float32) if image else None
+        gm = gl = None
+        dptr = None
+        if d_image is not None:
+            d_image = np.ascontiguousarray(d_image, dtype=np.float32).reshape(n, 4)
+            dptr = d_image.ctypes.data
+            gm = np.zeros((self.material_count, 4), dtype=np.float64)
+            gl = np.zeros(params.light_count + 1, dtype=np.float64)
+        sptr = C.addressof(sampling) if sampling is not None else None
+        self.ctx.check(lib().ge_rt_path_adjoint(self.ctx.h, self.h, C.byref(params), dptr, sptr, _ptr(img),
+                                                _ptr(gm), _ptr(gl), flags))
+        return img, gm, gl
 
     def bvh(self, mesh):
         n, p = C.c_uint32(), C.c_uint32()

@@ -394,6 +394,13 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
  * tlas_references pass writes); follow with ge_rt_bvh_build(GE_RT_BUILD_TOP). */
 int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first,
                                      uint32_t count, const ge_instance_desc* instances);
+/* Replaces material records [first, first + count) (the GpuScene's material buffer update); the
+ * structures are unaffected. Host pointer. */
+int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
+                                     const ge_resolve_material* materials);
+/* Replaces light records [first, first + count). Host pointer. */
+int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
+                                  const ge_resolve_light* lights);
 /* Sizes of the built structure of mesh `mesh` (or of the top level for mesh == ~0). */
 int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, uint32_t* node_count,
                        uint32_t* prim_count);
@@ -428,6 +435,20 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene,
  * pointers (null takes the block's own `accum` / `coverage` address). */
 int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
                          float* accum, uint32_t* coverage, uint32_t flags);
+/* The path-replay adjoint of the path tracer, for inverse rendering. No counterpart in the reference
+ * (its path tracer has no derivatives); it replays path_trace_main's paths (path_trace.slang:257-495)
+ * from their counter RNG (path_trace.h:17-21) with sampling detached: directions, lobe choices,
+ * pdfs and roulette come from `sampling` (material_count records; null: the scene's own), the
+ * BSDF values from the scene's materials, so with equal tables the estimate is path_trace's.
+ *   image (optional, 4 floats a pixel): the samples' radiance sum and their count, as accum;
+ *   d_image (optional, 4 floats a pixel, the fourth unused): dLoss/dI for I = sum / spp; with it,
+ *   grad_materials (material_count x 4 doubles: albedo r, g, b, roughness) and grad_lights
+ *   (1 + light_count doubles: the sun's intensity sun[3], then each light's color_intensity[3])
+ *   receive dLoss/dparameter, summed over every path in double.
+ * Materials with their own specular record and max_bounces > 15 return GE_RT_ERROR_UNSUPPORTED. */
+int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
+                           const float* d_image, const ge_resolve_material* sampling, float* image,
+                           double* grad_materials, double* grad_lights, uint32_t flags);
 /* The tonemap (tonemap_main, path_trace.slang:511-543): accum + coverage -> packed RGBA8. */
 int32_t ge_rt_tonemap(ge_rt_ctx* ctx, const ge_path_trace_params* params, const float* accum,
                       const uint32_t* coverage, uint32_t* output, uint32_t flags);

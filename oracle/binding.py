@@ -64,6 +64,13 @@ def lib():
         L.ge_oracle_trace_closest.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32]
         L.ge_oracle_trace_any.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32]
         L.ge_oracle_ray_visibility.argtypes = [vp, C.POINTER(abi.RayVisibilityParams), vp, C.c_int32]
+        L.ge_oracle_path_adjoint.restype = C.c_int32
+        L.ge_oracle_path_adjoint.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, C.c_uint32, vp, vp, vp, vp,
+                                             C.c_int32]
+        L.ge_oracle_update_materials.restype = C.c_int32
+        L.ge_oracle_update_materials.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+        L.ge_oracle_update_lights.restype = C.c_int32
+        L.ge_oracle_update_lights.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
         L.ge_oracle_shadow_resolve.restype = C.c_int32
         L.ge_oracle_shadow_resolve.argtypes = [vp, C.POINTER(abi.ShadowResolveParams), vp, vp, vp, C.c_int32]
         L.ge_oracle_path_trace.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, C.c_uint32,
@@ -97,6 +104,7 @@ class Scene:
         self.h = C.c_void_p(lib().ge_oracle_scene_create(C.byref(desc), C.byref(eye) if eye else None))
         if not self.h:
             raise ValueError("oracle scene creation failed")
+        self.desc_material_count = desc.material_count
 
     def close(self):
         if self.h:
@@ -163,6 +171,35 @@ class Scene:
         lib().ge_oracle_path_trace64(self.h, C.byref(params), accum.ctypes.data, coverage.ctypes.data,
                                      r0, r1, threads)
         return accum, coverage
+
+    def path_adjoint(self, params, d_image=None, sampling=None, mode=FP32, threads=0):
+        """(image, grad_materials, grad_lights): the detached estimator's per-pixel sample sums (4
+        values a pixel) and, given dLoss/dI (4 floats a pixel), the gradients (material_count x 4:
+        albedo rgb, roughness; then the sun and each light's intensity)."""
+        n = params.width * params.height
+        nm = self.desc_material_count
+        img32 = np.zeros((n, 4), dtype=np.float32)
+        img64 = np.zeros((n, 4), dtype=np.float64)
+        gm = np.zeros((nm, 4), dtype=np.float64)
+        gl = np.zeros(params.light_count + 1, dtype=np.float64)
+        dptr = None
+        if d_image is not None:
+            d_image = np.ascontiguousarray(d_image, dtype=np.float32).reshape(n, 4)
+            dptr = d_image.ctypes.data
+        sptr = None
+        if sampling is not None:
+            sptr = C.addressof(sampling)
+        st = lib().ge_oracle_path_adjoint(self.h, C.byref(params), dptr, sptr, mode, img32.ctypes.data,
+                                          img64.ctypes.data, gm.ctypes.data, gl.ctypes.data, threads)
+        if st != 0:
+            raise ValueError(f"oracle path adjoint: status {st}")
+        return (img64 if mode == FP64 else img32), gm, gl
+
+    def update_materials(self, first, records):
+        assert lib().ge_oracle_update_materials(self.h, first, len(records), C.addressof(records)) == 0
+
+    def update_lights(self, first, records):
+        assert lib().ge_oracle_update_lights(self.h, first, len(records), C.addressof(records)) == 0
 
     def shadow_resolve(self, params, words, threads=0):
         """(out, rays): the entries' answers and their rays (origin xyz, t_max; t_max -1: none)."""

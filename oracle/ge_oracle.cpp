@@ -787,6 +787,49 @@ template <class R> inline V3<R> brdf_direct(const BrdfSurface<R>& s, V3<R> l, V3
   return brdf_eval(s, l) * maxr(dot(s.normal, l), R(0)) * radiance;
 }
 
+// The derivatives of brdf_eval for a surface without its own specular record: per channel c,
+// df_c/d albedo_c (the only albedo channel f_c depends on) and df_c/d roughness, from the terms of
+// brdf_eval (Schlick F with F0 = lerp(0.04, albedo, metallic), GGX D, Smith-correlated V, the
+// (1 - F)(1 - metallic)/pi diffuse lobe). Zero where f is zero, and the roughness term zero where
+// brdf_alpha clamps.
+template <class R> inline void brdf_eval_grad(const BrdfSurface<R>& s, V3<R> l, V3<R>& df_da, V3<R>& df_dr) {
+  df_da = splat(R(0));
+  df_dr = splat(R(0));
+  const R n_dot_l = dot(s.normal, l);
+  if (n_dot_l <= R(0)) return;
+  const R n_dot_v = maxr(dot(s.normal, s.view), KC(1e-4));
+  const V3<R> half_sum = l + s.view;
+  const V3<R> h = dot(half_sum, half_sum) > KC(1e-12) ? normalize(half_sum) : s.normal;
+  const R n_dot_h = clampr(dot(s.normal, h), R(0), R(1));
+  const R v_dot_h = clampr(dot(s.view, h), R(0), R(1));
+  const R rc = clampr(s.roughness, KC(0.045), R(1));
+  const R alpha = rc * rc;
+  const R dalpha = (s.roughness > KC(0.045) && s.roughness < R(1)) ? R(2) * rc : R(0);
+  const R m = s.metallic;
+  const R mm = clampr(R(1) - v_dot_h, R(0), R(1));
+  const R m2 = mm * mm;
+  const R w5 = m2 * m2 * mm;
+  const V3<R> f = brdf_fresnel(s, v_dot_h);
+  const R d = brdf_d_ggx(n_dot_h, alpha);
+  const R vis = brdf_v_smith(n_dot_v, n_dot_l, alpha);
+  const R kd = (R(1) - m) / k_pi<R>();
+  const R df0 = m * (R(1) - w5);  // dF_c / d albedo_c
+  df_da = splat(df0 * (d * vis)) - splat(df0 * kd) * s.albedo + (splat(R(1)) - f) * kd;
+  const R a2 = alpha * alpha;
+  const R dd = n_dot_h * n_dot_h * (a2 - R(1)) + R(1);
+  const R pi = k_pi<R>();
+  const R dd_dalpha = pi * dd * dd > KC(1e-9)
+                          ? R(2) * alpha / (pi * dd * dd) - R(4) * alpha * a2 * n_dot_h * n_dot_h / (pi * dd * dd * dd)
+                          : R(2) * alpha / KC(1e-9);
+  const R xv = n_dot_v * n_dot_v * (R(1) - a2) + a2;
+  const R xl = n_dot_l * n_dot_l * (R(1) - a2) + a2;
+  const R lambda = n_dot_l * std::sqrt(xv) + n_dot_v * std::sqrt(xl);
+  const R dlambda = n_dot_l * alpha * (R(1) - n_dot_v * n_dot_v) / std::sqrt(xv) +
+                    n_dot_v * alpha * (R(1) - n_dot_l * n_dot_l) / std::sqrt(xl);
+  const R dv_dalpha = lambda > KC(1e-9) ? -KC(0.5) * dlambda / (lambda * lambda) : R(0);
+  df_dr = f * ((dd_dalpha * vis + d * dv_dalpha) * dalpha);
+}
+
 template <class R> inline void onb_from_normal(V3<R> n, V3<R>& t, V3<R>& b) {
   const R s = n.z >= R(0) ? R(1) : R(-1);
   const R a = R(-1) / (s + n.z);
@@ -861,7 +904,8 @@ template <class R> inline R spot_attenuation(V3<R> to_surface, V3<R> direction, 
 }
 template <class R>
 inline bool light_sample(const ge_resolve_light& light, V3<R> position, V3<R>& direction, V3<R>& radiance,
-                         R& distance) {
+                         R& distance, R* attenuation_out = nullptr) {
+  if (attenuation_out) *attenuation_out = R(1);
   if (R(light.direction_type[3]) >= KC(1.5)) {
     direction = normalize(load3<R>(light.position_radius));
     radiance = load3<R>(light.color_intensity) * R(light.color_intensity[3]);
@@ -879,6 +923,7 @@ inline bool light_sample(const ge_resolve_light& light, V3<R> position, V3<R>& d
   if (R(light.direction_type[3]) >= KC(0.5))
     attenuation *= spot_attenuation(-direction, normalize(load3<R>(light.direction_type)), R(light.spot[0]), R(light.spot[1]));
   radiance = load3<R>(light.color_intensity) * (R(light.color_intensity[3]) * attenuation);
+  if (attenuation_out) *attenuation_out = attenuation;
   distance = std::sqrt(distance_sq);
   return true;
 }
@@ -1147,6 +1192,229 @@ void path_trace_pixel(const PtCtx<R>& c, uint32_t x, uint32_t y, R sum[3], bool&
     bool hit_primary = false;
     acc = acc + trace_path(c, origin, dir, rng, hit_primary);
     covered = covered || hit_primary;
+  }
+  sum[0] = acc.x;
+  sum[1] = acc.y;
+  sum[2] = acc.z;
+}
+
+// ---- the path-replay adjoint ---------------------------------------------------------------------
+// The estimator differentiated is the path tracer's with its sampling detached: directions, lobe
+// choices, pdfs and roulette come from the sampling materials (the scene's own when no table is
+// given) and the BSDF values from the scene's materials, so at equal tables it is the forward
+// tracer's estimate. Along a path, L = sum_k T_k C_k with T_{k+1} = T_k w_k (C_k the vertex's
+// emission and next-event light, w_k = f cos / (pdf survive)); dL = sum_k T_k dC_k +
+// sum_k T_k dw_k U_{k+1}, U_k = C_k + w_k U_{k+1}: the light and BSDF terms of the vertex's
+// next-event sum at once, the throughput terms from the path's recorded vertices, backwards.
+constexpr uint32_t k_adjoint_vertices = 17;  // max_bounces <= 15: the vertices and the escape
+
+template <class R> struct AdjointVertex {
+  V3<R> C, T, w, dw_a, dw_r;
+  uint32_t mat;
+  bool cont;
+};
+
+template <class R> struct Grads {
+  std::vector<double> mat, light;  // material_count x (albedo rgb, roughness); sun, then lights
+};
+
+template <class R>
+V3<R> replay_path(const PtCtx<R>& c, const ge_resolve_material* sampling, V3<R> origin, V3<R> direction,
+                  uint32_t& rng, bool& hit_primary, const V3<R>* delta, Grads<R>* g) {
+  const ge_path_trace_params& p = *c.p;
+  const ge_oracle_scene& s = *c.s;
+  hit_primary = false;
+  AdjointVertex<R> vx[k_adjoint_vertices];
+  uint32_t nv = 0;
+  V3<R> radiance = splat(R(0));
+  V3<R> throughput = splat(R(1));
+  V3<R> throughput_s = splat(R(1));
+  for (uint32_t bounce = 0;; ++bounce) {
+    ++c.tally->closest;
+    const float o[3] = {float(origin.x), float(origin.y), float(origin.z)};
+    const float d[3] = {float(direction.x), float(direction.y), float(direction.z)};
+    const TRay ray = make_ray(o, d, 0.0f);
+    HitRec hit;
+    traverse<false>(s, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, *c.tally);
+    AdjointVertex<R>& v = vx[nv++];
+    v.T = throughput;
+    v.cont = false;
+    v.mat = GE_RT_MISS;
+    if (!hit.found) {
+      v.C = environment(c, direction, bounce == 0);
+      radiance = radiance + throughput * v.C;
+      // The lit ground below the horizon carries the sun's intensity (environment above).
+      if (g && bounce != 0 && !(p.flags & GE_PT_UNIFORM_SKY) && !(direction.y >= R(0)))
+        g->light[0] += double(dot(*delta, throughput * load3<R>(p.ground) * (maxr(R(p.sun[1]), R(0)) / k_pi<R>())));
+      break;
+    }
+    if (bounce == 0) hit_primary = true;
+    const ge_instance_desc* inst;
+    uint32_t cluster_index;
+    const Surf<R> sf = surface_of_hit<R>(s, hit, origin, p.eye, &inst, &cluster_index);
+    const uint32_t mat_index = s.cluster_materials[cluster_index] + inst->material_base;
+    const ge_resolve_material& mat = s.materials[mat_index];
+    const ge_resolve_material& smat = sampling ? sampling[mat_index] : mat;
+    V3<R> shading_normal = sf.normal;
+    if (dot(shading_normal, sf.geometric) < R(0)) shading_normal = -shading_normal;
+    const V3<R> emission = load3<R>(mat.emissive);
+    radiance = radiance + throughput * emission;
+    BrdfSurface<R> surface, ssurface;
+    surface.albedo = load3<R>(mat.albedo);
+    surface.normal = shading_normal;
+    surface.view = -direction;
+    surface.roughness = R(mat.albedo[3]);
+    surface.metallic = R(mat.emissive[3]);
+    for (int k = 0; k < 4; ++k) surface.specular[k] = R(mat.specular[k]);
+    ssurface = surface;
+    ssurface.albedo = load3<R>(smat.albedo);
+    ssurface.roughness = R(smat.albedo[3]);
+    ssurface.metallic = R(smat.emissive[3]);
+    const ge_mesh_desc& mesh = s.meshes[inst->mesh];
+    const R bias = R(p.ray_bias) + R(p.ray_bias_steps) * R(mesh.quant[3]) * R(inst->scale_max);
+    const V3<R> front = dot(sf.geometric, surface.view) >= R(0) ? sf.geometric : -sf.geometric;
+    // next_event, with each unblocked light's derivatives beside its term.
+    V3<R> direct = splat(R(0));
+    V3<R> dd_a = splat(R(0)), dd_r = splat(R(0));
+    const V3<R> shadow_origin = sf.position + front * bias;
+    const V3<R> sun = load3<R>(p.sun);
+    if (R(p.sun[3]) > R(0) && dot(surface.normal, sun) > R(0) && !occluded(c, shadow_origin, sun, KC(1e30))) {
+      direct = direct + brdf_direct(surface, sun, splat(R(p.sun[3])));
+      if (g) {
+        V3<R> ga, gr;
+        brdf_eval_grad(surface, sun, ga, gr);
+        const R cosl = maxr(dot(surface.normal, sun), R(0));
+        dd_a = dd_a + ga * (cosl * R(p.sun[3]));
+        dd_r = dd_r + gr * (cosl * R(p.sun[3]));
+        g->light[0] += double(dot(*delta, throughput * (brdf_eval(surface, sun) * cosl)));
+      }
+    }
+    for (uint32_t i = 0; i < p.light_count; ++i) {
+      const ge_resolve_light& light = s.lights[i];
+      V3<R> light_dir, light_radiance;
+      R light_distance, att;
+      if (!light_sample(light, sf.position, light_dir, light_radiance, light_distance, &att)) continue;
+      if (dot(surface.normal, light_dir) <= R(0)) continue;
+      if (occluded(c, shadow_origin, light_dir, light_distance - bias - R(light.spot[2]))) continue;
+      direct = direct + brdf_direct(surface, light_dir, light_radiance);
+      if (g) {
+        V3<R> ga, gr;
+        brdf_eval_grad(surface, light_dir, ga, gr);
+        const R cosl = maxr(dot(surface.normal, light_dir), R(0));
+        dd_a = dd_a + ga * cosl * light_radiance;
+        dd_r = dd_r + gr * cosl * light_radiance;
+        g->light[1 + i] +=
+            double(dot(*delta, throughput * (brdf_eval(surface, light_dir) * cosl * load3<R>(light.color_intensity) * att)));
+      }
+    }
+    radiance = radiance + throughput * direct;
+    v.C = emission + direct;
+    v.mat = mat_index;
+    if (g) {
+      const V3<R> t_a = *delta * throughput * dd_a;
+      g->mat[size_t(mat_index) * 4 + 0] += double(t_a.x);
+      g->mat[size_t(mat_index) * 4 + 1] += double(t_a.y);
+      g->mat[size_t(mat_index) * 4 + 2] += double(t_a.z);
+      g->mat[size_t(mat_index) * 4 + 3] += double(dot(*delta, throughput * dd_r));
+    }
+    if (bounce >= p.max_bounces) break;
+    const R n_dot_v = dot(surface.normal, surface.view);
+    if (n_dot_v <= R(0)) break;
+    V3<R> tangent, bitangent;
+    onb_from_normal(surface.normal, tangent, bitangent);
+    const V3<R> v_local = to_local(surface.view, tangent, bitangent, surface.normal);
+    const R alpha = brdf_alpha(ssurface.roughness);
+    const V3<R> fresnel = brdf_fresnel(ssurface, n_dot_v);
+    const R p_specular = specular_probability(ssurface.albedo, ssurface.metallic, n_dot_v, fresnel);
+    V3<R> l_local;
+    if (rng_next<R>(rng) < p_specular) {
+      const R u0 = rng_next<R>(rng);
+      const R u1 = rng_next<R>(rng);
+      const V3<R> hh = sample_ggx_vndf(v_local, alpha, u0, u1);
+      l_local = reflect_(-v_local, hh);
+    } else {
+      const R u0 = rng_next<R>(rng);
+      const R u1 = rng_next<R>(rng);
+      l_local = sample_cosine_hemisphere(u0, u1);
+    }
+    if (l_local.z <= R(0)) break;
+    const R pdf = bsdf_pdf(v_local, l_local, alpha, p_specular);
+    if (!(pdf > R(0))) break;
+    const V3<R> next = to_world(l_local, tangent, bitangent, surface.normal);
+    const R scale = l_local.z / pdf;
+    const V3<R> weight = brdf_eval(surface, next) * scale;
+    const V3<R> weight_s = sampling ? brdf_eval(ssurface, next) * scale : weight;
+    throughput = throughput * weight;
+    throughput_s = throughput_s * weight_s;
+    if (throughput_s.x <= R(0) && throughput_s.y <= R(0) && throughput_s.z <= R(0)) break;
+    R survive = R(1);
+    if (bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
+      survive = clampr(maxr(throughput_s.x, maxr(throughput_s.y, throughput_s.z)), R(0), KC(0.95));
+      if (rng_next<R>(rng) >= survive) break;
+      throughput = throughput / maxr(survive, KC(1e-6));
+      throughput_s = throughput_s / maxr(survive, KC(1e-6));
+      survive = maxr(survive, KC(1e-6));
+    }
+    if (g) {
+      V3<R> ga, gr;
+      brdf_eval_grad(surface, next, ga, gr);
+      v.w = weight / survive;
+      v.dw_a = ga * (scale / survive);
+      v.dw_r = gr * (scale / survive);
+      v.cont = true;
+    }
+    if (nv >= k_adjoint_vertices - 1) break;  // unreachable: max_bounces <= 15 is checked
+    origin = sf.position + (dot(next, front) >= R(0) ? front : -front) * bias;
+    direction = next;
+  }
+  if (g) {
+    // The throughput terms, backwards over the recorded vertices: U_k = C_k + w_k U_{k+1}.
+    V3<R> u = splat(R(0));
+    for (uint32_t k = nv; k-- > 0;) {
+      const AdjointVertex<R>& v = vx[k];
+      if (v.cont) {
+        const V3<R> t_a = *delta * v.T * v.dw_a * u;
+        g->mat[size_t(v.mat) * 4 + 0] += double(t_a.x);
+        g->mat[size_t(v.mat) * 4 + 1] += double(t_a.y);
+        g->mat[size_t(v.mat) * 4 + 2] += double(t_a.z);
+        g->mat[size_t(v.mat) * 4 + 3] += double(dot(*delta, v.T * v.dw_r * u));
+        u = v.C + v.w * u;
+      } else {
+        u = v.C;
+      }
+    }
+  }
+  return radiance;
+}
+
+template <class R>
+void adjoint_pixel(const PtCtx<R>& c, const ge_resolve_material* sampling, uint32_t x, uint32_t y, const float* d_image,
+                   R sum[3], Grads<R>* g) {
+  const ge_path_trace_params& p = *c.p;
+  const R sx = R(float(p.width)), sy = R(float(p.height));
+  const bool centered = (p.flags & GE_PT_PIXEL_CENTER) != 0;
+  const size_t pix = size_t(y) * p.width + x;
+  V3<R> delta = splat(R(0));
+  if (d_image) delta = v3(R(d_image[pix * 4]), R(d_image[pix * 4 + 1]), R(d_image[pix * 4 + 2])) / R(float(p.spp));
+  V3<R> acc = splat(R(0));
+  for (uint32_t si = 0; si < p.spp; ++si) {
+    uint32_t rng = rng_seed(x, y, p.sample_base + si, p.seed);
+    R jx = KC(0.5), jy = KC(0.5);
+    if (!centered) {
+      jx = rng_next<R>(rng);
+      jy = rng_next<R>(rng);
+    }
+    const R ndx = (R(float(x)) + jx) / sx * R(2) - R(1);
+    const R ndy = R(1) - (R(float(y)) + jy) / sy * R(2);
+    const float* m = p.clip_to_ray;
+    V3<R> dir;
+    dir.x = ((R(m[0]) * ndx + R(m[4]) * ndy) + R(m[8]) * R(0)) + R(m[12]) * R(1);
+    dir.y = ((R(m[1]) * ndx + R(m[5]) * ndy) + R(m[9]) * R(0)) + R(m[13]) * R(1);
+    dir.z = ((R(m[2]) * ndx + R(m[6]) * ndy) + R(m[10]) * R(0)) + R(m[14]) * R(1);
+    dir = normalize(dir);
+    bool hit_primary = false;
+    acc = acc + replay_path(c, sampling, load3<R>(p.camera), dir, rng, hit_primary, d_image ? &delta : nullptr,
+                            d_image ? g : nullptr);
   }
   sum[0] = acc.x;
   sum[1] = acc.y;
@@ -1597,6 +1865,82 @@ int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resol
   return GE_RT_OK;
 }
 
+int32_t ge_oracle_path_adjoint(const ge_oracle_scene* s, const ge_path_trace_params* p, const float* d_image,
+                               const ge_resolve_material* sampling, uint32_t mode, float* image_sum, double* image64,
+                               double* grad_materials, double* grad_lights, int32_t threads) {
+  if (p->max_bounces + 2 > k_adjoint_vertices) return GE_RT_ERROR_UNSUPPORTED;
+  for (const ge_resolve_material& m : s->materials)
+    if (m.specular[0] != 0.0f || m.specular[1] != 0.0f || m.specular[2] != 0.0f || m.specular[3] != 0.0f)
+      return GE_RT_ERROR_UNSUPPORTED;
+  const size_t nm = s->materials.size() * 4, nl = size_t(p->light_count) + 1;
+  const int nt = resolve_threads(threads);
+  std::vector<std::vector<double>> gm(size_t(nt), std::vector<double>(nm, 0.0)), gl(size_t(nt), std::vector<double>(nl, 0.0));
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t y = 0; y < int64_t(p->height); ++y) {
+      for (uint32_t x = 0; x < p->width; ++x) {
+        const size_t pix = size_t(y) * p->width + x;
+        if (mode == GE_ORACLE_FP64) {
+          PtCtx<double> c{s, p, &tally};
+          Grads<double> g;
+          g.mat.swap(gm[size_t(tid)]);
+          g.light.swap(gl[size_t(tid)]);
+          double sum[3];
+          adjoint_pixel(c, sampling, x, uint32_t(y), d_image, sum, &g);
+          g.mat.swap(gm[size_t(tid)]);
+          g.light.swap(gl[size_t(tid)]);
+          if (image64)
+            for (int k = 0; k < 3; ++k) image64[pix * 4 + k] = sum[k];
+          if (image64) image64[pix * 4 + 3] = double(p->spp);
+        } else {
+          PtCtx<float> c{s, p, &tally};
+          Grads<float> g;
+          g.mat.swap(gm[size_t(tid)]);
+          g.light.swap(gl[size_t(tid)]);
+          float sum[3];
+          adjoint_pixel(c, sampling, x, uint32_t(y), d_image, sum, &g);
+          g.mat.swap(gm[size_t(tid)]);
+          g.light.swap(gl[size_t(tid)]);
+          if (image_sum)
+            for (int k = 0; k < 3; ++k) image_sum[pix * 4 + k] = sum[k];
+          if (image_sum) image_sum[pix * 4 + 3] = float(p->spp);
+        }
+      }
+    }
+    tally.flush();
+  }
+  if (grad_materials) {
+    std::fill(grad_materials, grad_materials + nm, 0.0);
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < nm; ++k) grad_materials[k] += gm[size_t(t)][k];
+  }
+  if (grad_lights) {
+    std::fill(grad_lights, grad_lights + nl, 0.0);
+    for (int t = 0; t < nt; ++t)
+      for (size_t k = 0; k < nl; ++k) grad_lights[k] += gl[size_t(t)][k];
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_oracle_update_materials(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_resolve_material* m) {
+  if (uint64_t(first) + count > s->materials.size()) return GE_RT_ERROR_INVALID_ARGUMENT;
+  std::memcpy(&s->materials[first], m, size_t(count) * sizeof(ge_resolve_material));
+  return GE_RT_OK;
+}
+
+int32_t ge_oracle_update_lights(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_resolve_light* l) {
+  if (uint64_t(first) + count > s->lights.size()) return GE_RT_ERROR_INVALID_ARGUMENT;
+  std::memcpy(&s->lights[first], l, size_t(count) * sizeof(ge_resolve_light));
+  return GE_RT_OK;
+}
+
 void ge_oracle_counters(uint64_t out[8]) {
   for (int i = 0; i < 8; ++i) out[i] = g_counters[i].load();
 }
@@ -1698,6 +2042,13 @@ template <class R> int32_t probe(uint32_t kind, const double* in, double* out) {
       return 0;
     }
     case 16: put(0, brdf_direct(surface(0), V(15), V(18))); return 0;
+    case 17: {
+      V3<R> ga, gr;
+      brdf_eval_grad(surface(0), V(15), ga, gr);
+      put(0, ga);
+      put(3, gr);
+      return 0;
+    }
     default: return GE_RT_ERROR_INVALID_ARGUMENT;
   }
 }

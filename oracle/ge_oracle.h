@@ -67,6 +67,18 @@ void ge_oracle_shadow_pass(const ge_oracle_scene* scene, const ge_shadow_params*
 int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resolve_params* p, const uint64_t* words,
                                  uint32_t* out, float* rays, int32_t threads);
 
+/* The path-replay adjoint (ge_rt_path_adjoint's contract) in float (mode FP32, the GPU's mirror)
+ * or double (FP64): per pixel the sum over its samples of the detached estimator (sampling from
+ * `sampling`, or the scene's materials when null; `image_sum` in FP32 mode, `image64` in FP64, 4
+ * values a pixel, the fourth the sample count) and, when `d_image` (dLoss/dI, 4 floats a pixel) is
+ * given, the loss gradient over material albedo rgb and roughness (4 doubles a material) and light
+ * intensities (sun, then each light). */
+int32_t ge_oracle_path_adjoint(const ge_oracle_scene* s, const ge_path_trace_params* p, const float* d_image,
+                               const ge_resolve_material* sampling, uint32_t mode, float* image_sum, double* image64,
+                               double* grad_materials, double* grad_lights, int32_t threads);
+int32_t ge_oracle_update_materials(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_resolve_material* m);
+int32_t ge_oracle_update_lights(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_resolve_light* l);
+
 /* Traversal counters accumulated since the last reset: closest rays, any rays, then nodes,
  * triangles and instances of the closest-hit walks, then those of the any-hit walks. */
 void ge_oracle_counters(uint64_t out[8]);
@@ -84,7 +96,7 @@ void ge_oracle_sincos(float x, float* s, float* c);
  * source. kind: 0 cosine hemisphere, 1 GGX VNDF sample, 2 ggx_vndf_pdf, 3 brdf_eval, 4 bsdf_pdf,
  * 5 specular_probability, 6 onb_from_normal, 7 ggx_g1, 8 v_smith, 9 d_ggx, 10 fresnel,
  * 11 mis_power_heuristic, 12 light_sample, 13 decode_normal_oct, 14 display_output,
- * 15 transform_normal, 16 brdf_direct. */
+ * 15 transform_normal, 16 brdf_direct, 17 brdf_eval_grad (df/d albedo per channel, df/d roughness). */
 int32_t ge_oracle_probe(uint32_t kind, uint32_t mode, const double* in, double* out);
 /* Watertight ray/triangle test: out = {hit, t, u, v}. */
 void ge_oracle_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3],
