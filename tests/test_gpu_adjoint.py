@@ -165,7 +165,7 @@ def test_autograd_inverse_rendering_lowers_the_loss(ctx):
         opt.step()
         with torch.no_grad():
             albedo.clamp_(0.0, 1.0)
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     print(f"loss {losses[0]:.4g} -> {losses[-1]:.4g}")
     assert losses[-1] < 0.5 * losses[0]
     err0 = (albedo0[visible] - hidden[visible]).abs().mean()
