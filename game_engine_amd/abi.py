@@ -103,6 +103,19 @@ class SceneDesc(C.Structure):
                 ("pad", u32)]
 
 
+# The radiance cache's records (no counterpart in the reference).
+CACHE_INPUTS = 192
+CACHE_WIDTH = 64
+CACHE_OUTPUTS = 16
+CACHE_LAYERS = 6
+CACHE_WEIGHTS = 29696
+CACHE_DIMS = (CACHE_INPUTS, CACHE_WIDTH, CACHE_WIDTH, CACHE_WIDTH, CACHE_WIDTH, CACHE_WIDTH, CACHE_OUTPUTS)
+CACHE_QUERY_DTYPE = np.dtype([("position", "<f4", 3), ("direction", "<f4", 2), ("normal", "<f4", 2),
+                              ("albedo", "<f4", 3), ("roughness", "<f4"), ("tag", "<u4")])
+CACHE_SAMPLE_DTYPE = np.dtype([("query", CACHE_QUERY_DTYPE), ("radiance", "<f4", 3), ("path", "<u4")])
+assert CACHE_QUERY_DTYPE.itemsize == 48 and CACHE_SAMPLE_DTYPE.itemsize == 64
+assert sum(a * b for a, b in zip(CACHE_DIMS[:-1], CACHE_DIMS[1:])) == CACHE_WEIGHTS
+
 # numpy views of the backend's own records.
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instance_id", "<u4"),

@@ -10,8 +10,13 @@
 #include <vector>
 
 #include "rt_internal.h"
+#include "rt_mlp.h"
 
 using namespace ge;
+
+struct ge_rt_cache {
+  ge::RadianceCache net;
+};
 
 namespace {
 
@@ -674,8 +679,14 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   return GE_RT_OK;
 }
 
-int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params, float* accum,
-                         uint32_t* coverage, uint32_t flags) {
+}  // extern "C"
+
+namespace {
+
+// A path-trace dispatch's checks, its parameter block with the scene's addresses (as the
+// renderer's GpuScene fills them), and the scene view over the block's lights.
+int32_t path_setup(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
+                   ge_path_trace_params* p, SceneView* svl) {
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
@@ -685,37 +696,66 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
   GE_CHECK(ctx, params->ground_detail == 0, GE_RT_ERROR_UNSUPPORTED, "ground detail is not supported");
   GE_CHECK(ctx, params->light_count <= scene->dev.light_count, GE_RT_ERROR_INVALID_ARGUMENT,
            "light_count exceeds the scene's uploaded lights");
+  *p = *params;
+  const SceneView sv = scene_view(scene->dev);
+  p->clusters = reinterpret_cast<uint64_t>(sv.clusters);
+  p->mesh = reinterpret_cast<uint64_t>(sv.meshes);
+  p->triangles = reinterpret_cast<uint64_t>(sv.triangles);
+  p->instances = reinterpret_cast<uint64_t>(sv.instances);
+  p->attributes = reinterpret_cast<uint64_t>(sv.attributes);
+  p->materials = reinterpret_cast<uint64_t>(sv.materials);
+  p->cluster_materials = reinterpret_cast<uint64_t>(sv.cluster_materials);
+  p->lights = reinterpret_cast<uint64_t>(sv.lights);
+  *svl = sv;
+  svl->light_count = params->light_count;
+  return GE_RT_OK;
+}
+
+// The accumulator and coverage a path-trace dispatch adds into: the caller's device pointers (or
+// the block's), or staging copies of host buffers (copied in when resuming at sample_base > 0).
+int32_t path_outputs(ge_rt_ctx* ctx, const ge_path_trace_params* params, float* accum, uint32_t* coverage,
+                     uint32_t flags, float4** d_accum, uint32_t** d_cov) {
   const size_t n = size_t(params->width) * params->height;
-  const bool dev = flags & GE_RT_DEVICE_PTR;
+  if (flags & GE_RT_DEVICE_PTR) {
+    *d_accum = reinterpret_cast<float4*>(accum ? accum : reinterpret_cast<float*>(params->accum));
+    *d_cov = coverage ? coverage : reinterpret_cast<uint32_t*>(params->coverage);
+    GE_CHECK(ctx, *d_accum && *d_cov, GE_RT_ERROR_INVALID_ARGUMENT, "null accumulator or coverage");
+    return GE_RT_OK;
+  }
+  GE_CHECK(ctx, accum && coverage, GE_RT_ERROR_INVALID_ARGUMENT, "null accumulator or coverage");
+  const bool resume = params->sample_base != 0;
+  float* da;
+  int32_t r;
+  if ((r = stage(ctx, 0, accum, n * 4, resume, &da)) != GE_RT_OK) return r;
+  *d_accum = reinterpret_cast<float4*>(da);
+  return stage(ctx, 1, coverage, n, resume, d_cov);
+}
+
+int32_t path_copy_out(ge_rt_ctx* ctx, const ge_path_trace_params* params, float* accum, uint32_t* coverage,
+                      uint32_t flags, const float4* d_accum, const uint32_t* d_cov) {
+  if (flags & GE_RT_DEVICE_PTR) return GE_RT_OK;
+  const size_t n = size_t(params->width) * params->height;
+  GE_HIP(ctx, hipMemcpyAsync(accum, d_accum, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  GE_HIP(ctx, hipMemcpyAsync(coverage, d_cov, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params, float* accum,
+                         uint32_t* coverage, uint32_t flags) {
+  ge_path_trace_params p;
+  SceneView svl;
+  int32_t r = path_setup(ctx, scene, params, &p, &svl);
+  if (r != GE_RT_OK) return r;
   float4* d_accum;
   uint32_t* d_cov;
-  if (dev) {
-    d_accum = reinterpret_cast<float4*>(accum ? accum : reinterpret_cast<float*>(params->accum));
-    d_cov = coverage ? coverage : reinterpret_cast<uint32_t*>(params->coverage);
-    GE_CHECK(ctx, d_accum && d_cov, GE_RT_ERROR_INVALID_ARGUMENT, "null accumulator or coverage");
-  } else {
-    GE_CHECK(ctx, accum && coverage, GE_RT_ERROR_INVALID_ARGUMENT, "null accumulator or coverage");
-    const bool resume = params->sample_base != 0;
-    float* da;
-    if ((r = stage(ctx, 0, accum, n * 4, resume, &da)) != GE_RT_OK) return r;
-    d_accum = reinterpret_cast<float4*>(da);
-    if ((r = stage(ctx, 1, coverage, n, resume, &d_cov)) != GE_RT_OK) return r;
-  }
-  ge_path_trace_params p = *params;
-  const SceneView sv = scene_view(scene->dev);
-  // The block's scene addresses, as the renderer's GpuScene would have filled them.
-  p.clusters = reinterpret_cast<uint64_t>(sv.clusters);
-  p.mesh = reinterpret_cast<uint64_t>(sv.meshes);
-  p.triangles = reinterpret_cast<uint64_t>(sv.triangles);
-  p.instances = reinterpret_cast<uint64_t>(sv.instances);
-  p.attributes = reinterpret_cast<uint64_t>(sv.attributes);
-  p.materials = reinterpret_cast<uint64_t>(sv.materials);
-  p.cluster_materials = reinterpret_cast<uint64_t>(sv.cluster_materials);
-  p.lights = reinterpret_cast<uint64_t>(sv.lights);
+  if ((r = path_outputs(ctx, params, accum, coverage, flags, &d_accum, &d_cov)) != GE_RT_OK) return r;
   p.accum = reinterpret_cast<uint64_t>(d_accum);
   p.coverage = reinterpret_cast<uint64_t>(d_cov);
-  SceneView svl = sv;
-  svl.light_count = params->light_count;
   GE_HIP(ctx, begin_timing(ctx));
   unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
   if (flags & GE_RT_PT_MEGAKERNEL) {
@@ -724,12 +764,7 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
     GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
   }
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
-  if (!dev) {
-    GE_HIP(ctx, hipMemcpyAsync(accum, d_accum, n * 16, hipMemcpyDeviceToHost, ctx->stream));
-    GE_HIP(ctx, hipMemcpyAsync(coverage, d_cov, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  }
-  return GE_RT_OK;
+  return path_copy_out(ctx, params, accum, coverage, flags, d_accum, d_cov);
 }
 
 int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
@@ -943,6 +978,212 @@ int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+}
+
+// ---- the radiance cache ----------------------------------------------------------------------------
+int32_t ge_rt_cache_create(ge_rt_ctx* ctx, const float box_min[3], const float box_max[3], uint32_t seed,
+                           ge_rt_cache** out) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, out && box_min && box_max, GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  for (int k = 0; k < 3; ++k)
+    GE_CHECK(ctx, std::isfinite(box_min[k]) && std::isfinite(box_max[k]) && box_max[k] > box_min[k],
+             GE_RT_ERROR_INVALID_ARGUMENT, "the cache's box is empty or not finite");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  auto* c = new (std::nothrow) ge_rt_cache();
+  GE_CHECK(ctx, c, GE_RT_ERROR_OUT_OF_MEMORY, "out of host memory");
+  for (int k = 0; k < 3; ++k) {
+    c->net.box.lo[k] = box_min[k];
+    c->net.box.inv[k] = 1.0f / (box_max[k] - box_min[k]);
+  }
+  const hipError_t e = c->net.init(ctx->stream, seed);
+  if (e != hipSuccess) {
+    c->net.release();
+    delete c;
+    GE_HIP(ctx, e);
+  }
+  *out = c;
+  return GE_RT_OK;
+}
+
+void ge_rt_cache_destroy(ge_rt_ctx* ctx, ge_rt_cache* cache) {
+  if (!cache) return;
+  if (ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  cache->net.release();
+  delete cache;
+}
+
+int32_t ge_rt_cache_weights(ge_rt_ctx* ctx, ge_rt_cache* cache, float* get, const float* set) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache, GE_RT_ERROR_INVALID_ARGUMENT, "null cache");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  if (get) GE_HIP(ctx, cache->net.get_weights(ctx->stream, get));
+  if (set) {
+    for (uint32_t i = 0; i < GE_CACHE_WEIGHTS; ++i)
+      GE_CHECK(ctx, std::isfinite(set[i]), GE_RT_ERROR_INVALID_ARGUMENT, "weights must be finite");
+    GE_HIP(ctx, cache->net.set_weights(ctx->stream, set));
+    GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_cache_infer(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_query* queries, uint32_t n,
+                          float* radiance, uint32_t flags) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache && (n == 0 || (queries && radiance)), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  if (n == 0) return GE_RT_OK;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  const bool dev = flags & GE_RT_DEVICE_PTR;
+  const ge_cache_query* dq = queries;
+  float* dout = radiance;
+  int32_t r;
+  if (!dev) {
+    ge_cache_query* t;
+    if ((r = stage(ctx, 0, queries, n, true, &t)) != GE_RT_OK) return r;
+    dq = t;
+    if ((r = stage(ctx, 1, radiance, size_t(n) * 3, false, &dout)) != GE_RT_OK) return r;
+  }
+  GE_HIP(ctx, begin_timing(ctx));
+  GE_HIP(ctx, cache->net.infer(ctx->stream, reinterpret_cast<const float*>(dq), sizeof(ge_cache_query) / 4, n,
+                               nullptr, dout));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (!dev) GE_HIP(ctx, hipMemcpyAsync(radiance, dout, size_t(n) * 12, hipMemcpyDeviceToHost, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+namespace {
+
+int32_t read_loss(ge_rt_ctx* ctx, ge_rt_cache* cache, double* loss) {
+  if (!loss) return GE_RT_OK;
+  GE_HIP(ctx, hipMemcpyAsync(cache->net.host_loss, cache->net.loss.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *loss = *cache->net.host_loss;
+  return GE_RT_OK;
+}
+
+bool finite_rate(float lr) { return std::isfinite(lr) && lr >= 0.0f; }
+
+// Traces params' paths through the wavefront, recording a sample at every hit vertex into the
+// cache's own buffer (capacity samples); *count receives how many vertices there were.
+int32_t collect_samples(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
+                        const ge_path_trace_params* params, uint32_t capacity, uint32_t* count) {
+  ge_path_trace_params p;
+  SceneView svl;
+  int32_t r = path_setup(ctx, scene, params, &p, &svl);
+  if (r != GE_RT_OK) return r;
+  RadianceCache& net = cache->net;
+  GE_HIP(ctx, net.samples.reserve(std::max<size_t>(size_t(capacity) * sizeof(ge_cache_sample), 256)));
+  GE_HIP(ctx, net.weights.reserve(std::max<size_t>(size_t(capacity) * 16, 256)));
+  CacheHook hook{};
+  hook.mode = CacheHook::kRecord;
+  hook.box = net.box;
+  hook.cache = &net;
+  hook.samples = net.samples.as<ge_cache_sample>();
+  hook.weights = net.weights.as<float4>();
+  hook.capacity = capacity;
+  GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, nullptr, nullptr, nullptr, &hook));
+  GE_HIP(ctx, read_cache_count(ctx, count));
+  return GE_RT_OK;
+}
+
+}  // namespace
+
+int32_t ge_rt_cache_train(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_sample* samples, uint32_t n,
+                          float learning_rate, double* loss, uint32_t flags) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache && (n == 0 || samples), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  GE_CHECK(ctx, finite_rate(learning_rate), GE_RT_ERROR_INVALID_ARGUMENT, "the learning rate must be finite and >= 0");
+  if (loss) *loss = 0.0;
+  if (n == 0) return GE_RT_OK;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  const ge_cache_sample* ds = samples;
+  if (!(flags & GE_RT_DEVICE_PTR)) {
+    ge_cache_sample* t;
+    const int32_t r = stage(ctx, 0, samples, n, true, &t);
+    if (r != GE_RT_OK) return r;
+    ds = t;
+  }
+  GE_HIP(ctx, cache->net.train(ctx->stream, ds, 1, n, learning_rate));
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return read_loss(ctx, cache, loss);
+}
+
+int32_t ge_rt_cache_collect(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
+                            const ge_path_trace_params* params, ge_cache_sample* samples, uint32_t capacity,
+                            uint32_t* count, uint32_t flags) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache && count && (capacity == 0 || samples), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  GE_CHECK(ctx, !(flags & GE_RT_PT_MEGAKERNEL), GE_RT_ERROR_UNSUPPORTED, "the cache's paths run on the wavefront");
+  *count = 0;
+  int32_t r = collect_samples(ctx, scene, cache, params, capacity, count);
+  if (r != GE_RT_OK) return r;
+  const size_t kept = std::min(*count, capacity);
+  if (kept) {
+    const hipMemcpyKind kind = (flags & GE_RT_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    GE_HIP(ctx, hipMemcpyAsync(samples, cache->net.samples.ptr, kept * sizeof(ge_cache_sample), kind, ctx->stream));
+  }
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
+}
+
+int32_t ge_rt_cache_update(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
+                           const ge_path_trace_params* params, uint32_t samples, uint32_t steps, float learning_rate,
+                           double* loss, uint32_t flags) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache && steps > 0, GE_RT_ERROR_INVALID_ARGUMENT, "null cache or no steps");
+  GE_CHECK(ctx, finite_rate(learning_rate), GE_RT_ERROR_INVALID_ARGUMENT, "the learning rate must be finite and >= 0");
+  GE_CHECK(ctx, !(flags & GE_RT_PT_MEGAKERNEL), GE_RT_ERROR_UNSUPPORTED, "the cache's paths run on the wavefront");
+  if (loss) *loss = 0.0;
+  uint32_t count = 0;
+  int32_t r = collect_samples(ctx, scene, cache, params, samples, &count);
+  if (r != GE_RT_OK) return r;
+  const uint32_t n = std::min(count, samples);
+  const uint32_t k = std::min(steps, std::max(n, 1u));
+  const uint32_t per = n / k;
+  if (per == 0) return GE_RT_OK;
+  // Step s trains on samples s, s + k, s + 2k, ...: every step sees every bounce.
+  for (uint32_t st = 0; st < k; ++st)
+    GE_HIP(ctx, cache->net.train(ctx->stream, cache->net.samples.as<ge_cache_sample>() + st, k, per, learning_rate));
+  return read_loss(ctx, cache, loss);
+}
+
+int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
+                                const ge_path_trace_params* params, uint32_t depth, float* accum, uint32_t* coverage,
+                                uint32_t flags) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache, GE_RT_ERROR_INVALID_ARGUMENT, "null cache");
+  GE_CHECK(ctx, !(flags & GE_RT_PT_MEGAKERNEL), GE_RT_ERROR_UNSUPPORTED, "the cached tracer runs on the wavefront");
+  ge_path_trace_params p;
+  SceneView svl;
+  int32_t r = path_setup(ctx, scene, params, &p, &svl);
+  if (r != GE_RT_OK) return r;
+  const uint64_t npaths = uint64_t(params->width) * params->height * params->spp;
+  GE_CHECK(ctx, npaths < (1ull << 31), GE_RT_ERROR_INVALID_ARGUMENT, "too many paths in one dispatch");
+  float4* d_accum;
+  uint32_t* d_cov;
+  if ((r = path_outputs(ctx, params, accum, coverage, flags, &d_accum, &d_cov)) != GE_RT_OK) return r;
+  p.accum = reinterpret_cast<uint64_t>(d_accum);
+  p.coverage = reinterpret_cast<uint64_t>(d_cov);
+  RadianceCache& net = cache->net;
+  GE_HIP(ctx, net.samples.reserve(std::max<size_t>(npaths * sizeof(ge_cache_query), 256)));
+  GE_HIP(ctx, net.weights.reserve(std::max<size_t>(npaths * 16, 256)));
+  CacheHook hook{};
+  hook.mode = CacheHook::kCached;
+  hook.depth = depth;
+  hook.box = net.box;
+  hook.cache = &net;
+  hook.queries = net.samples.as<ge_cache_query>();
+  hook.weights = net.weights.as<float4>();
+  hook.capacity = uint32_t(npaths);
+  GE_HIP(ctx, begin_timing(ctx));
+  unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
+  GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters, &hook));
+  if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  return path_copy_out(ctx, params, accum, coverage, flags, d_accum, d_cov);
 }
 
 int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {

@@ -214,8 +214,11 @@ hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, con
                           const uint32_t* coverage, uint32_t* out);
 // The wavefront form of launch_path_trace (rt_wavefront.hip); synchronizes once a bounce to size
 // the next launches.
+// With a hook (rt_mlp.h) the paths also feed the radiance cache.
+struct CacheHook;
 hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
-                                float4* accum, uint32_t* coverage, unsigned long long* counters);
+                                float4* accum, uint32_t* coverage, unsigned long long* counters,
+                                const CacheHook* hook = nullptr);
 hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
                               const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters);
 

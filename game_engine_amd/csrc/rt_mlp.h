@@ -1,0 +1,96 @@
+// rt_mlp.h — the radiance cache's network: sizes, the encoding of a query, and the host-side state
+// (FP32 master weights, Adam moments, BF16 copies in three layouts, training activations).
+#pragma once
+
+#include "rt_internal.h"
+
+namespace ge {
+
+constexpr int kCacheFeatures = 11;   // position 3, direction 2, normal 2, albedo 3, roughness 1
+constexpr int kCacheBands = 12;      // sin and cos of 2^b pi x, b = 0..11, for the 7 geometric values
+constexpr int kCacheInputs = GE_CACHE_INPUTS;    // 168 encoded + 4 material + 1 constant, padded to 192
+constexpr int kCacheWidth = GE_CACHE_WIDTH;      // 64
+constexpr int kCacheOutPad = GE_CACHE_OUTPUTS;   // 16 output rows, the first 3 RGB
+constexpr int kCacheLayers = GE_CACHE_LAYERS;    // 6 weight matrices: 5 hidden ReLU layers
+constexpr uint32_t kCacheWeights = GE_CACHE_WEIGHTS;
+static_assert(kCacheInputs * kCacheWidth + 4 * kCacheWidth * kCacheWidth + kCacheWidth * kCacheOutPad ==
+                  GE_CACHE_WEIGHTS,
+              "weight count");
+static_assert(sizeof(ge_cache_query) == 48 && sizeof(ge_cache_sample) == 64, "cache record sizes");
+
+// Layer l maps cache_dim(l) inputs to cache_dim(l + 1) outputs; its weights are [out][in] from
+// cache_offset(l) in every flat layout.
+__host__ __device__ constexpr uint32_t cache_dim(int l) {
+  return l == 0 ? uint32_t(kCacheInputs) : (l == kCacheLayers ? uint32_t(kCacheOutPad) : uint32_t(kCacheWidth));
+}
+__host__ __device__ constexpr uint32_t cache_offset(int l) {
+  uint32_t at = 0;
+  for (int i = 0; i < l; ++i) at += cache_dim(i) * cache_dim(i + 1);
+  return at;
+}
+
+// The value at input index `idx` (0..191) of the encoded query with features f. The index splits
+// as idx = 32 kb + 8 g + j with group G = 6 g + kb: groups 0..20 hold sin, cos pairs of bands
+// 4 (G % 3) .. + 3 of geometric value G / 3 (even j sin, odd j cos); group 21 holds albedo rgb,
+// roughness, then 1 (the bias input) and zeros; groups 22 and 23 are zero. The order is what lets the
+// fused inference kernel compute a lane's inputs from compile-time band numbers.
+// sin(2^b pi x) is evaluated as sin(2 pi frac(2^(b-1) x)), exact argument reduction for any x.
+__device__ __forceinline__ float cache_band(float x, int b, bool cosine) {
+  const float r = __builtin_amdgcn_fractf(x * __builtin_ldexpf(1.0f, b - 1));
+  return cosine ? __builtin_amdgcn_cosf(r) : __builtin_amdgcn_sinf(r);  // arguments in revolutions
+}
+
+// The cache's normalisation: position -> (p - lo) * inv.
+struct CacheBox {
+  float lo[3], inv[3];
+};
+
+struct RadianceCache {
+  CacheBox box{{0, 0, 0}, {1, 1, 1}};
+  DevBuf w, m, v, grad;  // FP32 master weights, Adam moments, gradient (kCacheWeights each)
+  DevBuf wb, wt, frag;   // BF16 weights: [out][in], [in][out], and the inference kernel's fragments
+  DevBuf partial;        // split-k partial weight gradients
+  DevBuf loss;           // double
+  DevBuf x, act[kCacheLayers - 1], out, delta[2];  // training activations (bf16), output (fp32), deltas (bf16)
+  DevBuf samples, weights;  // the tracer's cache records (ge_cache_sample / ge_cache_query) and float4 side records
+  uint32_t capacity = 0;    // rows the training buffers hold
+  uint32_t step = 0;        // Adam steps taken
+  double* host_loss = nullptr;  // pinned
+
+  hipError_t init(hipStream_t st, uint32_t seed);
+  hipError_t set_weights(hipStream_t st, const float* host);
+  hipError_t get_weights(hipStream_t st, float* host);
+  // Raw network outputs for n queries (stride floats apart, the first 11 the features): out n x 3.
+  // With n_dev (device count, clamped to n) the launch reads the count from the device.
+  hipError_t infer(hipStream_t st, const float* queries, uint32_t stride, uint32_t n, const uint32_t* n_dev, float* out);
+  // The cached tracer's apply: result[tag] += w.xyz * max(prediction, 0) for each query (tag = the
+  // path id), the queries counted on the device.
+  hipError_t apply(hipStream_t st, const ge_cache_query* queries, const float4* w, uint32_t capacity,
+                   const uint32_t* n_dev, float4* result);
+  // One Adam step on the relative L2 loss over n samples `stride` records apart from `samples`;
+  // the batch's mean loss lands in `loss` (device double).
+  hipError_t train(hipStream_t st, const ge_cache_sample* samples, uint32_t stride, uint32_t n, float lr);
+  void release();
+
+ private:
+  hipError_t reserve_rows(uint32_t n);
+  hipError_t publish(hipStream_t st, float lr, float c1, float c2, bool update);
+};
+
+// How the wavefront tracer feeds the cache (wavefront_path_trace's `hook`).
+struct CacheHook {
+  enum Mode { kCached = 1, kRecord = 2 } mode;
+  uint32_t depth;             // kCached: the bounce whose hit ends each path into the cache
+  CacheBox box;
+  RadianceCache* cache;       // kCached: the network the ended paths query
+  ge_cache_query* queries;    // kCached: capacity records
+  ge_cache_sample* samples;   // kRecord: capacity records
+  float4* weights;            // capacity records
+  uint32_t capacity;
+};
+
+// The number of cache records the last wavefront_path_trace with a hook appended (all of them,
+// including any past its capacity). Synchronizes the context's stream.
+hipError_t read_cache_count(ge_rt_ctx* ctx, uint32_t* out);
+
+}  // namespace ge

@@ -1,0 +1,590 @@
+// rt_mlp.hip — the radiance cache's network on CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16: BF16
+// operands, FP32 accumulation).
+//
+// Inference is one fused kernel: a workgroup stages every weight in LDS once (58 KB, already laid
+// out as MFMA fragments), and each wave pushes 32 queries at a time through all six layers with
+// the activations in registers. It computes H^T = W X^T, so a layer's accumulator (lane l: neurons
+// 16 t + 4 (l / 16) + i of query l % 16) becomes the next layer's B operand in the same lane once
+// the next layer's weights are stored with their inputs in that order; nothing moves between lanes.
+//
+// Training runs layer by layer through one strided GEMM kernel (the activations of 2^16 samples are
+// a few MB and stay in L2): the forward products, the deltas through ReLU's derivative, and the
+// weight gradients split over the batch, then Adam on the FP32 master weights, which also refreshes
+// the three BF16 copies (the GEMM's [out][in], the backward pass's [in][out], and the fragments).
+#include <cmath>
+#include <vector>
+
+#include "rt_internal.h"
+#include "rt_mlp.h"
+
+namespace ge {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+  // Round to nearest even; NaN stays a quiet NaN.
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ __bf16 to_bf16(float f) { return __builtin_bit_cast(__bf16, bf16_bits(f)); }
+__device__ __forceinline__ float from_bf16(__bf16 h) {
+  return __uint_as_float(uint32_t(__builtin_bit_cast(uint16_t, h)) << 16);
+}
+
+// The encoded query's input `idx` (the order rt_mlp.h describes), from its 11 features.
+__device__ float encode_value(const float* f, uint32_t idx) {
+  const uint32_t kb = idx / 32, g = (idx % 32) / 8, j = idx % 8, G = 6 * g + kb;
+  if (G < 21) return cache_band(f[G / 3], int(4 * (G % 3) + j / 2), (j & 1u) != 0);
+  if (G == 21) return j < 4 ? f[7 + j] : (j == 4 ? 1.0f : 0.0f);
+  return 0.0f;
+}
+
+// Which fragment slot holds weight (row r, column c) of layer l: rows in 16-row tiles t, columns in
+// 32-wide k blocks kb, lane = 16 g + r % 16, element j. Layer 0 takes its inputs in order
+// (c % 32 = 8 g + j); later layers in the order the previous layer's accumulators sit in a lane
+// (c % 32 = 4 g + j for j < 4, 16 + 4 g + j - 4 after).
+__host__ __device__ inline uint32_t frag_index(int l, uint32_t r, uint32_t c) {
+  const uint32_t t = r / 16, kb = c / 32, w = c % 32;
+  uint32_t g, j;
+  if (l == 0) {
+    g = w / 8;
+    j = w % 8;
+  } else if (w < 16) {
+    g = w / 4;
+    j = w % 4;
+  } else {
+    g = (w - 16) / 4;
+    j = 4 + (w - 16) % 4;
+  }
+  const uint32_t lane = 16 * g + r % 16;
+  return cache_offset(l) + ((t * (cache_dim(l) / 32) + kb) * 64 + lane) * 8 + j;
+}
+
+// ---- fused inference -----------------------------------------------------------------------------
+struct InferArgs {
+  const float* queries;  // the first 11 floats of each record are the features, the 12th the tag
+  uint32_t stride;       // floats between records (a multiple of 4)
+  uint32_t n;
+  const uint32_t* n_dev;  // optional device count (clamped to n)
+  const uint4* frag;      // kCacheWeights bf16 in fragment order
+  float* out;             // n x 3 (infer)
+  const float4* w;        // per query: throughput (apply)
+  float4* result;         // per path: radiance (apply)
+};
+
+__device__ __forceinline__ bf16x8 pack8(const float* v) {
+  bf16x8 r;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = to_bf16(v[j]);
+  return r;
+}
+
+// ReLU, then the two B operands of the next layer from this layer's four accumulator tiles.
+__device__ __forceinline__ void relu_pack(const f32x4 acc[4], bf16x8 out[2]) {
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = acc[2 * kb][i] > 0.0f ? acc[2 * kb][i] : 0.0f;
+      v[4 + i] = acc[2 * kb + 1][i] > 0.0f ? acc[2 * kb + 1][i] : 0.0f;
+    }
+    out[kb] = pack8(v);
+  }
+}
+
+constexpr uint32_t kInferWaves = 4;
+constexpr uint32_t kInferQueries = 32;  // a wave's queries per pass: two 16-query tiles
+
+template <bool Apply>
+__global__ void __launch_bounds__(256, 2) k_cache_infer(InferArgs a) {
+  __shared__ uint4 frag[kCacheWeights / 8];
+  for (uint32_t i = threadIdx.x; i < kCacheWeights / 8; i += blockDim.x) frag[i] = a.frag[i];
+  __syncthreads();
+  const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, g = lane >> 4, r = lane & 15u;
+  for (uint32_t base = (blockIdx.x * kInferWaves + wave) * kInferQueries; base < n;
+       base += gridDim.x * kInferWaves * kInferQueries) {
+    // The weights are the same every pass; an offset the compiler cannot see through keeps it
+    // from hoisting all 58 KB of fragment reads into registers ahead of the loop.
+    uint32_t lane_at = lane;
+    asm volatile("" : "+v"(lane_at));
+    const bf16x8* fr = reinterpret_cast<const bf16x8*>(frag) + lane_at;
+    // ---- the two queries' features this lane encodes ----
+    // Lane group g encodes geometric values 2g (k blocks 0..2) and 2g + 1 (3..5) of each query;
+    // group 3's second half is the material tail (albedo, roughness, the constant 1).
+    float xa[2], xb[2], tail[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t q = base + 16 * h + r;
+      float f[12];
+      if (q < n) {
+        const float4* rec = reinterpret_cast<const float4*>(a.queries + size_t(q) * a.stride);
+        const float4 f0 = rec[0], f1 = rec[1], f2 = rec[2];
+        f[0] = f0.x; f[1] = f0.y; f[2] = f0.z; f[3] = f0.w;
+        f[4] = f1.x; f[5] = f1.y; f[6] = f1.z; f[7] = f1.w;
+        f[8] = f2.x; f[9] = f2.y; f[10] = f2.z; f[11] = 0.0f;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) f[k] = 0.0f;
+      }
+      xa[h] = g == 0 ? f[0] : (g == 1 ? f[2] : (g == 2 ? f[4] : f[6]));
+      xb[h] = g == 0 ? f[1] : (g == 1 ? f[3] : f[5]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) tail[h][k] = f[7 + k];
+    }
+    f32x4 acc[2][4];
+    // ---- layer 0: 192 -> 64, each k block encoded just before its products ----
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 6; ++kb) {
+      bf16x8 x[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float v[8];
+        if (kb < 3 || g < 3) {
+          const float xv = kb < 3 ? xa[h] : xb[h];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int b = 4 * (kb % 3) + jj;
+            v[2 * jj] = cache_band(xv, b, false);
+            v[2 * jj + 1] = cache_band(xv, b, true);
+          }
+        } else if (kb == 3) {
+          v[0] = tail[h][0]; v[1] = tail[h][1]; v[2] = tail[h][2]; v[3] = tail[h][3];
+          v[4] = 1.0f; v[5] = 0.0f; v[6] = 0.0f; v[7] = 0.0f;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) v[k] = 0.0f;
+        }
+        x[h] = pack8(v);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const bf16x8 wf = fr[(t * 6 + kb) * 64];
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, x[0], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, x[1], acc[1][t], 0, 0, 0);
+      }
+    }
+    // ---- hidden layers 1..4: 64 -> 64 ----
+#pragma unroll
+    for (int l = 1; l < kCacheLayers - 1; ++l) {
+      bf16x8 hin[2][2];
+      relu_pack(acc[0], hin[0]);
+      relu_pack(acc[1], hin[1]);
+      const bf16x8* lw = fr + cache_offset(l) / 8;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const bf16x8 wf = lw[(t * 2 + kb) * 64];
+          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[0][kb], acc[0][t], 0, 0, 0);
+          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[1][kb], acc[1][t], 0, 0, 0);
+        }
+      }
+    }
+    // ---- output layer: 64 -> 16 (rows 0..2 used) ----
+    bf16x8 hin[2][2];
+    relu_pack(acc[0], hin[0]);
+    relu_pack(acc[1], hin[1]);
+    const bf16x8* lw = fr + cache_offset(kCacheLayers - 1) / 8;
+    f32x4 o[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      const bf16x8 wf = lw[kb * 64];
+      o[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[0][kb], o[0], 0, 0, 0);
+      o[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[1][kb], o[1], 0, 0, 0);
+    }
+    // Lanes 0..15 hold rows 0..3 of query r.
+    if (g == 0) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t q = base + 16 * h + r;
+        if (q >= n) continue;
+        if (Apply) {
+          const uint32_t pid = __float_as_uint(a.queries[size_t(q) * a.stride + 11]);
+          const float4 w = a.w[q];
+          float4 res = a.result[pid];
+          res.x += w.x * (o[h][0] > 0.0f ? o[h][0] : 0.0f);
+          res.y += w.y * (o[h][1] > 0.0f ? o[h][1] : 0.0f);
+          res.z += w.z * (o[h][2] > 0.0f ? o[h][2] : 0.0f);
+          a.result[pid] = res;
+        } else {
+          a.out[size_t(q) * 3 + 0] = o[h][0];
+          a.out[size_t(q) * 3 + 1] = o[h][1];
+          a.out[size_t(q) * 3 + 2] = o[h][2];
+        }
+      }
+    }
+  }
+}
+
+// ---- training --------------------------------------------------------------------------------------
+constexpr int kTile = 64;  // GEMM output tile: 64 x 64, four waves of 16 rows
+constexpr int kStep = 32;  // k per MFMA step
+
+enum : uint32_t {
+  kStoreF32 = 0,       // C (fp32) = acc, split z at c + z * c_z
+  kStoreBf16Relu = 1,  // C (bf16) = max(acc, 0)
+  kStoreBf16Mask = 2,  // C (bf16) = acc where mask(m, n) > 0, else 0 (ReLU's derivative)
+};
+
+struct GemmArgs {
+  const __bf16* a;  // A(m, k) = a[m * a_m + k * a_k]
+  const __bf16* b;  // B(n, k) = b[n * b_n + k * b_k]
+  void* c;          // C(m, n) = c[m * c_m + n]
+  const __bf16* mask;
+  uint32_t m, n, k;
+  uint32_t a_m, a_k, b_n, b_k, c_m, c_z;
+  uint32_t k_split;  // k a split covers (a multiple of kStep); gridDim.z splits
+  uint32_t epilogue;
+};
+
+// C = A B^T over 64 x 64 tiles. Tiles of A and B are staged in LDS 32 k at a time, threads running
+// along whichever of the operand's axes is contiguous in memory.
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) __bf16 sa[kTile][kStep + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 sb[kTile][kStep + 8];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t m0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
+  const uint32_t k_begin = blockIdx.z * g.k_split;
+  const uint32_t k_end = min(k_begin + g.k_split, g.k);
+  const bool a_rows = g.a_m == 1, b_rows = g.b_n == 1;  // contiguous along m (n) rather than k
+  f32x4 acc[4];
+  for (int j = 0; j < 4; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t k0 = k_begin; k0 < k_end; k0 += kStep) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const uint32_t idx = uint32_t(e) * 256u + t;
+      {
+        const uint32_t r = a_rows ? idx % kTile : idx / kStep, kk = a_rows ? idx / kTile : idx % kStep;
+        const uint32_t gm = m0 + r, gk = k0 + kk;
+        sa[r][kk] = (gm < g.m && gk < k_end) ? g.a[size_t(gm) * g.a_m + size_t(gk) * g.a_k] : __bf16(0.0f);
+      }
+      {
+        const uint32_t r = b_rows ? idx % kTile : idx / kStep, kk = b_rows ? idx / kTile : idx % kStep;
+        const uint32_t gn = n0 + r, gk = k0 + kk;
+        sb[r][kk] = (gn < g.n && gk < k_end) ? g.b[size_t(gn) * g.b_n + size_t(gk) * g.b_k] : __bf16(0.0f);
+      }
+    }
+    __syncthreads();
+    const bf16x8 af = *reinterpret_cast<const bf16x8*>(&sa[wave * 16 + (lane & 15u)][8 * (lane >> 4)]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bf16x8 bf = *reinterpret_cast<const bf16x8*>(&sb[j * 16 + (lane & 15u)][8 * (lane >> 4)]);
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t gm = m0 + wave * 16 + 4 * (lane >> 4) + uint32_t(i);
+      const uint32_t gn = n0 + uint32_t(j) * 16 + (lane & 15u);
+      if (gm >= g.m || gn >= g.n) continue;
+      const size_t at = size_t(gm) * g.c_m + gn;
+      const float v = acc[j][i];
+      if (g.epilogue == kStoreF32) {
+        static_cast<float*>(g.c)[size_t(blockIdx.z) * g.c_z + at] = v;
+      } else if (g.epilogue == kStoreBf16Relu) {
+        static_cast<__bf16*>(g.c)[at] = to_bf16(v > 0.0f ? v : 0.0f);
+      } else {
+        static_cast<__bf16*>(g.c)[at] = from_bf16(g.mask[at]) > 0.0f ? to_bf16(v) : __bf16(0.0f);
+      }
+    }
+  }
+}
+
+__global__ void k_reduce_splits(uint32_t count, uint32_t splits, const float* __restrict__ partial,
+                                float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.0f;
+  for (uint32_t z = 0; z < splits; ++z) s += partial[size_t(z) * count + i];
+  out[i] = s;
+}
+
+// Row q of the encoded batch from record q (stride floats apart).
+__global__ void k_encode(const float* __restrict__ records, uint32_t stride, uint32_t n, __bf16* __restrict__ x) {
+  const uint32_t q = blockIdx.x * blockDim.y + threadIdx.y;
+  if (q >= n) return;
+  float f[kCacheFeatures];
+  for (int k = 0; k < kCacheFeatures; ++k) f[k] = records[size_t(q) * stride + k];
+  for (uint32_t idx = threadIdx.x; idx < uint32_t(kCacheInputs); idx += blockDim.x)
+    x[size_t(q) * kCacheInputs + idx] = to_bf16(encode_value(f, idx));
+}
+
+// Relative L2 against the samples' radiance: the output layer's delta (bf16, n x 16, columns 3..
+// zero) and the batch's mean loss added into `loss` (double).
+__global__ void k_loss(const float* __restrict__ pred, const ge_cache_sample* __restrict__ samples, uint32_t stride,
+                       uint32_t n, __bf16* __restrict__ delta, double* loss) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const float inv_n = 1.0f / float(n);
+  double sum = 0.0;
+  if (q < n) {
+    const ge_cache_sample& s = samples[size_t(q) * stride];
+    for (int c = 0; c < kCacheOutPad; ++c) {
+      float d = 0.0f;
+      if (c < 3) {
+        const float p = pred[size_t(q) * kCacheOutPad + c], y = s.radiance[c];
+        const float denom = p * p + 0.01f;
+        d = 2.0f * (p - y) / denom * inv_n;
+        sum += double((p - y) * (p - y) / denom);
+      }
+      delta[size_t(q) * kCacheOutPad + c] = to_bf16(d);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+  if ((threadIdx.x & 63u) == 0 && sum != 0.0) atomicAdd(loss, sum / double(n));
+}
+
+// Adam on the master weights (when `update`), then the three BF16 copies.
+__global__ void k_publish(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                          const float* __restrict__ grad, __bf16* __restrict__ wb, __bf16* __restrict__ wt,
+                          __bf16* __restrict__ frag, float lr, float beta1, float beta2, float eps, float c1, float c2,
+                          int update) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kCacheWeights) return;
+  float wi = w[i];
+  if (update) {
+    const float gr = grad[i];
+    const float mi = beta1 * m[i] + (1.0f - beta1) * gr;
+    const float vi = beta2 * v[i] + (1.0f - beta2) * gr * gr;
+    m[i] = mi;
+    v[i] = vi;
+    wi = wi - lr * (mi / c1) / (__builtin_sqrtf(vi / c2) + eps);
+    w[i] = wi;
+  }
+  int l = 0;
+  while (l + 1 < kCacheLayers && i >= cache_offset(l + 1)) ++l;
+  const uint32_t off = cache_offset(l), rows = cache_dim(l + 1), cols = cache_dim(l);
+  const uint32_t r = (i - off) / cols, c = (i - off) % cols;
+  const __bf16 b = to_bf16(wi);
+  wb[i] = b;
+  wt[off + c * rows + r] = b;
+  frag[frag_index(l, r, c)] = b;
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+
+// Splits of k: each a multiple of kStep, the last possibly short. Returns how many there are.
+uint32_t split_k(uint32_t k, uint32_t want, uint32_t& per) {
+  if (want < 1) want = 1;
+  per = ((k + want - 1) / want + kStep - 1) / kStep * kStep;
+  return (k + per - 1) / per;
+}
+
+hipError_t gemm(hipStream_t st, GemmArgs g, uint32_t want_splits) {
+  const uint32_t splits = split_k(g.k, want_splits, g.k_split);
+  k_gemm<<<dim3(blocks(g.m, kTile), blocks(g.n, kTile), splits), 256, 0, st>>>(g);
+  return hipGetLastError();
+}
+
+constexpr uint32_t kMaxSplits = 64;
+
+}  // namespace
+
+// ---- host side -----------------------------------------------------------------------------------
+hipError_t RadianceCache::init(hipStream_t st, uint32_t seed) {
+  hipError_t e;
+  for (DevBuf* b : {&w, &m, &v, &grad})
+    if ((e = b->reserve(size_t(kCacheWeights) * 4)) != hipSuccess) return e;
+  for (DevBuf* b : {&wb, &wt, &frag})
+    if ((e = b->reserve(size_t(kCacheWeights) * 2)) != hipSuccess) return e;
+  if ((e = partial.reserve(size_t(kMaxSplits) * kCacheInputs * kCacheWidth * 4)) != hipSuccess) return e;
+  if ((e = loss.reserve(8)) != hipSuccess) return e;
+  if (!host_loss && (e = hipHostMalloc(reinterpret_cast<void**>(&host_loss), 8)) != hipSuccess) return e;
+  // He-uniform weights from a splitmix64 sequence; the unused output rows 3.. start at zero.
+  std::vector<float> h(kCacheWeights);
+  uint64_t state = 0x9e3779b97f4a7c15ull * (uint64_t(seed) + 1);
+  for (int l = 0; l < kCacheLayers; ++l) {
+    const uint32_t rows = cache_dim(l + 1), cols = cache_dim(l), off = cache_offset(l);
+    const float bound = std::sqrt(6.0f / float(cols));
+    for (uint32_t i = 0; i < rows * cols; ++i) {
+      state += 0x9e3779b97f4a7c15ull;
+      uint64_t z = state;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+      z ^= z >> 31;
+      const float u = float(z >> 40) * (1.0f / 16777216.0f);
+      const bool unused = l == kCacheLayers - 1 && i / cols >= 3;
+      h[off + i] = unused ? 0.0f : (2.0f * u - 1.0f) * bound;
+    }
+  }
+  return set_weights(st, h.data());
+}
+
+hipError_t RadianceCache::set_weights(hipStream_t st, const float* host) {
+  const size_t bytes = size_t(kCacheWeights) * 4;
+  hipError_t e = hipMemcpyAsync(w.ptr, host, bytes, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m.ptr, 0, bytes, st);
+  if (e == hipSuccess) e = hipMemsetAsync(v.ptr, 0, bytes, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  step = 0;
+  return publish(st, 0.0f, 1.0f, 1.0f, false);
+}
+
+hipError_t RadianceCache::get_weights(hipStream_t st, float* host) {
+  hipError_t e = hipMemcpyAsync(host, w.ptr, size_t(kCacheWeights) * 4, hipMemcpyDeviceToHost, st);
+  return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
+hipError_t RadianceCache::publish(hipStream_t st, float lr, float c1, float c2, bool update) {
+  k_publish<<<blocks(kCacheWeights, 256), 256, 0, st>>>(w.as<float>(), m.as<float>(), v.as<float>(), grad.as<float>(),
+                                                       wb.as<__bf16>(), wt.as<__bf16>(), frag.as<__bf16>(), lr, 0.9f,
+                                                       0.99f, 1e-8f, c1, c2, update ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t RadianceCache::reserve_rows(uint32_t n) {
+  if (n <= capacity) return hipSuccess;
+  hipError_t e;
+  if ((e = x.reserve(size_t(n) * kCacheInputs * 2)) != hipSuccess) return e;
+  for (DevBuf& b : act)
+    if ((e = b.reserve(size_t(n) * kCacheWidth * 2)) != hipSuccess) return e;
+  if ((e = out.reserve(size_t(n) * kCacheOutPad * 4)) != hipSuccess) return e;
+  for (DevBuf& b : delta)
+    if ((e = b.reserve(size_t(n) * kCacheWidth * 2)) != hipSuccess) return e;
+  capacity = n;
+  return hipSuccess;
+}
+
+static uint32_t infer_grid(uint32_t n) {
+  const uint32_t per_block = kInferWaves * kInferQueries;
+  return std::max(1u, std::min(blocks(n, per_block), 512u));  // two workgroups a CU, 256 CUs
+}
+
+hipError_t RadianceCache::infer(hipStream_t st, const float* queries, uint32_t stride, uint32_t n,
+                                const uint32_t* n_dev, float* dst) {
+  if (n == 0) return hipSuccess;
+  if (stride % 4 != 0 || stride < 12) return hipErrorInvalidValue;
+  InferArgs a{};
+  a.queries = queries;
+  a.stride = stride;
+  a.n = n;
+  a.n_dev = n_dev;
+  a.frag = frag.as<uint4>();
+  a.out = dst;
+  k_cache_infer<false><<<infer_grid(n), kInferWaves * 64, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t RadianceCache::apply(hipStream_t st, const ge_cache_query* queries, const float4* wts, uint32_t cap,
+                                const uint32_t* n_dev, float4* result) {
+  if (cap == 0) return hipSuccess;
+  InferArgs a{};
+  a.queries = reinterpret_cast<const float*>(queries);
+  a.stride = sizeof(ge_cache_query) / 4;
+  a.n = cap;
+  a.n_dev = n_dev;
+  a.frag = frag.as<uint4>();
+  a.w = wts;
+  a.result = result;
+  k_cache_infer<true><<<infer_grid(cap), kInferWaves * 64, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t RadianceCache::train(hipStream_t st, const ge_cache_sample* smp, uint32_t stride, uint32_t n, float lr) {
+  if (n == 0) return hipSuccess;
+  hipError_t e;
+  if ((e = reserve_rows(n)) != hipSuccess) return e;
+  k_encode<<<blocks(n, 4), dim3(64, 4), 0, st>>>(reinterpret_cast<const float*>(smp), stride * 16, n, x.as<__bf16>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // Forward, keeping every layer's activations.
+  const __bf16* in = x.as<__bf16>();
+  for (int l = 0; l < kCacheLayers; ++l) {
+    const bool last = l == kCacheLayers - 1;
+    GemmArgs g{};
+    g.a = in;
+    g.a_m = cache_dim(l);
+    g.a_k = 1;
+    g.b = wb.as<__bf16>() + cache_offset(l);
+    g.b_n = cache_dim(l);
+    g.b_k = 1;
+    g.m = n;
+    g.n = cache_dim(l + 1);
+    g.k = cache_dim(l);
+    g.c = last ? out.ptr : act[l].ptr;
+    g.c_m = cache_dim(l + 1);
+    g.epilogue = last ? kStoreF32 : kStoreBf16Relu;
+    if ((e = gemm(st, g, 1)) != hipSuccess) return e;
+    if (!last) in = act[l].as<__bf16>();
+  }
+  if ((e = hipMemsetAsync(loss.ptr, 0, 8, st)) != hipSuccess) return e;
+  k_loss<<<blocks(n, 256), 256, 0, st>>>(out.as<float>(), smp, stride, n, delta[0].as<__bf16>(), loss.as<double>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // Backward: each layer's weight gradient (split over the batch, then summed), then its input's
+  // delta through ReLU's derivative.
+  const uint32_t want = std::max(1u, std::min(kMaxSplits, n / 512u));
+  int cur = 0;
+  for (int l = kCacheLayers - 1; l >= 0; --l) {
+    const uint32_t dout = cache_dim(l + 1), din = cache_dim(l);
+    const __bf16* dz = delta[cur].as<__bf16>();
+    const __bf16* a_in = l == 0 ? x.as<__bf16>() : act[l - 1].as<__bf16>();
+    GemmArgs gw{};
+    gw.a = dz;
+    gw.a_m = 1;
+    gw.a_k = dout;
+    gw.b = a_in;
+    gw.b_n = 1;
+    gw.b_k = din;
+    gw.m = dout;
+    gw.n = din;
+    gw.k = n;
+    gw.c = partial.ptr;
+    gw.c_m = din;
+    gw.c_z = dout * din;
+    gw.epilogue = kStoreF32;
+    uint32_t per;
+    const uint32_t used = split_k(n, want, per);
+    if ((e = gemm(st, gw, want)) != hipSuccess) return e;
+    k_reduce_splits<<<blocks(dout * din, 256), 256, 0, st>>>(dout * din, used, partial.as<float>(),
+                                                             grad.as<float>() + cache_offset(l));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (l == 0) break;
+    GemmArgs gd{};
+    gd.a = dz;
+    gd.a_m = dout;
+    gd.a_k = 1;
+    gd.b = wt.as<__bf16>() + cache_offset(l);
+    gd.b_n = dout;
+    gd.b_k = 1;
+    gd.m = n;
+    gd.n = din;
+    gd.k = dout;
+    gd.c = delta[cur ^ 1].ptr;
+    gd.c_m = din;
+    gd.mask = act[l - 1].as<__bf16>();
+    gd.epilogue = kStoreBf16Mask;
+    if ((e = gemm(st, gd, 1)) != hipSuccess) return e;
+    cur ^= 1;
+  }
+  ++step;
+  const float c1 = 1.0f - std::pow(0.9f, float(step)), c2 = 1.0f - std::pow(0.99f, float(step));
+  return publish(st, lr, c1, c2, true);
+}
+
+void RadianceCache::release() {
+  for (DevBuf* b : {&w, &m, &v, &grad, &wb, &wt, &frag, &partial, &loss, &x, &out, &delta[0], &delta[1], &samples,
+                    &weights})
+    b->release();
+  for (DevBuf& b : act) b.release();
+  if (host_loss) (void)hipHostFree(host_loss);
+  host_loss = nullptr;
+  capacity = 0;
+}
+
+}  // namespace ge

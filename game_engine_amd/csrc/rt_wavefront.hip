@@ -17,6 +17,7 @@
 // idempotent: if its shadow records overflow their buffer, the host grows it and runs the launch
 // again.
 #include "rt_internal.h"
+#include "rt_mlp.h"
 #include "rt_shade.h"
 
 namespace ge {
@@ -165,7 +166,53 @@ struct ShadeArgs {
   ShadowRec* shadow;
   uint32_t* next_queue;
   uint32_t* counts;  // [0] next queue, [1] shadow records
+  // The radiance cache's records (kShadeCached, kShadeRecord), appended at *cache_count and written
+  // while they fit in cache_capacity.
+  ge_cache_query* queries;   // cached: the query where a path ends (tag = path id)
+  ge_cache_sample* samples;  // record: every hit vertex (radiance = L after its emission, for now)
+  float4* weights;           // per record: the throughput at the vertex
+  uint32_t* cache_count;
+  uint32_t cache_capacity;
+  uint32_t cache_depth;      // cached: the bounce whose hit ends the path into the cache
+  CacheBox box;
 };
+
+enum : int {
+  kShadePlain = 0,   // the path tracer
+  kShadeCached = 1,  // paths end into the cache at the hit of bounce cache_depth
+  kShadeRecord = 2,  // the path tracer, recording a training sample at every hit vertex
+};
+
+// Octahedral coordinates of a unit vector in [0, 1]^2: (x, z) / (|x| + |y| + |z|), folded over
+// the diagonals when y < 0.
+__device__ __forceinline__ void octahedral(f3 v, float& u, float& w) {
+  const float s = (fabsf(v.x) + fabsf(v.y)) + fabsf(v.z);
+  float px = v.x / s, pz = v.z / s;
+  if (v.y < 0.0f) {
+    const float fx = (1.0f - fabsf(pz)) * (px >= 0.0f ? 1.0f : -1.0f);
+    const float fz = (1.0f - fabsf(px)) * (pz >= 0.0f ? 1.0f : -1.0f);
+    px = fx;
+    pz = fz;
+  }
+  u = px * 0.5f + 0.5f;
+  w = pz * 0.5f + 0.5f;
+}
+
+// The cache's query at a surface point, looking back along `s.view`.
+__device__ __forceinline__ ge_cache_query cache_query(const CacheBox& box, f3 position, const Surface& s, uint32_t tag) {
+  ge_cache_query q;
+  q.position[0] = (position.x - box.lo[0]) * box.inv[0];
+  q.position[1] = (position.y - box.lo[1]) * box.inv[1];
+  q.position[2] = (position.z - box.lo[2]) * box.inv[2];
+  octahedral(s.view, q.direction[0], q.direction[1]);
+  octahedral(s.normal, q.normal[0], q.normal[1]);
+  q.albedo[0] = s.albedo.x;
+  q.albedo[1] = s.albedo.y;
+  q.albedo[2] = s.albedo.z;
+  q.roughness = s.roughness;
+  q.tag = tag;
+  return q;
+}
 
 // Which lights next-event estimation traces from this surface, in next_event's order: the sun, then
 // each light whose sample is not zero and that is in front of the shading normal.
@@ -173,8 +220,9 @@ __device__ __forceinline__ bool sun_asked(const ge_path_trace_params& p, const S
   return p.sun[3] > 0.0f && dot(s.normal, ld3(p.sun)) > 0.0f;
 }
 
+template <int Mode>
 __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
-  __shared__ uint32_t parts[2][k_shade_threads / 64 + 1];
+  __shared__ uint32_t parts[Mode == kShadePlain ? 2 : 3][k_shade_threads / 64 + 1];
   const uint32_t g = blockIdx.x * k_shade_threads + threadIdx.x;
   const bool valid = g < a.n;
   const uint32_t pid = valid ? a.queue[g] : 0u;
@@ -186,7 +234,7 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   float bias = 0.0f;
   f3 front = splat(0.0f);
   const ge_path_trace_params& p = a.p;
-  bool hit_found = false;
+  bool hit_found = false, cache_here = false;
   if (valid) {
     st = a.in[pid];
     const PathHit hit = a.hits[pid];
@@ -231,18 +279,40 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
       surface.spec[3] = spc.w;
       bias = p.ray_bias + p.ray_bias_steps * a.sv.meshes[in.mesh].quant[3] * in.scale_max;
       front = dot(vx.geometric, surface.view) >= 0.0f ? vx.geometric : -vx.geometric;
+      cache_here = Mode == kShadeCached && st.bounce == a.cache_depth;
       // Count the records first; the same tests run again when they are written.
-      nshadow = sun_asked(p, surface) ? 1u : 0u;
-      for (uint32_t i = 0; i < a.sv.light_count; ++i) {
-        f3 ld, rad;
-        float dist;
-        if (!light_sample(a.sv.lights[i], vx.position, ld, rad, dist)) continue;
-        if (dot(surface.normal, ld) <= 0.0f) continue;
-        ++nshadow;
+      if (!cache_here) {
+        nshadow = sun_asked(p, surface) ? 1u : 0u;
+        for (uint32_t i = 0; i < a.sv.light_count; ++i) {
+          f3 ld, rad;
+          float dist;
+          if (!light_sample(a.sv.lights[i], vx.position, ld, rad, dist)) continue;
+          if (dot(surface.normal, ld) <= 0.0f) continue;
+          ++nshadow;
+        }
       }
     }
   }
   const uint32_t first = block_append(&a.counts[1], nshadow, parts[0]);
+  if (Mode != kShadePlain) {
+    const bool record = hit_found && (Mode == kShadeRecord || cache_here);
+    const uint32_t slot = block_append(a.cache_count, record ? 1u : 0u, parts[Mode == kShadePlain ? 0 : 2]);
+    if (record && slot < a.cache_capacity) {
+      const ge_cache_query q = cache_query(a.box, vx.position, surface, Mode == kShadeCached ? pid : st.bounce);
+      if (Mode == kShadeCached) {
+        a.queries[slot] = q;
+      } else {
+        ge_cache_sample smp;
+        smp.query = q;
+        smp.radiance[0] = st.L[0];
+        smp.radiance[1] = st.L[1];
+        smp.radiance[2] = st.L[2];
+        smp.path = pid;
+        a.samples[slot] = smp;
+      }
+      a.weights[slot] = make_float4(st.T[0], st.T[1], st.T[2], 0.0f);
+    }
+  }
   if (valid && hit_found) {
     const bool fits = first + nshadow <= a.shadow_capacity;
     NeeRec nr;
@@ -281,7 +351,7 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
       }
     }
     // ---- the next direction (path_trace.slang:397-455) ----
-    alive = st.bounce < p.max_bounces;
+    alive = !cache_here && st.bounce < p.max_bounces;
     if (alive) {
       const float n_dot_v = dot(surface.normal, surface.view);
       alive = n_dot_v > 0.0f;
@@ -412,6 +482,20 @@ __global__ void __launch_bounds__(64) k_accumulate(ge_path_trace_params p, const
   coverage[pixel] = before | (covered ? 1u : 0u);
 }
 
+// A training sample's radiance: what the path gathered after the vertex's emission, over the
+// throughput that reached the vertex (zero in a channel the throughput has none of).
+__global__ void __launch_bounds__(256) k_cache_targets(ge_cache_sample* __restrict__ samples,
+                                                       const float4* __restrict__ weights, const uint32_t* count,
+                                                       uint32_t capacity, const float4* __restrict__ result) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= min(*count, capacity)) return;
+  ge_cache_sample& s = samples[i];
+  const float4 R = result[s.path], T = weights[i];
+  s.radiance[0] = T.x > 0.0f ? (R.x - s.radiance[0]) / T.x : 0.0f;
+  s.radiance[1] = T.y > 0.0f ? (R.y - s.radiance[1]) / T.y : 0.0f;
+  s.radiance[2] = T.z > 0.0f ? (R.z - s.radiance[2]) / T.z : 0.0f;
+}
+
 inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
 constexpr size_t k_lds = size_t(k_ring) * k_lanes * 4;
 
@@ -427,7 +511,8 @@ hipError_t stage_mark(ge_rt_ctx* ctx, int kind, int end) {
 }
 
 hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
-                                float4* accum, uint32_t* coverage, unsigned long long* counters) {
+                                float4* accum, uint32_t* coverage, unsigned long long* counters,
+                                const CacheHook* hook) {
   Wavefront& w = ctx->wave;
   const hipStream_t st = ctx->stream;
   const uint32_t pixels = p.width * p.height;
@@ -452,7 +537,8 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     if ((e = w.shadow.reserve(w.shadow_capacity * sizeof(ShadowRec))) != hipSuccess) return e;
     if ((e = w.blocked.reserve(w.shadow_capacity)) != hipSuccess) return e;
   }
-  uint32_t* counts = w.counts.as<uint32_t>();  // [0] first queue; [2] next queue, [3] shadow records, [4] [5] walk heads
+  // [0] first queue; [2] next queue, [3] shadow records, [4] [5] walk heads; [6] cache records
+  uint32_t* counts = w.counts.as<uint32_t>();
   if ((e = hipMemsetAsync(counts, 0, 32, st)) != hipSuccess) return e;
   const uint32_t tiles_x = (p.width + 7) / 8, tiles = tiles_x * ((p.height + 7) / 8);
   k_generate<<<tiles * p.spp, 64, 0, st>>>(p, tiles_x, tiles, w.state[0].as<PathState>(), w.queue[0].as<uint32_t>());
@@ -483,11 +569,25 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     a.nee = w.nee.as<NeeRec>();
     a.next_queue = w.queue[1].as<uint32_t>();
     a.counts = c2;
+    a.cache_count = counts + 6;
+    if (hook) {
+      a.queries = hook->queries;
+      a.samples = hook->samples;
+      a.weights = hook->weights;
+      a.cache_capacity = hook->capacity;
+      a.cache_depth = hook->depth;
+      a.box = hook->box;
+    }
     uint32_t nshadow = 0, next = 0;
     for (;;) {
       a.shadow = w.shadow.as<ShadowRec>();
       a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
-      k_shade<<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+      if (!hook)
+        k_shade<kShadePlain><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+      else if (hook->mode == CacheHook::kCached)
+        k_shade<kShadeCached><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+      else
+        k_shade<kShadeRecord><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if ((e = hipMemcpyAsync(w.host_counts, c2, 8, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
@@ -523,8 +623,27 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     std::swap(w.queue[0], w.queue[1]);
     n = next;
   }
+  if (hook && hook->mode == CacheHook::kRecord) {
+    k_cache_targets<<<blocks(hook->capacity, 256), 256, 0, st>>>(hook->samples, hook->weights, counts + 6,
+                                                                 hook->capacity, w.result.as<float4>());
+    return hipGetLastError();
+  }
+  if (hook) {
+    // Each cached path's radiance += throughput x max(prediction, 0) before the samples are summed.
+    if ((e = hook->cache->apply(st, hook->queries, hook->weights, hook->capacity, counts + 6,
+                                w.result.as<float4>())) != hipSuccess)
+      return e;
+  }
   k_accumulate<<<blocks(pixels, 64), 64, 0, st>>>(p, w.result.as<float4>(), accum, coverage);
   return hipGetLastError();
+}
+
+hipError_t read_cache_count(ge_rt_ctx* ctx, uint32_t* out) {
+  Wavefront& w = ctx->wave;
+  hipError_t e = hipMemcpyAsync(w.host_counts + 8, w.counts.as<uint32_t>() + 6, 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) *out = w.host_counts[8];
+  return e;
 }
 
 }  // namespace ge

@@ -67,6 +67,15 @@ def lib():
             "ge_rt_scene_update_materials": ([vp, vp, u32, u32, vp], i32),
             "ge_rt_scene_update_lights": ([vp, vp, u32, u32, vp], i32),
             "ge_rt_path_adjoint": ([vp, vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, vp, vp, u32], i32),
+            "ge_rt_cache_create": ([vp, vp, vp, u32, C.POINTER(vp)], i32),
+            "ge_rt_cache_destroy": ([vp, vp], None),
+            "ge_rt_cache_weights": ([vp, vp, vp, vp], i32),
+            "ge_rt_cache_infer": ([vp, vp, vp, u32, vp, u32], i32),
+            "ge_rt_cache_train": ([vp, vp, vp, u32, C.c_float, C.POINTER(C.c_double), u32], i32),
+            "ge_rt_cache_collect": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), vp, u32, C.POINTER(u32), u32], i32),
+            "ge_rt_cache_update": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), u32, u32, C.c_float,
+                                    C.POINTER(C.c_double), u32], i32),
+            "ge_rt_path_trace_cached": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), u32, vp, vp, u32], i32),
             "ge_rt_stats": ([vp, C.POINTER(abi.Counters)], i32),
             "ge_rt_reset_stats": ([vp], i32),
         }
@@ -82,7 +91,9 @@ EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchroni
            "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_bvh_info",
            "ge_rt_bvh_download", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
            "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve", "ge_rt_stats",
-           "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint")
+           "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint",
+           "ge_rt_cache_create", "ge_rt_cache_destroy", "ge_rt_cache_weights", "ge_rt_cache_infer",
+           "ge_rt_cache_train", "ge_rt_cache_collect", "ge_rt_cache_update", "ge_rt_path_trace_cached")
 
 
 def _ptr(a):
@@ -254,6 +265,88 @@ class Scene:
         flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
         self.ctx.check(lib().ge_rt_shadow_resolve(self.ctx.h, self.h, C.byref(params), _ptr(words), _ptr(out), flags))
         return out
+
+
+class RadianceCache:
+    """The radiance cache (ge_rt_cache_*): an MLP on the GPU's matrix cores, trained online on the
+    path tracer's own path suffixes, and the cached tracer that ends its paths into it."""
+
+    def __init__(self, ctx, box_min, box_max, seed=0):
+        self.ctx = ctx
+        lo = (C.c_float * 3)(*[float(v) for v in box_min])
+        hi = (C.c_float * 3)(*[float(v) for v in box_max])
+        h = C.c_void_p()
+        ctx.check(lib().ge_rt_cache_create(ctx.h, lo, hi, seed, C.byref(h)))
+        self.h = h
+        self.box_min = np.array(lo, dtype=np.float32)
+        self.box_max = np.array(hi, dtype=np.float32)
+
+    def close(self):
+        if self.h:
+            lib().ge_rt_cache_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def box_inv(self):
+        """The normalisation the backend applies: 1 / (max - min) in float32."""
+        return (np.float32(1.0) / (self.box_max - self.box_min)).astype(np.float32)
+
+    def weights(self):
+        w = np.zeros(abi.CACHE_WEIGHTS, dtype=np.float32)
+        self.ctx.check(lib().ge_rt_cache_weights(self.ctx.h, self.h, w.ctypes.data, None))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == abi.CACHE_WEIGHTS
+        self.ctx.check(lib().ge_rt_cache_weights(self.ctx.h, self.h, None, w.ctypes.data))
+
+    def infer(self, queries, out=None, device=False, n=None):
+        n = len(queries) if n is None else n
+        if not device:
+            queries = np.ascontiguousarray(queries, dtype=abi.CACHE_QUERY_DTYPE)
+            out = np.zeros((n, 3), dtype=np.float32) if out is None else out
+        self.ctx.check(lib().ge_rt_cache_infer(self.ctx.h, self.h, _ptr(queries), n, _ptr(out),
+                                               abi.DEVICE_PTR if device else 0))
+        return out
+
+    def train(self, samples, learning_rate, device=False, n=None):
+        n = len(samples) if n is None else n
+        if not device:
+            samples = np.ascontiguousarray(samples, dtype=abi.CACHE_SAMPLE_DTYPE)
+        loss = C.c_double()
+        self.ctx.check(lib().ge_rt_cache_train(self.ctx.h, self.h, _ptr(samples), n, learning_rate, C.byref(loss),
+                                               abi.DEVICE_PTR if device else 0))
+        return loss.value
+
+    def collect(self, scene, params, capacity):
+        """Training samples of params' paths: (samples[:min(count, capacity)], count)."""
+        out = np.zeros(capacity, dtype=abi.CACHE_SAMPLE_DTYPE)
+        count = C.c_uint32()
+        self.ctx.check(lib().ge_rt_cache_collect(self.ctx.h, scene.h, self.h, C.byref(params), out.ctypes.data,
+                                                 capacity, C.byref(count), 0))
+        return out[:min(count.value, capacity)], count.value
+
+    def update(self, scene, params, samples=1 << 16, steps=4, learning_rate=1e-2):
+        loss = C.c_double()
+        self.ctx.check(lib().ge_rt_cache_update(self.ctx.h, scene.h, self.h, C.byref(params), samples, steps,
+                                                learning_rate, C.byref(loss), 0))
+        return loss.value
+
+    def path_trace(self, scene, params, depth=1, accum=None, coverage=None, count=False, device=False):
+        n = params.width * params.height
+        if not device:
+            accum = np.zeros((n, 4), dtype=np.float32) if accum is None else accum
+            coverage = np.zeros(n, dtype=np.uint32) if coverage is None else coverage
+        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        self.ctx.check(lib().ge_rt_path_trace_cached(self.ctx.h, scene.h, self.h, C.byref(params), depth, _ptr(accum),
+                                                     _ptr(coverage), flags))
+        return accum, coverage
 
 
 def tonemap(ctx, params, accum, coverage, device=False, output=None):

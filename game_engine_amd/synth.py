@@ -122,6 +122,31 @@ class Synth:
             "instances": view(d.instances, abi.InstanceDesc, d.instance_count),
         }
 
+    def bounds(self, margin=0.01):
+        """The scene's box in the path tracer's frame (the world's origin: synth frames leave the
+        eye at zero): each instance's transform of its mesh's cluster spheres, grown by `margin`
+        of its size on each side."""
+        a = self.arrays()
+        cl, meshes, inst = a["clusters"], a["meshes"], a["instances"]
+        centers = cl["center"].astype(np.float64)
+        radius = cl["radius"].astype(np.float64)[:, None]
+        mesh_lo, mesh_hi = [], []
+        for first, count in zip(meshes["first_cluster"], meshes["cluster_count"]):
+            sl = slice(int(first), int(first) + int(count))
+            mesh_lo.append(np.min(centers[sl] - radius[sl], axis=0))
+            mesh_hi.append(np.max(centers[sl] + radius[sl], axis=0))
+        mesh_lo, mesh_hi = np.array(mesh_lo), np.array(mesh_hi)
+        rows = inst["rows"].astype(np.float64)  # (n, 3, 4)
+        t = rows[:, :, 3] + 64.0 * inst["cell"].astype(np.float64)
+        lo_m, hi_m = mesh_lo[inst["mesh"]], mesh_hi[inst["mesh"]]
+        # The box of a transformed box: centre through the linear part, half-size through |linear|.
+        c, h = 0.5 * (lo_m + hi_m), 0.5 * (hi_m - lo_m)
+        wc = np.einsum("nij,nj->ni", rows[:, :, :3], c) + t
+        wh = np.einsum("nij,nj->ni", np.abs(rows[:, :, :3]), h)
+        lo, hi = (wc - wh).min(axis=0), (wc + wh).max(axis=0)
+        pad = margin * (hi - lo) + 1e-3
+        return (lo - pad).astype(np.float32), (hi + pad).astype(np.float32)
+
     @property
     def triangle_count(self):
         return self._desc.triangle_count

@@ -469,6 +469,78 @@ int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_sha
 int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
                              const uint64_t* visibility, uint32_t* out, uint32_t flags);
 
+/* ---- The radiance cache ---------------------------------------------------------------------------
+ * No counterpart in the reference: its GI cache is a world-space hash grid filled by a compute pass
+ * (domain/gfx/include/domain/gfx/gi_cache.h:1-30, domain/gfx/shaders/gi_update.slang:85-115). Here
+ * a small MLP predicts the radiance a surface point scatters toward a direction (everything but its
+ * own emission), trained online on the path tracer's own path suffixes, and the cached tracer ends
+ * its paths into it. The network: a 12-band sin/cos encoding of position (3), direction (2) and
+ * normal (2), plus albedo (3), roughness (1) and a constant 1, padded to GE_CACHE_INPUTS; five
+ * hidden ReLU layers of GE_CACHE_WIDTH; GE_CACHE_OUTPUTS output rows of which the first three are
+ * RGB; no biases. BF16 weights and activations, FP32 accumulation (CDNA4 matrix cores), FP32 master
+ * weights and Adam (beta1 0.9, beta2 0.99, epsilon 1e-8) on the relative L2 loss
+ * (p - y)^2 / (p^2 + 0.01), p held fixed in the denominator. */
+#define GE_CACHE_INPUTS 192
+#define GE_CACHE_WIDTH 64
+#define GE_CACHE_OUTPUTS 16
+#define GE_CACHE_LAYERS 6
+#define GE_CACHE_WEIGHTS 29696 /* 192*64 + 4*64*64 + 64*16: layer by layer, each [out][in] row-major */
+
+/* One query, 48 bytes. position: the point in the cache's box, (p - box_min) / (box_max - box_min);
+ * direction (towards the previous path vertex) and normal (the shading normal): octahedral
+ * coordinates mapped to [0, 1]^2 (x, z over |x| + |y| + |z|, folded when y < 0). tag: the tracer's
+ * bookkeeping (the path's bounce in a training sample), not an input. */
+typedef struct ge_cache_query {
+  float position[3];
+  float direction[2];
+  float normal[2];
+  float albedo[3];
+  float roughness;
+  uint32_t tag;
+} ge_cache_query;
+
+/* One training sample, 64 bytes: a path vertex and the radiance its path suffix brought back to it,
+ * (path radiance - radiance up to the vertex's emission) / throughput at the vertex. */
+typedef struct ge_cache_sample {
+  ge_cache_query query;
+  float radiance[3];
+  uint32_t path; /* sample * width * height + y * width + x */
+} ge_cache_sample;
+
+typedef struct ge_rt_cache ge_rt_cache;
+
+/* A cache with He-uniform weights from `seed`, for positions in [box_min, box_max] (the path
+ * tracer's frame: eye-relative, as its hit positions). */
+int32_t ge_rt_cache_create(ge_rt_ctx* ctx, const float box_min[3], const float box_max[3], uint32_t seed,
+                           ge_rt_cache** out);
+void ge_rt_cache_destroy(ge_rt_ctx* ctx, ge_rt_cache* cache);
+/* The FP32 master weights (GE_CACHE_WEIGHTS floats, host memory): `get` receives them, `set`
+ * replaces them and restarts the optimizer. Either may be NULL. */
+int32_t ge_rt_cache_weights(ge_rt_ctx* ctx, ge_rt_cache* cache, float* get, const float* set);
+/* The network's raw outputs (n x 3 floats) for n queries. */
+int32_t ge_rt_cache_infer(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_query* queries, uint32_t n,
+                          float* radiance, uint32_t flags);
+/* One optimizer step over n samples; `loss` (host, may be NULL) receives the batch's mean loss. */
+int32_t ge_rt_cache_train(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_sample* samples, uint32_t n,
+                          float learning_rate, double* loss, uint32_t flags);
+/* Training samples from the path tracer: traces params' paths (its estimator, to max_bounces) and
+ * writes one sample per path vertex that hit a surface, up to `capacity`; *count receives how many
+ * vertices there were. Samples are in no particular order. */
+int32_t ge_rt_cache_collect(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
+                            const ge_path_trace_params* params, ge_cache_sample* samples, uint32_t capacity,
+                            uint32_t* count, uint32_t flags);
+/* Online training: collect (at most `samples` samples) and `steps` optimizer steps over interleaved
+ * slices of them. `loss` (host, may be NULL) receives the last step's mean loss. */
+int32_t ge_rt_cache_update(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
+                           const ge_path_trace_params* params, uint32_t samples, uint32_t steps, float learning_rate,
+                           double* loss, uint32_t flags);
+/* The cached tracer: ge_rt_path_trace with each path ended at the surface its ray of bounce `depth`
+ * hits, which adds its emission and throughput x max(cache prediction, 0) instead of next-event
+ * estimation and further bounces. Same accum / coverage contract as ge_rt_path_trace. */
+int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
+                                const ge_path_trace_params* params, uint32_t depth, float* accum, uint32_t* coverage,
+                                uint32_t flags);
+
 /* Counters and the last kernel/build times (ReferenceFrame::trace_ms, reference.h:95). */
 int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out);
 int32_t ge_rt_reset_stats(ge_rt_ctx* ctx);
@@ -492,6 +564,8 @@ static_assert(sizeof(ge_bvh_node) == 64, "a BVH node is one 64-byte fetch");
 static_assert(sizeof(ge_bvh_triangle) == 48, "a BVH triangle is 48 bytes");
 static_assert(sizeof(ge_shadow_receiver) == 32, "a shadow receiver is 32 bytes");
 static_assert(sizeof(ge_shadow_resolve_params) == 128, "shadow resolve params are 128 bytes");
+static_assert(sizeof(ge_cache_query) == 48, "a cache query is 48 bytes");
+static_assert(sizeof(ge_cache_sample) == 64, "a cache sample is 64 bytes");
 #endif
 
 #endif /* GE_RT_H */

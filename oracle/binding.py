@@ -78,6 +78,10 @@ def lib():
         L.ge_oracle_path_trace64.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, C.c_uint32,
                                              C.c_uint32, C.c_int32]
         L.ge_oracle_tonemap.argtypes = [C.POINTER(abi.PathTraceParams), vp, vp, vp]
+        L.ge_oracle_path_cached.argtypes = [vp, C.POINTER(abi.PathTraceParams), C.c_uint32, vp, vp, vp, vp, vp,
+                                            C.c_int32]
+        L.ge_oracle_cache_samples.restype = C.c_uint32
+        L.ge_oracle_cache_samples.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, C.c_uint32, C.c_int32]
         L.ge_oracle_shadow_pass.argtypes = [vp, C.POINTER(abi.ShadowParams), vp, vp, C.c_int32]
         L.ge_oracle_counters.argtypes = [vp]
         L.ge_oracle_max_threads.restype = C.c_int32
@@ -194,6 +198,29 @@ class Scene:
         if st != 0:
             raise ValueError(f"oracle path adjoint: status {st}")
         return (img64 if mode == FP64 else img32), gm, gl
+
+    def path_cached(self, params, depth, box_lo, box_inv, threads=0):
+        """The cached tracer's view of each path (sample-major ids): (radiance before the cache
+        term (paths, 3), weight (paths, 4): throughput and whether the path ended into the cache,
+        queries (paths,))."""
+        paths = params.width * params.height * params.spp
+        rad = np.zeros((paths, 3), dtype=np.float32)
+        w = np.zeros((paths, 4), dtype=np.float32)
+        q = np.zeros(paths, dtype=abi.CACHE_QUERY_DTYPE)
+        lo = np.ascontiguousarray(box_lo, dtype=np.float32)
+        inv = np.ascontiguousarray(box_inv, dtype=np.float32)
+        lib().ge_oracle_path_cached(self.h, C.byref(params), depth, lo.ctypes.data, inv.ctypes.data, rad.ctypes.data,
+                                    w.ctypes.data, q.ctypes.data, threads)
+        return rad, w, q
+
+    def cache_samples(self, params, box_lo, box_inv, capacity=1 << 20, threads=0):
+        """The plain tracer's training samples, ordered by path then bounce: (samples, count)."""
+        out = np.zeros(capacity, dtype=abi.CACHE_SAMPLE_DTYPE)
+        lo = np.ascontiguousarray(box_lo, dtype=np.float32)
+        inv = np.ascontiguousarray(box_inv, dtype=np.float32)
+        count = lib().ge_oracle_cache_samples(self.h, C.byref(params), lo.ctypes.data, inv.ctypes.data,
+                                              out.ctypes.data, capacity, threads)
+        return out[:min(count, capacity)], count
 
     def update_materials(self, first, records):
         assert lib().ge_oracle_update_materials(self.h, first, len(records), C.addressof(records)) == 0

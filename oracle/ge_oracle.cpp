@@ -1091,8 +1091,56 @@ V3<R> next_event(const PtCtx<R>& c, const BrdfSurface<R>& surface, V3<R> positio
   return direct;
 }
 
+// ---- the radiance cache's view of a path (no counterpart in the reference) ------------------------
+// Where the cached tracer hands a path to the cache (mode 1: the surface its ray of bounce `depth`
+// hits adds its emission and then the cache's answer, with no next-event estimation and no further
+// bounce), and the training samples of the plain tracer's paths (mode 2: every hit vertex, with the
+// radiance and throughput the path had there). FP32 only: the queries are compared to the bit.
+struct CacheTap {
+  int mode = 0;
+  uint32_t depth = 0;
+  const float* lo = nullptr;
+  const float* inv = nullptr;
+  bool queried = false;            // mode 1
+  ge_cache_query query{};
+  float T[3] = {0, 0, 0};
+  std::vector<ge_cache_sample> samples;  // mode 2: radiance = L after the vertex's emission, for now
+  std::vector<std::array<float, 3>> throughput;
+};
+
+// Octahedral coordinates in [0, 1]^2: (x, z) / (|x| + |y| + |z|), folded over the diagonals when
+// y < 0.
+inline void octahedral(V3<float> v, float& u, float& w) {
+  const float s = (std::fabs(v.x) + std::fabs(v.y)) + std::fabs(v.z);
+  float px = v.x / s, pz = v.z / s;
+  if (v.y < 0.0f) {
+    const float fx = (1.0f - std::fabs(pz)) * (px >= 0.0f ? 1.0f : -1.0f);
+    const float fz = (1.0f - std::fabs(px)) * (pz >= 0.0f ? 1.0f : -1.0f);
+    px = fx;
+    pz = fz;
+  }
+  u = px * 0.5f + 0.5f;
+  w = pz * 0.5f + 0.5f;
+}
+
+inline ge_cache_query cache_query(const CacheTap& tap, V3<float> position, const BrdfSurface<float>& s, uint32_t tag) {
+  ge_cache_query q;
+  q.position[0] = (position.x - tap.lo[0]) * tap.inv[0];
+  q.position[1] = (position.y - tap.lo[1]) * tap.inv[1];
+  q.position[2] = (position.z - tap.lo[2]) * tap.inv[2];
+  octahedral(s.view, q.direction[0], q.direction[1]);
+  octahedral(s.normal, q.normal[0], q.normal[1]);
+  q.albedo[0] = s.albedo.x;
+  q.albedo[1] = s.albedo.y;
+  q.albedo[2] = s.albedo.z;
+  q.roughness = s.roughness;
+  q.tag = tag;
+  return q;
+}
+
 template <class R>
-V3<R> trace_path(const PtCtx<R>& c, V3<R> origin, V3<R> direction, uint32_t& rng, bool& hit_primary) {
+V3<R> trace_path(const PtCtx<R>& c, V3<R> origin, V3<R> direction, uint32_t& rng, bool& hit_primary,
+                 CacheTap* tap = nullptr) {
   const ge_path_trace_params& p = *c.p;
   const ge_oracle_scene& s = *c.s;
   hit_primary = false;
@@ -1124,6 +1172,25 @@ V3<R> trace_path(const PtCtx<R>& c, V3<R> origin, V3<R> direction, uint32_t& rng
     surface.roughness = R(mat.albedo[3]);
     surface.metallic = R(mat.emissive[3]);
     for (int k = 0; k < 4; ++k) surface.specular[k] = R(mat.specular[k]);
+    if constexpr (std::is_same_v<R, float>) {
+      if (tap && tap->mode == 1 && bounce == tap->depth) {
+        tap->queried = true;
+        tap->query = cache_query(*tap, sf.position, surface, 0);
+        tap->T[0] = throughput.x;
+        tap->T[1] = throughput.y;
+        tap->T[2] = throughput.z;
+        break;
+      }
+      if (tap && tap->mode == 2) {
+        ge_cache_sample smp{};
+        smp.query = cache_query(*tap, sf.position, surface, bounce);
+        smp.radiance[0] = radiance.x;
+        smp.radiance[1] = radiance.y;
+        smp.radiance[2] = radiance.z;
+        tap->samples.push_back(smp);
+        tap->throughput.push_back({throughput.x, throughput.y, throughput.z});
+      }
+    }
     const ge_mesh_desc& mesh = s.meshes[inst->mesh];
     const R bias = R(p.ray_bias) + R(p.ray_bias_steps) * R(mesh.quant[3]) * R(inst->scale_max);
     const V3<R> front = dot(sf.geometric, surface.view) >= R(0) ? sf.geometric : -sf.geometric;
@@ -1196,6 +1263,27 @@ void path_trace_pixel(const PtCtx<R>& c, uint32_t x, uint32_t y, R sum[3], bool&
   sum[0] = acc.x;
   sum[1] = acc.y;
   sum[2] = acc.z;
+}
+
+// One path of pixel (x, y): sample si of the dispatch (path_trace_pixel's loop body).
+V3<float> trace_one(const PtCtx<float>& c, uint32_t x, uint32_t y, uint32_t si, bool& hit_primary, CacheTap* tap) {
+  const ge_path_trace_params& p = *c.p;
+  const float sx = float(p.width), sy = float(p.height);
+  uint32_t rng = rng_seed(x, y, p.sample_base + si, p.seed);
+  float jx = 0.5f, jy = 0.5f;
+  if (!(p.flags & GE_PT_PIXEL_CENTER)) {
+    jx = rng_next<float>(rng);
+    jy = rng_next<float>(rng);
+  }
+  const float ndx = (float(x) + jx) / sx * 2.0f - 1.0f;
+  const float ndy = 1.0f - (float(y) + jy) / sy * 2.0f;
+  const float* m = p.clip_to_ray;
+  V3<float> dir;
+  dir.x = ((m[0] * ndx + m[4] * ndy) + m[8] * 0.0f) + m[12] * 1.0f;
+  dir.y = ((m[1] * ndx + m[5] * ndy) + m[9] * 0.0f) + m[13] * 1.0f;
+  dir.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
+  dir = normalize(dir);
+  return trace_path<float>(c, load3<float>(p.camera), dir, rng, hit_primary, tap);
 }
 
 // ---- the path-replay adjoint ---------------------------------------------------------------------
@@ -1927,6 +2015,83 @@ int32_t ge_oracle_path_adjoint(const ge_oracle_scene* s, const ge_path_trace_par
       for (size_t k = 0; k < nl; ++k) grad_lights[k] += gl[size_t(t)][k];
   }
   return GE_RT_OK;
+}
+
+void ge_oracle_path_cached(const ge_oracle_scene* s, const ge_path_trace_params* p, uint32_t depth,
+                           const float box_lo[3], const float box_inv[3], float* radiance, float* weight,
+                           ge_cache_query* queries, int32_t threads) {
+  const int nt = resolve_threads(threads);
+  const uint32_t W = p->width, H = p->height;
+  const size_t pixels = size_t(W) * H;
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+    PtCtx<float> c{s, p, &tally};
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < int64_t(pixels) * p->spp; ++i) {
+      const uint32_t si = uint32_t(i / int64_t(pixels));
+      const uint32_t pix = uint32_t(i % int64_t(pixels));
+      CacheTap tap;
+      tap.mode = 1;
+      tap.depth = depth;
+      tap.lo = box_lo;
+      tap.inv = box_inv;
+      bool hit_primary = false;
+      const V3<float> L = trace_one(c, pix % W, pix / W, si, hit_primary, &tap);
+      radiance[i * 3 + 0] = L.x;
+      radiance[i * 3 + 1] = L.y;
+      radiance[i * 3 + 2] = L.z;
+      weight[i * 4 + 0] = tap.T[0];
+      weight[i * 4 + 1] = tap.T[1];
+      weight[i * 4 + 2] = tap.T[2];
+      weight[i * 4 + 3] = tap.queried ? 1.0f : 0.0f;
+      ge_cache_query q = tap.query;
+      q.tag = uint32_t(i);
+      queries[i] = q;
+    }
+    tally.flush();
+  }
+}
+
+uint32_t ge_oracle_cache_samples(const ge_oracle_scene* s, const ge_path_trace_params* p, const float box_lo[3],
+                                 const float box_inv[3], ge_cache_sample* out, uint32_t capacity, int32_t threads) {
+  const int nt = resolve_threads(threads);
+  const uint32_t W = p->width, H = p->height;
+  const size_t pixels = size_t(W) * H, paths = pixels * p->spp;
+  std::vector<std::vector<ge_cache_sample>> per(paths);
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+    PtCtx<float> c{s, p, &tally};
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < int64_t(paths); ++i) {
+      const uint32_t si = uint32_t(i / int64_t(pixels));
+      const uint32_t pix = uint32_t(i % int64_t(pixels));
+      CacheTap tap;
+      tap.mode = 2;
+      tap.lo = box_lo;
+      tap.inv = box_inv;
+      bool hit_primary = false;
+      const V3<float> L = trace_one(c, pix % W, pix / W, si, hit_primary, &tap);
+      // What the path gathered after each vertex's emission, over the throughput that reached it.
+      for (size_t k = 0; k < tap.samples.size(); ++k) {
+        ge_cache_sample& smp = tap.samples[k];
+        const std::array<float, 3>& T = tap.throughput[k];
+        const float R[3] = {L.x, L.y, L.z};
+        for (int ch = 0; ch < 3; ++ch) smp.radiance[ch] = T[ch] > 0.0f ? (R[ch] - smp.radiance[ch]) / T[ch] : 0.0f;
+        smp.path = uint32_t(i);
+      }
+      per[size_t(i)] = std::move(tap.samples);
+    }
+    tally.flush();
+  }
+  uint32_t count = 0;
+  for (const auto& v : per)
+    for (const ge_cache_sample& smp : v) {
+      if (count < capacity) out[count] = smp;
+      ++count;
+    }
+  return count;
 }
 
 int32_t ge_oracle_update_materials(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_resolve_material* m) {

@@ -73,6 +73,17 @@ int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resol
  * values a pixel, the fourth the sample count) and, when `d_image` (dLoss/dI, 4 floats a pixel) is
  * given, the loss gradient over material albedo rgb and roughness (4 doubles a material) and light
  * intensities (sun, then each light). */
+/* The cached tracer's view of each path of the dispatch (path i = sample * width * height + pixel):
+ * radiance[i * 3]: what the path gathered before the cache's term; weight[i * 4]: the throughput
+ * at the surface its ray of bounce `depth` hit and 1 when it ended there into the cache (0: it
+ * missed or ended earlier); queries[i]: that query (tag = i). No counterpart in the reference. */
+void ge_oracle_path_cached(const ge_oracle_scene* s, const ge_path_trace_params* p, uint32_t depth,
+                           const float box_lo[3], const float box_inv[3], float* radiance, float* weight,
+                           ge_cache_query* queries, int32_t threads);
+/* The plain tracer's training samples: one per hit vertex, ordered by path then bounce (query.tag
+ * = bounce, path = i), up to `capacity`; returns how many there were. */
+uint32_t ge_oracle_cache_samples(const ge_oracle_scene* s, const ge_path_trace_params* p, const float box_lo[3],
+                                 const float box_inv[3], ge_cache_sample* out, uint32_t capacity, int32_t threads);
 int32_t ge_oracle_path_adjoint(const ge_oracle_scene* s, const ge_path_trace_params* p, const float* d_image,
                                const ge_resolve_material* sampling, uint32_t mode, float* image_sum, double* image64,
                                double* grad_materials, double* grad_lights, int32_t threads);

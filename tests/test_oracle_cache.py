@@ -1,0 +1,133 @@
+"""Oracle pins for the radiance cache's view of the path tracer (no counterpart in the reference).
+
+The oracle's cached tracer ends a path at the surface its ray of bounce `depth` hits, and its
+training samples record every hit vertex of the plain tracer's paths. Pinned here against what is
+fixed independently of that code:
+- a depth no path reaches leaves the tracer unchanged (the same radiance as the plain tracer, to
+  the bit);
+- a path's radiance is the cached radiance at its bounce-1 vertex plus the throughput there times
+  that vertex's training target (the same path seen both ways, to float rounding);
+- at the camera vertex the sample's target is the path's whole radiance (throughput 1, no
+  emission), bit-exact against the plain tracer;
+- the camera vertex's query holds the primary ray's hit point in the cache's box and the reversed
+  ray direction, computed here from the camera and the closest hit in float64; octahedral
+  coordinates decode to unit vectors.
+"""
+import numpy as np
+import pytest
+
+from game_engine_amd import abi, synth
+from oracle import binding as ob
+
+
+def _decode_oct(uv):
+    """Inverse of the octahedral map (x, z over |x| + |y| + |z|, folded when y < 0)."""
+    p = uv.astype(np.float64) * 2.0 - 1.0
+    x, z = p[:, 0], p[:, 1]
+    y = 1.0 - np.abs(x) - np.abs(z)
+    neg = y < 0
+    fx = np.where(neg, (1.0 - np.abs(z)) * np.where(x >= 0, 1.0, -1.0), x)
+    fz = np.where(neg, (1.0 - np.abs(x)) * np.where(z >= 0, 1.0, -1.0), z)
+    v = np.stack([fx, y, fz], axis=1)
+    return v / np.linalg.norm(v, axis=1)[:, None]
+
+
+@pytest.fixture(scope="module")
+def scene():
+    s = synth.Synth.config(2, 0)  # 4,096 instances of 16 meshes, a sun and 64 point lights
+    o = ob.Scene(s.desc())
+    lo, hi = s.bounds()
+    inv = (np.float32(1) / (hi - lo)).astype(np.float32)
+    return s, o, lo, inv
+
+
+def _frame(s, w=48, h=32, spp=2, bounces=4, center=False):
+    pt, rv = s.frame(w, h)
+    pt.spp = spp
+    pt.max_bounces = bounces
+    pt.flags = abi.PT_PIXEL_CENTER if center else 0
+    return pt, rv
+
+
+def test_unreached_depth_is_the_plain_tracer(scene):
+    s, o, lo, inv = scene
+    pt, _ = _frame(s)
+    rad, w, q = o.path_cached(pt, 99, lo, inv)
+    assert not w[:, 3].any()
+    acc, _ = o.path_trace(pt)
+    n = pt.width * pt.height
+    total = np.zeros((n, 3), dtype=np.float32)
+    for si in range(pt.spp):  # the per-pixel sum in sample order, as the tracer forms it
+        total = total + rad[si * n:(si + 1) * n]
+    assert np.array_equal(total, acc[:, :3])
+
+
+def test_cached_radiance_plus_target_is_the_path(scene):
+    s, o, lo, inv = scene
+    pt, _ = _frame(s, spp=1)
+    rad, w, q = o.path_cached(pt, 1, lo, inv)
+    samples, count = o.cache_samples(pt, lo, inv)
+    assert count == len(samples) and count > 0.5 * pt.width * pt.height
+    acc, _ = o.path_trace(pt)
+    at1 = samples[samples["query"]["tag"] == 1]
+    assert len(at1) > 0.15 * pt.width * pt.height
+    paths = at1["path"]
+    assert w[paths, 3].all()  # every path with a bounce-1 vertex ended there into the cache
+    # The same vertex seen both ways: its query, and L = L_before + T * target.
+    for f in ("position", "direction", "normal", "albedo", "roughness"):
+        assert np.array_equal(q[paths][f], at1["query"][f]), f
+    whole = rad[paths].astype(np.float64) + w[paths, :3].astype(np.float64) * at1["radiance"].astype(np.float64)
+    ref = acc[paths, :3].astype(np.float64)
+    assert np.allclose(whole, ref, rtol=1e-5, atol=1e-6 * max(1.0, np.abs(ref).max()))
+    # Paths without a bounce-1 vertex never asked the cache.
+    others = np.setdiff1d(np.arange(len(w)), paths)
+    assert not w[others, 3].any()
+
+
+def test_camera_vertex_sample_is_the_whole_path(scene):
+    s, o, lo, inv = scene
+    pt, rv = _frame(s, spp=1, center=True)
+    samples, count = o.cache_samples(pt, lo, inv)
+    cam = samples[samples["query"]["tag"] == 0]
+    acc, _ = o.path_trace(pt)
+    # The synthetic scenes emit nothing: the target at the camera vertex is the path's radiance.
+    assert np.array_equal(cam["radiance"], acc[cam["path"], :3])
+    # The query: the primary hit in the box, looking back along the primary ray.
+    W, H = pt.width, pt.height
+    pix = cam["path"].astype(np.int64)
+    x, y = (pix % W).astype(np.float64), (pix // W).astype(np.float64)
+    ndx = (x + 0.5) / W * 2 - 1
+    ndy = 1 - (y + 0.5) / H * 2
+    m = np.array(pt.clip_to_ray, dtype=np.float64)
+    d = np.stack([m[r] * ndx + m[4 + r] * ndy + m[12 + r] for r in range(3)], axis=1)
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    assert np.allclose(_decode_oct(cam["query"]["direction"]), -d, atol=2e-6)
+    rays = np.zeros(len(cam), dtype=abi.RAY_DTYPE)
+    rays["origin"] = np.array(pt.camera[:3], dtype=np.float32)
+    rays["dir"] = d.astype(np.float32)
+    rays["tmax"] = np.float32(1e30)
+    hits = o.trace_closest(rays, mask=0xFE)
+    p = np.array(pt.camera[:3], dtype=np.float64) + d * hits["t"][:, None].astype(np.float64)
+    want = (p - lo.astype(np.float64)) * inv.astype(np.float64)
+    assert np.allclose(cam["query"]["position"], want, atol=2e-5)
+    assert ((cam["query"]["position"] >= 0) & (cam["query"]["position"] <= 1)).all()
+    n = _decode_oct(cam["query"]["normal"])
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0)
+    # Albedo in the synthetic range, roughness in (0, 1].
+    assert ((cam["query"]["albedo"] >= 0.05) & (cam["query"]["albedo"] <= 0.95)).all()
+    assert ((cam["query"]["roughness"] > 0) & (cam["query"]["roughness"] <= 1)).all()
+
+
+def test_octahedral_map_round_trips():
+    # The decoder the tests above use, against the map's definition on random unit vectors.
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(4096, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    s = np.abs(v).sum(axis=1)
+    px, pz = v[:, 0] / s, v[:, 2] / s
+    neg = v[:, 1] < 0
+    fx = np.where(neg, (1 - np.abs(pz)) * np.where(px >= 0, 1, -1), px)
+    fz = np.where(neg, (1 - np.abs(px)) * np.where(pz >= 0, 1, -1), pz)
+    uv = np.stack([fx * 0.5 + 0.5, fz * 0.5 + 0.5], axis=1)
+    assert ((uv >= 0) & (uv <= 1)).all()
+    assert np.allclose(_decode_oct(uv), v, atol=1e-12)
