@@ -185,6 +185,203 @@ def shadow_main(args):
     print(json.dumps(result), flush=True)
 
 
+CACHE_METRIC = "1080p time-to-equal-RMSE, cached vs plain tracer"
+PEAK_BF16_TFLOPS = 2500.0
+CACHE_FLOP_PER_QUERY = 2 * (172 * 64 + 4 * 64 * 64 + 64 * 3)  # the network's algorithmic products
+
+
+def cache_main(args):
+    """The radiance cache against the plain tracer on the same 4M-triangle scene at 1920 x 1080.
+
+    A reference of args.ref_spp samples (the plain tracer, seed 0). The plain tracer (seed 1) then
+    accumulates 1, 1, 2, 4, ... spp dispatches, its RMSE against the reference taken at each power
+    of two; the cached tracer (seed 2) trains online (warm-up frames, then one training update
+    and one 1-spp cached frame a step) with its RMSE taken after every frame. Times are wall
+    clock of the dispatches alone (the RMSE reductions run between them, untimed). For each plain
+    budget, the cached tracer's time to reach that RMSE is interpolated on log-log between frames;
+    value = that time over the plain tracer's at args.budget spp."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from game_engine_amd import abi, rt, synth
+
+    log("[bench] synthesizing the 4M-triangle instanced scene")
+    s = synth.Synth.config(2, 0, args.instances)
+    pt0, _ = s.frame(WIDTH, HEIGHT)
+    ctx = rt.Context(0)
+    scene = ctx.upload(s.desc())
+    scene.build()
+    lo, hi = s.bounds()
+    n = WIDTH * HEIGHT
+    flags = abi.PT_PIXEL_CENTER if args.center else 0
+    lights = 0 if args.sun_only else pt0.light_count
+
+    def params(seed, spp, base, width=WIDTH, height=HEIGHT):
+        p, _ = s.frame(width, height)
+        p.seed, p.spp, p.sample_base, p.flags, p.light_count = seed, spp, base, flags, lights
+        return p
+
+    def sync():
+        ctx.synchronize()
+        torch.cuda.synchronize()
+
+    # ---- the reference ----
+    ref_acc = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    cov = torch.zeros(n, dtype=torch.int32, device="cuda")
+    batch = 16
+    t0 = time.perf_counter()
+    last = t0
+    for k in range(args.ref_spp // batch):
+        scene.path_trace(params(0, batch, k * batch), ref_acc.data_ptr(), cov.data_ptr(), device=True)
+        if time.perf_counter() - last > 10:
+            sync()
+            last = time.perf_counter()
+            log(f"[bench] reference {(k + 1) * batch} of {args.ref_spp} spp, {last - t0:.1f} s")
+    sync()
+    ref_s = time.perf_counter() - t0
+    ref = (ref_acc[:, :3] / ref_acc[:, 3:4]).contiguous()
+    log(f"[bench] reference: {args.ref_spp} spp in {ref_s:.1f} s, mean radiance {ref.mean().item():.4f}")
+
+    def rmse(acc):
+        return math.sqrt(torch.mean((acc[:, :3] / acc[:, 3:4] - ref) ** 2).item())
+
+    # ---- the plain tracer ----
+    acc = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    plain = []  # (spp, seconds, rmse)
+    done, spent = 0, 0.0
+    warm = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    scene.path_trace(params(9, 1, 0), warm.data_ptr(), cov.data_ptr(), device=True)  # first-dispatch costs
+    sync()
+    while done < args.budget:
+        step = max(1, done)
+        sync()
+        t = time.perf_counter()
+        scene.path_trace(params(1, step, done), acc.data_ptr(), cov.data_ptr(), device=True)
+        sync()
+        spent += time.perf_counter() - t
+        done += step
+        plain.append((done, spent, rmse(acc)))
+        log(f"[bench] plain {done} spp: {spent * 1e3:.1f} ms, RMSE {plain[-1][2]:.5f}")
+
+    # ---- the cached tracer ----
+    # Warm-up: cached frames at the training resolution whose images are dropped, every path
+    # training; then full frames, 1 spp each, every args.train_every-th one training in-frame.
+    cache = rt.RadianceCache(ctx, lo, hi, seed=0)
+    warm_p = params(3, 1, 0, args.train_width, args.train_height)
+    warm_acc = torch.zeros((args.train_width * args.train_height, 4), dtype=torch.float32, device="cuda")
+    warm_cov = torch.zeros(args.train_width * args.train_height, dtype=torch.int32, device="cuda")
+    tp_warm = abi.CacheTrainParams(samples=args.samples, steps=args.train_steps, learning_rate=args.lr, stride=1)
+    tp = abi.CacheTrainParams(samples=args.samples, steps=args.train_steps, learning_rate=args.lr, stride=0)
+    # First-dispatch costs (the cached tracer's buffers at full size) outside the timed region.
+    cache.path_trace(scene, params(9, 1, 0), depth=1, accum=warm.data_ptr(), coverage=cov.data_ptr(), device=True,
+                     train=tp)
+    cache.set_weights(cache.weights())  # the same initial weights, the optimizer restarted
+    acc.zero_()
+    cached = []  # (frame, seconds, rmse)
+    losses, frame_ms = [], []
+    sync()
+    t = time.perf_counter()
+    for w in range(args.cache_warmup):
+        warm_p.sample_base = 100000 + w
+        _, _, loss = cache.path_trace(scene, warm_p, depth=1, accum=warm_acc.data_ptr(),
+                                      coverage=warm_cov.data_ptr(), device=True, train=tp_warm, want_loss=True)
+        losses.append(loss)
+    sync()
+    spent = time.perf_counter() - t
+    warm_s = spent
+    log(f"[bench] cache warm-up: {args.cache_warmup} frames at {args.train_width}x{args.train_height} in "
+        f"{spent * 1e3:.1f} ms, loss {losses[0]:.3f} -> {losses[-1]:.3f}" if losses else "[bench] no warm-up")
+    target_rmse = min(e for _, _, e in plain)
+    for f in range(args.cache_frames):
+        trains = args.train_every > 0 and f % args.train_every == 0
+        sync()
+        t = time.perf_counter()
+        out = cache.path_trace(scene, params(2, 1, f), depth=1, accum=acc.data_ptr(), coverage=cov.data_ptr(),
+                               device=True, train=tp if trains else None, want_loss=trains)
+        sync()
+        dt = time.perf_counter() - t
+        frame_ms.append((dt * 1e3, trains))
+        spent += dt
+        if trains:
+            losses.append(out[2])
+        cached.append((f + 1, spent, rmse(acc)))
+        if f < 4 or (f + 1) % 16 == 0:
+            log(f"[bench] cached frame {f + 1}: {spent * 1e3:.1f} ms, RMSE {cached[-1][2]:.5f}, last frame "
+                f"{dt * 1e3:.2f} ms{' (trained)' if trains else ''}, loss {losses[-1]:.3f}")
+        if cached[-1][2] <= target_rmse and f >= 8:
+            break
+    train_frames = [m for m, tr in frame_ms if tr]
+    plain_frames = [m for m, tr in frame_ms if not tr]
+
+    def time_to(target):
+        prev = None
+        for fr, t_s, e in cached:
+            if e <= target:
+                if prev is None or prev[2] <= e:
+                    return t_s
+                lt0, lt1 = math.log(prev[1]), math.log(t_s)
+                le0, le1 = math.log(prev[2]), math.log(e)
+                return math.exp(lt0 + (math.log(target) - le0) * (lt1 - lt0) / (le1 - le0))
+            prev = (fr, t_s, e)
+        return None
+
+    table = []
+    for spp, t_s, e in plain:
+        tc = time_to(e)
+        table.append({"plain_spp": spp, "plain_ms": round(t_s * 1e3, 2), "rmse": round(e, 6),
+                      "cached_ms": None if tc is None else round(tc * 1e3, 2),
+                      "ratio": None if tc is None else round(tc / t_s, 4)})
+        log(f"[bench] plain {spp} spp RMSE {e:.5f} in {t_s * 1e3:.1f} ms; cached "
+            + ("never" if tc is None else f"{tc * 1e3:.1f} ms ({tc / t_s:.3f}x)"))
+    head = next((row for row in table if row["plain_spp"] == args.budget), table[-1])
+    frames = len(cached)
+
+    # ---- the network's inference kernel alone, at one frame's query count ----
+    nq = int(0.5 * n)
+    queries = torch.rand((nq, 12), dtype=torch.float32, device="cuda")
+    out = torch.zeros((nq, 3), dtype=torch.float32, device="cuda")
+    infer_ms = []
+    for _ in range(5):
+        cache.infer(queries.data_ptr(), out.data_ptr(), device=True, n=nq)
+        infer_ms.append(ctx.stats()["kernel_ms"])
+    launch_ms = sorted(infer_ms)[len(infer_ms) // 2]
+    achieved = nq * CACHE_FLOP_PER_QUERY / (launch_ms * 1e-3) / 1e12
+    log(f"[bench] k_cache_infer: {nq} queries in {['%.3f' % x for x in infer_ms]} ms, {achieved:.1f} TFLOP/s")
+    value = head["ratio"]
+    result = {
+        "metric": CACHE_METRIC, "value": value, "unit": "x plain tracer's time", "n_gpus": 1,
+        "steps": frames, "warmup": args.cache_warmup,
+        "ms_per_step": round(sum(m for m, _ in frame_ms) / max(frames, 1), 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network, f32 tracer", "data": "synthetic",
+        "config": {"workload": f"radiance cache on the 4M-triangle instanced scene ({args.instances} instances of "
+                               f"16 x 1,024-triangle meshes), {WIDTH}x{HEIGHT}, 1 spp a frame ending into the cache "
+                               f"after bounce 1, trained in-frame (every {args.train_every} frames, at most "
+                               f"{args.samples} self-training samples, {args.train_steps} steps) after "
+                               f"{args.cache_warmup} warm-up frames at {args.train_width}x{args.train_height}, "
+                               f"RMSE against a {args.ref_spp}-spp plain render "
+                               f"({'pixel centres' if args.center else 'jittered'}, "
+                               f"{'sun and sky' if args.sun_only else 'sun + 64 point lights'}, 8 bounces)",
+                   "width": WIDTH, "height": HEIGHT, "reference_spp": args.ref_spp, "plain_budget_spp": args.budget,
+                   "warmup_paths": args.train_width * args.train_height, "samples": args.samples,
+                   "train_every": args.train_every,
+                   "train_steps": args.train_steps, "learning_rate": args.lr, "pixel_center": bool(args.center),
+                   "point_lights": lights, "parallelism": "single GPU"},
+        "plain_vs_cached": table,
+        "cached_frames": frames, "cached_rmse_final": round(cached[-1][2], 6),
+        "cached_warmup_ms": round(warm_s * 1e3, 2),
+        "cached_ms_per_frame": {"training": round(sum(train_frames) / max(len(train_frames), 1), 3),
+                                "not_training": round(sum(plain_frames) / max(len(plain_frames), 1), 3)},
+        "reference_seconds": round(ref_s, 1),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_BF16_TFLOPS, 4), "traffic": None, "kernel": "k_cache_infer",
+                     "launch_ms": round(launch_ms, 4), "queries_per_launch": nq,
+                     "flop_per_query": CACHE_FLOP_PER_QUERY},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,9 +391,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--megakernel", action="store_true", help="one path per lane instead of the wavefront")
-    ap.add_argument("--workload", choices=["path", "shadow"], default="path",
+    ap.add_argument("--workload", choices=["path", "shadow", "cache"], default="path",
                     help="path: the path tracer on the 4M-triangle scene (the headline); shadow: the sun shadow "
-                         "pass on the 50M-triangle scene with per-frame rebuilds")
+                         "pass on the 50M-triangle scene with per-frame rebuilds; cache: time to equal RMSE, "
+                         "radiance-cache tracer against the plain tracer, on the 4M-triangle scene")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="cache workload: the reference's samples")
+    ap.add_argument("--budget", type=int, default=64, help="cache workload: the plain tracer's samples")
+    ap.add_argument("--cache-warmup", type=int, default=32, help="cache workload: warm-up training frames")
+    ap.add_argument("--cache-frames", type=int, default=512, help="cache workload: at most this many frames")
+    ap.add_argument("--samples", type=int, default=1 << 16, help="cache workload: training samples a frame")
+    ap.add_argument("--train-steps", type=int, default=4, help="cache workload: optimizer steps a frame")
+    ap.add_argument("--train-width", type=int, default=256, help="cache workload: warm-up frame width")
+    ap.add_argument("--train-height", type=int, default=144, help="cache workload: warm-up frame height")
+    ap.add_argument("--lr", type=float, default=5e-3, help="cache workload: Adam's learning rate")
+    ap.add_argument("--train-every", type=int, default=4, help="cache workload: frames a training frame")
+    ap.add_argument("--center", action="store_true", help="cache workload: pixel-centre rays throughout")
+    ap.add_argument("--sun-only", action="store_true", help="cache workload: the point lights off throughout")
     ap.add_argument("--groups", type=int, default=16384, help="shadow workload: cluster groups")
     ap.add_argument("--terrain", type=int, default=4096, help="shadow workload: heightfield samples a side")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -209,6 +419,9 @@ def main():
         sys.exit(2)
     if args.workload == "shadow":
         shadow_main(args)
+        return
+    if args.workload == "cache":
+        cache_main(args)
         return
 
     import numpy as np

@@ -116,6 +116,10 @@ CACHE_SAMPLE_DTYPE = np.dtype([("query", CACHE_QUERY_DTYPE), ("radiance", "<f4",
 assert CACHE_QUERY_DTYPE.itemsize == 48 and CACHE_SAMPLE_DTYPE.itemsize == 64
 assert sum(a * b for a, b in zip(CACHE_DIMS[:-1], CACHE_DIMS[1:])) == CACHE_WEIGHTS
 
+class CacheTrainParams(C.Structure):
+    _fields_ = [("samples", u32), ("steps", u32), ("learning_rate", f32), ("stride", u32)]
+
+
 # numpy views of the backend's own records.
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instance_id", "<u4"),
@@ -129,7 +133,7 @@ RECEIVER_DTYPE = np.dtype([("position", "<f4", 3), ("instance_id", "<u4"), ("geo
 
 SIZES = {ClusterDesc: 48, VertexAttributes: 8, MeshDesc: 64, InstanceDesc: 96, FrameEye: 48,
          ResolveMaterial: 128, ResolveLight: 64, PathTraceParams: 368, RayVisibilityParams: 256,
-         ShadowParams: 48, Counters: 104, ShadowResolveParams: 128}
+         ShadowParams: 48, Counters: 104, ShadowResolveParams: 128, CacheTrainParams: 16}
 for _t, _n in SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32

@@ -1077,17 +1077,31 @@ int32_t collect_samples(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* c
   if (r != GE_RT_OK) return r;
   RadianceCache& net = cache->net;
   GE_HIP(ctx, net.samples.reserve(std::max<size_t>(size_t(capacity) * sizeof(ge_cache_sample), 256)));
-  GE_HIP(ctx, net.weights.reserve(std::max<size_t>(size_t(capacity) * 16, 256)));
+  GE_HIP(ctx, net.sample_weights.reserve(std::max<size_t>(size_t(capacity) * 16, 256)));
   CacheHook hook{};
   hook.mode = CacheHook::kRecord;
   hook.box = net.box;
   hook.cache = &net;
   hook.samples = net.samples.as<ge_cache_sample>();
-  hook.weights = net.weights.as<float4>();
-  hook.capacity = capacity;
+  hook.sample_weights = net.sample_weights.as<float4>();
+  hook.sample_capacity = capacity;
   GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, nullptr, nullptr, nullptr, &hook));
-  GE_HIP(ctx, read_cache_count(ctx, count));
+  GE_HIP(ctx, read_cache_counts(ctx, nullptr, count));
+  net.kept = std::min(*count, capacity);
   return GE_RT_OK;
+}
+
+// `steps` optimizer steps over interleaved slices of the first n of the cache's samples: step s
+// trains on samples s, s + k, s + 2k, ... so that every step sees every part of the frame.
+hipError_t train_slices(ge_rt_ctx* ctx, RadianceCache& net, uint32_t n, uint32_t steps, float lr) {
+  const uint32_t k = std::min(std::max(steps, 1u), std::max(n, 1u));
+  const uint32_t per = n / k;
+  if (per == 0) return hipSuccess;
+  for (uint32_t st = 0; st < k; ++st) {
+    const hipError_t e = net.train(ctx->stream, net.samples.as<ge_cache_sample>() + st, k, per, lr);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace
@@ -1110,6 +1124,21 @@ int32_t ge_rt_cache_train(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_sam
   GE_HIP(ctx, cache->net.train(ctx->stream, ds, 1, n, learning_rate));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return read_loss(ctx, cache, loss);
+}
+
+int32_t ge_rt_cache_samples(ge_rt_ctx* ctx, ge_rt_cache* cache, ge_cache_sample* out, uint32_t capacity,
+                            uint32_t* count, uint32_t flags) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache && count && (capacity == 0 || out), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  const uint32_t n = std::min(cache->net.kept, capacity);
+  *count = n;
+  if (n) {
+    const hipMemcpyKind kind = (flags & GE_RT_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    GE_HIP(ctx, hipMemcpyAsync(out, cache->net.samples.ptr, size_t(n) * sizeof(ge_cache_sample), kind, ctx->stream));
+  }
+  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GE_RT_OK;
 }
 
 int32_t ge_rt_cache_collect(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
@@ -1141,48 +1170,72 @@ int32_t ge_rt_cache_update(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache
   uint32_t count = 0;
   int32_t r = collect_samples(ctx, scene, cache, params, samples, &count);
   if (r != GE_RT_OK) return r;
-  const uint32_t n = std::min(count, samples);
-  const uint32_t k = std::min(steps, std::max(n, 1u));
-  const uint32_t per = n / k;
-  if (per == 0) return GE_RT_OK;
-  // Step s trains on samples s, s + k, s + 2k, ...: every step sees every bounce.
-  for (uint32_t st = 0; st < k; ++st)
-    GE_HIP(ctx, cache->net.train(ctx->stream, cache->net.samples.as<ge_cache_sample>() + st, k, per, learning_rate));
+  GE_HIP(ctx, train_slices(ctx, cache->net, std::min(count, samples), steps, learning_rate));
   return read_loss(ctx, cache, loss);
 }
 
 int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
-                                const ge_path_trace_params* params, uint32_t depth, float* accum, uint32_t* coverage,
+                                const ge_path_trace_params* params, uint32_t depth,
+                                const ge_cache_train_params* train, float* accum, uint32_t* coverage, double* loss,
                                 uint32_t flags) {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache, GE_RT_ERROR_INVALID_ARGUMENT, "null cache");
   GE_CHECK(ctx, !(flags & GE_RT_PT_MEGAKERNEL), GE_RT_ERROR_UNSUPPORTED, "the cached tracer runs on the wavefront");
+  const bool training = train && train->samples > 0 && train->steps > 0;
+  GE_CHECK(ctx, !train || finite_rate(train->learning_rate), GE_RT_ERROR_INVALID_ARGUMENT,
+           "the learning rate must be finite and >= 0");
+  if (loss) *loss = 0.0;
   ge_path_trace_params p;
   SceneView svl;
   int32_t r = path_setup(ctx, scene, params, &p, &svl);
   if (r != GE_RT_OK) return r;
   const uint64_t npaths = uint64_t(params->width) * params->height * params->spp;
-  GE_CHECK(ctx, npaths < (1ull << 31), GE_RT_ERROR_INVALID_ARGUMENT, "too many paths in one dispatch");
+  GE_CHECK(ctx, npaths < (1ull << 30), GE_RT_ERROR_INVALID_ARGUMENT, "too many paths in one dispatch");
   float4* d_accum;
   uint32_t* d_cov;
   if ((r = path_outputs(ctx, params, accum, coverage, flags, &d_accum, &d_cov)) != GE_RT_OK) return r;
   p.accum = reinterpret_cast<uint64_t>(d_accum);
   p.coverage = reinterpret_cast<uint64_t>(d_cov);
   RadianceCache& net = cache->net;
-  GE_HIP(ctx, net.samples.reserve(std::max<size_t>(npaths * sizeof(ge_cache_query), 256)));
-  GE_HIP(ctx, net.weights.reserve(std::max<size_t>(npaths * 16, 256)));
   CacheHook hook{};
   hook.mode = CacheHook::kCached;
   hook.depth = depth;
   hook.box = net.box;
   hook.cache = &net;
-  hook.queries = net.samples.as<ge_cache_query>();
-  hook.weights = net.weights.as<float4>();
-  hook.capacity = uint32_t(npaths);
+  if (training) {
+    uint32_t stride = train->stride;
+    if (stride == 0) {
+      const double want = double(train->samples) / std::max(double(net.hit_rate), 1e-3);
+      stride = uint32_t(std::max(1.0, std::floor(double(npaths) / want)));
+    }
+    hook.train_stride = stride;
+    hook.train_phase = params->sample_base % stride;
+    hook.sample_capacity = train->samples;
+    GE_HIP(ctx, net.samples.reserve(size_t(train->samples) * sizeof(ge_cache_sample)));
+    GE_HIP(ctx, net.sample_weights.reserve(size_t(train->samples) * 16));
+    hook.samples = net.samples.as<ge_cache_sample>();
+    hook.sample_weights = net.sample_weights.as<float4>();
+  }
+  // A query where each path ends, and one more where each training path does.
+  const uint64_t qcap = npaths + (training ? npaths / hook.train_stride + 1 : 0);
+  GE_HIP(ctx, net.queries.reserve(std::max<size_t>(qcap * sizeof(ge_cache_query), 256)));
+  GE_HIP(ctx, net.query_weights.reserve(std::max<size_t>(qcap * 16, 256)));
+  hook.queries = net.queries.as<ge_cache_query>();
+  hook.query_weights = net.query_weights.as<float4>();
+  hook.query_capacity = uint32_t(qcap);
   GE_HIP(ctx, begin_timing(ctx));
   unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
   GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters, &hook));
+  if (training) {
+    uint32_t found = 0;
+    GE_HIP(ctx, read_cache_counts(ctx, nullptr, &found));
+    const uint64_t started = npaths / hook.train_stride + (npaths % hook.train_stride > hook.train_phase ? 1 : 0);
+    if (started) net.hit_rate = float(double(found) / double(started));
+    net.kept = std::min(found, train->samples);
+    GE_HIP(ctx, train_slices(ctx, net, std::min(found, train->samples), train->steps, train->learning_rate));
+  }
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
+  if (training && (r = read_loss(ctx, cache, loss)) != GE_RT_OK) return r;
   return path_copy_out(ctx, params, accum, coverage, flags, d_accum, d_cov);
 }
 

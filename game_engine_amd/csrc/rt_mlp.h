@@ -52,9 +52,12 @@ struct RadianceCache {
   DevBuf partial;        // split-k partial weight gradients
   DevBuf loss;           // double
   DevBuf x, act[kCacheLayers - 1], out, delta[2];  // training activations (bf16), output (fp32), deltas (bf16)
-  DevBuf samples, weights;  // the tracer's cache records (ge_cache_sample / ge_cache_query) and float4 side records
+  // The tracer's records: queries and training samples, each with its float4 throughput.
+  DevBuf queries, query_weights, samples, sample_weights;
   uint32_t capacity = 0;    // rows the training buffers hold
   uint32_t step = 0;        // Adam steps taken
+  float hit_rate = 0.5f;    // the share of training paths that reached the cache's vertex, last frame
+  uint32_t kept = 0;        // samples with targets in `samples`, from the last frame that gathered them
   double* host_loss = nullptr;  // pinned
 
   hipError_t init(hipStream_t st, uint32_t seed);
@@ -80,17 +83,21 @@ struct RadianceCache {
 // How the wavefront tracer feeds the cache (wavefront_path_trace's `hook`).
 struct CacheHook {
   enum Mode { kCached = 1, kRecord = 2 } mode;
-  uint32_t depth;             // kCached: the bounce whose hit ends each path into the cache
+  uint32_t depth;              // kCached: the bounce whose hit ends each path into the cache
+  uint32_t train_stride;       // kCached: paths with pid % stride == phase go on to train (0: none)
+  uint32_t train_phase;
   CacheBox box;
-  RadianceCache* cache;       // kCached: the network the ended paths query
-  ge_cache_query* queries;    // kCached: capacity records
-  ge_cache_sample* samples;   // kRecord: capacity records
-  float4* weights;            // capacity records
-  uint32_t capacity;
+  RadianceCache* cache;        // kCached: the network the ended paths query
+  ge_cache_query* queries;     // kCached: query_capacity records (at most two a path)
+  float4* query_weights;
+  uint32_t query_capacity;
+  ge_cache_sample* samples;    // kRecord, or kCached with training: sample_capacity records
+  float4* sample_weights;
+  uint32_t sample_capacity;
 };
 
-// The number of cache records the last wavefront_path_trace with a hook appended (all of them,
-// including any past its capacity). Synchronizes the context's stream.
-hipError_t read_cache_count(ge_rt_ctx* ctx, uint32_t* out);
+// The cache queries and training samples the last wavefront_path_trace with a hook appended (all of
+// them, including any past the capacities). Synchronizes the context's stream.
+hipError_t read_cache_counts(ge_rt_ctx* ctx, uint32_t* queries, uint32_t* samples);
 
 }  // namespace ge

@@ -392,7 +392,7 @@ hipError_t gemm(hipStream_t st, GemmArgs g, uint32_t want_splits) {
   return hipGetLastError();
 }
 
-constexpr uint32_t kMaxSplits = 64;
+constexpr uint32_t kMaxSplits = 256;
 
 }  // namespace
 
@@ -528,7 +528,7 @@ hipError_t RadianceCache::train(hipStream_t st, const ge_cache_sample* smp, uint
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // Backward: each layer's weight gradient (split over the batch, then summed), then its input's
   // delta through ReLU's derivative.
-  const uint32_t want = std::max(1u, std::min(kMaxSplits, n / 512u));
+  const uint32_t want = std::max(1u, std::min(kMaxSplits, n / 128u));  // 4 k steps a workgroup
   int cur = 0;
   for (int l = kCacheLayers - 1; l >= 0; --l) {
     const uint32_t dout = cache_dim(l + 1), din = cache_dim(l);
@@ -578,8 +578,8 @@ hipError_t RadianceCache::train(hipStream_t st, const ge_cache_sample* smp, uint
 }
 
 void RadianceCache::release() {
-  for (DevBuf* b : {&w, &m, &v, &grad, &wb, &wt, &frag, &partial, &loss, &x, &out, &delta[0], &delta[1], &samples,
-                    &weights})
+  for (DevBuf* b : {&w, &m, &v, &grad, &wb, &wt, &frag, &partial, &loss, &x, &out, &delta[0], &delta[1], &queries,
+                    &query_weights, &samples, &sample_weights})
     b->release();
   for (DevBuf& b : act) b.release();
   if (host_loss) (void)hipHostFree(host_loss);

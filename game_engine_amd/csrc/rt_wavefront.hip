@@ -166,22 +166,31 @@ struct ShadeArgs {
   ShadowRec* shadow;
   uint32_t* next_queue;
   uint32_t* counts;  // [0] next queue, [1] shadow records
-  // The radiance cache's records (kShadeCached, kShadeRecord), appended at *cache_count and written
-  // while they fit in cache_capacity.
-  ge_cache_query* queries;   // cached: the query where a path ends (tag = path id)
-  ge_cache_sample* samples;  // record: every hit vertex (radiance = L after its emission, for now)
-  float4* weights;           // per record: the throughput at the vertex
-  uint32_t* cache_count;
-  uint32_t cache_capacity;
+  // The radiance cache (kShadeCached, kShadeRecord). Queries (with their throughput in
+  // query_weights) are appended at counts[6] while they fit in query_capacity; training samples
+  // (with theirs in sample_weights) at counts[7] while they fit in sample_capacity.
+  ge_cache_query* queries;   // cached: where a path ends into the cache (tag = the result slot)
+  float4* query_weights;
+  ge_cache_sample* samples;  // radiance = L after the vertex's emission, until the targets pass
+  float4* sample_weights;
+  uint32_t* cache_counts;
+  uint32_t query_capacity, sample_capacity;
   uint32_t cache_depth;      // cached: the bounce whose hit ends the path into the cache
+  uint32_t train_stride;     // cached: paths with pid % stride == phase train (0: none do)
+  uint32_t train_phase;
+  uint32_t npaths;           // a training path's own radiance goes to result[npaths + pid]
+  float4* result;            // cached: the image's radiance of a path that goes on to train
   CacheBox box;
 };
 
 enum : int {
   kShadePlain = 0,   // the path tracer
-  kShadeCached = 1,  // paths end into the cache at the hit of bounce cache_depth
+  kShadeCached = 1,  // paths end into the cache at the hit of bounce cache_depth; the training
+                     // paths among them record a sample there and go on one more bounce (next-event
+                     // estimation, then the cache at the next hit: self-training)
   kShadeRecord = 2,  // the path tracer, recording a training sample at every hit vertex
 };
+constexpr uint32_t kTrainingPath = 4u;  // PathState::flags: the path goes on past the cache to train
 
 // Octahedral coordinates of a unit vector in [0, 1]^2: (x, z) / (|x| + |y| + |z|), folded over
 // the diagonals when y < 0.
@@ -222,7 +231,7 @@ __device__ __forceinline__ bool sun_asked(const ge_path_trace_params& p, const S
 
 template <int Mode>
 __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
-  __shared__ uint32_t parts[Mode == kShadePlain ? 2 : 3][k_shade_threads / 64 + 1];
+  __shared__ uint32_t parts[Mode == kShadePlain ? 2 : 4][k_shade_threads / 64 + 1];
   const uint32_t g = blockIdx.x * k_shade_threads + threadIdx.x;
   const bool valid = g < a.n;
   const uint32_t pid = valid ? a.queue[g] : 0u;
@@ -234,7 +243,7 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   float bias = 0.0f;
   f3 front = splat(0.0f);
   const ge_path_trace_params& p = a.p;
-  bool hit_found = false, cache_here = false;
+  bool hit_found = false, cache_here = false, train_here = false;
   if (valid) {
     st = a.in[pid];
     const PathHit hit = a.hits[pid];
@@ -279,7 +288,12 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
       surface.spec[3] = spc.w;
       bias = p.ray_bias + p.ray_bias_steps * a.sv.meshes[in.mesh].quant[3] * in.scale_max;
       front = dot(vx.geometric, surface.view) >= 0.0f ? vx.geometric : -vx.geometric;
-      cache_here = Mode == kShadeCached && st.bounce == a.cache_depth;
+      if (Mode == kShadeCached) {
+        const bool training = (st.flags & kTrainingPath) != 0;
+        train_here = !training && st.bounce == a.cache_depth && a.train_stride != 0 &&
+                     pid % a.train_stride == a.train_phase;
+        cache_here = training ? st.bounce == a.cache_depth + 1 : (st.bounce == a.cache_depth && !train_here);
+      }
       // Count the records first; the same tests run again when they are written.
       if (!cache_here) {
         nshadow = sun_asked(p, surface) ? 1u : 0u;
@@ -295,22 +309,36 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   }
   const uint32_t first = block_append(&a.counts[1], nshadow, parts[0]);
   if (Mode != kShadePlain) {
-    const bool record = hit_found && (Mode == kShadeRecord || cache_here);
-    const uint32_t slot = block_append(a.cache_count, record ? 1u : 0u, parts[Mode == kShadePlain ? 0 : 2]);
-    if (record && slot < a.cache_capacity) {
-      const ge_cache_query q = cache_query(a.box, vx.position, surface, Mode == kShadeCached ? pid : st.bounce);
-      if (Mode == kShadeCached) {
-        a.queries[slot] = q;
-      } else {
+    // A query where the path ends into the cache, or where a training path leaves the image's.
+    const bool query = Mode == kShadeCached && (cache_here || train_here);
+    const bool sample = hit_found && (Mode == kShadeRecord || train_here);
+    const uint32_t qslot = block_append(&a.cache_counts[0], query ? 1u : 0u, parts[2]);
+    const uint32_t sslot = block_append(&a.cache_counts[1], sample ? 1u : 0u, parts[3]);
+    if (query || sample) {
+      const ge_cache_query q = cache_query(a.box, vx.position, surface, st.bounce);
+      const float4 T = make_float4(st.T[0], st.T[1], st.T[2], 0.0f);
+      if (query && qslot < a.query_capacity) {
+        ge_cache_query qq = q;
+        qq.tag = (st.flags & kTrainingPath) ? a.npaths + pid : pid;
+        a.queries[qslot] = qq;
+        a.query_weights[qslot] = T;
+      }
+      if (sample && sslot < a.sample_capacity) {
         ge_cache_sample smp;
         smp.query = q;
         smp.radiance[0] = st.L[0];
         smp.radiance[1] = st.L[1];
         smp.radiance[2] = st.L[2];
         smp.path = pid;
-        a.samples[slot] = smp;
+        a.samples[sslot] = smp;
+        a.sample_weights[sslot] = T;
       }
-      a.weights[slot] = make_float4(st.T[0], st.T[1], st.T[2], 0.0f);
+      if (train_here) {
+        // The image's radiance for this path stops here (the cache's answer is added to it); the
+        // path goes on with its own, into result[npaths + pid].
+        a.result[pid] = make_float4(st.L[0], st.L[1], st.L[2], __uint_as_float(st.flags & 1u));
+        st.flags |= kTrainingPath;
+      }
     }
   }
   if (valid && hit_found) {
@@ -440,7 +468,8 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_any(Bvh bvh, const ShadowRe
 __global__ void __launch_bounds__(64) k_resolve(PathState* __restrict__ state, const uint32_t* __restrict__ queue,
                                                 uint32_t n, const NeeRec* __restrict__ nee,
                                                 const ShadowRec* __restrict__ shadow,
-                                                const uint8_t* __restrict__ blocked, float4* __restrict__ result) {
+                                                const uint8_t* __restrict__ blocked, float4* __restrict__ result,
+                                                uint32_t npaths) {
   const uint32_t g = blockIdx.x * 64 + threadIdx.x;
   if (g >= n) return;
   const uint32_t pid = queue[g];
@@ -459,7 +488,9 @@ __global__ void __launch_bounds__(64) k_resolve(PathState* __restrict__ state, c
     st.L[1] = L.y;
     st.L[2] = L.z;
   }
-  if (st.flags & 2u) result[pid] = make_float4(L.x, L.y, L.z, __uint_as_float(st.flags & 1u));
+  // A training path's own radiance goes past the image's (kShadeCached).
+  if (st.flags & 2u)
+    result[(st.flags & kTrainingPath) ? npaths + pid : pid] = make_float4(L.x, L.y, L.z, __uint_as_float(st.flags & 1u));
 }
 
 // accum[p] += (sum over samples in order, spp); coverage |= any primary hit (path_trace.slang:490-494).
@@ -486,11 +517,12 @@ __global__ void __launch_bounds__(64) k_accumulate(ge_path_trace_params p, const
 // throughput that reached the vertex (zero in a channel the throughput has none of).
 __global__ void __launch_bounds__(256) k_cache_targets(ge_cache_sample* __restrict__ samples,
                                                        const float4* __restrict__ weights, const uint32_t* count,
-                                                       uint32_t capacity, const float4* __restrict__ result) {
+                                                       uint32_t capacity, const float4* __restrict__ result,
+                                                       uint32_t offset) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= min(*count, capacity)) return;
   ge_cache_sample& s = samples[i];
-  const float4 R = result[s.path], T = weights[i];
+  const float4 R = result[offset + s.path], T = weights[i];
   s.radiance[0] = T.x > 0.0f ? (R.x - s.radiance[0]) / T.x : 0.0f;
   s.radiance[1] = T.y > 0.0f ? (R.y - s.radiance[1]) / T.y : 0.0f;
   s.radiance[2] = T.z > 0.0f ? (R.z - s.radiance[2]) / T.z : 0.0f;
@@ -525,7 +557,8 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   if ((e = w.state[1].reserve(size_t(npaths) * sizeof(PathState))) != hipSuccess) return e;
   if ((e = w.hits.reserve(size_t(npaths) * sizeof(PathHit))) != hipSuccess) return e;
   if ((e = w.nee.reserve(size_t(npaths) * sizeof(NeeRec))) != hipSuccess) return e;
-  if ((e = w.result.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
+  const bool self_train = hook && hook->mode == CacheHook::kCached && hook->train_stride != 0;
+  if ((e = w.result.reserve(size_t(npaths) * (self_train ? 32 : 16))) != hipSuccess) return e;
   if ((e = w.queue[0].reserve(size_t(npaths) * 4)) != hipSuccess) return e;
   if ((e = w.queue[1].reserve(size_t(npaths) * 4)) != hipSuccess) return e;
   if ((e = w.counts.reserve(256)) != hipSuccess) return e;
@@ -569,13 +602,19 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     a.nee = w.nee.as<NeeRec>();
     a.next_queue = w.queue[1].as<uint32_t>();
     a.counts = c2;
-    a.cache_count = counts + 6;
+    a.cache_counts = counts + 6;
+    a.npaths = npaths;
+    a.result = w.result.as<float4>();
     if (hook) {
       a.queries = hook->queries;
+      a.query_weights = hook->query_weights;
+      a.query_capacity = hook->query_capacity;
       a.samples = hook->samples;
-      a.weights = hook->weights;
-      a.cache_capacity = hook->capacity;
+      a.sample_weights = hook->sample_weights;
+      a.sample_capacity = hook->sample_capacity;
       a.cache_depth = hook->depth;
+      a.train_stride = hook->train_stride;
+      a.train_phase = hook->train_phase;
       a.box = hook->box;
     }
     uint32_t nshadow = 0, next = 0;
@@ -617,32 +656,41 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     }
     k_resolve<<<blocks(n, 64), 64, 0, st>>>(w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n,
                                            w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(), w.blocked.as<uint8_t>(),
-                                           w.result.as<float4>());
+                                           w.result.as<float4>(), npaths);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     cur ^= 1;
     std::swap(w.queue[0], w.queue[1]);
     n = next;
   }
   if (hook && hook->mode == CacheHook::kRecord) {
-    k_cache_targets<<<blocks(hook->capacity, 256), 256, 0, st>>>(hook->samples, hook->weights, counts + 6,
-                                                                 hook->capacity, w.result.as<float4>());
+    k_cache_targets<<<blocks(hook->sample_capacity, 256), 256, 0, st>>>(
+        hook->samples, hook->sample_weights, counts + 7, hook->sample_capacity, w.result.as<float4>(), 0);
     return hipGetLastError();
   }
   if (hook) {
-    // Each cached path's radiance += throughput x max(prediction, 0) before the samples are summed.
-    if ((e = hook->cache->apply(st, hook->queries, hook->weights, hook->capacity, counts + 6,
+    // Each query's result slot += throughput x max(prediction, 0): the image's paths, and the
+    // training paths' own radiance; then the training samples' targets from the latter.
+    if ((e = hook->cache->apply(st, hook->queries, hook->query_weights, hook->query_capacity, counts + 6,
                                 w.result.as<float4>())) != hipSuccess)
       return e;
+    if (self_train) {
+      k_cache_targets<<<blocks(hook->sample_capacity, 256), 256, 0, st>>>(
+          hook->samples, hook->sample_weights, counts + 7, hook->sample_capacity, w.result.as<float4>(), npaths);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
   }
   k_accumulate<<<blocks(pixels, 64), 64, 0, st>>>(p, w.result.as<float4>(), accum, coverage);
   return hipGetLastError();
 }
 
-hipError_t read_cache_count(ge_rt_ctx* ctx, uint32_t* out) {
+hipError_t read_cache_counts(ge_rt_ctx* ctx, uint32_t* queries, uint32_t* samples) {
   Wavefront& w = ctx->wave;
-  hipError_t e = hipMemcpyAsync(w.host_counts + 8, w.counts.as<uint32_t>() + 6, 4, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e = hipMemcpyAsync(w.host_counts + 8, w.counts.as<uint32_t>() + 6, 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e == hipSuccess) *out = w.host_counts[8];
+  if (e == hipSuccess) {
+    if (queries) *queries = w.host_counts[8];
+    if (samples) *samples = w.host_counts[9];
+  }
   return e;
 }
 

@@ -73,9 +73,11 @@ def lib():
             "ge_rt_cache_infer": ([vp, vp, vp, u32, vp, u32], i32),
             "ge_rt_cache_train": ([vp, vp, vp, u32, C.c_float, C.POINTER(C.c_double), u32], i32),
             "ge_rt_cache_collect": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), vp, u32, C.POINTER(u32), u32], i32),
+            "ge_rt_cache_samples": ([vp, vp, vp, u32, C.POINTER(u32), u32], i32),
             "ge_rt_cache_update": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), u32, u32, C.c_float,
                                     C.POINTER(C.c_double), u32], i32),
-            "ge_rt_path_trace_cached": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), u32, vp, vp, u32], i32),
+            "ge_rt_path_trace_cached": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), u32, vp, vp, vp,
+                                         C.POINTER(C.c_double), u32], i32),
             "ge_rt_stats": ([vp, C.POINTER(abi.Counters)], i32),
             "ge_rt_reset_stats": ([vp], i32),
         }
@@ -93,7 +95,8 @@ EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchroni
            "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve", "ge_rt_stats",
            "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint",
            "ge_rt_cache_create", "ge_rt_cache_destroy", "ge_rt_cache_weights", "ge_rt_cache_infer",
-           "ge_rt_cache_train", "ge_rt_cache_collect", "ge_rt_cache_update", "ge_rt_path_trace_cached")
+           "ge_rt_cache_train", "ge_rt_cache_collect", "ge_rt_cache_update", "ge_rt_path_trace_cached",
+           "ge_rt_cache_samples")
 
 
 def _ptr(a):
@@ -332,20 +335,34 @@ class RadianceCache:
                                                  capacity, C.byref(count), 0))
         return out[:min(count.value, capacity)], count.value
 
+    def samples(self, capacity=1 << 20):
+        """The samples the last gathering call kept, with their targets."""
+        out = np.zeros(capacity, dtype=abi.CACHE_SAMPLE_DTYPE)
+        count = C.c_uint32()
+        self.ctx.check(lib().ge_rt_cache_samples(self.ctx.h, self.h, out.ctypes.data, capacity, C.byref(count), 0))
+        return out[:count.value]
+
     def update(self, scene, params, samples=1 << 16, steps=4, learning_rate=1e-2):
         loss = C.c_double()
         self.ctx.check(lib().ge_rt_cache_update(self.ctx.h, scene.h, self.h, C.byref(params), samples, steps,
                                                 learning_rate, C.byref(loss), 0))
         return loss.value
 
-    def path_trace(self, scene, params, depth=1, accum=None, coverage=None, count=False, device=False):
+    def path_trace(self, scene, params, depth=1, accum=None, coverage=None, count=False, device=False, train=None,
+                   want_loss=False):
+        """The cached tracer (ge_rt_path_trace_cached): (accum, coverage), plus the training loss
+        when `train` (an abi.CacheTrainParams) is given and want_loss is set."""
         n = params.width * params.height
         if not device:
             accum = np.zeros((n, 4), dtype=np.float32) if accum is None else accum
             coverage = np.zeros(n, dtype=np.uint32) if coverage is None else coverage
         flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
-        self.ctx.check(lib().ge_rt_path_trace_cached(self.ctx.h, scene.h, self.h, C.byref(params), depth, _ptr(accum),
-                                                     _ptr(coverage), flags))
+        loss = C.c_double()
+        self.ctx.check(lib().ge_rt_path_trace_cached(self.ctx.h, scene.h, self.h, C.byref(params), depth,
+                                                     C.byref(train) if train is not None else None, _ptr(accum),
+                                                     _ptr(coverage), C.byref(loss) if want_loss else None, flags))
+        if want_loss:
+            return accum, coverage, loss.value
         return accum, coverage
 
 

@@ -529,16 +529,36 @@ int32_t ge_rt_cache_train(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_sam
 int32_t ge_rt_cache_collect(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
                             const ge_path_trace_params* params, ge_cache_sample* samples, uint32_t capacity,
                             uint32_t* count, uint32_t flags);
+/* The samples the last ge_rt_cache_collect, ge_rt_cache_update or training ge_rt_path_trace_cached
+ * gathered, their targets filled in: up to `capacity` into `out`; *count receives how many were
+ * kept. */
+int32_t ge_rt_cache_samples(ge_rt_ctx* ctx, ge_rt_cache* cache, ge_cache_sample* out, uint32_t capacity,
+                            uint32_t* count, uint32_t flags);
 /* Online training: collect (at most `samples` samples) and `steps` optimizer steps over interleaved
  * slices of them. `loss` (host, may be NULL) receives the last step's mean loss. */
 int32_t ge_rt_cache_update(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
                            const ge_path_trace_params* params, uint32_t samples, uint32_t steps, float learning_rate,
                            double* loss, uint32_t flags);
+/* Online training inside a cached frame. Every stride-th path (phase sample_base % stride) goes on
+ * past the cache's vertex: it records a training sample there, adds next-event estimation, takes
+ * one more bounce and ends into the cache at the next hit (self-training); its own radiance is
+ * the sample's target. The image keeps the cache's answer at that vertex for these paths too.
+ * After the frame, `steps` optimizer steps run over interleaved slices of the samples. */
+typedef struct ge_cache_train_params {
+  uint32_t samples;    /* the most samples to gather; 0 trains nothing */
+  uint32_t steps;      /* optimizer steps over them */
+  float learning_rate;
+  uint32_t stride;     /* 0: chosen from the last frame so that about `samples` paths reach the vertex */
+} ge_cache_train_params;
+
 /* The cached tracer: ge_rt_path_trace with each path ended at the surface its ray of bounce `depth`
  * hits, which adds its emission and throughput x max(cache prediction, 0) instead of next-event
- * estimation and further bounces. Same accum / coverage contract as ge_rt_path_trace. */
+ * estimation and further bounces. Same accum / coverage contract as ge_rt_path_trace. With `train`
+ * (may be NULL) the frame also trains the cache; `loss` (host, may be NULL) receives the last
+ * step's mean loss, which synchronizes. */
 int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
-                                const ge_path_trace_params* params, uint32_t depth, float* accum, uint32_t* coverage,
+                                const ge_path_trace_params* params, uint32_t depth,
+                                const ge_cache_train_params* train, float* accum, uint32_t* coverage, double* loss,
                                 uint32_t flags);
 
 /* Counters and the last kernel/build times (ReferenceFrame::trace_ms, reference.h:95). */
@@ -566,6 +586,7 @@ static_assert(sizeof(ge_shadow_receiver) == 32, "a shadow receiver is 32 bytes")
 static_assert(sizeof(ge_shadow_resolve_params) == 128, "shadow resolve params are 128 bytes");
 static_assert(sizeof(ge_cache_query) == 48, "a cache query is 48 bytes");
 static_assert(sizeof(ge_cache_sample) == 64, "a cache sample is 64 bytes");
+static_assert(sizeof(ge_cache_train_params) == 16, "cache training params are 16 bytes");
 #endif
 
 #endif /* GE_RT_H */

@@ -80,6 +80,8 @@ def lib():
         L.ge_oracle_tonemap.argtypes = [C.POINTER(abi.PathTraceParams), vp, vp, vp]
         L.ge_oracle_path_cached.argtypes = [vp, C.POINTER(abi.PathTraceParams), C.c_uint32, vp, vp, vp, vp, vp,
                                             C.c_int32]
+        L.ge_oracle_path_cached_training.argtypes = [vp, C.POINTER(abi.PathTraceParams), C.c_uint32, C.c_uint32,
+                                                     C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int32]
         L.ge_oracle_cache_samples.restype = C.c_uint32
         L.ge_oracle_cache_samples.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, C.c_uint32, C.c_int32]
         L.ge_oracle_shadow_pass.argtypes = [vp, C.POINTER(abi.ShadowParams), vp, vp, C.c_int32]
@@ -212,6 +214,24 @@ class Scene:
         lib().ge_oracle_path_cached(self.h, C.byref(params), depth, lo.ctypes.data, inv.ctypes.data, rad.ctypes.data,
                                     w.ctypes.data, q.ctypes.data, threads)
         return rad, w, q
+
+    def path_cached_training(self, params, depth, stride, phase, box_lo, box_inv, threads=0):
+        """The cached tracer with self-training, per path: a dict of the image's radiance, weight
+        and query, the training sample and its weight, and the training path's own radiance,
+        weight and query at its end (see ge_oracle.h)."""
+        paths = params.width * params.height * params.spp
+        out = {"radiance": np.zeros((paths, 3), np.float32), "weight": np.zeros((paths, 4), np.float32),
+               "queries": np.zeros(paths, abi.CACHE_QUERY_DTYPE), "samples": np.zeros(paths, abi.CACHE_SAMPLE_DTYPE),
+               "sample_weight": np.zeros((paths, 4), np.float32), "train_radiance": np.zeros((paths, 3), np.float32),
+               "train_weight": np.zeros((paths, 4), np.float32),
+               "train_queries": np.zeros(paths, abi.CACHE_QUERY_DTYPE)}
+        lo = np.ascontiguousarray(box_lo, dtype=np.float32)
+        inv = np.ascontiguousarray(box_inv, dtype=np.float32)
+        lib().ge_oracle_path_cached_training(
+            self.h, C.byref(params), depth, stride, phase, lo.ctypes.data, inv.ctypes.data,
+            *[out[k].ctypes.data for k in ("radiance", "weight", "queries", "samples", "sample_weight",
+                                           "train_radiance", "train_weight", "train_queries")], threads)
+        return out
 
     def cache_samples(self, params, box_lo, box_inv, capacity=1 << 20, threads=0):
         """The plain tracer's training samples, ordered by path then bounce: (samples, count)."""

@@ -1097,7 +1097,7 @@ V3<R> next_event(const PtCtx<R>& c, const BrdfSurface<R>& surface, V3<R> positio
 // bounce), and the training samples of the plain tracer's paths (mode 2: every hit vertex, with the
 // radiance and throughput the path had there). FP32 only: the queries are compared to the bit.
 struct CacheTap {
-  int mode = 0;
+  int mode = 0;          // 1 cached, 2 record, 3 cached with this path training (self-training)
   uint32_t depth = 0;
   const float* lo = nullptr;
   const float* inv = nullptr;
@@ -1106,6 +1106,12 @@ struct CacheTap {
   float T[3] = {0, 0, 0};
   std::vector<ge_cache_sample> samples;  // mode 2: radiance = L after the vertex's emission, for now
   std::vector<std::array<float, 3>> throughput;
+  // mode 3: the image's radiance where the path left it (L after the emission at `depth`), and
+  // where the path ended into the cache one bounce later.
+  V3<float> image{0, 0, 0};
+  bool queried2 = false;
+  ge_cache_query query2{};
+  float T2[3] = {0, 0, 0};
 };
 
 // Octahedral coordinates in [0, 1]^2: (x, z) / (|x| + |y| + |z|), folded over the diagonals when
@@ -1179,6 +1185,30 @@ V3<R> trace_path(const PtCtx<R>& c, V3<R> origin, V3<R> direction, uint32_t& rng
         tap->T[0] = throughput.x;
         tap->T[1] = throughput.y;
         tap->T[2] = throughput.z;
+        break;
+      }
+      if (tap && tap->mode == 3 && bounce == tap->depth) {
+        // A training path: the image's value stops here; its sample is recorded and it goes on.
+        tap->queried = true;
+        tap->query = cache_query(*tap, sf.position, surface, 0);
+        tap->T[0] = throughput.x;
+        tap->T[1] = throughput.y;
+        tap->T[2] = throughput.z;
+        tap->image = radiance;
+        ge_cache_sample smp{};
+        smp.query = cache_query(*tap, sf.position, surface, bounce);
+        smp.radiance[0] = radiance.x;
+        smp.radiance[1] = radiance.y;
+        smp.radiance[2] = radiance.z;
+        tap->samples.push_back(smp);
+        tap->throughput.push_back({throughput.x, throughput.y, throughput.z});
+      }
+      if (tap && tap->mode == 3 && bounce == tap->depth + 1) {
+        tap->queried2 = true;
+        tap->query2 = cache_query(*tap, sf.position, surface, 0);
+        tap->T2[0] = throughput.x;
+        tap->T2[1] = throughput.y;
+        tap->T2[2] = throughput.z;
         break;
       }
       if (tap && tap->mode == 2) {
@@ -2048,6 +2078,64 @@ void ge_oracle_path_cached(const ge_oracle_scene* s, const ge_path_trace_params*
       ge_cache_query q = tap.query;
       q.tag = uint32_t(i);
       queries[i] = q;
+    }
+    tally.flush();
+  }
+}
+
+void ge_oracle_path_cached_training(const ge_oracle_scene* s, const ge_path_trace_params* p, uint32_t depth,
+                                    uint32_t stride, uint32_t phase, const float box_lo[3], const float box_inv[3],
+                                    float* radiance, float* weight, ge_cache_query* queries, ge_cache_sample* samples,
+                                    float* sample_weight, float* train_radiance, float* train_weight,
+                                    ge_cache_query* train_queries, int32_t threads) {
+  const int nt = resolve_threads(threads);
+  const uint32_t W = p->width, H = p->height;
+  const size_t pixels = size_t(W) * H;
+#pragma omp parallel num_threads(nt)
+  {
+    Tally tally;
+    PtCtx<float> c{s, p, &tally};
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < int64_t(pixels) * p->spp; ++i) {
+      const uint32_t si = uint32_t(i / int64_t(pixels));
+      const uint32_t pix = uint32_t(i % int64_t(pixels));
+      const bool trains = stride != 0 && uint64_t(i) % stride == phase;
+      CacheTap tap;
+      tap.mode = trains ? 3 : 1;
+      tap.depth = depth;
+      tap.lo = box_lo;
+      tap.inv = box_inv;
+      bool hit_primary = false;
+      const V3<float> L = trace_one(c, pix % W, pix / W, si, hit_primary, &tap);
+      const bool sampled = trains && !tap.samples.empty();
+      // The image: the path's radiance, or where a training path left it.
+      const V3<float> img = sampled ? tap.image : L;
+      radiance[i * 3 + 0] = img.x;
+      radiance[i * 3 + 1] = img.y;
+      radiance[i * 3 + 2] = img.z;
+      for (int k = 0; k < 3; ++k) weight[i * 4 + k] = tap.T[k];
+      weight[i * 4 + 3] = tap.queried ? 1.0f : 0.0f;
+      ge_cache_query q = tap.query;
+      q.tag = uint32_t(i);
+      queries[i] = q;
+      ge_cache_sample smp{};
+      std::array<float, 3> T1{0, 0, 0};
+      if (sampled) {
+        smp = tap.samples[0];
+        T1 = tap.throughput[0];
+      }
+      smp.path = uint32_t(i);
+      samples[i] = smp;
+      for (int k = 0; k < 3; ++k) sample_weight[i * 4 + k] = T1[k];
+      sample_weight[i * 4 + 3] = sampled ? 1.0f : 0.0f;
+      train_radiance[i * 3 + 0] = sampled ? L.x : 0.0f;
+      train_radiance[i * 3 + 1] = sampled ? L.y : 0.0f;
+      train_radiance[i * 3 + 2] = sampled ? L.z : 0.0f;
+      for (int k = 0; k < 3; ++k) train_weight[i * 4 + k] = tap.T2[k];
+      train_weight[i * 4 + 3] = tap.queried2 ? 1.0f : 0.0f;
+      ge_cache_query q2 = tap.query2;
+      q2.tag = uint32_t(i);
+      train_queries[i] = q2;
     }
     tally.flush();
   }

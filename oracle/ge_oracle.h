@@ -80,6 +80,18 @@ int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resol
 void ge_oracle_path_cached(const ge_oracle_scene* s, const ge_path_trace_params* p, uint32_t depth,
                            const float box_lo[3], const float box_inv[3], float* radiance, float* weight,
                            ge_cache_query* queries, int32_t threads);
+/* The cached tracer with self-training: paths with i % stride == phase record a sample at the
+ * surface of bounce `depth` (its query, the radiance after its emission as `radiance`, path i) and
+ * go on one bounce, ending into the cache at the next hit. Per path: the image's radiance before
+ * the cache term, its weight and query (as ge_oracle_path_cached); the sample and its throughput
+ * (sample_weight[i * 4 + 3] = 1 when there is one); the training path's own radiance before its
+ * cache term, that term's weight (train_weight[i * 4 + 3] = 1 when it ended into the cache) and
+ * query. No counterpart in the reference. */
+void ge_oracle_path_cached_training(const ge_oracle_scene* s, const ge_path_trace_params* p, uint32_t depth,
+                                    uint32_t stride, uint32_t phase, const float box_lo[3], const float box_inv[3],
+                                    float* radiance, float* weight, ge_cache_query* queries, ge_cache_sample* samples,
+                                    float* sample_weight, float* train_radiance, float* train_weight,
+                                    ge_cache_query* train_queries, int32_t threads);
 /* The plain tracer's training samples: one per hit vertex, ordered by path then bounce (query.tag
  * = bounce, path = i), up to `capacity`; returns how many there were. */
 uint32_t ge_oracle_cache_samples(const ge_oracle_scene* s, const ge_path_trace_params* p, const float box_lo[3],

@@ -181,6 +181,57 @@ def test_cached_tracer_adds_the_network(ctx, c2):
     cache.close()
 
 
+@pytest.mark.parametrize("silent", [True, False])
+def test_self_training_frame_matches_the_oracle(ctx, c2, silent):
+    # One cached frame that trains: every third path goes on past the cache's vertex. The image and,
+    # with a cache that answers zero, the samples and their targets are the oracle's to the bit;
+    # with a real network the targets carry the reference's answer at the training path's end.
+    s, g, o, lo, hi = c2
+    cache = rt.RadianceCache(ctx, lo, hi, seed=6)
+    if silent:
+        cache.set_weights(np.zeros(abi.CACHE_WEIGHTS, dtype=np.float32))
+    w = cache.weights()
+    pt = _frame(s, spp=2, bounces=6)
+    pt.sample_base = 5
+    tp = abi.CacheTrainParams(samples=1 << 16, steps=1, learning_rate=0.0, stride=3)
+    acc, _ = cache.path_trace(g, pt, depth=1, train=tp)
+    assert np.array_equal(cache.weights(), w)  # a zero learning rate moves nothing
+    v = o.path_cached_training(pt, 1, 3, pt.sample_base % 3, cache.box_min, cache.box_inv())
+    n = pt.width * pt.height
+    asked = v["weight"][:, 3:4] > 0
+    pred = np.maximum(ref.forward(w, v["queries"]).numpy().astype(np.float64), 0.0)
+    image = v["radiance"].astype(np.float64) + np.where(asked, v["weight"][:, :3] * pred, 0.0)
+    sampled = v["sample_weight"][:, 3] > 0
+    ended = v["train_weight"][:, 3:4] > 0
+    pred2 = np.maximum(ref.forward(w, v["train_queries"]).numpy().astype(np.float64), 0.0)
+    R = v["train_radiance"].astype(np.float64) + np.where(ended, v["train_weight"][:, :3] * pred2, 0.0)
+    T1 = v["sample_weight"][:, :3]
+    got = cache.samples()
+    got = got[np.argsort(got["path"])]
+    want = v["samples"][sampled]
+    assert len(got) == sampled.sum() > 0
+    assert np.array_equal(got["path"], want["path"])
+    for f in ("position", "direction", "normal", "albedo", "roughness", "tag"):
+        assert np.array_equal(got["query"][f], want["query"][f]), f
+    if silent:
+        assert np.array_equal(acc[:, :3], _by_sample(v["radiance"], n, pt.spp))
+        Rf = v["train_radiance"][sampled]
+        L1 = want["radiance"]
+        tgt = np.where(T1[sampled] > 0, (Rf - L1) / np.where(T1[sampled] > 0, T1[sampled], 1), np.float32(0))
+        assert np.array_equal(got["radiance"], tgt.astype(np.float32))
+    else:
+        scale = max(pred[asked[:, 0]].max(), pred2[ended[:, 0]].max(), 1e-3)
+        err = np.abs(acc[:, :3] - _by_sample(image, n, pt.spp))
+        assert (err <= 3e-2 * scale * _by_sample(np.where(asked, v["weight"][:, :3], 0.0), n, pt.spp)
+                + 1e-5 * (1 + np.abs(image).max())).all()
+        L1 = want["radiance"].astype(np.float64)
+        t = T1[sampled].astype(np.float64)
+        tgt = np.where(t > 0, (R[sampled] - L1) / np.where(t > 0, t, 1), 0.0)
+        bound = 3e-2 * scale * np.where(t > 0, v["train_weight"][sampled, :3] / np.where(t > 0, t, 1), 0.0)
+        assert (np.abs(got["radiance"] - tgt) <= bound + 1e-4 * (1 + np.abs(tgt))).all()
+    cache.close()
+
+
 def test_training_samples_match_the_oracle(ctx, c2):
     s, g, o, lo, hi = c2
     cache = rt.RadianceCache(ctx, lo, hi, seed=0)
@@ -202,9 +253,11 @@ def test_training_samples_match_the_oracle(ctx, c2):
     cache.close()
 
 
-def test_online_training_lowers_the_cached_error(ctx, c2):
+@pytest.mark.parametrize("mode", ["separate", "in_frame"])
+def test_online_training_lowers_the_cached_error(ctx, c2, mode):
     # Sun and sky only, pixel-centre rays: the camera vertex is the same in every sample, so the
-    # error left is the light the cache stands in for.
+    # error left is the light the cache stands in for. Training either on separate plain paths
+    # (ge_rt_cache_update) or inside the cached frames themselves (self-training).
     s, g, o, lo, hi = c2
     cache = rt.RadianceCache(ctx, lo, hi, seed=4)
     view = _frame(s, w=96, h=64, spp=16, bounces=6)
@@ -219,18 +272,26 @@ def test_online_training_lowers_the_cached_error(ctx, c2):
 
     def cached_error():
         view.sample_base = 0
+        view.spp = 16
         acc, _ = cache.path_trace(g, view, depth=1)
         return float(np.sqrt(np.mean((acc[:, :3] / acc[:, 3:4] - truth) ** 2)))
 
     before = cached_error()
-    train = _frame(s, w=96, h=64, spp=1, bounces=6)
-    train.light_count = 0
     losses = []
-    for frame in range(150):
-        train.sample_base = 1000 + frame
-        losses.append(cache.update(g, train, samples=8192, steps=2, learning_rate=5e-3))
+    if mode == "separate":
+        train = _frame(s, w=96, h=64, spp=1, bounces=6)
+        train.light_count = 0
+        for frame in range(150):
+            train.sample_base = 1000 + frame
+            losses.append(cache.update(g, train, samples=8192, steps=2, learning_rate=5e-3))
+    else:
+        tp = abi.CacheTrainParams(samples=8192, steps=2, learning_rate=5e-3, stride=0)
+        view.spp = 4
+        for frame in range(150):
+            view.sample_base = 1000 + 4 * frame
+            losses.append(cache.path_trace(g, view, depth=1, train=tp, want_loss=True)[2])
     after = cached_error()
-    print(f"[online] loss {np.mean(losses[:5]):.4f} -> {np.mean(losses[-5:]):.4f}; "
+    print(f"[online {mode}] loss {np.mean(losses[:5]):.4f} -> {np.mean(losses[-5:]):.4f}; "
           f"cached RMSE {before:.4f} -> {after:.4f}", flush=True)
     assert after < 0.5 * before
     cache.close()

@@ -131,3 +131,32 @@ def test_octahedral_map_round_trips():
     uv = np.stack([fx * 0.5 + 0.5, fz * 0.5 + 0.5], axis=1)
     assert ((uv >= 0) & (uv <= 1)).all()
     assert np.allclose(_decode_oct(uv), v, atol=1e-12)
+
+
+def test_self_training_paths_are_the_tracer_one_bounce_on(scene):
+    # A training path goes on past the cache's vertex with next-event estimation and ends into the
+    # cache at the next hit: with nothing emissive, its radiance before the cache's term is the
+    # plain tracer's at max_bounces = depth (next-event estimation at the vertex, then the end).
+    s, o, lo, inv = scene
+    pt, _ = _frame(s, spp=1, bounces=6)
+    v = o.path_cached_training(pt, 1, 1, 0, lo, inv)
+    short = abi.PathTraceParams.from_buffer_copy(pt)
+    short.max_bounces = 1
+    acc, _ = o.path_trace(short)
+    ended = v["train_weight"][:, 3] > 0
+    assert ended.sum() > 0.05 * len(ended)
+    assert np.array_equal(v["train_radiance"][ended], acc[ended, :3])
+    # The image part is the cached tracer's; the sample is the vertex the image left.
+    rad, w, q = o.path_cached(pt, 1, lo, inv)
+    assert np.array_equal(v["radiance"], rad) and np.array_equal(v["weight"], w)
+    sampled = v["sample_weight"][:, 3] > 0
+    assert np.array_equal(sampled, w[:, 3] > 0)
+    assert np.array_equal(v["sample_weight"][sampled, :3], w[sampled, :3])
+    for f in ("position", "direction", "normal", "albedo", "roughness"):
+        assert np.array_equal(v["samples"]["query"][f][sampled], q[f][sampled]), f
+    assert np.array_equal(v["samples"]["radiance"][sampled], rad[sampled])
+    # Paths off the stride do not train.
+    v3 = o.path_cached_training(pt, 1, 3, 1, lo, inv)
+    idx = np.arange(len(w))
+    assert not v3["sample_weight"][idx % 3 != 1, 3].any()
+    assert np.array_equal(v3["sample_weight"][idx % 3 == 1, 3], sampled[idx % 3 == 1].astype(np.float32))
