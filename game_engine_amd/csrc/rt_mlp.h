@@ -1,5 +1,5 @@
 // rt_mlp.h — the radiance cache's network: sizes, the encoding of a query, and the host-side state
-// (FP32 master weights, Adam moments, BF16 copies in three layouts, training activations).
+// (FP32 master weights, Adam moments, BF16 copies as MFMA fragments, training activations).
 #pragma once
 
 #include "rt_internal.h"
@@ -48,10 +48,10 @@ struct CacheBox {
 struct RadianceCache {
   CacheBox box{{0, 0, 0}, {1, 1, 1}};
   DevBuf w, m, v, grad;  // FP32 master weights, Adam moments, gradient (kCacheWeights each)
-  DevBuf wb, wt, frag;   // BF16 weights: [out][in], [in][out], and the inference kernel's fragments
-  DevBuf partial;        // split-k partial weight gradients
+  DevBuf frag, frag_t;   // BF16 weights as the forward pass's MFMA fragments, and W^T as the backward pass's
+  DevBuf partial;        // the weight gradients' per-split partial sums
   DevBuf loss;           // double
-  DevBuf x, act[kCacheLayers - 1], out, delta[2];  // training activations (bf16), output (fp32), deltas (bf16)
+  DevBuf x, act, dz;     // a training batch's encoded inputs, activations and deltas (bf16)
   // The tracer's records: queries and training samples, each with its float4 throughput.
   DevBuf queries, query_weights, samples, sample_weights;
   uint32_t capacity = 0;    // rows the training buffers hold

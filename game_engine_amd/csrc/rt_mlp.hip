@@ -31,17 +31,6 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) {
   return uint16_t(u >> 16);
 }
 __device__ __forceinline__ __bf16 to_bf16(float f) { return __builtin_bit_cast(__bf16, bf16_bits(f)); }
-__device__ __forceinline__ float from_bf16(__bf16 h) {
-  return __uint_as_float(uint32_t(__builtin_bit_cast(uint16_t, h)) << 16);
-}
-
-// The encoded query's input `idx` (the order rt_mlp.h describes), from its 11 features.
-__device__ float encode_value(const float* f, uint32_t idx) {
-  const uint32_t kb = idx / 32, g = (idx % 32) / 8, j = idx % 8, G = 6 * g + kb;
-  if (G < 21) return cache_band(f[G / 3], int(4 * (G % 3) + j / 2), (j & 1u) != 0);
-  if (G == 21) return j < 4 ? f[7 + j] : (j == 4 ? 1.0f : 0.0f);
-  return 0.0f;
-}
 
 // Which fragment slot holds weight (row r, column c) of layer l: rows in 16-row tiles t, columns in
 // 32-wide k blocks kb, lane = 16 g + r % 16, element j. Layer 0 takes its inputs in order
@@ -231,36 +220,244 @@ __global__ void __launch_bounds__(256, 2) k_cache_infer(InferArgs a) {
 }
 
 // ---- training --------------------------------------------------------------------------------------
-constexpr int kTile = 64;  // GEMM output tile: 64 x 64, four waves of 16 rows
-constexpr int kStep = 32;  // k per MFMA step
+// The backward pass's fragments: layer l >= 1 as W_l^T, rows = its 64 inputs in 16-row tiles t,
+// k = its outputs in 32-wide blocks kb (the output layer's 16 padded with zeros to one block), in
+// the order a layer's accumulators sit in a lane (the forward order of the hidden layers).
+__host__ __device__ constexpr uint32_t frag_t_blocks(int l) { return (cache_dim(l + 1) + 31) / 32; }
+__host__ __device__ constexpr uint32_t frag_t_offset(int l) {
+  uint32_t at = 0;
+  for (int i = 1; i < l; ++i) at += cache_dim(i) * 32 * frag_t_blocks(i);
+  return at;
+}
+constexpr uint32_t kFragT = frag_t_offset(kCacheLayers);  // 18432 bf16
 
-enum : uint32_t {
-  kStoreF32 = 0,       // C (fp32) = acc, split z at c + z * c_z
-  kStoreBf16Relu = 1,  // C (bf16) = max(acc, 0)
-  kStoreBf16Mask = 2,  // C (bf16) = acc where mask(m, n) > 0, else 0 (ReLU's derivative)
+__host__ __device__ inline uint32_t frag_t_index(int l, uint32_t o, uint32_t c) {
+  const uint32_t t = c / 16, kb = o / 32, w = o % 32;
+  const uint32_t g = w < 16 ? w / 4 : (w - 16) / 4, j = w < 16 ? w % 4 : 4 + (w - 16) % 4;
+  return frag_t_offset(l) + ((t * frag_t_blocks(l) + kb) * 64 + 16 * g + c % 16) * 8 + j;
+}
+
+struct TrainArgs {
+  const ge_cache_sample* samples;
+  uint32_t stride;  // records between samples
+  uint32_t n;
+  const uint4* frag;      // forward fragments (staged in LDS)
+  const bf16x8* frag_t;   // backward fragments
+  __bf16* x;              // n x 192: the encoded inputs
+  __bf16* act;            // layer l's activations at act + l n 64 (n x 64), l = 0..4
+  __bf16* dz;             // dLoss/dZ_l at dz + l n 64 (n x 64), l = 0..4; the output's at dz + 5 n 64 (n x 16)
+  double* loss;
 };
+
+__device__ __forceinline__ uint2 pack4(const float* v) {
+  return make_uint2(uint32_t(bf16_bits(v[0])) | (uint32_t(bf16_bits(v[1])) << 16),
+                    uint32_t(bf16_bits(v[2])) | (uint32_t(bf16_bits(v[3])) << 16));
+}
+__device__ __forceinline__ bf16x8 join8(uint2 lo, uint2 hi) {
+  const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+// The relative L2 loss's forward and backward pass, 32 samples a wave: the encoding, six products
+// with the activations kept (bf16, in accumulator order) for ReLU's derivative, the loss and its
+// gradient, and five products back through W^T. Writes the encoded inputs, every activation and
+// every delta for the weight gradients; adds the batch's mean loss into `loss`.
+__global__ void __launch_bounds__(256, 2) k_cache_train_pass(TrainArgs a) {
+  __shared__ uint4 frag[kCacheWeights / 8];
+  for (uint32_t i = threadIdx.x; i < kCacheWeights / 8; i += blockDim.x) frag[i] = a.frag[i];
+  __syncthreads();
+  const uint32_t n = a.n;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, g = lane >> 4, r = lane & 15u;
+  const float inv_n = 1.0f / float(n);
+  double loss_sum = 0.0;
+  for (uint32_t base = (blockIdx.x * kInferWaves + wave) * kInferQueries; base < n;
+       base += gridDim.x * kInferWaves * kInferQueries) {
+    uint32_t lane_at = lane;
+    asm volatile("" : "+v"(lane_at));
+    const bf16x8* fr = reinterpret_cast<const bf16x8*>(frag) + lane_at;
+    const bf16x8* ft = a.frag_t + lane_at;
+    uint32_t q[2];
+    bool live[2];
+    float xa[2], xb[2], tail[2][4], y[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      q[h] = base + 16 * h + r;
+      live[h] = q[h] < n;
+      float f[12];
+      y[h][0] = y[h][1] = y[h][2] = 0.0f;
+      if (live[h]) {
+        const ge_cache_sample& smp = a.samples[size_t(q[h]) * a.stride];
+        const float4* rec = reinterpret_cast<const float4*>(&smp);
+        const float4 f0 = rec[0], f1 = rec[1], f2 = rec[2], f3 = rec[3];
+        f[0] = f0.x; f[1] = f0.y; f[2] = f0.z; f[3] = f0.w;
+        f[4] = f1.x; f[5] = f1.y; f[6] = f1.z; f[7] = f1.w;
+        f[8] = f2.x; f[9] = f2.y; f[10] = f2.z; f[11] = 0.0f;
+        y[h][0] = f3.x; y[h][1] = f3.y; y[h][2] = f3.z;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) f[k] = 0.0f;
+      }
+      xa[h] = g == 0 ? f[0] : (g == 1 ? f[2] : (g == 2 ? f[4] : f[6]));
+      xb[h] = g == 0 ? f[1] : (g == 1 ? f[3] : f[5]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) tail[h][k] = f[7 + k];
+    }
+    // ---- forward ----
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 6; ++kb) {
+      bf16x8 xv[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float v[8];
+        if (kb < 3 || g < 3) {
+          const float val = kb < 3 ? xa[h] : xb[h];
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            const int b = 4 * (kb % 3) + jj;
+            v[2 * jj] = cache_band(val, b, false);
+            v[2 * jj + 1] = cache_band(val, b, true);
+          }
+        } else if (kb == 3) {
+          v[0] = tail[h][0]; v[1] = tail[h][1]; v[2] = tail[h][2]; v[3] = tail[h][3];
+          v[4] = 1.0f; v[5] = 0.0f; v[6] = 0.0f; v[7] = 0.0f;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) v[k] = 0.0f;
+        }
+        xv[h] = pack8(v);
+        if (live[h]) *reinterpret_cast<bf16x8*>(a.x + size_t(q[h]) * kCacheInputs + 32 * kb + 8 * g) = xv[h];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const bf16x8 wf = fr[(t * 6 + kb) * 64];
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xv[0], acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xv[1], acc[1][t], 0, 0, 0);
+      }
+    }
+    uint2 act[kCacheLayers - 1][2][4];  // bf16 activations of hidden layer l, tile t: neurons 16 t + 4 g + i
+#pragma unroll
+    for (int l = 0; l < kCacheLayers - 1; ++l) {
+      bf16x8 hin[2][2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          float v[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = acc[h][t][i] > 0.0f ? acc[h][t][i] : 0.0f;
+          act[l][h][t] = pack4(v);
+          if (live[h])
+            *reinterpret_cast<uint2*>(a.act + (size_t(l) * n + q[h]) * kCacheWidth + 16 * t + 4 * g) = act[l][h][t];
+        }
+        hin[h][0] = join8(act[l][h][0], act[l][h][1]);
+        hin[h][1] = join8(act[l][h][2], act[l][h][3]);
+      }
+      const int nl = l + 1;  // the next layer's product
+      const bf16x8* lw = fr + cache_offset(nl) / 8;
+      const int tiles = int(cache_dim(nl + 1) / 16);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          if (t >= tiles) continue;
+          const bf16x8 wf = lw[(t * 2 + kb) * 64];
+          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[0][kb], acc[0][t], 0, 0, 0);
+          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[1][kb], acc[1][t], 0, 0, 0);
+        }
+      }
+    }
+    // ---- the loss: acc[h][0] holds output rows 4 g + i of sample r ----
+    bf16x8 dzb[2][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float d[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (g == 0 && live[h]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float p = acc[h][0][c], yv = y[h][c];
+          const float denom = p * p + 0.01f;
+          d[c] = 2.0f * (p - yv) / denom * inv_n;
+          loss_sum += double((p - yv) * (p - yv) / denom);
+        }
+      }
+      const uint2 db = pack4(d);
+      if (live[h]) *reinterpret_cast<uint2*>(a.dz + size_t(kCacheLayers - 1) * n * kCacheWidth + size_t(q[h]) * kCacheOutPad + 4 * g) = db;
+      dzb[h][0] = join8(db, make_uint2(0u, 0u));
+      dzb[h][1] = join8(make_uint2(0u, 0u), make_uint2(0u, 0u));
+    }
+    // ---- backward: dA_{l-1} = W_l^T dZ_l, dZ_{l-1} = dA_{l-1} where A_{l-1} > 0 ----
+#pragma unroll
+    for (int l = kCacheLayers - 1; l >= 1; --l) {
+      const bf16x8* lt = ft + frag_t_offset(l) / 8;
+      const int kbs = int(frag_t_blocks(l));
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        if (kb >= kbs) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const bf16x8 wf = lt[(t * kbs + kb) * 64];
+          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, dzb[0][kb], acc[0][t], 0, 0, 0);
+          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, dzb[1][kb], acc[1][t], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        uint2 db[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const uint2 m = act[l - 1][h][t];
+          const uint32_t bits[4] = {m.x & 0xffffu, m.x >> 16, m.y & 0xffffu, m.y >> 16};
+          float v[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = bits[i] != 0u ? acc[h][t][i] : 0.0f;
+          db[t] = pack4(v);
+          if (live[h])
+            *reinterpret_cast<uint2*>(a.dz + (size_t(l - 1) * n + q[h]) * kCacheWidth + 16 * t + 4 * g) = db[t];
+        }
+        dzb[h][0] = join8(db[0], db[1]);
+        dzb[h][1] = join8(db[2], db[3]);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) loss_sum += __shfl_xor(loss_sum, off);
+  if (lane == 0 && loss_sum != 0.0) atomicAdd(a.loss, loss_sum * double(inv_n));
+}
+
+// ---- the weight gradients: dW_l = dZ_l^T A_{l-1}, every layer in one launch, split over samples ----
+constexpr int kTile = 64;  // output tile: 64 x 64, four waves of 16 rows
+constexpr int kStep = 32;  // k per MFMA step
 
 struct GemmArgs {
   const __bf16* a;  // A(m, k) = a[m * a_m + k * a_k]
   const __bf16* b;  // B(n, k) = b[n * b_n + k * b_k]
-  void* c;          // C(m, n) = c[m * c_m + n]
-  const __bf16* mask;
+  float* c;         // C(m, n) of split z at c[z * c_z + m * c_m + n]
   uint32_t m, n, k;
   uint32_t a_m, a_k, b_n, b_k, c_m, c_z;
-  uint32_t k_split;  // k a split covers (a multiple of kStep); gridDim.z splits
-  uint32_t epilogue;
+  uint32_t k_split;  // k a split covers (a multiple of kStep)
 };
 
-// C = A B^T over 64 x 64 tiles. Tiles of A and B are staged in LDS 32 k at a time, threads running
-// along whichever of the operand's axes is contiguous in memory.
-__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+// One 64 x 64 tile (m0, n0) of split z of C = A B^T, A and B staged in LDS 32 k at a time with the
+// threads running along whichever of the operand's axes is contiguous in memory.
+__device__ void gemm_tile(const GemmArgs& g, uint32_t m0, uint32_t n0, uint32_t z) {
   __shared__ __attribute__((aligned(16))) __bf16 sa[kTile][kStep + 8];
   __shared__ __attribute__((aligned(16))) __bf16 sb[kTile][kStep + 8];
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-  const uint32_t m0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
-  const uint32_t k_begin = blockIdx.z * g.k_split;
+  const uint32_t k_begin = z * g.k_split;
   const uint32_t k_end = min(k_begin + g.k_split, g.k);
-  const bool a_rows = g.a_m == 1, b_rows = g.b_n == 1;  // contiguous along m (n) rather than k
+  const bool a_rows = g.a_m == 1, b_rows = g.b_n == 1;
   f32x4 acc[4];
   for (int j = 0; j < 4; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (uint32_t k0 = k_begin; k0 < k_end; k0 += kStep) {
@@ -268,14 +465,14 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     for (int e = 0; e < 8; ++e) {
       const uint32_t idx = uint32_t(e) * 256u + t;
       {
-        const uint32_t r = a_rows ? idx % kTile : idx / kStep, kk = a_rows ? idx / kTile : idx % kStep;
-        const uint32_t gm = m0 + r, gk = k0 + kk;
-        sa[r][kk] = (gm < g.m && gk < k_end) ? g.a[size_t(gm) * g.a_m + size_t(gk) * g.a_k] : __bf16(0.0f);
+        const uint32_t rr = a_rows ? idx % kTile : idx / kStep, kk = a_rows ? idx / kTile : idx % kStep;
+        const uint32_t gm = m0 + rr, gk = k0 + kk;
+        sa[rr][kk] = (gm < g.m && gk < k_end) ? g.a[size_t(gm) * g.a_m + size_t(gk) * g.a_k] : __bf16(0.0f);
       }
       {
-        const uint32_t r = b_rows ? idx % kTile : idx / kStep, kk = b_rows ? idx / kTile : idx % kStep;
-        const uint32_t gn = n0 + r, gk = k0 + kk;
-        sb[r][kk] = (gn < g.n && gk < k_end) ? g.b[size_t(gn) * g.b_n + size_t(gk) * g.b_k] : __bf16(0.0f);
+        const uint32_t rr = b_rows ? idx % kTile : idx / kStep, kk = b_rows ? idx / kTile : idx % kStep;
+        const uint32_t gn = n0 + rr, gk = k0 + kk;
+        sb[rr][kk] = (gn < g.n && gk < k_end) ? g.b[size_t(gn) * g.b_n + size_t(gk) * g.b_k] : __bf16(0.0f);
       }
     }
     __syncthreads();
@@ -293,18 +490,23 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     for (int i = 0; i < 4; ++i) {
       const uint32_t gm = m0 + wave * 16 + 4 * (lane >> 4) + uint32_t(i);
       const uint32_t gn = n0 + uint32_t(j) * 16 + (lane & 15u);
-      if (gm >= g.m || gn >= g.n) continue;
-      const size_t at = size_t(gm) * g.c_m + gn;
-      const float v = acc[j][i];
-      if (g.epilogue == kStoreF32) {
-        static_cast<float*>(g.c)[size_t(blockIdx.z) * g.c_z + at] = v;
-      } else if (g.epilogue == kStoreBf16Relu) {
-        static_cast<__bf16*>(g.c)[at] = to_bf16(v > 0.0f ? v : 0.0f);
-      } else {
-        static_cast<__bf16*>(g.c)[at] = from_bf16(g.mask[at]) > 0.0f ? to_bf16(v) : __bf16(0.0f);
-      }
+      if (gm < g.m && gn < g.n) g.c[size_t(z) * g.c_z + size_t(gm) * g.c_m + gn] = acc[j][i];
     }
   }
+}
+
+// Tiles of every layer's weight gradient: layer 0 is 64 x 192 (three tiles), the hidden layers one
+// tile each, the output layer 16 x 64 (one partial tile).
+constexpr int kGradTiles = 3 + (kCacheLayers - 2) + 1;
+struct GradArgs {
+  GemmArgs layer[kCacheLayers];
+};
+
+__global__ void __launch_bounds__(256) k_weight_grads(GradArgs a) {
+  const int tile = int(blockIdx.x);
+  const int l = tile < 3 ? 0 : tile - 2;
+  const uint32_t n0 = tile < 3 ? uint32_t(tile) * kTile : 0u;
+  gemm_tile(a.layer[l], 0, n0, blockIdx.y);
 }
 
 __global__ void k_reduce_splits(uint32_t count, uint32_t splits, const float* __restrict__ partial,
@@ -316,45 +518,11 @@ __global__ void k_reduce_splits(uint32_t count, uint32_t splits, const float* __
   out[i] = s;
 }
 
-// Row q of the encoded batch from record q (stride floats apart).
-__global__ void k_encode(const float* __restrict__ records, uint32_t stride, uint32_t n, __bf16* __restrict__ x) {
-  const uint32_t q = blockIdx.x * blockDim.y + threadIdx.y;
-  if (q >= n) return;
-  float f[kCacheFeatures];
-  for (int k = 0; k < kCacheFeatures; ++k) f[k] = records[size_t(q) * stride + k];
-  for (uint32_t idx = threadIdx.x; idx < uint32_t(kCacheInputs); idx += blockDim.x)
-    x[size_t(q) * kCacheInputs + idx] = to_bf16(encode_value(f, idx));
-}
-
-// Relative L2 against the samples' radiance: the output layer's delta (bf16, n x 16, columns 3..
-// zero) and the batch's mean loss added into `loss` (double).
-__global__ void k_loss(const float* __restrict__ pred, const ge_cache_sample* __restrict__ samples, uint32_t stride,
-                       uint32_t n, __bf16* __restrict__ delta, double* loss) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  const float inv_n = 1.0f / float(n);
-  double sum = 0.0;
-  if (q < n) {
-    const ge_cache_sample& s = samples[size_t(q) * stride];
-    for (int c = 0; c < kCacheOutPad; ++c) {
-      float d = 0.0f;
-      if (c < 3) {
-        const float p = pred[size_t(q) * kCacheOutPad + c], y = s.radiance[c];
-        const float denom = p * p + 0.01f;
-        d = 2.0f * (p - y) / denom * inv_n;
-        sum += double((p - y) * (p - y) / denom);
-      }
-      delta[size_t(q) * kCacheOutPad + c] = to_bf16(d);
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-  if ((threadIdx.x & 63u) == 0 && sum != 0.0) atomicAdd(loss, sum / double(n));
-}
-
-// Adam on the master weights (when `update`), then the three BF16 copies.
+// Adam on the master weights (when `update`), then the BF16 copies: the forward fragments, and
+// the backward ones of every layer after the first.
 __global__ void k_publish(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
-                          const float* __restrict__ grad, __bf16* __restrict__ wb, __bf16* __restrict__ wt,
-                          __bf16* __restrict__ frag, float lr, float beta1, float beta2, float eps, float c1, float c2,
-                          int update) {
+                          const float* __restrict__ grad, __bf16* __restrict__ frag, __bf16* __restrict__ frag_t,
+                          float lr, float beta1, float beta2, float eps, float c1, float c2, int update) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kCacheWeights) return;
   float wi = w[i];
@@ -369,12 +537,11 @@ __global__ void k_publish(float* __restrict__ w, float* __restrict__ m, float* _
   }
   int l = 0;
   while (l + 1 < kCacheLayers && i >= cache_offset(l + 1)) ++l;
-  const uint32_t off = cache_offset(l), rows = cache_dim(l + 1), cols = cache_dim(l);
-  const uint32_t r = (i - off) / cols, c = (i - off) % cols;
+  const uint32_t off = cache_offset(l), cols = cache_dim(l);
+  const uint32_t o = (i - off) / cols, c = (i - off) % cols;
   const __bf16 b = to_bf16(wi);
-  wb[i] = b;
-  wt[off + c * rows + r] = b;
-  frag[frag_index(l, r, c)] = b;
+  frag[frag_index(l, o, c)] = b;
+  if (l >= 1) frag_t[frag_t_index(l, o, c)] = b;
 }
 
 inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
@@ -386,12 +553,6 @@ uint32_t split_k(uint32_t k, uint32_t want, uint32_t& per) {
   return (k + per - 1) / per;
 }
 
-hipError_t gemm(hipStream_t st, GemmArgs g, uint32_t want_splits) {
-  const uint32_t splits = split_k(g.k, want_splits, g.k_split);
-  k_gemm<<<dim3(blocks(g.m, kTile), blocks(g.n, kTile), splits), 256, 0, st>>>(g);
-  return hipGetLastError();
-}
-
 constexpr uint32_t kMaxSplits = 256;
 
 }  // namespace
@@ -401,9 +562,11 @@ hipError_t RadianceCache::init(hipStream_t st, uint32_t seed) {
   hipError_t e;
   for (DevBuf* b : {&w, &m, &v, &grad})
     if ((e = b->reserve(size_t(kCacheWeights) * 4)) != hipSuccess) return e;
-  for (DevBuf* b : {&wb, &wt, &frag})
-    if ((e = b->reserve(size_t(kCacheWeights) * 2)) != hipSuccess) return e;
-  if ((e = partial.reserve(size_t(kMaxSplits) * kCacheInputs * kCacheWidth * 4)) != hipSuccess) return e;
+  if ((e = frag.reserve(size_t(kCacheWeights) * 2)) != hipSuccess) return e;
+  if ((e = frag_t.reserve(size_t(kFragT) * 2)) != hipSuccess) return e;
+  // The output layer's backward fragments are padded with zeros that nothing writes again.
+  if ((e = hipMemsetAsync(frag_t.ptr, 0, size_t(kFragT) * 2, st)) != hipSuccess) return e;
+  if ((e = partial.reserve(size_t(kMaxSplits) * kCacheWeights * 4)) != hipSuccess) return e;
   if ((e = loss.reserve(8)) != hipSuccess) return e;
   if (!host_loss && (e = hipHostMalloc(reinterpret_cast<void**>(&host_loss), 8)) != hipSuccess) return e;
   // He-uniform weights from a splitmix64 sequence; the unused output rows 3.. start at zero.
@@ -444,8 +607,8 @@ hipError_t RadianceCache::get_weights(hipStream_t st, float* host) {
 
 hipError_t RadianceCache::publish(hipStream_t st, float lr, float c1, float c2, bool update) {
   k_publish<<<blocks(kCacheWeights, 256), 256, 0, st>>>(w.as<float>(), m.as<float>(), v.as<float>(), grad.as<float>(),
-                                                       wb.as<__bf16>(), wt.as<__bf16>(), frag.as<__bf16>(), lr, 0.9f,
-                                                       0.99f, 1e-8f, c1, c2, update ? 1 : 0);
+                                                       frag.as<__bf16>(), frag_t.as<__bf16>(), lr, 0.9f, 0.99f, 1e-8f,
+                                                       c1, c2, update ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -453,11 +616,8 @@ hipError_t RadianceCache::reserve_rows(uint32_t n) {
   if (n <= capacity) return hipSuccess;
   hipError_t e;
   if ((e = x.reserve(size_t(n) * kCacheInputs * 2)) != hipSuccess) return e;
-  for (DevBuf& b : act)
-    if ((e = b.reserve(size_t(n) * kCacheWidth * 2)) != hipSuccess) return e;
-  if ((e = out.reserve(size_t(n) * kCacheOutPad * 4)) != hipSuccess) return e;
-  for (DevBuf& b : delta)
-    if ((e = b.reserve(size_t(n) * kCacheWidth * 2)) != hipSuccess) return e;
+  if ((e = act.reserve(size_t(n) * (kCacheLayers - 1) * kCacheWidth * 2)) != hipSuccess) return e;
+  if ((e = dz.reserve(size_t(n) * ((kCacheLayers - 1) * kCacheWidth + kCacheOutPad) * 2)) != hipSuccess) return e;
   capacity = n;
   return hipSuccess;
 }
@@ -501,87 +661,54 @@ hipError_t RadianceCache::train(hipStream_t st, const ge_cache_sample* smp, uint
   if (n == 0) return hipSuccess;
   hipError_t e;
   if ((e = reserve_rows(n)) != hipSuccess) return e;
-  k_encode<<<blocks(n, 4), dim3(64, 4), 0, st>>>(reinterpret_cast<const float*>(smp), stride * 16, n, x.as<__bf16>());
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  // Forward, keeping every layer's activations.
-  const __bf16* in = x.as<__bf16>();
-  for (int l = 0; l < kCacheLayers; ++l) {
-    const bool last = l == kCacheLayers - 1;
-    GemmArgs g{};
-    g.a = in;
-    g.a_m = cache_dim(l);
-    g.a_k = 1;
-    g.b = wb.as<__bf16>() + cache_offset(l);
-    g.b_n = cache_dim(l);
-    g.b_k = 1;
-    g.m = n;
-    g.n = cache_dim(l + 1);
-    g.k = cache_dim(l);
-    g.c = last ? out.ptr : act[l].ptr;
-    g.c_m = cache_dim(l + 1);
-    g.epilogue = last ? kStoreF32 : kStoreBf16Relu;
-    if ((e = gemm(st, g, 1)) != hipSuccess) return e;
-    if (!last) in = act[l].as<__bf16>();
-  }
   if ((e = hipMemsetAsync(loss.ptr, 0, 8, st)) != hipSuccess) return e;
-  k_loss<<<blocks(n, 256), 256, 0, st>>>(out.as<float>(), smp, stride, n, delta[0].as<__bf16>(), loss.as<double>());
+  TrainArgs ta{};
+  ta.samples = smp;
+  ta.stride = stride;
+  ta.n = n;
+  ta.frag = frag.as<uint4>();
+  ta.frag_t = frag_t.as<bf16x8>();
+  ta.x = x.as<__bf16>();
+  ta.act = act.as<__bf16>();
+  ta.dz = dz.as<__bf16>();
+  ta.loss = loss.as<double>();
+  k_cache_train_pass<<<infer_grid(n), kInferWaves * 64, 0, st>>>(ta);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  // Backward: each layer's weight gradient (split over the batch, then summed), then its input's
-  // delta through ReLU's derivative.
-  const uint32_t want = std::max(1u, std::min(kMaxSplits, n / 128u));  // 4 k steps a workgroup
-  int cur = 0;
-  for (int l = kCacheLayers - 1; l >= 0; --l) {
+  // dW_l = dZ_l^T A_{l-1} for every layer in one launch, split over the samples; then the splits summed.
+  uint32_t per;
+  const uint32_t splits = split_k(n, std::max(1u, std::min(kMaxSplits, n / 128u)), per);
+  GradArgs ga{};
+  for (int l = 0; l < kCacheLayers; ++l) {
+    GemmArgs& g = ga.layer[l];
     const uint32_t dout = cache_dim(l + 1), din = cache_dim(l);
-    const __bf16* dz = delta[cur].as<__bf16>();
-    const __bf16* a_in = l == 0 ? x.as<__bf16>() : act[l - 1].as<__bf16>();
-    GemmArgs gw{};
-    gw.a = dz;
-    gw.a_m = 1;
-    gw.a_k = dout;
-    gw.b = a_in;
-    gw.b_n = 1;
-    gw.b_k = din;
-    gw.m = dout;
-    gw.n = din;
-    gw.k = n;
-    gw.c = partial.ptr;
-    gw.c_m = din;
-    gw.c_z = dout * din;
-    gw.epilogue = kStoreF32;
-    uint32_t per;
-    const uint32_t used = split_k(n, want, per);
-    if ((e = gemm(st, gw, want)) != hipSuccess) return e;
-    k_reduce_splits<<<blocks(dout * din, 256), 256, 0, st>>>(dout * din, used, partial.as<float>(),
-                                                             grad.as<float>() + cache_offset(l));
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (l == 0) break;
-    GemmArgs gd{};
-    gd.a = dz;
-    gd.a_m = dout;
-    gd.a_k = 1;
-    gd.b = wt.as<__bf16>() + cache_offset(l);
-    gd.b_n = dout;
-    gd.b_k = 1;
-    gd.m = n;
-    gd.n = din;
-    gd.k = dout;
-    gd.c = delta[cur ^ 1].ptr;
-    gd.c_m = din;
-    gd.mask = act[l - 1].as<__bf16>();
-    gd.epilogue = kStoreBf16Mask;
-    if ((e = gemm(st, gd, 1)) != hipSuccess) return e;
-    cur ^= 1;
+    g.a = dz.as<__bf16>() + size_t(l) * n * kCacheWidth;
+    g.a_m = 1;
+    g.a_k = dout;
+    g.b = l == 0 ? x.as<__bf16>() : act.as<__bf16>() + size_t(l - 1) * n * kCacheWidth;
+    g.b_n = 1;
+    g.b_k = din;
+    g.c = partial.as<float>() + cache_offset(l);
+    g.c_m = din;
+    g.c_z = kCacheWeights;
+    g.m = dout;
+    g.n = din;
+    g.k = n;
+    g.k_split = per;
   }
+  k_weight_grads<<<dim3(kGradTiles, splits), 256, 0, st>>>(ga);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  k_reduce_splits<<<blocks(kCacheWeights, 256), 256, 0, st>>>(kCacheWeights, splits, partial.as<float>(),
+                                                               grad.as<float>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   ++step;
   const float c1 = 1.0f - std::pow(0.9f, float(step)), c2 = 1.0f - std::pow(0.99f, float(step));
   return publish(st, lr, c1, c2, true);
 }
 
 void RadianceCache::release() {
-  for (DevBuf* b : {&w, &m, &v, &grad, &wb, &wt, &frag, &partial, &loss, &x, &out, &delta[0], &delta[1], &queries,
-                    &query_weights, &samples, &sample_weights})
+  for (DevBuf* b : {&w, &m, &v, &grad, &frag, &frag_t, &partial, &loss, &x, &act, &dz, &queries, &query_weights,
+                    &samples, &sample_weights})
     b->release();
-  for (DevBuf& b : act) b.release();
   if (host_loss) (void)hipHostFree(host_loss);
   host_loss = nullptr;
   capacity = 0;
