@@ -36,6 +36,7 @@ class InverseRenderer:
         n = self.width * self.height
         self._image = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
         self._d_image = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        self._coverage = torch.zeros(n, dtype=torch.int32, device="cuda")
         self._gm = torch.zeros((desc.material_count, 4), dtype=torch.float64, device="cuda")
         self._gl = torch.zeros(self.params.light_count + 1, dtype=torch.float64, device="cuda")
 
@@ -64,8 +65,13 @@ class InverseRenderer:
         self.params.sun[3] = float(sun.detach().reshape(-1)[0])
 
     def render(self):
-        """The detached estimator's per-pixel mean (H, W, 3) at the current parameters."""
-        self.scene.path_adjoint(self.params, None, None, self._image.data_ptr(), None, device=True)
+        """The detached estimator's per-pixel mean (H, W, 3) at the current parameters. At equal
+        sampling and scene tables the estimator is the path tracer's, to the bit, so the wavefront
+        tracer renders it."""
+        self._image.zero_()  # a sample_base past 0 resumes into the accumulator
+        torch.cuda.synchronize()  # torch's stream has written what the library's stream reads
+        self.scene.path_trace(self.params, self._image.data_ptr(), self._coverage.data_ptr(), device=True)
+        self.scene.ctx.synchronize()
         torch.cuda.synchronize()
         img = self._image[:, :3] / float(self.params.spp)
         return img.reshape(self.height, self.width, 3).clone()
@@ -74,6 +80,7 @@ class InverseRenderer:
         """dLoss/dparameters for dLoss/dI = grad_image (H, W, 3): float64 (M, 4) and (1 + L,)."""
         self._d_image.zero_()
         self._d_image[:, :3] = grad_image.reshape(-1, 3).to(torch.float32)
+        torch.cuda.synchronize()  # torch's stream has written what the library's stream reads
         self.scene.path_adjoint(self.params, self._d_image.data_ptr(), None, None,
                                 (self._gm.data_ptr(), self._gl.data_ptr()), device=True)
         torch.cuda.synchronize()
