@@ -29,15 +29,29 @@ __host__ __device__ constexpr uint32_t cache_offset(int l) {
   return at;
 }
 
-// The value at input index `idx` (0..191) of the encoded query with features f. The index splits
-// as idx = 32 kb + 8 g + j with group G = 6 g + kb: groups 0..20 hold sin, cos pairs of bands
+// The encoded query's inputs, in the order the fused kernels compute them: input index
+// idx = 32 kb + 8 g + j with group G = 6 g + kb; groups 0..20 hold sin, cos pairs of bands
 // 4 (G % 3) .. + 3 of geometric value G / 3 (even j sin, odd j cos); group 21 holds albedo rgb,
-// roughness, then 1 (the bias input) and zeros; groups 22 and 23 are zero. The order is what lets the
-// fused inference kernel compute a lane's inputs from compile-time band numbers.
-// sin(2^b pi x) is evaluated as sin(2 pi frac(2^(b-1) x)), exact argument reduction for any x.
-__device__ __forceinline__ float cache_band(float x, int b, bool cosine) {
-  const float r = __builtin_amdgcn_fractf(x * __builtin_ldexpf(1.0f, b - 1));
-  return cosine ? __builtin_amdgcn_cosf(r) : __builtin_amdgcn_sinf(r);  // arguments in revolutions
+// roughness, then 1 (the bias input) and zeros; groups 22 and 23 are zero. The order lets lane
+// group g of a wave encode values 2 g and 2 g + 1 with compile-time band numbers.
+//
+// One group's four bands of x: sin and cos of 2^b pi x for b = 4 q .. 4 q + 3. The first pair is
+// sin(2 pi frac(2^(4q-1) x)) from the hardware (exact argument reduction for any x), the next three
+// by doubling the angle (sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin a sin a).
+__device__ __forceinline__ void cache_bands4(float x, int q, float v[8]) {
+  const float r = __builtin_amdgcn_fractf(x * __builtin_ldexpf(1.0f, 4 * q - 1));
+  float sn = __builtin_amdgcn_sinf(r), cs = __builtin_amdgcn_cosf(r);  // arguments in revolutions
+  v[0] = sn;
+  v[1] = cs;
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {
+    const float twice = sn + sn;
+    const float s2 = twice * cs, c2 = 1.0f - twice * sn;
+    sn = s2;
+    cs = c2;
+    v[2 * i] = sn;
+    v[2 * i + 1] = cs;
+  }
 }
 
 // The cache's normalisation: position -> (p - lo) * inv.

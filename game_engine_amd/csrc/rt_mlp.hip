@@ -23,14 +23,9 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
-__device__ __forceinline__ uint16_t bf16_bits(float f) {
-  // Round to nearest even; NaN stays a quiet NaN.
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
-}
-__device__ __forceinline__ __bf16 to_bf16(float f) { return __builtin_bit_cast(__bf16, bf16_bits(f)); }
+// Round to nearest even (v_cvt_pk_bf16_f32, two values an instruction).
+__device__ __forceinline__ __bf16 to_bf16(float f) { return static_cast<__bf16>(f); }
+__device__ __forceinline__ uint16_t bf16_bits(float f) { return __builtin_bit_cast(uint16_t, to_bf16(f)); }
 
 // Which fragment slot holds weight (row r, column c) of layer l: rows in 16-row tiles t, columns in
 // 32-wide k blocks kb, lane = 16 g + r % 16, element j. Layer 0 takes its inputs in order
@@ -72,25 +67,46 @@ __device__ __forceinline__ bf16x8 pack8(const float* v) {
   return r;
 }
 
-// ReLU, then the two B operands of the next layer from this layer's four accumulator tiles.
-__device__ __forceinline__ void relu_pack(const f32x4 acc[4], bf16x8 out[2]) {
-#pragma unroll
-  for (int kb = 0; kb < 2; ++kb) {
-    float v[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      v[i] = acc[2 * kb][i] > 0.0f ? acc[2 * kb][i] : 0.0f;
-      v[4 + i] = acc[2 * kb + 1][i] > 0.0f ? acc[2 * kb + 1][i] : 0.0f;
-    }
-    out[kb] = pack8(v);
-  }
+__device__ __forceinline__ uint2 pack4(const float* v) {
+  return make_uint2(uint32_t(bf16_bits(v[0])) | (uint32_t(bf16_bits(v[1])) << 16),
+                    uint32_t(bf16_bits(v[2])) | (uint32_t(bf16_bits(v[3])) << 16));
+}
+__device__ __forceinline__ bf16x8 join8(uint2 lo, uint2 hi) {
+  const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
+  return __builtin_bit_cast(bf16x8, u);
 }
 
-constexpr uint32_t kInferWaves = 4;
+typedef short __attribute__((ext_vector_type(2))) short2v;
+
+// max(x, 0) of the two bf16 in a word, as one packed int16 max: a negative bf16 (-0 included) is a
+// negative int16, a positive one a positive int16. Rounding first and clamping after gives the
+// same bits as clamping in float and rounding.
+__device__ __forceinline__ uint32_t relu2(uint32_t w) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(short2v, w), short2v{0, 0}));
+}
+
+// One accumulator tile, ReLU'd, as four bf16 (the hidden activations neurons 16 t + 4 g + i).
+__device__ __forceinline__ uint2 relu4(const f32x4& a) {
+  const float v[4] = {a[0], a[1], a[2], a[3]};
+  const uint2 p = pack4(v);
+  return make_uint2(relu2(p.x), relu2(p.y));
+}
+
+// ReLU, then the two B operands of the next layer from this layer's four accumulator tiles.
+__device__ __forceinline__ void relu_pack(const f32x4 acc[4], bf16x8 out[2]) {
+  out[0] = join8(relu4(acc[0]), relu4(acc[1]));
+  out[1] = join8(relu4(acc[2]), relu4(acc[3]));
+}
+
+#ifndef GE_CACHE_WAVES
+#define GE_CACHE_WAVES 16
+#endif
+// One workgroup a CU shares one LDS copy of the weights: 16 waves, 4 a SIMD.
+constexpr uint32_t kInferWaves = GE_CACHE_WAVES;
 constexpr uint32_t kInferQueries = 32;  // a wave's queries per pass: two 16-query tiles
 
 template <bool Apply>
-__global__ void __launch_bounds__(256, 2) k_cache_infer(InferArgs a) {
+__global__ void __launch_bounds__(kInferWaves * 64) k_cache_infer(InferArgs a) {
   __shared__ uint4 frag[kCacheWeights / 8];
   for (uint32_t i = threadIdx.x; i < kCacheWeights / 8; i += blockDim.x) frag[i] = a.frag[i];
   __syncthreads();
@@ -139,13 +155,7 @@ __global__ void __launch_bounds__(256, 2) k_cache_infer(InferArgs a) {
       for (int h = 0; h < 2; ++h) {
         float v[8];
         if (kb < 3 || g < 3) {
-          const float xv = kb < 3 ? xa[h] : xb[h];
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            const int b = 4 * (kb % 3) + jj;
-            v[2 * jj] = cache_band(xv, b, false);
-            v[2 * jj + 1] = cache_band(xv, b, true);
-          }
+          cache_bands4(kb < 3 ? xa[h] : xb[h], kb % 3, v);
         } else if (kb == 3) {
           v[0] = tail[h][0]; v[1] = tail[h][1]; v[2] = tail[h][2]; v[3] = tail[h][3];
           v[4] = 1.0f; v[5] = 0.0f; v[6] = 0.0f; v[7] = 0.0f;
@@ -249,20 +259,14 @@ struct TrainArgs {
   double* loss;
 };
 
-__device__ __forceinline__ uint2 pack4(const float* v) {
-  return make_uint2(uint32_t(bf16_bits(v[0])) | (uint32_t(bf16_bits(v[1])) << 16),
-                    uint32_t(bf16_bits(v[2])) | (uint32_t(bf16_bits(v[3])) << 16));
-}
-__device__ __forceinline__ bf16x8 join8(uint2 lo, uint2 hi) {
-  const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
-  return __builtin_bit_cast(bf16x8, u);
-}
 
 // The relative L2 loss's forward and backward pass, 32 samples a wave: the encoding, six products
 // with the activations kept (bf16, in accumulator order) for ReLU's derivative, the loss and its
 // gradient, and five products back through W^T. Writes the encoded inputs, every activation and
 // every delta for the weight gradients; adds the batch's mean loss into `loss`.
-__global__ void __launch_bounds__(256, 2) k_cache_train_pass(TrainArgs a) {
+constexpr uint32_t kTrainWaves = 4;
+
+__global__ void __launch_bounds__(kTrainWaves * 64, 2) k_cache_train_pass(TrainArgs a) {
   __shared__ uint4 frag[kCacheWeights / 8];
   for (uint32_t i = threadIdx.x; i < kCacheWeights / 8; i += blockDim.x) frag[i] = a.frag[i];
   __syncthreads();
@@ -270,8 +274,8 @@ __global__ void __launch_bounds__(256, 2) k_cache_train_pass(TrainArgs a) {
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, g = lane >> 4, r = lane & 15u;
   const float inv_n = 1.0f / float(n);
   double loss_sum = 0.0;
-  for (uint32_t base = (blockIdx.x * kInferWaves + wave) * kInferQueries; base < n;
-       base += gridDim.x * kInferWaves * kInferQueries) {
+  for (uint32_t base = (blockIdx.x * kTrainWaves + wave) * kInferQueries; base < n;
+       base += gridDim.x * kTrainWaves * kInferQueries) {
     uint32_t lane_at = lane;
     asm volatile("" : "+v"(lane_at));
     const bf16x8* fr = reinterpret_cast<const bf16x8*>(frag) + lane_at;
@@ -315,13 +319,7 @@ __global__ void __launch_bounds__(256, 2) k_cache_train_pass(TrainArgs a) {
       for (int h = 0; h < 2; ++h) {
         float v[8];
         if (kb < 3 || g < 3) {
-          const float val = kb < 3 ? xa[h] : xb[h];
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            const int b = 4 * (kb % 3) + jj;
-            v[2 * jj] = cache_band(val, b, false);
-            v[2 * jj + 1] = cache_band(val, b, true);
-          }
+          cache_bands4(kb < 3 ? xa[h] : xb[h], kb % 3, v);
         } else if (kb == 3) {
           v[0] = tail[h][0]; v[1] = tail[h][1]; v[2] = tail[h][2]; v[3] = tail[h][3];
           v[4] = 1.0f; v[5] = 0.0f; v[6] = 0.0f; v[7] = 0.0f;
@@ -347,10 +345,7 @@ __global__ void __launch_bounds__(256, 2) k_cache_train_pass(TrainArgs a) {
       for (int h = 0; h < 2; ++h) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          float v[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] = acc[h][t][i] > 0.0f ? acc[h][t][i] : 0.0f;
-          act[l][h][t] = pack4(v);
+          act[l][h][t] = relu4(acc[h][t]);
           if (live[h])
             *reinterpret_cast<uint2*>(a.act + (size_t(l) * n + q[h]) * kCacheWidth + 16 * t + 4 * g) = act[l][h][t];
         }
@@ -622,9 +617,11 @@ hipError_t RadianceCache::reserve_rows(uint32_t n) {
   return hipSuccess;
 }
 
-static uint32_t infer_grid(uint32_t n) {
-  const uint32_t per_block = kInferWaves * kInferQueries;
-  return std::max(1u, std::min(blocks(n, per_block), 512u));  // two workgroups a CU, 256 CUs
+// Persistent grids: 256 CUs, one 16-wave inference workgroup a CU (its LDS copy of the weights
+// serves 4 waves a SIMD), or two 4-wave training workgroups (their 224 VGPRs allow 2 waves a SIMD).
+static uint32_t persistent_grid(uint32_t n, uint32_t waves) {
+  const uint32_t per_cu = waves >= 16 ? 1u : 2u;
+  return std::max(1u, std::min(blocks(n, waves * kInferQueries), 256u * per_cu));
 }
 
 hipError_t RadianceCache::infer(hipStream_t st, const float* queries, uint32_t stride, uint32_t n,
@@ -638,7 +635,7 @@ hipError_t RadianceCache::infer(hipStream_t st, const float* queries, uint32_t s
   a.n_dev = n_dev;
   a.frag = frag.as<uint4>();
   a.out = dst;
-  k_cache_infer<false><<<infer_grid(n), kInferWaves * 64, 0, st>>>(a);
+  k_cache_infer<false><<<persistent_grid(n, kInferWaves), kInferWaves * 64, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -653,7 +650,7 @@ hipError_t RadianceCache::apply(hipStream_t st, const ge_cache_query* queries, c
   a.frag = frag.as<uint4>();
   a.w = wts;
   a.result = result;
-  k_cache_infer<true><<<infer_grid(cap), kInferWaves * 64, 0, st>>>(a);
+  k_cache_infer<true><<<persistent_grid(cap, kInferWaves), kInferWaves * 64, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -672,7 +669,7 @@ hipError_t RadianceCache::train(hipStream_t st, const ge_cache_sample* smp, uint
   ta.act = act.as<__bf16>();
   ta.dz = dz.as<__bf16>();
   ta.loss = loss.as<double>();
-  k_cache_train_pass<<<infer_grid(n), kInferWaves * 64, 0, st>>>(ta);
+  k_cache_train_pass<<<persistent_grid(n, kTrainWaves), kTrainWaves * 64, 0, st>>>(ta);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // dW_l = dZ_l^T A_{l-1} for every layer in one launch, split over the samples; then the splits summed.
   uint32_t per;
