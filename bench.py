@@ -193,6 +193,12 @@ CACHE_FLOP_PER_QUERY = 2 * (172 * 64 + 4 * 64 * 64 + 64 * 3)  # the network's al
 def cache_main(args):
     """The radiance cache against the plain tracer on the same 4M-triangle scene at 1920 x 1080.
 
+    By default the camera rays go through pixel centres and the scene is lit by the sun and sky. With
+    jitter, near-mirror GGX surfaces (roughness down to 0.05) catch the delta sun across a pixel's
+    footprint, and point lights next to surfaces give 1/d^2 spikes: camera-vertex integrands whose
+    variance is so heavy-tailed that even a 4096-spp reference does not settle (--jitter and
+    --point-lights turn them on; both tracers share the camera vertex exactly either way).
+
     A reference of args.ref_spp samples (the plain tracer, seed 0). The plain tracer (seed 1) then
     accumulates 1, 1, 2, 4, ... spp dispatches, its RMSE against the reference taken at each power
     of two; the cached tracer (seed 2) trains online (warm-up frames, then one training update
@@ -215,8 +221,8 @@ def cache_main(args):
     scene.build()
     lo, hi = s.bounds()
     n = WIDTH * HEIGHT
-    flags = abi.PT_PIXEL_CENTER if args.center else 0
-    lights = 0 if args.sun_only else pt0.light_count
+    flags = 0 if args.jitter else abi.PT_PIXEL_CENTER
+    lights = pt0.light_count if args.point_lights else 0
 
     def params(seed, spp, base, width=WIDTH, height=HEIGHT):
         p, _ = s.frame(width, height)
@@ -361,12 +367,12 @@ def cache_main(args):
                                f"{args.samples} self-training samples, {args.train_steps} steps) after "
                                f"{args.cache_warmup} warm-up frames at {args.train_width}x{args.train_height}, "
                                f"RMSE against a {args.ref_spp}-spp plain render "
-                               f"({'pixel centres' if args.center else 'jittered'}, "
-                               f"{'sun and sky' if args.sun_only else 'sun + 64 point lights'}, 8 bounces)",
+                               f"({'jittered' if args.jitter else 'pixel centres'}, "
+                               f"{'sun + 64 point lights' if args.point_lights else 'sun and sky'}, 8 bounces)",
                    "width": WIDTH, "height": HEIGHT, "reference_spp": args.ref_spp, "plain_budget_spp": args.budget,
                    "warmup_paths": args.train_width * args.train_height, "samples": args.samples,
                    "train_every": args.train_every,
-                   "train_steps": args.train_steps, "learning_rate": args.lr, "pixel_center": bool(args.center),
+                   "train_steps": args.train_steps, "learning_rate": args.lr, "pixel_center": not args.jitter,
                    "point_lights": lights, "parallelism": "single GPU"},
         "plain_vs_cached": table,
         "cached_frames": frames, "cached_rmse_final": round(cached[-1][2], 6),
@@ -494,9 +500,11 @@ def main():
     ap.add_argument("--train-width", type=int, default=256, help="cache workload: warm-up frame width")
     ap.add_argument("--train-height", type=int, default=144, help="cache workload: warm-up frame height")
     ap.add_argument("--lr", type=float, default=5e-3, help="cache workload: Adam's learning rate")
-    ap.add_argument("--train-every", type=int, default=4, help="cache workload: frames a training frame")
-    ap.add_argument("--center", action="store_true", help="cache workload: pixel-centre rays throughout")
-    ap.add_argument("--sun-only", action="store_true", help="cache workload: the point lights off throughout")
+    ap.add_argument("--train-every", type=int, default=1, help="cache workload: frames a training frame")
+    ap.add_argument("--jitter", action="store_true",
+                    help="cache workload: jittered camera rays (default: pixel centres, where the reference converges)")
+    ap.add_argument("--point-lights", action="store_true",
+                    help="cache workload: the 64 point lights on (default: sun and sky, a finite-variance estimator)")
     ap.add_argument("--groups", type=int, default=16384, help="shadow workload: cluster groups")
     ap.add_argument("--terrain", type=int, default=4096, help="shadow workload: heightfield samples a side")
     ap.add_argument("--copy-calibration", action="store_true",
