@@ -190,7 +190,8 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
   ctx->resolve_rays.release();
   for (DevBuf& b : ctx->tmp) b.release();
   for (DevBuf* b : {&ctx->wave.state[0], &ctx->wave.state[1], &ctx->wave.hits, &ctx->wave.nee, &ctx->wave.result,
-                    &ctx->wave.queue[0], &ctx->wave.queue[1], &ctx->wave.counts, &ctx->wave.shadow, &ctx->wave.blocked})
+                    &ctx->wave.queue[0], &ctx->wave.queue[1], &ctx->wave.counts, &ctx->wave.shadow, &ctx->wave.blocked,
+                    &ctx->wave.sgrad, &ctx->wave.ts, &ctx->wave.adj_a, &ctx->wave.adj_r, &ctx->wave.lpath})
     b->release();
   if (ctx->wave.host_counts) (void)hipHostFree(ctx->wave.host_counts);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -851,8 +852,43 @@ int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_pa
   SceneView sv = scene_view(s);
   sv.light_count = params->light_count;
   GE_HIP(ctx, begin_timing(ctx));
-  GE_HIP(ctx, launch_path_adjoint(ctx->stream, bvh_view(s), sv, p, dsamp, reinterpret_cast<const float4*>(dd),
-                                  reinterpret_cast<float4*>(dimg), d_image ? dgm : nullptr, d_image ? dgl : nullptr));
+  if (flags & GE_RT_PT_MEGAKERNEL) {
+    GE_HIP(ctx, launch_path_adjoint(ctx->stream, bvh_view(s), sv, p, dsamp, reinterpret_cast<const float4*>(dd),
+                                    reinterpret_cast<float4*>(dimg), d_image ? dgm : nullptr, d_image ? dgl : nullptr));
+  } else {
+    // The wavefront: a primal pass (the detached estimate of every path, and the image), then with
+    // d_image a gradient pass that replays the same paths.
+    const uint64_t npaths = uint64_t(p.width) * p.height * p.spp;
+    GE_CHECK(ctx, npaths < (1ull << 31), GE_RT_ERROR_INVALID_ARGUMENT, "too many paths in one dispatch");
+    p.clusters = reinterpret_cast<uint64_t>(sv.clusters);
+    p.mesh = reinterpret_cast<uint64_t>(sv.meshes);
+    p.triangles = reinterpret_cast<uint64_t>(sv.triangles);
+    p.instances = reinterpret_cast<uint64_t>(sv.instances);
+    p.attributes = reinterpret_cast<uint64_t>(sv.attributes);
+    p.materials = reinterpret_cast<uint64_t>(sv.materials);
+    p.cluster_materials = reinterpret_cast<uint64_t>(sv.cluster_materials);
+    p.lights = reinterpret_cast<uint64_t>(sv.lights);
+    AdjointHook h{};
+    h.sampling = dsamp;
+    uint32_t* cov = nullptr;
+    if (dimg) {
+      GE_HIP(ctx, ctx->tmp[8].reserve(std::max<size_t>(n * 4, 256)));
+      cov = ctx->tmp[8].as<uint32_t>();
+      GE_HIP(ctx, hipMemsetAsync(dimg, 0, n * 16, ctx->stream));  // the image holds this dispatch's samples alone
+    }
+    GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(s), sv, p, reinterpret_cast<float4*>(dimg), cov, nullptr, nullptr, &h));
+    if (d_image) {
+      Wavefront& w = ctx->wave;
+      GE_HIP(ctx, w.lpath.reserve(npaths * 16));
+      GE_HIP(ctx, hipMemcpyAsync(w.lpath.ptr, w.result.ptr, npaths * 16, hipMemcpyDeviceToDevice, ctx->stream));
+      h.grad = true;
+      h.d_image = reinterpret_cast<const float4*>(dd);
+      h.lpath = w.lpath.as<float4>();
+      h.grad_materials = dgm;
+      h.grad_lights = dgl;
+      GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(s), sv, p, nullptr, nullptr, nullptr, nullptr, &h));
+    }
+  }
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     if (image) GE_HIP(ctx, hipMemcpyAsync(image, dimg, n * 16, hipMemcpyDeviceToHost, ctx->stream));

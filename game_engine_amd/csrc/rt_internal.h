@@ -105,10 +105,34 @@ struct ShadowRec {  // 48 bytes: one shadow ray and what its light adds if nothi
   float c[3];
   uint32_t pad1;
 };
+struct ShadowGrad {  // 32 bytes: what one unblocked shadow ray adds to the gradient (the adjoint)
+  float ga[3];    // delta T df/dalbedo cos radiance, for the vertex's material
+  float gr;       // delta . T df/droughness cos radiance
+  float gl;       // delta . T f cos dradiance/dintensity, for light `light`
+  uint32_t light; // 0 the sun, 1 + i light i
+  uint32_t pad[2];
+};
 struct Wavefront {
   DevBuf state[2], hits, nee, result, queue[2], counts, shadow, blocked;
+  DevBuf sgrad;               // one ShadowGrad a shadow record (the adjoint)
+  DevBuf ts, adj_a, adj_r;    // per path (the adjoint): sampling throughput; dlnf/dalbedo + material, dlnf/droughness + going on
+  DevBuf lpath;               // per path (the adjoint): the primal pass's radiance
   uint64_t shadow_capacity = 0;
   uint32_t* host_counts = nullptr;  // pinned
+};
+
+// The path-replay adjoint on the wavefront (wavefront_path_trace's `adjoint`): directions, lobe
+// choices, pdfs and roulette from `sampling` (null: the scene's materials), BSDF values from the
+// scene's. With grad, the pass replays the primal pass's paths and adds each vertex's gradient:
+// its next-event term, and (df/f) times what the path gathered after it, L - prefix (path replay
+// backpropagation: no stored vertices once the primal radiance `lpath` is known).
+struct AdjointHook {
+  const ge_resolve_material* sampling;
+  bool grad;
+  const float4* d_image;    // dLoss/dI per pixel
+  const float4* lpath;      // per path: the primal pass's radiance
+  double* grad_materials;   // per material: albedo rgb, roughness
+  double* grad_lights;      // the sun, then each light
 };
 
 }  // namespace ge
@@ -218,7 +242,7 @@ hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, con
 struct CacheHook;
 hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
                                 float4* accum, uint32_t* coverage, unsigned long long* counters,
-                                const CacheHook* hook = nullptr);
+                                const CacheHook* hook = nullptr, const AdjointHook* adjoint = nullptr);
 hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
                               const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters);
 

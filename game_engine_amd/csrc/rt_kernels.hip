@@ -317,11 +317,16 @@ struct AdjointArgs {
 
 struct AdjointVertex {
   f3 C, T, w, dw_a, dw_r;
+  f3 na;     // the next-event term's albedo gradient, delta T dC/dalbedo
+  float nr;  // and its roughness gradient
   uint32_t mat;
   bool cont;
 };
 
-__device__ f3 replay_path(const AdjointArgs& a, f3 origin, f3 direction, uint32_t& rng, bool grad, f3 delta) {
+// light_acc: the workgroup's LDS sums of the sun's and each light's gradient (every lane adds to
+// the same few addresses; one global atomic a workgroup at the end instead of one a vertex).
+__device__ f3 replay_path(const AdjointArgs& a, f3 origin, f3 direction, uint32_t& rng, bool grad, f3 delta,
+                         double* light_acc) {
   const Pt& c = a.c;
   const ge_path_trace_params& p = c.p;
   AdjointVertex vx[k_adjoint_vertices];
@@ -339,11 +344,13 @@ __device__ f3 replay_path(const AdjointArgs& a, f3 origin, f3 direction, uint32_
     v.T = throughput;
     v.cont = false;
     v.mat = GE_RT_MISS;
+    v.na = splat(0.0f);
+    v.nr = 0.0f;
     if (!hit.found) {
       v.C = environment(p, direction, bounce == 0);
       radiance = radiance + throughput * v.C;
       if (grad && bounce != 0 && !(p.flags & GE_PT_UNIFORM_SKY) && !(direction.y >= 0.0f))
-        atomicAdd(&a.grad_lights[0], double(dot(delta, throughput * ld3(p.ground) * (maxr(p.sun[1], 0.0f) / k_pi))));
+        atomicAdd(&light_acc[0], double(dot(delta, throughput * ld3(p.ground) * (maxr(p.sun[1], 0.0f) / k_pi))));
       break;
     }
     const ge_instance_desc in = c.sv.instances[hit.inst];
@@ -389,7 +396,7 @@ __device__ f3 replay_path(const AdjointArgs& a, f3 origin, f3 direction, uint32_
         const float cosl = maxr(dot(surface.normal, sun), 0.0f);
         dd_a = dd_a + ga * (cosl * p.sun[3]);
         dd_r = dd_r + gr * (cosl * p.sun[3]);
-        atomicAdd(&a.grad_lights[0], double(dot(delta, throughput * (brdf_eval(surface, sun) * cosl))));
+        atomicAdd(&light_acc[0], double(dot(delta, throughput * (brdf_eval(surface, sun) * cosl))));
       }
     }
     for (uint32_t i = 0; i < c.sv.light_count; ++i) {
@@ -406,20 +413,16 @@ __device__ f3 replay_path(const AdjointArgs& a, f3 origin, f3 direction, uint32_
         const float cosl = maxr(dot(surface.normal, light_dir), 0.0f);
         dd_a = dd_a + ga * cosl * light_radiance;
         dd_r = dd_r + gr * cosl * light_radiance;
-        atomicAdd(&a.grad_lights[1 + i],
+        atomicAdd(&light_acc[1 + i],
                   double(dot(delta, throughput * (brdf_eval(surface, light_dir) * cosl * ld3(light.color_intensity) * att))));
       }
     }
     radiance = radiance + throughput * direct;
     v.C = emission + direct;
     v.mat = mat_index;
-    if (grad) {
-      const f3 t_a = delta * throughput * dd_a;
-      double* gm = a.grad_materials + size_t(mat_index) * 4;
-      atomicAdd(&gm[0], double(t_a.x));
-      atomicAdd(&gm[1], double(t_a.y));
-      atomicAdd(&gm[2], double(t_a.z));
-      atomicAdd(&gm[3], double(dot(delta, throughput * dd_r)));
+    if (grad) {  // added with the vertex's throughput term in the backward sweep
+      v.na = delta * throughput * dd_a;
+      v.nr = dot(delta, throughput * dd_r);
     }
     if (bounce >= p.max_bounces) break;
     const float n_dot_v = dot(surface.normal, surface.view);
@@ -475,29 +478,46 @@ __device__ f3 replay_path(const AdjointArgs& a, f3 origin, f3 direction, uint32_
     f3 u = splat(0.0f);
     for (uint32_t k = nv; k-- > 0;) {
       const AdjointVertex& v = vx[k];
-      if (v.cont) {
-        const f3 t_a = delta * v.T * v.dw_a * u;
+      if (v.mat != GE_RT_MISS) {
+        // One update a vertex: its next-event term, plus (when the path went on) dw_k U_{k+1}.
         double* gm = a.grad_materials + size_t(v.mat) * 4;
-        atomicAdd(&gm[0], double(t_a.x));
-        atomicAdd(&gm[1], double(t_a.y));
-        atomicAdd(&gm[2], double(t_a.z));
-        atomicAdd(&gm[3], double(dot(delta, v.T * v.dw_r * u)));
-        u = v.C + v.w * u;
-      } else {
-        u = v.C;
+        atomicAdd(&gm[0], double(v.na.x) + (v.cont ? double((delta * v.T * v.dw_a * u).x) : 0.0));
+        atomicAdd(&gm[1], double(v.na.y) + (v.cont ? double((delta * v.T * v.dw_a * u).y) : 0.0));
+        atomicAdd(&gm[2], double(v.na.z) + (v.cont ? double((delta * v.T * v.dw_a * u).z) : 0.0));
+        atomicAdd(&gm[3], double(v.nr) + (v.cont ? double(dot(delta, v.T * v.dw_r * u)) : 0.0));
       }
+      u = v.cont ? v.C + v.w * u : v.C;
     }
   }
   return radiance;
 }
 
+__device__ void adjoint_pixel(const AdjointArgs& a, uint32_t x, uint32_t y, bool grad, double* light_acc);
+
 __global__ void __launch_bounds__(64) k_path_adjoint(AdjointArgs a) {
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
   const uint32_t y = blockIdx.y * 8 + (threadIdx.x >> 3);
   const ge_path_trace_params& p = a.c.p;
-  if (x >= p.width || y >= p.height) return;
-  const size_t pixel = size_t(y) * p.width + x;
   const bool grad = a.d_image != nullptr;
+  double* light_acc = reinterpret_cast<double*>(g_lds + k_ring * k_lanes);
+  const uint32_t nl = a.c.sv.light_count + 1;
+  if (grad) {
+    for (uint32_t i = threadIdx.x; i < nl; i += 64) light_acc[i] = 0.0;
+    __syncthreads();
+  }
+  if (x < p.width && y < p.height) adjoint_pixel(a, x, y, grad, light_acc);
+  if (grad) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nl; i += 64) {
+      const double v = light_acc[i];
+      if (v != 0.0) atomicAdd(&a.grad_lights[i], v);
+    }
+  }
+}
+
+__device__ void adjoint_pixel(const AdjointArgs& a, uint32_t x, uint32_t y, bool grad, double* light_acc) {
+  const ge_path_trace_params& p = a.c.p;
+  const size_t pixel = size_t(y) * p.width + x;
   f3 delta = splat(0.0f);
   if (grad) {
     const float4 d = a.d_image[pixel];
@@ -521,7 +541,7 @@ __global__ void __launch_bounds__(64) k_path_adjoint(AdjointArgs a) {
     dir.y = ((m[1] * ndx + m[5] * ndy) + m[9] * 0.0f) + m[13] * 1.0f;
     dir.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
     dir = normalize(dir);
-    sum = sum + replay_path(a, ld3(p.camera), dir, rng, grad, delta);
+    sum = sum + replay_path(a, ld3(p.camera), dir, rng, grad, delta, light_acc);
   }
   if (a.image) a.image[pixel] = make_float4(sum.x, sum.y, sum.z, float(p.spp));
 }
@@ -694,7 +714,7 @@ hipError_t launch_path_adjoint(hipStream_t stream, const Bvh& bvh, const SceneVi
   a.grad_materials = grad_materials;
   a.grad_lights = grad_lights;
   const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
-  k_path_adjoint<<<grid, 64, lds(bvh), stream>>>(a);
+  k_path_adjoint<<<grid, 64, lds(bvh) + (size_t(sv.light_count) + 1) * 8, stream>>>(a);
   return hipGetLastError();
 }
 

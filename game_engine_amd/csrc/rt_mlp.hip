@@ -102,8 +102,12 @@ __device__ __forceinline__ void relu_pack(const f32x4 acc[4], bf16x8 out[2]) {
 #define GE_CACHE_WAVES 16
 #endif
 // One workgroup a CU shares one LDS copy of the weights: 16 waves, 4 a SIMD.
+#ifndef GE_CACHE_TILES
+#define GE_CACHE_TILES 2
+#endif
 constexpr uint32_t kInferWaves = GE_CACHE_WAVES;
-constexpr uint32_t kInferQueries = 32;  // a wave's queries per pass: two 16-query tiles
+constexpr int kTiles = GE_CACHE_TILES;  // 16-query tiles a wave pushes through the network per pass
+constexpr uint32_t kInferQueries = 16 * kTiles;
 
 template <bool Apply>
 __global__ void __launch_bounds__(kInferWaves * 64) k_cache_infer(InferArgs a) {
@@ -122,9 +126,9 @@ __global__ void __launch_bounds__(kInferWaves * 64) k_cache_infer(InferArgs a) {
     // ---- the two queries' features this lane encodes ----
     // Lane group g encodes geometric values 2g (k blocks 0..2) and 2g + 1 (3..5) of each query;
     // group 3's second half is the material tail (albedo, roughness, the constant 1).
-    float xa[2], xb[2], tail[2][4];
+    float xa[kTiles], xb[kTiles], tail[kTiles][4];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kTiles; ++h) {
       const uint32_t q = base + 16 * h + r;
       float f[12];
       if (q < n) {
@@ -142,17 +146,17 @@ __global__ void __launch_bounds__(kInferWaves * 64) k_cache_infer(InferArgs a) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) tail[h][k] = f[7 + k];
     }
-    f32x4 acc[2][4];
+    f32x4 acc[kTiles][4];
     // ---- layer 0: 192 -> 64, each k block encoded just before its products ----
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int h = 0; h < kTiles; ++h)
 #pragma unroll
       for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kb = 0; kb < 6; ++kb) {
-      bf16x8 x[2];
+      bf16x8 x[kTiles];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < kTiles; ++h) {
         float v[8];
         if (kb < 3 || g < 3) {
           cache_bands4(kb < 3 ? xa[h] : xb[h], kb % 3, v);
@@ -168,19 +172,20 @@ __global__ void __launch_bounds__(kInferWaves * 64) k_cache_infer(InferArgs a) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const bf16x8 wf = fr[(t * 6 + kb) * 64];
-        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, x[0], acc[0][t], 0, 0, 0);
-        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, x[1], acc[1][t], 0, 0, 0);
+#pragma unroll
+        for (int h = 0; h < kTiles; ++h)
+          acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, x[h], acc[h][t], 0, 0, 0);
       }
     }
     // ---- hidden layers 1..4: 64 -> 64 ----
 #pragma unroll
     for (int l = 1; l < kCacheLayers - 1; ++l) {
-      bf16x8 hin[2][2];
-      relu_pack(acc[0], hin[0]);
-      relu_pack(acc[1], hin[1]);
+      bf16x8 hin[kTiles][2];
+#pragma unroll
+      for (int h = 0; h < kTiles; ++h) relu_pack(acc[h], hin[h]);
       const bf16x8* lw = fr + cache_offset(l) / 8;
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < kTiles; ++h)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[h][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -188,27 +193,30 @@ __global__ void __launch_bounds__(kInferWaves * 64) k_cache_infer(InferArgs a) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const bf16x8 wf = lw[(t * 2 + kb) * 64];
-          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[0][kb], acc[0][t], 0, 0, 0);
-          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[1][kb], acc[1][t], 0, 0, 0);
+#pragma unroll
+          for (int h = 0; h < kTiles; ++h)
+            acc[h][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[h][kb], acc[h][t], 0, 0, 0);
         }
       }
     }
     // ---- output layer: 64 -> 16 (rows 0..2 used) ----
-    bf16x8 hin[2][2];
-    relu_pack(acc[0], hin[0]);
-    relu_pack(acc[1], hin[1]);
+    bf16x8 hin[kTiles][2];
+#pragma unroll
+    for (int h = 0; h < kTiles; ++h) relu_pack(acc[h], hin[h]);
     const bf16x8* lw = fr + cache_offset(kCacheLayers - 1) / 8;
-    f32x4 o[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+    f32x4 o[kTiles];
+#pragma unroll
+    for (int h = 0; h < kTiles; ++h) o[h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
       const bf16x8 wf = lw[kb * 64];
-      o[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[0][kb], o[0], 0, 0, 0);
-      o[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[1][kb], o[1], 0, 0, 0);
+#pragma unroll
+      for (int h = 0; h < kTiles; ++h) o[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, hin[h][kb], o[h], 0, 0, 0);
     }
     // Lanes 0..15 hold rows 0..3 of query r.
     if (g == 0) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < kTiles; ++h) {
         const uint32_t q = base + 16 * h + r;
         if (q >= n) continue;
         if (Apply) {
@@ -265,6 +273,7 @@ struct TrainArgs {
 // gradient, and five products back through W^T. Writes the encoded inputs, every activation and
 // every delta for the weight gradients; adds the batch's mean loss into `loss`.
 constexpr uint32_t kTrainWaves = 4;
+constexpr uint32_t kTrainQueries = 32;  // two 16-sample tiles a wave pass
 
 __global__ void __launch_bounds__(kTrainWaves * 64, 2) k_cache_train_pass(TrainArgs a) {
   __shared__ uint4 frag[kCacheWeights / 8];
@@ -274,8 +283,8 @@ __global__ void __launch_bounds__(kTrainWaves * 64, 2) k_cache_train_pass(TrainA
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, g = lane >> 4, r = lane & 15u;
   const float inv_n = 1.0f / float(n);
   double loss_sum = 0.0;
-  for (uint32_t base = (blockIdx.x * kTrainWaves + wave) * kInferQueries; base < n;
-       base += gridDim.x * kTrainWaves * kInferQueries) {
+  for (uint32_t base = (blockIdx.x * kTrainWaves + wave) * kTrainQueries; base < n;
+       base += gridDim.x * kTrainWaves * kTrainQueries) {
     uint32_t lane_at = lane;
     asm volatile("" : "+v"(lane_at));
     const bf16x8* fr = reinterpret_cast<const bf16x8*>(frag) + lane_at;
@@ -619,9 +628,9 @@ hipError_t RadianceCache::reserve_rows(uint32_t n) {
 
 // Persistent grids: 256 CUs, one 16-wave inference workgroup a CU (its LDS copy of the weights
 // serves 4 waves a SIMD), or two 4-wave training workgroups (their 224 VGPRs allow 2 waves a SIMD).
-static uint32_t persistent_grid(uint32_t n, uint32_t waves) {
+static uint32_t persistent_grid(uint32_t n, uint32_t waves, uint32_t per_wave) {
   const uint32_t per_cu = waves >= 16 ? 1u : 2u;
-  return std::max(1u, std::min(blocks(n, waves * kInferQueries), 256u * per_cu));
+  return std::max(1u, std::min(blocks(n, waves * per_wave), 256u * per_cu));
 }
 
 hipError_t RadianceCache::infer(hipStream_t st, const float* queries, uint32_t stride, uint32_t n,
@@ -635,7 +644,7 @@ hipError_t RadianceCache::infer(hipStream_t st, const float* queries, uint32_t s
   a.n_dev = n_dev;
   a.frag = frag.as<uint4>();
   a.out = dst;
-  k_cache_infer<false><<<persistent_grid(n, kInferWaves), kInferWaves * 64, 0, st>>>(a);
+  k_cache_infer<false><<<persistent_grid(n, kInferWaves, kInferQueries), kInferWaves * 64, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -650,7 +659,7 @@ hipError_t RadianceCache::apply(hipStream_t st, const ge_cache_query* queries, c
   a.frag = frag.as<uint4>();
   a.w = wts;
   a.result = result;
-  k_cache_infer<true><<<persistent_grid(cap, kInferWaves), kInferWaves * 64, 0, st>>>(a);
+  k_cache_infer<true><<<persistent_grid(cap, kInferWaves, kInferQueries), kInferWaves * 64, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -669,7 +678,7 @@ hipError_t RadianceCache::train(hipStream_t st, const ge_cache_sample* smp, uint
   ta.act = act.as<__bf16>();
   ta.dz = dz.as<__bf16>();
   ta.loss = loss.as<double>();
-  k_cache_train_pass<<<persistent_grid(n, kTrainWaves), kTrainWaves * 64, 0, st>>>(ta);
+  k_cache_train_pass<<<persistent_grid(n, kTrainWaves, kTrainQueries), kTrainWaves * 64, 0, st>>>(ta);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // dW_l = dZ_l^T A_{l-1} for every layer in one launch, split over the samples; then the splits summed.
   uint32_t per;

@@ -181,6 +181,18 @@ struct ShadeArgs {
   uint32_t npaths;           // a training path's own radiance goes to result[npaths + pid]
   float4* result;            // cached: the image's radiance of a path that goes on to train
   CacheBox box;
+  // The adjoint (kShadeAdjoint): sampling table, per-path sampling throughput, and with grad the
+  // per-record and per-vertex gradient terms.
+  const ge_resolve_material* sampling;  // null: the scene's materials
+  float4* ts;
+  bool grad;
+  const float4* d_image;
+  uint32_t pixels;
+  float spp;
+  ShadowGrad* sgrad;
+  float4* adj_a;
+  float4* adj_r;
+  double* grad_lights;
 };
 
 enum : int {
@@ -189,6 +201,7 @@ enum : int {
                      // paths among them record a sample there and go on one more bounce (next-event
                      // estimation, then the cache at the next hit: self-training)
   kShadeRecord = 2,  // the path tracer, recording a training sample at every hit vertex
+  kShadeAdjoint = 3, // the detached estimator (sampling from its own table), and with grad its gradient terms
 };
 constexpr uint32_t kTrainingPath = 4u;  // PathState::flags: the path goes on past the cache to train
 
@@ -232,26 +245,46 @@ __device__ __forceinline__ bool sun_asked(const ge_path_trace_params& p, const S
 template <int Mode>
 __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   __shared__ uint32_t parts[Mode == kShadePlain ? 2 : 4][k_shade_threads / 64 + 1];
+  __shared__ double ground_acc;  // kShadeAdjoint: the workgroup's sum of the sun's ground-radiance term
+  const bool grad = Mode == kShadeAdjoint && a.grad;
+  if (grad) {
+    if (threadIdx.x == 0) ground_acc = 0.0;
+    __syncthreads();
+  }
   const uint32_t g = blockIdx.x * k_shade_threads + threadIdx.x;
   const bool valid = g < a.n;
   const uint32_t pid = valid ? a.queue[g] : 0u;
   PathState st;
   bool alive = false;
   uint32_t nshadow = 0;
-  Surface surface;
+  Surface surface, ssurface;
   Vertex vx;
   float bias = 0.0f;
   f3 front = splat(0.0f);
   const ge_path_trace_params& p = a.p;
   bool hit_found = false, cache_here = false, train_here = false;
+  f3 Ts = splat(1.0f), delta = splat(0.0f);
+  uint32_t mat_index = GE_RT_MISS;
   if (valid) {
     st = a.in[pid];
     const PathHit hit = a.hits[pid];
     const f3 dir = f3{st.d[0], st.d[1], st.d[2]};
     f3 L = f3{st.L[0], st.L[1], st.L[2]};
     const f3 T = f3{st.T[0], st.T[1], st.T[2]};
+    if (Mode == kShadeAdjoint && st.bounce != 0) {
+      const float4 t4 = a.ts[pid];
+      Ts = f3{t4.x, t4.y, t4.z};
+    }
+    if (grad) {
+      const float4 d = a.d_image[pid % a.pixels];
+      delta = f3{d.x, d.y, d.z} / a.spp;
+    }
     if (hit.inst == GE_RT_MISS) {
       L = L + T * environment(p, dir, st.bounce == 0);
+      // An escaped ray below the horizon sees the sun-lit ground (path_trace.slang:166-176).
+      if (grad && st.bounce != 0 && !(p.flags & GE_PT_UNIFORM_SKY) && !(dir.y >= 0.0f))
+        atomicAdd(&ground_acc, double(dot(delta, T * ld3(p.ground) * (maxr(p.sun[1], 0.0f) / k_pi))));
+      if (grad) a.adj_a[pid] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(GE_RT_MISS));
       st.L[0] = L.x;
       st.L[1] = L.y;
       st.L[2] = L.z;
@@ -267,7 +300,8 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
       const f3 origin = f3{st.o[0], st.o[1], st.o[2]};
       uint32_t cluster_index;
       vx = vertex_of(a.sv, p, hit, origin, in, cluster_index);
-      const ge_resolve_material* mat = a.sv.materials + (a.sv.cluster_materials[cluster_index] + in.material_base);
+      mat_index = a.sv.cluster_materials[cluster_index] + in.material_base;
+      const ge_resolve_material* mat = a.sv.materials + mat_index;
       f3 shading_normal = vx.normal;
       if (dot(shading_normal, vx.geometric) < 0.0f) shading_normal = -shading_normal;
       const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
@@ -286,6 +320,14 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
       surface.spec[1] = spc.y;
       surface.spec[2] = spc.z;
       surface.spec[3] = spc.w;
+      ssurface = surface;
+      if (Mode == kShadeAdjoint && a.sampling) {
+        const ge_resolve_material* smat = a.sampling + mat_index;
+        const float4 salb = *reinterpret_cast<const float4*>(smat->albedo);
+        ssurface.albedo = f3{salb.x, salb.y, salb.z};
+        ssurface.roughness = salb.w;
+        ssurface.metallic = smat->emissive[3];
+      }
       bias = p.ray_bias + p.ray_bias_steps * a.sv.meshes[in.mesh].quant[3] * in.scale_max;
       front = dot(vx.geometric, surface.view) >= 0.0f ? vx.geometric : -vx.geometric;
       if (Mode == kShadeCached) {
@@ -353,6 +395,24 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
     if (fits && nshadow) {
       const f3 shadow_origin = vx.position + front * bias;
       uint32_t k = first;
+      const f3 T = f3{st.T[0], st.T[1], st.T[2]};
+      // kShadeAdjoint with grad: what the record adds to the gradient if nothing blocks it. `unit`
+      // is the light's radiance per unit intensity.
+      auto emit_grad = [&](uint32_t k, uint32_t light, f3 ld, f3 radiance, f3 unit) {
+        f3 ga, gr;
+        brdf_eval_grad(surface, ld, ga, gr);
+        const float cosl = maxr(dot(surface.normal, ld), 0.0f);
+        ShadowGrad q;
+        const f3 ta = delta * T * (ga * cosl * radiance);
+        q.ga[0] = ta.x;
+        q.ga[1] = ta.y;
+        q.ga[2] = ta.z;
+        q.gr = dot(delta, T * (gr * cosl * radiance));
+        q.gl = dot(delta, T * (brdf_eval(surface, ld) * cosl * unit));
+        q.light = light;
+        q.pad[0] = q.pad[1] = 0;
+        a.sgrad[k] = q;
+      };
       auto emit = [&](f3 ld, float tmax, f3 c) {
         ShadowRec r;
         r.o[0] = shadow_origin.x;
@@ -368,16 +428,21 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
         r.pad0 = r.pad1 = 0;
         a.shadow[k++] = r;
       };
-      if (sun_asked(p, surface)) emit(ld3(p.sun), 1e30f, brdf_direct(surface, ld3(p.sun), splat(p.sun[3])));
+      if (sun_asked(p, surface)) {
+        if (grad) emit_grad(k, 0, ld3(p.sun), splat(p.sun[3]), splat(1.0f));
+        emit(ld3(p.sun), 1e30f, brdf_direct(surface, ld3(p.sun), splat(p.sun[3])));
+      }
       for (uint32_t i = 0; i < a.sv.light_count; ++i) {
         const ge_resolve_light light = a.sv.lights[i];
         f3 ld, rad;
-        float dist;
-        if (!light_sample(light, vx.position, ld, rad, dist)) continue;
+        float dist, att;
+        if (!light_sample(light, vx.position, ld, rad, dist, &att)) continue;
         if (dot(surface.normal, ld) <= 0.0f) continue;
+        if (grad) emit_grad(k, 1 + i, ld, rad, ld3(light.color_intensity) * att);
         emit(ld, dist - bias - light.spot[2], brdf_direct(surface, ld, rad));
       }
     }
+    f3 ratio_a = splat(0.0f), ratio_r = splat(0.0f);  // kShadeAdjoint: (df/dparameter) / f along the next direction
     // ---- the next direction (path_trace.slang:397-455) ----
     alive = !cache_here && st.bounce < p.max_bounces;
     if (alive) {
@@ -387,9 +452,10 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
         f3 tangent, bitangent;
         onb(surface.normal, tangent, bitangent);
         const f3 v_local = f3{dot(surface.view, tangent), dot(surface.view, bitangent), dot(surface.view, surface.normal)};
-        const float alpha = brdf_alpha(surface.roughness);
-        const f3 f = fresnel(surface, n_dot_v);
-        const float p_specular = specular_probability(surface.albedo, surface.metallic, f);
+        // The sampling decisions come from the sampling surface (the scene's own outside the adjoint).
+        const float alpha = brdf_alpha(ssurface.roughness);
+        const f3 f = fresnel(ssurface, n_dot_v);
+        const float p_specular = specular_probability(ssurface.albedo, ssurface.metallic, f);
         uint32_t rng = st.rng;
         f3 l_local;
         if (rng_next(rng) < p_specular) {
@@ -410,13 +476,38 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
         }
         if (alive) {
           const f3 next = tangent * l_local.x + bitangent * l_local.y + surface.normal * l_local.z;
-          const f3 weight = brdf_eval(surface, next) * (l_local.z / pdf);
+          const float scale = l_local.z / pdf;
+          const f3 fe = brdf_eval(surface, next);
+          const f3 weight = fe * scale;
           f3 T = f3{st.T[0], st.T[1], st.T[2]} * weight;
-          alive = !(T.x <= 0.0f && T.y <= 0.0f && T.z <= 0.0f);
-          if (alive && st.bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
-            const float survive = clampr(maxr(T.x, maxr(T.y, T.z)), 0.0f, 0.95f);
-            if (rng_next(rng) >= survive) alive = false;
-            else T = T / maxr(survive, 1e-6f);
+          if (Mode == kShadeAdjoint) {
+            // Roulette and the end test follow the sampling throughput (the scene's: T itself).
+            Ts = Ts * (a.sampling ? brdf_eval(ssurface, next) * scale : weight);
+            alive = !(Ts.x <= 0.0f && Ts.y <= 0.0f && Ts.z <= 0.0f);
+            if (alive && st.bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
+              const float survive = clampr(maxr(Ts.x, maxr(Ts.y, Ts.z)), 0.0f, 0.95f);
+              if (rng_next(rng) >= survive) {
+                alive = false;
+              } else {
+                T = T / maxr(survive, 1e-6f);
+                Ts = Ts / maxr(survive, 1e-6f);
+              }
+            }
+            if (alive && grad) {
+              f3 ga, gr;
+              brdf_eval_grad(surface, next, ga, gr);
+              ratio_a = f3{fe.x > 0.0f ? ga.x / fe.x : 0.0f, fe.y > 0.0f ? ga.y / fe.y : 0.0f,
+                           fe.z > 0.0f ? ga.z / fe.z : 0.0f};
+              ratio_r = f3{fe.x > 0.0f ? gr.x / fe.x : 0.0f, fe.y > 0.0f ? gr.y / fe.y : 0.0f,
+                           fe.z > 0.0f ? gr.z / fe.z : 0.0f};
+            }
+          } else {
+            alive = !(T.x <= 0.0f && T.y <= 0.0f && T.z <= 0.0f);
+            if (alive && st.bounce + 1 >= GE_PT_ROULETTE_DEPTH) {
+              const float survive = clampr(maxr(T.x, maxr(T.y, T.z)), 0.0f, 0.95f);
+              if (rng_next(rng) >= survive) alive = false;
+              else T = T / maxr(survive, 1e-6f);
+            }
           }
           if (alive) {
             const f3 o = vx.position + (dot(next, front) >= 0.0f ? front : -front) * bias;
@@ -435,13 +526,22 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
         st.rng = rng;
       }
     }
+    if (grad) {
+      a.adj_a[pid] = make_float4(ratio_a.x, ratio_a.y, ratio_a.z, __uint_as_float(mat_index));
+      a.adj_r[pid] = make_float4(ratio_r.x, ratio_r.y, ratio_r.z, alive ? 1.0f : 0.0f);
+    }
   }
   if (valid) {
     if (!alive) st.flags |= 2u;
     a.out[pid] = st;
+    if (Mode == kShadeAdjoint && alive) a.ts[pid] = make_float4(Ts.x, Ts.y, Ts.z, 0.0f);
   }
   const uint32_t slot = block_append(&a.counts[0], alive ? 1u : 0u, parts[1]);
   if (alive) a.next_queue[slot] = pid;
+  if (grad) {
+    __syncthreads();
+    if (threadIdx.x == 0 && ground_acc != 0.0) atomicAdd(&a.grad_lights[0], ground_acc);
+  }
 }
 
 template <bool Count>
@@ -465,32 +565,88 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_any(Bvh bvh, const ShadowRe
 
 // radiance + throughput * next_event(...): the unblocked lights summed in record order. A path that
 // ended this bounce (a miss, or shade's break) leaves its radiance in `result`.
+struct ResolveAdjoint {
+  const ShadowGrad* sgrad;
+  const float4* adj_a;
+  const float4* adj_r;
+  const float4* lpath;
+  const float4* d_image;
+  uint32_t pixels;
+  float spp;
+  uint32_t lights;  // the sun and the lights: 1 + light_count
+  double* grad_materials;
+  double* grad_lights;
+};
+
+// With Adjoint, also each vertex's gradient: its unblocked records' terms, and (df/f) x (L - prefix),
+// what the path gathered after this vertex, when the path went on from it. The lights' terms are
+// summed in LDS (every lane adds to the same few addresses) and flushed once a workgroup.
+template <bool Adjoint>
 __global__ void __launch_bounds__(64) k_resolve(PathState* __restrict__ state, const uint32_t* __restrict__ queue,
                                                 uint32_t n, const NeeRec* __restrict__ nee,
                                                 const ShadowRec* __restrict__ shadow,
                                                 const uint8_t* __restrict__ blocked, float4* __restrict__ result,
-                                                uint32_t npaths) {
-  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
-  if (g >= n) return;
-  const uint32_t pid = queue[g];
-  const NeeRec nr = nee[pid];
-  PathState& st = state[pid];
-  f3 L = f3{st.L[0], st.L[1], st.L[2]};
-  if (nr.count != GE_RT_MISS) {
-    f3 direct = splat(0.0f);
-    for (uint32_t k = 0; k < nr.count; ++k) {
-      const uint32_t j = nr.first + k;
-      if (blocked[j]) continue;
-      direct = direct + f3{shadow[j].c[0], shadow[j].c[1], shadow[j].c[2]};
-    }
-    L = L + f3{nr.T[0], nr.T[1], nr.T[2]} * direct;
-    st.L[0] = L.x;
-    st.L[1] = L.y;
-    st.L[2] = L.z;
+                                                uint32_t npaths, ResolveAdjoint adj) {
+  extern __shared__ double light_acc[];
+  if (Adjoint) {
+    for (uint32_t i = threadIdx.x; i < adj.lights; i += 64) light_acc[i] = 0.0;
+    __syncthreads();
   }
-  // A training path's own radiance goes past the image's (kShadeCached).
-  if (st.flags & 2u)
-    result[(st.flags & kTrainingPath) ? npaths + pid : pid] = make_float4(L.x, L.y, L.z, __uint_as_float(st.flags & 1u));
+  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+  if (g < n) {
+    const uint32_t pid = queue[g];
+    const NeeRec nr = nee[pid];
+    PathState& st = state[pid];
+    f3 L = f3{st.L[0], st.L[1], st.L[2]};
+    if (nr.count != GE_RT_MISS) {
+      f3 direct = splat(0.0f);
+      f3 ga = splat(0.0f);
+      float gr = 0.0f;
+      for (uint32_t k = 0; k < nr.count; ++k) {
+        const uint32_t j = nr.first + k;
+        if (blocked[j]) continue;
+        direct = direct + f3{shadow[j].c[0], shadow[j].c[1], shadow[j].c[2]};
+        if (Adjoint) {
+          const ShadowGrad q = adj.sgrad[j];
+          ga = ga + f3{q.ga[0], q.ga[1], q.ga[2]};
+          gr += q.gr;
+          atomicAdd(&light_acc[q.light], double(q.gl));
+        }
+      }
+      L = L + f3{nr.T[0], nr.T[1], nr.T[2]} * direct;
+      st.L[0] = L.x;
+      st.L[1] = L.y;
+      st.L[2] = L.z;
+      if (Adjoint) {
+        const float4 ra = adj.adj_a[pid], rr = adj.adj_r[pid];
+        const uint32_t mat = __float_as_uint(ra.w);
+        if (mat != GE_RT_MISS) {
+          if (rr.w != 0.0f) {
+            const float4 lp = adj.lpath[pid];
+            const f3 after = f3{lp.x, lp.y, lp.z} - L;
+            const float4 d = adj.d_image[pid % adj.pixels];
+            const f3 delta = f3{d.x, d.y, d.z} / adj.spp;
+            ga = ga + delta * f3{ra.x, ra.y, ra.z} * after;
+            gr += dot(delta, f3{rr.x, rr.y, rr.z} * after);
+          }
+          double* gm = adj.grad_materials + size_t(mat) * 4;
+          if (ga.x != 0.0f) atomicAdd(&gm[0], double(ga.x));
+          if (ga.y != 0.0f) atomicAdd(&gm[1], double(ga.y));
+          if (ga.z != 0.0f) atomicAdd(&gm[2], double(ga.z));
+          if (gr != 0.0f) atomicAdd(&gm[3], double(gr));
+        }
+      }
+    }
+    if (st.flags & 2u)
+      result[(st.flags & kTrainingPath) ? npaths + pid : pid] = make_float4(L.x, L.y, L.z, __uint_as_float(st.flags & 1u));
+  }
+  if (Adjoint) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < adj.lights; i += 64) {
+      const double v = light_acc[i];
+      if (v != 0.0) atomicAdd(&adj.grad_lights[i], v);
+    }
+  }
 }
 
 // accum[p] += (sum over samples in order, spp); coverage |= any primary hit (path_trace.slang:490-494).
@@ -544,7 +700,7 @@ hipError_t stage_mark(ge_rt_ctx* ctx, int kind, int end) {
 
 hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
                                 float4* accum, uint32_t* coverage, unsigned long long* counters,
-                                const CacheHook* hook) {
+                                const CacheHook* hook, const AdjointHook* adjoint) {
   Wavefront& w = ctx->wave;
   const hipStream_t st = ctx->stream;
   const uint32_t pixels = p.width * p.height;
@@ -563,13 +719,25 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   if ((e = w.queue[1].reserve(size_t(npaths) * 4)) != hipSuccess) return e;
   if ((e = w.counts.reserve(256)) != hipSuccess) return e;
   if (!w.host_counts && (e = hipHostMalloc(reinterpret_cast<void**>(&w.host_counts), 64)) != hipSuccess) return e;
-  if (w.shadow_capacity < 2ull * npaths) {
-    w.shadow_capacity = 2ull * npaths;
+  const bool grad = adjoint && adjoint->grad;
+  if (adjoint) {
+    if ((e = w.ts.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
+    if (grad) {
+      if ((e = w.adj_a.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
+      if ((e = w.adj_r.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
+    }
+  }
+  auto grow_shadow = [&](uint64_t capacity) -> hipError_t {
+    w.shadow_capacity = capacity;
     w.shadow.release();
     w.blocked.release();
-    if ((e = w.shadow.reserve(w.shadow_capacity * sizeof(ShadowRec))) != hipSuccess) return e;
-    if ((e = w.blocked.reserve(w.shadow_capacity)) != hipSuccess) return e;
-  }
+    w.sgrad.release();
+    hipError_t r = w.shadow.reserve(w.shadow_capacity * sizeof(ShadowRec));
+    if (r == hipSuccess) r = w.blocked.reserve(w.shadow_capacity);
+    return r;
+  };
+  if (w.shadow_capacity < 2ull * npaths && (e = grow_shadow(2ull * npaths)) != hipSuccess) return e;
+  if (grad && (e = w.sgrad.reserve(w.shadow_capacity * sizeof(ShadowGrad))) != hipSuccess) return e;
   // [0] first queue; [2] next queue, [3] shadow records, [4] [5] walk heads; [6] cache records
   uint32_t* counts = w.counts.as<uint32_t>();
   if ((e = hipMemsetAsync(counts, 0, 32, st)) != hipSuccess) return e;
@@ -591,7 +759,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
                                                counts + 4, w.hits.as<PathHit>(), counters);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = stage_mark(ctx, 0, 1)) != hipSuccess) return e;
-    ShadeArgs a;
+    ShadeArgs a{};
     a.sv = sv;
     a.p = p;
     a.in = w.state[cur].as<PathState>();
@@ -605,6 +773,18 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     a.cache_counts = counts + 6;
     a.npaths = npaths;
     a.result = w.result.as<float4>();
+    if (adjoint) {
+      a.sampling = adjoint->sampling;
+      a.ts = w.ts.as<float4>();
+      a.grad = grad;
+      a.d_image = adjoint->d_image;
+      a.pixels = pixels;
+      a.spp = float(p.spp);
+      a.sgrad = w.sgrad.as<ShadowGrad>();
+      a.adj_a = w.adj_a.as<float4>();
+      a.adj_r = w.adj_r.as<float4>();
+      a.grad_lights = adjoint->grad_lights;
+    }
     if (hook) {
       a.queries = hook->queries;
       a.query_weights = hook->query_weights;
@@ -621,7 +801,10 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     for (;;) {
       a.shadow = w.shadow.as<ShadowRec>();
       a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
-      if (!hook)
+      a.sgrad = w.sgrad.as<ShadowGrad>();
+      if (adjoint)
+        k_shade<kShadeAdjoint><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+      else if (!hook)
         k_shade<kShadePlain><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
       else if (hook->mode == CacheHook::kCached)
         k_shade<kShadeCached><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
@@ -635,11 +818,8 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       if (nshadow <= a.shadow_capacity) break;
       // The records did not fit: grow the buffer and shade this bounce again (idempotent: the
       // launch reads one state buffer and writes the other).
-      w.shadow_capacity = uint64_t(nshadow) + nshadow / 4;
-      w.shadow.release();
-      w.blocked.release();
-      if ((e = w.shadow.reserve(w.shadow_capacity * sizeof(ShadowRec))) != hipSuccess) return e;
-      if ((e = w.blocked.reserve(w.shadow_capacity)) != hipSuccess) return e;
+      if ((e = grow_shadow(uint64_t(nshadow) + nshadow / 4)) != hipSuccess) return e;
+      if (grad && (e = w.sgrad.reserve(w.shadow_capacity * sizeof(ShadowGrad))) != hipSuccess) return e;
       if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;
     }
     if (nshadow > 0) {
@@ -654,9 +834,27 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if ((e = stage_mark(ctx, 1, 1)) != hipSuccess) return e;
     }
-    k_resolve<<<blocks(n, 64), 64, 0, st>>>(w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n,
-                                           w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(), w.blocked.as<uint8_t>(),
-                                           w.result.as<float4>(), npaths);
+    if (grad) {
+      ResolveAdjoint ra;
+      ra.sgrad = w.sgrad.as<ShadowGrad>();
+      ra.adj_a = w.adj_a.as<float4>();
+      ra.adj_r = w.adj_r.as<float4>();
+      ra.lpath = adjoint->lpath;
+      ra.d_image = adjoint->d_image;
+      ra.pixels = pixels;
+      ra.spp = float(p.spp);
+      ra.lights = sv.light_count + 1;
+      ra.grad_materials = adjoint->grad_materials;
+      ra.grad_lights = adjoint->grad_lights;
+      k_resolve<true><<<blocks(n, 64), 64, size_t(ra.lights) * 8, st>>>(
+          w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n, w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
+          w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths, ra);
+    } else {
+      k_resolve<false><<<blocks(n, 64), 64, 0, st>>>(w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n,
+                                                    w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
+                                                    w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths,
+                                                    ResolveAdjoint{});
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     cur ^= 1;
     std::swap(w.queue[0], w.queue[1]);
@@ -679,6 +877,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
   }
+  if (grad || !accum) return hipSuccess;  // the gradient pass, or a caller that wants only `result`
   k_accumulate<<<blocks(pixels, 64), 64, 0, st>>>(p, w.result.as<float4>(), accum, coverage);
   return hipGetLastError();
 }

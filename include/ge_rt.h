@@ -445,7 +445,11 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
  *   grad_materials (material_count x 4 doubles: albedo r, g, b, roughness) and grad_lights
  *   (1 + light_count doubles: the sun's intensity sun[3], then each light's color_intensity[3])
  *   receive dLoss/dparameter, summed over every path in double.
- * Materials with their own specular record and max_bounces > 15 return GE_RT_ERROR_UNSUPPORTED. */
+ * Materials with their own specular record and max_bounces > 15 return GE_RT_ERROR_UNSUPPORTED.
+ * By default it runs on the wavefront in two passes: the detached estimate of every path, then a
+ * replay that adds each vertex's gradient from its next-event terms and (df/f) times what the path
+ * gathered after it (path replay backpropagation). GE_RT_PT_MEGAKERNEL replays one path a lane with
+ * its vertices stored and swept backwards; both give the same gradient up to float order. */
 int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
                            const float* d_image, const ge_resolve_material* sampling, float* image,
                            double* grad_materials, double* grad_lights, uint32_t flags);

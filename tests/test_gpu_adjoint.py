@@ -45,20 +45,25 @@ def _d_image(n, seed):
     return d
 
 
-def test_replay_image_is_the_path_tracer(c4):
+KERNELS = pytest.mark.parametrize("mega", [False, True], ids=["wavefront", "megakernel"])
+
+
+@KERNELS
+def test_replay_image_is_the_path_tracer(c4, mega):
     s, g, o, pt = c4
-    img, _, _ = g.path_adjoint(pt)
+    img, _, _ = g.path_adjoint(pt, megakernel=mega)
     acc, _ = g.path_trace(pt)
     assert np.array_equal(img, acc)
     oimg, _, _ = o.path_adjoint(pt)
     assert np.array_equal(img, oimg)
 
 
-def test_gradients_match_the_oracle(c4):
+@KERNELS
+def test_gradients_match_the_oracle(c4, mega):
     s, g, o, pt = c4
     n = pt.width * pt.height
     d = _d_image(n, 1)
-    _, gm, gl = g.path_adjoint(pt, d)
+    _, gm, gl = g.path_adjoint(pt, d, megakernel=mega)
     _, om, ol = o.path_adjoint(pt, d, mode=ob.FP32)
     scale = max(np.abs(om).max(), np.abs(ol).max())
     assert np.count_nonzero(om) > 100
@@ -72,7 +77,21 @@ def test_gradients_match_the_oracle(c4):
     assert cos > 0.999
 
 
-def test_gradients_match_central_differences(ctx, c4):
+def test_wavefront_and_megakernel_agree(c4):
+    # The wavefront adds (df/f) x (L - prefix) at each vertex, the megakernel dw U from its stored
+    # vertices: the same gradient, in other float orders.
+    s, g, o, pt = c4
+    n = pt.width * pt.height
+    d = _d_image(n, 4)
+    _, wm, wl = g.path_adjoint(pt, d)
+    _, mm, ml = g.path_adjoint(pt, d, megakernel=True)
+    scale = max(np.abs(mm).max(), np.abs(ml).max())
+    assert np.abs(wm - mm).max() <= 1e-5 * scale
+    assert np.abs(wl - ml).max() <= 1e-5 * scale
+
+
+@KERNELS
+def test_gradients_match_central_differences(ctx, c4, mega):
     s, g, o, pt = c4
     n = pt.width * pt.height
     d = _d_image(n, 2)
@@ -81,10 +100,10 @@ def test_gradients_match_central_differences(ctx, c4):
     sampling = (abi.ResolveMaterial * len(base))()
     C.memmove(sampling, base, C.sizeof(sampling))
     lights = (abi.ResolveLight * desc.light_count).from_address(desc.lights)
-    _, gm, gl = g.path_adjoint(pt, d)
+    _, gm, gl = g.path_adjoint(pt, d, megakernel=mega)
 
     def objective():
-        img, _, _ = g.path_adjoint(pt, sampling=sampling)
+        img, _, _ = g.path_adjoint(pt, sampling=sampling, megakernel=mega)
         return float(np.sum(d[:, :3].astype(np.float64) * img[:, :3].astype(np.float64) / pt.spp))
 
     rng = np.random.default_rng(3)
