@@ -143,6 +143,7 @@ OK = 0
 DEVICE_PTR = 0x1
 COUNT = 0x2
 PT_MEGAKERNEL = 0x4
+ADJOINT_REUSE_PRIMAL = 0x8
 BVH_LEAF = 0x80000000
 BVH_NONE = 0xFFFFFFFF
 MISS = 0xFFFFFFFF

@@ -298,6 +298,7 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
     }
   }
   s.h_materials.assign(d->materials, d->materials + d->material_count);
+  if (d->light_count) s.h_lights.assign(d->lights, d->lights + d->light_count);
   {
     // The largest cluster material of each mesh: an instance's materials are that plus its base.
     s.mesh_max_material.assign(d->mesh_count, 0u);
@@ -362,6 +363,7 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   if (st != GE_RT_OK) return st;
   std::memcpy(&s.h_instances[first], instances, size_t(count) * sizeof(ge_instance_desc));
   s.pairs_dirty = true;
+  ++s.version;
   GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, instances, size_t(count) * sizeof(ge_instance_desc),
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -518,6 +520,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
                " levels deep, past the traversal stack of " + std::to_string(k_stack));
   s.stack_cap = s.tlas_height_value + blas_max + 2;
   s.built = true;
+  ++s.version;
   return GE_RT_OK;
 }
 
@@ -712,6 +715,16 @@ int32_t path_setup(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace
   return GE_RT_OK;
 }
 
+// A path-trace dispatch's parameters as an adjoint's primal pass would see them: the addresses the
+// backend fills in or the caller's outputs cleared.
+ge_path_trace_params primal_params(const ge_path_trace_params* params) {
+  ge_path_trace_params k = *params;
+  k.accum = k.output = k.coverage = 0;
+  k.clusters = k.mesh = k.triangles = k.instances = k.attributes = k.materials = k.cluster_materials = 0;
+  k.visible = k.lights = k.hit_entries = k.extra = 0;
+  return k;
+}
+
 // The accumulator and coverage a path-trace dispatch adds into: the caller's device pointers (or
 // the block's), or staging copies of host buffers (copied in when resuming at sample_base > 0).
 int32_t path_outputs(ge_rt_ctx* ctx, const ge_path_trace_params* params, float* accum, uint32_t* coverage,
@@ -759,10 +772,16 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
   p.coverage = reinterpret_cast<uint64_t>(d_cov);
   GE_HIP(ctx, begin_timing(ctx));
   unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
+  ctx->primal.valid = false;
   if (flags & GE_RT_PT_MEGAKERNEL) {
     GE_HIP(ctx, launch_path_trace(ctx->stream, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
   } else {
     GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
+    // Every path's radiance is in the wavefront's result buffer until the next wavefront dispatch.
+    ctx->primal.valid = true;
+    ctx->primal.scene = scene;
+    ctx->primal.version = scene->dev.version;
+    ctx->primal.params = primal_params(params);
   }
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   return path_copy_out(ctx, params, accum, coverage, flags, d_accum, d_cov);
@@ -778,6 +797,8 @@ int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   if (st != GE_RT_OK) return st;
   if (!count) return GE_RT_OK;
   GE_HIP(ctx, hipSetDevice(ctx->device));
+  if (std::memcmp(&s.h_materials[first], materials, size_t(count) * sizeof(ge_resolve_material)) == 0) return GE_RT_OK;
+  ++s.version;
   std::memcpy(&s.h_materials[first], materials, size_t(count) * sizeof(ge_resolve_material));
   GE_HIP(ctx, hipMemcpyAsync(s.materials.as<ge_resolve_material>() + first, materials,
                              size_t(count) * sizeof(ge_resolve_material), hipMemcpyHostToDevice, ctx->stream));
@@ -793,6 +814,9 @@ int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t f
   GE_CHECK(ctx, uint64_t(first) + count <= s.light_count, GE_RT_ERROR_INVALID_ARGUMENT, "light range out of bounds");
   if (!count) return GE_RT_OK;
   GE_HIP(ctx, hipSetDevice(ctx->device));
+  if (std::memcmp(&s.h_lights[first], lights, size_t(count) * sizeof(ge_resolve_light)) == 0) return GE_RT_OK;
+  ++s.version;
+  std::memcpy(&s.h_lights[first], lights, size_t(count) * sizeof(ge_resolve_light));
   GE_HIP(ctx, hipMemcpyAsync(s.lights.as<ge_resolve_light>() + first, lights, size_t(count) * sizeof(ge_resolve_light),
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -870,13 +894,20 @@ int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_pa
     p.lights = reinterpret_cast<uint64_t>(sv.lights);
     AdjointHook h{};
     h.sampling = dsamp;
-    uint32_t* cov = nullptr;
-    if (dimg) {
-      GE_HIP(ctx, ctx->tmp[8].reserve(std::max<size_t>(n * 4, 256)));
-      cov = ctx->tmp[8].as<uint32_t>();
-      GE_HIP(ctx, hipMemsetAsync(dimg, 0, n * 16, ctx->stream));  // the image holds this dispatch's samples alone
+    const ge_path_trace_params key = primal_params(params);
+    const bool reuse = (flags & GE_RT_ADJOINT_REUSE_PRIMAL) && !sampling && d_image && !image && ctx->primal.valid &&
+                       ctx->primal.scene == scene && ctx->primal.version == s.version &&
+                       std::memcmp(&ctx->primal.params, &key, sizeof key) == 0;
+    ctx->primal.valid = false;
+    if (!reuse) {
+      uint32_t* cov = nullptr;
+      if (dimg) {
+        GE_HIP(ctx, ctx->tmp[8].reserve(std::max<size_t>(n * 4, 256)));
+        cov = ctx->tmp[8].as<uint32_t>();
+        GE_HIP(ctx, hipMemsetAsync(dimg, 0, n * 16, ctx->stream));  // the image holds this dispatch's samples alone
+      }
+      GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(s), sv, p, reinterpret_cast<float4*>(dimg), cov, nullptr, nullptr, &h));
     }
-    GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(s), sv, p, reinterpret_cast<float4*>(dimg), cov, nullptr, nullptr, &h));
     if (d_image) {
       Wavefront& w = ctx->wave;
       GE_HIP(ctx, w.lpath.reserve(npaths * 16));
@@ -1121,6 +1152,7 @@ int32_t collect_samples(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* c
   hook.samples = net.samples.as<ge_cache_sample>();
   hook.sample_weights = net.sample_weights.as<float4>();
   hook.sample_capacity = capacity;
+  ctx->primal.valid = false;
   GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, nullptr, nullptr, nullptr, &hook));
   GE_HIP(ctx, read_cache_counts(ctx, nullptr, count));
   net.kept = std::min(*count, capacity);
@@ -1261,6 +1293,7 @@ int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_
   hook.query_capacity = uint32_t(qcap);
   GE_HIP(ctx, begin_timing(ctx));
   unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
+  ctx->primal.valid = false;
   GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters, &hook));
   if (training) {
     uint32_t found = 0;

@@ -74,6 +74,18 @@ struct SceneDev {
   uint32_t pair_count = 0;
   bool pairs_dirty = true;
   ge_frame_eye eye{};
+  // Bumped by every change the traced image can see (materials, lights, instances, structures).
+  uint64_t version = 0;
+  std::vector<ge_resolve_light> h_lights;
+};
+
+// What the context's last wavefront path-trace dispatch traced, so an adjoint can reuse its
+// per-path radiance (`result`) as its primal pass.
+struct PrimalKey {
+  bool valid = false;
+  const void* scene = nullptr;
+  uint64_t version = 0;
+  ge_path_trace_params params{};  // with the output addresses cleared
 };
 
 // ---- the wavefront path tracer's records (rt_wavefront.hip) ----
@@ -161,6 +173,7 @@ struct ge_rt_ctx {
   double stage_ms[2] = {0.0, 0.0};
   uint32_t stage_launches[2] = {0, 0};
   ge::Wavefront wave;
+  ge::PrimalKey primal;
   ge::DevBuf resolve_rays;  // the visibility-buffer shadow pass's ray records
   uint32_t persistent_blocks = 8192;  // one-wave workgroups a persistent walk launches: CUs x 32
 };

@@ -81,8 +81,10 @@ class InverseRenderer:
         self._d_image.zero_()
         self._d_image[:, :3] = grad_image.reshape(-1, 3).to(torch.float32)
         torch.cuda.synchronize()  # torch's stream has written what the library's stream reads
+        # The forward render at these parameters is the adjoint's primal pass: the library reuses it
+        # when nothing has changed since (and traces it again otherwise).
         self.scene.path_adjoint(self.params, self._d_image.data_ptr(), None, None,
-                                (self._gm.data_ptr(), self._gl.data_ptr()), device=True)
+                                (self._gm.data_ptr(), self._gl.data_ptr()), device=True, reuse_primal=True)
         torch.cuda.synchronize()
         return self._gm.clone(), self._gl.clone()
 

@@ -187,15 +187,18 @@ class Scene:
         self.ctx.check(lib().ge_rt_scene_update_lights(self.ctx.h, self.h, first, len(records), C.addressof(records)))
 
     def path_adjoint(self, params, d_image=None, sampling=None, image=True, grads=None, device=False,
-                     megakernel=False):
+                     megakernel=False, reuse_primal=False):
         """The path-replay adjoint (ge_rt_path_adjoint): (image, grad_materials, grad_lights).
 
         Host arrays by default: image (n, 4) float32 sample sums, gradients (materials, 4) and
         (1 + lights,) float64 when d_image (n, 4 float32, dLoss/dI) is given. With device=True,
         every argument is a device pointer (int) or None and nothing is returned. megakernel=True
-        replays one path a lane with its vertices stored; the default is the wavefront."""
+        replays one path a lane with its vertices stored; the default is the wavefront.
+        reuse_primal=True takes the primal pass from the context's last path_trace of the same
+        parameters and scene when there is one (then no image: pass image=False or None)."""
         n = params.width * params.height
-        flags = (abi.DEVICE_PTR if device else 0) | (abi.PT_MEGAKERNEL if megakernel else 0)
+        flags = ((abi.DEVICE_PTR if device else 0) | (abi.PT_MEGAKERNEL if megakernel else 0) |
+                 (abi.ADJOINT_REUSE_PRIMAL if reuse_primal else 0))
         if device:
             gm, gl = grads if grads is not None else (None, None)
             self.ctx.check(lib().ge_rt_path_adjoint(self.ctx.h, self.h, C.byref(params), d_image, sampling, image,

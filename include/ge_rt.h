@@ -47,6 +47,11 @@ enum {
 /* Path-trace flag: run the integrator as one kernel, one path per lane, instead of the default
  * wavefront (a kernel per stage over compacted queues). Both give the same radiance to the bit. */
 #define GE_RT_PT_MEGAKERNEL 0x4u
+/* Adjoint flag: with the scene's own sampling (`sampling` NULL), reuse the per-path radiance of
+ * the context's last ge_rt_path_trace when it traced the same parameters over the same scene,
+ * materials, lights and structures (the adjoint's primal pass is that dispatch, to the bit);
+ * otherwise trace the primal pass as usual. `image` must then be NULL. */
+#define GE_RT_ADJOINT_REUSE_PRIMAL 0x8u
 
 /* ---- the engine's GPU records (byte layouts mirrored, sizes asserted below) ----------------- */
 

@@ -90,6 +90,35 @@ def test_wavefront_and_megakernel_agree(c4):
     assert np.abs(wl - ml).max() <= 1e-5 * scale
 
 
+def test_reused_primal_gives_the_same_gradient(c4):
+    # After a path_trace of the same parameters the adjoint takes its per-path radiance as the
+    # primal pass; the gradient is the same to the bit. A changed material or sample_base, or another
+    # dispatch in between, makes it trace the primal again (still the same gradient).
+    s, g, o, pt = c4
+    n = pt.width * pt.height
+    d = _d_image(n, 5)
+    _, gm, gl = g.path_adjoint(pt, d)
+    g.path_trace(pt)
+    _, rm, rl = g.path_adjoint(pt, d, image=False, reuse_primal=True)
+    assert np.array_equal(gm, rm) and np.array_equal(gl, rl)
+    other = abi.PathTraceParams.from_buffer_copy(pt)
+    other.sample_base = 7
+    g.path_trace(other)
+    _, sm, sl = g.path_adjoint(pt, d, image=False, reuse_primal=True)
+    assert np.array_equal(gm, sm) and np.array_equal(gl, sl)
+    desc = s.desc()
+    rec = (abi.ResolveMaterial * 1)()
+    C.memmove(rec, desc.materials, C.sizeof(rec))
+    g.path_trace(pt)
+    rec[0].albedo[0] = rec[0].albedo[0] * 0.5
+    g.update_materials(0, rec)  # the primal no longer matches the scene
+    _, cm, cl = g.path_adjoint(pt, d, image=False, reuse_primal=True)
+    _, fm, fl = g.path_adjoint(pt, d)
+    assert np.array_equal(cm, fm) and np.array_equal(cl, fl)
+    C.memmove(rec, desc.materials, C.sizeof(rec))
+    g.update_materials(0, rec)
+
+
 @KERNELS
 def test_gradients_match_central_differences(ctx, c4, mega):
     s, g, o, pt = c4
