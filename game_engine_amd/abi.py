@@ -100,7 +100,16 @@ class SceneDesc(C.Structure):
                 ("meshes", C.c_void_p), ("instances", C.c_void_p), ("instance_count", u32),
                 ("material_count", u32), ("materials", C.c_void_p),
                 ("cluster_materials", C.c_void_p), ("lights", C.c_void_p), ("light_count", u32),
-                ("pad", u32)]
+                ("texture_count", u32), ("textures", C.c_void_p), ("texels", C.c_void_p),
+                ("texel_bytes", u64), ("samplers", C.c_void_p), ("sampler_count", u32), ("pad", u32)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("format", u32), ("reserved", u32), ("offset", u64)]
+
+
+class SamplerDesc(C.Structure):
+    _fields_ = [("address_u", u32), ("address_v", u32), ("mag", u32), ("min", u32)]
 
 
 # The radiance cache's records (no counterpart in the reference).
@@ -133,7 +142,8 @@ RECEIVER_DTYPE = np.dtype([("position", "<f4", 3), ("instance_id", "<u4"), ("geo
 
 SIZES = {ClusterDesc: 48, VertexAttributes: 8, MeshDesc: 64, InstanceDesc: 96, FrameEye: 48,
          ResolveMaterial: 128, ResolveLight: 64, PathTraceParams: 368, RayVisibilityParams: 256,
-         ShadowParams: 48, Counters: 104, ShadowResolveParams: 128, CacheTrainParams: 16}
+         ShadowParams: 48, Counters: 104, ShadowResolveParams: 128, CacheTrainParams: 16,
+         TextureDesc: 24, SamplerDesc: 16, SceneDesc: 152}
 for _t, _n in SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
@@ -144,6 +154,8 @@ DEVICE_PTR = 0x1
 COUNT = 0x2
 PT_MEGAKERNEL = 0x4
 ADJOINT_REUSE_PRIMAL = 0x8
+ALPHA_MASKS = 0x10
+CULL_MASKED = 0x20
 BVH_LEAF = 0x80000000
 BVH_NONE = 0xFFFFFFFF
 MISS = 0xFFFFFFFF
@@ -154,6 +166,18 @@ PT_ALPHA_MASKS = 8
 BUILD_FULL, BUILD_SUBSET, BUILD_TOP = 0, 1, 2
 SHADOW_HALF_RATE = 0x1
 SHADOW_JITTER = 0x2
+SHADOW_ALPHA_MASKS = 0x4
+NO_TEXTURE = 0xFFFFFFFF
+SAME_SAMPLER = 0xFFFF
+MATERIAL_MIPPED = 1
+MATERIAL_NORMAL_RG = 2
+MATERIAL_UV_TRANSFORM = 4
+MATERIAL_MASKED = 32
+MATERIAL_STOCHASTIC = 131072
+FORMAT_RGBA8_UNORM = 37
+FORMAT_RGBA8_SRGB = 43
+FILTER_NEAREST, FILTER_LINEAR = 0, 1
+ADDRESS_REPEAT, ADDRESS_MIRRORED_REPEAT, ADDRESS_CLAMP_TO_EDGE, ADDRESS_CLAMP_TO_BORDER = 0, 1, 2, 3
 SHADOW_NOT_TAKEN = 0xFFFFFFFF
 
 

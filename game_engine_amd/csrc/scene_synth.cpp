@@ -62,11 +62,29 @@ uint32_t encode_oct(F3 n) {
   return (uint32_t(x) & 0xffffu) | ((uint32_t(y) & 0xffffu) << 16);
 }
 
-// A source mesh before clustering: positions, normals, and triangle corner indices.
+// A source mesh before clustering: positions, normals, UVs (two per vertex, or none), and triangle
+// corner indices.
 struct Source {
   std::vector<F3> pos, nrm;
+  std::vector<float> uv;
   std::vector<uint32_t> idx;
 };
+
+// A float as an IEEE half, rounded to nearest even (what a half2 UV stream holds).
+uint32_t float_to_half(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const float a = std::fabs(f);
+  if (!(a < 65520.0f)) return sign | 0x7c00u;
+  if (a < 6.103515625e-05f) return sign | uint32_t(std::nearbyint(a * 16777216.0f));
+  const uint32_t ax = x & 0x7fffffffu;
+  const uint32_t m = ax & 0x7fffffu;
+  uint32_t h = (((ax >> 23) - 112u) << 10) | (m >> 13);
+  const uint32_t rem = m & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+  return sign | h;
+}
 
 void icosphere(Source& m, F3 c, float radius, int subdivisions) {
   const float t = (1.0f + std::sqrt(5.0f)) * 0.5f;
@@ -169,6 +187,9 @@ struct ge_synth {
   std::vector<ge_resolve_material> materials;
   std::vector<uint32_t> cluster_materials;
   std::vector<ge_resolve_light> lights;
+  std::vector<ge_texture_desc> textures;
+  std::vector<ge_sampler_desc> samplers;
+  std::vector<uint8_t> texels;
   // Frame.
   F3 eye{0, 0, 0}, target{0, 0, 1};
   float fov_deg = 60.0f;
@@ -185,7 +206,8 @@ namespace {
 // Appends a mesh: the source's triangles in clusters of `cluster_size` in source order, each
 // cluster with its own block of vertices (cluster-ordered copies), quantized onto the mesh's own
 // 16-bit grid (quantize_positions, cluster.cpp:437-466), with `material` per cluster.
-uint32_t add_mesh(ge_synth& s, const Source& src, uint32_t cluster_size, const std::vector<uint32_t>& tri_material) {
+uint32_t add_mesh(ge_synth& s, const Source& src, uint32_t cluster_size, const std::vector<uint32_t>& tri_material,
+                  bool split_on_material = false) {
   ge_mesh_desc mesh{};
   mesh.first_cluster = uint32_t(s.clusters.size());
   const uint32_t vbase = uint32_t(s.vertices.size() / 3);
@@ -197,7 +219,8 @@ uint32_t add_mesh(ge_synth& s, const Source& src, uint32_t cluster_size, const s
     uint32_t t1 = t0;
     {
       std::vector<uint32_t> probe;
-      while (t1 < ntri && t1 - t0 < cluster_size) {
+      while (t1 < ntri && t1 - t0 < cluster_size &&
+             !(split_on_material && tri_material[t1] != tri_material[t0])) {
         uint32_t fresh = 0;
         for (int k = 0; k < 3; ++k) {
           const uint32_t sv = src.idx[size_t(t1) * 3 + k];
@@ -237,7 +260,10 @@ uint32_t add_mesh(ge_synth& s, const Source& src, uint32_t cluster_size, const s
     for (uint32_t sv : locals) {
       const F3 p = src.pos[sv];
       s.vertices.insert(s.vertices.end(), {p.x, p.y, p.z});
-      s.attributes.push_back({encode_oct(norm3(src.nrm[sv])), 0u});
+      const uint32_t uv = src.uv.empty() ? 0u
+                                         : float_to_half(src.uv[size_t(sv) * 2]) |
+                                               (float_to_half(src.uv[size_t(sv) * 2 + 1]) << 16);
+      s.attributes.push_back({encode_oct(norm3(src.nrm[sv])), uv});
       lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
       hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
     }
@@ -736,6 +762,242 @@ extern "C" {
 
 uint64_t ge_synth_splitmix64(uint64_t* state) { return splitmix64(*state); }
 
+namespace {
+
+// ---- config 5: an alpha-tested forest ------------------------------------------------------------
+float linear_to_srgb(float c) {
+  c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+  return c <= 0.0031308f ? 12.92f * c : 1.055f * std::pow(c, 1.0f / 2.4f) - 0.055f;
+}
+uint8_t unorm8(float c) { return uint8_t(std::lround((c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c)) * 255.0f)); }
+
+uint32_t add_texture(ge_synth& s, uint32_t w, uint32_t h, uint32_t format, const std::vector<uint8_t>& rgba) {
+  ge_texture_desc t{};
+  t.width = w;
+  t.height = h;
+  t.format = format;
+  t.offset = s.texels.size();
+  s.texels.insert(s.texels.end(), rgba.begin(), rgba.end());
+  s.textures.push_back(t);
+  return uint32_t(s.textures.size() - 1);
+}
+
+// A 256 x 256 sRGB atlas of 4 x 4 leaves, each a lobed ellipse at its own angle in a 64-texel cell,
+// alpha 1 inside and 0 outside with a two-texel ramp between.
+std::vector<uint8_t> leaf_atlas(uint32_t seed) {
+  std::vector<uint8_t> out(256 * 256 * 4);
+  for (uint32_t cell = 0; cell < 16; ++cell) {
+    Rng rng(seed * 131u + cell);
+    const float angle = rng.uniform(-0.6f, 0.6f) + 0.785f;
+    const float a = rng.uniform(24.0f, 29.0f), b = rng.uniform(9.0f, 14.0f);
+    const float lobes = float(int(rng.uniform(3.0f, 7.0f))), depth = rng.uniform(0.05f, 0.2f);
+    const float tint = rng.uniform(0.75f, 1.2f);
+    const float cs = std::cos(angle), sn = std::sin(angle);
+    for (uint32_t y = 0; y < 64; ++y)
+      for (uint32_t x = 0; x < 64; ++x) {
+        const float px = float(x) + 0.5f - 32.0f, py = float(y) + 0.5f - 32.0f;
+        const float u = cs * px + sn * py, v = -sn * px + cs * py;
+        const float theta = std::atan2(v, u);
+        const float r = std::sqrt((u / a) * (u / a) + (v / b) * (v / b)) / (1.0f + depth * std::sin(lobes * theta));
+        const float alpha = std::min(std::max((1.0f - r) * a * 0.5f, 0.0f), 1.0f);
+        const float vein = std::fabs(v) < 1.0f ? 0.7f : 1.0f;
+        const float lin[3] = {0.10f * tint * vein, 0.28f * tint * vein, 0.05f * tint};
+        uint8_t* t = &out[((size_t(cell / 4) * 64 + y) * 256 + size_t(cell % 4) * 64 + x) * 4];
+        for (int k = 0; k < 3; ++k) t[k] = unorm8(linear_to_srgb(lin[k]));
+        t[3] = unorm8(alpha);
+      }
+  }
+  return out;
+}
+
+// Bark: sRGB colour in vertical streaks and a UNORM normal map of the same grooves.
+void bark_textures(uint32_t seed, std::vector<uint8_t>& colour, std::vector<uint8_t>& normal) {
+  const uint32_t n = 128;
+  colour.assign(n * n * 4, 255);
+  normal.assign(n * n * 4, 255);
+  auto height = [&](float x, float y) {
+    return 0.5f + 0.35f * std::sin(x * 0.39269908f + 2.0f * value_noise(x * 0.1f, y * 0.05f, seed)) +
+           0.15f * value_noise(x * 0.3f, y * 0.3f, seed + 7u);
+  };
+  for (uint32_t y = 0; y < n; ++y)
+    for (uint32_t x = 0; x < n; ++x) {
+      const float h = height(float(x), float(y));
+      const float lin[3] = {0.22f * (0.55f + 0.45f * h), 0.14f * (0.55f + 0.45f * h), 0.08f * (0.55f + 0.45f * h)};
+      uint8_t* c = &colour[(size_t(y) * n + x) * 4];
+      for (int k = 0; k < 3; ++k) c[k] = unorm8(linear_to_srgb(lin[k]));
+      const float dx = (height(float(x) + 1.0f, float(y)) - height(float(x) - 1.0f, float(y))) * 2.0f;
+      const float dy = (height(float(x), float(y) + 1.0f) - height(float(x), float(y) - 1.0f)) * 2.0f;
+      const F3 nv = norm3({-dx, -dy, 1.0f});
+      uint8_t* m = &normal[(size_t(y) * n + x) * 4];
+      m[0] = unorm8(nv.x * 0.5f + 0.5f);
+      m[1] = unorm8(nv.y * 0.5f + 0.5f);
+      m[2] = unorm8(nv.z * 0.5f + 0.5f);
+    }
+}
+
+// `trees` trees of 8 meshes (an 8-sided trunk of 96 triangles and 320 to 480 alpha-tested leaf
+// cards of two triangles in a crown) on a jittered 6 m grid over gently rolling ground, under the
+// sun and sky. Textures: a leaf atlas (alpha-tested through a UV transform onto one of six cells),
+// bark colour with a normal map, ground colour with a metallic-roughness map.
+void build_config5(ge_synth& s, uint64_t seed, uint32_t trees) {
+  Rng rng(seed ^ 0xf0e57u);
+  const uint32_t leaf_tex = add_texture(s, 256, 256, GE_FORMAT_R8G8B8A8_SRGB, leaf_atlas(uint32_t(seed)));
+  std::vector<uint8_t> bark_c, bark_n;
+  bark_textures(uint32_t(seed) + 3u, bark_c, bark_n);
+  const uint32_t bark_tex = add_texture(s, 128, 128, GE_FORMAT_R8G8B8A8_SRGB, bark_c);
+  const uint32_t bark_nrm = add_texture(s, 128, 128, GE_FORMAT_R8G8B8A8_UNORM, bark_n);
+  std::vector<uint8_t> ground_c(256 * 256 * 4, 255), ground_mr(64 * 64 * 4, 255);
+  for (uint32_t y = 0; y < 256; ++y)
+    for (uint32_t x = 0; x < 256; ++x) {
+      const float t = value_noise(float(x) * 0.06f, float(y) * 0.06f, uint32_t(seed) + 11u);
+      const float lin[3] = {0.08f + 0.12f * t, 0.16f + 0.02f * t, 0.04f + 0.05f * t};
+      for (int k = 0; k < 3; ++k) ground_c[(size_t(y) * 256 + x) * 4 + k] = unorm8(linear_to_srgb(lin[k]));
+    }
+  for (uint32_t y = 0; y < 64; ++y)
+    for (uint32_t x = 0; x < 64; ++x) {
+      uint8_t* m = &ground_mr[(size_t(y) * 64 + x) * 4];
+      m[1] = unorm8(0.7f + 0.3f * value_noise(float(x) * 0.2f, float(y) * 0.2f, uint32_t(seed) + 13u));
+      m[2] = 0;
+    }
+  const uint32_t ground_tex = add_texture(s, 256, 256, GE_FORMAT_R8G8B8A8_SRGB, ground_c);
+  const uint32_t ground_mrt = add_texture(s, 64, 64, GE_FORMAT_R8G8B8A8_UNORM, ground_mr);
+  s.samplers.push_back({GE_ADDRESS_REPEAT, GE_ADDRESS_REPEAT, GE_FILTER_LINEAR, GE_FILTER_LINEAR});
+  s.samplers.push_back({GE_ADDRESS_CLAMP_TO_EDGE, GE_ADDRESS_CLAMP_TO_EDGE, GE_FILTER_LINEAR, GE_FILTER_LINEAR});
+  s.samplers.push_back({GE_ADDRESS_MIRRORED_REPEAT, GE_ADDRESS_MIRRORED_REPEAT, GE_FILTER_NEAREST, GE_FILTER_NEAREST});
+
+  ge_resolve_material ground = make_material(1.0f, 1.0f, 1.0f, 1.0f, 0.0f);
+  ground.albedo_texture = ground_tex;
+  ground.sampler = 0;
+  ground.uv_scale = 0.25f;
+  ground.metallic_roughness_texture = ground_mrt;
+  ground.samplers_mr_normal = 0xffff0002u;  // the map through the nearest, mirrored sampler
+  s.materials.push_back(ground);
+  ge_resolve_material bark = make_material(1.0f, 1.0f, 1.0f, 0.85f, 0.0f);
+  bark.albedo_texture = bark_tex;
+  bark.sampler = 0;
+  bark.normal_texture = bark_nrm;
+  s.materials.push_back(bark);
+  for (uint32_t k = 0; k < 6; ++k) {
+    ge_resolve_material leaf = make_material(1.0f, 1.0f, 1.0f, 0.6f, 0.0f);
+    leaf.flags = GE_MATERIAL_MASKED | GE_MATERIAL_UV_TRANSFORM;
+    leaf.albedo_texture = leaf_tex;
+    leaf.sampler = 1;
+    leaf.uv_transform[0] = 0.25f;
+    leaf.uv_transform[3] = 0.25f;
+    leaf.uv_offset[0] = 0.25f * float(k % 4);
+    leaf.uv_offset[1] = 0.25f * float(k / 4);
+    leaf.uv_offset[2] = 0.5f;  // the cutoff
+    leaf.uv_offset[3] = 1.0f;  // the base colour factor's alpha
+    s.materials.push_back(leaf);
+  }
+
+  for (uint32_t m = 0; m < 8; ++m) {
+    Source src;
+    std::vector<uint32_t> mats;
+    const float H = rng.uniform(8.0f, 14.0f), r0 = rng.uniform(0.25f, 0.4f);
+    const int sides = 8, rings = 6;
+    for (int j = 0; j <= rings; ++j) {
+      const float y = 0.55f * H * float(j) / float(rings);
+      const float r = r0 + (0.08f - r0) * float(j) / float(rings);
+      for (int i = 0; i <= sides; ++i) {
+        const float a = 6.2831853f * float(i) / float(sides);
+        src.pos.push_back({r * std::cos(a), y, r * std::sin(a)});
+        src.nrm.push_back({std::cos(a), 0.0f, std::sin(a)});
+        src.uv.push_back(2.0f * float(i) / float(sides));
+        src.uv.push_back(y * 0.5f);
+      }
+    }
+    for (int j = 0; j < rings; ++j)
+      for (int i = 0; i < sides; ++i) {
+        const uint32_t a = uint32_t(j * (sides + 1) + i), b = a + 1, c = a + uint32_t(sides + 1), d = c + 1;
+        src.idx.insert(src.idx.end(), {a, c, b, b, c, d});
+        mats.insert(mats.end(), {1u, 1u});
+      }
+    const uint32_t cards = 320 + uint32_t(rng.next() * 160.0);
+    for (uint32_t k = 0; k < cards; ++k) {
+      F3 c;
+      do {
+        c = {rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)};
+      } while (dot3(c, c) > 1.0f);
+      c = {c.x * 0.28f * H, 0.68f * H + c.y * 0.3f * H, c.z * 0.28f * H};
+      float rot[3][3];
+      random_rotation(rng, rot);
+      const float half = 0.5f * rng.uniform(0.35f, 0.7f);
+      const F3 ax{rot[0][0], rot[1][0], rot[2][0]}, ay{rot[0][1], rot[1][1], rot[2][1]}, an{rot[0][2], rot[1][2], rot[2][2]};
+      const uint32_t base = uint32_t(src.pos.size());
+      const float corner[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+      for (int q = 0; q < 4; ++q) {
+        src.pos.push_back(c + ax * (half * corner[q][0]) + ay * (half * corner[q][1]));
+        src.nrm.push_back(an);
+        src.uv.push_back(0.5f + 0.5f * corner[q][0]);
+        src.uv.push_back(0.5f - 0.5f * corner[q][1]);
+      }
+      src.idx.insert(src.idx.end(), {base, base + 1, base + 2, base, base + 2, base + 3});
+      const uint32_t leaf = 2u + (k / 64u) % 6u;  // 64 cards (one cluster) a leaf material
+      mats.insert(mats.end(), {leaf, leaf});
+    }
+    add_mesh(s, src, 128, mats, true);
+  }
+
+  const uint32_t per_side = uint32_t(std::ceil(std::sqrt(double(std::max(trees, 1u)))));
+  const float spacing = 6.0f, side = spacing * float(per_side);
+  auto terrain = [&](float x, float z) { return 1.5f * value_noise(x / 40.0f, z / 40.0f, uint32_t(seed) + 21u); };
+  {
+    Source g;
+    const int n = 128;
+    const float lo = -8.0f, step = (side + 16.0f) / float(n);
+    for (int j = 0; j <= n; ++j)
+      for (int i = 0; i <= n; ++i) {
+        const float x = lo + step * float(i), z = lo + step * float(j);
+        g.pos.push_back({x, terrain(x, z), z});
+        const float dx = terrain(x + 0.5f, z) - terrain(x - 0.5f, z), dz = terrain(x, z + 0.5f) - terrain(x, z - 0.5f);
+        g.nrm.push_back(norm3({-dx, 1.0f, -dz}));
+        g.uv.push_back(x);
+        g.uv.push_back(z);
+      }
+    std::vector<uint32_t> gm;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const uint32_t a = uint32_t(j * (n + 1) + i), b = a + 1, c = a + uint32_t(n + 1), d = c + 1;
+        g.idx.insert(g.idx.end(), {a, c, b, b, c, d});
+        gm.insert(gm.end(), {0u, 0u});
+      }
+    add_mesh(s, g, 128, gm, true);
+  }
+  uint32_t first_pair = 0;
+  const float identity[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  s.instances.push_back(make_instance(8, identity, {0, 0, 0}, 0, first_pair));
+  first_pair += s.meshes[8].cluster_count;
+  for (uint32_t t = 0; t < trees; ++t) {
+    const float x = spacing * (float(t % per_side) + 0.5f) + rng.uniform(-2.0f, 2.0f);
+    const float z = spacing * (float(t / per_side) + 0.5f) + rng.uniform(-2.0f, 2.0f);
+    const float yaw = rng.uniform(0.0f, 6.2831853f), scale = rng.uniform(0.8f, 1.25f);
+    const float rot[3][3] = {{scale * std::cos(yaw), 0, scale * std::sin(yaw)},
+                             {0, scale, 0},
+                             {-scale * std::sin(yaw), 0, scale * std::cos(yaw)}};
+    const uint32_t mesh = uint32_t(rng.next() * 8.0) & 7u;
+    ge_instance_desc in = make_instance(mesh, rot, {x, terrain(x, z) - 0.1f, z}, 0, first_pair);
+    in.scale_max = scale;
+    s.instances.push_back(in);
+    first_pair += s.meshes[mesh].cluster_count;
+  }
+  set_sun(s, -0.45f, 0.55f, 0.5f, 3.0f);
+  s.sky[0] = 0.45f;
+  s.sky[1] = 0.62f;
+  s.sky[2] = 0.9f;
+  s.ground[0] = 0.12f;
+  s.ground[1] = 0.16f;
+  s.ground[2] = 0.07f;
+  s.eye = {0.5f * side, 32.0f, -12.0f};
+  s.target = {0.5f * side, 4.0f, 0.3f * side};
+  s.fov_deg = 55.0f;
+  s.spp = 1;
+  s.max_bounces = 4;
+  s.reach = 1.5f * (side + 40.0f);
+}
+
+}  // namespace
+
 ge_synth* ge_synth_create(uint32_t config, uint64_t seed, uint32_t detail, uint32_t terrain) {
   auto* s = new ge_synth();
   s->config = config;
@@ -744,6 +1006,7 @@ ge_synth* ge_synth_create(uint32_t config, uint64_t seed, uint32_t detail, uint3
     case 2: build_config2(*s, seed, detail ? detail : 4096u); break;
     case 4: build_config4(*s, seed); break;
     case 3: build_config3(*s, seed, detail ? detail : 16384u, terrain ? std::max(terrain, 2u) : 4096u); break;
+    case 5: build_config5(*s, seed, detail ? detail : 4096u); break;
     default: delete s; return nullptr;
   }
   if (s->quantized.size() & 1u) s->quantized.push_back(0);  // padded to an even count (cluster.h:436)
@@ -751,7 +1014,7 @@ ge_synth* ge_synth_create(uint32_t config, uint64_t seed, uint32_t detail, uint3
 }
 
 ge_synth* ge_synth_from_triangles(const float* positions, uint32_t triangle_count, uint32_t cluster_size,
-                                  int32_t attributes, uint64_t seed) {
+                                  int32_t attributes, uint64_t seed, const float* uvs) {
   if (!positions || triangle_count == 0 || cluster_size == 0 || cluster_size > 128) return nullptr;
   auto* s = new ge_synth();
   s->config = 0;
@@ -766,6 +1029,7 @@ ge_synth* ge_synth_from_triangles(const float* positions, uint32_t triangle_coun
       src.nrm.push_back(n);
       src.idx.push_back(uint32_t(src.idx.size()));
     }
+    if (uvs) src.uv.insert(src.uv.end(), uvs + size_t(t) * 6, uvs + size_t(t) * 6 + 6);
   }
   std::vector<uint32_t> mats(triangle_count, 0u);
   add_mesh(*s, src, cluster_size, mats);
@@ -813,6 +1077,25 @@ void ge_synth_scene(const ge_synth* s, ge_scene_desc* out) {
   out->cluster_materials = s->cluster_materials.data();
   out->lights = s->lights.empty() ? nullptr : s->lights.data();
   out->light_count = uint32_t(s->lights.size());
+  out->texture_count = uint32_t(s->textures.size());
+  out->textures = s->textures.empty() ? nullptr : s->textures.data();
+  out->texels = s->texels.empty() ? nullptr : s->texels.data();
+  out->texel_bytes = s->texels.size();
+  out->samplers = s->samplers.empty() ? nullptr : s->samplers.data();
+  out->sampler_count = uint32_t(s->samplers.size());
+}
+
+void ge_synth_set_materials(ge_synth* s, const ge_resolve_material* materials, uint32_t count,
+                            const uint32_t* cluster_materials) {
+  s->materials.assign(materials, materials + count);
+  if (cluster_materials) s->cluster_materials.assign(cluster_materials, cluster_materials + s->clusters.size());
+}
+
+void ge_synth_set_textures(ge_synth* s, const ge_texture_desc* textures, uint32_t texture_count, const uint8_t* texels,
+                           uint64_t texel_bytes, const ge_sampler_desc* samplers, uint32_t sampler_count) {
+  s->textures.assign(textures, textures + texture_count);
+  s->texels.assign(texels, texels + texel_bytes);
+  s->samplers.assign(samplers, samplers + sampler_count);
 }
 
 void ge_synth_frame(const ge_synth* s, uint32_t width, uint32_t height, ge_path_trace_params* pt,

@@ -19,6 +19,12 @@
  *   config 4: 2,048 objects as config 1 (524,288 triangles) in a 200 m cube, 1,024 materials
  *             (albedo U[0.1, 0.9], roughness U[0.05, 1], one in five metal), the sun and eight
  *             point lights, 16 spp and 6 bounces: the inverse-rendering scene.
+ *   config 5: `detail` trees (default 4096) of 8 meshes, each an 8-sided trunk of 96 triangles and
+ *             320 to 480 alpha-tested leaf cards (two triangles each) in its crown, on a jittered
+ *             6 m grid over gently rolling ground of 32,768 triangles; textures at level 0 (a 4 x 4
+ *             leaf atlas whose alpha is the mask, read through a UV transform; bark colour and a
+ *             normal map; ground colour and a metallic-roughness map), three samplers (linear
+ *             repeat, linear clamp, nearest mirrored), the sun and sky: the alpha-masked workload.
  */
 #ifndef GE_SCENE_SYNTH_H
 #define GE_SCENE_SYNTH_H
@@ -31,14 +37,21 @@ extern "C" {
 
 typedef struct ge_synth ge_synth;
 
-/* A scene of configuration 1, 2 or 3 (above). `detail` = 0 takes the configuration's size;
+/* A scene of configuration 1 to 5 (above). `detail` = 0 takes the configuration's size;
  * `terrain` is config 3's heightfield samples per side (0: 4096). */
 ge_synth* ge_synth_create(uint32_t config, uint64_t seed, uint32_t detail, uint32_t terrain);
 /* One mesh, one identity instance, from raw triangles (9 floats each) in clusters of
- * `cluster_size` (<= 128) triangles, flat or smooth octahedral normals (`attributes`), one
- * material per cluster with albedo from `seed`. */
+ * `cluster_size` (<= 128) triangles, flat octahedral normals (`attributes`; 0: none), UVs from
+ * `uvs` (6 floats a triangle, stored as halves; NULL: zero), one material per cluster with albedo
+ * from `seed`. */
 ge_synth* ge_synth_from_triangles(const float* positions, uint32_t triangle_count, uint32_t cluster_size,
-                                  int32_t attributes, uint64_t seed);
+                                  int32_t attributes, uint64_t seed, const float* uvs);
+/* Replaces the materials (and, when not NULL, the per-cluster material indices). */
+void ge_synth_set_materials(ge_synth* s, const ge_resolve_material* materials, uint32_t count,
+                            const uint32_t* cluster_materials);
+/* Replaces the textures, their texel pool and the samplers (copied). */
+void ge_synth_set_textures(ge_synth* s, const ge_texture_desc* textures, uint32_t texture_count, const uint8_t* texels,
+                           uint64_t texel_bytes, const ge_sampler_desc* samplers, uint32_t sampler_count);
 void ge_synth_destroy(ge_synth* s);
 /* Pointers into the synth's arrays (valid until ge_synth_destroy). */
 void ge_synth_scene(const ge_synth* s, ge_scene_desc* out);

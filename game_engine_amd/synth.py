@@ -24,7 +24,10 @@ def lib():
         L.ge_synth_create.restype = C.c_void_p
         L.ge_synth_create.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]
         L.ge_synth_from_triangles.restype = C.c_void_p
-        L.ge_synth_from_triangles.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint64]
+        L.ge_synth_from_triangles.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint64, C.c_void_p]
+        L.ge_synth_set_materials.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ge_synth_set_textures.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p,
+                                            C.c_uint32]
         L.ge_synth_destroy.argtypes = [C.c_void_p]
         L.ge_synth_scene.argtypes = [C.c_void_p, C.POINTER(abi.SceneDesc)]
         L.ge_synth_frame.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32,
@@ -52,10 +55,31 @@ class Synth:
         return cls(lib().ge_synth_create(config, seed, detail, terrain))
 
     @classmethod
-    def from_triangles(cls, positions, cluster_size=128, attributes=True, seed=0):
+    def from_triangles(cls, positions, cluster_size=128, attributes=True, seed=0, uvs=None):
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
+        uv = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 6)
+        if uv is not None:
+            assert uv.shape[0] == pos.shape[0]
         return cls(lib().ge_synth_from_triangles(pos.ctypes.data, pos.shape[0], cluster_size,
-                                                  1 if attributes else 0, seed))
+                                                  1 if attributes else 0, seed,
+                                                  None if uv is None else uv.ctypes.data))
+
+    def set_materials(self, materials, cluster_materials=None):
+        """Replaces the materials (a ResolveMaterial array) and optionally the per-cluster indices."""
+        cm = None if cluster_materials is None else np.ascontiguousarray(cluster_materials, dtype=np.uint32)
+        if cm is not None:
+            assert len(cm) == self._desc.cluster_count
+        lib().ge_synth_set_materials(self.h, C.addressof(materials), len(materials),
+                                     None if cm is None else cm.ctypes.data)
+        lib().ge_synth_scene(self.h, C.byref(self._desc))
+
+    def set_textures(self, textures, texels, samplers):
+        """Replaces the textures (TextureDesc array), their texel pool (uint8) and the samplers."""
+        tx = np.ascontiguousarray(texels, dtype=np.uint8)
+        lib().ge_synth_set_textures(self.h, C.addressof(textures) if len(textures) else None, len(textures),
+                                    tx.ctypes.data, tx.nbytes, C.addressof(samplers) if len(samplers) else None,
+                                    len(samplers))
+        lib().ge_synth_scene(self.h, C.byref(self._desc))
 
     def close(self):
         if self.h:

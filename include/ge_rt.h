@@ -52,6 +52,16 @@ enum {
  * materials, lights and structures (the adjoint's primal pass is that dispatch, to the bit);
  * otherwise trace the primal pass as usual. `image` must then be NULL. */
 #define GE_RT_ADJOINT_REUSE_PRIMAL 0x8u
+/* Query flags for alpha-tested materials (a material with GE_MATERIAL_MASKED; every other triangle
+ * is opaque). Without either flag a query treats masked triangles as opaque, as a ray that forces
+ * opacity does (RAY_FLAG_FORCE_OPAQUE, path_trace.slang:196-203).
+ * GE_RT_ALPHA_MASKS: a masked candidate is kept where its mask keeps the hit at level 0
+ * (pt_candidate_keeps, path_trace.slang:117-135; mask_keeps_level0, alpha_mask.slang:66-75).
+ * GE_RT_CULL_MASKED: masked candidates are left out (RAY_FLAG_CULL_NON_OPAQUE, the camera ray of a
+ * frame without masks, path_trace.slang:280-281). ge_rt_trace_closest takes either,
+ * ge_rt_trace_any and ge_rt_shadow_pass GE_RT_ALPHA_MASKS. */
+#define GE_RT_ALPHA_MASKS 0x10u
+#define GE_RT_CULL_MASKED 0x20u
 
 /* ---- the engine's GPU records (byte layouts mirrored, sizes asserted below) ----------------- */
 
@@ -139,6 +149,51 @@ typedef struct ge_resolve_material {
   float specular[4]; /* rgb F0 - 0.04, w 1 - F90; zero: the 4 % and 1 */
 } ge_resolve_material;
 #define GE_NO_TEXTURE 0xffffffffu
+/* A per-slot sampler half that borrows the material's `sampler` (material.slang:67-68, 92-97):
+ * samplers_mr_normal holds the metallic-roughness slot's in bits 0-15 and the normal map's in
+ * 16-31, samplers_occlusion_emissive the emissive map's in bits 16-31. */
+#define GE_SAME_SAMPLER 0xffffu
+/* ResolveMaterial::flags (material.slang:35-59; scene.slang:360). Level 0 is all a ray samples, so
+ * GE_MATERIAL_MIPPED changes nothing here. GE_MATERIAL_MASKED: alpha-tested, alpha =
+ * uv_offset.w times the albedo texture's alpha, kept where alpha >= uv_offset.z. Ground detail and
+ * water are the resolve's and take no part in the path tracer without their tables; the stochastic
+ * flag (131072) and foliage optics (bits 24-31) are refused with GE_RT_ERROR_UNSUPPORTED. */
+#define GE_MATERIAL_MIPPED 1u
+#define GE_MATERIAL_NORMAL_RG 2u
+#define GE_MATERIAL_UV_TRANSFORM 4u
+#define GE_MATERIAL_MASKED 32u
+#define GE_MATERIAL_STOCHASTIC 131072u
+
+/* One texture of the scene's bindless array (g_textures, material.slang:30; the GpuScene's bindless
+ * set, gpu_scene.h:198-204), at level 0 only: width x height RGBA8 texels, rows top to bottom,
+ * starting `offset` bytes into the scene's texel pool. 24 bytes. */
+typedef struct ge_texture_desc {
+  uint32_t width;
+  uint32_t height;
+  uint32_t format; /* GE_FORMAT_*: gfx::Format's numbering (domain/gfx/include/domain/gfx/rhi.h:240-246) */
+  uint32_t reserved;
+  uint64_t offset;
+} ge_texture_desc;
+#define GE_FORMAT_R8G8B8A8_UNORM 37u
+#define GE_FORMAT_R8G8B8A8_SRGB 43u /* rgb through the sRGB transfer function, alpha linear */
+
+/* One sampler of the bindless array (g_samplers, material.slang:31): gfx::SamplerDesc
+ * (domain/gfx/include/domain/gfx/bindless.h:149-157) as level 0 reads it. A level-0 read
+ * (SampleLevel(..., 0)) has a level of detail of 0 and so filters with `mag`; `min` is kept for
+ * the record. Address modes and filters take gfx's numbering (rhi.h:352-359); the border of
+ * GE_ADDRESS_CLAMP_TO_BORDER is transparent black. 16 bytes. */
+typedef struct ge_sampler_desc {
+  uint32_t address_u;
+  uint32_t address_v;
+  uint32_t mag;
+  uint32_t min;
+} ge_sampler_desc;
+#define GE_FILTER_NEAREST 0u
+#define GE_FILTER_LINEAR 1u
+#define GE_ADDRESS_REPEAT 0u
+#define GE_ADDRESS_MIRRORED_REPEAT 1u
+#define GE_ADDRESS_CLAMP_TO_EDGE 2u
+#define GE_ADDRESS_CLAMP_TO_BORDER 3u
 
 /* gfx::ResolveLight, 64 bytes (material.slang:150-157). */
 typedef struct ge_resolve_light {
@@ -198,7 +253,10 @@ typedef struct ge_path_trace_params {
 #define GE_PT_UNIFORM_SKY 1u
 #define GE_PT_PIXEL_CENTER 2u
 #define GE_PT_DITHER 4u
-#define GE_PT_ALPHA_MASKS 8u /* unsupported: returns GE_RT_ERROR_UNSUPPORTED */
+/* The frame has alpha-tested materials: every ray decides its masked candidates by their masks
+ * (path_trace.slang:110-135, 278-292). Without it the camera ray leaves masked triangles out and
+ * every later ray stops on them. */
+#define GE_PT_ALPHA_MASKS 8u
 #define GE_PT_ROULETTE_DEPTH 3u
 
 /* gfx::RayVisibilityParams, 256 bytes (domain/gfx/include/domain/gfx/ray_visibility.h:39-61,
@@ -219,7 +277,11 @@ typedef struct ge_ray_visibility_params {
   uint64_t triangles;
   uint64_t clusters;
   uint64_t hit_entries;
-  uint64_t masks[4]; /* MaskTables (cluster_cull.h:154-167); must be 0: no alpha-tested material */
+  /* MaskTables (cluster_cull.h:154-167): materials, cluster_materials, attributes, fade. Non-zero
+   * `materials` marks a frame with alpha-tested materials (ray_visibility.h:32-36): masked
+   * candidates are decided by their masks from the scene's own tables; zero leaves them out as the
+   * cull flag does. The cross-fade (`fade`) must be 0. */
+  uint64_t masks[4];
 } ge_ray_visibility_params;
 
 /* ---- the backend's own records --------------------------------------------------------------- */
@@ -271,7 +333,7 @@ typedef struct ge_shadow_resolve_params {
   uint32_t width, height;  /* the visibility buffer's pixels */
   float shadow_bias;       /* ray_offset's floor (FrameLighting::shadow_bias) */
   float shadow_bias_steps; /* ray_offset's grid steps (FrameLighting::shadow_bias_steps) */
-  uint32_t flags;          /* GE_SHADOW_HALF_RATE, GE_SHADOW_JITTER */
+  uint32_t flags;          /* GE_SHADOW_HALF_RATE, GE_SHADOW_JITTER, GE_SHADOW_ALPHA_MASKS */
   uint32_t seed;           /* the jitter's seed */
   uint32_t pad[2];
 } ge_shadow_resolve_params;
@@ -280,6 +342,10 @@ typedef struct ge_shadow_resolve_params {
 /* The pixel's point at a per-pixel hash offset (rng_seed(x, y, 0, seed), two draws) instead of its
  * centre: the stochastic sampling of one ray per pixel. The resolve itself uses the centre. */
 #define GE_SHADOW_JITTER 0x2u
+/* Alpha-tested casters (k_shadow_alpha_masks, visibility_resolve.slang:357, 528-597): each masked
+ * candidate of the sun's ray is decided by its mask at level 0 and the first one kept ends the
+ * search. Without it masked casters are solid. */
+#define GE_SHADOW_ALPHA_MASKS 0x4u
 /* An entry with no ray: no surface, a pair with no instance, a surface facing away from the sun by
  * both normals, or (half rate) a block of more than one pair or at the picture's edge. */
 #define GE_SHADOW_NOT_TAKEN 0xffffffffu
@@ -359,6 +425,13 @@ typedef struct ge_scene_desc {
   const uint32_t* cluster_materials; /* per cluster */
   const ge_resolve_light* lights;
   uint32_t light_count;
+  /* The bindless textures and samplers the materials name (level 0; none: 0 and NULL). */
+  uint32_t texture_count;
+  const ge_texture_desc* textures;
+  const uint8_t* texels; /* the pool the textures' offsets point into */
+  uint64_t texel_bytes;
+  const ge_sampler_desc* samplers;
+  uint32_t sampler_count;
   uint32_t pad;
 } ge_scene_desc;
 
@@ -584,6 +657,8 @@ static_assert(sizeof(ge_instance_desc) == 96, "gfx::InstanceDesc is 96 bytes");
 static_assert(sizeof(ge_frame_eye) == 48, "gfx::FrameEye is 48 bytes");
 static_assert(sizeof(ge_resolve_material) == 128, "gfx::ResolveMaterial is 128 bytes");
 static_assert(sizeof(ge_resolve_light) == 64, "gfx::ResolveLight is 64 bytes");
+static_assert(sizeof(ge_texture_desc) == 24, "a texture record is 24 bytes");
+static_assert(sizeof(ge_sampler_desc) == 16, "a sampler record is 16 bytes");
 static_assert(sizeof(ge_path_trace_params) == 368, "gfx::PathTraceParams is 368 bytes");
 static_assert(sizeof(ge_ray_visibility_params) == 256, "gfx::RayVisibilityParams is 256 bytes");
 static_assert(sizeof(ge_rt_counters) == 104, "counters record");

@@ -61,8 +61,14 @@ def lib():
         L.ge_oracle_bvh_info.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.ge_oracle_bvh_download.argtypes = [vp, C.c_uint32, vp, vp, vp, vp]
         L.ge_oracle_update_instances.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
-        L.ge_oracle_trace_closest.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32]
-        L.ge_oracle_trace_any.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32]
+        L.ge_oracle_trace_closest.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32, C.c_uint32]
+        L.ge_oracle_trace_any.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32, C.c_uint32]
+        L.ge_oracle_sample.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, vp]
+        L.ge_oracle_shade.argtypes = [vp, vp, C.c_uint32, vp, vp]
+        L.ge_oracle_half.argtypes = [C.c_uint32]
+        L.ge_oracle_half.restype = C.c_float
+        L.ge_oracle_srgb.argtypes = [C.c_uint32]
+        L.ge_oracle_srgb.restype = C.c_float
         L.ge_oracle_ray_visibility.argtypes = [vp, C.POINTER(abi.RayVisibilityParams), vp, C.c_int32]
         L.ge_oracle_path_adjoint.restype = C.c_int32
         L.ge_oracle_path_adjoint.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, C.c_uint32, vp, vp, vp, vp,
@@ -84,7 +90,7 @@ def lib():
                                                      C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int32]
         L.ge_oracle_cache_samples.restype = C.c_uint32
         L.ge_oracle_cache_samples.argtypes = [vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, C.c_uint32, C.c_int32]
-        L.ge_oracle_shadow_pass.argtypes = [vp, C.POINTER(abi.ShadowParams), vp, vp, C.c_int32]
+        L.ge_oracle_shadow_pass.argtypes = [vp, C.POINTER(abi.ShadowParams), vp, vp, C.c_int32, C.c_uint32]
         L.ge_oracle_counters.argtypes = [vp]
         L.ge_oracle_max_threads.restype = C.c_int32
         L.ge_oracle_pcg_hash.restype = C.c_uint32
@@ -143,17 +149,31 @@ class Scene:
         lib().ge_oracle_update_instances(self.h, first, len(records), C.byref(records[0]) if len(records) else None,
                                          C.byref(eye) if eye else None)
 
-    def trace_closest(self, rays, mask=0xFF, brute=False, threads=0):
+    def trace_closest(self, rays, mask=0xFF, brute=False, threads=0, flags=0):
         hits = np.zeros(len(rays), dtype=abi.HIT_DTYPE)
         lib().ge_oracle_trace_closest(self.h, rays.ctypes.data, len(rays), mask, hits.ctypes.data,
-                                      1 if brute else 0, threads)
+                                      1 if brute else 0, threads, flags)
         return hits
 
-    def trace_any(self, rays, mask=0xFF, brute=False, threads=0):
+    def trace_any(self, rays, mask=0xFF, brute=False, threads=0, flags=0):
         bits = np.zeros((len(rays) + 31) // 32, dtype=np.uint32)
         lib().ge_oracle_trace_any(self.h, rays.ctypes.data, len(rays), mask, bits.ctypes.data,
-                                  1 if brute else 0, threads)
+                                  1 if brute else 0, threads, flags)
         return bits
+
+    def shade(self, hits, towards):
+        """(n, 16) float32: position, shading normal, albedo, roughness, metallic, emissive, uv per hit."""
+        out = np.zeros((len(hits), 16), dtype=np.float32)
+        t = np.ascontiguousarray(towards, dtype=np.float32)
+        lib().ge_oracle_shade(self.h, hits.ctypes.data, len(hits), t.ctypes.data, out.ctypes.data)
+        return out
+
+    def sample(self, texture, sampler, uv):
+        """(n, 4) float32 level-0 reads of `texture` through `sampler` at uv (n, 2)."""
+        uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), 4), dtype=np.float32)
+        lib().ge_oracle_sample(self.h, texture, sampler, uv.ctypes.data, len(uv), out.ctypes.data)
+        return out
 
     def ray_visibility(self, params, threads=0):
         words = np.zeros(params.width * params.height, dtype=np.uint64)
@@ -243,7 +263,9 @@ class Scene:
         return out[:min(count, capacity)], count
 
     def update_materials(self, first, records):
-        assert lib().ge_oracle_update_materials(self.h, first, len(records), C.addressof(records)) == 0
+        st = lib().ge_oracle_update_materials(self.h, first, len(records), C.addressof(records))
+        if st != 0:
+            raise ValueError(f"oracle material update: status {st}")
 
     def update_lights(self, first, records):
         assert lib().ge_oracle_update_lights(self.h, first, len(records), C.addressof(records)) == 0
@@ -259,9 +281,10 @@ class Scene:
             raise ValueError(f"oracle shadow resolve: status {st}")
         return out, rays
 
-    def shadow_pass(self, params, receivers, threads=0):
+    def shadow_pass(self, params, receivers, threads=0, flags=0):
         bits = np.zeros((params.count + 31) // 32, dtype=np.uint32)
-        lib().ge_oracle_shadow_pass(self.h, C.byref(params), receivers.ctypes.data, bits.ctypes.data, threads)
+        lib().ge_oracle_shadow_pass(self.h, C.byref(params), receivers.ctypes.data, bits.ctypes.data, threads,
+                                    flags)
         return bits
 
 
@@ -281,6 +304,14 @@ def counters():
 
 def reset_counters():
     lib().ge_oracle_reset_counters()
+
+
+def half(h):
+    return lib().ge_oracle_half(int(h))
+
+
+def srgb(code):
+    return lib().ge_oracle_srgb(int(code))
 
 
 def probe(kind, inputs, mode=FP64, n_out=8):

@@ -331,6 +331,11 @@ struct ge_oracle_scene {
   std::vector<ge_resolve_material> materials;
   std::vector<uint32_t> cluster_materials;
   std::vector<ge_resolve_light> lights;
+  std::vector<ge_texture_desc> textures;
+  std::vector<ge_sampler_desc> samplers;
+  std::vector<uint8_t> texels;
+  float srgb[256] = {};             // the sRGB transfer function of each 8-bit code
+  bool any_masked = false;          // a material with k_material_masked
   std::vector<Blas> blas;
   std::vector<InstRec> inst;
   Lbvh tlas;
@@ -349,6 +354,205 @@ inline float relative_axis(int cell, float local, int eye_cell, float eye_local,
 inline void instance_from_eye(const ge_instance_desc& inst, const ge_frame_eye& eye, float out[3]) {
   for (int r = 0; r < 3; ++r)
     out[r] = relative_axis(inst.cell[r], inst.rows[r][3], eye.cell[r], eye.local[r], eye.residual[r]);
+}
+
+// ---- textures at level 0 and the alpha test --------------------------------------------------------
+// Pins: tests/test_oracle_materials.py (the sampler against a float64 restatement of the Vulkan
+// filtering equations and on texel centres exactly; half floats against numpy; the sRGB table
+// against the transfer function; the mask against closed-form cut-outs; opaque-equivalent masks and
+// one-texel textures against untextured scenes to the bit).
+//
+// f16tof32 (decode_half2, material.slang:217): every half is a float exactly.
+inline float half_to_float(uint32_t h) {
+  const uint32_t sign = (h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1fu;
+  const uint32_t m = h & 0x3ffu;
+  uint32_t bits;
+  if (e == 0x1fu) {
+    bits = sign | 0x7f800000u | (m << 13);
+  } else if (e != 0) {
+    bits = sign | ((e + 112u) << 23) | (m << 13);
+  } else if (m == 0) {
+    bits = sign;
+  } else {  // subnormal: m * 2^-24
+    const float f = float(m) * 5.9604644775390625e-8f;
+    std::memcpy(&bits, &f, 4);
+    bits |= sign;
+  }
+  float out;
+  std::memcpy(&out, &bits, 4);
+  return out;
+}
+inline void decode_half2(uint32_t packed, float& u, float& v) {
+  u = half_to_float(packed & 0xffffu);
+  v = half_to_float(packed >> 16);
+}
+
+// The sRGB transfer function of an 8-bit code (the Vulkan sRGB formats' decode), in double, rounded
+// once to float.
+inline float srgb_to_linear(uint32_t code) {
+  const double c = double(code) / 255.0;
+  return float(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+}
+
+struct F4 {
+  float x = 0, y = 0, z = 0, w = 0;
+};
+
+// A texel coordinate in float kept where float still has a fraction (|x| <= 2^24); NaN reads 0.
+inline float texel_space(float x) {
+  if (!(x == x)) return 0.0f;
+  return x > 16777216.0f ? 16777216.0f : (x < -16777216.0f ? -16777216.0f : x);
+}
+
+// The Vulkan wrap of an integer texel coordinate (the spec's "Wrapping Operation"); false for a
+// texel of the border.
+inline bool wrap_texel(int64_t i, uint32_t size, uint32_t mode, int64_t& out) {
+  const int64_t n = int64_t(size);
+  switch (mode) {
+    case GE_ADDRESS_REPEAT:
+      out = ((i % n) + n) % n;
+      return true;
+    case GE_ADDRESS_MIRRORED_REPEAT: {
+      const int64_t t = ((i % (2 * n)) + 2 * n) % (2 * n) - n;
+      out = (n - 1) - (t >= 0 ? t : -(1 + t));
+      return true;
+    }
+    case GE_ADDRESS_CLAMP_TO_EDGE:
+      out = i < 0 ? 0 : (i >= n ? n - 1 : i);
+      return true;
+    default:
+      out = i >= 0 && i < n ? i : 0;
+      return i >= 0 && i < n;
+  }
+}
+
+inline F4 fetch_texel(const ge_oracle_scene& s, const ge_texture_desc& t, int64_t x, int64_t y) {
+  const uint8_t* p = s.texels.data() + t.offset + (uint64_t(y) * t.width + uint64_t(x)) * 4;
+  F4 c;
+  if (t.format == GE_FORMAT_R8G8B8A8_SRGB) {
+    c.x = s.srgb[p[0]];
+    c.y = s.srgb[p[1]];
+    c.z = s.srgb[p[2]];
+  } else {
+    c.x = float(p[0]) / 255.0f;
+    c.y = float(p[1]) / 255.0f;
+    c.z = float(p[2]) / 255.0f;
+  }
+  c.w = float(p[3]) / 255.0f;
+  return c;
+}
+
+// SampleLevel(sampler, uv, 0) (material.slang:645-677): level 0, the magnification filter. Nearest
+// reads texel floor(u W); linear weighs the four texels around (u W - 1/2, v H - 1/2) by the
+// fractions a, b as (1-a)(1-b), a(1-b), (1-a)b, ab, in that order (the Vulkan filtering equation);
+// a texel of the border is transparent black.
+F4 sample_level0(const ge_oracle_scene& s, uint32_t texture, uint32_t sampler, float u, float v) {
+  const ge_texture_desc& t = s.textures[texture];
+  const ge_sampler_desc& sm = s.samplers[sampler];
+  const float fw = float(t.width), fh = float(t.height);
+  if (sm.mag == GE_FILTER_NEAREST) {
+    const float x = texel_space(u * fw), y = texel_space(v * fh);
+    int64_t i, j;
+    const bool in_u = wrap_texel(int64_t(std::floor(x)), t.width, sm.address_u, i);
+    const bool in_v = wrap_texel(int64_t(std::floor(y)), t.height, sm.address_v, j);
+    return in_u && in_v ? fetch_texel(s, t, i, j) : F4{};
+  }
+  const float x = texel_space(u * fw - 0.5f), y = texel_space(v * fh - 0.5f);
+  const float x0 = std::floor(x), y0 = std::floor(y);
+  const float a = x - x0, b = y - y0;
+  int64_t i[2], j[2];
+  bool in_i[2], in_j[2];
+  for (int k = 0; k < 2; ++k) {
+    in_i[k] = wrap_texel(int64_t(x0) + k, t.width, sm.address_u, i[k]);
+    in_j[k] = wrap_texel(int64_t(y0) + k, t.height, sm.address_v, j[k]);
+  }
+  const float w[4] = {(1.0f - a) * (1.0f - b), a * (1.0f - b), (1.0f - a) * b, a * b};
+  F4 out;
+  for (int k = 0; k < 4; ++k) {
+    const int di = k & 1, dj = k >> 1;
+    const F4 c = in_i[di] && in_j[dj] ? fetch_texel(s, t, i[di], j[dj]) : F4{};
+    out.x = out.x + w[k] * c.x;
+    out.y = out.y + w[k] * c.y;
+    out.z = out.z + w[k] * c.z;
+    out.w = out.w + w[k] * c.w;
+  }
+  return out;
+}
+
+// slot_sampler (material.slang:94-97).
+inline uint32_t slot_sampler(const ge_resolve_material& m, uint32_t word, uint32_t shift) {
+  const uint32_t slot = (word >> shift) & 0xffffu;
+  return slot == GE_SAME_SAMPLER ? m.sampler : slot;
+}
+
+// material_uv (material.slang:136-141).
+inline void material_uv(const ge_resolve_material& m, float u, float v, float& su, float& sv) {
+  if (!(m.flags & GE_MATERIAL_UV_TRANSFORM)) {
+    su = u;
+    sv = v;
+    return;
+  }
+  su = (m.uv_transform[0] * u + m.uv_transform[1] * v) + m.uv_offset[0];
+  sv = (m.uv_transform[2] * u + m.uv_transform[3] * v) + m.uv_offset[1];
+}
+
+// mask_keeps_level0 (alpha_mask.slang:66-75): the factor's alpha times the albedo texture's at
+// level 0, against the cutoff.
+inline bool mask_keeps_level0(const ge_oracle_scene& s, const ge_resolve_material& m, float u, float v) {
+  float alpha = m.uv_offset[3];
+  if (m.albedo_texture != GE_NO_TEXTURE) {
+    float su, sv;
+    material_uv(m, u, v, su, sv);
+    alpha *= sample_level0(s, m.albedo_texture, m.sampler, su, sv).w;
+  }
+  return alpha >= m.uv_offset[2];
+}
+
+// How a walk treats a masked candidate: as opaque (RAY_FLAG_FORCE_OPAQUE), left out
+// (RAY_FLAG_CULL_NON_OPAQUE), or by its mask (the candidate loops of pt_candidate_keeps,
+// path_trace.slang:117-135, and shadow_ray_blocked, visibility_resolve.slang:528-597).
+enum AlphaMode { kAlphaOpaque = 0, kAlphaCull = 1, kAlphaTest = 2 };
+
+// Whether a candidate (instance, prim = cluster within the mesh << 8 | triangle, barycentrics u, v
+// of the second and third corners) is kept: hit_uv (alpha_mask.slang:79-88) on the pair's material
+// (pair_material, scene.slang:378-381). Non-masked triangles are opaque and always kept.
+inline bool candidate_kept(const ge_oracle_scene& s, uint32_t inst_id, uint32_t prim, float u, float v, int alpha) {
+  const ge_instance_desc& in = s.instances[inst_id];
+  const uint32_t cluster = s.meshes[in.mesh].first_cluster + (prim >> 8);
+  const ge_resolve_material& m = s.materials[s.cluster_materials[cluster] + in.material_base];
+  if (!(m.flags & GE_MATERIAL_MASKED)) return true;
+  if (alpha == kAlphaCull) return false;
+  float uv[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  if (s.has_attributes) {
+    const ge_cluster_desc& cl = s.clusters[cluster];
+    const uint32_t packed = s.triangles[cl.triangle_offset + (prim & 0xffu)];
+    for (int k = 0; k < 3; ++k)
+      decode_half2(s.attributes[cl.vertex_offset + ((packed >> (8 * k)) & 0xffu)].uv_half2, uv[k][0], uv[k][1]);
+  }
+  const float hu = (uv[0][0] + u * (uv[1][0] - uv[0][0])) + v * (uv[2][0] - uv[0][0]);
+  const float hv = (uv[0][1] + u * (uv[1][1] - uv[0][1])) + v * (uv[2][1] - uv[0][1]);
+  return mask_keeps_level0(s, m, hu, hv);
+}
+
+// A material's texture and sampler slots name records of the scene (the backend's upload checks
+// the same); the stochastic flag is not restated. Sets any_masked when the array is the scene's.
+inline bool slot_valid(const ge_oracle_scene& s, uint32_t texture, uint32_t sampler) {
+  return texture == GE_NO_TEXTURE || (texture < s.textures.size() && sampler < s.samplers.size());
+}
+bool materials_valid(ge_oracle_scene& s, const ge_resolve_material* m, uint32_t count) {
+  for (uint32_t k = 0; k < count; ++k) {
+    const ge_resolve_material& x = m[k];
+    if (x.flags & GE_MATERIAL_STOCHASTIC) return false;
+    if (!slot_valid(s, x.albedo_texture, x.sampler) ||
+        !slot_valid(s, x.metallic_roughness_texture, slot_sampler(x, x.samplers_mr_normal, 0)) ||
+        !slot_valid(s, x.normal_texture, slot_sampler(x, x.samplers_mr_normal, 16)) ||
+        !slot_valid(s, x.emissive_texture, slot_sampler(x, x.samplers_occlusion_emissive, 16)))
+      return false;
+  }
+  s.any_masked = false;
+  for (const ge_resolve_material& x : s.materials) s.any_masked = s.any_masked || (x.flags & GE_MATERIAL_MASKED) != 0;
+  return true;
 }
 
 // The instance's frame-to-mesh 3x4: the inverse of the linear part by its adjugate, in double,
@@ -568,7 +772,7 @@ constexpr uint32_t k_sentinel = 0xfffffffeu;
 // backend's kernels walk in this order, so their counters equal these.
 template <bool AnyHit>
 bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit,
-              Tally& tally) {
+              Tally& tally, int alpha = kAlphaOpaque) {
   hit.found = false;
   hit.t = tmax;
   hit.key = 0;
@@ -608,6 +812,7 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
         float t, u, v;
         if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
         if (!(t >= ray.tmin && t <= hit.t)) continue;
+        if (alpha != kAlphaOpaque && !candidate_kept(s, inst_id, tri.prim, u, v, alpha)) continue;
         const uint32_t key = (rec->first_pair << 8) + tri.prim;
         if (AnyHit) {
           hit.found = true;
@@ -660,7 +865,8 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
 }
 
 template <bool AnyHit>
-bool brute(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit) {
+bool brute(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit,
+           int alpha = kAlphaOpaque) {
   hit.found = false;
   hit.t = tmax;
   hit.key = 0;
@@ -674,6 +880,7 @@ bool brute(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mas
       float t, u, v;
       if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
       if (!(t >= ray.tmin && t <= hit.t)) continue;
+      if (alpha != kAlphaOpaque && !candidate_kept(s, i, tri.prim, u, v, alpha)) continue;
       const uint32_t key = (rec.first_pair << 8) + tri.prim;
       if (t < hit.t || !hit.found || key > hit.key) {
         hit = HitRec{t, u, v, i, tri.prim, key, true};
@@ -956,6 +1163,7 @@ template <class R> inline V3<R> load_quantized_position(const ge_oracle_scene& s
 
 template <class R> struct Surf {
   V3<R> position, normal, geometric;
+  R u = R(0), v = R(0);  // the interpolated UV (zero without attributes)
 };
 
 // fetch_triangle (material.slang:244-266): the corners in frame space, and the vertex indices.
@@ -993,10 +1201,96 @@ Surf<R> surface_at(const ge_oracle_scene& s, const ge_instance_desc& in, const V
     const V3<R> n2 = decode_normal_oct<R>(s.attributes[local[2]].normal_oct);
     const V3<R> n = b.x * n0 + b.y * n1 + b.z * n2;
     out.normal = transform_normal(in, n);
+    float uv[3][2];
+    for (int k = 0; k < 3; ++k) decode_half2(s.attributes[local[k]].uv_half2, uv[k][0], uv[k][1]);
+    out.u = b.x * R(uv[0][0]) + b.y * R(uv[1][0]) + b.z * R(uv[2][0]);
+    out.v = b.x * R(uv[0][1]) + b.y * R(uv[1][1]) + b.z * R(uv[2][1]);
   } else {
     out.normal = out.geometric;
   }
   return out;
+}
+
+// sample_material_plain (material.slang:645-677) at the shading UV, `R` arithmetic on float texels:
+// the albedo texture multiplies the base colour, a metallic-roughness map multiplies roughness by
+// G and metallic by B, an emissive texture multiplies the emissive factor.
+template <class R> struct MatSample {
+  V3<R> albedo, emissive;
+  R roughness, metallic;
+};
+template <class R> inline V3<R> rgb_of(const F4& c) { return v3(R(c.x), R(c.y), R(c.z)); }
+template <class R>
+MatSample<R> sample_material(const ge_oracle_scene& s, const ge_resolve_material& m, float u, float v) {
+  MatSample<R> out;
+  out.albedo = load3<R>(m.albedo);
+  if (m.albedo_texture != GE_NO_TEXTURE) out.albedo = out.albedo * rgb_of<R>(sample_level0(s, m.albedo_texture, m.sampler, u, v));
+  out.roughness = R(m.albedo[3]);
+  out.metallic = R(m.emissive[3]);
+  if (m.metallic_roughness_texture != GE_NO_TEXTURE) {
+    const F4 mr = sample_level0(s, m.metallic_roughness_texture, slot_sampler(m, m.samplers_mr_normal, 0), u, v);
+    out.roughness = out.roughness * R(mr.y);
+    out.metallic = out.metallic * R(mr.z);
+  }
+  out.emissive = load3<R>(m.emissive);
+  if (m.emissive_texture != GE_NO_TEXTURE)
+    out.emissive =
+        out.emissive * rgb_of<R>(sample_level0(s, m.emissive_texture, slot_sampler(m, m.samplers_occlusion_emissive, 16), u, v));
+  return out;
+}
+
+// tangent_frame (material.slang:571-614) without a tangent stream: the frame the positions and UVs
+// give, Gram-Schmidt against the normal, the bitangent negated. False where it degenerates.
+template <class R>
+bool tangent_frame(const ge_oracle_scene& s, const V3<R> p[3], const uint32_t local[3], V3<R> normal, V3<R>& tangent,
+                   V3<R>& bitangent) {
+  tangent = v3(R(1), R(0), R(0));
+  bitangent = v3(R(0), R(0), R(1));
+  if (!s.has_attributes) return false;
+  R uv[3][2];
+  for (int k = 0; k < 3; ++k) {
+    float a, b;
+    decode_half2(s.attributes[local[k]].uv_half2, a, b);
+    uv[k][0] = R(a);
+    uv[k][1] = R(b);
+  }
+  const V3<R> edge1 = p[1] - p[0];
+  const V3<R> edge2 = p[2] - p[0];
+  const R du1 = uv[1][0] - uv[0][0], dv1 = uv[1][1] - uv[0][1];
+  const R du2 = uv[2][0] - uv[0][0], dv2 = uv[2][1] - uv[0][1];
+  const R area = du1 * dv2 - du2 * dv1;
+  if (std::fabs(area) < KC(1e-12)) return false;
+  const R r = R(1) / area;
+  V3<R> t = (edge1 * dv2 - edge2 * dv1) * r;
+  V3<R> b = (edge2 * du1 - edge1 * du2) * r;
+  t = t - normal * dot(normal, t);
+  const R t_length = std::sqrt(dot(t, t));
+  if (t_length < KC(1e-12)) return false;
+  tangent = t / t_length;
+  b = b - (normal * dot(normal, b) + tangent * dot(tangent, b));
+  const R b_length = std::sqrt(dot(b, b));
+  if (b_length < KC(1e-12)) return false;
+  bitangent = b * (R(-1) / b_length);
+  return true;
+}
+
+// apply_normal_map_plain with normal_map_vector (material.slang:746-767).
+template <class R>
+V3<R> apply_normal_map(const ge_oracle_scene& s, const ge_resolve_material& m, float u, float v, V3<R> normal,
+                       V3<R> tangent, V3<R> bitangent) {
+  const F4 texel = sample_level0(s, m.normal_texture, slot_sampler(m, m.samplers_mr_normal, 16), u, v);
+  V3<R> tn;
+  if (!(m.flags & GE_MATERIAL_NORMAL_RG)) {
+    tn = v3(R(texel.x) * R(2) - R(1), R(texel.y) * R(2) - R(1), R(texel.z) * R(2) - R(1));
+  } else {
+    const R x = R(texel.x) * R(2) - R(1), y = R(texel.y) * R(2) - R(1);
+    tn = v3(x, y, std::sqrt(clampr(R(1) - (x * x + y * y), R(0), R(1))));
+  }
+  tn.x = tn.x * R(m.normal_scale);
+  tn.y = tn.y * R(m.normal_scale);
+  const R length_sq = dot(tn, tn);
+  if (length_sq <= KC(1e-12)) return normal;
+  tn = tn * rsqrt_(length_sq);
+  return normalize(tangent * tn.x + bitangent * tn.y + normal * tn.z);
 }
 
 template <class R>
@@ -1012,6 +1306,34 @@ Surf<R> surface_of_hit(const ge_oracle_scene& s, const HitRec& hit, V3<R> toward
   *inst_out = &in;
   *cluster_out = cluster_index;
   return surface_at<R>(s, in, p, local, b, towards);
+}
+
+// The hit decoded as trace_path does it (path_trace.slang:318-351): the surface, the material and
+// its level-0 sample at material_uv(uv * uv_scale), and the normal map on the shading normal.
+template <class R>
+Surf<R> shade_hit(const ge_oracle_scene& s, const HitRec& hit, V3<R> towards, const ge_frame_eye& eye,
+                  const ge_instance_desc** inst_out, uint32_t* cluster_out, const ge_resolve_material** mat_out,
+                  MatSample<R>& ms) {
+  const ge_instance_desc& in = s.instances[hit.inst];
+  const uint32_t cluster_index = s.meshes[in.mesh].first_cluster + (hit.prim >> 8);
+  V3<R> p[3];
+  uint32_t local[3];
+  corners_of<R>(s, in, eye, cluster_index, hit.prim & 0xffu, p, local);
+  const R bu = R(hit.u), bv = R(hit.v);
+  const V3<R> b = v3(R(1) - bu - bv, bu, bv);
+  *inst_out = &in;
+  *cluster_out = cluster_index;
+  Surf<R> sf = surface_at<R>(s, in, p, local, b, towards);
+  const ge_resolve_material& m = s.materials[s.cluster_materials[cluster_index] + in.material_base];
+  *mat_out = &m;
+  float su, sv;
+  material_uv(m, float(sf.u * R(m.uv_scale)), float(sf.v * R(m.uv_scale)), su, sv);
+  ms = sample_material<R>(s, m, su, sv);
+  if (m.normal_texture != GE_NO_TEXTURE) {
+    V3<R> t, bt;
+    if (tangent_frame<R>(s, p, local, sf.normal, t, bt)) sf.normal = apply_normal_map<R>(s, m, su, sv, sf.normal, t, bt);
+  }
+  return sf;
 }
 
 // screen_barycentrics (material.slang:386-416), the barycentrics alone, in float: where the ray
@@ -1051,7 +1373,16 @@ template <class R> inline bool occluded(const PtCtx<R>& c, V3<R> origin, V3<R> d
   const float d[3] = {float(direction.x), float(direction.y), float(direction.z)};
   const TRay ray = make_ray(o, d, 0.0f);
   HitRec hit;
-  return traverse<true>(*c.s, ray, float(t_max), 0x01u, hit, *c.tally);
+  const int alpha = (c.p->flags & GE_PT_ALPHA_MASKS) && c.s->any_masked ? kAlphaTest : kAlphaOpaque;
+  return traverse<true>(*c.s, ray, float(t_max), 0x01u, hit, *c.tally, alpha);
+}
+
+// The closest-hit ray's rule for masked candidates (path_trace.slang:278-292): by the mask in a
+// frame with GE_PT_ALPHA_MASKS, else left out by the camera ray and solid to every later ray.
+template <class R> inline int closest_alpha(const PtCtx<R>& c, uint32_t bounce) {
+  if (!c.s->any_masked) return kAlphaOpaque;
+  if (c.p->flags & GE_PT_ALPHA_MASKS) return kAlphaTest;
+  return bounce == 0 ? kAlphaCull : kAlphaOpaque;
 }
 
 template <class R> inline V3<R> environment(const PtCtx<R>& c, V3<R> direction, bool primary) {
@@ -1158,7 +1489,7 @@ V3<R> trace_path(const PtCtx<R>& c, V3<R> origin, V3<R> direction, uint32_t& rng
     const float d[3] = {float(direction.x), float(direction.y), float(direction.z)};
     const TRay ray = make_ray(o, d, 0.0f);
     HitRec hit;
-    traverse<false>(s, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, *c.tally);
+    traverse<false>(s, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, *c.tally, closest_alpha(c, bounce));
     if (!hit.found) {
       radiance = radiance + throughput * environment(c, direction, bounce == 0);
       break;
@@ -1166,17 +1497,19 @@ V3<R> trace_path(const PtCtx<R>& c, V3<R> origin, V3<R> direction, uint32_t& rng
     if (bounce == 0) hit_primary = true;
     const ge_instance_desc* inst;
     uint32_t cluster_index;
-    const Surf<R> sf = surface_of_hit<R>(s, hit, origin, p.eye, &inst, &cluster_index);
-    const ge_resolve_material& mat = s.materials[s.cluster_materials[cluster_index] + inst->material_base];
+    const ge_resolve_material* matp;
+    MatSample<R> ms;
+    const Surf<R> sf = shade_hit<R>(s, hit, origin, p.eye, &inst, &cluster_index, &matp, ms);
+    const ge_resolve_material& mat = *matp;
     V3<R> shading_normal = sf.normal;
     if (dot(shading_normal, sf.geometric) < R(0)) shading_normal = -shading_normal;
-    radiance = radiance + throughput * load3<R>(mat.emissive);
+    radiance = radiance + throughput * ms.emissive;
     BrdfSurface<R> surface;
-    surface.albedo = load3<R>(mat.albedo);
+    surface.albedo = ms.albedo;
     surface.normal = shading_normal;
     surface.view = -direction;
-    surface.roughness = R(mat.albedo[3]);
-    surface.metallic = R(mat.emissive[3]);
+    surface.roughness = ms.roughness;
+    surface.metallic = ms.metallic;
     for (int k = 0; k < 4; ++k) surface.specular[k] = R(mat.specular[k]);
     if constexpr (std::is_same_v<R, float>) {
       if (tap && tap->mode == 1 && bounce == tap->depth) {
@@ -1591,7 +1924,21 @@ ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* d, const ge_frame_e
     if (uint64_t(d->meshes[m].first_cluster) + d->meshes[m].cluster_count > d->cluster_count) return nullptr;
   for (uint32_t i = 0; i < d->instance_count; ++i)
     if (d->instances[i].mesh >= d->mesh_count) return nullptr;
+  for (uint32_t t = 0; t < d->texture_count; ++t) {
+    const ge_texture_desc& td = d->textures[t];
+    if (!td.width || !td.height || (td.format != GE_FORMAT_R8G8B8A8_UNORM && td.format != GE_FORMAT_R8G8B8A8_SRGB) ||
+        td.offset + uint64_t(td.width) * td.height * 4 > d->texel_bytes)
+      return nullptr;
+  }
+  for (uint32_t k = 0; k < d->sampler_count; ++k) {
+    const ge_sampler_desc& sd = d->samplers[k];
+    if (sd.address_u > 3 || sd.address_v > 3 || sd.mag > 1 || sd.min > 1) return nullptr;
+  }
   auto* s = new ge_oracle_scene();
+  if (d->texture_count) s->textures.assign(d->textures, d->textures + d->texture_count);
+  if (d->sampler_count) s->samplers.assign(d->samplers, d->samplers + d->sampler_count);
+  if (d->texel_bytes) s->texels.assign(d->texels, d->texels + d->texel_bytes);
+  for (uint32_t c = 0; c < 256; ++c) s->srgb[c] = srgb_to_linear(c);
   s->clusters.assign(d->clusters, d->clusters + d->cluster_count);
   s->vertices.assign(d->vertices, d->vertices + size_t(d->vertex_count) * 3);
   const size_t qn = (size_t(d->vertex_count) * 3 + 1) & ~size_t(1);
@@ -1602,6 +1949,10 @@ ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* d, const ge_frame_e
   s->meshes.assign(d->meshes, d->meshes + d->mesh_count);
   s->instances.assign(d->instances, d->instances + d->instance_count);
   s->materials.assign(d->materials, d->materials + d->material_count);
+  if (!materials_valid(*s, s->materials.data(), d->material_count)) {
+    delete s;
+    return nullptr;
+  }
   s->cluster_materials.assign(d->cluster_materials, d->cluster_materials + d->cluster_count);
   if (d->light_count) s->lights.assign(d->lights, d->lights + d->light_count);
   s->blas.resize(d->mesh_count);
@@ -1646,9 +1997,17 @@ int32_t ge_oracle_update_instances(ge_oracle_scene* s, uint32_t first, uint32_t 
   return GE_RT_OK;
 }
 
+// The walk's rule for masked candidates from a query's flags (GE_RT_ALPHA_MASKS, GE_RT_CULL_MASKED).
+int alpha_of_flags(const ge_oracle_scene* s, uint32_t flags) {
+  if (!s->any_masked) return kAlphaOpaque;
+  if (flags & GE_RT_ALPHA_MASKS) return kAlphaTest;
+  return (flags & GE_RT_CULL_MASKED) ? kAlphaCull : kAlphaOpaque;
+}
+
 void ge_oracle_trace_closest(const ge_oracle_scene* s, const ge_ray* rays, uint32_t count, uint32_t mask, ge_hit* hits,
-                             int32_t use_brute, int32_t threads) {
+                             int32_t use_brute, int32_t threads, uint32_t flags) {
   const int nt = resolve_threads(threads);
+  const int alpha = alpha_of_flags(s, flags);
 #pragma omp parallel num_threads(nt)
   {
     Tally tally;
@@ -1658,7 +2017,8 @@ void ge_oracle_trace_closest(const ge_oracle_scene* s, const ge_ray* rays, uint3
       const TRay ray = make_ray(r.origin, r.dir, r.tmin);
       HitRec h;
       ++tally.closest;
-      const bool found = use_brute ? brute<false>(*s, ray, r.tmax, mask, h) : traverse<false>(*s, ray, r.tmax, mask, h, tally);
+      const bool found =
+          use_brute ? brute<false>(*s, ray, r.tmax, mask, h, alpha) : traverse<false>(*s, ray, r.tmax, mask, h, tally, alpha);
       ge_hit out{};
       if (found) {
         out.t = h.t;
@@ -1680,7 +2040,8 @@ void ge_oracle_trace_closest(const ge_oracle_scene* s, const ge_ray* rays, uint3
 }
 
 void ge_oracle_trace_any(const ge_oracle_scene* s, const ge_ray* rays, uint32_t count, uint32_t mask,
-                         uint32_t* occluded_bits, int32_t use_brute, int32_t threads) {
+                         uint32_t* occluded_bits, int32_t use_brute, int32_t threads, uint32_t flags) {
+  const int alpha = alpha_of_flags(s, flags & GE_RT_ALPHA_MASKS);
   const uint32_t words = (count + 31) / 32;
   std::memset(occluded_bits, 0, size_t(words) * 4);
   const int nt = resolve_threads(threads);
@@ -1698,7 +2059,8 @@ void ge_oracle_trace_any(const ge_oracle_scene* s, const ge_ray* rays, uint32_t 
         ++tally.any;
         const TRay ray = make_ray(r.origin, r.dir, r.tmin);
         HitRec h;
-        const bool hit = use_brute ? brute<true>(*s, ray, r.tmax, mask, h) : traverse<true>(*s, ray, r.tmax, mask, h, tally);
+        const bool hit =
+            use_brute ? brute<true>(*s, ray, r.tmax, mask, h, alpha) : traverse<true>(*s, ray, r.tmax, mask, h, tally, alpha);
         if (hit) bits |= 1u << k;
       }
       occluded_bits[w] = bits;
@@ -1711,6 +2073,9 @@ void ge_oracle_ray_visibility(const ge_oracle_scene* s, const ge_ray_visibility_
                               int32_t threads) {
   const int nt = resolve_threads(threads);
   const uint32_t W = p->width, H = p->height;
+  // A frame with mask tables decides masked candidates by their masks; one without culls them
+  // (trace_main, ray_visibility.slang:130-177).
+  const int alpha = !s->any_masked ? kAlphaOpaque : (p->masks[0] != 0 ? kAlphaTest : kAlphaCull);
 #pragma omp parallel num_threads(nt)
   {
     Tally tally;
@@ -1730,7 +2095,7 @@ void ge_oracle_ray_visibility(const ge_oracle_scene* s, const ge_ray_visibility_
         HitRec h;
         ++tally.closest;
         uint64_t value = 0;
-        if (traverse<false>(*s, ray, 1e30f, 0xfeu, h, tally)) {
+        if (traverse<false>(*s, ray, 1e30f, 0xfeu, h, tally, alpha)) {
           float pt[3];
           for (int r = 0; r < 3; ++r) pt[r] = o[r] + d[r] * h.t;
           const float* vp = p->view_proj;
@@ -1843,7 +2208,8 @@ void ge_oracle_tonemap(const ge_path_trace_params* p, const float* accum, const 
 }
 
 void ge_oracle_shadow_pass(const ge_oracle_scene* s, const ge_shadow_params* p, const ge_shadow_receiver* rec,
-                           uint32_t* blocked, int32_t threads) {
+                           uint32_t* blocked, int32_t threads, uint32_t flags) {
+  const int alpha = alpha_of_flags(s, flags & GE_RT_ALPHA_MASKS);
   const uint32_t words = (p->count + 31) / 32;
   std::memset(blocked, 0, size_t(words) * 4);
   const int nt = resolve_threads(threads);
@@ -1866,7 +2232,7 @@ void ge_oracle_shadow_pass(const ge_oracle_scene* s, const ge_shadow_params* p, 
         ++tally.any;
         const TRay ray = make_ray(o, p->sun, 0.0f);
         HitRec h;
-        if (traverse<true>(*s, ray, p->t_max, 0x01u, h, tally)) bits |= 1u << k;
+        if (traverse<true>(*s, ray, p->t_max, 0x01u, h, tally, alpha)) bits |= 1u << k;
       }
       blocked[w] = bits;
     }
@@ -1935,6 +2301,7 @@ int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resol
   std::vector<std::array<uint32_t, 2>> pairs;
   if (!pair_table(*s, pairs)) return GE_RT_ERROR_INVALID_SCENE;
   const bool half = (p->flags & GE_SHADOW_HALF_RATE) != 0;
+  const int alpha = (p->flags & GE_SHADOW_ALPHA_MASKS) && s->any_masked ? kAlphaTest : kAlphaOpaque;
   const uint32_t W = p->width, H = p->height;
   const uint32_t cols = half ? (W + 1) / 2 : W, rows = half ? (H + 1) / 2 : H;
   const int nt = resolve_threads(threads);
@@ -1969,7 +2336,7 @@ int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resol
           const float d[3] = {p->sun[0], p->sun[1], p->sun[2]};
           const TRay r = make_ray(ray, d, 0.0f);
           HitRec h;
-          if (traverse<true>(*s, r, ray[3], 0x01u, h, tally)) value |= 1u;
+          if (traverse<true>(*s, r, ray[3], 0x01u, h, tally, alpha)) value |= 1u;
         } else {
           ray[3] = -1.0f;
         }
@@ -2184,7 +2551,13 @@ uint32_t ge_oracle_cache_samples(const ge_oracle_scene* s, const ge_path_trace_p
 
 int32_t ge_oracle_update_materials(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_resolve_material* m) {
   if (uint64_t(first) + count > s->materials.size()) return GE_RT_ERROR_INVALID_ARGUMENT;
+  const std::vector<ge_resolve_material> before = s->materials;
   std::memcpy(&s->materials[first], m, size_t(count) * sizeof(ge_resolve_material));
+  if (!materials_valid(*s, m, count)) {
+    s->materials = before;
+    materials_valid(*s, nullptr, 0);
+    return GE_RT_ERROR_UNSUPPORTED;
+  }
   return GE_RT_OK;
 }
 
@@ -2193,6 +2566,45 @@ int32_t ge_oracle_update_lights(ge_oracle_scene* s, uint32_t first, uint32_t cou
   std::memcpy(&s->lights[first], l, size_t(count) * sizeof(ge_resolve_light));
   return GE_RT_OK;
 }
+
+void ge_oracle_sample(const ge_oracle_scene* s, uint32_t texture, uint32_t sampler, const float* uv, uint32_t count,
+                      float* out) {
+  for (uint32_t i = 0; i < count; ++i) {
+    const F4 c = sample_level0(*s, texture, sampler, uv[2 * i], uv[2 * i + 1]);
+    out[4 * i] = c.x;
+    out[4 * i + 1] = c.y;
+    out[4 * i + 2] = c.z;
+    out[4 * i + 3] = c.w;
+  }
+}
+float ge_oracle_half(uint32_t h) { return half_to_float(h & 0xffffu); }
+
+void ge_oracle_shade(const ge_oracle_scene* s, const ge_hit* hits, uint32_t count, const float towards[3], float* out) {
+  for (uint32_t i = 0; i < count; ++i) {
+    float* o = out + size_t(i) * 16;
+    std::memset(o, 0, 16 * sizeof(float));
+    const ge_hit& h = hits[i];
+    if (h.instance_id == GE_RT_MISS) continue;
+    HitRec hit;
+    hit.found = true;
+    hit.t = h.t;
+    hit.u = h.u;
+    hit.v = h.v;
+    hit.inst = h.instance_id;
+    hit.prim = (h.geometry << 8) | h.primitive;
+    const ge_instance_desc* inst;
+    uint32_t cluster;
+    const ge_resolve_material* m;
+    MatSample<float> ms;
+    const Surf<float> sf =
+        shade_hit<float>(*s, hit, v3(towards[0], towards[1], towards[2]), s->eye, &inst, &cluster, &m, ms);
+    const float vals[16] = {sf.position.x, sf.position.y, sf.position.z, sf.normal.x, sf.normal.y, sf.normal.z,
+                            ms.albedo.x,   ms.albedo.y,   ms.albedo.z,   ms.roughness, ms.metallic, ms.emissive.x,
+                            ms.emissive.y, ms.emissive.z, sf.u,         sf.v};
+    std::memcpy(o, vals, sizeof(vals));
+  }
+}
+float ge_oracle_srgb(uint32_t code) { return srgb_to_linear(code & 0xffu); }
 
 void ge_oracle_counters(uint64_t out[8]) {
   for (int i = 0; i < 8; ++i) out[i] = g_counters[i].load();

@@ -43,11 +43,12 @@ int32_t ge_oracle_bvh_download(const ge_oracle_scene* scene, uint32_t mesh, ge_b
 int32_t ge_oracle_update_instances(ge_oracle_scene* scene, uint32_t first, uint32_t count,
                                    const ge_instance_desc* instances, const ge_frame_eye* eye);
 
-/* Queries. `brute` = 1 tests every triangle of every instance instead of walking the BVH. */
+/* Queries. `brute` = 1 tests every triangle of every instance instead of walking the BVH. `flags`:
+ * GE_RT_ALPHA_MASKS or GE_RT_CULL_MASKED (closest only), as ge_rt_trace_closest/any take them. */
 void ge_oracle_trace_closest(const ge_oracle_scene* scene, const ge_ray* rays, uint32_t count,
-                             uint32_t mask, ge_hit* hits, int32_t brute, int32_t threads);
+                             uint32_t mask, ge_hit* hits, int32_t brute, int32_t threads, uint32_t flags);
 void ge_oracle_trace_any(const ge_oracle_scene* scene, const ge_ray* rays, uint32_t count,
-                         uint32_t mask, uint32_t* occluded, int32_t brute, int32_t threads);
+                         uint32_t mask, uint32_t* occluded, int32_t brute, int32_t threads, uint32_t flags);
 void ge_oracle_ray_visibility(const ge_oracle_scene* scene, const ge_ray_visibility_params* params,
                               uint64_t* words, int32_t threads);
 /* One batch of path_trace_main over rows [row0, row1) (all rows: 0, height). */
@@ -61,7 +62,8 @@ void ge_oracle_path_trace64(const ge_oracle_scene* scene, const ge_path_trace_pa
 void ge_oracle_tonemap(const ge_path_trace_params* params, const float* accum,
                        const uint32_t* coverage, uint32_t* output);
 void ge_oracle_shadow_pass(const ge_oracle_scene* scene, const ge_shadow_params* params,
-                           const ge_shadow_receiver* receivers, uint32_t* blocked, int32_t threads);
+                           const ge_shadow_receiver* receivers, uint32_t* blocked, int32_t threads,
+                           uint32_t flags);
 /* The sun shadow pass over a visibility buffer (ge_rt_shadow_resolve's contract); `rays`, when not
  * null, receives each entry's ray origin and t_max (t_max -1: no ray), 4 floats an entry. */
 int32_t ge_oracle_shadow_resolve(const ge_oracle_scene* s, const ge_shadow_resolve_params* p, const uint64_t* words,
@@ -121,6 +123,18 @@ void ge_oracle_sincos(float x, float* s, float* c);
  * 11 mis_power_heuristic, 12 light_sample, 13 decode_normal_oct, 14 display_output,
  * 15 transform_normal, 16 brdf_direct, 17 brdf_eval_grad (df/d albedo per channel, df/d roughness). */
 int32_t ge_oracle_probe(uint32_t kind, uint32_t mode, const double* in, double* out);
+/* Level-0 texture reads: out[4 i .. 4 i + 3] = SampleLevel(texture, sampler, uv[2 i], uv[2 i + 1]). */
+void ge_oracle_sample(const ge_oracle_scene* scene, uint32_t texture, uint32_t sampler, const float* uv,
+                      uint32_t count, float* out);
+/* The hit decode of the path tracer for committed hits (instance_id ~0: zeros): 16 floats a hit,
+ * position, the shading normal after the normal map (before it is turned to the ray's side),
+ * albedo, roughness, metallic, emissive, and the interpolated UV; `towards` orients the geometric
+ * normal. */
+void ge_oracle_shade(const ge_oracle_scene* scene, const ge_hit* hits, uint32_t count, const float towards[3],
+                     float* out);
+/* f16tof32 of the low 16 bits, and the sRGB decode table's entry for an 8-bit code. */
+float ge_oracle_half(uint32_t h);
+float ge_oracle_srgb(uint32_t code);
 /* Watertight ray/triangle test: out = {hit, t, u, v}. */
 void ge_oracle_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3],
                          const float v2[3], float tmin, float tmax, double out[4]);
