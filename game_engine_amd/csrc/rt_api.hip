@@ -93,6 +93,14 @@ Bvh bvh_view(const SceneDev& s) {
   b.tris = s.blas_tris.as<ge_bvh_triangle>();
   b.tlas_node_count = s.tlas_node_count;
   b.stack_cap = s.stack_cap;
+  b.masks.inst = s.mask_inst.as<uint2>();
+  b.masks.cluster_materials = s.cluster_materials.as<uint32_t>();
+  b.masks.materials = s.materials.as<ge_resolve_material>();
+  b.masks.clusters = s.clusters.as<ge_cluster_desc>();
+  b.masks.triangles = s.triangles.as<uint32_t>();
+  b.masks.attributes = s.has_attributes ? s.attributes.as<ge_vertex_attributes>() : nullptr;
+  b.masks.tex = TexView{s.textures.as<ge_texture_desc>(), s.samplers.as<ge_sampler_desc>(), s.texels.as<uint8_t>(),
+                        s.srgb.as<float>()};
   return b;
 }
 
@@ -108,19 +116,71 @@ SceneView scene_view(const SceneDev& s) {
   v.cluster_materials = s.cluster_materials.as<uint32_t>();
   v.lights = s.lights.as<ge_resolve_light>();
   v.light_count = s.light_count;
+  v.features = (s.any_masked ? k_scene_masked : 0u) | (s.any_textured ? k_scene_textured : 0u);
+  v.tex = TexView{s.textures.as<ge_texture_desc>(), s.samplers.as<ge_sampler_desc>(), s.texels.as<uint8_t>(),
+                  s.srgb.as<float>()};
   return v;
 }
 
-int32_t validate_materials(ge_rt_ctx* ctx, const ge_resolve_material* m, uint32_t count, uint32_t first) {
+// Whether the walks and the shading need the textured and alpha-tested forms (scene_view's flag).
+bool full_scene(const SceneView& v) { return v.features != 0u; }
+
+// Every texture slot a material reads names a texture and its sampler a sampler of the scene.
+int32_t validate_materials(ge_rt_ctx* ctx, const SceneDev& s, const ge_resolve_material* m, uint32_t count,
+                           uint32_t first) {
+  auto slot_ok = [&](uint32_t texture, uint32_t sampler) {
+    return texture == GE_NO_TEXTURE || (texture < s.texture_count && sampler < s.sampler_count);
+  };
   for (uint32_t k = 0; k < count; ++k) {
     const ge_resolve_material& mat = m[k];
-    if (mat.albedo_texture != GE_NO_TEXTURE || mat.metallic_roughness_texture != GE_NO_TEXTURE ||
-        mat.normal_texture != GE_NO_TEXTURE || mat.emissive_texture != GE_NO_TEXTURE)
-      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(first + k) + " is textured");
-    if ((mat.flags >> 24) != 0u)
-      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, "material " + std::to_string(first + k) + " declares foliage optics");
+    const std::string name = "material " + std::to_string(first + k);
+    auto slot = [&](uint32_t word, uint32_t shift) {
+      const uint32_t v = (word >> shift) & 0xffffu;
+      return v == GE_SAME_SAMPLER ? mat.sampler : v;
+    };
+    if (!slot_ok(mat.albedo_texture, mat.sampler) ||
+        !slot_ok(mat.metallic_roughness_texture, slot(mat.samplers_mr_normal, 0)) ||
+        !slot_ok(mat.normal_texture, slot(mat.samplers_mr_normal, 16)) ||
+        !slot_ok(mat.emissive_texture, slot(mat.samplers_occlusion_emissive, 16)))
+      return set_error(ctx, GE_RT_ERROR_INVALID_SCENE, name + " names a texture or sampler the scene does not hold");
+    if (mat.flags & GE_MATERIAL_STOCHASTIC)
+      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, name + " asks for stochastic (hex-tiled) sampling");
+    if ((mat.flags >> 24) != 0u) return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, name + " declares foliage optics");
   }
   return GE_RT_OK;
+}
+
+// Recomputes the scene's alpha-tested and textured flags and, when it has masked materials, each
+// instance's mask record (its mesh's first cluster, its material base, bit 31 when one of its
+// clusters' materials is masked), uploaded synchronously.
+hipError_t refresh_masks(SceneDev& s, hipStream_t stream) {
+  s.any_masked = false;
+  s.any_textured = false;
+  for (const ge_resolve_material& m : s.h_materials) {
+    s.any_masked = s.any_masked || (m.flags & GE_MATERIAL_MASKED) != 0;
+    s.any_textured = s.any_textured || m.albedo_texture != GE_NO_TEXTURE || m.metallic_roughness_texture != GE_NO_TEXTURE ||
+                     m.normal_texture != GE_NO_TEXTURE || m.emissive_texture != GE_NO_TEXTURE;
+  }
+  if (!s.any_masked || s.instance_count == 0) return hipSuccess;
+  std::vector<uint2> rec(s.instance_count);
+  for (uint32_t i = 0; i < s.instance_count; ++i) {
+    const ge_instance_desc& in = s.h_instances[i];
+    bool masked = false;
+    for (uint32_t cm : s.mesh_materials[in.mesh])
+      masked = masked || (s.h_materials[cm + in.material_base].flags & GE_MATERIAL_MASKED) != 0;
+    rec[i] = make_uint2(s.h_meshes[in.mesh].first_cluster, in.material_base | (masked ? 0x80000000u : 0u));
+  }
+  hipError_t e = s.mask_inst.reserve(rec.size() * sizeof(uint2));
+  if (e == hipSuccess) e = hipMemcpyAsync(s.mask_inst.ptr, rec.data(), rec.size() * sizeof(uint2), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  return e;
+}
+
+// The walks' rule for masked candidates from a query's flags (GE_RT_ALPHA_MASKS, GE_RT_CULL_MASKED).
+uint32_t alpha_rule(const SceneDev& s, uint32_t flags) {
+  if (!s.any_masked) return k_alpha_opaque;
+  if (flags & GE_RT_ALPHA_MASKS) return k_alpha_test;
+  return (flags & GE_RT_CULL_MASKED) ? k_alpha_cull : k_alpha_opaque;
 }
 
 int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_desc* in, uint32_t count) {
@@ -290,8 +350,27 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
     return fail(GE_RT_ERROR_INVALID_SCENE, "the scene's meshes hold more than 2^31 triangles");
   s.tri_pool = uint32_t(tri_pool);
   s.node_pool = uint32_t(node_pool);
+  GE_CHECK(ctx, (d->textures || !d->texture_count) && (d->samplers || !d->sampler_count) && (d->texels || !d->texel_bytes),
+           GE_RT_ERROR_INVALID_ARGUMENT, "a texture array is null with a non-zero count");
+  for (uint32_t t = 0; t < d->texture_count; ++t) {
+    const ge_texture_desc& td = d->textures[t];
+    if (td.width == 0 || td.height == 0 || td.width > 32768 || td.height > 32768 ||
+        (td.format != GE_FORMAT_R8G8B8A8_UNORM && td.format != GE_FORMAT_R8G8B8A8_SRGB) || (td.offset & 3u) ||
+        td.offset + uint64_t(td.width) * td.height * 4 > d->texel_bytes)
+      return fail(GE_RT_ERROR_INVALID_SCENE, "texture " + std::to_string(t) +
+                                                 " is empty, larger than 32768 texels a side, of a format other than "
+                                                 "RGBA8 (UNORM or sRGB), unaligned or past the texel pool");
+  }
+  for (uint32_t k = 0; k < d->sampler_count; ++k) {
+    const ge_sampler_desc& sd = d->samplers[k];
+    if (sd.address_u > GE_ADDRESS_CLAMP_TO_BORDER || sd.address_v > GE_ADDRESS_CLAMP_TO_BORDER ||
+        sd.mag > GE_FILTER_LINEAR || sd.min > GE_FILTER_LINEAR)
+      return fail(GE_RT_ERROR_INVALID_SCENE, "sampler " + std::to_string(k) + " has an unknown address mode or filter");
+  }
+  s.texture_count = d->texture_count;
+  s.sampler_count = d->sampler_count;
   {
-    const int32_t st = validate_materials(ctx, d->materials, d->material_count, 0);
+    const int32_t st = validate_materials(ctx, s, d->materials, d->material_count, 0);
     if (st != GE_RT_OK) {
       const std::string msg = ctx->error;
       return fail(st, msg);
@@ -312,6 +391,14 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
       const std::string msg = ctx->error;
       return fail(st, msg);
     }
+    s.mesh_materials.assign(d->mesh_count, {});
+    for (uint32_t m = 0; m < d->mesh_count; ++m) {
+      std::vector<uint32_t>& list = s.mesh_materials[m];
+      const ge_mesh_desc& mesh = d->meshes[m];
+      for (uint32_t c = mesh.first_cluster; c < mesh.first_cluster + mesh.cluster_count; ++c) list.push_back(d->cluster_materials[c]);
+      std::sort(list.begin(), list.end());
+      list.erase(std::unique(list.begin(), list.end()), list.end());
+    }
   }
   s.h_instances.assign(d->instances, d->instances + d->instance_count);
   const hipStream_t st = ctx->stream;
@@ -329,7 +416,17 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
   if (e == hipSuccess) e = upload(s.materials, d->materials, d->material_count, st);
   if (e == hipSuccess) e = upload(s.cluster_materials, d->cluster_materials, d->cluster_count, st);
   if (e == hipSuccess) e = upload(s.lights, d->lights, d->light_count, st);
+  if (e == hipSuccess) e = upload(s.textures, d->textures, d->texture_count, st);
+  if (e == hipSuccess) e = upload(s.samplers, d->samplers, d->sampler_count, st);
+  if (e == hipSuccess) e = upload(s.texels, d->texels, size_t(d->texel_bytes), st);
+  float srgb[256];
+  for (uint32_t c = 0; c < 256; ++c) {
+    const double v = double(c) / 255.0;
+    srgb[c] = float(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+  }
+  if (e == hipSuccess) e = upload(s.srgb, srgb, 256, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = refresh_masks(s, st);
   if (e != hipSuccess)
     return fail(e == hipErrorOutOfMemory ? GE_RT_ERROR_OUT_OF_MEMORY : GE_RT_ERROR_DEVICE,
                 std::string("scene upload: ") + hipGetErrorString(e));
@@ -347,7 +444,8 @@ void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
   for (DevBuf* b : {&s.clusters, &s.vertices, &s.quantized, &s.attributes, &s.triangles, &s.meshes, &s.instances,
                     &s.materials, &s.cluster_materials, &s.lights, &s.tri_prefix, &s.tri_mesh_of, &s.blas_nodes,
                     &s.blas_tris, &s.blas_morton, &s.mesh_roots, &s.mesh_height, &s.tlas_nodes, &s.tlas_slots,
-                    &s.tlas_morton, &s.inst_recs, &s.inst_boxes, &s.live_ids, &s.tlas_root, &s.tlas_height, &s.pairs})
+                    &s.tlas_morton, &s.inst_recs, &s.inst_boxes, &s.live_ids, &s.tlas_root, &s.tlas_height, &s.pairs,
+                    &s.textures, &s.samplers, &s.texels, &s.srgb, &s.mask_inst})
     b->release();
   delete scene;
 }
@@ -367,6 +465,7 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, instances, size_t(count) * sizeof(ge_instance_desc),
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  GE_HIP(ctx, refresh_masks(s, ctx->stream));
   return GE_RT_OK;
 }
 
@@ -620,9 +719,12 @@ int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_r
     if ((r = stage(ctx, 1, hits, count, false, &d_hits)) != GE_RT_OK) return r;
   }
   GE_HIP(ctx, begin_timing(ctx));
+  GE_CHECK(ctx, (flags & (GE_RT_ALPHA_MASKS | GE_RT_CULL_MASKED)) != (GE_RT_ALPHA_MASKS | GE_RT_CULL_MASKED),
+           GE_RT_ERROR_INVALID_ARGUMENT, "GE_RT_ALPHA_MASKS and GE_RT_CULL_MASKED together");
   GE_HIP(ctx, launch_trace_closest(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_hits,
                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
-                                   ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
+                                   ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks,
+                                   alpha_rule(scene->dev, flags)));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(hits, d_hits, size_t(count) * sizeof(ge_hit), hipMemcpyDeviceToHost, ctx->stream));
@@ -647,9 +749,11 @@ int32_t ge_rt_trace_any(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* 
     if ((r = stage(ctx, 1, occluded, words, false, &d_bits)) != GE_RT_OK) return r;
   }
   GE_HIP(ctx, begin_timing(ctx));
+  GE_CHECK(ctx, !(flags & GE_RT_CULL_MASKED), GE_RT_ERROR_INVALID_ARGUMENT, "an any-hit query does not cull");
   GE_HIP(ctx, launch_trace_any(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_bits,
                                (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
-                               ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
+                               ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks,
+                               alpha_rule(scene->dev, flags)));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(occluded, d_bits, words * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -663,7 +767,7 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
-  GE_CHECK(ctx, params->masks[0] == 0, GE_RT_ERROR_UNSUPPORTED, "alpha-tested materials (MaskTables) are not supported");
+  GE_CHECK(ctx, params->masks[3] == 0, GE_RT_ERROR_UNSUPPORTED, "the cross-fade (MaskTables::fade) is not supported");
   GE_CHECK(ctx, params->width > 0 && params->height > 0, GE_RT_ERROR_INVALID_ARGUMENT, "empty extent");
   const size_t n = size_t(params->width) * params->height;
   const bool dev = flags & GE_RT_DEVICE_PTR;
@@ -672,9 +776,12 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   GE_CHECK(ctx, dev || words, GE_RT_ERROR_INVALID_ARGUMENT, "null output");
   if (!dev && (r = stage(ctx, 0, words, n, false, &out)) != GE_RT_OK) return r;
   GE_HIP(ctx, begin_timing(ctx));
+  // A frame with mask tables decides masked candidates by their masks, one without culls them
+  // (trace_main, ray_visibility.slang:130-177).
+  const uint32_t alpha = alpha_rule(scene->dev, params->masks[0] != 0 ? GE_RT_ALPHA_MASKS : GE_RT_CULL_MASKED);
   GE_HIP(ctx, launch_ray_visibility(ctx->stream, bvh_view(scene->dev), *params, out,
                                     (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
-                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
+                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks, alpha));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(words, out, n * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -695,7 +802,6 @@ int32_t path_setup(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
   GE_CHECK(ctx, params->width > 0 && params->height > 0, GE_RT_ERROR_INVALID_ARGUMENT, "empty extent");
-  GE_CHECK(ctx, !(params->flags & GE_PT_ALPHA_MASKS), GE_RT_ERROR_UNSUPPORTED, "alpha-tested materials are not supported");
   GE_CHECK(ctx, params->sky_params == 0, GE_RT_ERROR_UNSUPPORTED, "the physical sky (sky_params) is not supported");
   GE_CHECK(ctx, params->ground_detail == 0, GE_RT_ERROR_UNSUPPORTED, "ground detail is not supported");
   GE_CHECK(ctx, params->light_count <= scene->dev.light_count, GE_RT_ERROR_INVALID_ARGUMENT,
@@ -774,7 +880,7 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
   unsigned long long* counters = (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr;
   ctx->primal.valid = false;
   if (flags & GE_RT_PT_MEGAKERNEL) {
-    GE_HIP(ctx, launch_path_trace(ctx->stream, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
+    GE_HIP(ctx, launch_path_trace(ctx->stream, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters, full_scene(svl)));
   } else {
     GE_HIP(ctx, wavefront_path_trace(ctx, bvh_view(scene->dev), svl, p, d_accum, d_cov, counters));
     // Every path's radiance is in the wavefront's result buffer until the next wavefront dispatch.
@@ -793,7 +899,7 @@ int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_CHECK(ctx, scene && (materials || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or materials");
   SceneDev& s = scene->dev;
   GE_CHECK(ctx, uint64_t(first) + count <= s.material_count, GE_RT_ERROR_INVALID_ARGUMENT, "material range out of bounds");
-  const int32_t st = validate_materials(ctx, materials, count, first);
+  const int32_t st = validate_materials(ctx, s, materials, count, first);
   if (st != GE_RT_OK) return st;
   if (!count) return GE_RT_OK;
   GE_HIP(ctx, hipSetDevice(ctx->device));
@@ -803,6 +909,7 @@ int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_HIP(ctx, hipMemcpyAsync(s.materials.as<ge_resolve_material>() + first, materials,
                              size_t(count) * sizeof(ge_resolve_material), hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  GE_HIP(ctx, refresh_masks(s, ctx->stream));
   return GE_RT_OK;
 }
 
@@ -843,6 +950,8 @@ int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_pa
   for (const ge_resolve_material& m : s.h_materials)
     GE_CHECK(ctx, m.specular[0] == 0.0f && m.specular[1] == 0.0f && m.specular[2] == 0.0f && m.specular[3] == 0.0f,
              GE_RT_ERROR_UNSUPPORTED, "the adjoint covers materials without their own specular record");
+  GE_CHECK(ctx, !s.any_masked && !s.any_textured, GE_RT_ERROR_UNSUPPORTED,
+           "the adjoint covers untextured, opaque materials");
   const size_t n = size_t(params->width) * params->height;
   const size_t nm = size_t(s.material_count) * 4, nl = size_t(params->light_count) + 1;
   const bool dev = flags & GE_RT_DEVICE_PTR;
@@ -976,8 +1085,10 @@ int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_sha
     if ((r = stage(ctx, 1, blocked, words, false, &db)) != GE_RT_OK) return r;
   }
   GE_HIP(ctx, begin_timing(ctx));
+  GE_CHECK(ctx, !(flags & GE_RT_CULL_MASKED), GE_RT_ERROR_INVALID_ARGUMENT, "a shadow ray does not cull");
   GE_HIP(ctx, launch_shadow_pass(ctx->stream, bvh_view(scene->dev), scene_view(scene->dev), *params, dr, db,
-                                 (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr));
+                                 (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
+                                 alpha_rule(scene->dev, flags)));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(blocked, db, words * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -991,8 +1102,8 @@ int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null parameters");
-  GE_CHECK(ctx, (params->flags & ~(GE_SHADOW_HALF_RATE | GE_SHADOW_JITTER)) == 0, GE_RT_ERROR_UNSUPPORTED,
-           "unknown shadow resolve flags");
+  GE_CHECK(ctx, (params->flags & ~(GE_SHADOW_HALF_RATE | GE_SHADOW_JITTER | GE_SHADOW_ALPHA_MASKS)) == 0,
+           GE_RT_ERROR_UNSUPPORTED, "unknown shadow resolve flags");
   const uint64_t pixels = uint64_t(params->width) * params->height;
   const bool half = params->flags & GE_SHADOW_HALF_RATE;
   const uint64_t entries = half ? uint64_t((params->width + 1) / 2) * ((params->height + 1) / 2) : pixels;
@@ -1038,7 +1149,8 @@ int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
   GE_HIP(ctx, launch_shadow_resolve(ctx, bvh_view(s), scene_view(s), s.eye, *params, dw,
                                     s.pairs.as<uint2>(), s.pair_count, ctx->resolve_rays.as<float4>(), dout,
                                     (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
-                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks));
+                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks,
+                                    alpha_rule(s, (params->flags & GE_SHADOW_ALPHA_MASKS) ? GE_RT_ALPHA_MASKS : 0u)));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(out, dout, size_t(entries) * 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -95,6 +95,163 @@ struct InstRec {
 };
 static_assert(sizeof(InstRec) == 64, "one instance record is one 64-byte fetch");
 
+// ---- textures at level 0 and the alpha test ------------------------------------------------------
+// The scene's bindless textures (RGBA8, level 0), samplers and the sRGB decode of each 8-bit code.
+struct TexView {
+  const ge_texture_desc* textures;
+  const ge_sampler_desc* samplers;
+  const uint8_t* texels;
+  const float* srgb;  // 256 entries
+};
+
+// What a walk needs to decide a masked candidate: per instance its mesh's first cluster and its
+// material base with bit 31 set when one of its clusters' materials is alpha-tested, then the
+// tables pt_candidate_keeps reads (path_trace.slang:117-135).
+struct MaskView {
+  const uint2* inst;
+  const uint32_t* cluster_materials;
+  const ge_resolve_material* materials;
+  const ge_cluster_desc* clusters;
+  const uint32_t* triangles;
+  const ge_vertex_attributes* attributes;  // null: every UV is zero
+  TexView tex;
+};
+
+// How a walk treats a masked candidate: as opaque (RAY_FLAG_FORCE_OPAQUE), left out
+// (RAY_FLAG_CULL_NON_OPAQUE), or by its mask at level 0.
+constexpr uint32_t k_alpha_opaque = 0, k_alpha_cull = 1, k_alpha_test = 2;
+
+// f16tof32 (decode_half2, material.slang:217): exact.
+GE_DEV float half_to_float(uint32_t h) { return float(__builtin_bit_cast(_Float16, uint16_t(h))); }
+
+GE_DEV float texel_space(float x) {
+  if (!(x == x)) return 0.0f;
+  return x > 16777216.0f ? 16777216.0f : (x < -16777216.0f ? -16777216.0f : x);
+}
+// The Vulkan wrap of an integer texel coordinate; false for a texel of the border.
+GE_DEV bool wrap_texel(int i, int n, uint32_t mode, int& out) {
+  if (mode == GE_ADDRESS_REPEAT) {
+    out = ((i % n) + n) % n;
+    return true;
+  }
+  if (mode == GE_ADDRESS_MIRRORED_REPEAT) {
+    const int t = ((i % (2 * n)) + 2 * n) % (2 * n) - n;
+    out = (n - 1) - (t >= 0 ? t : -(1 + t));
+    return true;
+  }
+  if (mode == GE_ADDRESS_CLAMP_TO_EDGE) {
+    out = i < 0 ? 0 : (i >= n ? n - 1 : i);
+    return true;
+  }
+  const bool inside = i >= 0 && i < n;
+  out = inside ? i : 0;
+  return inside;
+}
+GE_DEV float4 fetch_texel(const TexView& tv, const ge_texture_desc& t, int x, int y) {
+  const uint32_t w = *reinterpret_cast<const uint32_t*>(tv.texels + t.offset + (uint64_t(uint32_t(y)) * t.width + uint32_t(x)) * 4);
+  float4 c;
+  if (t.format == GE_FORMAT_R8G8B8A8_SRGB) {
+    c.x = tv.srgb[w & 0xffu];
+    c.y = tv.srgb[(w >> 8) & 0xffu];
+    c.z = tv.srgb[(w >> 16) & 0xffu];
+  } else {
+    c.x = float(w & 0xffu) / 255.0f;
+    c.y = float((w >> 8) & 0xffu) / 255.0f;
+    c.z = float((w >> 16) & 0xffu) / 255.0f;
+  }
+  c.w = float(w >> 24) / 255.0f;
+  return c;
+}
+// SampleLevel(sampler, uv, 0): level 0 through the magnification filter; linear weighs the four
+// texels around (u W - 1/2, v H - 1/2) by (1-a)(1-b), a(1-b), (1-a)b, ab in that order (the Vulkan
+// filtering equation); a texel of the border is transparent black.
+GE_DEV float4 sample_level0(const TexView& tv, uint32_t texture, uint32_t sampler, float u, float v) {
+  const ge_texture_desc t = tv.textures[texture];
+  const ge_sampler_desc sm = tv.samplers[sampler];
+  const float fw = float(t.width), fh = float(t.height);
+  const int W = int(t.width), H = int(t.height);
+  if (sm.mag == GE_FILTER_NEAREST) {
+    const float x = texel_space(u * fw), y = texel_space(v * fh);
+    int i, j;
+    const bool in_u = wrap_texel(int(__builtin_floorf(x)), W, sm.address_u, i);
+    const bool in_v = wrap_texel(int(__builtin_floorf(y)), H, sm.address_v, j);
+    return in_u && in_v ? fetch_texel(tv, t, i, j) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  const float x = texel_space(u * fw - 0.5f), y = texel_space(v * fh - 0.5f);
+  const float x0 = __builtin_floorf(x), y0 = __builtin_floorf(y);
+  const float a = x - x0, b = y - y0;
+  int i[2], j[2];
+  bool in_i[2], in_j[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    in_i[k] = wrap_texel(int(x0) + k, W, sm.address_u, i[k]);
+    in_j[k] = wrap_texel(int(y0) + k, H, sm.address_v, j[k]);
+  }
+  const float w[4] = {(1.0f - a) * (1.0f - b), a * (1.0f - b), (1.0f - a) * b, a * b};
+  float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int di = k & 1, dj = k >> 1;
+    const float4 c = in_i[di] && in_j[dj] ? fetch_texel(tv, t, i[di], j[dj]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    out.x = out.x + w[k] * c.x;
+    out.y = out.y + w[k] * c.y;
+    out.z = out.z + w[k] * c.z;
+    out.w = out.w + w[k] * c.w;
+  }
+  return out;
+}
+// slot_sampler (material.slang:94-97).
+GE_DEV uint32_t slot_sampler(const ge_resolve_material& m, uint32_t word, uint32_t shift) {
+  const uint32_t slot = (word >> shift) & 0xffffu;
+  return slot == GE_SAME_SAMPLER ? m.sampler : slot;
+}
+// material_uv (material.slang:136-141).
+GE_DEV void material_uv(uint32_t flags, const float tr[4], const float off[4], float u, float v, float& su, float& sv) {
+  if (!(flags & GE_MATERIAL_UV_TRANSFORM)) {
+    su = u;
+    sv = v;
+    return;
+  }
+  su = (tr[0] * u + tr[1] * v) + off[0];
+  sv = (tr[2] * u + tr[3] * v) + off[1];
+}
+
+// pt_candidate_keeps for one candidate of a cluster (scene-wide index) with material `material`:
+// true for an opaque material; a masked one is left out under k_alpha_cull and otherwise kept where
+// mask_keeps_level0 (alpha_mask.slang:66-75) keeps its hit_uv (:79-88) at barycentrics u, v.
+GE_DEV bool candidate_kept(const MaskView& mv, uint32_t cluster, uint32_t material, uint32_t triangle, float u, float v,
+                           uint32_t alpha) {
+  const ge_resolve_material* m = mv.materials + material;
+  const uint32_t flags = m->flags;
+  if (!(flags & GE_MATERIAL_MASKED)) return true;
+  if (alpha == k_alpha_cull) return false;
+  float uv[3][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
+  if (mv.attributes) {
+    const ge_cluster_desc* cl = mv.clusters + cluster;
+    const uint32_t packed = mv.triangles[cl->triangle_offset + triangle];
+    const uint32_t base = cl->vertex_offset;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t h = mv.attributes[base + ((packed >> (8 * k)) & 0xffu)].uv_half2;
+      uv[k][0] = half_to_float(h & 0xffffu);
+      uv[k][1] = half_to_float(h >> 16);
+    }
+  }
+  const float hu = (uv[0][0] + u * (uv[1][0] - uv[0][0])) + v * (uv[2][0] - uv[0][0]);
+  const float hv = (uv[0][1] + u * (uv[1][1] - uv[0][1])) + v * (uv[2][1] - uv[0][1]);
+  const float4 offset = *reinterpret_cast<const float4*>(m->uv_offset);
+  float alpha_value = offset.w;
+  const uint32_t texture = m->albedo_texture;
+  if (texture != GE_NO_TEXTURE) {
+    const float4 tr4 = *reinterpret_cast<const float4*>(m->uv_transform);
+    const float tr[4] = {tr4.x, tr4.y, tr4.z, tr4.w}, off[4] = {offset.x, offset.y, offset.z, offset.w};
+    float su, sv;
+    material_uv(flags, tr, off, hu, hv, su, sv);
+    alpha_value *= sample_level0(mv.tex, texture, m->sampler, su, sv).w;
+  }
+  return alpha_value >= offset.z;
+}
+
 struct Bvh {
   const ge_bvh_node* tlas;       // the top level's nodes (node 0 the root)
   const uint32_t* tlas_slots;    // sorted slot -> instance id
@@ -103,6 +260,7 @@ struct Bvh {
   const ge_bvh_triangle* tris;   // every mesh's sorted triangles, pooled
   uint32_t tlas_node_count;      // 0: an empty scene
   uint32_t stack_cap;            // entries of the per-lane LDS stack: both structures' heights + 2
+  MaskView masks;                // read only by the walks that decide masked candidates
 };
 
 // A lane's traversal counts, [0] for closest-hit walks and [1] for any-hit walks.
@@ -269,16 +427,25 @@ GE_DEV void to_object(const InstRec& rec, const Ray& w, Ray& o) {
 //
 // The walk is a state machine, one node visit a step, so a persistent kernel can give a lane whose
 // ray has finished a new ray while the rest of its wave is still walking.
-template <bool AnyHit, bool Count>
+//
+// `Masks` compiles the candidate rule for alpha-tested materials (`alpha`, one of k_alpha_*): a
+// triangle that passes the intersection and distance tests is committed only where candidate_kept
+// keeps it, so the answer is the nearest kept triangle (the same order, the same tie rule) and the
+// walk and its counters are the opaque walk's but for the candidates left out.
+template <bool AnyHit, bool Count, bool Masks = false>
 struct Walker {
   Ray world, ray;
   Hit hit;
   int sp;
   uint32_t cur, node_base, tri_base, first_pair, inst_id;
-  bool in_blas;
+  uint32_t alpha, mask_cluster, mask_material;  // Masks: the rule, the instance's first cluster and base
+  bool in_blas, mask_live;                      // Masks: the instance has masked clusters
 
   // False when there is nothing to walk (mask 0 or an empty scene): the hit is already the answer.
-  GE_DEV bool start(const Bvh& bvh, const Ray& w, float tmax, uint32_t mask) {
+  GE_DEV bool start(const Bvh& bvh, const Ray& w, float tmax, uint32_t mask, uint32_t alpha_rule = k_alpha_opaque) {
+    alpha = alpha_rule;
+    mask_cluster = mask_material = 0;
+    mask_live = false;
     hit.found = false;
     hit.t = tmax;
     hit.key = 0;
@@ -317,6 +484,12 @@ struct Walker {
       node_base = rec.node_base;
       tri_base = rec.tri_base;
       first_pair = rec.first_pair;
+      if (Masks && alpha != k_alpha_opaque) {
+        const uint2 mi = bvh.masks.inst[inst_id];
+        mask_cluster = mi.x;
+        mask_material = mi.y & 0x7fffffffu;
+        mask_live = (mi.y >> 31) != 0u;
+      }
       to_object(rec, world, ray);
       push(stack, spill, cap, k_sentinel);
       in_blas = true;
@@ -345,6 +518,11 @@ struct Walker {
         float t, u, v;
         if (!intersect(ray, v0, v1, v2, t, u, v)) continue;
         if (!(t >= ray.tmin && t <= hit.t)) continue;
+        if (Masks && mask_live) {
+          const uint32_t cluster = mask_cluster + (prim >> 8);
+          const uint32_t material = bvh.masks.cluster_materials[cluster] + mask_material;
+          if (!candidate_kept(bvh.masks, cluster, material, prim & 0xffu, u, v, alpha)) continue;
+        }
         const uint32_t key = (first_pair << 8) + prim;
         if (AnyHit || t < hit.t || !hit.found || key > hit.key) {
           hit.found = true;
@@ -382,12 +560,12 @@ struct Walker {
   }
 };
 
-template <bool AnyHit, bool Count>
+template <bool AnyHit, bool Count, bool Masks = false>
 GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask, Hit& hit, Tally& tally,
-                     uint32_t* __restrict__ stack) {
-  Walker<AnyHit, Count> w;
+                     uint32_t* __restrict__ stack, uint32_t alpha = k_alpha_opaque) {
+  Walker<AnyHit, Count, Masks> w;
   uint32_t spill[k_stack - k_ring];
-  if (w.start(bvh, world, tmax, mask)) {
+  if (w.start(bvh, world, tmax, mask, alpha)) {
     while (!w.step(bvh, stack, spill, tally)) {
     }
   }
@@ -421,10 +599,10 @@ GE_DEV uint64_t lanes_below() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-template <bool AnyHit, bool Count, class Load, class Store>
+template <bool AnyHit, bool Count, bool Masks = false, class Load, class Store>
 GE_DEV void persistent_walk(const Bvh& bvh, uint32_t n, uint32_t* head, uint32_t* __restrict__ stack, Load load,
-                            Store store, uint64_t& issued, Tally& tally) {
-  Walker<AnyHit, Count> w;
+                            Store store, uint64_t& issued, Tally& tally, uint32_t alpha = k_alpha_opaque) {
+  Walker<AnyHit, Count, Masks> w;
   uint32_t spill[k_stack - k_ring];
   bool active = false, exhausted = false;
   uint32_t slot = 0;
@@ -444,7 +622,7 @@ GE_DEV void persistent_walk(const Bvh& bvh, uint32_t n, uint32_t* head, uint32_t
           uint32_t mask;
           if (load(i, slot, ray, tmax, mask)) {
             ++issued;
-            active = w.start(bvh, ray, tmax, mask);
+            active = w.start(bvh, ray, tmax, mask, alpha);
             if (!active) store(slot, w.hit);
           } else {
             Hit miss;

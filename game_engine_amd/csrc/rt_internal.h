@@ -77,6 +77,12 @@ struct SceneDev {
   // Bumped by every change the traced image can see (materials, lights, instances, structures).
   uint64_t version = 0;
   std::vector<ge_resolve_light> h_lights;
+  // The bindless textures (level 0), samplers, the sRGB decode table, and per instance its mesh's
+  // first cluster and material base, bit 31 set when a cluster's material is alpha-tested.
+  DevBuf textures, samplers, texels, srgb, mask_inst;
+  uint32_t texture_count = 0, sampler_count = 0;
+  bool any_masked = false, any_textured = false;
+  std::vector<std::vector<uint32_t>> mesh_materials;  // per mesh: its clusters' distinct material indices
 };
 
 // What the context's last wavefront path-trace dispatch traced, so an adjoint can reuse its
@@ -218,8 +224,12 @@ struct SceneView {
   const uint32_t* cluster_materials;
   const ge_resolve_light* lights;
   uint32_t light_count;
-  uint32_t pad;
+  uint32_t features;  // k_scene_masked | k_scene_textured
+  TexView tex;        // the scene's textures and samplers (none: null)
 };
+// SceneView::features: a material is alpha-tested (the walks' mask records are built), a material
+// reads a texture.
+constexpr uint32_t k_scene_masked = 1u, k_scene_textured = 2u;
 
 // Records the begin (end = 0) or end (end = 1) event of a traversal launch of `kind` (0 closest-hit,
 // 1 any-hit) on the context's stream, for ge_rt_stats; launches past the event pool go untimed.
@@ -236,17 +246,21 @@ hipError_t launch_path_adjoint(hipStream_t stream, const Bvh& bvh, const SceneVi
 hipError_t launch_shadow_resolve(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_frame_eye& eye,
                                  const ge_shadow_resolve_params& p, const uint64_t* words, const uint2* pairs,
                                  uint32_t pair_count, float4* rays, uint32_t* out, unsigned long long* counters,
-                                 uint32_t* head, uint32_t max_blocks);
+                                 uint32_t* head, uint32_t max_blocks, uint32_t alpha);
 
 // Persistent walks: `head` is a device counter the launch clears, `max_blocks` the grid's cap.
+// `alpha` (k_alpha_*) is the walks' rule for masked candidates; `full` compiles textured materials
+// and the masked rule into the path tracer.
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                                ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks);
+                                ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks,
+                                uint32_t alpha);
 hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                            uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks);
+                            uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks,
+                            uint32_t alpha);
 hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
-                                 unsigned long long* counters, uint32_t* head, uint32_t max_blocks);
+                                 unsigned long long* counters, uint32_t* head, uint32_t max_blocks, uint32_t alpha);
 hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
-                             float4* accum, uint32_t* coverage, unsigned long long* counters);
+                             float4* accum, uint32_t* coverage, unsigned long long* counters, bool full);
 hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, const float4* accum,
                           const uint32_t* coverage, uint32_t* out);
 // The wavefront form of launch_path_trace (rt_wavefront.hip); synchronizes once a bounce to size
@@ -257,6 +271,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
                                 float4* accum, uint32_t* coverage, unsigned long long* counters,
                                 const CacheHook* hook = nullptr, const AdjointHook* adjoint = nullptr);
 hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
-                              const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters);
+                              const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters,
+                              uint32_t alpha);
 
 }  // namespace ge

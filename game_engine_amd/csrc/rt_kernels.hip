@@ -17,10 +17,10 @@ namespace {
 extern __shared__ uint32_t g_lds[];
 __device__ __forceinline__ uint32_t* lane_stack() { return g_lds + threadIdx.x; }
 
-template <bool Count>
+template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_closest(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n,
                                                       uint32_t mask, uint32_t* head, ge_hit* __restrict__ hits,
-                                                      unsigned long long* counters) {
+                                                      unsigned long long* counters, uint32_t alpha) {
   Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& m) {
@@ -49,16 +49,16 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_closest(Bvh bvh, cons
     }
     hits[i] = out;
   };
-  persistent_walk<false, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
+  persistent_walk<false, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, issued, 0, tally);
 }
 
 // Occlusion bits by atomic OR into words the host cleared: a persistent walk finishes rays in any
 // order.
-template <bool Count>
+template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_any(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n, uint32_t mask,
                                                   uint32_t* head, uint32_t* __restrict__ bits,
-                                                  unsigned long long* counters) {
+                                                  unsigned long long* counters, uint32_t alpha) {
   Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& m) {
@@ -73,18 +73,18 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_any(Bvh bvh, const ge
   auto store = [&](uint32_t i, const Hit& h) {
     if (h.found) atomicOr(&bits[i >> 5], 1u << (i & 31u));
   };
-  persistent_walk<true, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
+  persistent_walk<true, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, 0, issued, tally);
 }
 
 // trace_main (ray_visibility.slang:106-217): the pixel centre's ray, the committed hit (whose
 // (t, key) order already settles a depth tie to the larger pair << 8 | triangle, the tie scan's
 // answer), and the word asuint(depth) << 32 | key.
-template <bool Count>
+template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) GE_WALK_ATTR k_ray_visibility(Bvh bvh, ge_ray_visibility_params p, uint32_t tiles_x,
                                                                      uint32_t n, uint32_t* head,
                                                                      uint64_t* __restrict__ words,
-                                                                     unsigned long long* counters) {
+                                                                     unsigned long long* counters, uint32_t alpha) {
   // Ray i is lane i % 64 of 8 x 8 tile i / 64 (tiles row-major), so a refill takes a tile's rays.
   auto direction = [&](uint32_t x, uint32_t y) {
     const float ndx = (float(x) + 0.5f) / float(p.width) * 2.0f - 1.0f;
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_ray_visibility(Bvh bvh, ge_
     }
     words[pixel] = value;
   };
-  persistent_walk<false, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
+  persistent_walk<false, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, issued, 0, tally);
 }
 
@@ -139,13 +139,24 @@ struct Pt {
   ge_path_trace_params p;
 };
 
-template <bool Count>
+// `Full` compiles textured materials and the alpha-tested candidate rule (a scene with either);
+// without it every material is its factors and every triangle opaque.
+template <bool Count, bool Full = false>
 __device__ bool occluded(const Pt& c, f3 origin, f3 direction, float t_max, uint64_t& any, Tally& tally) {
   if (t_max <= 0.0f) return false;
   ++any;
   const Ray ray = make_ray(origin, direction, 0.0f);
   Hit h;
-  return traverse<true, Count>(c.bvh, ray, t_max, 0x01u, h, tally, lane_stack());
+  const uint32_t alpha =
+      (c.sv.features & k_scene_masked) && (c.p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : k_alpha_opaque;
+  return traverse<true, Count, Full>(c.bvh, ray, t_max, 0x01u, h, tally, lane_stack(), alpha);
+}
+
+// The closest-hit ray's rule for masked candidates (path_trace.slang:278-292): by the mask in a frame
+// with GE_PT_ALPHA_MASKS, else left out by the camera ray and solid to every later ray.
+__device__ __forceinline__ uint32_t closest_alpha(const ge_path_trace_params& p, uint32_t bounce) {
+  if (p.flags & GE_PT_ALPHA_MASKS) return k_alpha_test;
+  return bounce == 0 ? k_alpha_cull : k_alpha_opaque;
 }
 
 __device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool primary) {
@@ -163,6 +174,8 @@ __device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool prim
 struct HitSurface {
   f3 position, normal, geometric;
   uint32_t cluster;
+  Corners corners;
+  f3 b;
 };
 
 // fetch_triangle + surface_at (material.slang:244-302) at the hit's barycentrics.
@@ -176,17 +189,19 @@ __device__ HitSurface surface_of_hit(const SceneView& sv, const ge_path_trace_pa
   out.normal = v.normal;
   out.geometric = v.geometric;
   out.cluster = cluster_index;
+  out.corners = c;
+  out.b = f3{1.0f - hit.u - hit.v, hit.u, hit.v};
   return out;
 }
 
-template <bool Count>
+template <bool Count, bool Full = false>
 __device__ f3 next_event(const Pt& c, const Surface& surface, f3 position, f3 front, float bias, uint64_t& any,
                          Tally& tally) {
   const ge_path_trace_params& p = c.p;
   f3 direct = splat(0.0f);
   const f3 shadow_origin = position + front * bias;
   const f3 sun = ld3(p.sun);
-  if (p.sun[3] > 0.0f && dot(surface.normal, sun) > 0.0f && !occluded<Count>(c, shadow_origin, sun, 1e30f, any, tally))
+  if (p.sun[3] > 0.0f && dot(surface.normal, sun) > 0.0f && !occluded<Count, Full>(c, shadow_origin, sun, 1e30f, any, tally))
     direct = direct + brdf_direct(surface, sun, splat(p.sun[3]));
   for (uint32_t i = 0; i < c.sv.light_count; ++i) {
     const ge_resolve_light light = c.sv.lights[i];
@@ -194,7 +209,7 @@ __device__ f3 next_event(const Pt& c, const Surface& surface, f3 position, f3 fr
     float light_distance;
     if (!light_sample(light, position, light_dir, radiance, light_distance)) continue;
     if (dot(surface.normal, light_dir) <= 0.0f) continue;
-    if (occluded<Count>(c, shadow_origin, light_dir, light_distance - bias - light.spot[2], any, tally)) continue;
+    if (occluded<Count, Full>(c, shadow_origin, light_dir, light_distance - bias - light.spot[2], any, tally)) continue;
     direct = direct + brdf_direct(surface, light_dir, radiance);
   }
   return direct;
@@ -208,7 +223,7 @@ struct PathVars {
 };
 
 // trace_path's bounce loop (path_trace.slang:257-461) from the path's current bounce to its end.
-template <bool Count>
+template <bool Count, bool Full = false>
 __device__ void continue_path(const Pt& c, PathVars& v, uint64_t& closest, uint64_t& any, Tally& tally) {
   const ge_path_trace_params& p = c.p;
   for (;; ++v.bounce) {
@@ -216,34 +231,49 @@ __device__ void continue_path(const Pt& c, PathVars& v, uint64_t& closest, uint6
     ++closest;
     const Ray ray = make_ray(v.origin, v.direction, 0.0f);
     Hit hit;
-    traverse<false, Count>(c.bvh, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, tally, lane_stack());
+    traverse<false, Count, Full>(c.bvh, ray, 1e30f, bounce == 0 ? 0xfeu : 0x01u, hit, tally, lane_stack(),
+                                 Full && (c.sv.features & k_scene_masked) ? closest_alpha(p, bounce) : k_alpha_opaque);
     if (!hit.found) {
       v.radiance = v.radiance + v.throughput * environment(p, v.direction, bounce == 0);
       break;
     }
     if (bounce == 0) v.hit_primary = true;
     const ge_instance_desc in = c.sv.instances[hit.inst];
-    const HitSurface sf = surface_of_hit(c.sv, p, hit, v.origin, in);
+    HitSurface sf = surface_of_hit(c.sv, p, hit, v.origin, in);
     const ge_resolve_material* mat = c.sv.materials + (c.sv.cluster_materials[sf.cluster] + in.material_base);
+    const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
+    f3 albedo, emissive;
+    float roughness, metallic;
+    if (Full) {
+      const MaterialSample ms = shade_material(c.sv, *mat, sf.corners, sf.b, sf.normal);
+      albedo = ms.albedo;
+      emissive = ms.emissive;
+      roughness = ms.roughness;
+      metallic = ms.metallic;
+    } else {
+      const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
+      const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
+      albedo = f3{alb.x, alb.y, alb.z};
+      emissive = f3{emi.x, emi.y, emi.z};
+      roughness = alb.w;
+      metallic = emi.w;
+    }
     f3 shading_normal = sf.normal;
     if (dot(shading_normal, sf.geometric) < 0.0f) shading_normal = -shading_normal;
-    const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
-    const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
-    const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
-    v.radiance = v.radiance + v.throughput * f3{emi.x, emi.y, emi.z};
+    v.radiance = v.radiance + v.throughput * emissive;
     Surface surface;
-    surface.albedo = f3{alb.x, alb.y, alb.z};
+    surface.albedo = albedo;
     surface.normal = shading_normal;
     surface.view = -v.direction;
-    surface.roughness = alb.w;
-    surface.metallic = emi.w;
+    surface.roughness = roughness;
+    surface.metallic = metallic;
     surface.spec[0] = spc.x;
     surface.spec[1] = spc.y;
     surface.spec[2] = spc.z;
     surface.spec[3] = spc.w;
     const float bias = p.ray_bias + p.ray_bias_steps * c.sv.meshes[in.mesh].quant[3] * in.scale_max;
     const f3 front = dot(sf.geometric, surface.view) >= 0.0f ? sf.geometric : -sf.geometric;
-    v.radiance = v.radiance + v.throughput * next_event<Count>(c, surface, sf.position, front, bias, any, tally);
+    v.radiance = v.radiance + v.throughput * next_event<Count, Full>(c, surface, sf.position, front, bias, any, tally);
     if (bounce >= p.max_bounces) break;
     const float n_dot_v = dot(surface.normal, surface.view);
     if (n_dot_v <= 0.0f) break;
@@ -281,7 +311,7 @@ __device__ void continue_path(const Pt& c, PathVars& v, uint64_t& closest, uint6
   }
 }
 
-template <bool Count>
+template <bool Count, bool Full = false>
 __device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bool& hit_primary, uint64_t& closest,
                          uint64_t& any, Tally& tally) {
   PathVars v;
@@ -292,7 +322,7 @@ __device__ f3 trace_path(const Pt& c, f3 origin, f3 direction, uint32_t& rng, bo
   v.rng = rng;
   v.bounce = 0;
   v.hit_primary = false;
-  continue_path<Count>(c, v, closest, any, tally);
+  continue_path<Count, Full>(c, v, closest, any, tally);
   rng = v.rng;
   hit_primary = v.hit_primary;
   return v.radiance;
@@ -546,7 +576,7 @@ __device__ void adjoint_pixel(const AdjointArgs& a, uint32_t x, uint32_t y, bool
   if (a.image) a.image[pixel] = make_float4(sum.x, sum.y, sum.z, float(p.spp));
 }
 
-template <bool Count>
+template <bool Count, bool Full>
 __global__ void __launch_bounds__(64) k_path_trace(Pt c, float4* __restrict__ accum, uint32_t* __restrict__ coverage,
                                                    unsigned long long* counters) {
   const uint32_t x = blockIdx.x * 8 + (threadIdx.x & 7u);
@@ -576,7 +606,7 @@ __global__ void __launch_bounds__(64) k_path_trace(Pt c, float4* __restrict__ ac
       dir.z = ((m[2] * ndx + m[6] * ndy) + m[10] * 0.0f) + m[14] * 1.0f;
       dir = normalize(dir);
       bool hit_primary = false;
-      sum = sum + trace_path<Count>(c, ld3(p.camera), dir, rng, hit_primary, closest, any, tally);
+      sum = sum + trace_path<Count, Full>(c, ld3(p.camera), dir, rng, hit_primary, closest, any, tally);
       covered = covered || hit_primary;
     }
     float4 prev = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -628,10 +658,10 @@ __global__ void __launch_bounds__(64) k_tonemap(ge_path_trace_params p, const fl
 // The resolve's sun query (shadow_visibility + shadow_ray_blocked, visibility_resolve.slang:516-605,
 // 742-755): origin = position + geometric * ray_offset, one terminate-on-first-hit ray with the caster
 // mask, a ballot of the answers per 32 receivers.
-template <bool Count>
+template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) k_shadow(Bvh bvh, SceneView sv, ge_shadow_params p,
                                                const ge_shadow_receiver* __restrict__ rec, uint32_t* __restrict__ bits,
-                                               unsigned long long* counters) {
+                                               unsigned long long* counters, uint32_t alpha) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   Tally tally;
   uint64_t issued = 0;
@@ -646,7 +676,7 @@ __global__ void __launch_bounds__(64) k_shadow(Bvh bvh, SceneView sv, ge_shadow_
       issued = 1;
       const Ray ray = make_ray(o, f3{p.sun[0], p.sun[1], p.sun[2]}, 0.0f);
       Hit h;
-      blocked = traverse<true, Count>(bvh, ray, p.t_max, 0x01u, h, tally, lane_stack());
+      blocked = traverse<true, Count, Masks>(bvh, ray, p.t_max, 0x01u, h, tally, lane_stack(), alpha);
     }
   }
   const uint64_t ballot = __ballot(blocked);
@@ -663,30 +693,42 @@ inline size_t lds(const Bvh&) { return size_t(k_ring) * k_lanes * 4; }
 }  // namespace
 
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                                ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks) {
+                                ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks,
+                                uint32_t alpha) {
   if (n == 0) return hipSuccess;
   hipError_t e = hipMemsetAsync(head, 0, 4, stream);
   if (e != hipSuccess) return e;
   const uint32_t grid = std::min(blocks(n, 64), max_blocks);
-  if (counters) k_trace_closest<true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters);
-  else k_trace_closest<false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters);
+  if (alpha != k_alpha_opaque) {
+    if (counters) k_trace_closest<true, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, alpha);
+    else k_trace_closest<false, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, alpha);
+  } else {
+    if (counters) k_trace_closest<true, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, 0u);
+    else k_trace_closest<false, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, 0u);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
-                            uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks) {
+                            uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks,
+                            uint32_t alpha) {
   if (n == 0) return hipSuccess;
   hipError_t e = hipMemsetAsync(head, 0, 4, stream);
   if (e == hipSuccess) e = hipMemsetAsync(bits, 0, size_t((n + 31) / 32) * 4, stream);
   if (e != hipSuccess) return e;
   const uint32_t grid = std::min(blocks(n, 64), max_blocks);
-  if (counters) k_trace_any<true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters);
-  else k_trace_any<false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters);
+  if (alpha != k_alpha_opaque) {
+    if (counters) k_trace_any<true, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters, alpha);
+    else k_trace_any<false, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters, alpha);
+  } else {
+    if (counters) k_trace_any<true, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters, 0u);
+    else k_trace_any<false, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, bits, counters, 0u);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ray_visibility_params& p, uint64_t* words,
-                                 unsigned long long* counters, uint32_t* head, uint32_t max_blocks) {
+                                 unsigned long long* counters, uint32_t* head, uint32_t max_blocks, uint32_t alpha) {
   const uint32_t tiles_x = blocks(p.width, 8), tiles_y = blocks(p.height, 8);
   const uint64_t n = uint64_t(tiles_x) * tiles_y * 64;
   if (n == 0) return hipSuccess;
@@ -694,10 +736,14 @@ hipError_t launch_ray_visibility(hipStream_t stream, const Bvh& bvh, const ge_ra
   hipError_t e = hipMemsetAsync(head, 0, 4, stream);
   if (e != hipSuccess) return e;
   const uint32_t grid = std::min(blocks(n, 64), max_blocks);
-  if (counters)
-    k_ray_visibility<true><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, uint32_t(n), head, words, counters);
-  else
-    k_ray_visibility<false><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, uint32_t(n), head, words, counters);
+  const uint32_t m = uint32_t(n);
+  if (alpha != k_alpha_opaque) {
+    if (counters) k_ray_visibility<true, true><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, m, head, words, counters, alpha);
+    else k_ray_visibility<false, true><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, m, head, words, counters, alpha);
+  } else {
+    if (counters) k_ray_visibility<true, false><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, m, head, words, counters, 0u);
+    else k_ray_visibility<false, false><<<grid, 64, lds(bvh), stream>>>(bvh, p, tiles_x, m, head, words, counters, 0u);
+  }
   return hipGetLastError();
 }
 
@@ -719,11 +765,16 @@ hipError_t launch_path_adjoint(hipStream_t stream, const Bvh& bvh, const SceneVi
 }
 
 hipError_t launch_path_trace(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_path_trace_params& p,
-                             float4* accum, uint32_t* coverage, unsigned long long* counters) {
+                             float4* accum, uint32_t* coverage, unsigned long long* counters, bool full) {
   const dim3 grid(blocks(p.width, 8), blocks(p.height, 8));
   Pt c{bvh, sv, p};
-  if (counters) k_path_trace<true><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
-  else k_path_trace<false><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
+  if (full) {
+    if (counters) k_path_trace<true, true><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
+    else k_path_trace<false, true><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
+  } else {
+    if (counters) k_path_trace<true, false><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
+    else k_path_trace<false, false><<<grid, 64, lds(bvh), stream>>>(c, accum, coverage, counters);
+  }
   return hipGetLastError();
 }
 
@@ -735,10 +786,17 @@ hipError_t launch_tonemap(hipStream_t stream, const ge_path_trace_params& p, con
 }
 
 hipError_t launch_shadow_pass(hipStream_t stream, const Bvh& bvh, const SceneView& sv, const ge_shadow_params& p,
-                              const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters) {
+                              const ge_shadow_receiver* rec, uint32_t* bits, unsigned long long* counters,
+                              uint32_t alpha) {
   if (p.count == 0) return hipSuccess;
-  if (counters) k_shadow<true><<<blocks(p.count, 64), 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters);
-  else k_shadow<false><<<blocks(p.count, 64), 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters);
+  const uint32_t g = blocks(p.count, 64);
+  if (alpha != k_alpha_opaque) {
+    if (counters) k_shadow<true, true><<<g, 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters, alpha);
+    else k_shadow<false, true><<<g, 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters, alpha);
+  } else {
+    if (counters) k_shadow<true, false><<<g, 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters, 0u);
+    else k_shadow<false, false><<<g, 64, lds(bvh), stream>>>(bvh, sv, p, rec, bits, counters, 0u);
+  }
   return hipGetLastError();
 }
 

@@ -288,6 +288,102 @@ GE_DEV Vertex surface_at(const SceneV& sv, const ge_instance_desc& in, const Cor
   return out;
 }
 
+// ---- textured materials (material.slang:571-677, 746-767) -------------------------------------------
+// A material after its level-0 textures.
+struct MaterialSample {
+  f3 albedo, emissive;
+  float roughness, metallic;
+};
+
+// The three corners' UVs (decode_half2), zero without attributes.
+template <class SceneV>
+GE_DEV void corner_uvs(const SceneV& sv, const Corners& c, float uv[3][2]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t h = sv.attributes ? sv.attributes[c.local[k]].uv_half2 : 0u;
+    uv[k][0] = half_to_float(h & 0xffffu);
+    uv[k][1] = half_to_float(h >> 16);
+  }
+}
+
+// tangent_frame without a tangent stream: the frame the positions and UVs give, Gram-Schmidt against
+// the normal, the bitangent negated; false where it degenerates.
+GE_DEV bool tangent_frame(const Corners& c, const float uv[3][2], f3 normal, f3& tangent, f3& bitangent) {
+  tangent = f3{1.0f, 0.0f, 0.0f};
+  bitangent = f3{0.0f, 0.0f, 1.0f};
+  const f3 edge1 = c.p[1] - c.p[0];
+  const f3 edge2 = c.p[2] - c.p[0];
+  const float du1 = uv[1][0] - uv[0][0], dv1 = uv[1][1] - uv[0][1];
+  const float du2 = uv[2][0] - uv[0][0], dv2 = uv[2][1] - uv[0][1];
+  const float area = du1 * dv2 - du2 * dv1;
+  if (__builtin_fabsf(area) < 1e-12f) return false;
+  const float r = 1.0f / area;
+  f3 t = (edge1 * dv2 - edge2 * dv1) * r;
+  f3 b = (edge2 * du1 - edge1 * du2) * r;
+  t = t - normal * dot(normal, t);
+  const float t_length = __builtin_sqrtf(dot(t, t));
+  if (t_length < 1e-12f) return false;
+  tangent = t / t_length;
+  b = b - (normal * dot(normal, b) + tangent * dot(tangent, b));
+  const float b_length = __builtin_sqrtf(dot(b, b));
+  if (b_length < 1e-12f) return false;
+  bitangent = b * (-1.0f / b_length);
+  return true;
+}
+
+// sample_material_plain at (u, v) already through material_uv, and the normal map on `normal` where
+// the material has one and the triangle a frame (path_trace.slang:326-349).
+template <class SceneV>
+GE_DEV MaterialSample shade_material(const SceneV& sv, const ge_resolve_material& m, const Corners& c, f3 b,
+                                     f3& normal) {
+  float uv[3][2];
+  corner_uvs(sv, c, uv);
+  const float su0 = (b.x * uv[0][0] + b.y * uv[1][0]) + b.z * uv[2][0];
+  const float sv0 = (b.x * uv[0][1] + b.y * uv[1][1]) + b.z * uv[2][1];
+  float su, sw;
+  material_uv(m.flags, m.uv_transform, m.uv_offset, su0 * m.uv_scale, sv0 * m.uv_scale, su, sw);
+  MaterialSample out;
+  out.albedo = ld3(m.albedo);
+  if (m.albedo_texture != GE_NO_TEXTURE) {
+    const float4 t = sample_level0(sv.tex, m.albedo_texture, m.sampler, su, sw);
+    out.albedo = out.albedo * f3{t.x, t.y, t.z};
+  }
+  out.roughness = m.albedo[3];
+  out.metallic = m.emissive[3];
+  if (m.metallic_roughness_texture != GE_NO_TEXTURE) {
+    const float4 t = sample_level0(sv.tex, m.metallic_roughness_texture, slot_sampler(m, m.samplers_mr_normal, 0), su, sw);
+    out.roughness = out.roughness * t.y;
+    out.metallic = out.metallic * t.z;
+  }
+  out.emissive = ld3(m.emissive);
+  if (m.emissive_texture != GE_NO_TEXTURE) {
+    const float4 t =
+        sample_level0(sv.tex, m.emissive_texture, slot_sampler(m, m.samplers_occlusion_emissive, 16), su, sw);
+    out.emissive = out.emissive * f3{t.x, t.y, t.z};
+  }
+  if (m.normal_texture != GE_NO_TEXTURE && sv.attributes) {
+    f3 t, bt;
+    if (tangent_frame(c, uv, normal, t, bt)) {
+      const float4 texel = sample_level0(sv.tex, m.normal_texture, slot_sampler(m, m.samplers_mr_normal, 16), su, sw);
+      f3 tn;
+      if (!(m.flags & GE_MATERIAL_NORMAL_RG)) {
+        tn = f3{texel.x * 2.0f - 1.0f, texel.y * 2.0f - 1.0f, texel.z * 2.0f - 1.0f};
+      } else {
+        const float x = texel.x * 2.0f - 1.0f, y = texel.y * 2.0f - 1.0f;
+        tn = f3{x, y, __builtin_sqrtf(clampr(1.0f - (x * x + y * y), 0.0f, 1.0f))};
+      }
+      tn.x = tn.x * m.normal_scale;
+      tn.y = tn.y * m.normal_scale;
+      const float length_sq = dot(tn, tn);
+      if (!(length_sq <= 1e-12f)) {
+        tn = tn * rsqrt_cr(length_sq);
+        normal = normalize(t * tn.x + bt * tn.y + normal * tn.z);
+      }
+    }
+  }
+  return out;
+}
+
 // screen_barycentrics (material.slang:386-416), the barycentrics alone: where the ray through
 // `pixel` (in pixels, y down) meets the triangle's plane, the edge functions on the plane's side
 // of zero kept, the rest zeroed, over their sum.

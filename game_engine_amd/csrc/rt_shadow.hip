@@ -89,11 +89,11 @@ __global__ void __launch_bounds__(256) k_receivers(ReceiverArgs a) {
   a.rays[i] = ray;
 }
 
-template <bool Count>
+template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) GE_WALK_ATTR k_sun_walk(Bvh bvh, const float4* __restrict__ rays, float sx,
                                                                float sy, float sz, uint32_t n, uint32_t* head,
                                                                uint32_t* __restrict__ out,
-                                                               unsigned long long* counters) {
+                                                               unsigned long long* counters, uint32_t alpha) {
   Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_sun_walk(Bvh bvh, const flo
   auto store = [&](uint32_t i, const Hit& h) {
     if (h.found) out[i] |= 1u;
   };
-  persistent_walk<true, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
+  persistent_walk<true, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, 0, issued, tally);
 }
 
@@ -119,7 +119,7 @@ inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t
 hipError_t launch_shadow_resolve(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView& sv, const ge_frame_eye& eye,
                                  const ge_shadow_resolve_params& p, const uint64_t* words, const uint2* pairs,
                                  uint32_t pair_count, float4* rays, uint32_t* out, unsigned long long* counters,
-                                 uint32_t* head, uint32_t max_blocks) {
+                                 uint32_t* head, uint32_t max_blocks, uint32_t alpha) {
   const hipStream_t stream = ctx->stream;
   const bool half = (p.flags & GE_SHADOW_HALF_RATE) != 0;
   ReceiverArgs a;
@@ -144,10 +144,14 @@ hipError_t launch_shadow_resolve(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView
   const uint32_t grid = std::min(blocks(n, 64), max_blocks);
   const size_t lds = size_t(k_ring) * k_lanes * 4;
   if ((e = stage_mark(ctx, 1, 0)) != hipSuccess) return e;
-  if (counters)
-    k_sun_walk<true><<<grid, 64, lds, stream>>>(bvh, rays, p.sun[0], p.sun[1], p.sun[2], a.n, head, out, counters);
-  else
-    k_sun_walk<false><<<grid, 64, lds, stream>>>(bvh, rays, p.sun[0], p.sun[1], p.sun[2], a.n, head, out, counters);
+  const float sx = p.sun[0], sy = p.sun[1], sz = p.sun[2];
+  if (alpha != k_alpha_opaque) {
+    if (counters) k_sun_walk<true, true><<<grid, 64, lds, stream>>>(bvh, rays, sx, sy, sz, a.n, head, out, counters, alpha);
+    else k_sun_walk<false, true><<<grid, 64, lds, stream>>>(bvh, rays, sx, sy, sz, a.n, head, out, counters, alpha);
+  } else {
+    if (counters) k_sun_walk<true, false><<<grid, 64, lds, stream>>>(bvh, rays, sx, sy, sz, a.n, head, out, counters, 0u);
+    else k_sun_walk<false, false><<<grid, 64, lds, stream>>>(bvh, rays, sx, sy, sz, a.n, head, out, counters, 0u);
+  }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return stage_mark(ctx, 1, 1);
 }

@@ -105,10 +105,11 @@ __global__ void __launch_bounds__(64) k_generate(ge_path_trace_params p, uint32_
   queue[slot] = pid;
 }
 
-template <bool Count>
+template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) GE_WALK_ATTR k_closest(Bvh bvh, const PathState* __restrict__ state,
                                                 const uint32_t* __restrict__ queue, uint32_t n,
-                                                uint32_t* head, PathHit* __restrict__ hits, unsigned long long* counters) {
+                                                uint32_t* head, PathHit* __restrict__ hits, unsigned long long* counters,
+                                                uint32_t alpha) {
   Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_closest(Bvh bvh, const Path
     out.prim = h.prim;
     hits[pid] = out;
   };
-  persistent_walk<false, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
+  persistent_walk<false, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, issued, 0, tally);
 }
 
@@ -147,9 +148,10 @@ __device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool prim
 
 // fetch_triangle + surface_at (material.slang:244-302) at the hit's barycentrics.
 __device__ Vertex vertex_of(const SceneView& sv, const ge_path_trace_params& p, const PathHit& hit, f3 towards,
-                            const ge_instance_desc& in, uint32_t& cluster_index) {
+                            const ge_instance_desc& in, uint32_t& cluster_index, Corners* corners = nullptr) {
   cluster_index = sv.meshes[in.mesh].first_cluster + (hit.prim >> 8);
   const Corners c = fetch_corners(sv, p.eye, in, cluster_index, hit.prim & 0xffu);
+  if (corners) *corners = c;
   return surface_at(sv, in, c, f3{1.0f - hit.u - hit.v, hit.u, hit.v}, towards);
 }
 
@@ -242,7 +244,9 @@ __device__ __forceinline__ bool sun_asked(const ge_path_trace_params& p, const S
   return p.sun[3] > 0.0f && dot(s.normal, ld3(p.sun)) > 0.0f;
 }
 
-template <int Mode>
+// `Full`: textured materials (sample_material and the normal map at level 0); without it every
+// material is its factors.
+template <int Mode, bool Full = false>
 __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   __shared__ uint32_t parts[Mode == kShadePlain ? 2 : 4][k_shade_threads / 64 + 1];
   __shared__ double ground_acc;  // kShadeAdjoint: the workgroup's sum of the sun's ground-radiance term
@@ -299,23 +303,38 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
       const ge_instance_desc in = a.sv.instances[hit.inst];
       const f3 origin = f3{st.o[0], st.o[1], st.o[2]};
       uint32_t cluster_index;
-      vx = vertex_of(a.sv, p, hit, origin, in, cluster_index);
+      Corners corners;
+      vx = vertex_of(a.sv, p, hit, origin, in, cluster_index, Full ? &corners : nullptr);
       mat_index = a.sv.cluster_materials[cluster_index] + in.material_base;
       const ge_resolve_material* mat = a.sv.materials + mat_index;
+      const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
+      f3 albedo, emissive;
+      float roughness, metallic;
+      if (Full) {
+        const MaterialSample ms = shade_material(a.sv, *mat, corners, f3{1.0f - hit.u - hit.v, hit.u, hit.v}, vx.normal);
+        albedo = ms.albedo;
+        emissive = ms.emissive;
+        roughness = ms.roughness;
+        metallic = ms.metallic;
+      } else {
+        const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
+        const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
+        albedo = f3{alb.x, alb.y, alb.z};
+        emissive = f3{emi.x, emi.y, emi.z};
+        roughness = alb.w;
+        metallic = emi.w;
+      }
       f3 shading_normal = vx.normal;
       if (dot(shading_normal, vx.geometric) < 0.0f) shading_normal = -shading_normal;
-      const float4 alb = *reinterpret_cast<const float4*>(mat->albedo);
-      const float4 emi = *reinterpret_cast<const float4*>(mat->emissive);
-      const float4 spc = *reinterpret_cast<const float4*>(mat->specular);
-      L = L + T * f3{emi.x, emi.y, emi.z};
+      L = L + T * emissive;
       st.L[0] = L.x;
       st.L[1] = L.y;
       st.L[2] = L.z;
-      surface.albedo = f3{alb.x, alb.y, alb.z};
+      surface.albedo = albedo;
       surface.normal = shading_normal;
       surface.view = -dir;
-      surface.roughness = alb.w;
-      surface.metallic = emi.w;
+      surface.roughness = roughness;
+      surface.metallic = metallic;
       surface.spec[0] = spc.x;
       surface.spec[1] = spc.y;
       surface.spec[2] = spc.z;
@@ -544,9 +563,9 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   }
 }
 
-template <bool Count>
+template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) GE_WALK_ATTR k_any(Bvh bvh, const ShadowRec* __restrict__ shadow, uint32_t n, uint32_t* head,
-                                            uint8_t* __restrict__ blocked, unsigned long long* counters) {
+                                            uint8_t* __restrict__ blocked, unsigned long long* counters, uint32_t alpha) {
   Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t j, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
@@ -559,7 +578,7 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_any(Bvh bvh, const ShadowRe
     return true;
   };
   auto store = [&](uint32_t j, const Hit& h) { blocked[j] = h.found ? 1 : 0; };
-  persistent_walk<true, Count>(bvh, n, head, lane_stack(), load, store, issued, tally);
+  persistent_walk<true, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, 0, issued, tally);
 }
 
@@ -747,16 +766,29 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   int cur = 0;
   uint32_t n = npaths;  // every path starts live
   uint32_t* c2 = counts + 2;
-  while (n > 0) {
+  // Every live path is at the same bounce: the loop's iteration. The walks' rule for masked
+  // candidates follows it (path_trace.slang:278-292), and the shading's textured form the scene.
+  const bool full = sv.features != 0u;
+  const bool masked = (sv.features & k_scene_masked) != 0u;
+  const uint32_t any_alpha = masked && (p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : k_alpha_opaque;
+  for (uint32_t bounce = 0; n > 0; ++bounce) {
+    const uint32_t alpha = !masked ? k_alpha_opaque
+                                   : ((p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : (bounce == 0 ? k_alpha_cull : k_alpha_opaque));
     if ((e = hipMemsetAsync(c2, 0, 16, st)) != hipSuccess) return e;  // queue and shadow counts, walk heads
     const uint32_t grid = std::min(blocks(n, 64), ctx->persistent_blocks);
     if ((e = stage_mark(ctx, 0, 0)) != hipSuccess) return e;
-    if (counters)
-      k_closest<true><<<grid, 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
-                                              counts + 4, w.hits.as<PathHit>(), counters);
-    else
-      k_closest<false><<<grid, 64, k_lds, st>>>(bvh, w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n,
-                                               counts + 4, w.hits.as<PathHit>(), counters);
+    {
+      const PathState* ps = w.state[cur].as<PathState>();
+      const uint32_t* q = w.queue[0].as<uint32_t>();
+      PathHit* ph = w.hits.as<PathHit>();
+      if (alpha != k_alpha_opaque) {
+        if (counters) k_closest<true, true><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, alpha);
+        else k_closest<false, true><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, alpha);
+      } else {
+        if (counters) k_closest<true, false><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, 0u);
+        else k_closest<false, false><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, 0u);
+      }
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = stage_mark(ctx, 0, 1)) != hipSuccess) return e;
     ShadeArgs a{};
@@ -802,14 +834,18 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       a.shadow = w.shadow.as<ShadowRec>();
       a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
       a.sgrad = w.sgrad.as<ShadowGrad>();
+      const uint32_t sg = blocks(n, k_shade_threads);
       if (adjoint)
-        k_shade<kShadeAdjoint><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+        k_shade<kShadeAdjoint><<<sg, k_shade_threads, 0, st>>>(a);
       else if (!hook)
-        k_shade<kShadePlain><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+        full ? k_shade<kShadePlain, true><<<sg, k_shade_threads, 0, st>>>(a)
+             : k_shade<kShadePlain><<<sg, k_shade_threads, 0, st>>>(a);
       else if (hook->mode == CacheHook::kCached)
-        k_shade<kShadeCached><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+        full ? k_shade<kShadeCached, true><<<sg, k_shade_threads, 0, st>>>(a)
+             : k_shade<kShadeCached><<<sg, k_shade_threads, 0, st>>>(a);
       else
-        k_shade<kShadeRecord><<<blocks(n, k_shade_threads), k_shade_threads, 0, st>>>(a);
+        full ? k_shade<kShadeRecord, true><<<sg, k_shade_threads, 0, st>>>(a)
+             : k_shade<kShadeRecord><<<sg, k_shade_threads, 0, st>>>(a);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if ((e = hipMemcpyAsync(w.host_counts, c2, 8, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
@@ -825,12 +861,15 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     if (nshadow > 0) {
       const uint32_t grid = std::min(blocks(nshadow, 64), ctx->persistent_blocks);
       if ((e = stage_mark(ctx, 1, 0)) != hipSuccess) return e;
-      if (counters)
-        k_any<true><<<grid, 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow, counts + 5,
-                                            w.blocked.as<uint8_t>(), counters);
-      else
-        k_any<false><<<grid, 64, k_lds, st>>>(bvh, w.shadow.as<ShadowRec>(), nshadow, counts + 5,
-                                             w.blocked.as<uint8_t>(), counters);
+      const ShadowRec* sr = w.shadow.as<ShadowRec>();
+      uint8_t* bl = w.blocked.as<uint8_t>();
+      if (any_alpha != k_alpha_opaque) {
+        if (counters) k_any<true, true><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, any_alpha);
+        else k_any<false, true><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, any_alpha);
+      } else {
+        if (counters) k_any<true, false><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, 0u);
+        else k_any<false, false><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, 0u);
+      }
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if ((e = stage_mark(ctx, 1, 1)) != hipSuccess) return e;
     }

@@ -230,17 +230,18 @@ class Scene:
             self.ctx.check(lib().ge_rt_bvh_download(self.ctx.h, self.h, mesh, _ptr(nodes), _ptr(prims), None, _ptr(morton)))
         return nodes, prims, morton
 
-    def trace_closest(self, rays, mask=0xFF, count=False, hits=None, device=False, n=None):
+    def trace_closest(self, rays, mask=0xFF, count=False, hits=None, device=False, n=None, flags=0):
+        """`flags` may add abi.ALPHA_MASKS or abi.CULL_MASKED (the masked-candidate rule)."""
         n = len(rays) if n is None else n
         hits = np.zeros(n, dtype=abi.HIT_DTYPE) if hits is None else hits
-        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        flags |= (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
         self.ctx.check(lib().ge_rt_trace_closest(self.ctx.h, self.h, _ptr(rays), n, mask, _ptr(hits), flags))
         return hits
 
-    def trace_any(self, rays, mask=0xFF, count=False, bits=None, device=False, n=None):
+    def trace_any(self, rays, mask=0xFF, count=False, bits=None, device=False, n=None, flags=0):
         n = len(rays) if n is None else n
         bits = np.zeros((n + 31) // 32, dtype=np.uint32) if bits is None else bits
-        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        flags |= (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
         self.ctx.check(lib().ge_rt_trace_any(self.ctx.h, self.h, _ptr(rays), n, mask, _ptr(bits), flags))
         return bits
 
@@ -260,9 +261,9 @@ class Scene:
         self.ctx.check(lib().ge_rt_path_trace(self.ctx.h, self.h, C.byref(params), _ptr(accum), _ptr(coverage), flags))
         return accum, coverage
 
-    def shadow_pass(self, params, receivers, count=False, bits=None, device=False):
+    def shadow_pass(self, params, receivers, count=False, bits=None, device=False, flags=0):
         bits = np.zeros((params.count + 31) // 32, dtype=np.uint32) if bits is None else bits
-        flags = (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
+        flags |= (abi.COUNT if count else 0) | (abi.DEVICE_PTR if device else 0)
         self.ctx.check(lib().ge_rt_shadow_pass(self.ctx.h, self.h, C.byref(params), _ptr(receivers), _ptr(bits), flags))
         return bits
 

@@ -248,11 +248,11 @@ def test_shadow_pass(c2):
 def test_errors_are_reported(ctx, c1):
     s, g, o = c1
     pt, _ = s.frame(8, 8)
-    pt.flags = abi.PT_ALPHA_MASKS
+    pt.sky_params = 1  # the physical sky is not restated
     with pytest.raises(rt.RtError) as e:
         g.path_trace(pt)
     assert e.value.status == 5
-    pt.flags = 0
+    pt.sky_params = 0
     pt.light_count = 99
     with pytest.raises(rt.RtError):
         g.path_trace(pt)
