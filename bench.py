@@ -185,6 +185,140 @@ def shadow_main(args):
     print(json.dumps(result), flush=True)
 
 
+MASKED_METRIC = "Mrays/s (any-hit sun shadow rays through alpha-tested casters, level-0 masks)"
+# Algorithmic bytes a mask evaluation reads: the cluster's material index (4), the material's flags,
+# albedo texture, sampler, UV transform and offset (44), the cluster's offsets (8), the packed
+# triangle (4), the three corners' half2 UVs (12), the texture and sampler records (24 + 16) and four
+# RGBA8 texels (16).
+MASK_TEST_BYTES = 128
+
+
+def masked_main(args):
+    """Alpha-tested shadows: the forest of `--trees` trees (8 meshes of an 8-sided trunk and 320 to
+    480 alpha-tested leaf cards, on rolling ground; every texture slot, format, filter and address
+    mode at level 0), 11520 x 2160 (the reference's forest frame), one sun ray a covered pixel. A step
+    is one frame: the visibility words (the camera rays decide masked candidates by their masks,
+    standing in for the rasterizer) and the sun shadow pass with GE_SHADOW_ALPHA_MASKS. value =
+    shadow rays / wall seconds of the whole frames. The same pass with every caster solid is timed
+    beside it on the same words (the reference's 20.6 ms masked against 3.2 ms solid)."""
+    import torch
+
+    from game_engine_amd import abi, rt, synth
+
+    W, H = args.width or 11520, args.height or 2160
+    t = time.perf_counter()
+    s = synth.Synth.config(5, 0, args.trees)
+    d = s.desc()
+    import numpy as np
+    arr = s.arrays()
+    cl_tris = arr["clusters"]["triangle_count"].astype(np.int64)
+    mesh_tris = np.array([cl_tris[m["first_cluster"]:m["first_cluster"] + m["cluster_count"]].sum()
+                          for m in arr["meshes"]])
+    instanced = int(mesh_tris[arr["instances"]["mesh"]].sum())
+    log(f"[bench] forest of {args.trees} trees: {instanced} triangles instanced ({d.triangle_count} unique), "
+        f"{d.instance_count} instances, {d.texture_count} textures in {time.perf_counter() - t:.1f} s")
+    ctx = rt.Context(0)
+    scene = ctx.upload(d)
+    scene.build()
+    log(f"[bench] full build {ctx.stats()['build_ms']:.2f} ms")
+    pt, rv = s.frame(W, H)
+    rv.masks[0] = 1  # a frame with alpha-tested materials
+    words = torch.zeros(W * H, dtype=torch.int64, device="cuda")
+    out = torch.zeros(W * H, dtype=torch.int32, device="cuda")
+    masked_p = synth.Synth.shadow_params(pt, rv, abi.SHADOW_ALPHA_MASKS, 0)
+    solid_p = synth.Synth.shadow_params(pt, rv, 0, 0)
+
+    def frame(count=False):
+        scene.ray_visibility(rv, words=words.data_ptr(), device=True, count=count)
+        vis = ctx.stats()
+        scene.shadow_resolve(masked_p, words.data_ptr(), count=count, out=out.data_ptr(), device=True)
+        return vis, ctx.stats()
+
+    for w in range(args.warmup):
+        v, st = frame()
+        log(f"[bench] warmup {w}: visibility {v['kernel_ms']:.2f} ms, masked shadow {st['kernel_ms']:.2f} ms")
+    rays = int((out != -1).sum().item())  # one any-hit ray an entry that asked one
+    torch.cuda.synchronize()
+    vis_ms, shadow_ms, walk_ms = [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        v, st = frame()
+        vis_ms.append(v["kernel_ms"])
+        shadow_ms.append(st["kernel_ms"])
+        walk_ms.append(st["any_ms"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    solid_ms = []
+    for k in range(args.steps):
+        scene.shadow_resolve(solid_p, words.data_ptr(), out=out.data_ptr(), device=True)
+        solid_ms.append(ctx.stats()["kernel_ms"])
+    blocked_solid = int(((out != -1) & ((out & 1) == 1)).sum().item())
+    scene.shadow_resolve(masked_p, words.data_ptr(), out=out.data_ptr(), device=True)
+    blocked = int(((out != -1) & ((out & 1) == 1)).sum().item())
+    log(f"[bench] timed {args.steps} frames: {wall * 1e3:.1f} ms wall; visibility {['%.2f' % x for x in vis_ms]}, "
+        f"masked shadow {['%.2f' % x for x in shadow_ms]} (walk {['%.2f' % x for x in walk_ms]}), solid shadow "
+        f"{['%.2f' % x for x in solid_ms]}; {rays} rays, {blocked} blocked masked, {blocked_solid} solid")
+    ctx.reset_stats()
+    frame(count=True)
+    st = ctx.stats()
+    per_ray = (48 * st["any"] + 64 * st["any_nodes"] + 48 * st["any_triangles"] + 72 * st["any_instances"] +
+               MASK_TEST_BYTES * st["any_mask_tests"]) / st["any"]
+    launch_ms = sum(walk_ms) / len(walk_ms)
+    achieved = per_ray * rays / (launch_ms * 1e-3) / 1e9
+    log(f"[bench] counted frame: {st}")
+    mean = lambda x: sum(x) / len(x)
+    cpu = None
+    if not args.no_cpu:
+        cpu = masked_cpu_baseline(s, min(args.cpu_seconds, 30.0))
+    result = {
+        "metric": MASKED_METRIC, "value": round(rays / (wall / args.steps) / 1e6, 2), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"alpha-tested forest shadows: {args.trees} trees ({instanced} triangles "
+                               f"instanced, {d.instance_count} instances, leaf cards masked by a level-0 RGBA8 sRGB "
+                               f"atlas), {W}x{H}, masked visibility words then one sun ray a covered pixel",
+                   "triangles": instanced, "width": W, "height": H, "parallelism": "single GPU"},
+        "shadow_rays_per_step": rays,
+        "masked_shadow_pass_mrays_per_s": round(rays / (mean(shadow_ms) * 1e-3) / 1e6, 1),
+        "solid_shadow_pass_mrays_per_s": round(rays / (mean(solid_ms) * 1e-3) / 1e6, 1),
+        "masked_over_solid_time": round(mean(shadow_ms) / mean(solid_ms), 3),
+        "blocked_fraction": {"masked": round(blocked / max(rays, 1), 4), "solid": round(blocked_solid / max(rays, 1), 4)},
+        "stage_ms": {"visibility": round(mean(vis_ms), 3), "masked_shadow": round(mean(shadow_ms), 3),
+                     "solid_shadow": round(mean(solid_ms), 3)},
+        "mask_tests_per_ray": round(st["any_mask_tests"] / st["any"], 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": "k_sun_walk (masks)",
+                     "launch_ms": round(launch_ms, 4), "bytes_per_ray": round(per_ray, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def masked_cpu_baseline(s, target_seconds):
+    """The oracle's masked visibility and shadow pass on frames of 1152 x 216 (a hundredth of the
+    pixels of the timed frame, same view and sun), all host threads, until `target_seconds` pass."""
+    from game_engine_amd import abi, synth
+    from oracle import binding as ob
+    sc = ob.Scene(s.desc())
+    threads = ob.lib().ge_oracle_max_threads()
+    pt, rv = s.frame(1152, 216)
+    rv.masks[0] = 1
+    words = sc.ray_visibility(rv)
+    p = synth.Synth.shadow_params(pt, rv, abi.SHADOW_ALPHA_MASKS, 0)
+    rays, frames = 0, 0
+    t = time.perf_counter()
+    while True:
+        out, _ = sc.shadow_resolve(p, words)
+        rays += int((out != abi.SHADOW_NOT_TAKEN).sum())
+        frames += 1
+        dt = time.perf_counter() - t
+        if dt >= target_seconds or frames >= 256:
+            break
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} masked sun shadow passes over the oracle's 1152x216 masked visibility words of the "
+                      f"same view: {rays} rays in {dt:.2f} s on {threads} threads (oracle/ scalar C++ port)"}
+
+
 CACHE_METRIC = "1080p time-to-equal-RMSE, cached vs plain tracer"
 PEAK_BF16_TFLOPS = 2500.0
 CACHE_FLOP_PER_QUERY = 2 * (172 * 64 + 4 * 64 * 64 + 64 * 3)  # the network's algorithmic products
@@ -494,11 +628,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--megakernel", action="store_true", help="one path per lane instead of the wavefront")
-    ap.add_argument("--workload", choices=["path", "shadow", "cache", "inverse"], default="path",
+    ap.add_argument("--workload", choices=["path", "shadow", "cache", "inverse", "masked"], default="path",
                     help="path: the path tracer on the 4M-triangle scene (the headline); shadow: the sun shadow "
                          "pass on the 50M-triangle scene with per-frame rebuilds; cache: time to equal RMSE, "
                          "radiance-cache tracer against the plain tracer, on the 4M-triangle scene; inverse: "
-                         "inverse-rendering iterations through the path-replay adjoint")
+                         "inverse-rendering iterations through the path-replay adjoint; masked: the sun shadow "
+                         "pass through alpha-tested casters on the textured forest")
+    ap.add_argument("--trees", type=int, default=4096, help="masked workload: trees in the forest")
+    ap.add_argument("--width", type=int, default=0, help="masked workload: frame width (0: 11520)")
+    ap.add_argument("--height", type=int, default=0, help="masked workload: frame height (0: 2160)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="cache workload: the reference's samples")
     ap.add_argument("--budget", type=int, default=64, help="cache workload: the plain tracer's samples")
     ap.add_argument("--cache-warmup", type=int, default=32, help="cache workload: warm-up training frames")
@@ -531,6 +669,9 @@ def main():
         return
     if args.workload == "inverse":
         inverse_main(args)
+        return
+    if args.workload == "masked":
+        masked_main(args)
         return
 
     import numpy as np

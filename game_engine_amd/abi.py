@@ -90,7 +90,7 @@ class Counters(C.Structure):
     _fields_ = [("closest_rays", u64), ("any_rays", u64), ("nodes", u64), ("triangles", u64),
                 ("instances", u64), ("last_kernel_ms", f64), ("last_build_ms", f64), ("any_nodes", u64),
                 ("any_triangles", u64), ("any_instances", u64), ("closest_ms", f64), ("any_ms", f64),
-                ("closest_launches", u32), ("any_launches", u32)]
+                ("closest_launches", u32), ("any_launches", u32), ("mask_tests", u64), ("any_mask_tests", u64)]
 
 
 class SceneDesc(C.Structure):
@@ -142,7 +142,7 @@ RECEIVER_DTYPE = np.dtype([("position", "<f4", 3), ("instance_id", "<u4"), ("geo
 
 SIZES = {ClusterDesc: 48, VertexAttributes: 8, MeshDesc: 64, InstanceDesc: 96, FrameEye: 48,
          ResolveMaterial: 128, ResolveLight: 64, PathTraceParams: 368, RayVisibilityParams: 256,
-         ShadowParams: 48, Counters: 104, ShadowResolveParams: 128, CacheTrainParams: 16,
+         ShadowParams: 48, Counters: 120, ShadowResolveParams: 128, CacheTrainParams: 16,
          TextureDesc: 24, SamplerDesc: 16, SceneDesc: 152}
 for _t, _n in SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)

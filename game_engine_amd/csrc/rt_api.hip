@@ -1440,8 +1440,10 @@ int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {
     }
     ctx->timing_pending = false;
   }
-  unsigned long long c[8];
+  unsigned long long c[11];  // [8] is the walks' head
   GE_HIP(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+  out->mask_tests = c[9];
+  out->any_mask_tests = c[10];
   out->closest_rays = c[0];
   out->any_rays = c[1];
   out->nodes = c[2] + c[5];
@@ -1462,7 +1464,7 @@ int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {
 int32_t ge_rt_reset_stats(ge_rt_ctx* ctx) {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_HIP(ctx, hipSetDevice(ctx->device));
-  GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 64, ctx->stream));
+  GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 88, ctx->stream));  // the walks' head is cleared with them
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
 }

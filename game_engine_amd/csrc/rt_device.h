@@ -219,12 +219,14 @@ GE_DEV void material_uv(uint32_t flags, const float tr[4], const float off[4], f
 // pt_candidate_keeps for one candidate of a cluster (scene-wide index) with material `material`:
 // true for an opaque material; a masked one is left out under k_alpha_cull and otherwise kept where
 // mask_keeps_level0 (alpha_mask.slang:66-75) keeps its hit_uv (:79-88) at barycentrics u, v.
+// `evaluated` is set when the mask was read (a masked candidate under k_alpha_test).
 GE_DEV bool candidate_kept(const MaskView& mv, uint32_t cluster, uint32_t material, uint32_t triangle, float u, float v,
-                           uint32_t alpha) {
+                           uint32_t alpha, bool& evaluated) {
   const ge_resolve_material* m = mv.materials + material;
   const uint32_t flags = m->flags;
   if (!(flags & GE_MATERIAL_MASKED)) return true;
   if (alpha == k_alpha_cull) return false;
+  evaluated = true;
   float uv[3][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
   if (mv.attributes) {
     const ge_cluster_desc* cl = mv.clusters + cluster;
@@ -265,23 +267,25 @@ struct Bvh {
 
 // A lane's traversal counts, [0] for closest-hit walks and [1] for any-hit walks.
 struct Tally {
-  uint32_t nodes[2] = {0, 0}, tris[2] = {0, 0}, inst[2] = {0, 0};
+  uint32_t nodes[2] = {0, 0}, tris[2] = {0, 0}, inst[2] = {0, 0}, masks[2] = {0, 0};
 };
 
 // Adds a wave's counts to the context's counters (closest rays, any rays, then nodes, triangles and
-// instances of the closest-hit walks and of the any-hit walks): a wave reduction, then one lane adds.
+// instances of the closest-hit walks and of the any-hit walks, then past the walks' head at [8] the
+// mask evaluations of each): a wave reduction, then one lane adds.
 GE_DEV void flush_counters(unsigned long long* counters, uint64_t closest, uint64_t any, const Tally& t) {
-  uint64_t v[8] = {closest, any, t.nodes[0], t.tris[0], t.inst[0], t.nodes[1], t.tris[1], t.inst[1]};
+  uint64_t v[10] = {closest, any, t.nodes[0], t.tris[0], t.inst[0], t.nodes[1], t.tris[1], t.inst[1],
+                    t.masks[0], t.masks[1]};
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 10; ++k) {
     uint64_t x = v[k];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     v[k] = x;
   }
   if ((threadIdx.x & 63u) == 0) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (v[k]) atomicAdd(&counters[k], (unsigned long long)v[k]);
+    for (int k = 0; k < 10; ++k)
+      if (v[k]) atomicAdd(&counters[k < 8 ? k : k + 1], (unsigned long long)v[k]);
   }
 }
 
@@ -521,7 +525,10 @@ struct Walker {
         if (Masks && mask_live) {
           const uint32_t cluster = mask_cluster + (prim >> 8);
           const uint32_t material = bvh.masks.cluster_materials[cluster] + mask_material;
-          if (!candidate_kept(bvh.masks, cluster, material, prim & 0xffu, u, v, alpha)) continue;
+          bool evaluated = false;
+          const bool kept = candidate_kept(bvh.masks, cluster, material, prim & 0xffu, u, v, alpha, evaluated);
+          if (Count && evaluated) ++tally.masks[AnyHit];
+          if (!kept) continue;
         }
         const uint32_t key = (first_pair << 8) + prim;
         if (AnyHit || t < hit.t || !hit.found || key > hit.key) {

@@ -140,7 +140,8 @@ class Context:
                 "instances": c.instances, "any_nodes": c.any_nodes, "any_triangles": c.any_triangles,
                 "any_instances": c.any_instances, "kernel_ms": c.last_kernel_ms, "build_ms": c.last_build_ms,
                 "closest_ms": c.closest_ms, "any_ms": c.any_ms, "closest_launches": c.closest_launches,
-                "any_launches": c.any_launches}
+                "any_launches": c.any_launches, "mask_tests": c.mask_tests + c.any_mask_tests,
+                "any_mask_tests": c.any_mask_tests}
 
     def reset_stats(self):
         self.check(lib().ge_rt_reset_stats(self.h))

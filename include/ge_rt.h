@@ -379,6 +379,9 @@ typedef struct ge_rt_counters {
   double any_ms;
   uint32_t closest_launches;
   uint32_t any_launches;
+  /* Masked candidates whose mask was evaluated (GE_RT_ALPHA_MASKS walks), closest-hit then any-hit. */
+  uint64_t mask_tests;
+  uint64_t any_mask_tests;
 } ge_rt_counters;
 #define GE_RT_STAGE_TIMED 32
 
@@ -661,7 +664,7 @@ static_assert(sizeof(ge_texture_desc) == 24, "a texture record is 24 bytes");
 static_assert(sizeof(ge_sampler_desc) == 16, "a sampler record is 16 bytes");
 static_assert(sizeof(ge_path_trace_params) == 368, "gfx::PathTraceParams is 368 bytes");
 static_assert(sizeof(ge_ray_visibility_params) == 256, "gfx::RayVisibilityParams is 256 bytes");
-static_assert(sizeof(ge_rt_counters) == 104, "counters record");
+static_assert(sizeof(ge_rt_counters) == 120, "counters record");
 static_assert(sizeof(ge_ray) == 32, "ge_ray is 32 bytes");
 static_assert(sizeof(ge_hit) == 32, "ge_hit is 32 bytes");
 static_assert(sizeof(ge_bvh_node) == 64, "a BVH node is one 64-byte fetch");

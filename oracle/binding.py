@@ -295,11 +295,12 @@ def tonemap(params, accum, coverage):
 
 
 def counters():
-    out = np.zeros(8, dtype=np.uint64)
+    out = np.zeros(10, dtype=np.uint64)
     lib().ge_oracle_counters(out.ctypes.data)
     c = [int(v) for v in out]
     return {"closest": c[0], "any": c[1], "nodes": c[2] + c[5], "triangles": c[3] + c[6],
-            "instances": c[4] + c[7], "any_nodes": c[5], "any_triangles": c[6], "any_instances": c[7]}
+            "instances": c[4] + c[7], "any_nodes": c[5], "any_triangles": c[6], "any_instances": c[7],
+            "mask_tests": c[8] + c[9], "any_mask_tests": c[9]}
 
 
 def reset_counters():

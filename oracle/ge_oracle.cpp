@@ -29,10 +29,10 @@ namespace {
 
 // ---- counters --------------------------------------------------------------------------------
 // closest rays, any rays, then nodes, triangles and instances of the closest-hit walks and of the
-// any-hit walks
-std::atomic<uint64_t> g_counters[8];
+// any-hit walks, then the mask evaluations of each
+std::atomic<uint64_t> g_counters[10];
 struct Tally {
-  uint64_t closest = 0, any = 0, nodes[2] = {0, 0}, tris[2] = {0, 0}, inst[2] = {0, 0};
+  uint64_t closest = 0, any = 0, nodes[2] = {0, 0}, tris[2] = {0, 0}, inst[2] = {0, 0}, masks[2] = {0, 0};
   void flush() {
     g_counters[0] += closest;
     g_counters[1] += any;
@@ -40,6 +40,7 @@ struct Tally {
       g_counters[2 + 3 * k] += nodes[k];
       g_counters[3 + 3 * k] += tris[k];
       g_counters[4 + 3 * k] += inst[k];
+      g_counters[8 + k] += masks[k];
     }
     *this = Tally();
   }
@@ -517,12 +518,15 @@ enum AlphaMode { kAlphaOpaque = 0, kAlphaCull = 1, kAlphaTest = 2 };
 // Whether a candidate (instance, prim = cluster within the mesh << 8 | triangle, barycentrics u, v
 // of the second and third corners) is kept: hit_uv (alpha_mask.slang:79-88) on the pair's material
 // (pair_material, scene.slang:378-381). Non-masked triangles are opaque and always kept.
-inline bool candidate_kept(const ge_oracle_scene& s, uint32_t inst_id, uint32_t prim, float u, float v, int alpha) {
+// `evaluated` counts the masks read (a masked candidate under kAlphaTest).
+inline bool candidate_kept(const ge_oracle_scene& s, uint32_t inst_id, uint32_t prim, float u, float v, int alpha,
+                           uint64_t* evaluated = nullptr) {
   const ge_instance_desc& in = s.instances[inst_id];
   const uint32_t cluster = s.meshes[in.mesh].first_cluster + (prim >> 8);
   const ge_resolve_material& m = s.materials[s.cluster_materials[cluster] + in.material_base];
   if (!(m.flags & GE_MATERIAL_MASKED)) return true;
   if (alpha == kAlphaCull) return false;
+  if (evaluated) ++*evaluated;
   float uv[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   if (s.has_attributes) {
     const ge_cluster_desc& cl = s.clusters[cluster];
@@ -812,7 +816,7 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
         float t, u, v;
         if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
         if (!(t >= ray.tmin && t <= hit.t)) continue;
-        if (alpha != kAlphaOpaque && !candidate_kept(s, inst_id, tri.prim, u, v, alpha)) continue;
+        if (alpha != kAlphaOpaque && !candidate_kept(s, inst_id, tri.prim, u, v, alpha, &tally.masks[AnyHit])) continue;
         const uint32_t key = (rec->first_pair << 8) + tri.prim;
         if (AnyHit) {
           hit.found = true;
@@ -2606,11 +2610,11 @@ void ge_oracle_shade(const ge_oracle_scene* s, const ge_hit* hits, uint32_t coun
 }
 float ge_oracle_srgb(uint32_t code) { return srgb_to_linear(code & 0xffu); }
 
-void ge_oracle_counters(uint64_t out[8]) {
-  for (int i = 0; i < 8; ++i) out[i] = g_counters[i].load();
+void ge_oracle_counters(uint64_t out[10]) {
+  for (int i = 0; i < 10; ++i) out[i] = g_counters[i].load();
 }
 void ge_oracle_reset_counters(void) {
-  for (int i = 0; i < 8; ++i) g_counters[i] = 0;
+  for (int i = 0; i < 10; ++i) g_counters[i] = 0;
 }
 int32_t ge_oracle_max_threads(void) { return resolve_threads(0); }
 

@@ -105,8 +105,9 @@ int32_t ge_oracle_update_materials(ge_oracle_scene* s, uint32_t first, uint32_t 
 int32_t ge_oracle_update_lights(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_resolve_light* l);
 
 /* Traversal counters accumulated since the last reset: closest rays, any rays, then nodes,
- * triangles and instances of the closest-hit walks, then those of the any-hit walks. */
-void ge_oracle_counters(uint64_t out[8]);
+ * triangles and instances of the closest-hit walks, then those of the any-hit walks, then the
+ * masks evaluated by the closest-hit and the any-hit walks. */
+void ge_oracle_counters(uint64_t out[10]);
 void ge_oracle_reset_counters(void);
 int32_t ge_oracle_max_threads(void);
 

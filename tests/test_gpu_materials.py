@@ -72,8 +72,10 @@ def test_masked_queries_match_the_oracle(ctx, forest, flags):
     gb = g.trace_any(rays, count=True, flags=any_flags)
     assert np.array_equal(gb, o.trace_any(rays, flags=any_flags))
     gs, os_ = ctx.stats(), ob.counters()
-    for k in ("closest", "any", "nodes", "triangles", "instances"):
+    for k in ("closest", "any", "nodes", "triangles", "instances", "mask_tests", "any_mask_tests"):
         assert gs[k] == os_[k], k
+    if flags & abi.ALPHA_MASKS:
+        assert gs["mask_tests"] > 0 and gs["any_mask_tests"] > 0
     print(f"hits {np.mean(got['instance_id'] != abi.MISS):.3f}, occluded {unpack_bits(gb, len(rays)).mean():.3f}")
 
 
@@ -100,7 +102,7 @@ def test_masked_shadow_passes(ctx, forest):
         want, _ = o.shadow_resolve(p, words)
         assert np.array_equal(got, want), flags
         gs, os_ = ctx.stats(), ob.counters()
-        for k in ("any", "nodes", "triangles", "instances"):
+        for k in ("any", "nodes", "triangles", "instances", "mask_tests"):
             assert gs[k] == os_[k], (flags, k)
     # The receiver-list pass.
     rays = s.primary_rays(96, 64, 2)
@@ -132,7 +134,7 @@ def test_textured_path_trace_matches_the_oracle(ctx, forest, megakernel, alpha):
     assert np.array_equal(cov, wcov)
     assert np.array_equal(acc, want), np.abs(acc - want).max()
     gs, os_ = ctx.stats(), ob.counters()
-    for k in ("closest", "any", "nodes", "triangles", "instances"):
+    for k in ("closest", "any", "nodes", "triangles", "instances", "mask_tests", "any_mask_tests"):
         assert gs[k] == os_[k], k
     assert acc[:, :3].mean() > 0
     # The sky a camera ray sees goes through pow, each side's own library's: within an ulp there.
