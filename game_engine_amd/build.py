@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 ARCH = os.environ.get("GE_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["rt_api.hip", "rt_bvh.hip", "rt_kernels.hip", "rt_wavefront.hip", "rt_shadow.hip", "rt_mlp.hip"]
+HIP_SOURCES = ["rt_api.hip", "rt_bvh.hip", "rt_kernels.hip", "rt_wavefront.hip", "rt_shadow.hip", "rt_mlp.hip",
+               "rt_clusters.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 

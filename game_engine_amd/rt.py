@@ -80,6 +80,11 @@ def lib():
                                          C.POINTER(C.c_double), u32], i32),
             "ge_rt_stats": ([vp, C.POINTER(abi.Counters)], i32),
             "ge_rt_reset_stats": ([vp], i32),
+            "ge_clusters_read": ([vp, C.c_uint64, u32, C.POINTER(vp), C.c_char_p, C.c_size_t], i32),
+            "ge_clusters_scene": ([vp, C.POINTER(abi.SceneDesc)], None),
+            "ge_clusters_counts": ([vp, C.POINTER(u32), C.POINTER(u32)], None),
+            "ge_clusters_destroy": ([vp], None),
+            "ge_clusters_hash": ([vp, C.c_uint64], C.c_uint64),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -96,7 +101,49 @@ EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchroni
            "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint",
            "ge_rt_cache_create", "ge_rt_cache_destroy", "ge_rt_cache_weights", "ge_rt_cache_infer",
            "ge_rt_cache_train", "ge_rt_cache_collect", "ge_rt_cache_update", "ge_rt_path_trace_cached",
-           "ge_rt_cache_samples")
+           "ge_rt_cache_samples", "ge_clusters_read", "ge_clusters_scene", "ge_clusters_counts",
+           "ge_clusters_destroy", "ge_clusters_hash")
+
+
+class ClustersFile:
+    """A `.clusters` container read from memory (ge_clusters_read): its finest level as a scene
+    descriptor for Context.upload. Raises ValueError with the reader's reason on a bad container."""
+
+    def __init__(self, data, images=False):
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(bytes(data))
+        err = C.create_string_buffer(512)
+        h = C.c_void_p()
+        st = lib().ge_clusters_read(buf, len(data), 1 if images else 0, C.byref(h), err, 512)
+        if st != 0:
+            raise ValueError(f"status {st}: {err.value.decode()}")
+        self.h = h
+        self._desc = abi.SceneDesc()
+        lib().ge_clusters_scene(self.h, C.byref(self._desc))
+
+    def desc(self):
+        return self._desc
+
+    def counts(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        lib().ge_clusters_counts(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            lib().ge_clusters_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def clusters_hash(data):
+    """hash_bytes over `data` with the engine's seed: the container's content hash."""
+    buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(bytes(data) or b"\0")
+    return lib().ge_clusters_hash(buf, len(data))
 
 
 def _ptr(a):

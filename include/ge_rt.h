@@ -650,6 +650,33 @@ int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_
 int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out);
 int32_t ge_rt_reset_stats(ge_rt_ctx* ctx);
 
+/* ---- The `.clusters` container ---------------------------------------------------------------
+ * The engine's built-mesh container (domain/geometry/include/domain/geometry/cluster_file.h), read
+ * from memory as geometry::read_cluster_file_memory reads it (domain/geometry/src/cluster_file.cpp:
+ * 419-515, 691-800, 1184-1215): the header and section table, every section inside the file, the
+ * content hash (core/hash hash_bytes over every byte after the header), the required sections at
+ * their element sizes (clusters, lod, vertices, attributes, triangles, vertex_source,
+ * level_cluster_counts, scalars), every cluster's runs inside the streams and every triangle's
+ * corners inside its cluster, the quantized grid (the file's, or quantize_positions' from the
+ * vertices), the optional materials and material map; unknown kinds are skipped. The scene it
+ * gives: the finest level (lod level 0) as one mesh, one identity instance of it, and the materials
+ * as GpuScene::fill_material fills their factors (systems/renderer/src/gpu_scene_textures.cpp:
+ * 477-540), the alpha-mask mode as set_alpha_mask packs it. Images are not decoded: with
+ * GE_CLUSTERS_IMAGES each texture slot names its image's index in the container's image list (the
+ * caller uploads them as the scene's textures, in that order, and sampler 0), otherwise none.
+ * Host code; no device is touched. On failure `error` holds the reason. */
+#define GE_CLUSTERS_IMAGES 0x1u
+typedef struct ge_clusters ge_clusters;
+int32_t ge_clusters_read(const uint8_t* bytes, uint64_t size, uint32_t flags, ge_clusters** out, char* error,
+                         size_t error_size);
+/* The scene's arrays (pointers into the reader's copy, valid until ge_clusters_destroy). */
+void ge_clusters_scene(const ge_clusters* clusters, ge_scene_desc* out);
+/* The container's clusters over every level, and its levels. */
+void ge_clusters_counts(const ge_clusters* clusters, uint32_t* all_clusters, uint32_t* levels);
+void ge_clusters_destroy(ge_clusters* clusters);
+/* The content hash the header holds (hash_bytes with the engine's seed). */
+uint64_t ge_clusters_hash(const uint8_t* bytes, uint64_t size);
+
 #ifdef __cplusplus
 } /* extern "C" */
 
@@ -662,6 +689,7 @@ static_assert(sizeof(ge_resolve_material) == 128, "gfx::ResolveMaterial is 128 b
 static_assert(sizeof(ge_resolve_light) == 64, "gfx::ResolveLight is 64 bytes");
 static_assert(sizeof(ge_texture_desc) == 24, "a texture record is 24 bytes");
 static_assert(sizeof(ge_sampler_desc) == 16, "a sampler record is 16 bytes");
+static_assert(sizeof(ge_scene_desc) == 152, "the scene descriptor is 152 bytes");
 static_assert(sizeof(ge_path_trace_params) == 368, "gfx::PathTraceParams is 368 bytes");
 static_assert(sizeof(ge_ray_visibility_params) == 256, "gfx::RayVisibilityParams is 256 bytes");
 static_assert(sizeof(ge_rt_counters) == 120, "counters record");

@@ -11,7 +11,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int32_t|void)\s+(ge_rt_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int32_t|void|uint64_t)\s+(ge_(?:rt|clusters)_\w+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_entry_point():
@@ -32,7 +32,9 @@ def test_record_sizes_match_the_engine():
         mirror = {"ge_cluster_desc": abi.ClusterDesc, "ge_mesh_desc": abi.MeshDesc, "ge_instance_desc": abi.InstanceDesc,
                   "ge_frame_eye": abi.FrameEye, "ge_resolve_material": abi.ResolveMaterial,
                   "ge_resolve_light": abi.ResolveLight, "ge_path_trace_params": abi.PathTraceParams,
-                  "ge_ray_visibility_params": abi.RayVisibilityParams, "ge_vertex_attributes": abi.VertexAttributes}
+                  "ge_ray_visibility_params": abi.RayVisibilityParams, "ge_vertex_attributes": abi.VertexAttributes,
+                  "ge_texture_desc": abi.TextureDesc, "ge_sampler_desc": abi.SamplerDesc,
+                  "ge_scene_desc": abi.SceneDesc, "ge_rt_counters": abi.Counters}
         if struct in mirror:
             assert C.sizeof(mirror[struct]) == int(size), struct
         dtypes = {"ge_ray": abi.RAY_DTYPE, "ge_hit": abi.HIT_DTYPE, "ge_bvh_node": abi.NODE_DTYPE,
