@@ -13,7 +13,7 @@ value = rays / wall seconds of the K timed frames. The BVH build is timed separa
 
 roofline: the dominant kernel of the frame (the traversal walk with the larger share of device time:
 k_closest, the closest-hit walk of one bounce, or k_any, the any-hit shadow walk of one bounce). Its
-algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 64 B per node popped + 48 B per
+algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 128 B per 4-wide node visited + 48 B per
 triangle tested + 64 B per instance entered, from the counted replay's per-kind counters divided by
 its launches per frame; its mean launch time is the library's own HIP events around each launch on
 its stream, over the timed frames. Peak: MI355X's 8 TB/s. `traffic` is that kernel's measured HBM
@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 WIDTH, HEIGHT = 1920, 1080
 PEAK_HBM_GBPS = 8000.0
+NODE_BYTES = 128  # a 4-wide node: four child boxes and references, what one node visit reads
 METRIC = "Mrays/s (closest+any-hit); 1080p time-to-equal-RMSE, cached vs plain tracer"
 
 
@@ -159,7 +160,7 @@ def shadow_main(args):
     ctx.reset_stats()
     frame(args.warmup + args.steps, count=True)
     st = ctx.stats()
-    per_ray = (48 * st["any"] + 64 * st["any_nodes"] + 48 * st["any_triangles"] + 64 * st["any_instances"]) / st["any"]
+    per_ray = (48 * st["any"] + NODE_BYTES * st["any_nodes"] + 48 * st["any_triangles"] + 64 * st["any_instances"]) / st["any"]
     rays_per_frame = rays / args.steps
     launch_ms = sum(walk_ms) / len(walk_ms)
     achieved = per_ray * rays_per_frame / (launch_ms * 1e-3) / 1e9
@@ -261,7 +262,7 @@ def masked_main(args):
     ctx.reset_stats()
     frame(count=True)
     st = ctx.stats()
-    per_ray = (48 * st["any"] + 64 * st["any_nodes"] + 48 * st["any_triangles"] + 72 * st["any_instances"] +
+    per_ray = (48 * st["any"] + NODE_BYTES * st["any_nodes"] + 48 * st["any_triangles"] + 72 * st["any_instances"] +
                MASK_TEST_BYTES * st["any_mask_tests"]) / st["any"]
     launch_ms = sum(walk_ms) / len(walk_ms)
     achieved = per_ray * rays / (launch_ms * 1e-3) / 1e9
@@ -590,7 +591,7 @@ def inverse_main(args):
     rays = st["closest"] + st["any"]
     f_ms = sum(fwd_ms) / len(fwd_ms)
     a_ms = sum(adj_ms) / len(adj_ms)
-    bytes_per_frame = (48 * rays + 64 * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"])
+    bytes_per_frame = (48 * rays + NODE_BYTES * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"])
     # The wavefront adjoint replays every ray in a primal pass (each path's radiance) and a gradient
     # pass; when the forward render traced the same parameters it reuses that as the primal. Passes
     # actually run = the adjoint's closest-hit launches over the forward's.
@@ -733,14 +734,14 @@ def main():
     # Algorithmic bytes of each walk kind over the timed frames (counts of the counted replay of the
     # same frames), and their HIP-event times.
     kinds = {
-        "k_closest": (48 * st["closest"] + 64 * (st["nodes"] - st["any_nodes"]) +
+        "k_closest": (48 * st["closest"] + NODE_BYTES * (st["nodes"] - st["any_nodes"]) +
                       48 * (st["triangles"] - st["any_triangles"]) + 64 * (st["instances"] - st["any_instances"]),
                       sum(x["closest_ms"] for x in per_step), sum(x["closest_launches"] for x in per_step)),
-        "k_any": (48 * st["any"] + 64 * st["any_nodes"] + 48 * st["any_triangles"] + 64 * st["any_instances"],
+        "k_any": (48 * st["any"] + NODE_BYTES * st["any_nodes"] + 48 * st["any_triangles"] + 64 * st["any_instances"],
                   sum(x["any_ms"] for x in per_step), sum(x["any_launches"] for x in per_step)),
     }
     if args.megakernel:
-        kinds = {"k_path_trace": (48 * rays + 64 * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"],
+        kinds = {"k_path_trace": (48 * rays + NODE_BYTES * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"],
                                   sum(kernel_ms), args.steps)}
     for name, (b, ms, launches) in kinds.items():
         log(f"[bench] {name}: {b / args.steps / 1e6:.1f} MB a frame over {launches / args.steps:.1f} launches, "

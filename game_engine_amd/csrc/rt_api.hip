@@ -90,6 +90,8 @@ Bvh bvh_view(const SceneDev& s) {
   b.tlas_slots = s.tlas_slots.as<uint32_t>();
   b.inst = s.inst_recs.as<InstRec>();
   b.nodes = s.blas_nodes.as<ge_bvh_node>();
+  b.wtlas = s.tlas_wide.as<WideNode>();
+  b.wnodes = s.blas_wide.as<WideNode>();
   b.tris = s.blas_tris.as<ge_bvh_triangle>();
   b.tlas_node_count = s.tlas_node_count;
   b.stack_cap = s.stack_cap;
@@ -445,7 +447,7 @@ void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
                     &s.materials, &s.cluster_materials, &s.lights, &s.tri_prefix, &s.tri_mesh_of, &s.blas_nodes,
                     &s.blas_tris, &s.blas_morton, &s.mesh_roots, &s.mesh_height, &s.tlas_nodes, &s.tlas_slots,
                     &s.tlas_morton, &s.inst_recs, &s.inst_boxes, &s.live_ids, &s.tlas_root, &s.tlas_height, &s.pairs,
-                    &s.textures, &s.samplers, &s.texels, &s.srgb, &s.mask_inst})
+                    &s.textures, &s.samplers, &s.texels, &s.srgb, &s.mask_inst, &s.blas_wide, &s.tlas_wide})
     b->release();
   delete scene;
 }
@@ -481,6 +483,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   const hipStream_t st = ctx->stream;
   s.eye = eye ? *eye : ge_frame_eye{};
   GE_HIP(ctx, s.blas_nodes.reserve(size_t(std::max(s.node_pool, 1u)) * sizeof(ge_bvh_node)));
+  GE_HIP(ctx, s.blas_wide.reserve(size_t(std::max(s.node_pool, 1u)) * sizeof(WideNode)));
   GE_HIP(ctx, s.blas_tris.reserve(size_t(std::max(s.tri_pool, 1u)) * sizeof(ge_bvh_triangle)));
   GE_HIP(ctx, s.blas_morton.reserve(size_t(std::max(s.tri_pool, 1u)) * 4));
   GE_HIP(ctx, s.mesh_roots.reserve(size_t(std::max(s.mesh_count, 1u)) * 24));
@@ -546,7 +549,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), uint32_t(seg_first.size()), ctx->tmp[3].as<uint32_t>(),
                                md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
                                ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
-                               std::max(s.node_pool, 1u)));
+                               std::max(s.node_pool, 1u), s.blas_wide.as<WideNode>()));
     GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
                                         ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
                                         ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));
@@ -576,6 +579,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, upload(s.inst_recs, recs.data(), recs.size(), st));
   GE_HIP(ctx, upload(s.live_ids, live.data(), live.size(), st));
   GE_HIP(ctx, s.tlas_nodes.reserve(size_t(std::max(s.tlas_node_count, 1u)) * sizeof(ge_bvh_node)));
+  GE_HIP(ctx, s.tlas_wide.reserve(size_t(std::max(s.tlas_node_count, 1u)) * sizeof(WideNode)));
   GE_HIP(ctx, s.tlas_slots.reserve(size_t(std::max(nl, 1u)) * 4));
   GE_HIP(ctx, s.tlas_morton.reserve(size_t(std::max(nl, 1u)) * 4));
   GE_HIP(ctx, s.tlas_root.reserve(32));
@@ -595,7 +599,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, launch_instance_boxes(st, nl, s.live_ids.as<uint32_t>(), s.instances.as<ge_instance_desc>(), d_T,
                                       s.mesh_roots.as<float>(), boxes, seg_of));
     GE_HIP(ctx, build_segments(ctx->build, st, nl, boxes, 1, seg_of, tm, tm + 1, tm + 2, s.tlas_nodes.as<ge_bvh_node>(), perm,
-                               codes, s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), nullptr, s.tlas_node_count));
+                               codes, s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), nullptr, s.tlas_node_count,
+                               s.tlas_wide.as<WideNode>()));
     GE_HIP(ctx, launch_gather_slots(st, nl, perm, s.live_ids.as<uint32_t>(), s.tlas_slots.as<uint32_t>()));
     GE_HIP(ctx, hipMemcpyAsync(s.tlas_morton.ptr, codes, size_t(nl) * 4, hipMemcpyDeviceToDevice, st));
   }
@@ -604,8 +609,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   float ms = 0.0f;
   GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->bev0, ctx->bev1));
   ctx->last_build_ms = ms;
-  // The traversal stack holds at most one deferred sibling per level of each structure plus the
-  // sentinel between them.
+  // The traversal stack holds at most three deferred children per wide level (two binary levels) of
+  // each structure plus the sentinel between them.
   s.h_mesh_height.assign(s.mesh_count, 0);
   if (s.mesh_count) GE_HIP(ctx, hipMemcpy(s.h_mesh_height.data(), s.mesh_height.ptr, size_t(s.mesh_count) * 4, hipMemcpyDeviceToHost));
   s.tlas_height_value = 0;
@@ -614,10 +619,11 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   for (uint32_t m = 0; m < s.mesh_count; ++m)
     if (s.mesh_tri_count[m] > 0) blas_max = std::max(blas_max, s.h_mesh_height[m]);
   s.built = false;
-  GE_CHECK(ctx, s.tlas_height_value + blas_max + 2 <= uint32_t(k_stack), GE_RT_ERROR_UNSUPPORTED,
+  const uint32_t need = 3 * ((s.tlas_height_value + 1) / 2) + 3 * ((blas_max + 1) / 2) + 2;
+  GE_CHECK(ctx, need <= uint32_t(k_stack), GE_RT_ERROR_UNSUPPORTED,
            "the structures are " + std::to_string(s.tlas_height_value) + " + " + std::to_string(blas_max) +
                " levels deep, past the traversal stack of " + std::to_string(k_stack));
-  s.stack_cap = s.tlas_height_value + blas_max + 2;
+  s.stack_cap = need;
   s.built = true;
   ++s.version;
   return GE_RT_OK;

@@ -547,6 +547,50 @@ __global__ void k_instance_boxes(uint32_t n, const uint32_t* __restrict__ live, 
   seg_of[k] = 0;
 }
 
+// The 4-wide collapse: each node at even depth gathers its children's children (a leaf child taken
+// as it is), in child order, with their boxes, into the wide node at its own binary index.
+__global__ void k_collapse(uint32_t n_total, const uint32_t* __restrict__ seg_of, const uint32_t* __restrict__ seg_first,
+                           const uint32_t* __restrict__ node_base, const uint32_t* __restrict__ dist,
+                           const ge_bvh_node* __restrict__ nodes, WideNode* __restrict__ wide) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_total) return;
+  const uint32_t d = dist[g];
+  if (d == kNoDepth || (d & 1u)) return;
+  const uint32_t s = seg_of[g];
+  const uint32_t nb = node_base[s];
+  const uint32_t i = g - seg_first[s];
+  const ge_bvh_node a = nodes[nb + i];
+  WideNode w;
+  int k = 0;
+  auto put = [&](uint32_t ref, const float* lo, const float* hi) {
+    w.child[k] = ref;
+    w.lox[k] = lo[0];
+    w.loy[k] = lo[1];
+    w.loz[k] = lo[2];
+    w.hix[k] = hi[0];
+    w.hiy[k] = hi[1];
+    w.hiz[k] = hi[2];
+    ++k;
+  };
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t ch = a.child[c];
+    if (ch == GE_BVH_NONE) continue;
+    if (ch & GE_BVH_LEAF) {
+      put(ch, c == 0 ? a.lo0 : a.lo1, c == 0 ? a.hi0 : a.hi1);
+      continue;
+    }
+    const ge_bvh_node m = nodes[nb + ch];
+    for (int e = 0; e < 2; ++e)
+      if (m.child[e] != GE_BVH_NONE) put(m.child[e], e == 0 ? m.lo0 : m.lo1, e == 0 ? m.hi0 : m.hi1);
+  }
+  for (; k < 4; ++k) {
+    w.child[k] = GE_BVH_NONE;
+    w.lox[k] = w.loy[k] = w.loz[k] = w.hix[k] = w.hiy[k] = w.hiz[k] = 0.0f;
+  }
+  for (int q = 0; q < 4; ++q) w.pad[q] = 0;
+  wide[nb + i] = w;
+}
+
 __global__ void k_gather_slots(uint32_t n, const uint32_t* perm, const uint32_t* live, uint32_t* slots) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < n) slots[g] = live[perm[g]];
@@ -559,7 +603,8 @@ inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t
 hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, const float* boxes, uint32_t nseg,
                           const uint32_t* d_seg_of, const uint32_t* d_seg_first, const uint32_t* d_seg_count,
                           const uint32_t* d_node_base, ge_bvh_node* nodes, uint32_t* perm, uint32_t* morton,
-                          float* roots, uint32_t* heights, const uint32_t* d_root_index, uint32_t node_pool) {
+                          float* roots, uint32_t* heights, const uint32_t* d_root_index, uint32_t node_pool,
+                          WideNode* wide) {
   if (n == 0) return hipSuccess;
   hipError_t e;
   const uint32_t tiles = blocks(n, kSortTile);
@@ -649,6 +694,10 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
                                                               d_root_index, nodes, sc.node_parent.as<uint32_t>(),
                                                               sc.node_box.as<float>(), roots, heights);
   }
+  // The depths are still those of the pointer jumping.
+  if (wide)
+    k_collapse<<<blocks(n, 256), 256, 0, stream>>>(n, d_seg_of, d_seg_first, d_node_base, sc.dist[jc].as<uint32_t>(),
+                                                   nodes, wide);
   return hipGetLastError();
 }
 

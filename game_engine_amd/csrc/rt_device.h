@@ -254,14 +254,27 @@ GE_DEV bool candidate_kept(const MaskView& mv, uint32_t cluster, uint32_t materi
   return alpha_value >= offset.z;
 }
 
+// A 4-wide node: two binary levels collapsed (an even-depth node's children's children, a leaf
+// child taken as it is), its four child boxes axis by axis and its four references (bit 31 a leaf
+// slot, GE_BVH_NONE empty, else the binary index of an even-depth node, whose wide node sits at the
+// same index). 128 bytes: two 64-byte lines, loaded as eight 16-byte words at once.
+struct WideNode {
+  float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
+  uint32_t child[4];
+  uint32_t pad[4];
+};
+static_assert(sizeof(WideNode) == 128, "a wide node is two 64-byte lines");
+
 struct Bvh {
   const ge_bvh_node* tlas;       // the top level's nodes (node 0 the root)
   const uint32_t* tlas_slots;    // sorted slot -> instance id
   const InstRec* inst;           // per instance id
   const ge_bvh_node* nodes;      // every mesh's nodes, pooled
   const ge_bvh_triangle* tris;   // every mesh's sorted triangles, pooled
+  const WideNode* wtlas;         // the top level's wide nodes, at the binary indices
+  const WideNode* wnodes;        // every mesh's wide nodes, at the binary indices of the pool
   uint32_t tlas_node_count;      // 0: an empty scene
-  uint32_t stack_cap;            // entries of the per-lane LDS stack: both structures' heights + 2
+  uint32_t stack_cap;            // entries of the per-lane stack the structures' depths need
   MaskView masks;                // read only by the walks that decide masked candidates
 };
 
@@ -292,13 +305,17 @@ GE_DEV void flush_counters(unsigned long long* counters, uint64_t closest, uint6
 constexpr float k_box_robust = 1.00000095367431640625f;  // 1 + 2^-20 on the far distance (Ize 2013)
 constexpr uint32_t k_sentinel = 0xfffffffeu;
 // The most entries a lane's traversal stack may need: the build refuses deeper structures.
-constexpr int k_stack = 96;
+constexpr int k_stack = 160;
 // Each kernel's workgroup is one wave of 64 lanes. The top k_ring entries of a lane's stack live in
 // an LDS ring (entry k of lane l at word 64 (k mod k_ring) + l, so a wave's pushes and pops hit 64
 // different banks); deeper entries spill to the lane's private memory and come back as the stack
 // shrinks, so LDS stays at 4 KiB a wave whatever the structures' depth.
 constexpr int k_lanes = 64;
-constexpr int k_ring = 16;
+#ifndef GE_WALK_RING
+#define GE_WALK_RING 16
+#endif
+constexpr int k_ring = GE_WALK_RING;
+static_assert(k_ring >= 4 && k_ring <= 64 && (k_ring & (k_ring - 1)) == 0, "the ring is indexed by a mask");
 
 struct Ray {
   float o[3], d[3], inv[3];
@@ -354,13 +371,19 @@ GE_DEV Ray make_ray(f3 o, f3 d, float tmin) {
 // The box test: the slab interval, its far end widened by 1 + 2^-20 (Ize 2013, "Robust BVH Ray
 // Traversal"), against [tmin, tmax], both widened by the ray's absolute slack `teps`, so that a
 // triangle a few coordinate ulps short of its box's computed entry (a hit very near an origin far
-// from the world's) is never culled with its box. min/max as compares: no operand is NaN.
+// from the world's) is never culled with its box. No operand is NaN (a direction's reciprocal is
+// finite and non-zero), so the hardware's min and max (v_min3/v_max3 after combining) give the
+// compare-and-select forms' values up to the sign of a zero, which no comparison below can see.
 GE_DEV bool slab(const float lo[3], const float hi[3], const Ray& r, float tmax, float& tnear) {
   const float tx0 = (lo[0] - r.o[0]) * r.inv[0], tx1 = (hi[0] - r.o[0]) * r.inv[0];
   const float ty0 = (lo[1] - r.o[1]) * r.inv[1], ty1 = (hi[1] - r.o[1]) * r.inv[1];
   const float tz0 = (lo[2] - r.o[2]) * r.inv[2], tz1 = (hi[2] - r.o[2]) * r.inv[2];
-  const float tn = maxr(maxr(minr(tx0, tx1), minr(ty0, ty1)), maxr(minr(tz0, tz1), r.tmin));
-  const float tf = minr(minr(minr(maxr(tx0, tx1), maxr(ty0, ty1)), maxr(tz0, tz1)) * k_box_robust, tmax);
+  const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
+                                   __builtin_fmaxf(__builtin_fminf(tz0, tz1), r.tmin));
+  const float tf = __builtin_fminf(
+      __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)), __builtin_fmaxf(tz0, tz1)) *
+          k_box_robust,
+      tmax);
   tnear = tn;
   return tn <= tf + r.teps;
 }
@@ -401,18 +424,6 @@ struct Hit {
   bool found;
 };
 
-GE_DEV void load_node(const ge_bvh_node* nodes, uint32_t i, float lo0[3], float hi0[3], float lo1[3], float hi1[3],
-                      uint32_t child[2]) {
-  const float4* p = reinterpret_cast<const float4*>(nodes + i);
-  const float4 a = p[0], b = p[1], c = p[2], d = p[3];
-  lo0[0] = a.x; lo0[1] = a.y; lo0[2] = a.z;
-  hi0[0] = a.w; hi0[1] = b.x; hi0[2] = b.y;
-  lo1[0] = b.z; lo1[1] = b.w; lo1[2] = c.x;
-  hi1[0] = c.y; hi1[1] = c.z; hi1[2] = c.w;
-  child[0] = __float_as_uint(d.x);
-  child[1] = __float_as_uint(d.y);
-}
-
 GE_DEV void to_object(const InstRec& rec, const Ray& w, Ray& o) {
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -423,9 +434,11 @@ GE_DEV void to_object(const InstRec& rec, const Ray& w, Ray& o) {
   prep(o);
 }
 
-// The two-level walk in its canonical order: the nearer child first (child 0 on a tie) and the
-// other pushed; a mesh's leaves tested as their node is visited; a top-level leaf enters its
-// instance (the ray goes into mesh space and a sentinel marks the way back). Closest hits are
+// The two-level walk over the 4-wide nodes in its canonical order: a node's four boxes against the
+// distance at entry; then a mesh's leaves among them in child order, each skipped if its box's entry
+// lies beyond the distance so far; then the inner children whose entry is within the distance after
+// the leaves, the nearest next (ties: the first) and the others pushed so that they come back in
+// child order; a top-level leaf enters its instance (the ray goes into mesh space and a sentinel marks the way back). Closest hits are
 // ordered by (t, then the larger pair << 8 | triangle), a total order, so the answer does not depend
 // on the walk; the counters do, and match the oracle's walk.
 //
@@ -501,57 +514,89 @@ struct Walker {
       cur = 0;
     }
     if (Count) ++tally.nodes[AnyHit];
-    float lo0[3], hi0[3], lo1[3], hi1[3];
-    uint32_t child[2];
-    load_node(in_blas ? bvh.nodes : bvh.tlas, in_blas ? node_base + cur : cur, lo0, hi0, lo1, hi1, child);
-    float tn0 = 0.0f, tn1 = 0.0f;
-    bool h0 = child[0] != GE_BVH_NONE && slab(lo0, hi0, ray, hit.t, tn0);
-    bool h1 = child[1] != GE_BVH_NONE && slab(lo1, hi1, ray, hit.t, tn1);
-    if (in_blas) {
+    const float4* np = reinterpret_cast<const float4*>(in_blas ? bvh.wnodes + node_base + cur : bvh.wtlas + cur);
+    const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+    const uint4 refs = reinterpret_cast<const uint4*>(np)[6];
+    const float lo[4][3] = {{lx.x, ly.x, lz.x}, {lx.y, ly.y, lz.y}, {lx.z, ly.z, lz.z}, {lx.w, ly.w, lz.w}};
+    const float hi[4][3] = {{hx.x, hy.x, hz.x}, {hx.y, hy.y, hz.y}, {hx.z, hy.z, hz.z}, {hx.w, hy.w, hz.w}};
+    const uint32_t ref[4] = {refs.x, refs.y, refs.z, refs.w};
+    // The four boxes against the distance at entry: one pass of box tests for the whole wave.
+    uint32_t hit_mask = 0, leaf_mask = 0;  // inner children to go on to; mesh leaves to test
+    float tn[4];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const bool hc = c == 0 ? h0 : h1;
-        if (!hc || !(child[c] & GE_BVH_LEAF)) continue;
-        if (c == 0) h0 = false; else h1 = false;
-        const ge_bvh_triangle* tri = bvh.tris + tri_base + (child[c] & ~GE_BVH_LEAF);
-        const float4* tp = reinterpret_cast<const float4*>(tri);
-        const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
-        const float v0[3] = {q0.x, q0.y, q0.z}, v1[3] = {q0.w, q1.x, q1.y}, v2[3] = {q1.z, q1.w, q2.x};
-        const uint32_t prim = __float_as_uint(q2.y);
-        if (Count) ++tally.tris[AnyHit];
-        float t, u, v;
-        if (!intersect(ray, v0, v1, v2, t, u, v)) continue;
-        if (!(t >= ray.tmin && t <= hit.t)) continue;
-        if (Masks && mask_live) {
-          const uint32_t cluster = mask_cluster + (prim >> 8);
-          const uint32_t material = bvh.masks.cluster_materials[cluster] + mask_material;
-          bool evaluated = false;
-          const bool kept = candidate_kept(bvh.masks, cluster, material, prim & 0xffu, u, v, alpha, evaluated);
-          if (Count && evaluated) ++tally.masks[AnyHit];
-          if (!kept) continue;
-        }
-        const uint32_t key = (first_pair << 8) + prim;
-        if (AnyHit || t < hit.t || !hit.found || key > hit.key) {
-          hit.found = true;
-          hit.t = t;
-          hit.u = u;
-          hit.v = v;
-          hit.inst = inst_id;
-          hit.prim = prim;
-          hit.key = key;
-          if (AnyHit) return true;
-        }
+    for (int c = 0; c < 4; ++c) {
+      // Branch-free, so that the node's seven 16-byte loads stay whole.
+      const bool in = slab(lo[c], hi[c], ray, hit.t, tn[c]) && ref[c] != GE_BVH_NONE;
+      const bool leaf = in_blas && (ref[c] & GE_BVH_LEAF) != 0u;
+      leaf_mask |= uint32_t(in && leaf) << c;
+      hit_mask |= uint32_t(in && !leaf) << c;
+    }
+    // The leaves in child order, one triangle test a trip for every lane that has one left, so lanes
+    // whose leaves sit in different slots share the trips (at most four, unrolled). A box that passed
+    // at entry is still within the distance so far exactly when tn <= t + teps (the slab test's other
+    // bound held at entry).
+#pragma unroll
+    for (int trip = 0; trip < 4; ++trip) {
+      if (!leaf_mask) break;
+      const int c = __builtin_ctz(leaf_mask);
+      leaf_mask &= leaf_mask - 1u;
+      const float tc = c == 0 ? tn[0] : c == 1 ? tn[1] : c == 2 ? tn[2] : tn[3];
+      if (!(tc <= hit.t + ray.teps)) continue;
+      const uint32_t rc = c == 0 ? ref[0] : c == 1 ? ref[1] : c == 2 ? ref[2] : ref[3];
+      const ge_bvh_triangle* tri = bvh.tris + tri_base + (rc & ~GE_BVH_LEAF);
+      const float4* tp = reinterpret_cast<const float4*>(tri);
+      const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
+      const float v0[3] = {q0.x, q0.y, q0.z}, v1[3] = {q0.w, q1.x, q1.y}, v2[3] = {q1.z, q1.w, q2.x};
+      const uint32_t prim = __float_as_uint(q2.y);
+      if (Count) ++tally.tris[AnyHit];
+      float t, u, v;
+      if (!intersect(ray, v0, v1, v2, t, u, v)) continue;
+      if (!(t >= ray.tmin && t <= hit.t)) continue;
+      if (Masks && mask_live) {
+        const uint32_t cluster = mask_cluster + (prim >> 8);
+        const uint32_t material = bvh.masks.cluster_materials[cluster] + mask_material;
+        bool evaluated = false;
+        const bool kept = candidate_kept(bvh.masks, cluster, material, prim & 0xffu, u, v, alpha, evaluated);
+        if (Count && evaluated) ++tally.masks[AnyHit];
+        if (!kept) continue;
+      }
+      const uint32_t key_hit = (first_pair << 8) + prim;
+      if (AnyHit || t < hit.t || !hit.found || key_hit > hit.key) {
+        hit.found = true;
+        hit.t = t;
+        hit.u = u;
+        hit.v = v;
+        hit.inst = inst_id;
+        hit.prim = prim;
+        hit.key = key_hit;
+        if (AnyHit) return true;
       }
     }
-    if (h0 && h1) {
-      const bool second_first = tn1 < tn0;
-      push(stack, spill, cap, second_first ? child[0] : child[1]);
-      cur = second_first ? child[1] : child[0];
-      return false;
-    }
-    if (h0 || h1) {
-      cur = h0 ? child[0] : child[1];
-      return false;
+    if (hit_mask) {
+      // The inner children still within the distance; the nearest next (ties: the first), the
+      // others pushed last child first, so they come back in child order.
+      int best = -1;
+      float best_t = 0.0f;
+      const float reach = hit.t + ray.teps;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (!((hit_mask >> c) & 1u)) continue;
+        if (!(tn[c] <= reach)) {
+          hit_mask &= ~(1u << c);
+          continue;
+        }
+        if (best < 0 || tn[c] < best_t) {
+          best = c;
+          best_t = tn[c];
+        }
+      }
+      if (hit_mask) {
+#pragma unroll
+        for (int c = 3; c >= 0; --c)
+          if ((hit_mask >> c) & 1u && c != best) push(stack, spill, cap, ref[c]);
+        cur = best == 0 ? ref[0] : best == 1 ? ref[1] : best == 2 ? ref[2] : ref[3];
+        return false;
+      }
     }
     for (;;) {
       if (sp == 0) return true;
@@ -585,10 +630,11 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
 // are not left idle until the wave's longest ray ends. `load(i, slot, ray, tmax, mask)` fetches ray
 // i (false: nothing to trace, the answer is a miss) and `store(slot, hit)` writes an answer. Every
 // wave leaves the loop once the counter has passed n and its lanes are idle.
-// Occupancy asked of the walk kernels, in waves per SIMD: 6 fits the walk in 80 VGPRs (5 waves
-// without the hint, 82-86 VGPRs), 5% faster on the 4M-triangle scene; 0 leaves it to the compiler.
+// Occupancy asked of the walk kernels, in waves per SIMD: 5 gives the 4-wide walk 96 VGPRs with a
+// few spilled; 6 (80 VGPRs) spills more of it and runs 8% slower, 4 runs 1-3% slower on the
+// 4M-triangle scene; 0 leaves it to the compiler.
 #ifndef GE_WALK_WAVES_PER_EU
-#define GE_WALK_WAVES_PER_EU 6
+#define GE_WALK_WAVES_PER_EU 5
 #endif
 #if GE_WALK_WAVES_PER_EU > 0
 #define GE_WALK_ATTR __attribute__((amdgpu_waves_per_eu(GE_WALK_WAVES_PER_EU)))

@@ -63,6 +63,7 @@ struct SceneDev {
   // The built structures.
   DevBuf blas_nodes, blas_tris, blas_morton, mesh_roots, mesh_height;  // roots: 6 floats per mesh
   DevBuf tlas_nodes, tlas_slots, tlas_morton, inst_recs, inst_boxes, live_ids, tlas_root, tlas_height;
+  DevBuf blas_wide, tlas_wide;  // the 4-wide collapse of each, at the binary nodes' indices
   uint32_t tlas_prims = 0, tlas_node_count = 0;
   std::vector<uint32_t> h_mesh_height;
   uint32_t tlas_height_value = 0;
@@ -197,7 +198,7 @@ hipError_t build_segments(BuildScratch& scratch, hipStream_t stream, uint32_t n,
                           uint32_t nseg, const uint32_t* d_seg_of, const uint32_t* d_seg_first,
                           const uint32_t* d_seg_count, const uint32_t* d_node_base, ge_bvh_node* nodes,
                           uint32_t* perm, uint32_t* morton, float* roots, uint32_t* heights,
-                          const uint32_t* d_root_index, uint32_t node_pool);
+                          const uint32_t* d_root_index, uint32_t node_pool, WideNode* wide);
 // Triangle records, boxes and segment ids of the clusters in `cluster_list` (cluster k's triangles
 // at batch index cluster_out[k] on; cluster_local[k] its index within its mesh).
 hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,

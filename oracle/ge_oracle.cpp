@@ -313,7 +313,19 @@ struct InstRec {
   bool live;
 };
 
+// A 4-wide node: four child boxes and references (bit 31: a leaf slot; GE_BVH_NONE: empty).
+struct Node4 {
+  float lo[4][3], hi[4][3];
+  uint32_t child[4];
+  uint32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "a wide node is two 64-byte lines");
+using Wide = std::vector<Node4>;
+struct Lbvh;
+void collapse(const Lbvh& b, Wide& w);
+
 struct Blas {
+  Wide wide;
   Lbvh bvh;
   std::vector<ge_bvh_triangle> tris;  // in sorted order
 };
@@ -340,6 +352,7 @@ struct ge_oracle_scene {
   std::vector<Blas> blas;
   std::vector<InstRec> inst;
   Lbvh tlas;
+  Wide tlas_wide;
   std::vector<uint32_t> tlas_slots;  // sorted slot -> instance id
   ge_frame_eye eye{};                // the frame the instance structure was built in
 };
@@ -635,6 +648,7 @@ void build_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
     ids.push_back(i);
   }
   lbvh_build(boxes, s.tlas);
+  collapse(s.tlas, s.tlas_wide);
   s.tlas_slots.resize(ids.size());
   for (size_t k = 0; k < ids.size(); ++k) s.tlas_slots[k] = ids[s.tlas.order[k]];
 }
@@ -667,6 +681,7 @@ void build_blas(ge_oracle_scene& s, uint32_t m) {
   }
   Blas& out = s.blas[m];
   lbvh_build(boxes, out.bvh);
+  collapse(out.bvh, out.wide);
   out.tris.resize(tris.size());
   for (size_t k = 0; k < tris.size(); ++k) out.tris[k] = tris[out.bvh.order[k]];
 }
@@ -775,8 +790,14 @@ constexpr uint32_t k_sentinel = 0xfffffffeu;
 // mesh's structure are tested as their node is visited; a top-level leaf enters its instance. The
 // backend's kernels walk in this order, so their counters equal these.
 template <bool AnyHit>
+bool traverse4(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit, Tally& tally,
+               int alpha);
+extern int g_walk_width;
+
+template <bool AnyHit>
 bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit,
               Tally& tally, int alpha = kAlphaOpaque) {
+  if (g_walk_width == 4) return traverse4<AnyHit>(s, world, tmax, mask, hit, tally, alpha);
   hit.found = false;
   hit.t = tmax;
   hit.key = 0;
@@ -852,6 +873,155 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
     }
     if (h1) {
       cur = node.child[1];
+      continue;
+    }
+    for (;;) {
+      if (sp == 0) return hit.found;
+      const uint32_t e = stack[--sp];
+      if (e == k_sentinel) {
+        in_blas = false;
+        ray = world;
+        continue;
+      }
+      cur = e;
+      break;
+    }
+  }
+}
+
+// ---- the 4-wide walk ------------------------------------------------------------------------------
+// Pins: tests/test_oracle_bvh.py (the wide walk against brute force and against the binary walk on
+// every ray; the collapse's node count and child boxes against the binary structure).
+//
+// Two binary levels a wide node: the internal nodes at even depth, each with its children's
+// children (a leaf child taken as it is), numbered in the order of their binary index.
+void collapse_levels(const Lbvh& b, Wide& w) {
+  const size_t n = b.nodes.size();
+  std::vector<uint32_t> depth(n, 0), index(n, GE_BVH_NONE);
+  for (size_t i = 0; i < n; ++i)  // parents precede children in no particular order: walk up
+    for (uint32_t p = b.nodes[i].parent; p != 0xffffffffu; p = b.nodes[p].parent) ++depth[i];
+  uint32_t count = 0;
+  for (size_t i = 0; i < n; ++i)
+    if ((depth[i] & 1u) == 0) index[i] = count++;
+  w.assign(count, Node4{});
+  for (size_t i = 0; i < n; ++i) {
+    if (index[i] == GE_BVH_NONE) continue;
+    Node4& node = w[index[i]];
+    int k = 0;
+    const ge_bvh_node& a = b.nodes[i];
+    for (int c = 0; c < 2; ++c) {
+      if (a.child[c] == GE_BVH_NONE) continue;
+      if (a.child[c] & GE_BVH_LEAF) {
+        node.child[k] = a.child[c];
+        std::memcpy(node.lo[k], c == 0 ? a.lo0 : a.lo1, 12);
+        std::memcpy(node.hi[k], c == 0 ? a.hi0 : a.hi1, 12);
+        ++k;
+        continue;
+      }
+      const ge_bvh_node& m = b.nodes[a.child[c]];
+      for (int d = 0; d < 2; ++d) {
+        if (m.child[d] == GE_BVH_NONE) continue;
+        node.child[k] = (m.child[d] & GE_BVH_LEAF) ? m.child[d] : index[m.child[d]];
+        std::memcpy(node.lo[k], d == 0 ? m.lo0 : m.lo1, 12);
+        std::memcpy(node.hi[k], d == 0 ? m.hi0 : m.hi1, 12);
+        ++k;
+      }
+    }
+    for (; k < 4; ++k) node.child[k] = GE_BVH_NONE;
+  }
+}
+
+void collapse(const Lbvh& b, Wide& w) {
+  w.clear();
+  if (!b.nodes.empty()) collapse_levels(b, w);
+}
+
+int g_walk_width = 4;  // 2: the walks take the binary structures as built
+
+// The walk over the collapsed structures: a node's four boxes tested against the distance at entry;
+// then a mesh's leaves among them in child order, each skipped if its box's entry lies beyond the
+// distance so far (the slab test against that distance, given that it passed at entry); then the
+// inner children whose entry is still within the distance, the nearest next (ties: the first) and
+// the others pushed so that they come back in child order; a top-level leaf enters its instance as
+// in the binary walk. The hits are the binary walk's (the order of the walk never changes the
+// answer); the counters count wide nodes.
+template <bool AnyHit>
+bool traverse4(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mask, HitRec& hit, Tally& tally,
+               int alpha) {
+  hit.found = false;
+  hit.t = tmax;
+  hit.key = 0;
+  if (mask == 0 || s.tlas_wide.empty()) return false;
+  uint32_t stack[256];
+  int sp = 0;
+  TRay ray = world;
+  bool in_blas = false;
+  const Blas* blas = nullptr;
+  const InstRec* rec = nullptr;
+  uint32_t inst_id = 0, cur = 0;
+  for (;;) {
+    if (!in_blas && (cur & GE_BVH_LEAF)) {
+      inst_id = s.tlas_slots[cur & ~GE_BVH_LEAF];
+      rec = &s.inst[inst_id];
+      blas = &s.blas[rec->mesh];
+      to_object(*rec, world, ray);
+      if (sp >= 250) std::abort();
+      stack[sp++] = k_sentinel;
+      in_blas = true;
+      ++tally.inst[AnyHit];
+      cur = 0;
+    }
+    ++tally.nodes[AnyHit];
+    const Node4& node = in_blas ? blas->wide[cur] : s.tlas_wide[cur];
+    float entry[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool passed[4] = {false, false, false, false};
+    for (int c = 0; c < 4; ++c)
+      passed[c] = node.child[c] != GE_BVH_NONE && slab(node.lo[c], node.hi[c], ray, hit.t, entry[c]);
+    for (int c = 0; in_blas && c < 4; ++c) {
+      if (!passed[c] || !(node.child[c] & GE_BVH_LEAF)) continue;
+      float again = 0.0f;
+      if (!slab(node.lo[c], node.hi[c], ray, hit.t, again)) continue;
+      const ge_bvh_triangle& tri = blas->tris[node.child[c] & ~GE_BVH_LEAF];
+      ++tally.tris[AnyHit];
+      float t, u, v;
+      if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
+      if (!(t >= ray.tmin && t <= hit.t)) continue;
+      if (alpha != kAlphaOpaque && !candidate_kept(s, inst_id, tri.prim, u, v, alpha, &tally.masks[AnyHit])) continue;
+      const uint32_t key = (rec->first_pair << 8) + tri.prim;
+      if (AnyHit || t < hit.t || !hit.found || key > hit.key) {
+        hit.found = true;
+        hit.t = t;
+        hit.u = u;
+        hit.v = v;
+        hit.inst = inst_id;
+        hit.prim = tri.prim;
+        hit.key = key;
+        if (AnyHit) return true;
+      }
+    }
+    uint32_t next[4];
+    float tn[4];
+    int nn = 0;
+    for (int c = 0; c < 4; ++c) {
+      if (!passed[c] || (in_blas && (node.child[c] & GE_BVH_LEAF))) continue;
+      float again = 0.0f;
+      if (!slab(node.lo[c], node.hi[c], ray, hit.t, again)) continue;
+      tn[nn] = entry[c];
+      next[nn++] = node.child[c];
+    }
+    if (nn > 0) {
+      // The nearest hit first (ties: the first in child order), the others after it in child order.
+      int best = 0;
+      for (int k = 1; k < nn; ++k)
+        if (tn[k] < tn[best]) best = k;
+      const uint32_t first = next[best];
+      for (int k = best; k > 0; --k) next[k] = next[k - 1];
+      next[0] = first;
+    }
+    if (nn > 0) {
+      if (sp + nn >= 250) std::abort();
+      for (int k = nn - 1; k >= 1; --k) stack[sp++] = next[k];
+      cur = next[0];
       continue;
     }
     for (;;) {
@@ -2582,6 +2752,7 @@ void ge_oracle_sample(const ge_oracle_scene* s, uint32_t texture, uint32_t sampl
   }
 }
 float ge_oracle_half(uint32_t h) { return half_to_float(h & 0xffffu); }
+void ge_oracle_walk_width(int32_t width) { g_walk_width = width == 4 ? 4 : 2; }
 
 void ge_oracle_shade(const ge_oracle_scene* s, const ge_hit* hits, uint32_t count, const float towards[3], float* out) {
   for (uint32_t i = 0; i < count; ++i) {

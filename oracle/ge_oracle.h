@@ -133,6 +133,8 @@ void ge_oracle_sample(const ge_oracle_scene* scene, uint32_t texture, uint32_t s
  * normal. */
 void ge_oracle_shade(const ge_oracle_scene* scene, const ge_hit* hits, uint32_t count, const float towards[3],
                      float* out);
+/* The structures every walk takes: 2, the binary structures as built; 4, their 4-wide collapse. */
+void ge_oracle_walk_width(int32_t width);
 /* f16tof32 of the low 16 bits, and the sRGB decode table's entry for an 8-bit code. */
 float ge_oracle_half(uint32_t h);
 float ge_oracle_srgb(uint32_t code);
