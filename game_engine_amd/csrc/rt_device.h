@@ -325,7 +325,14 @@ struct Ray {
   float Sx, Sy, Sz;
 };
 
-GE_DEV float sel3(const float v[3], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+GE_DEV float sel3(const float v[3], int k) {
+  // k is 0, 1 or 2. Bit masks rather than selects: the compiler turns chained selects on a lane's k
+  // into branches.
+  const uint32_t m0 = 0u - ((uint32_t(k) - 1u) >> 31);      // all ones where k == 0
+  const uint32_t m1 = 0u - ((uint32_t(k ^ 1) - 1u) >> 31);  // all ones where k == 1
+  const uint32_t r12 = (__float_as_uint(v[1]) & m1) | (__float_as_uint(v[2]) & ~m1);  // v_bfi_b32
+  return __uint_as_float((__float_as_uint(v[0]) & m0) | (r12 & ~m0));
+}
 
 GE_DEV void prep(Ray& r) {
 #pragma unroll
