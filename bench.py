@@ -255,6 +255,7 @@ def masked_main(args):
         solid_ms.append(ctx.stats()["kernel_ms"])
     blocked_solid = int(((out != -1) & ((out & 1) == 1)).sum().item())
     scene.shadow_resolve(masked_p, words.data_ptr(), out=out.data_ptr(), device=True)
+    ctx.synchronize()  # device-pointer calls return with their kernels queued on the library's stream
     blocked = int(((out != -1) & ((out & 1) == 1)).sum().item())
     log(f"[bench] timed {args.steps} frames: {wall * 1e3:.1f} ms wall; visibility {['%.2f' % x for x in vis_ms]}, "
         f"masked shadow {['%.2f' % x for x in shadow_ms]} (walk {['%.2f' % x for x in walk_ms]}), solid shadow "

@@ -39,7 +39,9 @@ enum {
 };
 
 /* Memory flag: every caller pointer of the call is a device pointer of the context's device.
- * Without it, caller pointers are host memory and the call copies in and out. */
+ * Without it, caller pointers are host memory and the call copies in and out. With it, the call
+ * returns once its kernels are queued on the context's stream: device outputs are complete after
+ * ge_rt_synchronize (or ge_rt_stats, which waits for the stream). */
 #define GE_RT_DEVICE_PTR 0x1u
 /* Trace flag: count internal nodes popped, triangle tests and instance entries into the context's
  * counters (ge_rt_stats). Off in timed runs. */
