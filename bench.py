@@ -314,7 +314,7 @@ def masked_cpu_baseline(s, target_seconds):
         rays += int((out != abi.SHADOW_NOT_TAKEN).sum())
         frames += 1
         dt = time.perf_counter() - t
-        if dt >= target_seconds or frames >= 256:
+        if dt >= target_seconds or frames >= 4096:
             break
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{frames} masked sun shadow passes over the oracle's 1152x216 masked visibility words of the "

@@ -648,8 +648,11 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
 #else
 #define GE_WALK_ATTR
 #endif
+// Refill threshold: 48 idle lanes before a wave fetches rays. A refill (the atomic, the ray loads,
+// a reciprocal and a shear per ray) runs as divergent code beside the walk, so batching it pays:
+// on the 4M-triangle scene 16 runs 7% slower, 24 to 48 alike, 8 a third slower.
 #ifndef GE_WALK_REFILL
-#define GE_WALK_REFILL 16
+#define GE_WALK_REFILL 48
 #endif
 constexpr uint32_t k_refill = GE_WALK_REFILL;
 static_assert(k_refill >= 1 && k_refill <= 64, "a refill needs at most a wave of idle lanes");
