@@ -581,26 +581,34 @@ struct Walker {
     }
     if (hit_mask) {
       // The inner children still within the distance; the nearest next (ties: the first), the
-      // others pushed last child first, so they come back in child order.
-      int best = -1;
-      float best_t = 0.0f;
+      // others pushed last child first, so they come back in child order. Selects, not branches.
       const float reach = hit.t + ray.teps;
+      uint32_t keep = 0;
+      int best = 0;
+      float best_t = 0.0f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        if (!((hit_mask >> c) & 1u)) continue;
-        if (!(tn[c] <= reach)) {
-          hit_mask &= ~(1u << c);
-          continue;
-        }
-        if (best < 0 || tn[c] < best_t) {
-          best = c;
-          best_t = tn[c];
-        }
+        const bool in = ((hit_mask >> c) & 1u) != 0u && tn[c] <= reach;
+        const bool nearer = in && (keep == 0u || tn[c] < best_t);
+        keep |= uint32_t(in) << c;
+        best = nearer ? c : best;
+        best_t = nearer ? tn[c] : best_t;
       }
-      if (hit_mask) {
+      if (keep) {
+        const uint32_t rest = keep & ~(1u << best);
+        if (sp + 4 <= k_ring && sp + 3 <= cap) {
+          // The ring has room below its first wrap: each child is written to the free slot at the
+          // top whether or not it is pushed (one that is not is overwritten by the next).
 #pragma unroll
-        for (int c = 3; c >= 0; --c)
-          if ((hit_mask >> c) & 1u && c != best) push(stack, spill, cap, ref[c]);
+          for (int c = 3; c >= 0; --c) {
+            stack[k_lanes * sp] = ref[c];
+            sp += int((rest >> c) & 1u);
+          }
+        } else {
+#pragma unroll
+          for (int c = 3; c >= 0; --c)
+            if ((rest >> c) & 1u) push(stack, spill, cap, ref[c]);
+        }
         cur = best == 0 ? ref[0] : best == 1 ? ref[1] : best == 2 ? ref[2] : ref[3];
         return false;
       }
