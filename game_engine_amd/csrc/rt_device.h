@@ -542,7 +542,7 @@ struct Walker {
     // whose leaves sit in different slots share the trips (at most four, unrolled). A box that passed
     // at entry is still within the distance so far exactly when tn <= t + teps (the slab test's other
     // bound held at entry).
-#pragma unroll
+#pragma unroll(Masks ? 1 : 4)
     for (int trip = 0; trip < 4; ++trip) {
       if (!leaf_mask) break;
       const int c = __builtin_ctz(leaf_mask);
