@@ -188,10 +188,15 @@ GE_DEV bool light_sample(const ge_resolve_light& light, f3 position, f3& directi
   }
   const f3 to_light = ld3(light.position_radius) - position;
   const float distance_sq = dot(to_light, to_light);
-  float attenuation = distance_attenuation(distance_sq, light.position_radius[3]);
   direction = f3{0.0f, 1.0f, 0.0f};
   radiance = splat(0.0f);
   distance = 0.0f;
+  // At or beyond the radius the window is exactly zero (d^2 / r^2 rounds to at least 1, clamped to
+  // 1), so the attenuation is zero: skip its two divisions. The shading loops visit every light
+  // twice a vertex, and most are out of range.
+  const float radius = light.position_radius[3];
+  if (distance_sq >= maxr(radius * radius, 1e-8f)) return false;
+  float attenuation = distance_attenuation(distance_sq, radius);
   if (attenuation <= 0.0f) return false;
   direction = normalize(to_light);
   if (light.direction_type[3] >= 0.5f)
