@@ -261,6 +261,7 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   PathState st;
   bool alive = false;
   uint32_t nshadow = 0;
+  uint64_t lit = 0;  // lights 0..63 that give this vertex a shadow record
   Surface surface, ssurface;
   Vertex vx;
   float bias = 0.0f;
@@ -364,6 +365,7 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
           if (!light_sample(a.sv.lights[i], vx.position, ld, rad, dist)) continue;
           if (dot(surface.normal, ld) <= 0.0f) continue;
           ++nshadow;
+          if (i < 64u) lit |= 1ull << i;
         }
       }
     }
@@ -451,15 +453,19 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
         if (grad) emit_grad(k, 0, ld3(p.sun), splat(p.sun[3]), splat(1.0f));
         emit(ld3(p.sun), 1e30f, brdf_direct(surface, ld3(p.sun), splat(p.sun[3])));
       }
-      for (uint32_t i = 0; i < a.sv.light_count; ++i) {
+      auto emit_light = [&](uint32_t i) {
         const ge_resolve_light light = a.sv.lights[i];
         f3 ld, rad;
         float dist, att;
-        if (!light_sample(light, vx.position, ld, rad, dist, &att)) continue;
-        if (dot(surface.normal, ld) <= 0.0f) continue;
+        if (!light_sample(light, vx.position, ld, rad, dist, &att)) return;
+        if (dot(surface.normal, ld) <= 0.0f) return;
         if (grad) emit_grad(k, 1 + i, ld, rad, ld3(light.color_intensity) * att);
         emit(ld, dist - bias - light.spot[2], brdf_direct(surface, ld, rad));
-      }
+      };
+      // The counting pass's lights again, in the same order: the first 64 from its mask, the rest
+      // tested in full.
+      for (uint64_t m = lit; m != 0u; m &= m - 1u) emit_light(uint32_t(__builtin_ctzll(m)));
+      for (uint32_t i = 64; i < a.sv.light_count; ++i) emit_light(i);
     }
     f3 ratio_a = splat(0.0f), ratio_r = splat(0.0f);  // kShadeAdjoint: (df/dparameter) / f along the next direction
     // ---- the next direction (path_trace.slang:397-455) ----
