@@ -11,6 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
+OBJ = os.path.join(HERE, "_build")
 ARCH = os.environ.get("GE_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["rt_api.hip", "rt_bvh.hip", "rt_kernels.hip", "rt_wavefront.hip", "rt_shadow.hip", "rt_mlp.hip",
@@ -25,10 +26,21 @@ def _run(cmd):
 
 
 def build(verbose=True):
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(LIB, exist_ok=True)
+    os.makedirs(OBJ, exist_ok=True)
     rt = os.path.join(LIB, "libge_rt.so")
     synth = os.path.join(LIB, "libge_synth.so")
-    _run(["hipcc", f"--offload-arch={ARCH}", *HIP_FLAGS, "-o", rt + ".tmp", *HIP_SOURCES])
+    # One object a source, compiled side by side (each carries its own device code), then linked.
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(OBJ, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
+    jobs = max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(_run, ["hipcc", f"--offload-arch={ARCH}", *compile_flags, "-c", "-o", obj, src])
+                  for src, obj in zip(HIP_SOURCES, objs)]:
+            f.result()
+    _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", rt + ".tmp", *objs])
     os.replace(rt + ".tmp", rt)
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-o", synth + ".tmp", "scene_synth.cpp"])
     os.replace(synth + ".tmp", synth)

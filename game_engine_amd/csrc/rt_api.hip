@@ -570,8 +570,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     rec.node_base = s.mesh_node_base[in.mesh];
     rec.tri_base = s.mesh_tri_base[in.mesh];
     rec.first_pair = in.first_pair;
-    rec.live = s.mesh_tri_count[in.mesh] > 0 ? 1u : 0u;
-    if (rec.live) live.push_back(i);
+    rec.mask = instance_ray_mask(in.flags);
+    if (s.mesh_tri_count[in.mesh] > 0) live.push_back(i);
   }
   const uint32_t nl = uint32_t(live.size());
   s.tlas_prims = nl;

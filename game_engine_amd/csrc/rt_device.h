@@ -91,9 +91,18 @@ struct InstRec {
   uint32_t node_base;  // the mesh's root in the node pool
   uint32_t tri_base;   // the mesh's first triangle in the triangle pool
   uint32_t first_pair;
-  uint32_t live;
+  uint32_t mask;  // the instance's 8-bit ray mask: a ray enters it only when (ray mask & mask) != 0
 };
 static_assert(sizeof(InstRec) == 64, "one instance record is one 64-byte fetch");
+
+// The top-level record's mask from the instance's flags (instance_ray_mask, scene.slang:480-492): an
+// instance flagged untraced (bit 6, never met by a light's ray) has every bit but the caster bit,
+// every other instance all eight. Camera rays trace with 0xfe and meet both; bounce and shadow rays
+// trace with the caster bit 0x01 alone (path_trace.slang:196-203, 289-298) and pass untraced
+// instances by.
+__host__ __device__ inline uint32_t instance_ray_mask(uint32_t flags) {
+  return (flags & GE_INSTANCE_UNTRACED) != 0u ? 0xfeu : 0xffu;
+}
 
 // ---- textures at level 0 and the alpha test ------------------------------------------------------
 // The scene's bindless textures (RGBA8, level 0), samplers and the sRGB decode of each 8-bit code.
@@ -461,7 +470,7 @@ struct Walker {
   Ray world, ray;
   Hit hit;
   int sp;
-  uint32_t cur, node_base, tri_base, first_pair, inst_id;
+  uint32_t cur, node_base, tri_base, first_pair, inst_id, ray_mask;
   uint32_t alpha, mask_cluster, mask_material;  // Masks: the rule, the instance's first cluster and base
   bool in_blas, mask_live;                      // Masks: the instance has masked clusters
 
@@ -480,6 +489,7 @@ struct Walker {
     sp = 0;
     cur = 0;
     node_base = tri_base = first_pair = inst_id = 0;
+    ray_mask = mask;
     in_blas = false;
     return mask != 0 && bvh.tlas_node_count != 0;
   }
@@ -498,13 +508,30 @@ struct Walker {
     if (sp >= k_ring) slot = spill[sp - k_ring];
     return x;
   }
+  // Takes the next node off the stack (a sentinel leads back to the top level); true when the stack
+  // is empty and the walk has ended.
+  GE_DEV bool next(uint32_t* __restrict__ stack, const uint32_t* __restrict__ spill) {
+    for (;;) {
+      if (sp == 0) return true;
+      const uint32_t e = pop(stack, spill);
+      if (e == k_sentinel) {
+        in_blas = false;
+        ray = world;
+        continue;
+      }
+      cur = e;
+      return false;
+    }
+  }
 
-  // One node visit; true when the walk has ended (stack empty, or an any-hit query found a hit).
+  // One node visit; true when the walk has ended (stack empty, or an any-hit query found a hit). A
+  // top-level leaf whose instance mask shares no bit with the ray's is passed by without a visit.
   GE_DEV bool step(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
     const int cap = int(bvh.stack_cap);
     if (!in_blas && (cur & GE_BVH_LEAF)) {
       inst_id = bvh.tlas_slots[cur & ~GE_BVH_LEAF];
       const InstRec rec = bvh.inst[inst_id];
+      if ((rec.mask & ray_mask) == 0u) return next(stack, spill);
       node_base = rec.node_base;
       tri_base = rec.tri_base;
       first_pair = rec.first_pair;
@@ -613,17 +640,7 @@ struct Walker {
         return false;
       }
     }
-    for (;;) {
-      if (sp == 0) return true;
-      const uint32_t e = pop(stack, spill);
-      if (e == k_sentinel) {
-        in_blas = false;
-        ray = world;
-        continue;
-      }
-      cur = e;
-      return false;
-    }
+    return next(stack, spill);
   }
 };
 

@@ -118,6 +118,10 @@ typedef struct ge_instance_desc {
 /* InstanceDesc::flags bit 7 (scene.slang:694): the instance traces its mesh's own structure and a
  * hit's geometry index is the cluster within the mesh. Every instance this backend builds is one. */
 #define GE_INSTANCE_STATIC_RT 128u
+/* InstanceDesc::flags bit 6, k_instance_untraced (scene.slang:473-492): never met by a light's ray.
+ * The instance's top-level mask is then 0xfe instead of 0xff (instance_ray_mask), so camera rays
+ * (mask 0xfe) meet it and bounce and shadow rays (the caster bit 0x01 alone) pass it by. */
+#define GE_INSTANCE_UNTRACED 64u
 #define GE_INVALID_DEFORM 0xffffffffu
 
 /* gfx::FrameEye, 48 bytes (scene.slang:96-104): the frame's origin. All zero is the world's. */

@@ -310,8 +310,16 @@ struct InstRec {
   float w2o[3][4];  // frame space to mesh space
   uint32_t mesh;
   uint32_t first_pair;
+  uint32_t mask;  // the top-level record's 8-bit ray mask
   bool live;
 };
+
+// instance_ray_mask (scene.slang:480-492; tlas_references.slang:164-172): an instance flagged
+// k_instance_untraced (bit 6) gets every bit but the caster bit, every other one 0xff. A ray meets
+// an instance only when its own mask and the instance's share a bit (the TraceRayInline mask
+// argument): camera rays 0xfe meet both, light and bounce rays 0x01 (path_trace.slang:196-203,
+// 289-298) only the traced ones.
+inline uint32_t instance_ray_mask(uint32_t flags) { return (flags & 64u) != 0u ? 0xfeu : 0xffu; }
 
 // A 4-wide node: four child boxes and references (bit 31: a leaf slot; GE_BVH_NONE: empty).
 struct Node4 {
@@ -642,6 +650,7 @@ void build_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
     make_w2o(in, T, rec.w2o);
     rec.mesh = in.mesh;
     rec.first_pair = in.first_pair;
+    rec.mask = instance_ray_mask(in.flags);
     rec.live = s.blas[in.mesh].bvh.n > 0;
     if (!rec.live) continue;
     boxes.push_back(instance_box(in, T, s.blas[in.mesh].bvh.root));
@@ -811,9 +820,27 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
   uint32_t inst_id = 0;
   uint32_t cur = 0;
   for (;;) {
+    bool skip = false;  // a top-level leaf whose instance the ray's mask does not meet
     if (!in_blas && (cur & GE_BVH_LEAF)) {
       inst_id = s.tlas_slots[cur & ~GE_BVH_LEAF];
       rec = &s.inst[inst_id];
+      skip = (rec->mask & mask) == 0u;
+    }
+    if (skip) {
+      for (;;) {
+        if (sp == 0) return hit.found;
+        const uint32_t e = stack[--sp];
+        if (e == k_sentinel) {
+          in_blas = false;
+          ray = world;
+          continue;
+        }
+        cur = e;
+        break;
+      }
+      continue;
+    }
+    if (!in_blas && (cur & GE_BVH_LEAF)) {
       blas = &s.blas[rec->mesh];
       to_object(*rec, world, ray);
       if (sp >= 190) std::abort();  // deeper than any structure the build accepts
@@ -960,9 +987,27 @@ bool traverse4(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t
   const InstRec* rec = nullptr;
   uint32_t inst_id = 0, cur = 0;
   for (;;) {
+    bool skip = false;  // a top-level leaf whose instance the ray's mask does not meet
     if (!in_blas && (cur & GE_BVH_LEAF)) {
       inst_id = s.tlas_slots[cur & ~GE_BVH_LEAF];
       rec = &s.inst[inst_id];
+      skip = (rec->mask & mask) == 0u;
+    }
+    if (skip) {
+      for (;;) {
+        if (sp == 0) return hit.found;
+        const uint32_t e = stack[--sp];
+        if (e == k_sentinel) {
+          in_blas = false;
+          ray = world;
+          continue;
+        }
+        cur = e;
+        break;
+      }
+      continue;
+    }
+    if (!in_blas && (cur & GE_BVH_LEAF)) {
       blas = &s.blas[rec->mesh];
       to_object(*rec, world, ray);
       if (sp >= 250) std::abort();
@@ -1047,7 +1092,7 @@ bool brute(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mas
   if (mask == 0) return false;
   for (uint32_t i = 0; i < s.instances.size(); ++i) {
     const InstRec& rec = s.inst[i];
-    if (!rec.live) continue;
+    if (!rec.live || (rec.mask & mask) == 0u) continue;
     TRay ray;
     to_object(rec, world, ray);
     for (const ge_bvh_triangle& tri : s.blas[rec.mesh].tris) {
