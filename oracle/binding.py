@@ -22,17 +22,18 @@ _REF = None
 
 def build():
     """Compiles the oracle with its Makefile (g++ -O3 -march=x86-64-v3 -ffp-contract=off), and,
-    where the reference tree is present, its double-precision BSDF twin into oracle/_ref/."""
+    where the reference tree is present, the reference's header-only twins into oracle/_ref/."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     if os.path.isdir(REFERENCE):
         subprocess.run(["make", "-s", "-C", _HERE, "ref", f"REFERENCE={REFERENCE}"], check=True)
 
 
 def ref_lib():
-    """The reference's BSDF twin (oracle/_ref/libbrdf_ref.so) if it was built, else None."""
+    """The reference's header-only twins (oracle/_ref/libge_ref.so: the BSDF twin and the world-cell
+    arithmetic) if they were built, else None."""
     global _REF
     if _REF is None:
-        path = os.path.join(_HERE, "_ref", "libbrdf_ref.so")
+        path = os.path.join(_HERE, "_ref", "libge_ref.so")
         if not os.path.exists(path):
             return None
         L = C.CDLL(path)
@@ -41,7 +42,11 @@ def ref_lib():
                            ("ge_ref_distance_attenuation", [d, d]), ("ge_ref_spot_attenuation", [vp, vp, d, d])):
             getattr(L, name).restype = d
             getattr(L, name).argtypes = args
-        for name in ("ge_ref_eval", "ge_ref_fresnel", "ge_ref_direct"):
+        L.ge_ref_hash_bytes.restype = C.c_uint64
+        L.ge_ref_hash_bytes.argtypes = [vp, C.c_uint64]
+        L.ge_ref_relative_axis.restype = C.c_float
+        L.ge_ref_relative_axis.argtypes = [C.c_int32, C.c_float, C.c_int32, C.c_float, C.c_float]
+        for name in ("ge_ref_eval", "ge_ref_fresnel", "ge_ref_direct", "ge_ref_sky_radiance", "ge_ref_ground_radiance"):
             getattr(L, name).argtypes = [vp, vp]
         _REF = L
     return _REF
@@ -102,6 +107,8 @@ def lib():
         L.ge_oracle_morton3.restype = C.c_uint32
         L.ge_oracle_morton3.argtypes = [C.c_float, C.c_float, C.c_float, vp, vp]
         L.ge_oracle_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.ge_oracle_relative_axis.restype = C.c_float
+        L.ge_oracle_relative_axis.argtypes = [C.c_int32, C.c_float, C.c_int32, C.c_float, C.c_float]
         L.ge_oracle_probe.restype = C.c_int32
         L.ge_oracle_probe.argtypes = [C.c_uint32, C.c_uint32, vp, vp]
         L.ge_oracle_intersect.argtypes = [vp, vp, vp, vp, vp, C.c_float, C.c_float, vp]

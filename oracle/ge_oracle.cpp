@@ -2844,6 +2844,9 @@ uint32_t ge_oracle_morton3(float cx, float cy, float cz, const float lo[3], cons
   return morton3(c, lo, scale);
 }
 void ge_oracle_sincos(float x, float* s, float* c) { sincos_f32(x, s, c); }
+float ge_oracle_relative_axis(int32_t cell, float local, int32_t eye_cell, float eye_local, float eye_residual) {
+  return relative_axis(cell, local, eye_cell, eye_local, eye_residual);
+}
 
 void ge_oracle_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
                          float tmin, float tmax, double out[4]) {
@@ -2932,6 +2935,18 @@ template <class R> int32_t probe(uint32_t kind, const double* in, double* out) {
       brdf_eval_grad(surface(0), V(15), ga, gr);
       put(0, ga);
       put(3, gr);
+      return 0;
+    }
+    case 18: {  // environment(): in = sky[4] ground[4] sun[4] direction[3] primary flags
+      ge_path_trace_params p{};
+      for (int k = 0; k < 4; ++k) {
+        p.sky[k] = float(in[k]);
+        p.ground[k] = float(in[4 + k]);
+        p.sun[k] = float(in[8 + k]);
+      }
+      p.flags = uint32_t(in[16]);
+      PtCtx<R> c{nullptr, &p, nullptr};
+      put(0, environment(c, V(12), in[15] != 0.0));
       return 0;
     }
     default: return GE_RT_ERROR_INVALID_ARGUMENT;

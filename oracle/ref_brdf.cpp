@@ -50,6 +50,15 @@ double ge_ref_distance_attenuation(double distance_sq, double radius) {
   return ref::distance_attenuation(distance_sq, radius);
 }
 
+// in: sky[3]: the sky's radiance above the horizon (material.slang's sky_radiance)
+void ge_ref_sky_radiance(const double* in, double* out) { put(out, ref::sky_radiance(d3(in))); }
+
+// in: ground albedo[3], sun direction[3], sun intensity, sky[3]: the lit ground below the horizon
+// (brdf.slang's brdf_ground_radiance through material.slang's ground_radiance)
+void ge_ref_ground_radiance(const double* in, double* out) {
+  put(out, ref::ground_radiance(d3(in), d3(in + 3), in[6], d3(in + 7)));
+}
+
 double ge_ref_spot_attenuation(const double* to_surface, const double* direction, double cos_inner, double cos_outer) {
   return ref::spot_attenuation(d3(to_surface), d3(direction), cos_inner, cos_outer);
 }

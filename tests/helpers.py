@@ -65,3 +65,41 @@ def with_materials(synth_scene, albedo, roughness, metallic):
         m.uv_scale = 1.0
     d.materials = C.addressof(mats)
     return d, mats
+
+
+def far_from_origin(s, pt, rv, shift=(150000, 3, -200000), residual=(7e-4, -3e-4, 5e-4)):
+    """The synth scene `s` and its frame (pt, rv), moved `shift` world cells (64 m each) from the
+    world's origin and seen from a frame whose eye (FrameEye, cluster_cull.h:939-960) is the camera:
+    every instance's cell moves by `shift`, the eye is the camera's world position as a cell, a
+    float local and a small residual, and the frame's camera, lights and projection become
+    eye-relative (the camera at the frame's origin). Changes the instances of `s` in place.
+
+    Returns (desc, eye, pt, rv, keep): the scene descriptor with the eye-relative lights, the eye,
+    the two parameter blocks, and what must stay alive while desc is used."""
+    import ctypes as C
+
+    inst = s.arrays()["instances"]
+    inst["cell"] += np.asarray(shift, dtype=np.int32)
+    cam = np.array(pt.camera[:3], dtype=np.float64)
+    base = np.floor(cam / 64.0)
+    eye = abi.FrameEye()
+    eye.cell[:] = [int(base[a]) + int(shift[a]) for a in range(3)]
+    eye.local[:] = [float(np.float32(cam[a] - 64.0 * base[a] - residual[a])) for a in range(3)]
+    eye.residual[:] = [float(np.float32(r)) for r in residual]
+    d = abi.SceneDesc.from_buffer_copy(s.desc())
+    lights = (abi.ResolveLight * max(d.light_count, 1))()
+    if d.light_count:
+        C.memmove(lights, d.lights, C.sizeof(abi.ResolveLight) * d.light_count)
+        for l in lights[: d.light_count]:
+            if l.direction_type[3] != 2.0:  # a positioned light: from the eye
+                l.position_radius[:3] = [float(np.float32(l.position_radius[a] - cam[a])) for a in range(3)]
+        d.lights = C.addressof(lights)
+    pt2 = abi.PathTraceParams.from_buffer_copy(pt)
+    rv2 = abi.RayVisibilityParams.from_buffer_copy(rv)
+    pt2.camera[:3] = [0.0, 0.0, 0.0]
+    rv2.camera[:3] = [0.0, 0.0, 0.0]
+    pt2.eye = eye
+    # The projection of a camera at the frame's origin: the inverse of clip_to_ray (the rotation).
+    m = np.array(pt.clip_to_ray, dtype=np.float64).reshape(4, 4).T
+    rv2.view_proj[:] = [float(x) for x in np.linalg.inv(m).T.reshape(-1).astype(np.float32)]
+    return d, eye, pt2, rv2, lights

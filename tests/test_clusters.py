@@ -3,12 +3,12 @@
 The reference holds no container files, so the reader is pinned by what the format fixes: a writer
 of the documented layout (tests/clusters_writer.py) round-trips a synthetic mesh into a scene whose
 walk gives the oracle's hits on the original to the bit; the content hash is checked against the
-writer's Python restatement of core/hash and the properties the reference's hash tests hold
-(core/hash/tests/hash_tests.cpp:25-31); the level-0 selection, the material conversion and every
-refusal of the engine's reader (a truncated or foreign file, a section past the end, a hash
-mismatch, a missing or mis-sized section, a cluster or a triangle outside its streams) are checked
-one by one. The hash's constants beyond those properties are parity unpinned: no reference vector
-fixes them.
+reference's own hash_bytes (core/hash/include/core/hash/hash.h, compiled into oracle/_ref/ where
+the reference tree is present), the writer's Python restatement of it and the properties the
+reference's hash tests hold (core/hash/tests/hash_tests.cpp:25-31); the level-0 selection, the
+material conversion and every refusal of the engine's reader (a truncated or foreign file, a section
+past the end, a hash mismatch, a missing or mis-sized section, a cluster or a triangle outside its
+streams) are checked one by one.
 """
 import struct
 
@@ -29,6 +29,15 @@ def test_hash_properties_and_restatement():
     assert rt.clusters_hash(b"ab") != rt.clusters_hash(b"ab\0")
     assert rt.clusters_hash(b"") != rt.clusters_hash(b"\0")
     assert rt.clusters_hash(b"abcdefgh") != rt.clusters_hash(b"abcdefgi")
+
+
+@pytest.mark.skipif(ob.ref_lib() is None, reason="oracle/_ref not built: no reference tree here")
+def test_hash_equals_the_reference():
+    ref = ob.ref_lib()
+    rng = np.random.default_rng(2)
+    for n in list(range(0, 70)) + [127, 128, 129, 4096, 100003]:
+        data = rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
+        assert rt.clusters_hash(data) == ref.ge_ref_hash_bytes(data, n), n
 
 
 @pytest.fixture(scope="module")
