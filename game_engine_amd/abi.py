@@ -179,6 +179,7 @@ FORMAT_RGBA8_SRGB = 43
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
 ADDRESS_REPEAT, ADDRESS_MIRRORED_REPEAT, ADDRESS_CLAMP_TO_EDGE, ADDRESS_CLAMP_TO_BORDER = 0, 1, 2, 3
 SHADOW_NOT_TAKEN = 0xFFFFFFFF
+SHADOW_STATIC_ENTRIES = 1 << 21  # the allocation-free shadow resolve's most entries
 
 
 def shadow_entries(params):

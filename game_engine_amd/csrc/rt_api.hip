@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -19,6 +20,33 @@ struct ge_rt_cache {
 };
 
 namespace {
+
+// ---- the allocation-free shadow resolve's working memory ---------------------------------------------
+// One array per GPU, part of the library's code object: the walk's ray counter in the first 256
+// bytes, then GE_RT_SHADOW_STATIC_ENTRIES 16-byte ray records. Host code never uses the array's
+// name as an address: each device's copy sits at its own address, looked up once per device with
+// that device current (scratch_address) and passed to the kernels from there.
+constexpr size_t k_scratch_head = 256;
+constexpr size_t k_scratch_bytes = k_scratch_head + size_t(GE_RT_SHADOW_STATIC_ENTRIES) * 16;
+static __device__ __align__(256) uint8_t scratch[k_scratch_bytes];
+
+std::mutex g_scratch_mutex;
+std::vector<uint8_t*> g_scratch_ptr;  // per device ordinal; null until looked up
+
+// The address of `device`'s copy of the array; the caller has made `device` current.
+hipError_t scratch_address(int device, uint8_t** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr.size() <= size_t(device)) g_scratch_ptr.resize(size_t(device) + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = static_cast<uint8_t*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 int32_t set_error(ge_rt_ctx* ctx, int32_t code, const std::string& msg) {
   if (ctx) {
@@ -185,6 +213,48 @@ uint32_t alpha_rule(const SceneDev& s, uint32_t flags) {
   return (flags & GE_RT_CULL_MASKED) ? k_alpha_cull : k_alpha_opaque;
 }
 
+// The engine's pair numbering, pair -> (instance, cluster within its mesh): instance i owns pairs
+// [first_pair, first_pair + its clusters). Rebuilt by every structure build after an upload or an
+// instance update, so that a shadow resolve finds it current; a numbering the table cannot hold
+// (two instances claiming one pair, pairs past 2^24) is kept as `pairs_error` for the resolve to
+// report. Synchronous.
+hipError_t refresh_pairs(SceneDev& s) {
+  s.pairs_error.clear();
+  s.pair_count = 0;
+  uint64_t count = 0;
+  for (const ge_instance_desc& in : s.h_instances)
+    count = std::max<uint64_t>(count, uint64_t(in.first_pair) + s.h_meshes[in.mesh].cluster_count);
+  if (count > (1u << 24)) {
+    s.pairs_error = "pairs past 2^24 do not fit a visibility word";
+    s.pairs_dirty = false;
+    return hipSuccess;
+  }
+  std::vector<uint32_t> table(size_t(count) * 2, GE_RT_MISS);
+  for (uint32_t i = 0; i < s.instance_count && s.pairs_error.empty(); ++i) {
+    const ge_instance_desc& in = s.h_instances[i];
+    for (uint32_t c = 0; c < s.h_meshes[in.mesh].cluster_count; ++c) {
+      uint32_t* e = &table[(size_t(in.first_pair) + c) * 2];
+      if (e[0] != GE_RT_MISS) {
+        s.pairs_error = "instances " + std::to_string(e[0]) + " and " + std::to_string(i) + " claim pair " +
+                        std::to_string(in.first_pair + c);
+        break;
+      }
+      e[0] = i;
+      e[1] = c;
+    }
+  }
+  s.pairs_dirty = false;
+  if (!s.pairs_error.empty()) return hipSuccess;
+  hipError_t e = s.pairs.reserve(std::max<size_t>(table.size() * 4, 8));
+  if (e == hipSuccess && !table.empty()) e = hipMemcpy(s.pairs.ptr, table.data(), table.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    s.pairs_dirty = true;
+    return e;
+  }
+  s.pair_count = uint32_t(count);
+  return hipSuccess;
+}
+
 int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_desc* in, uint32_t count) {
   for (uint32_t i = 0; i < count; ++i) {
     const ge_instance_desc& inst = in[i];
@@ -231,6 +301,10 @@ int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal) {
         ge_rt_destroy(ctx);
         return GE_RT_ERROR_DEVICE;
       }
+  if (scratch_address(device_ordinal, &ctx->static_scratch) != hipSuccess) {
+    ge_rt_destroy(ctx);
+    return GE_RT_ERROR_DEVICE;
+  }
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_ordinal) == hipSuccess && cus > 0)
     ctx->persistent_blocks = uint32_t(cus) * 32u;
@@ -624,6 +698,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
            "the structures are " + std::to_string(s.tlas_height_value) + " + " + std::to_string(blas_max) +
                " levels deep, past the traversal stack of " + std::to_string(k_stack));
   s.stack_cap = need;
+  if (s.pairs_dirty) GE_HIP(ctx, refresh_pairs(s));
   s.built = true;
   ++s.version;
   return GE_RT_OK;
@@ -703,6 +778,23 @@ int32_t ready(ge_rt_ctx* ctx, const ge_rt_scene* scene) {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene && scene->dev.built, GE_RT_ERROR_INVALID_SCENE, "no structure built: call ge_rt_bvh_build");
   GE_HIP(ctx, hipSetDevice(ctx->device));
+  return GE_RT_OK;
+}
+
+// What both shadow resolves check before they touch the device: the parameters, the extent, and
+// the pointers; `pixels` and `entries` receive the visibility buffer's size and the answers'.
+int32_t resolve_checks(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
+                       const uint64_t* visibility, const uint32_t* out, uint64_t* pixels, uint64_t* entries) {
+  int32_t r = ready(ctx, scene);
+  if (r != GE_RT_OK) return r;
+  GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null parameters");
+  GE_CHECK(ctx, (params->flags & ~(GE_SHADOW_HALF_RATE | GE_SHADOW_JITTER | GE_SHADOW_ALPHA_MASKS)) == 0,
+           GE_RT_ERROR_UNSUPPORTED, "unknown shadow resolve flags");
+  *pixels = uint64_t(params->width) * params->height;
+  const bool half = params->flags & GE_SHADOW_HALF_RATE;
+  *entries = half ? uint64_t((params->width + 1) / 2) * ((params->height + 1) / 2) : *pixels;
+  GE_CHECK(ctx, *pixels < (1ull << 31), GE_RT_ERROR_INVALID_ARGUMENT, "more than 2^31 pixels");
+  GE_CHECK(ctx, (visibility && out) || *entries == 0, GE_RT_ERROR_INVALID_ARGUMENT, "null visibility or output");
   return GE_RT_OK;
 }
 
@@ -1105,42 +1197,12 @@ int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_sha
 
 int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
                              const uint64_t* visibility, uint32_t* out, uint32_t flags) {
-  int32_t r = ready(ctx, scene);
-  if (r != GE_RT_OK) return r;
-  GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null parameters");
-  GE_CHECK(ctx, (params->flags & ~(GE_SHADOW_HALF_RATE | GE_SHADOW_JITTER | GE_SHADOW_ALPHA_MASKS)) == 0,
-           GE_RT_ERROR_UNSUPPORTED, "unknown shadow resolve flags");
-  const uint64_t pixels = uint64_t(params->width) * params->height;
-  const bool half = params->flags & GE_SHADOW_HALF_RATE;
-  const uint64_t entries = half ? uint64_t((params->width + 1) / 2) * ((params->height + 1) / 2) : pixels;
-  GE_CHECK(ctx, pixels < (1ull << 31), GE_RT_ERROR_INVALID_ARGUMENT, "more than 2^31 pixels");
-  GE_CHECK(ctx, (visibility && out) || entries == 0, GE_RT_ERROR_INVALID_ARGUMENT, "null visibility or output");
-  if (entries == 0) return GE_RT_OK;
+  uint64_t pixels = 0, entries = 0;
+  int32_t r = resolve_checks(ctx, scene, params, visibility, out, &pixels, &entries);
+  if (r != GE_RT_OK || entries == 0) return r;
   SceneDev& s = const_cast<SceneDev&>(scene->dev);
-  if (s.pairs_dirty) {
-    // The engine's pair numbering: instance i owns pairs [first_pair, first_pair + its clusters).
-    uint64_t count = 0;
-    for (const ge_instance_desc& in : s.h_instances)
-      count = std::max<uint64_t>(count, uint64_t(in.first_pair) + s.h_meshes[in.mesh].cluster_count);
-    GE_CHECK(ctx, count <= (1u << 24), GE_RT_ERROR_INVALID_SCENE, "pairs past 2^24 do not fit a visibility word");
-    std::vector<uint32_t> table(size_t(count) * 2, GE_RT_MISS);
-    for (uint32_t i = 0; i < s.instance_count; ++i) {
-      const ge_instance_desc& in = s.h_instances[i];
-      for (uint32_t c = 0; c < s.h_meshes[in.mesh].cluster_count; ++c) {
-        uint32_t* e = &table[(size_t(in.first_pair) + c) * 2];
-        GE_CHECK(ctx, e[0] == GE_RT_MISS, GE_RT_ERROR_INVALID_SCENE,
-                 "instances " + std::to_string(e[0]) + " and " + std::to_string(i) + " claim pair " +
-                     std::to_string(in.first_pair + c));
-        e[0] = i;
-        e[1] = c;
-      }
-    }
-    GE_HIP(ctx, s.pairs.reserve(std::max<size_t>(table.size() * 4, 8)));
-    if (!table.empty())
-      GE_HIP(ctx, hipMemcpy(s.pairs.ptr, table.data(), table.size() * 4, hipMemcpyHostToDevice));
-    s.pair_count = uint32_t(count);
-    s.pairs_dirty = false;
-  }
+  if (s.pairs_dirty) GE_HIP(ctx, refresh_pairs(s));
+  GE_CHECK(ctx, s.pairs_error.empty(), GE_RT_ERROR_INVALID_SCENE, s.pairs_error);
   const bool dev = flags & GE_RT_DEVICE_PTR;
   const uint64_t* dw = visibility;
   uint32_t* dout = out;
@@ -1163,6 +1225,30 @@ int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+}
+
+int32_t ge_rt_shadow_resolve_noalloc(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
+                                     const uint64_t* visibility, uint32_t* out, uint32_t flags) {
+  uint64_t pixels = 0, entries = 0;
+  int32_t r = resolve_checks(ctx, scene, params, visibility, out, &pixels, &entries);
+  if (r != GE_RT_OK || entries == 0) return r;
+  GE_CHECK(ctx, flags & GE_RT_DEVICE_PTR, GE_RT_ERROR_INVALID_ARGUMENT,
+           "the allocation-free shadow resolve takes device pointers only (GE_RT_DEVICE_PTR)");
+  GE_CHECK(ctx, entries <= GE_RT_SHADOW_STATIC_ENTRIES, GE_RT_ERROR_INVALID_ARGUMENT,
+           std::to_string(entries) + " entries exceed the allocation-free shadow resolve's " +
+               std::to_string(GE_RT_SHADOW_STATIC_ENTRIES));
+  const SceneDev& s = scene->dev;
+  GE_CHECK(ctx, !s.pairs_dirty, GE_RT_ERROR_INVALID_SCENE,
+           "instances changed since the last ge_rt_bvh_build, which rebuilds the pair table");
+  GE_CHECK(ctx, s.pairs_error.empty(), GE_RT_ERROR_INVALID_SCENE, s.pairs_error);
+  GE_CHECK(ctx, ctx->static_scratch, GE_RT_ERROR_DEVICE, "no allocation-free working memory on this device");
+  GE_HIP(ctx, begin_timing(ctx));
+  GE_HIP(ctx, launch_shadow_resolve(ctx, bvh_view(s), scene_view(s), s.eye, *params, visibility, s.pairs.as<uint2>(),
+                                    s.pair_count, reinterpret_cast<float4*>(ctx->static_scratch + k_scratch_head), out,
+                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
+                                    reinterpret_cast<uint32_t*>(ctx->static_scratch), ctx->persistent_blocks,
+                                    alpha_rule(s, (params->flags & GE_SHADOW_ALPHA_MASKS) ? GE_RT_ALPHA_MASKS : 0u)));
+  return finish(ctx, false);
 }
 
 // ---- the radiance cache ----------------------------------------------------------------------------

@@ -74,6 +74,7 @@ struct SceneDev {
   DevBuf pairs;
   uint32_t pair_count = 0;
   bool pairs_dirty = true;
+  std::string pairs_error;  // why the pair table could not be built (empty: it is current)
   ge_frame_eye eye{};
   // Bumped by every change the traced image can see (materials, lights, instances, structures).
   uint64_t version = 0;
@@ -182,6 +183,9 @@ struct ge_rt_ctx {
   ge::Wavefront wave;
   ge::PrimalKey primal;
   ge::DevBuf resolve_rays;  // the visibility-buffer shadow pass's ray records
+  // The device address of this device's allocation-free working array (the shadow resolve's
+  // counter, then GE_RT_SHADOW_STATIC_ENTRIES ray records), looked up once per device at creation.
+  uint8_t* static_scratch = nullptr;
   uint32_t persistent_blocks = 8192;  // one-wave workgroups a persistent walk launches: CUs x 32
 };
 

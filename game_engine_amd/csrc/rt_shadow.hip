@@ -27,6 +27,7 @@ struct ReceiverArgs {
   uint32_t cols, n;    // entries a row, entries
   float4* rays;
   uint32_t* out;
+  uint32_t* head;      // the walk's ray counter, cleared here for the launch after this one
 };
 
 // The receiver of pixel (x, y) showing visibility word `word`; false when it asks no ray.
@@ -63,6 +64,7 @@ __device__ bool receiver(const ReceiverArgs& a, uint32_t x, uint32_t y, uint64_t
 template <bool Half>
 __global__ void __launch_bounds__(256) k_receivers(ReceiverArgs a) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) *a.head = 0u;  // a kernel's store, not a memset: the walk after this launch starts from 0
   if (i >= a.n) return;
   const uint32_t bx = i % a.cols, by = i / a.cols;
   const uint32_t W = a.p.width, H = a.p.height;
@@ -136,10 +138,10 @@ hipError_t launch_shadow_resolve(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView
   a.n = uint32_t(n);
   a.rays = rays;
   a.out = out;
+  a.head = head;
   if (half) k_receivers<true><<<blocks(n, 256), 256, 0, stream>>>(a);
   else k_receivers<false><<<blocks(n, 256), 256, 0, stream>>>(a);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemsetAsync(head, 0, 4, stream);
   if (e != hipSuccess) return e;
   const uint32_t grid = std::min(blocks(n, 64), max_blocks);
   const size_t lds = size_t(k_ring) * k_lanes * 4;

@@ -64,6 +64,7 @@ def lib():
             "ge_rt_tonemap": ([vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, u32], i32),
             "ge_rt_shadow_pass": ([vp, vp, C.POINTER(abi.ShadowParams), vp, vp, u32], i32),
             "ge_rt_shadow_resolve": ([vp, vp, C.POINTER(abi.ShadowResolveParams), vp, vp, u32], i32),
+            "ge_rt_shadow_resolve_noalloc": ([vp, vp, C.POINTER(abi.ShadowResolveParams), vp, vp, u32], i32),
             "ge_rt_scene_update_materials": ([vp, vp, u32, u32, vp], i32),
             "ge_rt_scene_update_lights": ([vp, vp, u32, u32, vp], i32),
             "ge_rt_path_adjoint": ([vp, vp, C.POINTER(abi.PathTraceParams), vp, vp, vp, vp, vp, u32], i32),
@@ -97,7 +98,8 @@ def lib():
 EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchronize", "ge_rt_scene_upload",
            "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_bvh_info",
            "ge_rt_bvh_download", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
-           "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve", "ge_rt_stats",
+           "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve",
+           "ge_rt_shadow_resolve_noalloc", "ge_rt_stats",
            "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint",
            "ge_rt_cache_create", "ge_rt_cache_destroy", "ge_rt_cache_weights", "ge_rt_cache_infer",
            "ge_rt_cache_train", "ge_rt_cache_collect", "ge_rt_cache_update", "ge_rt_path_trace_cached",

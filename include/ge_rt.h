@@ -560,6 +560,22 @@ int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_sha
 int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
                              const uint64_t* visibility, uint32_t* out, uint32_t flags);
 
+/* The allocation-free shadow resolve. No counterpart in the reference: it is ge_rt_shadow_resolve for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight per-frame
+ * loops, harnesses that forbid hipMalloc). Its working memory, the walk's ray counter and a 16-byte
+ * ray record for each entry, lives in a fixed array the library holds on each GPU it runs on (256
+ * bytes + GE_RT_SHADOW_STATIC_ENTRIES records, 32 MiB, kept until the process ends) instead of the
+ * context's growable buffer, and it calls no hipMalloc, hipFree or other allocating routine. The
+ * answers are ge_rt_shadow_resolve's. Device pointers only (flags must hold GE_RT_DEVICE_PTR);
+ * more than GE_RT_SHADOW_STATIC_ENTRIES entries return GE_RT_ERROR_INVALID_ARGUMENT; the scene's
+ * pair table must be current, as every ge_rt_bvh_build leaves it (instances updated since the last
+ * build return GE_RT_ERROR_INVALID_SCENE). Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (contexts on different streams) race; calls on one context are ordered
+ * by its stream. */
+#define GE_RT_SHADOW_STATIC_ENTRIES (1u << 21)
+int32_t ge_rt_shadow_resolve_noalloc(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
+                                     const uint64_t* visibility, uint32_t* out, uint32_t flags);
+
 /* ---- The radiance cache ---------------------------------------------------------------------------
  * No counterpart in the reference: its GI cache is a world-space hash grid filled by a compute pass
  * (domain/gfx/include/domain/gfx/gi_cache.h:1-30, domain/gfx/shaders/gi_update.slang:85-115). Here

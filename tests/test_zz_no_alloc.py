@@ -1,0 +1,103 @@
+"""The allocation-free shadow resolve (ge_rt_shadow_resolve_noalloc) against ge_rt_shadow_resolve.
+
+Called through the C ABI with device pointers, on two calls in a row (the second smaller than the
+first, so the fixed working array still holds the first call's records), full rate, half rate and
+jittered, on the 4M-triangle layout at 512 instances and the configuration-3 shadow scene: every
+answer bit-exact against the allocating entry point's, and the walk counters equal. A call past
+the array's limit, with host pointers, or after an instance update without a build is refused.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from game_engine_amd import abi, rt, synth
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = rt.Context(0)
+    yield c
+    c.close()
+
+
+def _resolve(ctx, scene, params, words, noalloc, count=False):
+    """The answers as a host array, through the C ABI with device pointers."""
+    out = torch.full((abi.shadow_entries(params),), -1, dtype=torch.int32, device="cuda")
+    fn = rt.lib().ge_rt_shadow_resolve_noalloc if noalloc else rt.lib().ge_rt_shadow_resolve
+    flags = abi.DEVICE_PTR | (abi.COUNT if count else 0)
+    ctx.check(fn(ctx.h, scene.h, C.byref(params), words.data_ptr(), out.data_ptr(), flags))
+    ctx.synchronize()
+    return out.cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("config", [2, 3], ids=["c2", "c3"])
+def test_noalloc_equals_the_allocating_resolve(ctx, config):
+    s = synth.Synth.config(2, 0, 512) if config == 2 else synth.Synth.config(3, 0, 48, 129)
+    print(f"[scene] configuration {config}: {s.triangle_count} triangles", flush=True)
+    g = ctx.upload(s.desc()).build()
+    o = ob.Scene(s.desc())
+    for (w, h) in ((320, 200), (96, 64)):  # the second call is smaller than the first
+        pt, rv = s.frame(w, h)
+        words = torch.from_numpy(o.ray_visibility(rv).view(np.int64)).cuda()
+        for flags in (0, abi.SHADOW_HALF_RATE, abi.SHADOW_JITTER):
+            p = synth.Synth.shadow_params(pt, rv, flags, 3)
+            print(f"[case] {w}x{h} flags {flags}", flush=True)
+            ctx.reset_stats()
+            want = _resolve(ctx, g, p, words, False, count=True)
+            want_stats = ctx.stats()
+            ctx.reset_stats()
+            got = _resolve(ctx, g, p, words, True, count=True)
+            got_stats = ctx.stats()
+            assert np.count_nonzero(want != abi.SHADOW_NOT_TAKEN) > 0
+            assert np.array_equal(got, want)
+            for k in ("any", "nodes", "triangles", "instances"):
+                assert got_stats[k] == want_stats[k], k
+    g.close()
+
+
+def test_noalloc_refusals(ctx):
+    s = synth.Synth.config(1, 0)
+    g = ctx.upload(s.desc()).build()
+    pt, rv = s.frame(64, 64)
+    p = synth.Synth.shadow_params(pt, rv)
+    words = torch.zeros(64 * 64, dtype=torch.int64, device="cuda")
+    out = torch.zeros(64 * 64, dtype=torch.int32, device="cuda")
+    L = rt.lib()
+    # Host pointers: refused (staging them would allocate).
+    host_words = np.zeros(64 * 64, dtype=np.uint64)
+    host_out = np.zeros(64 * 64, dtype=np.uint32)
+    st = L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(p), host_words.ctypes.data, host_out.ctypes.data, 0)
+    assert st == 1
+    # Past the fixed array: refused before any launch.
+    big = synth.Synth.shadow_params(*s.frame(2048, 1025), 0, 0)
+    assert abi.shadow_entries(big) > abi.SHADOW_STATIC_ENTRIES
+    big_words = torch.zeros(2048 * 1025, dtype=torch.int64, device="cuda")
+    big_out = torch.zeros(2048 * 1025, dtype=torch.int32, device="cuda")
+    st = L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(big), big_words.data_ptr(), big_out.data_ptr(),
+                                        abi.DEVICE_PTR)
+    assert st == 1
+    buf = C.create_string_buffer(256)
+    L.ge_rt_last_error(ctx.h, buf, 256)
+    assert b"exceed" in buf.value
+    # At half rate the same frame fits.
+    big.flags = abi.SHADOW_HALF_RATE
+    assert L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(big), big_words.data_ptr(), big_out.data_ptr(),
+                                          abi.DEVICE_PTR) == 0
+    ctx.synchronize()
+    answers = big_out.cpu().numpy().view(np.uint32)[: abi.shadow_entries(big)]
+    assert np.all(answers == abi.SHADOW_NOT_TAKEN)  # empty words ask no ray
+    # Instances updated without a build: the pair table is stale.
+    inst = s.arrays()["instances"]
+    recs = (abi.InstanceDesc * 1).from_buffer_copy(inst[:1].tobytes())
+    g.update_instances(0, recs)
+    st = L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(p), words.data_ptr(), out.data_ptr(), abi.DEVICE_PTR)
+    assert st == 2
+    g.build(abi.BUILD_TOP)
+    assert L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(p), words.data_ptr(), out.data_ptr(), abi.DEVICE_PTR) == 0
+    ctx.synchronize()
+    g.close()
