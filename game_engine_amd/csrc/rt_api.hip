@@ -2,6 +2,7 @@
 // orchestration (every mesh's LBVH in one segmented batch, then the instance level), and the
 // traced passes. Errors are a status and a per-context message, never an exception across the ABI.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -47,6 +48,11 @@ hipError_t scratch_address(int device, uint8_t** out) {
   *out = g_scratch_ptr[device];
   return hipSuccess;
 }
+
+// Scene versions come from one process-wide counter that only goes up, so a version names one
+// state of one scene: a scene destroyed and another uploaded at the same address never share one.
+std::atomic<uint64_t> g_version{0};
+uint64_t next_version() { return g_version.fetch_add(1, std::memory_order_relaxed) + 1; }
 
 int32_t set_error(ge_rt_ctx* ctx, int32_t code, const std::string& msg) {
   if (ctx) {
@@ -370,6 +376,7 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
            GE_RT_ERROR_INVALID_ARGUMENT, "a scene array is null with a non-zero count");
   GE_HIP(ctx, hipSetDevice(ctx->device));
   auto* scene = new (std::nothrow) ge_rt_scene();
+  if (scene) scene->dev.version = next_version();
   GE_CHECK(ctx, scene, GE_RT_ERROR_OUT_OF_MEMORY, "host allocation");
   SceneDev& s = scene->dev;
   s.cluster_count = d->cluster_count;
@@ -515,6 +522,7 @@ void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
   if (ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->primal.scene == scene) ctx->primal.valid = false;
   }
   SceneDev& s = scene->dev;
   for (DevBuf* b : {&s.clusters, &s.vertices, &s.quantized, &s.attributes, &s.triangles, &s.meshes, &s.instances,
@@ -537,7 +545,7 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   if (st != GE_RT_OK) return st;
   std::memcpy(&s.h_instances[first], instances, size_t(count) * sizeof(ge_instance_desc));
   s.pairs_dirty = true;
-  ++s.version;
+  s.version = next_version();
   GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, instances, size_t(count) * sizeof(ge_instance_desc),
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -700,7 +708,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   s.stack_cap = need;
   if (s.pairs_dirty) GE_HIP(ctx, refresh_pairs(s));
   s.built = true;
-  ++s.version;
+  s.version = next_version();
   return GE_RT_OK;
 }
 
@@ -1002,7 +1010,7 @@ int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   if (!count) return GE_RT_OK;
   GE_HIP(ctx, hipSetDevice(ctx->device));
   if (std::memcmp(&s.h_materials[first], materials, size_t(count) * sizeof(ge_resolve_material)) == 0) return GE_RT_OK;
-  ++s.version;
+  s.version = next_version();
   std::memcpy(&s.h_materials[first], materials, size_t(count) * sizeof(ge_resolve_material));
   GE_HIP(ctx, hipMemcpyAsync(s.materials.as<ge_resolve_material>() + first, materials,
                              size_t(count) * sizeof(ge_resolve_material), hipMemcpyHostToDevice, ctx->stream));
@@ -1020,7 +1028,7 @@ int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t f
   if (!count) return GE_RT_OK;
   GE_HIP(ctx, hipSetDevice(ctx->device));
   if (std::memcmp(&s.h_lights[first], lights, size_t(count) * sizeof(ge_resolve_light)) == 0) return GE_RT_OK;
-  ++s.version;
+  s.version = next_version();
   std::memcpy(&s.h_lights[first], lights, size_t(count) * sizeof(ge_resolve_light));
   GE_HIP(ctx, hipMemcpyAsync(s.lights.as<ge_resolve_light>() + first, lights, size_t(count) * sizeof(ge_resolve_light),
                              hipMemcpyHostToDevice, ctx->stream));

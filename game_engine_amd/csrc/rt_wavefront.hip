@@ -186,7 +186,8 @@ struct ShadeArgs {
   // The adjoint (kShadeAdjoint): sampling table, per-path sampling throughput, and with grad the
   // per-record and per-vertex gradient terms.
   const ge_resolve_material* sampling;  // null: the scene's materials
-  float4* ts;
+  const float4* ts_in;  // the sampling throughput, double-buffered as the path state is
+  float4* ts_out;
   bool grad;
   const float4* d_image;
   uint32_t pixels;
@@ -195,6 +196,9 @@ struct ShadeArgs {
   float4* adj_a;
   float4* adj_r;
   double* grad_lights;
+  // A launch run again after its shadow records overflowed: its one non-idempotent effect, the
+  // workgroup's ground term added to grad_lights[0], was made by the first run and is not repeated.
+  bool rerun;
 };
 
 enum : int {
@@ -277,7 +281,7 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
     f3 L = f3{st.L[0], st.L[1], st.L[2]};
     const f3 T = f3{st.T[0], st.T[1], st.T[2]};
     if (Mode == kShadeAdjoint && st.bounce != 0) {
-      const float4 t4 = a.ts[pid];
+      const float4 t4 = a.ts_in[pid];
       Ts = f3{t4.x, t4.y, t4.z};
     }
     if (grad) {
@@ -559,13 +563,13 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   if (valid) {
     if (!alive) st.flags |= 2u;
     a.out[pid] = st;
-    if (Mode == kShadeAdjoint && alive) a.ts[pid] = make_float4(Ts.x, Ts.y, Ts.z, 0.0f);
+    if (Mode == kShadeAdjoint && alive) a.ts_out[pid] = make_float4(Ts.x, Ts.y, Ts.z, 0.0f);
   }
   const uint32_t slot = block_append(&a.counts[0], alive ? 1u : 0u, parts[1]);
   if (alive) a.next_queue[slot] = pid;
   if (grad) {
     __syncthreads();
-    if (threadIdx.x == 0 && ground_acc != 0.0) atomicAdd(&a.grad_lights[0], ground_acc);
+    if (threadIdx.x == 0 && ground_acc != 0.0 && !a.rerun) atomicAdd(&a.grad_lights[0], ground_acc);
   }
 }
 
@@ -746,7 +750,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   if (!w.host_counts && (e = hipHostMalloc(reinterpret_cast<void**>(&w.host_counts), 64)) != hipSuccess) return e;
   const bool grad = adjoint && adjoint->grad;
   if (adjoint) {
-    if ((e = w.ts.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
+    if ((e = w.ts.reserve(size_t(npaths) * 32)) != hipSuccess) return e;  // two buffers, as the state
     if (grad) {
       if ((e = w.adj_a.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
       if ((e = w.adj_r.reserve(size_t(npaths) * 16)) != hipSuccess) return e;
@@ -771,6 +775,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   if ((e = hipGetLastError()) != hipSuccess) return e;
   int cur = 0;
   uint32_t n = npaths;  // every path starts live
+  uint32_t cache_before[2] = {0u, 0u};  // the cache's query and sample counts before this bounce's shade
   uint32_t* c2 = counts + 2;
   // Every live path is at the same bounce: the loop's iteration. The walks' rule for masked
   // candidates follows it (path_trace.slang:278-292), and the shading's textured form the scene.
@@ -813,7 +818,8 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     a.result = w.result.as<float4>();
     if (adjoint) {
       a.sampling = adjoint->sampling;
-      a.ts = w.ts.as<float4>();
+      a.ts_in = w.ts.as<float4>() + size_t(cur) * npaths;
+      a.ts_out = w.ts.as<float4>() + size_t(cur ^ 1) * npaths;
       a.grad = grad;
       a.d_image = adjoint->d_image;
       a.pixels = pixels;
@@ -836,6 +842,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       a.box = hook->box;
     }
     uint32_t nshadow = 0, next = 0;
+    a.rerun = false;
     for (;;) {
       a.shadow = w.shadow.as<ShadowRec>();
       a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
@@ -853,16 +860,26 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
         full ? k_shade<kShadeRecord, true><<<sg, k_shade_threads, 0, st>>>(a)
              : k_shade<kShadeRecord><<<sg, k_shade_threads, 0, st>>>(a);
       if ((e = hipGetLastError()) != hipSuccess) return e;
-      if ((e = hipMemcpyAsync(w.host_counts, c2, 8, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+      // The next queue's and the shadow records' counts, and the cache's query and sample counts.
+      if ((e = hipMemcpyAsync(w.host_counts, c2, 24, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
       next = w.host_counts[0];
       nshadow = w.host_counts[1];
-      if (nshadow <= a.shadow_capacity) break;
+      if (nshadow <= a.shadow_capacity) {
+        cache_before[0] = w.host_counts[4];
+        cache_before[1] = w.host_counts[5];
+        break;
+      }
       // The records did not fit: grow the buffer and shade this bounce again (idempotent: the
-      // launch reads one state buffer and writes the other).
+      // launch reads one state buffer and writes the other), from the counts it started with: the
+      // queue and shadow counts at zero and the cache's counts as the previous bounce left them.
       if ((e = grow_shadow(uint64_t(nshadow) + nshadow / 4)) != hipSuccess) return e;
       if (grad && (e = w.sgrad.reserve(w.shadow_capacity * sizeof(ShadowGrad))) != hipSuccess) return e;
       if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;
+      w.host_counts[6] = cache_before[0];
+      w.host_counts[7] = cache_before[1];
+      if ((e = hipMemcpyAsync(counts + 6, w.host_counts + 6, 8, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+      a.rerun = true;
     }
     if (nshadow > 0) {
       const uint32_t grid = std::min(blocks(nshadow, 64), ctx->persistent_blocks);

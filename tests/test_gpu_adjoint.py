@@ -126,6 +126,29 @@ def test_reused_primal_gives_the_same_gradient(c4):
     g.update_materials(0, rec)
 
 
+def test_reused_primal_never_outlives_its_scene(ctx):
+    # A scene traced, destroyed, and a scene of other materials uploaded in its place (possibly at
+    # the same address, with the same number of builds): the adjoint must trace its own primal.
+    s = synth.Synth.config(4, 0)
+    pt, _ = s.frame(48, 48)
+    pt.spp = 2
+    d = _d_image(pt.width * pt.height, 6)
+    first = ctx.upload(s.desc()).build()
+    first.path_trace(pt)
+    first.close()
+    desc = abi.SceneDesc.from_buffer_copy(s.desc())
+    mats = (abi.ResolveMaterial * desc.material_count)()
+    C.memmove(mats, desc.materials, C.sizeof(mats))
+    for m in mats:
+        m.albedo[1] = m.albedo[1] * 0.5
+    desc.materials = C.addressof(mats)
+    second = ctx.upload(desc).build()
+    _, rm, rl = second.path_adjoint(pt, d, image=False, reuse_primal=True)
+    _, fm, fl = second.path_adjoint(pt, d)
+    assert _same_sums(rm, fm) and _same_sums(rl, fl)
+    second.close()
+
+
 @KERNELS
 def test_gradients_match_central_differences(ctx, c4, mega):
     s, g, o, pt = c4

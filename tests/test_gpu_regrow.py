@@ -1,0 +1,218 @@
+"""Shading launches run again after their shadow records overflow, in every wavefront mode.
+
+A fresh context's first shadow buffer holds two records a path; with the sun and 8 directional
+lights a vertex asks up to nine, so the first shade launch of a dispatch overflows and is run
+again into a grown buffer. That re-run must leave nothing twice: the cached tracer's queries and
+samples (their counts are restored), the adjoint's ground term (added by the first run alone) and
+its per-path sampling throughput (double-buffered). Each dispatch is made on a fresh context and
+checked against the oracle, to the bars of the mode's own parity tests, and against the same
+dispatch on a context whose buffer has already grown.
+
+Lights past the 64th: the record-writing pass visits lights 0..63 through the counting pass's
+64-bit mask and tests the rest in full. A scene with 80 point lights, several past index 64 in
+range of visible surfaces, is checked against the oracle in plain and adjoint modes.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from game_engine_amd import abi, rt, synth
+from oracle import binding as ob
+from tests import cache_reference as ref
+from tests.test_gpu_cache import _by_sample
+
+pytestmark = pytest.mark.gpu
+
+
+def _eight_lights(s):
+    lights = (abi.ResolveLight * 8)()
+    rng = np.random.default_rng(2)
+    for l in lights:
+        d = rng.normal(size=3)
+        d[1] = abs(d[1]) + 0.5
+        l.position_radius[:] = [*d, 0.0]
+        l.color_intensity[:] = [1.0, 0.9, 0.8, 0.5]
+        l.direction_type[:] = [0.0, 0.0, 0.0, 2.0]
+    desc = abi.SceneDesc.from_buffer_copy(s.desc())
+    desc.lights = C.addressof(lights)
+    desc.light_count = 8
+    return desc, lights
+
+
+@pytest.fixture(scope="module")
+def scene():
+    s = synth.Synth.config(1, 0)
+    desc, lights = _eight_lights(s)
+    o = ob.Scene(desc)
+    lo, hi = s.bounds()
+    print(f"[scene] {s.triangle_count} triangles, 8 directional lights and the sun", flush=True)
+    yield s, desc, lights, o, lo, hi
+
+
+def _frame(s, w=64, h=48, spp=2, bounces=4):
+    pt, _ = s.frame(w, h)
+    pt.spp, pt.max_bounces, pt.light_count = spp, bounces, 8
+    return pt
+
+
+def _fresh(desc):
+    ctx = rt.Context(0)
+    return ctx, ctx.upload(desc).build()
+
+
+def test_cached_tracer_regrows(scene):
+    s, desc, lights, o, lo, hi = scene
+    pt = _frame(s)
+    n = pt.width * pt.height
+    # A cache that answers zero: the oracle's cached tracer to the bit.
+    ctx, g = _fresh(desc)
+    silent = rt.RadianceCache(ctx, lo, hi, seed=0)
+    silent.set_weights(np.zeros(abi.CACHE_WEIGHTS, dtype=np.float32))
+    sacc, _ = silent.path_trace(g, pt, depth=1)
+    rad, wt, q = o.path_cached(pt, 1, silent.box_min, silent.box_inv())
+    assert np.array_equal(sacc[:, :3], _by_sample(rad, n, pt.spp))
+    silent.close(), g.close(), ctx.close()
+    # A real network: one query a path end, so a fresh and a warm dispatch agree to the bit, and
+    # both are the oracle's radiance plus throughput times the reference network's answer.
+    runs = []
+    for warm in (False, True):
+        ctx, g = _fresh(desc)
+        if warm:
+            g.path_trace(pt)  # grows the shadow buffer first
+        cache = rt.RadianceCache(ctx, lo, hi, seed=9)
+        acc, cov = cache.path_trace(g, pt, depth=1)
+        runs.append((acc, cov, cache.weights()))
+        cache.close(), g.close(), ctx.close()
+    (a0, c0, w0), (a1, c1, w1) = runs
+    assert np.array_equal(c0, c1)
+    assert a0.tobytes() == a1.tobytes()
+    pred = np.maximum(ref.forward(w0, q).numpy().astype(np.float64), 0.0)
+    asked = wt[:, 3:4] > 0
+    want = _by_sample(rad.astype(np.float64) + np.where(asked, wt[:, :3].astype(np.float64) * pred, 0.0), n, pt.spp)
+    weight = _by_sample(np.where(asked, wt[:, :3].astype(np.float64), 0.0), n, pt.spp)
+    scale = max(pred[asked[:, 0]].max(), 1e-3)
+    assert (np.abs(a0[:, :3] - want) <= 3e-2 * scale * weight + 1e-5 * (1 + np.abs(want))).all()
+
+
+def test_self_training_frame_regrows(scene):
+    s, desc, lights, o, lo, hi = scene
+    ctx, g = _fresh(desc)
+    cache = rt.RadianceCache(ctx, lo, hi, seed=6)
+    cache.set_weights(np.zeros(abi.CACHE_WEIGHTS, dtype=np.float32))
+    pt = _frame(s, spp=2, bounces=6)
+    pt.sample_base = 5
+    tp = abi.CacheTrainParams(samples=1 << 16, steps=1, learning_rate=0.0, stride=3)
+    acc, _ = cache.path_trace(g, pt, depth=1, train=tp)
+    v = o.path_cached_training(pt, 1, 3, pt.sample_base % 3, cache.box_min, cache.box_inv())
+    n = pt.width * pt.height
+    assert np.array_equal(acc[:, :3], _by_sample(v["radiance"], n, pt.spp))
+    sampled = v["sample_weight"][:, 3] > 0
+    got = cache.samples()
+    got = got[np.argsort(got["path"])]
+    want = v["samples"][sampled]
+    assert len(got) == sampled.sum() > 0  # each sample once
+    assert np.array_equal(got["path"], want["path"])
+    T1 = v["sample_weight"][:, :3]
+    Rf = v["train_radiance"][sampled]
+    tgt = np.where(T1[sampled] > 0, (Rf - want["radiance"]) / np.where(T1[sampled] > 0, T1[sampled], 1), np.float32(0))
+    assert np.array_equal(got["radiance"], tgt.astype(np.float32))
+    cache.close(), g.close(), ctx.close()
+
+
+def test_recorded_samples_regrow(scene):
+    s, desc, lights, o, lo, hi = scene
+    ctx, g = _fresh(desc)
+    cache = rt.RadianceCache(ctx, lo, hi, seed=0)
+    pt = _frame(s, spp=1, bounces=6)
+    got, count = cache.collect(g, pt, 1 << 16)
+    want, wcount = o.cache_samples(pt, cache.box_min, cache.box_inv())
+    assert count == wcount and len(got) == count > 0
+    key = lambda a: np.lexsort((a["query"]["tag"], a["path"]))  # noqa: E731
+    got, want = got[key(got)], want[key(want)]
+    assert np.array_equal(got["path"], want["path"])
+    for f in ("position", "direction", "normal", "albedo", "roughness", "tag"):
+        assert np.array_equal(got["query"][f], want["query"][f]), f
+    assert np.array_equal(got["radiance"], want["radiance"])
+    cache.close(), g.close(), ctx.close()
+
+
+def _d_image(n, seed):
+    d = np.zeros((n, 4), dtype=np.float32)
+    d[:, :3] = np.random.default_rng(seed).normal(size=(n, 3)).astype(np.float32)
+    return d
+
+
+def _adjoint_check(g, o, pt):
+    n = pt.width * pt.height
+    d = _d_image(n, 1)
+    img, gm, gl = g.path_adjoint(pt, d)
+    oimg, om, ol = o.path_adjoint(pt, d, mode=ob.FP32)
+    assert np.array_equal(img, oimg)
+    scale = max(np.abs(om).max(), np.abs(ol).max())
+    assert np.count_nonzero(om) > 10
+    assert np.abs(gm - om).max() <= 1e-5 * scale
+    assert np.abs(gl - ol).max() <= 1e-5 * scale
+    return gm, gl
+
+
+def test_adjoint_regrows(scene):
+    s, desc, lights, o, lo, hi = scene
+    pt = _frame(s, 48, 48, spp=2, bounces=4)
+    ctx, g = _fresh(desc)
+    fresh = _adjoint_check(g, o, pt)
+    warm = _adjoint_check(g, o, pt)  # the same context, its buffer now grown
+    assert abs(fresh[1][0]) > 0  # the sun's term, which carries the ground's share
+    for a, b in zip(fresh, warm):
+        assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(b).max())
+    g.close(), ctx.close()
+
+
+@pytest.fixture(scope="module")
+def many_lights():
+    """Configuration 1 with 80 point lights near its objects; lights 64..79 sit next to the
+    objects the camera sees, so their records come from the pass's full tests."""
+    s = synth.Synth.config(1, 0)
+    a = s.arrays()
+    cl = a["clusters"]
+    rng = np.random.default_rng(8)
+    lights = (abi.ResolveLight * 80)()
+    centers = cl["center"][rng.choice(len(cl), 80, replace=False)]
+    for i, l in enumerate(lights):
+        p = centers[i] + rng.normal(size=3) * 2.0
+        l.position_radius[:] = [*p, 25.0 if i >= 64 else 12.0]
+        l.color_intensity[:] = [1.0, 0.8, 0.6, 40.0]
+        l.direction_type[:] = [0.0, -1.0, 0.0, 0.0]
+        l.spot[:] = [0.0, 0.0, 0.05, 0.0]
+    desc = abi.SceneDesc.from_buffer_copy(s.desc())
+    desc.lights = C.addressof(lights)
+    desc.light_count = 80
+    yield s, desc, lights
+
+
+def test_more_than_64_lights(many_lights):
+    s, desc, lights = many_lights
+    ctx = rt.Context(0)
+    g = ctx.upload(desc).build()
+    o = ob.Scene(desc)
+    pt, _ = s.frame(96, 64)
+    pt.light_count = 80
+    for megakernel in (False, True):
+        ctx.reset_stats()
+        ob.reset_counters()
+        ga, gc = g.path_trace(pt, megakernel=megakernel, count=True)
+        oa, oc = o.path_trace(pt)
+        assert np.array_equal(gc, oc)
+        rel = np.abs(ga[:, :3] - oa[:, :3]) / np.maximum(np.abs(oa[:, :3]), 1e-6)
+        print(f"80 lights (megakernel={megakernel}): max rel err {rel.max():.3g}", flush=True)
+        assert rel.max() <= 1e-4
+        gs, os_ = ctx.stats(), ob.counters()
+        for k in ("closest", "any", "nodes", "any_nodes", "any_triangles"):
+            assert gs[k] == os_[k], k
+    # The lights past 64 matter: without them the picture changes.
+    pt.light_count = 64
+    fewer, _ = o.path_trace(pt)
+    assert not np.array_equal(fewer, oa)
+    pt.light_count = 80
+    _adjoint_check(g, o, pt)
+    g.close(), ctx.close()
