@@ -13,14 +13,26 @@ value = rays / wall seconds of the K timed frames. The BVH build is timed separa
 
 roofline: the dominant kernel of the frame (the traversal walk with the larger share of device time:
 k_closest, the closest-hit walk of one bounce, or k_any, the any-hit shadow walk of one bounce). Its
-algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 128 B per 4-wide node visited + 48 B per
-triangle tested + 64 B per instance entered, from the counted replay's per-kind counters divided by
-its launches per frame; its mean launch time is the library's own HIP events around each launch on
-its stream, over the timed frames. Peak: MI355X's 8 TB/s. `traffic` is that kernel's measured HBM
-bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, when one is there.
+algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 112 B per 4-wide node visited (the
+seven 16-byte loads of its boxes and references) + 48 B per triangle tested + 64 B per instance
+entered, from the counted replay's per-kind counters divided by its launches per frame; its mean
+launch time is the library's own HIP events around each launch on its stream, over the timed
+frames. Peak: MI355X's 8 TB/s. `traffic` is that kernel's measured HBM bytes per launch from the
+committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, and `valu_busy`, `valu_thread_util` and `l2_hit`
+its vector-issue share, lane utilisation and L2 hit rate from the same summary: the walk is limited
+by vector issue, its node fetches served by the L2.
 
-cpu_baseline: the oracle (oracle/, a scalar C++ port of the same tracer over the same LBVH) on a
-band of rows of the same frame, all host threads, same seeds.
+cpu_baseline: the oracle (oracle/, a scalar C++ port of the same tracer over the same LBVH) on whole
+frames of the same workload (same scene, seeds and sample numbering), all host threads, for about
+--cpu-seconds.
+
+parity: one whole frame of the GPU (samples from warmup x spp) against the oracle's frame of the
+same samples: coverage bit for bit and the largest relative radiance error (bar 1e-4); and the error
+against the FP64 oracle on a band of rows.
+
+time_to_equal_rmse: the other half of the metric on the same scene (see cache_measure): the
+configuration as stated (jittered camera rays, the sun and 64 point lights, RMSE against a 4096-spp
+plain render), and at pixel centres under the sun and sky.
 """
 import argparse
 import json
@@ -33,7 +45,7 @@ sys.path.insert(0, ROOT)
 
 WIDTH, HEIGHT = 1920, 1080
 PEAK_HBM_GBPS = 8000.0
-NODE_BYTES = 128  # a 4-wide node: four child boxes and references, what one node visit reads
+NODE_BYTES = 112  # a 4-wide node visit loads its four child boxes and references: 7 x 16 B
 METRIC = "Mrays/s (closest+any-hit); 1080p time-to-equal-RMSE, cached vs plain tracer"
 
 
@@ -41,12 +53,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if one is there."""
+def read_profile(kernel, figure):
+    """A figure of `kernel` from the committed rocprofv3 PMC summary (lines `figure kernel: value`):
+    traffic_bytes_per_launch (calibrated FETCH_SIZE + WRITE_SIZE), valu_busy (issued VALU cycles over
+    the SIMDs' cycles), valu_thread_util (SQ_THREAD_CYCLES_VALU over 64 x SQ_ACTIVE_INST_VALU),
+    l2_hit (TCC hits over requests); None when the summary or the line is not there."""
     path = os.path.join(ROOT, "profiles", "path_trace_c2_pmc.md")
     if not os.path.exists(path):
         return None
-    key = f"traffic_bytes_per_launch {kernel}:"
+    key = f"{figure} {kernel}:"
     for line in open(path):
         if line.startswith(key):
             return float(line.split(":", 1)[1].strip())
@@ -76,6 +91,36 @@ def cpu_baseline(s, pt, target_seconds, sample_base):
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{frames} whole {WIDTH}x{HEIGHT} frames at {p.spp} spp from sample_base {sample_base}: "
                       f"{rays} rays in {dt:.2f} s on {threads} threads (oracle/ scalar C++ port, OpenMP 8x8 tiles)"}
+
+
+def frame_parity(s, pt, sample_base, gpu_acc, gpu_cov, fp64_rows=(520, 556)):
+    """The GPU's frame of `sample_base` against the oracle's frame of the same samples: coverage
+    bit for bit, radiance relative error per pixel (the FP32 oracle runs the shader's operation
+    order), and the error against the FP64 oracle on a band of rows."""
+    import numpy as np
+
+    from oracle import binding as ob
+    sc = ob.Scene(s.desc())
+    p = type(pt).from_buffer_copy(pt)
+    p.sample_base = sample_base
+    t = time.perf_counter()
+    oa, oc = sc.path_trace(p)
+    t32 = time.perf_counter() - t
+    rel = np.abs(gpu_acc[:, :3] - oa[:, :3]) / np.maximum(np.abs(oa[:, :3]), 1e-6)
+    out = {"sample_base": sample_base, "pixels": int(len(oc)), "coverage_equal": bool(np.array_equal(gpu_cov, oc)),
+           "max_rel_err": float(rel.max()), "bit_exact_pixels": float(np.mean(np.all(gpu_acc == oa, axis=1))),
+           "bar": 1e-4, "oracle_seconds": round(t32, 2)}
+    r0, r1 = fp64_rows
+    t = time.perf_counter()
+    da, _ = sc.path_trace64(p, rows=(r0, r1))
+    band = slice(r0 * WIDTH, r1 * WIDTH)
+    g, d = gpu_acc[band, :3].astype(np.float64), da[band, :3]
+    e = np.abs(g - d) / np.maximum(np.abs(d), 1e-6)
+    out["fp64"] = {"rows": [r0, r1], "rel_l2": float(np.linalg.norm(g - d) / np.linalg.norm(d)),
+                   "median_rel": float(np.median(e)), "p99_rel": float(np.quantile(e, 0.99)),
+                   "within_1e-4": float(np.mean(e <= 1e-4)), "oracle_seconds": round(time.perf_counter() - t, 2)}
+    sc.close()
+    return out
 
 
 SHADOW_METRIC = "Mrays/s (any-hit sun shadow rays, whole frames with 10% of instances rebuilt)"
@@ -327,6 +372,11 @@ CACHE_FLOP_PER_QUERY = 2 * (172 * 64 + 4 * 64 * 64 + 64 * 3)  # the network's al
 
 
 def cache_main(args):
+    """The radiance cache workload as its own JSON line (`--workload cache`)."""
+    print(json.dumps(cache_measure(args, args.jitter, args.point_lights)), flush=True)
+
+
+def cache_measure(args, jitter, point_lights, scene_parts=None):
     """The radiance cache against the plain tracer on the same 4M-triangle scene at 1920 x 1080.
 
     By default the camera rays go through pixel centres and the scene is lit by the sun and sky. With
@@ -349,16 +399,19 @@ def cache_main(args):
 
     from game_engine_amd import abi, rt, synth
 
-    log("[bench] synthesizing the 4M-triangle instanced scene")
-    s = synth.Synth.config(2, 0, args.instances)
+    if scene_parts is None:
+        log("[bench] synthesizing the 4M-triangle instanced scene")
+        s = synth.Synth.config(2, 0, args.instances)
+        ctx = rt.Context(0)
+        scene = ctx.upload(s.desc())
+        scene.build()
+    else:
+        s, ctx, scene = scene_parts
     pt0, _ = s.frame(WIDTH, HEIGHT)
-    ctx = rt.Context(0)
-    scene = ctx.upload(s.desc())
-    scene.build()
     lo, hi = s.bounds()
     n = WIDTH * HEIGHT
-    flags = 0 if args.jitter else abi.PT_PIXEL_CENTER
-    lights = pt0.light_count if args.point_lights else 0
+    flags = 0 if jitter else abi.PT_PIXEL_CENTER
+    lights = pt0.light_count if point_lights else 0
 
     def params(seed, spp, base, width=WIDTH, height=HEIGHT):
         p, _ = s.frame(width, height)
@@ -503,12 +556,12 @@ def cache_main(args):
                                f"{args.samples} self-training samples, {args.train_steps} steps) after "
                                f"{args.cache_warmup} warm-up frames at {args.train_width}x{args.train_height}, "
                                f"RMSE against a {args.ref_spp}-spp plain render "
-                               f"({'jittered' if args.jitter else 'pixel centres'}, "
-                               f"{'sun + 64 point lights' if args.point_lights else 'sun and sky'}, 8 bounces)",
+                               f"({'jittered' if jitter else 'pixel centres'}, "
+                               f"{'sun + 64 point lights' if point_lights else 'sun and sky'}, 8 bounces)",
                    "width": WIDTH, "height": HEIGHT, "reference_spp": args.ref_spp, "plain_budget_spp": args.budget,
                    "warmup_paths": args.train_width * args.train_height, "samples": args.samples,
                    "train_every": args.train_every,
-                   "train_steps": args.train_steps, "learning_rate": args.lr, "pixel_center": not args.jitter,
+                   "train_steps": args.train_steps, "learning_rate": args.lr, "pixel_center": not jitter,
                    "point_lights": lights, "parallelism": "single GPU"},
         "plain_vs_cached": table,
         "cached_frames": frames, "cached_rmse_final": round(cached[-1][2], 6),
@@ -521,7 +574,8 @@ def cache_main(args):
                      "launch_ms": round(launch_ms, 4), "queries_per_launch": nq,
                      "flop_per_query": CACHE_FLOP_PER_QUERY},
     }
-    print(json.dumps(result), flush=True)
+    cache.close()
+    return result
 
 
 INVERSE_METRIC = "inverse-rendering iterations/s (path-replay adjoint)"
@@ -628,6 +682,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true", help="skip the whole-frame check against the oracle")
+    ap.add_argument("--no-c5", action="store_true", help="skip the time-to-equal-RMSE measurement")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--megakernel", action="store_true", help="one path per lane instead of the wavefront")
     ap.add_argument("--workload", choices=["path", "shadow", "cache", "inverse", "masked"], default="path",
@@ -728,6 +784,12 @@ def main():
     for k in range(args.steps):
         frame(args.warmup + k, count=True)
     st = ctx.stats()
+    # One more frame on its own, samples warmup * spp onwards: the frame the oracle renders first.
+    accum.zero_()
+    coverage.zero_()
+    frame(args.warmup)
+    torch.cuda.synchronize()
+    check_acc, check_cov = accum.cpu().numpy(), coverage.cpu().numpy().view(np.uint32)
     rays = st["closest"] + st["any"]
     log(f"[bench] counted: {st}")
     value = rays / wall / 1e6
@@ -766,9 +828,15 @@ def main():
         "bvh_build_ms": round(min(builds), 3),
         "frame_kernel_ms": round(mean_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": read_traffic(dominant),
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": read_profile(dominant, "traffic_bytes_per_launch"),
                      "kernel": dominant, "launch_ms": round(launch_ms, 4), "launches_per_step": launches / args.steps,
                      "bytes_per_launch": int(per_launch),
+                     # What limits the walk: vector issue (VALU cycles over SIMD cycles) and divergence
+                     # (lanes doing work in an issued instruction), from the committed counter passes;
+                     # the L2 serves the node fetches, so HBM traffic is far below the bytes above.
+                     "limiter": "valu issue", "valu_busy": read_profile(dominant, "valu_busy"),
+                     "valu_thread_util": read_profile(dominant, "valu_thread_util"),
+                     "l2_hit": read_profile(dominant, "l2_hit"),
                      "walk_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in kinds.items()}},
     }
     if not args.no_cpu:
@@ -776,6 +844,24 @@ def main():
         cpu = cpu_baseline(s, pt, args.cpu_seconds, sample_base=args.warmup * pt.spp)
         out["cpu_baseline"] = cpu
         out["gpu_over_cpu"] = round(value / cpu["value"], 1) if cpu["value"] > 0 else None
+    if not args.no_parity:
+        log("[bench] parity of one whole frame against the oracle")
+        out["parity"] = frame_parity(s, pt, args.warmup * pt.spp, check_acc, check_cov)
+        log(f"[bench] parity: {out['parity']}")
+    if not args.no_c5:
+        # The time-to-equal-RMSE half of the metric, on the same scene: as configured (jittered
+        # camera rays, the sun and 64 point lights) and at pixel centres under the sun and sky.
+        c5 = {}
+        for name, jit, pl in (("stated", True, True), ("pixel_centres_sun_sky", False, False)):
+            log(f"[bench] time to equal RMSE: {name}")
+            r = cache_measure(args, jit, pl, scene_parts=(s, ctx, scene))
+            head = next((row for row in r["plain_vs_cached"] if row["plain_spp"] == args.budget),
+                        r["plain_vs_cached"][-1])
+            c5[name] = {"ratio": r["value"], "plain_spp": head["plain_spp"], "plain_ms": head["plain_ms"],
+                        "cached_ms": head["cached_ms"], "rmse": head["rmse"], "workload": r["config"]["workload"],
+                        "cached_frames": r["cached_frames"], "cached_ms_per_frame": r["cached_ms_per_frame"],
+                        "table": r["plain_vs_cached"], "k_cache_infer": r["roofline"]}
+        out["time_to_equal_rmse"] = c5
     print(json.dumps(out), flush=True)
 
 

@@ -33,7 +33,8 @@ def build(verbose=True):
     rt = os.path.join(LIB, "libge_rt.so")
     synth = os.path.join(LIB, "libge_synth.so")
     # One object a source, compiled side by side (each carries its own device code), then linked.
-    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    # GE_HIP_DEFINES: extra -D flags for the walk's tuning macros (GE_WALK_BATCH, GE_WALK_REFILL, ...).
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"] + os.environ.get("GE_HIP_DEFINES", "").split()
     objs = [os.path.join(OBJ, os.path.splitext(src)[0] + ".o") for src in HIP_SOURCES]
     jobs = max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 8))
     with ThreadPoolExecutor(jobs) as pool:

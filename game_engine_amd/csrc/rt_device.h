@@ -428,9 +428,10 @@ GE_DEV bool intersect(const Ray& r, const float* v0, const float* v1, const floa
   if (det == 0.0f) return false;
   const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
   const float T = U * Az + V * Bz + W * Cz;
-  t = T / det;
-  u = V / det;
-  v = W / det;
+  const float rcp = 1.0f / det;  // one division, the paper's scaling of T, V and W
+  t = T * rcp;
+  u = V * rcp;
+  v = W * rcp;
   return true;
 }
 
@@ -454,17 +455,28 @@ GE_DEV void to_object(const InstRec& rec, const Ray& w, Ray& o) {
 // distance at entry; then a mesh's leaves among them in child order, each skipped if its box's entry
 // lies beyond the distance so far; then the inner children whose entry is within the distance after
 // the leaves, the nearest next (ties: the first) and the others pushed so that they come back in
-// child order; a top-level leaf enters its instance (the ray goes into mesh space and a sentinel marks the way back). Closest hits are
-// ordered by (t, then the larger pair << 8 | triangle), a total order, so the answer does not depend
-// on the walk; the counters do, and match the oracle's walk.
+// child order; a top-level leaf enters its instance (the ray goes into mesh space and a sentinel
+// marks the way back). Closest hits are ordered by (t, then the larger pair << 8 | triangle), a
+// total order, so the answer does not depend on the walk; the counters do, and match the oracle's
+// walk.
 //
-// The walk is a state machine, one node visit a step, so a persistent kernel can give a lane whose
-// ray has finished a new ray while the rest of its wave is still walking.
+// The walk is a state machine of three kinds of work, so that a wave can run each kind for the
+// lanes that need it together instead of one lane at a time:
+//   kNode  — test the four boxes of node `cur`; a mesh node with leaves among them goes to kLeaf
+//            with the node's entries kept, any other goes straight on to its inner children;
+//   kLeaf  — the kept leaves' triangle tests in child order, then the inner children;
+//   kEnter — `cur` is a top-level leaf: the ray goes into its instance's mesh space (or passes it
+//            by when the instance's mask shares no bit with the ray's);
+// and kDone. The inner children's step chooses the next node (the nearest, the rest pushed) or pops
+// one, which sets the next state. Every lane runs the same sequence of operations whenever the
+// wave schedules them, so hits and counters do not depend on the schedule.
 //
 // `Masks` compiles the candidate rule for alpha-tested materials (`alpha`, one of k_alpha_*): a
 // triangle that passes the intersection and distance tests is committed only where candidate_kept
 // keeps it, so the answer is the nearest kept triangle (the same order, the same tie rule) and the
 // walk and its counters are the opaque walk's but for the candidates left out.
+constexpr uint32_t k_walk_node = 0, k_walk_leaf = 1, k_walk_enter = 2, k_walk_done = 3;
+
 template <bool AnyHit, bool Count, bool Masks = false>
 struct Walker {
   Ray world, ray;
@@ -473,6 +485,12 @@ struct Walker {
   uint32_t cur, node_base, tri_base, first_pair, inst_id, ray_mask;
   uint32_t alpha, mask_cluster, mask_material;  // Masks: the rule, the instance's first cluster and base
   bool in_blas, mask_live;                      // Masks: the instance has masked clusters
+  uint32_t state;
+  // The node whose boxes were tested last: its references and entry distances, the leaves among
+  // them still to test and the inner children that passed at entry.
+  uint32_t nref[4];
+  float ntn[4];
+  uint32_t leaf_mask, hit_mask;
 
   // False when there is nothing to walk (mask 0 or an empty scene): the hit is already the answer.
   GE_DEV bool start(const Bvh& bvh, const Ray& w, float tmax, uint32_t mask, uint32_t alpha_rule = k_alpha_opaque) {
@@ -491,7 +509,10 @@ struct Walker {
     node_base = tri_base = first_pair = inst_id = 0;
     ray_mask = mask;
     in_blas = false;
-    return mask != 0 && bvh.tlas_node_count != 0;
+    leaf_mask = hit_mask = 0;
+    const bool go = mask != 0 && bvh.tlas_node_count != 0;
+    state = go ? k_walk_node : k_walk_done;
+    return go;
   }
 
   GE_DEV void push(uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, int cap, uint32_t x) {
@@ -508,75 +529,99 @@ struct Walker {
     if (sp >= k_ring) slot = spill[sp - k_ring];
     return x;
   }
-  // Takes the next node off the stack (a sentinel leads back to the top level); true when the stack
-  // is empty and the walk has ended.
-  GE_DEV bool next(uint32_t* __restrict__ stack, const uint32_t* __restrict__ spill) {
+  // The state that goes with node reference `cur`: a top-level leaf is entered, any other node visited.
+  GE_DEV void go_to(uint32_t node) {
+    cur = node;
+    state = (!in_blas && (node & GE_BVH_LEAF)) ? k_walk_enter : k_walk_node;
+  }
+  // Takes the next node off the stack (a sentinel leads back to the top level), or ends the walk.
+  GE_DEV void next(uint32_t* __restrict__ stack, const uint32_t* __restrict__ spill) {
     for (;;) {
-      if (sp == 0) return true;
+      if (sp == 0) {
+        state = k_walk_done;
+        return;
+      }
       const uint32_t e = pop(stack, spill);
       if (e == k_sentinel) {
         in_blas = false;
         ray = world;
         continue;
       }
-      cur = e;
-      return false;
+      go_to(e);
+      return;
     }
   }
 
-  // One node visit; true when the walk has ended (stack empty, or an any-hit query found a hit). A
-  // top-level leaf whose instance mask shares no bit with the ray's is passed by without a visit.
-  GE_DEV bool step(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
-    const int cap = int(bvh.stack_cap);
-    if (!in_blas && (cur & GE_BVH_LEAF)) {
-      inst_id = bvh.tlas_slots[cur & ~GE_BVH_LEAF];
-      const InstRec rec = bvh.inst[inst_id];
-      if ((rec.mask & ray_mask) == 0u) return next(stack, spill);
-      node_base = rec.node_base;
-      tri_base = rec.tri_base;
-      first_pair = rec.first_pair;
-      if (Masks && alpha != k_alpha_opaque) {
-        const uint2 mi = bvh.masks.inst[inst_id];
-        mask_cluster = mi.x;
-        mask_material = mi.y & 0x7fffffffu;
-        mask_live = (mi.y >> 31) != 0u;
-      }
-      to_object(rec, world, ray);
-      push(stack, spill, cap, k_sentinel);
-      in_blas = true;
-      if (Count) ++tally.inst[AnyHit];
-      cur = 0;
+  // kEnter: into the instance of top-level leaf `cur`, or past it.
+  GE_DEV void enter(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
+    inst_id = bvh.tlas_slots[cur & ~GE_BVH_LEAF];
+    const InstRec rec = bvh.inst[inst_id];
+    if ((rec.mask & ray_mask) == 0u) {
+      next(stack, spill);
+      return;
     }
+    node_base = rec.node_base;
+    tri_base = rec.tri_base;
+    first_pair = rec.first_pair;
+    if (Masks && alpha != k_alpha_opaque) {
+      const uint2 mi = bvh.masks.inst[inst_id];
+      mask_cluster = mi.x;
+      mask_material = mi.y & 0x7fffffffu;
+      mask_live = (mi.y >> 31) != 0u;
+    }
+    to_object(rec, world, ray);
+    push(stack, spill, int(bvh.stack_cap), k_sentinel);
+    in_blas = true;
+    if (Count) ++tally.inst[AnyHit];
+    cur = 0;
+    state = k_walk_node;
+  }
+
+  // kNode: the four boxes of node `cur` against the distance at entry (one pass of box tests for
+  // the whole wave, branch-free, so that the node's seven 16-byte loads stay whole). A mesh node
+  // with leaves among its passing children goes to kLeaf; any other node goes on to its children.
+  GE_DEV void visit(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
     if (Count) ++tally.nodes[AnyHit];
     const float4* np = reinterpret_cast<const float4*>(in_blas ? bvh.wnodes + node_base + cur : bvh.wtlas + cur);
     const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
     const uint4 refs = reinterpret_cast<const uint4*>(np)[6];
     const float lo[4][3] = {{lx.x, ly.x, lz.x}, {lx.y, ly.y, lz.y}, {lx.z, ly.z, lz.z}, {lx.w, ly.w, lz.w}};
     const float hi[4][3] = {{hx.x, hy.x, hz.x}, {hx.y, hy.y, hz.y}, {hx.z, hy.z, hz.z}, {hx.w, hy.w, hz.w}};
-    const uint32_t ref[4] = {refs.x, refs.y, refs.z, refs.w};
-    // The four boxes against the distance at entry: one pass of box tests for the whole wave.
-    uint32_t hit_mask = 0, leaf_mask = 0;  // inner children to go on to; mesh leaves to test
-    float tn[4];
+    nref[0] = refs.x;
+    nref[1] = refs.y;
+    nref[2] = refs.z;
+    nref[3] = refs.w;
+    uint32_t lm = 0, hm = 0;  // mesh leaves to test; inner children to go on to
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      // Branch-free, so that the node's seven 16-byte loads stay whole.
-      const bool in = slab(lo[c], hi[c], ray, hit.t, tn[c]) && ref[c] != GE_BVH_NONE;
-      const bool leaf = in_blas && (ref[c] & GE_BVH_LEAF) != 0u;
-      leaf_mask |= uint32_t(in && leaf) << c;
-      hit_mask |= uint32_t(in && !leaf) << c;
+      const bool in = slab(lo[c], hi[c], ray, hit.t, ntn[c]) && nref[c] != GE_BVH_NONE;
+      const bool leaf = in_blas && (nref[c] & GE_BVH_LEAF) != 0u;
+      lm |= uint32_t(in && leaf) << c;
+      hm |= uint32_t(in && !leaf) << c;
     }
-    // The leaves in child order, one triangle test a trip for every lane that has one left, so lanes
-    // whose leaves sit in different slots share the trips (at most four, unrolled). A box that passed
-    // at entry is still within the distance so far exactly when tn <= t + teps (the slab test's other
-    // bound held at entry).
+    leaf_mask = lm;
+    hit_mask = hm;
+    if (lm) {
+      state = k_walk_leaf;
+      return;
+    }
+    children(stack, spill, int(bvh.stack_cap));
+  }
+
+  // kLeaf: the kept leaves in child order, one triangle test a trip for every lane that has one
+  // left, so lanes whose leaves sit in different slots share the trips (at most four, unrolled). A
+  // box that passed at entry is still within the distance so far exactly when tn <= t + teps (the
+  // slab test's other bound held at entry). Then the inner children.
+  GE_DEV void leaves(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
+    uint32_t lm = leaf_mask;
 #pragma unroll(Masks ? 1 : 4)
     for (int trip = 0; trip < 4; ++trip) {
-      if (!leaf_mask) break;
-      const int c = __builtin_ctz(leaf_mask);
-      leaf_mask &= leaf_mask - 1u;
-      const float tc = c == 0 ? tn[0] : c == 1 ? tn[1] : c == 2 ? tn[2] : tn[3];
+      if (!lm) break;
+      const int c = __builtin_ctz(lm);
+      lm &= lm - 1u;
+      const float tc = c == 0 ? ntn[0] : c == 1 ? ntn[1] : c == 2 ? ntn[2] : ntn[3];
       if (!(tc <= hit.t + ray.teps)) continue;
-      const uint32_t rc = c == 0 ? ref[0] : c == 1 ? ref[1] : c == 2 ? ref[2] : ref[3];
+      const uint32_t rc = c == 0 ? nref[0] : c == 1 ? nref[1] : c == 2 ? nref[2] : nref[3];
       const ge_bvh_triangle* tri = bvh.tris + tri_base + (rc & ~GE_BVH_LEAF);
       const float4* tp = reinterpret_cast<const float4*>(tri);
       const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
@@ -603,23 +648,32 @@ struct Walker {
         hit.inst = inst_id;
         hit.prim = prim;
         hit.key = key_hit;
-        if (AnyHit) return true;
+        if (AnyHit) {
+          state = k_walk_done;
+          return;
+        }
       }
     }
+    leaf_mask = 0;
+    children(stack, spill, int(bvh.stack_cap));
+  }
+
+  // The inner children of the node just tested that are still within the distance: the nearest
+  // next (ties: the first), the others pushed last child first, so that they come back in child
+  // order; none: the next node off the stack. Selects, not branches.
+  GE_DEV void children(uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, int cap) {
     if (hit_mask) {
-      // The inner children still within the distance; the nearest next (ties: the first), the
-      // others pushed last child first, so they come back in child order. Selects, not branches.
       const float reach = hit.t + ray.teps;
       uint32_t keep = 0;
       int best = 0;
       float best_t = 0.0f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const bool in = ((hit_mask >> c) & 1u) != 0u && tn[c] <= reach;
-        const bool nearer = in && (keep == 0u || tn[c] < best_t);
+        const bool in = ((hit_mask >> c) & 1u) != 0u && ntn[c] <= reach;
+        const bool nearer = in && (keep == 0u || ntn[c] < best_t);
         keep |= uint32_t(in) << c;
         best = nearer ? c : best;
-        best_t = nearer ? tn[c] : best_t;
+        best_t = nearer ? ntn[c] : best_t;
       }
       if (keep) {
         const uint32_t rest = keep & ~(1u << best);
@@ -628,19 +682,27 @@ struct Walker {
           // top whether or not it is pushed (one that is not is overwritten by the next).
 #pragma unroll
           for (int c = 3; c >= 0; --c) {
-            stack[k_lanes * sp] = ref[c];
+            stack[k_lanes * sp] = nref[c];
             sp += int((rest >> c) & 1u);
           }
         } else {
 #pragma unroll
           for (int c = 3; c >= 0; --c)
-            if ((rest >> c) & 1u) push(stack, spill, cap, ref[c]);
+            if ((rest >> c) & 1u) push(stack, spill, cap, nref[c]);
         }
-        cur = best == 0 ? ref[0] : best == 1 ? ref[1] : best == 2 ? ref[2] : ref[3];
-        return false;
+        go_to(best == 0 ? nref[0] : best == 1 ? nref[1] : best == 2 ? nref[2] : nref[3]);
+        return;
       }
     }
-    return next(stack, spill);
+    next(stack, spill);
+  }
+
+  // Everything up to the next node test, in order; true when the walk has ended.
+  GE_DEV bool step(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
+    if (state == k_walk_enter) enter(bvh, stack, spill, tally);
+    if (state == k_walk_node) visit(bvh, stack, spill, tally);
+    if (state == k_walk_leaf) leaves(bvh, stack, spill, tally);
+    return state == k_walk_done;
   }
 };
 
@@ -662,6 +724,12 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
 // are not left idle until the wave's longest ray ends. `load(i, slot, ray, tmax, mask)` fetches ray
 // i (false: nothing to trace, the answer is a miss) and `store(slot, hit)` writes an answer. Every
 // wave leaves the loop once the counter has passed n and its lanes are idle.
+//
+// Each pass of the loop runs the box tests of every lane whose walk is at a node. A lane with
+// triangle tests to make, or an instance to enter, waits for them until k_batch lanes of the wave
+// wait for the same kind of work (or until fewer than k_batch lanes are left at a node), and those
+// lanes then run it together: a triangle test or an instance entry run for a few lanes at a time
+// costs the wave as much as for all of them.
 // Occupancy asked of the walk kernels, in waves per SIMD: 5 gives the 4-wide walk 96 VGPRs with a
 // few spilled; 6 (80 VGPRs) spills more of it and runs 8% slower, 4 runs 1-3% slower on the
 // 4M-triangle scene; 0 leaves it to the compiler.
@@ -673,14 +741,22 @@ GE_DEV bool traverse(const Bvh& bvh, const Ray& world, float tmax, uint32_t mask
 #else
 #define GE_WALK_ATTR
 #endif
-// Refill threshold: 48 idle lanes before a wave fetches rays. A refill (the atomic, the ray loads,
-// a reciprocal and a shear per ray) runs as divergent code beside the walk, so batching it pays:
-// on the 4M-triangle scene 16 runs 7% slower, 24 to 48 alike, 8 a third slower.
+// Refill threshold: 32 idle lanes before a wave fetches rays. A refill (the atomic, the dependent
+// ray loads, three divisions and a shear per ray) stalls the wave and runs as divergent code beside
+// the walk, so batching it pays: on the 4M-triangle scene 48 runs 0.8% slower, 24 1.6%, 16 14%.
 #ifndef GE_WALK_REFILL
-#define GE_WALK_REFILL 48
+#define GE_WALK_REFILL 32
 #endif
 constexpr uint32_t k_refill = GE_WALK_REFILL;
 static_assert(k_refill >= 1 && k_refill <= 64, "a refill needs at most a wave of idle lanes");
+// Lanes that wait for triangle tests or an instance entry before the wave runs them: on the
+// 4M-triangle scene 16 is 5% faster than running them as they come; 12 and 20 measure alike, 8 and
+// 24 0.5% slower, 32 3% slower.
+#ifndef GE_WALK_BATCH
+#define GE_WALK_BATCH 16
+#endif
+constexpr uint32_t k_batch = GE_WALK_BATCH;
+static_assert(k_batch >= 1 && k_batch <= 64, "a batch is at most a wave");
 
 GE_DEV uint64_t lanes_below() {
   const uint32_t lane = threadIdx.x & 63u;
@@ -727,7 +803,16 @@ GE_DEV void persistent_walk(const Bvh& bvh, uint32_t n, uint32_t* head, uint32_t
       if (exhausted) break;
       continue;
     }
-    if (active && w.step(bvh, stack, spill, tally)) {
+    // What each kind of work waits for: the wave runs a kind when k_batch lanes want it, or when
+    // fewer than k_batch lanes are left at a node (so that a wave never waits on itself).
+    const uint32_t at_node = uint32_t(__popcll(__ballot(active && w.state == k_walk_node)));
+    const uint32_t at_leaf = uint32_t(__popcll(__ballot(active && w.state == k_walk_leaf)));
+    const uint32_t at_enter = uint32_t(__popcll(__ballot(active && w.state == k_walk_enter)));
+    const bool few = at_node < k_batch;
+    if (at_enter && (few || at_enter >= k_batch) && active && w.state == k_walk_enter) w.enter(bvh, stack, spill, tally);
+    if (at_leaf && (few || at_leaf >= k_batch) && active && w.state == k_walk_leaf) w.leaves(bvh, stack, spill, tally);
+    if (active && w.state == k_walk_node) w.visit(bvh, stack, spill, tally);
+    if (active && w.state == k_walk_done) {
       store(slot, w.hit);
       active = false;
     }

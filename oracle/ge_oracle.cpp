@@ -772,9 +772,10 @@ inline bool intersect(const TRay& r, const float* v0, const float* v1, const flo
   if (det == 0.0f) return false;
   const float Az = r.Sz * A[r.kz], Bz = r.Sz * B[r.kz], Cz = r.Sz * C[r.kz];
   const float T = U * Az + V * Bz + W * Cz;
-  t = T / det;
-  u = V / det;
-  v = W / det;
+  const float rcp = 1.0f / det;  // one division, the paper's scaling of T, V and W
+  t = T * rcp;
+  u = V * rcp;
+  v = W * rcp;
   return true;
 }
 
