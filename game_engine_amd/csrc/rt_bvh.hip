@@ -583,9 +583,11 @@ __global__ void k_collapse(uint32_t n_total, const uint32_t* __restrict__ seg_of
     for (int e = 0; e < 2; ++e)
       if (m.child[e] != GE_BVH_NONE) put(m.child[e], e == 0 ? m.lo0 : m.lo1, e == 0 ? m.hi0 : m.hi1);
   }
+  // An empty slot's box sits at +infinity on every axis: every slab test against it fails (its near
+  // distance is +inf, or both its distances -inf), so the walk needs no test of the reference.
   for (; k < 4; ++k) {
     w.child[k] = GE_BVH_NONE;
-    w.lox[k] = w.loy[k] = w.loz[k] = w.hix[k] = w.hiy[k] = w.hiz[k] = 0.0f;
+    w.lox[k] = w.loy[k] = w.loz[k] = w.hix[k] = w.hiy[k] = w.hiz[k] = __builtin_huge_valf();
   }
   for (int q = 0; q < 4; ++q) w.pad[q] = 0;
   wide[nb + i] = w;

@@ -266,7 +266,8 @@ GE_DEV bool candidate_kept(const MaskView& mv, uint32_t cluster, uint32_t materi
 // A 4-wide node: two binary levels collapsed (an even-depth node's children's children, a leaf
 // child taken as it is), its four child boxes axis by axis and its four references (bit 31 a leaf
 // slot, GE_BVH_NONE empty, else the binary index of an even-depth node, whose wide node sits at the
-// same index). 128 bytes: two 64-byte lines, loaded as eight 16-byte words at once.
+// same index). An empty slot's box is +inf on every axis. 128 bytes: two 64-byte lines, of which a
+// visit loads the first 112 as seven 16-byte words.
 struct WideNode {
   float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
   uint32_t child[4];
@@ -332,6 +333,10 @@ struct Ray {
   float teps;  // absolute slack on the box test, in t: 2^-19 of the origin's largest coordinate
   int kx, ky, kz;
   float Sx, Sy, Sz;
+  // Byte offsets within a wide node of each axis's near and far planes, axis a in bits 8a..8a+7:
+  // where the ray's reciprocal is positive the low plane (16 a) is near and the high one (48 + 16 a)
+  // far; where it is negative the other way round.
+  uint32_t near, far;
 };
 
 GE_DEV float sel3(const float v[3], int k) {
@@ -367,6 +372,10 @@ GE_DEV void prep(Ray& r) {
   r.Sz = sel3(r.inv, kz);
   r.Sx = sel3(r.d, kx) * r.Sz;
   r.Sy = sel3(r.d, ky) * r.Sz;
+  const uint32_t flip = ((__float_as_uint(r.inv[0]) >> 31) * 48u) | ((__float_as_uint(r.inv[1]) >> 31) * (48u << 8)) |
+                        ((__float_as_uint(r.inv[2]) >> 31) * (48u << 16));
+  r.near = 0x00201000u + flip;           // 0, 16, 32, each + 48 where flipped
+  r.far = 0x00504030u - flip;            // 48, 64, 80, each - 48 where flipped
   r.teps = maxr(maxr(__builtin_fabsf(r.o[0]), __builtin_fabsf(r.o[1])), __builtin_fabsf(r.o[2])) * 1.9073486328125e-6f *
            __builtin_fabsf(r.Sz);
 }
@@ -400,6 +409,19 @@ GE_DEV bool slab(const float lo[3], const float hi[3], const Ray& r, float tmax,
       __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)), __builtin_fmaxf(tz0, tz1)) *
           k_box_robust,
       tmax);
+  tnear = tn;
+  return tn <= tf + r.teps;
+}
+
+// slab() with each axis's near and far plane given (the low plane is near where the ray's
+// reciprocal is positive): the same comparisons on the same distances, without a min or max per
+// axis. Requires lo <= hi on every axis, which every built box satisfies.
+GE_DEV bool slab_ordered(const float nearp[3], const float farp[3], const Ray& r, float tmax, float& tnear) {
+  const float nx = (nearp[0] - r.o[0]) * r.inv[0], fx = (farp[0] - r.o[0]) * r.inv[0];
+  const float ny = (nearp[1] - r.o[1]) * r.inv[1], fy = (farp[1] - r.o[1]) * r.inv[1];
+  const float nz = (nearp[2] - r.o[2]) * r.inv[2], fz = (farp[2] - r.o[2]) * r.inv[2];
+  const float tn = __builtin_fmaxf(__builtin_fmaxf(nx, ny), __builtin_fmaxf(nz, r.tmin));
+  const float tf = __builtin_fminf(__builtin_fminf(__builtin_fminf(fx, fy), fz) * k_box_robust, tmax);
   tnear = tn;
   return tn <= tf + r.teps;
 }
@@ -582,11 +604,19 @@ struct Walker {
   // with leaves among its passing children goes to kLeaf; any other node goes on to its children.
   GE_DEV void visit(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
     if (Count) ++tally.nodes[AnyHit];
-    const float4* np = reinterpret_cast<const float4*>(in_blas ? bvh.wnodes + node_base + cur : bvh.wtlas + cur);
-    const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-    const uint4 refs = reinterpret_cast<const uint4*>(np)[6];
-    const float lo[4][3] = {{lx.x, ly.x, lz.x}, {lx.y, ly.y, lz.y}, {lx.z, ly.z, lz.z}, {lx.w, ly.w, lz.w}};
-    const float hi[4][3] = {{hx.x, hy.x, hz.x}, {hx.y, hy.y, hz.y}, {hx.z, hy.z, hz.z}, {hx.w, hy.w, hz.w}};
+    const char* np = reinterpret_cast<const char*>(in_blas ? bvh.wnodes + node_base + cur : bvh.wtlas + cur);
+    // Each axis's near and far planes of the four boxes, loaded by the ray's direction signs: the
+    // slab test needs no min or max per axis (with lo <= hi, the near plane's distance is the
+    // smaller one: exactly what the min of the two would choose).
+    const float4 ax = *reinterpret_cast<const float4*>(np + (ray.near & 0xffu));
+    const float4 ay = *reinterpret_cast<const float4*>(np + ((ray.near >> 8) & 0xffu));
+    const float4 az = *reinterpret_cast<const float4*>(np + (ray.near >> 16));
+    const float4 bx = *reinterpret_cast<const float4*>(np + (ray.far & 0xffu));
+    const float4 by = *reinterpret_cast<const float4*>(np + ((ray.far >> 8) & 0xffu));
+    const float4 bz = *reinterpret_cast<const float4*>(np + (ray.far >> 16));
+    const uint4 refs = *reinterpret_cast<const uint4*>(np + 96);
+    const float nearp[4][3] = {{ax.x, ay.x, az.x}, {ax.y, ay.y, az.y}, {ax.z, ay.z, az.z}, {ax.w, ay.w, az.w}};
+    const float farp[4][3] = {{bx.x, by.x, bz.x}, {bx.y, by.y, bz.y}, {bx.z, by.z, bz.z}, {bx.w, by.w, bz.w}};
     nref[0] = refs.x;
     nref[1] = refs.y;
     nref[2] = refs.z;
@@ -594,7 +624,7 @@ struct Walker {
     uint32_t lm = 0, hm = 0;  // mesh leaves to test; inner children to go on to
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const bool in = slab(lo[c], hi[c], ray, hit.t, ntn[c]) && nref[c] != GE_BVH_NONE;
+      const bool in = slab_ordered(nearp[c], farp[c], ray, hit.t, ntn[c]);  // an empty slot's box never passes
       const bool leaf = in_blas && (nref[c] & GE_BVH_LEAF) != 0u;
       lm |= uint32_t(in && leaf) << c;
       hm |= uint32_t(in && !leaf) << c;

@@ -955,7 +955,10 @@ void collapse_levels(const Lbvh& b, Wide& w) {
         ++k;
       }
     }
-    for (; k < 4; ++k) node.child[k] = GE_BVH_NONE;
+    for (; k < 4; ++k) {  // an empty slot: no reference, a box at +inf that no slab test passes
+      node.child[k] = GE_BVH_NONE;
+      for (int a = 0; a < 3; ++a) node.lo[k][a] = node.hi[k][a] = INFINITY;
+    }
   }
 }
 
