@@ -584,6 +584,9 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     std::sort(list.begin(), list.end());
     list.erase(std::unique(list.begin(), list.end()), list.end());
   }
+  // Meshes too large for a one-workgroup build first: they take the device-wide passes.
+  std::stable_partition(list.begin(), list.end(), [&](uint32_t m) { return s.mesh_tri_count[m] > k_block_build_max; });
+  uint32_t nseg_global = 0, n_global = 0;
   std::vector<uint32_t> seg_first, seg_count, seg_node, seg_out, seg_root;
   std::vector<uint32_t> cl_list, cl_out, cl_seg, cl_local;
   uint32_t n = 0;
@@ -595,6 +598,10 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     seg_node.push_back(s.mesh_node_base[m]);
     seg_out.push_back(s.mesh_tri_base[m]);
     seg_root.push_back(m);
+    if (s.mesh_tri_count[m] > k_block_build_max) {
+      nseg_global = sidx + 1;
+      n_global = n + s.mesh_tri_count[m];
+    }
     const ge_mesh_desc& mesh = s.h_meshes[m];
     uint32_t off = n;
     for (uint32_t c = 0; c < mesh.cluster_count; ++c) {
@@ -609,7 +616,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     }
     n = off;
   }
-  std::vector<uint32_t> meta;  // every small array of the batch, packed, one upload
+  std::vector<uint32_t>& meta = ctx->host_meta;  // every small array of the batch, packed, one upload
+  meta.clear();
   auto pack = [&](const std::vector<uint32_t>& v) {
     const size_t at = meta.size();
     meta.insert(meta.end(), v.begin(), v.end());
@@ -631,18 +639,20 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), uint32_t(seg_first.size()), ctx->tmp[3].as<uint32_t>(),
                                md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
                                ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
-                               std::max(s.node_pool, 1u), s.blas_wide.as<WideNode>()));
+                               std::max(s.node_pool, 1u), s.blas_wide.as<WideNode>(), nseg_global, n_global));
     GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
                                         ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
                                         ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));
-    GE_HIP(ctx, hipStreamSynchronize(st));  // the packed metadata lives in host memory until here
   }
 
   // ---- the instances ----
   const uint32_t ni = s.instance_count;
-  std::vector<InstRec> recs(ni);
-  std::vector<float> T(size_t(std::max(ni, 1u)) * 3, 0.0f);
-  std::vector<uint32_t> live;
+  std::vector<InstRec>& recs = ctx->host_recs;
+  std::vector<float>& T = ctx->host_T;
+  std::vector<uint32_t>& live = ctx->host_live;
+  recs.assign(ni, InstRec{});
+  T.assign(size_t(std::max(ni, 1u)) * 3, 0.0f);
+  live.clear();
   for (uint32_t i = 0; i < ni; ++i) {
     const ge_instance_desc& in = s.h_instances[i];
     for (int r = 0; r < 3; ++r)
@@ -667,7 +677,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, s.tlas_root.reserve(32));
   GE_HIP(ctx, s.tlas_height.reserve(4));
   if (nl > 0) {
-    const std::vector<uint32_t> tmeta = {0u, nl, 0u};  // seg_first, seg_count, node_base
+    std::vector<uint32_t>& tmeta = ctx->host_tmeta;
+    tmeta = {0u, nl, 0u};  // seg_first, seg_count, node_base
     GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + size_t(ni) * 12 + 256));
     float* boxes = ctx->tmp[6].as<float>();
     float* d_T = boxes + size_t(nl) * 6;
@@ -682,7 +693,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
                                       s.mesh_roots.as<float>(), boxes, seg_of));
     GE_HIP(ctx, build_segments(ctx->build, st, nl, boxes, 1, seg_of, tm, tm + 1, tm + 2, s.tlas_nodes.as<ge_bvh_node>(), perm,
                                codes, s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), nullptr, s.tlas_node_count,
-                               s.tlas_wide.as<WideNode>()));
+                               s.tlas_wide.as<WideNode>(), nl > k_block_build_max ? 1u : 0u,
+                               nl > k_block_build_max ? nl : 0u));
     GE_HIP(ctx, launch_gather_slots(st, nl, perm, s.live_ids.as<uint32_t>(), s.tlas_slots.as<uint32_t>()));
     GE_HIP(ctx, hipMemcpyAsync(s.tlas_morton.ptr, codes, size_t(nl) * 4, hipMemcpyDeviceToDevice, st));
   }

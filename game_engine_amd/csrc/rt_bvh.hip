@@ -598,6 +598,286 @@ __global__ void k_gather_slots(uint32_t n, const uint32_t* perm, const uint32_t*
   if (g < n) slots[g] = live[perm[g]];
 }
 
+// ---- a whole mesh's structure in one workgroup --------------------------------------------------
+// A segment of at most kBlockMax primitives is built by one workgroup from start to end in LDS: its
+// centroid bounds, Morton codes, a bitonic sort of (code, index) (the stable order of the global
+// sort), Karras' emission, the fit bottom-up (the second child to finish a node fits it: LDS
+// counters, a workgroup-scope fence between a box's store and the counter), the binary nodes, each
+// node's depth and the 4-wide collapse. One launch builds every small mesh of a batch, where the
+// device-wide passes cost a launch a level; the results are the device-wide build's to the bit.
+constexpr uint32_t kBlockMax = k_block_build_max;
+constexpr uint32_t kBlockThreads = 256;
+constexpr uint16_t kBlockLeaf = 0x8000u;
+
+__global__ void __launch_bounds__(kBlockThreads) k_block_build(
+    uint32_t seg_base, const float* __restrict__ boxes, const uint32_t* __restrict__ seg_first,
+    const uint32_t* __restrict__ seg_count, const uint32_t* __restrict__ node_base,
+    const uint32_t* __restrict__ root_index, ge_bvh_node* __restrict__ nodes, uint32_t* __restrict__ perm,
+    uint32_t* __restrict__ morton, float* __restrict__ roots, uint32_t* __restrict__ heights,
+    WideNode* __restrict__ wide) {
+  __shared__ uint64_t keys[kBlockMax];     // (code << 32 | local index), sorted in place
+  __shared__ float nbox[kBlockMax][6];     // internal nodes' boxes
+  __shared__ uint16_t child[kBlockMax][2]; // internal children; kBlockLeaf | sorted position for a leaf
+  __shared__ uint16_t parent[kBlockMax];   // internal node -> its parent
+  __shared__ uint16_t leaf_parent[kBlockMax];
+  __shared__ uint32_t arrived[kBlockMax];  // fit counters
+  __shared__ uint8_t height[kBlockMax];
+  __shared__ uint32_t red[6];
+  const uint32_t t = threadIdx.x;
+  const uint32_t s = seg_base + blockIdx.x;
+  const uint32_t first = seg_first[s], n = seg_count[s], nb = node_base[s];
+  const uint32_t out = root_index ? root_index[s] : s;
+  const float* sb = boxes + size_t(first) * 6;
+  auto leaf_box = [&](uint32_t sorted, float lo[3], float hi[3]) {
+    const uint32_t o = uint32_t(keys[sorted]);
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = sb[size_t(o) * 6 + a];
+      hi[a] = sb[size_t(o) * 6 + 3 + a];
+    }
+  };
+
+  // Centroid bounds, as order-preserving integers (min and max are the float ones).
+  if (t < 6) red[t] = t < 3 ? 0xffffffffu : 0u;
+  __syncthreads();
+  {
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    for (uint32_t i = t; i < n; i += kBlockThreads)
+      for (int a = 0; a < 3; ++a) {
+        const uint32_t c = ordered((sb[size_t(i) * 6 + a] + sb[size_t(i) * 6 + 3 + a]) * 0.5f);
+        lo[a] = min(lo[a], c);
+        hi[a] = max(hi[a], c);
+      }
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&red[a], lo[a]);
+      atomicMax(&red[3 + a], hi[a]);
+    }
+  }
+  __syncthreads();
+  float clo[3], cscale[3];
+  for (int a = 0; a < 3; ++a) {
+    clo[a] = unordered(red[a]);
+    const float ext = unordered(red[3 + a]) - clo[a];
+    cscale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+  }
+  // Codes, and the bitonic sort over the next power of two (padding keys sort last).
+  uint32_t p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  for (uint32_t i = t; i < p2; i += kBlockThreads) {
+    if (i < n) {
+      uint32_t q[3];
+      for (int a = 0; a < 3; ++a) {
+        const float c = (sb[size_t(i) * 6 + a] + sb[size_t(i) * 6 + 3 + a]) * 0.5f;
+        q[a] = quantize10(c, clo[a], cscale[a]);
+      }
+      const uint32_t code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+      keys[i] = (uint64_t(code) << 32) | i;
+    } else {
+      keys[i] = ~0ull;
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t; i < p2; i += kBlockThreads) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const uint64_t a = keys[i], b = keys[l];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = t; i < n; i += kBlockThreads) {
+    perm[first + i] = first + uint32_t(keys[i]);
+    morton[first + i] = uint32_t(keys[i] >> 32);
+    arrived[i] = 0;
+  }
+  if (n == 1) {
+    if (t == 0) {
+      float lo[3], hi[3];
+      leaf_box(0, lo, hi);
+      ge_bvh_node node;
+      node.child[0] = GE_BVH_LEAF;
+      node.child[1] = GE_BVH_NONE;
+      node.parent = GE_BVH_NONE;
+      node.count = 1;
+      WideNode w;
+      for (int a = 0; a < 3; ++a) {
+        node.lo0[a] = lo[a];
+        node.hi0[a] = hi[a];
+        node.lo1[a] = __uint_as_float(0x7f800000u);
+        node.hi1[a] = __uint_as_float(0xff800000u);
+        roots[out * 6 + a] = lo[a];
+        roots[out * 6 + 3 + a] = hi[a];
+      }
+      nodes[nb] = node;
+      heights[out] = 1;
+      for (int k = 0; k < 4; ++k) {
+        w.child[k] = k == 0 ? GE_BVH_LEAF : GE_BVH_NONE;
+        w.lox[k] = k == 0 ? lo[0] : __builtin_huge_valf();
+        w.loy[k] = k == 0 ? lo[1] : __builtin_huge_valf();
+        w.loz[k] = k == 0 ? lo[2] : __builtin_huge_valf();
+        w.hix[k] = k == 0 ? hi[0] : __builtin_huge_valf();
+        w.hiy[k] = k == 0 ? hi[1] : __builtin_huge_valf();
+        w.hiz[k] = k == 0 ? hi[2] : __builtin_huge_valf();
+        w.pad[k] = 0;
+      }
+      wide[nb] = w;
+    }
+    return;
+  }
+  // Karras 2012 over the sorted codes, equal codes split by position.
+  auto delta = [&](int i, int j) -> int {
+    if (j < 0 || j > int(n) - 1) return -1;
+    const uint32_t a = uint32_t(keys[i] >> 32), b = uint32_t(keys[j] >> 32);
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz(uint32_t(i) ^ uint32_t(j));
+  };
+  __syncthreads();
+  if (t == 0) parent[0] = 0xffffu;
+  for (uint32_t ii = t; ii + 1 < n; ii += kBlockThreads) {
+    const int i = int(ii);
+    const int d = (delta(i, i + 1) - delta(i, i - 1)) > 0 ? 1 : -1;
+    const int dmin = delta(i, i - d);
+    int lmax = 2;
+    while (delta(i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int st = lmax / 2; st >= 1; st /= 2)
+      if (delta(i, i + (l + st) * d) > dmin) l += st;
+    const int j = i + l * d;
+    const int dnode = delta(i, j);
+    int sp = 0, st = l;
+    do {
+      st = (st + 1) >> 1;
+      if (delta(i, i + (sp + st) * d) > dnode) sp += st;
+    } while (st > 1);
+    const int gamma = i + sp * d + (d < 0 ? d : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const bool l0 = lo == gamma, l1 = hi == gamma + 1;
+    child[i][0] = l0 ? uint16_t(kBlockLeaf | gamma) : uint16_t(gamma);
+    child[i][1] = l1 ? uint16_t(kBlockLeaf | (gamma + 1)) : uint16_t(gamma + 1);
+    if (l0) leaf_parent[gamma] = uint16_t(i); else parent[gamma] = uint16_t(i);
+    if (l1) leaf_parent[gamma + 1] = uint16_t(i); else parent[gamma + 1] = uint16_t(i);
+  }
+  __syncthreads();
+  // The fit: every leaf walks up; the second arrival at a node fits it and goes on.
+  for (uint32_t leaf = t; leaf < n; leaf += kBlockThreads) {
+    uint32_t p = leaf_parent[leaf];
+    for (;;) {
+      __threadfence_block();
+      if (atomicAdd(&arrived[p], 1u) == 0u) break;
+      float b[2][6];
+      uint32_t h[2];
+      for (int c = 0; c < 2; ++c) {
+        const uint16_t ch = child[p][c];
+        if (ch & kBlockLeaf) {
+          leaf_box(ch & ~kBlockLeaf, &b[c][0], &b[c][3]);
+          h[c] = 0;
+        } else {
+          for (int a = 0; a < 6; ++a) b[c][a] = nbox[ch][a];
+          h[c] = height[ch];
+        }
+      }
+      for (int a = 0; a < 3; ++a) {
+        nbox[p][a] = __builtin_fminf(b[0][a], b[1][a]);
+        nbox[p][3 + a] = __builtin_fmaxf(b[0][3 + a], b[1][3 + a]);
+      }
+      height[p] = uint8_t(1 + (h[0] > h[1] ? h[0] : h[1]));
+      if (p == 0) break;
+      p = parent[p];
+    }
+  }
+  __syncthreads();
+  // The binary nodes, the root, and each node's depth for the collapse.
+  for (uint32_t i = t; i + 1 < n; i += kBlockThreads) {
+    ge_bvh_node node;
+    for (int c = 0; c < 2; ++c) {
+      const uint16_t ch = child[i][c];
+      float lo[3], hi[3];
+      if (ch & kBlockLeaf) {
+        leaf_box(ch & ~kBlockLeaf, lo, hi);
+        node.child[c] = GE_BVH_LEAF | uint32_t(ch & ~kBlockLeaf);
+      } else {
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = nbox[ch][a];
+          hi[a] = nbox[ch][3 + a];
+        }
+        node.child[c] = ch;
+      }
+      float* dlo = c == 0 ? node.lo0 : node.lo1;
+      float* dhi = c == 0 ? node.hi0 : node.hi1;
+      for (int a = 0; a < 3; ++a) {
+        dlo[a] = lo[a];
+        dhi[a] = hi[a];
+      }
+    }
+    // The primitives below: the node's key range, from its children.
+    uint32_t lo_i = 0, hi_i = 0;
+    {
+      // Leftmost and rightmost leaves: descend the first and the last child.
+      uint16_t c = child[i][0];
+      while (!(c & kBlockLeaf)) c = child[c][0];
+      lo_i = c & ~kBlockLeaf;
+      c = child[i][1];
+      while (!(c & kBlockLeaf)) c = child[c][1];
+      hi_i = c & ~kBlockLeaf;
+    }
+    node.count = hi_i - lo_i + 1;
+    node.parent = i == 0 ? GE_BVH_NONE : uint32_t(parent[i]);
+    nodes[nb + i] = node;
+  }
+  if (t == 0) {
+    for (int a = 0; a < 6; ++a) roots[out * 6 + a] = nbox[0][a];
+    heights[out] = height[0];
+  }
+  for (uint32_t i = t; i + 1 < n; i += kBlockThreads) {
+    uint32_t d = 0;
+    for (uint32_t q = i; q != 0; q = parent[q]) ++d;
+    if (d & 1u) continue;
+    WideNode w;
+    int k = 0;
+    auto put = [&](uint32_t ref, const float* lo, const float* hi) {
+      w.child[k] = ref;
+      w.lox[k] = lo[0];
+      w.loy[k] = lo[1];
+      w.loz[k] = lo[2];
+      w.hix[k] = hi[0];
+      w.hiy[k] = hi[1];
+      w.hiz[k] = hi[2];
+      ++k;
+    };
+    auto take = [&](uint16_t ch) {
+      float lo[3], hi[3];
+      if (ch & kBlockLeaf) {
+        leaf_box(ch & ~kBlockLeaf, lo, hi);
+        put(GE_BVH_LEAF | uint32_t(ch & ~kBlockLeaf), lo, hi);
+      } else {
+        put(ch, &nbox[ch][0], &nbox[ch][3]);
+      }
+    };
+    for (int c = 0; c < 2; ++c) {
+      const uint16_t ch = child[i][c];
+      if (ch & kBlockLeaf) {
+        take(ch);
+        continue;
+      }
+      take(child[ch][0]);
+      take(child[ch][1]);
+    }
+    for (; k < 4; ++k) {
+      w.child[k] = GE_BVH_NONE;
+      w.lox[k] = w.loy[k] = w.loz[k] = w.hix[k] = w.hiy[k] = w.hiz[k] = __builtin_huge_valf();
+    }
+    for (int q = 0; q < 4; ++q) w.pad[q] = 0;
+    wide[nb + i] = w;
+  }
+}
+
 inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
 
 }  // namespace
@@ -606,9 +886,19 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
                           const uint32_t* d_seg_of, const uint32_t* d_seg_first, const uint32_t* d_seg_count,
                           const uint32_t* d_node_base, ge_bvh_node* nodes, uint32_t* perm, uint32_t* morton,
                           float* roots, uint32_t* heights, const uint32_t* d_root_index, uint32_t node_pool,
-                          WideNode* wide) {
+                          WideNode* wide, uint32_t nseg_global, uint32_t n_global) {
   if (n == 0) return hipSuccess;
   hipError_t e;
+  // Segments [nseg_global, nseg) fit a workgroup's LDS: one launch builds them all.
+  if (nseg_global < nseg) {
+    k_block_build<<<nseg - nseg_global, kBlockThreads, 0, stream>>>(nseg_global, boxes, d_seg_first, d_seg_count,
+                                                                    d_node_base, d_root_index, nodes, perm, morton,
+                                                                    roots, heights, wide);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  if (nseg_global == 0) return hipSuccess;
+  n = n_global;
+  nseg = nseg_global;
   const uint32_t tiles = blocks(n, kSortTile);
   if ((e = sc.keys[0].reserve(size_t(n) * 8)) != hipSuccess) return e;
   if ((e = sc.keys[1].reserve(size_t(n) * 8)) != hipSuccess) return e;

@@ -170,6 +170,11 @@ struct ge_rt_ctx {
   ge::BuildScratch build;
   ge::DevBuf io[6];    // host-pointer staging
   ge::DevBuf tmp[9];   // build batches: metadata, records, boxes, segments, order, codes, instances
+  // A build's host arrays, copied to the device asynchronously: kept until the next build, which
+  // starts after the last one's closing synchronization.
+  std::vector<uint32_t> host_meta, host_tmeta, host_live;
+  std::vector<ge::InstRec> host_recs;
+  std::vector<float> host_T;
   // 8 x u64 (closest rays, any rays, closest-hit nodes, triangles, instances, any-hit nodes,
   // triangles, instances), then the persistent walks' head counter at byte 64
   ge::DevBuf counters;
@@ -198,11 +203,16 @@ namespace ge {
 // order (batch indices) to perm[seg_first[s] ..], its sorted codes to morton[seg_first[s] ..], and
 // its root box and height to roots[6 r ..] and heights[r], r = root_index[s] (or s when null).
 // `node_pool` bounds every node_base[s] + node count. All pointers are device pointers.
+// Segments of at most k_block_build_max primitives are built by one workgroup each, in LDS, in one
+// launch; larger ones by device-wide passes. The caller puts the larger ones first: segments
+// [0, nseg_global), primitives [0, n_global), take the device-wide passes.
+constexpr uint32_t k_block_build_max = 1024;
 hipError_t build_segments(BuildScratch& scratch, hipStream_t stream, uint32_t n, const float* boxes,
                           uint32_t nseg, const uint32_t* d_seg_of, const uint32_t* d_seg_first,
                           const uint32_t* d_seg_count, const uint32_t* d_node_base, ge_bvh_node* nodes,
                           uint32_t* perm, uint32_t* morton, float* roots, uint32_t* heights,
-                          const uint32_t* d_root_index, uint32_t node_pool, WideNode* wide);
+                          const uint32_t* d_root_index, uint32_t node_pool, WideNode* wide, uint32_t nseg_global,
+                          uint32_t n_global);
 // Triangle records, boxes and segment ids of the clusters in `cluster_list` (cluster k's triangles
 // at batch index cluster_out[k] on; cluster_local[k] its index within its mesh).
 hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,
