@@ -25,7 +25,7 @@ def build():
     where the reference tree is present, the reference's header-only twins into oracle/_ref/."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     if os.path.isdir(REFERENCE):
-        subprocess.run(["make", "-s", "-C", _HERE, "ref", f"REFERENCE={REFERENCE}"], check=True)
+        subprocess.run(["make", "-s", "-C", _HERE, "ref", "abi", f"REFERENCE={REFERENCE}"], check=True)
 
 
 def ref_lib():

@@ -3,8 +3,12 @@ its records have the engine's sizes, and calls that need no device fail cleanly.
 import ctypes as C
 import os
 import re
+import subprocess
+
+import pytest
 
 from game_engine_amd import abi, rt
+from oracle import binding as ob
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "ge_rt.h")
 
@@ -41,6 +45,53 @@ def test_record_sizes_match_the_engine():
                   "ge_bvh_triangle": abi.TRI_DTYPE, "ge_shadow_receiver": abi.RECEIVER_DTYPE}
         if struct in dtypes:
             assert dtypes[struct].itemsize == int(size), struct
+
+
+@pytest.mark.skipif(not os.path.isdir(ob.REFERENCE), reason="the reference's headers are not on this machine")
+def test_record_fields_match_the_engines_headers():
+    """oracle/abi_pins.cpp compiles include/ge_rt.h beside the reference's path_trace.h,
+    ray_visibility.h, cluster_cull.h, visibility_resolve.h and geometry/cluster.h, with a
+    static_assert on the offset and size of every mirrored field and the value of every mirrored
+    constant; the same file with one field compared against its neighbour must not compile."""
+    run = lambda *extra: subprocess.run(["make", "-s", "-C", os.path.dirname(ob.__file__), "abi",  # noqa: E731
+                                         f"REFERENCE={ob.REFERENCE}", *extra], capture_output=True, text=True)
+    good = run()
+    assert good.returncode == 0, good.stderr
+    bad = run("ABI_FLAGS=-DGE_ABI_PINS_PLANT_MISMATCH")
+    assert bad.returncode != 0 and "ge_path_trace_params::width is not at" in bad.stderr
+    pins = open(os.path.join(os.path.dirname(ob.__file__), "abi_pins.cpp")).read()
+    fields = re.findall(r"^GE_PIN\((\w+), [\w:]+, (\w+)\);", pins, re.M)
+    for struct, mirror in (("ge_path_trace_params", abi.PathTraceParams), ("ge_ray_visibility_params", abi.RayVisibilityParams),
+                           ("ge_instance_desc", abi.InstanceDesc), ("ge_mesh_desc", abi.MeshDesc),
+                           ("ge_frame_eye", abi.FrameEye), ("ge_resolve_material", abi.ResolveMaterial),
+                           ("ge_resolve_light", abi.ResolveLight), ("ge_cluster_desc", abi.ClusterDesc)):
+        assert sorted(f for s, f in fields if s == struct) == sorted(f[0] for f in mirror._fields_), struct
+
+
+MIRRORS = {"ge_path_trace_params": abi.PathTraceParams, "ge_ray_visibility_params": abi.RayVisibilityParams,
+           "ge_instance_desc": abi.InstanceDesc, "ge_mesh_desc": abi.MeshDesc, "ge_frame_eye": abi.FrameEye,
+           "ge_resolve_material": abi.ResolveMaterial, "ge_resolve_light": abi.ResolveLight,
+           "ge_cluster_desc": abi.ClusterDesc, "ge_texture_desc": abi.TextureDesc, "ge_sampler_desc": abi.SamplerDesc,
+           "ge_scene_desc": abi.SceneDesc, "ge_rt_counters": abi.Counters}
+
+
+def test_python_records_match_the_header_field_by_field(tmp_path):
+    """The ctypes mirrors against the C compiler's own offsets of include/ge_rt.h's fields."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for struct, mirror in MIRRORS.items():
+        for name, *_ in mirror._fields_:
+            lines.append(f'  printf("{struct} {name} %zu\\n", offsetof({struct}, {name}));')
+    lines.append("  return 0;\n}")
+    src, exe = tmp_path / "offsets.c", tmp_path / "offsets"
+    src.write_text("\n".join(lines))
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for line in filter(None, out):
+        struct, name, off = line.split()
+        assert getattr(MIRRORS[struct], name).offset == int(off), (struct, name)
+        seen += 1
+    assert seen == sum(len(m._fields_) for m in MIRRORS.values())
 
 
 def test_null_handles_fail_cleanly():
