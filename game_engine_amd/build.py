@@ -1,7 +1,7 @@
 """Builds the backend in-tree: libge_rt.so (HIP, gfx950) and libge_synth.so (host C++).
 
-HIP flags: -ffp-contract=off (no fused multiply-add: every kernel's arithmetic is the IEEE
-operation order written in the source), -fno-fast-math, correctly rounded division and square root
+HIP flags: -ffp-contract=off (no contraction: every kernel's arithmetic is the IEEE operation
+order written in the source, with a fused multiply-add only where one is written), -fno-fast-math, correctly rounded division and square root
 (hipcc's HIP default), and -packed-fp32-ops disabled.
 """
 import os

@@ -1321,6 +1321,14 @@ int32_t ge_rt_cache_weights(ge_rt_ctx* ctx, ge_rt_cache* cache, float* get, cons
   return GE_RT_OK;
 }
 
+int32_t ge_rt_cache_gradient(ge_rt_ctx* ctx, ge_rt_cache* cache, float* out) {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, cache && out, GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  GE_HIP(ctx, cache->net.get_gradient(ctx->stream, out));
+  return GE_RT_OK;
+}
+
 int32_t ge_rt_cache_infer(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_query* queries, uint32_t n,
                           float* radiance, uint32_t flags) {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;

@@ -47,24 +47,38 @@ GE_DEV f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
 
 constexpr float k_pi = 3.14159265358979323846f;
 
-// sin and cos: reduction by pi/2 and the Taylor series of both on [-pi/4, pi/4], in double,
-// rounded once to float. Correctly rounded except within ~1e-10 relative of a rounding boundary.
-GE_DEV void sincos_cr(float x, float* s, float* c) {
-  const double xd = double(x);
-  const double q = __builtin_rint(xd * 0.6366197723675814);
-  const double r = xd - q * 1.5707963267948966;
-  const double r2 = r * r;
-  const double sp =
-      r + r * r2 * (-1.0 / 6.0 + r2 * (1.0 / 120.0 + r2 * (-1.0 / 5040.0 + r2 * (1.0 / 362880.0 +
-                                                                                  r2 * (-1.0 / 39916800.0)))));
-  const double cp =
-      1.0 + r2 * (-0.5 + r2 * (1.0 / 24.0 + r2 * (-1.0 / 720.0 + r2 * (1.0 / 40320.0 +
-                                                                         r2 * (-1.0 / 3628800.0 + r2 * (1.0 / 479001600.0))))));
-  const int quadrant = int(int64_t(q) & 3);
-  const double sv = quadrant == 0 ? sp : (quadrant == 1 ? cp : (quadrant == 2 ? -sp : -cp));
-  const double cv = quadrant == 0 ? cp : (quadrant == 1 ? -sp : (quadrant == 2 ? -cp : sp));
-  *s = float(sv);
-  *c = float(cv);
+// sin and cos in float arithmetic alone, accurate to within one ulp of the exact value for
+// |x| <= 64 (callers pass angles in [0, 2 pi]).
+// Reduction (Cody and Waite 1980): q = rint(x 2/pi), r = x - q pi/2 with pi/2 split into
+// C1 + C2 + C3, C1 and C2 of 16 significant bits so q*C1 and q*C2 are exact for |q| < 2^8; x - q*C1
+// is exact (Sterbenz), the subtraction of q*C2 keeps its rounding error (Knuth's TwoSum), and
+// r = r_hi + r_lo to about 2^-48 relative.
+// Polynomials on [-pi/4, pi/4], z = r_hi^2: sin r = r + r^3 S(z) and cos r = 1 - z/2 + z^2 K(z),
+// S and K of degree 3, minimax in relative error (a Remez exchange, game_engine_amd/
+// derive_sincos.py; 3.7e-9 and 7.0e-10 with these float coefficients). r_lo enters to first order
+// (sin: r_lo (1 - z/2); cos: -r_hi r_lo), and cos's 1 - z/2 is summed with its rounding error
+// (Kahan's trick) and z's own. The result's one final rounding dominates the error.
+GE_DEV void sincos_f32(float x, float* s, float* c) {
+  const float q = __builtin_rintf(x * 0.636619747f);
+  const float r1 = __builtin_fmaf(-q, 1.57080078125f, x);
+  const float p2 = q * -4.45439946e-06f;
+  const float rh = r1 - p2;
+  const float bb = rh - r1;
+  const float e2 = (r1 - (rh - bb)) - (p2 + bb);
+  const float rl = __builtin_fmaf(-q, -5.56443156e-11f, e2);
+  const float z = rh * rh;
+  const float ze = __builtin_fmaf(rh, rh, -z);
+  const float sp = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(2.71812155e-06f, z, -1.98393129e-04f), z,
+                                                 8.33332911e-03f), z, -1.66666672e-01f);
+  const float sn = rh + __builtin_fmaf(rh * z, sp, __builtin_fmaf(-0.5f * z, rl, rl));
+  const float kp = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(-2.72057548e-07f, z, 2.47994594e-05f), z,
+                                                 -1.38888834e-03f), z, 4.16666679e-02f);
+  const float hz = 0.5f * z;
+  const float w = 1.0f - hz;
+  const float cs = w + __builtin_fmaf(z * z, kp, ((1.0f - w) - hz) - __builtin_fmaf(rh, rl, 0.5f * ze));
+  const int quadrant = int(q) & 3;
+  *s = quadrant == 0 ? sn : (quadrant == 1 ? cs : (quadrant == 2 ? -sn : -cs));
+  *c = quadrant == 0 ? cs : (quadrant == 1 ? -sn : (quadrant == 2 ? -cs : sn));
 }
 
 // ---- the random stream: PCG hash over (pixel, sample, seed) ------------------------------------
@@ -426,8 +440,17 @@ GE_DEV bool slab_ordered(const float nearp[3], const float farp[3], const Ray& r
   return tn <= tf + r.teps;
 }
 
-// Watertight ray/triangle intersection (Woop, Benthin, Wald 2013), with the double-precision
-// recomputation of an edge function that is exactly zero. No face culling.
+// a*b - c*d to within 2 ulp, exact in sign and exactly zero only when the value is (Kahan's
+// algorithm with one fused multiply-add; Jeannerod, Louvet and Muller 2013, Math. Comp. 82).
+GE_DEV float det2(float a, float b, float c, float d) {
+  const float w = c * d;
+  const float e = __builtin_fmaf(-c, d, w);
+  return __builtin_fmaf(a, b, -w) + e;
+}
+
+// Watertight ray/triangle intersection (Woop, Benthin, Wald 2013): an edge function that is
+// exactly zero in float is recomputed accurately (det2), the paper's higher-precision fallback.
+// No face culling.
 GE_DEV bool intersect(const Ray& r, const float* v0, const float* v1, const float* v2, float& t, float& u,
                       float& v) {
   const float A[3] = {v0[0] - r.o[0], v0[1] - r.o[1], v0[2] - r.o[2]};
@@ -441,9 +464,9 @@ GE_DEV bool intersect(const Ray& r, const float* v0, const float* v1, const floa
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
   if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
-    V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
-    W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    U = det2(Cx, By, Cy, Bx);
+    V = det2(Ax, Cy, Ay, Cx);
+    W = det2(Bx, Ay, By, Ax);
   }
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
   const float det = U + V + W;

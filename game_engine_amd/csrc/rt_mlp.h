@@ -77,6 +77,7 @@ struct RadianceCache {
   hipError_t init(hipStream_t st, uint32_t seed);
   hipError_t set_weights(hipStream_t st, const float* host);
   hipError_t get_weights(hipStream_t st, float* host);
+  hipError_t get_gradient(hipStream_t st, float* host);  // the last step's FP32 weight gradient
   // Raw network outputs for n queries (stride floats apart, the first 11 the features): out n x 3.
   // With n_dev (device count, clamped to n) the launch reads the count from the device.
   hipError_t infer(hipStream_t st, const float* queries, uint32_t stride, uint32_t n, const uint32_t* n_dev, float* out);

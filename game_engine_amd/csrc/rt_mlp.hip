@@ -598,6 +598,7 @@ hipError_t RadianceCache::set_weights(hipStream_t st, const float* host) {
   hipError_t e = hipMemcpyAsync(w.ptr, host, bytes, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemsetAsync(m.ptr, 0, bytes, st);
   if (e == hipSuccess) e = hipMemsetAsync(v.ptr, 0, bytes, st);
+  if (e == hipSuccess) e = hipMemsetAsync(grad.ptr, 0, bytes, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return e;
   step = 0;
@@ -606,6 +607,11 @@ hipError_t RadianceCache::set_weights(hipStream_t st, const float* host) {
 
 hipError_t RadianceCache::get_weights(hipStream_t st, float* host) {
   hipError_t e = hipMemcpyAsync(host, w.ptr, size_t(kCacheWeights) * 4, hipMemcpyDeviceToHost, st);
+  return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
+hipError_t RadianceCache::get_gradient(hipStream_t st, float* host) {
+  hipError_t e = hipMemcpyAsync(host, grad.ptr, size_t(kCacheWeights) * 4, hipMemcpyDeviceToHost, st);
   return e == hipSuccess ? hipStreamSynchronize(st) : e;
 }
 

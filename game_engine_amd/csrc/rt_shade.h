@@ -126,7 +126,7 @@ GE_DEV f3 cosine_hemisphere(float ux, float uy) {
   const float r = __builtin_sqrtf(ux);
   const float phi = 2.0f * k_pi * uy;
   float sn, cs;
-  sincos_cr(phi, &sn, &cs);
+  sincos_f32(phi, &sn, &cs);
   return f3{r * cs, r * sn, __builtin_sqrtf(maxr(0.0f, 1.0f - ux))};
 }
 GE_DEV float ggx_g1(float n_dot_v, float alpha) {
@@ -142,7 +142,7 @@ GE_DEV f3 ggx_vndf(f3 v, float alpha, float ux, float uy) {
   const float r = __builtin_sqrtf(ux);
   const float phi = 2.0f * k_pi * uy;
   float sn, cs;
-  sincos_cr(phi, &sn, &cs);
+  sincos_f32(phi, &sn, &cs);
   const float p1 = r * cs;
   float p2 = r * sn;
   const float s = 0.5f * (1.0f + vh.z);

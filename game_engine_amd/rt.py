@@ -71,6 +71,7 @@ def lib():
             "ge_rt_cache_create": ([vp, vp, vp, u32, C.POINTER(vp)], i32),
             "ge_rt_cache_destroy": ([vp, vp], None),
             "ge_rt_cache_weights": ([vp, vp, vp, vp], i32),
+            "ge_rt_cache_gradient": ([vp, vp, vp], i32),
             "ge_rt_cache_infer": ([vp, vp, vp, u32, vp, u32], i32),
             "ge_rt_cache_train": ([vp, vp, vp, u32, C.c_float, C.POINTER(C.c_double), u32], i32),
             "ge_rt_cache_collect": ([vp, vp, vp, C.POINTER(abi.PathTraceParams), vp, u32, C.POINTER(u32), u32], i32),
@@ -101,7 +102,7 @@ EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchroni
            "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve",
            "ge_rt_shadow_resolve_noalloc", "ge_rt_stats",
            "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint",
-           "ge_rt_cache_create", "ge_rt_cache_destroy", "ge_rt_cache_weights", "ge_rt_cache_infer",
+           "ge_rt_cache_create", "ge_rt_cache_destroy", "ge_rt_cache_weights", "ge_rt_cache_gradient", "ge_rt_cache_infer",
            "ge_rt_cache_train", "ge_rt_cache_collect", "ge_rt_cache_update", "ge_rt_path_trace_cached",
            "ge_rt_cache_samples", "ge_clusters_read", "ge_clusters_scene", "ge_clusters_counts",
            "ge_clusters_destroy", "ge_clusters_hash")
@@ -359,6 +360,12 @@ class RadianceCache:
         w = np.zeros(abi.CACHE_WEIGHTS, dtype=np.float32)
         self.ctx.check(lib().ge_rt_cache_weights(self.ctx.h, self.h, w.ctypes.data, None))
         return w
+
+    def gradient(self):
+        """The last optimizer step's FP32 weight gradient (d mean loss / d weights)."""
+        g = np.zeros(abi.CACHE_WEIGHTS, dtype=np.float32)
+        self.ctx.check(lib().ge_rt_cache_gradient(self.ctx.h, self.h, g.ctypes.data))
+        return g
 
     def set_weights(self, w):
         w = np.ascontiguousarray(w, dtype=np.float32)

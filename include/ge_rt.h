@@ -624,6 +624,12 @@ void ge_rt_cache_destroy(ge_rt_ctx* ctx, ge_rt_cache* cache);
 /* The FP32 master weights (GE_CACHE_WEIGHTS floats, host memory): `get` receives them, `set`
  * replaces them and restarts the optimizer. Either may be NULL. */
 int32_t ge_rt_cache_weights(ge_rt_ctx* ctx, ge_rt_cache* cache, float* get, const float* set);
+/* The FP32 weight gradient of the last optimizer step (ge_rt_cache_train, or the last step of
+ * ge_rt_cache_update or a training ge_rt_path_trace_cached): d(batch mean loss)/d(weights), the
+ * value Adam consumed, GE_CACHE_WEIGHTS floats into host `out`. Zero before the first step and
+ * after weights are set. No counterpart in the reference, whose cache trains inside its frame: it
+ * is for callers that check or log the backward pass. */
+int32_t ge_rt_cache_gradient(ge_rt_ctx* ctx, ge_rt_cache* cache, float* out);
 /* The network's raw outputs (n x 3 floats) for n queries. */
 int32_t ge_rt_cache_infer(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_query* queries, uint32_t n,
                           float* radiance, uint32_t flags);

@@ -107,6 +107,8 @@ def lib():
         L.ge_oracle_morton3.restype = C.c_uint32
         L.ge_oracle_morton3.argtypes = [C.c_float, C.c_float, C.c_float, vp, vp]
         L.ge_oracle_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.ge_oracle_det2.restype = C.c_float
+        L.ge_oracle_det2.argtypes = [C.c_float] * 4
         L.ge_oracle_relative_axis.restype = C.c_float
         L.ge_oracle_relative_axis.argtypes = [C.c_int32, C.c_float, C.c_int32, C.c_float, C.c_float]
         L.ge_oracle_probe.restype = C.c_int32

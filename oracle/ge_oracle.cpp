@@ -92,32 +92,36 @@ template <class R> inline V3<R> splat(R s) { return {s, s, s}; }
 template <class R> inline V3<R> load3(const float* p) { return {R(p[0]), R(p[1]), R(p[2])}; }
 
 // ---- sin and cos ---------------------------------------------------------------------------------
-// FP32 mode: the backend's own sincos, restated. Both sides reduce by pi/2 and evaluate the Taylor
-// series of sin and cos on [-pi/4, pi/4] in double, then round once to float, so the result is the
-// correctly rounded value except within ~1e-10 relative of a rounding boundary (pinned against
-// libm's double sin/cos in tests/test_oracle_math.py). FP64 mode: libm.
+// FP32 mode: the backend's float sincos, restated operation for operation: Cody-Waite reduction by
+// pi/2 (C1, C2 of 16 significant bits, C3; the second step's rounding error kept by TwoSum),
+// degree-3 minimax polynomials in z = r^2 for sin r = r + r^3 S(z) and cos r = 1 - z/2 + z^2 K(z)
+// (coefficients from game_engine_amd/derive_sincos.py), r's low part to first order and cos's
+// 1 - z/2 compensated. Within one ulp of libm's double sin and cos for |x| <= 64 (pinned in
+// tests/test_oracle_math.py). FP64 mode: libm.
 inline void sincos_f32(float x, float* s, float* c) {
-  const double k_half_pi = 1.5707963267948966;
-  const double xd = double(x);
-  const double q = std::nearbyint(xd * 0.6366197723675814);
-  const double r = xd - q * k_half_pi;
-  const double r2 = r * r;
-  const double sp =
-      r + r * r2 * (-1.0 / 6.0 + r2 * (1.0 / 120.0 + r2 * (-1.0 / 5040.0 + r2 * (1.0 / 362880.0 +
-                                                                                  r2 * (-1.0 / 39916800.0)))));
-  const double cp =
-      1.0 + r2 * (-0.5 + r2 * (1.0 / 24.0 + r2 * (-1.0 / 720.0 + r2 * (1.0 / 40320.0 +
-                                                                         r2 * (-1.0 / 3628800.0 + r2 * (1.0 / 479001600.0))))));
-  const int quadrant = int(int64_t(q) & 3);
-  double sv, cv;
-  switch (quadrant) {
-    case 0: sv = sp; cv = cp; break;
-    case 1: sv = cp; cv = -sp; break;
-    case 2: sv = -sp; cv = -cp; break;
-    default: sv = -cp; cv = sp; break;
+  const float q = std::nearbyint(x * 0.636619747f);
+  const float r1 = std::fma(-q, 1.57080078125f, x);
+  const float p2 = q * -4.45439946e-06f;
+  const float rh = r1 - p2;
+  const float bb = rh - r1;
+  const float e2 = (r1 - (rh - bb)) - (p2 + bb);
+  const float rl = std::fma(-q, -5.56443156e-11f, e2);
+  const float z = rh * rh;
+  const float ze = std::fma(rh, rh, -z);
+  const float sp = std::fma(std::fma(std::fma(2.71812155e-06f, z, -1.98393129e-04f), z, 8.33332911e-03f), z,
+                            -1.66666672e-01f);
+  const float sn = rh + std::fma(rh * z, sp, std::fma(-0.5f * z, rl, rl));
+  const float kp = std::fma(std::fma(std::fma(-2.72057548e-07f, z, 2.47994594e-05f), z, -1.38888834e-03f), z,
+                            4.16666679e-02f);
+  const float hz = 0.5f * z;
+  const float w = 1.0f - hz;
+  const float cs = w + std::fma(z * z, kp, ((1.0f - w) - hz) - std::fma(rh, rl, 0.5f * ze));
+  switch (int(q) & 3) {
+    case 0: *s = sn; *c = cs; break;
+    case 1: *s = cs; *c = -sn; break;
+    case 2: *s = -sn; *c = -cs; break;
+    default: *s = -cs; *c = sn; break;
   }
-  *s = float(sv);
-  *c = float(cv);
 }
 template <class R> inline void sincos_r(R x, R* s, R* c) {
   if constexpr (std::is_same_v<R, float>) {
@@ -749,8 +753,16 @@ inline bool slab(const float lo[3], const float hi[3], const TRay& r, float tmax
   return tn <= tf + r.teps;
 }
 
-// Woop, Benthin, Wald 2013, "Watertight Ray/Triangle Intersection", with the double-precision
-// fallback for an edge function that is exactly zero. No face culling.
+// a*b - c*d by Kahan's algorithm with one fused multiply-add: within 2 ulp, exact in sign, zero
+// only when the value is (Jeannerod, Louvet and Muller 2013). The backend's det2.
+inline float det2(float a, float b, float c, float d) {
+  const float w = c * d;
+  const float e = std::fma(-c, d, w);
+  return std::fma(a, b, -w) + e;
+}
+
+// Woop, Benthin, Wald 2013, "Watertight Ray/Triangle Intersection": an edge function exactly zero
+// in float is recomputed accurately (det2), the paper's higher-precision fallback. No face culling.
 inline bool intersect(const TRay& r, const float* v0, const float* v1, const float* v2, float& t,
                       float& u, float& v) {
   const float A[3] = {v0[0] - r.o[0], v0[1] - r.o[1], v0[2] - r.o[2]};
@@ -763,9 +775,9 @@ inline bool intersect(const TRay& r, const float* v0, const float* v1, const flo
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
   if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
-    V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
-    W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    U = det2(Cx, By, Cy, Bx);
+    V = det2(Ax, Cy, Ay, Cx);
+    W = det2(Bx, Ay, By, Ax);
   }
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
   const float det = U + V + W;
@@ -2848,6 +2860,7 @@ uint32_t ge_oracle_morton3(float cx, float cy, float cz, const float lo[3], cons
   return morton3(c, lo, scale);
 }
 void ge_oracle_sincos(float x, float* s, float* c) { sincos_f32(x, s, c); }
+float ge_oracle_det2(float a, float b, float c, float d) { return det2(a, b, c, d); }
 float ge_oracle_relative_axis(int32_t cell, float local, int32_t eye_cell, float eye_local, float eye_residual) {
   return relative_axis(cell, local, eye_cell, eye_local, eye_residual);
 }

@@ -118,6 +118,8 @@ uint32_t ge_oracle_rng_seed(uint32_t x, uint32_t y, uint32_t sample_index, uint3
 float ge_oracle_rng_next(uint32_t* state);
 uint32_t ge_oracle_morton3(float cx, float cy, float cz, const float lo[3], const float scale[3]);
 void ge_oracle_sincos(float x, float* s, float* c);
+/* a*b - c*d by the watertight test's accurate fallback. */
+float ge_oracle_det2(float a, float b, float c, float d);
 /* relative_axis (world.h:181-188): a position's axis from the frame's eye. */
 float ge_oracle_relative_axis(int32_t cell, float local, int32_t eye_cell, float eye_local, float eye_residual);
 /* Sampler and BSDF, in double (mode FP64) or float (FP32): `in`/`out` layouts per kind, see the

@@ -73,8 +73,12 @@ def test_gradients_match_the_oracle(c4, mega):
     a = np.concatenate([gm.ravel(), gl])
     b = np.concatenate([dm.ravel(), dl])
     cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
-    print(f"GPU vs FP64 oracle: cosine {cos:.6f}, relative norm {np.linalg.norm(a - b) / np.linalg.norm(b):.3g}")
-    assert cos > 0.999
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"GPU vs FP64 oracle: cosine {cos:.6f}, relative norm {rel:.3g}")
+    # The FP32 and FP64 replays part where a decision (roulette, a lobe, a hit) falls the other way
+    # on a rounding; measured on MI355X: cosine 0.99980, relative norm 0.0202.
+    assert cos > 0.9995
+    assert rel <= 0.03
 
 
 def test_wavefront_and_megakernel_agree(c4):
