@@ -62,6 +62,21 @@ int32_t set_error(ge_rt_ctx* ctx, int32_t code, const std::string& msg) {
   return code;
 }
 
+// What an entry point returns when host code under it throws (std::bad_alloc from a container
+// that could not grow, the only exception the backend's own code can raise): out of memory, with
+// the message set where that itself allocates nothing new. Nothing crosses the C boundary.
+int32_t host_failure(ge_rt_ctx* ctx) noexcept {
+  if (ctx) {
+    ctx->last_status = GE_RT_ERROR_OUT_OF_MEMORY;
+    try {
+      ctx->error = "out of host memory";
+    } catch (...) {
+      ctx->error.clear();
+    }
+  }
+  return GE_RT_ERROR_OUT_OF_MEMORY;
+}
+
 #define GE_HIP(ctx, call)                                                                                   \
   do {                                                                                                      \
     const hipError_t e_ = (call);                                                                           \
@@ -285,7 +300,7 @@ int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_
 
 extern "C" {
 
-int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal) {
+int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal) try {
   if (!out) return GE_RT_ERROR_INVALID_ARGUMENT;
   *out = nullptr;
   int count = 0;
@@ -316,6 +331,8 @@ int32_t ge_rt_create(ge_rt_ctx** out, int32_t device_ordinal) {
     ctx->persistent_blocks = uint32_t(cus) * 32u;
   *out = ctx;
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(nullptr);
 }
 
 void ge_rt_destroy(ge_rt_ctx* ctx) {
@@ -347,7 +364,7 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
   delete ctx;
 }
 
-int32_t ge_rt_last_error(const ge_rt_ctx* ctx, char* buffer, size_t size) {
+int32_t ge_rt_last_error(const ge_rt_ctx* ctx, char* buffer, size_t size) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   if (buffer && size) {
     const size_t n = std::min(size - 1, ctx->error.size());
@@ -355,16 +372,20 @@ int32_t ge_rt_last_error(const ge_rt_ctx* ctx, char* buffer, size_t size) {
     buffer[n] = 0;
   }
   return ctx->last_status;
+} catch (...) {
+  return host_failure(nullptr);
 }
 
-int32_t ge_rt_synchronize(ge_rt_ctx* ctx) {
+int32_t ge_rt_synchronize(ge_rt_ctx* ctx) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_HIP(ctx, hipSetDevice(ctx->device));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
-int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene** out) {
+int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene** out) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, d && out, GE_RT_ERROR_INVALID_ARGUMENT, "null scene or output");
   *out = nullptr;
@@ -515,6 +536,8 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
                 std::string("scene upload: ") + hipGetErrorString(e));
   *out = scene;
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
@@ -535,7 +558,7 @@ void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
 }
 
 int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
-                                     const ge_instance_desc* instances) {
+                                     const ge_instance_desc* instances) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene && (instances || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or instances");
   SceneDev& s = scene->dev;
@@ -551,10 +574,12 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   GE_HIP(ctx, refresh_masks(s, ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, const ge_frame_eye* eye, const uint32_t* meshes,
-                        uint32_t count) {
+                        uint32_t count) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene, GE_RT_ERROR_INVALID_ARGUMENT, "null scene");
   GE_CHECK(ctx, flags <= GE_RT_BUILD_TOP, GE_RT_ERROR_INVALID_ARGUMENT, "unknown build flags");
@@ -722,9 +747,11 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   s.built = true;
   s.version = next_version();
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
-int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, uint32_t* node_count, uint32_t* prim_count) {
+int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, uint32_t* node_count, uint32_t* prim_count) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene && scene->dev.built, GE_RT_ERROR_INVALID_SCENE, "no structure built");
   const SceneDev& s = scene->dev;
@@ -737,10 +764,12 @@ int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, 
   if (node_count) *node_count = s.mesh_node_count[mesh];
   if (prim_count) *prim_count = s.mesh_tri_count[mesh];
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_bvh_download(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, ge_bvh_node* nodes,
-                           ge_bvh_triangle* triangles, uint32_t* instance_slots, uint32_t* morton) {
+                           ge_bvh_triangle* triangles, uint32_t* instance_slots, uint32_t* morton) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene && scene->dev.built, GE_RT_ERROR_INVALID_SCENE, "no structure built");
   const SceneDev& s = scene->dev;
@@ -764,6 +793,8 @@ int32_t ge_rt_bvh_download(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t me
   if (morton && np)
     GE_HIP(ctx, hipMemcpy(morton, s.blas_morton.as<uint32_t>() + s.mesh_tri_base[mesh], size_t(np) * 4, hipMemcpyDeviceToHost));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 }  // extern "C"
@@ -823,7 +854,7 @@ int32_t resolve_checks(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow
 extern "C" {
 
 int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* rays, uint32_t count, uint32_t mask,
-                            ge_hit* hits, uint32_t flags) {
+                            ge_hit* hits, uint32_t flags) try {
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, (rays && hits) || count == 0, GE_RT_ERROR_INVALID_ARGUMENT, "null rays or hits");
@@ -849,10 +880,12 @@ int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_r
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_trace_any(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* rays, uint32_t count, uint32_t mask,
-                        uint32_t* occluded, uint32_t flags) {
+                        uint32_t* occluded, uint32_t flags) try {
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, (rays && occluded) || count == 0, GE_RT_ERROR_INVALID_ARGUMENT, "null rays or output");
@@ -878,10 +911,12 @@ int32_t ge_rt_trace_any(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray* 
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_ray_visibility_params* params,
-                             uint64_t* words, uint32_t flags) {
+                             uint64_t* words, uint32_t flags) try {
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
@@ -906,6 +941,8 @@ int32_t ge_rt_ray_visibility(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 }  // extern "C"
@@ -984,7 +1021,7 @@ int32_t path_copy_out(ge_rt_ctx* ctx, const ge_path_trace_params* params, float*
 extern "C" {
 
 int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params, float* accum,
-                         uint32_t* coverage, uint32_t flags) {
+                         uint32_t* coverage, uint32_t flags) try {
   ge_path_trace_params p;
   SceneView svl;
   int32_t r = path_setup(ctx, scene, params, &p, &svl);
@@ -1009,10 +1046,12 @@ int32_t ge_rt_path_trace(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path
   }
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   return path_copy_out(ctx, params, accum, coverage, flags, d_accum, d_cov);
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
-                                     const ge_resolve_material* materials) {
+                                     const ge_resolve_material* materials) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene && (materials || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or materials");
   SceneDev& s = scene->dev;
@@ -1029,10 +1068,12 @@ int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   GE_HIP(ctx, refresh_masks(s, ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
-                                  const ge_resolve_light* lights) {
+                                  const ge_resolve_light* lights) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene && (lights || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or lights");
   SceneDev& s = scene->dev;
@@ -1046,11 +1087,13 @@ int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t f
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_path_trace_params* params,
                            const float* d_image, const ge_resolve_material* sampling, float* image,
-                           double* grad_materials, double* grad_lights, uint32_t flags) {
+                           double* grad_materials, double* grad_lights, uint32_t flags) try {
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, params, GE_RT_ERROR_INVALID_ARGUMENT, "null params");
@@ -1157,10 +1200,12 @@ int32_t ge_rt_path_adjoint(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_pa
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_tonemap(ge_rt_ctx* ctx, const ge_path_trace_params* params, const float* accum, const uint32_t* coverage,
-                      uint32_t* output, uint32_t flags) {
+                      uint32_t* output, uint32_t flags) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, params && accum && coverage && output, GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
   GE_CHECK(ctx, params->sky_params == 0, GE_RT_ERROR_UNSUPPORTED, "the physical sky (sky_params) is not supported");
@@ -1184,10 +1229,12 @@ int32_t ge_rt_tonemap(ge_rt_ctx* ctx, const ge_path_trace_params* params, const 
   if (!dev) GE_HIP(ctx, hipMemcpyAsync(output, dout, n * 4, hipMemcpyDeviceToHost, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_params* params,
-                          const ge_shadow_receiver* receivers, uint32_t* blocked, uint32_t flags) {
+                          const ge_shadow_receiver* receivers, uint32_t* blocked, uint32_t flags) try {
   int32_t r = ready(ctx, scene);
   if (r != GE_RT_OK) return r;
   GE_CHECK(ctx, params && ((receivers && blocked) || params->count == 0), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
@@ -1213,10 +1260,12 @@ int32_t ge_rt_shadow_pass(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_sha
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
-                             const uint64_t* visibility, uint32_t* out, uint32_t flags) {
+                             const uint64_t* visibility, uint32_t* out, uint32_t flags) try {
   uint64_t pixels = 0, entries = 0;
   int32_t r = resolve_checks(ctx, scene, params, visibility, out, &pixels, &entries);
   if (r != GE_RT_OK || entries == 0) return r;
@@ -1245,10 +1294,12 @@ int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_shadow_resolve_noalloc(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_shadow_resolve_params* params,
-                                     const uint64_t* visibility, uint32_t* out, uint32_t flags) {
+                                     const uint64_t* visibility, uint32_t* out, uint32_t flags) try {
   uint64_t pixels = 0, entries = 0;
   int32_t r = resolve_checks(ctx, scene, params, visibility, out, &pixels, &entries);
   if (r != GE_RT_OK || entries == 0) return r;
@@ -1269,11 +1320,13 @@ int32_t ge_rt_shadow_resolve_noalloc(ge_rt_ctx* ctx, const ge_rt_scene* scene, c
                                     reinterpret_cast<uint32_t*>(ctx->static_scratch), ctx->persistent_blocks,
                                     alpha_rule(s, (params->flags & GE_SHADOW_ALPHA_MASKS) ? GE_RT_ALPHA_MASKS : 0u)));
   return finish(ctx, false);
+} catch (...) {
+  return host_failure(ctx);
 }
 
 // ---- the radiance cache ----------------------------------------------------------------------------
 int32_t ge_rt_cache_create(ge_rt_ctx* ctx, const float box_min[3], const float box_max[3], uint32_t seed,
-                           ge_rt_cache** out) {
+                           ge_rt_cache** out) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, out && box_min && box_max, GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
@@ -1295,6 +1348,8 @@ int32_t ge_rt_cache_create(ge_rt_ctx* ctx, const float box_min[3], const float b
   }
   *out = c;
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 void ge_rt_cache_destroy(ge_rt_ctx* ctx, ge_rt_cache* cache) {
@@ -1307,7 +1362,7 @@ void ge_rt_cache_destroy(ge_rt_ctx* ctx, ge_rt_cache* cache) {
   delete cache;
 }
 
-int32_t ge_rt_cache_weights(ge_rt_ctx* ctx, ge_rt_cache* cache, float* get, const float* set) {
+int32_t ge_rt_cache_weights(ge_rt_ctx* ctx, ge_rt_cache* cache, float* get, const float* set) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache, GE_RT_ERROR_INVALID_ARGUMENT, "null cache");
   GE_HIP(ctx, hipSetDevice(ctx->device));
@@ -1319,18 +1374,22 @@ int32_t ge_rt_cache_weights(ge_rt_ctx* ctx, ge_rt_cache* cache, float* get, cons
     GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
-int32_t ge_rt_cache_gradient(ge_rt_ctx* ctx, ge_rt_cache* cache, float* out) {
+int32_t ge_rt_cache_gradient(ge_rt_ctx* ctx, ge_rt_cache* cache, float* out) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache && out, GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
   GE_HIP(ctx, hipSetDevice(ctx->device));
   GE_HIP(ctx, cache->net.get_gradient(ctx->stream, out));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_cache_infer(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_query* queries, uint32_t n,
-                          float* radiance, uint32_t flags) {
+                          float* radiance, uint32_t flags) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache && (n == 0 || (queries && radiance)), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
   if (n == 0) return GE_RT_OK;
@@ -1352,6 +1411,8 @@ int32_t ge_rt_cache_infer(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_que
   if (!dev) GE_HIP(ctx, hipMemcpyAsync(radiance, dout, size_t(n) * 12, hipMemcpyDeviceToHost, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 namespace {
@@ -1407,7 +1468,7 @@ hipError_t train_slices(ge_rt_ctx* ctx, RadianceCache& net, uint32_t n, uint32_t
 }  // namespace
 
 int32_t ge_rt_cache_train(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_sample* samples, uint32_t n,
-                          float learning_rate, double* loss, uint32_t flags) {
+                          float learning_rate, double* loss, uint32_t flags) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache && (n == 0 || samples), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
   GE_CHECK(ctx, finite_rate(learning_rate), GE_RT_ERROR_INVALID_ARGUMENT, "the learning rate must be finite and >= 0");
@@ -1424,10 +1485,12 @@ int32_t ge_rt_cache_train(ge_rt_ctx* ctx, ge_rt_cache* cache, const ge_cache_sam
   GE_HIP(ctx, cache->net.train(ctx->stream, ds, 1, n, learning_rate));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return read_loss(ctx, cache, loss);
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_cache_samples(ge_rt_ctx* ctx, ge_rt_cache* cache, ge_cache_sample* out, uint32_t capacity,
-                            uint32_t* count, uint32_t flags) {
+                            uint32_t* count, uint32_t flags) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache && count && (capacity == 0 || out), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
   GE_HIP(ctx, hipSetDevice(ctx->device));
@@ -1439,11 +1502,13 @@ int32_t ge_rt_cache_samples(ge_rt_ctx* ctx, ge_rt_cache* cache, ge_cache_sample*
   }
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_cache_collect(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
                             const ge_path_trace_params* params, ge_cache_sample* samples, uint32_t capacity,
-                            uint32_t* count, uint32_t flags) {
+                            uint32_t* count, uint32_t flags) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache && count && (capacity == 0 || samples), GE_RT_ERROR_INVALID_ARGUMENT, "null argument");
   GE_CHECK(ctx, !(flags & GE_RT_PT_MEGAKERNEL), GE_RT_ERROR_UNSUPPORTED, "the cache's paths run on the wavefront");
@@ -1457,11 +1522,13 @@ int32_t ge_rt_cache_collect(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cach
   }
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_cache_update(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
                            const ge_path_trace_params* params, uint32_t samples, uint32_t steps, float learning_rate,
-                           double* loss, uint32_t flags) {
+                           double* loss, uint32_t flags) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache && steps > 0, GE_RT_ERROR_INVALID_ARGUMENT, "null cache or no steps");
   GE_CHECK(ctx, finite_rate(learning_rate), GE_RT_ERROR_INVALID_ARGUMENT, "the learning rate must be finite and >= 0");
@@ -1472,12 +1539,14 @@ int32_t ge_rt_cache_update(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache
   if (r != GE_RT_OK) return r;
   GE_HIP(ctx, train_slices(ctx, cache->net, std::min(count, samples), steps, learning_rate));
   return read_loss(ctx, cache, loss);
+} catch (...) {
+  return host_failure(ctx);
 }
 
 int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_cache* cache,
                                 const ge_path_trace_params* params, uint32_t depth,
                                 const ge_cache_train_params* train, float* accum, uint32_t* coverage, double* loss,
-                                uint32_t flags) {
+                                uint32_t flags) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, cache, GE_RT_ERROR_INVALID_ARGUMENT, "null cache");
   GE_CHECK(ctx, !(flags & GE_RT_PT_MEGAKERNEL), GE_RT_ERROR_UNSUPPORTED, "the cached tracer runs on the wavefront");
@@ -1538,9 +1607,11 @@ int32_t ge_rt_path_trace_cached(ge_rt_ctx* ctx, const ge_rt_scene* scene, ge_rt_
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (training && (r = read_loss(ctx, cache, loss)) != GE_RT_OK) return r;
   return path_copy_out(ctx, params, accum, coverage, flags, d_accum, d_cov);
+} catch (...) {
+  return host_failure(ctx);
 }
 
-int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {
+int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, out, GE_RT_ERROR_INVALID_ARGUMENT, "null output");
   GE_HIP(ctx, hipSetDevice(ctx->device));
@@ -1579,14 +1650,18 @@ int32_t ge_rt_stats(ge_rt_ctx* ctx, ge_rt_counters* out) {
   out->closest_launches = ctx->stage_launches[0];
   out->any_launches = ctx->stage_launches[1];
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
-int32_t ge_rt_reset_stats(ge_rt_ctx* ctx) {
+int32_t ge_rt_reset_stats(ge_rt_ctx* ctx) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_HIP(ctx, hipSetDevice(ctx->device));
   GE_HIP(ctx, hipMemsetAsync(ctx->counters.ptr, 0, 88, ctx->stream));  // the walks' head is cleared with them
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
 }
 
 }  // extern "C"

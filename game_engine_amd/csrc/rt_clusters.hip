@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -175,14 +176,10 @@ ge_resolve_material resolve_material(const FileMaterial& f, uint32_t flags) {
   return m;
 }
 
-}  // namespace
-
-extern "C" {
-
-int32_t ge_clusters_read(const uint8_t* bytes, uint64_t size, uint32_t flags, ge_clusters** out, char* error,
-                         size_t error_size) {
-  if (!out || (!bytes && size)) return GE_RT_ERROR_INVALID_ARGUMENT;
-  *out = nullptr;
+// The reader proper; ge_clusters_read turns a host allocation failure anywhere in it into
+// GE_RT_ERROR_OUT_OF_MEMORY, and `c` frees a half-built result on every early return.
+int32_t read_clusters(const uint8_t* bytes, uint64_t size, uint32_t flags, ge_clusters** out, char* error,
+                      size_t error_size) {
   if (size < sizeof(Header))
     return fail(error, error_size, "cluster file is truncated: " + std::to_string(size) +
                                        " bytes, the 32-byte header does not fit");
@@ -233,7 +230,7 @@ int32_t ge_clusters_read(const uint8_t* bytes, uint64_t size, uint32_t flags, ge
   if (!s_scalars) return fail(error, error_size, missing);
   if (s_scalars->element_count != 1) return fail(error, error_size, "cluster file scalars section holds other than one record");
 
-  auto* c = new (std::nothrow) ge_clusters();
+  std::unique_ptr<ge_clusters> c(new (std::nothrow) ge_clusters());
   if (!c) return GE_RT_ERROR_OUT_OF_MEMORY;
   std::vector<ge_cluster_desc> all;
   std::vector<LodDesc> lod;
@@ -246,10 +243,7 @@ int32_t ge_clusters_read(const uint8_t* bytes, uint64_t size, uint32_t flags, ge
   Scalars sc;
   std::memcpy(&sc, bytes + s_scalars->offset, sizeof(sc));
   const uint64_t nv = s_vertices->element_count, nt = s_triangles->element_count;
-  auto refuse = [&](const std::string& msg) {
-    delete c;
-    return fail(error, error_size, msg);
-  };
+  auto refuse = [&](const std::string& msg) { return fail(error, error_size, msg); };
   if (lod.size() != all.size()) return refuse("cluster file has " + std::to_string(lod.size()) + " lod records for " +
                                               std::to_string(all.size()) + " clusters");
   if (c->attributes.size() != nv) return refuse("cluster file attributes are not parallel to its vertices");
@@ -348,8 +342,29 @@ int32_t ge_clusters_read(const uint8_t* bytes, uint64_t size, uint32_t flags, ge
   in.scale_max = 1.0f;
   in.flags = GE_INSTANCE_STATIC_RT | 1u;
   in.deform = GE_INVALID_DEFORM;
-  *out = c;
+  *out = c.release();
   return GE_RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ge_clusters_read(const uint8_t* bytes, uint64_t size, uint32_t flags, ge_clusters** out, char* error,
+                         size_t error_size) {
+  if (!out || (!bytes && size)) return GE_RT_ERROR_INVALID_ARGUMENT;
+  *out = nullptr;
+  try {
+    return read_clusters(bytes, size, flags, out, error, error_size);
+  } catch (const std::bad_alloc&) {
+    static const char k_message[] = "out of host memory";
+    if (error && error_size) {
+      const size_t n = std::min(error_size - 1, sizeof(k_message) - 1);
+      std::memcpy(error, k_message, n);
+      error[n] = '\0';
+    }
+    return GE_RT_ERROR_OUT_OF_MEMORY;
+  }
 }
 
 void ge_clusters_scene(const ge_clusters* c, ge_scene_desc* out) {

@@ -667,7 +667,8 @@ struct Walker {
   // slab test's other bound held at entry). Then the inner children.
   GE_DEV void leaves(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
     uint32_t lm = leaf_mask;
-#pragma unroll(Masks ? 1 : 4)
+    constexpr int unroll = Masks ? 1 : 4;
+#pragma unroll unroll
     for (int trip = 0; trip < 4; ++trip) {
       if (!lm) break;
       const int c = __builtin_ctz(lm);

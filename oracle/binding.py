@@ -55,7 +55,8 @@ def ref_lib():
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "_build", "libge_oracle.so")
+        # GE_ORACLE_LIB: another build of the same oracle (the sanitized one, `make -C oracle asan`).
+        path = os.environ.get("GE_ORACLE_LIB") or os.path.join(_HERE, "_build", "libge_oracle.so")
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
