@@ -341,10 +341,10 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->build.keys[0], &ctx->build.keys[1], &ctx->build.vals[0], &ctx->build.vals[1], &ctx->build.hist,
                     &ctx->build.scan_tmp, &ctx->build.node_box, &ctx->build.node_parent, &ctx->build.seg_cbox,
-                    &ctx->build.seg_scale, &ctx->build.anc[0], &ctx->build.anc[1], &ctx->build.dist[0],
-                    &ctx->build.dist[1], &ctx->build.levels, &ctx->counters})
+                    &ctx->build.seg_scale, &ctx->build.depth, &ctx->build.buckets, &ctx->build.bucket_count,
+                    &ctx->build.flag, &ctx->counters})
     b->release();
-  if (ctx->build.host_levels) (void)hipHostFree(ctx->build.host_levels);
+  if (ctx->build.host_flag) (void)hipHostFree(ctx->build.host_flag);
   for (DevBuf& b : ctx->io) b.release();
   ctx->resolve_rays.release();
   for (DevBuf& b : ctx->tmp) b.release();
@@ -595,7 +595,12 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, s.blas_morton.reserve(size_t(std::max(s.tri_pool, 1u)) * 4));
   GE_HIP(ctx, s.mesh_roots.reserve(size_t(std::max(s.mesh_count, 1u)) * 24));
   GE_HIP(ctx, s.mesh_height.reserve(size_t(std::max(s.mesh_count, 1u)) * 4));
+  GE_HIP(ctx, ctx->build.flag.reserve(4));
+  if (!ctx->build.host_flag)
+    GE_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->build.host_flag), 4));
+  uint32_t* flag = ctx->build.flag.as<uint32_t>();
   GE_HIP(ctx, hipEventRecord(ctx->bev0, st));
+  GE_HIP(ctx, hipMemsetAsync(flag, 0, 4, st));
 
   // ---- the meshes ----
   std::vector<uint32_t> list;
@@ -664,7 +669,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), uint32_t(seg_first.size()), ctx->tmp[3].as<uint32_t>(),
                                md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
                                ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
-                               std::max(s.node_pool, 1u), s.blas_wide.as<WideNode>(), nseg_global, n_global));
+                               std::max(s.node_pool, 1u), s.blas_wide.as<WideNode>(), nseg_global, n_global,
+                               seg_count.data(), flag));
     GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
                                         ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
                                         ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));
@@ -719,15 +725,19 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, build_segments(ctx->build, st, nl, boxes, 1, seg_of, tm, tm + 1, tm + 2, s.tlas_nodes.as<ge_bvh_node>(), perm,
                                codes, s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), nullptr, s.tlas_node_count,
                                s.tlas_wide.as<WideNode>(), nl > k_block_build_max ? 1u : 0u,
-                               nl > k_block_build_max ? nl : 0u));
+                               nl > k_block_build_max ? nl : 0u, &nl, flag));
     GE_HIP(ctx, launch_gather_slots(st, nl, perm, s.live_ids.as<uint32_t>(), s.tlas_slots.as<uint32_t>()));
     GE_HIP(ctx, hipMemcpyAsync(s.tlas_morton.ptr, codes, size_t(nl) * 4, hipMemcpyDeviceToDevice, st));
   }
   GE_HIP(ctx, hipEventRecord(ctx->bev1, st));
+  GE_HIP(ctx, hipMemcpyAsync(ctx->build.host_flag, flag, 4, hipMemcpyDeviceToHost, st));
   GE_HIP(ctx, hipStreamSynchronize(st));
   float ms = 0.0f;
   GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->bev0, ctx->bev1));
   ctx->last_build_ms = ms;
+  if (*ctx->build.host_flag) s.built = false;
+  GE_CHECK(ctx, !*ctx->build.host_flag, GE_RT_ERROR_UNSUPPORTED,
+           "a structure deeper than 254 levels, or too unbalanced for the build's tiles");
   // The traversal stack holds at most three deferred children per wide level (two binary levels) of
   // each structure plus the sentinel between them.
   s.h_mesh_height.assign(s.mesh_count, 0);

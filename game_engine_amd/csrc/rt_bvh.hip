@@ -4,12 +4,13 @@
 // centroid bounds (10 bits an axis). A stable LSD radix sort on (segment << 30 | code) orders each
 // segment by code with ties in primitive order. Karras 2012 emits each segment's binary radix tree
 // from the sorted codes (one thread per internal node, the index breaking equal codes). The boxes
-// are fitted level by level from the deepest: each node's depth comes from pointer jumping over the
-// parent links, one pass of the radix sort groups the nodes by depth, and one launch per level
-// unions each node's two children, whose boxes the previous launch wrote. The kernel boundary is the
-// only synchronization: no per-node atomics or device-scope fences, which on a part whose L2 is
-// split over eight XCDs cost a cache write-back each. Everything is integer work or exact float
-// min/max, so the structure is bit-identical to the oracle's sequential build.
+// are fitted bottom-up tile by tile (a node's range of sorted primitives holds its own index, so the
+// nodes inside a run of primitives are one workgroup's work), the nodes crossing a tile's edge by
+// the next level's wider tiles, then the depths and the 4-wide collapse come back down the levels.
+// Kernel boundaries and workgroup-local counters are the only synchronization: no device-scope
+// atomics or fences, which on a part whose L2 is split over eight XCDs cost a cache write-back
+// each. Everything is integer work or exact float min/max, so the structure is bit-identical to the
+// oracle's sequential build.
 //
 // Memory: keys 8 B + values 4 B per primitive, double-buffered, a 256 x tiles histogram, and the
 // nodes themselves (64 B each). Bytes per triangle of the whole build are stated beside the
@@ -260,6 +261,180 @@ hipError_t exclusive_scan(uint32_t* data, uint32_t n, DevBuf& tmp, size_t tmp_of
   return hipGetLastError();
 }
 
+// ---- the fit and the collapse, tile by tile ------------------------------------------------
+// Karras' internal node i covers a contiguous range [lo, hi] of sorted primitives with i = lo or
+// i = hi, so a node whose range lies inside a run of primitives has its index inside that run too.
+// A node's level is the first in a ladder of tile widths (2^10 primitives, then 64 times wider at
+// each level) at which one tile holds its whole range; Karras' kernel, which knows the range,
+// drops every node above level 1 (a few in a hundred) into its tile's list. Then:
+// - the levels above 1, top first, give their nodes their depths (a walk up the tile to its top
+//   node, plus the depth of that node's parent, which the level above wrote);
+// - one launch takes every level-1 tile (99% of the nodes) in one workgroup from LDS: the leaf
+//   boxes and child links loaded once, the nodes fitted bottom-up (a node whose children are leaves
+//   starts; the last child to finish a node moves on to it, by LDS counters), the tile's top nodes'
+//   boxes handed up, each node's depth, and the collapse of the even-depth ones into 4-wide nodes;
+// - the levels above 1, bottom first, fit their nodes the same way from the boxes below, and one
+//   launch collapses their even-depth nodes.
+// The boxes are exact min/max, so the order of the unions does not change them, and every node is
+// fitted once. No pointer jumping, no depth sort, no level-per-launch fit and no host round trip.
+constexpr uint32_t kTileShift = 10;     // a level-1 tile: 1,024 primitives
+constexpr uint32_t kTileMax = 1u << kTileShift;
+constexpr uint32_t kTileThreads = 512;
+constexpr uint32_t kFanShift = 6;       // 64 tiles of a level make one of the next
+constexpr uint32_t kSparseCap = 4096;   // nodes a tile above level 1 holds
+constexpr uint32_t kSparseThreads = 512;
+constexpr uint32_t kFitMaxDepth = 254;  // deeper structures are refused (the flag)
+constexpr uint32_t kMaxLevels = 6;
+constexpr uint16_t kNoSlot = 0xffffu;
+constexpr uint16_t kLeafRef = 0x8000u;
+
+struct TileArgs {
+  uint32_t nseg, levels;
+  uint32_t bucket_base[kMaxLevels + 1];  // level l's lists start at bucket_base[l] (l >= 1)
+  const uint32_t* seg_first;
+  const uint32_t* seg_count;
+  const uint32_t* node_base;
+  const uint32_t* root_index;
+  const float* boxes;
+  const uint32_t* perm;
+  ge_bvh_node* nodes;
+  const uint32_t* node_parent;
+  float* node_box;  // 8 floats a node above level 1 or at a level-1 tile's top: box, height bits
+  float* roots;
+  uint32_t* heights;
+  uint32_t* list;   // kSparseCap entries a tile above level 1
+  uint32_t* count;
+  uint32_t* depth;  // per batch entry: the depth of the node of that index above level 1
+  WideNode* wide;
+  uint32_t* flag;   // set on a tile over capacity or a structure deeper than kFitMaxDepth
+};
+
+__device__ __forceinline__ uint32_t level_shift(uint32_t l) { return kTileShift + kFanShift * l; }
+__device__ __forceinline__ uint32_t tiles_of(uint32_t n, uint32_t shift) {
+  return uint32_t((uint64_t(n) + (1ull << shift) - 1) >> shift);
+}
+// Tile t of a level as (segment, tile within it), and back.
+__device__ void tile_place(const TileArgs& a, uint32_t shift, uint32_t t, uint32_t& s, uint32_t& k) {
+  for (s = 0; s + 1 < a.nseg; ++s) {
+    const uint32_t m = tiles_of(a.seg_count[s], shift);
+    if (t < m) break;
+    t -= m;
+  }
+  k = t;
+}
+__device__ uint32_t tile_index(const TileArgs& a, uint32_t shift, uint32_t s, uint32_t k) {
+  uint32_t base = 0;
+  for (uint32_t q = 0; q < s; ++q) base += tiles_of(a.seg_count[q], shift);
+  return base + k;
+}
+// The level of a node of range [lo, hi]; a node above level 1 goes into its tile's list.
+__device__ void file_node(const TileArgs& a, uint32_t s, uint32_t i, uint32_t lo, uint32_t hi) {
+  uint32_t l = 0;
+  while (l + 1 < a.levels && (lo >> level_shift(l)) != (hi >> level_shift(l))) ++l;
+  if (l == 0) return;
+  const uint32_t b = a.bucket_base[l] + tile_index(a, level_shift(l), s, i >> level_shift(l));
+  const uint32_t pos = atomicAdd(&a.count[b], 1u);
+  if (pos < kSparseCap) a.list[size_t(b) * kSparseCap + pos] = i;
+  else atomicOr(a.flag, 1u);
+}
+
+// Fits internal node i of segment s from global memory: its record's child boxes and parent, its
+// box and height (node_box), and the root's box and height.
+__device__ void fit_node(const TileArgs& a, uint32_t s, uint32_t i) {
+  const uint32_t first = a.seg_first[s], nb = a.node_base[s];
+  ge_bvh_node* nd = a.nodes + nb + i;
+  float b[2][6];
+  uint32_t h[2];
+  for (int c = 0; c < 2; ++c) {
+    const uint32_t ch = nd->child[c];
+    if (ch & GE_BVH_LEAF) {
+      const uint32_t p = a.perm[first + (ch & ~GE_BVH_LEAF)];
+      for (int q = 0; q < 6; ++q) b[c][q] = a.boxes[size_t(p) * 6 + q];
+      h[c] = 0;
+    } else {
+      const float* cb = a.node_box + size_t(nb + ch) * 8;
+      for (int q = 0; q < 6; ++q) b[c][q] = cb[q];
+      h[c] = __float_as_uint(cb[6]);
+    }
+  }
+  for (int q = 0; q < 3; ++q) {
+    nd->lo0[q] = b[0][q];
+    nd->hi0[q] = b[0][3 + q];
+    nd->lo1[q] = b[1][q];
+    nd->hi1[q] = b[1][3 + q];
+  }
+  nd->parent = i == 0 ? GE_BVH_NONE : a.node_parent[nb + i];
+  float u[6];
+  for (int q = 0; q < 3; ++q) {
+    u[q] = __builtin_fminf(b[0][q], b[1][q]);
+    u[3 + q] = __builtin_fmaxf(b[0][3 + q], b[1][3 + q]);
+  }
+  const uint32_t ht = 1 + (h[0] > h[1] ? h[0] : h[1]);
+  float* o = a.node_box + size_t(nb + i) * 8;
+  for (int q = 0; q < 6; ++q) o[q] = u[q];
+  o[6] = __uint_as_float(ht);
+  if (i == 0) {
+    const uint32_t out = a.root_index ? a.root_index[s] : s;
+    for (int q = 0; q < 6; ++q) a.roots[out * 6 + q] = u[q];
+    a.heights[out] = ht;
+  }
+}
+
+// Stores of whole records as 16-byte vectors (records and wide nodes are 16-byte aligned).
+__device__ __forceinline__ void store_record(ge_bvh_node* out, const float* lo0, const float* hi0, const float* lo1,
+                                             const float* hi1, uint32_t c0, uint32_t c1, uint32_t parent,
+                                             uint32_t count) {
+  uint4* o = reinterpret_cast<uint4*>(out);
+  o[0] = make_uint4(__float_as_uint(lo0[0]), __float_as_uint(lo0[1]), __float_as_uint(lo0[2]), __float_as_uint(hi0[0]));
+  o[1] = make_uint4(__float_as_uint(hi0[1]), __float_as_uint(hi0[2]), __float_as_uint(lo1[0]), __float_as_uint(lo1[1]));
+  o[2] = make_uint4(__float_as_uint(lo1[2]), __float_as_uint(hi1[0]), __float_as_uint(hi1[1]), __float_as_uint(hi1[2]));
+  o[3] = make_uint4(c0, c1, parent, count);
+}
+// A wide node from four (reference, box) entries, of which the first `valid` are real; the rest
+// get +infinity boxes (every slab test against them fails) and GE_BVH_NONE.
+__device__ __forceinline__ void store_wide(WideNode* out, const uint32_t ref[4], const float* box[4], int valid) {
+  float v[6][4];
+  uint32_t c[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool on = q < valid;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) v[f][q] = on ? box[q][f] : __builtin_huge_valf();
+    c[q] = on ? ref[q] : GE_BVH_NONE;
+  }
+  uint4* o = reinterpret_cast<uint4*>(out);
+#pragma unroll
+  for (int f = 0; f < 6; ++f)
+    o[f] = make_uint4(__float_as_uint(v[f][0]), __float_as_uint(v[f][1]), __float_as_uint(v[f][2]), __float_as_uint(v[f][3]));
+  o[6] = make_uint4(c[0], c[1], c[2], c[3]);
+  o[7] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// The 4-wide node of binary node i from the records in global memory: its children's children (a
+// leaf child taken as it is), in child order, with their boxes, at the node's own binary index.
+__device__ void collapse_node(const TileArgs& a, uint32_t nb, uint32_t i) {
+  const ge_bvh_node nd = a.nodes[nb + i];
+  const bool lx = nd.child[0] & GE_BVH_LEAF, ly = nd.child[1] & GE_BVH_LEAF;
+  const ge_bvh_node mx = a.nodes[nb + (lx ? i : nd.child[0])];
+  const ge_bvh_node my = a.nodes[nb + (ly ? i : nd.child[1])];
+  const uint32_t ref[4] = {lx ? nd.child[0] : mx.child[0], lx ? (ly ? nd.child[1] : my.child[0]) : mx.child[1],
+                           lx ? my.child[1] : (ly ? nd.child[1] : my.child[0]), my.child[1]};
+  const float* lo[4] = {lx ? nd.lo0 : mx.lo0, lx ? (ly ? nd.lo1 : my.lo0) : mx.lo1, lx ? my.lo1 : (ly ? nd.lo1 : my.lo0),
+                        my.lo1};
+  const float* hi[4] = {lx ? nd.hi0 : mx.hi0, lx ? (ly ? nd.hi1 : my.hi0) : mx.hi1, lx ? my.hi1 : (ly ? nd.hi1 : my.hi0),
+                        my.hi1};
+  float boxes[4][6];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      boxes[e][q] = lo[e][q];
+      boxes[e][3 + q] = hi[e][q];
+    }
+  const float* box[4] = {boxes[0], boxes[1], boxes[2], boxes[3]};
+  store_wide(a.wide + nb + i, ref, box, (lx ? 1 : 2) + (ly ? 1 : 2));
+}
+
 // ---- Karras 2012 --------------------------------------------------------------------------------
 __device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, int n, int i, int j) {
   if (j < 0 || j > n - 1) return -1;
@@ -268,9 +443,12 @@ __device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, int n, i
   return 32 + __clz(uint32_t(i) ^ uint32_t(j));
 }
 
+// Karras' emission; with more than one tile level, each node above level 1 is also filed into its
+// tile's list.
 __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_keys, const uint32_t* __restrict__ seg_of,
                          const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
-                         const uint32_t* __restrict__ node_base, ge_bvh_node* nodes, uint32_t* node_parent) {
+                         const uint32_t* __restrict__ node_base, ge_bvh_node* nodes, uint32_t* node_parent,
+                         TileArgs ta) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_total) return;
   const uint32_t s = seg_of[g];
@@ -320,150 +498,249 @@ __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_k
   if (i == 0) node.parent = GE_BVH_NONE;
   if (!(c0 & GE_BVH_LEAF)) node_parent[node_base[s] + gamma] = uint32_t(i);
   if (!(c1 & GE_BVH_LEAF)) node_parent[node_base[s] + gamma + 1] = uint32_t(i);
+  if (ta.levels > 1) file_node(ta, s, uint32_t(i), uint32_t(lo), uint32_t(hi));
 }
 
+// ---- levels above 1: a tile's list in LDS, sorted, each node's parent slot inside the tile ----
+struct SparseTile {
+  uint32_t list[kSparseCap];
+  uint32_t arrived[kSparseCap];
+  uint16_t pslot[kSparseCap];
+  uint8_t pending[kSparseCap];
+};
 
-// ---- the fit, level by level ----------------------------------------------------------------
-// A node is named by the batch index of its segment's primitive of the same index: internal node i
-// of segment s is batch entry seg_first[s] + i (i < n - 1, or i = 0 for a one-primitive segment),
-// so a batch of n primitives has at most n nodes and every array below is n long.
-constexpr uint32_t kNoNode = 0xffffffffu;
-constexpr uint32_t kNoDepth = 0xffffffffu;  // the distance of a batch entry that names no node
-constexpr uint32_t kNotANode = 255u;        // its depth key
-constexpr int kJumps = 8;                   // pointer-jumping rounds: depths up to 2^8 - 1
-
-__global__ void k_depth_init(uint32_t n_total, const uint32_t* __restrict__ seg_of,
-                             const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
-                             const uint32_t* __restrict__ node_base, const uint32_t* __restrict__ node_parent,
-                             uint32_t* anc, uint32_t* dist) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_total) return;
-  const uint32_t s = seg_of[g];
-  const uint32_t first_g = seg_first[s], n = seg_count[s];
-  const uint32_t i = g - first_g;
-  const bool node = n == 1 ? i == 0 : i < n - 1;
-  uint32_t a = kNoNode;
-  if (node && i != 0) a = first_g + node_parent[node_base[s] + i];
-  anc[g] = a;
-  dist[g] = node ? (a == kNoNode ? 0u : 1u) : kNoDepth;
-}
-
-// One round of pointer jumping: the distance to the ancestor doubles its reach.
-__global__ void k_depth_jump(uint32_t n_total, const uint32_t* __restrict__ anc, const uint32_t* __restrict__ dist,
-                             uint32_t* anc_out, uint32_t* dist_out) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_total) return;
-  const uint32_t a = anc[g];
-  uint32_t d = dist[g];
-  uint32_t up = kNoNode;
-  if (a != kNoNode) {
-    d += dist[a];
-    up = anc[a];
-  }
-  anc_out[g] = up;
-  dist_out[g] = d;
-}
-
-// The sort keys: a node's depth (kNotANode for the rest); a node short of its root after kJumps
-// rounds, or as deep as kNotANode, is deeper than any structure the traversal accepts and raises
-// the error flag.
-__global__ void k_depth_keys(uint32_t n_total, const uint32_t* __restrict__ anc, const uint32_t* __restrict__ dist,
-                             uint64_t* keys, uint32_t* vals, uint32_t* error) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_total) return;
-  uint32_t d = dist[g];
-  if (d == kNoDepth) {
-    d = kNotANode;
-  } else if (anc[g] != kNoNode || d >= kNotANode) {
-    atomicOr(error, 1u);
-    d = kNotANode - 1;
-  }
-  keys[g] = d;
-  vals[g] = g;
-}
-
-// Each level's first position in the depth-sorted order: the exclusive-scanned histogram's entry
-// for (digit d, tile 0). `out` gets 256 starts, then the error flag.
-__global__ void k_level_starts(const uint32_t* __restrict__ scanned, uint32_t tiles, const uint32_t* error,
-                               uint32_t* out) {
-  const uint32_t d = threadIdx.x;
-  out[d] = scanned[size_t(d) * tiles];
-  if (d == 0) out[256] = *error;
-}
-
-__device__ __forceinline__ void leaf_box(const float* boxes, const uint32_t* perm, uint32_t g, float lo[3], float hi[3]) {
-  const uint32_t p = perm[g];
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = boxes[size_t(p) * 6 + a];
-    hi[a] = boxes[size_t(p) * 6 + 3 + a];
-  }
-}
-
-// One level of the fit: each node unions its children (a leaf's box, or the box and height the
-// previous launch wrote for an internal child) into its record, its box and its height.
-__global__ void k_fit_level(const uint32_t* __restrict__ level, uint32_t count, const float* __restrict__ boxes,
-                            const uint32_t* __restrict__ perm, const uint32_t* __restrict__ seg_of,
-                            const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
-                            const uint32_t* __restrict__ node_base, const uint32_t* __restrict__ root_index,
-                            ge_bvh_node* nodes, const uint32_t* __restrict__ node_parent, float* node_box,
-                            float* roots, uint32_t* heights) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  const uint32_t g = level[k];
-  const uint32_t s = seg_of[g];
-  const uint32_t first_g = seg_first[s];
-  const uint32_t nb = node_base[s];
-  const uint32_t i = g - first_g;
-  const uint32_t out = root_index ? root_index[s] : s;
-  ge_bvh_node& node = nodes[nb + i];
-  if (seg_count[s] == 1) {
-    float lo[3], hi[3];
-    leaf_box(boxes, perm, first_g, lo, hi);
-    for (int a = 0; a < 3; ++a) {
-      node.lo0[a] = lo[a];
-      node.hi0[a] = hi[a];
-      roots[out * 6 + a] = lo[a];
-      roots[out * 6 + 3 + a] = hi[a];
-    }
-    heights[out] = 1;
-    return;
-  }
-  float clo[2][3], chi[2][3];
-  uint32_t ch_height[2];
-  for (int c = 0; c < 2; ++c) {
-    const uint32_t ch = node.child[c];
-    if (ch & GE_BVH_LEAF) {
-      leaf_box(boxes, perm, first_g + (ch & ~GE_BVH_LEAF), clo[c], chi[c]);
-      ch_height[c] = 0;
-    } else {
-      const float* b = node_box + size_t(nb + ch) * 8;
-      for (int a = 0; a < 3; ++a) {
-        clo[c][a] = b[a];
-        chi[c][a] = b[3 + a];
+__device__ uint32_t sparse_setup(const TileArgs& a, SparseTile& sh, uint32_t b, uint32_t s, bool fit) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t nb = a.node_base[s];
+  uint32_t cnt = a.count[b];
+  if (cnt > kSparseCap) cnt = kSparseCap;  // flagged where it was filled
+  uint32_t p2 = 1;
+  while (p2 < cnt) p2 <<= 1;
+  const uint32_t* src = a.list + size_t(b) * kSparseCap;
+  for (uint32_t j = t; j < p2; j += kSparseThreads) sh.list[j] = j < cnt ? src[j] : 0xffffffffu;
+  __syncthreads();
+  for (uint32_t w = 2; w <= p2; w <<= 1)
+    for (uint32_t v = w >> 1; v > 0; v >>= 1) {
+      for (uint32_t i = t; i < p2; i += kSparseThreads) {
+        const uint32_t l = i ^ v;
+        if (l > i) {
+          const uint32_t x = sh.list[i], y = sh.list[l];
+          if ((x > y) == ((i & w) == 0)) {
+            sh.list[i] = y;
+            sh.list[l] = x;
+          }
+        }
       }
-      ch_height[c] = __float_as_uint(b[6]);
+      __syncthreads();
+    }
+  auto slot_of = [&](uint32_t i) -> uint32_t {
+    uint32_t lo = 0, hi = cnt;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (sh.list[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    return (lo < cnt && sh.list[lo] == i) ? lo : 0xffffffffu;
+  };
+  for (uint32_t j = t; j < cnt; j += kSparseThreads) {
+    const uint32_t i = sh.list[j];
+    const uint32_t ps = i == 0 ? 0xffffffffu : slot_of(a.node_parent[nb + i]);
+    sh.pslot[j] = ps == 0xffffffffu ? kNoSlot : uint16_t(ps);
+    sh.arrived[j] = 0;
+    if (fit) {
+      const ge_bvh_node* nd = a.nodes + nb + i;
+      uint32_t pend = 0;
+      for (int c = 0; c < 2; ++c) {
+        const uint32_t ch = nd->child[c];
+        if (!(ch & GE_BVH_LEAF) && slot_of(ch) != 0xffffffffu) ++pend;
+      }
+      sh.pending[j] = uint8_t(pend);
     }
   }
-  for (int a = 0; a < 3; ++a) {
-    node.lo0[a] = clo[0][a];
-    node.hi0[a] = chi[0][a];
-    node.lo1[a] = clo[1][a];
-    node.hi1[a] = chi[1][a];
+  __syncthreads();
+  return cnt;
+}
+
+// Depths of a level's nodes, top level first.
+__global__ void __launch_bounds__(kSparseThreads) k_depth_sparse(TileArgs a, uint32_t l) {
+  __shared__ SparseTile sh;
+  uint32_t s, k;
+  tile_place(a, level_shift(l), blockIdx.x, s, k);
+  const uint32_t b = a.bucket_base[l] + blockIdx.x;
+  const uint32_t cnt = sparse_setup(a, sh, b, s, false);
+  const uint32_t first = a.seg_first[s], nb = a.node_base[s];
+  for (uint32_t j = threadIdx.x; j < cnt; j += kSparseThreads) {
+    uint32_t q = j, steps = 0;
+    while (sh.pslot[q] != kNoSlot) {
+      q = sh.pslot[q];
+      ++steps;
+    }
+    const uint32_t top = sh.list[q];
+    const uint32_t d = (top == 0 ? 0u : a.depth[first + a.node_parent[nb + top]] + 1u) + steps;
+    a.depth[first + sh.list[j]] = d;
+    if (d > kFitMaxDepth) atomicOr(a.flag, 1u);
   }
-  const uint32_t h = 1 + (ch_height[0] > ch_height[1] ? ch_height[0] : ch_height[1]);
-  float* b = node_box + size_t(nb + i) * 8;
-  for (int a = 0; a < 3; ++a) {
-    b[a] = __builtin_fminf(clo[0][a], clo[1][a]);
-    b[3 + a] = __builtin_fmaxf(chi[0][a], chi[1][a]);
+}
+
+// The fit of a level's nodes, level 2 first: children below the level are read from node_box.
+__global__ void __launch_bounds__(kSparseThreads) k_fit_sparse(TileArgs a, uint32_t l) {
+  __shared__ SparseTile sh;
+  uint32_t s, k;
+  tile_place(a, level_shift(l), blockIdx.x, s, k);
+  const uint32_t cnt = sparse_setup(a, sh, a.bucket_base[l] + blockIdx.x, s, true);
+  for (uint32_t j = threadIdx.x; j < cnt; j += kSparseThreads) {
+    if (sh.pending[j] != 0) continue;
+    uint32_t q = j;
+    for (;;) {
+      fit_node(a, s, sh.list[q]);
+      const uint16_t p = sh.pslot[q];
+      if (p == kNoSlot) break;
+      __threadfence_block();
+      if (atomicAdd(&sh.arrived[p], 1u) + 1u < sh.pending[p]) break;
+      __threadfence_block();
+      q = p;
+    }
   }
-  b[6] = __uint_as_float(h);
-  if (i == 0) {
-    node.parent = GE_BVH_NONE;
-    for (int a = 0; a < 6; ++a) roots[out * 6 + a] = b[a];
-    heights[out] = h;
-    return;
+}
+
+// The collapse of the even-depth nodes above level 1, every such tile in one launch.
+__global__ void __launch_bounds__(kSparseThreads) k_collapse_sparse(TileArgs a) {
+  const uint32_t b = blockIdx.x;
+  uint32_t l = 1;
+  while (l + 1 < a.levels && b >= a.bucket_base[l + 1]) ++l;
+  uint32_t s, k;
+  tile_place(a, level_shift(l), b - a.bucket_base[l], s, k);
+  const uint32_t first = a.seg_first[s], nb = a.node_base[s];
+  const uint32_t cnt = min(a.count[b], kSparseCap);
+  for (uint32_t j = threadIdx.x; j < cnt; j += kSparseThreads) {
+    const uint32_t i = a.list[size_t(b) * kSparseCap + j];
+    if (!(a.depth[first + i] & 1u)) collapse_node(a, nb, i);
   }
-  node.parent = node_parent[nb + i];
+}
+
+// ---- level 1: a tile of up to 1,024 primitives in one workgroup, from LDS ----
+__global__ void __launch_bounds__(kTileThreads) k_tile_build(TileArgs a) {
+  __shared__ float lbox[kTileMax][6];   // the tile's leaves, sorted order
+  __shared__ float nbox[kTileMax][6];   // its internal nodes
+  __shared__ uint16_t ch[kTileMax][2];  // kLeafRef | leaf slot, or internal slot; kNoSlot: not this tile's
+  __shared__ uint16_t pslot[kTileMax];
+  __shared__ uint16_t span[kTileMax];   // primitives below, minus one
+  __shared__ uint32_t arrived[kTileMax];
+  __shared__ uint8_t pending[kTileMax];
+  __shared__ uint8_t height[kTileMax];
+  const uint32_t t = threadIdx.x;
+  uint32_t s, k;
+  tile_place(a, kTileShift, blockIdx.x, s, k);
+  const uint32_t n = a.seg_count[s], first = a.seg_first[s], nb = a.node_base[s];
+  const uint32_t A = k << kTileShift;
+  const uint32_t B = min(n, A + kTileMax);
+  const uint32_t nleaf = B - A;
+  const uint32_t cnt = min(B, n - 1) > A ? min(B, n - 1) - A : 0u;  // internal node indices [A, A + cnt)
+  for (uint32_t x = t; x < nleaf; x += kTileThreads) {
+    const uint32_t p = a.perm[first + A + x];
+    for (int q = 0; q < 6; ++q) lbox[x][q] = a.boxes[size_t(p) * 6 + q];
+  }
+  // Child links; a node whose range leaves the tile is not this tile's (ch[j][0] = kNoSlot).
+  for (uint32_t j = t; j < cnt; j += kTileThreads) {
+    const uint32_t i = A + j;
+    const uint4 tail = reinterpret_cast<const uint4*>(a.nodes + nb + i)[3];  // child[2], parent, count
+    const uint32_t c0 = tail.x, c1 = tail.y, cn = tail.w;
+    const uint32_t gamma = c0 & ~GE_BVH_LEAF;
+    const uint32_t lo = i <= gamma ? i : i + 1 - cn, hi = i <= gamma ? i + cn - 1 : i;
+    arrived[j] = 0;
+    if (lo < A || hi >= B) {
+      ch[j][0] = ch[j][1] = kNoSlot;
+      continue;
+    }
+    const bool l0 = c0 & GE_BVH_LEAF, l1 = c1 & GE_BVH_LEAF;
+    ch[j][0] = uint16_t(l0 ? (kLeafRef | ((c0 & ~GE_BVH_LEAF) - A)) : (c0 - A));
+    ch[j][1] = uint16_t(l1 ? (kLeafRef | ((c1 & ~GE_BVH_LEAF) - A)) : (c1 - A));
+    pending[j] = uint8_t((l0 ? 0 : 1) + (l1 ? 0 : 1));
+    span[j] = uint16_t(cn - 1);
+  }
+  __syncthreads();
+  for (uint32_t j = t; j < cnt; j += kTileThreads) {
+    if (ch[j][0] == kNoSlot) continue;
+    const uint32_t i = A + j;
+    uint32_t ps = 0xffffffffu;
+    if (i != 0) {
+      const uint32_t p = a.node_parent[nb + i];
+      if (p >= A && p < A + cnt && ch[p - A][0] != kNoSlot) ps = p - A;
+    }
+    pslot[j] = ps == 0xffffffffu ? kNoSlot : uint16_t(ps);
+  }
+  __syncthreads();
+  // The fit, bottom-up in LDS alone (the records go out afterwards).
+  auto box_of = [&](uint16_t c) -> const float* { return (c & kLeafRef) ? lbox[c & (kTileMax - 1)] : nbox[c & (kTileMax - 1)]; };
+  for (uint32_t j0 = t; j0 < cnt; j0 += kTileThreads) {
+    if (ch[j0][0] == kNoSlot || pending[j0] != 0) continue;
+    uint32_t j = j0;
+    for (;;) {
+      const uint16_t x = ch[j][0], y = ch[j][1];
+      const float* b0 = box_of(x);
+      const float* b1 = box_of(y);
+      const uint32_t h0 = (x & kLeafRef) ? 0u : height[x], h1 = (y & kLeafRef) ? 0u : height[y];
+      for (int q = 0; q < 3; ++q) {
+        nbox[j][q] = __builtin_fminf(b0[q], b1[q]);
+        nbox[j][3 + q] = __builtin_fmaxf(b0[3 + q], b1[3 + q]);
+      }
+      const uint32_t ht = 1 + (h0 > h1 ? h0 : h1);
+      height[j] = uint8_t(ht < 255u ? ht : 255u);
+      const uint16_t p = pslot[j];
+      if (p == kNoSlot) break;
+      __threadfence_block();
+      if (atomicAdd(&arrived[p], 1u) + 1u < pending[p]) break;
+      __threadfence_block();
+      j = p;
+    }
+  }
+  __syncthreads();
+  // The records, the tile's top boxes, depths and the collapse.
+  for (uint32_t j = t; j < cnt; j += kTileThreads) {
+    const uint16_t x = ch[j][0], y = ch[j][1];
+    if (x == kNoSlot) continue;
+    const uint32_t i = A + j;
+    const bool lx = x & kLeafRef, ly = y & kLeafRef;
+    const float* bx = box_of(x);
+    const float* by = box_of(y);
+    const uint32_t gx = lx ? (GE_BVH_LEAF | (A + (x & (kTileMax - 1)))) : (A + x);
+    const uint32_t gy = ly ? (GE_BVH_LEAF | (A + (y & (kTileMax - 1)))) : (A + y);
+    const uint16_t ps = pslot[j];
+    const uint32_t parent = i == 0 ? GE_BVH_NONE : a.node_parent[nb + i];
+    store_record(a.nodes + nb + i, bx, bx + 3, by, by + 3, gx, gy, parent, uint32_t(span[j]) + 1);
+    uint32_t steps = 0, q = j;
+    while (pslot[q] != kNoSlot) {
+      q = pslot[q];
+      ++steps;
+    }
+    const uint32_t top = A + q;
+    if (ps == kNoSlot) {  // the tile's top: its box and height for the level above, or the root's
+      float* o = a.node_box + size_t(nb + i) * 8;
+      for (int f = 0; f < 6; ++f) o[f] = nbox[j][f];
+      o[6] = __uint_as_float(height[j]);
+      if (i == 0) {
+        const uint32_t out = a.root_index ? a.root_index[s] : s;
+        for (int f = 0; f < 6; ++f) a.roots[out * 6 + f] = nbox[j][f];
+        a.heights[out] = height[j];
+      }
+    }
+    const uint32_t d = (top == 0 ? 0u : a.depth[first + a.node_parent[nb + top]] + 1u) + steps;
+    if (d > kFitMaxDepth) atomicOr(a.flag, 1u);
+    if ((d & 1u) || !a.wide) continue;
+    // Entries: a leaf child as it is, an inner child's two children, in child order.
+    const uint16_t x0 = ch[x & (kTileMax - 1)][0], x1 = ch[x & (kTileMax - 1)][1];
+    const uint16_t y0 = ch[y & (kTileMax - 1)][0], y1 = ch[y & (kTileMax - 1)][1];
+    const uint16_t r[4] = {lx ? x : x0, lx ? (ly ? y : y0) : x1, lx ? y1 : (ly ? y : y0), y1};
+    uint32_t ref[4];
+    const float* box[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint16_t c = r[e];
+      ref[e] = (c & kLeafRef) ? (GE_BVH_LEAF | (A + (c & (kTileMax - 1)))) : (A + (c & (kTileMax - 1)));
+      box[e] = box_of(c);
+    }
+    store_wide(a.wide + nb + i, ref, box, (lx ? 1 : 2) + (ly ? 1 : 2));
+  }
 }
 
 __global__ void k_triangle_records(const ge_cluster_desc* __restrict__ clusters, const float* __restrict__ vertices,
@@ -547,52 +824,6 @@ __global__ void k_instance_boxes(uint32_t n, const uint32_t* __restrict__ live, 
   seg_of[k] = 0;
 }
 
-// The 4-wide collapse: each node at even depth gathers its children's children (a leaf child taken
-// as it is), in child order, with their boxes, into the wide node at its own binary index.
-__global__ void k_collapse(uint32_t n_total, const uint32_t* __restrict__ seg_of, const uint32_t* __restrict__ seg_first,
-                           const uint32_t* __restrict__ node_base, const uint32_t* __restrict__ dist,
-                           const ge_bvh_node* __restrict__ nodes, WideNode* __restrict__ wide) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_total) return;
-  const uint32_t d = dist[g];
-  if (d == kNoDepth || (d & 1u)) return;
-  const uint32_t s = seg_of[g];
-  const uint32_t nb = node_base[s];
-  const uint32_t i = g - seg_first[s];
-  const ge_bvh_node a = nodes[nb + i];
-  WideNode w;
-  int k = 0;
-  auto put = [&](uint32_t ref, const float* lo, const float* hi) {
-    w.child[k] = ref;
-    w.lox[k] = lo[0];
-    w.loy[k] = lo[1];
-    w.loz[k] = lo[2];
-    w.hix[k] = hi[0];
-    w.hiy[k] = hi[1];
-    w.hiz[k] = hi[2];
-    ++k;
-  };
-  for (int c = 0; c < 2; ++c) {
-    const uint32_t ch = a.child[c];
-    if (ch == GE_BVH_NONE) continue;
-    if (ch & GE_BVH_LEAF) {
-      put(ch, c == 0 ? a.lo0 : a.lo1, c == 0 ? a.hi0 : a.hi1);
-      continue;
-    }
-    const ge_bvh_node m = nodes[nb + ch];
-    for (int e = 0; e < 2; ++e)
-      if (m.child[e] != GE_BVH_NONE) put(m.child[e], e == 0 ? m.lo0 : m.lo1, e == 0 ? m.hi0 : m.hi1);
-  }
-  // An empty slot's box sits at +infinity on every axis: every slab test against it fails (its near
-  // distance is +inf, or both its distances -inf), so the walk needs no test of the reference.
-  for (; k < 4; ++k) {
-    w.child[k] = GE_BVH_NONE;
-    w.lox[k] = w.loy[k] = w.loz[k] = w.hix[k] = w.hiy[k] = w.hiz[k] = __builtin_huge_valf();
-  }
-  for (int q = 0; q < 4; ++q) w.pad[q] = 0;
-  wide[nb + i] = w;
-}
-
 __global__ void k_gather_slots(uint32_t n, const uint32_t* perm, const uint32_t* live, uint32_t* slots) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < n) slots[g] = live[perm[g]];
@@ -606,7 +837,7 @@ __global__ void k_gather_slots(uint32_t n, const uint32_t* perm, const uint32_t*
 // node's depth and the 4-wide collapse. One launch builds every small mesh of a batch, where the
 // device-wide passes cost a launch a level; the results are the device-wide build's to the bit.
 constexpr uint32_t kBlockMax = k_block_build_max;
-constexpr uint32_t kBlockThreads = 256;
+constexpr uint32_t kBlockThreads = 512;
 constexpr uint16_t kBlockLeaf = 0x8000u;
 
 __global__ void __launch_bounds__(kBlockThreads) k_block_build(
@@ -615,29 +846,26 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     const uint32_t* __restrict__ root_index, ge_bvh_node* __restrict__ nodes, uint32_t* __restrict__ perm,
     uint32_t* __restrict__ morton, float* __restrict__ roots, uint32_t* __restrict__ heights,
     WideNode* __restrict__ wide) {
-  __shared__ uint64_t keys[kBlockMax];     // (code << 32 | local index), sorted in place
+  // 64 KiB: the centroid bounds, then the sort keys, then the fit counters share one array (a node's
+  // counter holds its arrivals in bits 0-7 and its height above).
+  __shared__ union {
+    uint32_t red[6];
+    uint64_t keys[kBlockMax];  // (code << 32 | local index), sorted in place
+    uint32_t arrived[kBlockMax];
+  } u;
+  __shared__ float lbox[kBlockMax][6];     // the leaves' boxes, sorted order
   __shared__ float nbox[kBlockMax][6];     // internal nodes' boxes
   __shared__ uint16_t child[kBlockMax][2]; // internal children; kBlockLeaf | sorted position for a leaf
   __shared__ uint16_t parent[kBlockMax];   // internal node -> its parent
   __shared__ uint16_t leaf_parent[kBlockMax];
-  __shared__ uint32_t arrived[kBlockMax];  // fit counters
-  __shared__ uint8_t height[kBlockMax];
-  __shared__ uint32_t red[6];
   const uint32_t t = threadIdx.x;
   const uint32_t s = seg_base + blockIdx.x;
   const uint32_t first = seg_first[s], n = seg_count[s], nb = node_base[s];
   const uint32_t out = root_index ? root_index[s] : s;
   const float* sb = boxes + size_t(first) * 6;
-  auto leaf_box = [&](uint32_t sorted, float lo[3], float hi[3]) {
-    const uint32_t o = uint32_t(keys[sorted]);
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = sb[size_t(o) * 6 + a];
-      hi[a] = sb[size_t(o) * 6 + 3 + a];
-    }
-  };
 
   // Centroid bounds, as order-preserving integers (min and max are the float ones).
-  if (t < 6) red[t] = t < 3 ? 0xffffffffu : 0u;
+  if (t < 6) u.red[t] = t < 3 ? 0xffffffffu : 0u;
   __syncthreads();
   {
     uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
@@ -648,17 +876,18 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
         hi[a] = max(hi[a], c);
       }
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&red[a], lo[a]);
-      atomicMax(&red[3 + a], hi[a]);
+      atomicMin(&u.red[a], lo[a]);
+      atomicMax(&u.red[3 + a], hi[a]);
     }
   }
   __syncthreads();
   float clo[3], cscale[3];
   for (int a = 0; a < 3; ++a) {
-    clo[a] = unordered(red[a]);
-    const float ext = unordered(red[3 + a]) - clo[a];
+    clo[a] = unordered(u.red[a]);
+    const float ext = unordered(u.red[3 + a]) - clo[a];
     cscale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
   }
+  __syncthreads();  // red is read; the keys take its place
   // Codes, and the bitonic sort over the next power of two (padding keys sort last).
   uint32_t p2 = 1;
   while (p2 < n) p2 <<= 1;
@@ -670,9 +899,9 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
         q[a] = quantize10(c, clo[a], cscale[a]);
       }
       const uint32_t code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
-      keys[i] = (uint64_t(code) << 32) | i;
+      u.keys[i] = (uint64_t(code) << 32) | i;
     } else {
-      keys[i] = ~0ull;
+      u.keys[i] = ~0ull;
     }
   }
   __syncthreads();
@@ -681,11 +910,11 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
       for (uint32_t i = t; i < p2; i += kBlockThreads) {
         const uint32_t l = i ^ j;
         if (l > i) {
-          const uint64_t a = keys[i], b = keys[l];
+          const uint64_t a = u.keys[i], b = u.keys[l];
           const bool up = (i & k) == 0;
           if ((a > b) == up) {
-            keys[i] = b;
-            keys[l] = a;
+            u.keys[i] = b;
+            u.keys[l] = a;
           }
         }
       }
@@ -693,48 +922,31 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     }
   }
   for (uint32_t i = t; i < n; i += kBlockThreads) {
-    perm[first + i] = first + uint32_t(keys[i]);
-    morton[first + i] = uint32_t(keys[i] >> 32);
-    arrived[i] = 0;
+    const uint32_t o = uint32_t(u.keys[i]);
+    perm[first + i] = first + o;
+    morton[first + i] = uint32_t(u.keys[i] >> 32);
+    for (int a = 0; a < 6; ++a) lbox[i][a] = sb[size_t(o) * 6 + a];
   }
   if (n == 1) {
+    __syncthreads();
     if (t == 0) {
-      float lo[3], hi[3];
-      leaf_box(0, lo, hi);
-      ge_bvh_node node;
-      node.child[0] = GE_BVH_LEAF;
-      node.child[1] = GE_BVH_NONE;
-      node.parent = GE_BVH_NONE;
-      node.count = 1;
-      WideNode w;
-      for (int a = 0; a < 3; ++a) {
-        node.lo0[a] = lo[a];
-        node.hi0[a] = hi[a];
-        node.lo1[a] = __uint_as_float(0x7f800000u);
-        node.hi1[a] = __uint_as_float(0xff800000u);
-        roots[out * 6 + a] = lo[a];
-        roots[out * 6 + 3 + a] = hi[a];
-      }
-      nodes[nb] = node;
+      const float* lo = lbox[0];
+      const float* hi = lbox[0] + 3;
+      const float empty_lo[3] = {__uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u)};
+      const float empty_hi[3] = {__uint_as_float(0xff800000u), __uint_as_float(0xff800000u), __uint_as_float(0xff800000u)};
+      store_record(nodes + nb, lo, hi, empty_lo, empty_hi, GE_BVH_LEAF, GE_BVH_NONE, GE_BVH_NONE, 1u);
+      for (int a = 0; a < 6; ++a) roots[out * 6 + a] = lbox[0][a];
       heights[out] = 1;
-      for (int k = 0; k < 4; ++k) {
-        w.child[k] = k == 0 ? GE_BVH_LEAF : GE_BVH_NONE;
-        w.lox[k] = k == 0 ? lo[0] : __builtin_huge_valf();
-        w.loy[k] = k == 0 ? lo[1] : __builtin_huge_valf();
-        w.loz[k] = k == 0 ? lo[2] : __builtin_huge_valf();
-        w.hix[k] = k == 0 ? hi[0] : __builtin_huge_valf();
-        w.hiy[k] = k == 0 ? hi[1] : __builtin_huge_valf();
-        w.hiz[k] = k == 0 ? hi[2] : __builtin_huge_valf();
-        w.pad[k] = 0;
-      }
-      wide[nb] = w;
+      const uint32_t ref[4] = {GE_BVH_LEAF, 0u, 0u, 0u};
+      const float* box[4] = {lbox[0], lbox[0], lbox[0], lbox[0]};
+      store_wide(wide + nb, ref, box, 1);
     }
     return;
   }
   // Karras 2012 over the sorted codes, equal codes split by position.
   auto delta = [&](int i, int j) -> int {
     if (j < 0 || j > int(n) - 1) return -1;
-    const uint32_t a = uint32_t(keys[i] >> 32), b = uint32_t(keys[j] >> 32);
+    const uint32_t a = uint32_t(u.keys[i] >> 32), b = uint32_t(u.keys[j] >> 32);
     if (a != b) return __clz(a ^ b);
     return 32 + __clz(uint32_t(i) ^ uint32_t(j));
   };
@@ -764,117 +976,68 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     if (l0) leaf_parent[gamma] = uint16_t(i); else parent[gamma] = uint16_t(i);
     if (l1) leaf_parent[gamma + 1] = uint16_t(i); else parent[gamma + 1] = uint16_t(i);
   }
+  __syncthreads();  // the keys are done with; the counters take their place
+  for (uint32_t i = t; i < n; i += kBlockThreads) u.arrived[i] = 0;
   __syncthreads();
-  // The fit: every leaf walks up; the second arrival at a node fits it and goes on.
+  auto box_of = [&](uint16_t c) -> const float* {
+    return (c & kBlockLeaf) ? lbox[c & (kBlockMax - 1)] : nbox[c & (kBlockMax - 1)];
+  };
+  auto height_of = [&](uint16_t c) -> uint32_t { return (c & kBlockLeaf) ? 0u : (u.arrived[c] >> 8); };
+  // The fit, in LDS alone: every leaf walks up; the second arrival at a node fits it and goes on.
   for (uint32_t leaf = t; leaf < n; leaf += kBlockThreads) {
     uint32_t p = leaf_parent[leaf];
     for (;;) {
       __threadfence_block();
-      if (atomicAdd(&arrived[p], 1u) == 0u) break;
-      float b[2][6];
-      uint32_t h[2];
-      for (int c = 0; c < 2; ++c) {
-        const uint16_t ch = child[p][c];
-        if (ch & kBlockLeaf) {
-          leaf_box(ch & ~kBlockLeaf, &b[c][0], &b[c][3]);
-          h[c] = 0;
-        } else {
-          for (int a = 0; a < 6; ++a) b[c][a] = nbox[ch][a];
-          h[c] = height[ch];
-        }
-      }
+      if ((atomicAdd(&u.arrived[p], 1u) & 0xffu) == 0u) break;
+      __threadfence_block();
+      const uint16_t x = child[p][0], y = child[p][1];
+      const float* b0 = box_of(x);
+      const float* b1 = box_of(y);
       for (int a = 0; a < 3; ++a) {
-        nbox[p][a] = __builtin_fminf(b[0][a], b[1][a]);
-        nbox[p][3 + a] = __builtin_fmaxf(b[0][3 + a], b[1][3 + a]);
+        nbox[p][a] = __builtin_fminf(b0[a], b1[a]);
+        nbox[p][3 + a] = __builtin_fmaxf(b0[3 + a], b1[3 + a]);
       }
-      height[p] = uint8_t(1 + (h[0] > h[1] ? h[0] : h[1]));
+      const uint32_t h0 = height_of(x), h1 = height_of(y);
+      atomicOr(&u.arrived[p], (1u + (h0 > h1 ? h0 : h1)) << 8);
       if (p == 0) break;
       p = parent[p];
     }
   }
   __syncthreads();
-  // The binary nodes, the root, and each node's depth for the collapse.
+  // The binary nodes, the root, each node's depth and the collapse.
   for (uint32_t i = t; i + 1 < n; i += kBlockThreads) {
-    ge_bvh_node node;
-    for (int c = 0; c < 2; ++c) {
-      const uint16_t ch = child[i][c];
-      float lo[3], hi[3];
-      if (ch & kBlockLeaf) {
-        leaf_box(ch & ~kBlockLeaf, lo, hi);
-        node.child[c] = GE_BVH_LEAF | uint32_t(ch & ~kBlockLeaf);
-      } else {
-        for (int a = 0; a < 3; ++a) {
-          lo[a] = nbox[ch][a];
-          hi[a] = nbox[ch][3 + a];
-        }
-        node.child[c] = ch;
-      }
-      float* dlo = c == 0 ? node.lo0 : node.lo1;
-      float* dhi = c == 0 ? node.hi0 : node.hi1;
-      for (int a = 0; a < 3; ++a) {
-        dlo[a] = lo[a];
-        dhi[a] = hi[a];
-      }
-    }
-    // The primitives below: the node's key range, from its children.
-    uint32_t lo_i = 0, hi_i = 0;
-    {
-      // Leftmost and rightmost leaves: descend the first and the last child.
-      uint16_t c = child[i][0];
-      while (!(c & kBlockLeaf)) c = child[c][0];
-      lo_i = c & ~kBlockLeaf;
-      c = child[i][1];
-      while (!(c & kBlockLeaf)) c = child[c][1];
-      hi_i = c & ~kBlockLeaf;
-    }
-    node.count = hi_i - lo_i + 1;
-    node.parent = i == 0 ? GE_BVH_NONE : uint32_t(parent[i]);
-    nodes[nb + i] = node;
-  }
-  if (t == 0) {
-    for (int a = 0; a < 6; ++a) roots[out * 6 + a] = nbox[0][a];
-    heights[out] = height[0];
-  }
-  for (uint32_t i = t; i + 1 < n; i += kBlockThreads) {
+    const uint16_t x = child[i][0], y = child[i][1];
+    const bool lx = x & kBlockLeaf, ly = y & kBlockLeaf;
+    // The primitives below: the node's key range, from its leftmost and rightmost leaves.
+    uint16_t c = x;
+    while (!(c & kBlockLeaf)) c = child[c][0];
+    const uint32_t lo_i = c & ~kBlockLeaf;
+    c = y;
+    while (!(c & kBlockLeaf)) c = child[c][1];
+    const uint32_t hi_i = c & ~kBlockLeaf;
+    const float* bx = box_of(x);
+    const float* by = box_of(y);
+    store_record(nodes + nb + i, bx, bx + 3, by, by + 3, lx ? (GE_BVH_LEAF | uint32_t(x & ~kBlockLeaf)) : uint32_t(x),
+                 ly ? (GE_BVH_LEAF | uint32_t(y & ~kBlockLeaf)) : uint32_t(y), i == 0 ? GE_BVH_NONE : uint32_t(parent[i]),
+                 hi_i - lo_i + 1);
     uint32_t d = 0;
     for (uint32_t q = i; q != 0; q = parent[q]) ++d;
     if (d & 1u) continue;
-    WideNode w;
-    int k = 0;
-    auto put = [&](uint32_t ref, const float* lo, const float* hi) {
-      w.child[k] = ref;
-      w.lox[k] = lo[0];
-      w.loy[k] = lo[1];
-      w.loz[k] = lo[2];
-      w.hix[k] = hi[0];
-      w.hiy[k] = hi[1];
-      w.hiz[k] = hi[2];
-      ++k;
-    };
-    auto take = [&](uint16_t ch) {
-      float lo[3], hi[3];
-      if (ch & kBlockLeaf) {
-        leaf_box(ch & ~kBlockLeaf, lo, hi);
-        put(GE_BVH_LEAF | uint32_t(ch & ~kBlockLeaf), lo, hi);
-      } else {
-        put(ch, &nbox[ch][0], &nbox[ch][3]);
-      }
-    };
-    for (int c = 0; c < 2; ++c) {
-      const uint16_t ch = child[i][c];
-      if (ch & kBlockLeaf) {
-        take(ch);
-        continue;
-      }
-      take(child[ch][0]);
-      take(child[ch][1]);
+    const uint16_t x0 = child[x & (kBlockMax - 1)][0], x1 = child[x & (kBlockMax - 1)][1];
+    const uint16_t y0 = child[y & (kBlockMax - 1)][0], y1 = child[y & (kBlockMax - 1)][1];
+    const uint16_t r[4] = {lx ? x : x0, lx ? (ly ? y : y0) : x1, lx ? y1 : (ly ? y : y0), y1};
+    uint32_t ref[4];
+    const float* box[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      ref[e] = (r[e] & kBlockLeaf) ? (GE_BVH_LEAF | uint32_t(r[e] & ~kBlockLeaf)) : uint32_t(r[e]);
+      box[e] = box_of(r[e]);
     }
-    for (; k < 4; ++k) {
-      w.child[k] = GE_BVH_NONE;
-      w.lox[k] = w.loy[k] = w.loz[k] = w.hix[k] = w.hiy[k] = w.hiz[k] = __builtin_huge_valf();
-    }
-    for (int q = 0; q < 4; ++q) w.pad[q] = 0;
-    wide[nb + i] = w;
+    store_wide(wide + nb + i, ref, box, (lx ? 1 : 2) + (ly ? 1 : 2));
+  }
+  if (t == 0) {
+    for (int a = 0; a < 6; ++a) roots[out * 6 + a] = nbox[0][a];
+    heights[out] = u.arrived[0] >> 8;
   }
 }
 
@@ -886,7 +1049,8 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
                           const uint32_t* d_seg_of, const uint32_t* d_seg_first, const uint32_t* d_seg_count,
                           const uint32_t* d_node_base, ge_bvh_node* nodes, uint32_t* perm, uint32_t* morton,
                           float* roots, uint32_t* heights, const uint32_t* d_root_index, uint32_t node_pool,
-                          WideNode* wide, uint32_t nseg_global, uint32_t n_global) {
+                          WideNode* wide, uint32_t nseg_global, uint32_t n_global, const uint32_t* h_seg_count,
+                          uint32_t* flag) {
   if (n == 0) return hipSuccess;
   hipError_t e;
   // Segments [nseg_global, nseg) fit a workgroup's LDS: one launch builds them all.
@@ -908,12 +1072,7 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   if ((e = sc.scan_tmp.reserve(size_t(tiles) * 256 * 4 / 256 + 65536)) != hipSuccess) return e;
   if ((e = sc.node_box.reserve(size_t(node_pool) * 32)) != hipSuccess) return e;
   if ((e = sc.node_parent.reserve(size_t(node_pool) * 4)) != hipSuccess) return e;
-  for (int k = 0; k < 2; ++k) {
-    if ((e = sc.anc[k].reserve(size_t(n) * 4)) != hipSuccess) return e;
-    if ((e = sc.dist[k].reserve(size_t(n) * 4)) != hipSuccess) return e;
-  }
-  if ((e = sc.levels.reserve(260 * 4)) != hipSuccess) return e;
-  if (!sc.host_levels && (e = hipHostMalloc(reinterpret_cast<void**>(&sc.host_levels), 260 * 4)) != hipSuccess) return e;
+  if ((e = sc.depth.reserve(size_t(n) * 4)) != hipSuccess) return e;
   if ((e = sc.seg_cbox.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
   if ((e = sc.seg_scale.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
 
@@ -944,52 +1103,53 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   }
   const uint64_t* sorted_keys = sc.keys[cur].as<uint64_t>();
   const uint32_t* sorted_vals = sc.vals[cur].as<uint32_t>();
+  // ---- the tile ladder: how many levels, each level's tiles, and the lists above level 1 ----
+  uint32_t levels = 1;
+  for (uint32_t q = 0; q < nseg; ++q)
+    while (levels < kMaxLevels && (uint64_t(1) << (kTileShift + kFanShift * (levels - 1))) < h_seg_count[q]) ++levels;
+  uint32_t level_tiles[kMaxLevels] = {};
+  TileArgs ta{};
+  ta.nseg = nseg;
+  ta.levels = levels;
+  for (uint32_t l = 0; l < levels; ++l) {
+    const uint32_t shift = kTileShift + kFanShift * l;
+    uint64_t t = 0;
+    for (uint32_t q = 0; q < nseg; ++q) t += (uint64_t(h_seg_count[q]) + (1ull << shift) - 1) >> shift;
+    level_tiles[l] = uint32_t(t);
+    ta.bucket_base[l + 1] = ta.bucket_base[l] + (l == 0 ? 0u : level_tiles[l]);  // level 1 keeps no list
+  }
+  const uint32_t lists = ta.bucket_base[levels];
+  if ((e = sc.buckets.reserve(size_t(std::max(lists, 1u)) * kSparseCap * 4)) != hipSuccess) return e;
+  if ((e = sc.bucket_count.reserve(size_t(std::max(lists, 1u)) * 4)) != hipSuccess) return e;
+  if (lists && (e = hipMemsetAsync(sc.bucket_count.ptr, 0, size_t(lists) * 4, stream)) != hipSuccess) return e;
+  ta.seg_first = d_seg_first;
+  ta.seg_count = d_seg_count;
+  ta.node_base = d_node_base;
+  ta.root_index = d_root_index;
+  ta.boxes = boxes;
+  ta.nodes = nodes;
+  ta.node_parent = sc.node_parent.as<uint32_t>();
+  ta.node_box = sc.node_box.as<float>();
+  ta.roots = roots;
+  ta.heights = heights;
+  ta.list = sc.buckets.as<uint32_t>();
+  ta.count = sc.bucket_count.as<uint32_t>();
+  ta.depth = sc.depth.as<uint32_t>();
+  ta.wide = wide;
+  ta.flag = flag;
+
   k_karras<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, d_seg_of, d_seg_first, d_seg_count, d_node_base, nodes,
-                                               sc.node_parent.as<uint32_t>());
+                                               sc.node_parent.as<uint32_t>(), ta);
   k_codes<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, morton);
   if ((e = hipMemcpyAsync(perm, sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
+  ta.perm = sorted_vals;
 
-  // ---- the fit: depths by pointer jumping, nodes grouped by depth, one launch per level ----
-  uint32_t* error = sc.levels.as<uint32_t>() + 257;
-  if ((e = hipMemsetAsync(error, 0, 4, stream)) != hipSuccess) return e;
-  k_depth_init<<<blocks(n, 256), 256, 0, stream>>>(n, d_seg_of, d_seg_first, d_seg_count, d_node_base,
-                                                   sc.node_parent.as<uint32_t>(), sc.anc[0].as<uint32_t>(),
-                                                   sc.dist[0].as<uint32_t>());
-  int jc = 0;
-  for (int r = 0; r < kJumps; ++r, jc ^= 1)
-    k_depth_jump<<<blocks(n, 256), 256, 0, stream>>>(n, sc.anc[jc].as<uint32_t>(), sc.dist[jc].as<uint32_t>(),
-                                                     sc.anc[jc ^ 1].as<uint32_t>(), sc.dist[jc ^ 1].as<uint32_t>());
-  // The Morton keys and order stay in keys[cur] / vals[cur]; the depth sort uses the other pair
-  // and its own value buffer.
-  uint64_t* dkeys = sc.keys[cur ^ 1].as<uint64_t>();
-  uint32_t* dvals = sc.anc[jc ^ 1].as<uint32_t>();
-  uint32_t* dsorted = sc.vals[cur ^ 1].as<uint32_t>();
-  k_depth_keys<<<blocks(n, 256), 256, 0, stream>>>(n, sc.anc[jc].as<uint32_t>(), sc.dist[jc].as<uint32_t>(), dkeys,
-                                                   dvals, error);
-  k_sort_hist<<<tiles, kSortThreads, 0, stream>>>(dkeys, n, 0, sc.hist.as<uint32_t>(), tiles);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = exclusive_scan(sc.hist.as<uint32_t>(), tiles * 256, sc.scan_tmp, 0, stream)) != hipSuccess) return e;
-  // One 8-bit pass orders the nodes by depth (the sorted keys themselves are not kept).
-  k_sort_scatter<false><<<tiles, kSortThreads, 0, stream>>>(dkeys, dvals, nullptr, dsorted, n, 0,
-                                                            sc.hist.as<uint32_t>(), tiles);
-  k_level_starts<<<1, 256, 0, stream>>>(sc.hist.as<uint32_t>(), tiles, error, sc.levels.as<uint32_t>());
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(sc.host_levels, sc.levels.ptr, 257 * 4, hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
-  if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-  if (sc.host_levels[256]) return hipErrorNotSupported;  // deeper than 2^8 - 2 levels
-  for (int d = int(kNotANode) - 1; d >= 0; --d) {
-    const uint32_t start = sc.host_levels[d];
-    const uint32_t end = sc.host_levels[d + 1];
-    if (end <= start) continue;
-    k_fit_level<<<blocks(end - start, 256), 256, 0, stream>>>(dsorted + start, end - start, boxes, sorted_vals,
-                                                              d_seg_of, d_seg_first, d_seg_count, d_node_base,
-                                                              d_root_index, nodes, sc.node_parent.as<uint32_t>(),
-                                                              sc.node_box.as<float>(), roots, heights);
-  }
-  // The depths are still those of the pointer jumping.
-  if (wide)
-    k_collapse<<<blocks(n, 256), 256, 0, stream>>>(n, d_seg_of, d_seg_first, d_node_base, sc.dist[jc].as<uint32_t>(),
-                                                   nodes, wide);
+  // ---- depths above level 1 (top down), level 1 whole, the fit above level 1 (bottom up) ----
+  for (uint32_t l = levels; l-- > 1;)
+    k_depth_sparse<<<level_tiles[l], kSparseThreads, 0, stream>>>(ta, l);
+  k_tile_build<<<level_tiles[0], kTileThreads, 0, stream>>>(ta);
+  for (uint32_t l = 1; l < levels; ++l) k_fit_sparse<<<level_tiles[l], kSparseThreads, 0, stream>>>(ta, l);
+  if (lists && wide) k_collapse_sparse<<<lists, kSparseThreads, 0, stream>>>(ta);
   return hipGetLastError();
 }
 

@@ -38,9 +38,10 @@ struct DevBuf {
 // Everything a segmented LBVH build needs besides its inputs. One per context, reused.
 struct BuildScratch {
   DevBuf keys[2], vals[2], hist, scan_tmp, node_box, node_parent, seg_cbox, seg_scale;
-  DevBuf anc[2], dist[2];           // pointer jumping: each node's ancestor and distance to it
-  DevBuf levels;                    // 256 level starts, the depth error flag copy, the flag
-  uint32_t* host_levels = nullptr;  // pinned copy of `levels`
+  DevBuf depth;                   // per node of the device-wide build, for the levels above the first
+  DevBuf buckets, bucket_count;   // the nodes each tile hands up to the next level
+  DevBuf flag;                    // the build's refusal flag (a tile over capacity, too deep)
+  uint32_t* host_flag = nullptr;  // pinned copy of `flag`
 };
 
 struct SceneDev {
@@ -205,14 +206,16 @@ namespace ge {
 // `node_pool` bounds every node_base[s] + node count. All pointers are device pointers.
 // Segments of at most k_block_build_max primitives are built by one workgroup each, in LDS, in one
 // launch; larger ones by device-wide passes. The caller puts the larger ones first: segments
-// [0, nseg_global), primitives [0, n_global), take the device-wide passes.
+// [0, nseg_global), primitives [0, n_global), take the device-wide passes; h_seg_count holds their
+// counts on the host. A structure the build refuses (deeper than 254 levels, or too unbalanced for
+// a tile's capacity) sets the device word `flag`, which the caller clears before and reads after.
 constexpr uint32_t k_block_build_max = 1024;
 hipError_t build_segments(BuildScratch& scratch, hipStream_t stream, uint32_t n, const float* boxes,
                           uint32_t nseg, const uint32_t* d_seg_of, const uint32_t* d_seg_first,
                           const uint32_t* d_seg_count, const uint32_t* d_node_base, ge_bvh_node* nodes,
                           uint32_t* perm, uint32_t* morton, float* roots, uint32_t* heights,
                           const uint32_t* d_root_index, uint32_t node_pool, WideNode* wide, uint32_t nseg_global,
-                          uint32_t n_global);
+                          uint32_t n_global, const uint32_t* h_seg_count, uint32_t* flag);
 // Triangle records, boxes and segment ids of the clusters in `cluster_list` (cluster k's triangles
 // at batch index cluster_out[k] on; cluster_local[k] its index within its mesh).
 hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,
