@@ -344,7 +344,7 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
                     &ctx->build.seg_scale, &ctx->build.depth, &ctx->build.buckets, &ctx->build.bucket_count,
                     &ctx->build.flag, &ctx->counters})
     b->release();
-  if (ctx->build.host_flag) (void)hipHostFree(ctx->build.host_flag);
+  if (ctx->host_readback) (void)hipHostFree(ctx->host_readback);
   for (DevBuf& b : ctx->io) b.release();
   ctx->resolve_rays.release();
   for (DevBuf& b : ctx->tmp) b.release();
@@ -566,13 +566,27 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_HIP(ctx, hipSetDevice(ctx->device));
   const int32_t st = validate_instances(ctx, s, instances, count);
   if (st != GE_RT_OK) return st;
+  // A moved instance changes only its record; one that changes mesh or pair range changes the pair
+  // table and the list of instances with geometry, and one that changes materials the mask table.
+  bool topology = false, materials = false;
+  for (uint32_t k = 0; k < count; ++k) {
+    const ge_instance_desc& was = s.h_instances[first + k];
+    topology = topology || was.mesh != instances[k].mesh || was.first_pair != instances[k].first_pair;
+    materials = materials || was.mesh != instances[k].mesh || was.material_base != instances[k].material_base;
+  }
   std::memcpy(&s.h_instances[first], instances, size_t(count) * sizeof(ge_instance_desc));
-  s.pairs_dirty = true;
+  if (topology) {
+    s.pairs_dirty = true;
+    s.live_dirty = true;
+  }
+  if (count) {
+    s.rec_lo = std::min(s.rec_lo, first);
+    s.rec_hi = std::max(s.rec_hi, first + count);
+  }
   s.version = next_version();
-  GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, instances, size_t(count) * sizeof(ge_instance_desc),
-                             hipMemcpyHostToDevice, ctx->stream));
-  GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  GE_HIP(ctx, refresh_masks(s, ctx->stream));
+  GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, &s.h_instances[first],
+                             size_t(count) * sizeof(ge_instance_desc), hipMemcpyHostToDevice, ctx->stream));
+  if (materials) GE_HIP(ctx, refresh_masks(s, ctx->stream));
   return GE_RT_OK;
 } catch (...) {
   return host_failure(ctx);
@@ -596,8 +610,14 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, s.mesh_roots.reserve(size_t(std::max(s.mesh_count, 1u)) * 24));
   GE_HIP(ctx, s.mesh_height.reserve(size_t(std::max(s.mesh_count, 1u)) * 4));
   GE_HIP(ctx, ctx->build.flag.reserve(4));
-  if (!ctx->build.host_flag)
-    GE_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->build.host_flag), 4));
+  const size_t readback = size_t(s.mesh_count) + 2;  // the flag, the instance structure's height, the meshes'
+  if (ctx->host_readback_cap < readback) {
+    if (ctx->host_readback) GE_HIP(ctx, hipHostFree(ctx->host_readback));
+    ctx->host_readback = nullptr;
+    ctx->host_readback_cap = 0;
+    GE_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_readback), readback * 4));
+    ctx->host_readback_cap = readback;
+  }
   uint32_t* flag = ctx->build.flag.as<uint32_t>();
   GE_HIP(ctx, hipEventRecord(ctx->bev0, st));
   GE_HIP(ctx, hipMemsetAsync(flag, 0, 4, st));
@@ -676,31 +696,51 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
                                         ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));
   }
 
-  // ---- the instances ----
+  // ---- the instances: the records the walks read, recomputed where they changed ----
   const uint32_t ni = s.instance_count;
-  std::vector<InstRec>& recs = ctx->host_recs;
-  std::vector<float>& T = ctx->host_T;
-  std::vector<uint32_t>& live = ctx->host_live;
-  recs.assign(ni, InstRec{});
-  T.assign(size_t(std::max(ni, 1u)) * 3, 0.0f);
-  live.clear();
-  for (uint32_t i = 0; i < ni; ++i) {
+  const bool eye_moved = std::memcmp(&s.eye, &s.rec_eye, sizeof(ge_frame_eye)) != 0;
+  if (flags == GE_RT_BUILD_FULL || s.rec_all || eye_moved || s.h_recs.size() != ni) {
+    s.h_recs.assign(ni, InstRec{});
+    s.h_T.assign(size_t(std::max(ni, 1u)) * 3, 0.0f);
+    s.rec_lo = 0;
+    s.rec_hi = ni;
+    s.live_dirty = true;
+  }
+  const uint32_t rlo = std::min(s.rec_lo, ni), rhi = std::min(s.rec_hi, ni);
+  for (uint32_t i = rlo; i < rhi; ++i) {
     const ge_instance_desc& in = s.h_instances[i];
     for (int r = 0; r < 3; ++r)
-      T[size_t(i) * 3 + r] = relative_axis_host(in.cell[r], in.rows[r][3], s.eye.cell[r], s.eye.local[r], s.eye.residual[r]);
-    InstRec& rec = recs[i];
-    make_w2o(in, &T[size_t(i) * 3], rec.w2o);
+      s.h_T[size_t(i) * 3 + r] =
+          relative_axis_host(in.cell[r], in.rows[r][3], s.eye.cell[r], s.eye.local[r], s.eye.residual[r]);
+    InstRec& rec = s.h_recs[i];
+    make_w2o(in, &s.h_T[size_t(i) * 3], rec.w2o);
     rec.node_base = s.mesh_node_base[in.mesh];
     rec.tri_base = s.mesh_tri_base[in.mesh];
     rec.first_pair = in.first_pair;
     rec.mask = instance_ray_mask(in.flags);
-    if (s.mesh_tri_count[in.mesh] > 0) live.push_back(i);
   }
-  const uint32_t nl = uint32_t(live.size());
+  GE_HIP(ctx, s.inst_recs.reserve(size_t(std::max(ni, 1u)) * sizeof(InstRec)));
+  GE_HIP(ctx, s.inst_T.reserve(size_t(std::max(ni, 1u)) * 12));
+  if (rhi > rlo) {
+    GE_HIP(ctx, hipMemcpyAsync(s.inst_recs.as<InstRec>() + rlo, s.h_recs.data() + rlo, size_t(rhi - rlo) * sizeof(InstRec),
+                               hipMemcpyHostToDevice, st));
+    GE_HIP(ctx, hipMemcpyAsync(s.inst_T.as<float>() + size_t(rlo) * 3, s.h_T.data() + size_t(rlo) * 3,
+                               size_t(rhi - rlo) * 12, hipMemcpyHostToDevice, st));
+  }
+  if (s.live_dirty) {
+    s.h_live.clear();
+    for (uint32_t i = 0; i < ni; ++i)
+      if (s.mesh_tri_count[s.h_instances[i].mesh] > 0) s.h_live.push_back(i);
+    GE_HIP(ctx, upload(s.live_ids, s.h_live.data(), s.h_live.size(), st));
+  }
+  s.rec_lo = 0xffffffffu;
+  s.rec_hi = 0;
+  s.rec_all = false;
+  s.live_dirty = false;
+  s.rec_eye = s.eye;
+  const uint32_t nl = uint32_t(s.h_live.size());
   s.tlas_prims = nl;
   s.tlas_node_count = nl == 0 ? 0u : (nl == 1 ? 1u : nl - 1);
-  GE_HIP(ctx, upload(s.inst_recs, recs.data(), recs.size(), st));
-  GE_HIP(ctx, upload(s.live_ids, live.data(), live.size(), st));
   GE_HIP(ctx, s.tlas_nodes.reserve(size_t(std::max(s.tlas_node_count, 1u)) * sizeof(ge_bvh_node)));
   GE_HIP(ctx, s.tlas_wide.reserve(size_t(std::max(s.tlas_node_count, 1u)) * sizeof(WideNode)));
   GE_HIP(ctx, s.tlas_slots.reserve(size_t(std::max(nl, 1u)) * 4));
@@ -710,10 +750,9 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   if (nl > 0) {
     std::vector<uint32_t>& tmeta = ctx->host_tmeta;
     tmeta = {0u, nl, 0u};  // seg_first, seg_count, node_base
-    GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + size_t(ni) * 12 + 256));
+    GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + 256));
     float* boxes = ctx->tmp[6].as<float>();
-    float* d_T = boxes + size_t(nl) * 6;
-    GE_HIP(ctx, hipMemcpyAsync(d_T, T.data(), size_t(ni) * 12, hipMemcpyHostToDevice, st));
+    const float* d_T = s.inst_T.as<float>();
     GE_HIP(ctx, upload(ctx->tmp[7], tmeta.data(), tmeta.size(), st));
     GE_HIP(ctx, ctx->tmp[8].reserve(size_t(nl) * 12));
     uint32_t* seg_of = ctx->tmp[8].as<uint32_t>();
@@ -730,20 +769,22 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, hipMemcpyAsync(s.tlas_morton.ptr, codes, size_t(nl) * 4, hipMemcpyDeviceToDevice, st));
   }
   GE_HIP(ctx, hipEventRecord(ctx->bev1, st));
-  GE_HIP(ctx, hipMemcpyAsync(ctx->build.host_flag, flag, 4, hipMemcpyDeviceToHost, st));
+  uint32_t* rb = ctx->host_readback;
+  rb[1] = 0;
+  GE_HIP(ctx, hipMemcpyAsync(rb, flag, 4, hipMemcpyDeviceToHost, st));
+  if (nl > 0) GE_HIP(ctx, hipMemcpyAsync(rb + 1, s.tlas_height.ptr, 4, hipMemcpyDeviceToHost, st));
+  if (s.mesh_count) GE_HIP(ctx, hipMemcpyAsync(rb + 2, s.mesh_height.ptr, size_t(s.mesh_count) * 4, hipMemcpyDeviceToHost, st));
   GE_HIP(ctx, hipStreamSynchronize(st));
   float ms = 0.0f;
   GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->bev0, ctx->bev1));
   ctx->last_build_ms = ms;
-  if (*ctx->build.host_flag) s.built = false;
-  GE_CHECK(ctx, !*ctx->build.host_flag, GE_RT_ERROR_UNSUPPORTED,
+  if (rb[0]) s.built = false;
+  GE_CHECK(ctx, !rb[0], GE_RT_ERROR_UNSUPPORTED,
            "a structure deeper than 254 levels, or too unbalanced for the build's tiles");
   // The traversal stack holds at most three deferred children per wide level (two binary levels) of
   // each structure plus the sentinel between them.
-  s.h_mesh_height.assign(s.mesh_count, 0);
-  if (s.mesh_count) GE_HIP(ctx, hipMemcpy(s.h_mesh_height.data(), s.mesh_height.ptr, size_t(s.mesh_count) * 4, hipMemcpyDeviceToHost));
-  s.tlas_height_value = 0;
-  if (nl > 0) GE_HIP(ctx, hipMemcpy(&s.tlas_height_value, s.tlas_height.ptr, 4, hipMemcpyDeviceToHost));
+  s.h_mesh_height.assign(rb + 2, rb + 2 + s.mesh_count);
+  s.tlas_height_value = rb[1];
   uint32_t blas_max = 0;
   for (uint32_t m = 0; m < s.mesh_count; ++m)
     if (s.mesh_tri_count[m] > 0) blas_max = std::max(blas_max, s.h_mesh_height[m]);

@@ -41,7 +41,6 @@ struct BuildScratch {
   DevBuf depth;                   // per node of the device-wide build, for the levels above the first
   DevBuf buckets, bucket_count;   // the nodes each tile hands up to the next level
   DevBuf flag;                    // the build's refusal flag (a tile over capacity, too deep)
-  uint32_t* host_flag = nullptr;  // pinned copy of `flag`
 };
 
 struct SceneDev {
@@ -64,6 +63,16 @@ struct SceneDev {
   // The built structures.
   DevBuf blas_nodes, blas_tris, blas_morton, mesh_roots, mesh_height;  // roots: 6 floats per mesh
   DevBuf tlas_nodes, tlas_slots, tlas_morton, inst_recs, inst_boxes, live_ids, tlas_root, tlas_height;
+  DevBuf inst_T;  // per instance: its translation in the frame's space (3 floats)
+  // The instance records as last uploaded, kept between builds: entries [rec_lo, rec_hi) changed
+  // since (all of them when rec_all, or when the frame's eye moves), and the list of instances with
+  // geometry when live_dirty.
+  std::vector<InstRec> h_recs;
+  std::vector<float> h_T;
+  std::vector<uint32_t> h_live;
+  uint32_t rec_lo = 0xffffffffu, rec_hi = 0;
+  bool rec_all = true, live_dirty = true;
+  ge_frame_eye rec_eye{};
   DevBuf blas_wide, tlas_wide;  // the 4-wide collapse of each, at the binary nodes' indices
   uint32_t tlas_prims = 0, tlas_node_count = 0;
   std::vector<uint32_t> h_mesh_height;
@@ -173,9 +182,11 @@ struct ge_rt_ctx {
   ge::DevBuf tmp[9];   // build batches: metadata, records, boxes, segments, order, codes, instances
   // A build's host arrays, copied to the device asynchronously: kept until the next build, which
   // starts after the last one's closing synchronization.
-  std::vector<uint32_t> host_meta, host_tmeta, host_live;
-  std::vector<ge::InstRec> host_recs;
-  std::vector<float> host_T;
+  std::vector<uint32_t> host_meta, host_tmeta;
+  // Pinned host memory for the build's end: the refusal flag, the instance structure's height and
+  // every mesh's height, read back with one synchronization.
+  uint32_t* host_readback = nullptr;
+  size_t host_readback_cap = 0;
   // 8 x u64 (closest rays, any rays, closest-hit nodes, triangles, instances, any-hit nodes,
   // triangles, instances), then the persistent walks' head counter at byte 64
   ge::DevBuf counters;

@@ -568,8 +568,9 @@ int32_t ge_rt_shadow_resolve(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_
  * context's growable buffer, and it calls no hipMalloc, hipFree or other allocating routine. The
  * answers are ge_rt_shadow_resolve's. Device pointers only (flags must hold GE_RT_DEVICE_PTR);
  * more than GE_RT_SHADOW_STATIC_ENTRIES entries return GE_RT_ERROR_INVALID_ARGUMENT; the scene's
- * pair table must be current, as every ge_rt_bvh_build leaves it (instances updated since the last
- * build return GE_RT_ERROR_INVALID_SCENE). Every call on one GPU shares that GPU's array: calls
+ * pair table must be current, as every ge_rt_bvh_build leaves it (instances given another mesh or
+ * pair range since the last build return GE_RT_ERROR_INVALID_SCENE; instances that only move keep
+ * it current). Every call on one GPU shares that GPU's array: calls
  * that overlap on one GPU (contexts on different streams) race; calls on one context are ordered
  * by its stream. */
 #define GE_RT_SHADOW_STATIC_ENTRIES (1u << 21)

@@ -4,7 +4,8 @@ Called through the C ABI with device pointers, on two calls in a row (the second
 first, so the fixed working array still holds the first call's records), full rate, half rate and
 jittered, on the 4M-triangle layout at 512 instances and the configuration-3 shadow scene: every
 answer bit-exact against the allocating entry point's, and the walk counters equal. A call past
-the array's limit, with host pointers, or after an instance update without a build is refused.
+the array's limit, with host pointers, or after an instance is given another pair range without a
+build is refused; an instance that only moves leaves the pair table current.
 """
 import ctypes as C
 
@@ -91,13 +92,18 @@ def test_noalloc_refusals(ctx):
     ctx.synchronize()
     answers = big_out.cpu().numpy().view(np.uint32)[: abi.shadow_entries(big)]
     assert np.all(answers == abi.SHADOW_NOT_TAKEN)  # empty words ask no ray
-    # Instances updated without a build: the pair table is stale.
+    # An instance given another pair range without a build: the pair table is stale.
     inst = s.arrays()["instances"]
     recs = (abi.InstanceDesc * 1).from_buffer_copy(inst[:1].tobytes())
+    recs[0].first_pair += 7
     g.update_instances(0, recs)
     st = L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(p), words.data_ptr(), out.data_ptr(), abi.DEVICE_PTR)
     assert st == 2
     g.build(abi.BUILD_TOP)
+    assert L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(p), words.data_ptr(), out.data_ptr(), abi.DEVICE_PTR) == 0
+    # An instance that only moves leaves the pair table current.
+    recs[0].rows[0][3] += 1.0
+    g.update_instances(0, recs)
     assert L.ge_rt_shadow_resolve_noalloc(ctx.h, g.h, C.byref(p), words.data_ptr(), out.data_ptr(), abi.DEVICE_PTR) == 0
     ctx.synchronize()
     g.close()
