@@ -155,7 +155,7 @@ def shadow_main(args):
 
     host = {"move": 0.0, "update": 0.0, "build": 0.0, "visibility": 0.0, "shadow": 0.0}
 
-    def frame(k, count=False):
+    def frame(k, count=False, refit=False):
         t = time.perf_counter()
         first = 1 + (k * moving) % (groups - moving + 1)
         recs = s.move_instances(first, moving, 1000 + k, 0.5, 0.2)
@@ -163,7 +163,10 @@ def shadow_main(args):
         t1 = time.perf_counter()
         scene.update_instances(first, recs)
         t2 = time.perf_counter()
-        scene.build(abi.BUILD_SUBSET, meshes=meshes)
+        if refit:
+            scene.build(abi.BUILD_REFIT)  # the groups are rigid: only the instance structure changes
+        else:
+            scene.build(abi.BUILD_SUBSET, meshes=meshes)
         build_ms = ctx.stats()["build_ms"]
         t3 = time.perf_counter()
         scene.ray_visibility(rv, words=words.data_ptr(), device=True)
@@ -201,6 +204,16 @@ def shadow_main(args):
         f"{['%.2f' % x for x in walk_ms]})")
     log(f"[bench] wall ms a frame by call (host + device): " +
         ", ".join(f"{k} {v / args.steps:.2f}" for k, v in host.items()))
+    # The same frames with the refit option instead of the rebuild (reported beside, not `value`).
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    refit_stages, refit_rays = [], 0
+    for k in range(args.steps):
+        refit_stages.append(frame(args.warmup + args.steps + k, refit=True))
+        refit_rays += int((out != -1).sum().item())
+    torch.cuda.synchronize()
+    refit_wall = time.perf_counter() - t0
+    log(f"[bench] refit frames: {refit_wall * 1e3:.1f} ms wall; refit {['%.3f' % x[0] for x in refit_stages]}")
     # One counted frame more for the walk's algorithmic bytes a ray.
     ctx.reset_stats()
     frame(args.warmup + args.steps, count=True)
@@ -221,6 +234,10 @@ def shadow_main(args):
                    "parallelism": "single GPU"},
         "shadow_rays_per_step": int(rays_per_frame),
         "shadow_pass_mrays_per_s": round(rays_per_frame / (sum(shadow_ms) / len(shadow_ms) * 1e-3) / 1e6, 1),
+        "refit_option": {"build_ms": round(sum(x[0] for x in refit_stages) / len(refit_stages), 3),
+                         "ms_per_step": round(refit_wall / args.steps * 1e3, 3),
+                         "mrays_per_s": round(refit_rays / refit_wall / 1e6, 2),
+                         "note": "GE_RT_BUILD_REFIT: the moved groups are rigid, so only the instance structure is refitted"},
         "stage_ms": {"rebuild": round(sum(build_ms) / len(build_ms), 3),
                      "visibility": round(sum(vis_ms) / len(vis_ms), 3),
                      "shadow": round(sum(shadow_ms) / len(shadow_ms), 3), "full_build": round(full_build, 3)},

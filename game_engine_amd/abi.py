@@ -163,7 +163,9 @@ PT_UNIFORM_SKY = 1
 PT_PIXEL_CENTER = 2
 PT_DITHER = 4
 PT_ALPHA_MASKS = 8
-BUILD_FULL, BUILD_SUBSET, BUILD_TOP = 0, 1, 2
+BUILD_FULL, BUILD_SUBSET, BUILD_TOP, BUILD_REFIT = 0, 1, 2, 3
+# Status codes (ge_rt_status).
+OK, ERROR_INVALID_ARGUMENT, ERROR_INVALID_SCENE, ERROR_OUT_OF_MEMORY, ERROR_DEVICE, ERROR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 SHADOW_HALF_RATE = 0x1
 SHADOW_JITTER = 0x2
 SHADOW_ALPHA_MASKS = 0x4

@@ -596,10 +596,13 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
                         uint32_t count) try {
   if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
   GE_CHECK(ctx, scene, GE_RT_ERROR_INVALID_ARGUMENT, "null scene");
-  GE_CHECK(ctx, flags <= GE_RT_BUILD_TOP, GE_RT_ERROR_INVALID_ARGUMENT, "unknown build flags");
+  GE_CHECK(ctx, flags <= GE_RT_BUILD_REFIT, GE_RT_ERROR_INVALID_ARGUMENT, "unknown build flags");
   SceneDev& s = scene->dev;
   GE_CHECK(ctx, flags == GE_RT_BUILD_FULL || s.built, GE_RT_ERROR_INVALID_SCENE, "a partial build needs a full one first");
-  GE_CHECK(ctx, flags != GE_RT_BUILD_SUBSET || meshes || !count, GE_RT_ERROR_INVALID_ARGUMENT, "null mesh list");
+  const bool subset = flags == GE_RT_BUILD_SUBSET || flags == GE_RT_BUILD_REFIT;
+  GE_CHECK(ctx, !subset || meshes || !count, GE_RT_ERROR_INVALID_ARGUMENT, "null mesh list");
+  GE_CHECK(ctx, flags != GE_RT_BUILD_REFIT || !s.live_dirty, GE_RT_ERROR_INVALID_SCENE,
+           "a refit keeps the instance structure's slots: an instance given another mesh needs a rebuild");
   GE_HIP(ctx, hipSetDevice(ctx->device));
   const hipStream_t st = ctx->stream;
   s.eye = eye ? *eye : ge_frame_eye{};
@@ -626,7 +629,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   std::vector<uint32_t> list;
   if (flags == GE_RT_BUILD_FULL) {
     for (uint32_t m = 0; m < s.mesh_count; ++m) list.push_back(m);
-  } else if (flags == GE_RT_BUILD_SUBSET) {
+  } else if (subset) {
     for (uint32_t k = 0; k < count; ++k) {
       GE_CHECK(ctx, meshes[k] < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
       list.push_back(meshes[k]);
@@ -747,7 +750,20 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, s.tlas_morton.reserve(size_t(std::max(nl, 1u)) * 4));
   GE_HIP(ctx, s.tlas_root.reserve(32));
   GE_HIP(ctx, s.tlas_height.reserve(4));
-  if (nl > 0) {
+  const bool refit = flags == GE_RT_BUILD_REFIT && nl >= 2;
+  if (refit) {
+    // The slots' boxes from the instances' current placement, then the structure refitted.
+    std::vector<uint32_t>& tmeta = ctx->host_tmeta;
+    tmeta = {0u, nl, 0u};
+    GE_HIP(ctx, upload(ctx->tmp[7], tmeta.data(), tmeta.size(), st));
+    GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + 256));
+    GE_HIP(ctx, ctx->tmp[8].reserve(size_t(nl) * 4));
+    float* boxes = ctx->tmp[6].as<float>();
+    GE_HIP(ctx, launch_instance_boxes(st, nl, s.tlas_slots.as<uint32_t>(), s.instances.as<ge_instance_desc>(),
+                                      s.inst_T.as<float>(), s.mesh_roots.as<float>(), boxes, ctx->tmp[8].as<uint32_t>()));
+    GE_HIP(ctx, refit_segment(ctx->build, st, nl, boxes, ctx->tmp[7].as<uint32_t>(), s.tlas_nodes.as<ge_bvh_node>(),
+                              s.tlas_wide.as<WideNode>(), s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), flag));
+  } else if (nl > 0) {
     std::vector<uint32_t>& tmeta = ctx->host_tmeta;
     tmeta = {0u, nl, 0u};  // seg_first, seg_count, node_base
     GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + 256));

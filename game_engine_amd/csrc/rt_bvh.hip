@@ -1041,7 +1041,70 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
   }
 }
 
+// For a refit: the parent links and the tile lists from the records a build left, and the
+// identity as the sorted order (the leaves' boxes are given in slot order).
+__global__ void k_parents_from_records(uint32_t n, const ge_bvh_node* __restrict__ nodes, uint32_t* node_parent) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 < n) node_parent[i] = nodes[i].parent;
+}
+__global__ void k_file_records(uint32_t n, const ge_bvh_node* __restrict__ nodes, TileArgs ta) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= n) return;
+  const uint32_t gamma = nodes[i].child[0] & ~GE_BVH_LEAF, span = nodes[i].count;
+  const uint32_t lo = i <= gamma ? i : i + 1 - span, hi = i <= gamma ? i + span - 1 : i;
+  file_node(ta, 0, i, lo, hi);
+}
+__global__ void k_identity(uint32_t n, uint32_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i;
+}
+
 inline uint32_t blocks(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+
+// The tile ladder of a device-wide build: how many levels, each level's tiles, the lists above
+// level 1 (cleared).
+struct TileLadder {
+  TileArgs args{};
+  uint32_t levels = 1;
+  uint32_t tiles[kMaxLevels] = {};
+};
+
+hipError_t tile_ladder(BuildScratch& sc, hipStream_t stream, uint32_t nseg, const uint32_t* h_seg_count, TileLadder& lad) {
+  uint32_t levels = 1;
+  for (uint32_t q = 0; q < nseg; ++q)
+    while (levels < kMaxLevels && (uint64_t(1) << (kTileShift + kFanShift * (levels - 1))) < h_seg_count[q]) ++levels;
+  lad.levels = levels;
+  lad.args.nseg = nseg;
+  lad.args.levels = levels;
+  for (uint32_t l = 0; l < levels; ++l) {
+    const uint32_t shift = kTileShift + kFanShift * l;
+    uint64_t t = 0;
+    for (uint32_t q = 0; q < nseg; ++q) t += (uint64_t(h_seg_count[q]) + (1ull << shift) - 1) >> shift;
+    lad.tiles[l] = uint32_t(t);
+    lad.args.bucket_base[l + 1] = lad.args.bucket_base[l] + (l == 0 ? 0u : lad.tiles[l]);  // level 1 keeps no list
+  }
+  const uint32_t lists = lad.args.bucket_base[levels];
+  hipError_t e;
+  if ((e = sc.buckets.reserve(size_t(std::max(lists, 1u)) * kSparseCap * 4)) != hipSuccess) return e;
+  if ((e = sc.bucket_count.reserve(size_t(std::max(lists, 1u)) * 4)) != hipSuccess) return e;
+  if (lists && (e = hipMemsetAsync(sc.bucket_count.ptr, 0, size_t(lists) * 4, stream)) != hipSuccess) return e;
+  lad.args.list = sc.buckets.as<uint32_t>();
+  lad.args.count = sc.bucket_count.as<uint32_t>();
+  lad.args.depth = sc.depth.as<uint32_t>();
+  return hipSuccess;
+}
+
+// Depths above level 1 (top down), level 1 whole, the fit above level 1 (bottom up), the collapse
+// above level 1.
+hipError_t tile_fit(hipStream_t stream, const TileLadder& lad) {
+  const TileArgs& ta = lad.args;
+  for (uint32_t l = lad.levels; l-- > 1;) k_depth_sparse<<<lad.tiles[l], kSparseThreads, 0, stream>>>(ta, l);
+  k_tile_build<<<lad.tiles[0], kTileThreads, 0, stream>>>(ta);
+  for (uint32_t l = 1; l < lad.levels; ++l) k_fit_sparse<<<lad.tiles[l], kSparseThreads, 0, stream>>>(ta, l);
+  const uint32_t lists = ta.bucket_base[lad.levels];
+  if (lists && ta.wide) k_collapse_sparse<<<lists, kSparseThreads, 0, stream>>>(ta);
+  return hipGetLastError();
+}
 
 }  // namespace
 
@@ -1103,25 +1166,9 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   }
   const uint64_t* sorted_keys = sc.keys[cur].as<uint64_t>();
   const uint32_t* sorted_vals = sc.vals[cur].as<uint32_t>();
-  // ---- the tile ladder: how many levels, each level's tiles, and the lists above level 1 ----
-  uint32_t levels = 1;
-  for (uint32_t q = 0; q < nseg; ++q)
-    while (levels < kMaxLevels && (uint64_t(1) << (kTileShift + kFanShift * (levels - 1))) < h_seg_count[q]) ++levels;
-  uint32_t level_tiles[kMaxLevels] = {};
-  TileArgs ta{};
-  ta.nseg = nseg;
-  ta.levels = levels;
-  for (uint32_t l = 0; l < levels; ++l) {
-    const uint32_t shift = kTileShift + kFanShift * l;
-    uint64_t t = 0;
-    for (uint32_t q = 0; q < nseg; ++q) t += (uint64_t(h_seg_count[q]) + (1ull << shift) - 1) >> shift;
-    level_tiles[l] = uint32_t(t);
-    ta.bucket_base[l + 1] = ta.bucket_base[l] + (l == 0 ? 0u : level_tiles[l]);  // level 1 keeps no list
-  }
-  const uint32_t lists = ta.bucket_base[levels];
-  if ((e = sc.buckets.reserve(size_t(std::max(lists, 1u)) * kSparseCap * 4)) != hipSuccess) return e;
-  if ((e = sc.bucket_count.reserve(size_t(std::max(lists, 1u)) * 4)) != hipSuccess) return e;
-  if (lists && (e = hipMemsetAsync(sc.bucket_count.ptr, 0, size_t(lists) * 4, stream)) != hipSuccess) return e;
+  TileLadder lad;
+  if ((e = tile_ladder(sc, stream, nseg, h_seg_count, lad)) != hipSuccess) return e;
+  TileArgs& ta = lad.args;
   ta.seg_first = d_seg_first;
   ta.seg_count = d_seg_count;
   ta.node_base = d_node_base;
@@ -1132,25 +1179,46 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   ta.node_box = sc.node_box.as<float>();
   ta.roots = roots;
   ta.heights = heights;
-  ta.list = sc.buckets.as<uint32_t>();
-  ta.count = sc.bucket_count.as<uint32_t>();
-  ta.depth = sc.depth.as<uint32_t>();
   ta.wide = wide;
   ta.flag = flag;
-
   k_karras<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, d_seg_of, d_seg_first, d_seg_count, d_node_base, nodes,
                                                sc.node_parent.as<uint32_t>(), ta);
   k_codes<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, morton);
   if ((e = hipMemcpyAsync(perm, sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
   ta.perm = sorted_vals;
+  return tile_fit(stream, lad);
+}
 
-  // ---- depths above level 1 (top down), level 1 whole, the fit above level 1 (bottom up) ----
-  for (uint32_t l = levels; l-- > 1;)
-    k_depth_sparse<<<level_tiles[l], kSparseThreads, 0, stream>>>(ta, l);
-  k_tile_build<<<level_tiles[0], kTileThreads, 0, stream>>>(ta);
-  for (uint32_t l = 1; l < levels; ++l) k_fit_sparse<<<level_tiles[l], kSparseThreads, 0, stream>>>(ta, l);
-  if (lists && wide) k_collapse_sparse<<<lists, kSparseThreads, 0, stream>>>(ta);
-  return hipGetLastError();
+hipError_t refit_segment(BuildScratch& sc, hipStream_t stream, uint32_t n, const float* slot_boxes,
+                         const uint32_t* d_seg_tables, ge_bvh_node* nodes, WideNode* wide, float* root, uint32_t* height,
+                         uint32_t* flag) {
+  if (n < 2) return hipErrorInvalidValue;
+  hipError_t e;
+  if ((e = sc.node_box.reserve(size_t(n) * 32)) != hipSuccess) return e;
+  if ((e = sc.node_parent.reserve(size_t(n) * 4)) != hipSuccess) return e;
+  if ((e = sc.depth.reserve(size_t(n) * 4)) != hipSuccess) return e;
+  if ((e = sc.vals[0].reserve(size_t(n) * 4)) != hipSuccess) return e;
+  TileLadder lad;
+  if ((e = tile_ladder(sc, stream, 1, &n, lad)) != hipSuccess) return e;
+  TileArgs& ta = lad.args;
+  ta.seg_first = d_seg_tables;
+  ta.seg_count = d_seg_tables + 1;
+  ta.node_base = d_seg_tables + 2;
+  ta.root_index = nullptr;
+  ta.boxes = slot_boxes;
+  ta.nodes = nodes;
+  ta.node_parent = sc.node_parent.as<uint32_t>();
+  ta.node_box = sc.node_box.as<float>();
+  ta.roots = root;
+  ta.heights = height;
+  ta.wide = wide;
+  ta.flag = flag;
+  k_identity<<<blocks(n, 256), 256, 0, stream>>>(n, sc.vals[0].as<uint32_t>());
+  ta.perm = sc.vals[0].as<uint32_t>();
+  k_parents_from_records<<<blocks(n, 256), 256, 0, stream>>>(n, nodes, sc.node_parent.as<uint32_t>());
+  if (lad.levels > 1) k_file_records<<<blocks(n, 256), 256, 0, stream>>>(n, nodes, ta);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return tile_fit(stream, lad);
 }
 
 hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,

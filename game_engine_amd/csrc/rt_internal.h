@@ -227,6 +227,12 @@ hipError_t build_segments(BuildScratch& scratch, hipStream_t stream, uint32_t n,
                           uint32_t* perm, uint32_t* morton, float* roots, uint32_t* heights,
                           const uint32_t* d_root_index, uint32_t node_pool, WideNode* wide, uint32_t nseg_global,
                           uint32_t n_global, const uint32_t* h_seg_count, uint32_t* flag);
+// The refit of one structure of n >= 2 leaves (GE_RT_BUILD_REFIT): its records' topology kept,
+// the leaves' boxes given in slot order (6 floats each), every node's box, the records, the wide
+// nodes and the root's box and height recomputed. d_seg_tables holds {0, n, 0} on the device.
+hipError_t refit_segment(BuildScratch& scratch, hipStream_t stream, uint32_t n, const float* slot_boxes,
+                         const uint32_t* d_seg_tables, ge_bvh_node* nodes, WideNode* wide, float* root,
+                         uint32_t* height, uint32_t* flag);
 // Triangle records, boxes and segment ids of the clusters in `cluster_list` (cluster k's triangles
 // at batch index cluster_out[k] on; cluster_local[k] its index within its mesh).
 hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,

@@ -471,10 +471,19 @@ void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene);
  * domain/gfx/shaders/tlas_references.slang:75-104). `eye` may be NULL (the world's origin).
  * flags: GE_RT_BUILD_FULL builds everything; GE_RT_BUILD_SUBSET rebuilds the meshes named in
  * `meshes[0..count)` and then the top level; GE_RT_BUILD_TOP rebuilds the top level only (for
- * instances that moved: re-upload them with ge_rt_scene_update_instances first). */
+ * instances that moved: re-upload them with ge_rt_scene_update_instances first).
+ * GE_RT_BUILD_REFIT (no counterpart in the reference, whose structures are the driver's; the
+ * per-frame update's cheaper option) rebuilds the meshes named, as GE_RT_BUILD_SUBSET does, then
+ * refits the top level: the topology, sorted slots and codes of its last build are kept and its
+ * boxes and wide nodes recomputed from the instances' current placement in the frame of `eye`. Hits
+ * are a rebuild's (the walk's answer does not depend on the tree); the node visits grow as
+ * instances move away from where the structure was built. It needs the same instances holding
+ * geometry as at that build: an instance given another mesh since returns
+ * GE_RT_ERROR_INVALID_SCENE. */
 #define GE_RT_BUILD_FULL 0u
 #define GE_RT_BUILD_SUBSET 1u
 #define GE_RT_BUILD_TOP 2u
+#define GE_RT_BUILD_REFIT 3u
 int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, const ge_frame_eye* eye,
                         const uint32_t* meshes, uint32_t count);
 /* Replaces instance records [first, first + count) (the per-frame placement the engine's

@@ -67,6 +67,7 @@ def lib():
         L.ge_oracle_bvh_info.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.ge_oracle_bvh_download.argtypes = [vp, C.c_uint32, vp, vp, vp, vp]
         L.ge_oracle_update_instances.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
+        L.ge_oracle_refit_instances.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.ge_oracle_trace_closest.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32, C.c_uint32]
         L.ge_oracle_trace_any.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32, C.c_uint32]
         L.ge_oracle_sample.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, vp]
@@ -155,9 +156,12 @@ class Scene:
                                          morton.ctypes.data)
         return nodes, prims, morton
 
-    def update_instances(self, first, records, eye=None):
-        lib().ge_oracle_update_instances(self.h, first, len(records), C.byref(records[0]) if len(records) else None,
-                                         C.byref(eye) if eye else None)
+    def update_instances(self, first, records, eye=None, refit=False):
+        """Replaces instance records and rebuilds the instance structure (or refits it: its topology
+        and slots kept; returns the status, GE_RT_ERROR_INVALID_SCENE when that cannot be)."""
+        fn = lib().ge_oracle_refit_instances if refit else lib().ge_oracle_update_instances
+        return fn(self.h, first, len(records), C.byref(records[0]) if len(records) else None,
+                  C.byref(eye) if eye else None)
 
     def trace_closest(self, rays, mask=0xFF, brute=False, threads=0, flags=0):
         hits = np.zeros(len(rays), dtype=abi.HIT_DTYPE)

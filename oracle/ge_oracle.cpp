@@ -194,6 +194,8 @@ inline int delta(const std::vector<uint32_t>& codes, int n, int i, int j) {
   return 32 + __builtin_clz(uint32_t(i) ^ uint32_t(j));
 }
 
+template <class LeafBox> void lbvh_fit(Lbvh& out, const LeafBox& leaf_box);
+
 void lbvh_build(const std::vector<Box>& boxes, Lbvh& out) {
   const uint32_t n = uint32_t(boxes.size());
   out = Lbvh{};
@@ -273,7 +275,12 @@ void lbvh_build(const std::vector<Box>& boxes, Lbvh& out) {
     if (!(node.child[0] & GE_BVH_LEAF)) out.nodes[size_t(gamma)].parent = uint32_t(i);
     if (!(node.child[1] & GE_BVH_LEAF)) out.nodes[size_t(gamma + 1)].parent = uint32_t(i);
   }
-  // Bottom-up fit, written as a post-order walk; min and max are exact, so any order is the same.
+  lbvh_fit(out, leaf_box);
+}
+
+// Bottom-up fit, written as a post-order walk; min and max are exact, so any order is the same.
+template <class LeafBox> void lbvh_fit(Lbvh& out, const LeafBox& leaf_box) {
+  const uint32_t n = out.n;
   std::vector<Box> nb(n - 1);
   std::vector<uint8_t> done(n - 1, 0);
   std::vector<uint32_t> stack;
@@ -664,6 +671,55 @@ void build_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
   collapse(s.tlas, s.tlas_wide);
   s.tlas_slots.resize(ids.size());
   for (size_t k = 0; k < ids.size(); ++k) s.tlas_slots[k] = ids[s.tlas.order[k]];
+}
+
+// The instance records in the frame of `eye`; returns whether the instances with geometry are
+// still the ones the instance structure holds (its slots name the same set).
+void instance_records(ge_oracle_scene& s, const ge_frame_eye& eye) {
+  const uint32_t ni = uint32_t(s.instances.size());
+  s.inst.assign(ni, InstRec{});
+  for (uint32_t i = 0; i < ni; ++i) {
+    const ge_instance_desc& in = s.instances[i];
+    float T[3];
+    instance_from_eye(in, eye, T);
+    InstRec& rec = s.inst[i];
+    make_w2o(in, T, rec.w2o);
+    rec.mesh = in.mesh;
+    rec.first_pair = in.first_pair;
+    rec.mask = instance_ray_mask(in.flags);
+    rec.live = s.blas[in.mesh].bvh.n > 0;
+  }
+}
+
+// The refit of the instance structure (GE_RT_BUILD_REFIT, the backend's own option): the topology,
+// sorted slots and codes of the last build kept, each slot's box recomputed from its instance's
+// current placement in the frame of `eye`, the nodes' boxes fitted again bottom-up and the wide
+// nodes collapsed again. False (nothing changed) when the instances with geometry are no longer
+// the slots' set.
+bool refit_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
+  instance_records(s, eye);
+  uint32_t live = 0;
+  for (const InstRec& r : s.inst) live += r.live ? 1u : 0u;
+  if (live != s.tlas_slots.size()) return false;
+  for (uint32_t id : s.tlas_slots)
+    if (!s.inst[id].live) return false;
+  std::vector<Box> slot_box(s.tlas_slots.size());
+  for (size_t k = 0; k < slot_box.size(); ++k) {
+    const uint32_t id = s.tlas_slots[k];
+    const ge_instance_desc& in = s.instances[id];
+    float T[3];
+    instance_from_eye(in, eye, T);
+    slot_box[k] = instance_box(in, T, s.blas[in.mesh].bvh.root);
+  }
+  if (s.tlas.n == 1) {
+    std::memcpy(s.tlas.nodes[0].lo0, slot_box[0].lo, 12);
+    std::memcpy(s.tlas.nodes[0].hi0, slot_box[0].hi, 12);
+    s.tlas.root = slot_box[0];
+  } else if (s.tlas.n > 1) {
+    lbvh_fit(s.tlas, [&](uint32_t slot) { return slot_box[slot]; });
+  }
+  collapse(s.tlas, s.tlas_wide);
+  return true;
 }
 
 void build_blas(ge_oracle_scene& s, uint32_t m) {
@@ -2229,6 +2285,23 @@ int32_t ge_oracle_update_instances(ge_oracle_scene* s, uint32_t first, uint32_t 
   const ge_frame_eye zero{};
   s->eye = eye ? *eye : zero;
   build_tlas(*s, s->eye);
+  return GE_RT_OK;
+}
+
+int32_t ge_oracle_refit_instances(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_instance_desc* in,
+                                  const ge_frame_eye* eye) {
+  if (first + count > s->instances.size()) return GE_RT_ERROR_INVALID_ARGUMENT;
+  std::vector<ge_instance_desc> was(s->instances);
+  std::memcpy(&s->instances[first], in, size_t(count) * sizeof(ge_instance_desc));
+  const ge_frame_eye zero{};
+  const ge_frame_eye was_eye = s->eye;
+  s->eye = eye ? *eye : zero;
+  if (!refit_tlas(*s, s->eye)) {
+    s->instances = was;
+    s->eye = was_eye;
+    instance_records(*s, s->eye);
+    return GE_RT_ERROR_INVALID_SCENE;
+  }
   return GE_RT_OK;
 }
 

@@ -42,6 +42,11 @@ int32_t ge_oracle_bvh_download(const ge_oracle_scene* scene, uint32_t mesh, ge_b
 /* Replaces instance records and rebuilds the top level (the per-frame placement). */
 int32_t ge_oracle_update_instances(ge_oracle_scene* scene, uint32_t first, uint32_t count,
                                    const ge_instance_desc* instances, const ge_frame_eye* eye);
+/* The refit of the instance structure (GE_RT_BUILD_REFIT): its topology and slots kept, its boxes
+ * and wide nodes recomputed; GE_RT_ERROR_INVALID_SCENE (and nothing changed) when the instances
+ * with geometry are no longer the structure's slots. */
+int32_t ge_oracle_refit_instances(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_instance_desc* in,
+                                  const ge_frame_eye* eye);
 
 /* Queries. `brute` = 1 tests every triangle of every instance instead of walking the BVH. `flags`:
  * GE_RT_ALPHA_MASKS or GE_RT_CULL_MASKED (closest only), as ge_rt_trace_closest/any take them. */
