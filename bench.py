@@ -13,8 +13,8 @@ value = rays / wall seconds of the K timed frames. The BVH build is timed separa
 
 roofline: the dominant kernel of the frame (the traversal walk with the larger share of device time:
 k_closest, the closest-hit walk of one bounce, or k_any, the any-hit shadow walk of one bounce). Its
-algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 112 B per 4-wide node visited (the
-seven 16-byte loads of its boxes and references) + 48 B per triangle tested + 64 B per instance
+algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 64 B per 4-wide node visited (the
+whole quantized node: four 16-byte loads) + 48 B per triangle tested + 64 B per instance
 entered, from the counted replay's per-kind counters divided by its launches per frame; its mean
 launch time is the library's own HIP events around each launch on its stream, over the timed
 frames. Peak: MI355X's 8 TB/s. `traffic` is that kernel's measured HBM bytes per launch from the
@@ -45,7 +45,7 @@ sys.path.insert(0, ROOT)
 
 WIDTH, HEIGHT = 1920, 1080
 PEAK_HBM_GBPS = 8000.0
-NODE_BYTES = 112  # a 4-wide node visit loads its four child boxes and references: 7 x 16 B
+NODE_BYTES = 64  # a 4-wide node visit loads the whole quantized node: 4 x 16 B
 METRIC = "Mrays/s (closest+any-hit); 1080p time-to-equal-RMSE, cached vs plain tracer"
 
 

@@ -135,6 +135,8 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instance_id", 
                       ("geometry", "<u4"), ("primitive", "<u4"), ("key", "<u4"), ("pad", "<u4")])
 NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3), ("hi1", "<f4", 3),
                        ("child", "<u4", 2), ("parent", "<u4"), ("count", "<u4")])
+WIDE_DTYPE = np.dtype([("origin", "<f4", 3), ("meta", "<u4"), ("child", "<u4", 4), ("lo", "<u4", 3), ("hi", "<u4", 3),
+                       ("reserved", "<u4", 2)])
 TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3), ("prim", "<u4"),
                       ("index", "<u4"), ("pad", "<u4")])
 RECEIVER_DTYPE = np.dtype([("position", "<f4", 3), ("instance_id", "<u4"), ("geometric", "<f4", 3),
@@ -147,7 +149,7 @@ SIZES = {ClusterDesc: 48, VertexAttributes: 8, MeshDesc: 64, InstanceDesc: 96, F
 for _t, _n in SIZES.items():
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t), _n)
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
-assert NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and RECEIVER_DTYPE.itemsize == 32
+assert NODE_DTYPE.itemsize == 64 and WIDE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and RECEIVER_DTYPE.itemsize == 32
 
 OK = 0
 DEVICE_PTR = 0x1

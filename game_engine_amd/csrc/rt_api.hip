@@ -864,6 +864,27 @@ int32_t ge_rt_bvh_download(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t me
   return host_failure(ctx);
 }
 
+int32_t ge_rt_bvh_download_wide(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, ge_bvh_wide_node* wide) try {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && scene->dev.built, GE_RT_ERROR_INVALID_SCENE, "no structure built");
+  GE_CHECK(ctx, wide != nullptr, GE_RT_ERROR_INVALID_ARGUMENT, "no output");
+  const SceneDev& s = scene->dev;
+  GE_HIP(ctx, hipSetDevice(ctx->device));
+  if (mesh == 0xffffffffu) {
+    if (s.tlas_node_count)
+      GE_HIP(ctx, hipMemcpy(wide, s.tlas_wide.ptr, size_t(s.tlas_node_count) * sizeof(WideNode), hipMemcpyDeviceToHost));
+    return GE_RT_OK;
+  }
+  GE_CHECK(ctx, mesh < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
+  const uint32_t nn = s.mesh_node_count[mesh];
+  if (nn)
+    GE_HIP(ctx, hipMemcpy(wide, s.blas_wide.as<WideNode>() + s.mesh_node_base[mesh], size_t(nn) * sizeof(WideNode),
+                          hipMemcpyDeviceToHost));
+  return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
+}
+
 }  // extern "C"
 
 namespace {

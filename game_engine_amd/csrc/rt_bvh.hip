@@ -390,24 +390,84 @@ __device__ __forceinline__ void store_record(ge_bvh_node* out, const float* lo0,
   o[2] = make_uint4(__float_as_uint(lo1[2]), __float_as_uint(hi1[0]), __float_as_uint(hi1[1]), __float_as_uint(hi1[2]));
   o[3] = make_uint4(c0, c1, parent, count);
 }
+// The unbiased exponent of a finite x >= 0 (-127 for zero and subnormals).
+__device__ __forceinline__ int exponent_of(float x) { return int((__float_as_uint(x) >> 23) & 0xffu) - 127; }
+// 2^e for -126 <= e <= 127, exactly.
+__device__ __forceinline__ float pow2(int e) { return __uint_as_float(uint32_t(e + 127) << 23); }
+
+// The grid of one axis of a wide node whose children's low planes reach down to L and high planes
+// up to H: the smallest exponent e >= -100 at which L / 2^e and H / 2^e are below 2^23 in magnitude
+// (so the floors and ceilings below are exact in float) and ceil(H / 2^e) - floor(L / 2^e) <= 253
+// (so every index fits a byte with the extra step on each side); k0 = floor(L / 2^e). Every
+// operation is exact, so the grid is a function of the box planes alone. (Planes beyond 2^100 or
+// not finite end the search at e = 126 with indices that mean nothing; boxes of finite geometry
+// never get there.)
+__device__ void axis_grid(float L, float H, int& e, float& k0) {
+  const float m = __builtin_fmaxf(__builtin_fabsf(L), __builtin_fabsf(H));
+  int g = exponent_of(m) - 22;
+  const int span = exponent_of(H - L) - 7;
+  g = g > span ? g : span;
+  g = g > -100 ? g : -100;
+  float a = 0.0f;
+  for (; g < 126; ++g) {
+    const float down = pow2(-g);
+    a = __builtin_floorf(L * down);
+    const float b = __builtin_ceilf(H * down);
+    if (b - a <= 253.0f && __builtin_fabsf(a) < 8388608.0f && __builtin_fabsf(b) < 8388608.0f) break;
+  }
+  e = g;
+  k0 = a;
+}
+
+// A plane x, in grid steps, rounded outward: down (up) to the grid, and one step further when the
+// rounding left less than 2^-10 of a step (x - floor(x) and ceil(x) - x are exact), so that a
+// quantized box holds its child's box with at least 2^-10 of a step to spare on every side.
+__device__ __forceinline__ float grid_down(float x) {
+  const float f = __builtin_floorf(x);
+  return x - f < 0.0009765625f ? f - 1.0f : f;
+}
+__device__ __forceinline__ float grid_up(float x) {
+  const float c = __builtin_ceilf(x);
+  return c - x < 0.0009765625f ? c + 1.0f : c;
+}
+
 // A wide node from four (reference, box) entries, of which the first `valid` are real; the rest
-// get +infinity boxes (every slab test against them fails) and GE_BVH_NONE.
+// are empty (GE_BVH_NONE, no occupied bit, zero planes); a leaf reference sets its slot's leaf bit
+// (bit 28 + slot of meta, beside the occupied bit at 24 + slot). Per axis, the grid of axis_grid with its
+// origin at k0 - 1 steps, and each child's planes at grid_down(lo / 2^e) - (k0 - 1) and
+// grid_up(hi / 2^e) - (k0 - 1), within [0, 255].
 __device__ __forceinline__ void store_wide(WideNode* out, const uint32_t ref[4], const float* box[4], int valid) {
-  float v[6][4];
+  uint32_t meta = ((1u << valid) - 1u) << 24, lo[3] = {0u, 0u, 0u}, hi[3] = {0u, 0u, 0u};
+  float origin[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float L = box[0][a], H = box[0][3 + a];
+    for (int q = 1; q < valid; ++q) {
+      L = __builtin_fminf(L, box[q][a]);
+      H = __builtin_fmaxf(H, box[q][3 + a]);
+    }
+    int e;
+    float k0;
+    axis_grid(L, H, e, k0);
+    const float down = pow2(-e), base = k0 - 1.0f;
+    origin[a] = base * pow2(e);
+    meta |= uint32_t(e + 127) << (8 * a);
+    for (int q = 0; q < valid; ++q) {
+      lo[a] |= uint32_t(grid_down(box[q][a] * down) - base) << (8 * q);
+      hi[a] |= uint32_t(grid_up(box[q][3 + a] * down) - base) << (8 * q);
+    }
+  }
   uint32_t c[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const bool on = q < valid;
-#pragma unroll
-    for (int f = 0; f < 6; ++f) v[f][q] = on ? box[q][f] : __builtin_huge_valf();
-    c[q] = on ? ref[q] : GE_BVH_NONE;
+    c[q] = q < valid ? ref[q] : GE_BVH_NONE;
+    if (q < valid && (ref[q] & GE_BVH_LEAF)) meta |= 1u << (28 + q);
   }
   uint4* o = reinterpret_cast<uint4*>(out);
-#pragma unroll
-  for (int f = 0; f < 6; ++f)
-    o[f] = make_uint4(__float_as_uint(v[f][0]), __float_as_uint(v[f][1]), __float_as_uint(v[f][2]), __float_as_uint(v[f][3]));
-  o[6] = make_uint4(c[0], c[1], c[2], c[3]);
-  o[7] = make_uint4(0u, 0u, 0u, 0u);
+  o[0] = make_uint4(__float_as_uint(origin[0]), __float_as_uint(origin[1]), __float_as_uint(origin[2]), meta);
+  o[1] = make_uint4(c[0], c[1], c[2], c[3]);
+  o[2] = make_uint4(lo[0], lo[1], lo[2], hi[0]);
+  o[3] = make_uint4(hi[1], hi[2], 0u, 0u);
 }
 
 // The 4-wide node of binary node i from the records in global memory: its children's children (a

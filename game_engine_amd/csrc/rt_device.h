@@ -277,17 +277,13 @@ GE_DEV bool candidate_kept(const MaskView& mv, uint32_t cluster, uint32_t materi
   return alpha_value >= offset.z;
 }
 
-// A 4-wide node: two binary levels collapsed (an even-depth node's children's children, a leaf
-// child taken as it is), its four child boxes axis by axis and its four references (bit 31 a leaf
-// slot, GE_BVH_NONE empty, else the binary index of an even-depth node, whose wide node sits at the
-// same index). An empty slot's box is +inf on every axis. 128 bytes: two 64-byte lines, of which a
-// visit loads the first 112 as seven 16-byte words.
-struct WideNode {
-  float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
-  uint32_t child[4];
-  uint32_t pad[4];
-};
-static_assert(sizeof(WideNode) == 128, "a wide node is two 64-byte lines");
+// A 4-wide node (ge_bvh_wide_node): two binary levels collapsed, its four references and its four
+// child boxes on a per-axis grid of 8-bit plane indices, each plane rounded outward (one step
+// further where the rounding left less than 2^-10 of a step). 64 bytes, one line, loaded as four
+// 16-byte words: origin and exponents (byte 3 of word 3: bit c for an occupied slot c, bit 4 + c
+// for a leaf slot c), references, the low planes (byte c of lo[a] is slot c's) and the high planes.
+using WideNode = ge_bvh_wide_node;
+static_assert(sizeof(WideNode) == 64, "a wide node is one 64-byte line");
 
 struct Bvh {
   const ge_bvh_node* tlas;       // the top level's nodes (node 0 the root)
@@ -347,10 +343,6 @@ struct Ray {
   float teps;  // absolute slack on the box test, in t: 2^-19 of the origin's largest coordinate
   int kx, ky, kz;
   float Sx, Sy, Sz;
-  // Byte offsets within a wide node of each axis's near and far planes, axis a in bits 8a..8a+7:
-  // where the ray's reciprocal is positive the low plane (16 a) is near and the high one (48 + 16 a)
-  // far; where it is negative the other way round.
-  uint32_t near, far;
 };
 
 GE_DEV float sel3(const float v[3], int k) {
@@ -386,10 +378,6 @@ GE_DEV void prep(Ray& r) {
   r.Sz = sel3(r.inv, kz);
   r.Sx = sel3(r.d, kx) * r.Sz;
   r.Sy = sel3(r.d, ky) * r.Sz;
-  const uint32_t flip = ((__float_as_uint(r.inv[0]) >> 31) * 48u) | ((__float_as_uint(r.inv[1]) >> 31) * (48u << 8)) |
-                        ((__float_as_uint(r.inv[2]) >> 31) * (48u << 16));
-  r.near = 0x00201000u + flip;           // 0, 16, 32, each + 48 where flipped
-  r.far = 0x00504030u - flip;            // 48, 64, 80, each - 48 where flipped
   r.teps = maxr(maxr(__builtin_fabsf(r.o[0]), __builtin_fabsf(r.o[1])), __builtin_fabsf(r.o[2])) * 1.9073486328125e-6f *
            __builtin_fabsf(r.Sz);
 }
@@ -407,38 +395,28 @@ GE_DEV Ray make_ray(f3 o, f3 d, float tmin) {
   return r;
 }
 
-// The box test: the slab interval, its far end widened by 1 + 2^-20 (Ize 2013, "Robust BVH Ray
-// Traversal"), against [tmin, tmax], both widened by the ray's absolute slack `teps`, so that a
-// triangle a few coordinate ulps short of its box's computed entry (a hit very near an origin far
-// from the world's) is never culled with its box. No operand is NaN (a direction's reciprocal is
-// finite and non-zero), so the hardware's min and max (v_min3/v_max3 after combining) give the
-// compare-and-select forms' values up to the sign of a zero, which no comparison below can see.
-GE_DEV bool slab(const float lo[3], const float hi[3], const Ray& r, float tmax, float& tnear) {
-  const float tx0 = (lo[0] - r.o[0]) * r.inv[0], tx1 = (hi[0] - r.o[0]) * r.inv[0];
-  const float ty0 = (lo[1] - r.o[1]) * r.inv[1], ty1 = (hi[1] - r.o[1]) * r.inv[1];
-  const float tz0 = (lo[2] - r.o[2]) * r.inv[2], tz1 = (hi[2] - r.o[2]) * r.inv[2];
-  const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
-                                   __builtin_fmaxf(__builtin_fminf(tz0, tz1), r.tmin));
-  const float tf = __builtin_fminf(
-      __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)), __builtin_fmaxf(tz0, tz1)) *
-          k_box_robust,
-      tmax);
+// The box test of a quantized box (Ize 2013, "Robust BVH Ray Traversal"): the slab interval, its
+// far end widened by 1 + 2^-20, against [tmin, tmax], both widened by the ray's absolute slack
+// `teps`, so that a triangle a few coordinate ulps short of its box's computed entry (a hit very
+// near an origin far from the world's) is never culled with its box. Each axis's near and far
+// distances come in (nt, ft), computed as fma(q, inv 2^(e - 127), (origin - o) inv) from the
+// plane's grid index q. Against the exact distance of that plane, the computed one errs by at most
+// 3 2^-24 of itself plus 2^-15 of a grid step's length along the ray (the node's origin lies at
+// most 255 steps beyond the plane), so the 2^-10 of a step or more to spare on every side of a
+// quantized box covers the second part and the far end's widening the first: the test passes wherever the fp32 slab
+// test, (plane - o) inv on the child's own box, would. fma is monotone in q, so the near distance
+// is the smaller of the axis's two. No operand is NaN (a direction's reciprocal is finite and
+// non-zero), so the hardware's min and max (v_min3/v_max3) give the compare-and-select forms'
+// values up to the sign of a zero, which no comparison below can see.
+GE_DEV bool slab_q(const float nt[3], const float ft[3], const Ray& r, float tmax, float& tnear) {
+  const float tn = __builtin_fmaxf(__builtin_fmaxf(nt[0], nt[1]), __builtin_fmaxf(nt[2], r.tmin));
+  const float tf = __builtin_fminf(__builtin_fminf(__builtin_fminf(ft[0], ft[1]), ft[2]) * k_box_robust, tmax);
   tnear = tn;
   return tn <= tf + r.teps;
 }
 
-// slab() with each axis's near and far plane given (the low plane is near where the ray's
-// reciprocal is positive): the same comparisons on the same distances, without a min or max per
-// axis. Requires lo <= hi on every axis, which every built box satisfies.
-GE_DEV bool slab_ordered(const float nearp[3], const float farp[3], const Ray& r, float tmax, float& tnear) {
-  const float nx = (nearp[0] - r.o[0]) * r.inv[0], fx = (farp[0] - r.o[0]) * r.inv[0];
-  const float ny = (nearp[1] - r.o[1]) * r.inv[1], fy = (farp[1] - r.o[1]) * r.inv[1];
-  const float nz = (nearp[2] - r.o[2]) * r.inv[2], fz = (farp[2] - r.o[2]) * r.inv[2];
-  const float tn = __builtin_fmaxf(__builtin_fmaxf(nx, ny), __builtin_fmaxf(nz, r.tmin));
-  const float tf = __builtin_fminf(__builtin_fminf(__builtin_fminf(fx, fy), fz) * k_box_robust, tmax);
-  tnear = tn;
-  return tn <= tf + r.teps;
-}
+// Byte c of a plane word as a float (v_cvt_f32_ubyte0..3).
+GE_DEV float plane_byte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
 // a*b - c*d to within 2 ulp, exact in sign and exactly zero only when the value is (Kahan's
 // algorithm with one fused multiply-add; Jeannerod, Louvet and Muller 2013, Math. Comp. 82).
@@ -580,7 +558,7 @@ struct Walker {
     state = (!in_blas && (node & GE_BVH_LEAF)) ? k_walk_enter : k_walk_node;
   }
   // Takes the next node off the stack (a sentinel leads back to the top level), or ends the walk.
-  GE_DEV void next(uint32_t* __restrict__ stack, const uint32_t* __restrict__ spill) {
+  GE_DEV void next(const Bvh& bvh, uint32_t* __restrict__ stack, const uint32_t* __restrict__ spill) {
     for (;;) {
       if (sp == 0) {
         state = k_walk_done;
@@ -602,7 +580,7 @@ struct Walker {
     inst_id = bvh.tlas_slots[cur & ~GE_BVH_LEAF];
     const InstRec rec = bvh.inst[inst_id];
     if ((rec.mask & ray_mask) == 0u) {
-      next(stack, spill);
+      next(bvh, stack, spill);
       return;
     }
     node_base = rec.node_base;
@@ -623,42 +601,50 @@ struct Walker {
   }
 
   // kNode: the four boxes of node `cur` against the distance at entry (one pass of box tests for
-  // the whole wave, branch-free, so that the node's seven 16-byte loads stay whole). A mesh node
+  // the whole wave, branch-free, so that the node's four 16-byte loads stay whole). A mesh node
   // with leaves among its passing children goes to kLeaf; any other node goes on to its children.
   GE_DEV void visit(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, Tally& tally) {
     if (Count) ++tally.nodes[AnyHit];
-    const char* np = reinterpret_cast<const char*>(in_blas ? bvh.wnodes + node_base + cur : bvh.wtlas + cur);
-    // Each axis's near and far planes of the four boxes, loaded by the ray's direction signs: the
-    // slab test needs no min or max per axis (with lo <= hi, the near plane's distance is the
-    // smaller one: exactly what the min of the two would choose).
-    const float4 ax = *reinterpret_cast<const float4*>(np + (ray.near & 0xffu));
-    const float4 ay = *reinterpret_cast<const float4*>(np + ((ray.near >> 8) & 0xffu));
-    const float4 az = *reinterpret_cast<const float4*>(np + (ray.near >> 16));
-    const float4 bx = *reinterpret_cast<const float4*>(np + (ray.far & 0xffu));
-    const float4 by = *reinterpret_cast<const float4*>(np + ((ray.far >> 8) & 0xffu));
-    const float4 bz = *reinterpret_cast<const float4*>(np + (ray.far >> 16));
-    const uint4 refs = *reinterpret_cast<const uint4*>(np + 96);
-    const float nearp[4][3] = {{ax.x, ay.x, az.x}, {ax.y, ay.y, az.y}, {ax.z, ay.z, az.z}, {ax.w, ay.w, az.w}};
-    const float farp[4][3] = {{bx.x, by.x, bz.x}, {bx.y, by.y, bz.y}, {bx.z, by.z, bz.z}, {bx.w, by.w, bz.w}};
+    const uint4* np = reinterpret_cast<const uint4*>(in_blas ? bvh.wnodes + node_base + cur : bvh.wtlas + cur);
+    const uint4 w0 = np[0], refs = np[1], w2 = np[2], w3 = np[3];
+    const uint32_t lo[3] = {w2.x, w2.y, w2.z}, hi[3] = {w2.w, w3.x, w3.y};
+    const float origin[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+    // Per axis: the grid step times the reciprocal (exact: a power of two), the origin's distance,
+    // and the near and far plane words by the direction's sign (bit masks: v_bfi_b32).
+    float A[3], B[3];
+    uint32_t qn[3], qf[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      A[a] = ray.inv[a] * __uint_as_float(((w0.w >> (8 * a)) & 0xffu) << 23);
+      B[a] = (origin[a] - ray.o[a]) * ray.inv[a];
+      const uint32_t flip = uint32_t(int32_t(__float_as_uint(ray.inv[a])) >> 31);
+      qn[a] = (hi[a] & flip) | (lo[a] & ~flip);
+      qf[a] = (lo[a] & flip) | (hi[a] & ~flip);
+    }
     nref[0] = refs.x;
     nref[1] = refs.y;
     nref[2] = refs.z;
     nref[3] = refs.w;
-    uint32_t lm = 0, hm = 0;  // mesh leaves to test; inner children to go on to
+    uint32_t in = 0;  // the slots whose boxes pass
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const bool in = slab_ordered(nearp[c], farp[c], ray, hit.t, ntn[c]);  // an empty slot's box never passes
-      const bool leaf = in_blas && (nref[c] & GE_BVH_LEAF) != 0u;
-      lm |= uint32_t(in && leaf) << c;
-      hm |= uint32_t(in && !leaf) << c;
+      float nt[3], ft[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        nt[a] = __builtin_fmaf(plane_byte(qn[a], c), A[a], B[a]);
+        ft[a] = __builtin_fmaf(plane_byte(qf[a], c), A[a], B[a]);
+      }
+      in |= uint32_t(slab_q(nt, ft, ray, hit.t, ntn[c])) << c;
     }
+    in &= w0.w >> 24;                                      // an empty slot's planes are not a box
+    const uint32_t lm = in & (in_blas ? w0.w >> 28 : 0u);  // a mesh's leaves, to test
     leaf_mask = lm;
-    hit_mask = hm;
+    hit_mask = in & ~lm;                                   // inner children (and instances) to go on to
     if (lm) {
       state = k_walk_leaf;
       return;
     }
-    children(stack, spill, int(bvh.stack_cap));
+    children<false>(bvh, stack, spill);
   }
 
   // kLeaf: the kept leaves in child order, one triangle test a trip for every lane that has one
@@ -709,13 +695,18 @@ struct Walker {
       }
     }
     leaf_mask = 0;
-    children(stack, spill, int(bvh.stack_cap));
+    children<true>(bvh, stack, spill);
   }
 
   // The inner children of the node just tested that are still within the distance: the nearest
   // next (ties: the first), the others pushed last child first, so that they come back in child
   // order; none: the next node off the stack. Selects, not branches.
-  GE_DEV void children(uint32_t* __restrict__ stack, uint32_t* __restrict__ spill, int cap) {
+  // `Refilter`: the leaves have been tested since the boxes were, so the distance may have shrunk;
+  // straight after the box tests every inner child that passed is within it (its entry is at most
+  // t + teps), so the re-filter would keep them all.
+  template <bool Refilter>
+  GE_DEV void children(const Bvh& bvh, uint32_t* __restrict__ stack, uint32_t* __restrict__ spill) {
+    const int cap = int(bvh.stack_cap);
     if (hit_mask) {
       const float reach = hit.t + ray.teps;
       uint32_t keep = 0;
@@ -723,7 +714,7 @@ struct Walker {
       float best_t = 0.0f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const bool in = ((hit_mask >> c) & 1u) != 0u && ntn[c] <= reach;
+        const bool in = ((hit_mask >> c) & 1u) != 0u && (!Refilter || ntn[c] <= reach);
         const bool nearer = in && (keep == 0u || ntn[c] < best_t);
         keep |= uint32_t(in) << c;
         best = nearer ? c : best;
@@ -748,7 +739,7 @@ struct Walker {
         return;
       }
     }
-    next(stack, spill);
+    next(bvh, stack, spill);
   }
 
   // Everything up to the next node test, in order; true when the walk has ended.

@@ -57,6 +57,7 @@ def lib():
             "ge_rt_scene_update_instances": ([vp, vp, u32, u32, vp], i32),
             "ge_rt_bvh_info": ([vp, vp, u32, C.POINTER(u32), C.POINTER(u32)], i32),
             "ge_rt_bvh_download": ([vp, vp, u32, vp, vp, vp, vp], i32),
+            "ge_rt_bvh_download_wide": ([vp, vp, u32, vp], i32),
             "ge_rt_trace_closest": ([vp, vp, vp, u32, u32, vp, u32], i32),
             "ge_rt_trace_any": ([vp, vp, vp, u32, u32, vp, u32], i32),
             "ge_rt_ray_visibility": ([vp, vp, C.POINTER(abi.RayVisibilityParams), vp, u32], i32),
@@ -98,7 +99,7 @@ def lib():
 
 EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchronize", "ge_rt_scene_upload",
            "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_bvh_info",
-           "ge_rt_bvh_download", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
+           "ge_rt_bvh_download", "ge_rt_bvh_download_wide", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
            "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve",
            "ge_rt_shadow_resolve_noalloc", "ge_rt_stats",
            "ge_rt_reset_stats", "ge_rt_scene_update_materials", "ge_rt_scene_update_lights", "ge_rt_path_adjoint",
@@ -280,6 +281,14 @@ class Scene:
             prims = np.zeros(p.value, dtype=abi.TRI_DTYPE)
             self.ctx.check(lib().ge_rt_bvh_download(self.ctx.h, self.h, mesh, _ptr(nodes), _ptr(prims), None, _ptr(morton)))
         return nodes, prims, morton
+
+    def bvh_wide(self, mesh):
+        """The 4-wide nodes of mesh `mesh` (~0: the top level) at the binary nodes' indices."""
+        n, p = C.c_uint32(), C.c_uint32()
+        self.ctx.check(lib().ge_rt_bvh_info(self.ctx.h, self.h, mesh, C.byref(n), C.byref(p)))
+        wide = np.zeros(n.value, dtype=abi.WIDE_DTYPE)
+        self.ctx.check(lib().ge_rt_bvh_download_wide(self.ctx.h, self.h, mesh, _ptr(wide)))
+        return wide
 
     def trace_closest(self, rays, mask=0xFF, count=False, hits=None, device=False, n=None, flags=0):
         """`flags` may add abi.ALPHA_MASKS or abi.CULL_MASKED (the masked-candidate rule)."""

@@ -405,6 +405,25 @@ typedef struct ge_bvh_node {
 #define GE_BVH_LEAF 0x80000000u
 #define GE_BVH_NONE 0xffffffffu
 
+/* A node of the structures the walks take: two binary levels collapsed into one 4-wide node (an
+ * even-depth binary node's children's children, a leaf child taken as it is), 64 bytes. Its child
+ * boxes sit on a per-axis grid of 8-bit plane indices: plane q of axis a lies at
+ * origin[a] + q 2^(e_a - 127), e_a = (meta >> 8a) & 0xff; byte c of lo[a] (hi[a]) is slot c's low
+ * (high) plane, its box's plane rounded outward to the grid (one step further where that left less
+ * than 2^-10 of a step). Bit 24 + c of meta is set when slot c holds a child, bit 28 + c when that
+ * child is a leaf (a triangle, or an instance in the top level). child[c]: bit 31 set a
+ * leaf (its primitive slot), GE_BVH_NONE an empty slot, else the binary index of an even-depth node,
+ * whose wide node sits at that same index. No counterpart in the engine: its acceleration
+ * structures are the Vulkan driver's and opaque (acceleration.h:46-110). */
+typedef struct ge_bvh_wide_node {
+  float origin[3];
+  uint32_t meta;
+  uint32_t child[4];
+  uint32_t lo[3];
+  uint32_t hi[3];
+  uint32_t reserved[2];
+} ge_bvh_wide_node;
+
 typedef struct ge_bvh_triangle {
   float v0[3];
   float v1[3];
@@ -506,6 +525,10 @@ int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, 
 int32_t ge_rt_bvh_download(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh,
                            ge_bvh_node* nodes, ge_bvh_triangle* triangles, uint32_t* instance_slots,
                            uint32_t* morton);
+/* Copies out the 4-wide nodes of mesh `mesh` (~0: the top level): node_count entries (ge_rt_bvh_info)
+ * at the binary nodes' indices, of which those at even depth (the root's is 0) are the walks' nodes
+ * and the others unspecified. */
+int32_t ge_rt_bvh_download_wide(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, ge_bvh_wide_node* wide);
 
 /* Closest-hit queries (RayQuery<RAY_FLAG_NONE> committed hit, path_trace.slang:268-298; the tie
  * between hits at one t goes to the larger pair << 8 | triangle, ray_visibility.slang:13-30).
@@ -734,6 +757,7 @@ static_assert(sizeof(ge_rt_counters) == 120, "counters record");
 static_assert(sizeof(ge_ray) == 32, "ge_ray is 32 bytes");
 static_assert(sizeof(ge_hit) == 32, "ge_hit is 32 bytes");
 static_assert(sizeof(ge_bvh_node) == 64, "a BVH node is one 64-byte fetch");
+static_assert(sizeof(ge_bvh_wide_node) == 64, "a wide node is one 64-byte line");
 static_assert(sizeof(ge_bvh_triangle) == 48, "a BVH triangle is 48 bytes");
 static_assert(sizeof(ge_shadow_receiver) == 32, "a shadow receiver is 32 bytes");
 static_assert(sizeof(ge_shadow_resolve_params) == 128, "shadow resolve params are 128 bytes");

@@ -66,6 +66,7 @@ def lib():
         L.ge_oracle_scene_destroy.argtypes = [vp]
         L.ge_oracle_bvh_info.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.ge_oracle_bvh_download.argtypes = [vp, C.c_uint32, vp, vp, vp, vp]
+        L.ge_oracle_bvh_download_wide.argtypes = [vp, C.c_uint32, vp]
         L.ge_oracle_update_instances.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.ge_oracle_refit_instances.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.ge_oracle_trace_closest.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32, C.c_uint32]
@@ -155,6 +156,15 @@ class Scene:
             lib().ge_oracle_bvh_download(self.h, mesh, nodes.ctypes.data, prims.ctypes.data, None,
                                          morton.ctypes.data)
         return nodes, prims, morton
+
+    def bvh_wide(self, mesh):
+        """The 4-wide nodes of mesh `mesh` (~0: the top level) at the binary nodes' indices, zero at
+        odd depth."""
+        n, p = C.c_uint32(), C.c_uint32()
+        assert lib().ge_oracle_bvh_info(self.h, mesh, C.byref(n), C.byref(p)) == 0
+        wide = np.zeros(n.value, dtype=abi.WIDE_DTYPE)
+        assert lib().ge_oracle_bvh_download_wide(self.h, mesh, wide.ctypes.data) == 0
+        return wide
 
     def update_instances(self, first, records, eye=None, refit=False):
         """Replaces instance records and rebuilds the instance structure (or refits it: its topology

@@ -332,13 +332,19 @@ struct InstRec {
 // 289-298) only the traced ones.
 inline uint32_t instance_ray_mask(uint32_t flags) { return (flags & 64u) != 0u ? 0xfeu : 0xffu; }
 
-// A 4-wide node: four child boxes and references (bit 31: a leaf slot; GE_BVH_NONE: empty).
+// A 4-wide node: four references (bit 31: a leaf slot; GE_BVH_NONE: empty) and their boxes on a
+// per-axis grid of 8-bit indices: plane q of axis a at origin[a] + q 2^(exp[a] - 127). A child's
+// low plane is its box's plane rounded down to the grid, its high plane rounded up, each one step
+// further when the rounding left less than 2^-10 of a step. 64 bytes.
 struct Node4 {
-  float lo[4][3], hi[4][3];
+  float origin[3];
+  uint8_t exp[3];
+  uint8_t slots;  // bit c: slot c holds a child; bit 4 + c: that child is a leaf
   uint32_t child[4];
-  uint32_t pad[4];
+  uint8_t lo[3][4], hi[3][4];  // [axis][slot]
+  uint8_t pad[8];
 };
-static_assert(sizeof(Node4) == 128, "a wide node is two 64-byte lines");
+static_assert(sizeof(Node4) == 64, "a wide node is one 64-byte line");
 using Wide = std::vector<Node4>;
 struct Lbvh;
 void collapse(const Lbvh& b, Wide& w);
@@ -991,26 +997,122 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
 //
 // Two binary levels a wide node: the internal nodes at even depth, each with its children's
 // children (a leaf child taken as it is), numbered in the order of their binary index.
-void collapse_levels(const Lbvh& b, Wide& w) {
+// The grid of one axis: the smallest exponent e >= -100 at which the children's lowest and highest
+// planes L and H, over 2^e, are below 2^23 in magnitude and ceil(H / 2^e) - floor(L / 2^e) <= 253,
+// starting the search at the exponents of max(|L|, |H|) less 22 and of H - L less 7; k0 = floor(L /
+// 2^e). Each operation is exact (a power-of-two scaling, floor, ceiling, differences of integers
+// below 2^24), so the grid is a function of the planes alone.
+int float_exponent(float x) {
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  return int((b >> 23) & 0xffu) - 127;
+}
+float pow2f(int e) {
+  const uint32_t b = uint32_t(e + 127) << 23;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+void axis_grid(float L, float H, int& e, float& k0) {
+  const float m = mx(std::fabs(L), std::fabs(H));
+  e = std::max(std::max(float_exponent(m) - 22, float_exponent(H - L) - 7), -100);
+  k0 = 0.0f;
+  for (; e < 126; ++e) {
+    const float down = pow2f(-e);
+    k0 = std::floor(L * down);
+    const float k1 = std::ceil(H * down);
+    if (k1 - k0 <= 253.0f && std::fabs(k0) < 8388608.0f && std::fabs(k1) < 8388608.0f) return;
+  }
+}
+
+// A plane x (in grid steps) rounded outward: down (up) to the grid, and one step further when that
+// is within 2^-10 of a step. x - floor(x) and ceil(x) - x are exact.
+float grid_down(float x) {
+  const float f = std::floor(x);
+  return x - f < 0.0009765625f ? f - 1.0f : f;
+}
+float grid_up(float x) {
+  const float c = std::ceil(x);
+  return c - x < 0.0009765625f ? c + 1.0f : c;
+}
+
+// The first n slots (references already in place) onto the node's grid: their occupied and leaf
+// bits, and per axis the origin at k0 - 1 steps, a child's low
+// index grid_down(lo / 2^e) - (k0 - 1) and high index grid_up(hi / 2^e) - (k0 - 1), within
+// [0, 255].
+void quantize(Node4& node, const float lo[4][3], const float hi[4][3], int n) {
+  node.slots = uint8_t((1u << n) - 1u);
+  for (int c = 0; c < n; ++c)
+    if (node.child[c] & GE_BVH_LEAF) node.slots = uint8_t(node.slots | (16u << c));
+  for (int a = 0; a < 3; ++a) {
+    float L = lo[0][a], H = hi[0][a];
+    for (int c = 1; c < n; ++c) {
+      L = mn(L, lo[c][a]);
+      H = mx(H, hi[c][a]);
+    }
+    int e;
+    float k0;
+    axis_grid(L, H, e, k0);
+    const float down = pow2f(-e);
+    node.exp[a] = uint8_t(e + 127);
+    const float base = k0 - 1.0f;
+    node.origin[a] = base * pow2f(e);
+    for (int c = 0; c < n; ++c) {
+      node.lo[a][c] = uint8_t(grid_down(lo[c][a] * down) - base);
+      node.hi[a][c] = uint8_t(grid_up(hi[c][a] * down) - base);
+    }
+  }
+}
+
+// slab() on slot c's quantized box: each plane's distance as fma(q, inv 2^(e - 127), (origin - o)
+// inv). Against the exact distance of the quantized plane that errs by at most 3 2^-24 of itself
+// plus 2^-15 of a grid step's length along the ray, which the at least 2^-10 of a step to spare on
+// each side and the far end's 1 + 2^-20 cover, so the test passes wherever slab() on the child's
+// own box does.
+bool qslab(const Node4& n, int c, const TRay& r, float tmax, float& tnear) {
+  float t0[3], t1[3];
+  for (int a = 0; a < 3; ++a) {
+    const float A = r.inv[a] * pow2f(int(n.exp[a]) - 127);
+    const float B = (n.origin[a] - r.o[a]) * r.inv[a];
+    t0[a] = std::fma(float(n.lo[a][c]), A, B);
+    t1[a] = std::fma(float(n.hi[a][c]), A, B);
+  }
+  const float tn = mx(mx(mn(t0[0], t1[0]), mn(t0[1], t1[1])), mx(mn(t0[2], t1[2]), r.tmin));
+  const float tf = mn(mn(mn(mx(t0[0], t1[0]), mx(t0[1], t1[1])), mx(t0[2], t1[2])) * k_box_robust, tmax);
+  tnear = tn;
+  return tn <= tf + r.teps;
+}
+
+// The wide node of each binary node at even depth, numbered in the order of the binary indices
+// (GE_BVH_NONE at odd depth); returns how many.
+uint32_t even_depth_index(const Lbvh& b, std::vector<uint32_t>& index) {
   const size_t n = b.nodes.size();
-  std::vector<uint32_t> depth(n, 0), index(n, GE_BVH_NONE);
+  std::vector<uint32_t> depth(n, 0);
+  index.assign(n, GE_BVH_NONE);
   for (size_t i = 0; i < n; ++i)  // parents precede children in no particular order: walk up
     for (uint32_t p = b.nodes[i].parent; p != 0xffffffffu; p = b.nodes[p].parent) ++depth[i];
   uint32_t count = 0;
   for (size_t i = 0; i < n; ++i)
     if ((depth[i] & 1u) == 0) index[i] = count++;
-  w.assign(count, Node4{});
+  return count;
+}
+
+void collapse_levels(const Lbvh& b, Wide& w) {
+  const size_t n = b.nodes.size();
+  std::vector<uint32_t> index;
+  w.assign(even_depth_index(b, index), Node4{});
   for (size_t i = 0; i < n; ++i) {
     if (index[i] == GE_BVH_NONE) continue;
     Node4& node = w[index[i]];
     int k = 0;
+    float lo[4][3], hi[4][3];
     const ge_bvh_node& a = b.nodes[i];
     for (int c = 0; c < 2; ++c) {
       if (a.child[c] == GE_BVH_NONE) continue;
       if (a.child[c] & GE_BVH_LEAF) {
         node.child[k] = a.child[c];
-        std::memcpy(node.lo[k], c == 0 ? a.lo0 : a.lo1, 12);
-        std::memcpy(node.hi[k], c == 0 ? a.hi0 : a.hi1, 12);
+        std::memcpy(lo[k], c == 0 ? a.lo0 : a.lo1, 12);
+        std::memcpy(hi[k], c == 0 ? a.hi0 : a.hi1, 12);
         ++k;
         continue;
       }
@@ -1018,15 +1120,13 @@ void collapse_levels(const Lbvh& b, Wide& w) {
       for (int d = 0; d < 2; ++d) {
         if (m.child[d] == GE_BVH_NONE) continue;
         node.child[k] = (m.child[d] & GE_BVH_LEAF) ? m.child[d] : index[m.child[d]];
-        std::memcpy(node.lo[k], d == 0 ? m.lo0 : m.lo1, 12);
-        std::memcpy(node.hi[k], d == 0 ? m.hi0 : m.hi1, 12);
+        std::memcpy(lo[k], d == 0 ? m.lo0 : m.lo1, 12);
+        std::memcpy(hi[k], d == 0 ? m.hi0 : m.hi1, 12);
         ++k;
       }
     }
-    for (; k < 4; ++k) {  // an empty slot: no reference, a box at +inf that no slab test passes
-      node.child[k] = GE_BVH_NONE;
-      for (int a = 0; a < 3; ++a) node.lo[k][a] = node.hi[k][a] = INFINITY;
-    }
+    quantize(node, lo, hi, k);
+    for (; k < 4; ++k) node.child[k] = GE_BVH_NONE;  // an empty slot: no reference, no occupied bit
   }
 }
 
@@ -1092,12 +1192,11 @@ bool traverse4(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t
     const Node4& node = in_blas ? blas->wide[cur] : s.tlas_wide[cur];
     float entry[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     bool passed[4] = {false, false, false, false};
-    for (int c = 0; c < 4; ++c)
-      passed[c] = node.child[c] != GE_BVH_NONE && slab(node.lo[c], node.hi[c], ray, hit.t, entry[c]);
+    for (int c = 0; c < 4; ++c) passed[c] = ((node.slots >> c) & 1u) != 0u && qslab(node, c, ray, hit.t, entry[c]);
     for (int c = 0; in_blas && c < 4; ++c) {
       if (!passed[c] || !(node.child[c] & GE_BVH_LEAF)) continue;
       float again = 0.0f;
-      if (!slab(node.lo[c], node.hi[c], ray, hit.t, again)) continue;
+      if (!qslab(node, c, ray, hit.t, again)) continue;
       const ge_bvh_triangle& tri = blas->tris[node.child[c] & ~GE_BVH_LEAF];
       ++tally.tris[AnyHit];
       float t, u, v;
@@ -1122,7 +1221,7 @@ bool traverse4(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t
     for (int c = 0; c < 4; ++c) {
       if (!passed[c] || (in_blas && (node.child[c] & GE_BVH_LEAF))) continue;
       float again = 0.0f;
-      if (!slab(node.lo[c], node.hi[c], ray, hit.t, again)) continue;
+      if (!qslab(node, c, ray, hit.t, again)) continue;
       tn[nn] = entry[c];
       next[nn++] = node.child[c];
     }
@@ -2274,6 +2373,27 @@ int32_t ge_oracle_bvh_download(const ge_oracle_scene* s, uint32_t mesh, ge_bvh_n
     if (slots) std::memcpy(slots, s->tlas_slots.data(), s->tlas_slots.size() * 4);
   } else if (tris) {
     std::memcpy(tris, s->blas[mesh].tris.data(), s->blas[mesh].tris.size() * sizeof(ge_bvh_triangle));
+  }
+  return GE_RT_OK;
+}
+
+int32_t ge_oracle_bvh_download_wide(const ge_oracle_scene* s, uint32_t mesh, ge_bvh_wide_node* out) {
+  static_assert(sizeof(Node4) == sizeof(ge_bvh_wide_node), "the wide node's layout is the backend's");
+  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (mesh < s->blas.size() ? &s->blas[mesh].bvh : nullptr);
+  if (!b || !out) return GE_RT_ERROR_INVALID_ARGUMENT;
+  const Wide& w = mesh == 0xffffffffu ? s->tlas_wide : s->blas[mesh].wide;
+  std::vector<uint32_t> index;
+  even_depth_index(*b, index);
+  std::vector<uint32_t> binary(w.size(), 0);
+  for (size_t i = 0; i < index.size(); ++i)
+    if (index[i] != GE_BVH_NONE) binary[index[i]] = uint32_t(i);
+  std::memset(out, 0, b->nodes.size() * sizeof(ge_bvh_wide_node));
+  for (size_t i = 0; i < index.size(); ++i) {
+    if (index[i] == GE_BVH_NONE) continue;
+    Node4 node = w[index[i]];
+    for (int c = 0; c < 4; ++c)  // inner references by binary index, as the backend lays them out
+      if (node.child[c] != GE_BVH_NONE && !(node.child[c] & GE_BVH_LEAF)) node.child[c] = binary[node.child[c]];
+    std::memcpy(&out[i], &node, sizeof(Node4));
   }
   return GE_RT_OK;
 }

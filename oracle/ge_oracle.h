@@ -39,6 +39,9 @@ int32_t ge_oracle_bvh_info(const ge_oracle_scene* scene, uint32_t mesh, uint32_t
 int32_t ge_oracle_bvh_download(const ge_oracle_scene* scene, uint32_t mesh, ge_bvh_node* nodes,
                                ge_bvh_triangle* triangles, uint32_t* instance_slots,
                                uint32_t* morton);
+/* The 4-wide nodes of mesh `mesh` (~0: the top level) at the binary nodes' indices (node_count
+ * entries), references by binary index as the backend lays them out; zero at odd depth. */
+int32_t ge_oracle_bvh_download_wide(const ge_oracle_scene* scene, uint32_t mesh, ge_bvh_wide_node* wide);
 /* Replaces instance records and rebuilds the top level (the per-frame placement). */
 int32_t ge_oracle_update_instances(ge_oracle_scene* scene, uint32_t first, uint32_t count,
                                    const ge_instance_desc* instances, const ge_frame_eye* eye);

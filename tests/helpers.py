@@ -103,3 +103,16 @@ def far_from_origin(s, pt, rv, shift=(150000, 3, -200000), residual=(7e-4, -3e-4
     m = np.array(pt.clip_to_ray, dtype=np.float64).reshape(4, 4).T
     rv2.view_proj[:] = [float(x) for x in np.linalg.inv(m).T.reshape(-1).astype(np.float32)]
     return d, eye, pt2, rv2, lights
+
+
+def even_depth(nodes):
+    """Indices of the binary nodes at even depth (the root's is 0): those that carry a wide node."""
+    import numpy as np
+    parent = nodes["parent"].astype(np.int64)
+    depth = np.zeros(len(nodes), dtype=np.int64)
+    for i in range(len(nodes)):
+        p = parent[i]
+        while p != 0xFFFFFFFF:
+            depth[i] += 1
+            p = parent[p]
+    return np.nonzero((depth & 1) == 0)[0]

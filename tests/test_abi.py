@@ -42,7 +42,8 @@ def test_record_sizes_match_the_engine():
         if struct in mirror:
             assert C.sizeof(mirror[struct]) == int(size), struct
         dtypes = {"ge_ray": abi.RAY_DTYPE, "ge_hit": abi.HIT_DTYPE, "ge_bvh_node": abi.NODE_DTYPE,
-                  "ge_bvh_triangle": abi.TRI_DTYPE, "ge_shadow_receiver": abi.RECEIVER_DTYPE}
+                  "ge_bvh_wide_node": abi.WIDE_DTYPE, "ge_bvh_triangle": abi.TRI_DTYPE,
+                  "ge_shadow_receiver": abi.RECEIVER_DTYPE}
         if struct in dtypes:
             assert dtypes[struct].itemsize == int(size), struct
 
@@ -92,6 +93,24 @@ def test_python_records_match_the_header_field_by_field(tmp_path):
         assert getattr(MIRRORS[struct], name).offset == int(off), (struct, name)
         seen += 1
     assert seen == sum(len(m._fields_) for m in MIRRORS.values())
+
+
+def test_record_dtypes_match_the_header_field_by_field(tmp_path):
+    """The numpy views of the walks' records against the C compiler's offsets of their fields."""
+    dtypes = {"ge_bvh_node": abi.NODE_DTYPE, "ge_bvh_wide_node": abi.WIDE_DTYPE, "ge_bvh_triangle": abi.TRI_DTYPE}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for struct, dt in dtypes.items():
+        for name in dt.names:
+            lines.append(f'  printf("{struct} {name} %zu\\n", offsetof({struct}, {name}));')
+    lines.append("  return 0;\n}")
+    src, exe = tmp_path / "offsets.c", tmp_path / "offsets"
+    src.write_text("\n".join(lines))
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        struct, name, off = line.split()
+        assert dtypes[struct].fields[name][1] == int(off), (struct, name)
+    assert len(list(filter(None, out))) == sum(len(d.names) for d in dtypes.values())
 
 
 def test_null_handles_fail_cleanly():

@@ -19,7 +19,7 @@ import pytest
 
 from game_engine_amd import abi, rt, synth
 from oracle import binding as ob
-from tests.helpers import rays_towards
+from tests.helpers import even_depth, rays_towards
 
 pytestmark = pytest.mark.gpu
 TOP = 0xFFFFFFFF
@@ -48,6 +48,8 @@ def test_path_trace_4m_triangles_1080p(ctx):
     gn, gp, gm = g.bvh(TOP)
     on, op, om = o.bvh(TOP)
     assert np.array_equal(gm, om) and gp.tobytes() == op.tobytes() and gn.tobytes() == on.tobytes()
+    even = even_depth(on)
+    assert g.bvh_wide(TOP)[even].tobytes() == o.bvh_wide(TOP)[even].tobytes(), "quantized wide nodes"
     lo, hi = s.bounds()
     rays = rays_towards(200000, lo, hi, lo, hi, 5)
     assert g.trace_closest(rays).tobytes() == o.trace_closest(rays).tobytes()

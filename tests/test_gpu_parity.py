@@ -12,7 +12,7 @@ import pytest
 
 from game_engine_amd import abi, rt, synth
 from oracle import binding as ob
-from tests.helpers import random_rays, rays_towards, unpack_bits
+from tests.helpers import even_depth, random_rays, rays_towards, unpack_bits
 
 pytestmark = pytest.mark.gpu
 TOP = 0xFFFFFFFF
@@ -53,6 +53,8 @@ def _bvh_equal(g, o, mesh):
     assert np.array_equal(gm, om), "Morton codes"
     assert gp.tobytes() == op.tobytes(), "sorted primitives"
     assert gn.tobytes() == on.tobytes(), "nodes"
+    even = even_depth(on)
+    assert g.bvh_wide(mesh)[even].tobytes() == o.bvh_wide(mesh)[even].tobytes(), "quantized wide nodes"
     return len(gn)
 
 

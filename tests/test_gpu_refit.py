@@ -12,7 +12,7 @@ import pytest
 
 from game_engine_amd import abi, rt, synth
 from oracle import binding as ob
-from tests.helpers import far_from_origin, rays_towards
+from tests.helpers import even_depth, far_from_origin, rays_towards
 from tests.test_oracle_refit import _moved, with_empty_mesh
 
 pytestmark = pytest.mark.gpu
@@ -30,6 +30,8 @@ def _same_top(g, o):
     gn, gp, gm = g.bvh(TOP)
     on, op, om = o.bvh(TOP)
     assert np.array_equal(gm, om) and gp.tobytes() == op.tobytes() and gn.tobytes() == on.tobytes()
+    even = even_depth(on)
+    assert g.bvh_wide(TOP)[even].tobytes() == o.bvh_wide(TOP)[even].tobytes(), "quantized wide nodes"
     return gn
 
 

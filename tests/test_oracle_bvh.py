@@ -248,8 +248,9 @@ def test_wide_walk_is_the_binary_walk_with_fewer_nodes():
     assert c4["nodes"] < 0.65 * c2["nodes"]
     # A node's four children are its children's children. A wide node tests its leaves in child
     # order where the binary walk goes to the nearer half first, so a far leaf can be tested before
-    # a near one has shortened the ray: a few more triangle tests, never more than 5% here.
-    assert c4["triangles"] <= c2["triangles"] * 1.05 and c4["instances"] <= c2["instances"] * 1.05
+    # a near one has shortened the ray, and its boxes are the binary boxes rounded outward to an
+    # 8-bit grid: a few more triangle tests and instance entries, never more than 8% here.
+    assert c4["triangles"] <= c2["triangles"] * 1.08 and c4["instances"] <= c2["instances"] * 1.08
 
 
 def test_watertight_on_shared_edges_and_vertices():
