@@ -11,9 +11,8 @@ Rays = closest-hit queries + any-hit queries issued, counted by the kernel's own
 separate counted replay of the same frames (counters are compiled out of the timed kernel).
 value = rays / wall seconds of the K timed frames. The BVH build is timed separately.
 
-roofline: the dominant kernel of the frame (the traversal walk with the larger share of device time:
-k_closest, the closest-hit walk of one bounce, or k_any, the any-hit shadow walk of one bounce). Its
-algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 64 B per 4-wide node visited (the
+roofline: the dominant kernel of the frame, k_walks (one launch a bounce: that bounce's closest-hit
+rays, then the previous bounce's any-hit shadow rays). Its algorithmic bytes per launch are 48 B a ray (32 in, 16 out) + 64 B per 4-wide node visited (the
 whole quantized node: four 16-byte loads) + 48 B per triangle tested + 64 B per instance
 entered, from the counted replay's per-kind counters divided by its launches per frame; its mean
 launch time is the library's own HIP events around each launch on its stream, over the timed
@@ -53,12 +52,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_profile(kernel, figure):
-    """A figure of `kernel` from the committed rocprofv3 PMC summary (lines `figure kernel: value`):
+def read_profile(kernel, figure, summary="path_trace_c2_pmc.md"):
+    """A figure of `kernel` from a committed rocprofv3 PMC summary (lines `figure kernel: value`):
     traffic_bytes_per_launch (calibrated FETCH_SIZE + WRITE_SIZE), valu_busy (issued VALU cycles over
     the SIMDs' cycles), valu_thread_util (SQ_THREAD_CYCLES_VALU over 64 x SQ_ACTIVE_INST_VALU),
     l2_hit (TCC hits over requests); None when the summary or the line is not there."""
-    path = os.path.join(ROOT, "profiles", "path_trace_c2_pmc.md")
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     key = f"{figure} {kernel}:"
@@ -124,6 +123,9 @@ def frame_parity(s, pt, sample_base, gpu_acc, gpu_cov, fp64_rows=(520, 556)):
 
 
 SHADOW_METRIC = "Mrays/s (any-hit sun shadow rays, whole frames with 10% of instances rebuilt)"
+
+
+SHADOW_PMC = "shadow_c3_pmc.md"
 
 
 def shadow_main(args):
@@ -242,8 +244,12 @@ def shadow_main(args):
                      "visibility": round(sum(vis_ms) / len(vis_ms), 3),
                      "shadow": round(sum(shadow_ms) / len(shadow_ms), 3), "full_build": round(full_build, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": "k_sun_walk",
-                     "launch_ms": round(launch_ms, 4), "bytes_per_ray": round(per_ray, 1)},
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                     "traffic": read_profile("k_sun_walk", "traffic_bytes_per_launch", SHADOW_PMC),
+                     "kernel": "k_sun_walk", "launch_ms": round(launch_ms, 4), "bytes_per_ray": round(per_ray, 1),
+                     "limiter": "valu issue", "valu_busy": read_profile("k_sun_walk", "valu_busy", SHADOW_PMC),
+                     "valu_thread_util": read_profile("k_sun_walk", "valu_thread_util", SHADOW_PMC),
+                     "l2_hit": read_profile("k_sun_walk", "l2_hit", SHADOW_PMC)},
     }
     print(json.dumps(result), flush=True)
 
@@ -686,7 +692,7 @@ def inverse_main(args):
         "loss_first": losses[0], "loss_last": losses[-1],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
-                     "kernel": "the adjoint's wavefront passes (k_closest, k_shade<3>, k_any, k_resolve<true>)",
+                     "kernel": "the adjoint's wavefront passes (k_walks, k_shade<3>, k_resolve<true>)",
                      "launch_ms": round(a_ms, 4), "bytes_per_launch": int(passes * bytes_per_frame)},
     }
     print(json.dumps(result), flush=True)
@@ -736,6 +742,14 @@ def main():
         print(f"bench.py measures one GPU: the path tracer runs on one MI355X and has no multi-GPU path "
               f"(--gpus {args.gpus} asked)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.workload == "shadow":
         shadow_main(args)
         return
@@ -754,13 +768,6 @@ def main():
 
     from game_engine_amd import rt, synth
 
-    if args.copy_calibration:
-        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda").fill_(1)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     log("[bench] synthesizing the 4M-triangle instanced scene")
     s = synth.Synth.config(2, 0, args.instances)
     pt, _ = s.frame(WIDTH, HEIGHT)
@@ -794,8 +801,7 @@ def main():
     wall = time.perf_counter() - t0
     kernel_ms = [x["kernel_ms"] for x in per_step]
     log(f"[bench] timed {args.steps} frames: {wall * 1e3:.1f} ms wall, frame ms {['%.2f' % x for x in kernel_ms]}, "
-        f"closest-hit walks ms {['%.2f' % x['closest_ms'] for x in per_step]}, "
-        f"any-hit walks ms {['%.2f' % x['any_ms'] for x in per_step]}")
+        f"walks ms {['%.2f' % x['closest_ms'] for x in per_step]}")
 
     ctx.reset_stats()
     for k in range(args.steps):
@@ -811,14 +817,12 @@ def main():
     log(f"[bench] counted: {st}")
     value = rays / wall / 1e6
     mean_ms = sum(kernel_ms) / len(kernel_ms)
-    # Algorithmic bytes of each walk kind over the timed frames (counts of the counted replay of the
-    # same frames), and their HIP-event times.
+    # Algorithmic bytes of the walks over the timed frames (counts of the counted replay of the same
+    # frames: a launch walks one bounce's closest-hit rays and the previous bounce's shadow rays), and
+    # their HIP-event times.
     kinds = {
-        "k_closest": (48 * st["closest"] + NODE_BYTES * (st["nodes"] - st["any_nodes"]) +
-                      48 * (st["triangles"] - st["any_triangles"]) + 64 * (st["instances"] - st["any_instances"]),
-                      sum(x["closest_ms"] for x in per_step), sum(x["closest_launches"] for x in per_step)),
-        "k_any": (48 * st["any"] + NODE_BYTES * st["any_nodes"] + 48 * st["any_triangles"] + 64 * st["any_instances"],
-                  sum(x["any_ms"] for x in per_step), sum(x["any_launches"] for x in per_step)),
+        "k_walks": (48 * rays + NODE_BYTES * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"],
+                    sum(x["closest_ms"] for x in per_step), sum(x["closest_launches"] for x in per_step)),
     }
     if args.megakernel:
         kinds = {"k_path_trace": (48 * rays + NODE_BYTES * st["nodes"] + 48 * st["triangles"] + 64 * st["instances"],

@@ -105,13 +105,21 @@ __global__ void __launch_bounds__(64) k_generate(ge_path_trace_params p, uint32_
   queue[slot] = pid;
 }
 
+// One bounce's closest-hit rays: path `queue[i]`'s ray from its state, its hit into hits[pid].
+struct ClosestJob {
+  const PathState* state;
+  const uint32_t* queue;
+  uint32_t n;
+  uint32_t* head;
+  PathHit* hits;
+  uint32_t alpha;  // the masked-candidate rule of the bounce
+};
+
 template <bool Count, bool Masks>
-__global__ void __launch_bounds__(64) GE_WALK_ATTR k_closest(Bvh bvh, const PathState* __restrict__ state,
-                                                const uint32_t* __restrict__ queue, uint32_t n,
-                                                uint32_t* head, PathHit* __restrict__ hits, unsigned long long* counters,
-                                                uint32_t alpha) {
-  Tally tally;
-  uint64_t issued = 0;
+__device__ void closest_walks(const Bvh& bvh, const ClosestJob& job, uint64_t& issued, Tally& tally) {
+  const PathState* __restrict__ state = job.state;
+  const uint32_t* __restrict__ queue = job.queue;
+  PathHit* __restrict__ hits = job.hits;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
     const uint32_t pid = queue[i];
     const PathState& st = state[pid];
@@ -130,8 +138,7 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_closest(Bvh bvh, const Path
     out.prim = h.prim;
     hits[pid] = out;
   };
-  persistent_walk<false, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
-  if (Count) flush_counters(counters, issued, 0, tally);
+  persistent_walk<false, Count, Masks>(bvh, job.n, job.head, lane_stack(), load, store, issued, tally, job.alpha);
 }
 
 __device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool primary) {
@@ -573,11 +580,19 @@ __global__ void __launch_bounds__(k_shade_threads) k_shade(ShadeArgs a) {
   }
 }
 
+// One bounce's shadow rays: record j's ray, its answer into blocked[j].
+struct AnyJob {
+  const ShadowRec* shadow;
+  uint32_t n;
+  uint32_t* head;
+  uint8_t* blocked;
+  uint32_t alpha;  // the masked-candidate rule of light rays
+};
+
 template <bool Count, bool Masks>
-__global__ void __launch_bounds__(64) GE_WALK_ATTR k_any(Bvh bvh, const ShadowRec* __restrict__ shadow, uint32_t n, uint32_t* head,
-                                            uint8_t* __restrict__ blocked, unsigned long long* counters, uint32_t alpha) {
-  Tally tally;
-  uint64_t issued = 0;
+__device__ void any_walks(const Bvh& bvh, const AnyJob& job, uint64_t& issued, Tally& tally) {
+  const ShadowRec* __restrict__ shadow = job.shadow;
+  uint8_t* __restrict__ blocked = job.blocked;
   auto load = [&](uint32_t j, uint32_t& slot, Ray& ray, float& tmax, uint32_t& mask) {
     const ShadowRec& r = shadow[j];
     slot = j;
@@ -588,8 +603,21 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_any(Bvh bvh, const ShadowRe
     return true;
   };
   auto store = [&](uint32_t j, const Hit& h) { blocked[j] = h.found ? 1 : 0; };
-  persistent_walk<true, Count, Masks>(bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
-  if (Count) flush_counters(counters, 0, issued, tally);
+  persistent_walk<true, Count, Masks>(bvh, job.n, job.head, lane_stack(), load, store, issued, tally, job.alpha);
+}
+
+// The walks of one launch: a bounce's closest-hit rays, then the previous bounce's shadow rays (the
+// two do not depend on each other: both were written by the previous shading launch). Each wave
+// takes closest-hit rays until their queue is drained and then shadow rays, so the waves that run
+// out of the first kind fill the other's tail instead of idling until the launch ends.
+template <bool Count, bool Masks>
+__global__ void __launch_bounds__(64) GE_WALK_ATTR k_walks(Bvh bvh, ClosestJob closest, AnyJob any,
+                                                          unsigned long long* counters) {
+  Tally tally;
+  uint64_t closest_issued = 0, any_issued = 0;
+  if (closest.n) closest_walks<Count, Masks>(bvh, closest, closest_issued, tally);
+  if (any.n) any_walks<Count, Masks>(bvh, any, any_issued, tally);
+  if (Count) flush_counters(counters, closest_issued, any_issued, tally);
 }
 
 // radiance + throughput * next_event(...): the unblocked lights summed in record order. A path that
@@ -782,26 +810,65 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   const bool full = sv.features != 0u;
   const bool masked = (sv.features & k_scene_masked) != 0u;
   const uint32_t any_alpha = masked && (p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : k_alpha_opaque;
-  for (uint32_t bounce = 0; n > 0; ++bounce) {
+  // A bounce's resolve waits for its shadow rays, which are walked in the same launch as the next
+  // bounce's closest-hit rays: walks(b + 1 closest, b shadow), resolve(b), shade(b + 1). Each path's
+  // radiance still takes its additions in the order of the single-bounce loop (bounce b's lights
+  // before bounce b + 1's emission or environment).
+  struct Pending {
+    bool live = false;        // a shaded bounce whose resolve has not run
+    uint32_t n = 0;           // its paths
+    uint32_t shadows = 0;     // its shadow records
+    PathState* state = nullptr;
+    const uint32_t* queue = nullptr;
+  } pending;
+  for (uint32_t bounce = 0; n > 0 || pending.live; ++bounce) {
     const uint32_t alpha = !masked ? k_alpha_opaque
                                    : ((p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : (bounce == 0 ? k_alpha_cull : k_alpha_opaque));
     if ((e = hipMemsetAsync(c2, 0, 16, st)) != hipSuccess) return e;  // queue and shadow counts, walk heads
-    const uint32_t grid = std::min(blocks(n, 64), ctx->persistent_blocks);
-    if ((e = stage_mark(ctx, 0, 0)) != hipSuccess) return e;
-    {
-      const PathState* ps = w.state[cur].as<PathState>();
-      const uint32_t* q = w.queue[0].as<uint32_t>();
-      PathHit* ph = w.hits.as<PathHit>();
-      if (alpha != k_alpha_opaque) {
-        if (counters) k_closest<true, true><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, alpha);
-        else k_closest<false, true><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, alpha);
+    const uint64_t walk_rays = uint64_t(n) + pending.shadows;
+    if (walk_rays > 0) {
+      ClosestJob cj{w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n, counts + 4, w.hits.as<PathHit>(), alpha};
+      AnyJob aj{w.shadow.as<ShadowRec>(), pending.shadows, counts + 5, w.blocked.as<uint8_t>(), any_alpha};
+      const bool mask_walk = (n > 0 && alpha != k_alpha_opaque) || (pending.shadows > 0 && any_alpha != k_alpha_opaque);
+      const uint32_t grid = std::min(blocks(walk_rays, 64), ctx->persistent_blocks);
+      if ((e = stage_mark(ctx, 0, 0)) != hipSuccess) return e;
+      if (mask_walk) {
+        if (counters) k_walks<true, true><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
+        else k_walks<false, true><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
       } else {
-        if (counters) k_closest<true, false><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, 0u);
-        else k_closest<false, false><<<grid, 64, k_lds, st>>>(bvh, ps, q, n, counts + 4, ph, counters, 0u);
+        if (counters) k_walks<true, false><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
+        else k_walks<false, false><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
       }
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = stage_mark(ctx, 0, 1)) != hipSuccess) return e;
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = stage_mark(ctx, 0, 1)) != hipSuccess) return e;
+    if (pending.live) {
+      if (grad) {
+        ResolveAdjoint ra;
+        ra.sgrad = w.sgrad.as<ShadowGrad>();
+        ra.adj_a = w.adj_a.as<float4>();
+        ra.adj_r = w.adj_r.as<float4>();
+        ra.lpath = adjoint->lpath;
+        ra.d_image = adjoint->d_image;
+        ra.pixels = pixels;
+        ra.spp = float(p.spp);
+        ra.lights = sv.light_count + 1;
+        ra.grad_materials = adjoint->grad_materials;
+        ra.grad_lights = adjoint->grad_lights;
+        k_resolve<true><<<blocks(pending.n, 64), 64, size_t(ra.lights) * 8, st>>>(
+            pending.state, pending.queue, pending.n, w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
+            w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths, ra);
+      } else {
+        k_resolve<false><<<blocks(pending.n, 64), 64, 0, st>>>(pending.state, pending.queue, pending.n,
+                                                              w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
+                                                              w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths,
+                                                              ResolveAdjoint{});
+      }
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      pending.live = false;
+      pending.shadows = 0;
+    }
+    if (n == 0) break;
     ShadeArgs a{};
     a.sv = sv;
     a.p = p;
@@ -881,43 +948,11 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       if ((e = hipMemcpyAsync(counts + 6, w.host_counts + 6, 8, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
       a.rerun = true;
     }
-    if (nshadow > 0) {
-      const uint32_t grid = std::min(blocks(nshadow, 64), ctx->persistent_blocks);
-      if ((e = stage_mark(ctx, 1, 0)) != hipSuccess) return e;
-      const ShadowRec* sr = w.shadow.as<ShadowRec>();
-      uint8_t* bl = w.blocked.as<uint8_t>();
-      if (any_alpha != k_alpha_opaque) {
-        if (counters) k_any<true, true><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, any_alpha);
-        else k_any<false, true><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, any_alpha);
-      } else {
-        if (counters) k_any<true, false><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, 0u);
-        else k_any<false, false><<<grid, 64, k_lds, st>>>(bvh, sr, nshadow, counts + 5, bl, counters, 0u);
-      }
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-      if ((e = stage_mark(ctx, 1, 1)) != hipSuccess) return e;
-    }
-    if (grad) {
-      ResolveAdjoint ra;
-      ra.sgrad = w.sgrad.as<ShadowGrad>();
-      ra.adj_a = w.adj_a.as<float4>();
-      ra.adj_r = w.adj_r.as<float4>();
-      ra.lpath = adjoint->lpath;
-      ra.d_image = adjoint->d_image;
-      ra.pixels = pixels;
-      ra.spp = float(p.spp);
-      ra.lights = sv.light_count + 1;
-      ra.grad_materials = adjoint->grad_materials;
-      ra.grad_lights = adjoint->grad_lights;
-      k_resolve<true><<<blocks(n, 64), 64, size_t(ra.lights) * 8, st>>>(
-          w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n, w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
-          w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths, ra);
-    } else {
-      k_resolve<false><<<blocks(n, 64), 64, 0, st>>>(w.state[cur ^ 1].as<PathState>(), w.queue[0].as<uint32_t>(), n,
-                                                    w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
-                                                    w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths,
-                                                    ResolveAdjoint{});
-    }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    pending.live = true;
+    pending.n = n;
+    pending.shadows = nshadow;
+    pending.state = w.state[cur ^ 1].as<PathState>();
+    pending.queue = w.queue[0].as<uint32_t>();  // swapped to [1] below; the next shade writes it after the resolve
     cur ^= 1;
     std::swap(w.queue[0], w.queue[1]);
     n = next;

@@ -377,10 +377,11 @@ typedef struct ge_rt_counters {
   uint64_t any_nodes;      /* the part of nodes, triangles and instances that any-hit queries made */
   uint64_t any_triangles;
   uint64_t any_instances;
-  /* The last dispatch's traversal launches, timed with hipEvents on the ctx stream: the wavefront
-   * path tracer's closest-hit walks (one a bounce) and any-hit shadow walks, or the shadow
-   * resolve's sun walk (an any-hit launch), summed, and how many launches the sums cover (the first
-   * GE_RT_STAGE_TIMED of each kind are timed). */
+  /* The last dispatch's traversal launches, timed with hipEvents on the ctx stream, summed, and how
+   * many launches the sums cover (the first GE_RT_STAGE_TIMED of each kind are timed): in closest_*
+   * the wavefront path tracer's walk launches (one a bounce: its closest-hit rays together with the
+   * previous bounce's shadow rays, the last one shadow rays alone); in any_* the shadow resolve's
+   * sun walk. */
   double closest_ms;
   double any_ms;
   uint32_t closest_launches;
