@@ -161,6 +161,7 @@ CULL_MASKED = 0x20
 BVH_LEAF = 0x80000000
 BVH_NONE = 0xFFFFFFFF
 MISS = 0xFFFFFFFF
+FRAME_STRUCTURE = 0x80000000  # bvh_info / download selector: the frame structure of instance (sel & ~bit)
 PT_UNIFORM_SKY = 1
 PT_PIXEL_CENTER = 2
 PT_DITHER = 4

@@ -282,10 +282,6 @@ int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_
     if (inst.mesh >= s.mesh_count)
       return set_error(ctx, GE_RT_ERROR_INVALID_SCENE, "instance " + std::to_string(i) + " names mesh " +
                                                            std::to_string(inst.mesh) + " of " + std::to_string(s.mesh_count));
-    if (!(inst.flags & GE_INSTANCE_STATIC_RT))
-      return set_error(ctx, GE_RT_ERROR_UNSUPPORTED,
-                       "instance " + std::to_string(i) + " is not a static structure (InstanceDesc::flags bit 7); "
-                       "the backend traces every instance through its mesh's own structure");
     if (inst.deform != GE_INVALID_DEFORM)
       return set_error(ctx, GE_RT_ERROR_UNSUPPORTED, "instance " + std::to_string(i) + " is deformed");
     if (s.h_meshes[inst.mesh].cluster_count &&
@@ -294,6 +290,102 @@ int32_t validate_instances(ge_rt_ctx* ctx, const SceneDev& s, const ge_instance_
                        "instance " + std::to_string(i) + " reaches past the material array");
   }
   return GE_RT_OK;
+}
+
+
+// The visible list's entries against the scene's instances: each {instance, cluster} names an
+// instance in range and a cluster of its mesh, and none repeats.
+int32_t check_visible(ge_rt_ctx* ctx, const SceneDev& s, const uint32_t* e, uint32_t count, int32_t code) {
+  std::vector<uint64_t> keys(count);
+  for (uint32_t j = 0; j < count; ++j) {
+    const uint32_t i = e[2 * j], c = e[2 * j + 1];
+    if (i >= s.instance_count)
+      return set_error(ctx, code, "visible entry " + std::to_string(j) + " names instance " + std::to_string(i) + " of " +
+                                      std::to_string(s.instance_count));
+    const ge_mesh_desc& mesh = s.h_meshes[s.h_instances[i].mesh];
+    if (c < mesh.first_cluster || c - mesh.first_cluster >= mesh.cluster_count)
+      return set_error(ctx, code, "visible entry " + std::to_string(j) + " names cluster " + std::to_string(c) +
+                                      ", not one of instance " + std::to_string(i) + "'s mesh");
+    keys[j] = (uint64_t(i) << 32) | c;
+  }
+  std::sort(keys.begin(), keys.end());
+  const auto rep = std::adjacent_find(keys.begin(), keys.end());
+  if (rep != keys.end())
+    return set_error(ctx, code, "the visible list names instance " + std::to_string(*rep >> 32) + "'s cluster " +
+                                    std::to_string(uint32_t(*rep)) + " twice");
+  return GE_RT_OK;
+}
+
+// The frame structures' regions, laid out again when the instances changed: one for each instance
+// that is not static, as large as its mesh's own structure, after the meshes' regions in the pools;
+// its root slot mesh_count + k; its clusters' visible entries from geom_base in entry_of.
+int32_t layout_frames(ge_rt_ctx* ctx, SceneDev& s, hipStream_t st) {
+  const uint32_t ni = s.instance_count;
+  s.frame_of.assign(ni, 0xffffffffu);
+  s.h_geom_base.assign(std::max(ni, 1u), 0xffffffffu);
+  s.frame_inst.clear();
+  s.frame_tri_base.clear();
+  s.frame_node_base.clear();
+  std::vector<uint32_t> root_of(std::max(ni, 1u), 0u);
+  uint64_t tri = s.tri_pool, node = s.node_pool, geom = 0;
+  for (uint32_t i = 0; i < ni; ++i) {
+    const ge_instance_desc& in = s.h_instances[i];
+    root_of[i] = in.mesh;
+    if (in.flags & GE_INSTANCE_STATIC_RT) continue;
+    const uint32_t k = uint32_t(s.frame_inst.size());
+    s.frame_of[i] = k;
+    s.frame_inst.push_back(i);
+    s.frame_tri_base.push_back(uint32_t(tri));
+    s.frame_node_base.push_back(uint32_t(node));
+    s.h_geom_base[i] = uint32_t(geom);
+    root_of[i] = s.mesh_count + k;
+    tri += s.mesh_tri_count[in.mesh];
+    node += s.mesh_node_count[in.mesh];
+    geom += s.h_meshes[in.mesh].cluster_count;
+    GE_CHECK(ctx, tri < (1ull << 31) && node < (1ull << 31) && geom < (1ull << 31), GE_RT_ERROR_INVALID_SCENE,
+             "the frame's structures would hold more than 2^31 triangles");
+  }
+  s.frame_tri_pool = uint32_t(tri - s.tri_pool);
+  s.frame_node_pool = uint32_t(node - s.node_pool);
+  s.h_entry_of.assign(size_t(std::max<uint64_t>(geom, 1)), 0xffffffffu);
+  const size_t nf = s.frame_inst.size();
+  s.frame_clusters.assign(nf, {});
+  s.frame_tri_count.assign(nf, 0u);
+  s.frame_node_count.assign(nf, 0u);
+  GE_HIP(ctx, upload(s.root_of, root_of.data(), root_of.size(), st));
+  GE_HIP(ctx, upload(s.geom_base, s.h_geom_base.data(), s.h_geom_base.size(), st));
+  s.frame_layout_dirty = false;
+  return GE_RT_OK;
+}
+
+// Each frame structure's clusters from the visible list, in list order, and each cluster's entry.
+int32_t resolve_visible(ge_rt_ctx* ctx, SceneDev& s, hipStream_t st) {
+  const uint32_t n = uint32_t(s.h_visible.size() / 2);
+  const int32_t r = check_visible(ctx, s, s.h_visible.data(), n, GE_RT_ERROR_INVALID_SCENE);  // the instances may have changed since
+  if (r != GE_RT_OK) return r;
+  for (auto& v : s.frame_clusters) v.clear();
+  std::fill(s.h_entry_of.begin(), s.h_entry_of.end(), 0xffffffffu);
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t i = s.h_visible[2 * j], c = s.h_visible[2 * j + 1];
+    const uint32_t k = s.frame_of[i];
+    if (k == 0xffffffffu) continue;  // a static instance traces its mesh's structure
+    s.h_entry_of[s.h_geom_base[i] + (c - s.h_meshes[s.h_instances[i].mesh].first_cluster)] = j;
+    s.frame_clusters[k].push_back(c);
+  }
+  for (size_t k = 0; k < s.frame_clusters.size(); ++k) {
+    uint64_t t = 0;
+    for (uint32_t c : s.frame_clusters[k]) t += s.h_clusters[c].triangle_count;
+    s.frame_tri_count[k] = uint32_t(t);
+    s.frame_node_count[k] = t == 0 ? 0u : (t == 1 ? 1u : uint32_t(t - 1));
+  }
+  GE_HIP(ctx, upload(s.entry_of, s.h_entry_of.data(), s.h_entry_of.size(), st));
+  return GE_RT_OK;
+}
+
+// Whether instance i has geometry to trace: its mesh's triangles, or its frame structure's.
+bool instance_live(const SceneDev& s, uint32_t i) {
+  const uint32_t k = s.frame_of[i];
+  return k == 0xffffffffu ? s.mesh_tri_count[s.h_instances[i].mesh] > 0 : s.frame_tri_count[k] > 0;
 }
 
 }  // namespace
@@ -568,17 +660,19 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   if (st != GE_RT_OK) return st;
   // A moved instance changes only its record; one that changes mesh or pair range changes the pair
   // table and the list of instances with geometry, and one that changes materials the mask table.
-  bool topology = false, materials = false;
+  bool topology = false, materials = false, frames = false;
   for (uint32_t k = 0; k < count; ++k) {
     const ge_instance_desc& was = s.h_instances[first + k];
     topology = topology || was.mesh != instances[k].mesh || was.first_pair != instances[k].first_pair;
     materials = materials || was.mesh != instances[k].mesh || was.material_base != instances[k].material_base;
+    frames = frames || was.mesh != instances[k].mesh || ((was.flags ^ instances[k].flags) & GE_INSTANCE_STATIC_RT);
   }
   std::memcpy(&s.h_instances[first], instances, size_t(count) * sizeof(ge_instance_desc));
   if (topology) {
     s.pairs_dirty = true;
     s.live_dirty = true;
   }
+  if (frames) s.frame_layout_dirty = true;  // an instance changed mesh or static form: regions laid out again
   if (count) {
     s.rec_lo = std::min(s.rec_lo, first);
     s.rec_hi = std::max(s.rec_hi, first + count);
@@ -587,6 +681,51 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
   GE_HIP(ctx, hipMemcpyAsync(s.instances.as<ge_instance_desc>() + first, &s.h_instances[first],
                              size_t(count) * sizeof(ge_instance_desc), hipMemcpyHostToDevice, ctx->stream));
   if (materials) GE_HIP(ctx, refresh_masks(s, ctx->stream));
+  return GE_RT_OK;
+} catch (...) {
+  return host_failure(ctx);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The pool region a download names: mesh `sel`'s structure, or with GE_RT_FRAME_STRUCTURE the frame
+// structure of instance sel & ~GE_RT_FRAME_STRUCTURE; false when there is none.
+bool structure_of(const SceneDev& s, uint32_t sel, uint32_t& node_base, uint32_t& node_count, uint32_t& tri_base,
+                  uint32_t& tri_count) {
+  if (sel & GE_RT_FRAME_STRUCTURE) {
+    const uint32_t i = sel & ~GE_RT_FRAME_STRUCTURE;
+    if (i >= s.frame_of.size() || s.frame_of[i] == 0xffffffffu) return false;
+    const uint32_t k = s.frame_of[i];
+    node_base = s.frame_node_base[k];
+    node_count = s.frame_node_count[k];
+    tri_base = s.frame_tri_base[k];
+    tri_count = s.frame_tri_count[k];
+    return true;
+  }
+  if (sel >= s.mesh_count) return false;
+  node_base = s.mesh_node_base[sel];
+  node_count = s.mesh_node_count[sel];
+  tri_base = s.mesh_tri_base[sel];
+  tri_count = s.mesh_tri_count[sel];
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ge_rt_scene_set_visible(ge_rt_ctx* ctx, ge_rt_scene* scene, const uint32_t* entries, uint32_t count) try {
+  if (!ctx) return GE_RT_ERROR_INVALID_ARGUMENT;
+  GE_CHECK(ctx, scene && (entries || !count), GE_RT_ERROR_INVALID_ARGUMENT, "null scene or entries");
+  SceneDev& s = scene->dev;
+  const int32_t r = check_visible(ctx, s, entries, count, GE_RT_ERROR_INVALID_ARGUMENT);
+  if (r != GE_RT_OK) return r;
+  s.h_visible.assign(entries, entries + 2 * size_t(count));
+  s.has_visible = true;
+  s.visible_dirty = true;
+  s.version = next_version();
   return GE_RT_OK;
 } catch (...) {
   return host_failure(ctx);
@@ -601,19 +740,51 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_CHECK(ctx, flags == GE_RT_BUILD_FULL || s.built, GE_RT_ERROR_INVALID_SCENE, "a partial build needs a full one first");
   const bool subset = flags == GE_RT_BUILD_SUBSET || flags == GE_RT_BUILD_REFIT;
   GE_CHECK(ctx, !subset || meshes || !count, GE_RT_ERROR_INVALID_ARGUMENT, "null mesh list");
-  GE_CHECK(ctx, flags != GE_RT_BUILD_REFIT || !s.live_dirty, GE_RT_ERROR_INVALID_SCENE,
-           "a refit keeps the instance structure's slots: an instance given another mesh needs a rebuild");
   GE_HIP(ctx, hipSetDevice(ctx->device));
   const hipStream_t st = ctx->stream;
+
+  // ---- the frame's structures: their regions (when the instances changed), their clusters (when
+  // the visible list did), and which instances have geometry ----
+  bool frames_changed = false;
+  if (s.frame_layout_dirty) {
+    const int32_t r = layout_frames(ctx, s, st);
+    if (r != GE_RT_OK) return r;
+    s.rec_all = true;
+    s.visible_dirty = true;
+  }
+  GE_CHECK(ctx, s.frame_inst.empty() || s.has_visible, GE_RT_ERROR_INVALID_SCENE,
+           "instance " + std::to_string(s.frame_inst.empty() ? 0u : s.frame_inst[0]) +
+               " is not static (InstanceDesc::flags bit 7): its structure needs the frame's visible list");
+  if (s.visible_dirty) {
+    const int32_t r = resolve_visible(ctx, s, st);
+    if (r != GE_RT_OK) return r;
+    frames_changed = true;
+  }
+  const uint32_t ni = s.instance_count;
+  std::vector<uint32_t> live;
+  for (uint32_t i = 0; i < ni; ++i)
+    if (instance_live(s, i)) live.push_back(i);
+  if (live != s.h_live) s.live_dirty = true;
+  GE_CHECK(ctx, flags != GE_RT_BUILD_REFIT || !s.live_dirty, GE_RT_ERROR_INVALID_SCENE,
+           "a refit keeps the instance structure's slots: an instance given another mesh, or whose visible clusters came "
+           "or went, needs a rebuild");
   s.eye = eye ? *eye : ge_frame_eye{};
-  GE_HIP(ctx, s.blas_nodes.reserve(size_t(std::max(s.node_pool, 1u)) * sizeof(ge_bvh_node)));
-  GE_HIP(ctx, s.blas_wide.reserve(size_t(std::max(s.node_pool, 1u)) * sizeof(WideNode)));
-  GE_HIP(ctx, s.blas_tris.reserve(size_t(std::max(s.tri_pool, 1u)) * sizeof(ge_bvh_triangle)));
-  GE_HIP(ctx, s.blas_morton.reserve(size_t(std::max(s.tri_pool, 1u)) * 4));
-  GE_HIP(ctx, s.mesh_roots.reserve(size_t(std::max(s.mesh_count, 1u)) * 24));
-  GE_HIP(ctx, s.mesh_height.reserve(size_t(std::max(s.mesh_count, 1u)) * 4));
+  const uint32_t nf = uint32_t(s.frame_inst.size());
+  const uint32_t roots = s.mesh_count + nf;
+  const size_t node_all = size_t(std::max(s.node_pool + s.frame_node_pool, 1u)),
+               tri_all = size_t(std::max(s.tri_pool + s.frame_tri_pool, 1u));
+  // A pool that has to grow loses what it held: every structure is built again into it.
+  const bool regrow = s.blas_nodes.bytes < node_all * sizeof(ge_bvh_node) || s.blas_wide.bytes < node_all * sizeof(WideNode) ||
+                      s.blas_tris.bytes < tri_all * sizeof(ge_bvh_triangle) || s.blas_morton.bytes < tri_all * 4 ||
+                      s.mesh_roots.bytes < size_t(std::max(roots, 1u)) * 24 || s.mesh_height.bytes < size_t(std::max(roots, 1u)) * 4;
+  GE_HIP(ctx, s.blas_nodes.reserve(node_all * sizeof(ge_bvh_node)));
+  GE_HIP(ctx, s.blas_wide.reserve(node_all * sizeof(WideNode)));
+  GE_HIP(ctx, s.blas_tris.reserve(tri_all * sizeof(ge_bvh_triangle)));
+  GE_HIP(ctx, s.blas_morton.reserve(tri_all * 4));
+  GE_HIP(ctx, s.mesh_roots.reserve(size_t(std::max(roots, 1u)) * 24));
+  GE_HIP(ctx, s.mesh_height.reserve(size_t(std::max(roots, 1u)) * 4));
   GE_HIP(ctx, ctx->build.flag.reserve(4));
-  const size_t readback = size_t(s.mesh_count) + 2;  // the flag, the instance structure's height, the meshes'
+  const size_t readback = size_t(roots) + 2;  // the flag, the instance structure's height, the structures'
   if (ctx->host_readback_cap < readback) {
     if (ctx->host_readback) GE_HIP(ctx, hipHostFree(ctx->host_readback));
     ctx->host_readback = nullptr;
@@ -625,9 +796,14 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, hipEventRecord(ctx->bev0, st));
   GE_HIP(ctx, hipMemsetAsync(flag, 0, 4, st));
 
-  // ---- the meshes ----
+  // ---- the segments: meshes, then frame structures ----
+  struct Seg {
+    uint32_t count, node, out, root, mesh;
+    const std::vector<uint32_t>* clusters;  // a frame structure's clusters; null: the mesh's own
+  };
+  std::vector<Seg> segs;
   std::vector<uint32_t> list;
-  if (flags == GE_RT_BUILD_FULL) {
+  if (flags == GE_RT_BUILD_FULL || regrow) {
     for (uint32_t m = 0; m < s.mesh_count; ++m) list.push_back(m);
   } else if (subset) {
     for (uint32_t k = 0; k < count; ++k) {
@@ -637,35 +813,45 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     std::sort(list.begin(), list.end());
     list.erase(std::unique(list.begin(), list.end()), list.end());
   }
-  // Meshes too large for a one-workgroup build first: they take the device-wide passes.
-  std::stable_partition(list.begin(), list.end(), [&](uint32_t m) { return s.mesh_tri_count[m] > k_block_build_max; });
+  for (uint32_t m : list)
+    if (s.mesh_tri_count[m] > 0) segs.push_back({s.mesh_tri_count[m], s.mesh_node_base[m], s.mesh_tri_base[m], m, m, nullptr});
+  if (frames_changed || flags == GE_RT_BUILD_FULL || regrow)
+    for (uint32_t k = 0; k < nf; ++k)
+      if (s.frame_tri_count[k] > 0)
+        segs.push_back({s.frame_tri_count[k], s.frame_node_base[k], s.frame_tri_base[k], s.mesh_count + k,
+                        s.h_instances[s.frame_inst[k]].mesh, &s.frame_clusters[k]});
+  // Structures too large for a one-workgroup build first: they take the device-wide passes.
+  std::stable_partition(segs.begin(), segs.end(), [](const Seg& g) { return g.count > k_block_build_max; });
   uint32_t nseg_global = 0, n_global = 0;
   std::vector<uint32_t> seg_first, seg_count, seg_node, seg_out, seg_root;
   std::vector<uint32_t> cl_list, cl_out, cl_seg, cl_local;
   uint32_t n = 0;
-  for (uint32_t m : list) {
-    if (s.mesh_tri_count[m] == 0) continue;
+  for (const Seg& g : segs) {
     const uint32_t sidx = uint32_t(seg_first.size());
     seg_first.push_back(n);
-    seg_count.push_back(s.mesh_tri_count[m]);
-    seg_node.push_back(s.mesh_node_base[m]);
-    seg_out.push_back(s.mesh_tri_base[m]);
-    seg_root.push_back(m);
-    if (s.mesh_tri_count[m] > k_block_build_max) {
+    seg_count.push_back(g.count);
+    seg_node.push_back(g.node);
+    seg_out.push_back(g.out);
+    seg_root.push_back(g.root);
+    if (g.count > k_block_build_max) {
       nseg_global = sidx + 1;
-      n_global = n + s.mesh_tri_count[m];
+      n_global = n + g.count;
     }
-    const ge_mesh_desc& mesh = s.h_meshes[m];
+    const ge_mesh_desc& mesh = s.h_meshes[g.mesh];
     uint32_t off = n;
-    for (uint32_t c = 0; c < mesh.cluster_count; ++c) {
-      const uint32_t gc = mesh.first_cluster + c;
+    auto add = [&](uint32_t gc) {
       const uint32_t tc = s.h_clusters[gc].triangle_count;
-      if (tc == 0) continue;
+      if (tc == 0) return;
       cl_list.push_back(gc);
       cl_out.push_back(off);
       cl_seg.push_back(sidx);
-      cl_local.push_back(c);
+      cl_local.push_back(gc - mesh.first_cluster);
       off += tc;
+    };
+    if (g.clusters) {
+      for (uint32_t gc : *g.clusters) add(gc);
+    } else {
+      for (uint32_t c = 0; c < mesh.cluster_count; ++c) add(mesh.first_cluster + c);
     }
     n = off;
   }
@@ -692,7 +878,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), uint32_t(seg_first.size()), ctx->tmp[3].as<uint32_t>(),
                                md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
                                ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
-                               std::max(s.node_pool, 1u), s.blas_wide.as<WideNode>(), nseg_global, n_global,
+                               uint32_t(node_all), s.blas_wide.as<WideNode>(), nseg_global, n_global,
                                seg_count.data(), flag));
     GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
                                         ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
@@ -700,7 +886,6 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   }
 
   // ---- the instances: the records the walks read, recomputed where they changed ----
-  const uint32_t ni = s.instance_count;
   const bool eye_moved = std::memcmp(&s.eye, &s.rec_eye, sizeof(ge_frame_eye)) != 0;
   if (flags == GE_RT_BUILD_FULL || s.rec_all || eye_moved || s.h_recs.size() != ni) {
     s.h_recs.assign(ni, InstRec{});
@@ -717,8 +902,9 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
           relative_axis_host(in.cell[r], in.rows[r][3], s.eye.cell[r], s.eye.local[r], s.eye.residual[r]);
     InstRec& rec = s.h_recs[i];
     make_w2o(in, &s.h_T[size_t(i) * 3], rec.w2o);
-    rec.node_base = s.mesh_node_base[in.mesh];
-    rec.tri_base = s.mesh_tri_base[in.mesh];
+    const uint32_t k = s.frame_of[i];
+    rec.node_base = k == 0xffffffffu ? s.mesh_node_base[in.mesh] : s.frame_node_base[k];
+    rec.tri_base = k == 0xffffffffu ? s.mesh_tri_base[in.mesh] : s.frame_tri_base[k];
     rec.first_pair = in.first_pair;
     rec.mask = instance_ray_mask(in.flags);
   }
@@ -731,9 +917,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
                                size_t(rhi - rlo) * 12, hipMemcpyHostToDevice, st));
   }
   if (s.live_dirty) {
-    s.h_live.clear();
-    for (uint32_t i = 0; i < ni; ++i)
-      if (s.mesh_tri_count[s.h_instances[i].mesh] > 0) s.h_live.push_back(i);
+    s.h_live = live;
     GE_HIP(ctx, upload(s.live_ids, s.h_live.data(), s.h_live.size(), st));
   }
   s.rec_lo = 0xffffffffu;
@@ -750,6 +934,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, s.tlas_morton.reserve(size_t(std::max(nl, 1u)) * 4));
   GE_HIP(ctx, s.tlas_root.reserve(32));
   GE_HIP(ctx, s.tlas_height.reserve(4));
+  const uint32_t* root_of = s.root_of.as<uint32_t>();
   const bool refit = flags == GE_RT_BUILD_REFIT && nl >= 2;
   if (refit) {
     // The slots' boxes from the instances' current placement, then the structure refitted.
@@ -760,7 +945,8 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, ctx->tmp[8].reserve(size_t(nl) * 4));
     float* boxes = ctx->tmp[6].as<float>();
     GE_HIP(ctx, launch_instance_boxes(st, nl, s.tlas_slots.as<uint32_t>(), s.instances.as<ge_instance_desc>(),
-                                      s.inst_T.as<float>(), s.mesh_roots.as<float>(), boxes, ctx->tmp[8].as<uint32_t>()));
+                                      s.inst_T.as<float>(), s.mesh_roots.as<float>(), root_of, boxes,
+                                      ctx->tmp[8].as<uint32_t>()));
     GE_HIP(ctx, refit_segment(ctx->build, st, nl, boxes, ctx->tmp[7].as<uint32_t>(), s.tlas_nodes.as<ge_bvh_node>(),
                               s.tlas_wide.as<WideNode>(), s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), flag));
   } else if (nl > 0) {
@@ -776,7 +962,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     uint32_t* codes = perm + nl;
     const uint32_t* tm = ctx->tmp[7].as<uint32_t>();
     GE_HIP(ctx, launch_instance_boxes(st, nl, s.live_ids.as<uint32_t>(), s.instances.as<ge_instance_desc>(), d_T,
-                                      s.mesh_roots.as<float>(), boxes, seg_of));
+                                      s.mesh_roots.as<float>(), root_of, boxes, seg_of));
     GE_HIP(ctx, build_segments(ctx->build, st, nl, boxes, 1, seg_of, tm, tm + 1, tm + 2, s.tlas_nodes.as<ge_bvh_node>(), perm,
                                codes, s.tlas_root.as<float>(), s.tlas_height.as<uint32_t>(), nullptr, s.tlas_node_count,
                                s.tlas_wide.as<WideNode>(), nl > k_block_build_max ? 1u : 0u,
@@ -789,7 +975,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   rb[1] = 0;
   GE_HIP(ctx, hipMemcpyAsync(rb, flag, 4, hipMemcpyDeviceToHost, st));
   if (nl > 0) GE_HIP(ctx, hipMemcpyAsync(rb + 1, s.tlas_height.ptr, 4, hipMemcpyDeviceToHost, st));
-  if (s.mesh_count) GE_HIP(ctx, hipMemcpyAsync(rb + 2, s.mesh_height.ptr, size_t(s.mesh_count) * 4, hipMemcpyDeviceToHost, st));
+  if (roots) GE_HIP(ctx, hipMemcpyAsync(rb + 2, s.mesh_height.ptr, size_t(roots) * 4, hipMemcpyDeviceToHost, st));
   GE_HIP(ctx, hipStreamSynchronize(st));
   float ms = 0.0f;
   GE_HIP(ctx, hipEventElapsedTime(&ms, ctx->bev0, ctx->bev1));
@@ -797,13 +983,16 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   if (rb[0]) s.built = false;
   GE_CHECK(ctx, !rb[0], GE_RT_ERROR_UNSUPPORTED,
            "a structure deeper than 254 levels, or too unbalanced for the build's tiles");
+  s.visible_dirty = false;
   // The traversal stack holds at most three deferred children per wide level (two binary levels) of
   // each structure plus the sentinel between them.
-  s.h_mesh_height.assign(rb + 2, rb + 2 + s.mesh_count);
+  s.h_mesh_height.assign(rb + 2, rb + 2 + roots);
   s.tlas_height_value = rb[1];
   uint32_t blas_max = 0;
   for (uint32_t m = 0; m < s.mesh_count; ++m)
     if (s.mesh_tri_count[m] > 0) blas_max = std::max(blas_max, s.h_mesh_height[m]);
+  for (uint32_t k = 0; k < nf; ++k)
+    if (s.frame_tri_count[k] > 0) blas_max = std::max(blas_max, s.h_mesh_height[s.mesh_count + k]);
   s.built = false;
   const uint32_t need = 3 * ((s.tlas_height_value + 1) / 2) + 3 * ((blas_max + 1) / 2) + 2;
   GE_CHECK(ctx, need <= uint32_t(k_stack), GE_RT_ERROR_UNSUPPORTED,
@@ -827,9 +1016,11 @@ int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, 
     if (prim_count) *prim_count = s.tlas_prims;
     return GE_RT_OK;
   }
-  GE_CHECK(ctx, mesh < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
-  if (node_count) *node_count = s.mesh_node_count[mesh];
-  if (prim_count) *prim_count = s.mesh_tri_count[mesh];
+  uint32_t nb, nn, tb, nt;
+  GE_CHECK(ctx, structure_of(s, mesh, nb, nn, tb, nt), GE_RT_ERROR_INVALID_ARGUMENT,
+           "no such structure (mesh index out of range, or an instance without a frame structure)");
+  if (node_count) *node_count = nn;
+  if (prim_count) *prim_count = nt;
   return GE_RT_OK;
 } catch (...) {
   return host_failure(ctx);
@@ -849,16 +1040,15 @@ int32_t ge_rt_bvh_download(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t me
     if (morton && s.tlas_prims) GE_HIP(ctx, hipMemcpy(morton, s.tlas_morton.ptr, size_t(s.tlas_prims) * 4, hipMemcpyDeviceToHost));
     return GE_RT_OK;
   }
-  GE_CHECK(ctx, mesh < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
-  const uint32_t nn = s.mesh_node_count[mesh], np = s.mesh_tri_count[mesh];
+  uint32_t nb, nn, tb, np;
+  GE_CHECK(ctx, structure_of(s, mesh, nb, nn, tb, np), GE_RT_ERROR_INVALID_ARGUMENT,
+           "no such structure (mesh index out of range, or an instance without a frame structure)");
   if (nodes && nn)
-    GE_HIP(ctx, hipMemcpy(nodes, s.blas_nodes.as<ge_bvh_node>() + s.mesh_node_base[mesh], size_t(nn) * sizeof(ge_bvh_node),
-                          hipMemcpyDeviceToHost));
+    GE_HIP(ctx, hipMemcpy(nodes, s.blas_nodes.as<ge_bvh_node>() + nb, size_t(nn) * sizeof(ge_bvh_node), hipMemcpyDeviceToHost));
   if (triangles && np)
-    GE_HIP(ctx, hipMemcpy(triangles, s.blas_tris.as<ge_bvh_triangle>() + s.mesh_tri_base[mesh],
-                          size_t(np) * sizeof(ge_bvh_triangle), hipMemcpyDeviceToHost));
-  if (morton && np)
-    GE_HIP(ctx, hipMemcpy(morton, s.blas_morton.as<uint32_t>() + s.mesh_tri_base[mesh], size_t(np) * 4, hipMemcpyDeviceToHost));
+    GE_HIP(ctx, hipMemcpy(triangles, s.blas_tris.as<ge_bvh_triangle>() + tb, size_t(np) * sizeof(ge_bvh_triangle),
+                          hipMemcpyDeviceToHost));
+  if (morton && np) GE_HIP(ctx, hipMemcpy(morton, s.blas_morton.as<uint32_t>() + tb, size_t(np) * 4, hipMemcpyDeviceToHost));
   return GE_RT_OK;
 } catch (...) {
   return host_failure(ctx);
@@ -875,11 +1065,10 @@ int32_t ge_rt_bvh_download_wide(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32
       GE_HIP(ctx, hipMemcpy(wide, s.tlas_wide.ptr, size_t(s.tlas_node_count) * sizeof(WideNode), hipMemcpyDeviceToHost));
     return GE_RT_OK;
   }
-  GE_CHECK(ctx, mesh < s.mesh_count, GE_RT_ERROR_INVALID_ARGUMENT, "mesh index out of range");
-  const uint32_t nn = s.mesh_node_count[mesh];
-  if (nn)
-    GE_HIP(ctx, hipMemcpy(wide, s.blas_wide.as<WideNode>() + s.mesh_node_base[mesh], size_t(nn) * sizeof(WideNode),
-                          hipMemcpyDeviceToHost));
+  uint32_t nb, nn, tb, np;
+  GE_CHECK(ctx, structure_of(s, mesh, nb, nn, tb, np), GE_RT_ERROR_INVALID_ARGUMENT,
+           "no such structure (mesh index out of range, or an instance without a frame structure)");
+  if (nn) GE_HIP(ctx, hipMemcpy(wide, s.blas_wide.as<WideNode>() + nb, size_t(nn) * sizeof(WideNode), hipMemcpyDeviceToHost));
   return GE_RT_OK;
 } catch (...) {
   return host_failure(ctx);
@@ -961,7 +1150,9 @@ int32_t ge_rt_trace_closest(ge_rt_ctx* ctx, const ge_rt_scene* scene, const ge_r
   GE_HIP(ctx, launch_trace_closest(ctx->stream, bvh_view(scene->dev), d_rays, count, mask, d_hits,
                                    (flags & GE_RT_COUNT) ? ctx->counters.as<unsigned long long>() : nullptr,
                                    ctx->counters.as<uint32_t>() + 16, ctx->persistent_blocks,
-                                   alpha_rule(scene->dev, flags)));
+                                   alpha_rule(scene->dev, flags),
+                                   GeomMap{scene->dev.frame_inst.empty() ? nullptr : scene->dev.geom_base.as<uint32_t>(),
+                                           scene->dev.entry_of.as<uint32_t>()}));
   if ((r = finish(ctx, false)) != GE_RT_OK) return r;
   if (!dev) {
     GE_HIP(ctx, hipMemcpyAsync(hits, d_hits, size_t(count) * sizeof(ge_hit), hipMemcpyDeviceToHost, ctx->stream));

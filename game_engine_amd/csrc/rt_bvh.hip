@@ -856,14 +856,16 @@ __global__ void k_codes(uint32_t n, const uint64_t* keys, uint32_t* codes) {
   if (g < n) codes[g] = uint32_t(keys[g]) & 0x3fffffffu;
 }
 
+// Slot k's box: the 8 corners of its instance's structure's root box (its mesh's, or the frame's
+// structure's: root_of[id]) through the instance's 3x4, padded by 2^-20 of its largest coordinate.
 __global__ void k_instance_boxes(uint32_t n, const uint32_t* __restrict__ live, const ge_instance_desc* __restrict__ inst,
-                                 const float* __restrict__ T, const float* __restrict__ roots, float* boxes,
-                                 uint32_t* seg_of) {
+                                 const float* __restrict__ T, const float* __restrict__ roots,
+                                 const uint32_t* __restrict__ root_of, float* boxes, uint32_t* seg_of) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const uint32_t id = live[k];
   const ge_instance_desc in = inst[id];
-  const float* b = roots + size_t(in.mesh) * 6;
+  const float* b = roots + size_t(root_of[id]) * 6;
   float lo[3] = {__uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u)};
   float hi[3] = {__uint_as_float(0xff800000u), __uint_as_float(0xff800000u), __uint_as_float(0xff800000u)};
   for (int i = 0; i < 8; ++i) {
@@ -1301,9 +1303,10 @@ hipError_t launch_gather_triangles(hipStream_t stream, uint32_t n, const uint32_
 }
 
 hipError_t launch_instance_boxes(hipStream_t stream, uint32_t n, const uint32_t* live, const ge_instance_desc* inst,
-                                 const float* T, const float* roots, float* boxes, uint32_t* seg_of) {
+                                 const float* T, const float* roots, const uint32_t* root_of, float* boxes,
+                                 uint32_t* seg_of) {
   if (n == 0) return hipSuccess;
-  k_instance_boxes<<<blocks(n, 256), 256, 0, stream>>>(n, live, inst, T, roots, boxes, seg_of);
+  k_instance_boxes<<<blocks(n, 256), 256, 0, stream>>>(n, live, inst, T, roots, root_of, boxes, seg_of);
   return hipGetLastError();
 }
 

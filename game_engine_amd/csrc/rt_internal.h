@@ -95,6 +95,23 @@ struct SceneDev {
   uint32_t texture_count = 0, sampler_count = 0;
   bool any_masked = false, any_textured = false;
   std::vector<std::vector<uint32_t>> mesh_materials;  // per mesh: its clusters' distinct material indices
+  // The frame's structures (the engine's per-frame cluster chain): each instance that is not static
+  // (InstanceDesc::flags bit 7 clear) traces a structure built over the clusters the frame's visible
+  // list gives it. Frame structure k (the k-th such instance in index order) has a fixed region of
+  // the pools as large as its mesh's own structure, after the meshes' regions, and root slot
+  // mesh_count + k; geom_base[instance] is where its clusters' visible entries start in entry_of
+  // (~0 for a static instance).
+  bool has_visible = false;        // a visible list was given
+  bool visible_dirty = false;      // given since the last build
+  std::vector<uint32_t> h_visible;  // the list as given: {instance, cluster} pairs
+  std::vector<uint32_t> frame_of;   // per instance: its frame structure, ~0 if static
+  std::vector<uint32_t> frame_inst, frame_tri_base, frame_node_base, frame_tri_count, frame_node_count;
+  std::vector<std::vector<uint32_t>> frame_clusters;  // per frame structure: its clusters (scene-wide), in list order
+  std::vector<uint32_t> h_geom_base, h_entry_of;
+  uint32_t frame_tri_pool = 0, frame_node_pool = 0;  // the regions' total, after the meshes'
+  bool frame_layout_dirty = true;  // instances changed: the regions and root slots are laid out again
+  DevBuf root_of;                  // per instance: its structure's root slot (its mesh, or mesh_count + frame)
+  DevBuf geom_base, entry_of;
 };
 
 // What the context's last wavefront path-trace dispatch traced, so an adjoint can reuse its
@@ -105,6 +122,18 @@ struct PrimalKey {
   uint64_t version = 0;
   ge_path_trace_params params{};  // with the output addresses cleared
 };
+
+// A hit's geometry index as the API reports it (hit_visible_entry, scene.slang:718-724): the
+// cluster within the mesh for a static instance; the cluster's visible entry for an instance that
+// traces the frame's structure (base[instance] != ~0: its clusters' entries start there in `entry`).
+struct GeomMap {
+  const uint32_t* base;
+  const uint32_t* entry;
+};
+__device__ __forceinline__ uint32_t geom_of(const GeomMap& g, uint32_t instance, uint32_t local_cluster) {
+  const uint32_t b = g.base ? g.base[instance] : 0xffffffffu;
+  return b == 0xffffffffu ? local_cluster : g.entry[b + local_cluster];
+}
 
 // ---- the wavefront path tracer's records (rt_wavefront.hip) ----
 struct PathState {  // 64 bytes: one live path
@@ -243,7 +272,8 @@ hipError_t launch_gather_triangles(hipStream_t stream, uint32_t n, const uint32_
                                    const uint32_t* seg_first, const uint32_t* seg_out, const ge_bvh_triangle* in,
                                    ge_bvh_triangle* out, const uint32_t* morton_in, uint32_t* morton_out);
 hipError_t launch_instance_boxes(hipStream_t stream, uint32_t n, const uint32_t* live_ids, const ge_instance_desc* inst,
-                                 const float* translations, const float* mesh_roots, float* boxes, uint32_t* seg_of);
+                                 const float* translations, const float* roots, const uint32_t* root_of,
+                                 float* boxes, uint32_t* seg_of);
 hipError_t launch_gather_slots(hipStream_t stream, uint32_t n, const uint32_t* perm, const uint32_t* live_ids,
                                uint32_t* slots);
 
@@ -288,7 +318,7 @@ hipError_t launch_shadow_resolve(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView
 // and the masked rule into the path tracer.
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
                                 ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks,
-                                uint32_t alpha);
+                                uint32_t alpha, GeomMap geom);
 hipError_t launch_trace_any(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
                             uint32_t* bits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks,
                             uint32_t alpha);

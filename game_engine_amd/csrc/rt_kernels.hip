@@ -20,7 +20,7 @@ __device__ __forceinline__ uint32_t* lane_stack() { return g_lds + threadIdx.x; 
 template <bool Count, bool Masks>
 __global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_closest(Bvh bvh, const ge_ray* __restrict__ rays, uint32_t n,
                                                       uint32_t mask, uint32_t* head, ge_hit* __restrict__ hits,
-                                                      unsigned long long* counters, uint32_t alpha) {
+                                                      unsigned long long* counters, uint32_t alpha, GeomMap geom) {
   Tally tally;
   uint64_t issued = 0;
   auto load = [&](uint32_t i, uint32_t& slot, Ray& ray, float& tmax, uint32_t& m) {
@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_trace_closest(Bvh bvh, cons
       out.u = h.u;
       out.v = h.v;
       out.instance_id = h.inst;
-      out.geometry = h.prim >> 8;
+      out.geometry = geom_of(geom, h.inst, h.prim >> 8);
       out.primitive = h.prim & 0xffu;
       out.key = h.key;
     } else {
@@ -694,17 +694,17 @@ inline size_t lds(const Bvh&) { return size_t(k_ring) * k_lanes * 4; }
 
 hipError_t launch_trace_closest(hipStream_t stream, const Bvh& bvh, const ge_ray* rays, uint32_t n, uint32_t mask,
                                 ge_hit* hits, unsigned long long* counters, uint32_t* head, uint32_t max_blocks,
-                                uint32_t alpha) {
+                                uint32_t alpha, GeomMap geom) {
   if (n == 0) return hipSuccess;
   hipError_t e = hipMemsetAsync(head, 0, 4, stream);
   if (e != hipSuccess) return e;
   const uint32_t grid = std::min(blocks(n, 64), max_blocks);
   if (alpha != k_alpha_opaque) {
-    if (counters) k_trace_closest<true, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, alpha);
-    else k_trace_closest<false, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, alpha);
+    if (counters) k_trace_closest<true, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, alpha, geom);
+    else k_trace_closest<false, true><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, alpha, geom);
   } else {
-    if (counters) k_trace_closest<true, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, 0u);
-    else k_trace_closest<false, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, 0u);
+    if (counters) k_trace_closest<true, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, 0u, geom);
+    else k_trace_closest<false, false><<<grid, 64, lds(bvh), stream>>>(bvh, rays, n, mask, head, hits, counters, 0u, geom);
   }
   return hipGetLastError();
 }

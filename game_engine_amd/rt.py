@@ -58,6 +58,7 @@ def lib():
             "ge_rt_bvh_info": ([vp, vp, u32, C.POINTER(u32), C.POINTER(u32)], i32),
             "ge_rt_bvh_download": ([vp, vp, u32, vp, vp, vp, vp], i32),
             "ge_rt_bvh_download_wide": ([vp, vp, u32, vp], i32),
+            "ge_rt_scene_set_visible": ([vp, vp, vp, u32], i32),
             "ge_rt_trace_closest": ([vp, vp, vp, u32, u32, vp, u32], i32),
             "ge_rt_trace_any": ([vp, vp, vp, u32, u32, vp, u32], i32),
             "ge_rt_ray_visibility": ([vp, vp, C.POINTER(abi.RayVisibilityParams), vp, u32], i32),
@@ -98,7 +99,8 @@ def lib():
 
 
 EXPORTS = ("ge_rt_create", "ge_rt_destroy", "ge_rt_last_error", "ge_rt_synchronize", "ge_rt_scene_upload",
-           "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_bvh_info",
+           "ge_rt_scene_destroy", "ge_rt_bvh_build", "ge_rt_scene_update_instances", "ge_rt_scene_set_visible",
+           "ge_rt_bvh_info",
            "ge_rt_bvh_download", "ge_rt_bvh_download_wide", "ge_rt_trace_closest", "ge_rt_trace_any", "ge_rt_ray_visibility",
            "ge_rt_path_trace", "ge_rt_tonemap", "ge_rt_shadow_pass", "ge_rt_shadow_resolve",
            "ge_rt_shadow_resolve_noalloc", "ge_rt_stats",
@@ -231,6 +233,15 @@ class Scene:
 
     def update_instances(self, first, records):
         self.ctx.check(lib().ge_rt_scene_update_instances(self.ctx.h, self.h, first, len(records), C.byref(records[0])))
+
+    def set_visible(self, entries, check=True):
+        """The frame's visible list (ge_rt_scene_set_visible): (n, 2) {instance, cluster} entries.
+        Returns the status when `check` is false."""
+        e = np.ascontiguousarray(entries, dtype=np.uint32).reshape(-1, 2)
+        st = lib().ge_rt_scene_set_visible(self.ctx.h, self.h, _ptr(e) if len(e) else None, len(e))
+        if check:
+            self.ctx.check(st)
+        return st
 
     def update_materials(self, first, records):
         self.ctx.check(lib().ge_rt_scene_update_materials(self.ctx.h, self.h, first, len(records), C.addressof(records)))

@@ -116,7 +116,8 @@ typedef struct ge_instance_desc {
 } ge_instance_desc;
 
 /* InstanceDesc::flags bit 7 (scene.slang:694): the instance traces its mesh's own structure and a
- * hit's geometry index is the cluster within the mesh. Every instance this backend builds is one. */
+ * hit's geometry index is the cluster within the mesh. An instance without it traces the frame's
+ * structure of the clusters the visible list gives it (ge_rt_scene_set_visible). */
 #define GE_INSTANCE_STATIC_RT 128u
 /* InstanceDesc::flags bit 6, k_instance_untraced (scene.slang:473-492): never met by a light's ray.
  * The instance's top-level mask is then 0xfe instead of 0xff (instance_ray_mask), so camera rays
@@ -309,7 +310,8 @@ typedef struct ge_hit {
   float u; /* barycentric weight of the triangle's second corner */
   float v; /* barycentric weight of the third corner */
   uint32_t instance_id;
-  uint32_t geometry;  /* cluster index within the instance's mesh */
+  uint32_t geometry;  /* cluster index within the instance's mesh (a static instance), or the visible
+                         entry of the cluster (an instance tracing the frame's structure) */
   uint32_t primitive; /* triangle index within the cluster */
   uint32_t key;       /* pair << 8 | triangle */
   uint32_t pad;
@@ -517,6 +519,21 @@ int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
 /* Replaces light records [first, first + count). Host pointer. */
 int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
                                   const ge_resolve_light* lights);
+/* The frame's visible list (PathTraceParams::visible: cluster_cull's output, scene.slang:663-741):
+ * `count` entries {instance, cluster}, `cluster` scene-wide and within the instance's mesh. From the
+ * next ge_rt_bvh_build on, each instance without GE_INSTANCE_STATIC_RT traces a structure of this
+ * frame built over its entries' clusters in list order, as the engine's per-frame cluster chain
+ * builds one (cluster_acceleration.h:82-200); such an instance with no entry is met by no ray. Its
+ * hits name the entry as their geometry (hit_visible_entry then hit_pair), and their keys stay
+ * pair << 8 | triangle. Entries of static instances are checked and change nothing: those trace
+ * their mesh's own structure. Host pointer; the list is copied. An instance out of range, a cluster
+ * outside its instance's mesh or a repeated {instance, cluster} returns GE_RT_ERROR_INVALID_ARGUMENT
+ * and keeps the previous list. A build with such instances and no list returns
+ * GE_RT_ERROR_INVALID_SCENE. */
+int32_t ge_rt_scene_set_visible(ge_rt_ctx* ctx, ge_rt_scene* scene, const uint32_t* entries, uint32_t count);
+/* With ge_rt_bvh_info, ge_rt_bvh_download and ge_rt_bvh_download_wide, `mesh` =
+ * GE_RT_FRAME_STRUCTURE | instance names the frame's structure of that (non-static) instance. */
+#define GE_RT_FRAME_STRUCTURE 0x80000000u
 /* Sizes of the built structure of mesh `mesh` (or of the top level for mesh == ~0). */
 int32_t ge_rt_bvh_info(ge_rt_ctx* ctx, const ge_rt_scene* scene, uint32_t mesh, uint32_t* node_count,
                        uint32_t* prim_count);

@@ -67,6 +67,7 @@ def lib():
         L.ge_oracle_bvh_info.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.ge_oracle_bvh_download.argtypes = [vp, C.c_uint32, vp, vp, vp, vp]
         L.ge_oracle_bvh_download_wide.argtypes = [vp, C.c_uint32, vp]
+        L.ge_oracle_set_visible.argtypes = [vp, vp, C.c_uint32]
         L.ge_oracle_update_instances.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.ge_oracle_refit_instances.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp]
         L.ge_oracle_trace_closest.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, C.c_int32, C.c_int32, C.c_uint32]
@@ -165,6 +166,11 @@ class Scene:
         wide = np.zeros(n.value, dtype=abi.WIDE_DTYPE)
         assert lib().ge_oracle_bvh_download_wide(self.h, mesh, wide.ctypes.data) == 0
         return wide
+
+    def set_visible(self, entries):
+        """The frame's visible list, (n, 2) {instance, cluster}; returns the status."""
+        e = np.ascontiguousarray(entries, dtype=np.uint32).reshape(-1, 2)
+        return lib().ge_oracle_set_visible(self.h, e.ctypes.data if len(e) else None, len(e))
 
     def update_instances(self, first, records, eye=None, refit=False):
         """Replaces instance records and rebuilds the instance structure (or refits it: its topology

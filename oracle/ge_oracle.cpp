@@ -322,6 +322,7 @@ struct InstRec {
   uint32_t mesh;
   uint32_t first_pair;
   uint32_t mask;  // the top-level record's 8-bit ray mask
+  int32_t frame;  // its frame structure (an instance that is not static), or -1: its mesh's
   bool live;
 };
 
@@ -380,6 +381,13 @@ struct ge_oracle_scene {
   Wide tlas_wide;
   std::vector<uint32_t> tlas_slots;  // sorted slot -> instance id
   ge_frame_eye eye{};                // the frame the instance structure was built in
+  // The frame's structures (the engine's per-frame cluster chain): an instance without
+  // InstanceDesc::flags bit 7 traces a structure over the clusters its entries of the visible list
+  // name, in list order; frame_entry[k][c] is the visible entry of mesh-local cluster c of frame k.
+  std::vector<uint32_t> visible;  // {instance, cluster} pairs
+  std::vector<int32_t> frame_of;  // per instance: its frame structure, or -1 if static
+  std::vector<Blas> frame;
+  std::vector<std::vector<uint32_t>> frame_entry;
 };
 
 namespace {
@@ -654,6 +662,11 @@ Box instance_box(const ge_instance_desc& inst, const float T[3], const Box& b) {
   return r;
 }
 
+// The structure instance record `rec` traces.
+const Blas& blas_of(const ge_oracle_scene& s, const InstRec& rec) {
+  return rec.frame >= 0 ? s.frame[rec.frame] : s.blas[rec.mesh];
+}
+
 void build_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
   const uint32_t ni = uint32_t(s.instances.size());
   s.inst.assign(ni, InstRec{});
@@ -668,9 +681,10 @@ void build_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
     rec.mesh = in.mesh;
     rec.first_pair = in.first_pair;
     rec.mask = instance_ray_mask(in.flags);
-    rec.live = s.blas[in.mesh].bvh.n > 0;
+    rec.frame = s.frame_of[i];
+    rec.live = blas_of(s, rec).bvh.n > 0;
     if (!rec.live) continue;
-    boxes.push_back(instance_box(in, T, s.blas[in.mesh].bvh.root));
+    boxes.push_back(instance_box(in, T, blas_of(s, rec).bvh.root));
     ids.push_back(i);
   }
   lbvh_build(boxes, s.tlas);
@@ -693,7 +707,8 @@ void instance_records(ge_oracle_scene& s, const ge_frame_eye& eye) {
     rec.mesh = in.mesh;
     rec.first_pair = in.first_pair;
     rec.mask = instance_ray_mask(in.flags);
-    rec.live = s.blas[in.mesh].bvh.n > 0;
+    rec.frame = s.frame_of[i];
+    rec.live = blas_of(s, rec).bvh.n > 0;
   }
 }
 
@@ -715,7 +730,7 @@ bool refit_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
     const ge_instance_desc& in = s.instances[id];
     float T[3];
     instance_from_eye(in, eye, T);
-    slot_box[k] = instance_box(in, T, s.blas[in.mesh].bvh.root);
+    slot_box[k] = instance_box(in, T, blas_of(s, s.inst[id]).bvh.root);
   }
   if (s.tlas.n == 1) {
     std::memcpy(s.tlas.nodes[0].lo0, slot_box[0].lo, 12);
@@ -728,12 +743,14 @@ bool refit_tlas(ge_oracle_scene& s, const ge_frame_eye& eye) {
   return true;
 }
 
-void build_blas(ge_oracle_scene& s, uint32_t m) {
+// A structure over mesh m's clusters `local` (mesh-local indices, in order): each triangle's record
+// (positions, prim = cluster << 8 | triangle, index = its place in the list) and its LBVH.
+void build_structure(const ge_oracle_scene& s, uint32_t m, const std::vector<uint32_t>& local, Blas& out) {
   const ge_mesh_desc& mesh = s.meshes[m];
   std::vector<Box> boxes;
   std::vector<ge_bvh_triangle> tris;
   uint32_t index = 0;
-  for (uint32_t c = 0; c < mesh.cluster_count; ++c) {
+  for (uint32_t c : local) {
     const ge_cluster_desc& cl = s.clusters[mesh.first_cluster + c];
     for (uint32_t t = 0; t < cl.triangle_count; ++t, ++index) {
       const uint32_t packed = s.triangles[cl.triangle_offset + t];
@@ -754,11 +771,60 @@ void build_blas(ge_oracle_scene& s, uint32_t m) {
       tris.push_back(tri);
     }
   }
-  Blas& out = s.blas[m];
   lbvh_build(boxes, out.bvh);
   collapse(out.bvh, out.wide);
   out.tris.resize(tris.size());
   for (size_t k = 0; k < tris.size(); ++k) out.tris[k] = tris[out.bvh.order[k]];
+}
+
+void build_blas(ge_oracle_scene& s, uint32_t m) {
+  std::vector<uint32_t> all(s.meshes[m].cluster_count);
+  for (uint32_t c = 0; c < all.size(); ++c) all[c] = c;
+  build_structure(s, m, all, s.blas[m]);
+}
+
+// The visible list against the instances: each {instance, cluster} names an instance in range and
+// a cluster of its mesh, and none repeats.
+bool visible_fits(const ge_oracle_scene& s, const uint32_t* e, uint32_t n) {
+  std::vector<uint64_t> keys;
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t i = e[2 * j], c = e[2 * j + 1];
+    if (i >= s.instances.size()) return false;
+    const ge_mesh_desc& mesh = s.meshes[s.instances[i].mesh];
+    if (c < mesh.first_cluster || c - mesh.first_cluster >= mesh.cluster_count) return false;
+    keys.push_back((uint64_t(i) << 32) | c);
+  }
+  std::sort(keys.begin(), keys.end());
+  return std::adjacent_find(keys.begin(), keys.end()) == keys.end();
+}
+
+// The frame's structures from the visible list: one for each instance that is not static, over its
+// entries' clusters in list order (none: an empty structure, met by no ray).
+void build_frames(ge_oracle_scene& s) {
+  const uint32_t ni = uint32_t(s.instances.size());
+  s.frame_of.assign(ni, -1);
+  std::vector<std::vector<uint32_t>> local;
+  s.frame_entry.clear();
+  for (uint32_t i = 0; i < ni; ++i) {
+    if (s.instances[i].flags & 128u) continue;
+    s.frame_of[i] = int32_t(local.size());
+    local.emplace_back();
+    s.frame_entry.emplace_back(s.meshes[s.instances[i].mesh].cluster_count, 0xffffffffu);
+  }
+  for (uint32_t j = 0; j < s.visible.size() / 2; ++j) {
+    const uint32_t i = s.visible[2 * j], c = s.visible[2 * j + 1];
+    const int32_t k = s.frame_of[i];
+    if (k < 0) continue;
+    const uint32_t lc = c - s.meshes[s.instances[i].mesh].first_cluster;
+    local[k].push_back(lc);
+    s.frame_entry[k][lc] = j;
+  }
+  s.frame.assign(local.size(), Blas{});
+  for (size_t k = 0; k < local.size(); ++k) {
+    uint32_t i = 0;
+    while (s.frame_of[i] != int32_t(k)) ++i;
+    build_structure(s, s.instances[i].mesh, local[k], s.frame[k]);
+  }
 }
 
 // ---- traversal ---------------------------------------------------------------------------------
@@ -916,7 +982,7 @@ bool traverse(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t 
       continue;
     }
     if (!in_blas && (cur & GE_BVH_LEAF)) {
-      blas = &s.blas[rec->mesh];
+      blas = &blas_of(s, *rec);
       to_object(*rec, world, ray);
       if (sp >= 190) std::abort();  // deeper than any structure the build accepts
       stack[sp++] = k_sentinel;
@@ -1180,7 +1246,7 @@ bool traverse4(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t
       continue;
     }
     if (!in_blas && (cur & GE_BVH_LEAF)) {
-      blas = &s.blas[rec->mesh];
+      blas = &blas_of(s, *rec);
       to_object(*rec, world, ray);
       if (sp >= 250) std::abort();
       stack[sp++] = k_sentinel;
@@ -1266,7 +1332,7 @@ bool brute(const ge_oracle_scene& s, const TRay& world, float tmax, uint32_t mas
     if (!rec.live || (rec.mask & mask) == 0u) continue;
     TRay ray;
     to_object(rec, world, ray);
-    for (const ge_bvh_triangle& tri : s.blas[rec.mesh].tris) {
+    for (const ge_bvh_triangle& tri : blas_of(s, rec).tris) {
       float t, u, v;
       if (!intersect(ray, tri.v0, tri.v1, tri.v2, t, u, v)) continue;
       if (!(t >= ray.tmin && t <= hit.t)) continue;
@@ -2347,6 +2413,7 @@ ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* d, const ge_frame_e
   if (d->light_count) s->lights.assign(d->lights, d->lights + d->light_count);
   s->blas.resize(d->mesh_count);
   for (uint32_t m = 0; m < d->mesh_count; ++m) build_blas(*s, m);
+  build_frames(*s);
   const ge_frame_eye zero{};
   s->eye = eye ? *eye : zero;
   build_tlas(*s, s->eye);
@@ -2355,8 +2422,19 @@ ge_oracle_scene* ge_oracle_scene_create(const ge_scene_desc* d, const ge_frame_e
 
 void ge_oracle_scene_destroy(ge_oracle_scene* s) { delete s; }
 
+// The structure a download names: mesh `sel`'s, or with bit 31 the frame structure of instance
+// sel & ~bit 31 (null when there is none).
+const Blas* selected(const ge_oracle_scene* s, uint32_t sel) {
+  if (sel & 0x80000000u) {
+    const uint32_t i = sel & 0x7fffffffu;
+    return i < s->frame_of.size() && s->frame_of[i] >= 0 ? &s->frame[s->frame_of[i]] : nullptr;
+  }
+  return sel < s->blas.size() ? &s->blas[sel] : nullptr;
+}
+
 int32_t ge_oracle_bvh_info(const ge_oracle_scene* s, uint32_t mesh, uint32_t* node_count, uint32_t* prim_count) {
-  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (mesh < s->blas.size() ? &s->blas[mesh].bvh : nullptr);
+  const Blas* sb = mesh == 0xffffffffu ? nullptr : selected(s, mesh);
+  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (sb ? &sb->bvh : nullptr);
   if (!b) return GE_RT_ERROR_INVALID_ARGUMENT;
   if (node_count) *node_count = uint32_t(b->nodes.size());
   if (prim_count) *prim_count = b->n;
@@ -2365,28 +2443,32 @@ int32_t ge_oracle_bvh_info(const ge_oracle_scene* s, uint32_t mesh, uint32_t* no
 
 int32_t ge_oracle_bvh_download(const ge_oracle_scene* s, uint32_t mesh, ge_bvh_node* nodes, ge_bvh_triangle* tris,
                                uint32_t* slots, uint32_t* morton) {
-  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (mesh < s->blas.size() ? &s->blas[mesh].bvh : nullptr);
+  const Blas* sb = mesh == 0xffffffffu ? nullptr : selected(s, mesh);
+  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (sb ? &sb->bvh : nullptr);
   if (!b) return GE_RT_ERROR_INVALID_ARGUMENT;
-  if (nodes) std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(ge_bvh_node));
-  if (morton) std::memcpy(morton, b->morton.data(), b->morton.size() * 4);
+  // (an empty structure, a culled frame structure's, copies nothing: no null source to memcpy)
+  if (nodes && !b->nodes.empty()) std::memcpy(nodes, b->nodes.data(), b->nodes.size() * sizeof(ge_bvh_node));
+  if (morton && !b->morton.empty()) std::memcpy(morton, b->morton.data(), b->morton.size() * 4);
   if (mesh == 0xffffffffu) {
-    if (slots) std::memcpy(slots, s->tlas_slots.data(), s->tlas_slots.size() * 4);
-  } else if (tris) {
-    std::memcpy(tris, s->blas[mesh].tris.data(), s->blas[mesh].tris.size() * sizeof(ge_bvh_triangle));
+    if (slots && !s->tlas_slots.empty()) std::memcpy(slots, s->tlas_slots.data(), s->tlas_slots.size() * 4);
+  } else if (tris && !sb->tris.empty()) {
+    std::memcpy(tris, sb->tris.data(), sb->tris.size() * sizeof(ge_bvh_triangle));
   }
   return GE_RT_OK;
 }
 
 int32_t ge_oracle_bvh_download_wide(const ge_oracle_scene* s, uint32_t mesh, ge_bvh_wide_node* out) {
   static_assert(sizeof(Node4) == sizeof(ge_bvh_wide_node), "the wide node's layout is the backend's");
-  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (mesh < s->blas.size() ? &s->blas[mesh].bvh : nullptr);
+  const Blas* sb = mesh == 0xffffffffu ? nullptr : selected(s, mesh);
+  const Lbvh* b = mesh == 0xffffffffu ? &s->tlas : (sb ? &sb->bvh : nullptr);
   if (!b || !out) return GE_RT_ERROR_INVALID_ARGUMENT;
-  const Wide& w = mesh == 0xffffffffu ? s->tlas_wide : s->blas[mesh].wide;
+  const Wide& w = mesh == 0xffffffffu ? s->tlas_wide : sb->wide;
   std::vector<uint32_t> index;
   even_depth_index(*b, index);
   std::vector<uint32_t> binary(w.size(), 0);
   for (size_t i = 0; i < index.size(); ++i)
     if (index[i] != GE_BVH_NONE) binary[index[i]] = uint32_t(i);
+  if (b->nodes.empty()) return GE_RT_OK;
   std::memset(out, 0, b->nodes.size() * sizeof(ge_bvh_wide_node));
   for (size_t i = 0; i < index.size(); ++i) {
     if (index[i] == GE_BVH_NONE) continue;
@@ -2401,7 +2483,13 @@ int32_t ge_oracle_bvh_download_wide(const ge_oracle_scene* s, uint32_t mesh, ge_
 int32_t ge_oracle_update_instances(ge_oracle_scene* s, uint32_t first, uint32_t count, const ge_instance_desc* in,
                                    const ge_frame_eye* eye) {
   if (first + count > s->instances.size()) return GE_RT_ERROR_INVALID_ARGUMENT;
+  std::vector<ge_instance_desc> was(s->instances.begin() + first, s->instances.begin() + first + count);
   std::memcpy(&s->instances[first], in, size_t(count) * sizeof(ge_instance_desc));
+  if (!visible_fits(*s, s->visible.data(), uint32_t(s->visible.size() / 2))) {
+    std::copy(was.begin(), was.end(), s->instances.begin() + first);  // the list no longer fits: nothing changes
+    return GE_RT_ERROR_INVALID_SCENE;
+  }
+  build_frames(*s);
   const ge_frame_eye zero{};
   s->eye = eye ? *eye : zero;
   build_tlas(*s, s->eye);
@@ -2416,12 +2504,24 @@ int32_t ge_oracle_refit_instances(ge_oracle_scene* s, uint32_t first, uint32_t c
   const ge_frame_eye zero{};
   const ge_frame_eye was_eye = s->eye;
   s->eye = eye ? *eye : zero;
-  if (!refit_tlas(*s, s->eye)) {
+  const bool fits = visible_fits(*s, s->visible.data(), uint32_t(s->visible.size() / 2));
+  if (fits) build_frames(*s);
+  if (!fits || !refit_tlas(*s, s->eye)) {
     s->instances = was;
     s->eye = was_eye;
+    build_frames(*s);
     instance_records(*s, s->eye);
     return GE_RT_ERROR_INVALID_SCENE;
   }
+  return GE_RT_OK;
+}
+
+int32_t ge_oracle_set_visible(ge_oracle_scene* s, const uint32_t* entries, uint32_t count) {
+  if (!entries && count) return GE_RT_ERROR_INVALID_ARGUMENT;
+  if (!visible_fits(*s, entries, count)) return GE_RT_ERROR_INVALID_ARGUMENT;
+  s->visible.assign(entries, entries + 2 * size_t(count));
+  build_frames(*s);
+  build_tlas(*s, s->eye);
   return GE_RT_OK;
 }
 
@@ -2453,7 +2553,8 @@ void ge_oracle_trace_closest(const ge_oracle_scene* s, const ge_ray* rays, uint3
         out.u = h.u;
         out.v = h.v;
         out.instance_id = h.inst;
-        out.geometry = h.prim >> 8;
+        const int32_t k = s->frame_of[h.inst];
+        out.geometry = k < 0 ? h.prim >> 8 : s->frame_entry[k][h.prim >> 8];  // hit_visible_entry
         out.primitive = h.prim & 0xffu;
         out.key = h.key;
       } else {

@@ -42,6 +42,12 @@ int32_t ge_oracle_bvh_download(const ge_oracle_scene* scene, uint32_t mesh, ge_b
 /* The 4-wide nodes of mesh `mesh` (~0: the top level) at the binary nodes' indices (node_count
  * entries), references by binary index as the backend lays them out; zero at odd depth. */
 int32_t ge_oracle_bvh_download_wide(const ge_oracle_scene* scene, uint32_t mesh, ge_bvh_wide_node* wide);
+/* The frame's visible list ({instance, cluster} pairs): each instance without InstanceDesc::flags
+ * bit 7 gets a structure over its entries' clusters in list order, and the top level is rebuilt.
+ * GE_RT_ERROR_INVALID_ARGUMENT (nothing changed) for an instance out of range, a cluster outside its
+ * instance's mesh or a repeated pair. The download functions take mesh = bit 31 | instance for
+ * that instance's frame structure. */
+int32_t ge_oracle_set_visible(ge_oracle_scene* scene, const uint32_t* entries, uint32_t count);
 /* Replaces instance records and rebuilds the top level (the per-frame placement). */
 int32_t ge_oracle_update_instances(ge_oracle_scene* scene, uint32_t first, uint32_t count,
                                    const ge_instance_desc* instances, const ge_frame_eye* eye);
