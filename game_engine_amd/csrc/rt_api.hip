@@ -98,6 +98,38 @@ hipError_t upload(DevBuf& buf, const T* host, size_t count, hipStream_t stream) 
   return hipMemcpyAsync(buf.ptr, host, bytes, hipMemcpyHostToDevice, stream);
 }
 
+// The build's pinned staging (ge_rt_ctx::pin): `bytes` in all for this build.
+hipError_t pin_reserve(ge_rt_ctx* ctx, size_t bytes) {
+  ctx->pin_used = 0;
+  if (ctx->pin_cap >= bytes) return hipSuccess;
+  if (ctx->pin) {
+    const hipError_t e = hipHostFree(ctx->pin);
+    if (e != hipSuccess) return e;
+  }
+  ctx->pin = nullptr;
+  ctx->pin_cap = 0;
+  const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&ctx->pin), bytes);
+  if (e == hipSuccess) ctx->pin_cap = bytes;
+  return e;
+}
+constexpr size_t k_pin_align = 256;
+size_t pin_piece(size_t bytes) { return (bytes + k_pin_align - 1) / k_pin_align * k_pin_align; }
+// Copies `bytes` of host memory to `dev` through the next piece of the staging.
+hipError_t staged_copy(ge_rt_ctx* ctx, void* dev, const void* host, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  if (ctx->pin_used + pin_piece(bytes) > ctx->pin_cap) return hipErrorInvalidValue;  // sized at the build's start
+  uint8_t* piece = ctx->pin + ctx->pin_used;
+  ctx->pin_used += pin_piece(bytes);
+  std::memcpy(piece, host, bytes);
+  return hipMemcpyAsync(dev, piece, bytes, hipMemcpyHostToDevice, st);
+}
+template <class T>
+hipError_t staged_upload(ge_rt_ctx* ctx, DevBuf& buf, const T* host, size_t count, hipStream_t st) {
+  const hipError_t e = buf.reserve(count ? count * sizeof(T) : 256);
+  if (e != hipSuccess) return e;
+  return staged_copy(ctx, buf.ptr, host, count * sizeof(T), st);
+}
+
 // engine::relative_axis (core/math/world.h:181-195), operation for operation.
 float relative_axis_host(int cell, float local, int eye_cell, float eye_local, float eye_residual) {
   const float cells = float(cell - eye_cell) * 64.0f;
@@ -378,7 +410,7 @@ int32_t resolve_visible(ge_rt_ctx* ctx, SceneDev& s, hipStream_t st) {
     s.frame_tri_count[k] = uint32_t(t);
     s.frame_node_count[k] = t == 0 ? 0u : (t == 1 ? 1u : uint32_t(t - 1));
   }
-  GE_HIP(ctx, upload(s.entry_of, s.h_entry_of.data(), s.h_entry_of.size(), st));
+  GE_HIP(ctx, staged_upload(ctx, s.entry_of, s.h_entry_of.data(), s.h_entry_of.size(), st));
   return GE_RT_OK;
 }
 
@@ -437,6 +469,7 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
                     &ctx->build.flag, &ctx->counters})
     b->release();
   if (ctx->host_readback) (void)hipHostFree(ctx->host_readback);
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
   for (DevBuf& b : ctx->io) b.release();
   ctx->resolve_rays.release();
   for (DevBuf& b : ctx->tmp) b.release();
@@ -752,6 +785,17 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     s.rec_all = true;
     s.visible_dirty = true;
   }
+  {
+    // The staging for every upload below: the entry table, the batch's packed arrays (five per
+    // structure, four per cluster, each padded to 64 words), the instance records and translations,
+    // the live list and the instance structure's three words.
+    const size_t ni = s.instance_count, nstruct = size_t(s.mesh_count) + s.frame_inst.size();
+    const size_t nclusters = size_t(s.cluster_count) + s.h_entry_of.size();
+    const size_t bytes = pin_piece(s.h_entry_of.size() * 4) + pin_piece((5 * nstruct + 4 * nclusters + 9 * 64) * 4) +
+                         pin_piece(ni * sizeof(InstRec)) + pin_piece(ni * 12) + pin_piece(ni * 4) + pin_piece(12) +
+                         pin_piece(12);
+    GE_HIP(ctx, pin_reserve(ctx, bytes));
+  }
   GE_CHECK(ctx, s.frame_inst.empty() || s.has_visible, GE_RT_ERROR_INVALID_SCENE,
            "instance " + std::to_string(s.frame_inst.empty() ? 0u : s.frame_inst[0]) +
                " is not static (InstanceDesc::flags bit 7): its structure needs the frame's visible list");
@@ -866,7 +910,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   if (n > 0) {
     const size_t o_sf = pack(seg_first), o_sc = pack(seg_count), o_sn = pack(seg_node), o_so = pack(seg_out),
                  o_sr = pack(seg_root), o_cl = pack(cl_list), o_co = pack(cl_out), o_cs = pack(cl_seg), o_cc = pack(cl_local);
-    GE_HIP(ctx, upload(ctx->tmp[0], meta.data(), meta.size(), st));
+    GE_HIP(ctx, staged_upload(ctx, ctx->tmp[0], meta.data(), meta.size(), st));
     const uint32_t* md = ctx->tmp[0].as<uint32_t>();
     GE_HIP(ctx, ctx->tmp[1].reserve(size_t(n) * sizeof(ge_bvh_triangle)));  // records in batch order
     GE_HIP(ctx, ctx->tmp[2].reserve(size_t(n) * 24));                       // boxes
@@ -911,14 +955,13 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   GE_HIP(ctx, s.inst_recs.reserve(size_t(std::max(ni, 1u)) * sizeof(InstRec)));
   GE_HIP(ctx, s.inst_T.reserve(size_t(std::max(ni, 1u)) * 12));
   if (rhi > rlo) {
-    GE_HIP(ctx, hipMemcpyAsync(s.inst_recs.as<InstRec>() + rlo, s.h_recs.data() + rlo, size_t(rhi - rlo) * sizeof(InstRec),
-                               hipMemcpyHostToDevice, st));
-    GE_HIP(ctx, hipMemcpyAsync(s.inst_T.as<float>() + size_t(rlo) * 3, s.h_T.data() + size_t(rlo) * 3,
-                               size_t(rhi - rlo) * 12, hipMemcpyHostToDevice, st));
+    GE_HIP(ctx, staged_copy(ctx, s.inst_recs.as<InstRec>() + rlo, s.h_recs.data() + rlo, size_t(rhi - rlo) * sizeof(InstRec), st));
+    GE_HIP(ctx, staged_copy(ctx, s.inst_T.as<float>() + size_t(rlo) * 3, s.h_T.data() + size_t(rlo) * 3,
+                            size_t(rhi - rlo) * 12, st));
   }
   if (s.live_dirty) {
     s.h_live = live;
-    GE_HIP(ctx, upload(s.live_ids, s.h_live.data(), s.h_live.size(), st));
+    GE_HIP(ctx, staged_upload(ctx, s.live_ids, s.h_live.data(), s.h_live.size(), st));
   }
   s.rec_lo = 0xffffffffu;
   s.rec_hi = 0;
@@ -940,7 +983,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     // The slots' boxes from the instances' current placement, then the structure refitted.
     std::vector<uint32_t>& tmeta = ctx->host_tmeta;
     tmeta = {0u, nl, 0u};
-    GE_HIP(ctx, upload(ctx->tmp[7], tmeta.data(), tmeta.size(), st));
+    GE_HIP(ctx, staged_upload(ctx, ctx->tmp[7], tmeta.data(), tmeta.size(), st));
     GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + 256));
     GE_HIP(ctx, ctx->tmp[8].reserve(size_t(nl) * 4));
     float* boxes = ctx->tmp[6].as<float>();
@@ -955,7 +998,7 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     GE_HIP(ctx, ctx->tmp[6].reserve(size_t(nl) * 24 + 256));
     float* boxes = ctx->tmp[6].as<float>();
     const float* d_T = s.inst_T.as<float>();
-    GE_HIP(ctx, upload(ctx->tmp[7], tmeta.data(), tmeta.size(), st));
+    GE_HIP(ctx, staged_upload(ctx, ctx->tmp[7], tmeta.data(), tmeta.size(), st));
     GE_HIP(ctx, ctx->tmp[8].reserve(size_t(nl) * 12));
     uint32_t* seg_of = ctx->tmp[8].as<uint32_t>();
     uint32_t* perm = seg_of + nl;

@@ -894,13 +894,16 @@ __global__ void k_gather_slots(uint32_t n, const uint32_t* perm, const uint32_t*
 // ---- a whole mesh's structure in one workgroup --------------------------------------------------
 // A segment of at most kBlockMax primitives is built by one workgroup from start to end in LDS: its
 // centroid bounds, Morton codes, a bitonic sort of (code, index) (the stable order of the global
-// sort), Karras' emission, the fit bottom-up (the second child to finish a node fits it: LDS
-// counters, a workgroup-scope fence between a box's store and the counter), the binary nodes, each
-// node's depth and the 4-wide collapse. One launch builds every small mesh of a batch, where the
-// device-wide passes cost a launch a level; the results are the device-wide build's to the bit.
+// sort), Karras' emission (with each node's primitive count, its range's length), each node's
+// depth by pointer jumping, the fit level by level from the deepest (a barrier a level; a node's
+// internal children sit one level deeper), the binary nodes and the 4-wide collapse. One launch
+// builds every small mesh of a batch, where the device-wide passes cost a launch a level; the
+// results are the device-wide build's to the bit.
 constexpr uint32_t kBlockMax = k_block_build_max;
 constexpr uint32_t kBlockThreads = 512;
 constexpr uint16_t kBlockLeaf = 0x8000u;
+static_assert(kBlockMax == 2 * kBlockThreads && kBlockMax <= 0x8000u && (kBlockMax & (kBlockMax - 1)) == 0,
+              "a thread holds two keys of a power-of-two sort; 15-bit indices");
 
 __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     uint32_t seg_base, const float* __restrict__ boxes, const uint32_t* __restrict__ seg_first,
@@ -908,18 +911,23 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     const uint32_t* __restrict__ root_index, ge_bvh_node* __restrict__ nodes, uint32_t* __restrict__ perm,
     uint32_t* __restrict__ morton, float* __restrict__ roots, uint32_t* __restrict__ heights,
     WideNode* __restrict__ wide) {
-  // 64 KiB: the centroid bounds, then the sort keys, then the fit counters share one array (a node's
-  // counter holds its arrivals in bits 0-7 and its height above).
+  // The centroid bounds, then the sort keys, then each internal node's ancestor link, depth and
+  // height (the fit's level order) share one array.
   __shared__ union {
     uint32_t red[6];
     uint64_t keys[kBlockMax];  // (code << 32 | local index), sorted in place
-    uint32_t arrived[kBlockMax];
+    struct {
+      uint16_t anc[kBlockMax];  // pointer jumping: an ancestor 2^r levels up (0xffff past the root)
+      uint16_t dep[kBlockMax];  // the distance to it, then the node's depth
+      uint16_t hgt[kBlockMax];  // the node's height (a leaf's is 0)
+      uint32_t max_depth;
+    } f;
   } u;
   __shared__ float lbox[kBlockMax][6];     // the leaves' boxes, sorted order
   __shared__ float nbox[kBlockMax][6];     // internal nodes' boxes
   __shared__ uint16_t child[kBlockMax][2]; // internal children; kBlockLeaf | sorted position for a leaf
   __shared__ uint16_t parent[kBlockMax];   // internal node -> its parent
-  __shared__ uint16_t leaf_parent[kBlockMax];
+  __shared__ uint16_t count[kBlockMax];    // internal node -> the primitives below it
   const uint32_t t = threadIdx.x;
   const uint32_t s = seg_base + blockIdx.x;
   const uint32_t first = seg_first[s], n = seg_count[s], nb = node_base[s];
@@ -937,10 +945,17 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
         lo[a] = min(lo[a], c);
         hi[a] = max(hi[a], c);
       }
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&u.red[a], lo[a]);
-      atomicMax(&u.red[3 + a], hi[a]);
-    }
+    // The wave's bounds first: one LDS atomic a wave and axis, not one a lane on six addresses.
+    for (int a = 0; a < 3; ++a)
+      for (int off = 32; off > 0; off >>= 1) {
+        lo[a] = min(lo[a], uint32_t(__shfl_xor(lo[a], off)));
+        hi[a] = max(hi[a], uint32_t(__shfl_xor(hi[a], off)));
+      }
+    if ((t & 63u) == 0)
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(&u.red[a], lo[a]);
+        atomicMax(&u.red[3 + a], hi[a]);
+      }
   }
   __syncthreads();
   float clo[3], cscale[3];
@@ -950,10 +965,10 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     cscale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
   }
   __syncthreads();  // red is read; the keys take its place
-  // Codes, and the bitonic sort over the next power of two (padding keys sort last).
-  uint32_t p2 = 1;
-  while (p2 < n) p2 <<= 1;
-  for (uint32_t i = t; i < p2; i += kBlockThreads) {
+  // Codes: thread t holds keys 2t and 2t + 1, (code << 32 | index); past n, ~0, which sorts last.
+  uint64_t kv[2];
+  for (uint32_t e = 0; e < 2; ++e) {
+    const uint32_t i = 2 * t + e;
     if (i < n) {
       uint32_t q[3];
       for (int a = 0; a < 3; ++a) {
@@ -961,28 +976,52 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
         q[a] = quantize10(c, clo[a], cscale[a]);
       }
       const uint32_t code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
-      u.keys[i] = (uint64_t(code) << 32) | i;
+      kv[e] = (uint64_t(code) << 32) | i;
     } else {
-      u.keys[i] = ~0ull;
+      kv[e] = ~0ull;
     }
   }
-  __syncthreads();
-  for (uint32_t k = 2; k <= p2; k <<= 1) {
+  // Batcher's bitonic sort of the 1,024 keys (unique, so the order is the stable sort's): element i
+  // meets i ^ j, keeping the smaller where i is the lower index of an ascending block (i & k == 0)
+  // or the higher of a descending one. Partners 1 apart are in the thread, 2 to 64 apart in the
+  // wave (lane t ^ j/2, a shuffle), 128 and more apart in other waves (through LDS, a barrier each):
+  // 6 of the 55 stages wait at a barrier.
+  for (uint32_t k = 2; k <= kBlockMax; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = t; i < p2; i += kBlockThreads) {
-        const uint32_t l = i ^ j;
-        if (l > i) {
-          const uint64_t a = u.keys[i], b = u.keys[l];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            u.keys[i] = b;
-            u.keys[l] = a;
-          }
-        }
+      if (j == 1) {
+        const bool up = ((2 * t) & k) == 0;
+        const uint64_t lo = kv[0] < kv[1] ? kv[0] : kv[1], hi = kv[0] < kv[1] ? kv[1] : kv[0];
+        kv[0] = up ? lo : hi;
+        kv[1] = up ? hi : lo;
+        continue;
       }
-      __syncthreads();
+      uint64_t other[2];
+      if (j < 128) {
+        for (uint32_t e = 0; e < 2; ++e) {
+          const uint32_t m = j >> 1;
+          const uint32_t lo32 = uint32_t(__shfl_xor(uint32_t(kv[e]), int(m)));
+          const uint32_t hi32 = uint32_t(__shfl_xor(uint32_t(kv[e] >> 32), int(m)));
+          other[e] = (uint64_t(hi32) << 32) | lo32;
+        }
+      } else {
+        u.keys[2 * t] = kv[0];
+        u.keys[2 * t + 1] = kv[1];
+        __syncthreads();
+        other[0] = u.keys[(2 * t) ^ j];
+        other[1] = u.keys[(2 * t + 1) ^ j];
+        __syncthreads();
+      }
+      for (uint32_t e = 0; e < 2; ++e) {
+        const uint32_t i = 2 * t + e;
+        const bool keep_small = ((i & j) == 0) == ((i & k) == 0);
+        const uint64_t y = other[e];
+        kv[e] = keep_small ? (kv[e] < y ? kv[e] : y) : (kv[e] < y ? y : kv[e]);
+      }
     }
   }
+  u.keys[2 * t] = kv[0];
+  u.keys[2 * t + 1] = kv[1];
+  __syncthreads();
   for (uint32_t i = t; i < n; i += kBlockThreads) {
     const uint32_t o = uint32_t(u.keys[i]);
     perm[first + i] = first + o;
@@ -1006,9 +1045,10 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     return;
   }
   // Karras 2012 over the sorted codes, equal codes split by position.
+  const uint32_t* code32 = reinterpret_cast<const uint32_t*>(u.keys);  // a key's code is its high word
   auto delta = [&](int i, int j) -> int {
     if (j < 0 || j > int(n) - 1) return -1;
-    const uint32_t a = uint32_t(u.keys[i] >> 32), b = uint32_t(u.keys[j] >> 32);
+    const uint32_t a = code32[2 * i + 1], b = code32[2 * j + 1];
     if (a != b) return __clz(a ^ b);
     return 32 + __clz(uint32_t(i) ^ uint32_t(j));
   };
@@ -1035,56 +1075,89 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     const bool l0 = lo == gamma, l1 = hi == gamma + 1;
     child[i][0] = l0 ? uint16_t(kBlockLeaf | gamma) : uint16_t(gamma);
     child[i][1] = l1 ? uint16_t(kBlockLeaf | (gamma + 1)) : uint16_t(gamma + 1);
-    if (l0) leaf_parent[gamma] = uint16_t(i); else parent[gamma] = uint16_t(i);
-    if (l1) leaf_parent[gamma + 1] = uint16_t(i); else parent[gamma + 1] = uint16_t(i);
+    if (!l0) parent[gamma] = uint16_t(i);
+    if (!l1) parent[gamma + 1] = uint16_t(i);
+    count[i] = uint16_t(hi - lo + 1);
   }
-  __syncthreads();  // the keys are done with; the counters take their place
-  for (uint32_t i = t; i < n; i += kBlockThreads) u.arrived[i] = 0;
+  __syncthreads();  // the keys are done with; the depths take their place
+  // Each internal node's depth by pointer jumping: after round r a node's link is its ancestor
+  // 2^(r+1) levels up (or none) and `dep` the levels to it; ceil(log2 n) rounds reach the root.
+  constexpr uint32_t kPer = kBlockMax / kBlockThreads;  // internal nodes a thread holds
+  const uint32_t ni = n - 1;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = t + k * kBlockThreads;
+    if (i < ni) {
+      u.f.anc[i] = i == 0 ? 0xffffu : parent[i];
+      u.f.dep[i] = i == 0 ? 0u : 1u;
+    }
+  }
+  if (t == 0) u.f.max_depth = 0;
+  __syncthreads();
+  for (uint32_t span = 1; span < n; span <<= 1) {
+    uint16_t na[kPer], nd[kPer];
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t i = t + k * kBlockThreads;
+      if (i >= ni) continue;
+      const uint16_t a = u.f.anc[i];
+      na[k] = a == 0xffffu ? a : u.f.anc[a];
+      nd[k] = a == 0xffffu ? u.f.dep[i] : uint16_t(u.f.dep[i] + u.f.dep[a]);
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t i = t + k * kBlockThreads;
+      if (i >= ni) continue;
+      u.f.anc[i] = na[k];
+      u.f.dep[i] = nd[k];
+    }
+    __syncthreads();
+  }
+  {
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t i = t + k * kBlockThreads;
+      if (i < ni) m = u.f.dep[i] > m ? u.f.dep[i] : m;
+    }
+    atomicMax(&u.f.max_depth, m);
+  }
   __syncthreads();
   auto box_of = [&](uint16_t c) -> const float* {
     return (c & kBlockLeaf) ? lbox[c & (kBlockMax - 1)] : nbox[c & (kBlockMax - 1)];
   };
-  auto height_of = [&](uint16_t c) -> uint32_t { return (c & kBlockLeaf) ? 0u : (u.arrived[c] >> 8); };
-  // The fit, in LDS alone: every leaf walks up; the second arrival at a node fits it and goes on.
-  for (uint32_t leaf = t; leaf < n; leaf += kBlockThreads) {
-    uint32_t p = leaf_parent[leaf];
-    for (;;) {
-      __threadfence_block();
-      if ((atomicAdd(&u.arrived[p], 1u) & 0xffu) == 0u) break;
-      __threadfence_block();
-      const uint16_t x = child[p][0], y = child[p][1];
+  auto height_of = [&](uint16_t c) -> uint32_t { return (c & kBlockLeaf) ? 0u : u.f.hgt[c]; };
+  // The fit, level by level from the deepest: a node's internal children are one level deeper, so
+  // they are fitted a barrier earlier. Its box is the union of theirs (exact min and max), its
+  // height one more than the taller one's.
+  for (int level = int(u.f.max_depth); level >= 0; --level) {
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint32_t i = t + k * kBlockThreads;
+      if (i >= ni || u.f.dep[i] != uint32_t(level)) continue;
+      const uint16_t x = child[i][0], y = child[i][1];
       const float* b0 = box_of(x);
       const float* b1 = box_of(y);
-      for (int a = 0; a < 3; ++a) {
-        nbox[p][a] = __builtin_fminf(b0[a], b1[a]);
-        nbox[p][3 + a] = __builtin_fmaxf(b0[3 + a], b1[3 + a]);
+      float a[6], b[6];
+      for (int q = 0; q < 6; ++q) {
+        a[q] = b0[q];
+        b[q] = b1[q];
       }
       const uint32_t h0 = height_of(x), h1 = height_of(y);
-      atomicOr(&u.arrived[p], (1u + (h0 > h1 ? h0 : h1)) << 8);
-      if (p == 0) break;
-      p = parent[p];
+      for (int q = 0; q < 3; ++q) {
+        nbox[i][q] = __builtin_fminf(a[q], b[q]);
+        nbox[i][3 + q] = __builtin_fmaxf(a[3 + q], b[3 + q]);
+      }
+      u.f.hgt[i] = uint16_t(1u + (h0 > h1 ? h0 : h1));
     }
+    __syncthreads();
   }
-  __syncthreads();
   // The binary nodes, the root, each node's depth and the collapse.
   for (uint32_t i = t; i + 1 < n; i += kBlockThreads) {
     const uint16_t x = child[i][0], y = child[i][1];
     const bool lx = x & kBlockLeaf, ly = y & kBlockLeaf;
-    // The primitives below: the node's key range, from its leftmost and rightmost leaves.
-    uint16_t c = x;
-    while (!(c & kBlockLeaf)) c = child[c][0];
-    const uint32_t lo_i = c & ~kBlockLeaf;
-    c = y;
-    while (!(c & kBlockLeaf)) c = child[c][1];
-    const uint32_t hi_i = c & ~kBlockLeaf;
     const float* bx = box_of(x);
     const float* by = box_of(y);
     store_record(nodes + nb + i, bx, bx + 3, by, by + 3, lx ? (GE_BVH_LEAF | uint32_t(x & ~kBlockLeaf)) : uint32_t(x),
                  ly ? (GE_BVH_LEAF | uint32_t(y & ~kBlockLeaf)) : uint32_t(y), i == 0 ? GE_BVH_NONE : uint32_t(parent[i]),
-                 hi_i - lo_i + 1);
-    uint32_t d = 0;
-    for (uint32_t q = i; q != 0; q = parent[q]) ++d;
-    if (d & 1u) continue;
+                 count[i]);
+    if (u.f.dep[i] & 1u) continue;
     const uint16_t x0 = child[x & (kBlockMax - 1)][0], x1 = child[x & (kBlockMax - 1)][1];
     const uint16_t y0 = child[y & (kBlockMax - 1)][0], y1 = child[y & (kBlockMax - 1)][1];
     const uint16_t r[4] = {lx ? x : x0, lx ? (ly ? y : y0) : x1, lx ? y1 : (ly ? y : y0), y1};
@@ -1099,7 +1172,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
   }
   if (t == 0) {
     for (int a = 0; a < 6; ++a) roots[out * 6 + a] = nbox[0][a];
-    heights[out] = u.arrived[0] >> 8;
+    heights[out] = u.f.hgt[0];
   }
 }
 

@@ -216,6 +216,11 @@ struct ge_rt_ctx {
   // every mesh's height, read back with one synchronization.
   uint32_t* host_readback = nullptr;
   size_t host_readback_cap = 0;
+  // Pinned staging for a build's uploads (copies from pageable memory block the host while the
+  // runtime stages them): sized once at the build's start, then handed out piece by piece; the
+  // build's closing synchronization ends every copy before the next build writes it again.
+  uint8_t* pin = nullptr;
+  size_t pin_cap = 0, pin_used = 0;
   // 8 x u64 (closest rays, any rays, closest-hit nodes, triangles, instances, any-hit nodes,
   // triangles, instances), then the persistent walks' head counter at byte 64
   ge::DevBuf counters;
