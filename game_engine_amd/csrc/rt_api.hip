@@ -864,24 +864,35 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
       if (s.frame_tri_count[k] > 0)
         segs.push_back({s.frame_tri_count[k], s.frame_node_base[k], s.frame_tri_base[k], s.mesh_count + k,
                         s.h_instances[s.frame_inst[k]].mesh, &s.frame_clusters[k]});
-  // Structures too large for a one-workgroup build first: they take the device-wide passes.
-  std::stable_partition(segs.begin(), segs.end(), [](const Seg& g) { return g.count > k_block_build_max; });
-  uint32_t nseg_global = 0, n_global = 0;
+  // Structures too large for a one-workgroup build take the device-wide passes over a batch of
+  // triangle records (their clusters listed here); the others are built one workgroup each, each
+  // reading its own clusters, from a table of segments.
+  uint32_t nseg_global = 0, n = 0;
   std::vector<uint32_t> seg_first, seg_count, seg_node, seg_out, seg_root;
   std::vector<uint32_t> cl_list, cl_out, cl_seg, cl_local;
-  uint32_t n = 0;
+  std::vector<BlockSeg>& bsegs = ctx->host_bsegs;
+  std::vector<uint32_t>& blists = ctx->host_blists;
+  bsegs.clear();
+  blists.clear();
   for (const Seg& g : segs) {
-    const uint32_t sidx = uint32_t(seg_first.size());
+    const ge_mesh_desc& mesh = s.h_meshes[g.mesh];
+    if (g.count <= k_block_build_max) {
+      BlockSeg b{g.count, g.node, g.out, g.root, mesh.first_cluster, mesh.cluster_count, mesh.first_cluster, 0u};
+      if (g.clusters) {
+        b.first = uint32_t(blists.size());
+        b.ncl = uint32_t(g.clusters->size());
+        b.list = 1;
+        blists.insert(blists.end(), g.clusters->begin(), g.clusters->end());
+      }
+      bsegs.push_back(b);
+      continue;
+    }
+    const uint32_t sidx = nseg_global++;
     seg_first.push_back(n);
     seg_count.push_back(g.count);
     seg_node.push_back(g.node);
     seg_out.push_back(g.out);
     seg_root.push_back(g.root);
-    if (g.count > k_block_build_max) {
-      nseg_global = sidx + 1;
-      n_global = n + g.count;
-    }
-    const ge_mesh_desc& mesh = s.h_meshes[g.mesh];
     uint32_t off = n;
     auto add = [&](uint32_t gc) {
       const uint32_t tc = s.h_clusters[gc].triangle_count;
@@ -901,32 +912,41 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
   }
   std::vector<uint32_t>& meta = ctx->host_meta;  // every small array of the batch, packed, one upload
   meta.clear();
-  auto pack = [&](const std::vector<uint32_t>& v) {
+  auto pack_words = [&](const uint32_t* p, size_t count) {
     const size_t at = meta.size();
-    meta.insert(meta.end(), v.begin(), v.end());
+    meta.insert(meta.end(), p, p + count);
     while (meta.size() % 64) meta.push_back(0);
     return at;
   };
-  if (n > 0) {
+  auto pack = [&](const std::vector<uint32_t>& v) { return pack_words(v.data(), v.size()); };
+  static_assert(sizeof(BlockSeg) == 32, "eight words a segment");
+  if (n > 0 || !bsegs.empty()) {
     const size_t o_sf = pack(seg_first), o_sc = pack(seg_count), o_sn = pack(seg_node), o_so = pack(seg_out),
-                 o_sr = pack(seg_root), o_cl = pack(cl_list), o_co = pack(cl_out), o_cs = pack(cl_seg), o_cc = pack(cl_local);
+                 o_sr = pack(seg_root), o_cl = pack(cl_list), o_co = pack(cl_out), o_cs = pack(cl_seg), o_cc = pack(cl_local),
+                 o_bs = pack_words(reinterpret_cast<const uint32_t*>(bsegs.data()), bsegs.size() * 8), o_bl = pack(blists);
     GE_HIP(ctx, staged_upload(ctx, ctx->tmp[0], meta.data(), meta.size(), st));
     const uint32_t* md = ctx->tmp[0].as<uint32_t>();
-    GE_HIP(ctx, ctx->tmp[1].reserve(size_t(n) * sizeof(ge_bvh_triangle)));  // records in batch order
-    GE_HIP(ctx, ctx->tmp[2].reserve(size_t(n) * 24));                       // boxes
-    GE_HIP(ctx, ctx->tmp[3].reserve(size_t(n) * 4));                        // segment of each primitive
-    GE_HIP(ctx, ctx->tmp[4].reserve(size_t(n) * 4));                        // sorted order
-    GE_HIP(ctx, ctx->tmp[5].reserve(size_t(n) * 4));                        // sorted codes
-    GE_HIP(ctx, launch_triangle_records(st, s, uint32_t(cl_list.size()), md + o_cl, md + o_co, md + o_cs, md + o_cc,
-                                        ctx->tmp[1].as<ge_bvh_triangle>(), ctx->tmp[2].as<float>(), ctx->tmp[3].as<uint32_t>()));
-    GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), uint32_t(seg_first.size()), ctx->tmp[3].as<uint32_t>(),
-                               md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
-                               ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
-                               uint32_t(node_all), s.blas_wide.as<WideNode>(), nseg_global, n_global,
-                               seg_count.data(), flag));
-    GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
-                                        ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
-                                        ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));
+    GE_HIP(ctx, launch_block_meshes(st, s, uint32_t(bsegs.size()), reinterpret_cast<const BlockSeg*>(md + o_bs), md + o_bl,
+                                    s.blas_nodes.as<ge_bvh_node>(), s.blas_tris.as<ge_bvh_triangle>(),
+                                    s.blas_morton.as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(),
+                                    s.blas_wide.as<WideNode>()));
+    if (n > 0) {
+      GE_HIP(ctx, ctx->tmp[1].reserve(size_t(n) * sizeof(ge_bvh_triangle)));  // records in batch order
+      GE_HIP(ctx, ctx->tmp[2].reserve(size_t(n) * 24));                       // boxes
+      GE_HIP(ctx, ctx->tmp[3].reserve(size_t(n) * 4));                        // segment of each primitive
+      GE_HIP(ctx, ctx->tmp[4].reserve(size_t(n) * 4));                        // sorted order
+      GE_HIP(ctx, ctx->tmp[5].reserve(size_t(n) * 4));                        // sorted codes
+      GE_HIP(ctx, launch_triangle_records(st, s, uint32_t(cl_list.size()), md + o_cl, md + o_co, md + o_cs, md + o_cc,
+                                          ctx->tmp[1].as<ge_bvh_triangle>(), ctx->tmp[2].as<float>(), ctx->tmp[3].as<uint32_t>()));
+      GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), nseg_global, ctx->tmp[3].as<uint32_t>(),
+                                 md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
+                                 ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
+                                 uint32_t(node_all), s.blas_wide.as<WideNode>(), nseg_global, n,
+                                 seg_count.data(), flag));
+      GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
+                                          ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
+                                          ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));
+    }
   }
 
   // ---- the instances: the records the walks read, recomputed where they changed ----

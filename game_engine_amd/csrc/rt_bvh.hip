@@ -150,6 +150,161 @@ __global__ void k_morton(uint32_t n, const float* __restrict__ boxes, const uint
   vals[i] = i;
 }
 
+// ---- one segment of up to kSmallMax primitives: bounds, codes and the sort in one workgroup ----
+// The instance structure of a scene (tens of thousands of instances) and mid-sized meshes take
+// this path instead of the device-wide passes, whose ~25 launches (bounds, codes, then a
+// histogram, a scan and a scatter a digit) cost more than their work at this size. The keys
+// (30-bit codes) live in LDS between the four 8-bit LSD passes. A pass: each wave counts its
+// keys' digits (LDS atomics), the block scans the (digit, wave) counts in digit-major order into
+// places, and each wave scatters its keys a slot at a time, a key's place being its (digit, wave)
+// place plus the keys of that digit in the wave's earlier slots and lower lanes. The lanes that
+// share a digit (the peers) come from a 64-bit mask a digit in LDS that each lane ORs its bit
+// into: a few LDS operations a key, where ballots over the digit's 8 bits cost ~50 vector
+// instructions on the four SIMDs one workgroup has. The element order inside a wave is
+// slot-major (element s * 64 + lane of the wave's run), so places follow positions and every pass
+// is stable: the result is the device-wide sort's, key for key and index for index.
+constexpr uint32_t kSmallThreads = 1024;
+constexpr uint32_t kSmallItems = 17;  // keys a lane holds
+constexpr uint32_t kSmallWaves = kSmallThreads / 64;
+constexpr uint32_t kSmallMax = kSmallThreads * kSmallItems;
+static_assert(kSmallMax <= 0x10000u, "16-bit indices in LDS");
+
+__global__ void __launch_bounds__(kSmallThreads) k_sort_small(uint32_t n, const float* __restrict__ boxes,
+                                                               uint64_t* __restrict__ keys_out,
+                                                               uint32_t* __restrict__ vals_out,
+                                                               uint32_t* __restrict__ perm_out,
+                                                               uint32_t* __restrict__ codes_out) {
+  __shared__ uint32_t lkey[kSmallMax];
+  __shared__ uint16_t lval[kSmallMax];
+  __shared__ uint32_t cnt[kSmallWaves][256];            // a wave's count of each digit, then its place
+  __shared__ unsigned long long peer[kSmallWaves][256];  // the lanes holding each digit in this slot
+  __shared__ uint32_t wsum[kSmallWaves];
+  __shared__ uint32_t red[6];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t run = wave * 64u * kSmallItems;  // the wave's first element
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const unsigned long long me = 1ull << lane;
+
+  // Centroid bounds (the same order-preserving min and max as the device-wide passes).
+  if (t < 6) red[t] = t < 3 ? 0xffffffffu : 0u;
+  for (uint32_t d = lane; d < 256; d += 64) peer[wave][d] = 0;
+  __syncthreads();
+  {
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    for (uint32_t i = t; i < n; i += kSmallThreads)
+      for (int a = 0; a < 3; ++a) {
+        const uint32_t c = ordered((boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f);
+        lo[a] = min(lo[a], c);
+        hi[a] = max(hi[a], c);
+      }
+    for (int a = 0; a < 3; ++a)
+      for (int off = 32; off > 0; off >>= 1) {
+        lo[a] = min(lo[a], uint32_t(__shfl_xor(lo[a], off)));
+        hi[a] = max(hi[a], uint32_t(__shfl_xor(hi[a], off)));
+      }
+    if (lane == 0)
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(&red[a], lo[a]);
+        atomicMax(&red[3 + a], hi[a]);
+      }
+  }
+  __syncthreads();
+  float clo[3], cscale[3];
+  for (int a = 0; a < 3; ++a) {
+    clo[a] = unordered(red[a]);
+    const float ext = unordered(red[3 + a]) - clo[a];
+    cscale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+  }
+  // Codes, through LDS into the lanes' slots; past n, ~0 (above every 30-bit code: sorts last).
+  for (uint32_t i = t; i < kSmallMax; i += kSmallThreads) {
+    uint32_t code = 0xffffffffu;
+    if (i < n) {
+      uint32_t q[3];
+      for (int a = 0; a < 3; ++a) {
+        const float c = (boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f;
+        q[a] = quantize10(c, clo[a], cscale[a]);
+      }
+      code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    }
+    lkey[i] = code;
+  }
+  __syncthreads();
+  uint32_t key[kSmallItems], val[kSmallItems];
+#pragma unroll
+  for (uint32_t s = 0; s < kSmallItems; ++s) {
+    val[s] = run + s * 64u + lane;
+    key[s] = lkey[val[s]];
+  }
+  __syncthreads();  // lkey is rewritten by the first scatter
+#pragma unroll  // four passes, unrolled: the arrays are not carried through a loop as register tuples
+  for (uint32_t shift = 0; shift < 32; shift += 8) {
+    for (uint32_t d = lane; d < 256; d += 64) cnt[wave][d] = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t s = 0; s < kSmallItems; ++s) atomicAdd(&cnt[wave][(key[s] >> shift) & 255u], 1u);
+    __syncthreads();
+    // Places: the exclusive scan of the counts in (digit, wave) order; thread t holds digit t / 4,
+    // waves 4 (t % 4) to 4 (t % 4) + 3.
+    {
+      const uint32_t d = t >> 2, w0 = (t & 3u) * 4u;
+      uint32_t c[4], sum = 0;
+      for (uint32_t j = 0; j < 4; ++j) {
+        c[j] = cnt[w0 + j][d];
+        sum += c[j];
+      }
+      uint32_t incl = sum;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = uint32_t(__shfl_up(incl, off));
+        if (lane >= uint32_t(off)) incl += y;
+      }
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      uint32_t base = 0;
+      for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+      uint32_t at = base + incl - sum;
+      for (uint32_t j = 0; j < 4; ++j) {
+        cnt[w0 + j][d] = at;
+        at += c[j];
+      }
+    }
+    __syncthreads();
+    // The scatter, a slot at a time. LDS operations of one wave complete in order: every lane's OR
+    // lands before any lane reads the mask back, and the reads before the mask is cleared.
+#pragma unroll
+    for (uint32_t s = 0; s < kSmallItems; ++s) {
+      const uint32_t k = key[s];
+      const uint32_t d = (k >> shift) & 255u;
+      atomicOr(&peer[wave][d], me);
+      __builtin_amdgcn_wave_barrier();
+      const uint64_t peers = peer[wave][d];
+      const uint32_t at = cnt[wave][d];
+      __builtin_amdgcn_wave_barrier();
+      peer[wave][d] = 0;
+      const uint32_t p = at + uint32_t(__popcll(peers & lt));
+      lkey[p] = k;
+      lval[p] = uint16_t(val[s]);
+      if ((peers & lt) == 0) cnt[wave][d] = at + uint32_t(__popcll(peers));
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    if (shift + 8 < 32) {
+#pragma unroll
+      for (uint32_t s = 0; s < kSmallItems; ++s) {
+        key[s] = lkey[run + s * 64u + lane];
+        val[s] = lval[run + s * 64u + lane];
+      }
+      __syncthreads();  // cnt is cleared and lkey rewritten by the next pass
+    }
+  }
+  for (uint32_t i = t; i < n; i += kSmallThreads) {
+    const uint32_t k = lkey[i], v = lval[i];
+    keys_out[i] = k;
+    vals_out[i] = v;
+    perm_out[i] = v;
+    codes_out[i] = k & 0x3fffffffu;
+  }
+}
+
 // ---- stable LSD radix sort, 8-bit digits -------------------------------------------------------
 __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* __restrict__ keys, uint32_t n, int shift,
                                                             uint32_t* hist, uint32_t tiles) {
@@ -803,6 +958,19 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileArgs a) {
   }
 }
 
+// Triangle t of cluster cl: its three corners from the packed 8-bit vertex indices.
+__device__ __forceinline__ void fetch_corners(const ge_cluster_desc& cl, const float* __restrict__ vertices,
+                                              const uint32_t* __restrict__ triangles, uint32_t t, ge_bvh_triangle& tri) {
+  const uint32_t packed = triangles[cl.triangle_offset + t];
+  float* v[3] = {tri.v0, tri.v1, tri.v2};
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t vi = cl.vertex_offset + ((packed >> (8 * q)) & 0xffu);
+    v[q][0] = vertices[size_t(vi) * 3];
+    v[q][1] = vertices[size_t(vi) * 3 + 1];
+    v[q][2] = vertices[size_t(vi) * 3 + 2];
+  }
+}
+
 __global__ void k_triangle_records(const ge_cluster_desc* __restrict__ clusters, const float* __restrict__ vertices,
                                    const uint32_t* __restrict__ triangles, const uint32_t* __restrict__ cluster_list,
                                    const uint32_t* __restrict__ cluster_out, const uint32_t* __restrict__ cluster_seg,
@@ -815,15 +983,8 @@ __global__ void k_triangle_records(const ge_cluster_desc* __restrict__ clusters,
   const uint32_t local_cluster = cluster_local[k];
   const uint32_t s = cluster_seg[k];
   for (uint32_t t = threadIdx.x; t < cl.triangle_count; t += blockDim.x) {
-    const uint32_t packed = triangles[cl.triangle_offset + t];
     ge_bvh_triangle tri;
-    float* v[3] = {tri.v0, tri.v1, tri.v2};
-    for (int q = 0; q < 3; ++q) {
-      const uint32_t vi = cl.vertex_offset + ((packed >> (8 * q)) & 0xffu);
-      v[q][0] = vertices[size_t(vi) * 3];
-      v[q][1] = vertices[size_t(vi) * 3 + 1];
-      v[q][2] = vertices[size_t(vi) * 3 + 2];
-    }
+    fetch_corners(cl, vertices, triangles, t, tri);
     const uint32_t o = out0 + t;
     tri.prim = (local_cluster << 8) | t;
     tri.index = o;  // rewritten to the mesh-local index by the gather
@@ -899,22 +1060,54 @@ __global__ void k_gather_slots(uint32_t n, const uint32_t* perm, const uint32_t*
 // internal children sit one level deeper), the binary nodes and the 4-wide collapse. One launch
 // builds every small mesh of a batch, where the device-wide passes cost a launch a level; the
 // results are the device-wide build's to the bit.
+//
+// Over triangles (Tri), a workgroup also reads its segment's clusters itself (a mesh's own, or a
+// frame structure's list) and writes the sorted triangle records and codes straight to the
+// structure's place: no batch records, boxes, order or gather pass go through memory.
 constexpr uint32_t kBlockMax = k_block_build_max;
 constexpr uint32_t kBlockThreads = 512;
 constexpr uint16_t kBlockLeaf = 0x8000u;
 static_assert(kBlockMax == 2 * kBlockThreads && kBlockMax <= 0x8000u && (kBlockMax & (kBlockMax - 1)) == 0,
               "a thread holds two keys of a power-of-two sort; 15-bit indices");
 
-__global__ void __launch_bounds__(kBlockThreads) k_block_build(
-    uint32_t seg_base, const float* __restrict__ boxes, const uint32_t* __restrict__ seg_first,
-    const uint32_t* __restrict__ seg_count, const uint32_t* __restrict__ node_base,
-    const uint32_t* __restrict__ root_index, ge_bvh_node* __restrict__ nodes, uint32_t* __restrict__ perm,
-    uint32_t* __restrict__ morton, float* __restrict__ roots, uint32_t* __restrict__ heights,
-    WideNode* __restrict__ wide) {
+struct BlockArgs {
+  // Boxes given (Tri false): segment seg_base + blockIdx.x of the batch tables, its primitives'
+  // boxes at boxes[6 seg_first ..], its sorted order (batch indices) to perm[seg_first ..] and its
+  // codes to morton[seg_first ..].
+  uint32_t seg_base;
+  const float* boxes;
+  const uint32_t* seg_first;
+  const uint32_t* seg_count;
+  const uint32_t* node_base;
+  const uint32_t* root_index;
+  uint32_t* perm;
+  // Triangles (Tri): segment blockIdx.x of `segs`; its sorted records to tris[out ..] and codes to
+  // morton[out ..].
+  const BlockSeg* segs;
+  const uint32_t* lists;  // frame structures' cluster lists
+  const ge_cluster_desc* clusters;
+  const float* vertices;
+  const uint32_t* triangles;
+  ge_bvh_triangle* tris;
+  // Both.
+  ge_bvh_node* nodes;
+  uint32_t* morton;
+  float* roots;
+  uint32_t* heights;
+  WideNode* wide;
+};
+
+__device__ __forceinline__ uint32_t seg_cluster(const BlockSeg& g, const uint32_t* lists, uint32_t j) {
+  return g.list ? lists[g.first + j] : g.first + j;
+}
+
+template <bool Tri>
+__global__ void __launch_bounds__(kBlockThreads) k_block_build(BlockArgs ba) {
   // The centroid bounds, then the sort keys, then each internal node's ancestor link, depth and
   // height (the fit's level order) share one array.
   __shared__ union {
     uint32_t red[6];
+    uint32_t scan[kBlockThreads + 1 + kBlockThreads / 64];  // the triangles' cluster prefix
     uint64_t keys[kBlockMax];  // (code << 32 | local index), sorted in place
     struct {
       uint16_t anc[kBlockMax];  // pointer jumping: an ancestor 2^r levels up (0xffff past the root)
@@ -924,15 +1117,78 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     } f;
   } u;
   __shared__ float lbox[kBlockMax][6];     // the leaves' boxes, sorted order
-  __shared__ float nbox[kBlockMax][6];     // internal nodes' boxes
-  __shared__ uint16_t child[kBlockMax][2]; // internal children; kBlockLeaf | sorted position for a leaf
+  __shared__ float nbox[kBlockMax][6];     // internal nodes' boxes (first, over triangles, the leaves' in batch order)
+  __shared__ __align__(16) uint16_t child[kBlockMax][2];  // internal children; kBlockLeaf | sorted position for a leaf
   __shared__ uint16_t parent[kBlockMax];   // internal node -> its parent
   __shared__ uint16_t count[kBlockMax];    // internal node -> the primitives below it
   const uint32_t t = threadIdx.x;
-  const uint32_t s = seg_base + blockIdx.x;
-  const uint32_t first = seg_first[s], n = seg_count[s], nb = node_base[s];
-  const uint32_t out = root_index ? root_index[s] : s;
-  const float* sb = boxes + size_t(first) * 6;
+  uint32_t n, nb, out, first = 0;
+  BlockSeg g{};
+  if (Tri) {
+    g = ba.segs[blockIdx.x];
+    n = g.count;
+    nb = g.node;
+    out = g.root;
+  } else {
+    const uint32_t s = ba.seg_base + blockIdx.x;
+    first = ba.seg_first[s];
+    n = ba.seg_count[s];
+    nb = ba.node_base[s];
+    out = ba.root_index ? ba.root_index[s] : s;
+  }
+  // A triangle's source, (its cluster's place in the segment's list << 8 | triangle in the
+  // cluster), kept where the child links go later.
+  uint32_t* src = reinterpret_cast<uint32_t*>(child);
+  // Primitive i's box: from memory, or from the triangle phase's boxes in nbox.
+  auto pbox = [&](uint32_t i, int a) -> float { return Tri ? nbox[i][a] : ba.boxes[size_t(first + i) * 6 + a]; };
+
+  if (Tri) {
+    // The segment's triangles in batch order: its clusters in list order, a chunk of
+    // kBlockThreads clusters at a time (a scan of their triangle counts), each triangle finding
+    // its cluster by a binary search of the chunk's prefix. Boxes to nbox, sources to src.
+    uint32_t* pre = u.scan;
+    uint32_t* wsum = u.scan + kBlockThreads + 1;
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    uint32_t done = 0;
+    for (uint32_t c0 = 0; c0 < g.ncl; c0 += kBlockThreads) {
+      const uint32_t j = c0 + t;
+      const uint32_t tc = j < g.ncl ? ba.clusters[seg_cluster(g, ba.lists, j)].triangle_count : 0u;
+      uint32_t incl = tc;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = uint32_t(__shfl_up(incl, off));
+        if (lane >= uint32_t(off)) incl += y;
+      }
+      if (lane == 63) wsum[wave] = incl;
+      __syncthreads();
+      uint32_t base = 0, total = 0;
+      for (uint32_t w = 0; w < kBlockThreads / 64; ++w) {
+        base += w < wave ? wsum[w] : 0u;
+        total += wsum[w];
+      }
+      pre[t] = base + incl - tc;
+      __syncthreads();
+      const uint32_t m = min(kBlockThreads, g.ncl - c0);
+      for (uint32_t x = t; x < total; x += kBlockThreads) {
+        uint32_t lo = 0, hi = m - 1;  // the last cluster of the chunk that starts at or before x
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (pre[mid] <= x) lo = mid;
+          else hi = mid - 1;
+        }
+        const uint32_t tri_in = x - pre[lo];
+        const uint32_t o = done + x;
+        ge_bvh_triangle tri;
+        fetch_corners(ba.clusters[seg_cluster(g, ba.lists, c0 + lo)], ba.vertices, ba.triangles, tri_in, tri);
+        for (int a = 0; a < 3; ++a) {
+          nbox[o][a] = __builtin_fminf(__builtin_fminf(tri.v0[a], tri.v1[a]), tri.v2[a]);
+          nbox[o][3 + a] = __builtin_fmaxf(__builtin_fmaxf(tri.v0[a], tri.v1[a]), tri.v2[a]);
+        }
+        src[o] = ((c0 + lo) << 8) | tri_in;
+      }
+      done += total;
+      __syncthreads();
+    }
+  }
 
   // Centroid bounds, as order-preserving integers (min and max are the float ones).
   if (t < 6) u.red[t] = t < 3 ? 0xffffffffu : 0u;
@@ -941,7 +1197,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
     for (uint32_t i = t; i < n; i += kBlockThreads)
       for (int a = 0; a < 3; ++a) {
-        const uint32_t c = ordered((sb[size_t(i) * 6 + a] + sb[size_t(i) * 6 + 3 + a]) * 0.5f);
+        const uint32_t c = ordered((pbox(i, a) + pbox(i, 3 + a)) * 0.5f);
         lo[a] = min(lo[a], c);
         hi[a] = max(hi[a], c);
       }
@@ -972,7 +1228,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     if (i < n) {
       uint32_t q[3];
       for (int a = 0; a < 3; ++a) {
-        const float c = (sb[size_t(i) * 6 + a] + sb[size_t(i) * 6 + 3 + a]) * 0.5f;
+        const float c = (pbox(i, a) + pbox(i, 3 + a)) * 0.5f;
         q[a] = quantize10(c, clo[a], cscale[a]);
       }
       const uint32_t code = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
@@ -1024,9 +1280,23 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
   __syncthreads();
   for (uint32_t i = t; i < n; i += kBlockThreads) {
     const uint32_t o = uint32_t(u.keys[i]);
-    perm[first + i] = first + o;
-    morton[first + i] = uint32_t(u.keys[i] >> 32);
-    for (int a = 0; a < 6; ++a) lbox[i][a] = sb[size_t(o) * 6 + a];
+    const uint32_t code = uint32_t(u.keys[i] >> 32);
+    for (int a = 0; a < 6; ++a) lbox[i][a] = pbox(o, a);
+    if (Tri) {
+      // The record of the triangle sorted to place i, its corners fetched again (L2-resident).
+      const uint32_t from = src[o], j = from >> 8, tri_in = from & 0xffu;
+      const uint32_t gc = seg_cluster(g, ba.lists, j);
+      ge_bvh_triangle tri;
+      fetch_corners(ba.clusters[gc], ba.vertices, ba.triangles, tri_in, tri);
+      tri.prim = ((gc - g.cbase) << 8) | tri_in;
+      tri.index = o;
+      tri.pad = 0;
+      ba.tris[g.out + i] = tri;
+      ba.morton[g.out + i] = code;
+    } else {
+      ba.perm[first + i] = first + o;
+      ba.morton[first + i] = code;
+    }
   }
   if (n == 1) {
     __syncthreads();
@@ -1035,12 +1305,12 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
       const float* hi = lbox[0] + 3;
       const float empty_lo[3] = {__uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u), __uint_as_float(0x7f800000u)};
       const float empty_hi[3] = {__uint_as_float(0xff800000u), __uint_as_float(0xff800000u), __uint_as_float(0xff800000u)};
-      store_record(nodes + nb, lo, hi, empty_lo, empty_hi, GE_BVH_LEAF, GE_BVH_NONE, GE_BVH_NONE, 1u);
-      for (int a = 0; a < 6; ++a) roots[out * 6 + a] = lbox[0][a];
-      heights[out] = 1;
+      store_record(ba.nodes + nb, lo, hi, empty_lo, empty_hi, GE_BVH_LEAF, GE_BVH_NONE, GE_BVH_NONE, 1u);
+      for (int a = 0; a < 6; ++a) ba.roots[out * 6 + a] = lbox[0][a];
+      ba.heights[out] = 1;
       const uint32_t ref[4] = {GE_BVH_LEAF, 0u, 0u, 0u};
       const float* box[4] = {lbox[0], lbox[0], lbox[0], lbox[0]};
-      store_wide(wide + nb, ref, box, 1);
+      store_wide(ba.wide + nb, ref, box, 1);
     }
     return;
   }
@@ -1154,7 +1424,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
     const bool lx = x & kBlockLeaf, ly = y & kBlockLeaf;
     const float* bx = box_of(x);
     const float* by = box_of(y);
-    store_record(nodes + nb + i, bx, bx + 3, by, by + 3, lx ? (GE_BVH_LEAF | uint32_t(x & ~kBlockLeaf)) : uint32_t(x),
+    store_record(ba.nodes + nb + i, bx, bx + 3, by, by + 3, lx ? (GE_BVH_LEAF | uint32_t(x & ~kBlockLeaf)) : uint32_t(x),
                  ly ? (GE_BVH_LEAF | uint32_t(y & ~kBlockLeaf)) : uint32_t(y), i == 0 ? GE_BVH_NONE : uint32_t(parent[i]),
                  count[i]);
     if (u.f.dep[i] & 1u) continue;
@@ -1168,11 +1438,11 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(
       ref[e] = (r[e] & kBlockLeaf) ? (GE_BVH_LEAF | uint32_t(r[e] & ~kBlockLeaf)) : uint32_t(r[e]);
       box[e] = box_of(r[e]);
     }
-    store_wide(wide + nb + i, ref, box, (lx ? 1 : 2) + (ly ? 1 : 2));
+    store_wide(ba.wide + nb + i, ref, box, (lx ? 1 : 2) + (ly ? 1 : 2));
   }
   if (t == 0) {
-    for (int a = 0; a < 6; ++a) roots[out * 6 + a] = nbox[0][a];
-    heights[out] = u.f.hgt[0];
+    for (int a = 0; a < 6; ++a) ba.roots[out * 6 + a] = nbox[0][a];
+    ba.heights[out] = u.f.hgt[0];
   }
 }
 
@@ -1253,9 +1523,20 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   hipError_t e;
   // Segments [nseg_global, nseg) fit a workgroup's LDS: one launch builds them all.
   if (nseg_global < nseg) {
-    k_block_build<<<nseg - nseg_global, kBlockThreads, 0, stream>>>(nseg_global, boxes, d_seg_first, d_seg_count,
-                                                                    d_node_base, d_root_index, nodes, perm, morton,
-                                                                    roots, heights, wide);
+    BlockArgs ba{};
+    ba.seg_base = nseg_global;
+    ba.boxes = boxes;
+    ba.seg_first = d_seg_first;
+    ba.seg_count = d_seg_count;
+    ba.node_base = d_node_base;
+    ba.root_index = d_root_index;
+    ba.perm = perm;
+    ba.nodes = nodes;
+    ba.morton = morton;
+    ba.roots = roots;
+    ba.heights = heights;
+    ba.wide = wide;
+    k_block_build<false><<<nseg - nseg_global, kBlockThreads, 0, stream>>>(ba);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (nseg_global == 0) return hipSuccess;
@@ -1274,30 +1555,38 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   if ((e = sc.seg_cbox.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
   if ((e = sc.seg_scale.reserve(size_t(nseg) * 24)) != hipSuccess) return e;
 
-  k_init_bounds<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>());
-  // Runs of 64 x steps primitives a wave: at least 2,048 waves, at most 64 steps a run.
-  const uint32_t steps = std::max(1u, std::min(kBoundsMaxSteps, n / (64u * 2048u)));
-  const uint32_t bound_waves = blocks(n, 64u * steps);
-  k_centroid_bounds<<<blocks(uint64_t(bound_waves) * 64, 256), 256, 0, stream>>>(n, steps, boxes, d_seg_of,
-                                                                                  sc.seg_cbox.as<uint32_t>());
-  k_segment_scale<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>(), sc.seg_scale.as<float>());
-  k_morton<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, d_seg_of, sc.seg_scale.as<float>(), sc.keys[0].as<uint64_t>(),
-                                               sc.vals[0].as<uint32_t>());
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-
-  uint32_t seg_bits = 0;
-  while ((1u << seg_bits) < nseg) ++seg_bits;
-  const int key_bits = 30 + int(seg_bits);
+  // One segment small enough for one workgroup's LDS: bounds, codes, the sort, the codes and the
+  // order written out, in one launch.
+  const bool small = nseg == 1 && n <= kSmallMax;
   int cur = 0;
-  for (int shift = 0; shift < key_bits; shift += 8) {
-    k_sort_hist<<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), n, shift, sc.hist.as<uint32_t>(), tiles);
+  if (small) {
+    k_sort_small<<<1, kSmallThreads, 0, stream>>>(n, boxes, sc.keys[0].as<uint64_t>(), sc.vals[0].as<uint32_t>(), perm,
+                                                  morton);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = exclusive_scan(sc.hist.as<uint32_t>(), tiles * 256, sc.scan_tmp, 0, stream)) != hipSuccess) return e;
-    k_sort_scatter<true><<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), sc.vals[cur].as<uint32_t>(),
-                                                       sc.keys[cur ^ 1].as<uint64_t>(), sc.vals[cur ^ 1].as<uint32_t>(),
-                                                       n, shift, sc.hist.as<uint32_t>(), tiles);
+  } else {
+    k_init_bounds<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>());
+    // Runs of 64 x steps primitives a wave: at least 2,048 waves, at most 64 steps a run.
+    const uint32_t steps = std::max(1u, std::min(kBoundsMaxSteps, n / (64u * 2048u)));
+    const uint32_t bound_waves = blocks(n, 64u * steps);
+    k_centroid_bounds<<<blocks(uint64_t(bound_waves) * 64, 256), 256, 0, stream>>>(n, steps, boxes, d_seg_of,
+                                                                                    sc.seg_cbox.as<uint32_t>());
+    k_segment_scale<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>(), sc.seg_scale.as<float>());
+    k_morton<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, d_seg_of, sc.seg_scale.as<float>(), sc.keys[0].as<uint64_t>(),
+                                                 sc.vals[0].as<uint32_t>());
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    cur ^= 1;
+    uint32_t seg_bits = 0;
+    while ((1u << seg_bits) < nseg) ++seg_bits;
+    const int key_bits = 30 + int(seg_bits);
+    for (int shift = 0; shift < key_bits; shift += 8) {
+      k_sort_hist<<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), n, shift, sc.hist.as<uint32_t>(), tiles);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = exclusive_scan(sc.hist.as<uint32_t>(), tiles * 256, sc.scan_tmp, 0, stream)) != hipSuccess) return e;
+      k_sort_scatter<true><<<tiles, kSortThreads, 0, stream>>>(sc.keys[cur].as<uint64_t>(), sc.vals[cur].as<uint32_t>(),
+                                                         sc.keys[cur ^ 1].as<uint64_t>(), sc.vals[cur ^ 1].as<uint32_t>(),
+                                                         n, shift, sc.hist.as<uint32_t>(), tiles);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      cur ^= 1;
+    }
   }
   const uint64_t* sorted_keys = sc.keys[cur].as<uint64_t>();
   const uint32_t* sorted_vals = sc.vals[cur].as<uint32_t>();
@@ -1318,8 +1607,10 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   ta.flag = flag;
   k_karras<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, d_seg_of, d_seg_first, d_seg_count, d_node_base, nodes,
                                                sc.node_parent.as<uint32_t>(), ta);
-  k_codes<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, morton);
-  if ((e = hipMemcpyAsync(perm, sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
+  if (!small) {
+    k_codes<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, morton);
+    if ((e = hipMemcpyAsync(perm, sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
+  }
   ta.perm = sorted_vals;
   return tile_fit(stream, lad);
 }
@@ -1363,6 +1654,26 @@ hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32
   k_triangle_records<<<ncl, 128, 0, stream>>>(s.clusters.as<ge_cluster_desc>(), s.vertices.as<float>(),
                                                s.triangles.as<uint32_t>(), cluster_list, cluster_out, cluster_seg,
                                                cluster_local, tris, boxes, seg_of);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_meshes(hipStream_t stream, const SceneDev& s, uint32_t nseg, const BlockSeg* segs,
+                               const uint32_t* lists, ge_bvh_node* nodes, ge_bvh_triangle* tris, uint32_t* morton,
+                               float* roots, uint32_t* heights, WideNode* wide) {
+  if (nseg == 0) return hipSuccess;
+  BlockArgs ba{};
+  ba.segs = segs;
+  ba.lists = lists;
+  ba.clusters = s.clusters.as<ge_cluster_desc>();
+  ba.vertices = s.vertices.as<float>();
+  ba.triangles = s.triangles.as<uint32_t>();
+  ba.tris = tris;
+  ba.nodes = nodes;
+  ba.morton = morton;
+  ba.roots = roots;
+  ba.heights = heights;
+  ba.wide = wide;
+  k_block_build<true><<<nseg, kBlockThreads, 0, stream>>>(ba);
   return hipGetLastError();
 }
 

@@ -43,6 +43,16 @@ struct BuildScratch {
   DevBuf flag;                    // the build's refusal flag (a tile over capacity, too deep)
 };
 
+// A one-workgroup build over triangles (at most k_block_build_max): its `count` triangles are those
+// of clusters first .. first + ncl - 1 (list 0: a mesh's own) or lists[first .. first + ncl - 1]
+// (list 1: a frame structure's), in that order; cbase is the mesh's first cluster (a record's
+// cluster is numbered within its mesh). Its nodes go to nodes[node ..], its sorted triangle
+// records and codes to tris[out ..] and morton[out ..], its root box and height to roots[6 root ..]
+// and heights[root].
+struct BlockSeg {
+  uint32_t count, node, out, root, first, ncl, cbase, list;
+};
+
 struct SceneDev {
   // The engine's arrays, as uploaded.
   DevBuf clusters, vertices, quantized, attributes, triangles, meshes, instances, materials,
@@ -211,7 +221,8 @@ struct ge_rt_ctx {
   ge::DevBuf tmp[9];   // build batches: metadata, records, boxes, segments, order, codes, instances
   // A build's host arrays, copied to the device asynchronously: kept until the next build, which
   // starts after the last one's closing synchronization.
-  std::vector<uint32_t> host_meta, host_tmeta;
+  std::vector<uint32_t> host_meta, host_tmeta, host_blists;
+  std::vector<ge::BlockSeg> host_bsegs;
   // Pinned host memory for the build's end: the refusal flag, the instance structure's height and
   // every mesh's height, read back with one synchronization.
   uint32_t* host_readback = nullptr;
@@ -267,6 +278,10 @@ hipError_t build_segments(BuildScratch& scratch, hipStream_t stream, uint32_t n,
 hipError_t refit_segment(BuildScratch& scratch, hipStream_t stream, uint32_t n, const float* slot_boxes,
                          const uint32_t* d_seg_tables, ge_bvh_node* nodes, WideNode* wide, float* root,
                          uint32_t* height, uint32_t* flag);
+// Builds the BlockSeg segments, one workgroup each, in one launch.
+hipError_t launch_block_meshes(hipStream_t stream, const SceneDev& s, uint32_t nseg, const BlockSeg* segs,
+                               const uint32_t* lists, ge_bvh_node* nodes, ge_bvh_triangle* tris, uint32_t* morton,
+                               float* roots, uint32_t* heights, WideNode* wide);
 // Triangle records, boxes and segment ids of the clusters in `cluster_list` (cluster k's triangles
 // at batch index cluster_out[k] on; cluster_local[k] its index within its mesh).
 hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,
