@@ -865,51 +865,40 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
         segs.push_back({s.frame_tri_count[k], s.frame_node_base[k], s.frame_tri_base[k], s.mesh_count + k,
                         s.h_instances[s.frame_inst[k]].mesh, &s.frame_clusters[k]});
   // Structures too large for a one-workgroup build take the device-wide passes over a batch of
-  // triangle records (their clusters listed here); the others are built one workgroup each, each
-  // reading its own clusters, from a table of segments.
-  uint32_t nseg_global = 0, n = 0;
-  std::vector<uint32_t> seg_first, seg_count, seg_node, seg_out, seg_root;
-  std::vector<uint32_t> cl_list, cl_out, cl_seg, cl_local;
+  // triangle records; the others are built one workgroup each. Either kind reads its clusters on
+  // the device from a table of segments (a mesh's own clusters, or a frame structure's list).
+  uint32_t nseg_global = 0, n = 0, ncl_global = 0;
+  std::vector<uint32_t> seg_first, seg_count, seg_node, seg_out, seg_root, seg_pre;
+  std::vector<GlobalSeg> gsegs;
   std::vector<BlockSeg>& bsegs = ctx->host_bsegs;
-  std::vector<uint32_t>& blists = ctx->host_blists;
+  std::vector<uint32_t>& lists = ctx->host_blists;
   bsegs.clear();
-  blists.clear();
+  lists.clear();
   for (const Seg& g : segs) {
     const ge_mesh_desc& mesh = s.h_meshes[g.mesh];
+    uint32_t first = mesh.first_cluster, ncl = mesh.cluster_count, is_list = 0;
+    if (g.clusters) {
+      first = uint32_t(lists.size());
+      ncl = uint32_t(g.clusters->size());
+      is_list = 1;
+      lists.insert(lists.end(), g.clusters->begin(), g.clusters->end());
+    }
     if (g.count <= k_block_build_max) {
-      BlockSeg b{g.count, g.node, g.out, g.root, mesh.first_cluster, mesh.cluster_count, mesh.first_cluster, 0u};
-      if (g.clusters) {
-        b.first = uint32_t(blists.size());
-        b.ncl = uint32_t(g.clusters->size());
-        b.list = 1;
-        blists.insert(blists.end(), g.clusters->begin(), g.clusters->end());
-      }
-      bsegs.push_back(b);
+      bsegs.push_back({g.count, g.node, g.out, g.root, first, ncl, mesh.first_cluster, is_list});
       continue;
     }
-    const uint32_t sidx = nseg_global++;
+    ++nseg_global;
     seg_first.push_back(n);
     seg_count.push_back(g.count);
     seg_node.push_back(g.node);
     seg_out.push_back(g.out);
     seg_root.push_back(g.root);
-    uint32_t off = n;
-    auto add = [&](uint32_t gc) {
-      const uint32_t tc = s.h_clusters[gc].triangle_count;
-      if (tc == 0) return;
-      cl_list.push_back(gc);
-      cl_out.push_back(off);
-      cl_seg.push_back(sidx);
-      cl_local.push_back(gc - mesh.first_cluster);
-      off += tc;
-    };
-    if (g.clusters) {
-      for (uint32_t gc : *g.clusters) add(gc);
-    } else {
-      for (uint32_t c = 0; c < mesh.cluster_count; ++c) add(mesh.first_cluster + c);
-    }
-    n = off;
+    seg_pre.push_back(ncl_global);
+    gsegs.push_back({first, is_list, mesh.first_cluster, 0u});
+    ncl_global += ncl;
+    n += g.count;
   }
+  seg_pre.push_back(ncl_global);
   std::vector<uint32_t>& meta = ctx->host_meta;  // every small array of the batch, packed, one upload
   meta.clear();
   auto pack_words = [&](const uint32_t* p, size_t count) {
@@ -919,14 +908,15 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
     return at;
   };
   auto pack = [&](const std::vector<uint32_t>& v) { return pack_words(v.data(), v.size()); };
-  static_assert(sizeof(BlockSeg) == 32, "eight words a segment");
+  static_assert(sizeof(BlockSeg) == 32 && sizeof(GlobalSeg) == 16, "whole words a segment");
   if (n > 0 || !bsegs.empty()) {
     const size_t o_sf = pack(seg_first), o_sc = pack(seg_count), o_sn = pack(seg_node), o_so = pack(seg_out),
-                 o_sr = pack(seg_root), o_cl = pack(cl_list), o_co = pack(cl_out), o_cs = pack(cl_seg), o_cc = pack(cl_local),
-                 o_bs = pack_words(reinterpret_cast<const uint32_t*>(bsegs.data()), bsegs.size() * 8), o_bl = pack(blists);
+                 o_sr = pack(seg_root), o_sp = pack(seg_pre),
+                 o_gs = pack_words(reinterpret_cast<const uint32_t*>(gsegs.data()), gsegs.size() * 4),
+                 o_bs = pack_words(reinterpret_cast<const uint32_t*>(bsegs.data()), bsegs.size() * 8), o_li = pack(lists);
     GE_HIP(ctx, staged_upload(ctx, ctx->tmp[0], meta.data(), meta.size(), st));
     const uint32_t* md = ctx->tmp[0].as<uint32_t>();
-    GE_HIP(ctx, launch_block_meshes(st, s, uint32_t(bsegs.size()), reinterpret_cast<const BlockSeg*>(md + o_bs), md + o_bl,
+    GE_HIP(ctx, launch_block_meshes(st, s, uint32_t(bsegs.size()), reinterpret_cast<const BlockSeg*>(md + o_bs), md + o_li,
                                     s.blas_nodes.as<ge_bvh_node>(), s.blas_tris.as<ge_bvh_triangle>(),
                                     s.blas_morton.as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(),
                                     s.blas_wide.as<WideNode>()));
@@ -936,13 +926,17 @@ int32_t ge_rt_bvh_build(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t flags, cons
       GE_HIP(ctx, ctx->tmp[3].reserve(size_t(n) * 4));                        // segment of each primitive
       GE_HIP(ctx, ctx->tmp[4].reserve(size_t(n) * 4));                        // sorted order
       GE_HIP(ctx, ctx->tmp[5].reserve(size_t(n) * 4));                        // sorted codes
-      GE_HIP(ctx, launch_triangle_records(st, s, uint32_t(cl_list.size()), md + o_cl, md + o_co, md + o_cs, md + o_cc,
+      // The scan scratch at the size the sort's scan needs later, so no reserve frees it under the
+      // cluster scan queued before.
+      GE_HIP(ctx, ctx->build.scan_tmp.reserve(size_t(n) / 1024 + size_t(ncl_global) / 256 + 65536 + 256));
+      GE_HIP(ctx, launch_triangle_records(st, s, ctx->build, nseg_global, ncl_global, md + o_sp,
+                                          reinterpret_cast<const GlobalSeg*>(md + o_gs), md + o_li,
                                           ctx->tmp[1].as<ge_bvh_triangle>(), ctx->tmp[2].as<float>(), ctx->tmp[3].as<uint32_t>()));
       GE_HIP(ctx, build_segments(ctx->build, st, n, ctx->tmp[2].as<float>(), nseg_global, ctx->tmp[3].as<uint32_t>(),
                                  md + o_sf, md + o_sc, md + o_sn, s.blas_nodes.as<ge_bvh_node>(), ctx->tmp[4].as<uint32_t>(),
                                  ctx->tmp[5].as<uint32_t>(), s.mesh_roots.as<float>(), s.mesh_height.as<uint32_t>(), md + o_sr,
                                  uint32_t(node_all), s.blas_wide.as<WideNode>(), nseg_global, n,
-                                 seg_count.data(), flag));
+                                 seg_count.data(), flag, ncl_global));
       GE_HIP(ctx, launch_gather_triangles(st, n, ctx->tmp[4].as<uint32_t>(), ctx->tmp[3].as<uint32_t>(), md + o_sf, md + o_so,
                                           ctx->tmp[1].as<ge_bvh_triangle>(), s.blas_tris.as<ge_bvh_triangle>(),
                                           ctx->tmp[5].as<uint32_t>(), s.blas_morton.as<uint32_t>()));

@@ -61,10 +61,14 @@ __global__ void k_init_bounds(uint32_t nseg, uint32_t* cbox) {
 // per-lane running bounds while its lanes stay in one segment (segments are contiguous), so a
 // segment costs one wave reduction and six atomics per run instead of per primitive: same-address
 // atomics serialize, and one segment can hold tens of millions of primitives. `steps` grows with
-// the batch (bounds_steps) so that small batches still spread over many waves.
+// the batch (bounds_steps) so that small batches still spread over many waves. With Partials the
+// items are a batch's clusters, each carrying the lowest and highest ordered centroids of its
+// triangles (reduced by the records pass): the same bounds from a 1/128th of the reads.
 constexpr uint32_t kBoundsMaxSteps = 64;
 
+template <bool Partials>
 __global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, uint32_t steps, const float* __restrict__ boxes,
+                                                         const uint32_t* __restrict__ part,
                                                          const uint32_t* __restrict__ seg_of, uint32_t* cbox) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64u * steps;
@@ -93,9 +97,16 @@ __global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, uint32_t st
     const bool valid = i < n;
     if (!__any(valid)) break;
     const uint32_t s = valid ? seg_of[i] : 0xffffffffu;
-    uint32_t c[3];
-    for (int a = 0; a < 3; ++a)
-      c[a] = valid ? ordered((boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f) : 0u;
+    // An item's (lowest, highest) ordered centroid: a primitive's own, or a cluster's partial bounds.
+    uint32_t clo[3], chi[3];
+    for (int a = 0; a < 3; ++a) {
+      if (Partials) {
+        clo[a] = valid ? part[size_t(i) * 6 + a] : 0xffffffffu;
+        chi[a] = valid ? part[size_t(i) * 6 + 3 + a] : 0u;
+      } else {
+        clo[a] = chi[a] = valid ? ordered((boxes[size_t(i) * 6 + a] + boxes[size_t(i) * 6 + 3 + a]) * 0.5f) : 0u;
+      }
+    }
     const uint32_t s0 = uint32_t(__shfl(int(s), 0));
     if (__all(!valid || s == s0)) {
       if (s0 != cur) {
@@ -104,8 +115,8 @@ __global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, uint32_t st
       }
       if (valid) {
         for (int a = 0; a < 3; ++a) {
-          lo[a] = min(lo[a], c[a]);
-          hi[a] = max(hi[a], c[a]);
+          lo[a] = min(lo[a], clo[a]);
+          hi[a] = max(hi[a], chi[a]);
         }
       }
     } else {
@@ -114,8 +125,8 @@ __global__ void __launch_bounds__(256) k_centroid_bounds(uint32_t n, uint32_t st
       cur = 0xffffffffu;
       if (valid) {
         for (int a = 0; a < 3; ++a) {
-          atomicMin(&cbox[s * 6 + a], c[a]);
-          atomicMax(&cbox[s * 6 + 3 + a], c[a]);
+          atomicMin(&cbox[s * 6 + a], clo[a]);
+          atomicMax(&cbox[s * 6 + 3 + a], chi[a]);
         }
       }
     }
@@ -971,31 +982,85 @@ __device__ __forceinline__ void fetch_corners(const ge_cluster_desc& cl, const f
   }
 }
 
-__global__ void k_triangle_records(const ge_cluster_desc* __restrict__ clusters, const float* __restrict__ vertices,
-                                   const uint32_t* __restrict__ triangles, const uint32_t* __restrict__ cluster_list,
-                                   const uint32_t* __restrict__ cluster_out, const uint32_t* __restrict__ cluster_seg,
-                                   const uint32_t* __restrict__ cluster_local, ge_bvh_triangle* tris, float* boxes,
-                                   uint32_t* seg_of) {
-  const uint32_t k = blockIdx.x;
-  const uint32_t c = cluster_list[k];
-  const ge_cluster_desc cl = clusters[c];
-  const uint32_t out0 = cluster_out[k];
-  const uint32_t local_cluster = cluster_local[k];
-  const uint32_t s = cluster_seg[k];
+// The device-wide batch's clusters: segment s owns items [pre[s], pre[s + 1]) of the batch's cluster
+// sequence, in order.
+__device__ __forceinline__ uint32_t item_segment(const uint32_t* __restrict__ pre, uint32_t nseg, uint32_t b) {
+  uint32_t lo = 0, hi = nseg - 1;  // the last segment whose first item is at or before b
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t item_cluster(const GlobalSeg& g, const uint32_t* __restrict__ lists, uint32_t j) {
+  return g.list ? lists[g.first + j] : g.first + j;
+}
+
+__global__ void k_cluster_counts(uint32_t ncl, uint32_t nseg, const uint32_t* __restrict__ pre,
+                                 const GlobalSeg* __restrict__ segs, const uint32_t* __restrict__ lists,
+                                 const ge_cluster_desc* __restrict__ clusters, uint32_t* counts) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ncl) return;
+  const uint32_t s = item_segment(pre, nseg, b);
+  counts[b] = clusters[item_cluster(segs[s], lists, b - pre[s])].triangle_count;
+}
+
+// Item b's triangles (a workgroup a cluster) at batch index out[b] on (the scan of the counts): each
+// one's record, box and segment, and the cluster's lowest and highest ordered centroids.
+__global__ void __launch_bounds__(128) k_triangle_records(uint32_t nseg, const uint32_t* __restrict__ pre,
+                                                          const GlobalSeg* __restrict__ segs,
+                                                          const uint32_t* __restrict__ lists,
+                                                          const ge_cluster_desc* __restrict__ clusters,
+                                                          const float* __restrict__ vertices,
+                                                          const uint32_t* __restrict__ triangles,
+                                                          const uint32_t* __restrict__ out, ge_bvh_triangle* tris,
+                                                          float* boxes, uint32_t* seg_of, uint32_t* part,
+                                                          uint32_t* item_seg) {
+  __shared__ uint32_t red[2][6];
+  const uint32_t b = blockIdx.x;
+  const uint32_t s = item_segment(pre, nseg, b);
+  const GlobalSeg g = segs[s];
+  const uint32_t gc = item_cluster(g, lists, b - pre[s]);
+  const ge_cluster_desc cl = clusters[gc];
+  const uint32_t out0 = out[b];
+  uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   for (uint32_t t = threadIdx.x; t < cl.triangle_count; t += blockDim.x) {
     ge_bvh_triangle tri;
     fetch_corners(cl, vertices, triangles, t, tri);
     const uint32_t o = out0 + t;
-    tri.prim = (local_cluster << 8) | t;
+    tri.prim = ((gc - g.cbase) << 8) | t;
     tri.index = o;  // rewritten to the mesh-local index by the gather
     tri.pad = 0;
     tris[o] = tri;
     for (int a = 0; a < 3; ++a) {
-      boxes[size_t(o) * 6 + a] = __builtin_fminf(__builtin_fminf(tri.v0[a], tri.v1[a]), tri.v2[a]);
-      boxes[size_t(o) * 6 + 3 + a] = __builtin_fmaxf(__builtin_fmaxf(tri.v0[a], tri.v1[a]), tri.v2[a]);
+      const float blo = __builtin_fminf(__builtin_fminf(tri.v0[a], tri.v1[a]), tri.v2[a]);
+      const float bhi = __builtin_fmaxf(__builtin_fmaxf(tri.v0[a], tri.v1[a]), tri.v2[a]);
+      boxes[size_t(o) * 6 + a] = blo;
+      boxes[size_t(o) * 6 + 3 + a] = bhi;
+      const uint32_t c = ordered((blo + bhi) * 0.5f);
+      lo[a] = min(lo[a], c);
+      hi[a] = max(hi[a], c);
     }
     seg_of[o] = s;
   }
+  for (int a = 0; a < 3; ++a)
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[a] = min(lo[a], uint32_t(__shfl_xor(lo[a], off)));
+      hi[a] = max(hi[a], uint32_t(__shfl_xor(hi[a], off)));
+    }
+  const uint32_t w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0)
+    for (int a = 0; a < 3; ++a) {
+      red[w][a] = lo[a];
+      red[w][3 + a] = hi[a];
+    }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t x = red[0][threadIdx.x], y = red[1][threadIdx.x];
+    part[size_t(b) * 6 + threadIdx.x] = threadIdx.x < 3 ? min(x, y) : max(x, y);
+  }
+  if (threadIdx.x == 0) item_seg[b] = s;
 }
 
 __global__ void k_gather_triangles(uint32_t n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ seg_of,
@@ -1518,7 +1583,7 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
                           const uint32_t* d_node_base, ge_bvh_node* nodes, uint32_t* perm, uint32_t* morton,
                           float* roots, uint32_t* heights, const uint32_t* d_root_index, uint32_t node_pool,
                           WideNode* wide, uint32_t nseg_global, uint32_t n_global, const uint32_t* h_seg_count,
-                          uint32_t* flag) {
+                          uint32_t* flag, uint32_t nparts) {
   if (n == 0) return hipSuccess;
   hipError_t e;
   // Segments [nseg_global, nseg) fit a workgroup's LDS: one launch builds them all.
@@ -1565,11 +1630,16 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
     if ((e = hipGetLastError()) != hipSuccess) return e;
   } else {
     k_init_bounds<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>());
-    // Runs of 64 x steps primitives a wave: at least 2,048 waves, at most 64 steps a run.
-    const uint32_t steps = std::max(1u, std::min(kBoundsMaxSteps, n / (64u * 2048u)));
-    const uint32_t bound_waves = blocks(n, 64u * steps);
-    k_centroid_bounds<<<blocks(uint64_t(bound_waves) * 64, 256), 256, 0, stream>>>(n, steps, boxes, d_seg_of,
-                                                                                    sc.seg_cbox.as<uint32_t>());
+    // Runs of 64 x steps items a wave: at least 2,048 waves, at most 64 steps a run.
+    const uint32_t items = nparts ? nparts : n;
+    const uint32_t steps = std::max(1u, std::min(kBoundsMaxSteps, items / (64u * 2048u)));
+    const uint32_t bound_grid = blocks(uint64_t(blocks(items, 64u * steps)) * 64, 256);
+    if (nparts)
+      k_centroid_bounds<true><<<bound_grid, 256, 0, stream>>>(nparts, steps, nullptr, sc.cl_part.as<uint32_t>(),
+                                                              sc.cl_seg.as<uint32_t>(), sc.seg_cbox.as<uint32_t>());
+    else
+      k_centroid_bounds<false><<<bound_grid, 256, 0, stream>>>(n, steps, boxes, nullptr, d_seg_of,
+                                                               sc.seg_cbox.as<uint32_t>());
     k_segment_scale<<<blocks(nseg, 256), 256, 0, stream>>>(nseg, sc.seg_cbox.as<uint32_t>(), sc.seg_scale.as<float>());
     k_morton<<<blocks(n, 256), 256, 0, stream>>>(n, boxes, d_seg_of, sc.seg_scale.as<float>(), sc.keys[0].as<uint64_t>(),
                                                  sc.vals[0].as<uint32_t>());
@@ -1647,13 +1717,22 @@ hipError_t refit_segment(BuildScratch& sc, hipStream_t stream, uint32_t n, const
   return tile_fit(stream, lad);
 }
 
-hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,
-                                   const uint32_t* cluster_out, const uint32_t* cluster_seg, const uint32_t* cluster_local,
+hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, BuildScratch& sc, uint32_t nseg,
+                                   uint32_t ncl, const uint32_t* pre, const GlobalSeg* segs, const uint32_t* lists,
                                    ge_bvh_triangle* tris, float* boxes, uint32_t* seg_of) {
   if (ncl == 0) return hipSuccess;
-  k_triangle_records<<<ncl, 128, 0, stream>>>(s.clusters.as<ge_cluster_desc>(), s.vertices.as<float>(),
-                                               s.triangles.as<uint32_t>(), cluster_list, cluster_out, cluster_seg,
-                                               cluster_local, tris, boxes, seg_of);
+  hipError_t e;
+  if ((e = sc.cl_out.reserve(size_t(ncl) * 4)) != hipSuccess) return e;
+  if ((e = sc.cl_part.reserve(size_t(ncl) * 24)) != hipSuccess) return e;
+  if ((e = sc.cl_seg.reserve(size_t(ncl) * 4)) != hipSuccess) return e;
+  if ((e = sc.scan_tmp.reserve(size_t(ncl) / 256 + 65536)) != hipSuccess) return e;
+  const ge_cluster_desc* clusters = s.clusters.as<ge_cluster_desc>();
+  k_cluster_counts<<<blocks(ncl, 256), 256, 0, stream>>>(ncl, nseg, pre, segs, lists, clusters, sc.cl_out.as<uint32_t>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = exclusive_scan(sc.cl_out.as<uint32_t>(), ncl, sc.scan_tmp, 0, stream)) != hipSuccess) return e;
+  k_triangle_records<<<ncl, 128, 0, stream>>>(nseg, pre, segs, lists, clusters, s.vertices.as<float>(),
+                                              s.triangles.as<uint32_t>(), sc.cl_out.as<uint32_t>(), tris, boxes, seg_of,
+                                              sc.cl_part.as<uint32_t>(), sc.cl_seg.as<uint32_t>());
   return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ struct BuildScratch {
   DevBuf depth;                   // per node of the device-wide build, for the levels above the first
   DevBuf buckets, bucket_count;   // the nodes each tile hands up to the next level
   DevBuf flag;                    // the build's refusal flag (a tile over capacity, too deep)
+  DevBuf cl_out, cl_part, cl_seg; // the device-wide batch's clusters: first triangle, centroid bounds, segment
 };
 
 // A one-workgroup build over triangles (at most k_block_build_max): its `count` triangles are those
@@ -265,13 +266,15 @@ namespace ge {
 // [0, nseg_global), primitives [0, n_global), take the device-wide passes; h_seg_count holds their
 // counts on the host. A structure the build refuses (deeper than 254 levels, or too unbalanced for
 // a tile's capacity) sets the device word `flag`, which the caller clears before and reads after.
+// With nparts, the device-wide segments' centroid bounds come from the per-cluster bounds that
+// launch_triangle_records (over nparts clusters) left in the scratch, not from the boxes.
 constexpr uint32_t k_block_build_max = 1024;
 hipError_t build_segments(BuildScratch& scratch, hipStream_t stream, uint32_t n, const float* boxes,
                           uint32_t nseg, const uint32_t* d_seg_of, const uint32_t* d_seg_first,
                           const uint32_t* d_seg_count, const uint32_t* d_node_base, ge_bvh_node* nodes,
                           uint32_t* perm, uint32_t* morton, float* roots, uint32_t* heights,
                           const uint32_t* d_root_index, uint32_t node_pool, WideNode* wide, uint32_t nseg_global,
-                          uint32_t n_global, const uint32_t* h_seg_count, uint32_t* flag);
+                          uint32_t n_global, const uint32_t* h_seg_count, uint32_t* flag, uint32_t nparts = 0);
 // The refit of one structure of n >= 2 leaves (GE_RT_BUILD_REFIT): its records' topology kept,
 // the leaves' boxes given in slot order (6 floats each), every node's box, the records, the wide
 // nodes and the root's box and height recomputed. d_seg_tables holds {0, n, 0} on the device.
@@ -282,12 +285,18 @@ hipError_t refit_segment(BuildScratch& scratch, hipStream_t stream, uint32_t n, 
 hipError_t launch_block_meshes(hipStream_t stream, const SceneDev& s, uint32_t nseg, const BlockSeg* segs,
                                const uint32_t* lists, ge_bvh_node* nodes, ge_bvh_triangle* tris, uint32_t* morton,
                                float* roots, uint32_t* heights, WideNode* wide);
-// Triangle records, boxes and segment ids of the clusters in `cluster_list` (cluster k's triangles
-// at batch index cluster_out[k] on; cluster_local[k] its index within its mesh).
-hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, uint32_t ncl, const uint32_t* cluster_list,
-                                   const uint32_t* cluster_out, const uint32_t* cluster_seg,
-                                   const uint32_t* cluster_local, ge_bvh_triangle* tris, float* boxes,
-                                   uint32_t* seg_of);
+// The device-wide batch's segments over triangles: segment s's clusters are clusters first + j (list
+// 0: a mesh's own) or lists[first + j] (list 1: a frame structure's), j < pre[s + 1] - pre[s], its
+// triangles at batch index pre-scan on; cbase is its mesh's first cluster.
+struct GlobalSeg {
+  uint32_t first, list, cbase, pad;
+};
+// Triangle records, boxes and segment ids of the batch's ncl clusters (nseg segments, `pre` their
+// first clusters, nseg + 1 entries), and each cluster's centroid bounds and segment in scratch for
+// build_segments (nparts = ncl).
+hipError_t launch_triangle_records(hipStream_t stream, const SceneDev& s, BuildScratch& scratch, uint32_t nseg,
+                                   uint32_t ncl, const uint32_t* pre, const GlobalSeg* segs, const uint32_t* lists,
+                                   ge_bvh_triangle* tris, float* boxes, uint32_t* seg_of);
 hipError_t launch_gather_triangles(hipStream_t stream, uint32_t n, const uint32_t* perm, const uint32_t* seg_of,
                                    const uint32_t* seg_first, const uint32_t* seg_out, const ge_bvh_triangle* in,
                                    ge_bvh_triangle* out, const uint32_t* morton_in, uint32_t* morton_out);
