@@ -478,6 +478,7 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
                     &ctx->wave.sgrad, &ctx->wave.ts, &ctx->wave.adj_a, &ctx->wave.adj_r, &ctx->wave.lpath})
     b->release();
   if (ctx->wave.host_counts) (void)hipHostFree(ctx->wave.host_counts);
+  if (ctx->wave.counted) (void)hipEventDestroy(ctx->wave.counted);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->bev0) (void)hipEventDestroy(ctx->bev0);

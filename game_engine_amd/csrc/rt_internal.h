@@ -189,6 +189,7 @@ struct Wavefront {
   DevBuf lpath;               // per path (the adjoint): the primal pass's radiance
   uint64_t shadow_capacity = 0;
   uint32_t* host_counts = nullptr;  // pinned
+  hipEvent_t counted = nullptr;     // the shading's counts have reached host_counts
 };
 
 // The path-replay adjoint on the wavefront (wavefront_path_trace's `adjoint`): directions, lobe

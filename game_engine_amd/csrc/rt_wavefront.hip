@@ -113,6 +113,7 @@ struct ClosestJob {
   uint32_t* head;
   PathHit* hits;
   uint32_t alpha;  // the masked-candidate rule of the bounce
+  const uint32_t* count = nullptr;  // set: the rays are the first min(n, *count), read on the device
 };
 
 template <bool Count, bool Masks>
@@ -587,6 +588,7 @@ struct AnyJob {
   uint32_t* head;
   uint8_t* blocked;
   uint32_t alpha;  // the masked-candidate rule of light rays
+  const uint32_t* count = nullptr;  // set: the records are the first min(n, *count), read on the device
 };
 
 template <bool Count, bool Masks>
@@ -615,6 +617,10 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_walks(Bvh bvh, ClosestJob c
                                                           unsigned long long* counters) {
   Tally tally;
   uint64_t closest_issued = 0, any_issued = 0;
+  if (closest.count) closest.n = min(closest.n, *closest.count);
+  if (any.count) any.n = min(any.n, *any.count);
+  // A wave past the rays' count would find the counters spent: it leaves without contending for them.
+  if (uint64_t(blockIdx.x) * 64u >= uint64_t(closest.n) + any.n) return;
   if (closest.n) closest_walks<Count, Masks>(bvh, closest, closest_issued, tally);
   if (any.n) any_walks<Count, Masks>(bvh, any, any_issued, tally);
   if (Count) flush_counters(counters, closest_issued, any_issued, tally);
@@ -776,6 +782,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   if ((e = w.queue[1].reserve(size_t(npaths) * 4)) != hipSuccess) return e;
   if ((e = w.counts.reserve(256)) != hipSuccess) return e;
   if (!w.host_counts && (e = hipHostMalloc(reinterpret_cast<void**>(&w.host_counts), 64)) != hipSuccess) return e;
+  if (!w.counted && (e = hipEventCreateWithFlags(&w.counted, hipEventDisableTiming)) != hipSuccess) return e;
   const bool grad = adjoint && adjoint->grad;
   if (adjoint) {
     if ((e = w.ts.reserve(size_t(npaths) * 32)) != hipSuccess) return e;  // two buffers, as the state
@@ -821,27 +828,43 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     PathState* state = nullptr;
     const uint32_t* queue = nullptr;
   } pending;
+  auto alpha_of = [&](uint32_t bounce) {
+    return !masked ? k_alpha_opaque
+                   : ((p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : (bounce == 0 ? k_alpha_cull : k_alpha_opaque));
+  };
+  // One walk launch: closest-hit rays of `cj`, shadow records of `aj` (the walk heads cleared first).
+  auto launch_walks = [&](const ClosestJob& cj, const AnyJob& aj, bool mask_walk, uint32_t grid) -> hipError_t {
+    hipError_t r;
+    if ((r = hipMemsetAsync(counts + 4, 0, 8, st)) != hipSuccess) return r;
+    if ((r = stage_mark(ctx, 0, 0)) != hipSuccess) return r;
+    if (mask_walk) {
+      if (counters) k_walks<true, true><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
+      else k_walks<false, true><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
+    } else {
+      if (counters) k_walks<true, false><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
+      else k_walks<false, false><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
+    }
+    if ((r = hipGetLastError()) != hipSuccess) return r;
+    return stage_mark(ctx, 0, 1);
+  };
+  // Plain tracing enqueues the next bounce's walks before it waits for the shading's counts (they
+  // read the counts on the device), so the host's round trip hides behind the walk; a shading that
+  // overflows its shadow records runs again, and so does that walk. Counted walks, the cache's and
+  // the adjoint's passes wait first, as a walk that ran twice would count twice.
+  const bool ahead = !counters && !hook && !adjoint;
+  bool walked = false;        // this bounce's walks were enqueued ahead
+  uint32_t last_shadows = 0;  // the previous bounce's shadow records: the size of the walk ahead's grid
   for (uint32_t bounce = 0; n > 0 || pending.live; ++bounce) {
-    const uint32_t alpha = !masked ? k_alpha_opaque
-                                   : ((p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : (bounce == 0 ? k_alpha_cull : k_alpha_opaque));
-    if ((e = hipMemsetAsync(c2, 0, 16, st)) != hipSuccess) return e;  // queue and shadow counts, walk heads
+    const uint32_t alpha = alpha_of(bounce);
     const uint64_t walk_rays = uint64_t(n) + pending.shadows;
-    if (walk_rays > 0) {
+    if (!walked && walk_rays > 0) {
       ClosestJob cj{w.state[cur].as<PathState>(), w.queue[0].as<uint32_t>(), n, counts + 4, w.hits.as<PathHit>(), alpha};
       AnyJob aj{w.shadow.as<ShadowRec>(), pending.shadows, counts + 5, w.blocked.as<uint8_t>(), any_alpha};
       const bool mask_walk = (n > 0 && alpha != k_alpha_opaque) || (pending.shadows > 0 && any_alpha != k_alpha_opaque);
-      const uint32_t grid = std::min(blocks(walk_rays, 64), ctx->persistent_blocks);
-      if ((e = stage_mark(ctx, 0, 0)) != hipSuccess) return e;
-      if (mask_walk) {
-        if (counters) k_walks<true, true><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
-        else k_walks<false, true><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
-      } else {
-        if (counters) k_walks<true, false><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
-        else k_walks<false, false><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
-      }
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-      if ((e = stage_mark(ctx, 0, 1)) != hipSuccess) return e;
+      if ((e = launch_walks(cj, aj, mask_walk, std::min(blocks(walk_rays, 64), ctx->persistent_blocks))) != hipSuccess)
+        return e;
     }
+    walked = false;
     if (pending.live) {
       if (grad) {
         ResolveAdjoint ra;
@@ -910,6 +933,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     }
     uint32_t nshadow = 0, next = 0;
     a.rerun = false;
+    if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;  // the next queue's and the shadow records' counts
     for (;;) {
       a.shadow = w.shadow.as<ShadowRec>();
       a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
@@ -929,7 +953,25 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       if ((e = hipGetLastError()) != hipSuccess) return e;
       // The next queue's and the shadow records' counts, and the cache's query and sample counts.
       if ((e = hipMemcpyAsync(w.host_counts, c2, 24, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      if (ahead) {
+        // The next bounce's closest-hit rays (at most this bounce's n) and this bounce's shadow
+        // records (at most the buffer's capacity), counted on the device. The host waits for the
+        // counts only, while the walk runs. The grid is sized by the previous bounce's shadow
+        // records (the first bounce: the capacity): a persistent walk covers every ray whatever
+        // its grid, but spare waves contend for the ray counter.
+        if ((e = hipEventRecord(w.counted, st)) != hipSuccess) return e;
+        const uint32_t alpha1 = alpha_of(bounce + 1);
+        ClosestJob cj{w.state[cur ^ 1].as<PathState>(), w.queue[1].as<uint32_t>(), n, counts + 4, w.hits.as<PathHit>(),
+                      alpha1, c2};
+        AnyJob aj{w.shadow.as<ShadowRec>(), a.shadow_capacity, counts + 5, w.blocked.as<uint8_t>(), any_alpha, c2 + 1};
+        const bool mask_walk = alpha1 != k_alpha_opaque || any_alpha != k_alpha_opaque;
+        const uint64_t guess = uint64_t(n) + (bounce == 0 ? a.shadow_capacity : last_shadows);
+        if ((e = launch_walks(cj, aj, mask_walk, std::min(blocks(guess, 64), ctx->persistent_blocks))) != hipSuccess) return e;
+        walked = true;
+        if ((e = hipEventSynchronize(w.counted)) != hipSuccess) return e;
+      } else if ((e = hipStreamSynchronize(st)) != hipSuccess) {
+        return e;
+      }
       next = w.host_counts[0];
       nshadow = w.host_counts[1];
       if (nshadow <= a.shadow_capacity) {
@@ -940,6 +982,8 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       // The records did not fit: grow the buffer and shade this bounce again (idempotent: the
       // launch reads one state buffer and writes the other), from the counts it started with: the
       // queue and shadow counts at zero and the cache's counts as the previous bounce left them.
+      // A walk enqueued ahead reads the buffers being replaced: it ends first.
+      if (ahead && (e = hipStreamSynchronize(st)) != hipSuccess) return e;
       if ((e = grow_shadow(uint64_t(nshadow) + nshadow / 4)) != hipSuccess) return e;
       if (grad && (e = w.sgrad.reserve(w.shadow_capacity * sizeof(ShadowGrad))) != hipSuccess) return e;
       if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;
@@ -948,6 +992,7 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       if ((e = hipMemcpyAsync(counts + 6, w.host_counts + 6, 8, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
       a.rerun = true;
     }
+    last_shadows = nshadow;
     pending.live = true;
     pending.n = n;
     pending.shadows = nshadow;
