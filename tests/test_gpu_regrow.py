@@ -61,6 +61,21 @@ def _fresh(desc):
     return ctx, ctx.upload(desc).build()
 
 
+def test_plain_tracer_regrows(scene):
+    """Plain tracing enqueues the next bounce's walks before it reads the shading's counts: after an
+    overflow both the shading and that walk run again. A fresh dispatch, a second one on the grown
+    buffer and a counted one (which waits before it walks) equal the oracle's to the bit."""
+    s, desc, lights, o, lo, hi = scene
+    pt = _frame(s)
+    oa, oc = o.path_trace(pt)
+    ctx, g = _fresh(desc)
+    for count in (False, False, True):
+        ga, gc = g.path_trace(pt, count=count)
+        assert np.array_equal(gc, oc)
+        assert ga.tobytes() == oa.tobytes()
+    g.close(), ctx.close()
+
+
 def test_cached_tracer_regrows(scene):
     s, desc, lights, o, lo, hi = scene
     pt = _frame(s)
