@@ -192,7 +192,11 @@ __global__ void __launch_bounds__(kSmallThreads) k_sort_small(uint32_t n, const 
   __shared__ uint32_t wsum[kSmallWaves];
   __shared__ uint32_t red[6];
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-  const uint32_t run = wave * 64u * kSmallItems;  // the wave's first element
+  // Slots in use: a wave's run is 64 x slots elements (padded past n), so a smaller segment takes
+  // fewer passes of each loop below (a uniform branch a slot).
+  const uint32_t slots = (n + kSmallThreads - 1) / kSmallThreads;
+  const uint32_t padded = slots * kSmallThreads;
+  const uint32_t run = wave * 64u * slots;  // the wave's first element
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const unsigned long long me = 1ull << lane;
 
@@ -227,7 +231,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_sort_small(uint32_t n, const 
     cscale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
   }
   // Codes, through LDS into the lanes' slots; past n, ~0 (above every 30-bit code: sorts last).
-  for (uint32_t i = t; i < kSmallMax; i += kSmallThreads) {
+  for (uint32_t i = t; i < padded; i += kSmallThreads) {
     uint32_t code = 0xffffffffu;
     if (i < n) {
       uint32_t q[3];
@@ -244,7 +248,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_sort_small(uint32_t n, const 
 #pragma unroll
   for (uint32_t s = 0; s < kSmallItems; ++s) {
     val[s] = run + s * 64u + lane;
-    key[s] = lkey[val[s]];
+    key[s] = s < slots ? lkey[val[s]] : 0u;
   }
   __syncthreads();  // lkey is rewritten by the first scatter
 #pragma unroll  // four passes, unrolled: the arrays are not carried through a loop as register tuples
@@ -252,7 +256,8 @@ __global__ void __launch_bounds__(kSmallThreads) k_sort_small(uint32_t n, const 
     for (uint32_t d = lane; d < 256; d += 64) cnt[wave][d] = 0;
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (uint32_t s = 0; s < kSmallItems; ++s) atomicAdd(&cnt[wave][(key[s] >> shift) & 255u], 1u);
+    for (uint32_t s = 0; s < kSmallItems; ++s)
+      if (s < slots) atomicAdd(&cnt[wave][(key[s] >> shift) & 255u], 1u);
     __syncthreads();
     // Places: the exclusive scan of the counts in (digit, wave) order; thread t holds digit t / 4,
     // waves 4 (t % 4) to 4 (t % 4) + 3.
@@ -283,6 +288,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_sort_small(uint32_t n, const 
     // lands before any lane reads the mask back, and the reads before the mask is cleared.
 #pragma unroll
     for (uint32_t s = 0; s < kSmallItems; ++s) {
+      if (s >= slots) break;
       const uint32_t k = key[s];
       const uint32_t d = (k >> shift) & 255u;
       atomicOr(&peer[wave][d], me);
@@ -301,6 +307,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_sort_small(uint32_t n, const 
     if (shift + 8 < 32) {
 #pragma unroll
       for (uint32_t s = 0; s < kSmallItems; ++s) {
+        if (s >= slots) break;
         key[s] = lkey[run + s * 64u + lane];
         val[s] = lval[run + s * 64u + lane];
       }
