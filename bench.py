@@ -31,7 +31,8 @@ against the FP64 oracle on a band of rows.
 
 time_to_equal_rmse: the other half of the metric on the same scene (see cache_measure): the
 configuration as stated (jittered camera rays, the sun and 64 point lights, RMSE against a 4096-spp
-plain render), and at pixel centres under the sun and sky.
+plain render), the same lights with pixel-centre camera rays (the scene's coherent primaries), and
+pixel centres under the sun and sky.
 """
 import argparse
 import json
@@ -871,9 +872,11 @@ def main():
         log(f"[bench] parity: {out['parity']}")
     if not args.no_c5:
         # The time-to-equal-RMSE half of the metric, on the same scene: as configured (jittered
-        # camera rays, the sun and 64 point lights) and at pixel centres under the sun and sky.
+        # camera rays, the sun and 64 point lights), the same lights at pixel centres, and pixel
+        # centres under the sun and sky.
         c5 = {}
-        for name, jit, pl in (("stated", True, True), ("pixel_centres_sun_sky", False, False)):
+        for name, jit, pl in (("stated", True, True), ("pixel_centres", False, True),
+                              ("pixel_centres_sun_sky", False, False)):
             log(f"[bench] time to equal RMSE: {name}")
             r = cache_measure(args, jit, pl, scene_parts=(s, ctx, scene))
             head = next((row for row in r["plain_vs_cached"] if row["plain_spp"] == args.budget),

@@ -338,48 +338,105 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint64_t* __re
   hist[size_t(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// A tile's keys in place, stable: wave w holds the tile's elements [1024 w, 1024 (w + 1)), slot s
+// of a lane element 1024 w + 64 s + lane, so ranks follow positions. Each wave counts its digits
+// (LDS atomics); a digit's place for a wave within the tile is the tile's earlier digits plus the
+// earlier waves' counts of it. The keys go to their places in the tile in LDS, a slot at a time (a
+// key's place is its wave's place for the digit plus the keys of that digit in the wave's earlier
+// slots and lower lanes, the lanes sharing a digit from a 64-bit LDS mask a digit that each lane
+// ORs its bit into), and then out in tile order, each to the tile's place for its digit (the scan
+// of the histograms) plus its rank in the digit's run: neighbouring lanes write neighbouring words.
 template <bool Keys>
 __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint64_t* __restrict__ kin,
                                                                const uint32_t* __restrict__ vin, uint64_t* kout,
                                                                uint32_t* vout, uint32_t n, int shift,
                                                                const uint32_t* __restrict__ offsets, uint32_t tiles) {
-  __shared__ uint32_t running[256];
-  __shared__ uint32_t wave_counts[kSortThreads / 64][256];
-  const uint32_t t = threadIdx.x;
-  const uint32_t lane = t & 63u, wave = t >> 6;
-  running[t] = offsets[size_t(t) * tiles + blockIdx.x];
-  for (int w = 0; w < kSortThreads / 64; ++w) wave_counts[w][t] = 0;
-  __syncthreads();
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  constexpr uint32_t kWaves = kSortThreads / 64;
+  static_assert(kSortItems * 64 * kWaves == kSortTile, "a wave holds a contiguous quarter of the tile");
+  static_assert(kSortThreads == 256, "a thread a digit");
+  __shared__ uint32_t cnt[kWaves][256];             // a wave's count of each digit, then its place in the tile
+  __shared__ unsigned long long peer[kWaves][256];  // the lanes holding each digit in this slot
+  __shared__ uint32_t start[256];                   // a digit's first place in the tile
+  __shared__ uint32_t dest[256];                    // a digit's first place in the output
+  __shared__ uint32_t wsum[kWaves];
+  __shared__ uint64_t lkey[kSortTile];
+  __shared__ uint32_t lval[kSortTile];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const unsigned long long me = 1ull << lane;
+  for (uint32_t w = 0; w < kWaves; ++w) {
+    cnt[w][t] = 0;
+    peer[w][t] = 0;
+  }
   const uint32_t base = blockIdx.x * kSortTile;
-  for (int k = 0; k < kSortItems; ++k) {
-    const uint32_t i = base + uint32_t(k) * kSortThreads + t;
-    const bool valid = i < n;
-    const uint64_t key = valid ? kin[i] : 0ull;
-    const uint32_t d = uint32_t(key >> shift) & 255u;
-    uint64_t match = __ballot(valid);
+  const uint32_t count = min(uint32_t(kSortTile), n - base);  // the tile's keys
+  const uint32_t first = wave * 64u * kSortItems;             // the wave's first element in the tile
+  uint64_t key[kSortItems];
+  uint32_t val[kSortItems];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bits = __ballot((d >> b) & 1u);
-      match &= ((d >> b) & 1u) ? bits : ~bits;
+  for (uint32_t s = 0; s < kSortItems; ++s) {
+    const uint32_t i = first + s * 64u + lane;
+    key[s] = i < count ? kin[base + i] : 0ull;
+    val[s] = i < count ? vin[base + i] : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t s = 0; s < kSortItems; ++s)
+    if (first + s * 64u + lane < count) atomicAdd(&cnt[wave][uint32_t(key[s] >> shift) & 255u], 1u);
+  __syncthreads();
+  {
+    // Digit t: its count in the tile, the exclusive scan over digits (a wave scan, then the waves'
+    // sums), and each wave's place for it.
+    uint32_t c[kWaves], sum = 0;
+    for (uint32_t w = 0; w < kWaves; ++w) {
+      c[w] = cnt[w][t];
+      sum += c[w];
     }
-    const uint32_t below = uint32_t(__popcll(match & lt));
-    if (valid && below == 0) wave_counts[wave][d] = uint32_t(__popcll(match));
+    uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = uint32_t(__shfl_up(incl, off));
+      if (lane >= uint32_t(off)) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
     __syncthreads();
+    uint32_t at = incl - sum;
+    for (uint32_t w = 0; w < wave; ++w) at += wsum[w];
+    start[t] = at;
+    dest[t] = offsets[size_t(t) * tiles + blockIdx.x];
+    for (uint32_t w = 0; w < kWaves; ++w) {
+      cnt[w][t] = at;
+      at += c[w];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t s = 0; s < kSortItems; ++s) {
+    if (first + s * 64u >= count) break;  // the wave's remaining slots are past the keys
+    const bool valid = first + s * 64u + lane < count;
+    uint64_t k = key[s];
+    asm volatile("" : "+v"(k));  // recomputed, not kept from the count above: registers
+    const uint32_t d = uint32_t(k >> shift) & 255u;
+    if (valid) atomicOr(&peer[wave][d], me);
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t peers = valid ? peer[wave][d] : 0ull;
+    const uint32_t at = valid ? cnt[wave][d] : 0u;
+    __builtin_amdgcn_wave_barrier();
     if (valid) {
-      uint32_t pos = running[d] + below;
-      for (uint32_t w = 0; w < wave; ++w) pos += wave_counts[w][d];
-      if (Keys) kout[pos] = key;
-      vout[pos] = vin[i];
+      peer[wave][d] = 0;
+      const uint32_t p = at + uint32_t(__popcll(peers & lt));
+      lkey[p] = k;
+      lval[p] = val[s];
+      if ((peers & lt) == 0) cnt[wave][d] = at + uint32_t(__popcll(peers));
     }
-    __syncthreads();
-    uint32_t sum = 0;
-    for (int w = 0; w < kSortThreads / 64; ++w) {
-      sum += wave_counts[w][t];
-      wave_counts[w][t] = 0;
-    }
-    running[t] += sum;
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  for (uint32_t j = t; j < count; j += kSortThreads) {
+    const uint64_t k = lkey[j];
+    const uint32_t d = uint32_t(k >> shift) & 255u;
+    const uint32_t p = dest[d] + (j - start[d]);
+    if (Keys) kout[p] = k;
+    vout[p] = lval[j];
   }
 }
 
