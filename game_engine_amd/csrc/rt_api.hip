@@ -466,7 +466,8 @@ void ge_rt_destroy(ge_rt_ctx* ctx) {
   for (DevBuf* b : {&ctx->build.keys[0], &ctx->build.keys[1], &ctx->build.vals[0], &ctx->build.vals[1], &ctx->build.hist,
                     &ctx->build.scan_tmp, &ctx->build.node_box, &ctx->build.node_parent, &ctx->build.seg_cbox,
                     &ctx->build.seg_scale, &ctx->build.depth, &ctx->build.buckets, &ctx->build.bucket_count,
-                    &ctx->build.flag, &ctx->counters})
+                    &ctx->build.flag, &ctx->build.cl_out, &ctx->build.cl_part, &ctx->build.cl_seg, &ctx->build.links,
+                    &ctx->counters})
     b->release();
   if (ctx->host_readback) (void)hipHostFree(ctx->host_readback);
   if (ctx->pin) (void)hipHostFree(ctx->pin);

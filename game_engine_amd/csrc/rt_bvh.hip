@@ -528,6 +528,7 @@ struct TileArgs {
   const float* boxes;
   const uint32_t* perm;
   ge_bvh_node* nodes;
+  const uint4* links;  // per batch entry first + i: internal node i's child[0], child[1], -, primitive count
   const uint32_t* node_parent;
   float* node_box;  // 8 floats a node above level 1 or at a level-1 tile's top: box, height bits
   float* roots;
@@ -568,15 +569,28 @@ __device__ void file_node(const TileArgs& a, uint32_t s, uint32_t i, uint32_t lo
   else atomicOr(a.flag, 1u);
 }
 
+// Stores of whole records as 16-byte vectors (records and wide nodes are 16-byte aligned).
+__device__ __forceinline__ void store_record(ge_bvh_node* out, const float* lo0, const float* hi0, const float* lo1,
+                                             const float* hi1, uint32_t c0, uint32_t c1, uint32_t parent,
+                                             uint32_t count) {
+  uint4* o = reinterpret_cast<uint4*>(out);
+  o[0] = make_uint4(__float_as_uint(lo0[0]), __float_as_uint(lo0[1]), __float_as_uint(lo0[2]), __float_as_uint(hi0[0]));
+  o[1] = make_uint4(__float_as_uint(hi0[1]), __float_as_uint(hi0[2]), __float_as_uint(lo1[0]), __float_as_uint(lo1[1]));
+  o[2] = make_uint4(__float_as_uint(lo1[2]), __float_as_uint(hi1[0]), __float_as_uint(hi1[1]), __float_as_uint(hi1[2]));
+  o[3] = make_uint4(c0, c1, parent, count);
+}
+
 // Fits internal node i of segment s from global memory: its record's child boxes and parent, its
 // box and height (node_box), and the root's box and height.
 __device__ void fit_node(const TileArgs& a, uint32_t s, uint32_t i) {
   const uint32_t first = a.seg_first[s], nb = a.node_base[s];
   ge_bvh_node* nd = a.nodes + nb + i;
+  const uint4 link = a.links[first + i];
+  const uint32_t child[2] = {link.x, link.y};
   float b[2][6];
   uint32_t h[2];
   for (int c = 0; c < 2; ++c) {
-    const uint32_t ch = nd->child[c];
+    const uint32_t ch = child[c];
     if (ch & GE_BVH_LEAF) {
       const uint32_t p = a.perm[first + (ch & ~GE_BVH_LEAF)];
       for (int q = 0; q < 6; ++q) b[c][q] = a.boxes[size_t(p) * 6 + q];
@@ -587,13 +601,8 @@ __device__ void fit_node(const TileArgs& a, uint32_t s, uint32_t i) {
       h[c] = __float_as_uint(cb[6]);
     }
   }
-  for (int q = 0; q < 3; ++q) {
-    nd->lo0[q] = b[0][q];
-    nd->hi0[q] = b[0][3 + q];
-    nd->lo1[q] = b[1][q];
-    nd->hi1[q] = b[1][3 + q];
-  }
-  nd->parent = i == 0 ? GE_BVH_NONE : a.node_parent[nb + i];
+  store_record(nd, b[0], b[0] + 3, b[1], b[1] + 3, child[0], child[1], i == 0 ? GE_BVH_NONE : a.node_parent[nb + i],
+               link.w);
   float u[6];
   for (int q = 0; q < 3; ++q) {
     u[q] = __builtin_fminf(b[0][q], b[1][q]);
@@ -610,16 +619,6 @@ __device__ void fit_node(const TileArgs& a, uint32_t s, uint32_t i) {
   }
 }
 
-// Stores of whole records as 16-byte vectors (records and wide nodes are 16-byte aligned).
-__device__ __forceinline__ void store_record(ge_bvh_node* out, const float* lo0, const float* hi0, const float* lo1,
-                                             const float* hi1, uint32_t c0, uint32_t c1, uint32_t parent,
-                                             uint32_t count) {
-  uint4* o = reinterpret_cast<uint4*>(out);
-  o[0] = make_uint4(__float_as_uint(lo0[0]), __float_as_uint(lo0[1]), __float_as_uint(lo0[2]), __float_as_uint(hi0[0]));
-  o[1] = make_uint4(__float_as_uint(hi0[1]), __float_as_uint(hi0[2]), __float_as_uint(lo1[0]), __float_as_uint(lo1[1]));
-  o[2] = make_uint4(__float_as_uint(lo1[2]), __float_as_uint(hi1[0]), __float_as_uint(hi1[1]), __float_as_uint(hi1[2]));
-  o[3] = make_uint4(c0, c1, parent, count);
-}
 // The unbiased exponent of a finite x >= 0 (-127 for zero and subnormals).
 __device__ __forceinline__ int exponent_of(float x) { return int((__float_as_uint(x) >> 23) & 0xffu) - 127; }
 // 2^e for -126 <= e <= 127, exactly.
@@ -738,7 +737,7 @@ __device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, int n, i
 __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_keys, const uint32_t* __restrict__ seg_of,
                          const uint32_t* __restrict__ seg_first, const uint32_t* __restrict__ seg_count,
                          const uint32_t* __restrict__ node_base, ge_bvh_node* nodes, uint32_t* node_parent,
-                         TileArgs ta) {
+                         uint4* __restrict__ links, TileArgs ta) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_total) return;
   const uint32_t s = seg_of[g];
@@ -779,13 +778,9 @@ __global__ void k_karras(uint32_t n_total, const uint64_t* __restrict__ sorted_k
   } while (t > 1);
   const int gamma = i + sp * d + (d < 0 ? d : 0);
   const int lo = i < j ? i : j, hi = i < j ? j : i;
-  ge_bvh_node& node = seg_nodes[i];
   const uint32_t c0 = (lo == gamma) ? (GE_BVH_LEAF | uint32_t(gamma)) : uint32_t(gamma);
   const uint32_t c1 = (hi == gamma + 1) ? (GE_BVH_LEAF | uint32_t(gamma + 1)) : uint32_t(gamma + 1);
-  node.child[0] = c0;
-  node.child[1] = c1;
-  node.count = uint32_t(hi - lo + 1);
-  if (i == 0) node.parent = GE_BVH_NONE;
+  links[g] = make_uint4(c0, c1, 0u, uint32_t(hi - lo + 1));  // the fit writes the whole record
   if (!(c0 & GE_BVH_LEAF)) node_parent[node_base[s] + gamma] = uint32_t(i);
   if (!(c1 & GE_BVH_LEAF)) node_parent[node_base[s] + gamma + 1] = uint32_t(i);
   if (ta.levels > 1) file_node(ta, s, uint32_t(i), uint32_t(lo), uint32_t(hi));
@@ -838,10 +833,10 @@ __device__ uint32_t sparse_setup(const TileArgs& a, SparseTile& sh, uint32_t b, 
     sh.pslot[j] = ps == 0xffffffffu ? kNoSlot : uint16_t(ps);
     sh.arrived[j] = 0;
     if (fit) {
-      const ge_bvh_node* nd = a.nodes + nb + i;
+      const uint4 link = a.links[a.seg_first[s] + i];
       uint32_t pend = 0;
       for (int c = 0; c < 2; ++c) {
-        const uint32_t ch = nd->child[c];
+        const uint32_t ch = c == 0 ? link.x : link.y;
         if (!(ch & GE_BVH_LEAF) && slot_of(ch) != 0xffffffffu) ++pend;
       }
       sh.pending[j] = uint8_t(pend);
@@ -933,7 +928,7 @@ __global__ void __launch_bounds__(kTileThreads) k_tile_build(TileArgs a) {
   // Child links; a node whose range leaves the tile is not this tile's (ch[j][0] = kNoSlot).
   for (uint32_t j = t; j < cnt; j += kTileThreads) {
     const uint32_t i = A + j;
-    const uint4 tail = reinterpret_cast<const uint4*>(a.nodes + nb + i)[3];  // child[2], parent, count
+    const uint4 tail = a.links[first + i];  // child[2], -, count
     const uint32_t c0 = tail.x, c1 = tail.y, cn = tail.w;
     const uint32_t gamma = c0 & ~GE_BVH_LEAF;
     const uint32_t lo = i <= gamma ? i : i + 1 - cn, hi = i <= gamma ? i + cn - 1 : i;
@@ -1577,9 +1572,13 @@ __global__ void __launch_bounds__(kBlockThreads) k_block_build(BlockArgs ba) {
 
 // For a refit: the parent links and the tile lists from the records a build left, and the
 // identity as the sorted order (the leaves' boxes are given in slot order).
-__global__ void k_parents_from_records(uint32_t n, const ge_bvh_node* __restrict__ nodes, uint32_t* node_parent) {
+__global__ void k_parents_from_records(uint32_t n, const ge_bvh_node* __restrict__ nodes, uint32_t* node_parent,
+                                       uint4* links) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + 1 < n) node_parent[i] = nodes[i].parent;
+  if (i + 1 >= n) return;
+  const ge_bvh_node nd = nodes[i];
+  node_parent[i] = nd.parent;
+  links[i] = make_uint4(nd.child[0], nd.child[1], 0u, nd.count);
 }
 __global__ void k_file_records(uint32_t n, const ge_bvh_node* __restrict__ nodes, TileArgs ta) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1739,8 +1738,10 @@ hipError_t build_segments(BuildScratch& sc, hipStream_t stream, uint32_t n, cons
   ta.heights = heights;
   ta.wide = wide;
   ta.flag = flag;
+  if ((e = sc.links.reserve(size_t(n) * 16)) != hipSuccess) return e;
+  ta.links = sc.links.as<uint4>();
   k_karras<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, d_seg_of, d_seg_first, d_seg_count, d_node_base, nodes,
-                                               sc.node_parent.as<uint32_t>(), ta);
+                                               sc.node_parent.as<uint32_t>(), sc.links.as<uint4>(), ta);
   if (!small) {
     k_codes<<<blocks(n, 256), 256, 0, stream>>>(n, sorted_keys, morton);
     if ((e = hipMemcpyAsync(perm, sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
@@ -1775,7 +1776,9 @@ hipError_t refit_segment(BuildScratch& sc, hipStream_t stream, uint32_t n, const
   ta.flag = flag;
   k_identity<<<blocks(n, 256), 256, 0, stream>>>(n, sc.vals[0].as<uint32_t>());
   ta.perm = sc.vals[0].as<uint32_t>();
-  k_parents_from_records<<<blocks(n, 256), 256, 0, stream>>>(n, nodes, sc.node_parent.as<uint32_t>());
+  if ((e = sc.links.reserve(size_t(n) * 16)) != hipSuccess) return e;
+  ta.links = sc.links.as<uint4>();
+  k_parents_from_records<<<blocks(n, 256), 256, 0, stream>>>(n, nodes, sc.node_parent.as<uint32_t>(), sc.links.as<uint4>());
   if (lad.levels > 1) k_file_records<<<blocks(n, 256), 256, 0, stream>>>(n, nodes, ta);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return tile_fit(stream, lad);

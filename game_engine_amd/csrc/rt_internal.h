@@ -42,6 +42,7 @@ struct BuildScratch {
   DevBuf buckets, bucket_count;   // the nodes each tile hands up to the next level
   DevBuf flag;                    // the build's refusal flag (a tile over capacity, too deep)
   DevBuf cl_out, cl_part, cl_seg; // the device-wide batch's clusters: first triangle, centroid bounds, segment
+  DevBuf links;                   // Karras' child links and counts, 16 B a batch entry, until the fit writes the records
 };
 
 // A one-workgroup build over triangles (at most k_block_build_max): its `count` triangles are those
