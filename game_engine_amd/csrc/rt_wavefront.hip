@@ -175,7 +175,7 @@ struct ShadeArgs {
   NeeRec* nee;
   ShadowRec* shadow;
   uint32_t* next_queue;
-  uint32_t* counts;  // [0] next queue, [1] shadow records
+  uint32_t* counts;  // [0] next queue, [1] shadow records; [2] [3] the walk heads, cleared for the next walk
   // The radiance cache (kShadeCached, kShadeRecord). Queries (with their throughput in
   // query_weights) are appended at counts[6] while they fit in query_capacity; training samples
   // (with theirs in sample_weights) at counts[7] while they fit in sample_capacity.
@@ -274,6 +274,8 @@ __global__ void __launch_bounds__(k_shade_threads)
     if (threadIdx.x == 0) ground_acc = 0.0;
     __syncthreads();
   }
+  // The walk heads: this bounce's walk is over, the next starts after this launch.
+  if (blockIdx.x == 0 && threadIdx.x < 2) a.counts[2 + threadIdx.x] = 0u;
   const uint32_t g = blockIdx.x * k_shade_threads + threadIdx.x;
   const bool valid = g < a.n;
   const uint32_t pid = valid ? a.queue[g] : 0u;
@@ -656,7 +658,9 @@ __global__ void __launch_bounds__(64) k_resolve(PathState* __restrict__ state, c
                                                 uint32_t n, const NeeRec* __restrict__ nee,
                                                 const ShadowRec* __restrict__ shadow,
                                                 const uint8_t* __restrict__ blocked, float4* __restrict__ result,
-                                                uint32_t npaths, ResolveAdjoint adj) {
+                                                uint32_t npaths, ResolveAdjoint adj, uint32_t* clear) {
+  // The next shading's queue and shadow-record counts (the walk that read them has ended).
+  if (blockIdx.x == 0 && threadIdx.x < 2) clear[threadIdx.x] = 0u;
   extern __shared__ double light_acc[];
   if (Adjoint) {
     for (uint32_t i = threadIdx.x; i < adj.lights; i += 64) light_acc[i] = 0.0;
@@ -839,10 +843,10 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
     return !masked ? k_alpha_opaque
                    : ((p.flags & GE_PT_ALPHA_MASKS) ? k_alpha_test : (bounce == 0 ? k_alpha_cull : k_alpha_opaque));
   };
-  // One walk launch: closest-hit rays of `cj`, shadow records of `aj` (the walk heads cleared first).
+  // One walk launch: closest-hit rays of `cj`, shadow records of `aj`. Its heads (counts[4], [5])
+  // are zero: the frame's first clear, then each shading's.
   auto launch_walks = [&](const ClosestJob& cj, const AnyJob& aj, bool mask_walk, uint32_t grid) -> hipError_t {
     hipError_t r;
-    if ((r = hipMemsetAsync(counts + 4, 0, 8, st)) != hipSuccess) return r;
     if ((r = stage_mark(ctx, 0, 0)) != hipSuccess) return r;
     if (mask_walk) {
       if (counters) k_walks<true, true><<<grid, 64, k_lds, st>>>(bvh, cj, aj, counters);
@@ -887,12 +891,12 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
         ra.grad_lights = adjoint->grad_lights;
         k_resolve<true><<<blocks(pending.n, 64), 64, size_t(ra.lights) * 8, st>>>(
             pending.state, pending.queue, pending.n, w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
-            w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths, ra);
+            w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths, ra, c2);
       } else {
         k_resolve<false><<<blocks(pending.n, 64), 64, 0, st>>>(pending.state, pending.queue, pending.n,
                                                               w.nee.as<NeeRec>(), w.shadow.as<ShadowRec>(),
                                                               w.blocked.as<uint8_t>(), w.result.as<float4>(), npaths,
-                                                              ResolveAdjoint{});
+                                                              ResolveAdjoint{}, c2);
       }
       if ((e = hipGetLastError()) != hipSuccess) return e;
       pending.live = false;
@@ -938,9 +942,10 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
       a.train_phase = hook->train_phase;
       a.box = hook->box;
     }
+    // The next queue's and the shadow records' counts are zero: the frame's first clear, then the
+    // previous bounce's resolve's (every bounce after the first resolves one before it shades).
     uint32_t nshadow = 0, next = 0;
     a.rerun = false;
-    if ((e = hipMemsetAsync(c2, 0, 8, st)) != hipSuccess) return e;  // the next queue's and the shadow records' counts
     for (;;) {
       a.shadow = w.shadow.as<ShadowRec>();
       a.shadow_capacity = uint32_t(std::min<uint64_t>(w.shadow_capacity, 0xffffffffu));
