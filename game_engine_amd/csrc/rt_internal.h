@@ -251,7 +251,7 @@ struct ge_rt_ctx {
   // The device address of this device's allocation-free working array (the shadow resolve's
   // counter, then GE_RT_SHADOW_STATIC_ENTRIES ray records), looked up once per device at creation.
   uint8_t* static_scratch = nullptr;
-  uint32_t persistent_blocks = 8192;  // one-wave workgroups a persistent walk launches: CUs x 32
+  uint32_t persistent_blocks = 5120;  // one-wave workgroups a persistent walk launches: CUs x 20
 };
 
 namespace ge {
