@@ -264,9 +264,13 @@ __device__ __forceinline__ bool sun_asked(const ge_path_trace_params& p, const S
 #ifndef GE_SHADE_WAVES_PER_EU
 #define GE_SHADE_WAVES_PER_EU 5
 #endif
+#ifndef GE_SHADE_ADJOINT_WAVES_PER_EU
+#define GE_SHADE_ADJOINT_WAVES_PER_EU 3
+#endif
 template <int Mode, bool Full = false>
 __global__ void __launch_bounds__(k_shade_threads)
-    __attribute__((amdgpu_waves_per_eu(Mode == kShadeAdjoint ? 3 : GE_SHADE_WAVES_PER_EU))) k_shade(ShadeArgs a) {
+    __attribute__((amdgpu_waves_per_eu(Mode == kShadeAdjoint ? GE_SHADE_ADJOINT_WAVES_PER_EU : GE_SHADE_WAVES_PER_EU)))
+    k_shade(ShadeArgs a) {
   __shared__ uint32_t parts[Mode == kShadePlain ? 2 : 4][k_shade_threads / 64 + 1];
   __shared__ double ground_acc;  // kShadeAdjoint: the workgroup's sum of the sun's ground-radiance term
   const bool grad = Mode == kShadeAdjoint && a.grad;

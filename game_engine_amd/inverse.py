@@ -13,6 +13,7 @@ same detached estimator.
 """
 import ctypes as C
 
+import numpy as np
 import torch
 
 from game_engine_amd import abi
@@ -32,6 +33,10 @@ class InverseRenderer:
         if desc.light_count:
             C.memmove(self.lights, desc.lights, C.sizeof(abi.ResolveLight) * desc.light_count)
         self.light_count = desc.light_count
+        # Float views of the records (albedo at words 0..3 of a material, intensity at word 7 of a
+        # light): the parameters go in as whole columns.
+        self._material_words = np.frombuffer(self.materials, dtype=np.float32).reshape(desc.material_count, -1)
+        self._light_words = np.frombuffer(self.lights, dtype=np.float32).reshape(len(self.lights), -1)
         self.width, self.height = params.width, params.height
         n = self.width * self.height
         self._image = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
@@ -49,18 +54,11 @@ class InverseRenderer:
         return albedo, rough, lights, sun
 
     def set_parameters(self, albedo, roughness, lights, sun):
-        a = albedo.detach().to("cpu", torch.float32).tolist()
-        r = roughness.detach().to("cpu", torch.float32).tolist()
-        for k in range(len(self.materials)):
-            self.materials[k].albedo[0] = a[k][0]
-            self.materials[k].albedo[1] = a[k][1]
-            self.materials[k].albedo[2] = a[k][2]
-            self.materials[k].albedo[3] = r[k]
+        self._material_words[:, 0:3] = albedo.detach().to("cpu", torch.float32).numpy()
+        self._material_words[:, 3] = roughness.detach().to("cpu", torch.float32).numpy()
         self.scene.update_materials(0, self.materials)
         if self.light_count:
-            li = lights.detach().to("cpu", torch.float32).tolist()
-            for k in range(self.light_count):
-                self.lights[k].color_intensity[3] = li[k]
+            self._light_words[:self.light_count, 7] = lights.detach().to("cpu", torch.float32).numpy()
             self.scene.update_lights(0, (abi.ResolveLight * self.light_count).from_buffer(self.lights))
         self.params.sun[3] = float(sun.detach().reshape(-1)[0])
 
