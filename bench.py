@@ -721,7 +721,8 @@ def main():
     ap.add_argument("--height", type=int, default=0, help="masked workload: frame height (0: 2160)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="cache workload: the reference's samples")
     ap.add_argument("--budget", type=int, default=64, help="cache workload: the plain tracer's samples")
-    ap.add_argument("--cache-warmup", type=int, default=32, help="cache workload: warm-up training frames")
+    ap.add_argument("--cache-warmup", type=int, default=8,
+                    help="cache workload: warm-up training frames (their time counts in the cached tracer's)")
     ap.add_argument("--cache-frames", type=int, default=512, help="cache workload: at most this many frames")
     ap.add_argument("--samples", type=int, default=1 << 16, help="cache workload: training samples a frame")
     ap.add_argument("--train-steps", type=int, default=4, help="cache workload: optimizer steps a frame")
