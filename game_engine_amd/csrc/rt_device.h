@@ -802,13 +802,37 @@ static_assert(k_refill >= 1 && k_refill <= 64, "a refill needs at most a wave of
 #endif
 constexpr uint32_t k_batch = GE_WALK_BATCH;
 static_assert(k_batch >= 1 && k_batch <= 64, "a batch is at most a wave");
+// Per kind of ray, where it measured better than the defaults above (the 20-wave-a-CU grid):
+// - the path tracer's bounces refill at 40 idle lanes (0.3% faster than 32, 48 0.5% slower);
+// - the visibility words' camera rays at 56 (their walk 8.6% faster than at 32, 48 8%): rays of
+//   one 8 x 8 tile end close together, so a wave waits for most of them before its next tile;
+// - the sun rays of the shadow resolve batch 24 lanes (3% faster than 16; refills at 48 1%).
+// The walks that decide masked candidates keep the defaults: through the alpha-tested forest the
+// camera rays' walk ran 7% slower refilling at 56, the sun rays' 1.4% slower batching 24.
+#ifndef GE_WALK_REFILL_PATHS
+#define GE_WALK_REFILL_PATHS 40
+#endif
+#ifndef GE_WALK_REFILL_CAMERA
+#define GE_WALK_REFILL_CAMERA 56
+#endif
+#ifndef GE_WALK_REFILL_SUN
+#define GE_WALK_REFILL_SUN 32
+#endif
+#ifndef GE_WALK_BATCH_SUN
+#define GE_WALK_BATCH_SUN 24
+#endif
+constexpr uint32_t k_refill_paths = GE_WALK_REFILL_PATHS;    // the path tracer's bounces
+constexpr uint32_t k_refill_camera = GE_WALK_REFILL_CAMERA;  // the visibility words' camera rays
+constexpr uint32_t k_refill_sun = GE_WALK_REFILL_SUN;        // the shadow resolve's sun rays
+constexpr uint32_t k_batch_sun = GE_WALK_BATCH_SUN;
 
 GE_DEV uint64_t lanes_below() {
   const uint32_t lane = threadIdx.x & 63u;
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-template <bool AnyHit, bool Count, bool Masks = false, class Load, class Store>
+template <bool AnyHit, bool Count, bool Masks = false, uint32_t Refill = k_refill, uint32_t Batch = k_batch,
+          class Load, class Store>
 GE_DEV void persistent_walk(const Bvh& bvh, uint32_t n, uint32_t* head, uint32_t* __restrict__ stack, Load load,
                             Store store, uint64_t& issued, Tally& tally, uint32_t alpha = k_alpha_opaque) {
   Walker<AnyHit, Count, Masks> w;
@@ -818,7 +842,7 @@ GE_DEV void persistent_walk(const Bvh& bvh, uint32_t n, uint32_t* head, uint32_t
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = uint32_t(__popcll(idle));
-    if (!exhausted && nidle >= k_refill) {
+    if (!exhausted && nidle >= Refill) {
       uint32_t base = 0;
       if ((threadIdx.x & 63u) == 0) base = atomicAdd(head, nidle);
       base = __builtin_amdgcn_readfirstlane(base);  // lane 0 took part: one value for the wave
@@ -848,14 +872,14 @@ GE_DEV void persistent_walk(const Bvh& bvh, uint32_t n, uint32_t* head, uint32_t
       if (exhausted) break;
       continue;
     }
-    // What each kind of work waits for: the wave runs a kind when k_batch lanes want it, or when
-    // fewer than k_batch lanes are left at a node (so that a wave never waits on itself).
+    // What each kind of work waits for: the wave runs a kind when Batch lanes want it, or when
+    // fewer than Batch lanes are left at a node (so that a wave never waits on itself).
     const uint32_t at_node = uint32_t(__popcll(__ballot(active && w.state == k_walk_node)));
     const uint32_t at_leaf = uint32_t(__popcll(__ballot(active && w.state == k_walk_leaf)));
     const uint32_t at_enter = uint32_t(__popcll(__ballot(active && w.state == k_walk_enter)));
-    const bool few = at_node < k_batch;
-    if (at_enter && (few || at_enter >= k_batch) && active && w.state == k_walk_enter) w.enter(bvh, stack, spill, tally);
-    if (at_leaf && (few || at_leaf >= k_batch) && active && w.state == k_walk_leaf) w.leaves(bvh, stack, spill, tally);
+    const bool few = at_node < Batch;
+    if (at_enter && (few || at_enter >= Batch) && active && w.state == k_walk_enter) w.enter(bvh, stack, spill, tally);
+    if (at_leaf && (few || at_leaf >= Batch) && active && w.state == k_walk_leaf) w.leaves(bvh, stack, spill, tally);
     if (active && w.state == k_walk_node) w.visit(bvh, stack, spill, tally);
     if (active && w.state == k_walk_done) {
       store(slot, w.hit);

@@ -139,7 +139,8 @@ __device__ void closest_walks(const Bvh& bvh, const ClosestJob& job, uint64_t& i
     out.prim = h.prim;
     hits[pid] = out;
   };
-  persistent_walk<false, Count, Masks>(bvh, job.n, job.head, lane_stack(), load, store, issued, tally, job.alpha);
+  persistent_walk<false, Count, Masks, k_refill_paths>(bvh, job.n, job.head, lane_stack(), load, store, issued, tally,
+                                                       job.alpha);
 }
 
 __device__ f3 environment(const ge_path_trace_params& p, f3 direction, bool primary) {
@@ -618,7 +619,8 @@ __device__ void any_walks(const Bvh& bvh, const AnyJob& job, uint64_t& issued, T
     return true;
   };
   auto store = [&](uint32_t j, const Hit& h) { blocked[j] = h.found ? 1 : 0; };
-  persistent_walk<true, Count, Masks>(bvh, job.n, job.head, lane_stack(), load, store, issued, tally, job.alpha);
+  persistent_walk<true, Count, Masks, k_refill_paths>(bvh, job.n, job.head, lane_stack(), load, store, issued, tally,
+                                                      job.alpha);
 }
 
 // The walks of one launch: a bounce's closest-hit rays, then the previous bounce's shadow rays (the
