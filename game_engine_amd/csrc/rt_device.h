@@ -808,7 +808,8 @@ static_assert(k_batch >= 1 && k_batch <= 64, "a batch is at most a wave");
 //   one 8 x 8 tile end close together, so a wave waits for most of them before its next tile;
 // - the sun rays of the shadow resolve batch 24 lanes (3% faster than 16; refills at 48 1%).
 // The walks that decide masked candidates keep the defaults: through the alpha-tested forest the
-// camera rays' walk ran 7% slower refilling at 56, the sun rays' 1.4% slower batching 24.
+// camera rays' walk ran 7% slower refilling at 56, the sun rays' 1.4% slower batching 24, and
+// refills at 24 or 40 or batches of 8 or 24 measured no better for either.
 #ifndef GE_WALK_REFILL_PATHS
 #define GE_WALK_REFILL_PATHS 40
 #endif
@@ -825,6 +826,14 @@ constexpr uint32_t k_refill_paths = GE_WALK_REFILL_PATHS;    // the path tracer'
 constexpr uint32_t k_refill_camera = GE_WALK_REFILL_CAMERA;  // the visibility words' camera rays
 constexpr uint32_t k_refill_sun = GE_WALK_REFILL_SUN;        // the shadow resolve's sun rays
 constexpr uint32_t k_batch_sun = GE_WALK_BATCH_SUN;
+#ifndef GE_WALK_REFILL_MASKED
+#define GE_WALK_REFILL_MASKED GE_WALK_REFILL
+#endif
+#ifndef GE_WALK_BATCH_MASKED
+#define GE_WALK_BATCH_MASKED GE_WALK_BATCH
+#endif
+constexpr uint32_t k_refill_masked = GE_WALK_REFILL_MASKED;  // walks that decide masked candidates
+constexpr uint32_t k_batch_masked = GE_WALK_BATCH_MASKED;
 
 GE_DEV uint64_t lanes_below() {
   const uint32_t lane = threadIdx.x & 63u;

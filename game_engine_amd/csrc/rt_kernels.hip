@@ -128,8 +128,8 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_ray_visibility(Bvh bvh, ge_
     }
     words[pixel] = value;
   };
-  persistent_walk<false, Count, Masks, Masks ? k_refill : k_refill_camera>(bvh, n, head, lane_stack(), load, store, issued,
-                                                                          tally, alpha);
+  persistent_walk<false, Count, Masks, Masks ? k_refill_masked : k_refill_camera, Masks ? k_batch_masked : k_batch>(
+      bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, issued, 0, tally);
 }
 

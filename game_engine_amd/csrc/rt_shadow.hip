@@ -110,8 +110,8 @@ __global__ void __launch_bounds__(64) GE_WALK_ATTR k_sun_walk(Bvh bvh, const flo
   auto store = [&](uint32_t i, const Hit& h) {
     if (h.found) out[i] |= 1u;
   };
-  persistent_walk<true, Count, Masks, k_refill_sun, Masks ? k_batch : k_batch_sun>(bvh, n, head, lane_stack(), load, store,
-                                                                                   issued, tally, alpha);
+  persistent_walk<true, Count, Masks, Masks ? k_refill_masked : k_refill_sun, Masks ? k_batch_masked : k_batch_sun>(
+      bvh, n, head, lane_stack(), load, store, issued, tally, alpha);
   if (Count) flush_counters(counters, 0, issued, tally);
 }
 
