@@ -873,9 +873,10 @@ hipError_t wavefront_path_trace(ge_rt_ctx* ctx, const Bvh& bvh, const SceneView&
   };
   // Plain tracing enqueues the next bounce's walks before it waits for the shading's counts (they
   // read the counts on the device), so the host's round trip hides behind the walk; a shading that
-  // overflows its shadow records runs again, and so does that walk. Counted walks, the cache's and
-  // the adjoint's passes wait first, as a walk that ran twice would count twice.
-  const bool ahead = !counters && !hook && !adjoint;
+  // overflows its shadow records runs again, and so does that walk. Counted walks, the cache's
+  // recording and the adjoint's passes wait first (a walk run twice would count twice; the
+  // recording and the adjoint keep their plain order).
+  const bool ahead = !counters && !adjoint && (!hook || hook->mode == CacheHook::kCached);
   bool walked = false;        // this bounce's walks were enqueued ahead
   uint32_t last_shadows = 0;  // the previous bounce's shadow records: the size of the walk ahead's grid
   for (uint32_t bounce = 0; n > 0 || pending.live; ++bounce) {
