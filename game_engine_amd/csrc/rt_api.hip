@@ -187,6 +187,83 @@ Bvh bvh_view(const SceneDev& s) {
   return b;
 }
 
+// The light grid over the box the lights' ranges reach (point and spot lights: a light at or
+// beyond its radius adds nothing, light_sample's test). A cell's mask holds the lights whose
+// sphere, grown by a thousandth of its radius and of a cell, meets the cell's box grown by a
+// thousandth of a cell, so a point the shading places in the cell is within range only of lights
+// in its mask; directional lights are in every cell. No light with a range: no grid.
+hipError_t build_light_grid(SceneDev& s, hipStream_t stream) {
+  s.light_grid = false;
+  const uint32_t nl = std::min<uint32_t>(uint32_t(s.h_lights.size()), 64u);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool ranged = false;
+  auto radius_of = [](const ge_resolve_light& l) { return std::max(std::fabs(l.position_radius[3]), 1e-4f); };
+  for (uint32_t i = 0; i < nl; ++i) {
+    const ge_resolve_light& l = s.h_lights[i];
+    if (l.direction_type[3] >= 1.5f) continue;
+    if (!std::isfinite(l.direction_type[3])) return hipSuccess;
+    for (int a = 0; a < 4; ++a)
+      if (!std::isfinite(l.position_radius[a])) return hipSuccess;
+    ranged = true;
+    const float r = radius_of(l);
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], l.position_radius[a] - r);
+      hi[a] = std::max(hi[a], l.position_radius[a] + r);
+    }
+  }
+  if (!ranged || !(std::isfinite(lo[0]) && std::isfinite(lo[1]) && std::isfinite(lo[2]) && std::isfinite(hi[0]) &&
+                   std::isfinite(hi[1]) && std::isfinite(hi[2])))
+    return hipSuccess;
+  // Grown by a thousandth of its largest extent: a point outside is that far from every range.
+  double pad = 0.0;
+  for (int a = 0; a < 3; ++a) pad = std::max(pad, double(hi[a]) - double(lo[a]));
+  pad = 1e-3 * std::max(pad, 1e-3);
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = float(double(lo[a]) - pad);
+    hi[a] = float(double(hi[a]) + pad);
+  }
+  constexpr uint32_t G = k_light_grid;
+  double cell[3];
+  s.light_far = 0;
+  for (uint32_t i = 0; i < nl; ++i)
+    if (s.h_lights[i].direction_type[3] >= 1.5f) s.light_far |= 1ull << i;
+  for (int a = 0; a < 3; ++a) {
+    cell[a] = std::max(double(hi[a]) - double(lo[a]), 1e-3) / G;
+    s.light_lo[a] = lo[a];
+    s.light_inv[a] = float(1.0 / cell[a]);
+  }
+  std::vector<uint64_t> masks(size_t(G) * G * G, 0ull);
+  for (uint32_t i = 0; i < nl; ++i) {
+    const ge_resolve_light& l = s.h_lights[i];
+    const bool everywhere = l.direction_type[3] >= 1.5f;
+    const double r = radius_of(l);
+    for (uint32_t z = 0; z < G; ++z)
+      for (uint32_t y = 0; y < G; ++y)
+        for (uint32_t x = 0; x < G; ++x) {
+          bool in = everywhere;
+          if (!in) {
+            const uint32_t c[3] = {x, y, z};
+            double d2 = 0.0, grow = 0.0;
+            for (int a = 0; a < 3; ++a) grow = std::max(grow, 1e-3 * cell[a]);
+            for (int a = 0; a < 3; ++a) {
+              const double b0 = double(lo[a]) + c[a] * cell[a] - grow, b1 = double(lo[a]) + (c[a] + 1) * cell[a] + grow;
+              const double p = l.position_radius[a];
+              const double d = p < b0 ? b0 - p : (p > b1 ? p - b1 : 0.0);
+              d2 += d * d;
+            }
+            const double reach = r * (1.0 + 1e-3) + grow;
+            in = d2 < reach * reach;
+          }
+          if (in) masks[(size_t(z) * G + y) * G + x] |= 1ull << i;
+        }
+  }
+  hipError_t e = s.light_cells.reserve(masks.size() * 8);
+  if (e == hipSuccess) e = hipMemcpyAsync(s.light_cells.ptr, masks.data(), masks.size() * 8, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e == hipSuccess) s.light_grid = true;
+  return e;
+}
+
 SceneView scene_view(const SceneDev& s) {
   SceneView v{};
   v.clusters = s.clusters.as<ge_cluster_desc>();
@@ -199,6 +276,12 @@ SceneView scene_view(const SceneDev& s) {
   v.cluster_materials = s.cluster_materials.as<uint32_t>();
   v.lights = s.lights.as<ge_resolve_light>();
   v.light_count = s.light_count;
+  v.light_cells = s.light_grid ? s.light_cells.as<uint64_t>() : nullptr;
+  v.light_far = s.light_far;
+  for (int a = 0; a < 3; ++a) {
+    v.light_lo[a] = s.light_lo[a];
+    v.light_inv[a] = s.light_inv[a];
+  }
   v.features = (s.any_masked ? k_scene_masked : 0u) | (s.any_textured ? k_scene_textured : 0u);
   v.tex = TexView{s.textures.as<ge_texture_desc>(), s.samplers.as<ge_sampler_desc>(), s.texels.as<uint8_t>(),
                   s.srgb.as<float>()};
@@ -669,6 +752,7 @@ int32_t ge_rt_scene_upload(ge_rt_ctx* ctx, const ge_scene_desc* d, ge_rt_scene**
   if (e == hipSuccess) e = upload(s.srgb, srgb, 256, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e == hipSuccess) e = refresh_masks(s, st);
+  if (e == hipSuccess) e = build_light_grid(s, st);
   if (e != hipSuccess)
     return fail(e == hipErrorOutOfMemory ? GE_RT_ERROR_OUT_OF_MEMORY : GE_RT_ERROR_DEVICE,
                 std::string("scene upload: ") + hipGetErrorString(e));
@@ -690,7 +774,8 @@ void ge_rt_scene_destroy(ge_rt_ctx* ctx, ge_rt_scene* scene) {
                     &s.materials, &s.cluster_materials, &s.lights, &s.tri_prefix, &s.tri_mesh_of, &s.blas_nodes,
                     &s.blas_tris, &s.blas_morton, &s.mesh_roots, &s.mesh_height, &s.tlas_nodes, &s.tlas_slots,
                     &s.tlas_morton, &s.inst_recs, &s.inst_boxes, &s.live_ids, &s.tlas_root, &s.tlas_height, &s.pairs,
-                    &s.textures, &s.samplers, &s.texels, &s.srgb, &s.mask_inst, &s.blas_wide, &s.tlas_wide})
+                    &s.textures, &s.samplers, &s.texels, &s.srgb, &s.mask_inst, &s.blas_wide, &s.tlas_wide,
+                    &s.light_cells})
     b->release();
   delete scene;
 }
@@ -1435,6 +1520,7 @@ int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t f
   GE_HIP(ctx, hipMemcpyAsync(s.lights.as<ge_resolve_light>() + first, lights, size_t(count) * sizeof(ge_resolve_light),
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  GE_HIP(ctx, build_light_grid(s, ctx->stream));
   return GE_RT_OK;
 } catch (...) {
   return host_failure(ctx);

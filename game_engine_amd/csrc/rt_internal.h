@@ -101,6 +101,10 @@ struct SceneDev {
   // Bumped by every change the traced image can see (materials, lights, instances, structures).
   uint64_t version = 0;
   std::vector<ge_resolve_light> h_lights;
+  DevBuf light_cells;               // k_light_grid^3 masks (empty: no grid)
+  bool light_grid = false;
+  uint64_t light_far = 0;
+  float light_lo[3] = {0, 0, 0}, light_inv[3] = {0, 0, 0};
   // The bindless textures (level 0), samplers, the sRGB decode table, and per instance its mesh's
   // first cluster and material base, bit 31 set when a cluster's material is alpha-tested.
   DevBuf textures, samplers, texels, srgb, mask_inst;
@@ -322,7 +326,15 @@ struct SceneView {
   uint32_t light_count;
   uint32_t features;  // k_scene_masked | k_scene_textured
   TexView tex;        // the scene's textures and samplers (none: null)
+  // Lights 0..63 by cell of a k_light_grid^3 grid over the box their ranges reach: bit i of a cell's
+  // mask is set when light i may reach a point of the cell (a light without a range everywhere).
+  // A point outside the box takes light_far (the lights without a range); light_cells null: every
+  // light.
+  const uint64_t* light_cells;
+  uint64_t light_far;
+  float light_lo[3], light_inv[3];  // the box's low corner, cells per unit length
 };
+constexpr uint32_t k_light_grid = 16;
 // SceneView::features: a material is alpha-tested (the walks' mask records are built), a material
 // reads a texture.
 constexpr uint32_t k_scene_masked = 1u, k_scene_textured = 2u;

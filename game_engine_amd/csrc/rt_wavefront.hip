@@ -385,13 +385,38 @@ __global__ void __launch_bounds__(k_shade_threads)
       // Count the records first; the same tests run again when they are written.
       if (!cache_here) {
         nshadow = sun_asked(p, surface) ? 1u : 0u;
-        for (uint32_t i = 0; i < a.sv.light_count; ++i) {
+        // Lights 0..63 through the light grid's cell mask (a light not in the cell's mask is out
+        // of range there), the rest in full.
+        const uint32_t low = a.sv.light_count < 64u ? a.sv.light_count : 64u;
+        uint64_t cand = low == 64u ? ~0ull : (1ull << low) - 1ull;
+        if (a.sv.light_cells) {
+          const float gx = (vx.position.x - a.sv.light_lo[0]) * a.sv.light_inv[0];
+          const float gy = (vx.position.y - a.sv.light_lo[1]) * a.sv.light_inv[1];
+          const float gz = (vx.position.z - a.sv.light_lo[2]) * a.sv.light_inv[2];
+          constexpr float G = float(k_light_grid);
+          if (gx >= 0.0f && gx < G && gy >= 0.0f && gy < G && gz >= 0.0f && gz < G) {
+            const uint32_t cell = (uint32_t(gz) * k_light_grid + uint32_t(gy)) * k_light_grid + uint32_t(gx);
+            cand &= a.sv.light_cells[cell];
+          } else {
+            cand &= a.sv.light_far;
+          }
+        }
+        while (cand) {
+          const uint32_t i = uint32_t(__builtin_ctzll(cand));
+          cand &= cand - 1ull;
           f3 ld, rad;
           float dist;
           if (!light_sample(a.sv.lights[i], vx.position, ld, rad, dist)) continue;
           if (dot(surface.normal, ld) <= 0.0f) continue;
           ++nshadow;
-          if (i < 64u) lit |= 1ull << i;
+          lit |= 1ull << i;
+        }
+        for (uint32_t i = 64u; i < a.sv.light_count; ++i) {
+          f3 ld, rad;
+          float dist;
+          if (!light_sample(a.sv.lights[i], vx.position, ld, rad, dist)) continue;
+          if (dot(surface.normal, ld) <= 0.0f) continue;
+          ++nshadow;
         }
       }
     }

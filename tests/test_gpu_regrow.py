@@ -231,3 +231,58 @@ def test_more_than_64_lights(many_lights):
     pt.light_count = 80
     _adjoint_check(g, o, pt)
     g.close(), ctx.close()
+
+
+def test_light_grid_follows_moved_lights(many_lights):
+    """The shading's count of shadow records visits lights 0..63 through a grid of per-cell light
+    masks, built from the lights at upload and again at each light update. With directional lights
+    among the point lights (in every cell's mask, and the only ones outside the grid's box), the
+    point lights then moved, shrunk and widened through the light update, and all of them moved
+    far from the scene (surfaces outside the box), the counts and the picture equal the oracle's."""
+    s, desc, lights = many_lights
+    mixed = (abi.ResolveLight * 48)()
+    C.memmove(mixed, lights, C.sizeof(mixed))
+    for i in (0, 17, 40):
+        mixed[i].position_radius[:] = [0.3, 1.0, -0.2, 0.0]
+        mixed[i].color_intensity[:] = [1.0, 1.0, 1.0, 0.4]
+        mixed[i].direction_type[:] = [0.0, 0.0, 0.0, 2.0]
+    d = abi.SceneDesc.from_buffer_copy(desc)
+    d.lights, d.light_count = C.addressof(mixed), 48
+    ctx = rt.Context(0)
+    g = ctx.upload(d).build()
+    o = ob.Scene(d)
+    pt, _ = s.frame(96, 64)
+    pt.light_count = 48
+    rng = np.random.default_rng(12)
+
+    def check(tag):
+        ctx.reset_stats()
+        ob.reset_counters()
+        ga, gc = g.path_trace(pt, count=True)
+        oa, oc = o.path_trace(pt)
+        assert np.array_equal(gc, oc), tag
+        rel = np.abs(ga[:, :3] - oa[:, :3]) / np.maximum(np.abs(oa[:, :3]), 1e-6)
+        assert rel.max() <= 1e-4, tag
+        gs, os_ = ctx.stats(), ob.counters()
+        for k in ("closest", "any", "any_nodes"):
+            assert gs[k] == os_[k], (tag, k)
+        return gs["any"]
+
+    base = check("upload")
+    moved = (abi.ResolveLight * 48)()
+    C.memmove(moved, mixed, C.sizeof(moved))
+    for i, l in enumerate(moved):
+        if l.direction_type[3] < 1.5:
+            l.position_radius[0] += float(rng.normal()) * 3.0
+            l.position_radius[2] += float(rng.normal()) * 3.0
+            l.position_radius[3] = [0.5, 30.0, 8.0][i % 3]
+    g.update_lights(0, moved)
+    o.update_lights(0, moved)
+    assert check("moved") != base
+    for l in moved:
+        if l.direction_type[3] < 1.5:
+            l.position_radius[1] += 1.0e4
+    g.update_lights(0, moved)
+    o.update_lights(0, moved)
+    check("far")
+    g.close(), ctx.close()
