@@ -516,7 +516,9 @@ int32_t ge_rt_scene_update_instances(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_
  * structures are unaffected. Host pointer. */
 int32_t ge_rt_scene_update_materials(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
                                      const ge_resolve_material* materials);
-/* Replaces light records [first, first + count). Host pointer. */
+/* Replaces light records [first, first + count). Host pointer. Also rebuilds, on the host, the
+ * scene's grid of per-cell light masks over lights 0..63 (16^3 cells, 32 KiB uploaded) that the
+ * shading uses to skip lights out of range; an update equal to the records held does nothing. */
 int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t first, uint32_t count,
                                   const ge_resolve_light* lights);
 /* The frame's visible list (PathTraceParams::visible: cluster_cull's output, scene.slang:663-741):
