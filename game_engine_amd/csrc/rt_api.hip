@@ -1516,11 +1516,19 @@ int32_t ge_rt_scene_update_lights(ge_rt_ctx* ctx, ge_rt_scene* scene, uint32_t f
   GE_HIP(ctx, hipSetDevice(ctx->device));
   if (std::memcmp(&s.h_lights[first], lights, size_t(count) * sizeof(ge_resolve_light)) == 0) return GE_RT_OK;
   s.version = next_version();
+  // The light grid reads lights 0..63's positions, radii and kinds: an update of colours and
+  // intensities alone (the inverse-rendering loop's) keeps it.
+  bool placed = false;
+  for (uint32_t i = 0; i < count && first + i < 64u && !placed; ++i) {
+    const ge_resolve_light& was = s.h_lights[first + i];
+    placed = std::memcmp(was.position_radius, lights[i].position_radius, sizeof(was.position_radius)) != 0 ||
+             std::memcmp(&was.direction_type[3], &lights[i].direction_type[3], sizeof(float)) != 0;
+  }
   std::memcpy(&s.h_lights[first], lights, size_t(count) * sizeof(ge_resolve_light));
   GE_HIP(ctx, hipMemcpyAsync(s.lights.as<ge_resolve_light>() + first, lights, size_t(count) * sizeof(ge_resolve_light),
                              hipMemcpyHostToDevice, ctx->stream));
   GE_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  GE_HIP(ctx, build_light_grid(s, ctx->stream));
+  if (placed) GE_HIP(ctx, build_light_grid(s, ctx->stream));
   return GE_RT_OK;
 } catch (...) {
   return host_failure(ctx);
